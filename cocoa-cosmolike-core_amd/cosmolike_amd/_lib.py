@@ -1,0 +1,97 @@
+"""ctypes binding of libcosmolike_hip.so (the C ABI in include/cosmolike_hip.h).
+
+The library is required: importing a compute path without it raises, there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcosmolike_hip.so")
+
+CLH_OK = 0
+ERRORS = {-1: "CLH_ERR_ARG", -2: "CLH_ERR_NAN", -3: "CLH_ERR_STATE", -4: "CLH_ERR_DOMAIN",
+          -5: "CLH_ERR_HIP", -6: "CLH_ERR_ALLOC"}
+
+D = ctypes.POINTER(ctypes.c_double)
+I = ctypes.POINTER(ctypes.c_int)
+VP = ctypes.c_void_p
+
+
+class clh_draw(ctypes.Structure):
+    _fields_ = [
+        ("omega_m", ctypes.c_double),
+        ("chi_z", D), ("chi", D), ("chi_n", ctypes.c_int),
+        ("G_z", D), ("G", D), ("G_n", ctypes.c_int),
+        ("log10k", D), ("pk_z", D), ("lnP", D), ("nk", ctypes.c_int), ("npz", ctypes.c_int),
+        ("src_dz", D), ("lens_dz", D), ("lens_stretch", D),
+        ("b1", D), ("bmag", D), ("shear_m", D),
+        ("ia_A", ctypes.c_double), ("ia_eta", ctypes.c_double), ("ia_bin", D),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(python -m cosmolike_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        i32 = ctypes.c_int
+        f64 = ctypes.c_double
+        sig = {
+            "clh_create": (i32, [ctypes.POINTER(VP), i32]),
+            "clh_destroy": (None, [VP]),
+            "clh_last_error": (ctypes.c_char_p, [VP]),
+            "clh_default": (VP, []),
+            "clh_set_legendre_mode": (i32, [VP, i32]),
+            "clh_init_ntable": (i32, [VP, i32, i32, i32, i32]),
+            "clh_init_binning_real_space": (i32, [VP, i32, f64, f64]),
+            "clh_init_ia": (i32, [VP, i32, i32]),
+            "clh_set_cosmological_parameters": (i32, [VP, f64, f64]),
+            "clh_set_distances": (i32, [VP, D, D, i32]),
+            "clh_set_growth": (i32, [VP, D, D, i32]),
+            "clh_set_non_linear_power_spectrum": (i32, [VP, D, i32, D, i32, D]),
+            "clh_set_source_sample": (i32, [VP, D, i32, i32]),
+            "clh_set_lens_sample": (i32, [VP, D, i32, i32]),
+            "clh_set_nuisance_shear_calib": (i32, [VP, D, i32]),
+            "clh_set_nuisance_shear_photoz": (i32, [VP, D, i32]),
+            "clh_set_nuisance_clustering_photoz": (i32, [VP, D, i32]),
+            "clh_set_nuisance_clustering_photoz_stretch": (i32, [VP, D, i32]),
+            "clh_set_nuisance_linear_bias": (i32, [VP, D, i32]),
+            "clh_set_nuisance_magnification_bias": (i32, [VP, D, i32]),
+            "clh_set_nuisance_ia": (i32, [VP, D, i32]),
+            "clh_datavector_size": (i32, [VP]),
+            "clh_compute_3x2pt_real": (i32, [VP, I, D, i32]),
+            "clh_xi_pm": (i32, [VP, D, D]),
+            "clh_gammat": (i32, [VP, D]),
+            "clh_wtheta": (i32, [VP, D]),
+            "clh_cl_tables": (i32, [VP, D, D, D]),
+            "clh_cl_nointerp": (i32, [VP, i32, D, i32, D]),
+            "clh_batch_upload": (i32, [VP, i32, ctypes.POINTER(clh_draw)]),
+            "clh_batch_forward": (i32, [VP, I, VP]),
+            "clh_synchronize": (i32, [VP]),
+            "clh_last_kernel_times": (i32, [VP, ctypes.POINTER(ctypes.c_char_p),
+                                            ctypes.POINTER(ctypes.c_float), i32]),
+            "clh_enable_timing": (i32, [VP, i32]),
+            "xi_pm_tomo": (f64, [i32, i32, i32, i32, i32]),
+            "w_gammat_tomo": (f64, [i32, i32, i32, i32]),
+            "w_gg_tomo": (f64, [i32, i32, i32, i32]),
+            "C_ss_tomo_limber": (f64, [f64, i32, i32, i32]),
+            "C_gs_tomo_limber": (f64, [f64, i32, i32]),
+            "C_gg_tomo_limber": (f64, [f64, i32, i32]),
+            "C_ss_tomo_limber_nointerp": (f64, [f64, i32, i32, i32, i32]),
+            "C_gs_tomo_limber_nointerp": (f64, [f64, i32, i32, i32]),
+            "C_gg_tomo_limber_nointerp": (f64, [f64, i32, i32, i32]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib

@@ -1,0 +1,255 @@
+"""Python mirror of the cosmolike_interface calls Cocoa makes for the
+real-space 3x2pt path (cosmolike/generic_interface.{hpp,cpp},
+cosmolike/cosmo2D_wrapper.cpp), backed by the HIP library.
+
+Method names follow the reference; errors raise ``CosmolikeError`` (the
+reference logs and exits).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import D, clh_draw
+
+
+class CosmolikeError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_lib.ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _arr(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def _p(x):
+    return None if x is None else x.ctypes.data_as(D)
+
+
+LEGENDRE_FOLDED = 0
+LEGENDRE_FULL = 1
+
+
+class Cosmolike:
+    """One library context: survey layout + the current draw."""
+
+    def __init__(self, device: int = 0, handle=None):
+        self.L = _lib.lib()
+        if handle is not None:
+            self.h = ctypes.c_void_p(handle)
+            self._owned = False
+        else:
+            h = ctypes.c_void_p()
+            rc = self.L.clh_create(ctypes.byref(h), device)
+            if rc:
+                raise CosmolikeError(rc, "clh_create failed")
+            self.h = h
+            self._owned = True
+        self.ns = self.nl = self.ntheta = 0
+        self._keep = []
+
+    @classmethod
+    def default(cls):
+        """The process-wide context behind xi_pm_tomo & co."""
+        L = _lib.lib()
+        return cls(handle=L.clh_default())
+
+    def close(self):
+        if getattr(self, "_owned", False) and self.h:
+            self.L.clh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise CosmolikeError(rc, self.L.clh_last_error(self.h).decode())
+
+    # ------------------------------------------------------------- init/set
+    def init_ntable(self, n_ell=300, n_a=225, lmax=50000, high_def=0):
+        self._chk(self.L.clh_init_ntable(self.h, n_ell, n_a, lmax, high_def))
+
+    def init_binning_real_space(self, ntheta, tmin_arcmin, tmax_arcmin):
+        self.ntheta = ntheta
+        self._chk(self.L.clh_init_binning_real_space(self.h, ntheta, tmin_arcmin, tmax_arcmin))
+
+    def init_IA(self, ia_model=0, ia_redshift_evol=3):
+        self._chk(self.L.clh_init_ia(self.h, ia_model, ia_redshift_evol))
+
+    def set_legendre_mode(self, mode):
+        self._chk(self.L.clh_set_legendre_mode(self.h, mode))
+
+    def set_cosmological_parameters(self, omega_matter, hubble=70.0):
+        self._chk(self.L.clh_set_cosmological_parameters(self.h, float(omega_matter), float(hubble)))
+
+    def set_distances(self, z, chi):
+        z, chi = _arr(z), _arr(chi)
+        self._chk(self.L.clh_set_distances(self.h, _p(z), _p(chi), len(z)))
+
+    def set_growth(self, z, G):
+        z, G = _arr(z), _arr(G)
+        self._chk(self.L.clh_set_growth(self.h, _p(z), _p(G), len(z)))
+
+    def set_non_linear_power_spectrum(self, log10k, z, lnP):
+        lk, z, lnP = _arr(log10k), _arr(z), _arr(lnP)
+        self._chk(self.L.clh_set_non_linear_power_spectrum(self.h, _p(lk), len(lk), _p(z), len(z), _p(lnP)))
+
+    def set_source_sample(self, table):
+        t = _arr(table)
+        self.ns = t.shape[1] - 1
+        self._chk(self.L.clh_set_source_sample(self.h, _p(t), t.shape[0], self.ns))
+
+    def set_lens_sample(self, table):
+        t = _arr(table)
+        self.nl = t.shape[1] - 1
+        self._chk(self.L.clh_set_lens_sample(self.h, _p(t), t.shape[0], self.nl))
+
+    def _setv(self, fn, v):
+        v = _arr(v)
+        self._chk(fn(self.h, _p(v), len(v)))
+
+    def set_nuisance_shear_calib(self, m):
+        self._setv(self.L.clh_set_nuisance_shear_calib, m)
+
+    def set_nuisance_shear_photoz(self, dz):
+        self._setv(self.L.clh_set_nuisance_shear_photoz, dz)
+
+    def set_nuisance_clustering_photoz(self, dz):
+        self._setv(self.L.clh_set_nuisance_clustering_photoz, dz)
+
+    def set_nuisance_clustering_photoz_stretch(self, s):
+        self._setv(self.L.clh_set_nuisance_clustering_photoz_stretch, s)
+
+    def set_nuisance_linear_bias(self, b1):
+        self._setv(self.L.clh_set_nuisance_linear_bias, b1)
+
+    def set_nuisance_magnification_bias(self, bmag):
+        self._setv(self.L.clh_set_nuisance_magnification_bias, bmag)
+
+    def set_nuisance_IA(self, A1):
+        self._setv(self.L.clh_set_nuisance_ia, A1)
+
+    def load(self, prob, legendre_mode=LEGENDRE_FOLDED):
+        """Configure the layout and current draw from a synth.Problem."""
+        self.init_ntable(prob.n_ell, 225, prob.lmax, 0)
+        self.init_binning_real_space(prob.ntheta, prob.theta_min_arcmin, prob.theta_max_arcmin)
+        self.init_IA(0, prob.ia_mode)
+        self.set_legendre_mode(legendre_mode)
+        self.set_source_sample(prob.source)
+        if prob.lens is not None:
+            self.set_lens_sample(prob.lens)
+        self.set_draw(prob)
+        return self
+
+    def set_draw(self, prob):
+        self.set_cosmological_parameters(prob.omega_m, 100.0 * prob.h)
+        self.set_distances(prob.chi_z, prob.chi)
+        self.set_growth(prob.G_z, prob.G)
+        self.set_non_linear_power_spectrum(prob.log10k, prob.pk_z, prob.lnP)
+        self.set_nuisance_shear_photoz(prob.src_dz)
+        self.set_nuisance_shear_calib(prob.shear_m)
+        if prob.lens is not None:
+            self.set_nuisance_clustering_photoz(prob.lens_dz)
+            self.set_nuisance_clustering_photoz_stretch(prob.lens_stretch)
+            self.set_nuisance_linear_bias(prob.b1)
+            self.set_nuisance_magnification_bias(prob.bmag)
+        if prob.ia_mode == 3:
+            self.set_nuisance_IA([prob.ia_A, prob.ia_eta])
+
+    # -------------------------------------------------------------- compute
+    @property
+    def n_ss(self):
+        return self.ns * (self.ns + 1) // 2
+
+    @property
+    def n_gs(self):
+        return self.ns * self.nl
+
+    def datavector_size(self):
+        return self.L.clh_datavector_size(self.h)
+
+    def compute_3x2pt_real(self, ord=(0, 1, 2)):
+        """compute_Mx2pt_N_masked<0,3>(ord) with an all-ones mask."""
+        n = self.datavector_size()
+        out = np.empty(n)
+        o = (ctypes.c_int * 3)(*ord)
+        self._chk(self.L.clh_compute_3x2pt_real(self.h, o, _p(out), n))
+        return out
+
+    def xi_pm(self):
+        xp = np.empty(self.n_ss * self.ntheta)
+        xm = np.empty(self.n_ss * self.ntheta)
+        self._chk(self.L.clh_xi_pm(self.h, _p(xp), _p(xm)))
+        return xp.reshape(self.n_ss, self.ntheta), xm.reshape(self.n_ss, self.ntheta)
+
+    def gammat(self):
+        out = np.empty(self.n_gs * self.ntheta)
+        self._chk(self.L.clh_gammat(self.h, _p(out)))
+        return out.reshape(self.n_gs, self.ntheta)
+
+    def wtheta(self):
+        out = np.empty(self.nl * self.ntheta)
+        self._chk(self.L.clh_wtheta(self.h, _p(out)))
+        return out.reshape(self.nl, self.ntheta)
+
+    def cl_tables(self, n_ell):
+        ss = np.empty((self.n_ss, n_ell))
+        gs = np.empty((max(self.n_gs, 1), n_ell))
+        gg = np.empty((max(self.nl, 1), n_ell))
+        self._chk(self.L.clh_cl_tables(self.h, _p(ss), _p(gs), _p(gg)))
+        return ss, gs[: self.n_gs], gg[: self.nl]
+
+    def cl_nointerp(self, probe, ells):
+        ells = _arr(ells)
+        idx = {"ss": 0, "gs": 1, "gg": 2}[probe]
+        n = {"ss": self.n_ss, "gs": self.n_gs, "gg": self.nl}[probe]
+        out = np.empty((n, len(ells)))
+        self._chk(self.L.clh_cl_nointerp(self.h, idx, _p(ells), len(ells), _p(out)))
+        return out
+
+    # ------------------------------------------------------------- batches
+    def batch_upload(self, problems):
+        """Upload a list of synth.Problem draws (same layout as the context)."""
+        arr = (clh_draw * len(problems))()
+        keep = []
+        for i, p in enumerate(problems):
+            vals = {k: _arr(getattr(p, k)) for k in ("chi_z", "chi", "G_z", "G", "log10k", "pk_z", "lnP",
+                                                     "src_dz", "shear_m")}
+            for k in ("lens_dz", "lens_stretch", "b1", "bmag"):
+                vals[k] = _arr(getattr(p, k)) if p.lens is not None else None
+            keep.append(vals)
+            d = arr[i]
+            d.omega_m = p.omega_m
+            d.chi_z, d.chi, d.chi_n = _p(vals["chi_z"]), _p(vals["chi"]), len(vals["chi_z"])
+            d.G_z, d.G, d.G_n = _p(vals["G_z"]), _p(vals["G"]), len(vals["G_z"])
+            d.log10k, d.pk_z, d.lnP = _p(vals["log10k"]), _p(vals["pk_z"]), _p(vals["lnP"])
+            d.nk, d.npz = len(vals["log10k"]), len(vals["pk_z"])
+            d.src_dz, d.shear_m = _p(vals["src_dz"]), _p(vals["shear_m"])
+            d.lens_dz, d.lens_stretch = _p(vals["lens_dz"]), _p(vals["lens_stretch"])
+            d.b1, d.bmag = _p(vals["b1"]), _p(vals["bmag"])
+            d.ia_A, d.ia_eta = p.ia_A, p.ia_eta
+        self._chk(self.L.clh_batch_upload(self.h, len(problems), arr))
+        self.batch_size = len(problems)
+
+    def batch_forward(self, dv_device_ptr: int, ord=(0, 1, 2)):
+        o = (ctypes.c_int * 3)(*ord)
+        self._chk(self.L.clh_batch_forward(self.h, o, ctypes.c_void_p(dv_device_ptr)))
+
+    def synchronize(self):
+        self._chk(self.L.clh_synchronize(self.h))
+
+    def enable_timing(self, on=True):
+        self._chk(self.L.clh_enable_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        names = (ctypes.c_char_p * 8)()
+        ms = (ctypes.c_float * 8)()
+        n = self.L.clh_last_kernel_times(self.h, names, ms, 8)
+        return {names[i].decode(): float(ms[i]) for i in range(n)}

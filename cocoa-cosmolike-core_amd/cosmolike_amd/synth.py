@@ -1,0 +1,264 @@
+"""Synthetic inputs for the tomographic two-point projection path.
+
+Everything here is plain numpy FP64 and seeded, so the oracle and the HIP path
+see bit-identical inputs.  The shapes follow what Cocoa hands to CosmoLike's
+setters (cosmolike/generic_interface.cpp):
+
+* ``set_distances(z, chi)``                       chi in Mpc/h      (:1085)
+* ``set_growth(z, G)``                            G = D/a           (:1149)
+* ``set_non_linear_power_spectrum(log10k, z, lnP)`` k-major lnP    (:1276, :1343)
+* ``set_source_sample`` / ``set_lens_sample``     histogram tables whose
+  column 0 is z and columns 1..N are n_i(z)      (:1907, :1802)
+
+Cosmology: flat LCDM, P(k,z) = A k^ns T_BBKS(k)^2 D(z)^2 with A fixed by sigma8.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+from scipy import integrate, special
+
+C_OVER_H0 = 2997.92458  # Mpc/h, cosmolike/structs.c:119
+
+
+@dataclasses.dataclass
+class Cosmology:
+    omega_m: float = 0.3
+    h: float = 0.7
+    ns: float = 0.965
+    sigma8: float = 0.8
+
+
+def _E(z, om):
+    return np.sqrt(om * (1.0 + z) ** 3 + (1.0 - om))
+
+
+def background_grid() -> np.ndarray:
+    """z grid for chi(z) and G(z): dense below z=4, sparser up to z=50.
+
+    chi_all reads chi[j+2] (cosmolike/cosmo3D.c:66-67) and the gg RSD term
+    queries chi(a_min) at z=40 (cosmo2D.c:2540, structs.c:41), so the grid has
+    to reach well past z=40.
+    """
+    z1 = np.linspace(0.0, 4.0, 401)
+    z2 = np.geomspace(4.05, 50.0, 120)
+    return np.concatenate([z1, z2])
+
+
+def comoving_distance(z: np.ndarray, om: float) -> np.ndarray:
+    """chi(z) in Mpc/h for flat LCDM (high-accuracy cumulative Simpson)."""
+    zf = np.linspace(0.0, float(z.max()), 200001)
+    inv = 1.0 / _E(zf, om)
+    cum = integrate.cumulative_simpson(inv, x=zf, initial=0.0)
+    return C_OVER_H0 * np.interp(z, zf, cum)
+
+
+def growth_D(z: np.ndarray, om: float) -> np.ndarray:
+    """Linear growth D(z) normalised to D(0)=1: D(a) ~ E(a) int_0^a da'/(a' E(a'))^3
+    (Heath 1977), by cumulative Simpson on a fine grid in a."""
+    af = np.linspace(0.0, 1.0, 400001)
+    e = np.empty_like(af)
+    e[1:] = np.sqrt(om / af[1:] ** 3 + (1.0 - om))
+    e[0] = np.inf
+    integrand = np.zeros_like(af)
+    integrand[1:] = 1.0 / (af[1:] * e[1:]) ** 3
+    cum = integrate.cumulative_simpson(integrand, x=af, initial=0.0)
+    d = np.zeros_like(af)
+    d[1:] = 2.5 * om * e[1:] * cum[1:]
+    a = 1.0 / (1.0 + np.asarray(z, dtype=np.float64))
+    da = np.interp(a, af, d)
+    return da / d[-1]
+
+
+def t_bbks(k_hmpc: np.ndarray, om: float, h: float) -> np.ndarray:
+    q = k_hmpc / (om * h)
+    x = 2.34 * q
+    return (np.log1p(x) / x) * (1.0 + 3.89 * q + (16.1 * q) ** 2 + (5.46 * q) ** 3
+                                + (6.71 * q) ** 4) ** -0.25
+
+
+def sigma8_unnorm(om: float, h: float, ns: float) -> float:
+    lk = np.linspace(-5.0, 3.0, 40001)
+    k = 10.0**lk
+    x = 8.0 * k
+    w = 3.0 * (np.sin(x) - x * np.cos(x)) / x**3
+    pk = k**ns * t_bbks(k, om, h) ** 2
+    integrand = k**3 * pk * w * w / (2.0 * np.pi**2) * np.log(10.0)
+    return math.sqrt(integrate.simpson(integrand, x=lk))
+
+
+def power_table(cosmo: Cosmology, nk: int = 400, nz: int = 100, kmin_log10=-4.0,
+                kmax_log10=2.0, zmax=3.0):
+    """(log10k[nk], z[nz], lnP[nk*nz] k-major) in (Mpc/h)^3, k in h/Mpc."""
+    log10k = np.linspace(kmin_log10, kmax_log10, nk)
+    z = np.linspace(0.0, zmax, nz)
+    k = 10.0**log10k
+    amp = (cosmo.sigma8 / sigma8_unnorm(cosmo.omega_m, cosmo.h, cosmo.ns)) ** 2
+    p0 = amp * k**cosmo.ns * t_bbks(k, cosmo.omega_m, cosmo.h) ** 2
+    d = growth_D(z, cosmo.omega_m)
+    lnP = np.log(p0)[:, None] + 2.0 * np.log(d)[None, :]
+    return log10k, z, np.ascontiguousarray(lnP.reshape(-1))
+
+
+def background(cosmo: Cosmology):
+    z = background_grid()
+    chi = comoving_distance(z, cosmo.omega_m)
+    d = growth_D(z, cosmo.omega_m)
+    G = d * (1.0 + z)  # G = D/a
+    return z, chi, z.copy(), G
+
+
+# ----------------------------------------------------------------------------
+# n(z) histograms (300 rows on [0, 3), dz = 0.01; column 0 = left edge)
+# ----------------------------------------------------------------------------
+
+def _zhisto(nrows=300, zmax=3.0):
+    edges = np.linspace(0.0, zmax, nrows + 1)
+    return edges[:-1], 0.5 * (edges[:-1] + edges[1:])
+
+
+def source_table(nbin: int, z0: float, alpha=2.0, beta=1.5, sigma0=0.05, nrows=300,
+                 zmax=3.0) -> np.ndarray:
+    """Equal-number photo-z bins of n(z) ~ z^alpha exp(-(z/z0)^beta).
+
+    Photo-z scatter is Gaussian with sigma = sigma0 (1+z).  Returns a
+    (nrows, nbin+1) table: z_left, n_1(z), ..., n_nbin(z).
+    """
+    zl, zc = _zhisto(nrows, zmax)
+    zf = np.linspace(0.0, zmax, 6001)
+    nf = zf**alpha * np.exp(-((zf / z0) ** beta))
+    # photo-z distribution of the full sample, for equal-number edges
+    zp = np.linspace(-0.5, zmax + 1.0, 4001)
+    sig = sigma0 * (1.0 + zf)
+    kern = np.exp(-0.5 * ((zp[:, None] - zf[None, :]) / sig[None, :]) ** 2) / sig[None, :]
+    nph = integrate.simpson(kern * nf[None, :], x=zf, axis=1)
+    cph = integrate.cumulative_simpson(nph, x=zp, initial=0.0)
+    cph /= cph[-1]
+    edges = np.interp(np.linspace(0.0, 1.0, nbin + 1), cph, zp)
+    edges[0], edges[-1] = -np.inf, np.inf
+    nzc = zc**alpha * np.exp(-((zc / z0) ** beta))
+    s = sigma0 * (1.0 + zc)
+    out = np.empty((nrows, nbin + 1))
+    out[:, 0] = zl
+    for i in range(nbin):
+        lo = special.erf((edges[i] - zc) / (np.sqrt(2.0) * s)) if np.isfinite(edges[i]) else -1.0
+        hi = special.erf((edges[i + 1] - zc) / (np.sqrt(2.0) * s)) if np.isfinite(edges[i + 1]) else 1.0
+        out[:, i + 1] = nzc * 0.5 * (hi - lo)
+    return out
+
+
+def lens_table(centres, sigma=0.05, nrows=300, zmax=3.0) -> np.ndarray:
+    zl, zc = _zhisto(nrows, zmax)
+    out = np.empty((nrows, len(centres) + 1))
+    out[:, 0] = zl
+    for i, c in enumerate(centres):
+        out[:, i + 1] = np.exp(-0.5 * ((zc - c) / sigma) ** 2)
+    return out
+
+
+# ----------------------------------------------------------------------------
+# Full problem descriptions
+# ----------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class Problem:
+    """Everything one data vector needs (one draw)."""
+    omega_m: float
+    chi_z: np.ndarray
+    chi: np.ndarray
+    G_z: np.ndarray
+    G: np.ndarray
+    log10k: np.ndarray
+    pk_z: np.ndarray
+    lnP: np.ndarray
+    source: np.ndarray           # (nrows, ns+1)
+    lens: np.ndarray | None      # (nrows, nl+1)
+    ntheta: int
+    theta_min_arcmin: float
+    theta_max_arcmin: float
+    n_ell: int = 300
+    lmax: int = 50000
+    src_dz: np.ndarray | None = None
+    lens_dz: np.ndarray | None = None
+    lens_stretch: np.ndarray | None = None
+    b1: np.ndarray | None = None
+    bmag: np.ndarray | None = None
+    shear_m: np.ndarray | None = None
+    ia_A: float = 0.5
+    ia_eta: float = 0.0
+    ia_mode: int = 3             # IA_REDSHIFT_EVOLUTION (cosmolike/IA.h:13)
+    h: float = 0.7
+
+    @property
+    def ns(self):
+        return self.source.shape[1] - 1
+
+    @property
+    def nl(self):
+        return 0 if self.lens is None else self.lens.shape[1] - 1
+
+    def fill_defaults(self):
+        ns, nl = self.ns, max(self.nl, 1)
+        if self.src_dz is None:
+            self.src_dz = np.zeros(ns)
+        if self.shear_m is None:
+            self.shear_m = np.zeros(ns)
+        if self.lens_dz is None:
+            self.lens_dz = np.zeros(nl)
+        if self.lens_stretch is None:
+            self.lens_stretch = np.ones(nl)
+        if self.b1 is None:
+            self.b1 = np.linspace(1.2, 2.0, nl) if nl > 1 else np.array([1.2])
+        if self.bmag is None:
+            self.bmag = np.zeros(nl)
+        return self
+
+
+_BG_CACHE: dict = {}
+
+
+def _cosmo_tables(cosmo: Cosmology):
+    key = (cosmo.omega_m, cosmo.h, cosmo.ns, cosmo.sigma8)
+    if key not in _BG_CACHE:
+        z, chi, gz, G = background(cosmo)
+        log10k, pz, lnP = power_table(cosmo)
+        _BG_CACHE[key] = (z, chi, gz, G, log10k, pz, lnP)
+    return _BG_CACHE[key]
+
+
+def shear_only_problem(cosmo: Cosmology | None = None) -> Problem:
+    """Cosmic shear, 5 source bins, 20 theta bins 2.5'-250', N_ell=300."""
+    cosmo = cosmo or Cosmology()
+    z, chi, gz, G, log10k, pz, lnP = _cosmo_tables(cosmo)
+    src = source_table(5, 0.5)
+    return Problem(cosmo.omega_m, z, chi, gz, G, log10k, pz, lnP, src, None, 20, 2.5,
+                   250.0, n_ell=300, h=cosmo.h).fill_defaults()
+
+
+def full_3x2pt_problem(cosmo: Cosmology | None = None, n_ell: int = 500) -> Problem:
+    """10 Smail source bins (z0=0.6), 5 Gaussian lens bins, 26 theta bins 2.5'-900'."""
+    cosmo = cosmo or Cosmology()
+    z, chi, gz, G, log10k, pz, lnP = _cosmo_tables(cosmo)
+    src = source_table(10, 0.6)
+    lens = lens_table(np.linspace(0.3, 0.9, 5), 0.05)
+    p = Problem(cosmo.omega_m, z, chi, gz, G, log10k, pz, lnP, src, lens, 26, 2.5, 900.0,
+                n_ell=n_ell, h=cosmo.h)
+    p.b1 = np.linspace(1.2, 2.0, 5)
+    return p.fill_defaults()
+
+
+def random_draws(n: int, seed: int = 20261203, ragged: bool = True):
+    """Cosmology/nuisance draws; ragged layouts when ``ragged``."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        c = Cosmology(omega_m=rng.uniform(0.2, 0.4), h=rng.uniform(0.6, 0.8),
+                      sigma8=rng.uniform(0.6, 1.0))
+        if ragged:
+            ns, nl, nt = int(rng.integers(3, 11)), int(rng.integers(2, 7)), int(rng.integers(20, 31))
+        else:
+            ns, nl, nt = 10, 5, 26
+        out.append((c, ns, nl, nt))
+    return out

@@ -1,0 +1,187 @@
+// Device-side restatements of the per-point physics the projection path
+// evaluates: background interpolation (cosmolike/cosmo3D.c), P(k,z) lookup,
+// n(z) splines (cosmolike/redshift_spline.c), lens efficiencies and radial
+// weights (cosmolike/radial_weights.c), NLA amplitude (cosmolike/IA.c).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "clh_internal.hpp"
+
+// ilo/ihi bisection used by every table lookup of cosmo3D.c (e.g. :46-57)
+__device__ __forceinline__ int clh_bs(const double* __restrict__ x, int n, double v) {
+  int ilo = 0, ihi = n - 1;
+  while (ihi > ilo + 1) {
+    const int ll = (ihi + ilo) >> 1;
+    if (x[ll] > v) ihi = ll; else ilo = ll;
+  }
+  return ilo;
+}
+
+// chi_all: cosmo3D.c:40-92 (chi and dchi/da, in c/H0 units).  The table is
+// validated on the host to extend two nodes beyond every queried z.
+__device__ __forceinline__ void clh_chi_all(const DrawDesc& d, double a, double& chi,
+                                            double& dchida) {
+  const double z = 1.0 / a - 1.0;
+  const double* __restrict__ zz = d.chi_z;
+  const double* __restrict__ cc = d.chi_v;
+  int j = clh_bs(zz, d.chi_n, z);
+  if (j > d.chi_n - 3) j = d.chi_n - 3;  // host guarantees this never triggers
+  const double dy = (z - zz[j]) / (zz[j + 1] - zz[j]);
+  const double out0 = cc[j] + dy * (cc[j + 1] - cc[j]);
+  const double up = (cc[j + 2] - cc[j]) / (zz[j + 2] - zz[j]);
+  double down;
+  if (j > 0) down = (cc[j + 1] - cc[j - 1]) / (zz[j + 1] - zz[j - 1]);
+  else down = (cc[j + 1] - cc[j]) / (zz[j + 1] - zz[j]);
+  double out1 = down + dy * (up - down);
+  out1 = out1 / CLH_COVERH0;
+  out1 = out1 / (a * a);
+  chi = out0 / CLH_COVERH0;
+  dchida = out1;
+}
+
+__device__ __forceinline__ double clh_chi(const DrawDesc& d, double a) {
+  double c, dc;
+  clh_chi_all(d, a, c, dc);
+  return c;
+}
+
+// hoverh0v2: cosmo3D.c:104-107
+__device__ __forceinline__ double clh_hoverh0v2(double a, double dchida) {
+  return 1.0 / ((a * a) * dchida);
+}
+
+// a_chi: cosmo3D.c:109-133
+__device__ __forceinline__ double clh_a_chi(const DrawDesc& d, double io_chi) {
+  const double chi = io_chi * CLH_COVERH0;
+  const int j = clh_bs(d.chi_v, d.chi_n, chi);
+  const double dy = (chi - d.chi_v[j]) / (d.chi_v[j + 1] - d.chi_v[j]);
+  const double z = d.chi_z[j] + dy * (d.chi_z[j + 1] - d.chi_z[j]);
+  return 1.0 / (1.0 + z);
+}
+
+// norm_growfac(a, true): cosmo3D.c:148-198 (G(0) precomputed per draw)
+__device__ __forceinline__ double clh_growfac(const DrawDesc& d, double a) {
+  const double z = 1.0 / a - 1.0;
+  const int j = clh_bs(d.G_z, d.G_n, z);
+  const double dy = (z - d.G_z[j]) / (d.G_z[j + 1] - d.G_z[j]);
+  const double G = d.G_v[j] + dy * (d.G_v[j + 1] - d.G_v[j]);
+  return (G * a) / d.G0;
+}
+
+// f_growth: cosmo3D.c:200-228
+__device__ __forceinline__ double clh_f_growth(const DrawDesc& d, double z) {
+  const int j = clh_bs(d.G_z, d.G_n, z);
+  const double dy = (z - d.G_z[j]) / (d.G_z[j + 1] - d.G_z[j]);
+  const double G = d.G_v[j] + dy * (d.G_v[j + 1] - d.G_v[j]);
+  const double dlnGdlnz = ((d.G_v[j + 1] - d.G_v[j]) / (d.G_z[j + 1] - d.G_z[j])) * z / G;
+  const double dlnGdlna = -dlnGdlnz * (1 + z) / z;
+  return 1 + dlnGdlna;
+}
+
+// p_nonlin without baryons: cosmo3D.c:362-417 (bilinear in log10 k, z of ln P;
+// linear extrapolation outside the table)
+__device__ __forceinline__ double clh_p_nonlin(const DrawDesc& d, double k, double a) {
+  const double log10k = log10(k / CLH_COVERH0);
+  const double z = 1.0 / a - 1.0;
+  const int nz = d.nzp;
+  const int i = clh_bs(d.lk, d.nk, log10k);
+  const int j = clh_bs(d.pz, nz, z);
+  const double dx = (log10k - d.lk[i]) / (d.lk[i + 1] - d.lk[i]);
+  const double dy = (z - d.pz[j]) / (d.pz[j + 1] - d.pz[j]);
+  const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
+  const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +
+                     dx * dy * P[nz + 1];
+  return exp(out) / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
+}
+
+// natural cubic spline through histogram centres (gsl cspline evaluation)
+__device__ __forceinline__ double clh_spline(const double* __restrict__ zc,
+                                             const double* __restrict__ coef, int rows, double x) {
+  int i = clh_bs(zc, rows, x);
+  if (i > rows - 2) i = rows - 2;
+  const double* c = coef + 4 * i;
+  const double dx = x - zc[i];
+  return c[0] + dx * (c[1] + dx * (c[2] + dx * c[3]));
+}
+
+// nz_source_photoz evaluation: redshift_spline.c:513-537
+__device__ __forceinline__ double clh_nz_src(const DrawDesc& d, double zz, int b) {
+  zz = zz - d.src_dz[b];
+  const int n = d.s_rows;
+  if (zz < d.s_zc[0] || zz > d.s_zc[n - 1]) return 0.0;
+  return clh_spline(d.s_zc, d.s_coef + (size_t)b * 4 * (n - 1), n, zz);
+}
+
+// nz_lens_photoz evaluation: redshift_spline.c:736-762
+__device__ __forceinline__ double clh_nz_lens(const DrawDesc& d, double zz, int b) {
+  zz = (zz - d.lens_dz[b] - d.l_zmean[b]) / d.lens_st[b] + d.l_zmean[b];
+  const int n = d.l_rows;
+  if (zz <= d.l_zc[0] || zz >= d.l_zc[n - 1]) return 0.0;
+  const double r = clh_spline(d.l_zc, d.l_coef + (size_t)b * 4 * (n - 1), n, zz);
+  return r / d.lens_st[b];
+}
+
+// interpol1d: basics.c:457-480 (constant extrapolation)
+__device__ __forceinline__ double clh_interpol1d(const double* __restrict__ f, int n, double a,
+                                                 double dx, double x) {
+  if (x < a) return f[0];
+  const double r = (x - a) / dx;
+  const int i = (int)floor(r);
+  if (i + 1 >= n) return f[n - 1];
+  return (r - i) * (f[i + 1] - f[i]) + f[i];
+}
+
+// g_tomo evaluation: redshift_spline.c:919-922
+__device__ __forceinline__ double clh_g_tomo(const DrawDesc& d, double a, int b) {
+  const double amin = 1.0 / (d.s_zmax_all + 1.0), amax = 0.999999;
+  const double dac = (amax - amin) / ((double)d.n_a - 1.0);
+  return (a <= amin || a > 1.0 - dac) ? 0.0
+         : clh_interpol1d(d.gtab + (size_t)b * d.n_a, d.n_a, amin, dac, a);
+}
+
+// g_lens evaluation: redshift_spline.c:1106-1108
+__device__ __forceinline__ double clh_g_lens(const DrawDesc& d, double a, int b) {
+  const double amin = 1.0 / (d.l_zmax_all + 1.0), amax = 0.999999;
+  const double dac = (amax - amin) / ((double)d.n_a - 1.0);
+  return (a < amin || a > 1.0 - dac) ? 0.0
+         : clh_interpol1d(d.gtab + (size_t)(d.ns + b) * d.n_a, d.n_a, amin, dac, a);
+}
+
+// IA_A1_Z1Z2 amplitude times Omega_m c1 rho_crit / D: IA.c:238-296
+__device__ __forceinline__ double clh_ia_c1(const DrawDesc& d, double a, double growfac_a, int b) {
+  double A = 0.0;
+  if (d.ia_mode == 2) A = d.ia_bin[b];
+  else if (d.ia_mode == 3) A = d.ia_A * pow((1.0 / a) / 1.62, d.ia_eta);
+  return A * (d.omega_m * 0.01389 / growfac_a);
+}
+
+// W_RSD: radial_weights.c:87-105 (hoverh0 via dchi_dz, f via f_growth)
+__device__ __forceinline__ double clh_w_rsd(const DrawDesc& d, double l, double a0, double a1,
+                                            int ni) {
+  double c0, dc0, c1, dc1;
+  clh_chi_all(d, a0, c0, dc0);
+  clh_chi_all(d, a1, c1, dc1);
+  const double h0 = 1.0 / ((a0 * a0) * dc0);
+  const double h1 = 1.0 / ((a1 * a1) * dc1);
+  double w = (1 + 8. * l) / ((2. * l + 1.) * (2. * l + 1.)) * clh_nz_lens(d, 1. / a0 - 1., ni) * h0 *
+             clh_f_growth(d, 1.0 / a0 - 1.0);
+  w -= 4. / (2 * l + 3.) * sqrt((2 * l + 1.) / (2 * l + 3.)) * clh_nz_lens(d, 1. / a1 - 1., ni) *
+       h1 * clh_f_growth(d, 1.0 / a1 - 1.0);
+  return w;
+}
+
+__device__ __forceinline__ void clh_ss_pair(int ns, int q, int& i, int& j) {
+  i = 0;
+  while (q >= ns - i) { q -= ns - i; i++; }
+  j = i + q;
+}
+
+// amin_lens / amax_lens: redshift_spline.c:50-79
+__device__ __forceinline__ double clh_amin_lens(const DrawDesc& d, int ni) {
+  const double zmax = (d.l_zmax[ni] - d.l_zmean[ni]) * d.lens_st[ni] + d.l_zmean[ni];
+  return 1. / (1 + zmax + 2. * fabs(d.lens_dz[ni]));
+}
+__device__ __forceinline__ double clh_amax_lens(const DrawDesc& d, int ni) {
+  const double zmin = (d.l_zmin[ni] - d.l_zmean[ni]) * d.lens_st[ni] + d.l_zmean[ni];
+  if (d.bmag[ni] != 0) return 1. / (1. + fmax(d.s_zmin_all, 0.001));
+  return 1. / (1 + fmax(zmin - 2. * fabs(d.lens_dz[ni]), 0.001));
+}

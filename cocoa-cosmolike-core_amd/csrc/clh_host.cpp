@@ -1,0 +1,1222 @@
+// Host runtime of the projection library: context, setters, cosmology-
+// independent precomputation (n(z) splines, lens zmean, Gauss-Legendre rules,
+// bin-averaged Legendre kernels and their fold), batch assembly and launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/cosmolike_hip.h"
+#include "clh_internal.hpp"
+
+hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_limber(const DrawDesc* D, const int4* work, int n, hipStream_t s);
+hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow,
+                               hipStream_t s);
+hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, int max_nrow,
+                                    hipStream_t s);
+
+namespace {
+
+using dvec = std::vector<double>;
+
+// ----------------------------------------------------------------- numerics
+// n-point Gauss-Legendre rule on [-1,1], ascending (Newton on P_n); the rule
+// gsl_integration_glfixed tabulates (cosmo2D.c:1583,1651,2226,2641).
+void gauss_legendre(int n, double* x, double* w) {
+  const int m = (n + 1) / 2;
+  for (int i = 0; i < m; i++) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; it++) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; j++) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      const double pp = n * (z * p1 - p2) / (z * z - 1.0);
+      const double z1 = z;
+      z = z1 - p1 / pp;
+      if (std::fabs(z - z1) < 1e-16) break;
+    }
+    double p1 = 1.0, p2 = 0.0;
+    for (int j = 1; j <= n; j++) {
+      const double p3 = p2;
+      p2 = p1;
+      p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+    }
+    const double pp = n * (z * p1 - p2) / (z * z - 1.0);
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+  }
+  if (n & 1) x[n / 2] = 0.0;
+}
+
+int bs_lo(const double* x, int n, double v) {
+  int ilo = 0, ihi = n - 1;
+  while (ihi > ilo + 1) {
+    const int ll = (ihi + ilo) >> 1;
+    if (x[ll] > v) ihi = ll; else ilo = ll;
+  }
+  return ilo;
+}
+
+// natural cubic spline (the gsl cspline the reference picks for
+// photoz_interpolation_type = 0, basics.c:31-44): per-interval (y, b, c, d)
+void natural_spline(int n, const double* x, const double* y, double* coef) {
+  std::vector<double> c(n, 0.0);
+  const int sys = n - 2;
+  if (sys > 0) {
+    std::vector<double> diag(sys), off(sys), g(sys);
+    for (int i = 0; i < sys; i++) {
+      const double hi = x[i + 1] - x[i], hip = x[i + 2] - x[i + 1];
+      off[i] = hip;
+      diag[i] = 2.0 * (hip + hi);
+      g[i] = 3.0 * ((y[i + 2] - y[i + 1]) / hip - (y[i + 1] - y[i]) / hi);
+    }
+    for (int i = 1; i < sys; i++) {
+      const double m = off[i - 1] / diag[i - 1];
+      diag[i] -= m * off[i - 1];
+      g[i] -= m * g[i - 1];
+    }
+    c[sys] = g[sys - 1] / diag[sys - 1];
+    for (int i = sys - 2; i >= 0; i--) c[i + 1] = (g[i] - off[i] * c[i + 2]) / diag[i];
+  }
+  for (int i = 0; i < n - 1; i++) {
+    const double h = x[i + 1] - x[i];
+    coef[4 * i + 0] = y[i];
+    coef[4 * i + 1] = (y[i + 1] - y[i]) / h - h * (c[i + 1] + 2.0 * c[i]) / 3.0;
+    coef[4 * i + 2] = c[i];
+    coef[4 * i + 3] = (c[i + 1] - c[i]) / (3.0 * h);
+  }
+}
+
+double spline_eval(const dvec& zc, const double* coef, double x) {
+  const int n = (int)zc.size();
+  int i = bs_lo(zc.data(), n, x);
+  if (i > n - 2) i = n - 2;
+  const double* c = coef + 4 * i;
+  const double dx = x - zc[i];
+  return c[0] + dx * (c[1] + dx * (c[2] + dx * c[3]));
+}
+
+// -------------------------------------------------------------- structures
+struct Sample {       // one n(z) sample as set_*_sample stores and splines it
+  bool set = false;
+  int nbin = 0, rows = 0;
+  dvec zc, coef;      // spline knots, [bin][rows-1][4]
+  double zmin_all = 0, zmax_all = 0;
+  double zmin[CLH_MAXB] = {0}, zmax[CLH_MAXB] = {0};
+};
+
+struct Layout {
+  Sample src, lens;
+  double zmean[CLH_MAXB] = {0};
+  int ntheta = 0;
+  double vtmin = 0, vtmax = 0;
+  int n_ell = 300, n_a = 225, lmax = 50000, hdi = 0;
+  int ia_model = 0, ia_mode = 0;
+  bool kern_valid = false;
+  dvec kfold;  // [4][nrow][ntheta]
+  dvec kfull;  // [4][lmax][ntheta] (FULL mode only)
+  dvec ell_ss, ell_gx;
+  int nrow() const { return CLH_NLOW + n_ell; }
+  int n_ss() const { return src.nbin * (src.nbin + 1) / 2; }
+  int n_gs() const { return src.nbin * lens.nbin; }
+  int dv_size() const { return ntheta * (2 * n_ss() + n_gs() + lens.nbin); }
+};
+
+struct Draw {  // one draw's cosmology + nuisance (host copy)
+  double omega_m = 0.0;
+  dvec chi_z, chi, G_z, G, lk, pz, lnP;
+  double src_dz[CLH_MAXB] = {0}, lens_dz[CLH_MAXB] = {0}, lens_st[CLH_MAXB],
+         b1[CLH_MAXB] = {0}, bmag[CLH_MAXB] = {0}, shear_m[CLH_MAXB] = {0};
+  double ia_bin[CLH_MAXB] = {0};
+  double ia_A = 0.0, ia_eta = 0.0;
+  Draw() { for (int i = 0; i < CLH_MAXB; i++) lens_st[i] = 1.0; }
+};
+
+template <class T>
+struct DBuf {  // device buffer that grows
+  T* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T));
+    if (e == hipSuccess) cap = n;
+    return e;
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+};
+
+struct Batch {
+  int B = 0;
+  int max_nrow = 0;
+  std::vector<DrawDesc> desc;
+  DBuf<DrawDesc> d_desc;
+  DBuf<double> d_tab, d_work, d_dv, d_ells;
+  DBuf<int2> d_wg, d_wn, d_wl2;
+  DBuf<int4> d_wl;
+  int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0;
+  int dv_size = 0;
+  void release() {
+    d_desc.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
+    d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release();
+  }
+};
+
+}  // namespace
+
+struct clh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Layout L;
+  Draw cur;
+  int legendre_mode = CLH_LEGENDRE_FOLDED;
+  uint64_t version = 1, cached_version = 0;
+  dvec cache_raw;   // raw (uncalibrated, ord 0,1,2) data vector of `cur`
+  dvec cache_cl;    // cl rows of `cur`
+  // device copies of the layout
+  bool dev_layout_valid = false;
+  DBuf<double> d_layout;
+  size_t off_s_zc = 0, off_s_coef = 0, off_l_zc = 0, off_l_coef = 0, off_ell_ss = 0,
+         off_ell_gx = 0, off_kfold = 0, off_gl = 0, off_kfull = 0, off_fill = 0;
+  Batch single, batch, scratch;
+  bool timing = false;
+  hipEvent_t ev[8] = {nullptr};
+  float last_ms[4] = {0, 0, 0, 0};
+};
+
+static int fail(clh_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(c, e_ == hipErrorOutOfMemory ? CLH_ERR_ALLOC : CLH_ERR_HIP, "%s: %s", \
+                  #call, hipGetErrorString(e_));                                         \
+  } while (0)
+
+// first GPU use of a context: device check, stream and timing events
+static int ensure_device(clh_ctx* c) {
+  if (c->stream) return CLH_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(c, CLH_ERR_HIP, "no HIP device");
+  if (c->device >= n) return fail(c, CLH_ERR_ARG, "device %d of %d", c->device, n);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+  return CLH_OK;
+}
+
+static bool has_nan(const double* x, int n) {
+  for (int i = 0; i < n; i++) if (std::isnan(x[i])) return true;
+  return false;
+}
+
+// ------------------------------------------------------------ n(z) samples
+// set_source_sample / set_lens_sample (generic_interface.cpp:1802-1999) and
+// the table build of nz_source_photoz / nz_lens_photoz (redshift_spline.c:446-512,
+// 665-731)
+static int build_sample(clh_ctx* c, Sample& S, const double* table, int nrows, int ntomo,
+                        bool source) {
+  if (!table || ntomo <= 0 || ntomo > CLH_MAXB || nrows < 5)
+    return fail(c, CLH_ERR_ARG, "n(z) sample: ntomo=%d (1..%d), nrows=%d (>=5)", ntomo, CLH_MAXB, nrows);
+  if (has_nan(table, nrows * (ntomo + 1))) return fail(c, CLH_ERR_NAN, "n(z) sample: NaN");
+  dvec z(nrows);
+  std::vector<dvec> col(ntomo, dvec(nrows));
+  for (int i = 0; i < nrows; i++) {
+    z[i] = table[(size_t)i * (ntomo + 1)];
+    for (int k = 0; k < ntomo; k++) col[k][i] = table[(size_t)i * (ntomo + 1) + k + 1];
+  }
+  for (int i = 1; i < nrows; i++)
+    if (!(z[i] > z[i - 1])) return fail(c, CLH_ERR_DOMAIN, "bad n(z) file (z vector not monotonic)");
+  Sample T;
+  T.nbin = ntomo;
+  T.rows = nrows;
+  T.zmin_all = std::fmax(z[0], 1.e-5);
+  T.zmax_all = z[nrows - 1] + (z[nrows - 1] - z[0]) / ((double)nrows - 1.);
+  for (int k = 0; k < ntomo; k++) {
+    double mx = col[k][0];
+    for (int i = 1; i < nrows; i++) mx = std::max(mx, col[k][i]);
+    int first = -1, last = -1;
+    for (int i = 0; i < nrows; i++)
+      if (col[k][i] > 0.999e-8 * mx) { if (first < 0) first = i; last = i; }
+    if (first < 0) return fail(c, CLH_ERR_DOMAIN, "zero/negative n(z) in bin %d", k);
+    T.zmin[k] = source ? std::fmax(z[first], 1.001e-5) : z[first];
+    T.zmax[k] = z[last];
+  }
+  if (source && (T.zmax_all < T.zmax[ntomo - 1] || T.zmin_all > T.zmin[0]))
+    return fail(c, CLH_ERR_DOMAIN, "source n(z) range inconsistent");
+  // histogram re-binned onto centres, normalised per bin
+  auto histo = [&](double zz, int ni) -> double {
+    if (zz >= T.zmin_all && zz < T.zmax_all) {
+      const double dzh = (z[nrows - 1] - z[0]) / ((double)nrows - 1.);
+      const int nj = (int)std::floor((zz - z[0]) / dzh);
+      if (nj < 0 || nj > nrows - 1) return 0.0;
+      return col[ni][nj];
+    }
+    return 0.0;
+  };
+  const double dz = (T.zmax_all - T.zmin_all) / ((double)nrows);
+  T.zc.resize(nrows);
+  for (int k = 0; k < nrows; k++) T.zc[k] = T.zmin_all + (k + 0.5) * dz;
+  T.coef.assign((size_t)ntomo * 4 * (nrows - 1), 0.0);
+  dvec y(nrows);
+  for (int i = 0; i < ntomo; i++) {
+    double NORM = 0.0;
+    for (int k = 0; k < nrows; k++) NORM += histo(T.zc[k], i) * dz;
+    if (!(NORM > 0.0)) return fail(c, CLH_ERR_DOMAIN, "zero/negative n(z) normalization for bin %d", i);
+    for (int k = 0; k < nrows; k++) y[k] = histo(T.zc[k], i) / NORM;
+    natural_spline(nrows, T.zc.data(), y.data(), T.coef.data() + (size_t)i * 4 * (nrows - 1));
+  }
+  T.set = true;
+  S = std::move(T);
+  return CLH_OK;
+}
+
+// zmean (redshift_spline.c:794-844) as set_lens_sample evaluates it
+// (generic_interface.cpp:1874-1876): zero shift, unit stretch, 256-point GL
+static void lens_zmean(Layout& L) {
+  const int n = 256;
+  std::vector<double> x(n), w(n);
+  gauss_legendre(n, x.data(), w.data());
+  const int rows = L.lens.rows;
+  for (int b = 0; b < L.lens.nbin; b++) {
+    const double lo = L.lens.zmin[b], hi = L.lens.zmax[b];
+    const double A = 0.5 * (hi - lo), B = 0.5 * (hi + lo);
+    double num = 0.0, den = 0.0;
+    const double* coef = L.lens.coef.data() + (size_t)b * 4 * (rows - 1);
+    for (int p = 0; p < n; p++) {
+      const double zz = B + A * x[p];
+      double v = 0.0;
+      if (!(zz <= L.lens.zc[0] || zz >= L.lens.zc[rows - 1])) v = spline_eval(L.lens.zc, coef, zz);
+      num += A * w[p] * zz * v;
+      den += A * w[p] * v;
+    }
+    L.zmean[b] = num / den;
+  }
+}
+
+// ------------------------------------------------------- Legendre kernels
+// set_bin_average (basics.c:382-455: theta edges and P_l, P'_l at both edges;
+// P'_l from (1-x^2) P'_l = l (P_{l-1} - x P_l)) and the bin-averaged kernels of
+// xi_pm_tomo (cosmo2D.c:430-453), w_gammat_tomo (:631-639) and w_gg_tomo
+// (:808-815); then the fold of the linear-in-ln(ell) fill (cosmo2D.c:1800-1807)
+// into the kernels.
+static void build_kernels(Layout& L, bool full) {
+  const int nt = L.ntheta, Lm = L.lmax, nrow = L.nrow();
+  L.kfold.assign((size_t)4 * nrow * nt, 0.0);
+  if (full) L.kfull.assign((size_t)4 * Lm * nt, 0.0); else L.kfull.clear();
+  double lim_ss[3], lim_gx[3];
+  lim_ss[0] = std::log(std::fmax(CLH_LMIN_TAB - 1., 1.0));
+  lim_gx[0] = std::log(std::fmax((double)CLH_LMIN_TAB, 1.0));
+  lim_ss[1] = lim_gx[1] = std::log((double)(Lm + 1));
+  lim_ss[2] = (lim_ss[1] - lim_ss[0]) / ((double)L.n_ell - 1.);
+  lim_gx[2] = (lim_gx[1] - lim_gx[0]) / ((double)L.n_ell - 1.);
+  const double logdt = (std::log(L.vtmax) - std::log(L.vtmin)) / nt;
+  auto work = [&](int i) {
+    const double xmin = std::cos(std::exp(std::log(L.vtmin) + (i + 0.) * logdt));
+    const double xmax = std::cos(std::exp(std::log(L.vtmin) + (i + 1.) * logdt));
+    dvec Pmin(Lm + 1), Pmax(Lm + 1), dPmin(Lm + 1), dPmax(Lm + 1);
+    auto leg = [&](double x, dvec& P, dvec& dP) {
+      P[0] = 1.0;
+      P[1] = x;
+      double pm2 = 1.0, pm1 = x;
+      for (int l = 2; l <= Lm; l++) {
+        const double p = (x * (2 * l - 1) * pm1 - (l - 1) * pm2) / l;
+        pm2 = pm1;
+        pm1 = p;
+        P[l] = p;
+      }
+      dP[0] = 0.0;
+      dP[1] = 1.0;
+      const double da = 1.0 + x, db = 1.0 - x;
+      for (int l = 2; l <= Lm; l++) dP[l] = -l * (x * P[l] - P[l - 1]) / (da * db);
+    };
+    leg(xmin, Pmin, dPmin);
+    leg(xmax, Pmax, dPmax);
+    for (int l = 1; l < Lm; l++) {
+      const double dx = xmin - xmax;
+      const double common =
+          -l * (l - 1.) / 2 * (l + 2. / (2 * l + 1)) * (Pmin[l - 1] - Pmax[l - 1]) -
+          l * (l - 1.) * (2. - l) / 2 * (xmin * Pmin[l] - xmax * Pmax[l]) +
+          l * (l - 1.) / (2. * l + 1) * (Pmin[l + 1] - Pmax[l + 1]) + (4 - l) * (dPmin[l] - dPmax[l]) +
+          (l + 2) * (xmin * dPmin[l - 1] - xmax * dPmax[l - 1] - Pmin[l - 1] + Pmax[l - 1]);
+      const double t1 = 2 * (l - 1) * (xmin * dPmin[l] - xmax * dPmax[l] - Pmin[l] + Pmax[l]);
+      const double t2 = 2 * (l + 2) * (dPmin[l - 1] - dPmax[l - 1]);
+      const double pre = (2. * l + 1) / (2. * M_PI * l * l * (l + 1) * (l + 1));
+      double g[4];
+      g[0] = pre * (common + t1 - t2) / dx;
+      g[1] = pre * (common - t1 + t2) / dx;
+      g[2] = (2. * l + 1) / (4. * M_PI * l * (l + 1) * dx) *
+             ((l + 2. / (2 * l + 1.)) * (Pmin[l - 1] - Pmax[l - 1]) +
+              (2 - l) * (xmin * Pmin[l] - xmax * Pmax[l]) - 2. / (2 * l + 1.) * (Pmin[l + 1] - Pmax[l + 1]));
+      g[3] = (1.0 / dx) * (1. / (4.0 * M_PI)) * (Pmin[l + 1] - Pmax[l + 1] - Pmin[l - 1] + Pmax[l - 1]);
+      for (int k = 0; k < 4; k++) {
+        if (full) L.kfull[((size_t)k * Lm + l) * nt + i] = g[k];
+        double* kf = L.kfold.data() + (size_t)k * nrow * nt;
+        if (l < CLH_LMIN_TAB) {
+          kf[(size_t)(l - 1) * nt + i] += g[k];
+        } else {
+          const double* lim = (k <= 1) ? lim_ss : lim_gx;
+          const double r = (std::log((double)l) - lim[0]) * (1.0 / lim[2]);
+          const int ii = (int)std::floor(r);
+          const int ic = ii < 0 ? 0 : (ii >= L.n_ell - 1 ? L.n_ell - 2 : ii);
+          const double t = r - ic;
+          kf[(size_t)(CLH_NLOW + ic) * nt + i] += g[k] * (1.0 - t);
+          kf[(size_t)(CLH_NLOW + ic + 1) * nt + i] += g[k] * t;
+        }
+      }
+    }
+  };
+  const int nthr = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 16u));
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthr; t++)
+    th.emplace_back([&, t]() { for (int i = t; i < nt; i += nthr) work(i); });
+  for (auto& x : th) x.join();
+  // the rows' own ell values
+  L.ell_ss.resize(nrow);
+  L.ell_gx.resize(nrow);
+  for (int r = 0; r < CLH_NLOW; r++) L.ell_ss[r] = L.ell_gx[r] = (double)(r + 1);
+  for (int i = 0; i < L.n_ell; i++) {
+    L.ell_ss[CLH_NLOW + i] = std::exp(lim_ss[0] + i * lim_ss[2]);
+    L.ell_gx[CLH_NLOW + i] = std::exp(lim_gx[0] + i * lim_gx[2]);
+  }
+  L.kern_valid = true;
+}
+
+// ------------------------------------------------------------- validation
+static void host_chi_all(const Draw& d, double a, double& chi, double& dchida) {
+  const double z = 1.0 / a - 1.0;
+  const int n = (int)d.chi_z.size();
+  int j = bs_lo(d.chi_z.data(), n, z);
+  if (j > n - 3) j = n - 3;
+  const double* zz = d.chi_z.data();
+  const double* cc = d.chi.data();
+  const double dy = (z - zz[j]) / (zz[j + 1] - zz[j]);
+  const double out0 = cc[j] + dy * (cc[j + 1] - cc[j]);
+  const double up = (cc[j + 2] - cc[j]) / (zz[j + 2] - zz[j]);
+  const double down = (j > 0) ? (cc[j + 1] - cc[j - 1]) / (zz[j + 1] - zz[j - 1])
+                              : (cc[j + 1] - cc[j]) / (zz[j + 1] - zz[j]);
+  chi = out0 / CLH_COVERH0;
+  dchida = (down + dy * (up - down)) / CLH_COVERH0 / (a * a);
+}
+
+static int check_draw(clh_ctx* c, const Draw& d, int idx) {
+  const Layout& L = c->L;
+  const int nc = (int)d.chi_z.size();
+  if (nc < 5 || d.chi.size() != (size_t)nc) return fail(c, CLH_ERR_STATE, "draw %d: distances not set", idx);
+  if (d.G_z.size() < 2 || d.G.size() != d.G_z.size()) return fail(c, CLH_ERR_STATE, "draw %d: growth not set", idx);
+  if (d.lk.size() < 2 || d.pz.size() < 2 || d.lnP.size() != d.lk.size() * d.pz.size())
+    return fail(c, CLH_ERR_STATE, "draw %d: P(k,z) not set", idx);
+  if (!(d.omega_m > 0)) return fail(c, CLH_ERR_STATE, "draw %d: Omega_m not set", idx);
+  for (int i = 1; i < nc; i++)
+    if (!(d.chi_z[i] > d.chi_z[i - 1]) || !(d.chi[i] > d.chi[i - 1]))
+      return fail(c, CLH_ERR_DOMAIN, "draw %d: chi(z) table not strictly increasing", idx);
+  for (size_t i = 1; i < d.G_z.size(); i++)
+    if (!(d.G_z[i] > d.G_z[i - 1])) return fail(c, CLH_ERR_DOMAIN, "draw %d: growth z not increasing", idx);
+  for (size_t i = 1; i < d.lk.size(); i++)
+    if (!(d.lk[i] > d.lk[i - 1])) return fail(c, CLH_ERR_DOMAIN, "draw %d: log10k not increasing", idx);
+  for (size_t i = 1; i < d.pz.size(); i++)
+    if (!(d.pz[i] > d.pz[i - 1])) return fail(c, CLH_ERR_DOMAIN, "draw %d: P(k,z) z not increasing", idx);
+  // chi_all reads chi[j+2] (cosmo3D.c:66): the largest queried z must lie below
+  // the table's next-to-last node.  gg queries chi(a_min) at z = 40.
+  double zq = L.src.zmax_all;
+  if (L.lens.nbin > 0) zq = std::max(zq, 1.0 / CLH_A_MIN - 1.0);
+  if (!(d.chi_z[nc - 2] > zq))
+    return fail(c, CLH_ERR_DOMAIN, "draw %d: chi(z) table must extend two nodes beyond z=%g", idx, zq);
+  for (int b = 0; b < L.lens.nbin; b++) {
+    if (!(d.lens_st[b] != 0.0)) return fail(c, CLH_ERR_DOMAIN, "draw %d: zero lens stretch", idx);
+    const double zmax = (L.lens.zmax[b] - L.zmean[b]) * d.lens_st[b] + L.zmean[b];
+    const double amin = 1. / (1 + zmax + 2. * std::fabs(d.lens_dz[b]));
+    const double zmin = (L.lens.zmin[b] - L.zmean[b]) * d.lens_st[b] + L.zmean[b];
+    const double amax = (d.bmag[b] != 0) ? 1. / (1. + std::fmax(L.src.zmin_all, 0.001))
+                                         : 1. / (1 + std::fmax(zmin - 2. * std::fabs(d.lens_dz[b]), 0.001));
+    if (!(amin > 0) || !(amin < 1) || !(amax > 0) || !(amax < 1) || !(amin < amax))
+      return fail(c, CLH_ERR_DOMAIN, "draw %d lens bin %d: 0 < amin < amax < 1 not true", idx, b);
+  }
+  return CLH_OK;
+}
+
+// ----------------------------------------------------------- device layout
+static int upload_layout(clh_ctx* c) {
+  Layout& L = c->L;
+  const bool full = c->legendre_mode == CLH_LEGENDRE_FULL;
+  if (!L.kern_valid || (full && L.kfull.empty())) build_kernels(L, full);
+  double glbuf[2 * CLH_NP_SS + 2 * CLH_NP_GG];
+  gauss_legendre(CLH_NP_SS, glbuf, glbuf + CLH_NP_SS);
+  gauss_legendre(CLH_NP_GG, glbuf + 2 * CLH_NP_SS, glbuf + 2 * CLH_NP_SS + CLH_NP_GG);
+  size_t off = 0;
+  auto take = [&](size_t n) { size_t o = off; off += (n + 31) & ~size_t(31); return o; };
+  c->off_s_zc = take(L.src.zc.size());
+  c->off_s_coef = take(L.src.coef.size());
+  c->off_l_zc = take(L.lens.zc.size());
+  c->off_l_coef = take(L.lens.coef.size());
+  c->off_ell_ss = take(L.ell_ss.size());
+  c->off_ell_gx = take(L.ell_gx.size());
+  c->off_kfold = take(L.kfold.size());
+  c->off_gl = take(sizeof glbuf / sizeof(double));
+  c->off_kfull = take(full ? L.kfull.size() : 0);
+  // FULL mode: index and weight of the ln-ell fill at every integer ell
+  // (C_*_tomo_limber_fill, cosmo2D.c:1800-1807)
+  dvec fill;
+  if (full) {
+    fill.assign((size_t)4 * L.lmax, 0.0);
+    for (int g = 0; g < 2; g++) {
+      double lim0 = std::log(std::fmax(g == 0 ? CLH_LMIN_TAB - 1. : (double)CLH_LMIN_TAB, 1.0));
+      const double lim1 = std::log((double)(L.lmax + 1));
+      const double lim2 = (lim1 - lim0) / ((double)L.n_ell - 1.);
+      const double inv_dx = 1.0 / lim2;
+      for (int l = CLH_LMIN_TAB; l < L.lmax; l++) {
+        const double r = (std::log((double)l) - lim0) * inv_dx;
+        const int i = (int)std::floor(r);
+        const int ic = i < 0 ? 0 : (i >= L.n_ell - 1 ? L.n_ell - 2 : i);
+        fill[((size_t)g * L.lmax + l) * 2 + 0] = (double)ic;
+        fill[((size_t)g * L.lmax + l) * 2 + 1] = r - ic;
+      }
+    }
+  }
+  c->off_fill = take(fill.size());
+  HIPCHK(c, c->d_layout.ensure(off));
+  double* base = c->d_layout.p;
+  auto put = [&](size_t o, const double* src, size_t n) -> hipError_t {
+    if (n == 0) return hipSuccess;
+    return hipMemcpyAsync(base + o, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  };
+  HIPCHK(c, put(c->off_s_zc, L.src.zc.data(), L.src.zc.size()));
+  HIPCHK(c, put(c->off_s_coef, L.src.coef.data(), L.src.coef.size()));
+  HIPCHK(c, put(c->off_l_zc, L.lens.zc.data(), L.lens.zc.size()));
+  HIPCHK(c, put(c->off_l_coef, L.lens.coef.data(), L.lens.coef.size()));
+  HIPCHK(c, put(c->off_ell_ss, L.ell_ss.data(), L.ell_ss.size()));
+  HIPCHK(c, put(c->off_ell_gx, L.ell_gx.data(), L.ell_gx.size()));
+  HIPCHK(c, put(c->off_kfold, L.kfold.data(), L.kfold.size()));
+  HIPCHK(c, put(c->off_gl, glbuf, sizeof glbuf / sizeof(double)));
+  if (full) HIPCHK(c, put(c->off_kfull, L.kfull.data(), L.kfull.size()));
+  if (full) HIPCHK(c, put(c->off_fill, fill.data(), fill.size()));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->dev_layout_valid = true;
+  return CLH_OK;
+}
+
+static int check_layout(clh_ctx* c) {
+  const Layout& L = c->L;
+  if (!L.src.set) return fail(c, CLH_ERR_STATE, "source sample not set");
+  if (L.ntheta <= 0) return fail(c, CLH_ERR_STATE, "Ntable.Ntheta not initialized");
+  if (L.ntheta > 32) return fail(c, CLH_ERR_ARG, "Ntheta=%d > 32 not supported", L.ntheta);
+  if (L.n_a - 1 > 1024) return fail(c, CLH_ERR_ARG, "N_a=%d > 1025 not supported", L.n_a);
+  return CLH_OK;
+}
+
+// Builds the per-draw descriptors, device tables and work lists of a batch.
+// `ells`: if non-null, nrow custom ell rows shared by all probes (the direct
+// *_nointerp evaluation); otherwise the layout's rows.
+static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& draws, int apply_calib,
+                       const int ord[3], const dvec* ells, double* dv_out) {
+  int rc = check_layout(c);
+  if (rc) return rc;
+  if ((rc = ensure_device(c))) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  const Layout& L = c->L;
+  const int B = (int)draws.size();
+  for (int i = 0; i < B; i++)
+    if ((rc = check_draw(c, *draws[i], i))) return rc;
+  if (!c->dev_layout_valid && (rc = upload_layout(c))) return rc;
+  const int ns = L.src.nbin, nl = L.lens.nbin;
+  const int nrow = ells ? (int)ells->size() : L.nrow();
+  const int n_ss = L.n_ss(), n_gs = L.n_gs();
+  const int ncol = n_ss + n_gs + nl;
+  const int nt = L.ntheta;
+  // data-vector block starts: compute_data_vector_Mx2pt_N_starts (generic_interface.hpp:770-791)
+  const int sizes[3] = {2 * nt * n_ss, nt * n_gs, nt * nl};
+  int idx[3] = {0, 1, 2};
+  for (int i = 1; i < 3; i++)
+    for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) std::swap(idx[j], idx[j - 1]);
+  int start[3] = {0, 0, 0};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+  const int dvsz = sizes[0] + sizes[1] + sizes[2];
+  // node layout
+  const int node_ss = 0;
+  const int ss_sz = CLH_NP_SS * (3 + ns);
+  const int gs_stride = CLH_NP_SS * (5 + ns);
+  const int gg_stride = CLH_NP_GG * 5;
+  const int node_gs = ss_sz;
+  const int node_gg = node_gs + nl * gs_stride;
+  const int node_sz = node_gg + nl * gg_stride;
+  const size_t gtab_sz = (size_t)(ns + nl) * L.n_a;
+  const size_t cl_sz = (size_t)ncol * nrow;
+  const size_t work_per = ((gtab_sz + 31) & ~size_t(31)) + ((node_sz + 31) & ~size_t(31)) + ((cl_sz + 31) & ~size_t(31));
+  // per-draw tables
+  std::vector<size_t> toff(B);
+  size_t ttot = 0;
+  for (int i = 0; i < B; i++) {
+    const Draw& d = *draws[i];
+    toff[i] = ttot;
+    ttot += 2 * d.chi_z.size() + 2 * d.G_z.size() + d.lk.size() + d.pz.size() + d.lnP.size() + 32;
+  }
+  HIPCHK(c, bt.d_tab.ensure(ttot));
+  HIPCHK(c, bt.d_work.ensure(work_per * B));
+  if (!dv_out) HIPCHK(c, bt.d_dv.ensure((size_t)dvsz * B));
+  if (ells) {
+    HIPCHK(c, bt.d_ells.ensure(nrow));
+    HIPCHK(c, hipMemcpyAsync(bt.d_ells.p, ells->data(), nrow * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  std::vector<double> htab(ttot);
+  bt.desc.assign(B, DrawDesc());
+  const double* lay = c->d_layout.p;
+  for (int i = 0; i < B; i++) {
+    const Draw& d = *draws[i];
+    DrawDesc& q = bt.desc[i];
+    std::memset(&q, 0, sizeof q);
+    size_t o = toff[i];
+    auto push = [&](const dvec& v) { const size_t oo = o; std::copy(v.begin(), v.end(), htab.begin() + o); o += v.size(); return bt.d_tab.p + oo; };
+    q.chi_z = push(d.chi_z); q.chi_v = push(d.chi);
+    q.G_z = push(d.G_z); q.G_v = push(d.G);
+    q.lk = push(d.lk); q.pz = push(d.pz); q.lnP = push(d.lnP);
+    q.chi_n = (int)d.chi_z.size(); q.G_n = (int)d.G_z.size();
+    q.nk = (int)d.lk.size(); q.nzp = (int)d.pz.size();
+    q.omega_m = d.omega_m;
+    q.ns = ns; q.nl = nl; q.ntheta = nt; q.n_ell = L.n_ell; q.n_a = L.n_a; q.nrow = nrow;
+    q.n_ss = n_ss; q.n_gs = n_gs; q.lmax = L.lmax;
+    q.s_zc = lay + c->off_s_zc; q.s_coef = lay + c->off_s_coef; q.s_rows = L.src.rows;
+    q.l_zc = lay + c->off_l_zc; q.l_coef = lay + c->off_l_coef; q.l_rows = L.lens.rows;
+    q.s_zmin_all = L.src.zmin_all; q.s_zmax_all = L.src.zmax_all;
+    q.l_zmin_all = L.lens.zmin_all; q.l_zmax_all = L.lens.zmax_all;
+    for (int b = 0; b < CLH_MAXB; b++) {
+      q.l_zmin[b] = L.lens.zmin[b]; q.l_zmax[b] = L.lens.zmax[b]; q.l_zmean[b] = L.zmean[b];
+      q.src_dz[b] = d.src_dz[b]; q.lens_dz[b] = d.lens_dz[b]; q.lens_st[b] = d.lens_st[b];
+      q.b1[b] = d.b1[b]; q.bmag[b] = d.bmag[b]; q.shear_m[b] = d.shear_m[b]; q.ia_bin[b] = d.ia_bin[b];
+    }
+    q.ia_A = d.ia_A; q.ia_eta = d.ia_eta; q.ia_mode = L.ia_mode;
+    q.ell_ss = ells ? bt.d_ells.p : lay + c->off_ell_ss;
+    q.ell_gx = ells ? bt.d_ells.p : lay + c->off_ell_gx;
+    q.kfold = lay + c->off_kfold;
+    q.kfull = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + c->off_kfull : nullptr;
+    q.fill = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + c->off_fill : nullptr;
+    q.gl = lay + c->off_gl;
+    double* wb = bt.d_work.p + work_per * i;
+    q.gtab = wb;
+    q.nodes = wb + ((gtab_sz + 31) & ~size_t(31));
+    q.cl = q.nodes + ((node_sz + 31) & ~size_t(31));
+    q.dv = dv_out ? dv_out + (size_t)dvsz * i : bt.d_dv.p + (size_t)dvsz * i;
+    for (int k = 0; k < 3; k++) q.dv_start[k] = start[k];
+    q.node_ss = node_ss; q.node_gs = node_gs; q.node_gg = node_gg;
+    q.node_gs_stride = gs_stride; q.node_gg_stride = gg_stride;
+    double chi_amin, dd;
+    host_chi_all(d, CLH_A_MIN, chi_amin, dd);
+    q.chi_amin = chi_amin;
+    {  // G(z=0) exactly as norm_growfac interpolates it (cosmo3D.c:151-167)
+      const int j = bs_lo(d.G_z.data(), (int)d.G_z.size(), 0.0);
+      const double dy = (0.0 - d.G_z[j]) / (d.G_z[j + 1] - d.G_z[j]);
+      q.G0 = d.G[j] + dy * (d.G[j + 1] - d.G[j]);
+    }
+    q.apply_calib = apply_calib;
+  }
+  HIPCHK(c, hipMemcpyAsync(bt.d_tab.p, htab.data(), ttot * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, bt.d_desc.ensure(B));
+  HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
+  // work lists
+  std::vector<int2> wg, wn, wl2;
+  std::vector<int4> wl;
+  for (int i = 0; i < B; i++) {
+    for (int b = 0; b < ns + nl; b++) wg.push_back(make_int2(i, b));
+    for (int s = 0; s <= 2 * nl; s++) wn.push_back(make_int2(i, s));
+  }
+  for (int i = 0; i < B; i++)  // gg tiles first: the heaviest per block
+    for (int zl = 0; zl < nl; zl++)
+      for (int r = 0; r < nrow; r += CLH_GG_ROWS) wl.push_back(make_int4(i, CLH_W_GG, zl, r));
+  for (int i = 0; i < B; i++) {
+    for (int zl = 0; zl < nl; zl++)
+      for (int r = 0; r < nrow; r += CLH_SS_ROWS) wl.push_back(make_int4(i, CLH_W_GS, zl, r));
+    for (int r = 0; r < nrow; r += CLH_SS_ROWS) wl.push_back(make_int4(i, CLH_W_SS, 0, r));
+  }
+  const int ntask = 2 * n_ss + n_gs + nl;
+  for (int i = 0; i < B; i++)
+    for (int t = 0; t < ntask; t += 8) wl2.push_back(make_int2(i, t));
+  HIPCHK(c, bt.d_wg.ensure(wg.size()));
+  HIPCHK(c, bt.d_wn.ensure(wn.size()));
+  HIPCHK(c, bt.d_wl.ensure(wl.size()));
+  HIPCHK(c, bt.d_wl2.ensure(wl2.size()));
+  HIPCHK(c, hipMemcpyAsync(bt.d_wg.p, wg.data(), wg.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(bt.d_wn.p, wn.data(), wn.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(bt.d_wl.p, wl.data(), wl.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(bt.d_wl2.p, wl2.data(), wl2.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+  bt.n_wg = (int)wg.size(); bt.n_wn = (int)wn.size(); bt.n_wl = (int)wl.size(); bt.n_wl2 = (int)wl2.size();
+  bt.B = B;
+  bt.max_nrow = nrow;
+  bt.dv_size = dvsz;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLH_OK;
+}
+
+static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
+  hipStream_t s = c->stream;
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], s));
+  HIPCHK(c, clh_launch_gtab(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], s));
+  HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], s));
+  HIPCHK(c, clh_launch_limber(bt.d_desc.p, bt.d_wl.p, bt.n_wl, s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], s));
+  if (legendre) {
+    if (c->legendre_mode == CLH_LEGENDRE_FULL)
+      HIPCHK(c, clh_launch_legendre_full(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s));
+    else
+      HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s));
+  }
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
+  return CLH_OK;
+}
+
+// single-draw compute of the current state, cached by version
+static int ensure_current(clh_ctx* c) {
+  if (c->cached_version == c->version) return CLH_OK;
+  const int ord[3] = {0, 1, 2};
+  std::vector<const Draw*> v{&c->cur};
+  int rc = build_batch(c, c->single, v, 0, ord, nullptr, nullptr);
+  if (rc) return rc;
+  if ((rc = run_batch(c, c->single, true))) return rc;
+  c->cache_raw.resize(c->single.dv_size);
+  const size_t ncl = (size_t)(c->L.n_ss() + c->L.n_gs() + c->L.lens.nbin) * c->L.nrow();
+  c->cache_cl.resize(ncl);
+  HIPCHK(c, hipMemcpyAsync(c->cache_raw.data(), c->single.desc[0].dv, c->cache_raw.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->cache_cl.data(), c->single.desc[0].cl, ncl * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->cached_version = c->version;
+  return CLH_OK;
+}
+
+// ============================================================== C ABI
+extern "C" {
+
+int clh_create(clh_ctx** out, int dev) {
+  if (!out) return CLH_ERR_ARG;
+  *out = nullptr;
+  if (dev < 0) return CLH_ERR_ARG;
+  clh_ctx* c = new (std::nothrow) clh_ctx();
+  if (!c) return CLH_ERR_ALLOC;
+  c->device = dev;  // the device, stream and events are set up on first use
+  *out = c;
+  return CLH_OK;
+}
+
+void clh_destroy(clh_ctx* c) {
+  if (!c) return;
+  if (c->stream) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+  }
+  c->single.release(); c->batch.release(); c->scratch.release();
+  c->d_layout.release();
+  for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* clh_last_error(const clh_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int clh_set_legendre_mode(clh_ctx* c, int mode) {
+  if (!c) return CLH_ERR_ARG;
+  if (mode != CLH_LEGENDRE_FOLDED && mode != CLH_LEGENDRE_FULL) return fail(c, CLH_ERR_ARG, "bad mode %d", mode);
+  if (mode != c->legendre_mode) {
+    c->legendre_mode = mode;
+    c->dev_layout_valid = false;
+    c->version++;
+  }
+  return CLH_OK;
+}
+
+int clh_init_ntable(clh_ctx* c, int n_ell, int n_a, int lmax, int hdi) {
+  if (!c) return CLH_ERR_ARG;
+  if (n_ell < 2 || n_a < 2 || n_a > 1025 || lmax < CLH_LMIN_TAB + 2)
+    return fail(c, CLH_ERR_ARG, "bad Ntable (N_ell=%d, N_a=%d, LMAX=%d)", n_ell, n_a, lmax);
+  if (hdi != 0) return fail(c, CLH_ERR_ARG, "high_def_integration=%d not implemented (0 only)", hdi);
+  c->L.n_ell = n_ell; c->L.n_a = n_a; c->L.lmax = lmax; c->L.hdi = hdi;
+  c->L.kern_valid = false;
+  c->dev_layout_valid = false;
+  c->version++;
+  return CLH_OK;
+}
+
+int clh_init_binning_real_space(clh_ctx* c, int ntheta, double tmin, double tmax) {
+  if (!c) return CLH_ERR_ARG;
+  if (!(ntheta > 0) || ntheta > 32 || !(tmin > 0) || !(tmax > tmin))
+    return fail(c, CLH_ERR_ARG, "bad theta binning (Ntheta=%d in 1..32, %g, %g)", ntheta, tmin, tmax);
+  c->L.ntheta = ntheta;
+  c->L.vtmin = tmin * 2.90888208665721580e-4;
+  c->L.vtmax = tmax * 2.90888208665721580e-4;
+  c->L.kern_valid = false;
+  c->dev_layout_valid = false;
+  c->version++;
+  return CLH_OK;
+}
+
+int clh_init_ia(clh_ctx* c, int model, int evol) {
+  if (!c) return CLH_ERR_ARG;
+  if (model != 0) return fail(c, CLH_ERR_ARG, "IA_MODEL=%d not implemented (NLA=0 only)", model);
+  if (evol != 0 && evol != 2 && evol != 3) return fail(c, CLH_ERR_ARG, "IA redshift evolution %d not supported", evol);
+  c->L.ia_model = model;
+  c->L.ia_mode = evol;
+  c->version++;
+  return CLH_OK;
+}
+
+int clh_set_cosmological_parameters(clh_ctx* c, double om, double hubble) {
+  if (!c) return CLH_ERR_ARG;
+  if (std::isnan(om) || std::isnan(hubble)) return fail(c, CLH_ERR_NAN, "NaN cosmological parameter");
+  c->cur.omega_m = om;
+  c->version++;
+  return CLH_OK;
+}
+
+static int set_pair(clh_ctx* c, dvec& X, dvec& Y, const double* x, const double* y, int n, int nmin, const char* what) {
+  if (!c) return CLH_ERR_ARG;
+  if (!x || !y || n < nmin) return fail(c, CLH_ERR_ARG, "%s: size %d (>= %d)", what, n, nmin);
+  if (has_nan(x, n) || has_nan(y, n)) return fail(c, CLH_ERR_NAN, "%s: NaN", what);
+  X.assign(x, x + n);
+  Y.assign(y, y + n);
+  c->version++;
+  return CLH_OK;
+}
+
+int clh_set_distances(clh_ctx* c, const double* z, const double* chi, int n) {
+  return set_pair(c, c->cur.chi_z, c->cur.chi, z, chi, n, 5, "set_distances");
+}
+int clh_set_growth(clh_ctx* c, const double* z, const double* G, int n) {
+  return set_pair(c, c->cur.G_z, c->cur.G, z, G, n, 2, "set_growth");
+}
+
+int clh_set_non_linear_power_spectrum(clh_ctx* c, const double* lk, int nk, const double* z, int nz,
+                                      const double* lnP) {
+  if (!c) return CLH_ERR_ARG;
+  if (!lk || !z || !lnP || nk < 2 || nz < 2) return fail(c, CLH_ERR_ARG, "set_non_linear_power_spectrum: sizes");
+  if (has_nan(lk, nk) || has_nan(z, nz) || has_nan(lnP, nk * nz)) return fail(c, CLH_ERR_NAN, "P(k,z): NaN");
+  c->cur.lk.assign(lk, lk + nk);
+  c->cur.pz.assign(z, z + nz);
+  c->cur.lnP.assign(lnP, lnP + (size_t)nk * nz);
+  c->version++;
+  return CLH_OK;
+}
+
+int clh_set_source_sample(clh_ctx* c, const double* t, int nrows, int ntomo) {
+  if (!c) return CLH_ERR_ARG;
+  int rc = build_sample(c, c->L.src, t, nrows, ntomo, true);
+  if (rc) return rc;
+  c->dev_layout_valid = false;
+  c->version++;
+  return CLH_OK;
+}
+
+int clh_set_lens_sample(clh_ctx* c, const double* t, int nrows, int ntomo) {
+  if (!c) return CLH_ERR_ARG;
+  int rc = build_sample(c, c->L.lens, t, nrows, ntomo, false);
+  if (rc) return rc;
+  lens_zmean(c->L);
+  c->dev_layout_valid = false;
+  c->version++;
+  return CLH_OK;
+}
+
+static int set_vec(clh_ctx* c, double* dst, const double* v, int n, int need, const char* what) {
+  if (!c) return CLH_ERR_ARG;
+  if (need <= 0) return fail(c, CLH_ERR_STATE, "%s: sample not set", what);
+  if (!v || n != need) return fail(c, CLH_ERR_ARG, "%s: size %d (!= %d)", what, n, need);
+  if (has_nan(v, n)) return fail(c, CLH_ERR_NAN, "%s: NaN", what);
+  for (int i = 0; i < n; i++) dst[i] = v[i];
+  c->version++;
+  return CLH_OK;
+}
+
+int clh_set_nuisance_shear_calib(clh_ctx* c, const double* m, int n) {
+  return set_vec(c, c->cur.shear_m, m, n, c ? c->L.src.nbin : 0, "set_nuisance_shear_calib");
+}
+int clh_set_nuisance_shear_photoz(clh_ctx* c, const double* v, int n) {
+  return set_vec(c, c->cur.src_dz, v, n, c ? c->L.src.nbin : 0, "set_nuisance_shear_photoz");
+}
+int clh_set_nuisance_clustering_photoz(clh_ctx* c, const double* v, int n) {
+  return set_vec(c, c->cur.lens_dz, v, n, c ? c->L.lens.nbin : 0, "set_nuisance_clustering_photoz");
+}
+int clh_set_nuisance_clustering_photoz_stretch(clh_ctx* c, const double* v, int n) {
+  return set_vec(c, c->cur.lens_st, v, n, c ? c->L.lens.nbin : 0, "set_nuisance_clustering_photoz_stretch");
+}
+int clh_set_nuisance_linear_bias(clh_ctx* c, const double* v, int n) {
+  return set_vec(c, c->cur.b1, v, n, c ? c->L.lens.nbin : 0, "set_nuisance_linear_bias");
+}
+int clh_set_nuisance_magnification_bias(clh_ctx* c, const double* v, int n) {
+  return set_vec(c, c->cur.bmag, v, n, c ? c->L.lens.nbin : 0, "set_nuisance_magnification_bias");
+}
+int clh_set_nuisance_ia(clh_ctx* c, const double* A1, int n) {
+  if (!c) return CLH_ERR_ARG;
+  if (!A1 || n <= 0) return fail(c, CLH_ERR_ARG, "set_nuisance_IA: empty");
+  if (has_nan(A1, n)) return fail(c, CLH_ERR_NAN, "set_nuisance_IA: NaN");
+  if (c->L.ia_mode == 3) {
+    if (n < 2) return fail(c, CLH_ERR_ARG, "set_nuisance_IA: need (A, eta)");
+    c->cur.ia_A = A1[0];
+    c->cur.ia_eta = A1[1];
+  } else if (c->L.ia_mode == 2) {
+    if (n < c->L.src.nbin) return fail(c, CLH_ERR_ARG, "set_nuisance_IA: need one A per source bin");
+    for (int i = 0; i < c->L.src.nbin; i++) c->cur.ia_bin[i] = A1[i];
+  }
+  c->version++;
+  return CLH_OK;
+}
+
+int clh_datavector_size(clh_ctx* c) { return c ? c->L.dv_size() : 0; }
+
+int clh_compute_3x2pt_real(clh_ctx* c, const int ord[3], double* dv, int n) {
+  if (!c || !ord || !dv) return CLH_ERR_ARG;
+  if (n != c->L.dv_size()) return fail(c, CLH_ERR_ARG, "dv size %d != %d", n, c->L.dv_size());
+  for (int i = 0; i < 3; i++)
+    if (ord[i] < 0) return fail(c, CLH_ERR_ARG, "bad ord");
+  std::vector<const Draw*> v{&c->cur};
+  int rc = build_batch(c, c->single, v, 1, ord, nullptr, nullptr);
+  if (rc) return rc;
+  if ((rc = run_batch(c, c->single, true))) return rc;
+  HIPCHK(c, hipMemcpyAsync(dv, c->single.desc[0].dv, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->cached_version = 0;  // the single batch now holds a calibrated vector
+  return CLH_OK;
+}
+
+int clh_xi_pm(clh_ctx* c, double* xip, double* xim) {
+  if (!c || !xip || !xim) return CLH_ERR_ARG;
+  int rc = ensure_current(c);
+  if (rc) return rc;
+  const int m = c->L.n_ss() * c->L.ntheta;
+  std::copy(c->cache_raw.begin(), c->cache_raw.begin() + m, xip);
+  std::copy(c->cache_raw.begin() + m, c->cache_raw.begin() + 2 * m, xim);
+  return CLH_OK;
+}
+
+int clh_gammat(clh_ctx* c, double* out) {
+  if (!c || !out) return CLH_ERR_ARG;
+  int rc = ensure_current(c);
+  if (rc) return rc;
+  const int m = c->L.n_ss() * c->L.ntheta;
+  std::copy(c->cache_raw.begin() + 2 * m, c->cache_raw.begin() + 2 * m + c->L.n_gs() * c->L.ntheta, out);
+  return CLH_OK;
+}
+
+int clh_wtheta(clh_ctx* c, double* out) {
+  if (!c || !out) return CLH_ERR_ARG;
+  int rc = ensure_current(c);
+  if (rc) return rc;
+  const int m = c->L.ntheta * (2 * c->L.n_ss() + c->L.n_gs());
+  std::copy(c->cache_raw.begin() + m, c->cache_raw.begin() + m + c->L.lens.nbin * c->L.ntheta, out);
+  return CLH_OK;
+}
+
+int clh_cl_tables(clh_ctx* c, double* ss, double* gs, double* gg) {
+  if (!c) return CLH_ERR_ARG;
+  int rc = ensure_current(c);
+  if (rc) return rc;
+  const int nrow = c->L.nrow(), ne = c->L.n_ell;
+  const int nss = c->L.n_ss(), ngs = c->L.n_gs(), nl = c->L.lens.nbin;
+  for (int q = 0; q < nss + ngs + nl; q++) {
+    double* dst = q < nss ? (ss ? ss + (size_t)q * ne : nullptr)
+                : q < nss + ngs ? (gs ? gs + (size_t)(q - nss) * ne : nullptr)
+                : (gg ? gg + (size_t)(q - nss - ngs) * ne : nullptr);
+    if (dst) std::copy(c->cache_cl.begin() + (size_t)q * nrow + CLH_NLOW, c->cache_cl.begin() + (size_t)(q + 1) * nrow, dst);
+  }
+  return CLH_OK;
+}
+
+int clh_cl_nointerp(clh_ctx* c, int probe, const double* ell, int nl_, double* out) {
+  if (!c || !ell || !out || nl_ <= 0) return CLH_ERR_ARG;
+  if (probe < 0 || probe > 2) return fail(c, CLH_ERR_ARG, "probe %d", probe);
+  if (probe > 0 && c->L.lens.nbin == 0) return fail(c, CLH_ERR_STATE, "lens sample not set");
+  if (has_nan(ell, nl_)) return fail(c, CLH_ERR_NAN, "ell: NaN");
+  dvec ells(ell, ell + nl_);
+  const int ord[3] = {0, 1, 2};
+  std::vector<const Draw*> v{&c->cur};
+  int rc = build_batch(c, c->scratch, v, 0, ord, &ells, nullptr);
+  if (rc) return rc;
+  if ((rc = run_batch(c, c->scratch, false))) return rc;
+  const int nss = c->L.n_ss(), ngs = c->L.n_gs(), nl = c->L.lens.nbin;
+  const int col0 = probe == 0 ? 0 : probe == 1 ? nss : nss + ngs;
+  const int ncol = probe == 0 ? nss : probe == 1 ? ngs : nl;
+  HIPCHK(c, hipMemcpyAsync(out, c->scratch.desc[0].cl + (size_t)col0 * nl_, (size_t)ncol * nl_ * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLH_OK;
+}
+
+static void fill_draw(Draw& d, const clh_draw& s, const Layout& L) {
+  d.omega_m = s.omega_m;
+  if (s.chi_z && s.chi && s.chi_n > 0) { d.chi_z.assign(s.chi_z, s.chi_z + s.chi_n); d.chi.assign(s.chi, s.chi + s.chi_n); }
+  if (s.G_z && s.G && s.G_n > 0) { d.G_z.assign(s.G_z, s.G_z + s.G_n); d.G.assign(s.G, s.G + s.G_n); }
+  if (s.log10k && s.pk_z && s.lnP && s.nk > 0 && s.npz > 0) {
+    d.lk.assign(s.log10k, s.log10k + s.nk);
+    d.pz.assign(s.pk_z, s.pk_z + s.npz);
+    d.lnP.assign(s.lnP, s.lnP + (size_t)s.nk * s.npz);
+  }
+  for (int i = 0; i < L.src.nbin; i++) {
+    d.src_dz[i] = s.src_dz ? s.src_dz[i] : 0.0;
+    d.shear_m[i] = s.shear_m ? s.shear_m[i] : 0.0;
+    d.ia_bin[i] = s.ia_bin ? s.ia_bin[i] : 0.0;
+  }
+  for (int i = 0; i < L.lens.nbin; i++) {
+    d.lens_dz[i] = s.lens_dz ? s.lens_dz[i] : 0.0;
+    d.lens_st[i] = s.lens_stretch ? s.lens_stretch[i] : 1.0;
+    d.b1[i] = s.b1 ? s.b1[i] : 0.0;
+    d.bmag[i] = s.bmag ? s.bmag[i] : 0.0;
+  }
+  d.ia_A = s.ia_A;
+  d.ia_eta = s.ia_eta;
+}
+
+int clh_batch_upload(clh_ctx* c, int B, const clh_draw* draws) {
+  if (!c || B <= 0 || !draws) return CLH_ERR_ARG;
+  std::vector<Draw> ds(B);
+  std::vector<const Draw*> ptr(B);
+  for (int i = 0; i < B; i++) {
+    const clh_draw& s = draws[i];
+    if (s.chi_n < 5 || s.G_n < 2 || s.nk < 2 || s.npz < 2) return fail(c, CLH_ERR_ARG, "draw %d: table sizes", i);
+    if (has_nan(s.chi_z, s.chi_n) || has_nan(s.chi, s.chi_n) || has_nan(s.G_z, s.G_n) || has_nan(s.G, s.G_n) ||
+        has_nan(s.log10k, s.nk) || has_nan(s.pk_z, s.npz) || has_nan(s.lnP, s.nk * s.npz))
+      return fail(c, CLH_ERR_NAN, "draw %d: NaN in a table", i);
+    if (c->L.lens.nbin > 0 && !s.b1) return fail(c, CLH_ERR_ARG, "draw %d: b1 missing", i);
+    fill_draw(ds[i], s, c->L);
+    ptr[i] = &ds[i];
+  }
+  const int ord[3] = {0, 1, 2};
+  return build_batch(c, c->batch, ptr, 1, ord, nullptr, nullptr);
+}
+
+int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
+  if (!c || !ord || !dv_device) return CLH_ERR_ARG;
+  if (c->batch.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
+  if (!c->dev_layout_valid) return fail(c, CLH_ERR_STATE, "layout changed since clh_batch_upload");
+  // per-call placement: output pointer and block order go into the descriptors
+  const Layout& L = c->L;
+  const int nt = L.ntheta;
+  const int sizes[3] = {2 * nt * L.n_ss(), nt * L.n_gs(), nt * L.lens.nbin};
+  int idx[3] = {0, 1, 2};
+  for (int i = 1; i < 3; i++)
+    for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) std::swap(idx[j], idx[j - 1]);
+  int start[3] = {0, 0, 0};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+  bool changed = false;
+  for (int i = 0; i < c->batch.B; i++) {
+    DrawDesc& q = c->batch.desc[i];
+    double* want = dv_device + (size_t)c->batch.dv_size * i;
+    if (q.dv != want || q.dv_start[0] != start[0] || q.dv_start[1] != start[1] || q.dv_start[2] != start[2]) {
+      q.dv = want;
+      for (int k = 0; k < 3; k++) q.dv_start[k] = start[k];
+      changed = true;
+    }
+  }
+  if (changed)
+    HIPCHK(c, hipMemcpyAsync(c->batch.d_desc.p, c->batch.desc.data(), c->batch.B * sizeof(DrawDesc),
+                             hipMemcpyHostToDevice, c->stream));
+  return run_batch(c, c->batch, true);
+}
+
+int clh_synchronize(clh_ctx* c) {
+  if (!c) return CLH_ERR_ARG;
+  if (!c->stream) return CLH_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->timing)
+    for (int k = 0; k < 4; k++) HIPCHK(c, hipEventElapsedTime(&c->last_ms[k], c->ev[k], c->ev[k + 1]));
+  return CLH_OK;
+}
+
+int clh_enable_timing(clh_ctx* c, int on) {
+  if (!c) return CLH_ERR_ARG;
+  c->timing = on != 0;
+  return CLH_OK;
+}
+
+int clh_last_kernel_times(clh_ctx* c, const char** names, float* ms, int cap) {
+  static const char* nm[4] = {"k_gtab", "k_nodes", "k_limber", "k_legendre"};
+  if (!c) return CLH_ERR_ARG;
+  for (int k = 0; k < 4 && k < cap; k++) {
+    if (names) names[k] = nm[k];
+    if (ms) ms[k] = c->last_ms[k];
+  }
+  return 4;
+}
+
+}  // extern "C"
+
+// ================================================= reference-named entries
+// Scalar accessors with the signatures of cosmolike/cosmo2D.h:30-96, served
+// from a process-wide context.  Errors return NaN and record the message
+// (the reference logs and exits instead).
+#include <limits>
+#include <mutex>
+
+static clh_ctx* g_default = nullptr;
+static std::mutex g_default_mu;
+
+static double nanv() { return std::numeric_limits<double>::quiet_NaN(); }
+
+static int n_shear_idx(int ns, int ni, int nj) {  // N_shear: redshift_spline.c:277-303
+  const int i = std::min(ni, nj), j = std::max(ni, nj);
+  int q = 0;
+  for (int k = 0; k < i; k++) q += ns - k;
+  return q + (j - i);
+}
+
+// interpol1d over one table column (basics.c:457-480)
+static double table_lookup(const clh_ctx* c, int col, bool ss, double l) {
+  const Layout& L = c->L;
+  const double lim0 = std::log(std::fmax(ss ? CLH_LMIN_TAB - 1. : (double)CLH_LMIN_TAB, 1.0));
+  const double lim1 = std::log((double)(L.lmax + 1));
+  const double dx = (lim1 - lim0) / ((double)L.n_ell - 1.);
+  const double* f = c->cache_cl.data() + (size_t)col * L.nrow() + CLH_NLOW;
+  const double x = std::log(l);
+  if (x < lim0) return f[0];
+  const double r = (x - lim0) / dx;
+  const int i = (int)std::floor(r);
+  if (i + 1 >= L.n_ell) return f[L.n_ell - 1];
+  return (r - i) * (f[i + 1] - f[i]) + f[i];
+}
+
+extern "C" {
+
+clh_ctx* clh_default(void) {
+  std::lock_guard<std::mutex> g(g_default_mu);
+  if (!g_default) clh_create(&g_default, 0);
+  return g_default;
+}
+
+double xi_pm_tomo(const int pm, const int nt, const int ni, const int nj, const int limber) {
+  clh_ctx* c = clh_default();
+  if (!c) return nanv();
+  if (limber != 1) { fail(c, CLH_ERR_ARG, "NonLimber not implemented"); return nanv(); }
+  const int ns = c->L.src.nbin;
+  if (nt < 0 || nt > c->L.ntheta - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number nt = %d", nt); return nanv(); }
+  if (ni < 0 || ni > ns - 1 || nj < 0 || nj > ns - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number (ni,nj) = [%d,%d]", ni, nj); return nanv(); }
+  if (ensure_current(c)) return nanv();
+  const int q = n_shear_idx(ns, ni, nj) * c->L.ntheta + nt;
+  const int m = c->L.n_ss() * c->L.ntheta;
+  return (pm > 0) ? c->cache_raw[q] : c->cache_raw[m + q];
+}
+
+double w_gammat_tomo(const int nt, const int ni, const int nj, const int limber) {
+  clh_ctx* c = clh_default();
+  if (!c) return nanv();
+  if (limber != 1) { fail(c, CLH_ERR_ARG, "NonLimber not implemented"); return nanv(); }
+  if (nt < 0 || nt > c->L.ntheta - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number nt = %d", nt); return nanv(); }
+  if (ni < 0 || ni > c->L.lens.nbin - 1 || nj < 0 || nj > c->L.src.nbin - 1) {
+    fail(c, CLH_ERR_ARG, "error in selecting bin number (ni, nj) = [%d,%d]", ni, nj);
+    return nanv();
+  }
+  if (ensure_current(c)) return nanv();
+  const int q = (ni * c->L.src.nbin + nj) * c->L.ntheta + nt;
+  return c->cache_raw[2 * c->L.n_ss() * c->L.ntheta + q];
+}
+
+double w_gg_tomo(const int nt, const int ni, const int nj, const int limber) {
+  clh_ctx* c = clh_default();
+  if (!c) return nanv();
+  if (limber != 1) { fail(c, CLH_ERR_ARG, "non-Limber w(theta) is not implemented (limber=1 only)"); return nanv(); }
+  if (nt < 0 || nt > c->L.ntheta - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number nt = %d", nt); return nanv(); }
+  if (ni < 0 || ni > c->L.lens.nbin - 1 || nj < 0 || nj > c->L.lens.nbin - 1) {
+    fail(c, CLH_ERR_ARG, "error in selecting bin number (ni,nj) = [%d,%d]", ni, nj);
+    return nanv();
+  }
+  if (ni != nj) { fail(c, CLH_ERR_ARG, "ni != nj tomography not supported"); return nanv(); }
+  if (ensure_current(c)) return nanv();
+  return c->cache_raw[(2 * c->L.n_ss() + c->L.n_gs()) * c->L.ntheta + ni * c->L.ntheta + nt];
+}
+
+double C_ss_tomo_limber(const double l, const int ni, const int nj, const int EE) {
+  clh_ctx* c = clh_default();
+  if (!c) return nanv();
+  const int ns = c->L.src.nbin;
+  if (ni < 0 || ni > ns - 1 || nj < 0 || nj > ns - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number (ni,nj) = [%d,%d]", ni, nj); return nanv(); }
+  if (ensure_current(c)) return nanv();
+  if (EE != 1) return 0.0;  // BB table is identically zero under NLA
+  return table_lookup(c, n_shear_idx(ns, ni, nj), true, l);
+}
+
+double C_gs_tomo_limber(const double l, const int ni, const int nj) {
+  clh_ctx* c = clh_default();
+  if (!c) return nanv();
+  if (ni < 0 || ni > c->L.lens.nbin - 1 || nj < 0 || nj > c->L.src.nbin - 1) {
+    fail(c, CLH_ERR_ARG, "error in selecting bin number (ni, nj) = [%d,%d]", ni, nj);
+    return nanv();
+  }
+  if (ensure_current(c)) return nanv();
+  return table_lookup(c, c->L.n_ss() + ni * c->L.src.nbin + nj, false, l);
+}
+
+double C_gg_tomo_limber(const double l, const int ni, const int nj) {
+  clh_ctx* c = clh_default();
+  if (!c) return nanv();
+  if (ni < 0 || ni > c->L.lens.nbin - 1 || nj < 0 || nj > c->L.lens.nbin - 1) {
+    fail(c, CLH_ERR_ARG, "error in selecting bin number (ni,nj) = [%d,%d]", ni, nj);
+    return nanv();
+  }
+  if (ni != nj) { fail(c, CLH_ERR_ARG, "cross-tomography not supported"); return nanv(); }
+  if (ensure_current(c)) return nanv();
+  return table_lookup(c, c->L.n_ss() + c->L.n_gs() + ni, false, l);
+}
+
+// init = 1 only primes the reference's static caches; its value is discarded
+// by every caller, so it returns 0 here.
+double C_ss_tomo_limber_nointerp(const double l, const int ni, const int nj, const int EE, const int init) {
+  clh_ctx* c = clh_default();
+  if (!c) return nanv();
+  const int ns = c->L.src.nbin;
+  if (ni < 0 || ni > ns - 1 || nj < 0 || nj > ns - 1) { fail(c, CLH_ERR_ARG, "invalid bin input (ni, nj) = (%d, %d)", ni, nj); return nanv(); }
+  if (init == 1 || EE != 1) return 0.0;
+  std::vector<double> out(c->L.n_ss());
+  if (clh_cl_nointerp(c, 0, &l, 1, out.data())) return nanv();
+  return out[n_shear_idx(ns, ni, nj)];
+}
+
+double C_gs_tomo_limber_nointerp(const double l, const int nl, const int ns, const int init) {
+  clh_ctx* c = clh_default();
+  if (!c) return nanv();
+  if (nl < 0 || nl > c->L.lens.nbin - 1 || ns < 0 || ns > c->L.src.nbin - 1) {
+    fail(c, CLH_ERR_ARG, "invalid bin input (ni, nj) = (%d, %d)", nl, ns);
+    return nanv();
+  }
+  if (init == 1) return 0.0;
+  std::vector<double> out(c->L.n_gs());
+  if (clh_cl_nointerp(c, 1, &l, 1, out.data())) return nanv();
+  return out[nl * c->L.src.nbin + ns];
+}
+
+double C_gg_tomo_limber_nointerp(const double l, const int ni, const int nj, const int init) {
+  clh_ctx* c = clh_default();
+  if (!c) return nanv();
+  if (ni < 0 || ni > c->L.lens.nbin - 1 || nj < 0 || nj > c->L.lens.nbin - 1) {
+    fail(c, CLH_ERR_ARG, "error in selecting bin number (ni, nj) = [%d,%d]", ni, nj);
+    return nanv();
+  }
+  if (ni != nj) { fail(c, CLH_ERR_ARG, "cross-tomography (ni,nj) = (%d,%d) bins not supported", ni, nj); return nanv(); }
+  if (init == 1) return 0.0;
+  std::vector<double> out(c->L.lens.nbin);
+  if (clh_cl_nointerp(c, 2, &l, 1, out.data())) return nanv();
+  return out[ni];
+}
+
+}  // extern "C"

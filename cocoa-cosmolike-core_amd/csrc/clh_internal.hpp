@@ -1,0 +1,79 @@
+// Shared host/device definitions of the projection library.
+#pragma once
+#include <stdint.h>
+
+#define CLH_MAXB 20          // MAX_SIZE_ARRAYS (cosmolike/structs.h:11)
+#define CLH_LMIN_TAB 20      // limits.LMIN_tab (cosmolike/structs.c:44)
+#define CLH_NLOW 19          // directly integrated ell = 1 .. LMIN_tab-1
+#define CLH_COVERH0 2997.92458
+#define CLH_A_MIN (1.0 / (1.0 + 40.0))  // limits.a_min (structs.c:41)
+#define CLH_NP_SS 64         // GL order for ss/gs at high_def_integration = 0
+#define CLH_NP_GG 96         // GL order for gg at high_def_integration = 0
+
+// Work-item kinds of the Limber kernel
+#define CLH_W_SS 0
+#define CLH_W_GS 1
+#define CLH_W_GG 2
+
+// Rows of a Limber table column: CLH_NLOW direct rows (ell = r+1), then the
+// N_ell log-spaced rows.  Rows per ss/gs work tile and per gg tile:
+#define CLH_SS_ROWS 4
+#define CLH_GG_ROWS 8
+
+// Everything one draw needs on the device.  Plain data; lives in an array in
+// device memory and is read with uniform (scalar) loads.
+struct DrawDesc {
+  // cosmology tables (cosmopara, cosmolike/structs.h:128-191)
+  const double* chi_z;
+  const double* chi_v;
+  const double* G_z;
+  const double* G_v;
+  const double* lk;   // log10 k [h/Mpc]
+  const double* pz;   // z of the P(k,z) table
+  const double* lnP;  // [nk][nzp]
+  int chi_n, G_n, nk, nzp;
+  double omega_m;
+
+  // survey layout
+  int ns, nl, ntheta, n_ell, n_a, nrow, n_ss, n_gs;
+  const double* s_zc;    // [s_rows] n(z) spline knots (histogram centres)
+  const double* s_coef;  // [ns][s_rows-1][4] (y, b, c, d) per interval
+  const double* l_zc;
+  const double* l_coef;
+  int s_rows, l_rows;
+  double s_zmin_all, s_zmax_all, l_zmin_all, l_zmax_all;
+  double l_zmin[CLH_MAXB], l_zmax[CLH_MAXB], l_zmean[CLH_MAXB];
+  const double* ell_ss;   // [nrow] ell of every ss row
+  const double* ell_gx;   // [nrow] ell of every gs/gg row
+  const double* kfold;    // [4][nrow][ntheta] (xi+, xi-, gamma_t, w), row-major in r
+  const double* kfull;    // [4][lmax][ntheta] full-length kernels (FULL mode) or null
+  const double* fill;     // [2][lmax][2] (index, weight) of the ln-ell fill, ss then gs/gg
+  double lim_ss[3], lim_gx[3];
+  int lmax;
+
+  // nuisance (nuisanceparams, cosmolike/structs.h:203-298)
+  double src_dz[CLH_MAXB], lens_dz[CLH_MAXB], lens_st[CLH_MAXB];
+  double b1[CLH_MAXB], bmag[CLH_MAXB], shear_m[CLH_MAXB];
+  double ia_bin[CLH_MAXB];
+  double ia_A, ia_eta;
+  int ia_mode;
+
+  // per-draw work buffers
+  double* gtab;    // [ns + nl][n_a] lens-efficiency tables (g_tomo, then g_lens)
+  double* nodes;   // node data, offsets below
+  double* cl;      // [n_ss + n_gs + nl][nrow]
+  double* dv;      // data vector
+  int dv_start[3];
+  int node_ss, node_gs, node_gg;  // offsets into nodes
+  int node_gs_stride, node_gg_stride;
+  double chi_amin;  // chi(a_min) for the gg RSD cut
+  double G0;        // G(z=0) by the growth table's own interpolation
+  const double* gl; // GL abscissae/weights on [-1,1]: x64 w64 x96 w96
+  int apply_calib;  // 1: shear calibration applied to the data vector
+};
+
+// Node buffer layout (doubles):
+//  ss block  : a[64] fK[64] fac[64] A[ns][64]            fac = wt dchi/da / fK^2
+//              A_b = W_kappa_b - C1 W_source_b
+//  gs block zl: a fK fac WGAL WMAG [64 each], B[ns][64]  B_zs = W_kappa_zs - C1_zs W_source_zs
+//  gg block zl: a fK fac WGAL WMAG [96 each]

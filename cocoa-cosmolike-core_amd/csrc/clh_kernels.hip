@@ -1,0 +1,451 @@
+// HIP kernels of the FP64 projection path (gfx950).
+//
+//  k_gtab     lens-efficiency tables g_tomo / g_lens: fine-grid integrand,
+//             per-interval trapezoid sums, block prefix scan
+//             (cosmolike/redshift_spline.c:852-923, 1014-1109)
+//  k_nodes    Gauss-Legendre nodes in a and every weight the Limber sums need
+//             (create_cosmo_nodes cosmo2D.c:1331-1356; radial_weights.c)
+//  k_limber   Limber node sums for ss / gs tiles (P(k,z) and weights staged in
+//             LDS) and the gg integrand with RSD (cosmo2D.c:1655-1747,
+//             2229-2348, 2490-2666) -> C_ell rows
+//  k_legendre contraction of C_ell rows with the bin-averaged Legendre
+//             kernels into xi+, xi-, gamma_t, w and data-vector placement with
+//             shear calibration (cosmo2D.c:487-534; generic_interface.hpp:800-862)
+#include <hip/hip_runtime.h>
+
+#include "clh_device.hpp"
+#include "clh_internal.hpp"
+
+#define CLH_BLOCK 256
+
+// ----------------------------------------------------------------------------
+// block-wide inclusive scan of one double per thread (256 threads, wave64)
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double clh_block_scan(double v, double* lds /* >= 4 */) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  if (lane == 63) lds[wid] = v;
+  __syncthreads();
+  double base = 0.0;
+  for (int w = 0; w < wid; w++) base += lds[w];
+  __syncthreads();
+  return v + base;
+}
+
+__device__ __forceinline__ double clh_block_sum(double v, double* lds) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  const double s = lds[0] + lds[1] + lds[2] + lds[3];
+  __syncthreads();
+  return s;
+}
+
+// ----------------------------------------------------------------------------
+// k_gtab: one block per (draw, bin).  bin < ns -> g_tomo of source bin,
+// otherwise g_lens of lens bin (bin - ns).
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__ D,
+                                                    const int2* __restrict__ work) {
+  __shared__ double sP[1024], sQ[1024];
+  __shared__ double red[8];
+  const int2 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const bool src = w.y < d.ns;
+  const int b = src ? w.y : w.y - d.ns;
+  const int x = 25;  // 25 (1 + |high_def_integration|), high_def_integration = 0
+  const int nint = d.n_a - 1;
+  const int Na = x * nint + 1;
+  const double amax = 0.999999;
+  const double amin = 1.0 / ((src ? d.s_zmax_all : d.l_zmax_all) + 1.0);
+  const double da = (amax - amin) / ((double)Na - 1.0);
+
+  // g_lens pre-integral from a_min of the shear sample (redshift_spline.c:1052-1071)
+  double P0 = 0.0, Q0 = 0.0;
+  if (!src) {
+    const double amin_shear = 1.0 / (d.s_zmax_all + 1.0);
+    const int Na0 = (amin_shear < amin) ? (int)ceil((amin - amin_shear) / da) + 1 : 1;
+    const double da0 = (Na0 > 1) ? (amin - amin_shear) / ((double)Na0 - 1.) : 0.0;
+    double p = 0.0, q = 0.0;
+    for (int i = 1 + threadIdx.x; i < Na0; i += CLH_BLOCK) {
+      const double apl = amin_shear + (i - 1) * da0;
+      const double aph = amin_shear + i * da0;
+      const double fl = clh_nz_lens(d, 1.0 / apl - 1.0, b) / (apl * apl);
+      const double fh = clh_nz_lens(d, 1.0 / aph - 1.0, b) / (aph * aph);
+      p += 0.5 * da0 * (fl + fh);
+      q += 0.5 * da0 * (fl / clh_chi(d, apl) + fh / clh_chi(d, aph));
+    }
+    P0 = clh_block_sum(p, red);
+    Q0 = clh_block_sum(q, red);
+  }
+
+  // per-interval trapezoid sums of n/a^2 and n/(chi a^2)
+  for (int t = threadIdx.x; t < nint; t += CLH_BLOCK) {
+    double p = 0.0, q = 0.0;
+    double fprev = 0.0, gprev = 0.0;
+    for (int s = 0; s <= x; s++) {
+      const int i = t * x + s;
+      const double a = amin + i * da;
+      const double f = (src ? clh_nz_src(d, 1. / a - 1., b) : clh_nz_lens(d, 1.0 / a - 1.0, b)) / (a * a);
+      const double g = f / clh_chi(d, amin + i * da);
+      if (s > 0) {
+        p += 0.5 * da * (fprev + f);
+        q += 0.5 * da * (gprev + g);
+      }
+      fprev = f;
+      gprev = g;
+    }
+    sP[t] = p;
+    sQ[t] = q;
+  }
+  __syncthreads();
+
+  // chunked prefix scan over intervals
+  const int cpt = (nint + CLH_BLOCK - 1) / CLH_BLOCK;
+  const int t0 = threadIdx.x * cpt;
+  const int t1 = min(t0 + cpt, nint);
+  double cp = 0.0, cq = 0.0;
+  for (int t = t0; t < t1; t++) { cp += sP[t]; cq += sQ[t]; }
+  const double ip = clh_block_scan(cp, red);
+  const double iq = clh_block_scan(cq, red);
+  double P = P0 + ip - cp, Q = Q0 + iq - cq;
+  double* tab = d.gtab + (size_t)w.y * d.n_a;
+  for (int t = t0; t < t1; t++) {
+    P += sP[t];
+    Q += sQ[t];
+    const int i = (t + 1) * x;
+    tab[t + 1] = P - clh_chi(d, amin + i * da) * Q;
+  }
+  if (threadIdx.x == 0) tab[0] = src ? 0.0 : P0 - clh_chi(d, amin) * Q0;
+}
+
+// ----------------------------------------------------------------------------
+// k_nodes: one block per (draw, segment); segment 0 = ss, 1..nl = gs lens
+// bin, nl+1..2nl = gg lens bin.  One thread per GL node.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_nodes(const DrawDesc* __restrict__ D,
+                                               const int2* __restrict__ work) {
+  const int2 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const int seg = w.y;
+  const int p = threadIdx.x;
+  const int np = (seg <= d.nl) ? CLH_NP_SS : CLH_NP_GG;
+  if (p >= np) return;
+  const double* glx = d.gl + (np == CLH_NP_SS ? 0 : 2 * CLH_NP_SS);
+  const double* glw = glx + np;
+  double amin, amax;
+  int zl = -1;
+  if (seg == 0) {
+    amin = 1. / (d.s_zmax_all + 1.);
+    amax = 1. / (1. + fmax(d.s_zmin_all, 1e-6));
+  } else {
+    zl = (seg <= d.nl) ? seg - 1 : seg - 1 - d.nl;
+    amin = clh_amin_lens(d, zl);
+    amax = clh_amax_lens(d, zl);
+  }
+  const double A = 0.5 * (amax - amin), B = 0.5 * (amin + amax);
+  const double a = B + A * glx[p];
+  const double wt = A * glw[p];
+  double fK, dchida;
+  clh_chi_all(d, a, fK, dchida);
+  const double gf = clh_growfac(d, a);
+  const double hh = clh_hoverh0v2(a, dchida);
+  const double fac = wt * (dchida / (fK * fK));
+  const double z = 1.0 / a - 1.0;
+  const double wk_pre = 1.5 * d.omega_m * fK / a;
+  if (seg == 0) {
+    double* o = d.nodes + d.node_ss;
+    o[p] = a;
+    o[np + p] = fK;
+    o[2 * np + p] = fac;
+    for (int bb = 0; bb < d.ns; bb++) {
+      const double WK = wk_pre * clh_g_tomo(d, a, bb);
+      const double WS = clh_nz_src(d, z, bb) * hh;
+      o[(3 + bb) * np + p] = WK - clh_ia_c1(d, a, gf, bb) * WS;
+    }
+  } else if (seg <= d.nl) {
+    double* o = d.nodes + d.node_gs + (size_t)zl * d.node_gs_stride;
+    o[p] = a;
+    o[np + p] = fK;
+    o[2 * np + p] = fac;
+    o[3 * np + p] = clh_nz_lens(d, z, zl) * hh;
+    o[4 * np + p] = wk_pre * clh_g_lens(d, a, zl);
+    for (int zs = 0; zs < d.ns; zs++) {
+      const double WK = wk_pre * clh_g_tomo(d, a, zs);
+      const double WS = clh_nz_src(d, z, zs) * hh;
+      o[(5 + zs) * np + p] = WK - WS * clh_ia_c1(d, a, gf, zs);
+    }
+  } else {
+    double* o = d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride;
+    o[p] = a;
+    o[np + p] = fK;
+    o[2 * np + p] = fac;
+    o[3 * np + p] = clh_nz_lens(d, z, zl) * hh;
+    o[4 * np + p] = wk_pre * clh_g_lens(d, a, zl);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// k_limber: work item (draw, kind, zl, row0)
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(CLH_BLOCK) void k_limber(const DrawDesc* __restrict__ D,
+                                                      const int4* __restrict__ work) {
+  __shared__ double S[CLH_GG_ROWS * CLH_NP_GG];          // P * fac per (row, node)
+  __shared__ double W[(CLH_MAXB + 2) * CLH_NP_SS];        // weights per (bin, node)
+  const int4 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const int kind = w.y, zl = w.z, row0 = w.w;
+  const int tid = threadIdx.x;
+
+  if (kind == CLH_W_SS || kind == CLH_W_GS) {
+    const int np = CLH_NP_SS;
+    const double* o = (kind == CLH_W_SS) ? d.nodes + d.node_ss
+                                         : d.nodes + d.node_gs + (size_t)zl * d.node_gs_stride;
+    const double* ells = (kind == CLH_W_SS) ? d.ell_ss : d.ell_gx;
+    {
+      const int rr = tid >> 6, p = tid & 63;
+      const int r = row0 + rr;
+      double s = 0.0;
+      if (r < d.nrow) {
+        const double ell = ells[r] + 0.5;
+        const double k = ell / o[np + p];
+        s = clh_p_nonlin(d, k, o[p]) * o[2 * np + p];
+      }
+      S[rr * np + p] = s;
+    }
+    const int nw = (kind == CLH_W_SS) ? d.ns : d.ns + 2;
+    const int woff = (kind == CLH_W_SS) ? 3 : 3;
+    for (int i = tid; i < nw * np; i += CLH_BLOCK) W[i] = o[woff * np + i];
+    __syncthreads();
+    if (kind == CLH_W_SS) {
+      const int nout = d.n_ss * CLH_SS_ROWS;
+      for (int q = tid; q < nout; q += CLH_BLOCK) {
+        const int pair = q / CLH_SS_ROWS, rr = q % CLH_SS_ROWS;
+        const int r = row0 + rr;
+        if (r >= d.nrow) continue;
+        int i, j;
+        clh_ss_pair(d.ns, pair, i, j);
+        const double* Ai = W + i * np;
+        const double* Aj = W + j * np;
+        const double* Sr = S + rr * np;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int p = 0; p < np; p++) acc += Ai[p] * Aj[p] * Sr[p];
+        const double l = ells[r], ell = l + 0.5, ell4 = ell * ell * ell * ell;
+        const double pref = l * (l - 1.) * (l + 1.) * (l + 2.) / ell4;
+        d.cl[(size_t)pair * d.nrow + r] = acc * pref;
+      }
+    } else {
+      // W rows: 0 = WGAL, 1 = WMAG, 2.. = B[zs]
+      const int nout = d.ns * CLH_SS_ROWS;
+      const double b1 = d.b1[zl], bmag = d.bmag[zl];
+      for (int q = tid; q < nout; q += CLH_BLOCK) {
+        const int zs = q / CLH_SS_ROWS, rr = q % CLH_SS_ROWS;
+        const int r = row0 + rr;
+        if (r >= d.nrow) continue;
+        const double l = ells[r], ell = l + 0.5;
+        const double pref1 = l * (l + 1.) / (ell * ell);
+        const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+        const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+        const double* Bz = W + (2 + zs) * np;
+        const double* Sr = S + rr * np;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int p = 0; p < np; p++) acc += Bz[p] * (W[p] * b1 + W[np + p] * pref1 * bmag) * Sr[p];
+        d.cl[(size_t)(d.n_ss + zl * d.ns + zs) * d.nrow + r] = acc * pref2;
+      }
+    }
+  } else {
+    // gg: full integrand per (row, node) incl. RSD (cosmo2D.c:2490-2608)
+    const int np = CLH_NP_GG;
+    const double* o = d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride;
+    const double b1 = d.b1[zl], bmag = d.bmag[zl];
+    for (int it = tid; it < CLH_GG_ROWS * np; it += CLH_BLOCK) {
+      const int rr = it / np, p = it % np;
+      const int r = row0 + rr;
+      double v = 0.0;
+      if (r < d.nrow) {
+        const double l = d.ell_gx[r], ell = l + 0.5;
+        const double a = o[p], fK = o[np + p];
+        const double k = ell / fK;
+        const double chi_0 = ell / k;
+        const double chi_1 = (ell + 1.0) / k;
+        if (!(chi_1 > d.chi_amin)) {
+          const double a_0 = clh_a_chi(d, chi_0);
+          const double a_1 = clh_a_chi(d, chi_1);
+          const double wr = clh_w_rsd(d, ell, a_0, a_1, zl);
+          const double pref = l * (l + 1.) / (ell * ell);
+          const double t = o[3 * np + p] * b1 + o[4 * np + p] * pref * bmag + wr;
+          v = t * t * clh_p_nonlin(d, k, a) * o[2 * np + p];
+        }
+      }
+      S[it] = v;
+    }
+    __syncthreads();
+    // one wave reduces the CLH_GG_ROWS rows
+    if (tid < 64) {
+      for (int rr = 0; rr < CLH_GG_ROWS; rr++) {
+        double v = S[rr * np + tid] + ((tid + 64 < np) ? S[rr * np + tid + 64] : 0.0);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        const int r = row0 + rr;
+        if (tid == 0 && r < d.nrow) d.cl[(size_t)(d.n_ss + d.n_gs + zl) * d.nrow + r] = v;
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// k_legendre (folded kernels): work item (draw, first task); 8 tasks per block,
+// 32 theta slots per task.  Task t of a draw: [0, n_ss) xi+, [n_ss, 2 n_ss)
+// xi-, then gamma_t pairs, then w bins.
+// ----------------------------------------------------------------------------
+#define CLH_LEG_TASKS 8
+__global__ __launch_bounds__(CLH_BLOCK) void k_legendre(const DrawDesc* __restrict__ D,
+                                                        const int2* __restrict__ work) {
+  extern __shared__ double colbuf[];  // [CLH_LEG_TASKS][nrow]
+  const int2 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const int nt = d.ntheta, nrow = d.nrow;
+  const int ntask = 2 * d.n_ss + d.n_gs + d.nl;
+  const int tl = threadIdx.x >> 5, th = threadIdx.x & 31;
+  const int task = w.y + tl;
+
+  for (int i = threadIdx.x; i < CLH_LEG_TASKS * nrow; i += CLH_BLOCK) {
+    const int tt = w.y + i / nrow;
+    double v = 0.0;
+    if (tt < ntask) {
+      int col;
+      if (tt < d.n_ss) col = tt;
+      else if (tt < 2 * d.n_ss) col = tt - d.n_ss;
+      else col = tt - d.n_ss;  // gs and gg columns follow the ss columns
+      v = d.cl[(size_t)col * nrow + (i % nrow)];
+    }
+    colbuf[i] = v;
+  }
+  __syncthreads();
+  if (task >= ntask || th >= nt) return;
+  int kind, q;
+  if (task < d.n_ss) { kind = 0; q = task; }
+  else if (task < 2 * d.n_ss) { kind = 1; q = task - d.n_ss; }
+  else if (task < 2 * d.n_ss + d.n_gs) { kind = 2; q = task - 2 * d.n_ss; }
+  else { kind = 3; q = task - 2 * d.n_ss - d.n_gs; }
+  const double* K = d.kfold + (size_t)kind * nrow * nt + th;
+  const double* c = colbuf + tl * nrow;
+  double acc = 0.0;
+  for (int r = 0; r < nrow; r++) acc += K[(size_t)r * nt] * c[r];
+
+  int idx;
+  double f = 1.0;
+  if (kind <= 1) {
+    int i, j;
+    clh_ss_pair(d.ns, q, i, j);
+    if (d.apply_calib) f = (1.0 + d.shear_m[i]) * (1.0 + d.shear_m[j]);
+    idx = d.dv_start[0] + nt * q + th + (kind == 1 ? nt * d.n_ss : 0);
+  } else if (kind == 2) {
+    if (d.apply_calib) f = 1.0 + d.shear_m[q % d.ns];
+    idx = d.dv_start[1] + nt * q + th;
+  } else {
+    idx = d.dv_start[2] + nt * q + th;
+  }
+  d.dv[idx] = acc * f;
+}
+
+// ----------------------------------------------------------------------------
+// k_legendre_full: the reference's interpolate-then-dot structure
+// (cosmo2D.c:468-533): C_ell at every integer ell < LMAX is generated on the
+// fly from the column's rows (ell < LMIN_tab direct, else the ln-ell fill) and
+// contracted with the full-length kernel.  Same task mapping as k_legendre.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(CLH_BLOCK) void k_legendre_full(const DrawDesc* __restrict__ D,
+                                                             const int2* __restrict__ work) {
+  extern __shared__ double colbuf[];
+  const int2 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const int nt = d.ntheta, nrow = d.nrow;
+  const int ntask = 2 * d.n_ss + d.n_gs + d.nl;
+  const int tl = threadIdx.x >> 5, th = threadIdx.x & 31;
+  const int task = w.y + tl;
+  for (int i = threadIdx.x; i < CLH_LEG_TASKS * nrow; i += CLH_BLOCK) {
+    const int tt = w.y + i / nrow;
+    double v = 0.0;
+    if (tt < ntask) {
+      const int col = (tt < d.n_ss) ? tt : tt - d.n_ss;
+      v = d.cl[(size_t)col * nrow + (i % nrow)];
+    }
+    colbuf[i] = v;
+  }
+  __syncthreads();
+  if (task >= ntask || th >= nt) return;
+  int kind, q;
+  if (task < d.n_ss) { kind = 0; q = task; }
+  else if (task < 2 * d.n_ss) { kind = 1; q = task - d.n_ss; }
+  else if (task < 2 * d.n_ss + d.n_gs) { kind = 2; q = task - 2 * d.n_ss; }
+  else { kind = 3; q = task - 2 * d.n_ss - d.n_gs; }
+  const int L = d.lmax;
+  const double* K = d.kfull + (size_t)kind * L * nt + th;
+  const double* fl = d.fill + (size_t)(kind <= 1 ? 0 : 1) * L * 2;
+  const double* c = colbuf + tl * nrow;
+  const double* tab = c + CLH_NLOW;
+  double acc = 0.0;
+  for (int l = 1; l < CLH_LMIN_TAB && l < L; l++) acc += K[(size_t)l * nt] * c[l - 1];
+  for (int l = CLH_LMIN_TAB; l < L; l++) {
+    const int ic = (int)fl[2 * l];
+    const double t = fl[2 * l + 1];
+    const double cl = tab[ic] + t * (tab[ic + 1] - tab[ic]);
+    acc += K[(size_t)l * nt] * cl;
+  }
+  int idx;
+  double f = 1.0;
+  if (kind <= 1) {
+    int i, j;
+    clh_ss_pair(d.ns, q, i, j);
+    if (d.apply_calib) f = (1.0 + d.shear_m[i]) * (1.0 + d.shear_m[j]);
+    idx = d.dv_start[0] + nt * q + th + (kind == 1 ? nt * d.n_ss : 0);
+  } else if (kind == 2) {
+    if (d.apply_calib) f = 1.0 + d.shear_m[q % d.ns];
+    idx = d.dv_start[1] + nt * q + th;
+  } else {
+    idx = d.dv_start[2] + nt * q + th;
+  }
+  d.dv[idx] = acc * f;
+}
+
+// ----------------------------------------------------------------------------
+// launch wrappers (called from the host runtime)
+// ----------------------------------------------------------------------------
+hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gtab, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  return hipGetLastError();
+}
+hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nodes, dim3(n), dim3(128), 0, s, D, work);
+  return hipGetLastError();
+}
+hipError_t clh_launch_limber(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_limber, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  return hipGetLastError();
+}
+hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const size_t lds = sizeof(double) * CLH_LEG_TASKS * (size_t)max_nrow;
+  hipLaunchKernelGGL(k_legendre, dim3(n), dim3(CLH_BLOCK), lds, s, D, work);
+  return hipGetLastError();
+}
+hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, int max_nrow,
+                                    hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const size_t lds = sizeof(double) * CLH_LEG_TASKS * (size_t)max_nrow;
+  hipLaunchKernelGGL(k_legendre_full, dim3(n), dim3(CLH_BLOCK), lds, s, D, work);
+  return hipGetLastError();
+}

@@ -1,0 +1,150 @@
+/*
+ * cosmolike_hip.h -- C ABI of the MI355X (gfx950) tomographic two-point
+ * projection library: Limber C_ell for shear-shear (ss), galaxy-shear (gs)
+ * and galaxy-galaxy (gg), and their bin-averaged Legendre transforms
+ * xi+-(theta), gamma_t(theta), w(theta).
+ *
+ * Plain pointers and sizes only.  Every entry point that can fail returns an
+ * int status (CLH_OK = 0, negative on error); it never exits the process and
+ * writes nothing to output buffers on error.  clh_last_error() gives the text.
+ *
+ * Three layers:
+ *  1. Context setters mirroring the cosmolike_interface setters that feed the
+ *     reference's global structs (each cites the routine it replaces).
+ *  2. Bulk compute calls: the real-space 3x2pt data vector in the reference's
+ *     order, per-probe arrays, C_ell tables, and a device-resident batch.
+ *  3. The reference's own scalar entry points (xi_pm_tomo, w_gammat_tomo,
+ *     w_gg_tomo, C_*_tomo_limber[_nointerp]) with identical signatures,
+ *     served from a process-wide default context; see clh_default().
+ */
+#ifndef COSMOLIKE_HIP_H
+#define COSMOLIKE_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLH_OK 0
+#define CLH_ERR_ARG (-1)     /* bad size, index or enum                        */
+#define CLH_ERR_NAN (-2)     /* NaN in an input table                          */
+#define CLH_ERR_STATE (-3)   /* an input the call needs has not been set       */
+#define CLH_ERR_DOMAIN (-4)  /* input outside the domain the algorithm needs   */
+#define CLH_ERR_HIP (-5)     /* HIP runtime failure                            */
+#define CLH_ERR_ALLOC (-6)   /* device or host allocation failed               */
+
+/* Legendre contraction variants (all FP64). FOLDED contracts the Limber
+ * table rows with cosmology-independent kernels into which the reference's
+ * linear-in-ln(ell) fill has been folded; FULL generates C_ell at every
+ * integer ell < LMAX on the fly and contracts over all of them, as the
+ * reference does. */
+#define CLH_LEGENDRE_FOLDED 0
+#define CLH_LEGENDRE_FULL 1
+
+typedef struct clh_ctx clh_ctx;
+
+/* ---------------------------------------------------------------- context */
+int clh_create(clh_ctx** out, int hip_device);
+void clh_destroy(clh_ctx* ctx);
+const char* clh_last_error(const clh_ctx* ctx);
+/* process-wide context behind the reference-named scalar entry points */
+clh_ctx* clh_default(void);
+int clh_set_legendre_mode(clh_ctx* ctx, int mode);
+
+/* ---------------------------------------------------------------- setters */
+/* init_accuracy_boost / init_ntable_lmax (generic_interface.cpp:284-342):
+ * N_ell log-spaced Limber ell, N_a lens-efficiency table size, LMAX,
+ * high_def_integration (only 0 is implemented: 64/96-point GL). */
+int clh_init_ntable(clh_ctx* ctx, int n_ell, int n_a, int lmax, int high_def);
+/* init_binning_real_space (generic_interface.cpp:479-505), arcmin */
+int clh_init_binning_real_space(clh_ctx* ctx, int ntheta, double theta_min_arcmin,
+                                double theta_max_arcmin);
+/* init_IA (generic_interface.cpp:628-634): ia_model 0 (NLA) only;
+ * ia_redshift_evol 0 (none), 2 (per-bin A) or 3 (A ((1+z)/1.62)^eta) */
+int clh_init_ia(clh_ctx* ctx, int ia_model, int ia_redshift_evol);
+/* set_cosmological_parameters (generic_interface.cpp:912-939) */
+int clh_set_cosmological_parameters(clh_ctx* ctx, double omega_matter, double hubble);
+/* set_distances (generic_interface.cpp:1085-1143): z, chi [Mpc/h], n >= 5 */
+int clh_set_distances(clh_ctx* ctx, const double* z, const double* chi, int n);
+/* set_growth (generic_interface.cpp:1149-1188): G = D/a */
+int clh_set_growth(clh_ctx* ctx, const double* z, const double* G, int n);
+/* set_non_linear_power_spectrum (generic_interface.cpp:1276-1352):
+ * lnP k-major, lnP[i*nz + j], P in (Mpc/h)^3, k in h/Mpc */
+int clh_set_non_linear_power_spectrum(clh_ctx* ctx, const double* log10k, int nk,
+                                      const double* z, int nz, const double* lnP);
+/* set_source_sample / set_lens_sample (generic_interface.cpp:1907-1999,
+ * 1802-1883): row-major (nrows, ntomo+1) table, column 0 = z */
+int clh_set_source_sample(clh_ctx* ctx, const double* table, int nrows, int ntomo);
+int clh_set_lens_sample(clh_ctx* ctx, const double* table, int nrows, int ntomo);
+/* set_nuisance_* (generic_interface.cpp:1358-1610, 1701-1775) */
+int clh_set_nuisance_shear_calib(clh_ctx* ctx, const double* m, int n);
+int clh_set_nuisance_shear_photoz(clh_ctx* ctx, const double* dz, int n);
+int clh_set_nuisance_clustering_photoz(clh_ctx* ctx, const double* dz, int n);
+int clh_set_nuisance_clustering_photoz_stretch(clh_ctx* ctx, const double* s, int n);
+int clh_set_nuisance_linear_bias(clh_ctx* ctx, const double* b1, int n);
+int clh_set_nuisance_magnification_bias(clh_ctx* ctx, const double* bmag, int n);
+/* set_nuisance_IA: mode 2 uses A1[0..ns-1]; mode 3 uses A1[0]=A, A1[1]=eta */
+int clh_set_nuisance_ia(clh_ctx* ctx, const double* A1, int n);
+
+/* ----------------------------------------------------------- bulk compute */
+int clh_datavector_size(clh_ctx* ctx);
+/* compute_Mx2pt_N_masked<0,3>(ord) (generic_interface.hpp:1137-1159) with an
+ * all-ones mask: xi+ | xi- | gamma_t | w blocks placed by ord, shear
+ * calibration applied; dv has clh_datavector_size() entries (host memory) */
+int clh_compute_3x2pt_real(clh_ctx* ctx, const int ord[3], double* dv, int n);
+/* per-probe arrays as the reference's static caches hold them (q = pair*Ntheta + i),
+ * without calibration: xi_pm_tomo (cosmo2D.c:353), w_gammat_tomo (:562),
+ * w_gg_tomo with limber=1 (:739) */
+int clh_xi_pm(clh_ctx* ctx, double* xip, double* xim);
+int clh_gammat(clh_ctx* ctx, double* out);
+int clh_wtheta(clh_ctx* ctx, double* out);
+/* Limber tables at the N_ell log-spaced ell (C_ss/gs/gg_tomo_limber tables,
+ * cosmo2D.c:1617, 2195, 2693): ss[n_ss][N_ell], gs[n_gs][N_ell], gg[nl][N_ell] */
+int clh_cl_tables(clh_ctx* ctx, double* ss, double* gs, double* gg);
+/* Direct GL C_ell at arbitrary ell (C_*_tomo_limber_nointerp,
+ * cosmo2D.c:1562, 2125, 2672): probe 0=ss(EE), 1=gs, 2=gg; out[pair][nl] */
+int clh_cl_nointerp(clh_ctx* ctx, int probe, const double* ell, int nl, double* out);
+
+/* ------------------------------------------------ device-resident batches */
+/* One draw's cosmology and nuisance (host pointers; NULL nuisance arrays
+ * mean zero shift / unit stretch / zero m / zero b_mag; b1 must be given when
+ * the layout has lens bins). The survey layout (n(z), binning, Ntable) is the
+ * context's. */
+typedef struct {
+  double omega_m;
+  const double* chi_z; const double* chi; int chi_n;
+  const double* G_z; const double* G; int G_n;
+  const double* log10k; const double* pk_z; const double* lnP; int nk; int npz;
+  const double* src_dz; const double* lens_dz; const double* lens_stretch;
+  const double* b1; const double* bmag; const double* shear_m;
+  double ia_A; double ia_eta; const double* ia_bin;
+} clh_draw;
+
+/* uploads B draws into device memory owned by the context */
+int clh_batch_upload(clh_ctx* ctx, int B, const clh_draw* draws);
+/* runs the whole path for every uploaded draw on the context's stream;
+ * dv_device: device pointer to B * clh_datavector_size() doubles */
+int clh_batch_forward(clh_ctx* ctx, const int ord[3], double* dv_device);
+int clh_synchronize(clh_ctx* ctx);
+/* per-kernel timing of the last clh_batch_forward (HIP events on the context
+ * stream): names[i], milliseconds[i]; returns the kernel count */
+int clh_last_kernel_times(clh_ctx* ctx, const char** names, float* ms, int cap);
+int clh_enable_timing(clh_ctx* ctx, int on);
+
+/* ---------------------------------------- reference-named scalar entries */
+/* Identical signatures to cosmolike/cosmo2D.h:30-96, served from clh_default().
+ * On an error they return NaN and record the message (clh_last_error on
+ * clh_default()) instead of exiting. */
+double xi_pm_tomo(const int pm, const int nt, const int ni, const int nj, const int limber);
+double w_gammat_tomo(const int nt, const int ni, const int nj, const int limber);
+double w_gg_tomo(const int nt, const int ni, const int nj, const int limber);
+double C_ss_tomo_limber(const double l, const int ni, const int nj, const int EE);
+double C_gs_tomo_limber(const double l, const int ni, const int nj);
+double C_gg_tomo_limber(const double l, const int ni, const int nj);
+double C_ss_tomo_limber_nointerp(const double l, const int ni, const int nj, const int EE,
+                                 const int init);
+double C_gs_tomo_limber_nointerp(const double l, const int nl, const int ns, const int init);
+double C_gg_tomo_limber_nointerp(const double l, const int ni, const int nj, const int init);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

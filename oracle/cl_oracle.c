@@ -1,0 +1,1362 @@
+/*
+ * cl_oracle.c -- TEST INFRASTRUCTURE ONLY (parity oracle + CPU baseline).
+ *
+ * FP64 CPU restatement of CosmoLike's tomographic two-point projection path.
+ * Only tests/, the smoke check and the benchmark's CPU-baseline leg load it;
+ * the HIP library never does.  Each routine cites the reference routine whose
+ * algorithm it restates.  Third-party arithmetic the reference delegates to
+ * GSL (version not pinned anywhere in the reference tree) is restated from the
+ * published algorithms:
+ *   - gsl_integration_glfixed: n-point Gauss-Legendre on [a,b]; nodes here come
+ *     from Newton iteration on P_n (pinned against numpy.leggauss and
+ *     polynomial exactness in the tests);
+ *   - gsl_interp_cspline: natural cubic spline (pinned against
+ *     scipy CubicSpline(bc_type='natural'));
+ *   - gsl_sf_legendre_Pl_deriv_array: Bonnet upward recurrence for P_l and
+ *     (1-x^2) P'_l = l (P_{l-1} - x P_l) (pinned against mpmath).
+ */
+#include "cl_oracle.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OC_MAXB 20
+#define OC_LMIN_TAB 20                 /* structs.c:44 */
+#define OC_A_MIN (1.0 / (1.0 + 40.0))  /* structs.c:41 */
+#define OC_COVERH0 2997.92458          /* structs.c:119 */
+
+struct oc_ctx {
+  int n_ell, n_a, lmax, hdi;
+  int ntheta;
+  double vtmin, vtmax;
+  double omega_m;
+  int chi_nz, G_nz, nk, nzp;
+  double *chi_z, *chi_v, *G_z, *G_v, *lk, *pz, *lnP;
+  /* source sample: histogram + spline through bin centres */
+  int ns, s_rows;
+  double *s_tab;  /* [ns+1][rows]; row ns = z column */
+  double s_zmin_all, s_zmax_all, s_zmin[OC_MAXB], s_zmax[OC_MAXB];
+  double *s_zc, *s_y, *s_c; /* centres [rows]; y,c [ns+1][rows] */
+  /* lens sample */
+  int nl, l_rows;
+  double *l_tab;
+  double l_zmin_all, l_zmax_all, l_zmin[OC_MAXB], l_zmax[OC_MAXB], l_zmean[OC_MAXB];
+  double *l_zc, *l_y, *l_c;
+  /* nuisance */
+  double src_dz[OC_MAXB], lens_dz[OC_MAXB], lens_st[OC_MAXB], b1[OC_MAXB],
+      bmag[OC_MAXB], shear_m[OC_MAXB];
+  int ia_mode;
+  double ia_bin[OC_MAXB], ia_A, ia_eta, oneplusz0, c1rhocrit;
+  /* caches */
+  int g_valid;
+  double *gt_tab, *gl_tab; /* [ns][N_a], [nl][N_a] */
+  int K_valid;
+  double *K; /* [4][ntheta][lmax] */
+  int status;
+  char err[256];
+};
+
+static void oc_set_err(oc_ctx* c, const char* fmt, ...) {
+  if (c->status == 0) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof c->err, fmt, ap);
+    va_end(ap);
+  }
+  c->status = -1;
+}
+
+const char* oc_last_error(oc_ctx* c) { return c->err; }
+
+oc_ctx* oc_create(void) {
+  oc_ctx* c = (oc_ctx*)calloc(1, sizeof(oc_ctx));
+  if (!c) return NULL;
+  /* defaults: structs.c:262-279, reset_nuisance_struct (structs.c:207-237) */
+  c->n_ell = 300;
+  c->n_a = 225;
+  c->lmax = 50000;
+  c->hdi = 0;
+  c->c1rhocrit = 0.01389;
+  c->oneplusz0 = 1.62; /* set_nuisance_IA, generic_interface.cpp:1756 */
+  for (int i = 0; i < OC_MAXB; i++) c->lens_st[i] = 1.0;
+  return c;
+}
+
+void oc_free(oc_ctx* c) {
+  if (!c) return;
+  free(c->chi_z); free(c->chi_v); free(c->G_z); free(c->G_v);
+  free(c->lk); free(c->pz); free(c->lnP);
+  free(c->s_tab); free(c->s_zc); free(c->s_y); free(c->s_c);
+  free(c->l_tab); free(c->l_zc); free(c->l_y); free(c->l_c);
+  free(c->gt_tab); free(c->gl_tab); free(c->K);
+  free(c);
+}
+
+void oc_touch(oc_ctx* c) { c->g_valid = 0; }
+
+static double* dup(const double* x, int n) {
+  double* y = (double*)malloc(sizeof(double) * (size_t)n);
+  memcpy(y, x, sizeof(double) * (size_t)n);
+  return y;
+}
+
+static int bsearch_lo(const double* x, int n, double v) {
+  /* the ilo/ihi bisection used throughout cosmo3D.c (e.g. :46-57) */
+  size_t ilo = 0, ihi = (size_t)n - 1;
+  while (ihi > ilo + 1) {
+    size_t ll = (ihi + ilo) / 2;
+    if (x[ll] > v) ihi = ll; else ilo = ll;
+  }
+  return (int)ilo;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Gauss-Legendre (restates the published gsl_integration_glfixed rule)    */
+/* ---------------------------------------------------------------------- */
+int oc_gl_nodes(int n, double* x, double* w) {
+  const int m = (n + 1) / 2;
+  for (int i = 0; i < m; i++) {
+    double z = cos(M_PI * (i + 0.75) / (n + 0.5));
+    double pp = 0.0;
+    for (int it = 0; it < 100; it++) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; j++) {
+        double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      double z1 = z;
+      z = z1 - p1 / pp;
+      if (fabs(z - z1) < 1e-16) break;
+    }
+    /* recompute derivative at the converged node */
+    {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; j++) {
+        double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+    }
+    const double wi = 2.0 / ((1.0 - z * z) * pp * pp);
+    x[i] = -z;          /* ascending order */
+    x[n - 1 - i] = z;
+    w[i] = wi;
+    w[n - 1 - i] = wi;
+  }
+  if (n % 2 == 1) x[n / 2] = 0.0;
+  return 0;
+}
+
+typedef struct { int n; double x[1024]; double w[1024]; } gl_rule;
+
+static void gl_make(gl_rule* r, int n) { r->n = n; oc_gl_nodes(n, r->x, r->w); }
+
+static void gl_point(const gl_rule* r, double a, double b, int p, double* xi, double* wi) {
+  const double A = 0.5 * (b - a), B = 0.5 * (a + b);
+  *xi = B + A * r->x[p];
+  *wi = A * r->w[p];
+}
+
+/* GL order selection: cosmo2D.c:1577-1581 (ss/gs) and 2634-2638 (gg) */
+static int gl_order_ssgs(int hdi) {
+  hdi = abs(hdi);
+  return hdi == 0 ? 64 : hdi == 1 ? 96 : hdi == 2 ? 128 : hdi == 3 ? 256 : 512;
+}
+static int gl_order_gg(int hdi) {
+  hdi = abs(hdi);
+  return hdi == 0 ? 96 : hdi == 1 ? 128 : hdi == 2 ? 256 : hdi == 3 ? 512 : 1024;
+}
+
+/* ---------------------------------------------------------------------- */
+/* natural cubic spline (restates gsl_interp_cspline)                      */
+/* ---------------------------------------------------------------------- */
+static void spline_init(int n, const double* x, const double* y, double* c) {
+  /* c = second derivative / 2; natural: c[0] = c[n-1] = 0 */
+  c[0] = 0.0;
+  c[n - 1] = 0.0;
+  const int sys = n - 2;
+  if (sys <= 0) return;
+  double* diag = (double*)malloc(sizeof(double) * sys);
+  double* off = (double*)malloc(sizeof(double) * sys);
+  double* g = (double*)malloc(sizeof(double) * sys);
+  for (int i = 0; i < sys; i++) {
+    const double h_i = x[i + 1] - x[i];
+    const double h_ip1 = x[i + 2] - x[i + 1];
+    const double ydiff_i = y[i + 1] - y[i];
+    const double ydiff_ip1 = y[i + 2] - y[i + 1];
+    const double g_i = (h_i != 0.0) ? 1.0 / h_i : 0.0;
+    const double g_ip1 = (h_ip1 != 0.0) ? 1.0 / h_ip1 : 0.0;
+    off[i] = h_ip1;
+    diag[i] = 2.0 * (h_ip1 + h_i);
+    g[i] = 3.0 * (ydiff_ip1 * g_ip1 - ydiff_i * g_i);
+  }
+  /* symmetric tridiagonal solve (Thomas) */
+  for (int i = 1; i < sys; i++) {
+    const double m = off[i - 1] / diag[i - 1];
+    diag[i] -= m * off[i - 1];
+    g[i] -= m * g[i - 1];
+  }
+  c[sys] = g[sys - 1] / diag[sys - 1];
+  for (int i = sys - 2; i >= 0; i--) c[i + 1] = (g[i] - off[i] * c[i + 2]) / diag[i];
+  free(diag); free(off); free(g);
+}
+
+static double spline_eval(int n, const double* x, const double* y, const double* c, double xx) {
+  int i = bsearch_lo(x, n, xx);
+  if (i > n - 2) i = n - 2;
+  const double h = x[i + 1] - x[i];
+  const double dy = y[i + 1] - y[i];
+  const double dx = xx - x[i];
+  const double b = dy / h - h * (c[i + 1] + 2.0 * c[i]) / 3.0;
+  const double d = (c[i + 1] - c[i]) / (3.0 * h);
+  return y[i] + dx * (b + dx * (c[i] + dx * d));
+}
+
+/* ---------------------------------------------------------------------- */
+/* background (cosmolike/cosmo3D.c)                                        */
+/* ---------------------------------------------------------------------- */
+/* chi_all: cosmo3D.c:40-92 */
+static void chi_all(oc_ctx* c, double a, double* chi, double* dchida) {
+  const double z = 1.0 / a - 1.0;
+  const double* zz = c->chi_z;
+  const double* cc = c->chi_v;
+  const int j = bsearch_lo(zz, c->chi_nz, z);
+  if (j + 2 > c->chi_nz - 1) { /* the reference reads chi[j+2] (cosmo3D.c:66) */
+    oc_set_err(c, "chi_all: z=%g needs chi table beyond its last two nodes", z);
+    *chi = NAN; *dchida = NAN; return;
+  }
+  const double dy = (z - zz[j]) / (zz[j + 1] - zz[j]);
+  const double out0 = cc[j] + dy * (cc[j + 1] - cc[j]);
+  double out1;
+  const double up = (cc[j + 2] - cc[j]) / (zz[j + 2] - zz[j]);
+  if (j > 0) {
+    const double down = (cc[j + 1] - cc[j - 1]) / (zz[j + 1] - zz[j - 1]);
+    out1 = down + dy * (up - down);
+  } else {
+    const double down = (cc[j + 1] - cc[j]) / (zz[j + 1] - zz[j]);
+    out1 = down + dy * (up - down);
+  }
+  out1 = out1 / OC_COVERH0;
+  out1 = out1 / (a * a);
+  *chi = out0 / OC_COVERH0;
+  *dchida = out1;
+}
+
+static double chi_of_a(oc_ctx* c, double a) { double x, d; chi_all(c, a, &x, &d); return x; }
+
+/* hoverh0v2: cosmo3D.c:104-107 */
+static double hoverh0v2(double a, double dchida) { return 1.0 / ((a * a) * dchida); }
+/* hoverh0: cosmo3D.c:99-102 via dchi_dz (:94-97) */
+static double hoverh0(oc_ctx* c, double a) {
+  double x, d;
+  chi_all(c, a, &x, &d);
+  return 1.0 / ((a * a) * d);
+}
+
+/* a_chi: cosmo3D.c:109-133 */
+static double a_chi(oc_ctx* c, double io_chi) {
+  const double chi = io_chi * OC_COVERH0;
+  const int j = bsearch_lo(c->chi_v, c->chi_nz, chi);
+  const double dy = (chi - c->chi_v[j]) / (c->chi_v[j + 1] - c->chi_v[j]);
+  const double z = c->chi_z[j] + dy * (c->chi_z[j + 1] - c->chi_z[j]);
+  return 1.0 / (1.0 + z);
+}
+
+/* norm_growfac(a, true): cosmo3D.c:148-198 */
+static double growfac(oc_ctx* c, double a) {
+  int j = bsearch_lo(c->G_z, c->G_nz, 0.0);
+  double dy = (0.0 - c->G_z[j]) / (c->G_z[j + 1] - c->G_z[j]);
+  const double g1 = c->G_v[j] + dy * (c->G_v[j + 1] - c->G_v[j]);
+  const double z = 1.0 / a - 1.0;
+  j = bsearch_lo(c->G_z, c->G_nz, z);
+  dy = (z - c->G_z[j]) / (c->G_z[j + 1] - c->G_z[j]);
+  const double G = c->G_v[j] + dy * (c->G_v[j + 1] - c->G_v[j]);
+  return (G * a) / g1;
+}
+
+/* f_growth: cosmo3D.c:200-228 */
+static double f_growth(oc_ctx* c, double z) {
+  const int j = bsearch_lo(c->G_z, c->G_nz, z);
+  const double dy = (z - c->G_z[j]) / (c->G_z[j + 1] - c->G_z[j]);
+  const double G = c->G_v[j] + dy * (c->G_v[j + 1] - c->G_v[j]);
+  const double dlnGdlnz = ((c->G_v[j + 1] - c->G_v[j]) / (c->G_z[j + 1] - c->G_z[j])) * z / G;
+  const double dlnGdlna = -dlnGdlnz * (1 + z) / z;
+  return 1 + dlnGdlna;
+}
+
+/* p_nonlin (no baryons): cosmo3D.c:362-417; Pdelta -> p_nonlin (:424-445) */
+static double p_nonlin(oc_ctx* c, double k, double a) {
+  const double log10k = log10(k / OC_COVERH0);
+  const double z = 1.0 / a - 1.0;
+  const int nk = c->nk, nz = c->nzp;
+  const int i = bsearch_lo(c->lk, nk, log10k);
+  const int j = bsearch_lo(c->pz, nz, z);
+  const double dx = (log10k - c->lk[i]) / (c->lk[i + 1] - c->lk[i]);
+  const double dy = (z - c->pz[j]) / (c->pz[j + 1] - c->pz[j]);
+  const double* P = c->lnP;
+  const double out = (1 - dx) * (1 - dy) * P[i * nz + j] + (1 - dx) * dy * P[i * nz + j + 1] +
+                     dx * (1 - dy) * P[(i + 1) * nz + j] + dx * dy * P[(i + 1) * nz + j + 1];
+  return exp(out) / (OC_COVERH0 * OC_COVERH0 * OC_COVERH0);
+}
+
+/* ---------------------------------------------------------------------- */
+/* redshift distributions (cosmolike/redshift_spline.c)                    */
+/* ---------------------------------------------------------------------- */
+/* zdistr_histo_n / pf_histo_n: redshift_spline.c:383-410, 597-628 */
+static double histo_n(const double* tab, int ntomo, int nzbins, double zmin_all,
+                      double zmax_all, double z, int ni) {
+  double res = 0.0;
+  if ((z >= zmin_all) && (z < zmax_all)) {
+    const double* z_v = tab + (size_t)ntomo * nzbins;
+    const double dz_histo = (z_v[nzbins - 1] - z_v[0]) / ((double)nzbins - 1.);
+    const int nj = (int)floor((z - z_v[0]) / dz_histo);
+    if (nj < 0 || nj > nzbins - 1) return NAN;
+    res = tab[(size_t)ni * nzbins + nj];
+  }
+  return res;
+}
+
+/* table + spline build of nz_source_photoz / nz_lens_photoz
+ * (redshift_spline.c:446-512 and 665-731) */
+static int build_nz_splines(oc_ctx* c, const double* tab, int ntomo, int nzbins, double zmin,
+                            double zmax, double** zc_out, double** y_out, double** c_out) {
+  double* zc = (double*)malloc(sizeof(double) * nzbins);
+  double* y = (double*)calloc((size_t)(ntomo + 1) * nzbins, sizeof(double));
+  double* cc = (double*)calloc((size_t)(ntomo + 1) * nzbins, sizeof(double));
+  const double dz_histo = (zmax - zmin) / ((double)nzbins);
+  for (int k = 0; k < nzbins; k++) zc[k] = zmin + (k + 0.5) * dz_histo;
+  double NORM[OC_MAXB];
+  double norm = 0;
+  for (int i = 0; i < ntomo; i++) {
+    NORM[i] = 0.0;
+    for (int k = 0; k < nzbins; k++)
+      NORM[i] += histo_n(tab, ntomo, nzbins, zmin, zmax, zc[k], i) * dz_histo;
+    if (!(NORM[i] > 0.0)) {
+      oc_set_err(c, "zero/negative n(z) normalization for bin %d", i);
+      free(zc); free(y); free(cc);
+      return -1;
+    }
+    norm += NORM[i];
+  }
+  for (int k = 0; k < nzbins; k++) {
+    y[k] = 0;
+    for (int i = 0; i < ntomo; i++) {
+      y[(size_t)(i + 1) * nzbins + k] = histo_n(tab, ntomo, nzbins, zmin, zmax, zc[k], i) / NORM[i];
+      y[k] += y[(size_t)(i + 1) * nzbins + k] * NORM[i] / norm;
+    }
+  }
+  for (int i = 0; i < ntomo + 1; i++)
+    spline_init(nzbins, zc, y + (size_t)i * nzbins, cc + (size_t)i * nzbins);
+  *zc_out = zc; *y_out = y; *c_out = cc;
+  return 0;
+}
+
+/* nz_source_photoz evaluation: redshift_spline.c:513-537 */
+static double nz_source(oc_ctx* c, double zz, int nj) {
+  const int n = c->s_rows;
+  zz = zz - c->src_dz[nj];
+  if (zz < c->s_zc[0] || zz > c->s_zc[n - 1]) return 0.0;
+  return spline_eval(n, c->s_zc, c->s_y + (size_t)(nj + 1) * n, c->s_c + (size_t)(nj + 1) * n, zz);
+}
+
+/* nz_lens_photoz evaluation: redshift_spline.c:736-762 */
+static double nz_lens(oc_ctx* c, double zz, int nj) {
+  const int n = c->l_rows;
+  zz = (zz - c->lens_dz[nj] - c->l_zmean[nj]) / c->lens_st[nj] + c->l_zmean[nj];
+  if (zz <= c->l_zc[0] || zz >= c->l_zc[n - 1]) return 0.0;
+  const double r = spline_eval(n, c->l_zc, c->l_y + (size_t)(nj + 1) * n, c->l_c + (size_t)(nj + 1) * n, zz);
+  return r / c->lens_st[nj];
+}
+
+/* bin limits from a histogram column: set_source_sample / set_lens_sample
+ * (generic_interface.cpp:1862-1867, 1966-1971) */
+static void bin_limits(const double* tab, int ntomo, int nrows, int k, double* zlo, double* zhi) {
+  const double* z_v = tab + (size_t)ntomo * nrows;
+  const double* col = tab + (size_t)k * nrows;
+  double mx = col[0];
+  for (int i = 1; i < nrows; i++) if (col[i] > mx) mx = col[i];
+  int first = -1, last = -1;
+  for (int i = 0; i < nrows; i++)
+    if (col[i] > 0.999e-8 * mx) { if (first < 0) first = i; last = i; }
+  if (first < 0) { first = 0; last = nrows - 1; }
+  *zlo = z_v[first];
+  *zhi = z_v[last];
+}
+
+int oc_set_source_sample(oc_ctx* c, const double* table, int nrows, int ntomo) {
+  c->status = 0;
+  if (ntomo <= 0 || ntomo > OC_MAXB || nrows < 5) { oc_set_err(c, "bad source sample size"); return -1; }
+  for (int i = 1; i < nrows; i++)
+    if (!(table[(size_t)i * (ntomo + 1)] > table[(size_t)(i - 1) * (ntomo + 1)])) {
+      oc_set_err(c, "bad n(z) file (z vector not monotonic)"); return -1;
+    }
+  free(c->s_tab);
+  c->s_tab = (double*)malloc(sizeof(double) * (size_t)(ntomo + 1) * nrows);
+  for (int i = 0; i < nrows; i++) {
+    c->s_tab[(size_t)ntomo * nrows + i] = table[(size_t)i * (ntomo + 1)];
+    for (int k = 0; k < ntomo; k++) c->s_tab[(size_t)k * nrows + i] = table[(size_t)i * (ntomo + 1) + k + 1];
+  }
+  c->ns = ntomo;
+  c->s_rows = nrows;
+  const double* z_v = c->s_tab + (size_t)ntomo * nrows;
+  c->s_zmin_all = fmax(z_v[0], 1.e-5);
+  c->s_zmax_all = z_v[nrows - 1] + (z_v[nrows - 1] - z_v[0]) / ((double)nrows - 1.);
+  for (int k = 0; k < ntomo; k++) {
+    double lo, hi;
+    bin_limits(c->s_tab, ntomo, nrows, k, &lo, &hi);
+    c->s_zmin[k] = fmax(lo, 1.001e-5);
+    c->s_zmax[k] = hi;
+  }
+  if (c->s_zmax_all < c->s_zmax[ntomo - 1] || c->s_zmin_all > c->s_zmin[0]) {
+    oc_set_err(c, "source n(z) range inconsistent"); return -1;
+  }
+  free(c->s_zc); free(c->s_y); free(c->s_c);
+  c->s_zc = c->s_y = c->s_c = NULL;
+  if (build_nz_splines(c, c->s_tab, ntomo, nrows, c->s_zmin_all, c->s_zmax_all, &c->s_zc, &c->s_y, &c->s_c))
+    return -1;
+  c->g_valid = 0;
+  return 0;
+}
+
+/* zmean (redshift_spline.c:794-844): 256-point GL of z n(z) / n(z) over the
+ * bin's support, evaluated when set_lens_sample runs (zero shift, unit stretch,
+ * zmean still 0 -- generic_interface.cpp:1874-1876) */
+static void compute_lens_zmean(oc_ctx* c) {
+  static gl_rule r; /* 256/512/1024 points by accuracy level (:813-815) */
+  const int hd = abs(c->hdi);
+  gl_make(&r, hd == 0 ? 256 : hd == 1 ? 512 : 1024);
+  double save_dz[OC_MAXB], save_st[OC_MAXB], save_zm[OC_MAXB];
+  memcpy(save_dz, c->lens_dz, sizeof save_dz);
+  memcpy(save_st, c->lens_st, sizeof save_st);
+  memcpy(save_zm, c->l_zmean, sizeof save_zm);
+  for (int i = 0; i < OC_MAXB; i++) { c->lens_dz[i] = 0.0; c->lens_st[i] = 1.0; c->l_zmean[i] = 0.0; }
+  double zm[OC_MAXB];
+  for (int i = 0; i < c->nl; i++) {
+    double num = 0.0, den = 0.0;
+    for (int p = 0; p < r.n; p++) {
+      double x, w;
+      gl_point(&r, c->l_zmin[i], c->l_zmax[i], p, &x, &w);
+      const double n = nz_lens(c, x, i);
+      num += w * x * n;
+      den += w * n;
+    }
+    zm[i] = num / den;
+  }
+  memcpy(c->lens_dz, save_dz, sizeof save_dz);
+  memcpy(c->lens_st, save_st, sizeof save_st);
+  for (int i = 0; i < c->nl; i++) c->l_zmean[i] = zm[i];
+  (void)save_zm;
+}
+
+int oc_set_lens_sample(oc_ctx* c, const double* table, int nrows, int ntomo) {
+  c->status = 0;
+  if (ntomo <= 0 || ntomo > OC_MAXB || nrows < 5) { oc_set_err(c, "bad lens sample size"); return -1; }
+  for (int i = 1; i < nrows; i++)
+    if (!(table[(size_t)i * (ntomo + 1)] > table[(size_t)(i - 1) * (ntomo + 1)])) {
+      oc_set_err(c, "bad n(z) file (z vector not monotonic)"); return -1;
+    }
+  free(c->l_tab);
+  c->l_tab = (double*)malloc(sizeof(double) * (size_t)(ntomo + 1) * nrows);
+  for (int i = 0; i < nrows; i++) {
+    c->l_tab[(size_t)ntomo * nrows + i] = table[(size_t)i * (ntomo + 1)];
+    for (int k = 0; k < ntomo; k++) c->l_tab[(size_t)k * nrows + i] = table[(size_t)i * (ntomo + 1) + k + 1];
+  }
+  c->nl = ntomo;
+  c->l_rows = nrows;
+  const double* z_v = c->l_tab + (size_t)ntomo * nrows;
+  c->l_zmin_all = fmax(z_v[0], 1.e-5);
+  c->l_zmax_all = z_v[nrows - 1] + (z_v[nrows - 1] - z_v[0]) / ((double)nrows - 1.);
+  for (int k = 0; k < ntomo; k++) bin_limits(c->l_tab, ntomo, nrows, k, &c->l_zmin[k], &c->l_zmax[k]);
+  free(c->l_zc); free(c->l_y); free(c->l_c);
+  c->l_zc = c->l_y = c->l_c = NULL;
+  if (build_nz_splines(c, c->l_tab, ntomo, nrows, c->l_zmin_all, c->l_zmax_all, &c->l_zc, &c->l_y, &c->l_c))
+    return -1;
+  compute_lens_zmean(c);
+  c->g_valid = 0;
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* lens efficiencies: g_tomo (redshift_spline.c:852-923), g_lens (1014-1109) */
+/* ---------------------------------------------------------------------- */
+static void build_g_tables(oc_ctx* c) {
+  const int x = 25 * (1 + abs(c->hdi));
+  const int Na = x * (c->n_a - 1) + 1;
+  const double amax = 0.999999;
+  free(c->gt_tab); free(c->gl_tab);
+  c->gt_tab = (double*)calloc((size_t)c->ns * c->n_a, sizeof(double));
+  c->gl_tab = c->nl > 0 ? (double*)calloc((size_t)c->nl * c->n_a, sizeof(double)) : NULL;
+  {
+    const double amin = 1.0 / (c->s_zmax_all + 1.0);
+    const double da = (amax - amin) / ((double)Na - 1.0);
+    #pragma omp parallel for schedule(static)
+    for (int j = 0; j < c->ns; j++) {
+      double* Pint = (double*)malloc(sizeof(double) * Na);
+      double* Qint = (double*)malloc(sizeof(double) * Na);
+      for (int i = 0; i < Na; i++) {
+        const double a = amin + i * da;
+        Pint[i] = nz_source(c, 1. / a - 1., j) / (a * a);
+        const double ch = chi_of_a(c, amin + i * da);
+        Qint[i] = Pint[i] / ch;
+      }
+      double P = 0.0, Q = 0.0;
+      double* tab = c->gt_tab + (size_t)j * c->n_a;
+      tab[0] = 0.0;
+      for (int i = 1; i < Na; i++) {
+        P += 0.5 * da * (Pint[i - 1] + Pint[i]);
+        Q += 0.5 * da * (Qint[i - 1] + Qint[i]);
+        if (i % x == 0) tab[i / x] = P - chi_of_a(c, amin + i * da) * Q;
+      }
+      free(Pint); free(Qint);
+    }
+  }
+  if (c->nl > 0) {
+    const double amin = 1.0 / (c->l_zmax_all + 1.0);
+    const double amin_shear = 1.0 / (c->s_zmax_all + 1.0);
+    const double da = (amax - amin) / ((double)Na - 1.0);
+    const int Na0 = (amin_shear < amin) ? (int)ceil((amin - amin_shear) / da) + 1 : 1;
+    const double da0 = (Na0 > 1) ? (amin - amin_shear) / ((double)Na0 - 1.) : 0.0;
+    #pragma omp parallel for schedule(static)
+    for (int j = 0; j < c->nl; j++) {
+      double P = 0.0, Q = 0.0;
+      for (int i = 1; i < Na0; i++) {
+        const double apl = amin_shear + (i - 1) * da0;
+        const double aph = amin_shear + i * da0;
+        const double fl = nz_lens(c, 1.0 / apl - 1.0, j) / (apl * apl);
+        const double fh = nz_lens(c, 1.0 / aph - 1.0, j) / (aph * aph);
+        P += 0.5 * da0 * (fl + fh);
+        Q += 0.5 * da0 * (fl / chi_of_a(c, apl) + fh / chi_of_a(c, aph));
+      }
+      double* Pint = (double*)malloc(sizeof(double) * Na);
+      double* Qint = (double*)malloc(sizeof(double) * Na);
+      for (int i = 0; i < Na; i++) {
+        const double ap = amin + i * da;
+        Pint[i] = nz_lens(c, 1.0 / ap - 1.0, j) / (ap * ap);
+        Qint[i] = Pint[i] / chi_of_a(c, amin + i * da);
+      }
+      double* tab = c->gl_tab + (size_t)j * c->n_a;
+      tab[0] = P - chi_of_a(c, amin) * Q;
+      for (int i = 1; i < Na; i++) {
+        P += 0.5 * da * (Pint[i - 1] + Pint[i]);
+        Q += 0.5 * da * (Qint[i - 1] + Qint[i]);
+        if (i % x == 0) tab[i / x] = P - chi_of_a(c, amin + i * da) * Q;
+      }
+      free(Pint); free(Qint);
+    }
+  }
+  c->g_valid = 1;
+}
+
+/* interpol1d: basics.c:457-480 */
+static double interpol1d(const double* f, int n, double a, double b, double dx, double x) {
+  (void)b;
+  if (x < a) return f[0];
+  const double r = (x - a) / dx;
+  const int i = (int)floor(r);
+  if (i + 1 >= n) return f[n - 1];
+  return (r - i) * (f[i + 1] - f[i]) + f[i];
+}
+
+static double g_tomo(oc_ctx* c, double a, int ni) {
+  const double amin = 1.0 / (c->s_zmax_all + 1.0), amax = 0.999999;
+  const double dac = (amax - amin) / ((double)c->n_a - 1.0);
+  return (a <= amin || a > 1.0 - dac) ? 0.0
+         : interpol1d(c->gt_tab + (size_t)ni * c->n_a, c->n_a, amin, amax, dac, a);
+}
+
+static double g_lens(oc_ctx* c, double a, int ni) {
+  const double amin = 1.0 / (c->l_zmax_all + 1.0), amax = 0.999999;
+  const double dac = (amax - amin) / ((double)c->n_a - 1.0);
+  return (a < amin || a > 1.0 - dac) ? 0.0
+         : interpol1d(c->gl_tab + (size_t)ni * c->n_a, c->n_a, amin, amax, dac, a);
+}
+
+/* ---------------------------------------------------------------------- */
+/* radial weights (radial_weights.c:15-105), IA (IA.c:238-303)             */
+/* ---------------------------------------------------------------------- */
+static double W_kappa(oc_ctx* c, double a, double fK, int nz) {
+  return (1.5 * c->omega_m * fK / a) * g_tomo(c, a, nz);
+}
+static double W_mag(oc_ctx* c, double a, double fK, int nz) {
+  return (1.5 * c->omega_m * fK / a) * g_lens(c, a, nz);
+}
+static double W_gal(oc_ctx* c, double a, int ni, double hh) {
+  const double z = 1. / a - 1;
+  return nz_lens(c, z, ni) * hh;
+}
+static double W_source(oc_ctx* c, double a, int ni, double hh) {
+  const double z = 1.0 / a - 1.0;
+  return nz_source(c, z, ni) * hh;
+}
+static double f_rsd(oc_ctx* c, double a) { return f_growth(c, 1.0 / a - 1.0); }
+static double W_RSD(oc_ctx* c, double l, double a0, double a1, int ni) {
+  double w = (1 + 8. * l) / ((2. * l + 1.) * (2. * l + 1.)) * nz_lens(c, 1. / a0 - 1., ni) *
+             hoverh0(c, a0) * f_rsd(c, a0);
+  w -= 4. / (2 * l + 3.) * sqrt((2 * l + 1.) / (2 * l + 3.)) * nz_lens(c, 1. / a1 - 1., ni) *
+       hoverh0(c, a1) * f_rsd(c, a1);
+  return w;
+}
+
+static void IA_A1_Z1Z2(oc_ctx* c, double a, double growfac_a, int n1, int n2, double res[2]) {
+  double A1 = 0.0, A2 = 0.0;
+  switch (c->ia_mode) {
+    case 0: break;
+    case 2: A1 = c->ia_bin[n1]; A2 = c->ia_bin[n2]; break;
+    case 3: A1 = c->ia_A * pow((1.0 / a) / c->oneplusz0, c->ia_eta); A2 = A1; break;
+    default: oc_set_err(c, "IA mode %d not supported", c->ia_mode);
+  }
+  const double x = c->omega_m * c->c1rhocrit / growfac_a;
+  res[0] = A1 * x;
+  res[1] = A2 * x;
+}
+
+/* amin_lens / amax_lens: redshift_spline.c:50-79 */
+static double amin_lens(oc_ctx* c, int ni) {
+  const double zmax = (c->l_zmax[ni] - c->l_zmean[ni]) * c->lens_st[ni] + c->l_zmean[ni];
+  return 1. / (1 + zmax + 2. * fabs(c->lens_dz[ni]));
+}
+static double amax_lens(oc_ctx* c, int ni) {
+  const double zmin = (c->l_zmin[ni] - c->l_zmean[ni]) * c->lens_st[ni] + c->l_zmean[ni];
+  if (c->bmag[ni] != 0) return 1. / (1. + fmax(c->s_zmin_all, 0.001));
+  return 1. / (1 + fmax(zmin - 2. * fabs(c->lens_dz[ni]), 0.001));
+}
+
+/* pair maps: Z1/Z2 (redshift_spline.c:229-275), ZL/ZS (150-193, no exclusions) */
+static void ss_pair(int ns, int q, int* z1, int* z2) {
+  *z1 = 0; *z2 = 0;
+  int n = 0;
+  for (int i = 0; i < ns; i++)
+    for (int j = i; j < ns; j++) { if (n == q) { *z1 = i; *z2 = j; return; } n++; }
+}
+int oc_npairs_ss(oc_ctx* c) { return c->ns * (c->ns + 1) / 2; }
+int oc_npairs_gs(oc_ctx* c) { return c->nl * c->ns; }
+
+/* ---------------------------------------------------------------------- */
+/* Limber integrands                                                       */
+/* ---------------------------------------------------------------------- */
+/* int_for_C_ss_tomo_limber_core, NLA branch: cosmo2D.c:1370-1515 */
+static double ss_core(oc_ctx* c, double a, double fK, double PK, double growfac_a,
+                      double dchida, double ell_prefactor, int n1, int n2, double WK1,
+                      double WK2, double WS1, double WS2, int EE) {
+  double ans;
+  if (1 == EE) {
+    double IA[2];
+    IA_A1_Z1Z2(c, a, growfac_a, n1, n2, IA);
+    const double C11 = IA[0], C12 = IA[1];
+    ans = WK1 * WK2 * PK - WS1 * WK2 * C11 * PK - WS2 * WK1 * C12 * PK + WS1 * WS2 * C11 * C12 * PK;
+  } else {
+    ans = 0.0;
+  }
+  return ans * (dchida / (fK * fK)) * ell_prefactor;
+}
+
+/* int_for_C_ss_tomo_limber: cosmo2D.c:1521-1555 */
+static double int_ss(oc_ctx* c, double a, int n1, int n2, double l, int EE) {
+  const double ell = l + 0.5;
+  double fK, dchida;
+  chi_all(c, a, &fK, &dchida);
+  const double growfac_a = growfac(c, a);
+  const double hh = hoverh0v2(a, dchida);
+  const double k = ell / fK;
+  const double PK = p_nonlin(c, k, a);
+  const double ell4 = ell * ell * ell * ell;
+  const double ell_prefactor = l * (l - 1.) * (l + 1.) * (l + 2.) / ell4;
+  const double WK1 = W_kappa(c, a, fK, n1), WK2 = W_kappa(c, a, fK, n2);
+  const double WS1 = W_source(c, a, n1, hh), WS2 = W_source(c, a, n2, hh);
+  return ss_core(c, a, fK, PK, growfac_a, dchida, ell_prefactor, n1, n2, WK1, WK2, WS1, WS2, EE);
+}
+
+/* int_for_C_gs_tomo_limber_core, NLA branch: cosmo2D.c:1890-2073 */
+static double gs_core(oc_ctx* c, double a, double fK, double PK, double growfac_a,
+                      double dchida, double ell_prefactor, double ell_prefactor2, int nl,
+                      int ns, double WK, double WS, double WGAL, double WMAG) {
+  const double b1 = c->b1[nl];
+  const double bmag = c->bmag[nl];
+  double IA[2];
+  IA_A1_Z1Z2(c, a, growfac_a, ns, ns, IA);
+  const double C1ZS = IA[0];
+  const double ans = (WK - WS * C1ZS) * ((WGAL * b1 + WMAG * ell_prefactor * bmag + 0.0) * PK + WGAL * 0.0);
+  return ans * (dchida / (fK * fK)) * ell_prefactor2;
+}
+
+/* int_for_C_gs_tomo_limber: cosmo2D.c:2079-2119 */
+static double int_gs(oc_ctx* c, double a, int nl, int ns, double l) {
+  const double growfac_a = growfac(c, a);
+  double fK, dchida;
+  chi_all(c, a, &fK, &dchida);
+  const double hh = hoverh0v2(a, dchida);
+  const double ell = l + 0.5;
+  const double k = ell / fK;
+  const double PK = p_nonlin(c, k, a);
+  const double ell_prefactor = l * (l + 1.) / (ell * ell);
+  const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+  const double ell_prefactor2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+  const double WK = W_kappa(c, a, fK, ns);
+  const double WS = W_source(c, a, ns, hh);
+  const double WGAL = W_gal(c, a, nl, hh);
+  const double WMAG = W_mag(c, a, fK, nl);
+  return gs_core(c, a, fK, PK, growfac_a, dchida, ell_prefactor, ell_prefactor2, nl, ns, WK, WS, WGAL, WMAG);
+}
+
+/* int_for_C_gg_tomo_limber (linear bias, RSD on, HOD off): cosmo2D.c:2490-2608 */
+static double int_gg(oc_ctx* c, double a, int ni, double l) {
+  double fK, dchida;
+  chi_all(c, a, &fK, &dchida);
+  const double hh = hoverh0v2(a, dchida);
+  const double ell = l + 0.5;
+  const double k = ell / fK;
+  const double b1i = c->b1[ni];
+  const double bmagi = c->bmag[ni];
+  const double WGALi = W_gal(c, a, ni, hh);
+  const double WMAGi = W_mag(c, a, fK, ni);
+  const double ell_prefactor = l * (l + 1.) / (ell * ell);
+  double res = 1.0;
+  const double chi_a_min = chi_of_a(c, OC_A_MIN);
+  const double chi_0 = ell / k;
+  const double chi_1 = (ell + 1.0) / k;
+  if (chi_1 > chi_a_min) return 0;
+  const double a_0 = a_chi(c, chi_0);
+  const double a_1 = a_chi(c, chi_1);
+  const double WRSDi = W_RSD(c, ell, a_0, a_1, ni);
+  res *= (WGALi * b1i + WMAGi * ell_prefactor * bmagi + WRSDi);
+  res *= (WGALi * b1i + WMAGi * ell_prefactor * bmagi + WRSDi);
+  res *= p_nonlin(c, k, a);
+  return (res + 0.0) * dchida / (fK * fK);
+}
+
+static int ensure_g(oc_ctx* c) {
+  if (!c->chi_z || !c->G_z || !c->lnP || !c->s_tab) { oc_set_err(c, "inputs not set"); return -1; }
+  if (!c->g_valid) build_g_tables(c);
+  return c->status;
+}
+
+/* C_ss_tomo_limber_nointerp: cosmo2D.c:1562-1604 (init=0) */
+static double C_ss_nointerp(oc_ctx* c, const gl_rule* r, double l, int ni, int nj, int EE) {
+  const double amin = 1. / (c->s_zmax_all + 1.);
+  const double amax = 1. / (1. + fmax(c->s_zmin_all, 1e-6));
+  double s = 0.0;
+  for (int p = 0; p < r->n; p++) {
+    double x, w;
+    gl_point(r, amin, amax, p, &x, &w);
+    s += w * int_ss(c, x, ni, nj, l, EE);
+  }
+  return s;
+}
+
+/* C_gs_tomo_limber_nointerp: cosmo2D.c:2125-2182 */
+static double C_gs_nointerp(oc_ctx* c, const gl_rule* r, double l, int nl, int ns) {
+  const double amin = amin_lens(c, nl), amax = amax_lens(c, nl);
+  if (!(amin > 0) || !(amin < 1) || !(amax > 0) || !(amax < 1) || !(amin < amax)) {
+    oc_set_err(c, "0 < amin < amax < 1 not true");
+    return NAN;
+  }
+  double s = 0.0;
+  for (int p = 0; p < r->n; p++) {
+    double x, w;
+    gl_point(r, amin, amax, p, &x, &w);
+    s += w * int_gs(c, x, nl, ns, l);
+  }
+  return s;
+}
+
+/* C_gg_tomo_limber_linpsopt_nointerp: cosmo2D.c:2614-2666 */
+static double C_gg_nointerp(oc_ctx* c, const gl_rule* r, double l, int ni) {
+  const double amin = amin_lens(c, ni), amax = amax_lens(c, ni);
+  if (!(amin > 0) || !(amin < 1) || !(amax > 0) || !(amax < 1) || !(amin < amax)) {
+    oc_set_err(c, "0 < amin < amax < 1 not true");
+    return NAN;
+  }
+  double s = 0.0;
+  for (int p = 0; p < r->n; p++) {
+    double x, w;
+    gl_point(r, amin, amax, p, &x, &w);
+    s += w * int_gg(c, x, ni, l);
+  }
+  return s;
+}
+
+/* ln-ell grids of the three tables (cosmo2D.c:1631-1634, 2204-2207, 2703-2706) */
+static void ss_lim(oc_ctx* c, double lim[3]) {
+  lim[0] = log(fmax(OC_LMIN_TAB - 1., 1.0));
+  lim[1] = log(c->lmax + 1);
+  lim[2] = (lim[1] - lim[0]) / ((double)c->n_ell - 1.);
+}
+static void gx_lim(oc_ctx* c, double lim[3]) {
+  lim[0] = log(fmax(OC_LMIN_TAB, 1.0));
+  lim[1] = log(c->lmax + 1);
+  lim[2] = (lim[1] - lim[0]) / ((double)c->n_ell - 1.0);
+}
+
+/* C_ss_tomo_limber table: cosmo2D.c:1655-1747 */
+static void table_ss(oc_ctx* c, double* tab /* [npairs][nell] */) {
+  const int nell = c->n_ell, npts = gl_order_ssgs(c->hdi), ns = c->ns;
+  const int np = oc_npairs_ss(c);
+  double lim[3];
+  ss_lim(c, lim);
+  gl_rule r;
+  gl_make(&r, npts);
+  const double amin = 1. / (c->s_zmax_all + 1.);
+  const double amax = 1. / (1. + fmax(c->s_zmin_all, 1e-6));
+  double *A = malloc(sizeof(double) * npts), *WT = malloc(sizeof(double) * npts),
+         *FK = malloc(sizeof(double) * npts), *GF = malloc(sizeof(double) * npts),
+         *HH = malloc(sizeof(double) * npts), *DC = malloc(sizeof(double) * npts);
+  for (int p = 0; p < npts; p++) { /* create_cosmo_nodes: cosmo2D.c:1331-1356 */
+    gl_point(&r, amin, amax, p, &A[p], &WT[p]);
+    chi_all(c, A[p], &FK[p], &DC[p]);
+    GF[p] = growfac(c, A[p]);
+    HH[p] = hoverh0v2(A[p], DC[p]);
+  }
+  double* lx = malloc(sizeof(double) * nell);
+  double* pref = malloc(sizeof(double) * nell);
+  for (int i = 0; i < nell; i++) {
+    lx[i] = exp(lim[0] + i * lim[2]);
+    const double ell = lx[i] + 0.5, ell4 = ell * ell * ell * ell;
+    pref[i] = lx[i] * (lx[i] - 1.) * (lx[i] + 1.) * (lx[i] + 2.) / ell4;
+  }
+  double* PK = malloc(sizeof(double) * (size_t)nell * npts);
+  #pragma omp parallel for schedule(static)
+  for (int i = 0; i < nell; i++)
+    for (int p = 0; p < npts; p++) PK[(size_t)i * npts + p] = p_nonlin(c, (lx[i] + 0.5) / FK[p], A[p]);
+  double* W = malloc(sizeof(double) * 2 * ns * npts);
+  for (int b = 0; b < ns; b++)
+    for (int p = 0; p < npts; p++) {
+      W[(0 * ns + b) * npts + p] = W_kappa(c, A[p], FK[p], b);
+      W[(1 * ns + b) * npts + p] = W_source(c, A[p], b, HH[p]);
+    }
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < nell; i++)
+    for (int k = 0; k < np; k++) {
+      int z1, z2;
+      ss_pair(ns, k, &z1, &z2);
+      double sum = 0.0;
+      for (int p = 0; p < npts; p++)
+        sum += ss_core(c, A[p], FK[p], PK[(size_t)i * npts + p], GF[p], DC[p], pref[i], z1, z2,
+                       W[z1 * npts + p], W[z2 * npts + p], W[(ns + z1) * npts + p],
+                       W[(ns + z2) * npts + p], 1) * WT[p];
+      tab[(size_t)k * nell + i] = sum;
+    }
+  free(A); free(WT); free(FK); free(GF); free(HH); free(DC);
+  free(lx); free(pref); free(PK); free(W);
+}
+
+/* C_gs_tomo_limber table: cosmo2D.c:2229-2348 */
+static void table_gs(oc_ctx* c, double* tab /* [nl*ns][nell] */) {
+  const int nell = c->n_ell, npts = gl_order_ssgs(c->hdi), ns = c->ns, nl = c->nl;
+  double lim[3];
+  gx_lim(c, lim);
+  gl_rule r;
+  gl_make(&r, npts);
+  double* lx = malloc(sizeof(double) * nell);
+  double* p1 = malloc(sizeof(double) * nell);
+  double* p2 = malloc(sizeof(double) * nell);
+  for (int i = 0; i < nell; i++) {
+    lx[i] = exp(lim[0] + i * lim[2]);
+    const double ell = lx[i] + 0.5;
+    p1[i] = lx[i] * (lx[i] + 1.) / (ell * ell);
+    const double tmp = (lx[i] - 1.) * lx[i] * (lx[i] + 1.) * (lx[i] + 2.);
+    p2[i] = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+  }
+  for (int zl = 0; zl < nl; zl++) {
+    const double amin = amin_lens(c, zl), amax = amax_lens(c, zl);
+    if (!(amin > 0) || !(amin < 1) || !(amax > 0) || !(amax < 1) || !(amin < amax)) {
+      oc_set_err(c, "0 < amin < amax < 1 not true"); break;
+    }
+    double A[1024], WT[1024], FK[1024], GF[1024], HH[1024], DC[1024];
+    for (int p = 0; p < npts; p++) {
+      gl_point(&r, amin, amax, p, &A[p], &WT[p]);
+      chi_all(c, A[p], &FK[p], &DC[p]);
+      GF[p] = growfac(c, A[p]);
+      HH[p] = hoverh0v2(A[p], DC[p]);
+    }
+    double* PK = malloc(sizeof(double) * (size_t)nell * npts);
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < nell; i++)
+      for (int p = 0; p < npts; p++) PK[(size_t)i * npts + p] = p_nonlin(c, (lx[i] + 0.5) / FK[p], A[p]);
+    double WG[1024], WM[1024];
+    double* WK = malloc(sizeof(double) * ns * npts);
+    double* WS = malloc(sizeof(double) * ns * npts);
+    for (int p = 0; p < npts; p++) {
+      WG[p] = W_gal(c, A[p], zl, HH[p]);
+      WM[p] = W_mag(c, A[p], FK[p], zl);
+      for (int zs = 0; zs < ns; zs++) {
+        WK[zs * npts + p] = W_kappa(c, A[p], FK[p], zs);
+        WS[zs * npts + p] = W_source(c, A[p], zs, HH[p]);
+      }
+    }
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int zs = 0; zs < ns; zs++)
+      for (int i = 0; i < nell; i++) {
+        double sum = 0.0;
+        for (int p = 0; p < npts; p++)
+          sum += gs_core(c, A[p], FK[p], PK[(size_t)i * npts + p], GF[p], DC[p], p1[i], p2[i], zl,
+                         zs, WK[zs * npts + p], WS[zs * npts + p], WG[p], WM[p]) * WT[p];
+        tab[(size_t)(zl * ns + zs) * nell + i] = sum;
+      }
+    free(PK); free(WK); free(WS);
+  }
+  free(lx); free(p1); free(p2);
+}
+
+/* C_gg_tomo_limber table: cosmo2D.c:2709-2732 */
+static void table_gg(oc_ctx* c, double* tab /* [nl][nell] */) {
+  const int nell = c->n_ell, nl = c->nl;
+  double lim[3];
+  gx_lim(c, lim);
+  gl_rule r;
+  gl_make(&r, gl_order_gg(c->hdi));
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < nl; k++)
+    for (int i = 0; i < nell; i++) {
+      const double lx = exp(lim[0] + i * lim[2]);
+      tab[(size_t)k * nell + i] = C_gg_nointerp(c, &r, lx, k);
+    }
+}
+
+/* *_fill: cosmo2D.c:1784-1807 (scalar branch) */
+static void fill(const double* tab, const double lim[3], int n, int lmin, int lmax, double* out) {
+  const double inv_dx = 1.0 / lim[2];
+  const double a = lim[0];
+  for (int l = lmin; l < lmax; l++) {
+    const double r = (log((double)l) - a) * inv_dx;
+    const int i = (int)floor(r);
+    const int ic = i < 0 ? 0 : (i >= n - 1 ? n - 2 : i);
+    const double t = r - ic;
+    out[l] = tab[ic] + t * (tab[ic + 1] - tab[ic]);
+  }
+}
+
+int oc_cl_tables(oc_ctx* c, double* ss, double* gs, double* gg) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  if (ss) table_ss(c, ss);
+  if (gs && c->nl > 0) table_gs(c, gs);
+  if (gg && c->nl > 0) table_gg(c, gg);
+  return c->status;
+}
+
+int oc_cl_ss_nointerp(oc_ctx* c, const double* l, int nl, double* out) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  gl_rule r;
+  gl_make(&r, gl_order_ssgs(c->hdi));
+  const int np = oc_npairs_ss(c);
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < np; k++)
+    for (int i = 0; i < nl; i++) {
+      int z1, z2;
+      ss_pair(c->ns, k, &z1, &z2);
+      out[(size_t)k * nl + i] = C_ss_nointerp(c, &r, l[i], z1, z2, 1);
+    }
+  return c->status;
+}
+
+int oc_cl_gs_nointerp(oc_ctx* c, const double* l, int nl, double* out) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  gl_rule r;
+  gl_make(&r, gl_order_ssgs(c->hdi));
+  const int np = oc_npairs_gs(c);
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < np; k++)
+    for (int i = 0; i < nl; i++) out[(size_t)k * nl + i] = C_gs_nointerp(c, &r, l[i], k / c->ns, k % c->ns);
+  return c->status;
+}
+
+int oc_cl_gg_nointerp(oc_ctx* c, const double* l, int nl, double* out) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  gl_rule r;
+  gl_make(&r, gl_order_gg(c->hdi));
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < c->nl; k++)
+    for (int i = 0; i < nl; i++) out[(size_t)k * nl + i] = C_gg_nointerp(c, &r, l[i], k);
+  return c->status;
+}
+
+/* Cl arrays exactly as xi_pm_tomo / w_gammat_tomo / w_gg_tomo(limber=1) build
+ * them: l=0 -> 0, 1 <= l < LMIN_tab direct GL, LMIN_tab <= l < LMAX fill
+ * (cosmo2D.c:468-496, 655-674, 831-849) */
+int oc_cl_full(oc_ctx* c, double* ss, double* gs, double* gg) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  const int L = c->lmax, nell = c->n_ell;
+  gl_rule r64, r96;
+  gl_make(&r64, gl_order_ssgs(c->hdi));
+  gl_make(&r96, gl_order_gg(c->hdi));
+  if (ss) {
+    const int np = oc_npairs_ss(c);
+    double* tab = malloc(sizeof(double) * (size_t)np * nell);
+    table_ss(c, tab);
+    double lim[3];
+    ss_lim(c, lim);
+    #pragma omp parallel for schedule(dynamic)
+    for (int k = 0; k < np; k++) {
+      int z1, z2;
+      ss_pair(c->ns, k, &z1, &z2);
+      double* out = ss + (size_t)k * L;
+      out[0] = 0.0;
+      for (int l = 1; l < OC_LMIN_TAB; l++) out[l] = C_ss_nointerp(c, &r64, (double)l, z1, z2, 1);
+      fill(tab + (size_t)k * nell, lim, nell, OC_LMIN_TAB, L, out);
+    }
+    free(tab);
+  }
+  if (gs && c->nl > 0) {
+    const int np = oc_npairs_gs(c);
+    double* tab = malloc(sizeof(double) * (size_t)np * nell);
+    table_gs(c, tab);
+    double lim[3];
+    gx_lim(c, lim);
+    #pragma omp parallel for schedule(dynamic)
+    for (int k = 0; k < np; k++) {
+      double* out = gs + (size_t)k * L;
+      out[0] = 0.0;
+      for (int l = 1; l < OC_LMIN_TAB; l++) out[l] = C_gs_nointerp(c, &r64, (double)l, k / c->ns, k % c->ns);
+      fill(tab + (size_t)k * nell, lim, nell, OC_LMIN_TAB, L, out);
+    }
+    free(tab);
+  }
+  if (gg && c->nl > 0) {
+    double* tab = malloc(sizeof(double) * (size_t)c->nl * nell);
+    table_gg(c, tab);
+    double lim[3];
+    gx_lim(c, lim);
+    #pragma omp parallel for schedule(dynamic)
+    for (int k = 0; k < c->nl; k++) {
+      double* out = gg + (size_t)k * L;
+      out[0] = 0.0;
+      for (int l = 1; l < OC_LMIN_TAB; l++) out[l] = C_gg_nointerp(c, &r96, (double)l, k);
+      fill(tab + (size_t)k * nell, lim, nell, OC_LMIN_TAB, L, out);
+    }
+    free(tab);
+  }
+  return c->status;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Legendre stage                                                          */
+/* ---------------------------------------------------------------------- */
+/* gsl_sf_legendre_Pl_deriv_array restated (called at basics.c:426,431) */
+int oc_legendre_pl_dpl(int lmax, double x, double* P, double* dP) {
+  if (lmax < 0 || x < -1.0 || x > 1.0) return -1;
+  P[0] = 1.0;
+  if (lmax >= 1) P[1] = x;
+  {
+    double pm2 = 1.0, pm1 = x;
+    for (int l = 2; l <= lmax; l++) {
+      const double p = (x * (2 * l - 1) * pm1 - (l - 1) * pm2) / l;
+      pm2 = pm1;
+      pm1 = p;
+      P[l] = p;
+    }
+  }
+  dP[0] = 0.0;
+  if (lmax >= 1) dP[1] = 1.0;
+  const double sqeps = 1.4901161193847656e-08;
+  if (fabs(x - 1.0) * (lmax + 1.0) * (lmax + 1.0) < sqeps) {
+    for (int l = 2; l <= lmax; l++) {
+      const double pre = 0.5 * l * (l + 1.0);
+      dP[l] = pre * (1.0 - 0.25 * (1.0 - x) * (l + 2.0) * (l - 1.0));
+    }
+  } else if (fabs(x + 1.0) * (lmax + 1.0) * (lmax + 1.0) < sqeps) {
+    for (int l = 2; l <= lmax; l++) {
+      const double sgn = (l & 1) ? 1.0 : -1.0;
+      const double pre = sgn * 0.5 * l * (l + 1.0);
+      dP[l] = pre * (1.0 - 0.25 * (1.0 + x) * (l + 2.0) * (l - 1.0));
+    }
+  } else {
+    const double da = 1.0 + x, db = 1.0 - x;
+    for (int l = 2; l <= lmax; l++) dP[l] = -l * (x * P[l] - P[l - 1]) / (da * db);
+  }
+  return 0;
+}
+
+/* set_bin_average (basics.c:382-455) + G builds (cosmo2D.c:377-460,
+ * 581-644, 758-823) */
+static int build_kernels(oc_ctx* c) {
+  const int nt = c->ntheta, L = c->lmax;
+  if (nt <= 0) { oc_set_err(c, "Ntable.Ntheta not initialized"); return -1; }
+  free(c->K);
+  c->K = (double*)calloc((size_t)4 * nt * L, sizeof(double));
+  double* Gp = c->K;
+  double* Gm = c->K + (size_t)nt * L;
+  double* Pgt = c->K + (size_t)2 * nt * L;
+  double* Pw = c->K + (size_t)3 * nt * L;
+  const double logdt = (log(c->vtmax) - log(c->vtmin)) / nt;
+  int bad = 0;
+  #pragma omp parallel for schedule(dynamic)
+  for (int i = 0; i < nt; i++) {
+    const double xmin = cos(exp(log(c->vtmin) + (i + 0.) * logdt));
+    const double xmax = cos(exp(log(c->vtmin) + (i + 1.) * logdt));
+    double* Pmin = malloc(sizeof(double) * (L + 1));
+    double* Pmax = malloc(sizeof(double) * (L + 1));
+    double* dPmin = malloc(sizeof(double) * (L + 1));
+    double* dPmax = malloc(sizeof(double) * (L + 1));
+    if (oc_legendre_pl_dpl(L, xmin, Pmin, dPmin) || oc_legendre_pl_dpl(L, xmax, Pmax, dPmax)) bad = 1;
+    double* gp = Gp + (size_t)i * L;
+    double* gm = Gm + (size_t)i * L;
+    double* pg = Pgt + (size_t)i * L;
+    double* pw = Pw + (size_t)i * L;
+    gp[0] = gm[0] = pg[0] = pw[0] = 0.0;
+    for (int l = 1; l < L; l++) {
+      gp[l] = (2. * l + 1) / (2. * M_PI * l * l * (l + 1) * (l + 1)) * (
+          -l * (l - 1.) / 2 * (l + 2. / (2 * l + 1)) * (Pmin[l - 1] - Pmax[l - 1])
+          - l * (l - 1.) * (2. - l) / 2 * (xmin * Pmin[l] - xmax * Pmax[l])
+          + l * (l - 1.) / (2. * l + 1) * (Pmin[l + 1] - Pmax[l + 1])
+          + (4 - l) * (dPmin[l] - dPmax[l])
+          + (l + 2) * (xmin * dPmin[l - 1] - xmax * dPmax[l - 1] - Pmin[l - 1] + Pmax[l - 1])
+          + 2 * (l - 1) * (xmin * dPmin[l] - xmax * dPmax[l] - Pmin[l] + Pmax[l])
+          - 2 * (l + 2) * (dPmin[l - 1] - dPmax[l - 1])) / (xmin - xmax);
+      gm[l] = (2. * l + 1) / (2. * M_PI * l * l * (l + 1) * (l + 1)) * (
+          -l * (l - 1.) / 2 * (l + 2. / (2 * l + 1)) * (Pmin[l - 1] - Pmax[l - 1])
+          - l * (l - 1.) * (2. - l) / 2 * (xmin * Pmin[l] - xmax * Pmax[l])
+          + l * (l - 1.) / (2. * l + 1) * (Pmin[l + 1] - Pmax[l + 1])
+          + (4 - l) * (dPmin[l] - dPmax[l])
+          + (l + 2) * (xmin * dPmin[l - 1] - xmax * dPmax[l - 1] - Pmin[l - 1] + Pmax[l - 1])
+          - 2 * (l - 1) * (xmin * dPmin[l] - xmax * dPmax[l] - Pmin[l] + Pmax[l])
+          + 2 * (l + 2) * (dPmin[l - 1] - dPmax[l - 1])) / (xmin - xmax);
+      pg[l] = (2. * l + 1) / (4. * M_PI * l * (l + 1) * (xmin - xmax)) *
+              ((l + 2. / (2 * l + 1.)) * (Pmin[l - 1] - Pmax[l - 1]) +
+               (2 - l) * (xmin * Pmin[l] - xmax * Pmax[l]) -
+               2. / (2 * l + 1.) * (Pmin[l + 1] - Pmax[l + 1]));
+      const double tmp = (1.0 / (xmin - xmax)) * (1. / (4.0 * M_PI));
+      pw[l] = tmp * (Pmin[l + 1] - Pmax[l + 1] - Pmin[l - 1] + Pmax[l - 1]);
+    }
+    free(Pmin); free(Pmax); free(dPmin); free(dPmax);
+  }
+  if (bad) { oc_set_err(c, "Legendre argument out of range"); return -1; }
+  c->K_valid = 1;
+  return 0;
+}
+
+int oc_kernels(oc_ctx* c, double* Gp, double* Gm, double* Pgt, double* Pw) {
+  c->status = 0;
+  if (!c->K_valid && build_kernels(c)) return -1;
+  const size_t n = (size_t)c->ntheta * c->lmax;
+  if (Gp) memcpy(Gp, c->K, n * sizeof(double));
+  if (Gm) memcpy(Gm, c->K + n, n * sizeof(double));
+  if (Pgt) memcpy(Pgt, c->K + 2 * n, n * sizeof(double));
+  if (Pw) memcpy(Pw, c->K + 3 * n, n * sizeof(double));
+  return 0;
+}
+
+/* plain dot over l = 1 .. LMAX-1 (cosmo2D.c:500-516 scalar branch) */
+static void contract(const double* K, const double* Cl, int npair, int nt, int L, double* out) {
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int nz = 0; nz < npair; nz++)
+    for (int i = 0; i < nt; i++) {
+      const double* g = K + (size_t)i * L;
+      const double* cl = Cl + (size_t)nz * L;
+      double s = 0.0;
+      for (int l = 1; l < L; l++) s += g[l] * cl[l];
+      out[nz * nt + i] = s;
+    }
+}
+
+int oc_xi_pm(oc_ctx* c, double* xip, double* xim) {
+  c->status = 0;
+  if (!c->K_valid && build_kernels(c)) return -1;
+  const int np = oc_npairs_ss(c), L = c->lmax;
+  double* cl = malloc(sizeof(double) * (size_t)np * L);
+  if (oc_cl_full(c, cl, NULL, NULL)) { free(cl); return -1; }
+  /* BB = 0 under NLA, so EE+BB = EE-BB = EE */
+  contract(c->K, cl, np, c->ntheta, L, xip);
+  contract(c->K + (size_t)c->ntheta * L, cl, np, c->ntheta, L, xim);
+  free(cl);
+  return c->status;
+}
+
+int oc_gammat(oc_ctx* c, double* out) {
+  c->status = 0;
+  if (!c->K_valid && build_kernels(c)) return -1;
+  const int np = oc_npairs_gs(c), L = c->lmax;
+  double* cl = malloc(sizeof(double) * (size_t)np * L);
+  if (oc_cl_full(c, NULL, cl, NULL)) { free(cl); return -1; }
+  contract(c->K + (size_t)2 * c->ntheta * L, cl, np, c->ntheta, L, out);
+  free(cl);
+  return c->status;
+}
+
+int oc_wtheta(oc_ctx* c, double* out) {
+  c->status = 0;
+  if (!c->K_valid && build_kernels(c)) return -1;
+  const int L = c->lmax;
+  double* cl = malloc(sizeof(double) * (size_t)c->nl * L);
+  if (oc_cl_full(c, NULL, NULL, cl)) { free(cl); return -1; }
+  contract(c->K + (size_t)3 * c->ntheta * L, cl, c->nl, c->ntheta, L, out);
+  free(cl);
+  return c->status;
+}
+
+int oc_datavector_size(oc_ctx* c) {
+  return c->ntheta * (2 * oc_npairs_ss(c) + oc_npairs_gs(c) + c->nl);
+}
+
+/* compute_Mx2pt_N_masked<0,3> with every mask entry 1 and no point mass
+ * (generic_interface.hpp:770-791 starts, 800-862 calibration, 968-1041 fill) */
+int oc_datavector_3x2pt(oc_ctx* c, const int ord[3], double* dv) {
+  c->status = 0;
+  const int nt = c->ntheta, nss = oc_npairs_ss(c), ngs = oc_npairs_gs(c), ngg = c->nl;
+  const int sizes[3] = {2 * nt * nss, nt * ngs, nt * ngg};
+  /* stable_sort_index(ord, "ascend") then the reference's start formula */
+  int idx[3] = {0, 1, 2};
+  for (int i = 1; i < 3; i++)
+    for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) { int t = idx[j]; idx[j] = idx[j - 1]; idx[j - 1] = t; }
+  int start[3] = {0, 0, 0};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+  const int n = sizes[0] + sizes[1] + sizes[2];
+  double* xp = malloc(sizeof(double) * nss * nt);
+  double* xm = malloc(sizeof(double) * nss * nt);
+  if (oc_xi_pm(c, xp, xm)) { free(xp); free(xm); return -1; }
+  double* gt = NULL;
+  double* w = NULL;
+  if (c->nl > 0) {
+    gt = malloc(sizeof(double) * ngs * nt);
+    w = malloc(sizeof(double) * ngg * nt);
+    if (oc_gammat(c, gt) || oc_wtheta(c, w)) { free(xp); free(xm); free(gt); free(w); return -1; }
+  }
+  for (int i = 0; i < n; i++) dv[i] = 0.0;
+  for (int nz = 0; nz < nss; nz++) {
+    int z1, z2;
+    ss_pair(c->ns, nz, &z1, &z2);
+    const double f = (1.0 + c->shear_m[z1]) * (1.0 + c->shear_m[z2]);
+    for (int i = 0; i < nt; i++) {
+      dv[start[0] + nt * nz + i] = xp[nz * nt + i] * f;
+      dv[start[0] + nt * nz + i + nt * nss] = xm[nz * nt + i] * f;
+    }
+  }
+  for (int nz = 0; nz < ngs; nz++) {
+    const int zs = nz % c->ns;
+    for (int i = 0; i < nt; i++) dv[start[1] + nt * nz + i] = gt[nz * nt + i] * (1.0 + c->shear_m[zs]);
+  }
+  for (int nz = 0; nz < ngg; nz++)
+    for (int i = 0; i < nt; i++) dv[start[2] + nt * nz + i] = w[nz * nt + i];
+  free(xp); free(xm); free(gt); free(w);
+  return c->status;
+}
+
+/* ---------------------------------------------------------------------- */
+/* setters and probes                                                      */
+/* ---------------------------------------------------------------------- */
+int oc_set_ntable(oc_ctx* c, int n_ell, int n_a, int lmax, int high_def) {
+  c->status = 0;
+  if (n_ell < 2 || n_a < 2 || lmax < OC_LMIN_TAB + 1) { oc_set_err(c, "bad Ntable"); return -1; }
+  c->n_ell = n_ell; c->n_a = n_a; c->lmax = lmax; c->hdi = high_def;
+  c->g_valid = 0; c->K_valid = 0;
+  if (c->nl > 0) compute_lens_zmean(c);
+  return 0;
+}
+
+int oc_set_binning_real(oc_ctx* c, int ntheta, double tmin, double tmax) {
+  c->status = 0;
+  if (ntheta <= 0 || !(tmax > tmin) || !(tmin > 0)) { oc_set_err(c, "bad theta binning"); return -1; }
+  c->ntheta = ntheta;
+  c->vtmin = tmin * 2.90888208665721580e-4;
+  c->vtmax = tmax * 2.90888208665721580e-4;
+  c->K_valid = 0;
+  return 0;
+}
+
+int oc_set_omega_m(oc_ctx* c, double om) { c->omega_m = om; c->g_valid = 0; return 0; }
+
+static int any_nan(const double* x, int n) { for (int i = 0; i < n; i++) if (isnan(x[i])) return 1; return 0; }
+
+int oc_set_distances(oc_ctx* c, const double* z, const double* chi, int n) {
+  c->status = 0;
+  if (n < 5 || any_nan(z, n) || any_nan(chi, n)) { oc_set_err(c, "bad distances"); return -1; }
+  free(c->chi_z); free(c->chi_v);
+  c->chi_z = dup(z, n); c->chi_v = dup(chi, n); c->chi_nz = n;
+  c->g_valid = 0;
+  return 0;
+}
+
+int oc_set_growth(oc_ctx* c, const double* z, const double* G, int n) {
+  c->status = 0;
+  if (n < 2 || any_nan(z, n) || any_nan(G, n)) { oc_set_err(c, "bad growth"); return -1; }
+  free(c->G_z); free(c->G_v);
+  c->G_z = dup(z, n); c->G_v = dup(G, n); c->G_nz = n;
+  c->g_valid = 0;
+  return 0;
+}
+
+int oc_set_pk(oc_ctx* c, const double* lk, int nk, const double* z, int nz, const double* lnP) {
+  c->status = 0;
+  if (nk < 2 || nz < 2 || any_nan(lk, nk) || any_nan(z, nz) || any_nan(lnP, nk * nz)) {
+    oc_set_err(c, "bad P(k,z)"); return -1;
+  }
+  free(c->lk); free(c->pz); free(c->lnP);
+  c->lk = dup(lk, nk); c->pz = dup(z, nz); c->lnP = dup(lnP, nk * nz);
+  c->nk = nk; c->nzp = nz;
+  c->g_valid = 0;
+  return 0;
+}
+
+int oc_set_nuisance(oc_ctx* c, const double* src_dz, const double* lens_dz, const double* lens_st,
+                    const double* b1, const double* bmag, const double* m) {
+  c->status = 0;
+  for (int i = 0; i < c->ns; i++) {
+    if (src_dz) c->src_dz[i] = src_dz[i];
+    if (m) c->shear_m[i] = m[i];
+  }
+  for (int i = 0; i < c->nl; i++) {
+    if (lens_dz) c->lens_dz[i] = lens_dz[i];
+    if (lens_st) c->lens_st[i] = lens_st[i];
+    if (b1) c->b1[i] = b1[i];
+    if (bmag) c->bmag[i] = bmag[i];
+  }
+  c->g_valid = 0;
+  return 0;
+}
+
+int oc_set_ia(oc_ctx* c, int mode, const double* A_bin, double A, double eta) {
+  c->status = 0;
+  if (mode != 0 && mode != 2 && mode != 3) { oc_set_err(c, "IA mode not supported"); return -1; }
+  c->ia_mode = mode;
+  c->ia_A = A;
+  c->ia_eta = eta;
+  if (A_bin) for (int i = 0; i < c->ns; i++) c->ia_bin[i] = A_bin[i];
+  return 0;
+}
+
+int oc_nz_source(oc_ctx* c, int bin, const double* z, int n, double* out) {
+  if (bin < 0 || bin >= c->ns) return -1;
+  for (int i = 0; i < n; i++) out[i] = nz_source(c, z[i], bin);
+  return 0;
+}
+int oc_nz_lens(oc_ctx* c, int bin, const double* z, int n, double* out) {
+  if (bin < 0 || bin >= c->nl) return -1;
+  for (int i = 0; i < n; i++) out[i] = nz_lens(c, z[i], bin);
+  return 0;
+}
+int oc_zmean_lens(oc_ctx* c, double* out) { for (int i = 0; i < c->nl; i++) out[i] = c->l_zmean[i]; return 0; }
+int oc_g_tomo(oc_ctx* c, int bin, const double* a, int n, double* out) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  for (int i = 0; i < n; i++) out[i] = g_tomo(c, a[i], bin);
+  return c->status;
+}
+int oc_g_lens(oc_ctx* c, int bin, const double* a, int n, double* out) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  for (int i = 0; i < n; i++) out[i] = g_lens(c, a[i], bin);
+  return c->status;
+}
+int oc_chi_all(oc_ctx* c, const double* a, int n, double* chi, double* dchida) {
+  c->status = 0;
+  for (int i = 0; i < n; i++) chi_all(c, a[i], &chi[i], &dchida[i]);
+  return c->status;
+}
+int oc_pdelta(oc_ctx* c, const double* k, const double* a, int n, double* out) {
+  for (int i = 0; i < n; i++) out[i] = p_nonlin(c, k[i], a[i]);
+  return 0;
+}

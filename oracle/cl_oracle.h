@@ -1,0 +1,86 @@
+/*
+ * cl_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 CPU restatement of CosmoLike's tomographic two-point projection path
+ * (Limber C_ell for ss/gs/gg + bin-averaged Legendre transform to xi+-,
+ * gamma_t, w).  It exists to check the HIP library and to time a CPU baseline;
+ * the shipped library never links, loads or calls it.
+ *
+ * Every function follows a reference routine cited at its definition
+ * (cosmolike/cosmo2D.c, cosmo3D.c, redshift_spline.c, radial_weights.c,
+ * basics.c, IA.c, bias.c, generic_interface.{cpp,hpp}).  State lives in an
+ * explicit context instead of the reference's global structs.
+ */
+#ifndef CL_ORACLE_H
+#define CL_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oc_ctx oc_ctx;
+
+oc_ctx* oc_create(void);
+void oc_free(oc_ctx* c);
+
+/* Ntable (structs.c:262-279): N_ell, N_a, LMAX, high_def_integration */
+int oc_set_ntable(oc_ctx* c, int n_ell, int n_a, int lmax, int high_def);
+/* init_binning_real_space (generic_interface.cpp:479-505), arcmin */
+int oc_set_binning_real(oc_ctx* c, int ntheta, double tmin_arcmin, double tmax_arcmin);
+int oc_set_omega_m(oc_ctx* c, double omega_m);
+int oc_set_distances(oc_ctx* c, const double* z, const double* chi, int n);
+int oc_set_growth(oc_ctx* c, const double* z, const double* G, int n);
+/* lnP is k-major: lnP[i*nz + j] (generic_interface.cpp:1343) */
+int oc_set_pk(oc_ctx* c, const double* log10k, int nk, const double* z, int nz,
+              const double* lnP);
+/* table is row-major (nrows, ntomo+1): column 0 = z, columns 1.. = n_i(z) */
+int oc_set_source_sample(oc_ctx* c, const double* table, int nrows, int ntomo);
+int oc_set_lens_sample(oc_ctx* c, const double* table, int nrows, int ntomo);
+int oc_set_nuisance(oc_ctx* c, const double* src_dz, const double* lens_dz,
+                    const double* lens_stretch, const double* b1, const double* bmag,
+                    const double* shear_m);
+/* IA: model 0 = NLA only; mode 0=none, 2=per-bin (A[i]), 3=A*(a^-1/1.62)^eta */
+int oc_set_ia(oc_ctx* c, int ia_mode, const double* A_per_bin, double A, double eta);
+
+/* ------------- pieces (for pinning tests) ------------------------------ */
+int oc_nz_source(oc_ctx* c, int bin, const double* z, int n, double* out);
+int oc_nz_lens(oc_ctx* c, int bin, const double* z, int n, double* out);
+int oc_zmean_lens(oc_ctx* c, double* out);
+int oc_g_tomo(oc_ctx* c, int bin, const double* a, int n, double* out);
+int oc_g_lens(oc_ctx* c, int bin, const double* a, int n, double* out);
+int oc_chi_all(oc_ctx* c, const double* a, int n, double* chi, double* dchida);
+int oc_pdelta(oc_ctx* c, const double* k, const double* a, int n, double* out);
+int oc_gl_nodes(int n, double* x, double* w);
+int oc_legendre_pl_dpl(int lmax, double x, double* P, double* dP);
+/* G kernels [ntheta][LMAX] for xi+, xi-, gamma_t, w (cosmo2D.c:430-453,631-639,808-815) */
+int oc_kernels(oc_ctx* c, double* Gp, double* Gm, double* Pgt, double* Pw);
+
+/* Limber tables at the N_ell log-spaced ell (cosmo2D.c:1617,2195,2693):
+ * ss[npairs][N_ell] (EE), gs[n_ggl][N_ell], gg[nlens][N_ell] */
+int oc_cl_tables(oc_ctx* c, double* ss, double* gs, double* gg);
+/* direct GL integrals at arbitrary ell (the *_nointerp routines):
+ * out[pair][nl] */
+int oc_cl_ss_nointerp(oc_ctx* c, const double* l, int nl, double* out);
+int oc_cl_gs_nointerp(oc_ctx* c, const double* l, int nl, double* out);
+int oc_cl_gg_nointerp(oc_ctx* c, const double* l, int nl, double* out);
+/* C_ell filled at integer ell 0..LMAX-1 as the real-space routines use them:
+ * ss[npairs][LMAX], gs[n_ggl][LMAX], gg[nlens][LMAX] (NULL to skip) */
+int oc_cl_full(oc_ctx* c, double* ss, double* gs, double* gg);
+
+/* xi_pm_tomo (cosmo2D.c:353-556): xip/xim[npairs*ntheta] (q = nz*Ntheta+i) */
+int oc_xi_pm(oc_ctx* c, double* xip, double* xim);
+int oc_gammat(oc_ctx* c, double* out);   /* w_gammat_tomo, [n_ggl*ntheta] */
+int oc_wtheta(oc_ctx* c, double* out);   /* w_gg_tomo limber=1, [nlens*ntheta] */
+/* compute_Mx2pt_N_masked<0,3> with an all-ones mask, incl. shear calibration */
+int oc_datavector_3x2pt(oc_ctx* c, const int ord[3], double* dv);
+int oc_datavector_size(oc_ctx* c);
+
+int oc_npairs_ss(oc_ctx* c);
+int oc_npairs_gs(oc_ctx* c);
+/* invalidates the cosmology-dependent caches (a new draw) */
+void oc_touch(oc_ctx* c);
+const char* oc_last_error(oc_ctx* c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,267 @@
+"""ctypes front end of the FP64 CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, the smoke check and the benchmark's CPU-baseline leg to
+check (never to produce) the HIP library's results.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        lib = ctypes.CDLL(_LIB)
+        vp = ctypes.c_void_p
+        lib.oc_create.restype = vp
+        lib.oc_free.argtypes = [vp]
+        lib.oc_last_error.restype = ctypes.c_char_p
+        lib.oc_last_error.argtypes = [vp]
+        lib.oc_touch.argtypes = [vp]
+        for name in ["oc_set_ntable", "oc_set_binning_real", "oc_set_omega_m", "oc_set_distances",
+                     "oc_set_growth", "oc_set_pk", "oc_set_source_sample", "oc_set_lens_sample",
+                     "oc_set_nuisance", "oc_set_ia", "oc_nz_source", "oc_nz_lens", "oc_zmean_lens",
+                     "oc_g_tomo", "oc_g_lens", "oc_chi_all", "oc_pdelta", "oc_kernels",
+                     "oc_cl_tables", "oc_cl_ss_nointerp", "oc_cl_gs_nointerp", "oc_cl_gg_nointerp",
+                     "oc_cl_full", "oc_xi_pm", "oc_gammat", "oc_wtheta", "oc_datavector_3x2pt",
+                     "oc_datavector_size", "oc_npairs_ss", "oc_npairs_gs"]:
+            getattr(lib, name).restype = ctypes.c_int
+        lib.oc_set_ntable.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.oc_set_binning_real.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        lib.oc_set_omega_m.argtypes = [vp, ctypes.c_double]
+        lib.oc_set_ia.argtypes = [vp, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double]
+        lib.oc_gl_nodes.argtypes = [ctypes.c_int, _D, _D]
+        lib.oc_legendre_pl_dpl.argtypes = [ctypes.c_int, ctypes.c_double, _D, _D]
+        _lib = lib
+    return _lib
+
+
+def _p(x):
+    if x is None:
+        return None
+    return x.ctypes.data_as(_D)
+
+
+def _arr(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def gl_nodes(n: int):
+    lib = _load()
+    x = np.empty(n)
+    w = np.empty(n)
+    lib.oc_gl_nodes(n, _p(x), _p(w))
+    return x, w
+
+
+def legendre(lmax: int, x: float):
+    lib = _load()
+    P = np.empty(lmax + 1)
+    dP = np.empty(lmax + 1)
+    if lib.oc_legendre_pl_dpl(lmax, x, _p(P), _p(dP)):
+        raise ValueError("domain error")
+    return P, dP
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Oracle:
+    """One oracle context holding one draw's inputs (mirrors the reference's
+    global structs, but explicit)."""
+
+    def __init__(self):
+        self.lib = _load()
+        self.c = ctypes.c_void_p(self.lib.oc_create())
+        self.ns = 0
+        self.nl = 0
+        self.ntheta = 0
+        self.lmax = 50000
+        self.n_ell = 300
+
+    def __del__(self):
+        try:
+            self.lib.oc_free(self.c)
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise OracleError(self.lib.oc_last_error(self.c).decode())
+
+    # -------------------------------------------------------------- setters
+    def set_ntable(self, n_ell=300, n_a=225, lmax=50000, high_def=0):
+        self.n_ell, self.lmax = n_ell, lmax
+        self._chk(self.lib.oc_set_ntable(self.c, n_ell, n_a, lmax, high_def))
+
+    def set_binning(self, ntheta, tmin, tmax):
+        self.ntheta = ntheta
+        self._chk(self.lib.oc_set_binning_real(self.c, ntheta, tmin, tmax))
+
+    def set_cosmology(self, omega_m, chi_z, chi, G_z, G, log10k, pk_z, lnP):
+        self._chk(self.lib.oc_set_omega_m(self.c, float(omega_m)))
+        chi_z, chi, G_z, G = map(_arr, (chi_z, chi, G_z, G))
+        self._chk(self.lib.oc_set_distances(self.c, _p(chi_z), _p(chi), len(chi_z)))
+        self._chk(self.lib.oc_set_growth(self.c, _p(G_z), _p(G), len(G_z)))
+        lk, pz, lnP = map(_arr, (log10k, pk_z, lnP))
+        self._chk(self.lib.oc_set_pk(self.c, _p(lk), len(lk), _p(pz), len(pz), _p(lnP)))
+
+    def set_source(self, table):
+        t = _arr(table)
+        self.ns = t.shape[1] - 1
+        self._chk(self.lib.oc_set_source_sample(self.c, _p(t), t.shape[0], self.ns))
+
+    def set_lens(self, table):
+        t = _arr(table)
+        self.nl = t.shape[1] - 1
+        self._chk(self.lib.oc_set_lens_sample(self.c, _p(t), t.shape[0], self.nl))
+
+    def set_nuisance(self, src_dz=None, lens_dz=None, lens_stretch=None, b1=None, bmag=None,
+                     shear_m=None):
+        a = [None if v is None else _arr(v) for v in (src_dz, lens_dz, lens_stretch, b1, bmag, shear_m)]
+        self._chk(self.lib.oc_set_nuisance(self.c, *[_p(v) for v in a]))
+
+    def set_ia(self, mode=3, A=0.5, eta=0.0, A_bin=None):
+        ab = None if A_bin is None else _arr(A_bin)
+        self._chk(self.lib.oc_set_ia(self.c, mode, _p(ab), float(A), float(eta)))
+
+    def load(self, prob):
+        """Configure from a ``cosmolike_amd.synth.Problem``."""
+        self.set_ntable(prob.n_ell, 225, prob.lmax, 0)
+        self.set_binning(prob.ntheta, prob.theta_min_arcmin, prob.theta_max_arcmin)
+        self.set_cosmology(prob.omega_m, prob.chi_z, prob.chi, prob.G_z, prob.G, prob.log10k,
+                           prob.pk_z, prob.lnP)
+        self.set_source(prob.source)
+        if prob.lens is not None:
+            self.set_lens(prob.lens)
+        self.set_nuisance(prob.src_dz, prob.lens_dz if prob.lens is not None else None,
+                          prob.lens_stretch if prob.lens is not None else None,
+                          prob.b1 if prob.lens is not None else None,
+                          prob.bmag if prob.lens is not None else None, prob.shear_m)
+        self.set_ia(prob.ia_mode, prob.ia_A, prob.ia_eta)
+        return self
+
+    def touch(self):
+        self.lib.oc_touch(self.c)
+
+    # -------------------------------------------------------------- pieces
+    @property
+    def n_ss(self):
+        return self.ns * (self.ns + 1) // 2
+
+    @property
+    def n_gs(self):
+        return self.ns * self.nl
+
+    def nz_source(self, b, z):
+        z = _arr(z)
+        out = np.empty_like(z)
+        self._chk(self.lib.oc_nz_source(self.c, b, _p(z), len(z), _p(out)))
+        return out
+
+    def nz_lens(self, b, z):
+        z = _arr(z)
+        out = np.empty_like(z)
+        self._chk(self.lib.oc_nz_lens(self.c, b, _p(z), len(z), _p(out)))
+        return out
+
+    def zmean_lens(self):
+        out = np.empty(self.nl)
+        self.lib.oc_zmean_lens(self.c, _p(out))
+        return out
+
+    def g_tomo(self, b, a):
+        a = _arr(a)
+        out = np.empty_like(a)
+        self._chk(self.lib.oc_g_tomo(self.c, b, _p(a), len(a), _p(out)))
+        return out
+
+    def g_lens(self, b, a):
+        a = _arr(a)
+        out = np.empty_like(a)
+        self._chk(self.lib.oc_g_lens(self.c, b, _p(a), len(a), _p(out)))
+        return out
+
+    def chi_all(self, a):
+        a = _arr(a)
+        chi = np.empty_like(a)
+        d = np.empty_like(a)
+        self._chk(self.lib.oc_chi_all(self.c, _p(a), len(a), _p(chi), _p(d)))
+        return chi, d
+
+    def pdelta(self, k, a):
+        k, a = _arr(k), _arr(a)
+        out = np.empty_like(k)
+        self._chk(self.lib.oc_pdelta(self.c, _p(k), _p(a), len(k), _p(out)))
+        return out
+
+    def kernels(self):
+        n = self.ntheta * self.lmax
+        out = [np.empty(n) for _ in range(4)]
+        self._chk(self.lib.oc_kernels(self.c, *[_p(o) for o in out]))
+        return [o.reshape(self.ntheta, self.lmax) for o in out]
+
+    def cl_tables(self):
+        ss = np.empty((self.n_ss, self.n_ell))
+        gs = np.empty((max(self.n_gs, 1), self.n_ell))
+        gg = np.empty((max(self.nl, 1), self.n_ell))
+        self._chk(self.lib.oc_cl_tables(self.c, _p(ss), _p(gs), _p(gg)))
+        return ss, gs[: self.n_gs], gg[: self.nl]
+
+    def cl_nointerp(self, probe, ells):
+        ells = _arr(ells)
+        n = {"ss": self.n_ss, "gs": self.n_gs, "gg": self.nl}[probe]
+        out = np.empty((n, len(ells)))
+        fn = {"ss": self.lib.oc_cl_ss_nointerp, "gs": self.lib.oc_cl_gs_nointerp,
+              "gg": self.lib.oc_cl_gg_nointerp}[probe]
+        self._chk(fn(self.c, _p(ells), len(ells), _p(out)))
+        return out
+
+    def cl_full(self, ss=True, gs=True, gg=True):
+        L = self.lmax
+        a = np.empty((self.n_ss, L)) if ss else None
+        b = np.empty((self.n_gs, L)) if (gs and self.nl) else None
+        c = np.empty((self.nl, L)) if (gg and self.nl) else None
+        self._chk(self.lib.oc_cl_full(self.c, _p(a), _p(b), _p(c)))
+        return a, b, c
+
+    def xi_pm(self):
+        xp = np.empty(self.n_ss * self.ntheta)
+        xm = np.empty(self.n_ss * self.ntheta)
+        self._chk(self.lib.oc_xi_pm(self.c, _p(xp), _p(xm)))
+        return xp.reshape(self.n_ss, self.ntheta), xm.reshape(self.n_ss, self.ntheta)
+
+    def gammat(self):
+        out = np.empty(self.n_gs * self.ntheta)
+        self._chk(self.lib.oc_gammat(self.c, _p(out)))
+        return out.reshape(self.n_gs, self.ntheta)
+
+    def wtheta(self):
+        out = np.empty(self.nl * self.ntheta)
+        self._chk(self.lib.oc_wtheta(self.c, _p(out)))
+        return out.reshape(self.nl, self.ntheta)
+
+    def datavector(self, ord=(0, 1, 2)):
+        n = self.lib.oc_datavector_size(self.c)
+        out = np.empty(n)
+        o = (ctypes.c_int * 3)(*ord)
+        self._chk(self.lib.oc_datavector_3x2pt(self.c, o, _p(out)))
+        return out
