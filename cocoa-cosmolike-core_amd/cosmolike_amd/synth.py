@@ -121,6 +121,16 @@ def _zhisto(nrows=300, zmax=3.0):
 
 def source_table(nbin: int, z0: float, alpha=2.0, beta=1.5, sigma0=0.05, nrows=300,
                  zmax=3.0) -> np.ndarray:
+    key = (nbin, z0, alpha, beta, sigma0, nrows, zmax)
+    if key not in _NZ_CACHE:
+        _NZ_CACHE[key] = _source_table(*key)
+    return _NZ_CACHE[key].copy()
+
+
+_NZ_CACHE: dict = {}
+
+
+def _source_table(nbin, z0, alpha, beta, sigma0, nrows, zmax):
     """Equal-number photo-z bins of n(z) ~ z^alpha exp(-(z/z0)^beta).
 
     Photo-z scatter is Gaussian with sigma = sigma0 (1+z).  Returns a

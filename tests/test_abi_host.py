@@ -1,0 +1,100 @@
+"""C-ABI surface and host-side logic of libcosmolike_hip.so, without a GPU.
+
+The library's setters and validation run on the host; a context touches the
+device only when something is computed, so these run in the CPU suite.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cosmolike_amd import Cosmolike, CosmolikeError, _lib, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "cosmolike_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(\w+)\s*\(", txt, flags=re.M)
+    return sorted({n for n in names if n not in ("if", "while", "return", "sizeof")})
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_reference_names_present():
+    names = set(header_functions())
+    for n in ("xi_pm_tomo", "w_gammat_tomo", "w_gg_tomo", "C_ss_tomo_limber", "C_gs_tomo_limber",
+              "C_gg_tomo_limber", "C_ss_tomo_limber_nointerp", "C_gs_tomo_limber_nointerp",
+              "C_gg_tomo_limber_nointerp"):
+        assert n in names
+
+
+@pytest.fixture()
+def ctx():
+    c = Cosmolike(device=0)
+    yield c
+    c.close()
+
+
+def test_setters_validate(ctx):
+    p = synth.shear_only_problem()
+    ctx.load(p)
+    assert ctx.datavector_size() == 2 * 15 * 20
+    with pytest.raises(CosmolikeError) as e:
+        ctx.init_binning_real_space(40, 2.5, 250.0)  # more than 32 theta bins
+    assert e.value.code == -1
+    with pytest.raises(CosmolikeError) as e:
+        ctx.init_ntable(300, 225, 50000, 1)           # high_def_integration != 0
+    assert e.value.code == -1
+    bad = p.chi.copy()
+    bad[3] = np.nan
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_distances(p.chi_z, bad)
+    assert e.value.code == -2
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_nuisance_shear_calib(np.zeros(4))     # 5 source bins
+    assert e.value.code == -1
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_nuisance_linear_bias(np.ones(5))      # no lens sample
+    assert e.value.code == -3
+    tab = p.source.copy()
+    tab[10, 0] = tab[9, 0]                            # z column not monotonic
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_source_sample(tab)
+    assert e.value.code == -4
+    tab = p.source.copy()
+    tab[:, 2] = 0.0                                   # empty bin
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_source_sample(tab)
+    assert e.value.code == -4
+
+
+def test_datavector_sizes(ctx):
+    p = synth.full_3x2pt_problem()
+    ctx.load(p)
+    assert ctx.datavector_size() == 26 * (2 * 55 + 50 + 5) == 4290
+
+
+def test_compute_without_device_fails_cleanly(ctx):
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    p = synth.shear_only_problem()
+    ctx.load(p)
+    out = np.full(ctx.datavector_size(), 7.0)
+    o = (ctypes.c_int * 3)(0, 1, 2)
+    rc = ctx.L.clh_compute_3x2pt_real(ctx.h, o, out.ctypes.data_as(_lib.D), len(out))
+    assert rc == -5
+    assert np.all(out == 7.0)

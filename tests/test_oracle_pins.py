@@ -1,0 +1,385 @@
+"""Pins of the FP64 CPU oracle (oracle/cl_oracle.c).
+
+The reference holds no tests, golden vectors or fixtures for this path and is
+not buildable here (GSL, FFTW, Armadillo, spdlog, Boost absent), so each piece
+of the oracle is pinned by what the mathematics fixes: library routines
+(numpy leggauss, scipy CubicSpline, np.interp, RegularGridInterpolator),
+mpmath at high precision, closed-form identities, invariants, and an
+independent continuum Limber integral for the assembled C_ell.
+"""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+from scipy import interpolate, special
+
+from cosmolike_amd import synth
+
+import oracle as O
+
+COVERH0 = 2997.92458
+
+
+@pytest.fixture(scope="module")
+def p3():
+    return synth.full_3x2pt_problem()
+
+
+@pytest.fixture(scope="module")
+def o3(p3):
+    return O.Oracle().load(p3)
+
+
+# ------------------------------------------------------------------ GL rule
+@pytest.mark.parametrize("n", [64, 96, 256])
+def test_gl_nodes_match_numpy(n):
+    x, w = O.gl_nodes(n)
+    xr, wr = np.polynomial.legendre.leggauss(n)
+    assert np.allclose(x, xr, rtol=0, atol=2e-15)
+    assert np.allclose(w, wr, rtol=1e-10, atol=0)  # numpy's own weights carry ~1e-11 error
+    mp.mp.dps = 40
+    for i in (0, n // 7, n // 2 - 1):
+        xe = mp.findroot(lambda t: mp.legendre(n, t), mp.mpf(x[i]))
+        we = 2 / ((1 - xe**2) * mp.diff(lambda t: mp.legendre(n, t), xe) ** 2)
+        assert abs(float(xe) - x[i]) < 2e-16 and abs(float(we) / w[i] - 1) < 1e-13
+
+
+def test_gl_exact_for_polynomials():
+    x, w = O.gl_nodes(64)
+    for k in (0, 1, 10, 63, 126, 127):
+        exact = (1.0 - (-1.0) ** (k + 1)) / (k + 1)
+        assert abs(np.dot(w, x**k) - exact) < 1e-13
+
+
+# ------------------------------------------------------- Legendre P_l, P'_l
+@pytest.mark.parametrize("theta_arcmin", [2.5, 40.0, 900.0])
+def test_legendre_vs_mpmath(theta_arcmin):
+    x = math.cos(theta_arcmin * 2.90888208665721580e-4)
+    L = 50000
+    P, dP = O.legendre(L, x)
+    mp.mp.dps = 40
+    for l in (2, 7, 100, 1234, 20000, 49999, 50000):
+        ref = mp.legendre(l, mp.mpf(x))
+        dref = mp.diff(lambda t: mp.legendre(l, t), mp.mpf(x))
+        scale = max(1.0, float(abs(ref)))
+        assert abs(P[l] - float(ref)) < 1e-14 * scale * max(l, 10), (l, P[l], ref)  # recurrence rounding ~ l eps
+        dscale = l * (l + 1) / 2.0
+        assert abs(dP[l] - float(dref)) < 1e-8 * dscale, (l, dP[l], dref)
+
+
+def test_legendre_identities():
+    P, dP = O.legendre(2000, 1.0)
+    l = np.arange(2001)
+    assert np.allclose(P, 1.0, rtol=0, atol=1e-15)
+    assert np.allclose(dP, l * (l + 1) / 2.0, rtol=1e-15)
+    P, dP = O.legendre(300, 0.3)
+    assert np.allclose(P, special.eval_legendre(l[:301], 0.3), atol=1e-13)
+
+
+# ------------------------------------------- bin-averaged Legendre kernels
+def _binavg(f, xlo, xhi):
+    return mp.quad(f, [xlo, xhi]) / (xhi - xlo)
+
+
+def test_kernels_are_bin_averages():
+    """w: (2l+1)/(4pi) P_l; gamma_t: (2l+1)/(4pi l(l+1)) P_l^2; xi+-: (2l+1)/(4pi)
+    (l-1)(l+2)/(l(l+1)) d^l_{2,+-2} (the G+-_{l,2} normalisation of cosmo2D.c:430-453,
+    i.e. 2 (G+ +- G-)/(l^2 (l+1)^2)), each averaged over x = cos(theta) across the bin.
+    xi- is compared on the scale of xi+: at small l its kernel is a tiny difference
+    of O(1) terms and carries the formula's own FP64 cancellation."""
+    o = O.Oracle()
+    o.set_ntable(300, 225, 64, 0)
+    o.set_binning(3, 60.0, 600.0)
+    Gp, Gm, Pgt, Pw = o.kernels()
+    mp.mp.dps = 30
+    logdt = (math.log(600.0) - math.log(60.0)) / 3
+    for i in range(3):
+        t0 = math.exp(math.log(60.0) + i * logdt) * 2.90888208665721580e-4
+        t1 = math.exp(math.log(60.0) + (i + 1) * logdt) * 2.90888208665721580e-4
+        xlo, xhi = mp.cos(mp.mpf(t1)), mp.cos(mp.mpf(t0))
+        for l in (2, 3, 5, 12, 30):
+            w = _binavg(lambda x: (2 * l + 1) / (4 * mp.pi) * mp.legendre(l, x), xlo, xhi)
+            gt = _binavg(lambda x: (2 * l + 1) / (4 * mp.pi * l * (l + 1)) * mp.legenp(l, 2, x), xlo, xhi)
+            # Wigner d^l_{2,2} = ((1+x)/2)^2 P^{(0,4)}_{l-2}(x); d^l_{2,-2} = ((1-x)/2)^2 P^{(4,0)}_{l-2}(x)
+            dp = _binavg(lambda x: (2 * l + 1) / (4 * mp.pi) * ((1 + x) / 2) ** 2 * mp.jacobi(l - 2, 0, 4, x), xlo, xhi)
+            dm = _binavg(lambda x: (2 * l + 1) / (4 * mp.pi) * ((1 - x) / 2) ** 2 * mp.jacobi(l - 2, 4, 0, x), xlo, xhi)
+            f = (l - 1) * (l + 2) / (l * (l + 1))
+            for got, ref, scale, name in ((Pw[i, l], w, w, "w"), (Pgt[i, l], gt, gt, "gt"),
+                                          (Gp[i, l], f * dp, f * dp, "xi+"), (Gm[i, l], f * dm, f * dp, "xi-")):
+                assert abs(got - float(ref)) <= 1e-9 * abs(float(scale)), (name, i, l, got, float(ref))
+
+
+def test_kernel_l1_zero():
+    o = O.Oracle()
+    o.set_ntable(300, 225, 64, 0)
+    o.set_binning(4, 2.5, 250.0)
+    Gp, Gm, Pgt, Pw = o.kernels()
+    assert np.all(Gp[:, :2] == 0) and np.all(Gm[:, :2] == 0)
+    assert np.all(Pgt[:, 0] == 0) and np.all(Pw[:, 0] == 0)
+
+
+# ---------------------------------------------------------------- n(z)
+def _centres(table):
+    z = table[:, 0]
+    n = len(z)
+    zmin, zmax = max(z[0], 1e-5), z[-1] + (z[-1] - z[0]) / (n - 1)
+    dz = (zmax - zmin) / n
+    return zmin + (np.arange(n) + 0.5) * dz, dz
+
+
+def test_source_nz_spline(p3, o3):
+    zc, dz = _centres(p3.source)
+    # histogram values looked up at the centres, normalised per bin
+    for b in (0, 4, 9):
+        h = p3.source[:, b + 1]
+        y = h / (h.sum() * dz)
+        cs = interpolate.CubicSpline(zc, y, bc_type="natural")
+        zz = np.linspace(zc[0], zc[-1], 997)
+        assert np.allclose(o3.nz_source(b, zz), cs(zz), rtol=0, atol=1e-12 * y.max())
+        assert np.allclose(o3.nz_source(b, zc), y, atol=1e-13 * y.max())
+        # zero outside the knots, unit normalisation
+        assert o3.nz_source(b, [zc[0] - 1e-9, zc[-1] + 1e-9]).tolist() == [0.0, 0.0]
+        zf = np.linspace(zc[0], zc[-1], 20001)
+        assert abs(np.trapezoid(o3.nz_source(b, zf), zf) - 1.0) < 2e-3
+
+
+def test_source_photoz_shift(p3):
+    o = O.Oracle().load(p3)
+    z = np.linspace(0.2, 2.0, 50)
+    base = o.nz_source(3, z - 0.02)
+    o.set_nuisance(src_dz=np.full(p3.ns, 0.02))
+    assert np.allclose(o.nz_source(3, z), base, rtol=0, atol=1e-15)
+
+
+def test_lens_stretch_conserves_norm(p3):
+    o = O.Oracle().load(p3)
+    zf = np.linspace(0.0, 3.0, 30001)
+    o.set_nuisance(lens_stretch=np.full(p3.nl, 1.3), lens_dz=np.full(p3.nl, 0.03))
+    for b in range(p3.nl):
+        assert abs(np.trapezoid(o.nz_lens(b, zf), zf) - 1.0) < 2e-3
+
+
+def test_lens_zmean(p3, o3):
+    zc, dz = _centres(p3.lens)
+    centres = np.linspace(0.3, 0.9, 5)
+    assert np.allclose(o3.zmean_lens(), centres, atol=5e-4)
+
+
+# ------------------------------------------------------------- background
+def test_chi_and_derivative(p3, o3):
+    a = np.array([0.9, 0.7, 0.5, 0.3, 0.26])
+    chi, dchida = o3.chi_all(a)
+    z = 1 / a - 1
+    ref = np.interp(z, p3.chi_z, p3.chi) / COVERH0
+    assert np.allclose(chi, ref, rtol=1e-14)
+    E = np.sqrt(p3.omega_m * (1 + z) ** 3 + 1 - p3.omega_m)
+    assert np.allclose(dchida, 1.0 / (a * a * E), rtol=2e-4)
+    # at table nodes the distance is the table value
+    zn = p3.chi_z[[10, 100, 300]]
+    chi_n, _ = o3.chi_all(1 / (1 + zn))
+    assert np.allclose(chi_n, p3.chi[[10, 100, 300]] / COVERH0, rtol=1e-13)
+
+
+def test_pdelta_bilinear_and_extrapolation(p3, o3):
+    lk, z, lnP = p3.log10k, p3.pk_z, p3.lnP.reshape(len(p3.log10k), len(p3.pk_z))
+    rgi = interpolate.RegularGridInterpolator((lk, z), lnP, method="linear", bounds_error=False,
+                                              fill_value=None)
+    rng = np.random.default_rng(3)
+    lq = rng.uniform(-4.5, 3.5, 300)   # beyond both ends of the log10 k column
+    zq = rng.uniform(0.0, 2.99, 300)
+    k = 10 ** lq * COVERH0
+    a = 1 / (1 + zq)
+    got = o3.pdelta(k, a)
+    ref = np.exp(rgi(np.c_[lq, zq])) / COVERH0**3
+    assert np.allclose(got, ref, rtol=1e-11)
+
+
+# ------------------------------------------------------ lens efficiencies
+def test_g_tomo_vs_direct_integral(p3, o3):
+    """At the coarse-table nodes g(a) = int_{z(a)} n(z') (1 - chi(a)/chi(z')) dz'."""
+    zmax_all = 3.0
+    amin = 1 / (zmax_all + 1)
+    dac = (0.999999 - amin) / 224
+    zc, dz = _centres(p3.source)
+    for b in (1, 6):
+        h = p3.source[:, b + 1]
+        cs = interpolate.CubicSpline(zc, h / (h.sum() * dz), bc_type="natural")
+        for k in (20, 90, 160, 210):
+            a = amin + k * dac
+            zk = 1 / a - 1
+            zz = np.linspace(zk, zc[-1], 40001)
+            chi_k = np.interp(zk, p3.chi_z, p3.chi)
+            chi_z = np.interp(zz, p3.chi_z, p3.chi)
+            ref = np.trapezoid(cs(zz) * (1 - chi_k / chi_z), zz)
+            got = o3.g_tomo(b, [a])[0]
+            assert abs(got - ref) < 2e-4 * max(ref, 1e-3), (b, k, got, ref)
+
+
+# ------------------------------------------------------------ Limber C_ell
+def _continuum_setup(p):
+    lk, pz = p.log10k, p.pk_z
+    rgi = interpolate.RegularGridInterpolator((lk, pz), p.lnP.reshape(len(lk), len(pz)), method="linear",
+                                              bounds_error=False, fill_value=None)
+
+    def P(k_c, z):  # k in c/H0 units
+        return np.exp(rgi(np.c_[np.log10(k_c / COVERH0), z])) / COVERH0**3
+
+    G0 = np.interp(0.0, p.G_z, p.G)
+
+    def D(z):
+        return np.interp(z, p.G_z, p.G) / (1 + z) / G0
+
+    def chi(z):
+        return np.interp(z, p.chi_z, p.chi) / COVERH0
+
+    def E(z):
+        return np.sqrt(p.omega_m * (1 + z) ** 3 + 1 - p.omega_m)
+
+    return P, D, chi, E
+
+
+def _nz_splines(table):
+    zc, dz = _centres(table)
+    out = []
+    for b in range(table.shape[1] - 1):
+        h = table[:, b + 1]
+        cs = interpolate.CubicSpline(zc, h / (h.sum() * dz), bc_type="natural")
+        out.append(lambda z, cs=cs, lo=zc[0], hi=zc[-1]: np.where((z >= lo) & (z <= hi), cs(z), 0.0))
+    return out
+
+
+def _lens_eff(n, chi, zgrid):
+    """g(z) = int_z n(z') (1 - chi(z)/chi(z')) dz' on zgrid (cumulative from the top)."""
+    nz = n(zgrid)
+    cz = chi(zgrid)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = np.where(cz > 0, nz / cz, 0.0)
+    # cumulative integrals from z to zmax
+    A = np.concatenate([[0], np.cumsum(0.5 * (nz[1:] + nz[:-1]) * np.diff(zgrid))])
+    B = np.concatenate([[0], np.cumsum(0.5 * (inv[1:] + inv[:-1]) * np.diff(zgrid))])
+    return (A[-1] - A) - cz * (B[-1] - B)
+
+
+def test_limber_ss_gs_gg_vs_continuum(p3, o3):
+    P, D, chi, E = _continuum_setup(p3)
+    src = _nz_splines(p3.source)
+    lens = _nz_splines(p3.lens)
+    z = np.linspace(1e-4, 3.0, 30001)
+    cz, Ez = chi(z), E(z)
+    a = 1 / (1 + z)
+    C1 = 0.5 * p3.omega_m * 0.01389 / D(z)
+    ells = np.array([30.0, 300.0, 3000.0])
+    ref_ss = o3.cl_nointerp("ss", ells)
+    ref_gs = o3.cl_nointerp("gs", ells)
+    ref_gg = o3.cl_nointerp("gg", np.array([5000.0]))
+
+    def q_shear(b):
+        g = _lens_eff(src[b], chi, z)
+        return 1.5 * p3.omega_m * cz / a * g - C1 * src[b](z) * Ez
+
+    for (i, j) in ((2, 4), (9, 9)):
+        q = q_shear(i) * q_shear(j)
+        pair = [(x, y) for x in range(p3.ns) for y in range(x, p3.ns)].index((i, j))
+        for n, l in enumerate(ells):
+            L = l + 0.5
+            integrand = q / cz**2 * P(L / cz, z) / Ez
+            c = np.trapezoid(integrand, z) * l * (l - 1) * (l + 1) * (l + 2) / L**4
+            assert abs(ref_ss[pair, n] / c - 1) < 2e-3, ("ss", i, j, l, ref_ss[pair, n], c)
+    for (zl, zs) in ((1, 7), (4, 9)):
+        q = q_shear(zs) * lens[zl](z) * Ez * p3.b1[zl]
+        for n, l in enumerate(ells):
+            L = l + 0.5
+            c = np.trapezoid(q / cz**2 * P(L / cz, z) / Ez, z) * math.sqrt((l - 1) * l * (l + 1) * (l + 2)) / L**2
+            assert abs(ref_gs[zl * p3.ns + zs, n] / c - 1) < 2e-3, ("gs", zl, zs, l)
+    for zl in (0, 3):
+        w = (lens[zl](z) * Ez * p3.b1[zl]) ** 2
+        L = 5000.5
+        c = np.trapezoid(w / cz**2 * P(L / cz, z) / Ez, z)
+        assert abs(ref_gg[zl, 0] / c - 1) < 3e-3, ("gg", zl, ref_gg[zl, 0], c)
+
+
+def test_table_equals_direct_at_table_ell(p3, o3):
+    ss, gs, gg = o3.cl_tables()
+    l_ss = np.exp(np.linspace(np.log(19.0), np.log(50001.0), p3.n_ell))
+    l_gx = np.exp(np.linspace(np.log(20.0), np.log(50001.0), p3.n_ell))
+    idx = np.array([0, 7, 150, 499])
+    assert np.allclose(o3.cl_nointerp("ss", l_ss[idx]), ss[:, idx], rtol=1e-12)
+    assert np.allclose(o3.cl_nointerp("gs", l_gx[idx]), gs[:, idx], rtol=1e-12)
+    assert np.allclose(o3.cl_nointerp("gg", l_gx[idx]), gg[:, idx], rtol=1e-12)
+
+
+def test_fill_is_linear_in_ln_ell(p3, o3):
+    ss, gs, gg = o3.cl_tables()
+    css, cgs, cgg = o3.cl_full()
+    l = np.arange(20, p3.lmax)
+    l_ss = np.linspace(np.log(19.0), np.log(50001.0), p3.n_ell)
+    l_gx = np.linspace(np.log(20.0), np.log(50001.0), p3.n_ell)
+    for q in (0, 30, 54):
+        assert np.allclose(css[q, 20:], np.interp(np.log(l), l_ss, ss[q]), rtol=1e-12, atol=0)
+    assert np.allclose(cgs[17, 20:], np.interp(np.log(l), l_gx, gs[17]), rtol=1e-12)
+    assert np.allclose(cgg[2, 20:], np.interp(np.log(l), l_gx, gg[2]), rtol=1e-12)
+    low = o3.cl_nointerp("ss", np.arange(1.0, 20.0))
+    assert np.allclose(css[:, 1:20], low, rtol=1e-14)
+    assert np.all(css[:, 0] == 0) and np.all(css[:, 1] == 0)  # l(l-1) prefactor
+
+
+def test_bias_linearity(p3):
+    o = O.Oracle().load(p3)
+    base_gs = o.cl_nointerp("gs", [50.0, 500.0])
+    gg = []
+    for s in (0.5, 1.0, 1.5, 2.0):
+        o.set_nuisance(b1=p3.b1 * s)
+        if s == 2.0:
+            assert np.allclose(o.cl_nointerp("gs", [50.0, 500.0]), 2 * base_gs, rtol=1e-13)
+        gg.append(o.cl_nointerp("gg", [30.0, 300.0]))
+    # gg with RSD is quadratic in b: third difference vanishes
+    third = gg[3] - 3 * gg[2] + 3 * gg[1] - gg[0]
+    assert np.all(np.abs(third) < 1e-10 * np.abs(gg[3]))
+
+
+def test_ia_amplitude_zero_ignores_eta(p3):
+    o = O.Oracle().load(p3)
+    o.set_ia(3, 0.0, 0.0)
+    a = o.cl_nointerp("ss", [100.0])
+    o.set_ia(3, 0.0, 2.0)
+    b = o.cl_nointerp("ss", [100.0])
+    assert np.array_equal(a, b)
+    o.set_ia(3, 0.5, 0.0)
+    c = o.cl_nointerp("ss", [100.0])
+    assert np.abs(c / a - 1).max() > 1e-3  # IA matters at this amplitude
+
+
+def test_shear_calibration_factors(p3):
+    o = O.Oracle().load(p3)
+    dv0 = o.datavector()
+    m = np.linspace(-0.02, 0.03, p3.ns)
+    o.set_nuisance(shear_m=m)
+    dv1 = o.datavector()
+    nt, ns = p3.ntheta, p3.ns
+    q = 0
+    for i in range(ns):
+        for j in range(i, ns):
+            f = (1 + m[i]) * (1 + m[j])
+            sl = slice(nt * q, nt * (q + 1))
+            assert np.allclose(dv1[sl], dv0[sl] * f, rtol=1e-15)
+            q += 1
+    start = 2 * nt * q
+    for zl in range(p3.nl):
+        for zs in range(ns):
+            sl = slice(start + nt * (zl * ns + zs), start + nt * (zl * ns + zs + 1))
+            assert np.allclose(dv1[sl], dv0[sl] * (1 + m[zs]), rtol=1e-15)
+
+
+def test_datavector_block_order(p3, o3):
+    """Block starts follow generic_interface.hpp:770-791 for the given ord."""
+    dv = o3.datavector((0, 1, 2))
+    nt, nss, ngs, nl = p3.ntheta, 55, 50, 5
+    s = [2 * nt * nss, nt * ngs, nt * nl]
+    rev = o3.datavector((2, 1, 0))
+    assert np.array_equal(rev[: s[2]], dv[s[0] + s[1]:])
+    assert np.array_equal(rev[s[2]: s[2] + s[1]], dv[s[0]: s[0] + s[1]])
+    assert np.array_equal(rev[s[2] + s[1]:], dv[: s[0]])
+    xp, xm = o3.xi_pm()
+    assert np.array_equal(dv[: nt * nss], xp.ravel())
+    assert np.array_equal(dv[nt * nss: 2 * nt * nss], xm.ravel())
