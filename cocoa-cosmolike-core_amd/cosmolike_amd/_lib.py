@@ -41,6 +41,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                                "(python -m cosmolike_amd.build)")
+        # torch bundles its own HIP runtime with the same soname
+        # (libamdhip64.so.7); loading torch first makes this library bind to
+        # that copy, so the process holds a single HIP runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         i32 = ctypes.c_int
         f64 = ctypes.c_double
@@ -76,6 +83,10 @@ def lib():
             "clh_batch_upload": (i32, [VP, i32, ctypes.POINTER(clh_draw)]),
             "clh_batch_forward": (i32, [VP, I, VP]),
             "clh_synchronize": (i32, [VP]),
+            "clh_set_stream": (i32, [VP, VP]),
+            "clh_batch_use_graph": (i32, [VP, i32]),
+            "clh_batch_set_params": (i32, [VP, VP, VP, VP, VP, VP, VP]),
+            "clh_batch_vjp": (i32, [VP, I, VP, VP, VP, VP, VP, VP, VP]),
             "clh_last_kernel_times": (i32, [VP, ctypes.POINTER(ctypes.c_char_p),
                                             ctypes.POINTER(ctypes.c_float), i32]),
             "clh_enable_timing": (i32, [VP, i32]),

@@ -242,6 +242,23 @@ class Cosmolike:
         o = (ctypes.c_int * 3)(*ord)
         self._chk(self.L.clh_batch_forward(self.h, o, ctypes.c_void_p(dv_device_ptr)))
 
+    def batch_use_graph(self, on=True):
+        self._chk(self.L.clh_batch_use_graph(self.h, 1 if on else 0))
+
+    def set_stream(self, stream_ptr):
+        self._chk(self.L.clh_set_stream(self.h, ctypes.c_void_p(stream_ptr)))
+
+    def batch_set_params(self, lnP=0, src_dz=0, lens_dz=0, b1=0, ia_A=0, shear_m=0):
+        """Device pointers (ints; 0 = keep) of [B][...] parameter arrays."""
+        v = [ctypes.c_void_p(x or None) for x in (lnP, src_dz, lens_dz, b1, ia_A, shear_m)]
+        self._chk(self.L.clh_batch_set_params(self.h, *v))
+
+    def batch_vjp(self, dv_bar, g_lnP=0, g_src_dz=0, g_lens_dz=0, g_b1=0, g_ia_A=0, g_m=0, ord=(0, 1, 2)):
+        """Device pointers (ints; 0 = skip) of the cotangent and the gradient outputs."""
+        v = [ctypes.c_void_p(x or None) for x in (dv_bar, g_lnP, g_src_dz, g_lens_dz, g_b1, g_ia_A, g_m)]
+        o = (ctypes.c_int * 3)(*ord)
+        self._chk(self.L.clh_batch_vjp(self.h, o, *v))
+
     def synchronize(self):
         self._chk(self.L.clh_synchronize(self.h))
 

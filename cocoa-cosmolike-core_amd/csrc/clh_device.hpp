@@ -105,7 +105,7 @@ __device__ __forceinline__ double clh_spline(const double* __restrict__ zc,
 
 // nz_source_photoz evaluation: redshift_spline.c:513-537
 __device__ __forceinline__ double clh_nz_src(const DrawDesc& d, double zz, int b) {
-  zz = zz - d.src_dz[b];
+  zz = zz - d.par->src_dz[b];
   const int n = d.s_rows;
   if (zz < d.s_zc[0] || zz > d.s_zc[n - 1]) return 0.0;
   return clh_spline(d.s_zc, d.s_coef + (size_t)b * 4 * (n - 1), n, zz);
@@ -113,11 +113,11 @@ __device__ __forceinline__ double clh_nz_src(const DrawDesc& d, double zz, int b
 
 // nz_lens_photoz evaluation: redshift_spline.c:736-762
 __device__ __forceinline__ double clh_nz_lens(const DrawDesc& d, double zz, int b) {
-  zz = (zz - d.lens_dz[b] - d.l_zmean[b]) / d.lens_st[b] + d.l_zmean[b];
+  zz = (zz - d.par->lens_dz[b] - d.l_zmean[b]) / d.par->lens_st[b] + d.l_zmean[b];
   const int n = d.l_rows;
   if (zz <= d.l_zc[0] || zz >= d.l_zc[n - 1]) return 0.0;
   const double r = clh_spline(d.l_zc, d.l_coef + (size_t)b * 4 * (n - 1), n, zz);
-  return r / d.lens_st[b];
+  return r / d.par->lens_st[b];
 }
 
 // interpol1d: basics.c:457-480 (constant extrapolation)
@@ -149,8 +149,8 @@ __device__ __forceinline__ double clh_g_lens(const DrawDesc& d, double a, int b)
 // IA_A1_Z1Z2 amplitude times Omega_m c1 rho_crit / D: IA.c:238-296
 __device__ __forceinline__ double clh_ia_c1(const DrawDesc& d, double a, double growfac_a, int b) {
   double A = 0.0;
-  if (d.ia_mode == 2) A = d.ia_bin[b];
-  else if (d.ia_mode == 3) A = d.ia_A * pow((1.0 / a) / 1.62, d.ia_eta);
+  if (d.ia_mode == 2) A = d.par->ia_bin[b];
+  else if (d.ia_mode == 3) A = d.par->ia_A * pow((1.0 / a) / 1.62, d.par->ia_eta);
   return A * (d.omega_m * 0.01389 / growfac_a);
 }
 
@@ -177,11 +177,102 @@ __device__ __forceinline__ void clh_ss_pair(int ns, int q, int& i, int& j) {
 
 // amin_lens / amax_lens: redshift_spline.c:50-79
 __device__ __forceinline__ double clh_amin_lens(const DrawDesc& d, int ni) {
-  const double zmax = (d.l_zmax[ni] - d.l_zmean[ni]) * d.lens_st[ni] + d.l_zmean[ni];
-  return 1. / (1 + zmax + 2. * fabs(d.lens_dz[ni]));
+  const double zmax = (d.l_zmax[ni] - d.l_zmean[ni]) * d.par->lens_st[ni] + d.l_zmean[ni];
+  return 1. / (1 + zmax + 2. * fabs(d.par->lens_dz[ni]));
 }
 __device__ __forceinline__ double clh_amax_lens(const DrawDesc& d, int ni) {
-  const double zmin = (d.l_zmin[ni] - d.l_zmean[ni]) * d.lens_st[ni] + d.l_zmean[ni];
-  if (d.bmag[ni] != 0) return 1. / (1. + fmax(d.s_zmin_all, 0.001));
-  return 1. / (1 + fmax(zmin - 2. * fabs(d.lens_dz[ni]), 0.001));
+  const double zmin = (d.l_zmin[ni] - d.l_zmean[ni]) * d.par->lens_st[ni] + d.l_zmean[ni];
+  if (d.par->bmag[ni] != 0) return 1. / (1. + fmax(d.s_zmin_all, 0.001));
+  return 1. / (1 + fmax(zmin - 2. * fabs(d.par->lens_dz[ni]), 0.001));
+}
+
+// ---------------------------------------------------------------- derivatives
+// d/dx of the spline piece: b + dx (2c + 3 dx d)
+__device__ __forceinline__ double clh_spline_d(const double* __restrict__ zc,
+                                               const double* __restrict__ coef, int rows, double x) {
+  int i = clh_bs(zc, rows, x);
+  if (i > rows - 2) i = rows - 2;
+  const double* c = coef + 4 * i;
+  const double dx = x - zc[i];
+  return c[1] + dx * (2.0 * c[2] + 3.0 * dx * c[3]);
+}
+
+// d n_src(z - dz_b) / d dz_b (zero where the forward value is clamped to zero)
+__device__ __forceinline__ double clh_dnz_src(const DrawDesc& d, double zz, int b) {
+  zz = zz - d.par->src_dz[b];
+  const int n = d.s_rows;
+  if (zz < d.s_zc[0] || zz > d.s_zc[n - 1]) return 0.0;
+  return -clh_spline_d(d.s_zc, d.s_coef + (size_t)b * 4 * (n - 1), n, zz);
+}
+
+// d n_lens / d dz_b of nz_lens_photoz: -spline'(z') / s^2
+__device__ __forceinline__ double clh_dnz_lens(const DrawDesc& d, double zz, int b) {
+  const double s = d.par->lens_st[b];
+  zz = (zz - d.par->lens_dz[b] - d.l_zmean[b]) / s + d.l_zmean[b];
+  const int n = d.l_rows;
+  if (zz <= d.l_zc[0] || zz >= d.l_zc[n - 1]) return 0.0;
+  return -clh_spline_d(d.l_zc, d.l_coef + (size_t)b * 4 * (n - 1), n, zz) / (s * s);
+}
+
+// d g / d dz of the bin's own shift: same interpolation over the tangent table
+__device__ __forceinline__ double clh_dg_tomo(const DrawDesc& d, double a, int b) {
+  const double amin = 1.0 / (d.s_zmax_all + 1.0), amax = 0.999999;
+  const double dac = (amax - amin) / ((double)d.n_a - 1.0);
+  return (a <= amin || a > 1.0 - dac) ? 0.0
+         : clh_interpol1d(d.dgtab + (size_t)b * d.n_a, d.n_a, amin, dac, a);
+}
+__device__ __forceinline__ double clh_dg_lens(const DrawDesc& d, double a, int b) {
+  const double amin = 1.0 / (d.l_zmax_all + 1.0), amax = 0.999999;
+  const double dac = (amax - amin) / ((double)d.n_a - 1.0);
+  return (a < amin || a > 1.0 - dac) ? 0.0
+         : clh_interpol1d(d.dgtab + (size_t)(d.ns + b) * d.n_a, d.n_a, amin, dac, a);
+}
+
+// d C1 / d A_IA for the redshift-evolution mode (C1 is linear in A)
+__device__ __forceinline__ double clh_ia_c1_unit(const DrawDesc& d, double a, double growfac_a) {
+  if (d.ia_mode != 3) return 0.0;
+  return pow((1.0 / a) / 1.62, d.par->ia_eta) * (d.omega_m * 0.01389 / growfac_a);
+}
+
+// p_nonlin plus the table cell and bilinear weights it read (for the adjoint)
+__device__ __forceinline__ double clh_p_nonlin_cells(const DrawDesc& d, double k, double a, int& cell,
+                                                     double w[4]) {
+  const double log10k = log10(k / CLH_COVERH0);
+  const double z = 1.0 / a - 1.0;
+  const int nz = d.nzp;
+  const int i = clh_bs(d.lk, d.nk, log10k);
+  const int j = clh_bs(d.pz, nz, z);
+  const double dx = (log10k - d.lk[i]) / (d.lk[i + 1] - d.lk[i]);
+  const double dy = (z - d.pz[j]) / (d.pz[j + 1] - d.pz[j]);
+  cell = i * nz + j;
+  w[0] = (1 - dx) * (1 - dy);
+  w[1] = (1 - dx) * dy;
+  w[2] = dx * (1 - dy);
+  w[3] = dx * dy;
+  const double* __restrict__ P = d.lnP + cell;
+  const double out = w[0] * P[0] + w[1] * P[1] + w[2] * P[nz] + w[3] * P[nz + 1];
+  return exp(out) / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
+}
+
+__device__ __forceinline__ void clh_scatter_lnP(const DrawDesc& d, int cell, const double w[4],
+                                                double g) {
+  if (g == 0.0) return;
+  double* G = d.g_lnP + cell;
+  atomicAdd(G, g * w[0]);
+  atomicAdd(G + 1, g * w[1]);
+  atomicAdd(G + d.nzp, g * w[2]);
+  atomicAdd(G + d.nzp + 1, g * w[3]);
+}
+
+// index of the ss pair (i, j), i <= j, in the row-major upper triangle
+__device__ __forceinline__ int clh_ss_index(int ns, int i, int j) {
+  if (i > j) { const int t = i; i = j; j = t; }
+  return i * ns - i * (i - 1) / 2 + (j - i);
+}
+
+// ----------------------------------------------------------- block helpers
+__device__ __forceinline__ double clh_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
 }

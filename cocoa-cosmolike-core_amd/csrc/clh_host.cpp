@@ -19,7 +19,16 @@ hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_limber(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow,
-                               hipStream_t s);
+                               hipStream_t s, int raw);
+hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_fill(double* p, size_t n, double v, hipStream_t s);
+hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nknz, const double* lnP,
+                                 const double* src, const double* lens, const double* b1,
+                                 const double* iaA, const double* m, hipStream_t s);
+hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
+hipError_t clh_launch_limber_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s);
+hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, int max_nrow,
                                     hipStream_t s);
 
@@ -164,15 +173,20 @@ struct Batch {
   int B = 0;
   int max_nrow = 0;
   std::vector<DrawDesc> desc;
+  std::vector<DrawParams> par;
   DBuf<DrawDesc> d_desc;
-  DBuf<double> d_tab, d_work, d_dv, d_ells;
-  DBuf<int2> d_wg, d_wn, d_wl2;
+  DBuf<DrawParams> d_par;
+  DBuf<double> d_tab, d_work, d_dv, d_ells, d_vjp, d_gscr;
+  DBuf<int2> d_wg, d_wn, d_wl2, d_wa;
   DBuf<int4> d_wl;
-  int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0;
+  int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0;
   int dv_size = 0;
+  int nknz = 0;        // common ln P table size (0 if the draws differ)
+  int ord[3] = {0, 1, 2};
   void release() {
-    d_desc.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
-    d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release();
+    d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
+    d_vjp.release(); d_gscr.release();
+    d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa.release();
   }
 };
 
@@ -180,7 +194,8 @@ struct Batch {
 
 struct clh_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // stream every launch goes to
+  hipStream_t own_stream = nullptr;  // the context's own stream
   std::string err;
   Layout L;
   Draw cur;
@@ -195,6 +210,9 @@ struct clh_ctx {
          off_ell_gx = 0, off_kfold = 0, off_gl = 0, off_kfull = 0, off_fill = 0;
   Batch single, batch, scratch;
   bool timing = false;
+  bool use_graph = false;             // replay the batch forward from a hipGraph
+  hipGraphExec_t gexec = nullptr;     // captured forward of `batch`
+  hipStream_t gstream = nullptr;      // stream the graph was captured on
   hipEvent_t ev[8] = {nullptr};
   float last_ms[4] = {0, 0, 0, 0};
 };
@@ -224,7 +242,8 @@ static int ensure_device(clh_ctx* c) {
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(c, CLH_ERR_HIP, "no HIP device");
   if (c->device >= n) return fail(c, CLH_ERR_ARG, "device %d of %d", c->device, n);
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+  c->stream = c->own_stream;
   for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
   return CLH_OK;
 }
@@ -335,44 +354,53 @@ static void build_kernels(Layout& L, bool full) {
   lim_ss[2] = (lim_ss[1] - lim_ss[0]) / ((double)L.n_ell - 1.);
   lim_gx[2] = (lim_gx[1] - lim_gx[0]) / ((double)L.n_ell - 1.);
   const double logdt = (std::log(L.vtmax) - std::log(L.vtmin)) / nt;
+  // The bin-averaged G+- kernels are small differences of O(l^2) terms divided
+  // by x_min - x_max (~1.5e-7 at 2.5'): evaluated in double they carry up to
+  // ~1e-4 relative rounding error in xi-(2.5'), so the recurrences and the
+  // kernel formulas run in 80-bit long double and are rounded to double once.
+  using ld = long double;
   auto work = [&](int i) {
-    const double xmin = std::cos(std::exp(std::log(L.vtmin) + (i + 0.) * logdt));
-    const double xmax = std::cos(std::exp(std::log(L.vtmin) + (i + 1.) * logdt));
-    dvec Pmin(Lm + 1), Pmax(Lm + 1), dPmin(Lm + 1), dPmax(Lm + 1);
-    auto leg = [&](double x, dvec& P, dvec& dP) {
-      P[0] = 1.0;
+    const double xmin_d = std::cos(std::exp(std::log(L.vtmin) + (i + 0.) * logdt));
+    const double xmax_d = std::cos(std::exp(std::log(L.vtmin) + (i + 1.) * logdt));
+    const ld xmin = xmin_d, xmax = xmax_d;
+    std::vector<ld> Pmin(Lm + 1), Pmax(Lm + 1), dPmin(Lm + 1), dPmax(Lm + 1);
+    auto leg = [&](ld x, std::vector<ld>& P, std::vector<ld>& dP) {
+      P[0] = 1.0L;
       P[1] = x;
-      double pm2 = 1.0, pm1 = x;
+      ld pm2 = 1.0L, pm1 = x;
       for (int l = 2; l <= Lm; l++) {
-        const double p = (x * (2 * l - 1) * pm1 - (l - 1) * pm2) / l;
+        const ld p = (x * (2 * l - 1) * pm1 - (l - 1) * pm2) / l;
         pm2 = pm1;
         pm1 = p;
         P[l] = p;
       }
-      dP[0] = 0.0;
-      dP[1] = 1.0;
-      const double da = 1.0 + x, db = 1.0 - x;
+      dP[0] = 0.0L;
+      dP[1] = 1.0L;
+      const ld da = 1.0L + x, db = 1.0L - x;
       for (int l = 2; l <= Lm; l++) dP[l] = -l * (x * P[l] - P[l - 1]) / (da * db);
     };
     leg(xmin, Pmin, dPmin);
     leg(xmax, Pmax, dPmax);
+    const ld pi = 3.14159265358979323846264338327950288L;
     for (int l = 1; l < Lm; l++) {
-      const double dx = xmin - xmax;
-      const double common =
-          -l * (l - 1.) / 2 * (l + 2. / (2 * l + 1)) * (Pmin[l - 1] - Pmax[l - 1]) -
-          l * (l - 1.) * (2. - l) / 2 * (xmin * Pmin[l] - xmax * Pmax[l]) +
-          l * (l - 1.) / (2. * l + 1) * (Pmin[l + 1] - Pmax[l + 1]) + (4 - l) * (dPmin[l] - dPmax[l]) +
-          (l + 2) * (xmin * dPmin[l - 1] - xmax * dPmax[l - 1] - Pmin[l - 1] + Pmax[l - 1]);
-      const double t1 = 2 * (l - 1) * (xmin * dPmin[l] - xmax * dPmax[l] - Pmin[l] + Pmax[l]);
-      const double t2 = 2 * (l + 2) * (dPmin[l - 1] - dPmax[l - 1]);
-      const double pre = (2. * l + 1) / (2. * M_PI * l * l * (l + 1) * (l + 1));
+      const ld dx = xmin - xmax;
+      const ld lf = l;
+      const ld common =
+          -lf * (lf - 1) / 2 * (lf + 2.0L / (2 * lf + 1)) * (Pmin[l - 1] - Pmax[l - 1]) -
+          lf * (lf - 1) * (2 - lf) / 2 * (xmin * Pmin[l] - xmax * Pmax[l]) +
+          lf * (lf - 1) / (2 * lf + 1) * (Pmin[l + 1] - Pmax[l + 1]) + (4 - lf) * (dPmin[l] - dPmax[l]) +
+          (lf + 2) * (xmin * dPmin[l - 1] - xmax * dPmax[l - 1] - Pmin[l - 1] + Pmax[l - 1]);
+      const ld t1 = 2 * (lf - 1) * (xmin * dPmin[l] - xmax * dPmax[l] - Pmin[l] + Pmax[l]);
+      const ld t2 = 2 * (lf + 2) * (dPmin[l - 1] - dPmax[l - 1]);
+      const ld pre = (2 * lf + 1) / (2 * pi * lf * lf * (lf + 1) * (lf + 1));
       double g[4];
-      g[0] = pre * (common + t1 - t2) / dx;
-      g[1] = pre * (common - t1 + t2) / dx;
-      g[2] = (2. * l + 1) / (4. * M_PI * l * (l + 1) * dx) *
-             ((l + 2. / (2 * l + 1.)) * (Pmin[l - 1] - Pmax[l - 1]) +
-              (2 - l) * (xmin * Pmin[l] - xmax * Pmax[l]) - 2. / (2 * l + 1.) * (Pmin[l + 1] - Pmax[l + 1]));
-      g[3] = (1.0 / dx) * (1. / (4.0 * M_PI)) * (Pmin[l + 1] - Pmax[l + 1] - Pmin[l - 1] + Pmax[l - 1]);
+      g[0] = (double)(pre * (common + t1 - t2) / dx);
+      g[1] = (double)(pre * (common - t1 + t2) / dx);
+      g[2] = (double)((2 * lf + 1) / (4 * pi * lf * (lf + 1) * dx) *
+                      ((lf + 2.0L / (2 * lf + 1)) * (Pmin[l - 1] - Pmax[l - 1]) +
+                       (2 - lf) * (xmin * Pmin[l] - xmax * Pmax[l]) -
+                       2.0L / (2 * lf + 1) * (Pmin[l + 1] - Pmax[l + 1])));
+      g[3] = (double)((1.0L / dx) * (1.0L / (4 * pi)) * (Pmin[l + 1] - Pmax[l + 1] - Pmin[l - 1] + Pmax[l - 1]));
       for (int k = 0; k < 4; k++) {
         if (full) L.kfull[((size_t)k * Lm + l) * nt + i] = g[k];
         double* kf = L.kfold.data() + (size_t)k * nrow * nt;
@@ -584,6 +612,8 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   }
   std::vector<double> htab(ttot);
   bt.desc.assign(B, DrawDesc());
+  bt.par.assign(B, DrawParams());
+  HIPCHK(c, bt.d_par.ensure(B));
   const double* lay = c->d_layout.p;
   for (int i = 0; i < B; i++) {
     const Draw& d = *draws[i];
@@ -603,12 +633,15 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.l_zc = lay + c->off_l_zc; q.l_coef = lay + c->off_l_coef; q.l_rows = L.lens.rows;
     q.s_zmin_all = L.src.zmin_all; q.s_zmax_all = L.src.zmax_all;
     q.l_zmin_all = L.lens.zmin_all; q.l_zmax_all = L.lens.zmax_all;
+    DrawParams& pp = bt.par[i];
     for (int b = 0; b < CLH_MAXB; b++) {
       q.l_zmin[b] = L.lens.zmin[b]; q.l_zmax[b] = L.lens.zmax[b]; q.l_zmean[b] = L.zmean[b];
-      q.src_dz[b] = d.src_dz[b]; q.lens_dz[b] = d.lens_dz[b]; q.lens_st[b] = d.lens_st[b];
-      q.b1[b] = d.b1[b]; q.bmag[b] = d.bmag[b]; q.shear_m[b] = d.shear_m[b]; q.ia_bin[b] = d.ia_bin[b];
+      pp.src_dz[b] = d.src_dz[b]; pp.lens_dz[b] = d.lens_dz[b]; pp.lens_st[b] = d.lens_st[b];
+      pp.b1[b] = d.b1[b]; pp.bmag[b] = d.bmag[b]; pp.shear_m[b] = d.shear_m[b]; pp.ia_bin[b] = d.ia_bin[b];
     }
-    q.ia_A = d.ia_A; q.ia_eta = d.ia_eta; q.ia_mode = L.ia_mode;
+    pp.ia_A = d.ia_A; pp.ia_eta = d.ia_eta;
+    q.par = bt.d_par.p + i;
+    q.ia_mode = L.ia_mode;
     q.ell_ss = ells ? bt.d_ells.p : lay + c->off_ell_ss;
     q.ell_gx = ells ? bt.d_ells.p : lay + c->off_ell_gx;
     q.kfold = lay + c->off_kfold;
@@ -636,6 +669,11 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   HIPCHK(c, hipMemcpyAsync(bt.d_tab.p, htab.data(), ttot * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, bt.d_desc.ensure(B));
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(bt.d_par.p, bt.par.data(), B * sizeof(DrawParams), hipMemcpyHostToDevice, c->stream));
+  bt.nknz = (int)draws[0]->lnP.size();
+  for (int i = 1; i < B; i++)
+    if ((int)draws[i]->lnP.size() != bt.nknz || draws[i]->pz.size() != draws[0]->pz.size()) bt.nknz = 0;
+  for (int k = 0; k < 3; k++) bt.ord[k] = ord[k];
   // work lists
   std::vector<int2> wg, wn, wl2;
   std::vector<int4> wl;
@@ -654,6 +692,12 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   const int ntask = 2 * n_ss + n_gs + nl;
   for (int i = 0; i < B; i++)
     for (int t = 0; t < ntask; t += 8) wl2.push_back(make_int2(i, t));
+  std::vector<int2> wa;  // Legendre adjoint: 8 C_ell columns per block
+  for (int i = 0; i < B; i++)
+    for (int col = 0; col < ncol; col += 8) wa.push_back(make_int2(i, col));
+  HIPCHK(c, bt.d_wa.ensure(wa.size()));
+  HIPCHK(c, hipMemcpyAsync(bt.d_wa.p, wa.data(), wa.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+  bt.n_wa = (int)wa.size();
   HIPCHK(c, bt.d_wg.ensure(wg.size()));
   HIPCHK(c, bt.d_wn.ensure(wn.size()));
   HIPCHK(c, bt.d_wl.ensure(wl.size()));
@@ -683,7 +727,7 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
     if (c->legendre_mode == CLH_LEGENDRE_FULL)
       HIPCHK(c, clh_launch_legendre_full(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s));
     else
-      HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s));
+      HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s, 0));
   }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
   return CLH_OK;
@@ -729,10 +773,11 @@ void clh_destroy(clh_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
   }
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   c->single.release(); c->batch.release(); c->scratch.release();
   c->d_layout.release();
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
 }
 
@@ -1003,6 +1048,7 @@ int clh_batch_upload(clh_ctx* c, int B, const clh_draw* draws) {
     ptr[i] = &ds[i];
   }
   const int ord[3] = {0, 1, 2};
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   return build_batch(c, c->batch, ptr, 1, ord, nullptr, nullptr);
 }
 
@@ -1030,10 +1076,41 @@ int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
       changed = true;
     }
   }
+  for (int k = 0; k < 3; k++) c->batch.ord[k] = ord[k];
   if (changed)
     HIPCHK(c, hipMemcpyAsync(c->batch.d_desc.p, c->batch.desc.data(), c->batch.B * sizeof(DrawDesc),
                              hipMemcpyHostToDevice, c->stream));
+  if (c->use_graph && !c->timing) {
+    // the kernels read placement and inputs from the descriptors in device
+    // memory, so one captured graph serves every later call on this batch
+    if (!c->gexec || c->gstream != c->stream) {
+      if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+      hipGraph_t g = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = run_batch(c, c->batch, true);
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+      if (e != hipSuccess) return fail(c, CLH_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) { c->gexec = nullptr; return fail(c, CLH_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+      c->gstream = c->stream;
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    return CLH_OK;
+  }
   return run_batch(c, c->batch, true);
+}
+
+int clh_set_stream(clh_ctx* c, void* stream) {
+  if (!c) return CLH_ERR_ARG;
+  int rc = ensure_device(c);
+  if (rc) return rc;
+  if (c->own_stream) {
+    HIPCHK(c, hipStreamSynchronize(c->own_stream));
+  }
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return CLH_OK;
 }
 
 int clh_synchronize(clh_ctx* c) {
@@ -1042,6 +1119,13 @@ int clh_synchronize(clh_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->timing)
     for (int k = 0; k < 4; k++) HIPCHK(c, hipEventElapsedTime(&c->last_ms[k], c->ev[k], c->ev[k + 1]));
+  return CLH_OK;
+}
+
+int clh_batch_use_graph(clh_ctx* c, int on) {
+  if (!c) return CLH_ERR_ARG;
+  c->use_graph = on != 0;
+  if (!c->use_graph && c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   return CLH_OK;
 }
 
@@ -1059,6 +1143,99 @@ int clh_last_kernel_times(clh_ctx* c, const char** names, float* ms, int cap) {
     if (ms) ms[k] = c->last_ms[k];
   }
   return 4;
+}
+
+}  // extern "C"
+
+// ============================================================ batch VJP
+extern "C" {
+
+int clh_batch_set_params(clh_ctx* c, const double* lnP, const double* src_dz, const double* lens_dz,
+                         const double* b1, const double* ia_A, const double* shear_m) {
+  if (!c) return CLH_ERR_ARG;
+  Batch& bt = c->batch;
+  if (bt.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
+  if (lnP && bt.nknz <= 0) return fail(c, CLH_ERR_ARG, "draws of the batch have different P(k,z) shapes");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, clh_launch_set_params(bt.d_desc.p, bt.d_par.p, bt.B, bt.nknz, lnP, src_dz, lens_dz, b1, ia_A,
+                                  shear_m, c->stream));
+  return CLH_OK;
+}
+
+int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_lnP, double* g_src_dz,
+                  double* g_lens_dz, double* g_b1, double* g_ia_A, double* g_m) {
+  if (!c || !dv_bar || !ord) return CLH_ERR_ARG;
+  Batch& bt = c->batch;
+  if (bt.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
+  if (bt.nknz <= 0) return fail(c, CLH_ERR_ARG, "draws of the batch have different P(k,z) shapes");
+  if (!c->dev_layout_valid) return fail(c, CLH_ERR_STATE, "layout changed since clh_batch_upload");
+  const Layout& L = c->L;
+  const int B = bt.B, ns = L.src.nbin, nl = L.lens.nbin, nrow = L.nrow();
+  const int ncol = L.n_ss() + L.n_gs() + nl;
+  HIPCHK(c, hipSetDevice(c->device));
+  // per-draw adjoint buffers
+  auto al = [](size_t n) { return (n + 31) & ~size_t(31); };
+  const size_t n_cbar = al((size_t)ncol * nrow), n_raw = al(bt.dv_size), n_abar = al((size_t)ns * CLH_NP_SS),
+               n_bbar = al((size_t)nl * ns * CLH_NP_SS),
+               n_wbar = al((size_t)nl * 2 * CLH_NP_SS + (size_t)nl * 2 * CLH_NP_GG),
+               n_dg = al((size_t)(ns + nl) * L.n_a);
+  const size_t per = n_cbar + n_raw + n_abar + n_bbar + n_wbar + n_dg;
+  HIPCHK(c, bt.d_vjp.ensure(per * B));
+  // scratch for gradients the caller does not want
+  const size_t gper = al(bt.nknz) + 4 * al(CLH_MAXB);
+  HIPCHK(c, bt.d_gscr.ensure(gper * B));
+  {  // block starts of dv_bar (generic_interface.hpp:770-791)
+    const int nt = L.ntheta;
+    const int sizes[3] = {2 * nt * L.n_ss(), nt * L.n_gs(), nt * nl};
+    int idx[3] = {0, 1, 2};
+    for (int i = 1; i < 3; i++)
+      for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) std::swap(idx[j], idx[j - 1]);
+    int start[3] = {0, 0, 0};
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+    for (int i = 0; i < B; i++)
+      for (int k = 0; k < 3; k++) bt.desc[i].dv_start[k] = start[k];
+  }
+  for (int i = 0; i < B; i++) {
+    DrawDesc& q = bt.desc[i];
+    double* v = bt.d_vjp.p + per * i;
+    q.cbar = v; v += n_cbar;
+    q.raw = v; v += n_raw;
+    q.abar = v; v += n_abar;
+    q.bbar = v; v += n_bbar;
+    q.wbar = v; v += n_wbar;
+    q.dgtab = v;
+    double* g = bt.d_gscr.p + gper * i;
+    q.dvbar = dv_bar + (size_t)bt.dv_size * i;
+    q.g_lnP = g_lnP ? g_lnP + (size_t)bt.nknz * i : g;
+    q.g_src = g_src_dz ? g_src_dz + (size_t)ns * i : g + al(bt.nknz);
+    q.g_lens = g_lens_dz ? g_lens_dz + (size_t)nl * i : g + al(bt.nknz) + al(CLH_MAXB);
+    q.g_b1 = g_b1 ? g_b1 + (size_t)nl * i : g + al(bt.nknz) + 2 * al(CLH_MAXB);
+    q.g_A = g_ia_A ? g_ia_A + i : g + al(bt.nknz) + 3 * al(CLH_MAXB);
+    q.g_m = g_m ? g_m + (size_t)ns * i : g + al(bt.nknz) + 3 * al(CLH_MAXB) + 1;
+  }
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), B * sizeof(DrawDesc), hipMemcpyHostToDevice, s));
+  // zero the accumulators (fill kernels, not memsets: safe under graph capture)
+  HIPCHK(c, clh_launch_fill(bt.d_vjp.p, per * B, 0.0, s));
+  HIPCHK(c, clh_launch_fill(bt.d_gscr.p, gper * B, 0.0, s));
+  if (g_lnP) HIPCHK(c, clh_launch_fill(g_lnP, (size_t)bt.nknz * B, 0.0, s));
+  if (g_src_dz) HIPCHK(c, clh_launch_fill(g_src_dz, (size_t)ns * B, 0.0, s));
+  if (g_lens_dz && nl) HIPCHK(c, clh_launch_fill(g_lens_dz, (size_t)nl * B, 0.0, s));
+  if (g_b1 && nl) HIPCHK(c, clh_launch_fill(g_b1, (size_t)nl * B, 0.0, s));
+  if (g_ia_A) HIPCHK(c, clh_launch_fill(g_ia_A, (size_t)B, 0.0, s));
+  // forward recompute (tables, nodes, C_ell rows, uncalibrated data vector)
+  HIPCHK(c, clh_launch_gtab(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
+  HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
+  HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
+  HIPCHK(c, clh_launch_limber(bt.d_desc.p, bt.d_wl.p, bt.n_wl, s));
+  HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s, 1));
+  // reverse sweep
+  HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa.p, bt.n_wa, s));
+  HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
+  HIPCHK(c, clh_launch_limber_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl, s));
+  HIPCHK(c, clh_launch_nodes_adj(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
+  return CLH_OK;
 }
 
 }  // extern "C"

@@ -22,6 +22,8 @@
 
 // Everything one draw needs on the device.  Plain data; lives in an array in
 // device memory and is read with uniform (scalar) loads.
+struct DrawParams;
+
 struct DrawDesc {
   // cosmology tables (cosmopara, cosmolike/structs.h:128-191)
   const double* chi_z;
@@ -30,7 +32,7 @@ struct DrawDesc {
   const double* G_v;
   const double* lk;   // log10 k [h/Mpc]
   const double* pz;   // z of the P(k,z) table
-  const double* lnP;  // [nk][nzp]
+  double* lnP;        // [nk][nzp] (writable: device-side parameter updates)
   int chi_n, G_n, nk, nzp;
   double omega_m;
 
@@ -51,11 +53,8 @@ struct DrawDesc {
   double lim_ss[3], lim_gx[3];
   int lmax;
 
-  // nuisance (nuisanceparams, cosmolike/structs.h:203-298)
-  double src_dz[CLH_MAXB], lens_dz[CLH_MAXB], lens_st[CLH_MAXB];
-  double b1[CLH_MAXB], bmag[CLH_MAXB], shear_m[CLH_MAXB];
-  double ia_bin[CLH_MAXB];
-  double ia_A, ia_eta;
+  // nuisance values of the draw (nuisanceparams, cosmolike/structs.h:203-298)
+  const struct DrawParams* par;
   int ia_mode;
 
   // per-draw work buffers
@@ -70,6 +69,28 @@ struct DrawDesc {
   double G0;        // G(z=0) by the growth table's own interpolation
   const double* gl; // GL abscissae/weights on [-1,1]: x64 w64 x96 w96
   int apply_calib;  // 1: shear calibration applied to the data vector
+
+  // adjoint (VJP) buffers of the draw
+  const double* dvbar;  // cotangent of the data vector
+  double* raw;          // uncalibrated data vector
+  double* cbar;         // [ncol][nrow] cotangent of the C_ell rows
+  double* abar;         // [ns][64] cotangent of A_b at the ss nodes
+  double* bbar;         // [nl][ns][64] cotangent of B_zs at the gs nodes
+  double* wbar;         // [nl][2][64] gs nodes (W_gal, W_mag), then [nl][2][96] gg nodes
+  double* dgtab;        // [ns+nl][n_a] d g / d(photo-z shift of its own bin)
+  double* g_lnP;        // [nk][nzp]
+  double* g_src;        // [ns]
+  double* g_lens;       // [nl]
+  double* g_b1;         // [nl]
+  double* g_A;          // [1]
+  double* g_m;          // [ns]
+};
+
+struct DrawParams {
+  double src_dz[CLH_MAXB], lens_dz[CLH_MAXB], lens_st[CLH_MAXB];
+  double b1[CLH_MAXB], bmag[CLH_MAXB], shear_m[CLH_MAXB];
+  double ia_bin[CLH_MAXB];
+  double ia_A, ia_eta;
 };
 
 // Node buffer layout (doubles):

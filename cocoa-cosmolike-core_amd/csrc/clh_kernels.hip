@@ -51,6 +51,9 @@ __device__ __forceinline__ double clh_block_sum(double v, double* lds) {
 // k_gtab: one block per (draw, bin).  bin < ns -> g_tomo of source bin,
 // otherwise g_lens of lens bin (bin - ns).
 // ----------------------------------------------------------------------------
+// TANGENT: the same prefix sums over d(integrand)/d(photo-z shift of the bin),
+// i.e. the derivative of the table with respect to that shift.
+template <bool TANGENT>
 __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__ D,
                                                     const int2* __restrict__ work) {
   __shared__ double sP[1024], sQ[1024];
@@ -76,8 +79,10 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
     for (int i = 1 + threadIdx.x; i < Na0; i += CLH_BLOCK) {
       const double apl = amin_shear + (i - 1) * da0;
       const double aph = amin_shear + i * da0;
-      const double fl = clh_nz_lens(d, 1.0 / apl - 1.0, b) / (apl * apl);
-      const double fh = clh_nz_lens(d, 1.0 / aph - 1.0, b) / (aph * aph);
+      const double fl = (TANGENT ? clh_dnz_lens(d, 1.0 / apl - 1.0, b)
+                                 : clh_nz_lens(d, 1.0 / apl - 1.0, b)) / (apl * apl);
+      const double fh = (TANGENT ? clh_dnz_lens(d, 1.0 / aph - 1.0, b)
+                                 : clh_nz_lens(d, 1.0 / aph - 1.0, b)) / (aph * aph);
       p += 0.5 * da0 * (fl + fh);
       q += 0.5 * da0 * (fl / clh_chi(d, apl) + fh / clh_chi(d, aph));
     }
@@ -92,7 +97,10 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
     for (int s = 0; s <= x; s++) {
       const int i = t * x + s;
       const double a = amin + i * da;
-      const double f = (src ? clh_nz_src(d, 1. / a - 1., b) : clh_nz_lens(d, 1.0 / a - 1.0, b)) / (a * a);
+      double n;
+      if (TANGENT) n = src ? clh_dnz_src(d, 1. / a - 1., b) : clh_dnz_lens(d, 1.0 / a - 1.0, b);
+      else n = src ? clh_nz_src(d, 1. / a - 1., b) : clh_nz_lens(d, 1.0 / a - 1.0, b);
+      const double f = n / (a * a);
       const double g = f / clh_chi(d, amin + i * da);
       if (s > 0) {
         p += 0.5 * da * (fprev + f);
@@ -115,7 +123,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   const double ip = clh_block_scan(cp, red);
   const double iq = clh_block_scan(cq, red);
   double P = P0 + ip - cp, Q = Q0 + iq - cq;
-  double* tab = d.gtab + (size_t)w.y * d.n_a;
+  double* tab = (TANGENT ? d.dgtab : d.gtab) + (size_t)w.y * d.n_a;
   for (int t = t0; t < t1; t++) {
     P += sP[t];
     Q += sQ[t];
@@ -244,7 +252,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber(const DrawDesc* __restrict
     } else {
       // W rows: 0 = WGAL, 1 = WMAG, 2.. = B[zs]
       const int nout = d.ns * CLH_SS_ROWS;
-      const double b1 = d.b1[zl], bmag = d.bmag[zl];
+      const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
       for (int q = tid; q < nout; q += CLH_BLOCK) {
         const int zs = q / CLH_SS_ROWS, rr = q % CLH_SS_ROWS;
         const int r = row0 + rr;
@@ -265,7 +273,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber(const DrawDesc* __restrict
     // gg: full integrand per (row, node) incl. RSD (cosmo2D.c:2490-2608)
     const int np = CLH_NP_GG;
     const double* o = d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride;
-    const double b1 = d.b1[zl], bmag = d.bmag[zl];
+    const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
     for (int it = tid; it < CLH_GG_ROWS * np; it += CLH_BLOCK) {
       const int rr = it / np, p = it % np;
       const int r = row0 + rr;
@@ -307,8 +315,9 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber(const DrawDesc* __restrict
 // xi-, then gamma_t pairs, then w bins.
 // ----------------------------------------------------------------------------
 #define CLH_LEG_TASKS 8
+// raw = 1: uncalibrated values into the draw's `raw` buffer (for the VJP)
 __global__ __launch_bounds__(CLH_BLOCK) void k_legendre(const DrawDesc* __restrict__ D,
-                                                        const int2* __restrict__ work) {
+                                                        const int2* __restrict__ work, int raw) {
   extern __shared__ double colbuf[];  // [CLH_LEG_TASKS][nrow]
   const int2 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
@@ -346,15 +355,15 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre(const DrawDesc* __restri
   if (kind <= 1) {
     int i, j;
     clh_ss_pair(d.ns, q, i, j);
-    if (d.apply_calib) f = (1.0 + d.shear_m[i]) * (1.0 + d.shear_m[j]);
+    if (d.apply_calib && !raw) f = (1.0 + d.par->shear_m[i]) * (1.0 + d.par->shear_m[j]);
     idx = d.dv_start[0] + nt * q + th + (kind == 1 ? nt * d.n_ss : 0);
   } else if (kind == 2) {
-    if (d.apply_calib) f = 1.0 + d.shear_m[q % d.ns];
+    if (d.apply_calib && !raw) f = 1.0 + d.par->shear_m[q % d.ns];
     idx = d.dv_start[1] + nt * q + th;
   } else {
     idx = d.dv_start[2] + nt * q + th;
   }
-  d.dv[idx] = acc * f;
+  (raw ? d.raw : d.dv)[idx] = acc * f;
 }
 
 // ----------------------------------------------------------------------------
@@ -406,10 +415,10 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre_full(const DrawDesc* __r
   if (kind <= 1) {
     int i, j;
     clh_ss_pair(d.ns, q, i, j);
-    if (d.apply_calib) f = (1.0 + d.shear_m[i]) * (1.0 + d.shear_m[j]);
+    if (d.apply_calib) f = (1.0 + d.par->shear_m[i]) * (1.0 + d.par->shear_m[j]);
     idx = d.dv_start[0] + nt * q + th + (kind == 1 ? nt * d.n_ss : 0);
   } else if (kind == 2) {
-    if (d.apply_calib) f = 1.0 + d.shear_m[q % d.ns];
+    if (d.apply_calib) f = 1.0 + d.par->shear_m[q % d.ns];
     idx = d.dv_start[1] + nt * q + th;
   } else {
     idx = d.dv_start[2] + nt * q + th;
@@ -422,7 +431,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre_full(const DrawDesc* __r
 // ----------------------------------------------------------------------------
 hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gtab, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  hipLaunchKernelGGL(k_gtab<false>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s) {
@@ -436,10 +445,10 @@ hipError_t clh_launch_limber(const DrawDesc* D, const int4* work, int n, hipStre
   return hipGetLastError();
 }
 hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow,
-                               hipStream_t s) {
+                               hipStream_t s, int raw) {
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * CLH_LEG_TASKS * (size_t)max_nrow;
-  hipLaunchKernelGGL(k_legendre, dim3(n), dim3(CLH_BLOCK), lds, s, D, work);
+  hipLaunchKernelGGL(k_legendre, dim3(n), dim3(CLH_BLOCK), lds, s, D, work, raw);
   return hipGetLastError();
 }
 hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, int max_nrow,
@@ -447,5 +456,10 @@ hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, 
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * CLH_LEG_TASKS * (size_t)max_nrow;
   hipLaunchKernelGGL(k_legendre_full, dim3(n), dim3(CLH_BLOCK), lds, s, D, work);
+  return hipGetLastError();
+}
+hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gtab<true>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }

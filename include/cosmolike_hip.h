@@ -124,6 +124,28 @@ int clh_batch_upload(clh_ctx* ctx, int B, const clh_draw* draws);
  * dv_device: device pointer to B * clh_datavector_size() doubles */
 int clh_batch_forward(clh_ctx* ctx, const int ord[3], double* dv_device);
 int clh_synchronize(clh_ctx* ctx);
+/* on != 0: clh_batch_forward captures its launches into a hipGraph on first
+ * use and replays it afterwards (re-captured after clh_batch_upload) */
+int clh_batch_use_graph(clh_ctx* ctx, int on);
+/* launch every subsequent call on `stream` (a hipStream_t; NULL = the
+ * context's own stream), e.g. the caller's current stream */
+int clh_set_stream(clh_ctx* ctx, void* stream);
+
+/* Device-side update of the uploaded batch's differentiable inputs (device
+ * pointers, [B][...] row-major, NULL = keep): lnP [B][nk*nz] (all draws must
+ * share the table shape), src_dz [B][ns], lens_dz [B][nl], b1 [B][nl],
+ * ia_A [B], shear_m [B][ns]. No counterpart in the reference, which has no
+ * gradients; it serves the autograd layer. */
+int clh_batch_set_params(clh_ctx* ctx, const double* lnP, const double* src_dz,
+                         const double* lens_dz, const double* b1, const double* ia_A,
+                         const double* shear_m);
+/* Vector-Jacobian product of the batch's data vectors (blocks placed by ord,
+ * as clh_batch_forward) with the cotangent dv_bar [B][n] (device). Writes, for
+ * every non-NULL device pointer, d<dv_bar, dv>/d(input): g_lnP [B][nk*nz],
+ * g_src_dz [B][ns], g_lens_dz [B][nl], g_b1 [B][nl], g_ia_A [B], g_m [B][ns].
+ * Differentiates the discrete algorithm (no reference counterpart). */
+int clh_batch_vjp(clh_ctx* ctx, const int ord[3], const double* dv_bar, double* g_lnP,
+                  double* g_src_dz, double* g_lens_dz, double* g_b1, double* g_ia_A, double* g_m);
 /* per-kernel timing of the last clh_batch_forward (HIP events on the context
  * stream): names[i], milliseconds[i]; returns the kernel count */
 int clh_last_kernel_times(clh_ctx* ctx, const char** names, float* ms, int cap);

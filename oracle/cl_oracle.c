@@ -58,6 +58,7 @@ struct oc_ctx {
   int g_valid;
   double *gt_tab, *gl_tab; /* [ns][N_a], [nl][N_a] */
   int K_valid;
+  int K_ld;  /* 1: kernels evaluated in long double (default), 0: in double */
   double *K; /* [4][ntheta][lmax] */
   int status;
   char err[256];
@@ -84,6 +85,7 @@ oc_ctx* oc_create(void) {
   c->lmax = 50000;
   c->hdi = 0;
   c->c1rhocrit = 0.01389;
+  c->K_ld = 1;
   c->oneplusz0 = 1.62; /* set_nuisance_IA, generic_interface.cpp:1756 */
   for (int i = 0; i < OC_MAXB; i++) c->lens_st[i] = 1.0;
   return c;
@@ -1081,7 +1083,66 @@ int oc_legendre_pl_dpl(int lmax, double x, double* P, double* dP) {
 }
 
 /* set_bin_average (basics.c:382-455) + G builds (cosmo2D.c:377-460,
- * 581-644, 758-823) */
+ * 581-644, 758-823).  The formulas are evaluated in REAL: the reference's
+ * double (K_ld = 0), or long double (K_ld = 1, default) because the G+-
+ * kernels are small differences of O(l^2) terms divided by x_min - x_max
+ * (~1.5e-7 at 2.5'), whose double evaluation carries up to ~1e-4 relative
+ * error in xi-(2.5') depending on the compiler's FMA contraction. */
+#define OC_KERNEL_BODY(REAL, PI)                                                              \
+  for (int i = i0; i < nt; i += stride) {                                                     \
+    const double xmin_d = cos(exp(log(c->vtmin) + (i + 0.) * logdt));                       \
+    const double xmax_d = cos(exp(log(c->vtmin) + (i + 1.) * logdt));                       \
+    const REAL xmin = xmin_d, xmax = xmax_d;                                                 \
+    REAL* Pmin = malloc(sizeof(REAL) * (L + 1));                                              \
+    REAL* Pmax = malloc(sizeof(REAL) * (L + 1));                                              \
+    REAL* dPmin = malloc(sizeof(REAL) * (L + 1));                                             \
+    REAL* dPmax = malloc(sizeof(REAL) * (L + 1));                                             \
+    for (int s = 0; s < 2; s++) {                                                             \
+      REAL* P = s ? Pmax : Pmin;                                                              \
+      REAL* dP = s ? dPmax : dPmin;                                                           \
+      const REAL x = s ? xmax : xmin;                                                         \
+      P[0] = 1; P[1] = x;                                                                     \
+      REAL pm2 = 1, pm1 = x;                                                                  \
+      for (int l = 2; l <= L; l++) {                                                          \
+        const REAL p = (x * (2 * l - 1) * pm1 - (l - 1) * pm2) / l;                           \
+        pm2 = pm1; pm1 = p; P[l] = p;                                                         \
+      }                                                                                       \
+      dP[0] = 0; dP[1] = 1;                                                                   \
+      const REAL da = 1 + x, db = 1 - x;                                                      \
+      for (int l = 2; l <= L; l++) dP[l] = -l * (x * P[l] - P[l - 1]) / (da * db);            \
+    }                                                                                         \
+    double* gp = Gp + (size_t)i * L;                                                          \
+    double* gm = Gm + (size_t)i * L;                                                          \
+    double* pg = Pgt + (size_t)i * L;                                                         \
+    double* pw = Pw + (size_t)i * L;                                                          \
+    gp[0] = gm[0] = pg[0] = pw[0] = 0.0;                                                      \
+    for (int l = 1; l < L; l++) {                                                             \
+      gp[l] = (double)((2. * l + 1) / (2. * PI * l * l * (l + 1) * (l + 1)) * (              \
+          -l * (l - 1.) / 2 * (l + 2. / (2 * l + 1)) * (Pmin[l - 1] - Pmax[l - 1])           \
+          - l * (l - 1.) * (2. - l) / 2 * (xmin * Pmin[l] - xmax * Pmax[l])                   \
+          + l * (l - 1.) / (2. * l + 1) * (Pmin[l + 1] - Pmax[l + 1])                         \
+          + (4 - l) * (dPmin[l] - dPmax[l])                                                   \
+          + (l + 2) * (xmin * dPmin[l - 1] - xmax * dPmax[l - 1] - Pmin[l - 1] + Pmax[l - 1]) \
+          + 2 * (l - 1) * (xmin * dPmin[l] - xmax * dPmax[l] - Pmin[l] + Pmax[l])             \
+          - 2 * (l + 2) * (dPmin[l - 1] - dPmax[l - 1])) / (xmin - xmax));                    \
+      gm[l] = (double)((2. * l + 1) / (2. * PI * l * l * (l + 1) * (l + 1)) * (              \
+          -l * (l - 1.) / 2 * (l + 2. / (2 * l + 1)) * (Pmin[l - 1] - Pmax[l - 1])           \
+          - l * (l - 1.) * (2. - l) / 2 * (xmin * Pmin[l] - xmax * Pmax[l])                   \
+          + l * (l - 1.) / (2. * l + 1) * (Pmin[l + 1] - Pmax[l + 1])                         \
+          + (4 - l) * (dPmin[l] - dPmax[l])                                                   \
+          + (l + 2) * (xmin * dPmin[l - 1] - xmax * dPmax[l - 1] - Pmin[l - 1] + Pmax[l - 1]) \
+          - 2 * (l - 1) * (xmin * dPmin[l] - xmax * dPmax[l] - Pmin[l] + Pmax[l])             \
+          + 2 * (l + 2) * (dPmin[l - 1] - dPmax[l - 1])) / (xmin - xmax));                    \
+      pg[l] = (double)((2. * l + 1) / (4. * PI * l * (l + 1) * (xmin - xmax)) *              \
+              ((l + 2. / (2 * l + 1.)) * (Pmin[l - 1] - Pmax[l - 1]) +                        \
+               (2 - l) * (xmin * Pmin[l] - xmax * Pmax[l]) -                                  \
+               2. / (2 * l + 1.) * (Pmin[l + 1] - Pmax[l + 1])));                             \
+      pw[l] = (double)((1.0 / (xmin - xmax)) * (1. / (4.0 * PI)) *                           \
+                       (Pmin[l + 1] - Pmax[l + 1] - Pmin[l - 1] + Pmax[l - 1]));              \
+    }                                                                                         \
+    free(Pmin); free(Pmax); free(dPmin); free(dPmax);                                         \
+  }
+
 static int build_kernels(oc_ctx* c) {
   const int nt = c->ntheta, L = c->lmax;
   if (nt <= 0) { oc_set_err(c, "Ntable.Ntheta not initialized"); return -1; }
@@ -1092,49 +1153,31 @@ static int build_kernels(oc_ctx* c) {
   double* Pgt = c->K + (size_t)2 * nt * L;
   double* Pw = c->K + (size_t)3 * nt * L;
   const double logdt = (log(c->vtmax) - log(c->vtmin)) / nt;
-  int bad = 0;
-  #pragma omp parallel for schedule(dynamic)
   for (int i = 0; i < nt; i++) {
-    const double xmin = cos(exp(log(c->vtmin) + (i + 0.) * logdt));
-    const double xmax = cos(exp(log(c->vtmin) + (i + 1.) * logdt));
-    double* Pmin = malloc(sizeof(double) * (L + 1));
-    double* Pmax = malloc(sizeof(double) * (L + 1));
-    double* dPmin = malloc(sizeof(double) * (L + 1));
-    double* dPmax = malloc(sizeof(double) * (L + 1));
-    if (oc_legendre_pl_dpl(L, xmin, Pmin, dPmin) || oc_legendre_pl_dpl(L, xmax, Pmax, dPmax)) bad = 1;
-    double* gp = Gp + (size_t)i * L;
-    double* gm = Gm + (size_t)i * L;
-    double* pg = Pgt + (size_t)i * L;
-    double* pw = Pw + (size_t)i * L;
-    gp[0] = gm[0] = pg[0] = pw[0] = 0.0;
-    for (int l = 1; l < L; l++) {
-      gp[l] = (2. * l + 1) / (2. * M_PI * l * l * (l + 1) * (l + 1)) * (
-          -l * (l - 1.) / 2 * (l + 2. / (2 * l + 1)) * (Pmin[l - 1] - Pmax[l - 1])
-          - l * (l - 1.) * (2. - l) / 2 * (xmin * Pmin[l] - xmax * Pmax[l])
-          + l * (l - 1.) / (2. * l + 1) * (Pmin[l + 1] - Pmax[l + 1])
-          + (4 - l) * (dPmin[l] - dPmax[l])
-          + (l + 2) * (xmin * dPmin[l - 1] - xmax * dPmax[l - 1] - Pmin[l - 1] + Pmax[l - 1])
-          + 2 * (l - 1) * (xmin * dPmin[l] - xmax * dPmax[l] - Pmin[l] + Pmax[l])
-          - 2 * (l + 2) * (dPmin[l - 1] - dPmax[l - 1])) / (xmin - xmax);
-      gm[l] = (2. * l + 1) / (2. * M_PI * l * l * (l + 1) * (l + 1)) * (
-          -l * (l - 1.) / 2 * (l + 2. / (2 * l + 1)) * (Pmin[l - 1] - Pmax[l - 1])
-          - l * (l - 1.) * (2. - l) / 2 * (xmin * Pmin[l] - xmax * Pmax[l])
-          + l * (l - 1.) / (2. * l + 1) * (Pmin[l + 1] - Pmax[l + 1])
-          + (4 - l) * (dPmin[l] - dPmax[l])
-          + (l + 2) * (xmin * dPmin[l - 1] - xmax * dPmax[l - 1] - Pmin[l - 1] + Pmax[l - 1])
-          - 2 * (l - 1) * (xmin * dPmin[l] - xmax * dPmax[l] - Pmin[l] + Pmax[l])
-          + 2 * (l + 2) * (dPmin[l - 1] - dPmax[l - 1])) / (xmin - xmax);
-      pg[l] = (2. * l + 1) / (4. * M_PI * l * (l + 1) * (xmin - xmax)) *
-              ((l + 2. / (2 * l + 1.)) * (Pmin[l - 1] - Pmax[l - 1]) +
-               (2 - l) * (xmin * Pmin[l] - xmax * Pmax[l]) -
-               2. / (2 * l + 1.) * (Pmin[l + 1] - Pmax[l + 1]));
-      const double tmp = (1.0 / (xmin - xmax)) * (1. / (4.0 * M_PI));
-      pw[l] = tmp * (Pmin[l + 1] - Pmax[l + 1] - Pmin[l - 1] + Pmax[l - 1]);
-    }
-    free(Pmin); free(Pmax); free(dPmin); free(dPmax);
+    const double xa = cos(exp(log(c->vtmin) + (i + 0.) * logdt));
+    const double xb = cos(exp(log(c->vtmin) + (i + 1.) * logdt));
+    if (fabs(xa) > 1 || fabs(xb) > 1) { oc_set_err(c, "Legendre argument out of range"); return -1; }
   }
-  if (bad) { oc_set_err(c, "Legendre argument out of range"); return -1; }
+  #pragma omp parallel
+  {
+#ifdef _OPENMP
+    const int i0 = omp_get_thread_num(), stride = omp_get_num_threads();
+#else
+    const int i0 = 0, stride = 1;
+#endif
+    if (c->K_ld) {
+      OC_KERNEL_BODY(long double, 3.14159265358979323846264338327950288L)
+    } else {
+      OC_KERNEL_BODY(double, M_PI)
+    }
+  }
   c->K_valid = 1;
+  return 0;
+}
+
+int oc_set_kernel_precision(oc_ctx* c, int extended) {
+  c->K_ld = extended ? 1 : 0;
+  c->K_valid = 0;
   return 0;
 }
 

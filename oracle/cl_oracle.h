@@ -51,6 +51,8 @@ int oc_chi_all(oc_ctx* c, const double* a, int n, double* chi, double* dchida);
 int oc_pdelta(oc_ctx* c, const double* k, const double* a, int n, double* out);
 int oc_gl_nodes(int n, double* x, double* w);
 int oc_legendre_pl_dpl(int lmax, double x, double* P, double* dP);
+/* 1 (default): kernels evaluated in long double; 0: in double, as the reference */
+int oc_set_kernel_precision(oc_ctx* c, int extended);
 /* G kernels [ntheta][LMAX] for xi+, xi-, gamma_t, w (cosmo2D.c:430-453,631-639,808-815) */
 int oc_kernels(oc_ctx* c, double* Gp, double* Gm, double* Pgt, double* Pw);
 

@@ -48,6 +48,7 @@ def _load():
         lib.oc_set_omega_m.argtypes = [vp, ctypes.c_double]
         lib.oc_set_ia.argtypes = [vp, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double]
         lib.oc_gl_nodes.argtypes = [ctypes.c_int, _D, _D]
+        lib.oc_set_kernel_precision.argtypes = [vp, ctypes.c_int]
         lib.oc_legendre_pl_dpl.argtypes = [ctypes.c_int, ctypes.c_double, _D, _D]
         _lib = lib
     return _lib
@@ -158,6 +159,9 @@ class Oracle:
                           prob.bmag if prob.lens is not None else None, prob.shear_m)
         self.set_ia(prob.ia_mode, prob.ia_A, prob.ia_eta)
         return self
+
+    def set_kernel_precision(self, extended: bool):
+        self.lib.oc_set_kernel_precision(self.c, 1 if extended else 0)
 
     def touch(self):
         self.lib.oc_touch(self.c)

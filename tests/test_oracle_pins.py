@@ -383,3 +383,31 @@ def test_datavector_block_order(p3, o3):
     xp, xm = o3.xi_pm()
     assert np.array_equal(dv[: nt * nss], xp.ravel())
     assert np.array_equal(dv[nt * nss: 2 * nt * nss], xm.ravel())
+
+
+def test_small_theta_kernel_precision():
+    """At theta = 2.5' the double evaluation of G- loses digits (the formula
+    divides differences of O(l^2) terms by x_min - x_max ~ 1.5e-7); the
+    long-double evaluation matches mpmath bin averages to 1e-9."""
+    o = O.Oracle()
+    o.set_ntable(300, 225, 3000, 0)
+    o.set_binning(20, 2.5, 250.0)
+    _, Gm_ld, _, _ = o.kernels()
+    o.set_kernel_precision(False)
+    _, Gm_d, _, _ = o.kernels()
+    mp.mp.dps = 40
+    logdt = (math.log(250.0) - math.log(2.5)) / 20
+    t0 = 2.5 * 2.90888208665721580e-4
+    t1 = math.exp(math.log(2.5) + logdt) * 2.90888208665721580e-4
+    xlo, xhi = mp.mpf(math.cos(t1)), mp.mpf(math.cos(t0))
+    scale = np.abs(Gm_ld[0]).max()
+    worst_ld = worst_d = 0.0
+    for l in (40, 300, 1000, 2500):
+        f = (l - 1) * (l + 2) / (l * (l + 1))
+        ref = f * _binavg(lambda x: (2 * l + 1) / (4 * mp.pi) * ((1 - x) / 2) ** 2 * mp.jacobi(l - 2, 4, 0, x), xlo, xhi)
+        ref = float(ref)
+        worst_ld = max(worst_ld, abs(Gm_ld[0, l] - ref) / scale)
+        worst_d = max(worst_d, abs(Gm_d[0, l] - ref) / scale)
+    print("G- abs err / max|G-| at 2.5': long double", worst_ld, "double", worst_d)
+    assert worst_ld < 1e-9
+    assert worst_d > 100 * worst_ld  # the reason the kernels are built in long double
