@@ -237,7 +237,7 @@ static int fail(clh_ctx* c, int code, const char* fmt, ...) {
 
 // first GPU use of a context: device check, stream and timing events
 static int ensure_device(clh_ctx* c) {
-  if (c->stream) return CLH_OK;
+  if (c->own_stream) return CLH_OK;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(c, CLH_ERR_HIP, "no HIP device");
   if (c->device >= n) return fail(c, CLH_ERR_ARG, "device %d of %d", c->device, n);
@@ -769,7 +769,7 @@ int clh_create(clh_ctx** out, int dev) {
 
 void clh_destroy(clh_ctx* c) {
   if (!c) return;
-  if (c->stream) {
+  if (c->own_stream) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
   }
@@ -1106,16 +1106,14 @@ int clh_set_stream(clh_ctx* c, void* stream) {
   if (!c) return CLH_ERR_ARG;
   int rc = ensure_device(c);
   if (rc) return rc;
-  if (c->own_stream) {
-    HIPCHK(c, hipStreamSynchronize(c->own_stream));
-  }
-  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  if (c->stream != (hipStream_t)stream) HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->stream = (hipStream_t)stream;  // NULL: the HIP default (legacy) stream
   return CLH_OK;
 }
 
 int clh_synchronize(clh_ctx* c) {
   if (!c) return CLH_ERR_ARG;
-  if (!c->stream) return CLH_OK;
+  if (!c->own_stream) return CLH_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->timing)
     for (int k = 0; k < 4; k++) HIPCHK(c, hipEventElapsedTime(&c->last_ms[k], c->ev[k], c->ev[k + 1]));

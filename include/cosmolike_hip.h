@@ -127,8 +127,9 @@ int clh_synchronize(clh_ctx* ctx);
 /* on != 0: clh_batch_forward captures its launches into a hipGraph on first
  * use and replays it afterwards (re-captured after clh_batch_upload) */
 int clh_batch_use_graph(clh_ctx* ctx, int on);
-/* launch every subsequent call on `stream` (a hipStream_t; NULL = the
- * context's own stream), e.g. the caller's current stream */
+/* launch every subsequent call on `stream` (a hipStream_t, e.g. the caller's
+ * current stream; NULL = the HIP default stream) instead of the context's own
+ * non-blocking stream */
 int clh_set_stream(clh_ctx* ctx, void* stream);
 
 /* Device-side update of the uploaded batch's differentiable inputs (device
