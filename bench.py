@@ -43,10 +43,10 @@ def algorithmic_work(p):
     """Per-draw algorithmic work of each kernel (FLOP), as defined in DESIGN.md."""
     nss, ngs, ngg = p.ns * (p.ns + 1) // 2, p.ns * p.nl, p.nl
     rows = p.n_ell + 19
-    pair_node = rows * (64 * nss + 64 * ngs + 96 * ngg)
-    p_lookups = rows * (64 + 64 * p.nl + 96 * p.nl)
     return {
-        "k_limber": 20.0 * pair_node + 40.0 * p_lookups,
+        # 20 FLOP per pair-node integrand term + 40 FLOP per P(k,z) lookup
+        "k_limber_sx": 20.0 * rows * 64 * (nss + ngs) + 40.0 * rows * 64 * (1 + p.nl),
+        "k_limber_gg": 20.0 * rows * 96 * ngg + 40.0 * rows * 96 * p.nl,
         "k_legendre": 2.0 * p.ntheta * rows * (2 * nss + ngs + ngg),
         "k_legendre_full": 2.0 * p.ntheta * (p.lmax - 1) * (2 * nss + ngs + ngg),
     }
@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--vjp", type=int, default=1, help="also time the batched VJP")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the draws of a batch "
@@ -125,6 +126,27 @@ def main():
         for k, v in c.kernel_times().items():
             acc[k] = acc.get(k, 0.0) + v / nt
     c.enable_timing(False)
+
+    # VJP throughput: one VJP of every draw's data vector (includes its forward recompute)
+    vjp_value = None
+    if args.vjp:
+        B = args.batch
+        ns, nl = draws[0].ns, draws[0].nl
+        nknz = draws[0].lnP.size
+        dvbar = torch.randn(B * n, dtype=torch.float64, device="cuda:0")
+        g = {k: torch.empty(B * m, dtype=torch.float64, device="cuda:0")
+             for k, m in (("lnP", nknz), ("src", ns), ("lens", nl), ("b1", nl), ("A", 1), ("m", ns))}
+        ptrs = [g[k].data_ptr() for k in ("lnP", "src", "lens", "b1", "A", "m")]
+        for _ in range(2):
+            c.batch_vjp(dvbar.data_ptr(), *ptrs)
+        c.synchronize()
+        t1 = time.perf_counter()
+        nv = max(3, args.steps // 2)
+        for _ in range(nv):
+            c.batch_vjp(dvbar.data_ptr(), *ptrs)
+        c.synchronize()
+        vjp_value = B * nv / (time.perf_counter() - t1)
+        log(f"bench: VJP {vjp_value:.1f} per s")
     work = algorithmic_work(draws[0])
     if args.mode == "full":
         acc["k_legendre_full"] = acc.pop("k_legendre")
@@ -154,6 +176,7 @@ def main():
                                "+ 5 gg pairs, 26 theta 2.5'-900', N_ell=500, LMAX=50000), batch of "
                                f"{args.batch} draws with their own cosmology",
                    "batch": args.batch, "legendre": args.mode, "datavector_len": n},
+        "vjp_value": vjp_value, "vjp_unit": "VJPs/s (full data vector -> lnP, dz_src, dz_lens, b1, A_IA, m)",
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

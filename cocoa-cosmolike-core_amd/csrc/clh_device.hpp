@@ -16,6 +16,14 @@ __device__ __forceinline__ int clh_bs(const double* __restrict__ x, int n, doubl
   return ilo;
 }
 
+// The same index as clh_bs for a strictly increasing table, found by walking
+// from a previous index j (cheap when successive queries move monotonically).
+__device__ __forceinline__ int clh_walk(const double* __restrict__ x, int n, double v, int j) {
+  while (j > 0 && x[j] > v) j--;
+  while (j < n - 2 && x[j + 1] <= v) j++;
+  return j;
+}
+
 // chi_all: cosmo3D.c:40-92 (chi and dchi/da, in c/H0 units).  The table is
 // validated on the host to extend two nodes beyond every queried z.
 __device__ __forceinline__ void clh_chi_all(const DrawDesc& d, double a, double& chi,

@@ -18,6 +18,8 @@
 hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_limber(const DrawDesc* D, const int4* work, int n, hipStream_t s);
+hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, hipStream_t s);
+hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow,
                                hipStream_t s, int raw);
 hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s);
@@ -178,8 +180,8 @@ struct Batch {
   DBuf<DrawParams> d_par;
   DBuf<double> d_tab, d_work, d_dv, d_ells, d_vjp, d_gscr;
   DBuf<int2> d_wg, d_wn, d_wl2, d_wa;
-  DBuf<int4> d_wl;
-  int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0;
+  DBuf<int4> d_wl, d_wsx, d_wgg;
+  int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0, n_wsx = 0, n_wgg = 0;
   int dv_size = 0;
   int nknz = 0;        // common ln P table size (0 if the draws differ)
   int ord[3] = {0, 1, 2};
@@ -187,6 +189,7 @@ struct Batch {
     d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
     d_vjp.release(); d_gscr.release();
     d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa.release();
+    d_wsx.release(); d_wgg.release();
   }
 };
 
@@ -214,7 +217,7 @@ struct clh_ctx {
   hipGraphExec_t gexec = nullptr;     // captured forward of `batch`
   hipStream_t gstream = nullptr;      // stream the graph was captured on
   hipEvent_t ev[8] = {nullptr};
-  float last_ms[4] = {0, 0, 0, 0};
+  float last_ms[5] = {0, 0, 0, 0, 0};
 };
 
 static int fail(clh_ctx* c, int code, const char* fmt, ...) {
@@ -692,6 +695,23 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   const int ntask = 2 * n_ss + n_gs + nl;
   for (int i = 0; i < B; i++)
     for (int t = 0; t < ntask; t += 8) wl2.push_back(make_int2(i, t));
+  // forward Limber tiles: gg (16 rows, heaviest, first) and ss/gs (16 rows)
+  std::vector<int4> wsx, wgg;
+  for (int i = 0; i < B; i++)
+    for (int zl = 0; zl < nl; zl++)
+      for (int r = 0; r < nrow; r += 16) wgg.push_back(make_int4(i, zl, r, 0));
+  for (int i = 0; i < B; i++) {
+    for (int zl = 0; zl < nl; zl++)
+      for (int r = 0; r < nrow; r += 16) wsx.push_back(make_int4(i, CLH_W_GS, zl, r));
+    for (int r = 0; r < nrow; r += 16) wsx.push_back(make_int4(i, CLH_W_SS, 0, r));
+  }
+  HIPCHK(c, bt.d_wsx.ensure(wsx.size()));
+  HIPCHK(c, bt.d_wgg.ensure(std::max<size_t>(wgg.size(), 1)));
+  HIPCHK(c, hipMemcpyAsync(bt.d_wsx.p, wsx.data(), wsx.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  if (!wgg.empty())
+    HIPCHK(c, hipMemcpyAsync(bt.d_wgg.p, wgg.data(), wgg.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  bt.n_wsx = (int)wsx.size();
+  bt.n_wgg = (int)wgg.size();
   std::vector<int2> wa;  // Legendre adjoint: 8 C_ell columns per block
   for (int i = 0; i < B; i++)
     for (int col = 0; col < ncol; col += 8) wa.push_back(make_int2(i, col));
@@ -721,15 +741,17 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], s));
   HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], s));
-  HIPCHK(c, clh_launch_limber(bt.d_desc.p, bt.d_wl.p, bt.n_wl, s));
+  HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], s));
+  HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
   if (legendre) {
     if (c->legendre_mode == CLH_LEGENDRE_FULL)
       HIPCHK(c, clh_launch_legendre_full(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s));
     else
       HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s, 0));
   }
-  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[5], s));
   return CLH_OK;
 }
 
@@ -1116,7 +1138,7 @@ int clh_synchronize(clh_ctx* c) {
   if (!c->own_stream) return CLH_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->timing)
-    for (int k = 0; k < 4; k++) HIPCHK(c, hipEventElapsedTime(&c->last_ms[k], c->ev[k], c->ev[k + 1]));
+    for (int k = 0; k < 5; k++) HIPCHK(c, hipEventElapsedTime(&c->last_ms[k], c->ev[k], c->ev[k + 1]));
   return CLH_OK;
 }
 
@@ -1134,13 +1156,13 @@ int clh_enable_timing(clh_ctx* c, int on) {
 }
 
 int clh_last_kernel_times(clh_ctx* c, const char** names, float* ms, int cap) {
-  static const char* nm[4] = {"k_gtab", "k_nodes", "k_limber", "k_legendre"};
+  static const char* nm[5] = {"k_gtab", "k_nodes", "k_limber_gg", "k_limber_sx", "k_legendre"};
   if (!c) return CLH_ERR_ARG;
-  for (int k = 0; k < 4 && k < cap; k++) {
+  for (int k = 0; k < 5 && k < cap; k++) {
     if (names) names[k] = nm[k];
     if (ms) ms[k] = c->last_ms[k];
   }
-  return 4;
+  return 5;
 }
 
 }  // extern "C"
@@ -1226,7 +1248,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   HIPCHK(c, clh_launch_gtab(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
   HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
   HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
-  HIPCHK(c, clh_launch_limber(bt.d_desc.p, bt.d_wl.p, bt.n_wl, s));
+  HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, s));
+  HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
   HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s, 1));
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa.p, bt.n_wa, s));
