@@ -2,11 +2,15 @@
 
 Workload: the full real-space 3x2pt layout (10 Smail source bins z0=0.6, 5
 Gaussian lens bins, 55 ss + 50 gs + 5 gg pairs, 26 theta bins 2.5'-900',
-N_ell = 500, LMAX = 50000, NLA IA, linear bias 1.2-2.0), plain FP64, over a
-batch of draws that each carry their own cosmology (Omega_m, sigma8, h drawn
+N_ell = 500, LMAX = 50000, NLA IA, linear bias 1.2-2.0), over a batch of
+draws that each carry their own cosmology (Omega_m, sigma8, h drawn
 uniformly) and hence their own chi(z), G(z) and 400x100 P(k,z) tables.  One
 step = the whole path (lens efficiencies, Limber C_ell, Legendre transform,
 data-vector assembly) for every draw of the batch, inputs resident in HBM.
+--precision picks the plain FP64 path or a mixed-precision one (FP32 Limber
+with an FP32 or 3-term split-BF16 MFMA Legendre contraction); for the mixed
+ones the per-probe error against the plain FP64 GPU path on the same batch is
+reported next to the throughput.
 
 Prints one JSON line (see README of the driver contract).
 """
@@ -24,6 +28,8 @@ sys.path.insert(0, os.path.join(ROOT, "cocoa-cosmolike-core_amd"))
 import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix rates coincide)
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector and matrix)
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 HBM_PEAK_GBS = 8000.0
 
 
@@ -52,6 +58,33 @@ def algorithmic_work(p):
     }
 
 
+def kernel_work_and_peak(name, work, precision):
+    """(algorithmic FLOP per draw, peak TFLOP/s) of a kernel by its name."""
+    base = name[:-2] if name.endswith("32") else name
+    if name == "k_legendre_mx":
+        base = "k_legendre"
+    flop = work.get(base, 0.0)
+    if name.endswith("32"):
+        return flop, FP32_PEAK_TFLOPS
+    if name == "k_legendre_mx":
+        return flop, (BF16_PEAK_TFLOPS if precision == "bf16x3" else FP32_PEAK_TFLOPS)
+    return flop, FP64_PEAK_TFLOPS
+
+
+def probe_errors(dv, ref, ns, nl, nt):
+    """Max and RMS of |dv - ref| / max(|ref|, 1e-3 max|ref| of the row) per probe."""
+    n_ss, n_gs = ns * (ns + 1) // 2, ns * nl
+    out, o = {}, 0
+    for name, rows in (("xi+", n_ss), ("xi-", n_ss), ("gammat", n_gs), ("w", nl)):
+        g = dv[:, o:o + rows * nt].reshape(dv.shape[0], rows, nt)
+        r = ref[:, o:o + rows * nt].reshape(dv.shape[0], rows, nt)
+        floor = 1e-3 * np.abs(r).max(axis=2, keepdims=True)
+        e = np.abs(g - r) / np.maximum(np.abs(r), floor)
+        out[name] = {"max": float(e.max()), "rms": float(np.sqrt((e ** 2).mean()))}
+        o += rows * nt
+    return out
+
+
 def cpu_baseline(draws, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the FP64 CPU restatement, timed as the CPU reference path
@@ -78,8 +111,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--mode", choices=["folded", "full"], default="folded")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16x3"], default="fp64")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -102,6 +136,20 @@ def main():
     c.batch_upload(draws)
     n = c.datavector_size()
     dv = torch.empty(args.batch * n, dtype=torch.float64, device="cuda:0")
+    accuracy = None
+    if args.precision != "fp64":
+        # error of the mixed path against the plain FP64 GPU path on this batch
+        ref = torch.empty_like(dv)
+        c.batch_forward(ref.data_ptr())
+        c.set_precision(args.precision)
+        c.batch_forward(dv.data_ptr())
+        c.synchronize()
+        torch.cuda.synchronize()
+        p0 = draws[0]
+        accuracy = probe_errors(dv.view(args.batch, n).cpu().numpy(), ref.view(args.batch, n).cpu().numpy(),
+                                p0.ns, p0.nl, p0.ntheta)
+        log(f"bench: {args.precision} vs FP64 path {accuracy}")
+        del ref
     for _ in range(args.warmup):
         c.batch_forward(dv.data_ptr())
     c.synchronize()
@@ -148,13 +196,12 @@ def main():
         vjp_value = B * nv / (time.perf_counter() - t1)
         log(f"bench: VJP {vjp_value:.1f} per s")
     work = algorithmic_work(draws[0])
-    if args.mode == "full":
-        acc["k_legendre_full"] = acc.pop("k_legendre")
     dom = max(acc, key=acc.get)
-    flop = work.get(dom, 0.0) * args.batch
+    flop1, peak = kernel_work_and_peak(dom, work, args.precision)
+    flop = flop1 * args.batch
     achieved = flop / (acc[dom] * 1e-3) / 1e12 if acc[dom] > 0 else 0.0
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": dom,
+    roofline = {"bound": "mfma" if "legendre" in dom else "valu", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None, "kernel": dom,
                 "kernel_ms": acc[dom], "kernel_ms_all": acc}
     log(f"bench: kernel ms {acc}")
 
@@ -171,11 +218,15 @@ def main():
         "metric": "3x2pt data vectors/s + VJPs/s at <=1e-4 rel err vs FP64 plain path, 1 GPU",
         "value": value, "unit": "data vectors/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "full 3x2pt real space, plain FP64 (10 src + 5 lens bins, 55 ss + 50 gs "
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "f32+bf16x3"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": f"full 3x2pt real space, {args.precision} (10 src + 5 lens bins, 55 ss + 50 gs "
                                "+ 5 gg pairs, 26 theta 2.5'-900', N_ell=500, LMAX=50000), batch of "
                                f"{args.batch} draws with their own cosmology",
-                   "batch": args.batch, "legendre": args.mode, "datavector_len": n},
+                   "batch": args.batch, "legendre": args.mode, "precision": args.precision,
+                   "datavector_len": n},
+        "accuracy_vs_fp64_path": accuracy,
         "vjp_value": vjp_value, "vjp_unit": "VJPs/s (full data vector -> lnP, dz_src, dz_lens, b1, A_IA, m)",
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -57,6 +57,7 @@ def lib():
             "clh_last_error": (ctypes.c_char_p, [VP]),
             "clh_default": (VP, []),
             "clh_set_legendre_mode": (i32, [VP, i32]),
+            "clh_set_precision": (i32, [VP, i32]),
             "clh_init_ntable": (i32, [VP, i32, i32, i32, i32]),
             "clh_init_binning_real_space": (i32, [VP, i32, f64, f64]),
             "clh_init_ia": (i32, [VP, i32, i32]),

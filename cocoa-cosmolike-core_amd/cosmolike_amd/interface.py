@@ -31,6 +31,7 @@ def _p(x):
 
 LEGENDRE_FOLDED = 0
 LEGENDRE_FULL = 1
+PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 
 class Cosmolike:
@@ -85,6 +86,12 @@ class Cosmolike:
 
     def set_legendre_mode(self, mode):
         self._chk(self.L.clh_set_legendre_mode(self.h, mode))
+
+    def set_precision(self, mode):
+        """``"fp64"`` (plain path), ``"fp32"`` or ``"bf16x3"``, or a CLH_PRECISION_* value."""
+        if isinstance(mode, str):
+            mode = PRECISION[mode]
+        self._chk(self.L.clh_set_precision(self.h, int(mode)))
 
     def set_cosmological_parameters(self, omega_matter, hubble=70.0):
         self._chk(self.L.clh_set_cosmological_parameters(self.h, float(omega_matter), float(hubble)))

@@ -31,6 +31,9 @@ hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int2* work, int n, hi
 hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
 hipError_t clh_launch_limber_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_limber_sx32(const DrawDesc* D, const int4* work, int n, hipStream_t s);
+hipError_t clh_launch_limber_gg32(const DrawDesc* D, const int4* work, int n, hipStream_t s);
+hipError_t clh_launch_legendre_mx(const DrawDesc* D, const int4* work, int n, int mode, hipStream_t s);
 hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, int max_nrow,
                                     hipStream_t s);
 
@@ -180,8 +183,8 @@ struct Batch {
   DBuf<DrawParams> d_par;
   DBuf<double> d_tab, d_work, d_dv, d_ells, d_vjp, d_gscr;
   DBuf<int2> d_wg, d_wn, d_wl2, d_wa;
-  DBuf<int4> d_wl, d_wsx, d_wgg;
-  int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0, n_wsx = 0, n_wgg = 0;
+  DBuf<int4> d_wl, d_wsx, d_wgg, d_wmx;
+  int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0, n_wsx = 0, n_wgg = 0, n_wmx = 0;
   int dv_size = 0;
   int nknz = 0;        // common ln P table size (0 if the draws differ)
   int ord[3] = {0, 1, 2};
@@ -189,7 +192,7 @@ struct Batch {
     d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
     d_vjp.release(); d_gscr.release();
     d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa.release();
-    d_wsx.release(); d_wgg.release();
+    d_wsx.release(); d_wgg.release(); d_wmx.release();
   }
 };
 
@@ -203,12 +206,16 @@ struct clh_ctx {
   Layout L;
   Draw cur;
   int legendre_mode = CLH_LEGENDRE_FOLDED;
+  int precision = CLH_PREC_FP64;
   uint64_t version = 1, cached_version = 0;
   dvec cache_raw;   // raw (uncalibrated, ord 0,1,2) data vector of `cur`
   dvec cache_cl;    // cl rows of `cur`
   // device copies of the layout
   bool dev_layout_valid = false;
   DBuf<double> d_layout;
+  DBuf<float> d_kmx32, d_kmxsc;   // folded kernels packed for the MFMA contraction
+  DBuf<uint16_t> d_kmxbf;
+  int kmx_ngrp = 0;
   size_t off_s_zc = 0, off_s_coef = 0, off_l_zc = 0, off_l_coef = 0, off_ell_ss = 0,
          off_ell_gx = 0, off_kfold = 0, off_gl = 0, off_kfull = 0, off_fill = 0;
   Batch single, batch, scratch;
@@ -491,6 +498,19 @@ static int check_draw(clh_ctx* c, const Draw& d, int idx) {
 }
 
 // ----------------------------------------------------------- device layout
+// bfloat16 of a float, rounded to nearest even (finite inputs)
+static uint16_t bf16_rne(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+static double bf16_value(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
 static int upload_layout(clh_ctx* c) {
   Layout& L = c->L;
   const bool full = c->legendre_mode == CLH_LEGENDRE_FULL;
@@ -545,6 +565,44 @@ static int upload_layout(clh_ctx* c) {
   HIPCHK(c, put(c->off_gl, glbuf, sizeof glbuf / sizeof(double)));
   if (full) HIPCHK(c, put(c->off_kfull, L.kfull.data(), L.kfull.size()));
   if (full) HIPCHK(c, put(c->off_fill, fill.data(), fill.size()));
+  // Folded kernels as MFMA A operands (k_legendre_mx): group g of 8 rows,
+  // lane l = theta + 32 h, item t <-> row 8 g + 4 h + t.  The BF16 parts are
+  // of the kernel scaled by 2^-e per 32-row band (e from the band's largest
+  // |K|); the contraction scales that band's C_ell rows by 2^e.
+  const int nrow = L.nrow(), nt = L.ntheta, ngrp = (nrow + 7) / 8;
+  std::vector<float> k32((size_t)4 * ngrp * 64 * 4, 0.f), ksc((size_t)4 * ngrp, 1.f);
+  std::vector<uint16_t> kbf((size_t)4 * ngrp * 64 * 8, 0);
+  for (int k = 0; k < 4; k++) {
+    const double* K = L.kfold.data() + (size_t)k * nrow * nt;
+    for (int g0 = 0; g0 < ngrp; g0 += 4) {
+      double mx = 0.0;
+      for (int r = 8 * g0; r < std::min(nrow, 8 * (g0 + 4)); r++)
+        for (int t = 0; t < nt; t++) mx = std::max(mx, std::fabs(K[(size_t)r * nt + t]));
+      int e = 0;
+      if (mx > 0.0) (void)std::frexp(mx, &e);
+      for (int g = g0; g < std::min(ngrp, g0 + 4); g++) {
+        ksc[(size_t)k * ngrp + g] = std::ldexp(1.0f, e);
+        for (int l = 0; l < 64; l++)
+          for (int t = 0; t < 4; t++) {
+            const int th = l & 31, r = 8 * g + 4 * (l >> 5) + t;
+            const double v = (th < nt && r < nrow) ? K[(size_t)r * nt + th] : 0.0;
+            const size_t o = ((size_t)k * ngrp + g) * 64 + l;
+            k32[o * 4 + t] = (float)v;
+            const double vs = std::ldexp(v, -e);
+            const uint16_t hi = bf16_rne((float)vs);
+            kbf[o * 8 + t] = hi;
+            kbf[o * 8 + 4 + t] = bf16_rne((float)(vs - bf16_value(hi)));
+          }
+      }
+    }
+  }
+  HIPCHK(c, c->d_kmx32.ensure(k32.size()));
+  HIPCHK(c, c->d_kmxbf.ensure(kbf.size()));
+  HIPCHK(c, c->d_kmxsc.ensure(ksc.size()));
+  HIPCHK(c, hipMemcpyAsync(c->d_kmx32.p, k32.data(), k32.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_kmxbf.p, kbf.data(), kbf.size() * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_kmxsc.p, ksc.data(), ksc.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  c->kmx_ngrp = ngrp;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->dev_layout_valid = true;
   return CLH_OK;
@@ -651,6 +709,8 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.kfull = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + c->off_kfull : nullptr;
     q.fill = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + c->off_fill : nullptr;
     q.gl = lay + c->off_gl;
+    q.kmx32 = c->d_kmx32.p; q.kmxbf = c->d_kmxbf.p; q.kmxsc = c->d_kmxsc.p;
+    q.kmx_ngrp = c->kmx_ngrp;
     double* wb = bt.d_work.p + work_per * i;
     q.gtab = wb;
     q.nodes = wb + ((gtab_sz + 31) & ~size_t(31));
@@ -705,6 +765,16 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
       for (int r = 0; r < nrow; r += 16) wsx.push_back(make_int4(i, CLH_W_GS, zl, r));
     for (int r = 0; r < nrow; r += 16) wsx.push_back(make_int4(i, CLH_W_SS, 0, r));
   }
+  std::vector<int4> wmx;  // MFMA Legendre tiles: 32 columns of one kind
+  {
+    const int nck[4] = {n_ss, n_ss, n_gs, nl};
+    for (int i = 0; i < B; i++)
+      for (int k = 0; k < 4; k++)
+        for (int c0 = 0; c0 < nck[k]; c0 += 32) wmx.push_back(make_int4(i, k, c0, 0));
+  }
+  HIPCHK(c, bt.d_wmx.ensure(wmx.size()));
+  HIPCHK(c, hipMemcpyAsync(bt.d_wmx.p, wmx.data(), wmx.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  bt.n_wmx = (int)wmx.size();
   HIPCHK(c, bt.d_wsx.ensure(wsx.size()));
   HIPCHK(c, bt.d_wgg.ensure(std::max<size_t>(wgg.size(), 1)));
   HIPCHK(c, hipMemcpyAsync(bt.d_wsx.p, wsx.data(), wsx.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
@@ -741,13 +811,22 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], s));
   HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], s));
-  HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, s));
+  const bool mixed = c->precision != CLH_PREC_FP64;
+  if (mixed)
+    HIPCHK(c, clh_launch_limber_gg32(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, s));
+  else
+    HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], s));
-  HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
+  if (mixed)
+    HIPCHK(c, clh_launch_limber_sx32(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
+  else
+    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
   if (legendre) {
     if (c->legendre_mode == CLH_LEGENDRE_FULL)
       HIPCHK(c, clh_launch_legendre_full(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s));
+    else if (mixed)
+      HIPCHK(c, clh_launch_legendre_mx(bt.d_desc.p, bt.d_wmx.p, bt.n_wmx, c->precision, s));
     else
       HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s, 0));
   }
@@ -798,6 +877,7 @@ void clh_destroy(clh_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   c->single.release(); c->batch.release(); c->scratch.release();
   c->d_layout.release();
+  c->d_kmx32.release(); c->d_kmxsc.release(); c->d_kmxbf.release();
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
@@ -812,6 +892,18 @@ int clh_set_legendre_mode(clh_ctx* c, int mode) {
     c->legendre_mode = mode;
     c->dev_layout_valid = false;
     c->version++;
+  }
+  return CLH_OK;
+}
+
+int clh_set_precision(clh_ctx* c, int mode) {
+  if (!c) return CLH_ERR_ARG;
+  if (mode != CLH_PREC_FP64 && mode != CLH_PREC_FP32 && mode != CLH_PREC_BF16X3)
+    return fail(c, CLH_ERR_ARG, "bad precision mode %d", mode);
+  if (mode != c->precision) {
+    c->precision = mode;
+    c->version++;
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   }
   return CLH_OK;
 }
@@ -1157,9 +1249,15 @@ int clh_enable_timing(clh_ctx* c, int on) {
 
 int clh_last_kernel_times(clh_ctx* c, const char** names, float* ms, int cap) {
   static const char* nm[5] = {"k_gtab", "k_nodes", "k_limber_gg", "k_limber_sx", "k_legendre"};
+  static const char* nm_full = "k_legendre_full";
+  static const char* nm32[3] = {"k_limber_gg32", "k_limber_sx32", "k_legendre_mx"};
   if (!c) return CLH_ERR_ARG;
+  const bool mixed = c->precision != CLH_PREC_FP64;
   for (int k = 0; k < 5 && k < cap; k++) {
-    if (names) names[k] = nm[k];
+    if (names) {
+      names[k] = (mixed && k >= 2) ? nm32[k - 2] : nm[k];
+      if (k == 4 && c->legendre_mode == CLH_LEGENDRE_FULL) names[k] = nm_full;
+    }
     if (ms) ms[k] = c->last_ms[k];
   }
   return 5;

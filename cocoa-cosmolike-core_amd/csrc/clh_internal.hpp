@@ -15,6 +15,11 @@
 #define CLH_W_GS 1
 #define CLH_W_GG 2
 
+// Precision modes (CLH_PRECISION_* of the C ABI)
+#define CLH_PREC_FP64 0
+#define CLH_PREC_FP32 1
+#define CLH_PREC_BF16X3 2
+
 // Rows of a Limber table column: CLH_NLOW direct rows (ell = r+1), then the
 // N_ell log-spaced rows.  Rows per ss/gs work tile and per gg tile:
 #define CLH_SS_ROWS 4
@@ -52,6 +57,12 @@ struct DrawDesc {
   const double* fill;     // [2][lmax][2] (index, weight) of the ln-ell fill, ss then gs/gg
   double lim_ss[3], lim_gx[3];
   int lmax;
+  // folded kernels packed as MFMA A operands, rows in groups of 8
+  // (group g, lane l = theta + 32 h, item t <-> row 8 g + 4 h + t):
+  const float* kmx32;        // [4][ngrp][64][4]   FP32
+  const uint16_t* kmxbf;     // [4][ngrp][64][8]   BF16 hi[4], lo[4] of the band-scaled kernel
+  const float* kmxsc;        // [4][ngrp] power-of-two factor applied to C_ell of the band
+  int kmx_ngrp;
 
   // nuisance values of the draw (nuisanceparams, cosmolike/structs.h:203-298)
   const struct DrawParams* par;

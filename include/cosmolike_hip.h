@@ -39,6 +39,18 @@ extern "C" {
 #define CLH_LEGENDRE_FOLDED 0
 #define CLH_LEGENDRE_FULL 1
 
+/* Arithmetic of the Limber and folded-Legendre stages.  FP64 is the plain
+ * path (the reference's arithmetic, default).  FP32: Limber integrands and
+ * Kahan-compensated node sums in FP32 (table searches, chi(z)/growth tables,
+ * lens efficiencies and Legendre kernels stay FP64-built), Legendre
+ * contraction on FP32 MFMA.  BF16X3: as FP32, with the Legendre contraction
+ * on BF16 MFMA using a 3-term hi/lo split of both operands (hi*hi + hi*lo +
+ * lo*hi), FP32 accumulation and power-of-two scaling of 32-row ell bands.
+ * FULL Legendre mode always contracts in FP64. */
+#define CLH_PRECISION_FP64 0
+#define CLH_PRECISION_FP32 1
+#define CLH_PRECISION_BF16X3 2
+
 typedef struct clh_ctx clh_ctx;
 
 /* ---------------------------------------------------------------- context */
@@ -48,6 +60,8 @@ const char* clh_last_error(const clh_ctx* ctx);
 /* process-wide context behind the reference-named scalar entry points */
 clh_ctx* clh_default(void);
 int clh_set_legendre_mode(clh_ctx* ctx, int mode);
+/* one of CLH_PRECISION_*; no reference counterpart (the reference is FP64 only) */
+int clh_set_precision(clh_ctx* ctx, int mode);
 
 /* ---------------------------------------------------------------- setters */
 /* init_accuracy_boost / init_ntable_lmax (generic_interface.cpp:284-342):

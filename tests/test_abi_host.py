@@ -76,6 +76,11 @@ def test_setters_validate(ctx):
     with pytest.raises(CosmolikeError) as e:
         ctx.set_source_sample(tab)
     assert e.value.code == -4
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_precision(3)                          # no such precision mode
+    assert e.value.code == -1
+    for m in ("fp32", "bf16x3", "fp64"):
+        ctx.set_precision(m)
 
 
 def test_datavector_sizes(ctx):
