@@ -209,6 +209,23 @@ def main():
     out = dv.view(args.batch, n)
     assert torch.isfinite(out).all().item()
 
+    # the plain FP64 drop-in path: one draw at a time through the setters and
+    # clh_compute_3x2pt_real (host buffers in and out), as the reference's
+    # interface calls it
+    c.set_precision("fp64")
+    cp = Cosmolike(device=0).load(draws[0], mode)
+    cp.compute_3x2pt_real()
+    npl, t1 = 0, time.perf_counter()
+    while True:
+        cp.set_draw(draws[npl % len(draws)])
+        cp.compute_3x2pt_real()
+        npl += 1
+        if time.perf_counter() - t1 > 2.0 and npl >= 10:
+            break
+    plain = {"value": npl / (time.perf_counter() - t1), "unit": "data vectors/s",
+             "path": "per draw: cosmology/nuisance setters + clh_compute_3x2pt_real, FP64, host buffers"}
+    log(f"bench: plain FP64 single-draw path {plain['value']:.1f} dv/s")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(draws, args.cpu_seconds)
@@ -228,6 +245,11 @@ def main():
                    "datavector_len": n},
         "accuracy_vs_fp64_path": accuracy,
         "vjp_value": vjp_value, "vjp_unit": "VJPs/s (full data vector -> lnP, dz_src, dz_lens, b1, A_IA, m)",
+        "fwd_plus_vjp_value": (1.0 / (1.0 / value + 1.0 / vjp_value)) if vjp_value else None,
+        "plain_fp64_single_draw": plain,
+        "speedup_vs_plain_path": {"forward": value / plain["value"],
+                                  "forward_plus_vjp": ((1.0 / (1.0 / value + 1.0 / vjp_value)) / plain["value"])
+                                  if vjp_value else None},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

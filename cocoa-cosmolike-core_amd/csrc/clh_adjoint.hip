@@ -10,8 +10,8 @@
 //
 //  k_set_params   device-side update of per-draw parameters (autograd inputs)
 //  k_leg_adjoint  C_ell-row cotangents from the data-vector cotangent
-//  k_m_grad       d/dm from the uncalibrated data vector
-//  k_limber_adj   ln P scatter + cotangents of the node weights
+//  k_m_grad       d/dm from cotangent-weighted uncalibrated column sums
+//  k_limber_sx_adj, k_limber_gg_adj   ln P cotangent + cotangents of the node weights
 //  k_nodes_adj    node weights -> photo-z shifts (n(z) and g tangents), A_IA
 #include <hip/hip_runtime.h>
 
@@ -54,13 +54,17 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_set_params(const DrawDesc* __rest
 }
 
 // ----------------------------------------------------------------------------
-// k_leg_adjoint: work item (draw, first column); 8 columns per block.
-// cbar[col][r] = sum_tasks sum_theta Kfold[kind][r][theta] * dvbar * calibration
-// (the FULL-mode forward has the same derivative with respect to the rows).
+// k_leg_adjoint: work item (draw, first column); 8 C_ell columns per block.
+//   u[col][r] = sum_tasks sum_theta Kfold[kind][r][theta] * dvbar
+//   cbar[col][r] = calibration(col) * u[col][r]
+//   T[col] = sum_r u[col][r] C[col][r]  (= dvbar . uncalibrated data vector)
+// T goes to the draw's `raw` buffer for the shear-calibration gradient.  (The
+// FULL-mode forward has the same derivative with respect to the rows.)
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(CLH_BLOCK) void k_leg_adjoint(const DrawDesc* __restrict__ D,
                                                            const int2* __restrict__ work) {
   __shared__ double vb[8][2][32];
+  __shared__ double cf[8], tq[8];
   const int2 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
   const int nt = d.ntheta, nrow = d.nrow, nss = d.n_ss, ngs = d.n_gs;
@@ -70,20 +74,24 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_leg_adjoint(const DrawDesc* __res
     const int col = w.y + c;
     double v = 0.0;
     if (col < ncol && th < nt) {
-      if (col < nss) {
-        int a, b;
-        clh_ss_pair(d.ns, col, a, b);
-        const double f = d.apply_calib ? (1.0 + d.par->shear_m[a]) * (1.0 + d.par->shear_m[b]) : 1.0;
-        v = d.dvbar[d.dv_start[0] + nt * col + th + (s ? nt * nss : 0)] * f;
-      } else if (s == 0 && col < nss + ngs) {
-        const int q = col - nss;
-        const double f = d.apply_calib ? 1.0 + d.par->shear_m[q % d.ns] : 1.0;
-        v = d.dvbar[d.dv_start[1] + nt * q + th] * f;
-      } else if (s == 0) {
-        v = d.dvbar[d.dv_start[2] + nt * (col - nss - ngs) + th];
-      }
+      if (col < nss) v = d.dvbar[d.dv_start[0] + nt * col + th + (s ? nt * nss : 0)];
+      else if (s == 0 && col < nss + ngs) v = d.dvbar[d.dv_start[1] + nt * (col - nss) + th];
+      else if (s == 0) v = d.dvbar[d.dv_start[2] + nt * (col - nss - ngs) + th];
     }
     vb[c][s][th] = v;
+  }
+  if (threadIdx.x < 8) {
+    const int col = w.y + threadIdx.x;
+    double f = 1.0;
+    if (d.apply_calib && col < nss) {
+      int a, b;
+      clh_ss_pair(d.ns, col, a, b);
+      f = (1.0 + d.par->shear_m[a]) * (1.0 + d.par->shear_m[b]);
+    } else if (d.apply_calib && col < nss + ngs) {
+      f = 1.0 + d.par->shear_m[(col - nss) % d.ns];
+    }
+    cf[threadIdx.x] = f;
+    tq[threadIdx.x] = 0.0;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 8 * nrow; i += CLH_BLOCK) {
@@ -98,237 +106,257 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_leg_adjoint(const DrawDesc* __res
       const double* K1 = d.kfold + ((size_t)nrow + r) * nt;
       for (int th = 0; th < nt; th++) acc += K1[th] * vb[c][1][th];
     }
-    d.cbar[(size_t)col * nrow + r] = acc;
+    d.cbar[(size_t)col * nrow + r] = acc * cf[c];
+    if (col < nss + ngs) atomicAdd(&tq[c], acc * d.cl[(size_t)col * nrow + r]);
   }
+  __syncthreads();
+  if (threadIdx.x < 8 && w.y + threadIdx.x < ncol) d.raw[w.y + threadIdx.x] = tq[threadIdx.x];
 }
 
 // ----------------------------------------------------------------------------
-// k_m_grad: one block per draw, one thread per source bin.
+// k_m_grad: one block per draw, one thread per source bin.  With T[col] the
+// cotangent-weighted uncalibrated column sums of k_leg_adjoint:
+//   d/dm_i = sum_{ss pairs (a,b) with i} d[(1+m_a)(1+m_b)]/dm_i T + sum_zl T[(zl, i)].
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_m_grad(const DrawDesc* __restrict__ D) {
   const DrawDesc& d = D[blockIdx.x];
   const int i = threadIdx.x;
-  if (i >= d.ns || !d.apply_calib) {
-    if (i < d.ns) d.g_m[i] = 0.0;
-    return;
-  }
-  const int nt = d.ntheta, nss = d.n_ss;
-  const double* m = d.par->shear_m;
+  if (i >= d.ns) return;
   double g = 0.0;
-  for (int q = 0; q < nss; q++) {
-    int a, b;
-    clh_ss_pair(d.ns, q, a, b);
-    if (a != i && b != i) continue;
-    const double der = (a == b) ? 2.0 * (1.0 + m[i]) : 1.0 + m[a == i ? b : a];
-    double s = 0.0;
-    for (int th = 0; th < nt; th++) {
-      const int ip = d.dv_start[0] + nt * q + th;
-      const int im = ip + nt * nss;
-      s += d.dvbar[ip] * d.raw[ip] + d.dvbar[im] * d.raw[im];
+  if (d.apply_calib) {
+    const double* m = d.par->shear_m;
+    for (int q = 0; q < d.n_ss; q++) {
+      int a, b;
+      clh_ss_pair(d.ns, q, a, b);
+      if (a != i && b != i) continue;
+      const double der = (a == b) ? 2.0 * (1.0 + m[i]) : 1.0 + m[a == i ? b : a];
+      g += der * d.raw[q];
     }
-    g += der * s;
-  }
-  for (int zl = 0; zl < d.nl; zl++) {
-    const int q = zl * d.ns + i;
-    for (int th = 0; th < nt; th++) {
-      const int idx = d.dv_start[1] + nt * q + th;
-      g += d.dvbar[idx] * d.raw[idx];
-    }
+    for (int zl = 0; zl < d.nl; zl++) g += d.raw[d.n_ss + zl * d.ns + i];
   }
   d.g_m[i] = g;
 }
 
 // ----------------------------------------------------------------------------
-// k_limber_adj: same work items as k_limber (draw, kind, zl, row0).
+// Cotangent of the ln P table gathered by one GL node (a fixed z column j)
+// along rows of nondecreasing k index i: the four bilinear weights of the
+// current cell accumulate in registers and go out with atomics only when the
+// row sequence leaves the cell.
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(CLH_BLOCK) void k_limber_adj(const DrawDesc* __restrict__ D,
-                                                          const int4* __restrict__ work) {
-  __shared__ double S[CLH_GG_ROWS * CLH_NP_GG];
-  __shared__ double W[(CLH_MAXB + 2) * CLH_NP_SS];
-  __shared__ double Cb[CLH_MAXB * (CLH_MAXB + 1) / 2 * CLH_SS_ROWS];
-  __shared__ double pr[3][CLH_GG_ROWS];
-  __shared__ double red[8];
+struct ClhLnPAcc {
+  int ci = -1;
+  double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
+  __device__ __forceinline__ void put(double* g, double v) {
+    if (v != 0.0) atomicAdd(g, v);
+  }
+  __device__ __forceinline__ void add(double* __restrict__ G, int nz, int j, int i, double dx, double dy,
+                                      double g) {
+    if (i != ci) {
+      if (ci >= 0) {
+        double* gp = G + (size_t)ci * nz + j;
+        put(gp, s00);
+        put(gp + 1, s01);
+        if (i == ci + 1) {
+          s00 = s10;
+          s01 = s11;
+        } else {
+          put(gp + nz, s10);
+          put(gp + nz + 1, s11);
+          s00 = s01 = 0.0;
+        }
+        s10 = s11 = 0.0;
+      }
+      ci = i;
+    }
+    s00 += g * ((1 - dx) * (1 - dy));
+    s01 += g * ((1 - dx) * dy);
+    s10 += g * (dx * (1 - dy));
+    s11 += g * (dx * dy);
+  }
+  __device__ __forceinline__ void done(double* __restrict__ G, int nz, int j) {
+    if (ci < 0) return;
+    double* gp = G + (size_t)ci * nz + j;
+    put(gp, s00);
+    put(gp + 1, s01);
+    put(gp + nz, s10);
+    put(gp + nz + 1, s11);
+  }
+};
+
+// ----------------------------------------------------------------------------
+// k_limber_sx_adj: adjoint of k_limber_sx.  One wave per work item (draw,
+// kind, zl, row0) of CLH_ADJ_ROWS rows; thread = GL node, walking its rows in
+// increasing ell with the forward's P(k,z) recomputation.  Per (row, node):
+//  ss: u_b = sum_j f_bj cbar_(b,j) A_j (f = 2 on the diagonal) gives both
+//      abar_b += pref0 S u_b and the ln P cotangent pref0 S sum_b A_b u_b / 2;
+//  gs: bbar_zs += pref2 cbar_zs g S, W_gal / W_mag / b1 cotangents and
+//      pref2 S g sum_zs cbar_zs B_zs for ln P, with g = W_gal b1 + W_mag pref1 bmag.
+// The C_ell-row cotangents of the tile sit in LDS (dynamic, npair x rows).
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict__ D,
+                                                      const int4* __restrict__ work) {
+  extern __shared__ double Cb[];  // [npair][CLH_ADJ_ROWS]
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
   const int kind = w.y, zl = w.z, row0 = w.w;
-  const int tid = threadIdx.x;
-  const int nrow = d.nrow;
-
-  if (kind == CLH_W_SS || kind == CLH_W_GS) {
-    const int np = CLH_NP_SS;
-    const double* o = (kind == CLH_W_SS) ? d.nodes + d.node_ss
-                                         : d.nodes + d.node_gs + (size_t)zl * d.node_gs_stride;
-    const double* ells = (kind == CLH_W_SS) ? d.ell_ss : d.ell_gx;
-    const int rr_own = tid >> 6, p_own = tid & 63;
-    const int r_own = row0 + rr_own;
-    double pk = 0.0, wts[4] = {0, 0, 0, 0};
-    int cell = 0;
-    if (r_own < nrow) {
-      const double ell = ells[r_own] + 0.5;
-      pk = clh_p_nonlin_cells(d, ell / o[np + p_own], o[p_own], cell, wts);
-    }
-    S[rr_own * np + p_own] = pk * o[2 * np + p_own];
-    const int nw = (kind == CLH_W_SS) ? d.ns : d.ns + 2;
-    for (int i = tid; i < nw * np; i += CLH_BLOCK) W[i] = o[3 * np + i];
-    const int npair = (kind == CLH_W_SS) ? d.n_ss : d.ns;
-    const int col0 = (kind == CLH_W_SS) ? 0 : d.n_ss + zl * d.ns;
-    for (int i = tid; i < npair * CLH_SS_ROWS; i += CLH_BLOCK) {
-      const int q = i / CLH_SS_ROWS, rr = i % CLH_SS_ROWS;
-      const int r = row0 + rr;
-      Cb[i] = (r < nrow) ? d.cbar[(size_t)(col0 + q) * nrow + r] : 0.0;
-    }
-    if (tid < CLH_SS_ROWS) {
-      const int r = row0 + tid;
-      double p0 = 0, p1 = 0, p2 = 0;
-      if (r < nrow) {
-        const double l = ells[r], ell = l + 0.5, ell4 = ell * ell * ell * ell;
-        p0 = l * (l - 1.) * (l + 1.) * (l + 2.) / ell4;
-        p1 = l * (l + 1.) / (ell * ell);
-        const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
-        p2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
-      }
-      pr[0][tid] = p0; pr[1][tid] = p1; pr[2][tid] = p2;
-    }
-    __syncthreads();
-    if (kind == CLH_W_SS) {
-      // ln P scatter of the own (row, node)
-      if (r_own < nrow) {
-        double sb = 0.0;
-        for (int q = 0; q < d.n_ss; q++) {
-          int i, j;
-          clh_ss_pair(d.ns, q, i, j);
-          sb += Cb[q * CLH_SS_ROWS + rr_own] * W[i * np + p_own] * W[j * np + p_own];
+  const int p = threadIdx.x;
+  constexpr int np = CLH_NP_SS, T = CLH_ADJ_ROWS;
+  const int nrow = d.nrow, ns = d.ns, nz = d.nzp;
+  const bool ss = kind == CLH_W_SS;
+  const double* o = ss ? d.nodes + d.node_ss : d.nodes + d.node_gs + (size_t)zl * d.node_gs_stride;
+  const double* ells = ss ? d.ell_ss : d.ell_gx;
+  const int npair = ss ? d.n_ss : ns;
+  const int col0 = ss ? 0 : d.n_ss + zl * ns;
+  __shared__ double L10[T];
+  for (int i = p; i < npair * T; i += 64) {
+    const int q = i / T, r = row0 + i % T;
+    Cb[i] = (r < nrow) ? d.cbar[(size_t)(col0 + q) * nrow + r] : 0.0;
+  }
+  if (p < T) L10[p] = log10((row0 + p < nrow ? ells[row0 + p] : 1.0) + 0.5);
+  __syncthreads();
+  const double a = o[p], fK = o[np + p], fac = o[2 * np + p];
+  const double z = 1.0 / a - 1.0;
+  const int j = clh_bs(d.pz, nz, z);
+  const double dy = (z - d.pz[j]) / (d.pz[j + 1] - d.pz[j]);
+  const int woff = ss ? 3 : 5;
+  double Wn[CLH_MAXB], acc[CLH_MAXB];
+#pragma unroll
+  for (int b = 0; b < CLH_MAXB; b++) {
+    Wn[b] = (b < ns) ? o[(woff + b) * np + p] : 0.0;
+    acc[b] = 0.0;
+  }
+  const double WG = ss ? 0.0 : o[3 * np + p], WM = ss ? 0.0 : o[4 * np + p];
+  const double b1 = ss ? 0.0 : d.par->b1[zl], bmag = ss ? 0.0 : d.par->bmag[zl];
+  double twg = 0.0, twm = 0.0;
+  ClhLnPAcc sc;
+  const double l10fk = log10(fK * CLH_COVERH0);
+  int i = -1;
+  double inv_k = 0.0;
+  for (int rr = 0; rr < T; rr++) {
+    const int r = row0 + rr;
+    if (r >= nrow) break;
+    const double l = ells[r], ell = l + 0.5;
+    const double log10k = L10[rr] - l10fk;
+    const int i0 = i;
+    i = (i < 0) ? clh_bs(d.lk, d.nk, log10k) : clh_walk(d.lk, d.nk, log10k, i);
+    if (i != i0) inv_k = 1.0 / (d.lk[i + 1] - d.lk[i]);
+    const double dx = (log10k - d.lk[i]) * inv_k;
+    const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
+    const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +
+                       dx * dy * P[nz + 1];
+    const double S = exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac;
+    double g;
+    if (ss) {
+      const double pref0 = l * (l - 1.) * (l + 1.) * (l + 2.) / (ell * ell * ell * ell);
+      const double c = pref0 * S;
+      double sb = 0.0;
+#pragma unroll
+      for (int b = 0; b < CLH_MAXB; b++) {
+        if (b < ns) {
+          double u = 0.0;
+#pragma unroll
+          for (int jj = 0; jj < CLH_MAXB; jj++)
+            if (jj < ns) u += (jj == b ? 2.0 : 1.0) * Cb[clh_ss_index(ns, b, jj) * T + rr] * Wn[jj];
+          acc[b] += c * u;
+          sb += Wn[b] * u;
         }
-        sb *= pr[0][rr_own] * o[2 * np + p_own];
-        clh_scatter_lnP(d, cell, wts, sb * pk);
       }
-      // A_b cotangents
-      for (int it = tid; it < d.ns * np; it += CLH_BLOCK) {
-        const int b = it / np, p = it % np;
-        double acc = 0.0;
-        for (int rr = 0; rr < CLH_SS_ROWS; rr++) {
-          double s = 0.0;
-          for (int j = 0; j < d.ns; j++) {
-            const double f = (j == b) ? 2.0 : 1.0;
-            s += f * Cb[clh_ss_index(d.ns, b, j) * CLH_SS_ROWS + rr] * W[j * np + p];
-          }
-          acc += pr[0][rr] * S[rr * np + p] * s;
-        }
-        if (acc != 0.0) atomicAdd(d.abar + (size_t)b * np + p, acc);
-      }
+      g = 0.5 * sb * c;
     } else {
-      const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
-      const double* WG = W;
-      const double* WM = W + np;
-      if (r_own < nrow) {
-        const double g = WG[p_own] * b1 + WM[p_own] * pr[1][rr_own] * bmag;
-        double sb = 0.0;
-        for (int zs = 0; zs < d.ns; zs++) sb += Cb[zs * CLH_SS_ROWS + rr_own] * W[(2 + zs) * np + p_own];
-        sb *= pr[2][rr_own] * g * o[2 * np + p_own];
-        clh_scatter_lnP(d, cell, wts, sb * pk);
-      }
-      // B_zs cotangents
-      for (int it = tid; it < d.ns * np; it += CLH_BLOCK) {
-        const int zs = it / np, p = it % np;
-        double acc = 0.0;
-        for (int rr = 0; rr < CLH_SS_ROWS; rr++)
-          acc += pr[2][rr] * Cb[zs * CLH_SS_ROWS + rr] * (WG[p] * b1 + WM[p] * pr[1][rr] * bmag) * S[rr * np + p];
-        if (acc != 0.0) atomicAdd(d.bbar + ((size_t)zl * d.ns + zs) * np + p, acc);
-      }
-      // W_gal, W_mag cotangents and b1
-      double b1p = 0.0;
-      if (tid < np) {
-        const int p = tid;
-        double t = 0.0, tm = 0.0;
-        for (int rr = 0; rr < CLH_SS_ROWS; rr++) {
-          double s = 0.0;
-          for (int zs = 0; zs < d.ns; zs++) s += Cb[zs * CLH_SS_ROWS + rr] * W[(2 + zs) * np + p];
-          s *= pr[2][rr] * S[rr * np + p];
-          t += s;
-          tm += s * pr[1][rr];
+      const double pref1 = l * (l + 1.) / (ell * ell);
+      const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+      const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+      const double gp = WG * b1 + WM * pref1 * bmag;
+      const double cg = pref2 * gp * S;
+      double s = 0.0;
+#pragma unroll
+      for (int b = 0; b < CLH_MAXB; b++) {
+        if (b < ns) {
+          const double cb = Cb[b * T + rr];
+          s += cb * Wn[b];
+          acc[b] += cg * cb;
         }
-        atomicAdd(d.wbar + ((size_t)zl * 2 + 0) * np + p, t * b1);
-        atomicAdd(d.wbar + ((size_t)zl * 2 + 1) * np + p, tm * bmag);
-        b1p = t * WG[p];
       }
-      if (tid < 64) {
-        b1p = clh_wave_sum(b1p);
-        if (tid == 0 && b1p != 0.0) atomicAdd(d.g_b1 + zl, b1p);
-      }
+      const double t = pref2 * S * s;
+      twg += t;
+      twm += t * pref1;
+      g = t * gp;
     }
+    sc.add(d.g_lnP, nz, j, i, dx, dy, g);
+  }
+  sc.done(d.g_lnP, nz, j);
+  if (ss) {
+#pragma unroll
+    for (int b = 0; b < CLH_MAXB; b++)
+      if (b < ns && acc[b] != 0.0) atomicAdd(d.abar + (size_t)b * np + p, acc[b]);
   } else {
-    const int np = CLH_NP_GG;
-    const double* o = d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride;
-    const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
-    const int colg = d.n_ss + d.n_gs + zl;
-    double b1p = 0.0, lzp = 0.0;
-    for (int it = tid; it < CLH_GG_ROWS * np; it += CLH_BLOCK) {
-      const int rr = it / np, p = it % np;
-      const int r = row0 + rr;
-      double tb = 0.0;
-      if (r < nrow) {
-        const double cb = d.cbar[(size_t)colg * nrow + r];
-        const double l = d.ell_gx[r], ell = l + 0.5;
-        const double a = o[p], fK = o[np + p], fac = o[2 * np + p];
-        const double k = ell / fK;
-        const double chi_0 = ell / k;
-        const double chi_1 = (ell + 1.0) / k;
-        if (!(chi_1 > d.chi_amin) && cb != 0.0) {
-          const double a_0 = clh_a_chi(d, chi_0);
-          const double a_1 = clh_a_chi(d, chi_1);
-          double c0, dc0, c1, dc1;
-          clh_chi_all(d, a_0, c0, dc0);
-          clh_chi_all(d, a_1, c1, dc1);
-          const double h0 = 1.0 / ((a_0 * a_0) * dc0);
-          const double h1 = 1.0 / ((a_1 * a_1) * dc1);
-          const double al = (1 + 8. * ell) / ((2. * ell + 1.) * (2. * ell + 1.)) * h0 * clh_f_growth(d, 1.0 / a_0 - 1.0);
-          const double be = 4. / (2 * ell + 3.) * sqrt((2 * ell + 1.) / (2 * ell + 3.)) * h1 *
-                            clh_f_growth(d, 1.0 / a_1 - 1.0);
-          const double wr = al * clh_nz_lens(d, 1. / a_0 - 1., zl) - be * clh_nz_lens(d, 1. / a_1 - 1., zl);
-          const double dwr = al * clh_dnz_lens(d, 1. / a_0 - 1., zl) - be * clh_dnz_lens(d, 1. / a_1 - 1., zl);
-          const double pref = l * (l + 1.) / (ell * ell);
-          const double t = o[3 * np + p] * b1 + o[4 * np + p] * pref * bmag + wr;
-          int cell;
-          double wts[4];
-          const double pk = clh_p_nonlin_cells(d, k, a, cell, wts);
-          clh_scatter_lnP(d, cell, wts, cb * t * t * fac * pk);
-          tb = cb * 2.0 * t * pk * fac;
-          b1p += tb * o[3 * np + p];
-          lzp += tb * dwr;
-        }
-      }
-      S[it] = tb;
-    }
-    // mag-bias weight needs pref per row: keep the row factor alongside
-    if (tid < CLH_GG_ROWS) {
-      const int r = row0 + tid;
-      double pf = 0.0;
-      if (r < nrow) { const double l = d.ell_gx[r], ell = l + 0.5; pf = l * (l + 1.) / (ell * ell); }
-      pr[1][tid] = pf;
-    }
-    __syncthreads();
-    if (tid < np) {
-      double tg = 0.0, tm = 0.0;
-      for (int rr = 0; rr < CLH_GG_ROWS; rr++) {
-        tg += S[rr * np + tid];
-        tm += S[rr * np + tid] * pr[1][rr];
-      }
-      const size_t base = (size_t)2 * d.nl * CLH_NP_SS + (size_t)zl * 2 * np;
-      if (tg != 0.0) atomicAdd(d.wbar + base + tid, tg * b1);
-      if (tm != 0.0 && bmag != 0.0) atomicAdd(d.wbar + base + np + tid, tm * bmag);
-    }
-    // block reductions of b1 and lens-shift partials
-    const int lane = tid & 63, wid = tid >> 6;
-    b1p = clh_wave_sum(b1p);
-    lzp = clh_wave_sum(lzp);
-    if (lane == 0) { red[wid] = b1p; red[4 + wid] = lzp; }
-    __syncthreads();
-    if (tid == 0) {
-      const double sb = red[0] + red[1] + red[2] + red[3];
-      const double sl = red[4] + red[5] + red[6] + red[7];
-      if (sb != 0.0) atomicAdd(d.g_b1 + zl, sb);
-      if (sl != 0.0) atomicAdd(d.g_lens + zl, sl);
-    }
+#pragma unroll
+    for (int b = 0; b < CLH_MAXB; b++)
+      if (b < ns && acc[b] != 0.0) atomicAdd(d.bbar + ((size_t)zl * ns + b) * np + p, acc[b]);
+    atomicAdd(d.wbar + ((size_t)zl * 2 + 0) * np + p, twg * b1);
+    atomicAdd(d.wbar + ((size_t)zl * 2 + 1) * np + p, twm * bmag);
+    const double b1p = clh_wave_sum(twg * WG);
+    if (p == 0 && b1p != 0.0) atomicAdd(d.g_b1 + zl, b1p);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// k_limber_gg_adj: adjoint of k_limber_gg, same work items and thread layout
+// (two groups of 96 node threads, each walking half of the rows; ClhGgNode).
+// Per (row, node) with integrand t^2 P fac, t = W_gal b1 + W_mag pref bmag +
+// W_RSD: ln P cotangent cbar t^2 P fac; tb = 2 cbar t P fac feeds b1 (W_gal),
+// the W_gal / W_mag node cotangents and the lens shift through dW_RSD/d dz.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restrict__ D,
+                                                       const int4* __restrict__ work) {
+  __shared__ ClhGgRow R[CLH_GGT_TILE];
+  __shared__ double Cr[CLH_GGT_TILE];
+  __shared__ double red[2][3];
+  const int4 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const int zl = w.y, row0 = w.z;
+  constexpr int np = CLH_NP_GG;
+  const int nrow = d.nrow;
+  if (threadIdx.x < CLH_GGT_TILE) {
+    const int r = row0 + threadIdx.x;
+    R[threadIdx.x] = clh_gg_row(r < nrow ? d.ell_gx[r] : 1.0);
+    Cr[threadIdx.x] = r < nrow ? d.cbar[(size_t)(d.n_ss + d.n_gs + zl) * nrow + r] : 0.0;
+  }
+  __syncthreads();
+  const int p = threadIdx.x % np, half = threadIdx.x / np;
+  ClhGgNode<true> nd;
+  nd.init(d, d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride, p, zl);
+  double tg = 0.0, tm = 0.0, b1p = 0.0, lzp = 0.0;
+  ClhLnPAcc sc;
+  constexpr int per = CLH_GGT_TILE / 2;
+  for (int q = 0; q < per; q++) {
+    const int rr = half * per + q;
+    const double cb = Cr[rr];
+    if (cb == 0.0) continue;  // also rows past the table
+    double t, pf, dwr, dx;
+    if (!nd.eval(d, R[rr], t, pf, dwr, dx)) continue;
+    sc.add(d.g_lnP, d.nzp, nd.j, nd.i, dx, nd.dy, cb * t * t * pf);
+    const double tb = cb * 2.0 * t * pf;
+    tg += tb;
+    tm += tb * R[rr].pref;
+    b1p += tb * nd.WG;
+    lzp += tb * dwr;
+  }
+  sc.done(d.g_lnP, d.nzp, nd.j);
+  const double b1 = nd.b1, bmag = nd.bmag;
+  const size_t base = (size_t)2 * d.nl * CLH_NP_SS + (size_t)zl * 2 * np;
+  if (tg != 0.0) atomicAdd(d.wbar + base + p, tg * b1);
+  if (tm != 0.0 && bmag != 0.0) atomicAdd(d.wbar + base + np + p, tm * bmag);
+  b1p = clh_wave_sum(b1p);
+  lzp = clh_wave_sum(lzp);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { red[0][wid] = b1p; red[1][wid] = lzp; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double sb = red[0][0] + red[0][1] + red[0][2];
+    const double sl = red[1][0] + red[1][1] + red[1][2];
+    if (sb != 0.0) atomicAdd(d.g_b1 + zl, sb);
+    if (sl != 0.0) atomicAdd(d.g_lens + zl, sl);
   }
 }
 
@@ -411,9 +439,16 @@ hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s) {
   hipLaunchKernelGGL(k_m_grad, dim3(B), dim3(64), 0, s, D);
   return hipGetLastError();
 }
-hipError_t clh_launch_limber_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
+hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int max_npair,
+                                   hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_limber_adj, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  const size_t lds = sizeof(double) * (size_t)max_npair * CLH_ADJ_ROWS;
+  hipLaunchKernelGGL(k_limber_sx_adj, dim3(n), dim3(64), lds, s, D, work);
+  return hipGetLastError();
+}
+hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_limber_gg_adj, dim3(n), dim3(192), 0, s, D, work);
   return hipGetLastError();
 }
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s) {

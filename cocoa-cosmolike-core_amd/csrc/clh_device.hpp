@@ -284,3 +284,148 @@ __device__ __forceinline__ double clh_wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+
+// ------------------------------------------------------------- gg integrand
+// Per-row constants of the gg integrand (ell = l + 1/2): log10(ell),
+// chi_1/chi = (ell+1)/ell, the two W_RSD coefficients (radial_weights.c:98-103)
+// and the magnification prefactor l(l+1)/ell^2.
+struct ClhGgRow {
+  double l10, rr1, c0, c1, pref;
+};
+
+__device__ __forceinline__ ClhGgRow clh_gg_row(double l) {
+  const double ell = l + 0.5;
+  ClhGgRow R;
+  R.l10 = log10(ell);
+  R.rr1 = (ell + 1.0) / ell;
+  R.c0 = (1 + 8. * ell) / ((2. * ell + 1.) * (2. * ell + 1.));
+  R.c1 = 4. / (2 * ell + 3.) * sqrt((2 * ell + 1.) / (2 * ell + 3.));
+  R.pref = l * (l + 1.) / (ell * ell);
+  return R;
+}
+
+// One gg GL node evaluating the integrand (W_gal b1 + W_mag pref bmag +
+// W_RSD)^2 P(ell/chi, a) fac (cosmo2D.c:2490-2666) row after row in
+// increasing ell.  The a_0 half of W_RSD depends only on the node (chi_0 =
+// chi) and is evaluated once; the a_1 half (chi_1 = chi (ell+1)/ell) moves
+// monotonically toward the node, so every table it reads -- chi(z) for
+// a_chi and the dchi/dz stencil (cosmo3D.c:40-133), G(z) for f_growth
+// (cosmo3D.c:200-228), the lens n(z) spline (redshift_spline.c:736-762) and
+// log10 k of P(k,z) (cosmo3D.c:362-417) -- is found by a walk from the
+// previous row, with the reciprocal spacing of the interval recomputed only
+// when the interval changes.  ADJ additionally returns d W_RSD / d dz_lens.
+template <bool ADJ>
+struct ClhGgNode {
+  double fK, fac, WG, WM, b1, bmag, l10fk, dy, t0, t0d, inv_st, dzl, zm, chi_amin;
+  double inv_c, inv_z, up, down, slG, inv_k;
+  int j, zl, jc, jz, jj, jg, jn, i;
+
+  __device__ __forceinline__ void set_z(const DrawDesc& d) {
+    const double* zz = d.chi_z;
+    const double* cc = d.chi_v;
+    inv_z = 1.0 / (zz[jj + 1] - zz[jj]);
+    up = (cc[jj + 2] - cc[jj]) / (zz[jj + 2] - zz[jj]);
+    down = (jj > 0) ? (cc[jj + 1] - cc[jj - 1]) / (zz[jj + 1] - zz[jj - 1]) : (cc[jj + 1] - cc[jj]) * inv_z;
+  }
+
+  __device__ __forceinline__ void init(const DrawDesc& d, const double* __restrict__ o, int p, int zl_) {
+    constexpr int np = CLH_NP_GG;
+    zl = zl_;
+    const double a = o[p];
+    fK = o[np + p];
+    fac = o[2 * np + p];
+    WG = o[3 * np + p];
+    WM = o[4 * np + p];
+    b1 = d.par->b1[zl];
+    bmag = d.par->bmag[zl];
+    const double z = 1.0 / a - 1.0;
+    j = clh_bs(d.pz, d.nzp, z);
+    dy = (z - d.pz[j]) / (d.pz[j + 1] - d.pz[j]);
+    l10fk = log10(fK * CLH_COVERH0);
+    inv_st = 1.0 / d.par->lens_st[zl];
+    dzl = d.par->lens_dz[zl];
+    zm = d.l_zmean[zl];
+    chi_amin = d.chi_amin;
+    // a_0 side of W_RSD at chi_0 = chi (radial_weights.c:98-99)
+    const double chi = fK * CLH_COVERH0;
+    jc = clh_bs(d.chi_v, d.chi_n, chi);
+    inv_c = 1.0 / (d.chi_v[jc + 1] - d.chi_v[jc]);
+    const double z0 = d.chi_z[jc] + (chi - d.chi_v[jc]) * inv_c * (d.chi_z[jc + 1] - d.chi_z[jc]);
+    const double a0 = 1.0 / (1.0 + z0);
+    double c0, dc0;
+    clh_chi_all(d, a0, c0, dc0);
+    const double zz0 = 1.0 / a0 - 1.0;
+    const double hf0 = 1.0 / ((a0 * a0) * dc0) * clh_f_growth(d, zz0);
+    t0 = clh_nz_lens(d, zz0, zl) * hf0;
+    t0d = ADJ ? clh_dnz_lens(d, zz0, zl) * hf0 : 0.0;
+    jz = clh_bs(d.chi_z, d.chi_n, zz0);
+    jj = jz > d.chi_n - 3 ? d.chi_n - 3 : jz;
+    set_z(d);
+    jg = clh_bs(d.G_z, d.G_n, zz0);
+    slG = (d.G_v[jg + 1] - d.G_v[jg]) / (d.G_z[jg + 1] - d.G_z[jg]);
+    jn = clh_bs(d.l_zc, d.l_rows, zz0);
+    i = -1;
+    inv_k = 0.0;
+  }
+
+  // false: chi_1 beyond chi(a_min), integrand zero (cosmo2D.c:2512-2514).
+  // t: the bracket; pf = P fac; dx: log10 k weight of the P cell (i, j).
+  __device__ __forceinline__ bool eval(const DrawDesc& d, const ClhGgRow& R, double& t, double& pf,
+                                       double& dwr, double& dx) {
+    const double chi_1 = fK * R.rr1;
+    if (chi_1 > chi_amin) return false;
+    const double c1v = chi_1 * CLH_COVERH0;
+    {
+      const int j0 = jc;
+      jc = clh_walk(d.chi_v, d.chi_n, c1v, jc);
+      if (jc != j0) inv_c = 1.0 / (d.chi_v[jc + 1] - d.chi_v[jc]);
+    }
+    const double z1 = d.chi_z[jc] + (c1v - d.chi_v[jc]) * inv_c * (d.chi_z[jc + 1] - d.chi_z[jc]);
+    jz = clh_walk(d.chi_z, d.chi_n, z1, jz);
+    {
+      const int j2 = jz > d.chi_n - 3 ? d.chi_n - 3 : jz;
+      if (j2 != jj) {
+        jj = j2;
+        set_z(d);
+      }
+    }
+    const double Dz = down + (z1 - d.chi_z[jj]) * inv_z * (up - down);  // dchi/dz
+    {
+      const int j0 = jg;
+      jg = clh_walk(d.G_z, d.G_n, z1, jg);
+      if (jg != j0) slG = (d.G_v[jg + 1] - d.G_v[jg]) / (d.G_z[jg + 1] - d.G_z[jg]);
+    }
+    const double G = d.G_v[jg] + (z1 - d.G_z[jg]) * slG;
+    // hoverh0 * f_growth = (c/H0)/(dchi/dz) * (1 - G'(1+z)/G)
+    const double hf1 = CLH_COVERH0 * (G - slG * (1.0 + z1)) / (Dz * G);
+    double n1 = 0.0, dn1 = 0.0;
+    {
+      const double zs = (z1 - dzl - zm) * inv_st + zm;
+      const int nr = d.l_rows;
+      if (!(zs <= d.l_zc[0] || zs >= d.l_zc[nr - 1])) {
+        jn = clh_walk(d.l_zc, nr, zs, jn);
+        const int jn2 = jn > nr - 2 ? nr - 2 : jn;
+        const double* cf = d.l_coef + (size_t)zl * 4 * (nr - 1) + 4 * jn2;
+        const double x = zs - d.l_zc[jn2];
+        n1 = (cf[0] + x * (cf[1] + x * (cf[2] + x * cf[3]))) * inv_st;
+        if (ADJ) dn1 = -(cf[1] + x * (2.0 * cf[2] + 3.0 * x * cf[3])) * (inv_st * inv_st);
+      }
+    }
+    const double wr = R.c0 * t0 - R.c1 * hf1 * n1;
+    if (ADJ) dwr = R.c0 * t0d - R.c1 * hf1 * dn1;
+    t = WG * b1 + WM * R.pref * bmag + wr;
+    const double log10k = R.l10 - l10fk;
+    {
+      const int i0 = i;
+      i = (i < 0) ? clh_bs(d.lk, d.nk, log10k) : clh_walk(d.lk, d.nk, log10k, i);
+      if (i != i0) inv_k = 1.0 / (d.lk[i + 1] - d.lk[i]);
+    }
+    dx = (log10k - d.lk[i]) * inv_k;
+    const int nz = d.nzp;
+    const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
+    const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +
+                       dx * dy * P[nz + 1];
+    pf = exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac;
+    return true;
+  }
+};

@@ -17,11 +17,9 @@
 
 hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
-hipError_t clh_launch_limber(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, hipStream_t s);
-hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow,
-                               hipStream_t s, int raw);
+hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow, hipStream_t s);
 hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_fill(double* p, size_t n, double v, hipStream_t s);
 hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nknz, const double* lnP,
@@ -29,7 +27,9 @@ hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nk
                                  const double* iaA, const double* m, hipStream_t s);
 hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
-hipError_t clh_launch_limber_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s);
+hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int max_npair,
+                                   hipStream_t s);
+hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_sx32(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_gg32(const DrawDesc* D, const int4* work, int n, hipStream_t s);
@@ -744,13 +744,10 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     for (int b = 0; b < ns + nl; b++) wg.push_back(make_int2(i, b));
     for (int s = 0; s <= 2 * nl; s++) wn.push_back(make_int2(i, s));
   }
-  for (int i = 0; i < B; i++)  // gg tiles first: the heaviest per block
+  for (int i = 0; i < B; i++) {  // adjoint ss/gs tiles of 32 rows, ss (heaviest) first
+    for (int r = 0; r < nrow; r += CLH_ADJ_ROWS) wl.push_back(make_int4(i, CLH_W_SS, 0, r));
     for (int zl = 0; zl < nl; zl++)
-      for (int r = 0; r < nrow; r += CLH_GG_ROWS) wl.push_back(make_int4(i, CLH_W_GG, zl, r));
-  for (int i = 0; i < B; i++) {
-    for (int zl = 0; zl < nl; zl++)
-      for (int r = 0; r < nrow; r += CLH_SS_ROWS) wl.push_back(make_int4(i, CLH_W_GS, zl, r));
-    for (int r = 0; r < nrow; r += CLH_SS_ROWS) wl.push_back(make_int4(i, CLH_W_SS, 0, r));
+      for (int r = 0; r < nrow; r += CLH_ADJ_ROWS) wl.push_back(make_int4(i, CLH_W_GS, zl, r));
   }
   const int ntask = 2 * n_ss + n_gs + nl;
   for (int i = 0; i < B; i++)
@@ -759,11 +756,11 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   std::vector<int4> wsx, wgg;
   for (int i = 0; i < B; i++)
     for (int zl = 0; zl < nl; zl++)
-      for (int r = 0; r < nrow; r += 16) wgg.push_back(make_int4(i, zl, r, 0));
+      for (int r = 0; r < nrow; r += CLH_GGT_TILE) wgg.push_back(make_int4(i, zl, r, 0));
   for (int i = 0; i < B; i++) {
     for (int zl = 0; zl < nl; zl++)
-      for (int r = 0; r < nrow; r += 16) wsx.push_back(make_int4(i, CLH_W_GS, zl, r));
-    for (int r = 0; r < nrow; r += 16) wsx.push_back(make_int4(i, CLH_W_SS, 0, r));
+      for (int r = 0; r < nrow; r += CLH_SX_TILE) wsx.push_back(make_int4(i, CLH_W_GS, zl, r));
+    for (int r = 0; r < nrow; r += CLH_SX_TILE) wsx.push_back(make_int4(i, CLH_W_SS, 0, r));
   }
   std::vector<int4> wmx;  // MFMA Legendre tiles: 32 columns of one kind
   {
@@ -828,7 +825,7 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
     else if (mixed)
       HIPCHK(c, clh_launch_legendre_mx(bt.d_desc.p, bt.d_wmx.p, bt.n_wmx, c->precision, s));
     else
-      HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s, 0));
+      HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s));
   }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[5], s));
   return CLH_OK;
@@ -1342,17 +1339,18 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (g_lens_dz && nl) HIPCHK(c, clh_launch_fill(g_lens_dz, (size_t)nl * B, 0.0, s));
   if (g_b1 && nl) HIPCHK(c, clh_launch_fill(g_b1, (size_t)nl * B, 0.0, s));
   if (g_ia_A) HIPCHK(c, clh_launch_fill(g_ia_A, (size_t)B, 0.0, s));
-  // forward recompute (tables, nodes, C_ell rows, uncalibrated data vector)
+  // forward recompute: lens-efficiency tables and their shift tangents, node
+  // weights, and the ss / gs C_ell rows (the shear-calibration gradient needs
+  // them; the gg rows carry no calibration and are not recomputed)
   HIPCHK(c, clh_launch_gtab(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
   HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
   HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
-  HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, s));
   HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
-  HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s, 1));
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa.p, bt.n_wa, s));
   HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
-  HIPCHK(c, clh_launch_limber_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl, s));
+  HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, s));
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl, std::max(c->L.n_ss(), c->L.src.nbin), s));
   HIPCHK(c, clh_launch_nodes_adj(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   return CLH_OK;
 }

@@ -21,9 +21,11 @@
 #define CLH_PREC_BF16X3 2
 
 // Rows of a Limber table column: CLH_NLOW direct rows (ell = r+1), then the
-// N_ell log-spaced rows.  Rows per ss/gs work tile and per gg tile:
-#define CLH_SS_ROWS 4
-#define CLH_GG_ROWS 8
+// N_ell log-spaced rows.  Rows per work tile of the ss/gs forward, the gg
+// forward and adjoint, and the ss/gs adjoint kernels:
+#define CLH_SX_TILE 16
+#define CLH_GGT_TILE 16
+#define CLH_ADJ_ROWS 32
 
 // Everything one draw needs on the device.  Plain data; lives in an array in
 // device memory and is read with uniform (scalar) loads.
@@ -83,7 +85,7 @@ struct DrawDesc {
 
   // adjoint (VJP) buffers of the draw
   const double* dvbar;  // cotangent of the data vector
-  double* raw;          // uncalibrated data vector
+  double* raw;          // [ncol] dvbar . uncalibrated data vector per C_ell column
   double* cbar;         // [ncol][nrow] cotangent of the C_ell rows
   double* abar;         // [ns][64] cotangent of A_b at the ss nodes
   double* bbar;         // [nl][ns][64] cotangent of B_zs at the gs nodes

@@ -5,9 +5,9 @@
 //             (cosmolike/redshift_spline.c:852-923, 1014-1109)
 //  k_nodes    Gauss-Legendre nodes in a and every weight the Limber sums need
 //             (create_cosmo_nodes cosmo2D.c:1331-1356; radial_weights.c)
-//  k_limber   Limber node sums for ss / gs tiles (P(k,z) and weights staged in
-//             LDS) and the gg integrand with RSD (cosmo2D.c:1655-1747,
-//             2229-2348, 2490-2666) -> C_ell rows
+//  k_limber_sx  Limber node sums for ss / gs tiles (P(k,z) and weights staged
+//               in LDS; cosmo2D.c:1655-1747, 2229-2348) -> C_ell rows
+//  k_limber_gg  gg integrand with RSD (cosmo2D.c:2490-2666) -> C_ell rows
 //  k_legendre contraction of C_ell rows with the bin-averaged Legendre
 //             kernels into xi+, xi-, gamma_t, w and data-vector placement with
 //             shear calibration (cosmo2D.c:487-534; generic_interface.hpp:800-862)
@@ -200,126 +200,16 @@ __global__ __launch_bounds__(128) void k_nodes(const DrawDesc* __restrict__ D,
 }
 
 // ----------------------------------------------------------------------------
-// k_limber: work item (draw, kind, zl, row0)
-// ----------------------------------------------------------------------------
-__global__ __launch_bounds__(CLH_BLOCK) void k_limber(const DrawDesc* __restrict__ D,
-                                                      const int4* __restrict__ work) {
-  __shared__ double S[CLH_GG_ROWS * CLH_NP_GG];          // P * fac per (row, node)
-  __shared__ double W[(CLH_MAXB + 2) * CLH_NP_SS];        // weights per (bin, node)
-  const int4 w = work[blockIdx.x];
-  const DrawDesc& d = D[w.x];
-  const int kind = w.y, zl = w.z, row0 = w.w;
-  const int tid = threadIdx.x;
-
-  if (kind == CLH_W_SS || kind == CLH_W_GS) {
-    const int np = CLH_NP_SS;
-    const double* o = (kind == CLH_W_SS) ? d.nodes + d.node_ss
-                                         : d.nodes + d.node_gs + (size_t)zl * d.node_gs_stride;
-    const double* ells = (kind == CLH_W_SS) ? d.ell_ss : d.ell_gx;
-    {
-      const int rr = tid >> 6, p = tid & 63;
-      const int r = row0 + rr;
-      double s = 0.0;
-      if (r < d.nrow) {
-        const double ell = ells[r] + 0.5;
-        const double k = ell / o[np + p];
-        s = clh_p_nonlin(d, k, o[p]) * o[2 * np + p];
-      }
-      S[rr * np + p] = s;
-    }
-    const int nw = (kind == CLH_W_SS) ? d.ns : d.ns + 2;
-    const int woff = (kind == CLH_W_SS) ? 3 : 3;
-    for (int i = tid; i < nw * np; i += CLH_BLOCK) W[i] = o[woff * np + i];
-    __syncthreads();
-    if (kind == CLH_W_SS) {
-      const int nout = d.n_ss * CLH_SS_ROWS;
-      for (int q = tid; q < nout; q += CLH_BLOCK) {
-        const int pair = q / CLH_SS_ROWS, rr = q % CLH_SS_ROWS;
-        const int r = row0 + rr;
-        if (r >= d.nrow) continue;
-        int i, j;
-        clh_ss_pair(d.ns, pair, i, j);
-        const double* Ai = W + i * np;
-        const double* Aj = W + j * np;
-        const double* Sr = S + rr * np;
-        double acc = 0.0;
-#pragma unroll 8
-        for (int p = 0; p < np; p++) acc += Ai[p] * Aj[p] * Sr[p];
-        const double l = ells[r], ell = l + 0.5, ell4 = ell * ell * ell * ell;
-        const double pref = l * (l - 1.) * (l + 1.) * (l + 2.) / ell4;
-        d.cl[(size_t)pair * d.nrow + r] = acc * pref;
-      }
-    } else {
-      // W rows: 0 = WGAL, 1 = WMAG, 2.. = B[zs]
-      const int nout = d.ns * CLH_SS_ROWS;
-      const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
-      for (int q = tid; q < nout; q += CLH_BLOCK) {
-        const int zs = q / CLH_SS_ROWS, rr = q % CLH_SS_ROWS;
-        const int r = row0 + rr;
-        if (r >= d.nrow) continue;
-        const double l = ells[r], ell = l + 0.5;
-        const double pref1 = l * (l + 1.) / (ell * ell);
-        const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
-        const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
-        const double* Bz = W + (2 + zs) * np;
-        const double* Sr = S + rr * np;
-        double acc = 0.0;
-#pragma unroll 8
-        for (int p = 0; p < np; p++) acc += Bz[p] * (W[p] * b1 + W[np + p] * pref1 * bmag) * Sr[p];
-        d.cl[(size_t)(d.n_ss + zl * d.ns + zs) * d.nrow + r] = acc * pref2;
-      }
-    }
-  } else {
-    // gg: full integrand per (row, node) incl. RSD (cosmo2D.c:2490-2608)
-    const int np = CLH_NP_GG;
-    const double* o = d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride;
-    const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
-    for (int it = tid; it < CLH_GG_ROWS * np; it += CLH_BLOCK) {
-      const int rr = it / np, p = it % np;
-      const int r = row0 + rr;
-      double v = 0.0;
-      if (r < d.nrow) {
-        const double l = d.ell_gx[r], ell = l + 0.5;
-        const double a = o[p], fK = o[np + p];
-        const double k = ell / fK;
-        const double chi_0 = ell / k;
-        const double chi_1 = (ell + 1.0) / k;
-        if (!(chi_1 > d.chi_amin)) {
-          const double a_0 = clh_a_chi(d, chi_0);
-          const double a_1 = clh_a_chi(d, chi_1);
-          const double wr = clh_w_rsd(d, ell, a_0, a_1, zl);
-          const double pref = l * (l + 1.) / (ell * ell);
-          const double t = o[3 * np + p] * b1 + o[4 * np + p] * pref * bmag + wr;
-          v = t * t * clh_p_nonlin(d, k, a) * o[2 * np + p];
-        }
-      }
-      S[it] = v;
-    }
-    __syncthreads();
-    // one wave reduces the CLH_GG_ROWS rows
-    if (tid < 64) {
-      for (int rr = 0; rr < CLH_GG_ROWS; rr++) {
-        double v = S[rr * np + tid] + ((tid + 64 < np) ? S[rr * np + tid + 64] : 0.0);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        const int r = row0 + rr;
-        if (tid == 0 && r < d.nrow) d.cl[(size_t)(d.n_ss + d.n_gs + zl) * d.nrow + r] = v;
-      }
-    }
-  }
-}
-
-// ----------------------------------------------------------------------------
 // k_limber_sx: ss and gs tiles of CLH_SX_TILE rows.  One thread per GL node
 // walks its rows in increasing ell, so the log10 k index of the P(k,z) lookup
 // moves by at most a step between rows and the z index is found once.
 // Work item (draw, kind, zl, row0).
 // ----------------------------------------------------------------------------
-#define CLH_SX_TILE 16
 __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restrict__ D,
                                                          const int4* __restrict__ work) {
   __shared__ double S[CLH_SX_TILE * CLH_NP_SS];
   __shared__ double W[(CLH_MAXB + 2) * CLH_NP_SS];
+  __shared__ double L10[CLH_SX_TILE];
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
   const int kind = w.y, zl = w.z, row0 = w.w;
@@ -328,6 +218,13 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
   const double* o = (kind == CLH_W_SS) ? d.nodes + d.node_ss
                                        : d.nodes + d.node_gs + (size_t)zl * d.node_gs_stride;
   const double* ells = (kind == CLH_W_SS) ? d.ell_ss : d.ell_gx;
+  if (tid < CLH_SX_TILE) {
+    const int r = row0 + tid;
+    L10[tid] = log10((r < d.nrow ? ells[r] : 1.0) + 0.5);
+  }
+  const int nw = (kind == CLH_W_SS) ? d.ns : d.ns + 2;
+  for (int i = tid; i < nw * np; i += CLH_BLOCK) W[i] = o[3 * np + i];
+  __syncthreads();
   {
     const int p = tid & 63, wv = tid >> 6;
     const double a = o[p], fK = o[np + p], fac = o[2 * np + p];
@@ -335,28 +232,28 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
     const int nz = d.nzp;
     const int j = clh_bs(d.pz, nz, z);
     const double dy = (z - d.pz[j]) / (d.pz[j + 1] - d.pz[j]);
+    const double l10fk = log10(fK * CLH_COVERH0);  // log10 k = log10(ell) - log10(chi)
     int i = -1;
+    double inv_k = 0.0;
     constexpr int per = CLH_SX_TILE / 4;
     for (int q = 0; q < per; q++) {
       const int rr = wv * per + q;
       const int r = row0 + rr;
       double s = 0.0;
       if (r < d.nrow) {
-        const double ell = ells[r] + 0.5;
-        const double k = ell / fK;
-        const double log10k = log10(k / CLH_COVERH0);
+        const double log10k = L10[rr] - l10fk;
+        const int i0 = i;
         i = (i < 0) ? clh_bs(d.lk, d.nk, log10k) : clh_walk(d.lk, d.nk, log10k, i);
-        const double dx = (log10k - d.lk[i]) / (d.lk[i + 1] - d.lk[i]);
+        if (i != i0) inv_k = 1.0 / (d.lk[i + 1] - d.lk[i]);
+        const double dx = (log10k - d.lk[i]) * inv_k;
         const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
         const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +
                            dx * dy * P[nz + 1];
-        s = exp(out) / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0) * fac;
+        s = exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac;
       }
       S[rr * np + p] = s;
     }
   }
-  const int nw = (kind == CLH_W_SS) ? d.ns : d.ns + 2;
-  for (int i = tid; i < nw * np; i += CLH_BLOCK) W[i] = o[3 * np + i];
   __syncthreads();
   if (kind == CLH_W_SS) {
     const int nout = d.n_ss * CLH_SX_TILE;
@@ -399,107 +296,32 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
 
 // ----------------------------------------------------------------------------
 // k_limber_gg: gg tiles of CLH_GGT_TILE rows, 192 threads = two groups of 96
-// (one thread per GL node), each group walking half of the rows in increasing
-// ell.  The a_0 half of W_RSD depends only on the node (chi_0 = ell/k = chi),
-// so it is evaluated once per node; the a_1 half and P(k,z) use index walks.
-// Work item (draw, zl, row0, -).
+// (one thread per GL node, ClhGgNode), each group walking half of the rows in
+// increasing ell; per-row constants in LDS.  Work item (draw, zl, row0, -).
 // ----------------------------------------------------------------------------
-#define CLH_GGT_TILE 16
 __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ D,
                                                    const int4* __restrict__ work) {
   __shared__ double V[CLH_GGT_TILE * CLH_NP_GG];
+  __shared__ ClhGgRow R[CLH_GGT_TILE];
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
   const int zl = w.y, row0 = w.z;
-  const int np = CLH_NP_GG;
-  const int p = threadIdx.x % np, half = threadIdx.x / np;
-  const double* o = d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride;
-  const double a = o[p], fK = o[np + p], fac = o[2 * np + p], WG = o[3 * np + p], WM = o[4 * np + p];
-  const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
-  const double z = 1.0 / a - 1.0;
-  const int nz = d.nzp;
-  const int j = clh_bs(d.pz, nz, z);
-  const double dy = (z - d.pz[j]) / (d.pz[j + 1] - d.pz[j]);
-  // a_0 side of W_RSD (radial_weights.c:98-99) at chi_0 = chi(a)
-  double t0;
-  int jc, jz, jg, jn;
-  {
-    const double chi = fK * CLH_COVERH0;
-    jc = clh_bs(d.chi_v, d.chi_n, chi);
-    const double dyc = (chi - d.chi_v[jc]) / (d.chi_v[jc + 1] - d.chi_v[jc]);
-    const double z0 = d.chi_z[jc] + dyc * (d.chi_z[jc + 1] - d.chi_z[jc]);
-    const double a0 = 1.0 / (1.0 + z0);
-    double c0, dc0;
-    clh_chi_all(d, a0, c0, dc0);
-    const double h0 = 1.0 / ((a0 * a0) * dc0);
-    const double zz0 = 1.0 / a0 - 1.0;
-    t0 = clh_nz_lens(d, zz0, zl) * h0 * clh_f_growth(d, zz0);
-    jz = clh_bs(d.chi_z, d.chi_n, zz0);
-    jg = clh_bs(d.G_z, d.G_n, zz0);
-    jn = clh_bs(d.l_zc, d.l_rows, zz0);
+  constexpr int np = CLH_NP_GG;
+  if (threadIdx.x < CLH_GGT_TILE) {
+    const int r = row0 + threadIdx.x;
+    R[threadIdx.x] = clh_gg_row(r < d.nrow ? d.ell_gx[r] : 1.0);
   }
-  int i = -1;
+  __syncthreads();
+  const int p = threadIdx.x % np, half = threadIdx.x / np;
+  ClhGgNode<false> nd;
+  nd.init(d, d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride, p, zl);
   constexpr int per = CLH_GGT_TILE / 2;
   for (int q = 0; q < per; q++) {
     const int rr = half * per + q;
-    const int r = row0 + rr;
     double v = 0.0;
-    if (r < d.nrow) {
-      const double l = d.ell_gx[r], ell = l + 0.5;
-      const double k = ell / fK;
-      const double chi_1 = (ell + 1.0) / k;
-      if (!(chi_1 > d.chi_amin)) {
-        // a_chi(chi_1) (cosmo3D.c:109-133)
-        const double c1v = chi_1 * CLH_COVERH0;
-        jc = clh_walk(d.chi_v, d.chi_n, c1v, jc);
-        const double dyc = (c1v - d.chi_v[jc]) / (d.chi_v[jc + 1] - d.chi_v[jc]);
-        const double z1i = d.chi_z[jc] + dyc * (d.chi_z[jc + 1] - d.chi_z[jc]);
-        const double a1 = 1.0 / (1.0 + z1i);
-        const double z1 = 1.0 / a1 - 1.0;
-        // chi_all(a1) derivative part -> hoverh0
-        jz = clh_walk(d.chi_z, d.chi_n, z1, jz);
-        int jj = jz > d.chi_n - 3 ? d.chi_n - 3 : jz;
-        const double* zz = d.chi_z;
-        const double* cc = d.chi_v;
-        const double dyz = (z1 - zz[jj]) / (zz[jj + 1] - zz[jj]);
-        const double up = (cc[jj + 2] - cc[jj]) / (zz[jj + 2] - zz[jj]);
-        const double down = (jj > 0) ? (cc[jj + 1] - cc[jj - 1]) / (zz[jj + 1] - zz[jj - 1])
-                                     : (cc[jj + 1] - cc[jj]) / (zz[jj + 1] - zz[jj]);
-        const double dchida1 = (down + dyz * (up - down)) / CLH_COVERH0 / (a1 * a1);
-        const double h1 = 1.0 / ((a1 * a1) * dchida1);
-        // f_growth(z1) (cosmo3D.c:200-228)
-        jg = clh_walk(d.G_z, d.G_n, z1, jg);
-        const double dyg = (z1 - d.G_z[jg]) / (d.G_z[jg + 1] - d.G_z[jg]);
-        const double G = d.G_v[jg] + dyg * (d.G_v[jg + 1] - d.G_v[jg]);
-        const double dlnGdlnz = ((d.G_v[jg + 1] - d.G_v[jg]) / (d.G_z[jg + 1] - d.G_z[jg])) * z1 / G;
-        const double f1 = 1 + (-dlnGdlnz * (1 + z1) / z1);
-        // nz_lens(z1) (redshift_spline.c:736-762)
-        double n1 = 0.0;
-        {
-          const double zs = (z1 - d.par->lens_dz[zl] - d.l_zmean[zl]) / d.par->lens_st[zl] + d.l_zmean[zl];
-          const int nr = d.l_rows;
-          if (!(zs <= d.l_zc[0] || zs >= d.l_zc[nr - 1])) {
-            jn = clh_walk(d.l_zc, nr, zs, jn);
-            const int jn2 = jn > nr - 2 ? nr - 2 : jn;
-            const double* cf = d.l_coef + (size_t)zl * 4 * (nr - 1) + 4 * jn2;
-            const double dxs = zs - d.l_zc[jn2];
-            n1 = (cf[0] + dxs * (cf[1] + dxs * (cf[2] + dxs * cf[3]))) / d.par->lens_st[zl];
-          }
-        }
-        const double wr = (1 + 8. * ell) / ((2. * ell + 1.) * (2. * ell + 1.)) * t0 -
-                          4. / (2 * ell + 3.) * sqrt((2 * ell + 1.) / (2 * ell + 3.)) * n1 * h1 * f1;
-        const double pref = l * (l + 1.) / (ell * ell);
-        const double t = WG * b1 + WM * pref * bmag + wr;
-        // p_nonlin(k, a) with the node's z index
-        const double log10k = log10(k / CLH_COVERH0);
-        i = (i < 0) ? clh_bs(d.lk, d.nk, log10k) : clh_walk(d.lk, d.nk, log10k, i);
-        const double dx = (log10k - d.lk[i]) / (d.lk[i + 1] - d.lk[i]);
-        const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
-        const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +
-                           dx * dy * P[nz + 1];
-        const double pk = exp(out) / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
-        v = t * t * pk * fac;
-      }
+    if (row0 + rr < d.nrow) {
+      double t, pf, dwr, dx;
+      if (nd.eval(d, R[rr], t, pf, dwr, dx)) v = t * t * pf;
     }
     V[rr * np + p] = v;
   }
@@ -520,9 +342,8 @@ __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ 
 // xi-, then gamma_t pairs, then w bins.
 // ----------------------------------------------------------------------------
 #define CLH_LEG_TASKS 8
-// raw = 1: uncalibrated values into the draw's `raw` buffer (for the VJP)
 __global__ __launch_bounds__(CLH_BLOCK) void k_legendre(const DrawDesc* __restrict__ D,
-                                                        const int2* __restrict__ work, int raw) {
+                                                        const int2* __restrict__ work) {
   extern __shared__ double colbuf[];  // [CLH_LEG_TASKS][nrow]
   const int2 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
@@ -560,15 +381,15 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre(const DrawDesc* __restri
   if (kind <= 1) {
     int i, j;
     clh_ss_pair(d.ns, q, i, j);
-    if (d.apply_calib && !raw) f = (1.0 + d.par->shear_m[i]) * (1.0 + d.par->shear_m[j]);
+    if (d.apply_calib) f = (1.0 + d.par->shear_m[i]) * (1.0 + d.par->shear_m[j]);
     idx = d.dv_start[0] + nt * q + th + (kind == 1 ? nt * d.n_ss : 0);
   } else if (kind == 2) {
-    if (d.apply_calib && !raw) f = 1.0 + d.par->shear_m[q % d.ns];
+    if (d.apply_calib) f = 1.0 + d.par->shear_m[q % d.ns];
     idx = d.dv_start[1] + nt * q + th;
   } else {
     idx = d.dv_start[2] + nt * q + th;
   }
-  (raw ? d.raw : d.dv)[idx] = acc * f;
+  d.dv[idx] = acc * f;
 }
 
 // ----------------------------------------------------------------------------
@@ -644,16 +465,11 @@ hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStrea
   hipLaunchKernelGGL(k_nodes, dim3(n), dim3(128), 0, s, D, work);
   return hipGetLastError();
 }
-hipError_t clh_launch_limber(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_limber, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
-  return hipGetLastError();
-}
 hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow,
-                               hipStream_t s, int raw) {
+                               hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * CLH_LEG_TASKS * (size_t)max_nrow;
-  hipLaunchKernelGGL(k_legendre, dim3(n), dim3(CLH_BLOCK), lds, s, D, work, raw);
+  hipLaunchKernelGGL(k_legendre, dim3(n), dim3(CLH_BLOCK), lds, s, D, work);
   return hipGetLastError();
 }
 hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, int max_nrow,

@@ -13,8 +13,6 @@
 #include "clh_internal.hpp"
 
 #define CLH_BLOCK 256
-#define CLH_SX_TILE 16
-#define CLH_GGT_TILE 16
 
 namespace {
 
