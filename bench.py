@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--vjp", type=int, default=1, help="also time the batched VJP")
+    ap.add_argument("--ragged", type=int, default=512,
+                    help="draws of the multi-layout leg (3-10 src x 2-6 lens bins, 20-30 theta bins; 0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the draws of a batch "
@@ -209,6 +211,47 @@ def main():
     out = dv.view(args.batch, n)
     assert torch.isfinite(out).all().item()
 
+    # multi-layout batch: every draw with its own cosmology and its own survey
+    # layout, all in one captured graph of per-stage launches
+    ragged = None
+    if args.ragged > 0:
+        from cosmolike_amd import synth
+        t1 = time.perf_counter()
+        spec = synth.random_draws(args.ragged, seed=args.seed + 1, ragged=True)
+        probs = [synth.ragged_3x2pt_problem(cs, ns, nl, nt) for cs, ns, nl, nt in spec]
+        cr = Cosmolike(device=0)
+        cache, lays = {}, []
+        for pr in probs:
+            key = (pr.ns, pr.nl, pr.ntheta)
+            if key not in cache:
+                cr.load(pr, mode)
+                cache[key] = cr.capture_layout()
+            lays.append(cache[key])
+        cr.set_precision(args.precision)
+        cr.batch_upload_layouts(probs, lays)
+        ntot = cr.batch_offsets()[-1]
+        log(f"bench: ragged batch of {len(probs)} draws, {len(cache)} layouts, {ntot} outputs, "
+            f"set up in {time.perf_counter() - t1:.1f}s")
+        rdv = torch.empty(ntot, dtype=torch.float64, device="cuda:0")
+        cr.batch_use_graph(True)
+        for _ in range(args.warmup):
+            cr.batch_forward(rdv.data_ptr())
+        cr.synchronize()
+        torch.cuda.synchronize()
+        nr = max(5, args.steps // 4)
+        t1 = time.perf_counter()
+        for _ in range(nr):
+            cr.batch_forward(rdv.data_ptr())
+        cr.synchronize()
+        torch.cuda.synchronize()
+        elr = time.perf_counter() - t1
+        assert torch.isfinite(rdv).all().item()
+        ragged = {"value": len(probs) * nr / elr, "unit": "data vectors/s", "draws": len(probs),
+                  "layouts": len(cache), "ms_per_step": 1e3 * elr / nr, "graph": True,
+                  "precision": args.precision}
+        log(f"bench: ragged {ragged['value']:.1f} dv/s")
+        del cr, rdv
+
     # the plain FP64 drop-in path: one draw at a time through the setters and
     # clh_compute_3x2pt_real (host buffers in and out), as the reference's
     # interface calls it
@@ -247,6 +290,7 @@ def main():
         "vjp_value": vjp_value, "vjp_unit": "VJPs/s (full data vector -> lnP, dz_src, dz_lens, b1, A_IA, m)",
         "fwd_plus_vjp_value": (1.0 / (1.0 / value + 1.0 / vjp_value)) if vjp_value else None,
         "plain_fp64_single_draw": plain,
+        "ragged_batch": ragged,
         "speedup_vs_plain_path": {"forward": value / plain["value"],
                                   "forward_plus_vjp": ((1.0 / (1.0 / value + 1.0 / vjp_value)) / plain["value"])
                                   if vjp_value else None},

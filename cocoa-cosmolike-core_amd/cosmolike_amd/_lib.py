@@ -57,7 +57,6 @@ def lib():
             "clh_last_error": (ctypes.c_char_p, [VP]),
             "clh_default": (VP, []),
             "clh_set_legendre_mode": (i32, [VP, i32]),
-            "clh_set_precision": (i32, [VP, i32]),
             "clh_init_ntable": (i32, [VP, i32, i32, i32, i32]),
             "clh_init_binning_real_space": (i32, [VP, i32, f64, f64]),
             "clh_init_ia": (i32, [VP, i32, i32]),
@@ -82,6 +81,12 @@ def lib():
             "clh_cl_tables": (i32, [VP, D, D, D]),
             "clh_cl_nointerp": (i32, [VP, i32, D, i32, D]),
             "clh_batch_upload": (i32, [VP, i32, ctypes.POINTER(clh_draw)]),
+            "clh_layout_capture": (i32, [VP, ctypes.POINTER(VP)]),
+            "clh_layout_destroy": (None, [VP]),
+            "clh_layout_datavector_size": (i32, [VP]),
+            "clh_batch_upload_layouts": (i32, [VP, i32, ctypes.POINTER(clh_draw), ctypes.POINTER(VP)]),
+            "clh_batch_datavector_offset": (ctypes.c_longlong, [VP, i32]),
+            "clh_set_precision": (i32, [VP, i32]),
             "clh_batch_forward": (i32, [VP, I, VP]),
             "clh_synchronize": (i32, [VP]),
             "clh_set_stream": (i32, [VP, VP]),

@@ -34,6 +34,25 @@ LEGENDRE_FULL = 1
 PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 
+class Layout:
+    """A captured survey layout (owns a clh_layout handle)."""
+
+    def __init__(self, lib, handle):
+        self.L = lib
+        self.h = handle
+
+    def datavector_size(self):
+        return self.L.clh_layout_datavector_size(self.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.clh_layout_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
 class Cosmolike:
     """One library context: survey layout + the current draw."""
 
@@ -222,8 +241,8 @@ class Cosmolike:
         return out
 
     # ------------------------------------------------------------- batches
-    def batch_upload(self, problems):
-        """Upload a list of synth.Problem draws (same layout as the context)."""
+    @staticmethod
+    def _draw_array(problems):
         arr = (clh_draw * len(problems))()
         keep = []
         for i, p in enumerate(problems):
@@ -242,8 +261,30 @@ class Cosmolike:
             d.lens_dz, d.lens_stretch = _p(vals["lens_dz"]), _p(vals["lens_stretch"])
             d.b1, d.bmag = _p(vals["b1"]), _p(vals["bmag"])
             d.ia_A, d.ia_eta = p.ia_A, p.ia_eta
+        return arr, keep
+
+    def batch_upload(self, problems):
+        """Upload a list of synth.Problem draws (same layout as the context)."""
+        arr, keep = self._draw_array(problems)
         self._chk(self.L.clh_batch_upload(self.h, len(problems), arr))
         self.batch_size = len(problems)
+
+    def capture_layout(self):
+        """Snapshot of the current survey layout (clh_layout_capture)."""
+        h = ctypes.c_void_p()
+        self._chk(self.L.clh_layout_capture(self.h, ctypes.byref(h)))
+        return Layout(self.L, h)
+
+    def batch_upload_layouts(self, problems, layouts):
+        """Upload draws with per-draw captured layouts (forward only)."""
+        arr, keep = self._draw_array(problems)
+        hs = (ctypes.c_void_p * len(layouts))(*[l.h.value for l in layouts])
+        self._chk(self.L.clh_batch_upload_layouts(self.h, len(problems), arr, hs))
+        self.batch_size = len(problems)
+
+    def batch_offsets(self):
+        """Data-vector offsets of the uploaded draws (length B+1)."""
+        return [int(self.L.clh_batch_datavector_offset(self.h, i)) for i in range(self.batch_size + 1)]
 
     def batch_forward(self, dv_device_ptr: int, ord=(0, 1, 2)):
         o = (ctypes.c_int * 3)(*ord)

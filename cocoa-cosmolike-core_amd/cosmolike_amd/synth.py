@@ -259,6 +259,18 @@ def full_3x2pt_problem(cosmo: Cosmology | None = None, n_ell: int = 500) -> Prob
     return p.fill_defaults()
 
 
+def ragged_3x2pt_problem(cosmo: Cosmology, ns: int, nl: int, ntheta: int, n_ell: int = 500) -> Problem:
+    """3x2pt layout with ns Smail source bins (z0=0.6), nl Gaussian lens bins
+    centred 0.3-0.9 (sigma_z=0.05) and ntheta theta bins 2.5'-900'."""
+    z, chi, gz, G, log10k, pz, lnP = _cosmo_tables(cosmo)
+    src = source_table(ns, 0.6)
+    lens = lens_table(np.linspace(0.3, 0.9, nl), 0.05)
+    p = Problem(cosmo.omega_m, z, chi, gz, G, log10k, pz, lnP, src, lens, ntheta, 2.5, 900.0,
+                n_ell=n_ell, h=cosmo.h)
+    p.b1 = np.linspace(1.2, 2.0, nl)
+    return p.fill_defaults()
+
+
 def random_draws(n: int, seed: int = 20261203, ragged: bool = True):
     """Cosmology/nuisance draws; ragged layouts when ``ragged``."""
     rng = np.random.default_rng(seed)

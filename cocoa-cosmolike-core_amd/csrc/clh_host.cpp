@@ -8,6 +8,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -174,9 +178,30 @@ struct DBuf {  // device buffer that grows
   void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
 };
 
+struct DevLayout {  // device copy of one survey layout
+  bool valid = false;
+  DBuf<double> buf;
+  DBuf<float> k32, ksc;  // folded kernels packed for the MFMA contraction
+  DBuf<uint16_t> kbf;
+  int ngrp = 0;
+  size_t off_s_zc = 0, off_s_coef = 0, off_l_zc = 0, off_l_coef = 0, off_ell_ss = 0,
+         off_ell_gx = 0, off_kfold = 0, off_gl = 0, off_kfull = 0, off_fill = 0;
+  void release() { buf.release(); k32.release(); ksc.release(); kbf.release(); valid = false; }
+};
+
+struct LayoutCopy {  // a context's own copy of a captured layout, with its device copy
+  Layout L;
+  DevLayout D;
+  uint64_t epoch = 0;
+};
+
 struct Batch {
   int B = 0;
-  int max_nrow = 0;
+  int max_nrow = 0, max_npair = 0;
+  bool ctx_layout = true;     // every draw uses the context's own layout
+  bool uniform = true;        // every draw has the same layout
+  std::vector<const Layout*> lays;
+  std::vector<long long> dv_off;  // [B+1] data-vector offsets of the draws
   std::vector<DrawDesc> desc;
   std::vector<DrawParams> par;
   DBuf<DrawDesc> d_desc;
@@ -211,13 +236,10 @@ struct clh_ctx {
   dvec cache_raw;   // raw (uncalibrated, ord 0,1,2) data vector of `cur`
   dvec cache_cl;    // cl rows of `cur`
   // device copies of the layout
-  bool dev_layout_valid = false;
-  DBuf<double> d_layout;
-  DBuf<float> d_kmx32, d_kmxsc;   // folded kernels packed for the MFMA contraction
-  DBuf<uint16_t> d_kmxbf;
-  int kmx_ngrp = 0;
-  size_t off_s_zc = 0, off_s_coef = 0, off_l_zc = 0, off_l_coef = 0, off_ell_ss = 0,
-         off_ell_gx = 0, off_kfold = 0, off_gl = 0, off_kfull = 0, off_fill = 0;
+  DevLayout dl;                                        // device copy of L
+  std::map<uint64_t, std::unique_ptr<LayoutCopy>> rl;  // layouts of a multi-layout batch
+  uint64_t mode_epoch = 0;                             // bumped when device layouts go stale
+  uint64_t batch_epoch = 0;
   Batch single, batch, scratch;
   bool timing = false;
   bool use_graph = false;             // replay the batch forward from a hipGraph
@@ -353,7 +375,51 @@ static void lens_zmean(Layout& L) {
 // xi_pm_tomo (cosmo2D.c:430-453), w_gammat_tomo (:631-639) and w_gg_tomo
 // (:808-815); then the fold of the linear-in-ln(ell) fill (cosmo2D.c:1800-1807)
 // into the kernels.
+struct KernelKey {
+  int nt, n_ell, lmax;
+  double vtmin, vtmax;
+  bool operator<(const KernelKey& o) const {
+    if (nt != o.nt) return nt < o.nt;
+    if (n_ell != o.n_ell) return n_ell < o.n_ell;
+    if (lmax != o.lmax) return lmax < o.lmax;
+    if (vtmin != o.vtmin) return vtmin < o.vtmin;
+    return vtmax < o.vtmax;
+  }
+};
+struct KernelSet {
+  dvec kfold, ell_ss, ell_gx;
+};
+// folded kernels depend only on the binning and Ntable: shared process-wide
+static std::map<KernelKey, std::shared_ptr<const KernelSet>> g_kernel_cache;
+static std::mutex g_kernel_mu;
+
+static void build_kernels_uncached(Layout& L, bool full);
+
 static void build_kernels(Layout& L, bool full) {
+  const KernelKey key{L.ntheta, L.n_ell, L.lmax, L.vtmin, L.vtmax};
+  if (!full) {
+    std::lock_guard<std::mutex> lk(g_kernel_mu);
+    auto it = g_kernel_cache.find(key);
+    if (it != g_kernel_cache.end()) {
+      L.kfold = it->second->kfold;
+      L.ell_ss = it->second->ell_ss;
+      L.ell_gx = it->second->ell_gx;
+      L.kfull.clear();
+      L.kern_valid = true;
+      return;
+    }
+  }
+  build_kernels_uncached(L, full);
+  std::lock_guard<std::mutex> lk(g_kernel_mu);
+  if (g_kernel_cache.size() >= 64) g_kernel_cache.clear();
+  auto ks = std::make_shared<KernelSet>();
+  ks->kfold = L.kfold;
+  ks->ell_ss = L.ell_ss;
+  ks->ell_gx = L.ell_gx;
+  g_kernel_cache[key] = ks;
+}
+
+static void build_kernels_uncached(Layout& L, bool full) {
   const int nt = L.ntheta, Lm = L.lmax, nrow = L.nrow();
   L.kfold.assign((size_t)4 * nrow * nt, 0.0);
   if (full) L.kfull.assign((size_t)4 * Lm * nt, 0.0); else L.kfull.clear();
@@ -461,8 +527,7 @@ static void host_chi_all(const Draw& d, double a, double& chi, double& dchida) {
   dchida = (down + dy * (up - down)) / CLH_COVERH0 / (a * a);
 }
 
-static int check_draw(clh_ctx* c, const Draw& d, int idx) {
-  const Layout& L = c->L;
+static int check_draw(clh_ctx* c, const Layout& L, const Draw& d, int idx) {
   const int nc = (int)d.chi_z.size();
   if (nc < 5 || d.chi.size() != (size_t)nc) return fail(c, CLH_ERR_STATE, "draw %d: distances not set", idx);
   if (d.G_z.size() < 2 || d.G.size() != d.G_z.size()) return fail(c, CLH_ERR_STATE, "draw %d: growth not set", idx);
@@ -511,24 +576,24 @@ static double bf16_value(uint16_t h) {
   return f;
 }
 
-static int upload_layout(clh_ctx* c) {
-  Layout& L = c->L;
+static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   const bool full = c->legendre_mode == CLH_LEGENDRE_FULL;
-  if (!L.kern_valid || (full && L.kfull.empty())) build_kernels(L, full);
+  if (!L.kern_valid || (full && L.kfull.empty()))
+    return fail(c, CLH_ERR_STATE, "layout kernels not built for the Legendre mode");
   double glbuf[2 * CLH_NP_SS + 2 * CLH_NP_GG];
   gauss_legendre(CLH_NP_SS, glbuf, glbuf + CLH_NP_SS);
   gauss_legendre(CLH_NP_GG, glbuf + 2 * CLH_NP_SS, glbuf + 2 * CLH_NP_SS + CLH_NP_GG);
   size_t off = 0;
   auto take = [&](size_t n) { size_t o = off; off += (n + 31) & ~size_t(31); return o; };
-  c->off_s_zc = take(L.src.zc.size());
-  c->off_s_coef = take(L.src.coef.size());
-  c->off_l_zc = take(L.lens.zc.size());
-  c->off_l_coef = take(L.lens.coef.size());
-  c->off_ell_ss = take(L.ell_ss.size());
-  c->off_ell_gx = take(L.ell_gx.size());
-  c->off_kfold = take(L.kfold.size());
-  c->off_gl = take(sizeof glbuf / sizeof(double));
-  c->off_kfull = take(full ? L.kfull.size() : 0);
+  D.off_s_zc = take(L.src.zc.size());
+  D.off_s_coef = take(L.src.coef.size());
+  D.off_l_zc = take(L.lens.zc.size());
+  D.off_l_coef = take(L.lens.coef.size());
+  D.off_ell_ss = take(L.ell_ss.size());
+  D.off_ell_gx = take(L.ell_gx.size());
+  D.off_kfold = take(L.kfold.size());
+  D.off_gl = take(sizeof glbuf / sizeof(double));
+  D.off_kfull = take(full ? L.kfull.size() : 0);
   // FULL mode: index and weight of the ln-ell fill at every integer ell
   // (C_*_tomo_limber_fill, cosmo2D.c:1800-1807)
   dvec fill;
@@ -548,23 +613,23 @@ static int upload_layout(clh_ctx* c) {
       }
     }
   }
-  c->off_fill = take(fill.size());
-  HIPCHK(c, c->d_layout.ensure(off));
-  double* base = c->d_layout.p;
+  D.off_fill = take(fill.size());
+  HIPCHK(c, D.buf.ensure(off));
+  double* base = D.buf.p;
   auto put = [&](size_t o, const double* src, size_t n) -> hipError_t {
     if (n == 0) return hipSuccess;
     return hipMemcpyAsync(base + o, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
   };
-  HIPCHK(c, put(c->off_s_zc, L.src.zc.data(), L.src.zc.size()));
-  HIPCHK(c, put(c->off_s_coef, L.src.coef.data(), L.src.coef.size()));
-  HIPCHK(c, put(c->off_l_zc, L.lens.zc.data(), L.lens.zc.size()));
-  HIPCHK(c, put(c->off_l_coef, L.lens.coef.data(), L.lens.coef.size()));
-  HIPCHK(c, put(c->off_ell_ss, L.ell_ss.data(), L.ell_ss.size()));
-  HIPCHK(c, put(c->off_ell_gx, L.ell_gx.data(), L.ell_gx.size()));
-  HIPCHK(c, put(c->off_kfold, L.kfold.data(), L.kfold.size()));
-  HIPCHK(c, put(c->off_gl, glbuf, sizeof glbuf / sizeof(double)));
-  if (full) HIPCHK(c, put(c->off_kfull, L.kfull.data(), L.kfull.size()));
-  if (full) HIPCHK(c, put(c->off_fill, fill.data(), fill.size()));
+  HIPCHK(c, put(D.off_s_zc, L.src.zc.data(), L.src.zc.size()));
+  HIPCHK(c, put(D.off_s_coef, L.src.coef.data(), L.src.coef.size()));
+  HIPCHK(c, put(D.off_l_zc, L.lens.zc.data(), L.lens.zc.size()));
+  HIPCHK(c, put(D.off_l_coef, L.lens.coef.data(), L.lens.coef.size()));
+  HIPCHK(c, put(D.off_ell_ss, L.ell_ss.data(), L.ell_ss.size()));
+  HIPCHK(c, put(D.off_ell_gx, L.ell_gx.data(), L.ell_gx.size()));
+  HIPCHK(c, put(D.off_kfold, L.kfold.data(), L.kfold.size()));
+  HIPCHK(c, put(D.off_gl, glbuf, sizeof glbuf / sizeof(double)));
+  if (full) HIPCHK(c, put(D.off_kfull, L.kfull.data(), L.kfull.size()));
+  if (full) HIPCHK(c, put(D.off_fill, fill.data(), fill.size()));
   // Folded kernels as MFMA A operands (k_legendre_mx): group g of 8 rows,
   // lane l = theta + 32 h, item t <-> row 8 g + 4 h + t.  The BF16 parts are
   // of the kernel scaled by 2^-e per 32-row band (e from the band's largest
@@ -596,20 +661,19 @@ static int upload_layout(clh_ctx* c) {
       }
     }
   }
-  HIPCHK(c, c->d_kmx32.ensure(k32.size()));
-  HIPCHK(c, c->d_kmxbf.ensure(kbf.size()));
-  HIPCHK(c, c->d_kmxsc.ensure(ksc.size()));
-  HIPCHK(c, hipMemcpyAsync(c->d_kmx32.p, k32.data(), k32.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->d_kmxbf.p, kbf.data(), kbf.size() * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->d_kmxsc.p, ksc.data(), ksc.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  c->kmx_ngrp = ngrp;
+  HIPCHK(c, D.k32.ensure(k32.size()));
+  HIPCHK(c, D.kbf.ensure(kbf.size()));
+  HIPCHK(c, D.ksc.ensure(ksc.size()));
+  HIPCHK(c, hipMemcpyAsync(D.k32.p, k32.data(), k32.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(D.kbf.p, kbf.data(), kbf.size() * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(D.ksc.p, ksc.data(), ksc.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  D.ngrp = ngrp;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->dev_layout_valid = true;
+  D.valid = true;
   return CLH_OK;
 }
 
-static int check_layout(clh_ctx* c) {
-  const Layout& L = c->L;
+static int check_layout(clh_ctx* c, const Layout& L) {
   if (!L.src.set) return fail(c, CLH_ERR_STATE, "source sample not set");
   if (L.ntheta <= 0) return fail(c, CLH_ERR_STATE, "Ntable.Ntheta not initialized");
   if (L.ntheta > 32) return fail(c, CLH_ERR_ARG, "Ntheta=%d > 32 not supported", L.ntheta);
@@ -617,67 +681,73 @@ static int check_layout(clh_ctx* c) {
   return CLH_OK;
 }
 
-// Builds the per-draw descriptors, device tables and work lists of a batch.
-// `ells`: if non-null, nrow custom ell rows shared by all probes (the direct
-// *_nointerp evaluation); otherwise the layout's rows.
-static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& draws, int apply_calib,
-                       const int ord[3], const dvec* ells, double* dv_out) {
-  int rc = check_layout(c);
-  if (rc) return rc;
-  if ((rc = ensure_device(c))) return rc;
-  HIPCHK(c, hipSetDevice(c->device));
-  const Layout& L = c->L;
-  const int B = (int)draws.size();
-  for (int i = 0; i < B; i++)
-    if ((rc = check_draw(c, *draws[i], i))) return rc;
-  if (!c->dev_layout_valid && (rc = upload_layout(c))) return rc;
-  const int ns = L.src.nbin, nl = L.lens.nbin;
-  const int nrow = ells ? (int)ells->size() : L.nrow();
-  const int n_ss = L.n_ss(), n_gs = L.n_gs();
-  const int ncol = n_ss + n_gs + nl;
+// data-vector block starts of compute_data_vector_Mx2pt_N_starts
+// (generic_interface.hpp:770-791) for a layout and a probe order
+static void dv_starts(const Layout& L, const int ord[3], int start[3]) {
   const int nt = L.ntheta;
-  // data-vector block starts: compute_data_vector_Mx2pt_N_starts (generic_interface.hpp:770-791)
-  const int sizes[3] = {2 * nt * n_ss, nt * n_gs, nt * nl};
+  const int sizes[3] = {2 * nt * L.n_ss(), nt * L.n_gs(), nt * L.lens.nbin};
   int idx[3] = {0, 1, 2};
   for (int i = 1; i < 3; i++)
     for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) std::swap(idx[j], idx[j - 1]);
-  int start[3] = {0, 0, 0};
+  for (int i = 0; i < 3; i++) start[i] = 0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
-  const int dvsz = sizes[0] + sizes[1] + sizes[2];
-  // node layout
-  const int node_ss = 0;
-  const int ss_sz = CLH_NP_SS * (3 + ns);
-  const int gs_stride = CLH_NP_SS * (5 + ns);
-  const int gg_stride = CLH_NP_GG * 5;
-  const int node_gs = ss_sz;
-  const int node_gg = node_gs + nl * gs_stride;
-  const int node_sz = node_gg + nl * gg_stride;
-  const size_t gtab_sz = (size_t)(ns + nl) * L.n_a;
-  const size_t cl_sz = (size_t)ncol * nrow;
-  const size_t work_per = ((gtab_sz + 31) & ~size_t(31)) + ((node_sz + 31) & ~size_t(31)) + ((cl_sz + 31) & ~size_t(31));
-  // per-draw tables
-  std::vector<size_t> toff(B);
-  size_t ttot = 0;
+}
+
+// Builds the per-draw descriptors, device tables and work lists of a batch;
+// draw i uses layout lays[i] with device copy devs[i].  `ells`: if non-null,
+// nrow custom ell rows shared by all probes (the direct *_nointerp
+// evaluation); otherwise the layout's rows.
+static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& draws,
+                       const std::vector<const Layout*>& lays, const std::vector<const DevLayout*>& devs,
+                       int apply_calib, const int ord[3], const dvec* ells, double* dv_out) {
+  int rc;
+  if ((rc = ensure_device(c))) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int B = (int)draws.size();
   for (int i = 0; i < B; i++) {
+    if ((rc = check_layout(c, *lays[i]))) return rc;
+    if ((rc = check_draw(c, *lays[i], *draws[i], i))) return rc;
+  }
+  auto al = [](size_t n) { return (n + 31) & ~size_t(31); };
+  // per-draw sizes and offsets (layouts may differ between draws)
+  std::vector<size_t> woff(B + 1, 0), toff(B + 1, 0);
+  std::vector<long long> dvoff(B + 1, 0);
+  int max_nrow = 0, max_npair = 0;
+  bool uniform = true;
+  for (int i = 0; i < B; i++) {
+    const Layout& L = *lays[i];
     const Draw& d = *draws[i];
-    toff[i] = ttot;
-    ttot += 2 * d.chi_z.size() + 2 * d.G_z.size() + d.lk.size() + d.pz.size() + d.lnP.size() + 32;
+    const int ns = L.src.nbin, nl = L.lens.nbin;
+    const int nrow = ells ? (int)ells->size() : L.nrow();
+    const size_t node_sz = (size_t)CLH_NP_SS * (3 + ns) + (size_t)nl * (CLH_NP_SS * (5 + ns) + CLH_NP_GG * 5);
+    woff[i + 1] = woff[i] + al((size_t)(ns + nl) * L.n_a) + al(node_sz) +
+                  al((size_t)(L.n_ss() + L.n_gs() + nl) * nrow);
+    toff[i + 1] = toff[i] + 2 * d.chi_z.size() + 2 * d.G_z.size() + d.lk.size() + d.pz.size() + d.lnP.size() + 32;
+    dvoff[i + 1] = dvoff[i] + L.dv_size();
+    max_nrow = std::max(max_nrow, nrow);
+    max_npair = std::max(max_npair, std::max(L.n_ss(), ns));
+    if (lays[i] != lays[0]) uniform = false;
   }
-  HIPCHK(c, bt.d_tab.ensure(ttot));
-  HIPCHK(c, bt.d_work.ensure(work_per * B));
-  if (!dv_out) HIPCHK(c, bt.d_dv.ensure((size_t)dvsz * B));
+  HIPCHK(c, bt.d_tab.ensure(toff[B]));
+  HIPCHK(c, bt.d_work.ensure(woff[B]));
+  if (!dv_out) HIPCHK(c, bt.d_dv.ensure((size_t)dvoff[B]));
   if (ells) {
-    HIPCHK(c, bt.d_ells.ensure(nrow));
-    HIPCHK(c, hipMemcpyAsync(bt.d_ells.p, ells->data(), nrow * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, bt.d_ells.ensure(ells->size()));
+    HIPCHK(c, hipMemcpyAsync(bt.d_ells.p, ells->data(), ells->size() * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
   }
-  std::vector<double> htab(ttot);
+  std::vector<double> htab(toff[B]);
   bt.desc.assign(B, DrawDesc());
   bt.par.assign(B, DrawParams());
   HIPCHK(c, bt.d_par.ensure(B));
-  const double* lay = c->d_layout.p;
   for (int i = 0; i < B; i++) {
     const Draw& d = *draws[i];
+    const Layout& L = *lays[i];
+    const DevLayout& DL = *devs[i];
+    const double* lay = DL.buf.p;
+    const int ns = L.src.nbin, nl = L.lens.nbin;
+    const int nrow = ells ? (int)ells->size() : L.nrow();
     DrawDesc& q = bt.desc[i];
     std::memset(&q, 0, sizeof q);
     size_t o = toff[i];
@@ -688,10 +758,10 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.chi_n = (int)d.chi_z.size(); q.G_n = (int)d.G_z.size();
     q.nk = (int)d.lk.size(); q.nzp = (int)d.pz.size();
     q.omega_m = d.omega_m;
-    q.ns = ns; q.nl = nl; q.ntheta = nt; q.n_ell = L.n_ell; q.n_a = L.n_a; q.nrow = nrow;
-    q.n_ss = n_ss; q.n_gs = n_gs; q.lmax = L.lmax;
-    q.s_zc = lay + c->off_s_zc; q.s_coef = lay + c->off_s_coef; q.s_rows = L.src.rows;
-    q.l_zc = lay + c->off_l_zc; q.l_coef = lay + c->off_l_coef; q.l_rows = L.lens.rows;
+    q.ns = ns; q.nl = nl; q.ntheta = L.ntheta; q.n_ell = L.n_ell; q.n_a = L.n_a; q.nrow = nrow;
+    q.n_ss = L.n_ss(); q.n_gs = L.n_gs(); q.lmax = L.lmax;
+    q.s_zc = lay + DL.off_s_zc; q.s_coef = lay + DL.off_s_coef; q.s_rows = L.src.rows;
+    q.l_zc = lay + DL.off_l_zc; q.l_coef = lay + DL.off_l_coef; q.l_rows = L.lens.rows;
     q.s_zmin_all = L.src.zmin_all; q.s_zmax_all = L.src.zmax_all;
     q.l_zmin_all = L.lens.zmin_all; q.l_zmax_all = L.lens.zmax_all;
     DrawParams& pp = bt.par[i];
@@ -703,22 +773,27 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     pp.ia_A = d.ia_A; pp.ia_eta = d.ia_eta;
     q.par = bt.d_par.p + i;
     q.ia_mode = L.ia_mode;
-    q.ell_ss = ells ? bt.d_ells.p : lay + c->off_ell_ss;
-    q.ell_gx = ells ? bt.d_ells.p : lay + c->off_ell_gx;
-    q.kfold = lay + c->off_kfold;
-    q.kfull = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + c->off_kfull : nullptr;
-    q.fill = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + c->off_fill : nullptr;
-    q.gl = lay + c->off_gl;
-    q.kmx32 = c->d_kmx32.p; q.kmxbf = c->d_kmxbf.p; q.kmxsc = c->d_kmxsc.p;
-    q.kmx_ngrp = c->kmx_ngrp;
-    double* wb = bt.d_work.p + work_per * i;
+    q.ell_ss = ells ? bt.d_ells.p : lay + DL.off_ell_ss;
+    q.ell_gx = ells ? bt.d_ells.p : lay + DL.off_ell_gx;
+    q.kfold = lay + DL.off_kfold;
+    q.kfull = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + DL.off_kfull : nullptr;
+    q.fill = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + DL.off_fill : nullptr;
+    q.gl = lay + DL.off_gl;
+    q.kmx32 = DL.k32.p; q.kmxbf = DL.kbf.p; q.kmxsc = DL.ksc.p;
+    q.kmx_ngrp = DL.ngrp;
+    const size_t gtab_sz = (size_t)(ns + nl) * L.n_a;
+    const size_t node_sz = (size_t)CLH_NP_SS * (3 + ns) + (size_t)nl * (CLH_NP_SS * (5 + ns) + CLH_NP_GG * 5);
+    double* wb = bt.d_work.p + woff[i];
     q.gtab = wb;
-    q.nodes = wb + ((gtab_sz + 31) & ~size_t(31));
-    q.cl = q.nodes + ((node_sz + 31) & ~size_t(31));
-    q.dv = dv_out ? dv_out + (size_t)dvsz * i : bt.d_dv.p + (size_t)dvsz * i;
-    for (int k = 0; k < 3; k++) q.dv_start[k] = start[k];
-    q.node_ss = node_ss; q.node_gs = node_gs; q.node_gg = node_gg;
-    q.node_gs_stride = gs_stride; q.node_gg_stride = gg_stride;
+    q.nodes = wb + al(gtab_sz);
+    q.cl = q.nodes + al(node_sz);
+    q.dv = dv_out ? dv_out + dvoff[i] : bt.d_dv.p + dvoff[i];
+    dv_starts(L, ord, q.dv_start);
+    q.node_ss = 0;
+    q.node_gs = CLH_NP_SS * (3 + ns);
+    q.node_gs_stride = CLH_NP_SS * (5 + ns);
+    q.node_gg_stride = CLH_NP_GG * 5;
+    q.node_gg = q.node_gs + nl * q.node_gs_stride;
     double chi_amin, dd;
     host_chi_all(d, CLH_A_MIN, chi_amin, dd);
     q.chi_amin = chi_amin;
@@ -729,7 +804,7 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     }
     q.apply_calib = apply_calib;
   }
-  HIPCHK(c, hipMemcpyAsync(bt.d_tab.p, htab.data(), ttot * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(bt.d_tab.p, htab.data(), toff[B] * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, bt.d_desc.ensure(B));
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(bt.d_par.p, bt.par.data(), B * sizeof(DrawParams), hipMemcpyHostToDevice, c->stream));
@@ -737,68 +812,81 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   for (int i = 1; i < B; i++)
     if ((int)draws[i]->lnP.size() != bt.nknz || draws[i]->pz.size() != draws[0]->pz.size()) bt.nknz = 0;
   for (int k = 0; k < 3; k++) bt.ord[k] = ord[k];
-  // work lists
-  std::vector<int2> wg, wn, wl2;
-  std::vector<int4> wl;
+  // work lists (one entry per block; every stage is one launch for the whole batch)
+  std::vector<int2> wg, wn, wl2, wa;
+  std::vector<int4> wl, wsx, wgg, wmx;
   for (int i = 0; i < B; i++) {
+    const Layout& L = *lays[i];
+    const int ns = L.src.nbin, nl = L.lens.nbin;
+    const int nrow = ells ? (int)ells->size() : L.nrow();
+    const int n_ss = L.n_ss(), n_gs = L.n_gs(), ncol = n_ss + n_gs + nl;
     for (int b = 0; b < ns + nl; b++) wg.push_back(make_int2(i, b));
-    for (int s = 0; s <= 2 * nl; s++) wn.push_back(make_int2(i, s));
-  }
-  for (int i = 0; i < B; i++) {  // adjoint ss/gs tiles of 32 rows, ss (heaviest) first
+    for (int sg = 0; sg <= 2 * nl; sg++) wn.push_back(make_int2(i, sg));
+    // adjoint ss/gs tiles, ss (heaviest) first
     for (int r = 0; r < nrow; r += CLH_ADJ_ROWS) wl.push_back(make_int4(i, CLH_W_SS, 0, r));
     for (int zl = 0; zl < nl; zl++)
       for (int r = 0; r < nrow; r += CLH_ADJ_ROWS) wl.push_back(make_int4(i, CLH_W_GS, zl, r));
-  }
-  const int ntask = 2 * n_ss + n_gs + nl;
-  for (int i = 0; i < B; i++)
+    const int ntask = 2 * n_ss + n_gs + nl;
     for (int t = 0; t < ntask; t += 8) wl2.push_back(make_int2(i, t));
-  // forward Limber tiles: gg (16 rows, heaviest, first) and ss/gs (16 rows)
-  std::vector<int4> wsx, wgg;
-  for (int i = 0; i < B; i++)
-    for (int zl = 0; zl < nl; zl++)
-      for (int r = 0; r < nrow; r += CLH_GGT_TILE) wgg.push_back(make_int4(i, zl, r, 0));
-  for (int i = 0; i < B; i++) {
     for (int zl = 0; zl < nl; zl++)
       for (int r = 0; r < nrow; r += CLH_SX_TILE) wsx.push_back(make_int4(i, CLH_W_GS, zl, r));
     for (int r = 0; r < nrow; r += CLH_SX_TILE) wsx.push_back(make_int4(i, CLH_W_SS, 0, r));
-  }
-  std::vector<int4> wmx;  // MFMA Legendre tiles: 32 columns of one kind
-  {
-    const int nck[4] = {n_ss, n_ss, n_gs, nl};
-    for (int i = 0; i < B; i++)
-      for (int k = 0; k < 4; k++)
-        for (int c0 = 0; c0 < nck[k]; c0 += 32) wmx.push_back(make_int4(i, k, c0, 0));
-  }
-  HIPCHK(c, bt.d_wmx.ensure(wmx.size()));
-  HIPCHK(c, hipMemcpyAsync(bt.d_wmx.p, wmx.data(), wmx.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-  bt.n_wmx = (int)wmx.size();
-  HIPCHK(c, bt.d_wsx.ensure(wsx.size()));
-  HIPCHK(c, bt.d_wgg.ensure(std::max<size_t>(wgg.size(), 1)));
-  HIPCHK(c, hipMemcpyAsync(bt.d_wsx.p, wsx.data(), wsx.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-  if (!wgg.empty())
-    HIPCHK(c, hipMemcpyAsync(bt.d_wgg.p, wgg.data(), wgg.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-  bt.n_wsx = (int)wsx.size();
-  bt.n_wgg = (int)wgg.size();
-  std::vector<int2> wa;  // Legendre adjoint: 8 C_ell columns per block
-  for (int i = 0; i < B; i++)
     for (int col = 0; col < ncol; col += 8) wa.push_back(make_int2(i, col));
-  HIPCHK(c, bt.d_wa.ensure(wa.size()));
-  HIPCHK(c, hipMemcpyAsync(bt.d_wa.p, wa.data(), wa.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-  bt.n_wa = (int)wa.size();
-  HIPCHK(c, bt.d_wg.ensure(wg.size()));
-  HIPCHK(c, bt.d_wn.ensure(wn.size()));
-  HIPCHK(c, bt.d_wl.ensure(wl.size()));
-  HIPCHK(c, bt.d_wl2.ensure(wl2.size()));
-  HIPCHK(c, hipMemcpyAsync(bt.d_wg.p, wg.data(), wg.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(bt.d_wn.p, wn.data(), wn.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(bt.d_wl.p, wl.data(), wl.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(bt.d_wl2.p, wl2.data(), wl2.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-  bt.n_wg = (int)wg.size(); bt.n_wn = (int)wn.size(); bt.n_wl = (int)wl.size(); bt.n_wl2 = (int)wl2.size();
+    const int nck[4] = {n_ss, n_ss, n_gs, nl};
+    for (int k = 0; k < 4; k++)
+      for (int c0 = 0; c0 < nck[k]; c0 += 32) wmx.push_back(make_int4(i, k, c0, 0));
+  }
+  for (int i = 0; i < B; i++) {  // gg tiles (heaviest per block) after the other lists
+    const Layout& L = *lays[i];
+    const int nrow = ells ? (int)ells->size() : L.nrow();
+    for (int zl = 0; zl < L.lens.nbin; zl++)
+      for (int r = 0; r < nrow; r += CLH_GGT_TILE) wgg.push_back(make_int4(i, zl, r, 0));
+  }
+  auto up2 = [&](DBuf<int2>& db, const std::vector<int2>& v, int& n) -> hipError_t {
+    n = (int)v.size();
+    hipError_t e = db.ensure(std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(db.p, v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream);
+  };
+  auto up4 = [&](DBuf<int4>& db, const std::vector<int4>& v, int& n) -> hipError_t {
+    n = (int)v.size();
+    hipError_t e = db.ensure(std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(db.p, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream);
+  };
+  HIPCHK(c, up2(bt.d_wg, wg, bt.n_wg));
+  HIPCHK(c, up2(bt.d_wn, wn, bt.n_wn));
+  HIPCHK(c, up2(bt.d_wl2, wl2, bt.n_wl2));
+  HIPCHK(c, up2(bt.d_wa, wa, bt.n_wa));
+  HIPCHK(c, up4(bt.d_wl, wl, bt.n_wl));
+  HIPCHK(c, up4(bt.d_wsx, wsx, bt.n_wsx));
+  HIPCHK(c, up4(bt.d_wgg, wgg, bt.n_wgg));
+  HIPCHK(c, up4(bt.d_wmx, wmx, bt.n_wmx));
   bt.B = B;
-  bt.max_nrow = nrow;
-  bt.dv_size = dvsz;
+  bt.max_nrow = max_nrow;
+  bt.max_npair = max_npair;
+  bt.uniform = uniform;
+  bt.lays = lays;
+  bt.dv_off = dvoff;
+  bt.dv_size = lays[0]->dv_size();
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CLH_OK;
+}
+
+// a batch whose draws all use the context's current layout
+static int build_batch_ctx(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& draws, int apply_calib,
+                           const int ord[3], const dvec* ells, double* dv_out) {
+  int rc = check_layout(c, c->L);
+  if (rc) return rc;
+  if ((rc = ensure_device(c))) return rc;
+  const bool full = c->legendre_mode == CLH_LEGENDRE_FULL;
+  if (!c->L.kern_valid || (full && c->L.kfull.empty())) build_kernels(c->L, full);
+  if (!c->dl.valid && (rc = upload_layout(c, c->L, c->dl))) return rc;
+  const std::vector<const Layout*> lays(draws.size(), &c->L);
+  const std::vector<const DevLayout*> devs(draws.size(), &c->dl);
+  rc = build_batch(c, bt, draws, lays, devs, apply_calib, ord, ells, dv_out);
+  bt.ctx_layout = true;
+  return rc;
 }
 
 static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
@@ -836,7 +924,7 @@ static int ensure_current(clh_ctx* c) {
   if (c->cached_version == c->version) return CLH_OK;
   const int ord[3] = {0, 1, 2};
   std::vector<const Draw*> v{&c->cur};
-  int rc = build_batch(c, c->single, v, 0, ord, nullptr, nullptr);
+  int rc = build_batch_ctx(c, c->single, v, 0, ord, nullptr, nullptr);
   if (rc) return rc;
   if ((rc = run_batch(c, c->single, true))) return rc;
   c->cache_raw.resize(c->single.dv_size);
@@ -873,8 +961,8 @@ void clh_destroy(clh_ctx* c) {
   }
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   c->single.release(); c->batch.release(); c->scratch.release();
-  c->d_layout.release();
-  c->d_kmx32.release(); c->d_kmxsc.release(); c->d_kmxbf.release();
+  c->dl.release();
+  c->rl.clear();
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
@@ -887,7 +975,8 @@ int clh_set_legendre_mode(clh_ctx* c, int mode) {
   if (mode != CLH_LEGENDRE_FOLDED && mode != CLH_LEGENDRE_FULL) return fail(c, CLH_ERR_ARG, "bad mode %d", mode);
   if (mode != c->legendre_mode) {
     c->legendre_mode = mode;
-    c->dev_layout_valid = false;
+    c->dl.valid = false;
+    c->mode_epoch++;
     c->version++;
   }
   return CLH_OK;
@@ -912,7 +1001,7 @@ int clh_init_ntable(clh_ctx* c, int n_ell, int n_a, int lmax, int hdi) {
   if (hdi != 0) return fail(c, CLH_ERR_ARG, "high_def_integration=%d not implemented (0 only)", hdi);
   c->L.n_ell = n_ell; c->L.n_a = n_a; c->L.lmax = lmax; c->L.hdi = hdi;
   c->L.kern_valid = false;
-  c->dev_layout_valid = false;
+  c->dl.valid = false;
   c->version++;
   return CLH_OK;
 }
@@ -925,7 +1014,7 @@ int clh_init_binning_real_space(clh_ctx* c, int ntheta, double tmin, double tmax
   c->L.vtmin = tmin * 2.90888208665721580e-4;
   c->L.vtmax = tmax * 2.90888208665721580e-4;
   c->L.kern_valid = false;
-  c->dev_layout_valid = false;
+  c->dl.valid = false;
   c->version++;
   return CLH_OK;
 }
@@ -981,7 +1070,7 @@ int clh_set_source_sample(clh_ctx* c, const double* t, int nrows, int ntomo) {
   if (!c) return CLH_ERR_ARG;
   int rc = build_sample(c, c->L.src, t, nrows, ntomo, true);
   if (rc) return rc;
-  c->dev_layout_valid = false;
+  c->dl.valid = false;
   c->version++;
   return CLH_OK;
 }
@@ -991,7 +1080,7 @@ int clh_set_lens_sample(clh_ctx* c, const double* t, int nrows, int ntomo) {
   int rc = build_sample(c, c->L.lens, t, nrows, ntomo, false);
   if (rc) return rc;
   lens_zmean(c->L);
-  c->dev_layout_valid = false;
+  c->dl.valid = false;
   c->version++;
   return CLH_OK;
 }
@@ -1048,7 +1137,7 @@ int clh_compute_3x2pt_real(clh_ctx* c, const int ord[3], double* dv, int n) {
   for (int i = 0; i < 3; i++)
     if (ord[i] < 0) return fail(c, CLH_ERR_ARG, "bad ord");
   std::vector<const Draw*> v{&c->cur};
-  int rc = build_batch(c, c->single, v, 1, ord, nullptr, nullptr);
+  int rc = build_batch_ctx(c, c->single, v, 1, ord, nullptr, nullptr);
   if (rc) return rc;
   if ((rc = run_batch(c, c->single, true))) return rc;
   HIPCHK(c, hipMemcpyAsync(dv, c->single.desc[0].dv, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1108,7 +1197,7 @@ int clh_cl_nointerp(clh_ctx* c, int probe, const double* ell, int nl_, double* o
   dvec ells(ell, ell + nl_);
   const int ord[3] = {0, 1, 2};
   std::vector<const Draw*> v{&c->cur};
-  int rc = build_batch(c, c->scratch, v, 0, ord, &ells, nullptr);
+  int rc = build_batch_ctx(c, c->scratch, v, 0, ord, &ells, nullptr);
   if (rc) return rc;
   if ((rc = run_batch(c, c->scratch, false))) return rc;
   const int nss = c->L.n_ss(), ngs = c->L.n_gs(), nl = c->L.lens.nbin;
@@ -1160,27 +1249,119 @@ int clh_batch_upload(clh_ctx* c, int B, const clh_draw* draws) {
   }
   const int ord[3] = {0, 1, 2};
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-  return build_batch(c, c->batch, ptr, 1, ord, nullptr, nullptr);
+  c->batch.B = 0;
+  c->batch_epoch = c->mode_epoch;
+  const int rc = build_batch_ctx(c, c->batch, ptr, 1, ord, nullptr, nullptr);
+  if (rc) c->batch.B = 0;
+  c->rl.clear();
+  return rc;
+}
+
+}  // extern "C"
+
+// ======================================================= captured layouts
+struct clh_layout {
+  Layout L;
+  uint64_t id = 0;
+};
+static std::atomic<uint64_t> g_layout_id{1};
+
+extern "C" {
+
+int clh_layout_capture(clh_ctx* c, clh_layout** out) {
+  if (!c || !out) return CLH_ERR_ARG;
+  *out = nullptr;
+  int rc = check_layout(c, c->L);
+  if (rc) return rc;
+  const bool full = c->legendre_mode == CLH_LEGENDRE_FULL;
+  if (!c->L.kern_valid || (full && c->L.kfull.empty())) build_kernels(c->L, full);
+  clh_layout* l = new (std::nothrow) clh_layout();
+  if (!l) return fail(c, CLH_ERR_ALLOC, "layout allocation");
+  l->L = c->L;
+  l->id = g_layout_id++;
+  *out = l;
+  return CLH_OK;
+}
+
+void clh_layout_destroy(clh_layout* l) { delete l; }
+
+int clh_layout_datavector_size(const clh_layout* l) { return l ? l->L.dv_size() : 0; }
+
+int clh_batch_upload_layouts(clh_ctx* c, int B, const clh_draw* draws, const clh_layout* const* layouts) {
+  if (!c || B <= 0 || !draws || !layouts) return CLH_ERR_ARG;
+  std::vector<Draw> ds(B);
+  std::vector<const Draw*> ptr(B);
+  for (int i = 0; i < B; i++) {
+    const clh_draw& s = draws[i];
+    if (!layouts[i]) return fail(c, CLH_ERR_ARG, "draw %d: no layout", i);
+    const Layout& L = layouts[i]->L;
+    if (s.chi_n < 5 || s.G_n < 2 || s.nk < 2 || s.npz < 2) return fail(c, CLH_ERR_ARG, "draw %d: table sizes", i);
+    if (has_nan(s.chi_z, s.chi_n) || has_nan(s.chi, s.chi_n) || has_nan(s.G_z, s.G_n) || has_nan(s.G, s.G_n) ||
+        has_nan(s.log10k, s.nk) || has_nan(s.pk_z, s.npz) || has_nan(s.lnP, s.nk * s.npz))
+      return fail(c, CLH_ERR_NAN, "draw %d: NaN in a table", i);
+    if (L.lens.nbin > 0 && !s.b1) return fail(c, CLH_ERR_ARG, "draw %d: b1 missing", i);
+    fill_draw(ds[i], s, L);
+    ptr[i] = &ds[i];
+  }
+  int rc = ensure_device(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  c->batch.B = 0;
+  // the context's copies of the layouts, with device copies (kept from the
+  // previous multi-layout batch where still valid)
+  const bool full = c->legendre_mode == CLH_LEGENDRE_FULL;
+  std::map<uint64_t, std::unique_ptr<LayoutCopy>> keep;
+  std::vector<const Layout*> lays(B);
+  std::vector<const DevLayout*> devs(B);
+  for (int i = 0; i < B; i++) {
+    const clh_layout* l = layouts[i];
+    auto it = keep.find(l->id);
+    if (it == keep.end()) {
+      std::unique_ptr<LayoutCopy> e;
+      auto old = c->rl.find(l->id);
+      if (old != c->rl.end() && old->second->epoch == c->mode_epoch && old->second->D.valid) {
+        e = std::move(old->second);
+      } else {
+        e.reset(new (std::nothrow) LayoutCopy());
+        if (!e) return fail(c, CLH_ERR_ALLOC, "layout copy");
+        e->L = l->L;
+        if (!e->L.kern_valid || (full && e->L.kfull.empty())) build_kernels(e->L, full);
+        if ((rc = check_layout(c, e->L))) return rc;
+        if ((rc = upload_layout(c, e->L, e->D))) return rc;
+        e->epoch = c->mode_epoch;
+      }
+      it = keep.emplace(l->id, std::move(e)).first;
+    }
+    lays[i] = &it->second->L;
+    devs[i] = &it->second->D;
+  }
+  const int ord[3] = {0, 1, 2};
+  rc = build_batch(c, c->batch, ptr, lays, devs, 1, ord, nullptr, nullptr);
+  c->rl = std::move(keep);
+  if (rc) { c->batch.B = 0; return rc; }
+  c->batch.ctx_layout = false;
+  c->batch_epoch = c->mode_epoch;
+  return CLH_OK;
+}
+
+long long clh_batch_datavector_offset(clh_ctx* c, int i) {
+  if (!c || i < 0 || i > c->batch.B) return -1;
+  return c->batch.dv_off[i];
 }
 
 int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
   if (!c || !ord || !dv_device) return CLH_ERR_ARG;
   if (c->batch.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
-  if (!c->dev_layout_valid) return fail(c, CLH_ERR_STATE, "layout changed since clh_batch_upload");
+  if ((c->batch.ctx_layout && !c->dl.valid) || c->batch_epoch != c->mode_epoch)
+    return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
   // per-call placement: output pointer and block order go into the descriptors
-  const Layout& L = c->L;
-  const int nt = L.ntheta;
-  const int sizes[3] = {2 * nt * L.n_ss(), nt * L.n_gs(), nt * L.lens.nbin};
-  int idx[3] = {0, 1, 2};
-  for (int i = 1; i < 3; i++)
-    for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) std::swap(idx[j], idx[j - 1]);
-  int start[3] = {0, 0, 0};
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
   bool changed = false;
   for (int i = 0; i < c->batch.B; i++) {
     DrawDesc& q = c->batch.desc[i];
-    double* want = dv_device + (size_t)c->batch.dv_size * i;
+    int start[3];
+    dv_starts(*c->batch.lays[i], ord, start);
+    double* want = dv_device + c->batch.dv_off[i];
     if (q.dv != want || q.dv_start[0] != start[0] || q.dv_start[1] != start[1] || q.dv_start[2] != start[2]) {
       q.dv = want;
       for (int k = 0; k < 3; k++) q.dv_start[k] = start[k];
@@ -1271,6 +1452,7 @@ int clh_batch_set_params(clh_ctx* c, const double* lnP, const double* src_dz, co
   Batch& bt = c->batch;
   if (bt.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
   if (lnP && bt.nknz <= 0) return fail(c, CLH_ERR_ARG, "draws of the batch have different P(k,z) shapes");
+  if (!bt.uniform) return fail(c, CLH_ERR_ARG, "per-draw parameter arrays need a batch with one layout");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, clh_launch_set_params(bt.d_desc.p, bt.d_par.p, bt.B, bt.nknz, lnP, src_dz, lens_dz, b1, ia_A,
                                   shear_m, c->stream));
@@ -1283,8 +1465,10 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   Batch& bt = c->batch;
   if (bt.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
   if (bt.nknz <= 0) return fail(c, CLH_ERR_ARG, "draws of the batch have different P(k,z) shapes");
-  if (!c->dev_layout_valid) return fail(c, CLH_ERR_STATE, "layout changed since clh_batch_upload");
-  const Layout& L = c->L;
+  if ((bt.ctx_layout && !c->dl.valid) || c->batch_epoch != c->mode_epoch)
+    return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
+  if (!bt.uniform) return fail(c, CLH_ERR_ARG, "VJP needs a batch with one layout for every draw");
+  const Layout& L = *bt.lays[0];
   const int B = bt.B, ns = L.src.nbin, nl = L.lens.nbin, nrow = L.nrow();
   const int ncol = L.n_ss() + L.n_gs() + nl;
   HIPCHK(c, hipSetDevice(c->device));
@@ -1300,14 +1484,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   const size_t gper = al(bt.nknz) + 4 * al(CLH_MAXB);
   HIPCHK(c, bt.d_gscr.ensure(gper * B));
   {  // block starts of dv_bar (generic_interface.hpp:770-791)
-    const int nt = L.ntheta;
-    const int sizes[3] = {2 * nt * L.n_ss(), nt * L.n_gs(), nt * nl};
-    int idx[3] = {0, 1, 2};
-    for (int i = 1; i < 3; i++)
-      for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) std::swap(idx[j], idx[j - 1]);
-    int start[3] = {0, 0, 0};
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+    int start[3];
+    dv_starts(L, ord, start);
     for (int i = 0; i < B; i++)
       for (int k = 0; k < 3; k++) bt.desc[i].dv_start[k] = start[k];
   }
@@ -1350,7 +1528,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa.p, bt.n_wa, s));
   HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
   HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, s));
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl, std::max(c->L.n_ss(), c->L.src.nbin), s));
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl, bt.max_npair, s));
   HIPCHK(c, clh_launch_nodes_adj(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   return CLH_OK;
 }

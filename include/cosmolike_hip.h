@@ -121,7 +121,7 @@ int clh_cl_nointerp(clh_ctx* ctx, int probe, const double* ell, int nl, double* 
 /* One draw's cosmology and nuisance (host pointers; NULL nuisance arrays
  * mean zero shift / unit stretch / zero m / zero b_mag; b1 must be given when
  * the layout has lens bins). The survey layout (n(z), binning, Ntable) is the
- * context's. */
+ * context's (clh_batch_upload) or a captured one (clh_batch_upload_layouts). */
 typedef struct {
   double omega_m;
   const double* chi_z; const double* chi; int chi_n;
@@ -134,8 +134,27 @@ typedef struct {
 
 /* uploads B draws into device memory owned by the context */
 int clh_batch_upload(clh_ctx* ctx, int B, const clh_draw* draws);
+/* Captured survey layouts, for batches whose draws differ in layout (number
+ * of source / lens bins, n(z), theta binning).  clh_layout_capture snapshots
+ * the context's current layout (the state set by clh_init_ntable,
+ * clh_init_binning_real_space, clh_init_ia, clh_set_source_sample,
+ * clh_set_lens_sample); later setter calls do not change the snapshot.  No
+ * counterpart in the reference, whose layout is one set of global structs. */
+typedef struct clh_layout clh_layout;
+int clh_layout_capture(clh_ctx* ctx, clh_layout** out);
+void clh_layout_destroy(clh_layout* layout);
+int clh_layout_datavector_size(const clh_layout* layout);
+/* uploads B draws, draw i with layout layouts[i] (the context keeps its own
+ * copies; the caller may destroy the layouts afterwards).  Forward only:
+ * clh_batch_set_params and clh_batch_vjp need every draw on one layout. */
+int clh_batch_upload_layouts(clh_ctx* ctx, int B, const clh_draw* draws,
+                             const clh_layout* const* layouts);
+/* offset (doubles) of draw i's data vector in the batch output; i = B gives
+ * the total length; -1 on a bad index */
+long long clh_batch_datavector_offset(clh_ctx* ctx, int i);
 /* runs the whole path for every uploaded draw on the context's stream;
- * dv_device: device pointer to B * clh_datavector_size() doubles */
+ * dv_device: device pointer to clh_batch_datavector_offset(ctx, B) doubles
+ * (B * clh_datavector_size() for clh_batch_upload), draw i at its offset */
 int clh_batch_forward(clh_ctx* ctx, const int ord[3], double* dv_device);
 int clh_synchronize(clh_ctx* ctx);
 /* on != 0: clh_batch_forward captures its launches into a hipGraph on first

@@ -103,3 +103,15 @@ def test_compute_without_device_fails_cleanly(ctx):
     rc = ctx.L.clh_compute_3x2pt_real(ctx.h, o, out.ctypes.data_as(_lib.D), len(out))
     assert rc == -5
     assert np.all(out == 7.0)
+
+
+def test_layout_capture_is_a_snapshot(ctx):
+    p = synth.full_3x2pt_problem()
+    ctx.load(p)
+    a = ctx.capture_layout()
+    assert a.datavector_size() == 4290
+    ctx.init_binning_real_space(20, 2.5, 250.0)
+    b = ctx.capture_layout()
+    assert b.datavector_size() == 20 * (2 * 55 + 50 + 5)
+    assert a.datavector_size() == 4290  # unchanged by later setters
+    del a, b
