@@ -60,11 +60,11 @@ def algorithmic_work(p):
 
 def kernel_work_and_peak(name, work, precision):
     """(algorithmic FLOP per draw, peak TFLOP/s) of a kernel by its name."""
-    base = name[:-2] if name.endswith("32") else name
+    base = name[:-4] if name.endswith("_f32") else name
     if name == "k_legendre_mx":
         base = "k_legendre"
     flop = work.get(base, 0.0)
-    if name.endswith("32"):
+    if name.endswith("_f32"):
         return flop, FP32_PEAK_TFLOPS
     if name == "k_legendre_mx":
         return flop, (BF16_PEAK_TFLOPS if precision == "bf16x3" else FP32_PEAK_TFLOPS)
@@ -138,6 +138,8 @@ def main():
     c.batch_upload(draws)
     n = c.datavector_size()
     dv = torch.empty(args.batch * n, dtype=torch.float64, device="cuda:0")
+    tile_rows = c.autotune()
+    log(f"bench: autotuned rows per tile (fwd ss/gs, fwd gg, adj ss/gs, adj gg) {tile_rows}")
     accuracy = None
     if args.precision != "fp64":
         # error of the mixed path against the plain FP64 GPU path on this batch
@@ -174,11 +176,13 @@ def main():
         c.batch_forward(dv.data_ptr())
         c.synchronize()
         for k, v in c.kernel_times().items():
-            acc[k] = acc.get(k, 0.0) + v / nt
+            if not k.endswith("_adj"):
+                acc[k] = acc.get(k, 0.0) + v / nt
     c.enable_timing(False)
 
     # VJP throughput: one VJP of every draw's data vector (includes its forward recompute)
     vjp_value = None
+    vjp_ms = {}
     if args.vjp:
         B = args.batch
         ns, nl = draws[0].ns, draws[0].nl
@@ -197,6 +201,15 @@ def main():
         c.synchronize()
         vjp_value = B * nv / (time.perf_counter() - t1)
         log(f"bench: VJP {vjp_value:.1f} per s")
+        c.enable_timing(True)
+        for _ in range(3):
+            c.batch_vjp(dvbar.data_ptr(), *ptrs)
+            c.synchronize()
+            for k, v in c.kernel_times().items():
+                if k.endswith("_adj"):
+                    vjp_ms[k] = vjp_ms.get(k, 0.0) + v / 3
+        c.enable_timing(False)
+        log(f"bench: VJP Limber kernel ms {vjp_ms}")
     work = algorithmic_work(draws[0])
     dom = max(acc, key=acc.get)
     flop1, peak = kernel_work_and_peak(dom, work, args.precision)
@@ -204,7 +217,7 @@ def main():
     achieved = flop / (acc[dom] * 1e-3) / 1e12 if acc[dom] > 0 else 0.0
     roofline = {"bound": "mfma" if "legendre" in dom else "valu", "achieved": achieved, "peak": peak,
                 "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None, "kernel": dom,
-                "kernel_ms": acc[dom], "kernel_ms_all": acc}
+                "kernel_ms": acc[dom], "kernel_ms_all": acc, "vjp_kernel_ms": vjp_ms}
     log(f"bench: kernel ms {acc}")
 
     # sanity: the last batch agrees with itself (finite values)
@@ -229,6 +242,7 @@ def main():
             lays.append(cache[key])
         cr.set_precision(args.precision)
         cr.batch_upload_layouts(probs, lays)
+        rtile = cr.autotune()
         ntot = cr.batch_offsets()[-1]
         log(f"bench: ragged batch of {len(probs)} draws, {len(cache)} layouts, {ntot} outputs, "
             f"set up in {time.perf_counter() - t1:.1f}s")
@@ -247,7 +261,7 @@ def main():
         elr = time.perf_counter() - t1
         assert torch.isfinite(rdv).all().item()
         ragged = {"value": len(probs) * nr / elr, "unit": "data vectors/s", "draws": len(probs),
-                  "layouts": len(cache), "ms_per_step": 1e3 * elr / nr, "graph": True,
+                  "layouts": len(cache), "ms_per_step": 1e3 * elr / nr, "graph": True, "tile_rows": rtile,
                   "precision": args.precision}
         log(f"bench: ragged {ragged['value']:.1f} dv/s")
         del cr, rdv
@@ -285,6 +299,7 @@ def main():
                                "+ 5 gg pairs, 26 theta 2.5'-900', N_ell=500, LMAX=50000), batch of "
                                f"{args.batch} draws with their own cosmology",
                    "batch": args.batch, "legendre": args.mode, "precision": args.precision,
+                   "tile_rows": tile_rows,
                    "datavector_len": n},
         "accuracy_vs_fp64_path": accuracy,
         "vjp_value": vjp_value, "vjp_unit": "VJPs/s (full data vector -> lnP, dz_src, dz_lens, b1, A_IA, m)",

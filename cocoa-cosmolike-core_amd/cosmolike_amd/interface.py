@@ -310,6 +310,21 @@ class Cosmolike:
     def synchronize(self):
         self._chk(self.L.clh_synchronize(self.h))
 
+    def autotune(self):
+        """Pick the rows per tile of the Limber stages by timing the uploaded batch."""
+        rows = (ctypes.c_int * 4)()
+        self._chk(self.L.clh_autotune(self.h, rows))
+        return list(rows)
+
+    def set_tile_rows(self, rows):
+        r = (ctypes.c_int * 4)(*rows)
+        self._chk(self.L.clh_set_tile_rows(self.h, r))
+
+    def tile_rows(self):
+        rows = (ctypes.c_int * 4)()
+        self._chk(self.L.clh_get_tile_rows(self.h, rows))
+        return list(rows)
+
     def enable_timing(self, on=True):
         self._chk(self.L.clh_enable_timing(self.h, 1 if on else 0))
 

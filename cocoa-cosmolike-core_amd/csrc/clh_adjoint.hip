@@ -185,39 +185,46 @@ struct ClhLnPAcc {
 
 // ----------------------------------------------------------------------------
 // k_limber_sx_adj: adjoint of k_limber_sx.  One wave per work item (draw,
-// kind, zl, row0) of CLH_ADJ_ROWS rows; thread = GL node, walking its rows in
+// kind + 4 zl, row0, nrows); thread = GL node, walking the tile's rows in
 // increasing ell with the forward's P(k,z) recomputation.  Per (row, node):
 //  ss: u_b = sum_j f_bj cbar_(b,j) A_j (f = 2 on the diagonal) gives both
-//      abar_b += pref0 S u_b and the ln P cotangent pref0 S sum_b A_b u_b / 2;
+//      abar_b += pref0 S u_b and the ln P cotangent pref0 S sum_b A_b u_b / 2
+//      (the cotangents sit in LDS as the full symmetric [ns][ns][rows] array
+//      with f folded in, so the b, j loops address it linearly);
 //  gs: bbar_zs += pref2 cbar_zs g S, W_gal / W_mag / b1 cotangents and
 //      pref2 S g sum_zs cbar_zs B_zs for ln P, with g = W_gal b1 + W_mag pref1 bmag.
-// The C_ell-row cotangents of the tile sit in LDS (dynamic, npair x rows).
+// LDS: ss [ns][ns][nrows], gs [ns][nrows] (dynamic).
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict__ D,
                                                       const int4* __restrict__ work) {
-  extern __shared__ double Cb[];  // [npair][CLH_ADJ_ROWS]
+  extern __shared__ double Cb[];  // ss: [ns][ns][T], gs: [ns][T]
+  __shared__ double L10[CLH_TILE_ROWS];
+  __shared__ double s_lk[CLH_LDS_LK];
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
-  const int kind = w.y, zl = w.z, row0 = w.w;
+  const int kind = w.y & 3, zl = w.y >> 2, row0 = w.z, T = w.w;
   const int p = threadIdx.x;
-  constexpr int np = CLH_NP_SS, T = CLH_ADJ_ROWS;
+  constexpr int np = CLH_NP_SS;
   const int nrow = d.nrow, ns = d.ns, nz = d.nzp;
   const bool ss = kind == CLH_W_SS;
-  const double* o = ss ? d.nodes + d.node_ss : d.nodes + d.node_gs + (size_t)zl * d.node_gs_stride;
+  const double* o = clh_sx_nodes(d, kind, zl);
+  const double* lut = o + clh_sx_lut_col(d, kind) * np;
   const double* ells = ss ? d.ell_ss : d.ell_gx;
-  const int npair = ss ? d.n_ss : ns;
-  const int col0 = ss ? 0 : d.n_ss + zl * ns;
-  __shared__ double L10[T];
-  for (int i = p; i < npair * T; i += 64) {
-    const int q = i / T, r = row0 + i % T;
-    Cb[i] = (r < nrow) ? d.cbar[(size_t)(col0 + q) * nrow + r] : 0.0;
+  if (ss) {
+    for (int i = p; i < ns * ns * T; i += 64) {
+      const int b = i / (ns * T), jj = (i / T) % ns, r = row0 + i % T;
+      Cb[i] = (b == jj ? 2.0 : 1.0) * d.cbar[(size_t)clh_ss_index(ns, b, jj) * nrow + r];
+    }
+  } else {
+    const int col0 = d.n_ss + zl * ns;
+    for (int i = p; i < ns * T; i += 64) Cb[i] = d.cbar[(size_t)(col0 + i / T) * nrow + row0 + i % T];
   }
-  if (p < T) L10[p] = log10((row0 + p < nrow ? ells[row0 + p] : 1.0) + 0.5);
+  for (int t = p; t < T; t += 64) L10[t] = log10(ells[row0 + t] + 0.5);
+  const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
   __syncthreads();
-  const double a = o[p], fK = o[np + p], fac = o[2 * np + p];
-  const double z = 1.0 / a - 1.0;
-  const int j = clh_bs(d.pz, nz, z);
-  const double dy = (z - d.pz[j]) / (d.pz[j + 1] - d.pz[j]);
+  const double fac = o[2 * np + p];
+  const int j = (int)lut[p];
+  const double dy = lut[np + p], l10fk = lut[2 * np + p];
   const int woff = ss ? 3 : 5;
   double Wn[CLH_MAXB], acc[CLH_MAXB];
 #pragma unroll
@@ -229,18 +236,15 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
   const double b1 = ss ? 0.0 : d.par->b1[zl], bmag = ss ? 0.0 : d.par->bmag[zl];
   double twg = 0.0, twm = 0.0;
   ClhLnPAcc sc;
-  const double l10fk = log10(fK * CLH_COVERH0);
   int i = -1;
   double inv_k = 0.0;
   for (int rr = 0; rr < T; rr++) {
-    const int r = row0 + rr;
-    if (r >= nrow) break;
-    const double l = ells[r], ell = l + 0.5;
+    const double l = ells[row0 + rr], ell = l + 0.5;
     const double log10k = L10[rr] - l10fk;
     const int i0 = i;
-    i = (i < 0) ? clh_bs(d.lk, d.nk, log10k) : clh_walk(d.lk, d.nk, log10k, i);
-    if (i != i0) inv_k = 1.0 / (d.lk[i + 1] - d.lk[i]);
-    const double dx = (log10k - d.lk[i]) * inv_k;
+    i = (i < 0) ? clh_bs(lk, d.nk, log10k) : clh_walk(lk, d.nk, log10k, i);
+    if (i != i0) inv_k = 1.0 / (lk[i + 1] - lk[i]);
+    const double dx = (log10k - lk[i]) * inv_k;
     const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
     const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +
                        dx * dy * P[nz + 1];
@@ -250,13 +254,14 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
       const double pref0 = l * (l - 1.) * (l + 1.) * (l + 2.) / (ell * ell * ell * ell);
       const double c = pref0 * S;
       double sb = 0.0;
+      const double* cr = Cb + rr;
 #pragma unroll
       for (int b = 0; b < CLH_MAXB; b++) {
         if (b < ns) {
           double u = 0.0;
 #pragma unroll
           for (int jj = 0; jj < CLH_MAXB; jj++)
-            if (jj < ns) u += (jj == b ? 2.0 : 1.0) * Cb[clh_ss_index(ns, b, jj) * T + rr] * Wn[jj];
+            if (jj < ns) u += cr[(b * ns + jj) * T] * Wn[jj];
           acc[b] += c * u;
           sb += Wn[b] * u;
         }
@@ -268,16 +273,16 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
       const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
       const double gp = WG * b1 + WM * pref1 * bmag;
       const double cg = pref2 * gp * S;
-      double s = 0.0;
+      double sv = 0.0;
 #pragma unroll
       for (int b = 0; b < CLH_MAXB; b++) {
         if (b < ns) {
           const double cb = Cb[b * T + rr];
-          s += cb * Wn[b];
+          sv += cb * Wn[b];
           acc[b] += cg * cb;
         }
       }
-      const double t = pref2 * S * s;
+      const double t = pref2 * S * sv;
       twg += t;
       twm += t * pref1;
       g = t * gp;
@@ -301,40 +306,41 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
 }
 
 // ----------------------------------------------------------------------------
-// k_limber_gg_adj: adjoint of k_limber_gg, same work items and thread layout
-// (two groups of 96 node threads, each walking half of the rows; ClhGgNode).
+// k_limber_gg_adj: adjoint of k_limber_gg; work item (draw, zl, row0,
+// nrows), two groups of 96 node threads, group h walking rows
+// h ceil(nrows/2) .. of the tile (ClhGgNode from the k_nodes search results).
 // Per (row, node) with integrand t^2 P fac, t = W_gal b1 + W_mag pref bmag +
 // W_RSD: ln P cotangent cbar t^2 P fac; tb = 2 cbar t P fac feeds b1 (W_gal),
 // the W_gal / W_mag node cotangents and the lens shift through dW_RSD/d dz.
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restrict__ D,
                                                        const int4* __restrict__ work) {
-  __shared__ ClhGgRow R[CLH_GGT_TILE];
-  __shared__ double Cr[CLH_GGT_TILE];
+  __shared__ ClhGgRow R[CLH_TILE_ROWS];
+  __shared__ double Cr[CLH_TILE_ROWS];
+  __shared__ double s_lk[CLH_LDS_LK];
   __shared__ double red[2][3];
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
-  const int zl = w.y, row0 = w.z;
+  const int zl = w.y, row0 = w.z, nr = w.w;
   constexpr int np = CLH_NP_GG;
-  const int nrow = d.nrow;
-  if (threadIdx.x < CLH_GGT_TILE) {
-    const int r = row0 + threadIdx.x;
-    R[threadIdx.x] = clh_gg_row(r < nrow ? d.ell_gx[r] : 1.0);
-    Cr[threadIdx.x] = r < nrow ? d.cbar[(size_t)(d.n_ss + d.n_gs + zl) * nrow + r] : 0.0;
+  for (int t = threadIdx.x; t < nr; t += 192) {
+    R[t] = clh_gg_row(d.ell_gx[row0 + t]);
+    Cr[t] = d.cbar[(size_t)(d.n_ss + d.n_gs + zl) * d.nrow + row0 + t];
   }
+  const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
   __syncthreads();
   const int p = threadIdx.x % np, half = threadIdx.x / np;
   ClhGgNode<true> nd;
-  nd.init(d, d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride, p, zl);
+  nd.init(d, clh_gg_nodes(d, zl), p, zl);
   double tg = 0.0, tm = 0.0, b1p = 0.0, lzp = 0.0;
   ClhLnPAcc sc;
-  constexpr int per = CLH_GGT_TILE / 2;
-  for (int q = 0; q < per; q++) {
-    const int rr = half * per + q;
+  const int hn = (nr + 1) / 2;
+  const int r_lo = half * hn, r_hi = half ? nr : hn;
+  for (int rr = r_lo; rr < r_hi; rr++) {
     const double cb = Cr[rr];
-    if (cb == 0.0) continue;  // also rows past the table
+    if (cb == 0.0) continue;
     double t, pf, dwr, dx;
-    if (!nd.eval(d, R[rr], t, pf, dwr, dx)) continue;
+    if (!nd.eval(d, lk, R[rr], t, pf, dwr, dx)) continue;
     sc.add(d.g_lnP, d.nzp, nd.j, nd.i, dx, nd.dy, cb * t * t * pf);
     const double tb = cb * 2.0 * t * pf;
     tg += tb;
@@ -441,8 +447,9 @@ hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s) {
 }
 hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int max_npair,
                                    hipStream_t s) {
+  // max_npair: largest npair x rows of any work item (doubles of LDS)
   if (n <= 0) return hipSuccess;
-  const size_t lds = sizeof(double) * (size_t)max_npair * CLH_ADJ_ROWS;
+  const size_t lds = sizeof(double) * (size_t)max_npair;  // max over items of npair x rows
   hipLaunchKernelGGL(k_limber_sx_adj, dim3(n), dim3(64), lds, s, D, work);
   return hipGetLastError();
 }

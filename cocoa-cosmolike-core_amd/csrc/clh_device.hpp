@@ -285,6 +285,28 @@ __device__ __forceinline__ double clh_wave_sum(double v) {
   return v;
 }
 
+// ------------------------------------------------------------- node blocks
+// node block of an ss (kind CLH_W_SS) or gs (CLH_W_GS, lens bin zl) tile and
+// the offset (in units of CLH_NP_SS) of its search-result columns
+__device__ __forceinline__ const double* clh_sx_nodes(const DrawDesc& d, int kind, int zl) {
+  return kind == CLH_W_SS ? d.nodes + d.node_ss : d.nodes + d.node_gs + (size_t)zl * d.node_gs_stride;
+}
+__device__ __forceinline__ int clh_sx_lut_col(const DrawDesc& d, int kind) {
+  return kind == CLH_W_SS ? 3 + d.ns : 5 + d.ns;
+}
+__device__ __forceinline__ const double* clh_gg_nodes(const DrawDesc& d, int zl) {
+  return d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride;
+}
+
+// stage a table of n doubles in LDS when it fits (cap entries), else use it
+// in place; every thread of the block calls this, then the block syncs
+__device__ __forceinline__ const double* clh_stage(const double* __restrict__ g, int n, double* lds,
+                                                   int cap) {
+  if (n > cap) return g;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
+  return lds;
+}
+
 // ------------------------------------------------------------- gg integrand
 // Per-row constants of the gg integrand (ell = l + 1/2): log10(ell),
 // chi_1/chi = (ell+1)/ell, the two W_RSD coefficients (radial_weights.c:98-103)
@@ -328,50 +350,47 @@ struct ClhGgNode {
     down = (jj > 0) ? (cc[jj + 1] - cc[jj - 1]) / (zz[jj + 1] - zz[jj - 1]) : (cc[jj + 1] - cc[jj]) * inv_z;
   }
 
+  // o: the node block (a, fK, fac, W_gal, W_mag, then the search results of
+  // k_nodes: P(k,z) z index and weight, log10(chi c/H0), the a_0 side of W_RSD
+  // and the table indices at a_0 from which the walks start)
   __device__ __forceinline__ void init(const DrawDesc& d, const double* __restrict__ o, int p, int zl_) {
     constexpr int np = CLH_NP_GG;
     zl = zl_;
-    const double a = o[p];
     fK = o[np + p];
     fac = o[2 * np + p];
     WG = o[3 * np + p];
     WM = o[4 * np + p];
     b1 = d.par->b1[zl];
     bmag = d.par->bmag[zl];
-    const double z = 1.0 / a - 1.0;
-    j = clh_bs(d.pz, d.nzp, z);
-    dy = (z - d.pz[j]) / (d.pz[j + 1] - d.pz[j]);
-    l10fk = log10(fK * CLH_COVERH0);
+    const double* lut = o + 5 * np;
+    j = (int)lut[p];
+    dy = lut[np + p];
+    l10fk = lut[2 * np + p];
+    jc = (int)lut[3 * np + p];
+    jz = (int)lut[4 * np + p];
+    jg = (int)lut[5 * np + p];
+    jn = (int)lut[6 * np + p];
+    t0 = lut[7 * np + p];
+    t0d = ADJ ? lut[8 * np + p] : 0.0;
     inv_st = 1.0 / d.par->lens_st[zl];
     dzl = d.par->lens_dz[zl];
     zm = d.l_zmean[zl];
     chi_amin = d.chi_amin;
-    // a_0 side of W_RSD at chi_0 = chi (radial_weights.c:98-99)
-    const double chi = fK * CLH_COVERH0;
-    jc = clh_bs(d.chi_v, d.chi_n, chi);
     inv_c = 1.0 / (d.chi_v[jc + 1] - d.chi_v[jc]);
-    const double z0 = d.chi_z[jc] + (chi - d.chi_v[jc]) * inv_c * (d.chi_z[jc + 1] - d.chi_z[jc]);
-    const double a0 = 1.0 / (1.0 + z0);
-    double c0, dc0;
-    clh_chi_all(d, a0, c0, dc0);
-    const double zz0 = 1.0 / a0 - 1.0;
-    const double hf0 = 1.0 / ((a0 * a0) * dc0) * clh_f_growth(d, zz0);
-    t0 = clh_nz_lens(d, zz0, zl) * hf0;
-    t0d = ADJ ? clh_dnz_lens(d, zz0, zl) * hf0 : 0.0;
-    jz = clh_bs(d.chi_z, d.chi_n, zz0);
     jj = jz > d.chi_n - 3 ? d.chi_n - 3 : jz;
     set_z(d);
-    jg = clh_bs(d.G_z, d.G_n, zz0);
     slG = (d.G_v[jg + 1] - d.G_v[jg]) / (d.G_z[jg + 1] - d.G_z[jg]);
-    jn = clh_bs(d.l_zc, d.l_rows, zz0);
     i = -1;
     inv_k = 0.0;
   }
 
   // false: chi_1 beyond chi(a_min), integrand zero (cosmo2D.c:2512-2514).
   // t: the bracket; pf = P fac; dx: log10 k weight of the P cell (i, j).
-  __device__ __forceinline__ bool eval(const DrawDesc& d, const ClhGgRow& R, double& t, double& pf,
-                                       double& dwr, double& dx) {
+  // lk: the draw's log10 k table (in LDS when it fits).  F32: P(k,z) as
+  // expf of the FP64 bilinear value.
+  template <bool F32 = false>
+  __device__ __forceinline__ bool eval(const DrawDesc& d, const double* __restrict__ lk, const ClhGgRow& R,
+                                       double& t, double& pf, double& dwr, double& dx) {
     const double chi_1 = fK * R.rr1;
     if (chi_1 > chi_amin) return false;
     const double c1v = chi_1 * CLH_COVERH0;
@@ -417,15 +436,17 @@ struct ClhGgNode {
     const double log10k = R.l10 - l10fk;
     {
       const int i0 = i;
-      i = (i < 0) ? clh_bs(d.lk, d.nk, log10k) : clh_walk(d.lk, d.nk, log10k, i);
-      if (i != i0) inv_k = 1.0 / (d.lk[i + 1] - d.lk[i]);
+      i = (i < 0) ? clh_bs(lk, d.nk, log10k) : clh_walk(lk, d.nk, log10k, i);
+      if (i != i0) inv_k = 1.0 / (lk[i + 1] - lk[i]);
     }
-    dx = (log10k - d.lk[i]) * inv_k;
+    dx = (log10k - lk[i]) * inv_k;
     const int nz = d.nzp;
     const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
     const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +
                        dx * dy * P[nz + 1];
-    pf = exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac;
+    constexpr double c3 = 1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
+    if (F32) pf = (double)(expf((float)out) * (float)(c3 * fac));
+    else pf = exp(out) * c3 * fac;
     return true;
   }
 };

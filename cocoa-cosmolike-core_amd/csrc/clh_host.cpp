@@ -21,8 +21,8 @@
 
 hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
-hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, hipStream_t s);
-hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, hipStream_t s);
+hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, hipStream_t s);
+hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, hipStream_t s);
 hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow, hipStream_t s);
 hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_fill(double* p, size_t n, double v, hipStream_t s);
@@ -31,12 +31,10 @@ hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nk
                                  const double* iaA, const double* m, hipStream_t s);
 hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
-hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int max_npair,
+hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles,
                                    hipStream_t s);
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
-hipError_t clh_launch_limber_sx32(const DrawDesc* D, const int4* work, int n, hipStream_t s);
-hipError_t clh_launch_limber_gg32(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_legendre_mx(const DrawDesc* D, const int4* work, int n, int mode, hipStream_t s);
 hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, int max_nrow,
                                     hipStream_t s);
@@ -197,7 +195,7 @@ struct LayoutCopy {  // a context's own copy of a captured layout, with its devi
 
 struct Batch {
   int B = 0;
-  int max_nrow = 0, max_npair = 0;
+  int max_nrow = 0, adj_lds = 0;  // adj_lds: LDS doubles of the ss/gs adjoint
   bool ctx_layout = true;     // every draw uses the context's own layout
   bool uniform = true;        // every draw has the same layout
   std::vector<const Layout*> lays;
@@ -208,8 +206,9 @@ struct Batch {
   DBuf<DrawParams> d_par;
   DBuf<double> d_tab, d_work, d_dv, d_ells, d_vjp, d_gscr;
   DBuf<int2> d_wg, d_wn, d_wl2, d_wa;
-  DBuf<int4> d_wl, d_wsx, d_wgg, d_wmx;
-  int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0, n_wsx = 0, n_wgg = 0, n_wmx = 0;
+  DBuf<int4> d_wl, d_wsx, d_wgg, d_wga, d_wmx;
+  int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0, n_wsx = 0, n_wgg = 0, n_wga = 0, n_wmx = 0;
+  int nrow_custom = 0;  // rows of a custom-ell batch (0: the layouts' rows)
   int dv_size = 0;
   int nknz = 0;        // common ln P table size (0 if the draws differ)
   int ord[3] = {0, 1, 2};
@@ -217,7 +216,7 @@ struct Batch {
     d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
     d_vjp.release(); d_gscr.release();
     d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa.release();
-    d_wsx.release(); d_wgg.release(); d_wmx.release();
+    d_wsx.release(); d_wgg.release(); d_wga.release(); d_wmx.release();
   }
 };
 
@@ -232,6 +231,7 @@ struct clh_ctx {
   Draw cur;
   int legendre_mode = CLH_LEGENDRE_FOLDED;
   int precision = CLH_PREC_FP64;
+  int tile_rows[4] = {16, 64, 64, 64};  // rows per tile: fwd ss/gs, fwd gg, adj ss/gs, adj gg
   uint64_t version = 1, cached_version = 0;
   dvec cache_raw;   // raw (uncalibrated, ord 0,1,2) data vector of `cur`
   dvec cache_cl;    // cl rows of `cur`
@@ -245,8 +245,10 @@ struct clh_ctx {
   bool use_graph = false;             // replay the batch forward from a hipGraph
   hipGraphExec_t gexec = nullptr;     // captured forward of `batch`
   hipStream_t gstream = nullptr;      // stream the graph was captured on
-  hipEvent_t ev[8] = {nullptr};
+  hipEvent_t ev[10] = {nullptr};
   float last_ms[5] = {0, 0, 0, 0, 0};
+  float last_vjp_ms[2] = {0, 0};
+  bool vjp_timed = false;
 };
 
 static int fail(clh_ctx* c, int code, const char* fmt, ...) {
@@ -681,6 +683,21 @@ static int check_layout(clh_ctx* c, const Layout& L) {
   return CLH_OK;
 }
 
+// node buffer of a draw: ss block, nl gs blocks, nl gg blocks, each with its
+// search-result columns (node layout in clh_internal.hpp)
+static size_t node_size(int ns, int nl) {
+  return (size_t)CLH_NP_SS * (3 + ns + CLH_LUT_SX) +
+         (size_t)nl * (CLH_NP_SS * (5 + ns + CLH_LUT_SX) + CLH_NP_GG * (5 + CLH_LUT_GG));
+}
+
+// the row tiles of one C_ell column: at most `cap` rows each, spread evenly
+static void row_tiles(int nrow, int cap, std::vector<int2>& out) {
+  out.clear();
+  const int nt = (nrow + cap - 1) / cap;
+  const int tr = (nrow + nt - 1) / nt;
+  for (int r = 0; r < nrow; r += tr) out.push_back(make_int2(r, std::min(tr, nrow - r)));
+}
+
 // data-vector block starts of compute_data_vector_Mx2pt_N_starts
 // (generic_interface.hpp:770-791) for a layout and a probe order
 static void dv_starts(const Layout& L, const int ord[3], int start[3]) {
@@ -692,6 +709,80 @@ static void dv_starts(const Layout& L, const int ord[3], int start[3]) {
   for (int i = 0; i < 3; i++) start[i] = 0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+}
+
+// Work lists of a batch (one entry per block; every stage is one launch for
+// the whole batch).  rows: rows per tile of the forward ss/gs, forward gg,
+// adjoint ss/gs and adjoint gg Limber stages (1..CLH_TILE_ROWS).
+static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
+  std::vector<int2> wg, wn, wl2, wa;
+  std::vector<int4> wl, wsx, wgg, wga, wmx;
+  int adj_lds = 0;
+  std::vector<int2> tiles, atiles_ss, atiles_gs;
+  for (int i = 0; i < bt.B; i++) {
+    const Layout& L = *bt.lays[i];
+    const int ns = L.src.nbin, nl = L.lens.nbin;
+    const int nrow = bt.nrow_custom ? bt.nrow_custom : L.nrow();
+    const int n_ss = L.n_ss(), n_gs = L.n_gs(), ncol = n_ss + n_gs + nl;
+    for (int b = 0; b < ns + nl; b++) wg.push_back(make_int2(i, b));
+    for (int sg = 0; sg <= 2 * nl; sg++) wn.push_back(make_int2(i, sg));
+    // forward Limber: ss, gs and gg tiles (draw, kind + 4 zl, row0, rows)
+    row_tiles(nrow, rows[0], tiles);
+    for (const int2& t : tiles) wsx.push_back(make_int4(i, CLH_W_SS, t.x, t.y));
+    for (int zl = 0; zl < nl; zl++)
+      for (const int2& t : tiles) wsx.push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
+    row_tiles(nrow, rows[1], tiles);
+    for (int zl = 0; zl < nl; zl++)
+      for (const int2& t : tiles) wgg.push_back(make_int4(i, zl, t.x, t.y));
+    // adjoint ss/gs tiles: the tile's cotangent rows ([ns][ns][rows] for ss,
+    // [ns][rows] for gs) fit in 64 KB of LDS
+    const int cap_ss = std::max(1, std::min(rows[2], 8192 / std::max(ns * ns, 1)));
+    const int cap_gs = std::max(1, std::min(rows[2], 8192 / std::max(ns, 1)));
+    row_tiles(nrow, cap_ss, atiles_ss);
+    row_tiles(nrow, cap_gs, atiles_gs);
+    for (const int2& t : atiles_ss) {
+      wl.push_back(make_int4(i, CLH_W_SS, t.x, t.y));
+      adj_lds = std::max(adj_lds, ns * ns * t.y);
+    }
+    for (int zl = 0; zl < nl; zl++)
+      for (const int2& t : atiles_gs) {
+        wl.push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
+        adj_lds = std::max(adj_lds, ns * t.y);
+      }
+    row_tiles(nrow, rows[3], tiles);
+    for (int zl = 0; zl < nl; zl++)
+      for (const int2& t : tiles) wga.push_back(make_int4(i, zl, t.x, t.y));
+    const int ntask = 2 * n_ss + n_gs + nl;
+    for (int t = 0; t < ntask; t += 8) wl2.push_back(make_int2(i, t));
+    for (int col = 0; col < ncol; col += 8) wa.push_back(make_int2(i, col));
+    const int nck[4] = {n_ss, n_ss, n_gs, nl};
+    for (int k = 0; k < 4; k++)
+      for (int c0 = 0; c0 < nck[k]; c0 += 32) wmx.push_back(make_int4(i, k, c0, 0));
+  }
+  auto up2 = [&](DBuf<int2>& db, const std::vector<int2>& v, int& n) -> hipError_t {
+    n = (int)v.size();
+    hipError_t e = db.ensure(std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(db.p, v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream);
+  };
+  auto up4 = [&](DBuf<int4>& db, const std::vector<int4>& v, int& n) -> hipError_t {
+    n = (int)v.size();
+    hipError_t e = db.ensure(std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(db.p, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream);
+  };
+  HIPCHK(c, up2(bt.d_wg, wg, bt.n_wg));
+  HIPCHK(c, up2(bt.d_wn, wn, bt.n_wn));
+  HIPCHK(c, up2(bt.d_wl2, wl2, bt.n_wl2));
+  HIPCHK(c, up2(bt.d_wa, wa, bt.n_wa));
+  HIPCHK(c, up4(bt.d_wl, wl, bt.n_wl));
+  HIPCHK(c, up4(bt.d_wsx, wsx, bt.n_wsx));
+  HIPCHK(c, up4(bt.d_wgg, wgg, bt.n_wgg));
+  HIPCHK(c, up4(bt.d_wga, wga, bt.n_wga));
+  HIPCHK(c, up4(bt.d_wmx, wmx, bt.n_wmx));
+  bt.adj_lds = adj_lds;
+  return CLH_OK;
+
 }
 
 // Builds the per-draw descriptors, device tables and work lists of a batch;
@@ -713,20 +804,19 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   // per-draw sizes and offsets (layouts may differ between draws)
   std::vector<size_t> woff(B + 1, 0), toff(B + 1, 0);
   std::vector<long long> dvoff(B + 1, 0);
-  int max_nrow = 0, max_npair = 0;
+  int max_nrow = 0;
   bool uniform = true;
   for (int i = 0; i < B; i++) {
     const Layout& L = *lays[i];
     const Draw& d = *draws[i];
     const int ns = L.src.nbin, nl = L.lens.nbin;
     const int nrow = ells ? (int)ells->size() : L.nrow();
-    const size_t node_sz = (size_t)CLH_NP_SS * (3 + ns) + (size_t)nl * (CLH_NP_SS * (5 + ns) + CLH_NP_GG * 5);
+    const size_t node_sz = node_size(ns, nl);
     woff[i + 1] = woff[i] + al((size_t)(ns + nl) * L.n_a) + al(node_sz) +
                   al((size_t)(L.n_ss() + L.n_gs() + nl) * nrow);
     toff[i + 1] = toff[i] + 2 * d.chi_z.size() + 2 * d.G_z.size() + d.lk.size() + d.pz.size() + d.lnP.size() + 32;
     dvoff[i + 1] = dvoff[i] + L.dv_size();
     max_nrow = std::max(max_nrow, nrow);
-    max_npair = std::max(max_npair, std::max(L.n_ss(), ns));
     if (lays[i] != lays[0]) uniform = false;
   }
   HIPCHK(c, bt.d_tab.ensure(toff[B]));
@@ -782,7 +872,7 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.kmx32 = DL.k32.p; q.kmxbf = DL.kbf.p; q.kmxsc = DL.ksc.p;
     q.kmx_ngrp = DL.ngrp;
     const size_t gtab_sz = (size_t)(ns + nl) * L.n_a;
-    const size_t node_sz = (size_t)CLH_NP_SS * (3 + ns) + (size_t)nl * (CLH_NP_SS * (5 + ns) + CLH_NP_GG * 5);
+    const size_t node_sz = node_size(ns, nl);
     double* wb = bt.d_work.p + woff[i];
     q.gtab = wb;
     q.nodes = wb + al(gtab_sz);
@@ -790,9 +880,9 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.dv = dv_out ? dv_out + dvoff[i] : bt.d_dv.p + dvoff[i];
     dv_starts(L, ord, q.dv_start);
     q.node_ss = 0;
-    q.node_gs = CLH_NP_SS * (3 + ns);
-    q.node_gs_stride = CLH_NP_SS * (5 + ns);
-    q.node_gg_stride = CLH_NP_GG * 5;
+    q.node_gs = CLH_NP_SS * (3 + ns + CLH_LUT_SX);
+    q.node_gs_stride = CLH_NP_SS * (5 + ns + CLH_LUT_SX);
+    q.node_gg_stride = CLH_NP_GG * (5 + CLH_LUT_GG);
     q.node_gg = q.node_gs + nl * q.node_gs_stride;
     double chi_amin, dd;
     host_chi_all(d, CLH_A_MIN, chi_amin, dd);
@@ -812,63 +902,14 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   for (int i = 1; i < B; i++)
     if ((int)draws[i]->lnP.size() != bt.nknz || draws[i]->pz.size() != draws[0]->pz.size()) bt.nknz = 0;
   for (int k = 0; k < 3; k++) bt.ord[k] = ord[k];
-  // work lists (one entry per block; every stage is one launch for the whole batch)
-  std::vector<int2> wg, wn, wl2, wa;
-  std::vector<int4> wl, wsx, wgg, wmx;
-  for (int i = 0; i < B; i++) {
-    const Layout& L = *lays[i];
-    const int ns = L.src.nbin, nl = L.lens.nbin;
-    const int nrow = ells ? (int)ells->size() : L.nrow();
-    const int n_ss = L.n_ss(), n_gs = L.n_gs(), ncol = n_ss + n_gs + nl;
-    for (int b = 0; b < ns + nl; b++) wg.push_back(make_int2(i, b));
-    for (int sg = 0; sg <= 2 * nl; sg++) wn.push_back(make_int2(i, sg));
-    // adjoint ss/gs tiles, ss (heaviest) first
-    for (int r = 0; r < nrow; r += CLH_ADJ_ROWS) wl.push_back(make_int4(i, CLH_W_SS, 0, r));
-    for (int zl = 0; zl < nl; zl++)
-      for (int r = 0; r < nrow; r += CLH_ADJ_ROWS) wl.push_back(make_int4(i, CLH_W_GS, zl, r));
-    const int ntask = 2 * n_ss + n_gs + nl;
-    for (int t = 0; t < ntask; t += 8) wl2.push_back(make_int2(i, t));
-    for (int zl = 0; zl < nl; zl++)
-      for (int r = 0; r < nrow; r += CLH_SX_TILE) wsx.push_back(make_int4(i, CLH_W_GS, zl, r));
-    for (int r = 0; r < nrow; r += CLH_SX_TILE) wsx.push_back(make_int4(i, CLH_W_SS, 0, r));
-    for (int col = 0; col < ncol; col += 8) wa.push_back(make_int2(i, col));
-    const int nck[4] = {n_ss, n_ss, n_gs, nl};
-    for (int k = 0; k < 4; k++)
-      for (int c0 = 0; c0 < nck[k]; c0 += 32) wmx.push_back(make_int4(i, k, c0, 0));
-  }
-  for (int i = 0; i < B; i++) {  // gg tiles (heaviest per block) after the other lists
-    const Layout& L = *lays[i];
-    const int nrow = ells ? (int)ells->size() : L.nrow();
-    for (int zl = 0; zl < L.lens.nbin; zl++)
-      for (int r = 0; r < nrow; r += CLH_GGT_TILE) wgg.push_back(make_int4(i, zl, r, 0));
-  }
-  auto up2 = [&](DBuf<int2>& db, const std::vector<int2>& v, int& n) -> hipError_t {
-    n = (int)v.size();
-    hipError_t e = db.ensure(std::max<size_t>(v.size(), 1));
-    if (e != hipSuccess || v.empty()) return e;
-    return hipMemcpyAsync(db.p, v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream);
-  };
-  auto up4 = [&](DBuf<int4>& db, const std::vector<int4>& v, int& n) -> hipError_t {
-    n = (int)v.size();
-    hipError_t e = db.ensure(std::max<size_t>(v.size(), 1));
-    if (e != hipSuccess || v.empty()) return e;
-    return hipMemcpyAsync(db.p, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream);
-  };
-  HIPCHK(c, up2(bt.d_wg, wg, bt.n_wg));
-  HIPCHK(c, up2(bt.d_wn, wn, bt.n_wn));
-  HIPCHK(c, up2(bt.d_wl2, wl2, bt.n_wl2));
-  HIPCHK(c, up2(bt.d_wa, wa, bt.n_wa));
-  HIPCHK(c, up4(bt.d_wl, wl, bt.n_wl));
-  HIPCHK(c, up4(bt.d_wsx, wsx, bt.n_wsx));
-  HIPCHK(c, up4(bt.d_wgg, wgg, bt.n_wgg));
-  HIPCHK(c, up4(bt.d_wmx, wmx, bt.n_wmx));
   bt.B = B;
   bt.max_nrow = max_nrow;
-  bt.max_npair = max_npair;
   bt.uniform = uniform;
   bt.lays = lays;
   bt.dv_off = dvoff;
   bt.dv_size = lays[0]->dv_size();
+  bt.nrow_custom = ells ? (int)ells->size() : 0;
+  if ((rc = build_work(c, bt, c->tile_rows))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CLH_OK;
 }
@@ -897,15 +938,9 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], s));
   const bool mixed = c->precision != CLH_PREC_FP64;
-  if (mixed)
-    HIPCHK(c, clh_launch_limber_gg32(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, s));
-  else
-    HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, s));
+  HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, mixed, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], s));
-  if (mixed)
-    HIPCHK(c, clh_launch_limber_sx32(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
-  else
-    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
+  HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, mixed, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
   if (legendre) {
     if (c->legendre_mode == CLH_LEGENDRE_FULL)
@@ -1407,8 +1442,11 @@ int clh_synchronize(clh_ctx* c) {
   if (!c) return CLH_ERR_ARG;
   if (!c->own_stream) return CLH_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (c->timing)
+  if (c->timing) {
     for (int k = 0; k < 5; k++) HIPCHK(c, hipEventElapsedTime(&c->last_ms[k], c->ev[k], c->ev[k + 1]));
+    if (c->vjp_timed)
+      for (int k = 0; k < 2; k++) HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[k], c->ev[6 + k], c->ev[7 + k]));
+  }
   return CLH_OK;
 }
 
@@ -1426,19 +1464,91 @@ int clh_enable_timing(clh_ctx* c, int on) {
 }
 
 int clh_last_kernel_times(clh_ctx* c, const char** names, float* ms, int cap) {
-  static const char* nm[5] = {"k_gtab", "k_nodes", "k_limber_gg", "k_limber_sx", "k_legendre"};
+  static const char* nm[7] = {"k_gtab", "k_nodes", "k_limber_gg", "k_limber_sx", "k_legendre",
+                              "k_limber_gg_adj", "k_limber_sx_adj"};
   static const char* nm_full = "k_legendre_full";
-  static const char* nm32[3] = {"k_limber_gg32", "k_limber_sx32", "k_legendre_mx"};
+  static const char* nm32[3] = {"k_limber_gg_f32", "k_limber_sx_f32", "k_legendre_mx"};
   if (!c) return CLH_ERR_ARG;
   const bool mixed = c->precision != CLH_PREC_FP64;
-  for (int k = 0; k < 5 && k < cap; k++) {
+  const int n = c->vjp_timed ? 7 : 5;
+  for (int k = 0; k < n && k < cap; k++) {
     if (names) {
-      names[k] = (mixed && k >= 2) ? nm32[k - 2] : nm[k];
+      names[k] = (mixed && k >= 2 && k < 5) ? nm32[k - 2] : nm[k];
       if (k == 4 && c->legendre_mode == CLH_LEGENDRE_FULL) names[k] = nm_full;
     }
-    if (ms) ms[k] = c->last_ms[k];
+    if (ms) ms[k] = k < 5 ? c->last_ms[k] : c->last_vjp_ms[k - 5];
   }
-  return 5;
+  return n;
+}
+
+int clh_set_tile_rows(clh_ctx* c, const int rows[4]) {
+  if (!c || !rows) return CLH_ERR_ARG;
+  for (int k = 0; k < 4; k++)
+    if (rows[k] < 1 || rows[k] > CLH_TILE_ROWS) return fail(c, CLH_ERR_ARG, "tile rows %d not in 1..%d", rows[k], CLH_TILE_ROWS);
+  for (int k = 0; k < 4; k++) c->tile_rows[k] = rows[k];
+  if (c->batch.B > 0) {
+    int rc = build_work(c, c->batch, c->tile_rows);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  }
+  return CLH_OK;
+}
+
+int clh_get_tile_rows(clh_ctx* c, int rows[4]) {
+  if (!c || !rows) return CLH_ERR_ARG;
+  for (int k = 0; k < 4; k++) rows[k] = c->tile_rows[k];
+  return CLH_OK;
+}
+
+int clh_autotune(clh_ctx* c, int rows_out[4]) {
+  if (!c) return CLH_ERR_ARG;
+  Batch& bt = c->batch;
+  if (bt.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
+  if ((bt.ctx_layout && !c->dl.valid) || c->batch_epoch != c->mode_epoch)
+    return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
+  HIPCHK(c, hipSetDevice(c->device));
+  // the forward writes into the batch's own output buffer while tuning
+  HIPCHK(c, bt.d_dv.ensure((size_t)bt.dv_off[bt.B]));
+  for (int i = 0; i < bt.B; i++) bt.desc[i].dv = bt.d_dv.p + bt.dv_off[i];
+  HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
+  const bool adj = bt.uniform && bt.nknz > 0;
+  DBuf<double> dvbar;  // unit cotangent for the adjoint stages
+  if (adj) {
+    HIPCHK(c, dvbar.ensure((size_t)bt.dv_off[bt.B]));
+    HIPCHK(c, clh_launch_fill(dvbar.p, (size_t)bt.dv_off[bt.B], 1.0, c->stream));
+  }
+  const bool was_timing = c->timing;
+  c->timing = true;
+  static const int cand[3] = {16, 32, 64};
+  float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+  int pick[4];
+  for (int k = 0; k < 4; k++) pick[k] = c->tile_rows[k];
+  const int ord[3] = {0, 1, 2};
+  int rc = CLH_OK;
+  for (int k = 0; k < 3 && rc == CLH_OK; k++) {
+    const int t[4] = {cand[k], cand[k], cand[k], cand[k]};
+    if ((rc = build_work(c, bt, t))) break;
+    for (int rep = 0; rep < 3 && rc == CLH_OK; rep++) {
+      if ((rc = run_batch(c, bt, false))) break;
+      if (adj && (rc = clh_batch_vjp(c, ord, dvbar.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr)))
+        break;
+      if ((rc = clh_synchronize(c))) break;
+      const float ms[4] = {c->last_ms[3], c->last_ms[2], adj ? c->last_vjp_ms[1] : 1e30f,
+                           adj ? c->last_vjp_ms[0] : 1e30f};
+      for (int s2 = 0; s2 < 4; s2++)
+        if (ms[s2] < best[s2]) { best[s2] = ms[s2]; pick[s2] = cand[k]; }
+    }
+  }
+  c->timing = was_timing;
+  if (rc) { (void)build_work(c, bt, c->tile_rows); return rc; }
+  for (int k = 0; k < 4; k++) c->tile_rows[k] = pick[k];
+  if ((rc = build_work(c, bt, c->tile_rows))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (rows_out)
+    for (int k = 0; k < 4; k++) rows_out[k] = pick[k];
+  return CLH_OK;
 }
 
 }  // extern "C"
@@ -1523,12 +1633,16 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   HIPCHK(c, clh_launch_gtab(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
   HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
   HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
-  HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
+  HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, 0, s));
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa.p, bt.n_wa, s));
   HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
-  HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, s));
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl, bt.max_npair, s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));
+  HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga, s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl, bt.adj_lds, s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
+  c->vjp_timed = c->timing;
   HIPCHK(c, clh_launch_nodes_adj(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   return CLH_OK;
 }

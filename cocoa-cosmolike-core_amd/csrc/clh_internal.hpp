@@ -21,11 +21,23 @@
 #define CLH_PREC_BF16X3 2
 
 // Rows of a Limber table column: CLH_NLOW direct rows (ell = r+1), then the
-// N_ell log-spaced rows.  Rows per work tile of the ss/gs forward, the gg
-// forward and adjoint, and the ss/gs adjoint kernels:
-#define CLH_SX_TILE 16
-#define CLH_GGT_TILE 16
-#define CLH_ADJ_ROWS 32
+// N_ell log-spaced rows.  A Limber work tile covers at most CLH_TILE_ROWS
+// consecutive rows (the host spreads a column's rows evenly over its tiles);
+// the forward kernels process a tile in sub-tiles of CLH_SUB_ROWS rows.
+#define CLH_TILE_ROWS 64
+#define CLH_SUB_ROWS 16
+#define CLH_LDS_LK 512       // log10 k tables up to this length are staged in LDS
+
+// Per-node search results written by k_nodes after each node block's weight
+// columns (shared by every row tile of the segment):
+//  ss / gs nodes: CLH_LUT_SX columns  z index of P(k,z), its weight dy,
+//                 log10(chi c/H0)
+//  gg nodes:      CLH_LUT_GG columns  the same three, then the a_0 side of
+//                 W_RSD (radial_weights.c:98-99): chi(z) index at chi,
+//                 z index of the dchi/dz stencil, growth index, lens n(z)
+//                 knot index, n_lens h f and d n_lens/d dz h f at a_0
+#define CLH_LUT_SX 3
+#define CLH_LUT_GG 9
 
 // Everything one draw needs on the device.  Plain data; lives in an array in
 // device memory and is read with uniform (scalar) loads.
@@ -107,7 +119,9 @@ struct DrawParams {
 };
 
 // Node buffer layout (doubles):
-//  ss block  : a[64] fK[64] fac[64] A[ns][64]            fac = wt dchi/da / fK^2
+//  ss block  : a[64] fK[64] fac[64] A[ns][64] LUT[3][64]      fac = wt dchi/da / fK^2
 //              A_b = W_kappa_b - C1 W_source_b
-//  gs block zl: a fK fac WGAL WMAG [64 each], B[ns][64]  B_zs = W_kappa_zs - C1_zs W_source_zs
-//  gg block zl: a fK fac WGAL WMAG [96 each]
+//  gs block zl: a fK fac WGAL WMAG [64 each], B[ns][64] LUT[3][64]
+//              B_zs = W_kappa_zs - C1_zs W_source_zs
+//  gg block zl: a fK fac WGAL WMAG [96 each] LUT[9][96]
+// Sizes: ss 64 (6 + ns), gs stride 64 (8 + ns), gg stride 96 (5 + 9).

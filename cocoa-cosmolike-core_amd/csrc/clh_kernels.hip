@@ -167,6 +167,35 @@ __global__ __launch_bounds__(128) void k_nodes(const DrawDesc* __restrict__ D,
   const double fac = wt * (dchida / (fK * fK));
   const double z = 1.0 / a - 1.0;
   const double wk_pre = 1.5 * d.omega_m * fK / a;
+  // search results every row tile of this segment starts from
+  double* lut;
+  {
+    const int jp = clh_bs(d.pz, d.nzp, z);
+    lut = (seg == 0) ? d.nodes + d.node_ss + (3 + d.ns) * np
+        : (seg <= d.nl) ? d.nodes + d.node_gs + (size_t)zl * d.node_gs_stride + (5 + d.ns) * np
+                        : d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride + 5 * np;
+    lut[p] = (double)jp;
+    lut[np + p] = (z - d.pz[jp]) / (d.pz[jp + 1] - d.pz[jp]);
+    lut[2 * np + p] = log10(fK * CLH_COVERH0);
+  }
+  if (seg > d.nl) {
+    // a_0 side of W_RSD at chi_0 = chi (radial_weights.c:98-99; a_chi cosmo3D.c:109-133)
+    const double chi = fK * CLH_COVERH0;
+    const int jc = clh_bs(d.chi_v, d.chi_n, chi);
+    const double z0 = d.chi_z[jc] + (chi - d.chi_v[jc]) * (1.0 / (d.chi_v[jc + 1] - d.chi_v[jc])) *
+                                        (d.chi_z[jc + 1] - d.chi_z[jc]);
+    const double a0 = 1.0 / (1.0 + z0);
+    double c0, dc0;
+    clh_chi_all(d, a0, c0, dc0);
+    const double zz0 = 1.0 / a0 - 1.0;
+    const double hf0 = 1.0 / ((a0 * a0) * dc0) * clh_f_growth(d, zz0);
+    lut[3 * np + p] = (double)jc;
+    lut[4 * np + p] = (double)clh_bs(d.chi_z, d.chi_n, zz0);
+    lut[5 * np + p] = (double)clh_bs(d.G_z, d.G_n, zz0);
+    lut[6 * np + p] = (double)clh_bs(d.l_zc, d.l_rows, zz0);
+    lut[7 * np + p] = clh_nz_lens(d, zz0, zl) * hf0;
+    lut[8 * np + p] = clh_dnz_lens(d, zz0, zl) * hf0;
+  }
   if (seg == 0) {
     double* o = d.nodes + d.node_ss;
     o[p] = a;
@@ -200,139 +229,182 @@ __global__ __launch_bounds__(128) void k_nodes(const DrawDesc* __restrict__ D,
 }
 
 // ----------------------------------------------------------------------------
-// k_limber_sx: ss and gs tiles of CLH_SX_TILE rows.  One thread per GL node
-// walks its rows in increasing ell, so the log10 k index of the P(k,z) lookup
-// moves by at most a step between rows and the z index is found once.
-// Work item (draw, kind, zl, row0).
+// k_limber_sx: ss and gs tiles.  Work item (draw, kind + 4 zl, row0, nrows):
+// up to CLH_TILE_ROWS consecutive rows, processed in sub-tiles of
+// CLH_SUB_ROWS.  One thread per GL node (4 waves x 64 nodes) walks 4 rows of
+// every sub-tile in increasing ell, so the log10 k index of the P(k,z) lookup
+// moves by a step or two between rows; the z index, its weight and
+// log10(chi c/H0) come from k_nodes.  P fac of the sub-tile goes to LDS, then
+// the pair sums read it with the node weights (also in LDS).
+// F32: ss tiles with FP32 P(k,z) (expf of the FP64 bilinear value), products
+// and Kahan-compensated node sums; gs tiles always run in FP64 (their lensing
+// and intrinsic-alignment terms can cancel to 1e-3 of their size, which FP32
+// rounding does not survive at 1e-4).
 // ----------------------------------------------------------------------------
+struct ClhKahan {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+template <bool F32>
 __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restrict__ D,
                                                          const int4* __restrict__ work) {
-  __shared__ double S[CLH_SX_TILE * CLH_NP_SS];
+  __shared__ double S[CLH_SUB_ROWS * CLH_NP_SS];
   __shared__ double W[(CLH_MAXB + 2) * CLH_NP_SS];
-  __shared__ double L10[CLH_SX_TILE];
+  __shared__ float Sf[CLH_SUB_ROWS * CLH_NP_SS];
+  __shared__ float Wf[CLH_MAXB * CLH_NP_SS];
+  __shared__ double L10[CLH_TILE_ROWS];
+  __shared__ double s_lk[CLH_LDS_LK];
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
-  const int kind = w.y, zl = w.z, row0 = w.w;
+  const int kind = w.y & 3, zl = w.y >> 2, row0 = w.z, nr = w.w;
+  const bool f32 = F32 && kind == CLH_W_SS;
   const int tid = threadIdx.x;
-  const int np = CLH_NP_SS;
-  const double* o = (kind == CLH_W_SS) ? d.nodes + d.node_ss
-                                       : d.nodes + d.node_gs + (size_t)zl * d.node_gs_stride;
+  constexpr int np = CLH_NP_SS;
+  const double* o = clh_sx_nodes(d, kind, zl);
+  const double* lut = o + clh_sx_lut_col(d, kind) * np;
   const double* ells = (kind == CLH_W_SS) ? d.ell_ss : d.ell_gx;
-  if (tid < CLH_SX_TILE) {
-    const int r = row0 + tid;
-    L10[tid] = log10((r < d.nrow ? ells[r] : 1.0) + 0.5);
-  }
+  for (int t = tid; t < nr; t += CLH_BLOCK) L10[t] = log10(ells[row0 + t] + 0.5);
   const int nw = (kind == CLH_W_SS) ? d.ns : d.ns + 2;
-  for (int i = tid; i < nw * np; i += CLH_BLOCK) W[i] = o[3 * np + i];
+  if (f32)
+    for (int i = tid; i < nw * np; i += CLH_BLOCK) Wf[i] = (float)o[3 * np + i];
+  else
+    for (int i = tid; i < nw * np; i += CLH_BLOCK) W[i] = o[3 * np + i];
+  const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
   __syncthreads();
-  {
-    const int p = tid & 63, wv = tid >> 6;
-    const double a = o[p], fK = o[np + p], fac = o[2 * np + p];
-    const double z = 1.0 / a - 1.0;
-    const int nz = d.nzp;
-    const int j = clh_bs(d.pz, nz, z);
-    const double dy = (z - d.pz[j]) / (d.pz[j + 1] - d.pz[j]);
-    const double l10fk = log10(fK * CLH_COVERH0);  // log10 k = log10(ell) - log10(chi)
-    int i = -1;
-    double inv_k = 0.0;
-    constexpr int per = CLH_SX_TILE / 4;
+  const int p = tid & 63, wv = tid >> 6;
+  const int j = (int)lut[p];
+  const double dy = lut[np + p], l10fk = lut[2 * np + p], fac = o[2 * np + p];
+  const int nz = d.nzp;
+  int i = -1;
+  double inv_k = 0.0;
+  constexpr double c3 = 1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
+  constexpr int per = CLH_SUB_ROWS / 4;
+  for (int sub = 0; sub * CLH_SUB_ROWS < nr; sub++) {
+    const int base = sub * CLH_SUB_ROWS;
     for (int q = 0; q < per; q++) {
-      const int rr = wv * per + q;
-      const int r = row0 + rr;
+      const int rl = wv * per + q;  // row within the sub-tile
+      const int rr = base + rl;     // row within the tile
       double s = 0.0;
-      if (r < d.nrow) {
+      if (rr < nr) {
         const double log10k = L10[rr] - l10fk;
         const int i0 = i;
-        i = (i < 0) ? clh_bs(d.lk, d.nk, log10k) : clh_walk(d.lk, d.nk, log10k, i);
-        if (i != i0) inv_k = 1.0 / (d.lk[i + 1] - d.lk[i]);
-        const double dx = (log10k - d.lk[i]) * inv_k;
+        i = (i < 0) ? clh_bs(lk, d.nk, log10k) : clh_walk(lk, d.nk, log10k, i);
+        if (i != i0) inv_k = 1.0 / (lk[i + 1] - lk[i]);
+        const double dx = (log10k - lk[i]) * inv_k;
         const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
         const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +
                            dx * dy * P[nz + 1];
-        s = exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac;
+        if (f32) Sf[rl * np + p] = expf((float)out) * (float)(c3 * fac);
+        else s = exp(out) * c3 * fac;
+      } else if (f32) {
+        Sf[rl * np + p] = 0.0f;
       }
-      S[rr * np + p] = s;
+      if (!f32) S[rl * np + p] = s;
     }
-  }
-  __syncthreads();
-  if (kind == CLH_W_SS) {
-    const int nout = d.n_ss * CLH_SX_TILE;
-    for (int q = tid; q < nout; q += CLH_BLOCK) {
-      const int pair = q / CLH_SX_TILE, rr = q % CLH_SX_TILE;
-      const int r = row0 + rr;
-      if (r >= d.nrow) continue;
-      int i, j;
-      clh_ss_pair(d.ns, pair, i, j);
-      const double* Ai = W + i * np;
-      const double* Aj = W + j * np;
-      const double* Sr = S + rr * np;
-      double acc = 0.0;
+    __syncthreads();
+    if (kind == CLH_W_SS) {
+      const int nout = d.n_ss * CLH_SUB_ROWS;
+      for (int qq = tid; qq < nout; qq += CLH_BLOCK) {
+        const int pair = qq / CLH_SUB_ROWS, rl = qq % CLH_SUB_ROWS;
+        const int rr = base + rl;
+        if (rr >= nr) continue;
+        int bi, bj;
+        clh_ss_pair(d.ns, pair, bi, bj);
+        double acc;
+        if (f32) {
+          const float* Ai = Wf + bi * np;
+          const float* Aj = Wf + bj * np;
+          const float* Sr = Sf + rl * np;
+          ClhKahan k;
 #pragma unroll 8
-      for (int p = 0; p < np; p++) acc += Ai[p] * Aj[p] * Sr[p];
-      const double l = ells[r], ell = l + 0.5, ell4 = ell * ell * ell * ell;
-      const double pref = l * (l - 1.) * (l + 1.) * (l + 2.) / ell4;
-      d.cl[(size_t)pair * d.nrow + r] = acc * pref;
-    }
-  } else {
-    const int nout = d.ns * CLH_SX_TILE;
-    const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
-    for (int q = tid; q < nout; q += CLH_BLOCK) {
-      const int zs = q / CLH_SX_TILE, rr = q % CLH_SX_TILE;
-      const int r = row0 + rr;
-      if (r >= d.nrow) continue;
-      const double l = ells[r], ell = l + 0.5;
-      const double pref1 = l * (l + 1.) / (ell * ell);
-      const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
-      const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
-      const double* Bz = W + (2 + zs) * np;
-      const double* Sr = S + rr * np;
-      double acc = 0.0;
+          for (int pp = 0; pp < np; pp++) k.add(Ai[pp] * Aj[pp] * Sr[pp]);
+          acc = (double)k.s;
+        } else {
+          const double* Ai = W + bi * np;
+          const double* Aj = W + bj * np;
+          const double* Sr = S + rl * np;
+          acc = 0.0;
 #pragma unroll 8
-      for (int p = 0; p < np; p++) acc += Bz[p] * (W[p] * b1 + W[np + p] * pref1 * bmag) * Sr[p];
-      d.cl[(size_t)(d.n_ss + zl * d.ns + zs) * d.nrow + r] = acc * pref2;
+          for (int pp = 0; pp < np; pp++) acc += Ai[pp] * Aj[pp] * Sr[pp];
+        }
+        const double l = ells[row0 + rr], ell = l + 0.5, ell4 = ell * ell * ell * ell;
+        const double pref = l * (l - 1.) * (l + 1.) * (l + 2.) / ell4;
+        d.cl[(size_t)pair * d.nrow + row0 + rr] = acc * pref;
+      }
+    } else {
+      const int nout = d.ns * CLH_SUB_ROWS;
+      const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
+      for (int qq = tid; qq < nout; qq += CLH_BLOCK) {
+        const int zs = qq / CLH_SUB_ROWS, rl = qq % CLH_SUB_ROWS;
+        const int rr = base + rl;
+        if (rr >= nr) continue;
+        const double l = ells[row0 + rr], ell = l + 0.5;
+        const double pref1 = l * (l + 1.) / (ell * ell);
+        const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+        const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+        const double* Bz = W + (2 + zs) * np;
+        const double* Sr = S + rl * np;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int pp = 0; pp < np; pp++) acc += Bz[pp] * (W[pp] * b1 + W[np + pp] * pref1 * bmag) * Sr[pp];
+        d.cl[(size_t)(d.n_ss + zl * d.ns + zs) * d.nrow + row0 + rr] = acc * pref2;
+      }
     }
+    __syncthreads();
   }
 }
 
 // ----------------------------------------------------------------------------
-// k_limber_gg: gg tiles of CLH_GGT_TILE rows, 192 threads = two groups of 96
-// (one thread per GL node, ClhGgNode), each group walking half of the rows in
-// increasing ell; per-row constants in LDS.  Work item (draw, zl, row0, -).
+// k_limber_gg: gg tiles.  Work item (draw, zl, row0, nrows); 192 threads = two
+// groups of 96 (one thread per GL node, ClhGgNode started from the k_nodes
+// search results).  Sub-tile s covers rows 16 s .. 16 s + 15 of the tile:
+// group h takes rows 16 s + 8 h .. + 7, so each thread still walks its rows in
+// increasing ell.  Row sums over the 96 nodes: one wave per row.
+// F32: P(k,z) as expf of the FP64 bilinear value.
 // ----------------------------------------------------------------------------
+template <bool F32>
 __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ D,
                                                    const int4* __restrict__ work) {
-  __shared__ double V[CLH_GGT_TILE * CLH_NP_GG];
-  __shared__ ClhGgRow R[CLH_GGT_TILE];
+  __shared__ double V[CLH_SUB_ROWS * CLH_NP_GG];
+  __shared__ ClhGgRow R[CLH_TILE_ROWS];
+  __shared__ double s_lk[CLH_LDS_LK];
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
-  const int zl = w.y, row0 = w.z;
+  const int zl = w.y, row0 = w.z, nr = w.w;
   constexpr int np = CLH_NP_GG;
-  if (threadIdx.x < CLH_GGT_TILE) {
-    const int r = row0 + threadIdx.x;
-    R[threadIdx.x] = clh_gg_row(r < d.nrow ? d.ell_gx[r] : 1.0);
-  }
+  for (int t = threadIdx.x; t < nr; t += 192) R[t] = clh_gg_row(d.ell_gx[row0 + t]);
+  const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
   __syncthreads();
   const int p = threadIdx.x % np, half = threadIdx.x / np;
   ClhGgNode<false> nd;
-  nd.init(d, d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride, p, zl);
-  constexpr int per = CLH_GGT_TILE / 2;
-  for (int q = 0; q < per; q++) {
-    const int rr = half * per + q;
-    double v = 0.0;
-    if (row0 + rr < d.nrow) {
-      double t, pf, dwr, dx;
-      if (nd.eval(d, R[rr], t, pf, dwr, dx)) v = t * t * pf;
-    }
-    V[rr * np + p] = v;
-  }
-  __syncthreads();
-  // row sums over the 96 nodes: one wave per row
+  nd.init(d, clh_gg_nodes(d, zl), p, zl);
+  constexpr int per = CLH_SUB_ROWS / 2;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int rr = wid; rr < CLH_GGT_TILE; rr += 3) {
-    double v = V[rr * np + lane] + ((lane + 64 < np) ? V[rr * np + lane + 64] : 0.0);
-    v = clh_wave_sum(v);
-    const int r = row0 + rr;
-    if (lane == 0 && r < d.nrow) d.cl[(size_t)(d.n_ss + d.n_gs + zl) * d.nrow + r] = v;
+  for (int sub = 0; sub * CLH_SUB_ROWS < nr; sub++) {
+    const int base = sub * CLH_SUB_ROWS;
+    for (int q = 0; q < per; q++) {
+      const int rl = half * per + q;
+      double v = 0.0;
+      if (base + rl < nr) {
+        double t, pf, dwr, dx;
+        if (nd.template eval<F32>(d, lk, R[base + rl], t, pf, dwr, dx)) v = t * t * pf;
+      }
+      V[rl * np + p] = v;
+    }
+    __syncthreads();
+    for (int rl = wid; rl < CLH_SUB_ROWS; rl += 3) {
+      double v = V[rl * np + lane] + ((lane + 64 < np) ? V[rl * np + lane + 64] : 0.0);
+      v = clh_wave_sum(v);
+      const int rr = base + rl;
+      if (lane == 0 && rr < nr) d.cl[(size_t)(d.n_ss + d.n_gs + zl) * d.nrow + row0 + rr] = v;
+    }
+    __syncthreads();
   }
 }
 
@@ -484,13 +556,15 @@ hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, h
   hipLaunchKernelGGL(k_gtab<true>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }
-hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
+hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_limber_sx, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  if (f32) hipLaunchKernelGGL(k_limber_sx<true>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  else hipLaunchKernelGGL(k_limber_sx<false>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }
-hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
+hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_limber_gg, dim3(n), dim3(192), 0, s, D, work);
+  if (f32) hipLaunchKernelGGL(k_limber_gg<true>, dim3(n), dim3(192), 0, s, D, work);
+  else hipLaunchKernelGGL(k_limber_gg<false>, dim3(n), dim3(192), 0, s, D, work);
   return hipGetLastError();
 }

@@ -81,6 +81,12 @@ def test_setters_validate(ctx):
     assert e.value.code == -1
     for m in ("fp32", "bf16x3", "fp64"):
         ctx.set_precision(m)
+    for bad in ([0, 16, 16, 16], [16, 65, 16, 16]):
+        with pytest.raises(CosmolikeError) as e:
+            ctx.set_tile_rows(bad)                    # rows per tile outside 1..64
+        assert e.value.code == -1
+    ctx.set_tile_rows([32, 32, 16, 64])
+    assert ctx.tile_rows() == [32, 32, 16, 64]
 
 
 def test_datavector_sizes(ctx):
