@@ -98,7 +98,7 @@ def lib():
             "clh_enable_timing": (i32, [VP, i32]),
             "clh_set_tile_rows": (i32, [VP, I]),
             "clh_get_tile_rows": (i32, [VP, I]),
-            "clh_autotune": (i32, [VP, I]),
+            "clh_autotune": (i32, [VP, I, I]),
             "xi_pm_tomo": (f64, [i32, i32, i32, i32, i32]),
             "w_gammat_tomo": (f64, [i32, i32, i32, i32]),
             "w_gg_tomo": (f64, [i32, i32, i32, i32]),

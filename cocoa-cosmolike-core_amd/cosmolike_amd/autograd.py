@@ -36,6 +36,7 @@ class Projection3x2pt(torch.autograd.Function):
         lib_ctx.batch_forward(out.data_ptr(), ord)
         ctx.lib_ctx = lib_ctx
         ctx.ord = ord
+        ctx.generation = lib_ctx.param_generation
         ctx.save_for_backward(*[t if t is not None else torch.empty(0) for t in ins])
         ctx.present = [t is not None for t in ins]
         return out
@@ -48,8 +49,10 @@ class Projection3x2pt(torch.autograd.Function):
         g = g.contiguous()
         dev = g.device
         c.set_stream(torch.cuda.current_stream(dev).cuda_stream)
-        # the batch may have been reused since the forward: restore its inputs
-        c.batch_set_params(*[_ptr(t) for t in ins])
+        # restore the inputs if another call changed the batch since this
+        # forward; otherwise the library reuses the forward's device state
+        if c.param_generation != ctx.generation:
+            c.batch_set_params(*[_ptr(t) for t in ins])
         grads = [torch.empty_like(t) if (t is not None and need) else None
                  for t, need in zip(ins, ctx.needs_input_grad[1:7])]
         c.batch_vjp(g.data_ptr(), *[_ptr(t) for t in grads], ord=ctx.ord)

@@ -268,6 +268,7 @@ class Cosmolike:
         arr, keep = self._draw_array(problems)
         self._chk(self.L.clh_batch_upload(self.h, len(problems), arr))
         self.batch_size = len(problems)
+        self.param_generation = getattr(self, "param_generation", 0) + 1
 
     def capture_layout(self):
         """Snapshot of the current survey layout (clh_layout_capture)."""
@@ -300,6 +301,7 @@ class Cosmolike:
         """Device pointers (ints; 0 = keep) of [B][...] parameter arrays."""
         v = [ctypes.c_void_p(x or None) for x in (lnP, src_dz, lens_dz, b1, ia_A, shear_m)]
         self._chk(self.L.clh_batch_set_params(self.h, *v))
+        self.param_generation = getattr(self, "param_generation", 0) + 1
 
     def batch_vjp(self, dv_bar, g_lnP=0, g_src_dz=0, g_lens_dz=0, g_b1=0, g_ia_A=0, g_m=0, ord=(0, 1, 2)):
         """Device pointers (ints; 0 = skip) of the cotangent and the gradient outputs."""
@@ -311,9 +313,12 @@ class Cosmolike:
         self._chk(self.L.clh_synchronize(self.h))
 
     def autotune(self):
-        """Pick the rows per tile of the Limber stages by timing the uploaded batch."""
+        """Pick the rows per tile of the Limber stages (and LDS staging of the
+        gg tables) by timing the uploaded batch; returns the rows per tile."""
         rows = (ctypes.c_int * 4)()
-        self._chk(self.L.clh_autotune(self.h, rows))
+        lds = (ctypes.c_int * 2)()
+        self._chk(self.L.clh_autotune(self.h, rows, lds))
+        self.gg_lds = list(lds)
         return list(rows)
 
     def set_tile_rows(self, rows):

@@ -314,11 +314,12 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
 // the W_gal / W_mag node cotangents and the lens shift through dW_RSD/d dz.
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restrict__ D,
-                                                       const int4* __restrict__ work) {
+                                                       const int4* __restrict__ work, int stage) {
   __shared__ ClhGgRow R[CLH_TILE_ROWS];
   __shared__ double Cr[CLH_TILE_ROWS];
   __shared__ double s_lk[CLH_LDS_LK];
   __shared__ double red[2][3];
+  extern __shared__ double dyn[];  // staged chi(z), G(z), lens n(z) tables
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
   const int zl = w.y, row0 = w.z, nr = w.w;
@@ -327,11 +328,11 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
     R[t] = clh_gg_row(d.ell_gx[row0 + t]);
     Cr[t] = d.cbar[(size_t)(d.n_ss + d.n_gs + zl) * d.nrow + row0 + t];
   }
-  const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
+  const ClhGgTabs T = clh_gg_tabs(d, zl, dyn, stage, s_lk);
   __syncthreads();
   const int p = threadIdx.x % np, half = threadIdx.x / np;
   ClhGgNode<true> nd;
-  nd.init(d, clh_gg_nodes(d, zl), p, zl);
+  nd.init(d, T, clh_gg_nodes(d, zl), p, zl);
   double tg = 0.0, tm = 0.0, b1p = 0.0, lzp = 0.0;
   ClhLnPAcc sc;
   const int hn = (nr + 1) / 2;
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
     const double cb = Cr[rr];
     if (cb == 0.0) continue;
     double t, pf, dwr, dx;
-    if (!nd.eval(d, lk, R[rr], t, pf, dwr, dx)) continue;
+    if (!nd.eval(d, T, R[rr], t, pf, dwr, dx)) continue;
     sc.add(d.g_lnP, d.nzp, nd.j, nd.i, dx, nd.dy, cb * t * t * pf);
     const double tb = cb * 2.0 * t * pf;
     tg += tb;
@@ -453,9 +454,11 @@ hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, 
   hipLaunchKernelGGL(k_limber_sx_adj, dim3(n), dim3(64), lds, s, D, work);
   return hipGetLastError();
 }
-hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
+hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles,
+                                   hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_limber_gg_adj, dim3(n), dim3(192), 0, s, D, work);
+  const size_t lds = sizeof(double) * (size_t)stage_doubles;
+  hipLaunchKernelGGL(k_limber_gg_adj, dim3(n), dim3(192), lds, s, D, work, stage_doubles > 0 ? 1 : 0);
   return hipGetLastError();
 }
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s) {

@@ -194,6 +194,50 @@ __device__ __forceinline__ double clh_amax_lens(const DrawDesc& d, int ni) {
   return 1. / (1 + fmax(zmin - 2. * fabs(d.par->lens_dz[ni]), 0.001));
 }
 
+// ------------------------------------------------ walked-index evaluations
+// The same values as clh_chi_all (chi only), clh_nz_src / clh_nz_lens and
+// their shift derivatives, for a thread that evaluates them along monotone z:
+// the table index is walked from the previous call (start with j = -1).
+__device__ __forceinline__ double clh_chi_w(const DrawDesc& d, double a, int& j) {
+  const double z = 1.0 / a - 1.0;
+  const double* __restrict__ zz = d.chi_z;
+  const double* __restrict__ cc = d.chi_v;
+  j = (j < 0) ? clh_bs(zz, d.chi_n, z) : clh_walk(zz, d.chi_n, z, j);
+  const int jj = j > d.chi_n - 3 ? d.chi_n - 3 : j;
+  const double dy = (z - zz[jj]) / (zz[jj + 1] - zz[jj]);
+  return (cc[jj] + dy * (cc[jj + 1] - cc[jj])) / CLH_COVERH0;
+}
+
+// n and d n / d dz of source bin b at z (D: also the derivative)
+template <bool DER>
+__device__ __forceinline__ void clh_nz_src_w(const DrawDesc& d, double z, int b, int& j, double& n,
+                                             double& dn) {
+  const double zz = z - d.par->src_dz[b];
+  const int rows = d.s_rows;
+  n = dn = 0.0;
+  if (zz < d.s_zc[0] || zz > d.s_zc[rows - 1]) return;
+  j = (j < 0) ? clh_bs(d.s_zc, rows, zz) : clh_walk(d.s_zc, rows, zz, j);
+  const double* c = d.s_coef + (size_t)b * 4 * (rows - 1) + 4 * j;
+  const double x = zz - d.s_zc[j];
+  n = c[0] + x * (c[1] + x * (c[2] + x * c[3]));
+  if (DER) dn = -(c[1] + x * (2.0 * c[2] + 3.0 * x * c[3]));
+}
+
+template <bool DER>
+__device__ __forceinline__ void clh_nz_lens_w(const DrawDesc& d, double z, int b, int& j, double& n,
+                                              double& dn) {
+  const double s = d.par->lens_st[b];
+  const double zz = (z - d.par->lens_dz[b] - d.l_zmean[b]) / s + d.l_zmean[b];
+  const int rows = d.l_rows;
+  n = dn = 0.0;
+  if (zz <= d.l_zc[0] || zz >= d.l_zc[rows - 1]) return;
+  j = (j < 0) ? clh_bs(d.l_zc, rows, zz) : clh_walk(d.l_zc, rows, zz, j);
+  const double* c = d.l_coef + (size_t)b * 4 * (rows - 1) + 4 * j;
+  const double x = zz - d.l_zc[j];
+  n = (c[0] + x * (c[1] + x * (c[2] + x * c[3]))) / s;
+  if (DER) dn = -(c[1] + x * (2.0 * c[2] + 3.0 * x * c[3])) / (s * s);
+}
+
 // ---------------------------------------------------------------- derivatives
 // d/dx of the spline piece: b + dx (2c + 3 dx d)
 __device__ __forceinline__ double clh_spline_d(const double* __restrict__ zc,
@@ -336,15 +380,51 @@ __device__ __forceinline__ ClhGgRow clh_gg_row(double l) {
 // log10 k of P(k,z) (cosmo3D.c:362-417) -- is found by a walk from the
 // previous row, with the reciprocal spacing of the interval recomputed only
 // when the interval changes.  ADJ additionally returns d W_RSD / d dz_lens.
+// The per-draw tables the gg integrand walks (chi(z), G(z), the lens n(z)
+// spline of one bin, log10 k): in LDS when the block staged them, else the
+// draw's own arrays.
+struct ClhGgTabs {
+  const double *cz, *cv, *gz, *gv, *lzc, *lcf, *lk;
+};
+
+// Stages the gg tables of lens bin zl into dyn (2 chi_n + 2 G_n + l_rows +
+// 4 (l_rows - 1) doubles) when stage != 0.  Every thread of the block calls
+// it; the block syncs before use.
+__device__ __forceinline__ ClhGgTabs clh_gg_tabs(const DrawDesc& d, int zl, double* dyn, int stage,
+                                                 double* s_lk) {
+  ClhGgTabs T;
+  const int nr = d.l_rows;
+  const double* lcf = d.l_coef + (size_t)zl * 4 * (nr - 1);
+  T.lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
+  if (!stage) {
+    T.cz = d.chi_z; T.cv = d.chi_v; T.gz = d.G_z; T.gv = d.G_v; T.lzc = d.l_zc; T.lcf = lcf;
+    return T;
+  }
+  double* p = dyn;
+  auto cp = [&](const double* src, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = src[i];
+    const double* r = p;
+    p += n;
+    return r;
+  };
+  T.cz = cp(d.chi_z, d.chi_n);
+  T.cv = cp(d.chi_v, d.chi_n);
+  T.gz = cp(d.G_z, d.G_n);
+  T.gv = cp(d.G_v, d.G_n);
+  T.lzc = cp(d.l_zc, nr);
+  T.lcf = cp(lcf, 4 * (nr - 1));
+  return T;
+}
+
 template <bool ADJ>
 struct ClhGgNode {
   double fK, fac, WG, WM, b1, bmag, l10fk, dy, t0, t0d, inv_st, dzl, zm, chi_amin;
   double inv_c, inv_z, up, down, slG, inv_k;
   int j, zl, jc, jz, jj, jg, jn, i;
 
-  __device__ __forceinline__ void set_z(const DrawDesc& d) {
-    const double* zz = d.chi_z;
-    const double* cc = d.chi_v;
+  __device__ __forceinline__ void set_z(const ClhGgTabs& T) {
+    const double* zz = T.cz;
+    const double* cc = T.cv;
     inv_z = 1.0 / (zz[jj + 1] - zz[jj]);
     up = (cc[jj + 2] - cc[jj]) / (zz[jj + 2] - zz[jj]);
     down = (jj > 0) ? (cc[jj + 1] - cc[jj - 1]) / (zz[jj + 1] - zz[jj - 1]) : (cc[jj + 1] - cc[jj]) * inv_z;
@@ -353,7 +433,8 @@ struct ClhGgNode {
   // o: the node block (a, fK, fac, W_gal, W_mag, then the search results of
   // k_nodes: P(k,z) z index and weight, log10(chi c/H0), the a_0 side of W_RSD
   // and the table indices at a_0 from which the walks start)
-  __device__ __forceinline__ void init(const DrawDesc& d, const double* __restrict__ o, int p, int zl_) {
+  __device__ __forceinline__ void init(const DrawDesc& d, const ClhGgTabs& T, const double* __restrict__ o,
+                                      int p, int zl_) {
     constexpr int np = CLH_NP_GG;
     zl = zl_;
     fK = o[np + p];
@@ -376,56 +457,56 @@ struct ClhGgNode {
     dzl = d.par->lens_dz[zl];
     zm = d.l_zmean[zl];
     chi_amin = d.chi_amin;
-    inv_c = 1.0 / (d.chi_v[jc + 1] - d.chi_v[jc]);
+    inv_c = 1.0 / (T.cv[jc + 1] - T.cv[jc]);
     jj = jz > d.chi_n - 3 ? d.chi_n - 3 : jz;
-    set_z(d);
-    slG = (d.G_v[jg + 1] - d.G_v[jg]) / (d.G_z[jg + 1] - d.G_z[jg]);
+    set_z(T);
+    slG = (T.gv[jg + 1] - T.gv[jg]) / (T.gz[jg + 1] - T.gz[jg]);
     i = -1;
     inv_k = 0.0;
   }
 
   // false: chi_1 beyond chi(a_min), integrand zero (cosmo2D.c:2512-2514).
   // t: the bracket; pf = P fac; dx: log10 k weight of the P cell (i, j).
-  // lk: the draw's log10 k table (in LDS when it fits).  F32: P(k,z) as
-  // expf of the FP64 bilinear value.
+  // T: the draw's tables (staged in LDS when they fit).  F32: P(k,z) as expf
+  // of the FP64 bilinear value.
   template <bool F32 = false>
-  __device__ __forceinline__ bool eval(const DrawDesc& d, const double* __restrict__ lk, const ClhGgRow& R,
+  __device__ __forceinline__ bool eval(const DrawDesc& d, const ClhGgTabs& T, const ClhGgRow& R,
                                        double& t, double& pf, double& dwr, double& dx) {
     const double chi_1 = fK * R.rr1;
     if (chi_1 > chi_amin) return false;
     const double c1v = chi_1 * CLH_COVERH0;
     {
       const int j0 = jc;
-      jc = clh_walk(d.chi_v, d.chi_n, c1v, jc);
-      if (jc != j0) inv_c = 1.0 / (d.chi_v[jc + 1] - d.chi_v[jc]);
+      jc = clh_walk(T.cv, d.chi_n, c1v, jc);
+      if (jc != j0) inv_c = 1.0 / (T.cv[jc + 1] - T.cv[jc]);
     }
-    const double z1 = d.chi_z[jc] + (c1v - d.chi_v[jc]) * inv_c * (d.chi_z[jc + 1] - d.chi_z[jc]);
-    jz = clh_walk(d.chi_z, d.chi_n, z1, jz);
+    const double z1 = T.cz[jc] + (c1v - T.cv[jc]) * inv_c * (T.cz[jc + 1] - T.cz[jc]);
+    jz = clh_walk(T.cz, d.chi_n, z1, jz);
     {
       const int j2 = jz > d.chi_n - 3 ? d.chi_n - 3 : jz;
       if (j2 != jj) {
         jj = j2;
-        set_z(d);
+        set_z(T);
       }
     }
-    const double Dz = down + (z1 - d.chi_z[jj]) * inv_z * (up - down);  // dchi/dz
+    const double Dz = down + (z1 - T.cz[jj]) * inv_z * (up - down);  // dchi/dz
     {
       const int j0 = jg;
-      jg = clh_walk(d.G_z, d.G_n, z1, jg);
-      if (jg != j0) slG = (d.G_v[jg + 1] - d.G_v[jg]) / (d.G_z[jg + 1] - d.G_z[jg]);
+      jg = clh_walk(T.gz, d.G_n, z1, jg);
+      if (jg != j0) slG = (T.gv[jg + 1] - T.gv[jg]) / (T.gz[jg + 1] - T.gz[jg]);
     }
-    const double G = d.G_v[jg] + (z1 - d.G_z[jg]) * slG;
+    const double G = T.gv[jg] + (z1 - T.gz[jg]) * slG;
     // hoverh0 * f_growth = (c/H0)/(dchi/dz) * (1 - G'(1+z)/G)
     const double hf1 = CLH_COVERH0 * (G - slG * (1.0 + z1)) / (Dz * G);
     double n1 = 0.0, dn1 = 0.0;
     {
       const double zs = (z1 - dzl - zm) * inv_st + zm;
       const int nr = d.l_rows;
-      if (!(zs <= d.l_zc[0] || zs >= d.l_zc[nr - 1])) {
-        jn = clh_walk(d.l_zc, nr, zs, jn);
+      if (!(zs <= T.lzc[0] || zs >= T.lzc[nr - 1])) {
+        jn = clh_walk(T.lzc, nr, zs, jn);
         const int jn2 = jn > nr - 2 ? nr - 2 : jn;
-        const double* cf = d.l_coef + (size_t)zl * 4 * (nr - 1) + 4 * jn2;
-        const double x = zs - d.l_zc[jn2];
+        const double* cf = T.lcf + 4 * jn2;
+        const double x = zs - T.lzc[jn2];
         n1 = (cf[0] + x * (cf[1] + x * (cf[2] + x * cf[3]))) * inv_st;
         if (ADJ) dn1 = -(cf[1] + x * (2.0 * cf[2] + 3.0 * x * cf[3])) * (inv_st * inv_st);
       }
@@ -436,10 +517,10 @@ struct ClhGgNode {
     const double log10k = R.l10 - l10fk;
     {
       const int i0 = i;
-      i = (i < 0) ? clh_bs(lk, d.nk, log10k) : clh_walk(lk, d.nk, log10k, i);
-      if (i != i0) inv_k = 1.0 / (lk[i + 1] - lk[i]);
+      i = (i < 0) ? clh_bs(T.lk, d.nk, log10k) : clh_walk(T.lk, d.nk, log10k, i);
+      if (i != i0) inv_k = 1.0 / (T.lk[i + 1] - T.lk[i]);
     }
-    dx = (log10k - lk[i]) * inv_k;
+    dx = (log10k - T.lk[i]) * inv_k;
     const int nz = d.nzp;
     const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
     const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +

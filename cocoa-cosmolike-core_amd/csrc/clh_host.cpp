@@ -22,7 +22,8 @@
 hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, hipStream_t s);
-hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, hipStream_t s);
+hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles,
+                                hipStream_t s);
 hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow, hipStream_t s);
 hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_fill(double* p, size_t n, double v, hipStream_t s);
@@ -33,7 +34,8 @@ hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int2* work, int n, hi
 hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
 hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles,
                                    hipStream_t s);
-hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s);
+hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles,
+                                   hipStream_t s);
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_legendre_mx(const DrawDesc* D, const int4* work, int n, int mode, hipStream_t s);
 hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, int max_nrow,
@@ -209,6 +211,11 @@ struct Batch {
   DBuf<int4> d_wl, d_wsx, d_wgg, d_wga, d_wmx;
   int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0, n_wsx = 0, n_wgg = 0, n_wga = 0, n_wmx = 0;
   int nrow_custom = 0;  // rows of a custom-ell batch (0: the layouts' rows)
+  int gg_stage = 0;     // LDS doubles for the gg kernels' staged tables (0: not staged)
+  // the device state (lens-efficiency tables, node weights, ss/gs C_ell rows)
+  // matches the draws' current parameters in FP64: set by an FP64 forward or
+  // a VJP, cleared by uploads and parameter updates
+  bool state_valid = false;
   int dv_size = 0;
   int nknz = 0;        // common ln P table size (0 if the draws differ)
   int ord[3] = {0, 1, 2};
@@ -232,6 +239,7 @@ struct clh_ctx {
   int legendre_mode = CLH_LEGENDRE_FOLDED;
   int precision = CLH_PREC_FP64;
   int tile_rows[4] = {16, 64, 64, 64};  // rows per tile: fwd ss/gs, fwd gg, adj ss/gs, adj gg
+  int gg_lds[2] = {0, 1};               // stage tables in LDS: fwd gg, adj gg
   uint64_t version = 1, cached_version = 0;
   dvec cache_raw;   // raw (uncalibrated, ord 0,1,2) data vector of `cur`
   dvec cache_cl;    // cl rows of `cur`
@@ -909,6 +917,17 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   bt.dv_off = dvoff;
   bt.dv_size = lays[0]->dv_size();
   bt.nrow_custom = ells ? (int)ells->size() : 0;
+  bt.state_valid = false;
+  {  // the gg kernels stage chi(z), G(z) and one lens n(z) spline in LDS when
+     // the largest draw's tables fit in 44 KB (64 KB per block with the static LDS)
+    size_t st = 0;
+    for (int i = 0; i < B; i++) {
+      const Layout& L = *lays[i];
+      const size_t lr = L.lens.nbin > 0 ? (size_t)L.lens.rows : 0;
+      st = std::max(st, 2 * draws[i]->chi_z.size() + 2 * draws[i]->G_z.size() + (lr ? lr + 4 * (lr - 1) : 0));
+    }
+    bt.gg_stage = st <= 5632 ? (int)st : 0;
+  }
   if ((rc = build_work(c, bt, c->tile_rows))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CLH_OK;
@@ -938,7 +957,7 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], s));
   const bool mixed = c->precision != CLH_PREC_FP64;
-  HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, mixed, s));
+  HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, mixed, c->gg_lds[0] ? bt.gg_stage : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], s));
   HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, mixed, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
@@ -951,6 +970,7 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
       HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s));
   }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[5], s));
+  bt.state_valid = !mixed;
   return CLH_OK;
 }
 
@@ -1024,6 +1044,7 @@ int clh_set_precision(clh_ctx* c, int mode) {
   if (mode != c->precision) {
     c->precision = mode;
     c->version++;
+    c->batch.state_valid = false;
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   }
   return CLH_OK;
@@ -1424,6 +1445,7 @@ int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
       c->gstream = c->stream;
     }
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    c->batch.state_valid = c->precision == CLH_PREC_FP64;
     return CLH_OK;
   }
   return run_batch(c, c->batch, true);
@@ -1501,7 +1523,7 @@ int clh_get_tile_rows(clh_ctx* c, int rows[4]) {
   return CLH_OK;
 }
 
-int clh_autotune(clh_ctx* c, int rows_out[4]) {
+int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   if (!c) return CLH_ERR_ARG;
   Batch& bt = c->batch;
   if (bt.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
@@ -1519,35 +1541,55 @@ int clh_autotune(clh_ctx* c, int rows_out[4]) {
     HIPCHK(c, clh_launch_fill(dvbar.p, (size_t)bt.dv_off[bt.B], 1.0, c->stream));
   }
   const bool was_timing = c->timing;
+  const int lds_was[2] = {c->gg_lds[0], c->gg_lds[1]};
   c->timing = true;
   static const int cand[3] = {16, 32, 64};
   float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
-  int pick[4];
+  int pick[4], pick_lds[2] = {c->gg_lds[0], c->gg_lds[1]};
   for (int k = 0; k < 4; k++) pick[k] = c->tile_rows[k];
   const int ord[3] = {0, 1, 2};
+  const int nst = bt.gg_stage > 0 ? 2 : 1;  // LDS staging of the gg tables is an option when they fit
   int rc = CLH_OK;
   for (int k = 0; k < 3 && rc == CLH_OK; k++) {
     const int t[4] = {cand[k], cand[k], cand[k], cand[k]};
     if ((rc = build_work(c, bt, t))) break;
-    for (int rep = 0; rep < 3 && rc == CLH_OK; rep++) {
-      if ((rc = run_batch(c, bt, false))) break;
-      if (adj && (rc = clh_batch_vjp(c, ord, dvbar.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr)))
-        break;
-      if ((rc = clh_synchronize(c))) break;
-      const float ms[4] = {c->last_ms[3], c->last_ms[2], adj ? c->last_vjp_ms[1] : 1e30f,
-                           adj ? c->last_vjp_ms[0] : 1e30f};
-      for (int s2 = 0; s2 < 4; s2++)
-        if (ms[s2] < best[s2]) { best[s2] = ms[s2]; pick[s2] = cand[k]; }
+    for (int st = 0; st < nst && rc == CLH_OK; st++) {
+      c->gg_lds[0] = c->gg_lds[1] = st;
+      for (int rep = 0; rep < 3 && rc == CLH_OK; rep++) {
+        if ((rc = run_batch(c, bt, false))) break;
+        if (adj && (rc = clh_batch_vjp(c, ord, dvbar.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr)))
+          break;
+        if ((rc = clh_synchronize(c))) break;
+        const float ms[4] = {c->last_ms[3], c->last_ms[2], adj ? c->last_vjp_ms[1] : 1e30f,
+                             adj ? c->last_vjp_ms[0] : 1e30f};
+        for (int s2 = 0; s2 < 4; s2++) {
+          if (ms[s2] < best[s2]) {
+            best[s2] = ms[s2];
+            pick[s2] = cand[k];
+            if (s2 == 1) pick_lds[0] = st;
+            if (s2 == 3) pick_lds[1] = st;
+          }
+        }
+      }
     }
   }
   c->timing = was_timing;
-  if (rc) { (void)build_work(c, bt, c->tile_rows); return rc; }
+  if (rc) {
+    c->gg_lds[0] = lds_was[0];
+    c->gg_lds[1] = lds_was[1];
+    (void)build_work(c, bt, c->tile_rows);
+    return rc;
+  }
   for (int k = 0; k < 4; k++) c->tile_rows[k] = pick[k];
+  c->gg_lds[0] = pick_lds[0];
+  c->gg_lds[1] = pick_lds[1];
   if ((rc = build_work(c, bt, c->tile_rows))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (rows_out)
     for (int k = 0; k < 4; k++) rows_out[k] = pick[k];
+  if (lds_out)
+    for (int k = 0; k < 2; k++) lds_out[k] = pick_lds[k];
   return CLH_OK;
 }
 
@@ -1566,6 +1608,7 @@ int clh_batch_set_params(clh_ctx* c, const double* lnP, const double* src_dz, co
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, clh_launch_set_params(bt.d_desc.p, bt.d_par.p, bt.B, bt.nknz, lnP, src_dz, lens_dz, b1, ia_A,
                                   shear_m, c->stream));
+  bt.state_valid = false;
   return CLH_OK;
 }
 
@@ -1630,19 +1673,21 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // forward recompute: lens-efficiency tables and their shift tangents, node
   // weights, and the ss / gs C_ell rows (the shear-calibration gradient needs
   // them; the gg rows carry no calibration and are not recomputed)
-  HIPCHK(c, clh_launch_gtab(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
-  HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
-  HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
-  HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, 0, s));
+  HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));  // tables and shift tangents
+  if (!bt.state_valid) {  // else the last FP64 forward / VJP left them for these parameters
+    HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
+    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, 0, s));
+  }
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa.p, bt.n_wa, s));
   HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));
-  HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga, s));
+  HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga, c->gg_lds[1] ? bt.gg_stage : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
   HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl, bt.adj_lds, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
   c->vjp_timed = c->timing;
+  bt.state_valid = true;
   HIPCHK(c, clh_launch_nodes_adj(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   return CLH_OK;
 }

@@ -49,14 +49,18 @@ __device__ __forceinline__ double clh_block_sum(double v, double* lds) {
 
 // ----------------------------------------------------------------------------
 // k_gtab: one block per (draw, bin).  bin < ns -> g_tomo of source bin,
-// otherwise g_lens of lens bin (bin - ns).
+// otherwise g_lens of lens bin (bin - ns).  Thread t integrates interval t of
+// the fine grid (x = 25 points, trapezoids of n/a^2 and n/(chi a^2)), walking
+// the n(z) spline and chi(z) indices along its points; a block scan turns
+// the interval sums into the table.
+// TANGENT: also the same prefix sums over d(n)/d(photo-z shift of the bin),
+// the derivative of the table with respect to that shift (into dgtab).
 // ----------------------------------------------------------------------------
-// TANGENT: the same prefix sums over d(integrand)/d(photo-z shift of the bin),
-// i.e. the derivative of the table with respect to that shift.
 template <bool TANGENT>
 __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__ D,
                                                     const int2* __restrict__ work) {
   __shared__ double sP[1024], sQ[1024];
+  __shared__ double sPd[TANGENT ? 1024 : 1], sQd[TANGENT ? 1024 : 1];
   __shared__ double red[8];
   const int2 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
@@ -70,47 +74,71 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   const double da = (amax - amin) / ((double)Na - 1.0);
 
   // g_lens pre-integral from a_min of the shear sample (redshift_spline.c:1052-1071)
-  double P0 = 0.0, Q0 = 0.0;
+  double P0 = 0.0, Q0 = 0.0, P0d = 0.0, Q0d = 0.0;
   if (!src) {
     const double amin_shear = 1.0 / (d.s_zmax_all + 1.0);
     const int Na0 = (amin_shear < amin) ? (int)ceil((amin - amin_shear) / da) + 1 : 1;
     const double da0 = (Na0 > 1) ? (amin - amin_shear) / ((double)Na0 - 1.) : 0.0;
-    double p = 0.0, q = 0.0;
+    double p = 0.0, q = 0.0, pd = 0.0, qd = 0.0;
     for (int i = 1 + threadIdx.x; i < Na0; i += CLH_BLOCK) {
       const double apl = amin_shear + (i - 1) * da0;
       const double aph = amin_shear + i * da0;
-      const double fl = (TANGENT ? clh_dnz_lens(d, 1.0 / apl - 1.0, b)
-                                 : clh_nz_lens(d, 1.0 / apl - 1.0, b)) / (apl * apl);
-      const double fh = (TANGENT ? clh_dnz_lens(d, 1.0 / aph - 1.0, b)
-                                 : clh_nz_lens(d, 1.0 / aph - 1.0, b)) / (aph * aph);
+      const double fl = clh_nz_lens(d, 1.0 / apl - 1.0, b) / (apl * apl);
+      const double fh = clh_nz_lens(d, 1.0 / aph - 1.0, b) / (aph * aph);
       p += 0.5 * da0 * (fl + fh);
       q += 0.5 * da0 * (fl / clh_chi(d, apl) + fh / clh_chi(d, aph));
+      if (TANGENT) {
+        const double gl = clh_dnz_lens(d, 1.0 / apl - 1.0, b) / (apl * apl);
+        const double gh = clh_dnz_lens(d, 1.0 / aph - 1.0, b) / (aph * aph);
+        pd += 0.5 * da0 * (gl + gh);
+        qd += 0.5 * da0 * (gl / clh_chi(d, apl) + gh / clh_chi(d, aph));
+      }
     }
     P0 = clh_block_sum(p, red);
     Q0 = clh_block_sum(q, red);
+    if (TANGENT) {
+      P0d = clh_block_sum(pd, red);
+      Q0d = clh_block_sum(qd, red);
+    }
   }
 
   // per-interval trapezoid sums of n/a^2 and n/(chi a^2)
   for (int t = threadIdx.x; t < nint; t += CLH_BLOCK) {
-    double p = 0.0, q = 0.0;
-    double fprev = 0.0, gprev = 0.0;
-    for (int s = 0; s <= x; s++) {
-      const int i = t * x + s;
+    double p = 0.0, q = 0.0, pd = 0.0, qd = 0.0;
+    double fprev = 0.0, gprev = 0.0, fdprev = 0.0, gdprev = 0.0;
+    int jn = -1, jc = -1;
+    for (int s2 = 0; s2 <= x; s2++) {
+      const int i = t * x + s2;
       const double a = amin + i * da;
-      double n;
-      if (TANGENT) n = src ? clh_dnz_src(d, 1. / a - 1., b) : clh_dnz_lens(d, 1.0 / a - 1.0, b);
-      else n = src ? clh_nz_src(d, 1. / a - 1., b) : clh_nz_lens(d, 1.0 / a - 1.0, b);
+      double n, dn;
+      if (src) clh_nz_src_w<TANGENT>(d, 1. / a - 1., b, jn, n, dn);
+      else clh_nz_lens_w<TANGENT>(d, 1.0 / a - 1.0, b, jn, n, dn);
+      const double chi = clh_chi_w(d, amin + i * da, jc);
       const double f = n / (a * a);
-      const double g = f / clh_chi(d, amin + i * da);
-      if (s > 0) {
+      const double g = f / chi;
+      if (s2 > 0) {
         p += 0.5 * da * (fprev + f);
         q += 0.5 * da * (gprev + g);
       }
       fprev = f;
       gprev = g;
+      if (TANGENT) {
+        const double fd = dn / (a * a);
+        const double gd = fd / chi;
+        if (s2 > 0) {
+          pd += 0.5 * da * (fdprev + fd);
+          qd += 0.5 * da * (gdprev + gd);
+        }
+        fdprev = fd;
+        gdprev = gd;
+      }
     }
     sP[t] = p;
     sQ[t] = q;
+    if (TANGENT) {
+      sPd[t] = pd;
+      sQd[t] = qd;
+    }
   }
   __syncthreads();
 
@@ -118,19 +146,44 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   const int cpt = (nint + CLH_BLOCK - 1) / CLH_BLOCK;
   const int t0 = threadIdx.x * cpt;
   const int t1 = min(t0 + cpt, nint);
-  double cp = 0.0, cq = 0.0;
-  for (int t = t0; t < t1; t++) { cp += sP[t]; cq += sQ[t]; }
+  double cp = 0.0, cq = 0.0, cpd = 0.0, cqd = 0.0;
+  for (int t = t0; t < t1; t++) {
+    cp += sP[t];
+    cq += sQ[t];
+    if (TANGENT) {
+      cpd += sPd[t];
+      cqd += sQd[t];
+    }
+  }
   const double ip = clh_block_scan(cp, red);
   const double iq = clh_block_scan(cq, red);
   double P = P0 + ip - cp, Q = Q0 + iq - cq;
-  double* tab = (TANGENT ? d.dgtab : d.gtab) + (size_t)w.y * d.n_a;
+  double Pd = 0.0, Qd = 0.0;
+  if (TANGENT) {
+    const double ipd = clh_block_scan(cpd, red);
+    const double iqd = clh_block_scan(cqd, red);
+    Pd = P0d + ipd - cpd;
+    Qd = Q0d + iqd - cqd;
+  }
+  double* tab = d.gtab + (size_t)w.y * d.n_a;
+  double* dtab = TANGENT ? d.dgtab + (size_t)w.y * d.n_a : nullptr;
   for (int t = t0; t < t1; t++) {
     P += sP[t];
     Q += sQ[t];
     const int i = (t + 1) * x;
-    tab[t + 1] = P - clh_chi(d, amin + i * da) * Q;
+    const double chi = clh_chi(d, amin + i * da);
+    tab[t + 1] = P - chi * Q;
+    if (TANGENT) {
+      Pd += sPd[t];
+      Qd += sQd[t];
+      dtab[t + 1] = Pd - chi * Qd;
+    }
   }
-  if (threadIdx.x == 0) tab[0] = src ? 0.0 : P0 - clh_chi(d, amin) * Q0;
+  if (threadIdx.x == 0) {
+    const double chi0 = clh_chi(d, amin);
+    tab[0] = src ? 0.0 : P0 - chi0 * Q0;
+    if (TANGENT) dtab[0] = src ? 0.0 : P0d - chi0 * Q0d;
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -363,27 +416,29 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
 // ----------------------------------------------------------------------------
 // k_limber_gg: gg tiles.  Work item (draw, zl, row0, nrows); 192 threads = two
 // groups of 96 (one thread per GL node, ClhGgNode started from the k_nodes
-// search results).  Sub-tile s covers rows 16 s .. 16 s + 15 of the tile:
+// search results; chi(z), G(z) and the lens n(z) spline staged in LDS when
+// stage != 0).  Sub-tile s covers rows 16 s .. 16 s + 15 of the tile:
 // group h takes rows 16 s + 8 h .. + 7, so each thread still walks its rows in
 // increasing ell.  Row sums over the 96 nodes: one wave per row.
 // F32: P(k,z) as expf of the FP64 bilinear value.
 // ----------------------------------------------------------------------------
 template <bool F32>
 __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ D,
-                                                   const int4* __restrict__ work) {
+                                                   const int4* __restrict__ work, int stage) {
   __shared__ double V[CLH_SUB_ROWS * CLH_NP_GG];
   __shared__ ClhGgRow R[CLH_TILE_ROWS];
   __shared__ double s_lk[CLH_LDS_LK];
+  extern __shared__ double dyn[];  // staged chi(z), G(z), lens n(z) tables
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
   const int zl = w.y, row0 = w.z, nr = w.w;
   constexpr int np = CLH_NP_GG;
   for (int t = threadIdx.x; t < nr; t += 192) R[t] = clh_gg_row(d.ell_gx[row0 + t]);
-  const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
+  const ClhGgTabs T = clh_gg_tabs(d, zl, dyn, stage, s_lk);
   __syncthreads();
   const int p = threadIdx.x % np, half = threadIdx.x / np;
   ClhGgNode<false> nd;
-  nd.init(d, clh_gg_nodes(d, zl), p, zl);
+  nd.init(d, T, clh_gg_nodes(d, zl), p, zl);
   constexpr int per = CLH_SUB_ROWS / 2;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int sub = 0; sub * CLH_SUB_ROWS < nr; sub++) {
@@ -393,7 +448,7 @@ __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ 
       double v = 0.0;
       if (base + rl < nr) {
         double t, pf, dwr, dx;
-        if (nd.template eval<F32>(d, lk, R[base + rl], t, pf, dwr, dx)) v = t * t * pf;
+        if (nd.template eval<F32>(d, T, R[base + rl], t, pf, dwr, dx)) v = t * t * pf;
       }
       V[rl * np + p] = v;
     }
@@ -562,9 +617,12 @@ hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int 
   else hipLaunchKernelGGL(k_limber_sx<false>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }
-hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, hipStream_t s) {
+hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles,
+                                hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (f32) hipLaunchKernelGGL(k_limber_gg<true>, dim3(n), dim3(192), 0, s, D, work);
-  else hipLaunchKernelGGL(k_limber_gg<false>, dim3(n), dim3(192), 0, s, D, work);
+  const size_t lds = sizeof(double) * (size_t)stage_doubles;
+  const int stage = stage_doubles > 0;
+  if (f32) hipLaunchKernelGGL(k_limber_gg<true>, dim3(n), dim3(192), lds, s, D, work, stage);
+  else hipLaunchKernelGGL(k_limber_gg<false>, dim3(n), dim3(192), lds, s, D, work, stage);
   return hipGetLastError();
 }

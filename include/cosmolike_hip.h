@@ -183,15 +183,17 @@ int clh_batch_vjp(clh_ctx* ctx, const int ord[3], const double* dv_bar, double* 
 /* Rows per work tile of the four Limber stages: rows[0] forward ss/gs,
  * rows[1] forward gg, rows[2] adjoint ss/gs, rows[3] adjoint gg (each
  * 1..64; the adjoint ss/gs tile is further capped so its cotangents fit in
- * 64 KB of LDS).  clh_autotune times 16, 32 and 64 rows for every stage on
- * the uploaded batch (HIP events, best of 3; the adjoint stages only for a
- * batch with one layout, with a unit cotangent) and keeps the fastest,
- * returning the choice in rows_out (may be NULL).  It overwrites the batch's
- * own output buffer and invalidates a captured graph.  No reference
- * counterpart. */
+ * 64 KB of LDS).  clh_autotune times 16, 32 and 64 rows for every stage, and
+ * for the two gg stages with and without their chi(z) / G(z) / lens n(z)
+ * tables staged in LDS, on the uploaded batch (HIP events, best of 3; the
+ * adjoint stages only for a batch with one layout, with a unit cotangent),
+ * keeps the fastest and returns the choice in rows_out and lds_out
+ * ([forward gg, adjoint gg], 1 = staged; either may be NULL).  It overwrites
+ * the batch's own output buffer and invalidates a captured graph.  No
+ * reference counterpart. */
 int clh_set_tile_rows(clh_ctx* ctx, const int rows[4]);
 int clh_get_tile_rows(clh_ctx* ctx, int rows[4]);
-int clh_autotune(clh_ctx* ctx, int rows_out[4]);
+int clh_autotune(clh_ctx* ctx, int rows_out[4], int lds_out[2]);
 /* per-kernel timing of the last clh_batch_forward (and, after a
  * clh_batch_vjp, its two Limber adjoint stages) with HIP events on the
  * context stream: names[i], milliseconds[i]; returns the kernel count */

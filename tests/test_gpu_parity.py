@@ -200,3 +200,16 @@ def test_error_paths_leave_output(full):
     c.set_distances(p.chi_z[:420], p.chi[:420])
     rc = c.L.clh_compute_3x2pt_real(c.h, o, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
     assert rc == -4 and np.all(out == 123.0)
+
+
+def test_large_tables_not_staged():
+    # chi(z) and G(z) tables too large for the gg kernels' LDS staging (the
+    # kernels then walk the tables in global memory): same bar
+    p = synth.full_3x2pt_problem()
+    z = np.linspace(0.0, p.chi_z[-1], 3000)
+    p.chi_z, p.chi = z, np.interp(z, p.chi_z, p.chi)
+    p.G_z, p.G = z, np.interp(z, p.G_z, p.G)
+    ref = O.Oracle().load(p).datavector()
+    got = Cosmolike().load(p).compute_3x2pt_real()
+    print("err", rel_err(got, ref))
+    assert rel_err(got, ref) < RTOL

@@ -143,3 +143,27 @@ def test_vjp_vs_oracle_fd(setup):
         checks.append((g[name][0][j], fd, f"{name}[{j}]"))
     for got, fd, name in checks:
         assert abs(got - fd) <= TOL * abs(fd), (name, got, fd)
+
+
+def test_vjp_reuse_matches_recompute(setup):
+    """A VJP right after an FP64 forward reuses its device state (node
+    weights, C_ell rows); after a parameter update it recomputes them."""
+    import torch
+    c, draws, P, v = setup
+    B, n = v.shape
+    g = {k: torch.empty_like(x) for k, x in P.items()}
+    ptrs = [g[k].data_ptr() for k in ("lnP", "src_dz", "lens_dz", "b1", "ia_A", "shear_m")]
+    out = torch.empty(B * n, dtype=torch.float64, device="cuda:0")
+    c.batch_set_params(*[P[k].data_ptr() for k in ("lnP", "src_dz", "lens_dz", "b1", "ia_A", "shear_m")])
+    c.batch_forward(out.data_ptr())
+    c.batch_vjp(v.data_ptr(), *ptrs)
+    c.synchronize()
+    torch.cuda.synchronize()
+    reuse = {k: x.cpu().numpy().copy() for k, x in g.items()}
+    c.batch_set_params(*[P[k].data_ptr() for k in ("lnP", "src_dz", "lens_dz", "b1", "ia_A", "shear_m")])
+    c.batch_vjp(v.data_ptr(), *ptrs)
+    c.synchronize()
+    torch.cuda.synchronize()
+    for k, x in g.items():
+        r = x.cpu().numpy()
+        assert np.abs(r - reuse[k]).max() <= 1e-11 * np.abs(r).max(), k
