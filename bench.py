@@ -85,6 +85,19 @@ def probe_errors(dv, ref, ns, nl, nt):
     return out
 
 
+def measured_traffic(kernel, batch, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE scaled by a plain device copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"projection_3x2pt_b{batch}_{precision}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def cpu_baseline(draws, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the FP64 CPU restatement, timed as the CPU reference path
@@ -118,6 +131,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--vjp", type=int, default=1, help="also time the batched VJP")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="also copy 512 MiB device to device once (HBM-counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--ragged", type=int, default=512,
                     help="draws of the multi-layout leg (3-10 src x 2-6 lens bins, 20-30 theta bins; 0 = skip)")
     args = ap.parse_args()
@@ -160,6 +175,13 @@ def main():
     c.synchronize()
     torch.cuda.synchronize()
     log("bench: warmup done")
+    if args.traffic_probe:
+        src = torch.ones(1 << 26, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: traffic probe copied {src.numel() * 8} bytes device to device")
+        del src, dst
     t0 = time.perf_counter()
     for _ in range(args.steps):
         c.batch_forward(dv.data_ptr())
@@ -226,7 +248,8 @@ def main():
     flop = flop1 * args.batch
     achieved = flop / (acc[dom] * 1e-3) / 1e12 if acc[dom] > 0 else 0.0
     roofline = {"bound": "mfma" if "legendre" in dom else "valu", "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None, "kernel": dom,
+                "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": measured_traffic(dom, args.batch, args.precision), "kernel": dom,
                 "kernel_ms": acc[dom], "kernel_ms_all": acc, "vjp_kernel_ms": vjp_ms}
     log(f"bench: kernel ms {acc}")
 
