@@ -225,6 +225,8 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
   const double fac = o[2 * np + p];
   const int j = (int)lut[p];
   const double dy = lut[np + p], l10fk = lut[2 * np + p];
+  ClhPkNode pkn;
+  pkn.init(d, j, dy);
   const int woff = ss ? 3 : 5;
   double Wn[CLH_MAXB], acc[CLH_MAXB];
 #pragma unroll
@@ -236,18 +238,11 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
   const double b1 = ss ? 0.0 : d.par->b1[zl], bmag = ss ? 0.0 : d.par->bmag[zl];
   double twg = 0.0, twm = 0.0;
   ClhLnPAcc sc;
-  int i = -1;
-  double inv_k = 0.0;
   for (int rr = 0; rr < T; rr++) {
     const double l = ells[row0 + rr], ell = l + 0.5;
-    const double log10k = L10[rr] - l10fk;
-    const int i0 = i;
-    i = (i < 0) ? clh_bs(lk, d.nk, log10k) : clh_walk(lk, d.nk, log10k, i);
-    if (i != i0) inv_k = 1.0 / (lk[i + 1] - lk[i]);
-    const double dx = (log10k - lk[i]) * inv_k;
-    const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
-    const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +
-                       dx * dy * P[nz + 1];
+    double dx;
+    const double out = pkn.lnp(lk, L10[rr] - l10fk, dx);
+    const int i = pkn.i;
     const double S = exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac;
     double g;
     if (ss) {
@@ -342,14 +337,14 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
     if (cb == 0.0) continue;
     double t, pf, dwr, dx;
     if (!nd.eval(d, T, R[rr], t, pf, dwr, dx)) continue;
-    sc.add(d.g_lnP, d.nzp, nd.j, nd.i, dx, nd.dy, cb * t * t * pf);
+    sc.add(d.g_lnP, d.nzp, nd.pk.j, nd.pk.i, dx, nd.pk.dy, cb * t * t * pf);
     const double tb = cb * 2.0 * t * pf;
     tg += tb;
     tm += tb * R[rr].pref;
     b1p += tb * nd.WG;
     lzp += tb * dwr;
   }
-  sc.done(d.g_lnP, d.nzp, nd.j);
+  sc.done(d.g_lnP, d.nzp, nd.pk.j);
   const double b1 = nd.b1, bmag = nd.bmag;
   const size_t base = (size_t)2 * d.nl * CLH_NP_SS + (size_t)zl * 2 * np;
   if (tg != 0.0) atomicAdd(d.wbar + base + p, tg * b1);

@@ -416,18 +416,100 @@ __device__ __forceinline__ ClhGgTabs clh_gg_tabs(const DrawDesc& d, int zl, doub
   return T;
 }
 
+// The P(k,z) lookup of one GL node (fixed z column j, weight dy) along
+// increasing log10 k (p_nonlin, cosmo3D.c:362-417): the bracketing log10 k
+// values, the reciprocal spacing and the four ln P values of the current cell
+// stay in registers; memory is read only when log10 k leaves the cell (the
+// index is the one clh_bs would return).
+struct ClhPkNode {
+  int i, j, nk, nz;
+  double dy, lo, hi, inv_k, p00, p01, p10, p11;
+  const double* __restrict__ lnP;
+
+  __device__ __forceinline__ void init(const DrawDesc& d, int j_, double dy_) {
+    i = -1;
+    j = j_;
+    dy = dy_;
+    nk = d.nk;
+    nz = d.nzp;
+    lnP = d.lnP;
+  }
+  __device__ __forceinline__ void set_cell(const double* __restrict__ lk) {
+    lo = lk[i];
+    hi = lk[i + 1];
+    inv_k = 1.0 / (hi - lo);
+    const double* __restrict__ P = lnP + (size_t)i * nz + j;
+    p00 = P[0];
+    p01 = P[1];
+    p10 = P[nz];
+    p11 = P[nz + 1];
+  }
+  // ln P at log10 k; dx: the log10 k weight of the cell (i, j)
+  __device__ __forceinline__ double lnp(const double* __restrict__ lk, double log10k, double& dx) {
+    if (i < 0) {
+      i = clh_bs(lk, nk, log10k);
+      set_cell(lk);
+    } else if (!((log10k >= lo || i == 0) && (log10k < hi || i == nk - 2))) {
+      i = clh_walk(lk, nk, log10k, i);
+      set_cell(lk);
+    }
+    dx = (log10k - lo) * inv_k;
+    return (1 - dx) * (1 - dy) * p00 + (1 - dx) * dy * p01 + dx * (1 - dy) * p10 + dx * dy * p11;
+  }
+};
+
+// One bracketed walk over a strictly increasing table: the index clh_bs
+// would return for v, found from the previous one; true when it moved.
+struct ClhBracket {
+  int j;
+  double lo, hi;
+  __device__ __forceinline__ void set(const double* __restrict__ x, int j_) {
+    j = j_;
+    lo = x[j];
+    hi = x[j + 1];
+  }
+  __device__ __forceinline__ bool move(const double* __restrict__ x, int n, double v) {
+    if ((v >= lo || j == 0) && (v < hi || j == n - 2)) return false;
+    set(x, clh_walk(x, n, v, j));
+    return true;
+  }
+};
+
 template <bool ADJ>
 struct ClhGgNode {
-  double fK, fac, WG, WM, b1, bmag, l10fk, dy, t0, t0d, inv_st, dzl, zm, chi_amin;
-  double inv_c, inv_z, up, down, slG, inv_k;
-  int j, zl, jc, jz, jj, jg, jn, i;
+  double fK, fac, WG, WM, b1, bmag, l10fk, t0, t0d, inv_st, dzl, zm, chi_amin;
+  ClhBracket bc, bz, bg, bn;           // chi_v, chi_z, G_z, lens n(z) knots
+  double cz_lo, cz_hi, inv_c;          // chi(z) interval payload
+  double zjj, inv_z, up, down;         // dchi/dz stencil payload
+  double gv_lo, slG;                   // G(z) interval payload
+  double cf0, cf1, cf2, cf3;           // lens spline piece
+  ClhPkNode pk;
+  int zl, jj;
 
-  __device__ __forceinline__ void set_z(const ClhGgTabs& T) {
+  __device__ __forceinline__ void set_c(const ClhGgTabs& T) {
+    cz_lo = T.cz[bc.j];
+    cz_hi = T.cz[bc.j + 1];
+    inv_c = 1.0 / (bc.hi - bc.lo);
+  }
+  __device__ __forceinline__ void set_z(const DrawDesc& d, const ClhGgTabs& T) {
+    jj = bz.j > d.chi_n - 3 ? d.chi_n - 3 : bz.j;
     const double* zz = T.cz;
     const double* cc = T.cv;
+    zjj = zz[jj];
     inv_z = 1.0 / (zz[jj + 1] - zz[jj]);
     up = (cc[jj + 2] - cc[jj]) / (zz[jj + 2] - zz[jj]);
     down = (jj > 0) ? (cc[jj + 1] - cc[jj - 1]) / (zz[jj + 1] - zz[jj - 1]) : (cc[jj + 1] - cc[jj]) * inv_z;
+  }
+  __device__ __forceinline__ void set_g(const ClhGgTabs& T) {
+    gv_lo = T.gv[bg.j];
+    slG = (T.gv[bg.j + 1] - gv_lo) / (bg.hi - bg.lo);
+  }
+  __device__ __forceinline__ void set_n(const ClhGgTabs& T) {
+    const double* cf = T.lcf + 4 * bn.j;
+    cf0 = cf[0];
+    cf1 = cf[1];
+    cf2 = cf[2];
+    cf3 = cf[3];
   }
 
   // o: the node block (a, fK, fac, W_gal, W_mag, then the search results of
@@ -444,29 +526,26 @@ struct ClhGgNode {
     b1 = d.par->b1[zl];
     bmag = d.par->bmag[zl];
     const double* lut = o + 5 * np;
-    j = (int)lut[p];
-    dy = lut[np + p];
+    pk.init(d, (int)lut[p], lut[np + p]);
     l10fk = lut[2 * np + p];
-    jc = (int)lut[3 * np + p];
-    jz = (int)lut[4 * np + p];
-    jg = (int)lut[5 * np + p];
-    jn = (int)lut[6 * np + p];
+    bc.set(T.cv, (int)lut[3 * np + p]);
+    bz.set(T.cz, (int)lut[4 * np + p]);
+    bg.set(T.gz, (int)lut[5 * np + p]);
+    bn.set(T.lzc, (int)lut[6 * np + p]);
     t0 = lut[7 * np + p];
     t0d = ADJ ? lut[8 * np + p] : 0.0;
     inv_st = 1.0 / d.par->lens_st[zl];
     dzl = d.par->lens_dz[zl];
     zm = d.l_zmean[zl];
     chi_amin = d.chi_amin;
-    inv_c = 1.0 / (T.cv[jc + 1] - T.cv[jc]);
-    jj = jz > d.chi_n - 3 ? d.chi_n - 3 : jz;
-    set_z(T);
-    slG = (T.gv[jg + 1] - T.gv[jg]) / (T.gz[jg + 1] - T.gz[jg]);
-    i = -1;
-    inv_k = 0.0;
+    set_c(T);
+    set_z(d, T);
+    set_g(T);
+    set_n(T);
   }
 
   // false: chi_1 beyond chi(a_min), integrand zero (cosmo2D.c:2512-2514).
-  // t: the bracket; pf = P fac; dx: log10 k weight of the P cell (i, j).
+  // t: the bracket; pf = P fac; dx: log10 k weight of the P cell (pk.i, pk.j).
   // T: the draw's tables (staged in LDS when they fit).  F32: P(k,z) as expf
   // of the FP64 bilinear value.
   template <bool F32 = false>
@@ -474,57 +553,34 @@ struct ClhGgNode {
                                        double& t, double& pf, double& dwr, double& dx) {
     const double chi_1 = fK * R.rr1;
     if (chi_1 > chi_amin) return false;
+    // a_chi(chi_1) (cosmo3D.c:109-133)
     const double c1v = chi_1 * CLH_COVERH0;
-    {
-      const int j0 = jc;
-      jc = clh_walk(T.cv, d.chi_n, c1v, jc);
-      if (jc != j0) inv_c = 1.0 / (T.cv[jc + 1] - T.cv[jc]);
-    }
-    const double z1 = T.cz[jc] + (c1v - T.cv[jc]) * inv_c * (T.cz[jc + 1] - T.cz[jc]);
-    jz = clh_walk(T.cz, d.chi_n, z1, jz);
-    {
-      const int j2 = jz > d.chi_n - 3 ? d.chi_n - 3 : jz;
-      if (j2 != jj) {
-        jj = j2;
-        set_z(T);
-      }
-    }
-    const double Dz = down + (z1 - T.cz[jj]) * inv_z * (up - down);  // dchi/dz
-    {
-      const int j0 = jg;
-      jg = clh_walk(T.gz, d.G_n, z1, jg);
-      if (jg != j0) slG = (T.gv[jg + 1] - T.gv[jg]) / (T.gz[jg + 1] - T.gz[jg]);
-    }
-    const double G = T.gv[jg] + (z1 - T.gz[jg]) * slG;
+    if (bc.move(T.cv, d.chi_n, c1v)) set_c(T);
+    const double z1 = cz_lo + (c1v - bc.lo) * inv_c * (cz_hi - cz_lo);
+    // dchi/dz at z1 (cosmo3D.c:64-81 stencils)
+    if (bz.move(T.cz, d.chi_n, z1)) set_z(d, T);
+    const double Dz = down + (z1 - zjj) * inv_z * (up - down);
+    // G(z1) and f_growth (cosmo3D.c:200-228)
+    if (bg.move(T.gz, d.G_n, z1)) set_g(T);
+    const double G = gv_lo + (z1 - bg.lo) * slG;
     // hoverh0 * f_growth = (c/H0)/(dchi/dz) * (1 - G'(1+z)/G)
     const double hf1 = CLH_COVERH0 * (G - slG * (1.0 + z1)) / (Dz * G);
+    // nz_lens(z1) (redshift_spline.c:736-762) and its shift derivative
     double n1 = 0.0, dn1 = 0.0;
     {
       const double zs = (z1 - dzl - zm) * inv_st + zm;
       const int nr = d.l_rows;
       if (!(zs <= T.lzc[0] || zs >= T.lzc[nr - 1])) {
-        jn = clh_walk(T.lzc, nr, zs, jn);
-        const int jn2 = jn > nr - 2 ? nr - 2 : jn;
-        const double* cf = T.lcf + 4 * jn2;
-        const double x = zs - T.lzc[jn2];
-        n1 = (cf[0] + x * (cf[1] + x * (cf[2] + x * cf[3]))) * inv_st;
-        if (ADJ) dn1 = -(cf[1] + x * (2.0 * cf[2] + 3.0 * x * cf[3])) * (inv_st * inv_st);
+        if (bn.move(T.lzc, nr, zs)) set_n(T);
+        const double x = zs - bn.lo;
+        n1 = (cf0 + x * (cf1 + x * (cf2 + x * cf3))) * inv_st;
+        if (ADJ) dn1 = -(cf1 + x * (2.0 * cf2 + 3.0 * x * cf3)) * (inv_st * inv_st);
       }
     }
     const double wr = R.c0 * t0 - R.c1 * hf1 * n1;
     if (ADJ) dwr = R.c0 * t0d - R.c1 * hf1 * dn1;
     t = WG * b1 + WM * R.pref * bmag + wr;
-    const double log10k = R.l10 - l10fk;
-    {
-      const int i0 = i;
-      i = (i < 0) ? clh_bs(T.lk, d.nk, log10k) : clh_walk(T.lk, d.nk, log10k, i);
-      if (i != i0) inv_k = 1.0 / (T.lk[i + 1] - T.lk[i]);
-    }
-    dx = (log10k - T.lk[i]) * inv_k;
-    const int nz = d.nzp;
-    const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
-    const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +
-                       dx * dy * P[nz + 1];
+    const double out = pk.lnp(T.lk, R.l10 - l10fk, dx);
     constexpr double c3 = 1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
     if (F32) pf = (double)(expf((float)out) * (float)(c3 * fac));
     else pf = exp(out) * c3 * fac;

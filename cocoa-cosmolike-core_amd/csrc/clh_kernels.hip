@@ -331,11 +331,9 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
   const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
   __syncthreads();
   const int p = tid & 63, wv = tid >> 6;
-  const int j = (int)lut[p];
-  const double dy = lut[np + p], l10fk = lut[2 * np + p], fac = o[2 * np + p];
-  const int nz = d.nzp;
-  int i = -1;
-  double inv_k = 0.0;
+  const double l10fk = lut[2 * np + p], fac = o[2 * np + p];
+  ClhPkNode pkn;
+  pkn.init(d, (int)lut[p], lut[np + p]);
   constexpr double c3 = 1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
   constexpr int per = CLH_SUB_ROWS / 4;
   for (int sub = 0; sub * CLH_SUB_ROWS < nr; sub++) {
@@ -345,14 +343,8 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
       const int rr = base + rl;     // row within the tile
       double s = 0.0;
       if (rr < nr) {
-        const double log10k = L10[rr] - l10fk;
-        const int i0 = i;
-        i = (i < 0) ? clh_bs(lk, d.nk, log10k) : clh_walk(lk, d.nk, log10k, i);
-        if (i != i0) inv_k = 1.0 / (lk[i + 1] - lk[i]);
-        const double dx = (log10k - lk[i]) * inv_k;
-        const double* __restrict__ P = d.lnP + (size_t)i * nz + j;
-        const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] +
-                           dx * dy * P[nz + 1];
+        double dx;
+        const double out = pkn.lnp(lk, L10[rr] - l10fk, dx);
         if (f32) Sf[rl * np + p] = expf((float)out) * (float)(c3 * fac);
         else s = exp(out) * c3 * fac;
       } else if (f32) {
