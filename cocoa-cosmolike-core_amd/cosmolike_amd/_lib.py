@@ -29,6 +29,7 @@ class clh_draw(ctypes.Structure):
         ("src_dz", D), ("lens_dz", D), ("lens_stretch", D),
         ("b1", D), ("bmag", D), ("shear_m", D),
         ("ia_A", ctypes.c_double), ("ia_eta", ctypes.c_double), ("ia_bin", D),
+        ("pm", D),
     ]
 
 
@@ -73,6 +74,12 @@ def lib():
             "clh_set_nuisance_linear_bias": (i32, [VP, D, i32]),
             "clh_set_nuisance_magnification_bias": (i32, [VP, D, i32]),
             "clh_set_nuisance_ia": (i32, [VP, D, i32]),
+            "clh_set_nuisance_pointmass": (i32, [VP, D, i32]),
+            "clh_like_set_mask": (i32, [VP, I, i32, I]),
+            "clh_like_set_data": (i32, [VP, D, i32, I]),
+            "clh_like_set_inv_cov": (i32, [VP, D, i32, I]),
+            "clh_chi2": (i32, [VP, D, i32, D]),
+            "clh_batch_chi2": (i32, [VP, I, VP, VP, VP]),
             "clh_datavector_size": (i32, [VP]),
             "clh_compute_3x2pt_real": (i32, [VP, I, D, i32]),
             "clh_xi_pm": (i32, [VP, D, D]),

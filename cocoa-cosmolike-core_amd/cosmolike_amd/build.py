@@ -8,7 +8,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(PKG), "csrc")
 LIB = os.path.join(PKG, "libcosmolike_hip.so")
 
-SOURCES = ["clh_kernels.hip", "clh_adjoint.hip", "clh_mixed.hip", "clh_host.cpp"]
+SOURCES = ["clh_kernels.hip", "clh_adjoint.hip", "clh_mixed.hip", "clh_like.hip", "clh_host.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
     # packed FP32 instructions stay off in every HIP build of this library
@@ -20,7 +20,7 @@ FLAGS = [
 def build_library(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB + ".tmp"
-    cmd = [hipcc, *FLAGS, *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
+    cmd = [hipcc, *FLAGS, *[os.path.join(CSRC, s) for s in SOURCES], "-lrocblas", "-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     msgs = [ln for ln in (res.stdout + res.stderr).splitlines() if "packed-fp32-ops" not in ln]
     if res.returncode != 0:

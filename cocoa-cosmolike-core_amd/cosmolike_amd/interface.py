@@ -186,8 +186,48 @@ class Cosmolike:
             self.set_nuisance_clustering_photoz_stretch(prob.lens_stretch)
             self.set_nuisance_linear_bias(prob.b1)
             self.set_nuisance_magnification_bias(prob.bmag)
+            if getattr(prob, "pm", None) is not None:
+                self.set_nuisance_pointmass(prob.pm)
         if prob.ia_mode == 3:
             self.set_nuisance_IA([prob.ia_A, prob.ia_eta])
+
+    def set_nuisance_pointmass(self, pm):
+        pm = _arr(pm)
+        self._chk(self.L.clh_set_nuisance_pointmass(self.h, _p(pm), len(pm)))
+
+    # ----------------------------------------------------------- likelihood
+    @staticmethod
+    def _ord(ord):
+        return (ctypes.c_int * 3)(*ord)
+
+    def like_set_mask(self, mask, ord=(0, 1, 2)):
+        """IP::set_mask: 0/1 per data-vector entry in the block order ord (None clears)."""
+        if mask is None:
+            self._chk(self.L.clh_like_set_mask(self.h, None, 0, self._ord(ord)))
+            return
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.int32))
+        self._chk(self.L.clh_like_set_mask(self.h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(m),
+                                           self._ord(ord)))
+
+    def like_set_data(self, data, ord=(0, 1, 2)):
+        d = _arr(data)
+        self._chk(self.L.clh_like_set_data(self.h, _p(d), len(d), self._ord(ord)))
+
+    def like_set_inv_cov(self, inv_cov, ord=(0, 1, 2)):
+        c = _arr(inv_cov)
+        self._chk(self.L.clh_like_set_inv_cov(self.h, _p(c), c.shape[0], self._ord(ord)))
+
+    def chi2(self, dv):
+        """IP::get_chi2 of one (host) data vector."""
+        d = _arr(dv)
+        out = ctypes.c_double()
+        self._chk(self.L.clh_chi2(self.h, _p(d), len(d), ctypes.byref(out)))
+        return out.value
+
+    def batch_chi2(self, dv_ptr, chi2_ptr, grad_ptr=0, ord=(0, 1, 2)):
+        """Device pointers: dv [B][n] in, chi2 [B] and d chi2/d dv [B][n] (0 = skip) out."""
+        self._chk(self.L.clh_batch_chi2(self.h, self._ord(ord), ctypes.c_void_p(dv_ptr), ctypes.c_void_p(chi2_ptr),
+                                        ctypes.c_void_p(grad_ptr or None)))
 
     # -------------------------------------------------------------- compute
     @property
@@ -248,8 +288,9 @@ class Cosmolike:
         for i, p in enumerate(problems):
             vals = {k: _arr(getattr(p, k)) for k in ("chi_z", "chi", "G_z", "G", "log10k", "pk_z", "lnP",
                                                      "src_dz", "shear_m")}
-            for k in ("lens_dz", "lens_stretch", "b1", "bmag"):
-                vals[k] = _arr(getattr(p, k)) if p.lens is not None else None
+            for k in ("lens_dz", "lens_stretch", "b1", "bmag", "pm"):
+                v = getattr(p, k, None)
+                vals[k] = _arr(v) if (p.lens is not None and v is not None) else None
             keep.append(vals)
             d = arr[i]
             d.omega_m = p.omega_m
@@ -260,6 +301,7 @@ class Cosmolike:
             d.src_dz, d.shear_m = _p(vals["src_dz"]), _p(vals["shear_m"])
             d.lens_dz, d.lens_stretch = _p(vals["lens_dz"]), _p(vals["lens_stretch"])
             d.b1, d.bmag = _p(vals["b1"]), _p(vals["bmag"])
+            d.pm = _p(vals["pm"])
             d.ia_A, d.ia_eta = p.ia_A, p.ia_eta
         return arr, keep
 

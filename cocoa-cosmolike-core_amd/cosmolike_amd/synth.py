@@ -200,6 +200,7 @@ class Problem:
     ia_eta: float = 0.0
     ia_mode: int = 3             # IA_REDSHIFT_EVOLUTION (cosmolike/IA.h:13)
     h: float = 0.7
+    pm: np.ndarray | None = None  # point-mass amplitudes per lens bin
 
     @property
     def ns(self):
@@ -223,6 +224,8 @@ class Problem:
             self.b1 = np.linspace(1.2, 2.0, nl) if nl > 1 else np.array([1.2])
         if self.bmag is None:
             self.bmag = np.zeros(nl)
+        if self.pm is None:
+            self.pm = np.zeros(nl)
         return self
 
 

@@ -74,9 +74,18 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_leg_adjoint(const DrawDesc* __res
     const int col = w.y + c;
     double v = 0.0;
     if (col < ncol && th < nt) {
-      if (col < nss) v = d.dvbar[d.dv_start[0] + nt * col + th + (s ? nt * nss : 0)];
-      else if (s == 0 && col < nss + ngs) v = d.dvbar[d.dv_start[1] + nt * (col - nss) + th];
-      else if (s == 0) v = d.dvbar[d.dv_start[2] + nt * (col - nss - ngs) + th];
+      int canon = -1;
+      if (col < nss) {
+        canon = nt * col + th + (s ? nt * nss : 0);
+        v = d.dvbar[d.dv_start[0] + canon];
+      } else if (s == 0 && col < nss + ngs) {
+        canon = 2 * nt * nss + nt * (col - nss) + th;
+        v = d.dvbar[d.dv_start[1] + nt * (col - nss) + th];
+      } else if (s == 0) {
+        canon = nt * (2 * nss + ngs) + nt * (col - nss - ngs) + th;
+        v = d.dvbar[d.dv_start[2] + nt * (col - nss - ngs) + th];
+      }
+      if (canon >= 0 && clh_masked(d, canon)) v = 0.0;  // masked entries are constant 0
     }
     vb[c][s][th] = v;
   }
@@ -110,7 +119,23 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_leg_adjoint(const DrawDesc* __res
     if (col < nss + ngs) atomicAdd(&tq[c], acc * d.cl[(size_t)col * nrow + r]);
   }
   __syncthreads();
-  if (threadIdx.x < 8 && w.y + threadIdx.x < ncol) d.raw[w.y + threadIdx.x] = tq[threadIdx.x];
+  if (threadIdx.x < 8 && w.y + threadIdx.x < ncol) {
+    const int col = w.y + threadIdx.x;
+    double t = tq[threadIdx.x];
+    if (d.apply_calib && col >= nss && col < nss + ngs) {
+      // point mass of gamma_t: (raw + pm / theta^2)(1 + m_zs) -> m and source-shift terms
+      const int q = col - nss, zl = q / d.ns, zs = q % d.ns;
+      const double pmf = clh_pm_fac(d, zl, zs);
+      if (pmf != 0.0) {
+        double sv = 0.0;
+        for (int th = 0; th < nt; th++) sv += vb[threadIdx.x][0][th] / (d.theta[th] * d.theta[th]);
+        t += pmf * sv;
+        const double dpm = clh_pm_fac<true>(d, zl, zs) * sv * cf[threadIdx.x];
+        if (dpm != 0.0) atomicAdd(d.g_src + zs, dpm);
+      }
+    }
+    d.raw[col] = t;
+  }
 }
 
 // ----------------------------------------------------------------------------

@@ -280,6 +280,26 @@ __device__ __forceinline__ double clh_dg_lens(const DrawDesc& d, double a, int b
          : clh_interpol1d(d.dgtab + (size_t)(d.ns + b) * d.n_a, d.n_a, amin, dac, a);
 }
 
+// ------------------------------------------------- data-vector placement
+// point-mass term of the gamma_t pair (lens zl, source zs), without its
+// 1/theta^2 (PointMass::get_pm, generic_interface.cpp:2626-2641; a_lens from
+// the lens bin's zmean); TANGENT: its derivative with respect to the source
+// bin's photo-z shift (through g_tomo)
+template <bool TANGENT = false>
+__device__ __forceinline__ double clh_pm_fac(const DrawDesc& d, int zl, int zs) {
+  const double pm = d.par->pm[zl];
+  if (pm == 0.0) return 0.0;
+  const double a = 1.0 / (1.0 + d.l_zmean[zl]);
+  const double g = TANGENT ? clh_dg_tomo(d, a, zs) : clh_g_tomo(d, a, zs);
+  return 4.0 * M_PI * 1.6e-23 * pm * 1.e13 * g / (clh_chi(d, a) * a * a * a);
+}
+
+// 0 where the likelihood mask removes data-vector entry `canon` (index in the
+// (0,1,2) block order)
+__device__ __forceinline__ bool clh_masked(const DrawDesc& d, int canon) {
+  return d.apply_calib && d.mask && d.mask[canon] == 0;
+}
+
 // d C1 / d A_IA for the redshift-evolution mode (C1 is linear in A)
 __device__ __forceinline__ double clh_ia_c1_unit(const DrawDesc& d, double a, double growfac_a) {
   if (d.ia_mode != 3) return 0.0;

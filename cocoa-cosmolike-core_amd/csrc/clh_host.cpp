@@ -2,6 +2,7 @@
 // independent precomputation (n(z) splines, lens zmean, Gauss-Legendre rules,
 // bin-averaged Legendre kernels and their fold), batch assembly and launches.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -27,6 +28,10 @@ hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int 
 hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow, hipStream_t s);
 hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_fill(double* p, size_t n, double v, hipStream_t s);
+hipError_t clh_launch_chi2_resid(int B, int n, const double* dv, const double* data, const double* mask,
+                                 double* R, hipStream_t s);
+hipError_t clh_launch_chi2_reduce(int B, int n, const double* R, const double* Y, const double* mask,
+                                  double* chi2, double* grad, hipStream_t s);
 hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nknz, const double* lnP,
                                  const double* src, const double* lens, const double* b1,
                                  const double* iaA, const double* m, hipStream_t s);
@@ -146,6 +151,8 @@ struct Layout {
   dvec kfold;  // [4][nrow][ntheta]
   dvec kfull;  // [4][lmax][ntheta] (FULL mode only)
   dvec ell_ss, ell_gx;
+  dvec theta;               // [ntheta] bin-averaged theta [rad] (compute_binning_real_space)
+  std::vector<int> mask;    // likelihood mask in the (0,1,2) block order; empty = all ones
   int nrow() const { return CLH_NLOW + n_ell; }
   int n_ss() const { return src.nbin * (src.nbin + 1) / 2; }
   int n_gs() const { return src.nbin * lens.nbin; }
@@ -158,6 +165,7 @@ struct Draw {  // one draw's cosmology + nuisance (host copy)
   double src_dz[CLH_MAXB] = {0}, lens_dz[CLH_MAXB] = {0}, lens_st[CLH_MAXB],
          b1[CLH_MAXB] = {0}, bmag[CLH_MAXB] = {0}, shear_m[CLH_MAXB] = {0};
   double ia_bin[CLH_MAXB] = {0};
+  double pm[CLH_MAXB] = {0};
   double ia_A = 0.0, ia_eta = 0.0;
   Draw() { for (int i = 0; i < CLH_MAXB; i++) lens_st[i] = 1.0; }
 };
@@ -184,9 +192,10 @@ struct DevLayout {  // device copy of one survey layout
   DBuf<float> k32, ksc;  // folded kernels packed for the MFMA contraction
   DBuf<uint16_t> kbf;
   int ngrp = 0;
+  DBuf<int> mask;       // likelihood mask (null when the layout has none)
   size_t off_s_zc = 0, off_s_coef = 0, off_l_zc = 0, off_l_coef = 0, off_ell_ss = 0,
-         off_ell_gx = 0, off_kfold = 0, off_gl = 0, off_kfull = 0, off_fill = 0;
-  void release() { buf.release(); k32.release(); ksc.release(); kbf.release(); valid = false; }
+         off_ell_gx = 0, off_kfold = 0, off_gl = 0, off_kfull = 0, off_fill = 0, off_theta = 0;
+  void release() { buf.release(); k32.release(); ksc.release(); kbf.release(); mask.release(); valid = false; }
 };
 
 struct LayoutCopy {  // a context's own copy of a captured layout, with its device copy
@@ -257,6 +266,13 @@ struct clh_ctx {
   float last_ms[5] = {0, 0, 0, 0, 0};
   float last_vjp_ms[2] = {0, 0};
   bool vjp_timed = false;
+  // likelihood (IP): data and inverse covariance in the block order like_ord
+  int like_n = 0, like_ord[3] = {0, 1, 2};
+  bool like_data_set = false, like_icov_set = false;
+  dvec like_data_h;
+  DBuf<double> like_data, like_icov, like_mask, like_R, like_Y;
+  uint64_t like_mask_version = ~0ull;  // layout version the device mask copy was made for
+  rocblas_handle blas = nullptr;
 };
 
 static int fail(clh_ctx* c, int code, const char* fmt, ...) {
@@ -603,6 +619,7 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   D.off_ell_gx = take(L.ell_gx.size());
   D.off_kfold = take(L.kfold.size());
   D.off_gl = take(sizeof glbuf / sizeof(double));
+  D.off_theta = take(L.theta.size());
   D.off_kfull = take(full ? L.kfull.size() : 0);
   // FULL mode: index and weight of the ln-ell fill at every integer ell
   // (C_*_tomo_limber_fill, cosmo2D.c:1800-1807)
@@ -638,6 +655,12 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   HIPCHK(c, put(D.off_ell_gx, L.ell_gx.data(), L.ell_gx.size()));
   HIPCHK(c, put(D.off_kfold, L.kfold.data(), L.kfold.size()));
   HIPCHK(c, put(D.off_gl, glbuf, sizeof glbuf / sizeof(double)));
+  HIPCHK(c, put(D.off_theta, L.theta.data(), L.theta.size()));
+  if (!L.mask.empty()) {
+    HIPCHK(c, D.mask.ensure(L.mask.size()));
+    HIPCHK(c, hipMemcpyAsync(D.mask.p, L.mask.data(), L.mask.size() * sizeof(int), hipMemcpyHostToDevice,
+                             c->stream));
+  }
   if (full) HIPCHK(c, put(D.off_kfull, L.kfull.data(), L.kfull.size()));
   if (full) HIPCHK(c, put(D.off_fill, fill.data(), fill.size()));
   // Folded kernels as MFMA A operands (k_legendre_mx): group g of 8 rows,
@@ -867,6 +890,7 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
       q.l_zmin[b] = L.lens.zmin[b]; q.l_zmax[b] = L.lens.zmax[b]; q.l_zmean[b] = L.zmean[b];
       pp.src_dz[b] = d.src_dz[b]; pp.lens_dz[b] = d.lens_dz[b]; pp.lens_st[b] = d.lens_st[b];
       pp.b1[b] = d.b1[b]; pp.bmag[b] = d.bmag[b]; pp.shear_m[b] = d.shear_m[b]; pp.ia_bin[b] = d.ia_bin[b];
+      pp.pm[b] = d.pm[b];
     }
     pp.ia_A = d.ia_A; pp.ia_eta = d.ia_eta;
     q.par = bt.d_par.p + i;
@@ -879,6 +903,8 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.gl = lay + DL.off_gl;
     q.kmx32 = DL.k32.p; q.kmxbf = DL.kbf.p; q.kmxsc = DL.ksc.p;
     q.kmx_ngrp = DL.ngrp;
+    q.theta = lay + DL.off_theta;
+    q.mask = L.mask.empty() ? nullptr : DL.mask.p;
     const size_t gtab_sz = (size_t)(ns + nl) * L.n_a;
     const size_t node_sz = node_size(ns, nl);
     double* wb = bt.d_work.p + woff[i];
@@ -1018,6 +1044,9 @@ void clh_destroy(clh_ctx* c) {
   c->single.release(); c->batch.release(); c->scratch.release();
   c->dl.release();
   c->rl.clear();
+  c->like_data.release(); c->like_icov.release(); c->like_mask.release();
+  c->like_R.release(); c->like_Y.release();
+  if (c->blas) (void)rocblas_destroy_handle(c->blas);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
@@ -1069,6 +1098,17 @@ int clh_init_binning_real_space(clh_ctx* c, int ntheta, double tmin, double tmax
   c->L.ntheta = ntheta;
   c->L.vtmin = tmin * 2.90888208665721580e-4;
   c->L.vtmax = tmax * 2.90888208665721580e-4;
+  // bin-averaged theta of compute_binning_real_space (generic_interface.cpp:2037-2063)
+  c->L.theta.resize(ntheta);
+  {
+    const double logdt = (std::log(c->L.vtmax) - std::log(c->L.vtmin)) / ntheta;
+    for (int i = 0; i < ntheta; i++) {
+      const double t0 = std::exp(std::log(c->L.vtmin) + (i + 0.) * logdt);
+      const double t1 = std::exp(std::log(c->L.vtmin) + (i + 1.) * logdt);
+      c->L.theta[i] = (2. / 3.) * (std::pow(t1, 3) - std::pow(t0, 3)) / (t1 * t1 - t0 * t0);
+    }
+  }
+  if (!c->L.mask.empty() && (int)c->L.mask.size() != c->L.dv_size()) c->L.mask.clear();
   c->L.kern_valid = false;
   c->dl.valid = false;
   c->version++;
@@ -1185,6 +1225,158 @@ int clh_set_nuisance_ia(clh_ctx* c, const double* A1, int n) {
   return CLH_OK;
 }
 
+// PointMass::set_pm_vector (generic_interface.hpp:362-365)
+int clh_set_nuisance_pointmass(clh_ctx* c, const double* pm, int n) {
+  return set_vec(c, c->cur.pm, pm, n, c ? c->L.lens.nbin : 0, "set_pointmass");
+}
+
+// data-vector index in the block order `ord` of index k in the (0,1,2) order
+static int dv_index_from_canonical(const Layout& L, const int ord[3], int k) {
+  const int nt = L.ntheta;
+  const int canon_start[3] = {0, 2 * nt * L.n_ss(), nt * (2 * L.n_ss() + L.n_gs())};
+  int start[3];
+  dv_starts(L, ord, start);
+  const int b = k >= canon_start[2] ? 2 : (k >= canon_start[1] ? 1 : 0);
+  return start[b] + (k - canon_start[b]);
+}
+
+static bool ord_ok(const int ord[3]) {
+  return ord && ord[0] >= 0 && ord[1] >= 0 && ord[2] >= 0 && ord[0] < 3 && ord[1] < 3 && ord[2] < 3 &&
+         ord[0] != ord[1] && ord[0] != ord[2] && ord[1] != ord[2];
+}
+
+// IP::set_mask (generic_interface.hpp:1234-1331): 0/1 per data-vector entry,
+// in the block order ord; NULL clears the mask
+int clh_like_set_mask(clh_ctx* c, const int* mask, int n, const int ord[3]) {
+  if (!c) return CLH_ERR_ARG;
+  if (!mask) {
+    c->L.mask.clear();
+  } else {
+    if (!ord_ok(ord)) return fail(c, CLH_ERR_ARG, "set_mask: ord must be a permutation of 0,1,2");
+    if (n != c->L.dv_size()) return fail(c, CLH_ERR_ARG, "set_mask: %d entries != data vector size %d", n, c->L.dv_size());
+    std::vector<int> m(n);
+    int kept = 0;
+    for (int k = 0; k < n; k++) {
+      const int v = mask[dv_index_from_canonical(c->L, ord, k)];
+      if (v != 0 && v != 1) return fail(c, CLH_ERR_DOMAIN, "set_mask: inconsistent mask (entry %d)", k);
+      m[k] = v;
+      kept += v;
+    }
+    if (kept == 0) return fail(c, CLH_ERR_DOMAIN, "set_mask: the mask leaves no data points");
+    c->L.mask = std::move(m);
+  }
+  c->dl.valid = false;
+  c->version++;
+  return CLH_OK;
+}
+
+// IP::set_data: the measured data vector, in the block order ord
+int clh_like_set_data(clh_ctx* c, const double* data, int n, const int ord[3]) {
+  if (!c || !data) return CLH_ERR_ARG;
+  if (!ord_ok(ord)) return fail(c, CLH_ERR_ARG, "set_data: ord must be a permutation of 0,1,2");
+  if (n != c->L.dv_size()) return fail(c, CLH_ERR_ARG, "set_data: %d entries != data vector size %d", n, c->L.dv_size());
+  if (has_nan(data, n)) return fail(c, CLH_ERR_NAN, "set_data: NaN");
+  if (c->like_icov_set && (c->like_n != n || std::memcmp(ord, c->like_ord, sizeof(int) * 3) != 0))
+    c->like_icov_set = false;  // a different size or order than the inverse covariance
+  c->like_data_h.assign(data, data + n);
+  c->like_n = n;
+  for (int k = 0; k < 3; k++) c->like_ord[k] = ord[k];
+  c->like_data_set = true;
+  return CLH_OK;
+}
+
+// IP::set_inv_cov after its inversion: the inverse of the masked covariance
+// (with any Hartlap factor), n x n, rows and columns in the block order ord.
+// Masked entries drop out of chi^2 through the residuals.
+int clh_like_set_inv_cov(clh_ctx* c, const double* inv_cov, int n, const int ord[3]) {
+  if (!c || !inv_cov) return CLH_ERR_ARG;
+  if (!ord_ok(ord)) return fail(c, CLH_ERR_ARG, "set_inv_cov: ord must be a permutation of 0,1,2");
+  if (n != c->L.dv_size()) return fail(c, CLH_ERR_ARG, "set_inv_cov: %d rows != data vector size %d", n, c->L.dv_size());
+  if (has_nan(inv_cov, n * n)) return fail(c, CLH_ERR_NAN, "set_inv_cov: NaN");
+  int rc = ensure_device(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, c->like_icov.ensure((size_t)n * n));
+  HIPCHK(c, hipMemcpyAsync(c->like_icov.p, inv_cov, sizeof(double) * n * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->like_data_set && (c->like_n != n || std::memcmp(ord, c->like_ord, sizeof(int) * 3) != 0))
+    c->like_data_set = false;
+  c->like_n = n;
+  for (int k = 0; k < 3; k++) c->like_ord[k] = ord[k];
+  c->like_icov_set = true;
+  return CLH_OK;
+}
+
+// chi^2 of B data vectors (device, [B][n] in the block order ord) against
+// the data: chi2[b] (device), and if grad != NULL d chi2 / d dv [B][n]
+static int chi2_run(clh_ctx* c, const Layout& L, int B, const int ord[3], const double* dv, double* chi2,
+                       double* grad) {
+  if (!c->like_data_set || !c->like_icov_set) return fail(c, CLH_ERR_STATE, "chi2: data or inverse covariance not set");
+  if (std::memcmp(ord, c->like_ord, sizeof(int) * 3) != 0)
+    return fail(c, CLH_ERR_ARG, "chi2: ord differs from the order of the data and inverse covariance");
+  const int n = c->like_n;
+  if (n != L.dv_size()) return fail(c, CLH_ERR_STATE, "chi2: data vector size %d != %d", L.dv_size(), n);
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->like_mask_version != c->version || !c->like_mask.p) {
+    std::vector<double> m(n, 1.0);
+    if (!L.mask.empty())
+      for (int k = 0; k < n; k++) m[dv_index_from_canonical(L, ord, k)] = L.mask[k];
+    HIPCHK(c, c->like_mask.ensure(n));
+    HIPCHK(c, c->like_data.ensure(n));
+    HIPCHK(c, hipMemcpyAsync(c->like_mask.p, m.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->like_data.p, c->like_data_h.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->like_mask_version = c->version;
+  }
+  if (!c->blas && rocblas_create_handle(&c->blas) != rocblas_status_success) {
+    c->blas = nullptr;
+    return fail(c, CLH_ERR_HIP, "rocblas_create_handle failed");
+  }
+  HIPCHK(c, c->like_R.ensure((size_t)B * n));
+  HIPCHK(c, c->like_Y.ensure((size_t)B * n));
+  HIPCHK(c, clh_launch_chi2_resid(B, n, dv, c->like_data.p, c->like_mask.p, c->like_R.p, c->stream));
+  if (rocblas_set_stream(c->blas, c->stream) != rocblas_status_success)
+    return fail(c, CLH_ERR_HIP, "rocblas_set_stream failed");
+  const double one = 1.0, zero = 0.0;
+  // column-major: Y (n x B) = C^-1 (n x n, symmetric) R (n x B)
+  if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, B, n, &one, c->like_icov.p, n,
+                    c->like_R.p, n, &zero, c->like_Y.p, n) != rocblas_status_success)
+    return fail(c, CLH_ERR_HIP, "rocblas_dgemm failed");
+  HIPCHK(c, clh_launch_chi2_reduce(B, n, c->like_R.p, c->like_Y.p, c->like_mask.p, chi2, grad, c->stream));
+  return CLH_OK;
+}
+
+int clh_batch_chi2(clh_ctx* c, const int ord[3], const double* dv_device, double* chi2_device, double* grad_device) {
+  if (!c || !ord || !dv_device || !chi2_device) return CLH_ERR_ARG;
+  Batch& bt = c->batch;
+  if (bt.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
+  if (!bt.uniform) return fail(c, CLH_ERR_ARG, "chi2 needs a batch with one layout for every draw");
+  return chi2_run(c, *bt.lays[0], bt.B, ord, dv_device, chi2_device, grad_device);
+}
+
+// IP::get_chi2 (generic_interface.cpp:2288-2325) of one host data vector
+int clh_chi2(clh_ctx* c, const double* dv, int n, double* chi2) {
+  if (!c || !dv || !chi2) return CLH_ERR_ARG;
+  if (n != c->L.dv_size()) return fail(c, CLH_ERR_ARG, "chi2: %d entries != data vector size %d", n, c->L.dv_size());
+  int rc = ensure_device(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  DBuf<double> d_dv, d_out;
+  HIPCHK(c, d_dv.ensure(n));
+  HIPCHK(c, d_out.ensure(1));
+  HIPCHK(c, hipMemcpyAsync(d_dv.p, dv, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if ((rc = chi2_run(c, c->L, 1, c->like_ord, d_dv.p, d_out.p, nullptr))) return rc;
+  double v = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&v, d_out.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  d_dv.release();
+  d_out.release();
+  if (!(v >= 0.0)) return fail(c, CLH_ERR_DOMAIN, "chi2 = %g (invalid)", v);
+  *chi2 = v;
+  return CLH_OK;
+}
+
 int clh_datavector_size(clh_ctx* c) { return c ? c->L.dv_size() : 0; }
 
 int clh_compute_3x2pt_real(clh_ctx* c, const int ord[3], double* dv, int n) {
@@ -1284,6 +1476,7 @@ static void fill_draw(Draw& d, const clh_draw& s, const Layout& L) {
     d.lens_st[i] = s.lens_stretch ? s.lens_stretch[i] : 1.0;
     d.b1[i] = s.b1 ? s.b1[i] : 0.0;
     d.bmag[i] = s.bmag ? s.bmag[i] : 0.0;
+    d.pm[i] = s.pm ? s.pm[i] : 0.0;
   }
   d.ia_A = s.ia_A;
   d.ia_eta = s.ia_eta;

@@ -93,7 +93,9 @@ struct DrawDesc {
   double chi_amin;  // chi(a_min) for the gg RSD cut
   double G0;        // G(z=0) by the growth table's own interpolation
   const double* gl; // GL abscissae/weights on [-1,1]: x64 w64 x96 w96
-  int apply_calib;  // 1: shear calibration applied to the data vector
+  int apply_calib;  // 1: shear calibration, point mass and mask applied to the data vector
+  const double* theta;  // [ntheta] bin-averaged theta [rad] (compute_binning_real_space)
+  const int* mask;      // [dv size] 0/1 in the (0,1,2) block order, or null (all ones)
 
   // adjoint (VJP) buffers of the draw
   const double* dvbar;  // cotangent of the data vector
@@ -115,6 +117,7 @@ struct DrawParams {
   double src_dz[CLH_MAXB], lens_dz[CLH_MAXB], lens_st[CLH_MAXB];
   double b1[CLH_MAXB], bmag[CLH_MAXB], shear_m[CLH_MAXB];
   double ia_bin[CLH_MAXB];
+  double pm[CLH_MAXB];  // point-mass amplitudes per lens bin (PointMass::pm_)
   double ia_A, ia_eta;
 };
 

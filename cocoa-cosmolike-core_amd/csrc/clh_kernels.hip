@@ -456,6 +456,41 @@ __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ 
 }
 
 // ----------------------------------------------------------------------------
+// Data-vector placement of one (kind, pair, theta) value, as
+// compute_Mx2pt_N_masked / add_calib_and_set_mask_X_N do it
+// (generic_interface.hpp:800-862, 1137-1159): gamma_t gets the point-mass
+// term, then shear calibration (1+m_i)(1+m_j) or (1+m_zs); masked entries are
+// set to 0.  Without apply_calib the raw value is stored (the reference's
+// xi_pm_tomo / w_gammat_tomo / w_gg_tomo).
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ void clh_place(const DrawDesc& d, int kind, int q, int th, double acc) {
+  const int nt = d.ntheta;
+  int idx, canon;
+  double f = 1.0;
+  if (kind <= 1) {
+    int i, j;
+    clh_ss_pair(d.ns, q, i, j);
+    if (d.apply_calib) f = (1.0 + d.par->shear_m[i]) * (1.0 + d.par->shear_m[j]);
+    const int o = nt * q + th + (kind == 1 ? nt * d.n_ss : 0);
+    idx = d.dv_start[0] + o;
+    canon = o;
+  } else if (kind == 2) {
+    const int zs = q % d.ns;
+    if (d.apply_calib) {
+      f = 1.0 + d.par->shear_m[zs];
+      const double pmf = clh_pm_fac(d, q / d.ns, zs);
+      if (pmf != 0.0) acc += pmf / (d.theta[th] * d.theta[th]);
+    }
+    idx = d.dv_start[1] + nt * q + th;
+    canon = 2 * nt * d.n_ss + nt * q + th;
+  } else {
+    idx = d.dv_start[2] + nt * q + th;
+    canon = nt * (2 * d.n_ss + d.n_gs) + nt * q + th;
+  }
+  d.dv[idx] = clh_masked(d, canon) ? 0.0 : acc * f;
+}
+
+// ----------------------------------------------------------------------------
 // k_legendre (folded kernels): work item (draw, first task); 8 tasks per block,
 // 32 theta slots per task.  Task t of a draw: [0, n_ss) xi+, [n_ss, 2 n_ss)
 // xi-, then gamma_t pairs, then w bins.
@@ -495,20 +530,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre(const DrawDesc* __restri
   double acc = 0.0;
   for (int r = 0; r < nrow; r++) acc += K[(size_t)r * nt] * c[r];
 
-  int idx;
-  double f = 1.0;
-  if (kind <= 1) {
-    int i, j;
-    clh_ss_pair(d.ns, q, i, j);
-    if (d.apply_calib) f = (1.0 + d.par->shear_m[i]) * (1.0 + d.par->shear_m[j]);
-    idx = d.dv_start[0] + nt * q + th + (kind == 1 ? nt * d.n_ss : 0);
-  } else if (kind == 2) {
-    if (d.apply_calib) f = 1.0 + d.par->shear_m[q % d.ns];
-    idx = d.dv_start[1] + nt * q + th;
-  } else {
-    idx = d.dv_start[2] + nt * q + th;
-  }
-  d.dv[idx] = acc * f;
+  clh_place(d, kind, q, th, acc);
 }
 
 // ----------------------------------------------------------------------------
@@ -555,20 +577,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre_full(const DrawDesc* __r
     const double cl = tab[ic] + t * (tab[ic + 1] - tab[ic]);
     acc += K[(size_t)l * nt] * cl;
   }
-  int idx;
-  double f = 1.0;
-  if (kind <= 1) {
-    int i, j;
-    clh_ss_pair(d.ns, q, i, j);
-    if (d.apply_calib) f = (1.0 + d.par->shear_m[i]) * (1.0 + d.par->shear_m[j]);
-    idx = d.dv_start[0] + nt * q + th + (kind == 1 ? nt * d.n_ss : 0);
-  } else if (kind == 2) {
-    if (d.apply_calib) f = 1.0 + d.par->shear_m[q % d.ns];
-    idx = d.dv_start[1] + nt * q + th;
-  } else {
-    idx = d.dv_start[2] + nt * q + th;
-  }
-  d.dv[idx] = acc * f;
+  clh_place(d, kind, q, th, acc);
 }
 
 // ----------------------------------------------------------------------------

@@ -87,23 +87,34 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre_mx(const DrawDesc* __res
     }
   }
   if (!colok) return;
-  int base;
-  double f = 1.0;
+  int base, canon0;
+  double f = 1.0, pmf = 0.0;
   if (kind <= 1) {
     int bi, bj;
     clh_ss_pair(d.ns, q, bi, bj);
     if (d.apply_calib) f = (1.0 + d.par->shear_m[bi]) * (1.0 + d.par->shear_m[bj]);
-    base = d.dv_start[0] + nt * q + (kind == 1 ? nt * d.n_ss : 0);
+    canon0 = nt * q + (kind == 1 ? nt * d.n_ss : 0);
+    base = d.dv_start[0] + canon0;
   } else if (kind == 2) {
-    if (d.apply_calib) f = 1.0 + d.par->shear_m[q % d.ns];
+    const int zs = q % d.ns;
+    if (d.apply_calib) {
+      f = 1.0 + d.par->shear_m[zs];
+      pmf = clh_pm_fac(d, q / d.ns, zs);  // point mass (generic_interface.cpp:2626-2641)
+    }
     base = d.dv_start[1] + nt * q;
+    canon0 = 2 * nt * d.n_ss + nt * q;
   } else {
     base = d.dv_start[2] + nt * q;
+    canon0 = nt * (2 * d.n_ss + d.n_gs) + nt * q;
   }
 #pragma unroll
   for (int v = 0; v < 16; v++) {
     const int th = 8 * (v >> 2) + 4 * h + (v & 3);
-    if (th < nt) d.dv[base + th] = (double)acc[v] * f;
+    if (th < nt) {
+      double x = (double)acc[v];
+      if (pmf != 0.0) x += pmf / (d.theta[th] * d.theta[th]);
+      d.dv[base + th] = clh_masked(d, canon0 + th) ? 0.0 : x * f;
+    }
   }
 }
 

@@ -99,6 +99,34 @@ int clh_set_nuisance_magnification_bias(clh_ctx* ctx, const double* bmag, int n)
 int clh_set_nuisance_ia(clh_ctx* ctx, const double* A1, int n);
 
 /* ----------------------------------------------------------- bulk compute */
+/* PointMass::set_pm_vector (generic_interface.hpp:362-365): point-mass
+ * amplitudes per lens bin; gamma_t of the data vectors gets
+ * 4 pi (G/c^2) pm 1e13 g_tomo(a_l, zs) / (theta^2 chi(a_l) a_l^3) with
+ * a_l = 1/(1+zmean(zl)) before shear calibration (PointMass::get_pm,
+ * generic_interface.cpp:2626-2641; add_calib_and_set_mask_X_N,
+ * generic_interface.hpp:843-858). Default 0. */
+int clh_set_nuisance_pointmass(clh_ctx* ctx, const double* pm, int n);
+
+/* ------------------------------------------------------------ likelihood */
+/* IP::set_mask (generic_interface.hpp:1234-1331): 0/1 per data-vector entry,
+ * in the block order ord (NULL clears). Masked entries of every computed
+ * data vector are 0 and carry no gradient. */
+int clh_like_set_mask(clh_ctx* ctx, const int* mask, int n, const int ord[3]);
+/* IP::set_data: the measured data vector (n entries, block order ord) */
+int clh_like_set_data(clh_ctx* ctx, const double* data, int n, const int ord[3]);
+/* IP::set_inv_cov after its inversion (generic_interface.cpp:2144-2278): the
+ * inverse of the masked covariance (Hartlap factor included), n x n row-major
+ * (symmetric), rows in the block order ord */
+int clh_like_set_inv_cov(clh_ctx* ctx, const double* inv_cov, int n, const int ord[3]);
+/* IP::get_chi2 (generic_interface.cpp:2288-2325) of one host data vector */
+int clh_chi2(clh_ctx* ctx, const double* dv, int n, double* chi2);
+/* chi^2 of every data vector of the uploaded batch (device dv [B][n] in the
+ * block order ord, which must be the data's): chi2 [B] (device) and, if
+ * grad != NULL, d chi2 / d dv [B][n] (device; zero on masked entries), the
+ * cotangent to chain into clh_batch_vjp. */
+int clh_batch_chi2(clh_ctx* ctx, const int ord[3], const double* dv_device, double* chi2_device,
+                   double* grad_device);
+
 int clh_datavector_size(clh_ctx* ctx);
 /* compute_Mx2pt_N_masked<0,3>(ord) (generic_interface.hpp:1137-1159) with an
  * all-ones mask: xi+ | xi- | gamma_t | w blocks placed by ord, shear
@@ -130,6 +158,7 @@ typedef struct {
   const double* src_dz; const double* lens_dz; const double* lens_stretch;
   const double* b1; const double* bmag; const double* shear_m;
   double ia_A; double ia_eta; const double* ia_bin;
+  const double* pm;  /* point-mass amplitudes per lens bin (NULL = 0) */
 } clh_draw;
 
 /* uploads B draws into device memory owned by the context */

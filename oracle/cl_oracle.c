@@ -52,6 +52,9 @@ struct oc_ctx {
   /* nuisance */
   double src_dz[OC_MAXB], lens_dz[OC_MAXB], lens_st[OC_MAXB], b1[OC_MAXB],
       bmag[OC_MAXB], shear_m[OC_MAXB];
+  double pm[OC_MAXB]; /* point-mass amplitudes per lens bin (PointMass::pm_) */
+  int* mask;          /* likelihood mask in the (0,1,2) block order, or NULL */
+  int mask_n;
   int ia_mode;
   double ia_bin[OC_MAXB], ia_A, ia_eta, oneplusz0, c1rhocrit;
   /* caches */
@@ -98,6 +101,7 @@ void oc_free(oc_ctx* c) {
   free(c->s_tab); free(c->s_zc); free(c->s_y); free(c->s_c);
   free(c->l_tab); free(c->l_zc); free(c->l_y); free(c->l_c);
   free(c->gt_tab); free(c->gl_tab); free(c->K);
+  free(c->mask);
   free(c);
 }
 
@@ -1246,6 +1250,26 @@ int oc_datavector_size(oc_ctx* c) {
 
 /* compute_Mx2pt_N_masked<0,3> with every mask entry 1 and no point mass
  * (generic_interface.hpp:770-791 starts, 800-862 calibration, 968-1041 fill) */
+/* bin-averaged theta [rad] (compute_binning_real_space, generic_interface.cpp:2037-2063) */
+static void binning_theta(oc_ctx* c, double* theta) {
+  const double logdt = (log(c->vtmax) - log(c->vtmin)) / c->ntheta;
+  for (int i = 0; i < c->ntheta; i++) {
+    const double t0 = exp(log(c->vtmin) + (i + 0.) * logdt);
+    const double t1 = exp(log(c->vtmin) + (i + 1.) * logdt);
+    theta[i] = 2. / 3. * (pow(t1, 3) - pow(t0, 3)) / (t1 * t1 - t0 * t0);
+  }
+}
+
+/* PointMass::get_pm (generic_interface.cpp:2626-2641) */
+static double point_mass(oc_ctx* c, int zl, int zs, double theta) {
+  if (c->pm[zl] == 0.0) return 0.0;
+  const double Goverc2 = 1.6e-23;
+  const double a_lens = 1.0 / (1.0 + c->l_zmean[zl]);
+  const double chi_lens = chi_of_a(c, a_lens);
+  return 4 * M_PI * Goverc2 * c->pm[zl] * 1.e+13 * g_tomo(c, a_lens, zs) / (theta * theta) /
+         (chi_lens * a_lens * a_lens * a_lens);
+}
+
 int oc_datavector_3x2pt(oc_ctx* c, const int ord[3], double* dv) {
   c->status = 0;
   const int nt = c->ntheta, nss = oc_npairs_ss(c), ngs = oc_npairs_gs(c), ngg = c->nl;
@@ -1278,14 +1302,65 @@ int oc_datavector_3x2pt(oc_ctx* c, const int ord[3], double* dv) {
       dv[start[0] + nt * nz + i + nt * nss] = xm[nz * nt + i] * f;
     }
   }
+  /* gamma_t: point mass, then (1 + m) (add_calib_and_set_mask_X_N,
+     generic_interface.hpp:843-858; PointMass::get_pm, generic_interface.cpp:2626-2641) */
+  double* theta = malloc(sizeof(double) * nt);
+  binning_theta(c, theta);
   for (int nz = 0; nz < ngs; nz++) {
-    const int zs = nz % c->ns;
-    for (int i = 0; i < nt; i++) dv[start[1] + nt * nz + i] = gt[nz * nt + i] * (1.0 + c->shear_m[zs]);
+    const int zs = nz % c->ns, zl = nz / c->ns;
+    for (int i = 0; i < nt; i++)
+      dv[start[1] + nt * nz + i] = (gt[nz * nt + i] + point_mass(c, zl, zs, theta[i])) * (1.0 + c->shear_m[zs]);
   }
   for (int nz = 0; nz < ngg; nz++)
     for (int i = 0; i < nt; i++) dv[start[2] + nt * nz + i] = w[nz * nt + i];
-  free(xp); free(xm); free(gt); free(w);
+  /* masked entries are 0 */
+  if (c->mask) {
+    const int cs[3] = {0, 2 * nt * nss, nt * (2 * nss + ngs)};
+    for (int k = 0; k < n; k++) {
+      if (c->mask[k]) continue;
+      const int b = k >= cs[2] ? 2 : (k >= cs[1] ? 1 : 0);
+      dv[start[b] + (k - cs[b])] = 0.0;
+    }
+  }
+  free(xp); free(xm); free(gt); free(w); free(theta);
   return c->status;
+}
+
+/* chi^2 = sum_ij r_i C^-1_ij r_j over unmasked entries, r = dv - data
+   (IP::get_chi2, generic_interface.cpp:2288-2325); mask and arrays in the
+   data-vector order of dv */
+double oc_chi2(const double* dv, const double* data, const double* inv_cov, const int* mask, int n) {
+  double chi2 = 0.0;
+  for (int i = 0; i < n; i++) {
+    if (mask && !mask[i]) continue;
+    const double x = dv[i] - data[i];
+    double row = 0.0;
+    for (int j = 0; j < n; j++) {
+      if (mask && !mask[j]) continue;
+      row += inv_cov[(size_t)i * n + j] * (dv[j] - data[j]);
+    }
+    chi2 += x * row;
+  }
+  return chi2;
+}
+
+int oc_set_pointmass(oc_ctx* c, const double* pm, int n) {
+  c->status = 0;
+  if (n != c->nl) { oc_set_err(c, "set_pointmass: %d values for %d lens bins", n, c->nl); return -1; }
+  for (int i = 0; i < n; i++) c->pm[i] = pm[i];
+  return 0;
+}
+
+int oc_set_mask(oc_ctx* c, const int* mask, int n) {
+  c->status = 0;
+  free(c->mask);
+  c->mask = NULL;
+  c->mask_n = 0;
+  if (!mask) return 0;
+  c->mask = malloc(sizeof(int) * n);
+  for (int i = 0; i < n; i++) c->mask[i] = mask[i];
+  c->mask_n = n;
+  return 0;
 }
 
 /* ---------------------------------------------------------------------- */

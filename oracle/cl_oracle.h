@@ -72,8 +72,16 @@ int oc_cl_full(oc_ctx* c, double* ss, double* gs, double* gg);
 int oc_xi_pm(oc_ctx* c, double* xip, double* xim);
 int oc_gammat(oc_ctx* c, double* out);   /* w_gammat_tomo, [n_ggl*ntheta] */
 int oc_wtheta(oc_ctx* c, double* out);   /* w_gg_tomo limber=1, [nlens*ntheta] */
-/* compute_Mx2pt_N_masked<0,3> with an all-ones mask, incl. shear calibration */
+/* compute_Mx2pt_N_masked<0,3>: shear calibration, point mass on gamma_t,
+ * masked entries 0 (all-ones mask unless oc_set_mask was called) */
 int oc_datavector_3x2pt(oc_ctx* c, const int ord[3], double* dv);
+/* PointMass::set_pm_vector: amplitudes per lens bin (default 0) */
+int oc_set_pointmass(oc_ctx* c, const double* pm, int n);
+/* likelihood mask in the (0,1,2) block order (NULL clears) */
+int oc_set_mask(oc_ctx* c, const int* mask, int n);
+/* IP::get_chi2: chi^2 of dv against data with the inverse covariance, over
+ * the entries where mask != 0 (mask NULL = all); arrays in dv's order */
+double oc_chi2(const double* dv, const double* data, const double* inv_cov, const int* mask, int n);
 int oc_datavector_size(oc_ctx* c);
 
 int oc_npairs_ss(oc_ctx* c);

@@ -50,6 +50,12 @@ def _load():
         lib.oc_gl_nodes.argtypes = [ctypes.c_int, _D, _D]
         lib.oc_set_kernel_precision.argtypes = [vp, ctypes.c_int]
         lib.oc_legendre_pl_dpl.argtypes = [ctypes.c_int, ctypes.c_double, _D, _D]
+        lib.oc_set_pointmass.restype = ctypes.c_int
+        lib.oc_set_pointmass.argtypes = [vp, _D, ctypes.c_int]
+        lib.oc_set_mask.restype = ctypes.c_int
+        lib.oc_set_mask.argtypes = [vp, _I, ctypes.c_int]
+        lib.oc_chi2.restype = ctypes.c_double
+        lib.oc_chi2.argtypes = [_D, _D, _D, _I, ctypes.c_int]
         _lib = lib
     return _lib
 
@@ -62,6 +68,17 @@ def _p(x):
 
 def _arr(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def chi2(dv, data, inv_cov, mask=None):
+    """IP::get_chi2 restated: arrays in one data-vector order."""
+    lib = _load()
+    dv, data, ic = _arr(dv), _arr(data), _arr(inv_cov)
+    mp = None
+    if mask is not None:
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.int32))
+        mp = m.ctypes.data_as(_I)
+    return lib.oc_chi2(_p(dv), _p(data), _p(ic), mp, len(dv))
 
 
 def gl_nodes(n: int):
@@ -158,7 +175,21 @@ class Oracle:
                           prob.b1 if prob.lens is not None else None,
                           prob.bmag if prob.lens is not None else None, prob.shear_m)
         self.set_ia(prob.ia_mode, prob.ia_A, prob.ia_eta)
+        if prob.lens is not None and getattr(prob, "pm", None) is not None:
+            self.set_pointmass(prob.pm)
         return self
+
+    def set_pointmass(self, pm):
+        pm = _arr(pm)
+        self._chk(self.lib.oc_set_pointmass(self.c, _p(pm), len(pm)))
+
+    def set_mask(self, mask):
+        """Mask in the (0,1,2) block order (None clears)."""
+        if mask is None:
+            self._chk(self.lib.oc_set_mask(self.c, None, 0))
+            return
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.int32))
+        self._chk(self.lib.oc_set_mask(self.c, m.ctypes.data_as(_I), len(m)))
 
     def set_kernel_precision(self, extended: bool):
         self.lib.oc_set_kernel_precision(self.c, 1 if extended else 0)

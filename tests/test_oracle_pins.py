@@ -13,6 +13,7 @@ import mpmath as mp
 import numpy as np
 import pytest
 from scipy import interpolate, special
+from scipy.integrate import quad
 
 from cosmolike_amd import synth
 
@@ -411,3 +412,83 @@ def test_small_theta_kernel_precision():
     print("G- abs err / max|G-| at 2.5': long double", worst_ld, "double", worst_d)
     assert worst_ld < 1e-9
     assert worst_d > 100 * worst_ld  # the reason the kernels are built in long double
+
+
+# --------------------------------------------------------------- likelihood end
+def _pm_problem():
+    p = synth.full_3x2pt_problem()
+    return p
+
+
+def test_point_mass_pins():
+    """Point-mass term of gamma_t (PointMass::get_pm): enters only gamma_t rows
+    of its lens bin, linear in the amplitude, times (1+m_zs), proportional to
+    1/theta_eff^2 with theta_eff the area-weighted mean of the annulus (checked
+    here by quadrature), and across source bins proportional to g_tomo at
+    a_lens (g_tomo pinned separately).  Parity partially pinned: the
+    normalisation 4 pi (G/c^2) 1e13 / (chi a^3) is restated from the reference."""
+    p = _pm_problem()
+    o = O.Oracle().load(p)
+    dv0 = o.datavector()
+    pm = np.zeros(p.nl)
+    pm[1] = 3.0
+    o.set_pointmass(pm)
+    d1 = o.datavector() - dv0
+    o.set_pointmass(2 * pm)
+    d2 = o.datavector() - dv0
+    nt, nss, ngs = p.ntheta, p.ns * (p.ns + 1) // 2, p.ns * p.nl
+    gt0 = 2 * nt * nss
+    assert np.all(d1[:gt0] == 0) and np.all(d1[gt0 + nt * ngs:] == 0)
+    blk = d1[gt0:gt0 + nt * ngs].reshape(p.nl, p.ns, nt)
+    assert np.all(blk[[0, 2, 3, 4]] == 0)
+    assert np.allclose(d2[gt0:gt0 + nt * ngs], 2 * d1[gt0:gt0 + nt * ngs], rtol=1e-12, atol=0)
+    # 1/theta_eff^2 with theta_eff = int theta^2 dtheta / int theta dtheta per annulus
+    t = np.exp(np.linspace(np.log(p.theta_min_arcmin), np.log(p.theta_max_arcmin), nt + 1)) * np.pi / 180 / 60
+    teff = np.array([quad(lambda x: x * x, t[i], t[i + 1])[0] / quad(lambda x: x, t[i], t[i + 1])[0]
+                     for i in range(nt)])
+    row = blk[1, p.ns - 1]
+    assert np.allclose(row * teff ** 2, row[0] * teff[0] ** 2, rtol=1e-9)
+    # source-bin dependence: g_tomo at a_lens = 1/(1+zmean)
+    zm = o.zmean_lens()[1]
+    g = np.array([o.g_tomo(b, [1 / (1 + zm)])[0] for b in range(p.ns)])
+    nz = g > 0
+    assert np.allclose(blk[1, nz, 0] / g[nz], blk[1, nz, 0][-1] / g[nz][-1], rtol=1e-12)
+    # (1 + m_zs) multiplies the point-mass term too
+    m = np.linspace(-0.02, 0.03, p.ns)
+    o.set_nuisance(shear_m=m)
+    o.set_pointmass(np.zeros(p.nl))
+    b0 = o.datavector()
+    o.set_pointmass(pm)
+    d3 = (o.datavector() - b0)[gt0:gt0 + nt * ngs].reshape(p.nl, p.ns, nt)
+    assert np.allclose(d3, blk * (1 + m)[None, :, None], rtol=1e-10, atol=0)
+
+
+def test_mask_zeroes_entries():
+    p = synth.full_3x2pt_problem()
+    o = O.Oracle().load(p)
+    ref = o.datavector()
+    rng = np.random.default_rng(1)
+    mask = (rng.random(ref.size) > 0.3).astype(np.int32)
+    o.set_mask(mask)
+    dv = o.datavector()
+    assert np.all(dv[mask == 0] == 0) and np.array_equal(dv[mask == 1], ref[mask == 1])
+    # block order: the mask is given in the (0,1,2) order; ord (2,0,1) moves blocks, not entries
+    dv2 = o.datavector((2, 0, 1))
+    n_w = p.nl * p.ntheta
+    assert np.array_equal(dv2[:n_w], dv[-n_w:])
+
+
+def test_chi2_matches_numpy():
+    rng = np.random.default_rng(2)
+    n = 200
+    A = rng.normal(size=(n, n))
+    cov = A @ A.T / n + np.eye(n)
+    mask = (rng.random(n) > 0.25).astype(np.int32)
+    data, dv = rng.normal(size=n), rng.normal(size=n)
+    keep = mask == 1
+    # the reference inverts the masked covariance and zeroes masked rows/columns
+    icov = np.zeros((n, n))
+    icov[np.ix_(keep, keep)] = np.linalg.inv(cov[np.ix_(keep, keep)])
+    r = (dv - data)[keep]
+    ref = r @ np.linalg.solve(cov[np.ix_(keep, keep)], r)
+    assert abs(O.chi2(dv, data, icov, mask) - ref) <= 1e-10 * ref
