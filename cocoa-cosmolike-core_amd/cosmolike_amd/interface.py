@@ -191,6 +191,27 @@ class Cosmolike:
         if prob.ia_mode == 3:
             self.set_nuisance_IA([prob.ia_A, prob.ia_eta])
 
+    def init_binning_fourier(self, nells, lmin, lmax, lmax_shear):
+        self._chk(self.L.clh_init_binning_fourier(self.h, int(nells), int(lmin), int(lmax), int(lmax_shear)))
+
+    def datavector_size_fourier(self):
+        return self.L.clh_datavector_size_fourier(self.h)
+
+    def compute_3x2pt_fourier(self, ord=(0, 1, 2), mask=None):
+        """compute_Mx2pt_N_masked<1,3>(ord); mask in the order of the output
+        (None = all ones)."""
+        n = self.L.clh_datavector_size_fourier(self.h)
+        out = np.empty(n)
+        m = None
+        if mask is not None:
+            m = np.ascontiguousarray(np.asarray(mask, dtype=np.int32))
+            if m.size != n:
+                raise ValueError(f"mask has {m.size} entries, data vector {n}")
+        I3 = (ctypes.c_int * 3)(*ord)
+        self._chk(self.L.clh_compute_3x2pt_fourier(
+            self.h, I3, None if m is None else m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(out), n))
+        return out
+
     def set_nuisance_pointmass(self, pm):
         pm = _arr(pm)
         self._chk(self.L.clh_set_nuisance_pointmass(self.h, _p(pm), len(pm)))

@@ -152,6 +152,8 @@ struct Layout {
   dvec kfull;  // [4][lmax][ntheta] (FULL mode only)
   dvec ell_ss, ell_gx;
   dvec theta;               // [ntheta] bin-averaged theta [rad] (compute_binning_real_space)
+  dvec ell_f;               // Fourier-space ell bins (init_binning_fourier), like.ell
+  double lmax_shear = 0;    // like.lmax_shear
   std::vector<int> mask;    // likelihood mask in the (0,1,2) block order; empty = all ones
   int nrow() const { return CLH_NLOW + n_ell; }
   int n_ss() const { return src.nbin * (src.nbin + 1) / 2; }
@@ -729,17 +731,53 @@ static void row_tiles(int nrow, int cap, std::vector<int2>& out) {
   for (int r = 0; r < nrow; r += tr) out.push_back(make_int2(r, std::min(tr, nrow - r)));
 }
 
-// data-vector block starts of compute_data_vector_Mx2pt_N_starts
-// (generic_interface.hpp:770-791) for a layout and a probe order
-static void dv_starts(const Layout& L, const int ord[3], int start[3]) {
-  const int nt = L.ntheta;
-  const int sizes[3] = {2 * nt * L.n_ss(), nt * L.n_gs(), nt * L.lens.nbin};
+// block starts of compute_data_vector_Mx2pt_N_starts
+// (generic_interface.hpp:770-791): indices = stable ascending sort of ord,
+// start(i) = sum of sizes(indices(j)) over j < indices(i)
+static void block_starts(const int sizes[3], const int ord[3], int start[3]) {
   int idx[3] = {0, 1, 2};
   for (int i = 1; i < 3; i++)
     for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) std::swap(idx[j], idx[j - 1]);
   for (int i = 0; i < 3; i++) start[i] = 0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+}
+
+// the formula above tiles the data vector only for an involutive order
+// ((0,1,2), (1,0,2), (2,1,0), (0,2,1)); a 3-cycle such as (1,2,0) makes
+// blocks overlap or run past the end, so such orders are refused
+static bool blocks_tile(const int sizes[3], const int ord[3]) {
+  if (!ord) return false;
+  for (int i = 0; i < 3; i++)
+    if (ord[i] < 0 || ord[i] > 2) return false;
+  if (ord[0] == ord[1] || ord[0] == ord[2] || ord[1] == ord[2]) return false;
+  int start[3];
+  block_starts(sizes, ord, start);
+  const int n = sizes[0] + sizes[1] + sizes[2];
+  for (int i = 0; i < 3; i++) {
+    if (start[i] < 0 || start[i] + sizes[i] > n) return false;
+    for (int j = 0; j < i; j++)
+      if (sizes[i] > 0 && sizes[j] > 0 && start[i] < start[j] + sizes[j] && start[j] < start[i] + sizes[i])
+        return false;
+  }
+  return true;
+}
+
+static void dv_sizes(const Layout& L, int sizes[3]) {
+  const int nt = L.ntheta;
+  sizes[0] = 2 * nt * L.n_ss(); sizes[1] = nt * L.n_gs(); sizes[2] = nt * L.lens.nbin;
+}
+
+static void dv_starts(const Layout& L, const int ord[3], int start[3]) {
+  int sizes[3];
+  dv_sizes(L, sizes);
+  block_starts(sizes, ord, start);
+}
+
+static bool ord_ok(const Layout& L, const int ord[3]) {
+  int sizes[3];
+  dv_sizes(L, sizes);
+  return blocks_tile(sizes, ord);
 }
 
 // Work lists of a batch (one entry per block; every stage is one launch for
@@ -1240,11 +1278,6 @@ static int dv_index_from_canonical(const Layout& L, const int ord[3], int k) {
   return start[b] + (k - canon_start[b]);
 }
 
-static bool ord_ok(const int ord[3]) {
-  return ord && ord[0] >= 0 && ord[1] >= 0 && ord[2] >= 0 && ord[0] < 3 && ord[1] < 3 && ord[2] < 3 &&
-         ord[0] != ord[1] && ord[0] != ord[2] && ord[1] != ord[2];
-}
-
 // IP::set_mask (generic_interface.hpp:1234-1331): 0/1 per data-vector entry,
 // in the block order ord; NULL clears the mask
 int clh_like_set_mask(clh_ctx* c, const int* mask, int n, const int ord[3]) {
@@ -1252,7 +1285,7 @@ int clh_like_set_mask(clh_ctx* c, const int* mask, int n, const int ord[3]) {
   if (!mask) {
     c->L.mask.clear();
   } else {
-    if (!ord_ok(ord)) return fail(c, CLH_ERR_ARG, "set_mask: ord must be a permutation of 0,1,2");
+    if (!ord_ok(c->L, ord)) return fail(c, CLH_ERR_ARG, "set_mask: ord must be a permutation of 0,1,2 whose blocks tile the data vector");
     if (n != c->L.dv_size()) return fail(c, CLH_ERR_ARG, "set_mask: %d entries != data vector size %d", n, c->L.dv_size());
     std::vector<int> m(n);
     int kept = 0;
@@ -1273,7 +1306,7 @@ int clh_like_set_mask(clh_ctx* c, const int* mask, int n, const int ord[3]) {
 // IP::set_data: the measured data vector, in the block order ord
 int clh_like_set_data(clh_ctx* c, const double* data, int n, const int ord[3]) {
   if (!c || !data) return CLH_ERR_ARG;
-  if (!ord_ok(ord)) return fail(c, CLH_ERR_ARG, "set_data: ord must be a permutation of 0,1,2");
+  if (!ord_ok(c->L, ord)) return fail(c, CLH_ERR_ARG, "set_data: ord must be a permutation of 0,1,2");
   if (n != c->L.dv_size()) return fail(c, CLH_ERR_ARG, "set_data: %d entries != data vector size %d", n, c->L.dv_size());
   if (has_nan(data, n)) return fail(c, CLH_ERR_NAN, "set_data: NaN");
   if (c->like_icov_set && (c->like_n != n || std::memcmp(ord, c->like_ord, sizeof(int) * 3) != 0))
@@ -1290,7 +1323,7 @@ int clh_like_set_data(clh_ctx* c, const double* data, int n, const int ord[3]) {
 // Masked entries drop out of chi^2 through the residuals.
 int clh_like_set_inv_cov(clh_ctx* c, const double* inv_cov, int n, const int ord[3]) {
   if (!c || !inv_cov) return CLH_ERR_ARG;
-  if (!ord_ok(ord)) return fail(c, CLH_ERR_ARG, "set_inv_cov: ord must be a permutation of 0,1,2");
+  if (!ord_ok(c->L, ord)) return fail(c, CLH_ERR_ARG, "set_inv_cov: ord must be a permutation of 0,1,2");
   if (n != c->L.dv_size()) return fail(c, CLH_ERR_ARG, "set_inv_cov: %d rows != data vector size %d", n, c->L.dv_size());
   if (has_nan(inv_cov, n * n)) return fail(c, CLH_ERR_NAN, "set_inv_cov: NaN");
   int rc = ensure_device(c);
@@ -1379,11 +1412,78 @@ int clh_chi2(clh_ctx* c, const double* dv, int n, double* chi2) {
 
 int clh_datavector_size(clh_ctx* c) { return c ? c->L.dv_size() : 0; }
 
+// init_binning_fourier (generic_interface.cpp:425-470): like.ell at the
+// log-centres of nells bins in [lmin, lmax]
+int clh_init_binning_fourier(clh_ctx* c, int nells, int lmin, int lmax, int lmax_shear) {
+  if (!c) return CLH_ERR_ARG;
+  if (!(nells > 0) || !(lmin > 0) || !(lmax > lmin))
+    return fail(c, CLH_ERR_ARG, "bad Fourier binning (nells=%d, lmin=%d, lmax=%d)", nells, lmin, lmax);
+  c->L.ell_f.resize(nells);
+  const double logdl = (std::log((double)lmax) - std::log((double)lmin)) / (double)nells;
+  for (int i = 0; i < nells; i++) c->L.ell_f[i] = std::exp(std::log((double)lmin) + (i + 0.5) * logdl);
+  c->L.lmax_shear = lmax_shear;
+  c->version++;
+  return CLH_OK;
+}
+
+int clh_datavector_size_fourier(clh_ctx* c) {
+  if (!c) return 0;
+  return (int)c->L.ell_f.size() * (c->L.n_ss() + c->L.n_gs() + c->L.lens.nbin);
+}
+
+// compute_Mx2pt_N_masked<1,3> (generic_interface.hpp:1137-1159, 968-1041)
+// C_ss (EE, zero at ell >= lmax_shear), C_gs, C_gg at like.ell by the direct
+// GL integrals, shear calibration and the mask in data-vector order
+// (add_calib_and_set_mask_X_N<1,M>, generic_interface.hpp:800-862; no point
+// mass in Fourier space); mask NULL = all ones
+int clh_compute_3x2pt_fourier(clh_ctx* c, const int ord[3], const int* mask, double* dv, int n) {
+  if (!c || !ord || !dv) return CLH_ERR_ARG;
+  const Layout& L = c->L;
+  const int ncl = (int)L.ell_f.size();
+  if (ncl == 0) return fail(c, CLH_ERR_STATE, "Fourier binning not initialized");
+  if (n != clh_datavector_size_fourier(c)) return fail(c, CLH_ERR_ARG, "dv size %d != %d", n, clh_datavector_size_fourier(c));
+  const int nss = L.n_ss(), ngs = L.n_gs(), nl = L.lens.nbin, ns = L.src.nbin;
+  const int sizes[3] = {ncl * nss, ncl * ngs, ncl * nl};
+  if (!blocks_tile(sizes, ord))
+    return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");
+  int start[3];
+  block_starts(sizes, ord, start);
+  const int o3[3] = {0, 1, 2};
+  std::vector<const Draw*> v{&c->cur};
+  int rc = build_batch_ctx(c, c->scratch, v, 0, o3, &L.ell_f, nullptr);
+  if (rc) return rc;
+  if ((rc = run_batch(c, c->scratch, false))) return rc;
+  dvec cl((size_t)(nss + ngs + nl) * ncl);
+  HIPCHK(c, hipMemcpyAsync(cl.data(), c->scratch.desc[0].cl, cl.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<double> out(n, 0.0);
+  const double* m = c->cur.shear_m;
+  for (int q = 0; q < nss; q++) {
+    int z1 = 0, z2 = q;
+    while (z2 >= ns - z1) { z2 -= ns - z1; z1++; }
+    z2 += z1;
+    for (int i = 0; i < ncl; i++)
+      if (L.ell_f[i] < L.lmax_shear)
+        out[start[0] + ncl * q + i] = cl[(size_t)q * ncl + i] * (1.0 + m[z1]) * (1.0 + m[z2]);
+  }
+  for (int q = 0; q < ngs; q++)
+    for (int i = 0; i < ncl; i++)
+      out[start[1] + ncl * q + i] = cl[(size_t)(nss + q) * ncl + i] * (1.0 + m[q % ns]);
+  for (int q = 0; q < nl; q++)
+    for (int i = 0; i < ncl; i++) out[start[2] + ncl * q + i] = cl[(size_t)(nss + ngs + q) * ncl + i];
+  if (mask)
+    for (int k = 0; k < n; k++)
+      if (!mask[k]) out[k] = 0.0;
+  std::copy(out.begin(), out.end(), dv);
+  return CLH_OK;
+}
+
 int clh_compute_3x2pt_real(clh_ctx* c, const int ord[3], double* dv, int n) {
   if (!c || !ord || !dv) return CLH_ERR_ARG;
   if (n != c->L.dv_size()) return fail(c, CLH_ERR_ARG, "dv size %d != %d", n, c->L.dv_size());
-  for (int i = 0; i < 3; i++)
-    if (ord[i] < 0) return fail(c, CLH_ERR_ARG, "bad ord");
+  if (!ord_ok(c->L, ord))
+    return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");
   std::vector<const Draw*> v{&c->cur};
   int rc = build_batch_ctx(c, c->single, v, 1, ord, nullptr, nullptr);
   if (rc) return rc;
@@ -1604,6 +1704,9 @@ int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
   if (c->batch.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
   if ((c->batch.ctx_layout && !c->dl.valid) || c->batch_epoch != c->mode_epoch)
     return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
+  for (int i = 0; i < c->batch.B; i++)
+    if ((i == 0 || c->batch.lays[i] != c->batch.lays[i - 1]) && !ord_ok(*c->batch.lays[i], ord))
+      return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");
   // per-call placement: output pointer and block order go into the descriptors
   bool changed = false;
   for (int i = 0; i < c->batch.B; i++) {
@@ -1815,6 +1918,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
   if (!bt.uniform) return fail(c, CLH_ERR_ARG, "VJP needs a batch with one layout for every draw");
   const Layout& L = *bt.lays[0];
+  if (!ord_ok(L, ord))
+    return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");
   const int B = bt.B, ns = L.src.nbin, nl = L.lens.nbin, nrow = L.nrow();
   const int ncol = L.n_ss() + L.n_gs() + nl;
   HIPCHK(c, hipSetDevice(c->device));

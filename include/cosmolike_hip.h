@@ -107,6 +107,19 @@ int clh_set_nuisance_ia(clh_ctx* ctx, const double* A1, int n);
  * generic_interface.hpp:843-858). Default 0. */
 int clh_set_nuisance_pointmass(clh_ctx* ctx, const double* pm, int n);
 
+/* ---------------------------------------------------- Fourier-space 3x2pt */
+/* init_binning_fourier (generic_interface.cpp:425-470): like.ell at the
+ * log-centres of nells bins in [lmin, lmax]; cosmic shear is kept below
+ * lmax_shear */
+int clh_init_binning_fourier(clh_ctx* ctx, int nells, int lmin, int lmax, int lmax_shear);
+int clh_datavector_size_fourier(clh_ctx* ctx);
+/* compute_Mx2pt_N_masked<1,3> (generic_interface.hpp:1137-1159): C_ss (EE,
+ * 0 at ell >= lmax_shear), C_gs, C_gg at like.ell by the direct GL integrals
+ * (C_*_tomo_limber_nointerp), shear calibration, blocks placed by ord as
+ * compute_data_vector_Mx2pt_N_starts<1,3>; mask (NULL = all ones) holds n
+ * entries in the same order as dv, masked entries are 0 */
+int clh_compute_3x2pt_fourier(clh_ctx* ctx, const int ord[3], const int* mask, double* dv, int n);
+
 /* ------------------------------------------------------------ likelihood */
 /* IP::set_mask (generic_interface.hpp:1234-1331): 0/1 per data-vector entry,
  * in the block order ord (NULL clears). Masked entries of every computed

@@ -1270,17 +1270,36 @@ static double point_mass(oc_ctx* c, int zl, int zs, double theta) {
          (chi_lens * a_lens * a_lens * a_lens);
 }
 
+/* compute_data_vector_Mx2pt_N_starts (generic_interface.hpp:770-791):
+ * stable_sort_index(ord, "ascend"), then start(i) = sum of sizes(indices(j))
+ * over j < indices(i).  Returns -1 unless the blocks tile [0, n): a 3-cycle
+ * ord such as (1,2,0) puts a block past the end of the data vector. */
+static int block_starts(const int sizes[3], const int ord[3], int start[3]) {
+  for (int i = 0; i < 3; i++)
+    if (ord[i] < 0 || ord[i] > 2) return -1;
+  if (ord[0] == ord[1] || ord[0] == ord[2] || ord[1] == ord[2]) return -1;
+  int idx[3] = {0, 1, 2};
+  for (int i = 1; i < 3; i++)
+    for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) { int t = idx[j]; idx[j] = idx[j - 1]; idx[j - 1] = t; }
+  for (int i = 0; i < 3; i++) {
+    start[i] = 0;
+    for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+  }
+  const int n = sizes[0] + sizes[1] + sizes[2];
+  for (int i = 0; i < 3; i++) {
+    if (start[i] + sizes[i] > n) return -1;
+    for (int j = 0; j < i; j++)
+      if (sizes[i] && sizes[j] && start[i] < start[j] + sizes[j] && start[j] < start[i] + sizes[i]) return -1;
+  }
+  return 0;
+}
+
 int oc_datavector_3x2pt(oc_ctx* c, const int ord[3], double* dv) {
   c->status = 0;
   const int nt = c->ntheta, nss = oc_npairs_ss(c), ngs = oc_npairs_gs(c), ngg = c->nl;
   const int sizes[3] = {2 * nt * nss, nt * ngs, nt * ngg};
-  /* stable_sort_index(ord, "ascend") then the reference's start formula */
-  int idx[3] = {0, 1, 2};
-  for (int i = 1; i < 3; i++)
-    for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) { int t = idx[j]; idx[j] = idx[j - 1]; idx[j - 1] = t; }
-  int start[3] = {0, 0, 0};
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+  int start[3];
+  if (block_starts(sizes, ord, start)) { oc_set_err(c, "ord: blocks do not tile the data vector"); return -1; }
   const int n = sizes[0] + sizes[1] + sizes[2];
   double* xp = malloc(sizeof(double) * nss * nt);
   double* xm = malloc(sizeof(double) * nss * nt);
@@ -1324,6 +1343,44 @@ int oc_datavector_3x2pt(oc_ctx* c, const int ord[3], double* dv) {
   }
   free(xp); free(xm); free(gt); free(w); free(theta);
   return c->status;
+}
+
+/* compute_Mx2pt_N_masked<1,3> (generic_interface.hpp:1137-1159): C_ss EE
+ * (0 at ell >= lmax_shear, compute_X_N_masked<1,0>, :968-1004), C_gs, C_gg
+ * at the given ells by the direct GL integrals, shear calibration
+ * (add_calib_and_set_mask_X_N<1,M>, :800-862, no point mass in Fourier
+ * space); mask (NULL = all ones) in the data-vector order of dv */
+int oc_datavector_fourier(oc_ctx* c, const double* ell, int ncl, double lmax_shear, const int ord[3],
+                          const int* mask, double* dv) {
+  const int nss = oc_npairs_ss(c), ngs = oc_npairs_gs(c), ngg = c->nl;
+  const int sizes[3] = {ncl * nss, ncl * ngs, ncl * ngg};
+  int start[3];
+  c->status = 0;
+  if (block_starts(sizes, ord, start)) { oc_set_err(c, "ord: blocks do not tile the data vector"); return -1; }
+  double* ss = malloc(sizeof(double) * (nss * ncl + 1));
+  double* gs = malloc(sizeof(double) * (ngs * ncl + 1));
+  double* gg = malloc(sizeof(double) * (ngg * ncl + 1));
+  int rc = oc_cl_ss_nointerp(c, ell, ncl, ss);
+  if (!rc && ngg > 0) rc = oc_cl_gs_nointerp(c, ell, ncl, gs) || oc_cl_gg_nointerp(c, ell, ncl, gg);
+  if (rc) { free(ss); free(gs); free(gg); return -1; }
+  for (int k = 0; k < sizes[0] + sizes[1] + sizes[2]; k++) dv[k] = 0.0;
+  for (int nz = 0; nz < nss; nz++) {
+    int z1, z2;
+    ss_pair(c->ns, nz, &z1, &z2);
+    for (int i = 0; i < ncl; i++) {
+      const int k = start[0] + ncl * nz + i;
+      dv[k] = ell[i] < lmax_shear ? ss[nz * ncl + i] * (1.0 + c->shear_m[z1]) * (1.0 + c->shear_m[z2]) : 0.0;
+    }
+  }
+  for (int nz = 0; nz < ngs; nz++)
+    for (int i = 0; i < ncl; i++) dv[start[1] + ncl * nz + i] = gs[nz * ncl + i] * (1.0 + c->shear_m[nz % c->ns]);
+  for (int nz = 0; nz < ngg; nz++)
+    for (int i = 0; i < ncl; i++) dv[start[2] + ncl * nz + i] = gg[nz * ncl + i];
+  if (mask)
+    for (int k = 0; k < sizes[0] + sizes[1] + sizes[2]; k++)
+      if (!mask[k]) dv[k] = 0.0;
+  free(ss); free(gs); free(gg);
+  return 0;
 }
 
 /* chi^2 = sum_ij r_i C^-1_ij r_j over unmasked entries, r = dv - data

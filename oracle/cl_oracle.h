@@ -64,6 +64,9 @@ int oc_cl_tables(oc_ctx* c, double* ss, double* gs, double* gg);
 int oc_cl_ss_nointerp(oc_ctx* c, const double* l, int nl, double* out);
 int oc_cl_gs_nointerp(oc_ctx* c, const double* l, int nl, double* out);
 int oc_cl_gg_nointerp(oc_ctx* c, const double* l, int nl, double* out);
+/* Fourier-space 3x2pt data vector at the given ells (compute_Mx2pt_N_masked<1,3>) */
+int oc_datavector_fourier(oc_ctx* c, const double* ell, int ncl, double lmax_shear, const int ord[3],
+                          const int* mask, double* dv);
 /* C_ell filled at integer ell 0..LMAX-1 as the real-space routines use them:
  * ss[npairs][LMAX], gs[n_ggl][LMAX], gg[nlens][LMAX] (NULL to skip) */
 int oc_cl_full(oc_ctx* c, double* ss, double* gs, double* gg);

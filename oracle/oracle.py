@@ -54,6 +54,9 @@ def _load():
         lib.oc_set_pointmass.argtypes = [vp, _D, ctypes.c_int]
         lib.oc_set_mask.restype = ctypes.c_int
         lib.oc_set_mask.argtypes = [vp, _I, ctypes.c_int]
+        lib.oc_datavector_fourier.restype = ctypes.c_int
+        lib.oc_datavector_fourier.argtypes = [vp, _D, ctypes.c_int, ctypes.c_double,
+                                              ctypes.POINTER(ctypes.c_int), _I, _D]
         lib.oc_chi2.restype = ctypes.c_double
         lib.oc_chi2.argtypes = [_D, _D, _D, _I, ctypes.c_int]
         _lib = lib
@@ -293,6 +296,16 @@ class Oracle:
         out = np.empty(self.nl * self.ntheta)
         self._chk(self.lib.oc_wtheta(self.c, _p(out)))
         return out.reshape(self.nl, self.ntheta)
+
+    def datavector_fourier(self, ells, lmax_shear, ord=(0, 1, 2), mask=None):
+        ells = _arr(ells)
+        n = len(ells) * (self.n_ss + self.n_gs + self.nl)
+        out = np.empty(n)
+        o = (ctypes.c_int * 3)(*ord)
+        m = None if mask is None else np.ascontiguousarray(np.asarray(mask, dtype=np.int32))
+        self._chk(self.lib.oc_datavector_fourier(self.c, _p(ells), len(ells), float(lmax_shear), o,
+                                                 None if m is None else m.ctypes.data_as(_I), _p(out)))
+        return out
 
     def datavector(self, ord=(0, 1, 2)):
         n = self.lib.oc_datavector_size(self.c)

@@ -95,6 +95,36 @@ def test_datavector_sizes(ctx):
     assert ctx.datavector_size() == 26 * (2 * 55 + 50 + 5) == 4290
 
 
+def test_fourier_binning_validates(ctx):
+    p = synth.full_3x2pt_problem()
+    ctx.load(p)
+    assert ctx.datavector_size_fourier() == 0
+    ctx.init_binning_fourier(15, 30, 3000, 1000)
+    assert ctx.datavector_size_fourier() == 15 * (55 + 50 + 5)
+    for bad in ((0, 30, 3000, 1000), (15, 0, 3000, 1000), (15, 3000, 30, 1000)):
+        with pytest.raises(CosmolikeError) as e:
+            ctx.init_binning_fourier(*bad)
+        assert e.value.code == -1
+    assert ctx.datavector_size_fourier() == 15 * 110
+    with pytest.raises(ValueError):
+        ctx.compute_3x2pt_fourier(mask=np.ones(5))
+
+
+@pytest.mark.parametrize("ord", [(1, 2, 0), (2, 0, 1), (0, 0, 1), (0, 1, 3), (-1, 0, 1)])
+def test_orders_that_do_not_tile_are_refused(ctx, ord):
+    """The reference's block-start formula (generic_interface.hpp:781-787)
+    tiles the data vector only for involutive orders; any other order is
+    refused before any device work, for real and Fourier space alike."""
+    p = synth.full_3x2pt_problem()
+    ctx.load(p)
+    ctx.init_binning_fourier(3, 30, 3000, 1000)
+    for fn in (lambda: ctx.compute_3x2pt_real(ord), lambda: ctx.compute_3x2pt_fourier(ord),
+               lambda: ctx.like_set_mask(np.ones(ctx.datavector_size(), np.int32), ord)):
+        with pytest.raises(CosmolikeError) as e:
+            fn()
+        assert e.value.code == -1
+
+
 def test_compute_without_device_fails_cleanly(ctx):
     try:
         import torch

@@ -472,8 +472,8 @@ def test_mask_zeroes_entries():
     o.set_mask(mask)
     dv = o.datavector()
     assert np.all(dv[mask == 0] == 0) and np.array_equal(dv[mask == 1], ref[mask == 1])
-    # block order: the mask is given in the (0,1,2) order; ord (2,0,1) moves blocks, not entries
-    dv2 = o.datavector((2, 0, 1))
+    # block order: the mask is given in the (0,1,2) order; ord (2,1,0) moves blocks, not entries
+    dv2 = o.datavector((2, 1, 0))
     n_w = p.nl * p.ntheta
     assert np.array_equal(dv2[:n_w], dv[-n_w:])
 
@@ -492,3 +492,60 @@ def test_chi2_matches_numpy():
     r = (dv - data)[keep]
     ref = r @ np.linalg.solve(cov[np.ix_(keep, keep)], r)
     assert abs(O.chi2(dv, data, icov, mask) - ref) <= 1e-10 * ref
+
+
+# ------------------------------------------------------- Fourier-space 3x2pt
+def fourier_ells(n, lmin, lmax):
+    """Geometric centres sqrt(e_i e_{i+1}) of n log-spaced bins in [lmin, lmax]."""
+    e = lmin * (lmax / lmin) ** (np.arange(n + 1) / n)
+    return np.sqrt(e[:-1] * e[1:])
+
+
+def test_fourier_datavector_pins(p3):
+    """compute_Mx2pt_N_masked<1,3>: ss EE cut at lmax_shear, (1+m) factors,
+    block order, mask; C_ell themselves are the direct integrals pinned by
+    test_limber_ss_gs_gg_vs_continuum."""
+    ells = fourier_ells(15, 30.0, 3000.0)
+    o = O.Oracle().load(p3)
+    ncl, nss, ngs, nl = 15, 55, 50, 5
+    dv0 = o.datavector_fourier(ells, 1e9)
+    assert dv0.size == ncl * (nss + ngs + nl)
+    # no cut, no calibration: the three direct-integral blocks
+    assert np.array_equal(dv0[: ncl * nss], o.cl_nointerp("ss", ells).ravel())
+    assert np.array_equal(dv0[ncl * (nss + ngs):], o.cl_nointerp("gg", ells).ravel())
+    # lmax_shear zeroes shear-shear only, at ell >= lmax_shear
+    cut = o.datavector_fourier(ells, 1000.0)
+    keep = ells < 1000.0
+    assert 0 < keep.sum() < ncl
+    ss0, ss1 = dv0[: ncl * nss].reshape(nss, ncl), cut[: ncl * nss].reshape(nss, ncl)
+    assert np.all(ss1[:, ~keep] == 0) and np.array_equal(ss1[:, keep], ss0[:, keep])
+    assert np.all(ss0[:, keep] > 0)
+    assert np.array_equal(cut[ncl * nss:], dv0[ncl * nss:])
+    # calibration: (1+m_i)(1+m_j) on ss, (1+m_s) on gs, none on gg
+    m = np.linspace(-0.02, 0.03, p3.ns)
+    o.set_nuisance(shear_m=m)
+    dvm = o.datavector_fourier(ells, 1e9)
+    iu = np.triu_indices(p3.ns)
+    f_ss = ((1 + m)[iu[0]] * (1 + m)[iu[1]])[:, None]
+    assert np.allclose(dvm[: ncl * nss].reshape(nss, ncl), ss0 * f_ss, rtol=1e-15, atol=0)
+    f_gs = np.tile(1 + m, nl)[:, None]
+    gs0 = dv0[ncl * nss: ncl * (nss + ngs)].reshape(ngs, ncl)
+    assert np.allclose(dvm[ncl * nss: ncl * (nss + ngs)].reshape(ngs, ncl), gs0 * f_gs, rtol=1e-15, atol=0)
+    assert np.array_equal(dvm[ncl * (nss + ngs):], dv0[ncl * (nss + ngs):])
+    # block order (generic_interface.hpp:770-791; start(i) sums the sizes of
+    # the blocks sorted before position indices(i), which for an involutive
+    # ord such as (1, 0, 2) puts gs first) and the mask in output order
+    rev = o.datavector_fourier(ells, 1e9, (1, 0, 2))
+    s = [ncl * nss, ncl * ngs, ncl * nl]
+    assert np.array_equal(rev[: s[1]], dvm[s[0]: s[0] + s[1]])
+    assert np.array_equal(rev[s[1]: s[1] + s[0]], dvm[: s[0]])
+    assert np.array_equal(rev[s[0] + s[1]:], dvm[s[0] + s[1]:])
+    mask = (np.arange(rev.size) % 3 != 0).astype(np.int32)
+    mrev = o.datavector_fourier(ells, 1e9, (1, 0, 2), mask)
+    assert np.all(mrev[mask == 0] == 0) and np.array_equal(mrev[mask == 1], rev[mask == 1])
+    # 3-cycles put blocks past the end of the data vector: refused
+    for bad in ((1, 2, 0), (2, 0, 1)):
+        with pytest.raises(O.OracleError):
+            o.datavector_fourier(ells, 1e9, bad)
+        with pytest.raises(O.OracleError):
+            o.datavector(bad)
