@@ -225,7 +225,7 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
   extern __shared__ double Cb[];  // ss: [ns][ns][T], gs: [ns][T]
   __shared__ double L10[CLH_TILE_ROWS];
   __shared__ double s_lk[CLH_LDS_LK];
-  const int4 w = work[blockIdx.x];
+  const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
   const int kind = w.y & 3, zl = w.y >> 2, row0 = w.z, T = w.w;
   const int p = threadIdx.x;
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
   __shared__ double s_lk[CLH_LDS_LK];
   __shared__ double red[2][3];
   extern __shared__ double dyn[];  // staged chi(z), G(z), lens n(z) tables
-  const int4 w = work[blockIdx.x];
+  const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
   const int zl = w.y, row0 = w.z, nr = w.w;
   constexpr int np = CLH_NP_GG;

@@ -5,24 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "clh_internal.hpp"
-
-// ilo/ihi bisection used by every table lookup of cosmo3D.c (e.g. :46-57)
-__device__ __forceinline__ int clh_bs(const double* __restrict__ x, int n, double v) {
-  int ilo = 0, ihi = n - 1;
-  while (ihi > ilo + 1) {
-    const int ll = (ihi + ilo) >> 1;
-    if (x[ll] > v) ihi = ll; else ilo = ll;
-  }
-  return ilo;
-}
-
-// The same index as clh_bs for a strictly increasing table, found by walking
-// from a previous index j (cheap when successive queries move monotonically).
-__device__ __forceinline__ int clh_walk(const double* __restrict__ x, int n, double v, int j) {
-  while (j > 0 && x[j] > v) j--;
-  while (j < n - 2 && x[j + 1] <= v) j++;
-  return j;
-}
+#include "clh_search.hpp"
 
 // chi_all: cosmo3D.c:40-92 (chi and dchi/da, in c/H0 units).  The table is
 // validated on the host to extend two nodes beyond every queried z.

@@ -313,7 +313,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
   __shared__ float Wf[CLH_MAXB * CLH_NP_SS];
   __shared__ double L10[CLH_TILE_ROWS];
   __shared__ double s_lk[CLH_LDS_LK];
-  const int4 w = work[blockIdx.x];
+  const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
   const int kind = w.y & 3, zl = w.y >> 2, row0 = w.z, nr = w.w;
   const bool f32 = F32 && kind == CLH_W_SS;
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ 
   __shared__ ClhGgRow R[CLH_TILE_ROWS];
   __shared__ double s_lk[CLH_LDS_LK];
   extern __shared__ double dyn[];  // staged chi(z), G(z), lens n(z) tables
-  const int4 w = work[blockIdx.x];
+  const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
   const int zl = w.y, row0 = w.z, nr = w.w;
   constexpr int np = CLH_NP_GG;
