@@ -199,7 +199,7 @@ def main():
         c.batch_forward(dv.data_ptr())
         c.synchronize()
         for k, v in c.kernel_times().items():
-            if not k.endswith("_adj"):
+            if not k.endswith("_adj") and k != "k_lnp_gather":
                 acc[k] = acc.get(k, 0.0) + v / nt
     c.enable_timing(False)
 
@@ -238,7 +238,7 @@ def main():
             c.batch_vjp(dvbar.data_ptr(), *ptrs)
             c.synchronize()
             for k, v in c.kernel_times().items():
-                if k.endswith("_adj"):
+                if k.endswith("_adj") or k == "k_lnp_gather":
                     vjp_ms[k] = vjp_ms.get(k, 0.0) + v / 3
         c.enable_timing(False)
         log(f"bench: VJP Limber kernel ms {vjp_ms}")

@@ -163,48 +163,51 @@ __global__ __launch_bounds__(64) void k_m_grad(const DrawDesc* __restrict__ D) {
 }
 
 // ----------------------------------------------------------------------------
-// Cotangent of the ln P table gathered by one GL node (a fixed z column j)
-// along rows of nondecreasing k index i: the four bilinear weights of the
-// current cell accumulate in registers and go out with atomics only when the
-// row sequence leaves the cell.
+// ln P cotangent of one GL node (a fixed z column j) over the rows of a tile,
+// kept as a run along the k index: R[i] = sum over rows with cell i of
+// g (1 - dx) plus sum over rows with cell i - 1 of g dx, where R is the node's
+// own row of the run slab (zeroed before the launch); k_lnp_gather folds the
+// z weights in.  Along increasing ell the cell index i never decreases, so a
+// cell strictly inside a tile's index range is reached by that tile alone and
+// takes a plain store; the last two cells of the range can be shared with the
+// next tile, the first three with the previous one (rows are sorted up to
+// rounding, which can move the first cell back by one), and take atomics.  A
+// decrease of i inside the run switches the rest of it to atomics.
 // ----------------------------------------------------------------------------
-struct ClhLnPAcc {
-  int ci = -1;
-  double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
-  __device__ __forceinline__ void put(double* g, double v) {
-    if (v != 0.0) atomicAdd(g, v);
+struct ClhLnPRun {
+  double* R;
+  int ci = -1, first = -1;
+  bool mono = true;
+  double a0 = 0.0, a1 = 0.0;
+  __device__ __forceinline__ void put(int i, double v, bool edge) {
+    if (v == 0.0) return;
+    if (edge || !mono || i <= first + 2) atomicAdd(R + i, v);
+    else R[i] = v;
   }
-  __device__ __forceinline__ void add(double* __restrict__ G, int nz, int j, int i, double dx, double dy,
-                                      double g) {
+  __device__ __forceinline__ void add(int i, double dx, double g) {
     if (i != ci) {
-      if (ci >= 0) {
-        double* gp = G + (size_t)ci * nz + j;
-        put(gp, s00);
-        put(gp + 1, s01);
+      if (ci < 0) {
+        first = i;
+      } else {
+        if (i < ci) mono = false;
+        put(ci, a0, false);
         if (i == ci + 1) {
-          s00 = s10;
-          s01 = s11;
+          a0 = a1;
         } else {
-          put(gp + nz, s10);
-          put(gp + nz + 1, s11);
-          s00 = s01 = 0.0;
+          put(ci + 1, a1, false);
+          a0 = 0.0;
         }
-        s10 = s11 = 0.0;
+        a1 = 0.0;
       }
       ci = i;
     }
-    s00 += g * ((1 - dx) * (1 - dy));
-    s01 += g * ((1 - dx) * dy);
-    s10 += g * (dx * (1 - dy));
-    s11 += g * (dx * dy);
+    a0 += g * (1 - dx);
+    a1 += g * dx;
   }
-  __device__ __forceinline__ void done(double* __restrict__ G, int nz, int j) {
+  __device__ __forceinline__ void done() {
     if (ci < 0) return;
-    double* gp = G + (size_t)ci * nz + j;
-    put(gp, s00);
-    put(gp + 1, s01);
-    put(gp + nz, s10);
-    put(gp + nz + 1, s11);
+    put(ci, a0, true);
+    put(ci + 1, a1, true);
   }
 };
 
@@ -230,7 +233,7 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
   const int kind = w.y & 3, zl = w.y >> 2, row0 = w.z, T = w.w;
   const int p = threadIdx.x;
   constexpr int np = CLH_NP_SS;
-  const int nrow = d.nrow, ns = d.ns, nz = d.nzp;
+  const int nrow = d.nrow, ns = d.ns;
   const bool ss = kind == CLH_W_SS;
   const double* o = clh_sx_nodes(d, kind, zl);
   const double* lut = o + clh_sx_lut_col(d, kind) * np;
@@ -262,7 +265,8 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
   const double WG = ss ? 0.0 : o[3 * np + p], WM = ss ? 0.0 : o[4 * np + p];
   const double b1 = ss ? 0.0 : d.par->b1[zl], bmag = ss ? 0.0 : d.par->bmag[zl];
   double twg = 0.0, twm = 0.0;
-  ClhLnPAcc sc;
+  ClhLnPRun sc;
+  sc.R = clh_run_row(d, kind, zl, p);
   for (int rr = 0; rr < T; rr++) {
     const double l = ells[row0 + rr], ell = l + 0.5;
     double dx;
@@ -307,9 +311,9 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
       twm += t * pref1;
       g = t * gp;
     }
-    sc.add(d.g_lnP, nz, j, i, dx, dy, g);
+    if (sc.R) sc.add(i, dx, g);
   }
-  sc.done(d.g_lnP, nz, j);
+  if (sc.R) sc.done();
   if (ss) {
 #pragma unroll
     for (int b = 0; b < CLH_MAXB; b++)
@@ -354,7 +358,8 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
   ClhGgNode<true> nd;
   nd.init(d, T, clh_gg_nodes(d, zl), p, zl);
   double tg = 0.0, tm = 0.0, b1p = 0.0, lzp = 0.0;
-  ClhLnPAcc sc;
+  ClhLnPRun sc;
+  sc.R = clh_run_row(d, CLH_W_GG, zl, p);
   const int hn = (nr + 1) / 2;
   const int r_lo = half * hn, r_hi = half ? nr : hn;
   for (int rr = r_lo; rr < r_hi; rr++) {
@@ -362,14 +367,14 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
     if (cb == 0.0) continue;
     double t, pf, dwr, dx;
     if (!nd.eval(d, T, R[rr], t, pf, dwr, dx)) continue;
-    sc.add(d.g_lnP, d.nzp, nd.pk.j, nd.pk.i, dx, nd.pk.dy, cb * t * t * pf);
+    if (sc.R) sc.add(nd.pk.i, dx, cb * t * t * pf);
     const double tb = cb * 2.0 * t * pf;
     tg += tb;
     tm += tb * R[rr].pref;
     b1p += tb * nd.WG;
     lzp += tb * dwr;
   }
-  sc.done(d.g_lnP, d.nzp, nd.pk.j);
+  if (sc.R) sc.done();
   const double b1 = nd.b1, bmag = nd.bmag;
   const size_t base = (size_t)2 * d.nl * CLH_NP_SS + (size_t)zl * 2 * np;
   if (tg != 0.0) atomicAdd(d.wbar + base + p, tg * b1);
@@ -445,6 +450,78 @@ __global__ __launch_bounds__(128) void k_nodes_adj(const DrawDesc* __restrict__ 
 }
 
 // ----------------------------------------------------------------------------
+// k_lnp_gather: the ln P cotangent of a draw from its per-node runs.  Work
+// item (draw, i0): lnP rows i0 .. i0 + ic - 1, every z column.  Node q (ss,
+// gs, gg blocks in run-slab order) with z column j and weight dy adds
+// R_q[i] (1 - dy) to cell (i, j) and R_q[i] dy to (i, j + 1) in an LDS tile
+// (LDS atomics: nodes of two groups may share a column); the tile then goes
+// out with coalesced stores and covers every cell, so g_lnP needs no zeroing.
+// The nodes' (j, dy) are staged in LDS first; group g of ic lanes takes nodes
+// g, g + 256/ic, ..., lane l reading R_q[i0 + l], four nodes per step.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_lnp_gather(const DrawDesc* __restrict__ D,
+                                                    const int2* __restrict__ work, int ic) {
+  extern __shared__ double A[];  // [ic][nz], then dy[total], then j[total]
+  const int2 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
+  const DrawDesc& d = D[w.x];
+  const int nz = d.nzp, nk = d.nk, i0 = w.y;
+  const int n_sx = CLH_NP_SS * (1 + d.nl), total = n_sx + CLH_NP_GG * d.nl;
+  double* s_dy = A + ic * nz;
+  int* s_j = (int*)(s_dy + total);
+  for (int t = threadIdx.x; t < ic * nz; t += 256) A[t] = 0.0;
+  for (int q = threadIdx.x; q < total; q += 256) {
+    const double* lut;
+    int p, np;
+    if (q < n_sx) {
+      const int kind = q < CLH_NP_SS ? CLH_W_SS : CLH_W_GS;
+      lut = clh_sx_nodes(d, kind, q / CLH_NP_SS - 1) + clh_sx_lut_col(d, kind) * CLH_NP_SS;
+      p = q % CLH_NP_SS;
+      np = CLH_NP_SS;
+    } else {
+      lut = clh_gg_nodes(d, (q - n_sx) / CLH_NP_GG) + 5 * CLH_NP_GG;
+      p = (q - n_sx) % CLH_NP_GG;
+      np = CLH_NP_GG;
+    }
+    s_j[q] = (int)lut[p];
+    s_dy[q] = lut[np + p];
+  }
+  __syncthreads();
+  const int li = threadIdx.x % ic, grp = threadIdx.x / ic, ng = 256 / ic;
+  const int i = i0 + li;
+  const bool in = i < nk;
+  const double* __restrict__ R = d.run + i;
+  double* Al = A + li * nz;
+  int q = grp;
+  for (; q + 3 * ng < total; q += 4 * ng) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = in ? R[(size_t)(q + u * ng) * nk] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (v[u] != 0.0) {
+        const int j = s_j[q + u * ng];
+        const double dy = s_dy[q + u * ng];
+        atomicAdd(Al + j, v[u] * (1 - dy));
+        atomicAdd(Al + j + 1, v[u] * dy);
+      }
+    }
+  }
+  for (; q < total; q += ng) {
+    const double v = in ? R[(size_t)q * nk] : 0.0;
+    if (v != 0.0) {
+      const int j = s_j[q];
+      const double dy = s_dy[q];
+      atomicAdd(Al + j, v * (1 - dy));
+      atomicAdd(Al + j + 1, v * dy);
+    }
+  }
+  __syncthreads();
+  const int rows = nk - i0 < ic ? nk - i0 : ic;
+  double* out = d.g_lnP + (size_t)i0 * nz;
+  for (int t = threadIdx.x; t < rows * nz; t += 256) out[t] = A[t];
+}
+
+// ----------------------------------------------------------------------------
 hipError_t clh_launch_fill(double* p, size_t n, double v, hipStream_t s) {
   if (!p || n == 0) return hipSuccess;
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
@@ -479,6 +556,14 @@ hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, 
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)stage_doubles;
   hipLaunchKernelGGL(k_limber_gg_adj, dim3(n), dim3(192), lds, s, D, work, stage_doubles > 0 ? 1 : 0);
+  return hipGetLastError();
+}
+hipError_t clh_launch_lnp_gather(const DrawDesc* D, const int2* work, int n, int ic, int nz, int nl,
+                                 hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const size_t total = (size_t)CLH_NP_SS * (1 + nl) + (size_t)CLH_NP_GG * nl;
+  const size_t lds = sizeof(double) * ((size_t)ic * nz + total) + sizeof(int) * total;
+  hipLaunchKernelGGL(k_lnp_gather, dim3(n), dim3(256), lds, s, D, work, ic);
   return hipGetLastError();
 }
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s) {

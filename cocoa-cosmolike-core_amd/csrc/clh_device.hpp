@@ -309,16 +309,6 @@ __device__ __forceinline__ double clh_p_nonlin_cells(const DrawDesc& d, double k
   return exp(out) / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
 }
 
-__device__ __forceinline__ void clh_scatter_lnP(const DrawDesc& d, int cell, const double w[4],
-                                                double g) {
-  if (g == 0.0) return;
-  double* G = d.g_lnP + cell;
-  atomicAdd(G, g * w[0]);
-  atomicAdd(G + 1, g * w[1]);
-  atomicAdd(G + d.nzp, g * w[2]);
-  atomicAdd(G + d.nzp + 1, g * w[3]);
-}
-
 // index of the ss pair (i, j), i <= j, in the row-major upper triangle
 __device__ __forceinline__ int clh_ss_index(int ns, int i, int j) {
   if (i > j) { const int t = i; i = j; j = t; }
@@ -343,6 +333,18 @@ __device__ __forceinline__ int clh_sx_lut_col(const DrawDesc& d, int kind) {
 }
 __device__ __forceinline__ const double* clh_gg_nodes(const DrawDesc& d, int zl) {
   return d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride;
+}
+
+// Row of the per-node ln P cotangent runs of GL node p: the ss nodes, then
+// 64 per gs lens bin, then 96 per gg lens bin (null when the caller did not
+// ask for the ln P gradient).
+__device__ __forceinline__ double* clh_run_row(const DrawDesc& d, int kind, int zl, int p) {
+  if (!d.run) return nullptr;
+  size_t r;
+  if (kind == CLH_W_SS) r = p;
+  else if (kind == CLH_W_GS) r = (size_t)CLH_NP_SS * (1 + zl) + p;
+  else r = (size_t)CLH_NP_SS * (1 + d.nl) + (size_t)CLH_NP_GG * zl + p;
+  return d.run + r * d.nk;
 }
 
 // stage a table of n doubles in LDS when it fits (cap entries), else use it

@@ -42,6 +42,8 @@ hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, 
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles,
                                    hipStream_t s);
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_lnp_gather(const DrawDesc* D, const int2* work, int n, int ic, int nz, int nl,
+                                 hipStream_t s);
 hipError_t clh_launch_legendre_mx(const DrawDesc* D, const int4* work, int n, int mode, hipStream_t s);
 hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, int max_nrow,
                                     hipStream_t s);
@@ -217,8 +219,9 @@ struct Batch {
   std::vector<DrawParams> par;
   DBuf<DrawDesc> d_desc;
   DBuf<DrawParams> d_par;
-  DBuf<double> d_tab, d_work, d_dv, d_ells, d_vjp, d_gscr;
-  DBuf<int2> d_wg, d_wn, d_wl2, d_wa;
+  DBuf<double> d_tab, d_work, d_dv, d_ells, d_vjp, d_gscr, d_run;
+  DBuf<int2> d_wg, d_wn, d_wl2, d_wa, d_wlg;
+  int n_wlg = 0, lnp_ic = 0;  // ln P gather: work items (draw, first k row), k rows per item
   DBuf<int4> d_wl, d_wsx, d_wgg, d_wga, d_wmx;
   int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0, n_wsx = 0, n_wgg = 0, n_wga = 0, n_wmx = 0;
   int nrow_custom = 0;  // rows of a custom-ell batch (0: the layouts' rows)
@@ -232,7 +235,7 @@ struct Batch {
   int ord[3] = {0, 1, 2};
   void release() {
     d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
-    d_vjp.release(); d_gscr.release();
+    d_vjp.release(); d_gscr.release(); d_run.release(); d_wlg.release();
     d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa.release();
     d_wsx.release(); d_wgg.release(); d_wga.release(); d_wmx.release();
   }
@@ -266,7 +269,7 @@ struct clh_ctx {
   hipStream_t gstream = nullptr;      // stream the graph was captured on
   hipEvent_t ev[10] = {nullptr};
   float last_ms[5] = {0, 0, 0, 0, 0};
-  float last_vjp_ms[2] = {0, 0};
+  float last_vjp_ms[3] = {0, 0, 0};
   bool vjp_timed = false;
   // likelihood (IP): data and inverse covariance in the block order like_ord
   int like_n = 0, like_ord[3] = {0, 1, 2};
@@ -1763,7 +1766,7 @@ int clh_synchronize(clh_ctx* c) {
   if (c->timing) {
     for (int k = 0; k < 5; k++) HIPCHK(c, hipEventElapsedTime(&c->last_ms[k], c->ev[k], c->ev[k + 1]));
     if (c->vjp_timed)
-      for (int k = 0; k < 2; k++) HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[k], c->ev[6 + k], c->ev[7 + k]));
+      for (int k = 0; k < 3; k++) HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[k], c->ev[6 + k], c->ev[7 + k]));
   }
   return CLH_OK;
 }
@@ -1782,13 +1785,13 @@ int clh_enable_timing(clh_ctx* c, int on) {
 }
 
 int clh_last_kernel_times(clh_ctx* c, const char** names, float* ms, int cap) {
-  static const char* nm[7] = {"k_gtab", "k_nodes", "k_limber_gg", "k_limber_sx", "k_legendre",
-                              "k_limber_gg_adj", "k_limber_sx_adj"};
+  static const char* nm[8] = {"k_gtab", "k_nodes", "k_limber_gg", "k_limber_sx", "k_legendre",
+                              "k_limber_gg_adj", "k_limber_sx_adj", "k_lnp_gather"};
   static const char* nm_full = "k_legendre_full";
   static const char* nm32[3] = {"k_limber_gg_f32", "k_limber_sx_f32", "k_legendre_mx"};
   if (!c) return CLH_ERR_ARG;
   const bool mixed = c->precision != CLH_PREC_FP64;
-  const int n = c->vjp_timed ? 7 : 5;
+  const int n = c->vjp_timed ? 8 : 5;
   for (int k = 0; k < n && k < cap; k++) {
     if (names) {
       names[k] = (mixed && k >= 2 && k < 5) ? nm32[k - 2] : nm[k];
@@ -1931,6 +1934,34 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
                n_dg = al((size_t)(ns + nl) * L.n_a);
   const size_t per = n_cbar + n_raw + n_abar + n_bbar + n_wbar + n_dg;
   HIPCHK(c, bt.d_vjp.ensure(per * B));
+  // per-node ln P cotangent runs and the gather work list (k rows per item:
+  // the LDS tile of k rows x nz columns plus the nodes' (j, dy) fits 64 KB)
+  const int nk = bt.desc[0].nk, nzp = bt.desc[0].nzp;
+  const size_t n_run = al((size_t)(CLH_NP_SS * (1 + nl) + CLH_NP_GG * nl) * nk);
+  if (g_lnP) {
+    const size_t nq = (size_t)CLH_NP_SS * (1 + nl) + (size_t)CLH_NP_GG * nl;
+    auto lds = [&](int ic) { return sizeof(double) * ((size_t)ic * nzp + nq) + sizeof(int) * nq; };
+    int ic = 16;
+    while (ic > 2 && lds(ic) > 65536) ic /= 2;
+    if (lds(ic) > 65536)
+      return fail(c, CLH_ERR_ARG, "ln P gradient: %d z values of P(k,z) with %d lens bins exceed the LDS tile",
+                  nzp, nl);
+    // rows in increasing ell up to rounding (exp(ln 19) sits a few ulp below
+    // the direct row 19), so a node's k cell moves back by at most one
+    for (const dvec* e : {&L.ell_ss, &L.ell_gx})
+      for (size_t r = 1; r < e->size(); r++)
+        if ((*e)[r] < (*e)[r - 1] * (1.0 - 1e-12)) return fail(c, CLH_ERR_STATE, "ell rows not sorted");
+    HIPCHK(c, bt.d_run.ensure(n_run * B));
+    if (bt.lnp_ic != ic || bt.n_wlg != B * ((nk + ic - 1) / ic)) {
+      std::vector<int2> wlg;
+      for (int i = 0; i < B; i++)
+        for (int i0 = 0; i0 < nk; i0 += ic) wlg.push_back(make_int2(i, i0));
+      HIPCHK(c, bt.d_wlg.ensure(wlg.size()));
+      HIPCHK(c, hipMemcpyAsync(bt.d_wlg.p, wlg.data(), wlg.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+      bt.n_wlg = (int)wlg.size();
+      bt.lnp_ic = ic;
+    }
+  }
   // scratch for gradients the caller does not want
   const size_t gper = al(bt.nknz) + 4 * al(CLH_MAXB);
   HIPCHK(c, bt.d_gscr.ensure(gper * B));
@@ -1952,6 +1983,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     double* g = bt.d_gscr.p + gper * i;
     q.dvbar = dv_bar + (size_t)bt.dv_size * i;
     q.g_lnP = g_lnP ? g_lnP + (size_t)bt.nknz * i : g;
+    q.run = g_lnP ? bt.d_run.p + n_run * i : nullptr;
     q.g_src = g_src_dz ? g_src_dz + (size_t)ns * i : g + al(bt.nknz);
     q.g_lens = g_lens_dz ? g_lens_dz + (size_t)nl * i : g + al(bt.nknz) + al(CLH_MAXB);
     q.g_b1 = g_b1 ? g_b1 + (size_t)nl * i : g + al(bt.nknz) + 2 * al(CLH_MAXB);
@@ -1963,7 +1995,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // zero the accumulators (fill kernels, not memsets: safe under graph capture)
   HIPCHK(c, clh_launch_fill(bt.d_vjp.p, per * B, 0.0, s));
   HIPCHK(c, clh_launch_fill(bt.d_gscr.p, gper * B, 0.0, s));
-  if (g_lnP) HIPCHK(c, clh_launch_fill(g_lnP, (size_t)bt.nknz * B, 0.0, s));
+  if (g_lnP) HIPCHK(c, clh_launch_fill(bt.d_run.p, n_run * B, 0.0, s));  // g_lnP: every cell written by the gather
   if (g_src_dz) HIPCHK(c, clh_launch_fill(g_src_dz, (size_t)ns * B, 0.0, s));
   if (g_lens_dz && nl) HIPCHK(c, clh_launch_fill(g_lens_dz, (size_t)nl * B, 0.0, s));
   if (g_b1 && nl) HIPCHK(c, clh_launch_fill(g_b1, (size_t)nl * B, 0.0, s));
@@ -1984,6 +2016,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
   HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl, bt.adj_lds, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
+  if (g_lnP) HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[9], s));
   c->vjp_timed = c->timing;
   bt.state_valid = true;
   HIPCHK(c, clh_launch_nodes_adj(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));

@@ -106,6 +106,7 @@ struct DrawDesc {
   double* wbar;         // [nl][2][64] gs nodes (W_gal, W_mag), then [nl][2][96] gg nodes
   double* dgtab;        // [ns+nl][n_a] d g / d(photo-z shift of its own bin)
   double* g_lnP;        // [nk][nzp]
+  double* run;          // [64 (1 + nl) + 96 nl][nk] per-GL-node ln P cotangent runs, or null
   double* g_src;        // [ns]
   double* g_lens;       // [nl]
   double* g_b1;         // [nl]
