@@ -210,7 +210,11 @@ struct LayoutCopy {  // a context's own copy of a captured layout, with its devi
 
 struct Batch {
   int B = 0;
-  int max_nrow = 0, adj_lds = 0;  // adj_lds: LDS doubles of the ss/gs adjoint
+  int max_nrow = 0;
+  // ss / gs adjoint: the work list holds every ss item, then every gs item,
+  // launched separately with their own LDS (doubles) so the small gs tiles
+  // are not held to the occupancy of the ss ones
+  int adj_lds_ss = 0, adj_lds_gs = 0, n_wl_ss = 0;
   bool ctx_layout = true;     // every draw uses the context's own layout
   bool uniform = true;        // every draw has the same layout
   std::vector<const Layout*> lays;
@@ -788,8 +792,8 @@ static bool ord_ok(const Layout& L, const int ord[3]) {
 // adjoint ss/gs and adjoint gg Limber stages (1..CLH_TILE_ROWS).
 static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   std::vector<int2> wg, wn, wl2, wa;
-  std::vector<int4> wl, wsx, wgg, wga, wmx;
-  int adj_lds = 0;
+  std::vector<int4> wl, wl_gs, wsx, wgg, wga, wmx;
+  int lds_ss = 0, lds_gs = 0;
   std::vector<int2> tiles, atiles_ss, atiles_gs;
   for (int i = 0; i < bt.B; i++) {
     const Layout& L = *bt.lays[i];
@@ -814,12 +818,12 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
     row_tiles(nrow, cap_gs, atiles_gs);
     for (const int2& t : atiles_ss) {
       wl.push_back(make_int4(i, CLH_W_SS, t.x, t.y));
-      adj_lds = std::max(adj_lds, ns * ns * t.y);
+      lds_ss = std::max(lds_ss, ns * ns * t.y);
     }
     for (int zl = 0; zl < nl; zl++)
       for (const int2& t : atiles_gs) {
-        wl.push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
-        adj_lds = std::max(adj_lds, ns * t.y);
+        wl_gs.push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
+        lds_gs = std::max(lds_gs, ns * t.y);
       }
     row_tiles(nrow, rows[3], tiles);
     for (int zl = 0; zl < nl; zl++)
@@ -847,12 +851,15 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   HIPCHK(c, up2(bt.d_wn, wn, bt.n_wn));
   HIPCHK(c, up2(bt.d_wl2, wl2, bt.n_wl2));
   HIPCHK(c, up2(bt.d_wa, wa, bt.n_wa));
+  bt.n_wl_ss = (int)wl.size();
+  wl.insert(wl.end(), wl_gs.begin(), wl_gs.end());
   HIPCHK(c, up4(bt.d_wl, wl, bt.n_wl));
   HIPCHK(c, up4(bt.d_wsx, wsx, bt.n_wsx));
   HIPCHK(c, up4(bt.d_wgg, wgg, bt.n_wgg));
   HIPCHK(c, up4(bt.d_wga, wga, bt.n_wga));
   HIPCHK(c, up4(bt.d_wmx, wmx, bt.n_wmx));
-  bt.adj_lds = adj_lds;
+  bt.adj_lds_ss = lds_ss;
+  bt.adj_lds_gs = lds_gs;
   return CLH_OK;
 
 }
@@ -2014,7 +2021,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));
   HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga, c->gg_lds[1] ? bt.gg_stage : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl, bt.adj_lds, s));
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, s));
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
   if (g_lnP) HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[9], s));

@@ -307,9 +307,13 @@ struct ClhKahan {
 template <bool F32>
 __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restrict__ D,
                                                          const int4* __restrict__ work) {
-  __shared__ double S[CLH_SUB_ROWS * CLH_NP_SS];
+  // rows of S / Sf padded by one element: the pair sums read 16 rows at one
+  // node index per wave, which an unpadded 64-element stride maps onto a
+  // single LDS bank
+  constexpr int sp = CLH_NP_SS + 1;
+  __shared__ double S[CLH_SUB_ROWS * sp];
   __shared__ double W[(CLH_MAXB + 2) * CLH_NP_SS];
-  __shared__ float Sf[CLH_SUB_ROWS * CLH_NP_SS];
+  __shared__ float Sf[CLH_SUB_ROWS * sp];
   __shared__ float Wf[CLH_MAXB * CLH_NP_SS];
   __shared__ double L10[CLH_TILE_ROWS];
   __shared__ double s_lk[CLH_LDS_LK];
@@ -345,12 +349,12 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
       if (rr < nr) {
         double dx;
         const double out = pkn.lnp(lk, L10[rr] - l10fk, dx);
-        if (f32) Sf[rl * np + p] = expf((float)out) * (float)(c3 * fac);
+        if (f32) Sf[rl * sp + p] = expf((float)out) * (float)(c3 * fac);
         else s = exp(out) * c3 * fac;
       } else if (f32) {
-        Sf[rl * np + p] = 0.0f;
+        Sf[rl * sp + p] = 0.0f;
       }
-      if (!f32) S[rl * np + p] = s;
+      if (!f32) S[rl * sp + p] = s;
     }
     __syncthreads();
     if (kind == CLH_W_SS) {
@@ -365,7 +369,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
         if (f32) {
           const float* Ai = Wf + bi * np;
           const float* Aj = Wf + bj * np;
-          const float* Sr = Sf + rl * np;
+          const float* Sr = Sf + rl * sp;
           ClhKahan k;
 #pragma unroll 8
           for (int pp = 0; pp < np; pp++) k.add(Ai[pp] * Aj[pp] * Sr[pp]);
@@ -373,7 +377,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
         } else {
           const double* Ai = W + bi * np;
           const double* Aj = W + bj * np;
-          const double* Sr = S + rl * np;
+          const double* Sr = S + rl * sp;
           acc = 0.0;
 #pragma unroll 8
           for (int pp = 0; pp < np; pp++) acc += Ai[pp] * Aj[pp] * Sr[pp];
@@ -394,7 +398,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
         const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
         const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
         const double* Bz = W + (2 + zs) * np;
-        const double* Sr = S + rl * np;
+        const double* Sr = S + rl * sp;
         double acc = 0.0;
 #pragma unroll 8
         for (int pp = 0; pp < np; pp++) acc += Bz[pp] * (W[pp] * b1 + W[np + pp] * pref1 * bmag) * Sr[pp];
