@@ -54,88 +54,129 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_set_params(const DrawDesc* __rest
 }
 
 // ----------------------------------------------------------------------------
-// k_leg_adjoint: work item (draw, first column); 8 C_ell columns per block.
-//   u[col][r] = sum_tasks sum_theta Kfold[kind][r][theta] * dvbar
-//   cbar[col][r] = calibration(col) * u[col][r]
-//   T[col] = sum_r u[col][r] C[col][r]  (= dvbar . uncalibrated data vector)
-// T goes to the draw's `raw` buffer for the shear-calibration gradient.  (The
-// FULL-mode forward has the same derivative with respect to the rows.)
+// k_leg_adjoint: cotangent of the C_ell rows from the data-vector cotangent,
+// on MFMA (v_mfma_f64_16x16x4f64).  One block of 4 waves per work item
+// (draw, kind, c0) = 16 columns of one probe (kind 0 ss, 2 gs, 3 gg):
+//   cbar[c][r] = f_c sum_theta u[c][theta] K_kind[r][theta]
+// (ss: the xi+ and xi- kernels with the xi+ and xi- cotangents, i.e. a
+// contraction over 2 Ntheta), with u the masked data-vector cotangent and f_c
+// the shear calibration.  M = columns, N = rows (16 per tile; wave w takes
+// tiles w, w + 4, ...), K = theta in steps of 4, kernels read as
+// [kind][theta][row] (coalesced along rows), every step of a tile loaded
+// before its MFMAs.
+//   T[c] = sum_r (cbar[c][r] / f_c) C[c][r]  (= dvbar . uncalibrated data
+// vector of the column), summed over the waves through LDS, goes to the
+// draw's `raw` buffer for the shear-calibration gradient, with the point-mass
+// term of gamma_t.  (The FULL-mode forward has the same derivative with
+// respect to the rows.)
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(CLH_BLOCK) void k_leg_adjoint(const DrawDesc* __restrict__ D,
-                                                           const int2* __restrict__ work) {
-  __shared__ double vb[8][2][32];
-  __shared__ double cf[8], tq[8];
-  const int2 w = work[blockIdx.x];
+__global__ __launch_bounds__(256) void k_leg_adjoint(const DrawDesc* __restrict__ D,
+                                                     const int4* __restrict__ work) {
+  __shared__ double vb[2][16][33];  // [xi+/xi-][column][theta], theta >= Ntheta zero
+  __shared__ double cf[16];
+  __shared__ double tsum[4][16];
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
+  const int kind = w.y, c0 = w.z;
   const int nt = d.ntheta, nrow = d.nrow, nss = d.n_ss, ngs = d.n_gs;
-  const int ncol = nss + ngs + d.nl;
-  for (int i = threadIdx.x; i < 8 * 2 * 32; i += CLH_BLOCK) {
-    const int c = i >> 6, s = (i >> 5) & 1, th = i & 31;
-    const int col = w.y + c;
+  const int ncolk = kind == 0 ? nss : (kind == 2 ? ngs : d.nl);
+  const int colbase = kind == 0 ? 0 : (kind == 2 ? nss : nss + ngs);
+  const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6, lm = l & 15, lk = l >> 4;
+  for (int i = tid; i < 2 * 16 * 33; i += 256) {
+    const int sk = i / (16 * 33), cc = (i / 33) % 16, th = i % 33;
+    const int q = c0 + cc;
     double v = 0.0;
-    if (col < ncol && th < nt) {
-      int canon = -1;
-      if (col < nss) {
-        canon = nt * col + th + (s ? nt * nss : 0);
-        v = d.dvbar[d.dv_start[0] + canon];
-      } else if (s == 0 && col < nss + ngs) {
-        canon = 2 * nt * nss + nt * (col - nss) + th;
-        v = d.dvbar[d.dv_start[1] + nt * (col - nss) + th];
-      } else if (s == 0) {
-        canon = nt * (2 * nss + ngs) + nt * (col - nss - ngs) + th;
-        v = d.dvbar[d.dv_start[2] + nt * (col - nss - ngs) + th];
+    if (q < ncolk && th < nt && (sk == 0 || kind == 0)) {
+      int canon, idx;
+      if (kind == 0) {
+        canon = nt * q + th + (sk ? nt * nss : 0);
+        idx = d.dv_start[0] + canon;
+      } else if (kind == 2) {
+        canon = 2 * nt * nss + nt * q + th;
+        idx = d.dv_start[1] + nt * q + th;
+      } else {
+        canon = nt * (2 * nss + ngs) + nt * q + th;
+        idx = d.dv_start[2] + nt * q + th;
       }
-      if (canon >= 0 && clh_masked(d, canon)) v = 0.0;  // masked entries are constant 0
+      v = clh_masked(d, canon) ? 0.0 : d.dvbar[idx];  // masked entries are constant 0
     }
-    vb[c][s][th] = v;
+    vb[sk][cc][th] = v;
   }
-  if (threadIdx.x < 8) {
-    const int col = w.y + threadIdx.x;
+  if (tid < 16) {
+    const int q = c0 + tid;
     double f = 1.0;
-    if (d.apply_calib && col < nss) {
+    if (d.apply_calib && q < ncolk && kind == 0) {
       int a, b;
-      clh_ss_pair(d.ns, col, a, b);
+      clh_ss_pair(d.ns, q, a, b);
       f = (1.0 + d.par->shear_m[a]) * (1.0 + d.par->shear_m[b]);
-    } else if (d.apply_calib && col < nss + ngs) {
-      f = 1.0 + d.par->shear_m[(col - nss) % d.ns];
+    } else if (d.apply_calib && q < ncolk && kind == 2) {
+      f = 1.0 + d.par->shear_m[q % d.ns];
     }
-    cf[threadIdx.x] = f;
-    tq[threadIdx.x] = 0.0;
+    cf[tid] = f;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 8 * nrow; i += CLH_BLOCK) {
-    const int c = i / nrow, r = i % nrow;
-    const int col = w.y + c;
-    if (col >= ncol) continue;
-    const int kind0 = col < nss ? 0 : (col < nss + ngs ? 2 : 3);
-    const double* K0 = d.kfold + ((size_t)kind0 * nrow + r) * nt;
-    double acc = 0.0;
-    for (int th = 0; th < nt; th++) acc += K0[th] * vb[c][0][th];
-    if (col < nss) {
-      const double* K1 = d.kfold + ((size_t)nrow + r) * nt;
-      for (int th = 0; th < nt; th++) acc += K1[th] * vb[c][1][th];
-    }
-    d.cbar[(size_t)col * nrow + r] = acc * cf[c];
-    if (col < nss + ngs) atomicAdd(&tq[c], acc * d.cl[(size_t)col * nrow + r]);
+  // the tile's K steps: up to 8 per kernel (Ntheta <= 32), two kernels for ss
+  const int nst = (nt + 3) >> 2;
+  const double* __restrict__ KT0 = d.kfoldT + (size_t)(kind == 0 ? 0 : kind) * nt * nrow;
+  const double* __restrict__ KT1 = d.kfoldT + (size_t)1 * nt * nrow;
+  double av[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int sk = u >> 3, th = 4 * (u & 7) + lk;
+    av[u] = (th < 33) ? vb[sk][lm][th < 33 ? th : 0] : 0.0;
   }
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int r0 = 16 * wv; r0 < nrow; r0 += 64) {
+    const int r = r0 + lm;
+    const bool rin = r < nrow;
+    double bv[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int sk = u >> 3, st = u & 7, th = 4 * st + lk;
+      const bool use = st < nst && (sk == 0 || kind == 0);
+      bv[u] = (use && rin && th < nt) ? (sk ? KT1 : KT0)[(size_t)th * nrow + r] : 0.0;
+    }
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int sk = u >> 3, st = u & 7;
+      if (st < nst && (sk == 0 || kind == 0)) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+    }
+    if (!rin) continue;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int q = c0 + 4 * v + lk;
+      if (q >= ncolk) continue;
+      const size_t o = (size_t)(colbase + q) * nrow + r;
+      d.cbar[o] = acc[v] * cf[4 * v + lk];
+      if (kind != 3) t[v] += acc[v] * d.cl[o];
+    }
+  }
+  // T per column: over the 16 lanes of each column group, then the 4 waves
+#pragma unroll
+  for (int v = 0; v < 4; v++)
+    for (int off = 8; off > 0; off >>= 1) t[v] += __shfl_xor(t[v], off, 16);
+  if (lm == 0)
+    for (int v = 0; v < 4; v++) tsum[wv][4 * v + lk] = t[v];
   __syncthreads();
-  if (threadIdx.x < 8 && w.y + threadIdx.x < ncol) {
-    const int col = w.y + threadIdx.x;
-    double t = tq[threadIdx.x];
-    if (d.apply_calib && col >= nss && col < nss + ngs) {
-      // point mass of gamma_t: (raw + pm / theta^2)(1 + m_zs) -> m and source-shift terms
-      const int q = col - nss, zl = q / d.ns, zs = q % d.ns;
-      const double pmf = clh_pm_fac(d, zl, zs);
-      if (pmf != 0.0) {
-        double sv = 0.0;
-        for (int th = 0; th < nt; th++) sv += vb[threadIdx.x][0][th] / (d.theta[th] * d.theta[th]);
-        t += pmf * sv;
-        const double dpm = clh_pm_fac<true>(d, zl, zs) * sv * cf[threadIdx.x];
-        if (dpm != 0.0) atomicAdd(d.g_src + zs, dpm);
-      }
+  if (tid >= 16) return;
+  const int c = tid, q = c0 + c;
+  if (q >= ncolk) return;
+  double tt = tsum[0][c] + tsum[1][c] + tsum[2][c] + tsum[3][c];
+  if (d.apply_calib && kind == 2) {
+    // point mass of gamma_t: (raw + pm / theta^2)(1 + m_zs) -> m and source-shift terms
+    const int zl = q / d.ns, zs = q % d.ns;
+    const double pmf = clh_pm_fac(d, zl, zs);
+    if (pmf != 0.0) {
+      double sv = 0.0;
+      for (int th = 0; th < nt; th++) sv += vb[0][c][th] / (d.theta[th] * d.theta[th]);
+      tt += pmf * sv;
+      const double dpm = clh_pm_fac<true>(d, zl, zs) * sv * cf[c];
+      if (dpm != 0.0) atomicAdd(d.g_src + zs, dpm);
     }
-    d.raw[col] = t;
   }
+  d.raw[colbase + q] = tt;
 }
 
 // ----------------------------------------------------------------------------
@@ -534,9 +575,9 @@ hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nk
   hipLaunchKernelGGL(k_set_params, dim3(B), dim3(CLH_BLOCK), 0, s, D, P, nknz, lnP, src, lens, b1, iaA, m);
   return hipGetLastError();
 }
-hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int2* work, int n, hipStream_t s) {
+hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_leg_adjoint, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  hipLaunchKernelGGL(k_leg_adjoint, dim3(n), dim3(256), 0, s, D, work);
   return hipGetLastError();
 }
 hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s) {

@@ -25,7 +25,7 @@ hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStrea
 hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, hipStream_t s);
 hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles,
                                 hipStream_t s);
-hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow, hipStream_t s);
+hipError_t clh_launch_legendre(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_fill(double* p, size_t n, double v, hipStream_t s);
 hipError_t clh_launch_chi2_resid(int B, int n, const double* dv, const double* data, const double* mask,
@@ -35,7 +35,7 @@ hipError_t clh_launch_chi2_reduce(int B, int n, const double* R, const double* Y
 hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nknz, const double* lnP,
                                  const double* src, const double* lens, const double* b1,
                                  const double* iaA, const double* m, hipStream_t s);
-hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
 hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles,
                                    hipStream_t s);
@@ -198,7 +198,7 @@ struct DevLayout {  // device copy of one survey layout
   int ngrp = 0;
   DBuf<int> mask;       // likelihood mask (null when the layout has none)
   size_t off_s_zc = 0, off_s_coef = 0, off_l_zc = 0, off_l_coef = 0, off_ell_ss = 0,
-         off_ell_gx = 0, off_kfold = 0, off_gl = 0, off_kfull = 0, off_fill = 0, off_theta = 0;
+         off_ell_gx = 0, off_kfold = 0, off_kfoldT = 0, off_gl = 0, off_kfull = 0, off_fill = 0, off_theta = 0;
   void release() { buf.release(); k32.release(); ksc.release(); kbf.release(); mask.release(); valid = false; }
 };
 
@@ -224,10 +224,11 @@ struct Batch {
   DBuf<DrawDesc> d_desc;
   DBuf<DrawParams> d_par;
   DBuf<double> d_tab, d_work, d_dv, d_ells, d_vjp, d_gscr, d_run;
-  DBuf<int2> d_wg, d_wn, d_wl2, d_wa, d_wlg;
+  DBuf<int2> d_wg, d_wn, d_wl2, d_wlg;
+  DBuf<int4> d_wa4;
   int n_wlg = 0, lnp_ic = 0;  // ln P gather: work items (draw, first k row), k rows per item
-  DBuf<int4> d_wl, d_wsx, d_wgg, d_wga, d_wmx;
-  int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0, n_wsx = 0, n_wgg = 0, n_wga = 0, n_wmx = 0;
+  DBuf<int4> d_wl, d_wsx, d_wgg, d_wga, d_wmx, d_wlf;
+  int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0, n_wsx = 0, n_wgg = 0, n_wga = 0, n_wmx = 0, n_wlf = 0;
   int nrow_custom = 0;  // rows of a custom-ell batch (0: the layouts' rows)
   int gg_stage = 0;     // LDS doubles for the gg kernels' staged tables (0: not staged)
   // the device state (lens-efficiency tables, node weights, ss/gs C_ell rows)
@@ -240,8 +241,8 @@ struct Batch {
   void release() {
     d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
     d_vjp.release(); d_gscr.release(); d_run.release(); d_wlg.release();
-    d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa.release();
-    d_wsx.release(); d_wgg.release(); d_wga.release(); d_wmx.release();
+    d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa4.release();
+    d_wsx.release(); d_wgg.release(); d_wga.release(); d_wmx.release(); d_wlf.release();
   }
 };
 
@@ -627,6 +628,7 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   D.off_ell_ss = take(L.ell_ss.size());
   D.off_ell_gx = take(L.ell_gx.size());
   D.off_kfold = take(L.kfold.size());
+  D.off_kfoldT = take(L.kfold.size());
   D.off_gl = take(sizeof glbuf / sizeof(double));
   D.off_theta = take(L.theta.size());
   D.off_kfull = take(full ? L.kfull.size() : 0);
@@ -663,6 +665,14 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   HIPCHK(c, put(D.off_ell_ss, L.ell_ss.data(), L.ell_ss.size()));
   HIPCHK(c, put(D.off_ell_gx, L.ell_gx.data(), L.ell_gx.size()));
   HIPCHK(c, put(D.off_kfold, L.kfold.data(), L.kfold.size()));
+  dvec kfoldT(L.kfold.size());  // [4][ntheta][nrow] for the adjoint contraction
+  {
+    const size_t nr = L.kfold.size() / (4 * (size_t)L.ntheta), nt = L.ntheta;
+    for (size_t k = 0; k < 4; k++)
+      for (size_t r = 0; r < nr; r++)
+        for (size_t t = 0; t < nt; t++) kfoldT[(k * nt + t) * nr + r] = L.kfold[(k * nr + r) * nt + t];
+  }
+  HIPCHK(c, put(D.off_kfoldT, kfoldT.data(), kfoldT.size()));
   HIPCHK(c, put(D.off_gl, glbuf, sizeof glbuf / sizeof(double)));
   HIPCHK(c, put(D.off_theta, L.theta.data(), L.theta.size()));
   if (!L.mask.empty()) {
@@ -791,15 +801,15 @@ static bool ord_ok(const Layout& L, const int ord[3]) {
 // the whole batch).  rows: rows per tile of the forward ss/gs, forward gg,
 // adjoint ss/gs and adjoint gg Limber stages (1..CLH_TILE_ROWS).
 static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
-  std::vector<int2> wg, wn, wl2, wa;
-  std::vector<int4> wl, wl_gs, wsx, wgg, wga, wmx;
+  std::vector<int2> wg, wn, wl2;
+  std::vector<int4> wl, wl_gs, wsx, wgg, wga, wmx, wlf, wa;
   int lds_ss = 0, lds_gs = 0;
   std::vector<int2> tiles, atiles_ss, atiles_gs;
   for (int i = 0; i < bt.B; i++) {
     const Layout& L = *bt.lays[i];
     const int ns = L.src.nbin, nl = L.lens.nbin;
     const int nrow = bt.nrow_custom ? bt.nrow_custom : L.nrow();
-    const int n_ss = L.n_ss(), n_gs = L.n_gs(), ncol = n_ss + n_gs + nl;
+    const int n_ss = L.n_ss(), n_gs = L.n_gs();
     for (int b = 0; b < ns + nl; b++) wg.push_back(make_int2(i, b));
     for (int sg = 0; sg <= 2 * nl; sg++) wn.push_back(make_int2(i, sg));
     // forward Limber: ss, gs and gg tiles (draw, kind + 4 zl, row0, rows)
@@ -830,7 +840,13 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
       for (const int2& t : tiles) wga.push_back(make_int4(i, zl, t.x, t.y));
     const int ntask = 2 * n_ss + n_gs + nl;
     for (int t = 0; t < ntask; t += 8) wl2.push_back(make_int2(i, t));
-    for (int col = 0; col < ncol; col += 8) wa.push_back(make_int2(i, col));
+    // FP64 Legendre on MFMA: 16 columns of one probe kind per item, forward
+    // (xi+, xi-, gamma_t, w kernels) and adjoint (ss with both xi kernels, gs, gg)
+    const int nkc[4] = {n_ss, n_ss, n_gs, nl};
+    for (int k = 0; k < 4; k++)
+      for (int c0 = 0; c0 < nkc[k]; c0 += 16) wlf.push_back(make_int4(i, k, c0, 0));
+    for (int k : {0, 2, 3})
+      for (int c0 = 0; c0 < nkc[k]; c0 += 16) wa.push_back(make_int4(i, k, c0, 0));
     const int nck[4] = {n_ss, n_ss, n_gs, nl};
     for (int k = 0; k < 4; k++)
       for (int c0 = 0; c0 < nck[k]; c0 += 32) wmx.push_back(make_int4(i, k, c0, 0));
@@ -850,7 +866,7 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   HIPCHK(c, up2(bt.d_wg, wg, bt.n_wg));
   HIPCHK(c, up2(bt.d_wn, wn, bt.n_wn));
   HIPCHK(c, up2(bt.d_wl2, wl2, bt.n_wl2));
-  HIPCHK(c, up2(bt.d_wa, wa, bt.n_wa));
+
   bt.n_wl_ss = (int)wl.size();
   wl.insert(wl.end(), wl_gs.begin(), wl_gs.end());
   HIPCHK(c, up4(bt.d_wl, wl, bt.n_wl));
@@ -858,6 +874,8 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   HIPCHK(c, up4(bt.d_wgg, wgg, bt.n_wgg));
   HIPCHK(c, up4(bt.d_wga, wga, bt.n_wga));
   HIPCHK(c, up4(bt.d_wmx, wmx, bt.n_wmx));
+  HIPCHK(c, up4(bt.d_wlf, wlf, bt.n_wlf));
+  HIPCHK(c, up4(bt.d_wa4, wa, bt.n_wa));
   bt.adj_lds_ss = lds_ss;
   bt.adj_lds_gs = lds_gs;
   return CLH_OK;
@@ -946,6 +964,7 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.ell_ss = ells ? bt.d_ells.p : lay + DL.off_ell_ss;
     q.ell_gx = ells ? bt.d_ells.p : lay + DL.off_ell_gx;
     q.kfold = lay + DL.off_kfold;
+    q.kfoldT = lay + DL.off_kfoldT;
     q.kfull = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + DL.off_kfull : nullptr;
     q.fill = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + DL.off_fill : nullptr;
     q.gl = lay + DL.off_gl;
@@ -1041,7 +1060,7 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
     else if (mixed)
       HIPCHK(c, clh_launch_legendre_mx(bt.d_desc.p, bt.d_wmx.p, bt.n_wmx, c->precision, s));
     else
-      HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s));
+      HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wlf.p, bt.n_wlf, s));
   }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[5], s));
   bt.state_valid = !mixed;
@@ -2016,7 +2035,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, 0, s));
   }
   // reverse sweep
-  HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa.p, bt.n_wa, s));
+  HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa4.p, bt.n_wa, s));
   HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));
   HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga, c->gg_lds[1] ? bt.gg_stage : 0, s));

@@ -67,6 +67,7 @@ struct DrawDesc {
   const double* ell_ss;   // [nrow] ell of every ss row
   const double* ell_gx;   // [nrow] ell of every gs/gg row
   const double* kfold;    // [4][nrow][ntheta] (xi+, xi-, gamma_t, w), row-major in r
+  const double* kfoldT;   // the same kernels as [4][ntheta][nrow]
   const double* kfull;    // [4][lmax][ntheta] full-length kernels (FULL mode) or null
   const double* fill;     // [2][lmax][2] (index, weight) of the ln-ell fill, ss then gs/gg
   double lim_ss[3], lim_gx[3];

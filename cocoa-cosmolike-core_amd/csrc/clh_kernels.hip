@@ -495,53 +495,79 @@ __device__ __forceinline__ void clh_place(const DrawDesc& d, int kind, int q, in
 }
 
 // ----------------------------------------------------------------------------
-// k_legendre (folded kernels): work item (draw, first task); 8 tasks per block,
-// 32 theta slots per task.  Task t of a draw: [0, n_ss) xi+, [n_ss, 2 n_ss)
-// xi-, then gamma_t pairs, then w bins.
+// k_legendre (FP64, folded kernels) on MFMA (v_mfma_f64_16x16x4f64): one block
+// of 4 waves per work item (draw, kind, c0) = 16 C_ell columns of one probe
+// kind (kind 0 xi+, 1 xi-, 2 gamma_t, 3 w) against every theta bin:
+//   out[c][theta] = sum_r C[c][r] K_kind[r][theta]
+// M = columns, N = theta (two 16-wide tiles when Ntheta > 16), K = rows in
+// steps of 4; wave w takes the steps w, w + 4, ... and the four partial tiles
+// are summed through LDS.  Operands come straight from the C_ell rows and the
+// layout's kernel rows, eight steps loaded ahead of their MFMAs.  Lane layout:
+// A[m][k] on lane m + 16 k, B[k][n] on lane n + 16 k, D[4 v + l / 16][l % 16]
+// in value v of lane l.
 // ----------------------------------------------------------------------------
-#define CLH_LEG_TASKS 8
-__global__ __launch_bounds__(CLH_BLOCK) void k_legendre(const DrawDesc* __restrict__ D,
-                                                        const int2* __restrict__ work) {
-  extern __shared__ double colbuf[];  // [CLH_LEG_TASKS][nrow]
-  const int2 w = work[blockIdx.x];
+typedef double clh_v4d __attribute__((ext_vector_type(4)));
+#define CLH_LEG_UNROLL 8
+__global__ __launch_bounds__(256) void k_legendre(const DrawDesc* __restrict__ D,
+                                                  const int4* __restrict__ work) {
+  __shared__ clh_v4d part[3][2][64];
+  const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
+  const int kind = w.y, c0 = w.z;
   const int nt = d.ntheta, nrow = d.nrow;
-  const int ntask = 2 * d.n_ss + d.n_gs + d.nl;
-  const int tl = threadIdx.x >> 5, th = threadIdx.x & 31;
-  const int task = w.y + tl;
-
-  for (int i = threadIdx.x; i < CLH_LEG_TASKS * nrow; i += CLH_BLOCK) {
-    const int tt = w.y + i / nrow;
-    double v = 0.0;
-    if (tt < ntask) {
-      int col;
-      if (tt < d.n_ss) col = tt;
-      else if (tt < 2 * d.n_ss) col = tt - d.n_ss;
-      else col = tt - d.n_ss;  // gs and gg columns follow the ss columns
-      v = d.cl[(size_t)col * nrow + (i % nrow)];
+  const int ncolk = kind <= 1 ? d.n_ss : (kind == 2 ? d.n_gs : d.nl);
+  const int colbase = kind <= 1 ? 0 : (kind == 2 ? d.n_ss : d.n_ss + d.n_gs);
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, lm = l & 15, lk = l >> 4;
+  const double* __restrict__ K = d.kfold + (size_t)kind * nrow * nt;
+  const int q = c0 + lm;
+  const double* __restrict__ C = d.cl + (size_t)(colbase + (q < ncolk ? q : 0)) * nrow;
+  const bool qin = q < ncolk, t0in = lm < nt, t1in = 16 + lm < nt, two = nt > 16;
+  clh_v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  const int nstep = (nrow + 3) >> 2;
+  for (int s0 = wv; s0 < nstep; s0 += 4 * CLH_LEG_UNROLL) {
+    double a[CLH_LEG_UNROLL], b0[CLH_LEG_UNROLL], b1[CLH_LEG_UNROLL];
+#pragma unroll
+    for (int u = 0; u < CLH_LEG_UNROLL; u++) {
+      const int r = 4 * (s0 + 4 * u) + lk;
+      const bool rin = r < nrow;
+      a[u] = (rin && qin) ? C[r] : 0.0;
+      b0[u] = (rin && t0in) ? K[(size_t)r * nt + lm] : 0.0;
+      b1[u] = (rin && t1in) ? K[(size_t)r * nt + 16 + lm] : 0.0;
     }
-    colbuf[i] = v;
+#pragma unroll
+    for (int u = 0; u < CLH_LEG_UNROLL; u++) {
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b0[u], acc0, 0, 0, 0);
+      if (two) acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b1[u], acc1, 0, 0, 0);
+    }
+  }
+  if (wv > 0) {
+    part[wv - 1][0][l] = acc0;
+    part[wv - 1][1][l] = acc1;
   }
   __syncthreads();
-  if (task >= ntask || th >= nt) return;
-  int kind, q;
-  if (task < d.n_ss) { kind = 0; q = task; }
-  else if (task < 2 * d.n_ss) { kind = 1; q = task - d.n_ss; }
-  else if (task < 2 * d.n_ss + d.n_gs) { kind = 2; q = task - 2 * d.n_ss; }
-  else { kind = 3; q = task - 2 * d.n_ss - d.n_gs; }
-  const double* K = d.kfold + (size_t)kind * nrow * nt + th;
-  const double* c = colbuf + tl * nrow;
-  double acc = 0.0;
-  for (int r = 0; r < nrow; r++) acc += K[(size_t)r * nt] * c[r];
-
-  clh_place(d, kind, q, th, acc);
+  if (wv > 0) return;
+  for (int k = 0; k < 3; k++) {
+    acc0 += part[k][0][l];
+    acc1 += part[k][1][l];
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int qq = c0 + 4 * v + lk;
+    if (qq >= ncolk) continue;
+    if (lm < nt) clh_place(d, kind, qq, lm, acc0[v]);
+    if (two && 16 + lm < nt) clh_place(d, kind, qq, 16 + lm, acc1[v]);
+  }
 }
+
+#define CLH_LEG_TASKS 8  // k_legendre_full: tasks per block, 32 theta slots each
 
 // ----------------------------------------------------------------------------
 // k_legendre_full: the reference's interpolate-then-dot structure
 // (cosmo2D.c:468-533): C_ell at every integer ell < LMAX is generated on the
 // fly from the column's rows (ell < LMIN_tab direct, else the ln-ell fill) and
-// contracted with the full-length kernel.  Same task mapping as k_legendre.
+// contracted with the full-length kernel.  Work item (draw, first task);
+// CLH_LEG_TASKS tasks per block, 32 theta slots per task.  Task t of a draw:
+// [0, n_ss) xi+, [n_ss, 2 n_ss) xi-, then gamma_t pairs, then w bins.
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(CLH_BLOCK) void k_legendre_full(const DrawDesc* __restrict__ D,
                                                              const int2* __restrict__ work) {
@@ -597,11 +623,9 @@ hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStrea
   hipLaunchKernelGGL(k_nodes, dim3(n), dim3(128), 0, s, D, work);
   return hipGetLastError();
 }
-hipError_t clh_launch_legendre(const DrawDesc* D, const int2* work, int n, int max_nrow,
-                               hipStream_t s) {
+hipError_t clh_launch_legendre(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const size_t lds = sizeof(double) * CLH_LEG_TASKS * (size_t)max_nrow;
-  hipLaunchKernelGGL(k_legendre, dim3(n), dim3(CLH_BLOCK), lds, s, D, work);
+  hipLaunchKernelGGL(k_legendre, dim3(n), dim3(256), 0, s, D, work);
   return hipGetLastError();
 }
 hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, int max_nrow,
