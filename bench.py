@@ -205,8 +205,9 @@ def main():
 
     # forward + VJP: one autograd step per draw (forward, then the VJP of the
     # whole data vector against a cotangent), timed as pairs; the VJP rate is
-    # the pair time minus the forward time.  After an FP64 forward the VJP
-    # reuses its device state; after a mixed-precision forward it recomputes it.
+    # the pair time minus the forward time.  As under torch.autograd, the
+    # forward keeps the gg integrand checkpoint and the VJP reuses the
+    # forward's device state (node weights, lens efficiencies, C_ell rows).
     vjp_value = None
     pair_value = None
     vjp_ms = {}
@@ -218,6 +219,7 @@ def main():
         g = {k: torch.empty(B * m, dtype=torch.float64, device="cuda:0")
              for k, m in (("lnP", nknz), ("src", ns), ("lens", nl), ("b1", nl), ("A", 1), ("m", ns))}
         ptrs = [g[k].data_ptr() for k in ("lnP", "src", "lens", "b1", "A", "m")]
+        c.set_vjp_checkpoint(True)
         for _ in range(2):
             c.batch_forward(dv.data_ptr())
             c.batch_vjp(dvbar.data_ptr(), *ptrs)
@@ -241,6 +243,7 @@ def main():
                 if k.endswith("_adj") or k == "k_lnp_gather":
                     vjp_ms[k] = vjp_ms.get(k, 0.0) + v / 3
         c.enable_timing(False)
+        c.set_vjp_checkpoint(False)
         log(f"bench: VJP Limber kernel ms {vjp_ms}")
     work = algorithmic_work(draws[0])
     dom = max(acc, key=acc.get)
@@ -336,8 +339,8 @@ def main():
                    "datavector_len": n},
         "accuracy_vs_fp64_path": accuracy,
         "vjp_value": vjp_value,
-        "vjp_unit": "VJPs/s (full data vector -> lnP, dz_src, dz_lens, b1, A_IA, m; FP64 adjoint, "
-                    "timed as forward+VJP pairs minus the forward)",
+        "vjp_unit": "VJPs/s (full data vector -> lnP, dz_src, dz_lens, b1, A_IA, m; FP64 adjoint kernels "
+                    "reading the forward's gg checkpoint, timed as forward+VJP pairs minus the forward)",
         "fwd_plus_vjp_value": pair_value,
         "plain_fp64_single_draw": plain,
         "ragged_batch": ragged,

@@ -75,6 +75,7 @@ def lib():
             "clh_set_nuisance_magnification_bias": (i32, [VP, D, i32]),
             "clh_set_nuisance_ia": (i32, [VP, D, i32]),
             "clh_set_nuisance_pointmass": (i32, [VP, D, i32]),
+            "clh_set_vjp_checkpoint": (i32, [VP, i32]),
             "clh_init_binning_fourier": (i32, [VP, i32, i32, i32, i32]),
             "clh_datavector_size_fourier": (i32, [VP]),
             "clh_compute_3x2pt_fourier": (i32, [VP, I, I, D, i32]),

@@ -33,6 +33,8 @@ class Projection3x2pt(torch.autograd.Function):
         out = torch.empty(B, n, dtype=torch.float64, device=dev)
         lib_ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
         lib_ctx.batch_set_params(*[_ptr(t) for t in ins])
+        # keep the gg integrand for the backward pass when one will follow
+        lib_ctx.set_vjp_checkpoint(any(ctx.needs_input_grad[1:7]))
         lib_ctx.batch_forward(out.data_ptr(), ord)
         ctx.lib_ctx = lib_ctx
         ctx.ord = ord

@@ -393,6 +393,10 @@ class Cosmolike:
         self._chk(self.L.clh_get_tile_rows(self.h, rows))
         return list(rows)
 
+    def set_vjp_checkpoint(self, on=True):
+        """Batch forwards keep the gg integrand for a following VJP."""
+        self._chk(self.L.clh_set_vjp_checkpoint(self.h, 1 if on else 0))
+
     def enable_timing(self, on=True):
         self._chk(self.L.clh_enable_timing(self.h, 1 if on else 0))
 

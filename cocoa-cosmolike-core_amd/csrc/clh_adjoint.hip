@@ -262,19 +262,26 @@ struct ClhLnPRun {
 //      with f folded in, so the b, j loops address it linearly);
 //  gs: bbar_zs += pref2 cbar_zs g S, W_gal / W_mag / b1 cotangents and
 //      pref2 S g sum_zs cbar_zs B_zs for ln P, with g = W_gal b1 + W_mag pref1 bmag.
-// LDS: ss [ns][ns][nrows], gs [ns][nrows] (dynamic).
+// LDS: ss [ns][ns][nrows], gs [ns][nrows] (dynamic).  With the forward's
+// checkpoint (d.sxk) the P(k,z) lookup is replaced by its stored S = P fac
+// and only the k cell is walked again.
 // ----------------------------------------------------------------------------
+template <int NS>
 __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict__ D,
                                                       const int4* __restrict__ work) {
+  // NS: the source-bin count when it is at most 10 (loops fully unrolled to
+  // it), else CLH_MAXB with the count checked at run time
+  constexpr bool EXACT = NS < CLH_MAXB;
   extern __shared__ double Cb[];  // ss: [ns][ns][T], gs: [ns][T]
   __shared__ double L10[CLH_TILE_ROWS];
+  __shared__ double PR[3][CLH_TILE_ROWS];  // per-row prefactors (ss: pref0; gs: pref1, pref2)
   __shared__ double s_lk[CLH_LDS_LK];
   const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
   const int kind = w.y & 3, zl = w.y >> 2, row0 = w.z, T = w.w;
   const int p = threadIdx.x;
   constexpr int np = CLH_NP_SS;
-  const int nrow = d.nrow, ns = d.ns;
+  const int nrow = d.nrow, ns = EXACT ? NS : d.ns;
   const bool ss = kind == CLH_W_SS;
   const double* o = clh_sx_nodes(d, kind, zl);
   const double* lut = o + clh_sx_lut_col(d, kind) * np;
@@ -288,19 +295,27 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
     const int col0 = d.n_ss + zl * ns;
     for (int i = p; i < ns * T; i += 64) Cb[i] = d.cbar[(size_t)(col0 + i / T) * nrow + row0 + i % T];
   }
-  for (int t = p; t < T; t += 64) L10[t] = log10(ells[row0 + t] + 0.5);
+  for (int t = p; t < T; t += 64) {
+    const double l = ells[row0 + t], ell = l + 0.5;
+    L10[t] = log10(ell);
+    if (ss) {
+      PR[0][t] = l * (l - 1.) * (l + 1.) * (l + 2.) / (ell * ell * ell * ell);
+    } else {
+      const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+      PR[1][t] = l * (l + 1.) / (ell * ell);
+      PR[2][t] = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+    }
+  }
   const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
   __syncthreads();
   const double fac = o[2 * np + p];
   const int j = (int)lut[p];
   const double dy = lut[np + p], l10fk = lut[2 * np + p];
-  ClhPkNode pkn;
-  pkn.init(d, j, dy);
   const int woff = ss ? 3 : 5;
-  double Wn[CLH_MAXB], acc[CLH_MAXB];
+  double Wn[NS], acc[NS];
 #pragma unroll
-  for (int b = 0; b < CLH_MAXB; b++) {
-    Wn[b] = (b < ns) ? o[(woff + b) * np + p] : 0.0;
+  for (int b = 0; b < NS; b++) {
+    Wn[b] = (EXACT || b < ns) ? o[(woff + b) * np + p] : 0.0;
     acc[b] = 0.0;
   }
   const double WG = ss ? 0.0 : o[3 * np + p], WM = ss ? 0.0 : o[4 * np + p];
@@ -308,40 +323,34 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
   double twg = 0.0, twm = 0.0;
   ClhLnPRun sc;
   sc.R = clh_run_row(d, kind, zl, p);
-  for (int rr = 0; rr < T; rr++) {
-    const double l = ells[row0 + rr], ell = l + 0.5;
-    double dx;
-    const double out = pkn.lnp(lk, L10[rr] - l10fk, dx);
-    const int i = pkn.i;
-    const double S = exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac;
+  // per row: S = P fac of the node (from the forward's checkpoint, or the
+  // P(k,z) lookup again), its k cell (i, dx), then the cotangent terms
+  auto row = [&](int rr, double S, int i, double dx) {
     double g;
     if (ss) {
-      const double pref0 = l * (l - 1.) * (l + 1.) * (l + 2.) / (ell * ell * ell * ell);
-      const double c = pref0 * S;
+      const double c = PR[0][rr] * S;
       double sb = 0.0;
       const double* cr = Cb + rr;
 #pragma unroll
-      for (int b = 0; b < CLH_MAXB; b++) {
-        if (b < ns) {
+      for (int b = 0; b < NS; b++) {
+        if (EXACT || b < ns) {
           double u = 0.0;
 #pragma unroll
-          for (int jj = 0; jj < CLH_MAXB; jj++)
-            if (jj < ns) u += cr[(b * ns + jj) * T] * Wn[jj];
+          for (int jj = 0; jj < NS; jj++)
+            if (EXACT || jj < ns) u += cr[(b * ns + jj) * T] * Wn[jj];
           acc[b] += c * u;
           sb += Wn[b] * u;
         }
       }
       g = 0.5 * sb * c;
     } else {
-      const double pref1 = l * (l + 1.) / (ell * ell);
-      const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
-      const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+      const double pref1 = PR[1][rr], pref2 = PR[2][rr];
       const double gp = WG * b1 + WM * pref1 * bmag;
       const double cg = pref2 * gp * S;
       double sv = 0.0;
 #pragma unroll
-      for (int b = 0; b < CLH_MAXB; b++) {
-        if (b < ns) {
+      for (int b = 0; b < NS; b++) {
+        if (EXACT || b < ns) {
           const double cb = Cb[b * T + rr];
           sv += cb * Wn[b];
           acc[b] += cg * cb;
@@ -353,16 +362,41 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
       g = t * gp;
     }
     if (sc.R) sc.add(i, dx, g);
+  };
+  if (d.sxk) {
+    // the forward's checkpoint of S; eight rows loaded ahead of their use
+    const double* sk = d.sxk + ((size_t)(ss ? 0 : 1 + zl) * nrow + row0) * np + p;
+    ClhKCell kc;
+    kc.nk = d.nk;
+    for (int r8 = 0; r8 < T; r8 += 8) {
+      double sv8[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) sv8[u] = sk[(size_t)(r8 + u < T ? r8 + u : T - 1) * np];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        if (r8 + u >= T) break;
+        const double dx = kc.dx(lk, L10[r8 + u] - l10fk);
+        row(r8 + u, sv8[u], kc.i, dx);
+      }
+    }
+  } else {
+    ClhPkNode pkn;
+    pkn.init(d, j, dy);
+    for (int rr = 0; rr < T; rr++) {
+      double dx;
+      const double out = pkn.lnp(lk, L10[rr] - l10fk, dx);
+      row(rr, exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac, pkn.i, dx);
+    }
   }
   if (sc.R) sc.done();
   if (ss) {
 #pragma unroll
-    for (int b = 0; b < CLH_MAXB; b++)
-      if (b < ns && acc[b] != 0.0) atomicAdd(d.abar + (size_t)b * np + p, acc[b]);
+    for (int b = 0; b < NS; b++)
+      if ((EXACT || b < ns) && acc[b] != 0.0) atomicAdd(d.abar + (size_t)b * np + p, acc[b]);
   } else {
 #pragma unroll
-    for (int b = 0; b < CLH_MAXB; b++)
-      if (b < ns && acc[b] != 0.0) atomicAdd(d.bbar + ((size_t)zl * ns + b) * np + p, acc[b]);
+    for (int b = 0; b < NS; b++)
+      if ((EXACT || b < ns) && acc[b] != 0.0) atomicAdd(d.bbar + ((size_t)zl * ns + b) * np + p, acc[b]);
     atomicAdd(d.wbar + ((size_t)zl * 2 + 0) * np + p, twg * b1);
     atomicAdd(d.wbar + ((size_t)zl * 2 + 1) * np + p, twm * bmag);
     const double b1p = clh_wave_sum(twg * WG);
@@ -396,27 +430,66 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
   const ClhGgTabs T = clh_gg_tabs(d, zl, dyn, stage, s_lk);
   __syncthreads();
   const int p = threadIdx.x % np, half = threadIdx.x / np;
-  ClhGgNode<true> nd;
-  nd.init(d, T, clh_gg_nodes(d, zl), p, zl);
   double tg = 0.0, tm = 0.0, b1p = 0.0, lzp = 0.0;
   ClhLnPRun sc;
   sc.R = clh_run_row(d, CLH_W_GG, zl, p);
   const int hn = (nr + 1) / 2;
   const int r_lo = half * hn, r_hi = half ? nr : hn;
-  for (int rr = r_lo; rr < r_hi; rr++) {
-    const double cb = Cr[rr];
-    if (cb == 0.0) continue;
-    double t, pf, dwr, dx;
-    if (!nd.eval(d, T, R[rr], t, pf, dwr, dx)) continue;
-    if (sc.R) sc.add(nd.pk.i, dx, cb * t * t * pf);
-    const double tb = cb * 2.0 * t * pf;
-    tg += tb;
-    tm += tb * R[rr].pref;
-    b1p += tb * nd.WG;
-    lzp += tb * dwr;
+  const double* o = clh_gg_nodes(d, zl);
+  const double WG = o[3 * np + p];
+  if (d.ggk) {
+    // the forward's checkpoint: (t, P fac, dW_RSD) per row; only the k cell
+    // of the ln P cotangent is walked again
+    ClhKCell kc;
+    kc.nk = d.nk;
+    const double l10fk = o[5 * np + 2 * np + p];
+    // eight rows of the checkpoint are loaded before any is used, so their
+    // latencies overlap
+    const double* gk = d.ggk + ((size_t)(zl * d.nrow + row0) * 3) * np + p;
+    for (int r8 = r_lo; r8 < r_hi; r8 += 8) {
+      double tv[8], pv[8], wv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int rr = r8 + u < r_hi ? r8 + u : r_hi - 1;
+        const double* g = gk + (size_t)rr * 3 * np;
+        tv[u] = g[0];
+        pv[u] = g[np];
+        wv[u] = g[2 * np];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int rr = r8 + u;
+        if (rr >= r_hi) break;
+        const double cb = Cr[rr];
+        const double t = tv[u], pf = pv[u], dwr = wv[u];
+        if (cb == 0.0 || pf == 0.0) continue;  // pf = 0: chi_1 beyond chi(a_min), zero integrand
+        const double dx = kc.dx(T.lk, R[rr].l10 - l10fk);
+        if (sc.R) sc.add(kc.i, dx, cb * t * t * pf);
+        const double tb = cb * 2.0 * t * pf;
+        tg += tb;
+        tm += tb * R[rr].pref;
+        b1p += tb * WG;
+        lzp += tb * dwr;
+      }
+    }
+  } else {
+    ClhGgNode<true> nd;
+    nd.init(d, T, o, p, zl);
+    for (int rr = r_lo; rr < r_hi; rr++) {
+      const double cb = Cr[rr];
+      if (cb == 0.0) continue;
+      double t, pf, dwr, dx;
+      if (!nd.eval(d, T, R[rr], t, pf, dwr, dx)) continue;
+      if (sc.R) sc.add(nd.pk.i, dx, cb * t * t * pf);
+      const double tb = cb * 2.0 * t * pf;
+      tg += tb;
+      tm += tb * R[rr].pref;
+      b1p += tb * WG;
+      lzp += tb * dwr;
+    }
   }
   if (sc.R) sc.done();
-  const double b1 = nd.b1, bmag = nd.bmag;
+  const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
   const size_t base = (size_t)2 * d.nl * CLH_NP_SS + (size_t)zl * 2 * np;
   if (tg != 0.0) atomicAdd(d.wbar + base + p, tg * b1);
   if (tm != 0.0 && bmag != 0.0) atomicAdd(d.wbar + base + np + p, tm * bmag);
@@ -584,12 +657,26 @@ hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s) {
   hipLaunchKernelGGL(k_m_grad, dim3(B), dim3(64), 0, s, D);
   return hipGetLastError();
 }
-hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int max_npair,
+hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int max_npair, int ns,
                                    hipStream_t s) {
   // max_npair: largest npair x rows of any work item (doubles of LDS)
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)max_npair;  // max over items of npair x rows
-  hipLaunchKernelGGL(k_limber_sx_adj, dim3(n), dim3(64), lds, s, D, work);
+#define CLH_SXADJ(NSV) hipLaunchKernelGGL(k_limber_sx_adj<NSV>, dim3(n), dim3(64), lds, s, D, work)
+  switch (ns) {
+    case 1: CLH_SXADJ(1); break;
+    case 2: CLH_SXADJ(2); break;
+    case 3: CLH_SXADJ(3); break;
+    case 4: CLH_SXADJ(4); break;
+    case 5: CLH_SXADJ(5); break;
+    case 6: CLH_SXADJ(6); break;
+    case 7: CLH_SXADJ(7); break;
+    case 8: CLH_SXADJ(8); break;
+    case 9: CLH_SXADJ(9); break;
+    case 10: CLH_SXADJ(10); break;
+    default: CLH_SXADJ(CLH_MAXB); break;
+  }
+#undef CLH_SXADJ
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles,

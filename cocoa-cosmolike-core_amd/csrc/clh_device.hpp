@@ -463,6 +463,28 @@ struct ClhPkNode {
   }
 };
 
+// The log10 k cell of a P(k,z) lookup along increasing k without the ln P
+// values: the same index and weight dx as ClhPkNode::lnp.
+struct ClhKCell {
+  int i = -1, nk;
+  double lo, hi, inv_k;
+  __device__ __forceinline__ double dx(const double* __restrict__ lk, double log10k) {
+    if (i < 0) {
+      i = clh_bs(lk, nk, log10k);
+      set(lk);
+    } else if (!((log10k >= lo || i == 0) && (log10k < hi || i == nk - 2))) {
+      i = clh_walk(lk, nk, log10k, i);
+      set(lk);
+    }
+    return (log10k - lo) * inv_k;
+  }
+  __device__ __forceinline__ void set(const double* __restrict__ lk) {
+    lo = lk[i];
+    hi = lk[i + 1];
+    inv_k = 1.0 / (hi - lo);
+  }
+};
+
 // One bracketed walk over a strictly increasing table: the index clh_bs
 // would return for v, found from the previous one; true when it moved.
 struct ClhBracket {

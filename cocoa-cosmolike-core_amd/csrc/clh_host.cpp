@@ -22,8 +22,8 @@
 
 hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
-hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, hipStream_t s);
-hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles,
+hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, int keep, hipStream_t s);
+hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles, int keep,
                                 hipStream_t s);
 hipError_t clh_launch_legendre(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s);
@@ -37,7 +37,7 @@ hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nk
                                  const double* iaA, const double* m, hipStream_t s);
 hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
-hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles,
+hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles, int ns,
                                    hipStream_t s);
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles,
                                    hipStream_t s);
@@ -232,15 +232,24 @@ struct Batch {
   int nrow_custom = 0;  // rows of a custom-ell batch (0: the layouts' rows)
   int gg_stage = 0;     // LDS doubles for the gg kernels' staged tables (0: not staged)
   // the device state (lens-efficiency tables, node weights, ss/gs C_ell rows)
-  // matches the draws' current parameters in FP64: set by an FP64 forward or
-  // a VJP, cleared by uploads and parameter updates
+  // matches the draws' current parameters: set by a forward or a VJP, cleared
+  // by uploads, parameter updates and precision changes.  The tables and
+  // weights are FP64 in every mode; after a mixed-precision forward the ss
+  // rows carry its FP32 node sums, which the VJP then uses only in the
+  // shear-calibration sums T (relative effect on d/dm about 1e-6)
   bool state_valid = false;
+  // checkpoint of the Limber integrands for the VJP (gg: t, P fac, dW_RSD;
+  // ss / gs: P fac): written by forwards with keep_ckpt, current while
+  // ckpt_fresh (cleared with state_valid and by forwards without it)
+  bool keep_ckpt = false, ckpt_fresh = false;
+  size_t ggk_per = 0, sxk_per = 0;  // doubles per draw
+  DBuf<double> d_ggk, d_sxk;
   int dv_size = 0;
   int nknz = 0;        // common ln P table size (0 if the draws differ)
   int ord[3] = {0, 1, 2};
   void release() {
     d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
-    d_vjp.release(); d_gscr.release(); d_run.release(); d_wlg.release();
+    d_vjp.release(); d_gscr.release(); d_run.release(); d_wlg.release(); d_ggk.release(); d_sxk.release();
     d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa4.release();
     d_wsx.release(); d_wgg.release(); d_wga.release(); d_wmx.release(); d_wlf.release();
   }
@@ -259,6 +268,7 @@ struct clh_ctx {
   int precision = CLH_PREC_FP64;
   int tile_rows[4] = {16, 64, 64, 64};  // rows per tile: fwd ss/gs, fwd gg, adj ss/gs, adj gg
   int gg_lds[2] = {0, 1};               // stage tables in LDS: fwd gg, adj gg
+  int vjp_ckpt = 0;                     // batch forwards keep the gg checkpoint for a VJP
   uint64_t version = 1, cached_version = 0;
   dvec cache_raw;   // raw (uncalibrated, ord 0,1,2) data vector of `cur`
   dvec cache_cl;    // cl rows of `cur`
@@ -1050,9 +1060,10 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], s));
   const bool mixed = c->precision != CLH_PREC_FP64;
-  HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, mixed, c->gg_lds[0] ? bt.gg_stage : 0, s));
+  HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, mixed, c->gg_lds[0] ? bt.gg_stage : 0,
+                                 bt.keep_ckpt && bt.ggk_per ? 1 : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], s));
-  HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, mixed, s));
+  HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, mixed, bt.keep_ckpt ? 1 : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
   if (legendre) {
     if (c->legendre_mode == CLH_LEGENDRE_FULL)
@@ -1063,7 +1074,8 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
       HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wlf.p, bt.n_wlf, s));
   }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[5], s));
-  bt.state_valid = !mixed;
+  bt.state_valid = true;
+  bt.ckpt_fresh = bt.keep_ckpt;
   return CLH_OK;
 }
 
@@ -1736,10 +1748,27 @@ int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
   for (int i = 0; i < c->batch.B; i++)
     if ((i == 0 || c->batch.lays[i] != c->batch.lays[i - 1]) && !ord_ok(*c->batch.lays[i], ord))
       return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");
+  // integrand checkpoint for a following VJP (batches with one layout)
+  Batch& cb = c->batch;
+  cb.keep_ckpt = c->vjp_ckpt && cb.uniform && !cb.nrow_custom;
+  if (cb.keep_ckpt) {
+    const size_t nl = cb.lays[0]->lens.nbin, nr = cb.lays[0]->nrow();
+    cb.ggk_per = (nl * nr * 3 * CLH_NP_GG + 31) & ~size_t(31);
+    cb.sxk_per = ((1 + nl) * nr * CLH_NP_SS + 31) & ~size_t(31);
+    if (cb.ggk_per) HIPCHK(c, cb.d_ggk.ensure(cb.ggk_per * cb.B));
+    HIPCHK(c, cb.d_sxk.ensure(cb.sxk_per * cb.B));
+  }
   // per-call placement: output pointer and block order go into the descriptors
   bool changed = false;
   for (int i = 0; i < c->batch.B; i++) {
     DrawDesc& q = c->batch.desc[i];
+    double* gk = (cb.keep_ckpt && cb.ggk_per) ? cb.d_ggk.p + cb.ggk_per * i : nullptr;
+    double* sk = cb.keep_ckpt ? cb.d_sxk.p + cb.sxk_per * i : nullptr;
+    if (q.ggk != gk || q.sxk != sk) {
+      q.ggk = gk;
+      q.sxk = sk;
+      changed = true;
+    }
     int start[3];
     dv_starts(*c->batch.lays[i], ord, start);
     double* want = dv_device + c->batch.dv_off[i];
@@ -1770,7 +1799,8 @@ int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
       c->gstream = c->stream;
     }
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
-    c->batch.state_valid = c->precision == CLH_PREC_FP64;
+    c->batch.state_valid = true;
+    c->batch.ckpt_fresh = c->batch.keep_ckpt;
     return CLH_OK;
   }
   return run_batch(c, c->batch, true);
@@ -1801,6 +1831,15 @@ int clh_batch_use_graph(clh_ctx* c, int on) {
   if (!c) return CLH_ERR_ARG;
   c->use_graph = on != 0;
   if (!c->use_graph && c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  return CLH_OK;
+}
+
+int clh_set_vjp_checkpoint(clh_ctx* c, int on) {
+  if (!c) return CLH_ERR_ARG;
+  if ((on != 0) != (c->vjp_ckpt != 0)) {
+    c->vjp_ckpt = on != 0;
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  }
   return CLH_OK;
 }
 
@@ -2010,6 +2049,10 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     q.dvbar = dv_bar + (size_t)bt.dv_size * i;
     q.g_lnP = g_lnP ? g_lnP + (size_t)bt.nknz * i : g;
     q.run = g_lnP ? bt.d_run.p + n_run * i : nullptr;
+    // the forward's checkpoints, if they still match the parameters
+    const bool ck = bt.state_valid && bt.ckpt_fresh;
+    q.ggk = (ck && bt.ggk_per) ? bt.d_ggk.p + bt.ggk_per * i : nullptr;
+    q.sxk = ck ? bt.d_sxk.p + bt.sxk_per * i : nullptr;
     q.g_src = g_src_dz ? g_src_dz + (size_t)ns * i : g + al(bt.nknz);
     q.g_lens = g_lens_dz ? g_lens_dz + (size_t)nl * i : g + al(bt.nknz) + al(CLH_MAXB);
     q.g_b1 = g_b1 ? g_b1 + (size_t)nl * i : g + al(bt.nknz) + 2 * al(CLH_MAXB);
@@ -2030,18 +2073,22 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // weights, and the ss / gs C_ell rows (the shear-calibration gradient needs
   // them; the gg rows carry no calibration and are not recomputed)
   HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));  // tables and shift tangents
-  if (!bt.state_valid) {  // else the last FP64 forward / VJP left them for these parameters
+  if (!bt.state_valid) {  // else the last forward / VJP left them for these parameters
+    bt.ckpt_fresh = false;  // a checkpoint predates these parameters
     HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
-    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, 0, s));
+    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, 0, 0, s));
   }
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa4.p, bt.n_wa, s));
   HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));
-  HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga, c->gg_lds[1] ? bt.gg_stage : 0, s));
+  const bool use_ggk = bt.state_valid && bt.ckpt_fresh && bt.ggk_per;
+  HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga,
+                                     (c->gg_lds[1] && !use_ggk) ? bt.gg_stage : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, s));
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, s));
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, s));
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, ns,
+                                     s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
   if (g_lnP) HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[9], s));

@@ -108,6 +108,10 @@ struct DrawDesc {
   double* dgtab;        // [ns+nl][n_a] d g / d(photo-z shift of its own bin)
   double* g_lnP;        // [nk][nzp]
   double* run;          // [64 (1 + nl) + 96 nl][nk] per-GL-node ln P cotangent runs, or null
+  double* sxk;          // [1 + nl][nrow][64] ss / gs checkpoint of the forward (S = P fac per
+                        // row and node) for the VJP, or null
+  double* ggk;          // [nl][nrow][3][96] gg integrand checkpoint of the forward (bracket t,
+                        // P fac, dW_RSD/d dz per row and node) for the VJP, or null
   double* g_src;        // [ns]
   double* g_lens;       // [nl]
   double* g_b1;         // [nl]

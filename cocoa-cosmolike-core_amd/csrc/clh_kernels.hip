@@ -293,6 +293,8 @@ __global__ __launch_bounds__(128) void k_nodes(const DrawDesc* __restrict__ D,
 // and Kahan-compensated node sums; gs tiles always run in FP64 (their lensing
 // and intrinsic-alignment terms can cancel to 1e-3 of their size, which FP32
 // rounding does not survive at 1e-4).
+// KEEP: S = P fac of every (row, node) also goes to the draw's checkpoint
+// (d.sxk) for the VJP.
 // ----------------------------------------------------------------------------
 struct ClhKahan {
   float s = 0.0f, c = 0.0f;
@@ -304,7 +306,7 @@ struct ClhKahan {
   }
 };
 
-template <bool F32>
+template <bool F32, bool KEEP>
 __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restrict__ D,
                                                          const int4* __restrict__ work) {
   // rows of S / Sf padded by one element: the pair sums read 16 rows at one
@@ -355,6 +357,9 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
         Sf[rl * sp + p] = 0.0f;
       }
       if (!f32) S[rl * sp + p] = s;
+      if (KEEP && rr < nr)
+        d.sxk[((size_t)(kind == CLH_W_SS ? 0 : 1 + zl) * d.nrow + row0 + rr) * np + p] =
+            f32 ? (double)Sf[rl * sp + p] : s;
     }
     __syncthreads();
     if (kind == CLH_W_SS) {
@@ -417,8 +422,11 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
 // group h takes rows 16 s + 8 h .. + 7, so each thread still walks its rows in
 // increasing ell.  Row sums over the 96 nodes: one wave per row.
 // F32: P(k,z) as expf of the FP64 bilinear value.
+// KEEP: also store the bracket t, P fac and dW_RSD / d dz of every (row, node)
+// in the draw's checkpoint (d.ggk), which the VJP then reads instead of
+// evaluating the integrand again.
 // ----------------------------------------------------------------------------
-template <bool F32>
+template <bool F32, bool KEEP>
 __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ D,
                                                    const int4* __restrict__ work, int stage) {
   __shared__ double V[CLH_SUB_ROWS * CLH_NP_GG];
@@ -433,7 +441,7 @@ __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ 
   const ClhGgTabs T = clh_gg_tabs(d, zl, dyn, stage, s_lk);
   __syncthreads();
   const int p = threadIdx.x % np, half = threadIdx.x / np;
-  ClhGgNode<false> nd;
+  ClhGgNode<KEEP> nd;
   nd.init(d, T, clh_gg_nodes(d, zl), p, zl);
   constexpr int per = CLH_SUB_ROWS / 2;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -443,8 +451,15 @@ __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ 
       const int rl = half * per + q;
       double v = 0.0;
       if (base + rl < nr) {
-        double t, pf, dwr, dx;
+        double t = 0.0, pf = 0.0, dwr = 0.0, dx;
         if (nd.template eval<F32>(d, T, R[base + rl], t, pf, dwr, dx)) v = t * t * pf;
+        else t = pf = dwr = 0.0;
+        if (KEEP) {
+          double* g = d.ggk + ((size_t)(zl * d.nrow + row0 + base + rl) * 3) * np + p;
+          g[0] = t;
+          g[np] = pf;
+          g[2 * np] = dwr;
+        }
       }
       V[rl * np + p] = v;
     }
@@ -640,18 +655,22 @@ hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, h
   hipLaunchKernelGGL(k_gtab<true>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }
-hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, hipStream_t s) {
+hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, int keep, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (f32) hipLaunchKernelGGL(k_limber_sx<true>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
-  else hipLaunchKernelGGL(k_limber_sx<false>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  if (f32 && keep) hipLaunchKernelGGL((k_limber_sx<true, true>), dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  else if (f32) hipLaunchKernelGGL((k_limber_sx<true, false>), dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  else if (keep) hipLaunchKernelGGL((k_limber_sx<false, true>), dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  else hipLaunchKernelGGL((k_limber_sx<false, false>), dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles,
-                                hipStream_t s) {
+                                int keep, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)stage_doubles;
   const int stage = stage_doubles > 0;
-  if (f32) hipLaunchKernelGGL(k_limber_gg<true>, dim3(n), dim3(192), lds, s, D, work, stage);
-  else hipLaunchKernelGGL(k_limber_gg<false>, dim3(n), dim3(192), lds, s, D, work, stage);
+  if (f32 && keep) hipLaunchKernelGGL((k_limber_gg<true, true>), dim3(n), dim3(192), lds, s, D, work, stage);
+  else if (f32) hipLaunchKernelGGL((k_limber_gg<true, false>), dim3(n), dim3(192), lds, s, D, work, stage);
+  else if (keep) hipLaunchKernelGGL((k_limber_gg<false, true>), dim3(n), dim3(192), lds, s, D, work, stage);
+  else hipLaunchKernelGGL((k_limber_gg<false, false>), dim3(n), dim3(192), lds, s, D, work, stage);
   return hipGetLastError();
 }

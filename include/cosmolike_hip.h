@@ -242,6 +242,13 @@ int clh_autotune(clh_ctx* ctx, int rows_out[4], int lds_out[2]);
  * kernel count */
 int clh_last_kernel_times(clh_ctx* ctx, const char** names, float* ms, int cap);
 int clh_enable_timing(clh_ctx* ctx, int on);
+/* on != 0: batch forwards (one layout per batch) also store the gg integrand
+ * of every (ell row, GL node) -- bracket t, P fac, dW_RSD/d dz, 3 doubles --
+ * so a following clh_batch_vjp with unchanged parameters reads it instead of
+ * evaluating the integrand again (nl x nrow x 288 doubles per draw of device
+ * memory).  After a mixed-precision forward the checkpoint holds its FP32
+ * P(k,z) products.  Default off.  No reference counterpart. */
+int clh_set_vjp_checkpoint(clh_ctx* ctx, int on);
 
 /* ---------------------------------------- reference-named scalar entries */
 /* Identical signatures to cosmolike/cosmo2D.h:30-96, served from clh_default().

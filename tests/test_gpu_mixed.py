@@ -105,3 +105,49 @@ def test_precision_switch_back_to_fp64(c2):
     dv = c.compute_3x2pt_real()
     errs = probe_errors(dv, ref, p.ns, p.nl, p.ntheta)
     assert max(v[0] for v in errs.values()) < 1e-6, errs
+
+
+@pytest.mark.parametrize("ckpt", [False, True])
+@pytest.mark.parametrize("mode", MODES)
+def test_vjp_after_mixed_forward(mode, ckpt):
+    """The VJP reuses the device state of a mixed-precision forward: lens
+    efficiencies and node weights are FP64 in every mode, so every gradient
+    but d/dm equals the one after an FP64 forward up to the order of the
+    adjoint's atomic sums (1e-11 of the largest entry); d/dm reads the ss
+    C_ell rows (FP32 node sums) through T and agrees to 1e-5 relative.  With
+    the integrand checkpoint the VJP also reads the forward's FP32 P(k,z)
+    products: every gradient then agrees to 1e-5 relative."""
+    import torch
+    draws = [synth.full_3x2pt_problem(cos) for cos, *_ in synth.random_draws(3, seed=12, ragged=False)]
+    c = Cosmolike().load(draws[0])
+    c.batch_upload(draws)
+    B, n, p = len(draws), c.datavector_size(), draws[0]
+    dev = torch.device("cuda:0")
+    out = torch.empty(B * n, dtype=torch.float64, device=dev)
+    dvbar = torch.linspace(-1.0, 1.0, B * n, dtype=torch.float64, device=dev)
+    sizes = (("lnP", p.lnP.size), ("src", p.ns), ("lens", p.nl), ("b1", p.nl), ("A", 1), ("m", p.ns))
+
+    def grads(prec):
+        c.set_precision(prec)
+        c.batch_forward(out.data_ptr())
+        g = {k: torch.empty(B * m, dtype=torch.float64, device=dev) for k, m in sizes}
+        c.batch_vjp(dvbar.data_ptr(), *[g[k].data_ptr() for k, _ in sizes])
+        c.synchronize()
+        torch.cuda.synchronize()
+        return {k: v.cpu().numpy() for k, v in g.items()}
+
+    try:
+        ref = grads("fp64")
+        c.set_vjp_checkpoint(ckpt)
+        got = grads(mode)
+    finally:
+        c.set_precision("fp64")
+        c.set_vjp_checkpoint(False)
+    tol = 1e-5 if ckpt else 1e-11
+    for k in ("lnP", "src", "lens", "b1", "A"):
+        e = np.abs(got[k] - ref[k]).max() / np.abs(ref[k]).max()
+        print(mode, ckpt, k, e)
+        assert e <= tol, k
+    e = np.abs(got["m"] - ref["m"]).max() / np.abs(ref["m"]).max()
+    print(mode, "d/dm rel", e)
+    assert e <= 1e-5

@@ -167,3 +167,37 @@ def test_vjp_reuse_matches_recompute(setup):
     for k, x in g.items():
         r = x.cpu().numpy()
         assert np.abs(r - reuse[k]).max() <= 1e-11 * np.abs(r).max(), k
+
+
+def test_vjp_checkpoint_matches_recompute(setup):
+    """With clh_set_vjp_checkpoint the VJP reads the forward's gg integrand;
+    it must equal the VJP that evaluates it again (up to the order of atomic
+    sums), and a checkpoint taken before a parameter update must not be used."""
+    import torch
+    c, draws, P, v = setup
+    B, n = v.shape
+    keys = ("lnP", "src_dz", "lens_dz", "b1", "ia_A", "shear_m")
+    g = {k: torch.empty_like(x) for k, x in P.items()}
+    out = torch.empty(B * n, dtype=torch.float64, device="cuda:0")
+
+    def run(PP, ckpt, update_after=None):
+        c.set_vjp_checkpoint(ckpt)
+        c.batch_set_params(*[PP[k].data_ptr() for k in keys])
+        c.batch_forward(out.data_ptr())
+        if update_after is not None:
+            c.batch_set_params(*[update_after[k].data_ptr() for k in keys])
+        c.batch_vjp(v.data_ptr(), *[g[k].data_ptr() for k in keys])
+        c.synchronize()
+        torch.cuda.synchronize()
+        return {k: x.cpu().numpy().copy() for k, x in g.items()}
+
+    def same(a, b):
+        for k in keys:
+            assert np.abs(a[k] - b[k]).max() <= 1e-11 * np.abs(b[k]).max(), k
+
+    P2 = dict(P, b1=P["b1"] * 1.1, lens_dz=P["lens_dz"] + 0.01)
+    try:
+        same(run(P, True), run(P, False))
+        same(run(P, True, update_after=P2), run(P2, False))
+    finally:
+        c.set_vjp_checkpoint(False)
