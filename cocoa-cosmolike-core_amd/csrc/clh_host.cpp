@@ -242,14 +242,14 @@ struct Batch {
   // ss / gs: P fac): written by forwards with keep_ckpt, current while
   // ckpt_fresh (cleared with state_valid and by forwards without it)
   bool keep_ckpt = false, ckpt_fresh = false;
-  size_t ggk_per = 0, sxk_per = 0;  // doubles per draw
-  DBuf<double> d_ggk, d_sxk;
+  size_t ggk_per = 0, sxk_per = 0, dgk_per = 0;  // doubles per draw
+  DBuf<double> d_ggk, d_sxk, d_dgk;  // dgk: lens-efficiency shift tangents
   int dv_size = 0;
   int nknz = 0;        // common ln P table size (0 if the draws differ)
   int ord[3] = {0, 1, 2};
   void release() {
     d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
-    d_vjp.release(); d_gscr.release(); d_run.release(); d_wlg.release(); d_ggk.release(); d_sxk.release();
+    d_vjp.release(); d_gscr.release(); d_run.release(); d_wlg.release(); d_ggk.release(); d_sxk.release(); d_dgk.release();
     d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa4.release();
     d_wsx.release(); d_wgg.release(); d_wga.release(); d_wmx.release(); d_wlf.release();
   }
@@ -1055,7 +1055,9 @@ static int build_batch_ctx(clh_ctx* c, Batch& bt, const std::vector<const Draw*>
 static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   hipStream_t s = c->stream;
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], s));
-  HIPCHK(c, clh_launch_gtab(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
+  // with the checkpoint, the tables come with their photo-z shift tangents
+  if (bt.keep_ckpt) HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
+  else HIPCHK(c, clh_launch_gtab(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], s));
   HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], s));
@@ -1755,8 +1757,10 @@ int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
     const size_t nl = cb.lays[0]->lens.nbin, nr = cb.lays[0]->nrow();
     cb.ggk_per = (nl * nr * 3 * CLH_NP_GG + 31) & ~size_t(31);
     cb.sxk_per = ((1 + nl) * nr * CLH_NP_SS + 31) & ~size_t(31);
+    cb.dgk_per = (((size_t)cb.lays[0]->src.nbin + nl) * cb.lays[0]->n_a + 31) & ~size_t(31);
     if (cb.ggk_per) HIPCHK(c, cb.d_ggk.ensure(cb.ggk_per * cb.B));
     HIPCHK(c, cb.d_sxk.ensure(cb.sxk_per * cb.B));
+    HIPCHK(c, cb.d_dgk.ensure(cb.dgk_per * cb.B));
   }
   // per-call placement: output pointer and block order go into the descriptors
   bool changed = false;
@@ -1767,6 +1771,10 @@ int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
     if (q.ggk != gk || q.sxk != sk) {
       q.ggk = gk;
       q.sxk = sk;
+      changed = true;
+    }
+    if (cb.keep_ckpt && q.dgtab != cb.d_dgk.p + cb.dgk_per * i) {
+      q.dgtab = cb.d_dgk.p + cb.dgk_per * i;
       changed = true;
     }
     int start[3];
@@ -2044,7 +2052,6 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     q.abar = v; v += n_abar;
     q.bbar = v; v += n_bbar;
     q.wbar = v; v += n_wbar;
-    q.dgtab = v;
     double* g = bt.d_gscr.p + gper * i;
     q.dvbar = dv_bar + (size_t)bt.dv_size * i;
     q.g_lnP = g_lnP ? g_lnP + (size_t)bt.nknz * i : g;
@@ -2053,6 +2060,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     const bool ck = bt.state_valid && bt.ckpt_fresh;
     q.ggk = (ck && bt.ggk_per) ? bt.d_ggk.p + bt.ggk_per * i : nullptr;
     q.sxk = ck ? bt.d_sxk.p + bt.sxk_per * i : nullptr;
+    q.dgtab = ck ? bt.d_dgk.p + bt.dgk_per * i : v;
     q.g_src = g_src_dz ? g_src_dz + (size_t)ns * i : g + al(bt.nknz);
     q.g_lens = g_lens_dz ? g_lens_dz + (size_t)nl * i : g + al(bt.nknz) + al(CLH_MAXB);
     q.g_b1 = g_b1 ? g_b1 + (size_t)nl * i : g + al(bt.nknz) + 2 * al(CLH_MAXB);
@@ -2072,7 +2080,9 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // forward recompute: lens-efficiency tables and their shift tangents, node
   // weights, and the ss / gs C_ell rows (the shear-calibration gradient needs
   // them; the gg rows carry no calibration and are not recomputed)
-  HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));  // tables and shift tangents
+  const bool ck_all = bt.state_valid && bt.ckpt_fresh;
+  if (!ck_all)  // else the checkpointing forward left tables and shift tangents
+    HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
   if (!bt.state_valid) {  // else the last forward / VJP left them for these parameters
     bt.ckpt_fresh = false;  // a checkpoint predates these parameters
     HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));

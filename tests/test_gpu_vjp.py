@@ -180,10 +180,13 @@ def test_vjp_checkpoint_matches_recompute(setup):
     g = {k: torch.empty_like(x) for k, x in P.items()}
     out = torch.empty(B * n, dtype=torch.float64, device="cuda:0")
 
+    fwd = {}
+
     def run(PP, ckpt, update_after=None):
         c.set_vjp_checkpoint(ckpt)
         c.batch_set_params(*[PP[k].data_ptr() for k in keys])
         c.batch_forward(out.data_ptr())
+        fwd[ckpt] = out.cpu().numpy().copy()
         if update_after is not None:
             c.batch_set_params(*[update_after[k].data_ptr() for k in keys])
         c.batch_vjp(v.data_ptr(), *[g[k].data_ptr() for k in keys])
@@ -198,6 +201,8 @@ def test_vjp_checkpoint_matches_recompute(setup):
     P2 = dict(P, b1=P["b1"] * 1.1, lens_dz=P["lens_dz"] + 0.01)
     try:
         same(run(P, True), run(P, False))
+        # the checkpointing forward (tables with tangents) gives the same data vector
+        assert np.array_equal(fwd[True], fwd[False])
         same(run(P, True, update_after=P2), run(P2, False))
     finally:
         c.set_vjp_checkpoint(False)
