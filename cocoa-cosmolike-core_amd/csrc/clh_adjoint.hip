@@ -18,6 +18,7 @@
 #include <algorithm>
 
 #include "clh_device.hpp"
+#include "clh_run.hpp"
 #include "clh_internal.hpp"
 
 #define CLH_BLOCK 256
@@ -204,55 +205,6 @@ __global__ __launch_bounds__(64) void k_m_grad(const DrawDesc* __restrict__ D) {
 }
 
 // ----------------------------------------------------------------------------
-// ln P cotangent of one GL node (a fixed z column j) over the rows of a tile,
-// kept as a run along the k index: R[i] = sum over rows with cell i of
-// g (1 - dx) plus sum over rows with cell i - 1 of g dx, where R is the node's
-// own row of the run slab (zeroed before the launch); k_lnp_gather folds the
-// z weights in.  Along increasing ell the cell index i never decreases, so a
-// cell strictly inside a tile's index range is reached by that tile alone and
-// takes a plain store; the last two cells of the range can be shared with the
-// next tile, the first three with the previous one (rows are sorted up to
-// rounding, which can move the first cell back by one), and take atomics.  A
-// decrease of i inside the run switches the rest of it to atomics.
-// ----------------------------------------------------------------------------
-struct ClhLnPRun {
-  double* R;
-  int ci = -1, first = -1;
-  bool mono = true;
-  double a0 = 0.0, a1 = 0.0;
-  __device__ __forceinline__ void put(int i, double v, bool edge) {
-    if (v == 0.0) return;
-    if (edge || !mono || i <= first + 2) atomicAdd(R + i, v);
-    else R[i] = v;
-  }
-  __device__ __forceinline__ void add(int i, double dx, double g) {
-    if (i != ci) {
-      if (ci < 0) {
-        first = i;
-      } else {
-        if (i < ci) mono = false;
-        put(ci, a0, false);
-        if (i == ci + 1) {
-          a0 = a1;
-        } else {
-          put(ci + 1, a1, false);
-          a0 = 0.0;
-        }
-        a1 = 0.0;
-      }
-      ci = i;
-    }
-    a0 += g * (1 - dx);
-    a1 += g * dx;
-  }
-  __device__ __forceinline__ void done() {
-    if (ci < 0) return;
-    put(ci, a0, true);
-    put(ci + 1, a1, true);
-  }
-};
-
-// ----------------------------------------------------------------------------
 // k_limber_sx_adj: adjoint of k_limber_sx.  One wave per work item (draw,
 // kind + 4 zl, row0, nrows); thread = GL node, walking the tile's rows in
 // increasing ell with the forward's P(k,z) recomputation.  Per (row, node):
@@ -266,11 +218,13 @@ struct ClhLnPRun {
 // checkpoint (d.sxk) the P(k,z) lookup is replaced by its stored S = P fac
 // and only the k cell is walked again.
 // ----------------------------------------------------------------------------
-template <int NS>
+template <int NS, bool CK>
 __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict__ D,
                                                       const int4* __restrict__ work) {
   // NS: the source-bin count when it is at most 10 (loops fully unrolled to
-  // it), else CLH_MAXB with the count checked at run time
+  // it), else CLH_MAXB with the count checked at run time.  CK: read the
+  // forward's checkpoint instead of looking P(k,z) up again (its own
+  // instantiation, so the lighter path gets its own register budget)
   constexpr bool EXACT = NS < CLH_MAXB;
   extern __shared__ double Cb[];  // ss: [ns][ns][T], gs: [ns][T]
   __shared__ double L10[CLH_TILE_ROWS];
@@ -363,8 +317,8 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
     }
     if (sc.R) sc.add(i, dx, g);
   };
-  if (d.sxk) {
-    // the forward's checkpoint of S; eight rows loaded ahead of their use
+  if constexpr (CK) {
+    // the forward's checkpoint of S (d.sxk); eight rows loaded ahead of their use
     const double* sk = d.sxk + ((size_t)(ss ? 0 : 1 + zl) * nrow + row0) * np + p;
     ClhKCell kc;
     kc.nk = d.nk;
@@ -412,6 +366,7 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
 // W_RSD: ln P cotangent cbar t^2 P fac; tb = 2 cbar t P fac feeds b1 (W_gal),
 // the W_gal / W_mag node cotangents and the lens shift through dW_RSD/d dz.
 // ----------------------------------------------------------------------------
+template <bool CK>
 __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restrict__ D,
                                                        const int4* __restrict__ work, int stage) {
   __shared__ ClhGgRow R[CLH_TILE_ROWS];
@@ -437,9 +392,9 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
   const int r_lo = half * hn, r_hi = half ? nr : hn;
   const double* o = clh_gg_nodes(d, zl);
   const double WG = o[3 * np + p];
-  if (d.ggk) {
-    // the forward's checkpoint: (t, P fac, dW_RSD) per row; only the k cell
-    // of the ln P cotangent is walked again
+  if constexpr (CK) {
+    // the forward's checkpoint (d.ggk): (t, P fac, dW_RSD) per row; only the
+    // k cell of the ln P cotangent is walked again
     ClhKCell kc;
     kc.nk = d.nk;
     const double l10fk = o[5 * np + 2 * np + p];
@@ -658,11 +613,15 @@ hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int max_npair, int ns,
-                                   hipStream_t s) {
+                                   int ck, hipStream_t s) {
   // max_npair: largest npair x rows of any work item (doubles of LDS)
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)max_npair;  // max over items of npair x rows
-#define CLH_SXADJ(NSV) hipLaunchKernelGGL(k_limber_sx_adj<NSV>, dim3(n), dim3(64), lds, s, D, work)
+#define CLH_SXADJ(NSV)                                                                            \
+  do {                                                                                            \
+    if (ck) hipLaunchKernelGGL((k_limber_sx_adj<NSV, true>), dim3(n), dim3(64), lds, s, D, work);  \
+    else hipLaunchKernelGGL((k_limber_sx_adj<NSV, false>), dim3(n), dim3(64), lds, s, D, work);    \
+  } while (0)
   switch (ns) {
     case 1: CLH_SXADJ(1); break;
     case 2: CLH_SXADJ(2); break;
@@ -679,11 +638,12 @@ hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, 
 #undef CLH_SXADJ
   return hipGetLastError();
 }
-hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles,
+hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
                                    hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)stage_doubles;
-  hipLaunchKernelGGL(k_limber_gg_adj, dim3(n), dim3(192), lds, s, D, work, stage_doubles > 0 ? 1 : 0);
+  if (ck) hipLaunchKernelGGL((k_limber_gg_adj<true>), dim3(n), dim3(192), lds, s, D, work, stage_doubles > 0 ? 1 : 0);
+  else hipLaunchKernelGGL((k_limber_gg_adj<false>), dim3(n), dim3(192), lds, s, D, work, stage_doubles > 0 ? 1 : 0);
   return hipGetLastError();
 }
 hipError_t clh_launch_lnp_gather(const DrawDesc* D, const int2* work, int n, int ic, int nz, int nl,

@@ -37,9 +37,9 @@ hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nk
                                  const double* iaA, const double* m, hipStream_t s);
 hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
-hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles, int ns,
+hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles, int ns, int ck,
                                    hipStream_t s);
-hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles,
+hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
                                    hipStream_t s);
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_lnp_gather(const DrawDesc* D, const int2* work, int n, int ic, int nz, int nl,
@@ -549,9 +549,12 @@ static void build_kernels_uncached(Layout& L, bool full) {
   L.ell_ss.resize(nrow);
   L.ell_gx.resize(nrow);
   for (int r = 0; r < CLH_NLOW; r++) L.ell_ss[r] = L.ell_gx[r] = (double)(r + 1);
+  // the first table row of ss is exp(ln 19), which rounds a few ulp below
+  // the direct row 19; the rows are kept non-decreasing (the adjoint's
+  // per-node ln P runs rely on it), which moves that row by at most an ulp
   for (int i = 0; i < L.n_ell; i++) {
-    L.ell_ss[CLH_NLOW + i] = std::exp(lim_ss[0] + i * lim_ss[2]);
-    L.ell_gx[CLH_NLOW + i] = std::exp(lim_gx[0] + i * lim_gx[2]);
+    L.ell_ss[CLH_NLOW + i] = std::max(std::exp(lim_ss[0] + i * lim_ss[2]), L.ell_ss[CLH_NLOW + i - 1]);
+    L.ell_gx[CLH_NLOW + i] = std::max(std::exp(lim_gx[0] + i * lim_gx[2]), L.ell_gx[CLH_NLOW + i - 1]);
   }
   L.kern_valid = true;
 }
@@ -2019,11 +2022,10 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     if (lds(ic) > 65536)
       return fail(c, CLH_ERR_ARG, "ln P gradient: %d z values of P(k,z) with %d lens bins exceed the LDS tile",
                   nzp, nl);
-    // rows in increasing ell up to rounding (exp(ln 19) sits a few ulp below
-    // the direct row 19), so a node's k cell moves back by at most one
+    // the per-node runs need rows in non-decreasing ell (as the layout builds them)
     for (const dvec* e : {&L.ell_ss, &L.ell_gx})
       for (size_t r = 1; r < e->size(); r++)
-        if ((*e)[r] < (*e)[r - 1] * (1.0 - 1e-12)) return fail(c, CLH_ERR_STATE, "ell rows not sorted");
+        if ((*e)[r] < (*e)[r - 1]) return fail(c, CLH_ERR_STATE, "ell rows not sorted");
     HIPCHK(c, bt.d_run.ensure(n_run * B));
     if (bt.lnp_ic != ic || bt.n_wlg != B * ((nk + ic - 1) / ic)) {
       std::vector<int2> wlg;
@@ -2094,11 +2096,11 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));
   const bool use_ggk = bt.state_valid && bt.ckpt_fresh && bt.ggk_per;
   HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga,
-                                     (c->gg_lds[1] && !use_ggk) ? bt.gg_stage : 0, s));
+                                     (c->gg_lds[1] && !use_ggk) ? bt.gg_stage : 0, use_ggk ? 1 : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, s));
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, ck_all ? 1 : 0, s));
   HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, ns,
-                                     s));
+                                     ck_all ? 1 : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
   if (g_lnP) HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[9], s));
