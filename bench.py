@@ -37,12 +37,34 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _build_one(item):
+    from cosmolike_amd import synth
+    kind, spec = item
+    if kind == "full":
+        return synth.full_3x2pt_problem(spec)
+    cs, ns, nl, nt = spec
+    return synth.ragged_3x2pt_problem(cs, ns, nl, nt)
+
+
+def _build(items):
+    """Synthetic draws, built by a pool of worker processes (forked before
+    anything touches the GPU)."""
+    workers = min(16, os.cpu_count() or 1)
+    if workers <= 1 or len(items) < 16:
+        return [_build_one(x) for x in items]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_build_one, items, chunksize=4)
+
+
 def make_draws(B, seed):
     from cosmolike_amd import synth
-    draws = []
-    for cos, *_ in synth.random_draws(B, seed=seed, ragged=False):
-        draws.append(synth.full_3x2pt_problem(cos))
-    return draws
+    return _build([("full", cos) for cos, *_ in synth.random_draws(B, seed=seed, ragged=False)])
+
+
+def make_ragged_draws(n, seed):
+    from cosmolike_amd import synth
+    return _build([("ragged", tuple(x)) for x in synth.random_draws(n, seed=seed, ragged=True)])
 
 
 def algorithmic_work(p):
@@ -124,7 +146,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--mode", choices=["folded", "full"], default="folded")
     ap.add_argument("--precision", choices=["fp64", "fp32", "bf16x3"], default="fp64")
     ap.add_argument("--seed", type=int, default=20261203)
@@ -142,12 +164,15 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
-    from cosmolike_amd import Cosmolike, LEGENDRE_FOLDED, LEGENDRE_FULL
-
+    # synthetic inputs first: the builder forks worker processes, which must
+    # happen before the HIP runtime is loaded
     t0 = time.perf_counter()
     draws = make_draws(args.batch, args.seed)
-    log(f"bench: built {len(draws)} draws in {time.perf_counter() - t0:.1f}s")
+    probs = make_ragged_draws(args.ragged, args.seed + 1) if args.ragged > 0 else []
+    log(f"bench: built {len(draws)} + {len(probs)} draws in {time.perf_counter() - t0:.1f}s")
+
+    import torch
+    from cosmolike_amd import Cosmolike, LEGENDRE_FOLDED, LEGENDRE_FULL
     mode = LEGENDRE_FULL if args.mode == "full" else LEGENDRE_FOLDED
     c = Cosmolike(device=0).load(draws[0], mode)
     c.batch_upload(draws)
@@ -220,6 +245,8 @@ def main():
              for k, m in (("lnP", nknz), ("src", ns), ("lens", nl), ("b1", nl), ("A", 1), ("m", ns))}
         ptrs = [g[k].data_ptr() for k in ("lnP", "src", "lens", "b1", "A", "m")]
         c.set_vjp_checkpoint(True)
+        pair_tiles = c.autotune()  # the checkpointed forward and adjoint have their own best tiles
+        log(f"bench: autotuned rows per tile for forward+VJP pairs {pair_tiles}, gg tables in LDS {c.gg_lds}")
         for _ in range(2):
             c.batch_forward(dv.data_ptr())
             c.batch_vjp(dvbar.data_ptr(), *ptrs)
@@ -244,6 +271,7 @@ def main():
                     vjp_ms[k] = vjp_ms.get(k, 0.0) + v / 3
         c.enable_timing(False)
         c.set_vjp_checkpoint(False)
+        c.set_tile_rows(tile_rows)
         log(f"bench: VJP Limber kernel ms {vjp_ms}")
     work = algorithmic_work(draws[0])
     dom = max(acc, key=acc.get)
@@ -264,10 +292,7 @@ def main():
     # layout, all in one captured graph of per-stage launches
     ragged = None
     if args.ragged > 0:
-        from cosmolike_amd import synth
         t1 = time.perf_counter()
-        spec = synth.random_draws(args.ragged, seed=args.seed + 1, ragged=True)
-        probs = [synth.ragged_3x2pt_problem(cs, ns, nl, nt) for cs, ns, nl, nt in spec]
         cr = Cosmolike(device=0)
         cache, lays = {}, []
         for pr in probs:

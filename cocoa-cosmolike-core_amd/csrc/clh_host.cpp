@@ -1745,16 +1745,11 @@ long long clh_batch_datavector_offset(clh_ctx* c, int i) {
   return c->batch.dv_off[i];
 }
 
-int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
-  if (!c || !ord || !dv_device) return CLH_ERR_ARG;
-  if (c->batch.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
-  if ((c->batch.ctx_layout && !c->dl.valid) || c->batch_epoch != c->mode_epoch)
-    return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
-  for (int i = 0; i < c->batch.B; i++)
-    if ((i == 0 || c->batch.lays[i] != c->batch.lays[i - 1]) && !ord_ok(*c->batch.lays[i], ord))
-      return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");
-  // integrand checkpoint for a following VJP (batches with one layout)
-  Batch& cb = c->batch;
+// Checkpoint buffers and descriptor pointers of the next forward: the forward
+// stores its Limber integrands (and the lens-efficiency shift tangents) when
+// the context asks for it and the batch has one layout.  `changed`: some
+// descriptor changed on the host and must be uploaded.
+static int prepare_checkpoint(clh_ctx* c, Batch& cb, bool& changed) {
   cb.keep_ckpt = c->vjp_ckpt && cb.uniform && !cb.nrow_custom;
   if (cb.keep_ckpt) {
     const size_t nl = cb.lays[0]->lens.nbin, nr = cb.lays[0]->nrow();
@@ -1765,10 +1760,8 @@ int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
     HIPCHK(c, cb.d_sxk.ensure(cb.sxk_per * cb.B));
     HIPCHK(c, cb.d_dgk.ensure(cb.dgk_per * cb.B));
   }
-  // per-call placement: output pointer and block order go into the descriptors
-  bool changed = false;
-  for (int i = 0; i < c->batch.B; i++) {
-    DrawDesc& q = c->batch.desc[i];
+  for (int i = 0; i < cb.B; i++) {
+    DrawDesc& q = cb.desc[i];
     double* gk = (cb.keep_ckpt && cb.ggk_per) ? cb.d_ggk.p + cb.ggk_per * i : nullptr;
     double* sk = cb.keep_ckpt ? cb.d_sxk.p + cb.sxk_per * i : nullptr;
     if (q.ggk != gk || q.sxk != sk) {
@@ -1780,6 +1773,25 @@ int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
       q.dgtab = cb.d_dgk.p + cb.dgk_per * i;
       changed = true;
     }
+  }
+  return CLH_OK;
+}
+
+int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
+  if (!c || !ord || !dv_device) return CLH_ERR_ARG;
+  if (c->batch.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
+  if ((c->batch.ctx_layout && !c->dl.valid) || c->batch_epoch != c->mode_epoch)
+    return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
+  for (int i = 0; i < c->batch.B; i++)
+    if ((i == 0 || c->batch.lays[i] != c->batch.lays[i - 1]) && !ord_ok(*c->batch.lays[i], ord))
+      return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");
+  // integrand checkpoint for a following VJP (batches with one layout)
+  bool changed = false;
+  int rc = prepare_checkpoint(c, c->batch, changed);
+  if (rc) return rc;
+  // per-call placement: output pointer and block order go into the descriptors
+  for (int i = 0; i < c->batch.B; i++) {
+    DrawDesc& q = c->batch.desc[i];
     int start[3];
     dv_starts(*c->batch.lays[i], ord, start);
     double* want = dv_device + c->batch.dv_off[i];
@@ -1907,13 +1919,22 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   HIPCHK(c, hipSetDevice(c->device));
   // the forward writes into the batch's own output buffer while tuning
   HIPCHK(c, bt.d_dv.ensure((size_t)bt.dv_off[bt.B]));
+  bool ck_changed = false;
+  int rc0 = prepare_checkpoint(c, bt, ck_changed);  // tune what the next forward will run
+  if (rc0) return rc0;
   for (int i = 0; i < bt.B; i++) bt.desc[i].dv = bt.d_dv.p + bt.dv_off[i];
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
   const bool adj = bt.uniform && bt.nknz > 0;
-  DBuf<double> dvbar;  // unit cotangent for the adjoint stages
+  DBuf<double> dvbar, glnp;  // unit cotangent and ln P gradient of the adjoint stages
   if (adj) {
-    HIPCHK(c, dvbar.ensure((size_t)bt.dv_off[bt.B]));
-    HIPCHK(c, clh_launch_fill(dvbar.p, (size_t)bt.dv_off[bt.B], 1.0, c->stream));
+    hipError_t e = dvbar.ensure((size_t)bt.dv_off[bt.B]);
+    if (e == hipSuccess) e = clh_launch_fill(dvbar.p, (size_t)bt.dv_off[bt.B], 1.0, c->stream);
+    if (e == hipSuccess) e = glnp.ensure((size_t)bt.nknz * bt.B);  // the ln P runs are part of what is timed
+    if (e != hipSuccess) {
+      dvbar.release();
+      glnp.release();
+      return fail(c, CLH_ERR_HIP, "autotune: %s", hipGetErrorString(e));
+    }
   }
   const bool was_timing = c->timing;
   const int lds_was[2] = {c->gg_lds[0], c->gg_lds[1]};
@@ -1932,7 +1953,7 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
       c->gg_lds[0] = c->gg_lds[1] = st;
       for (int rep = 0; rep < 3 && rc == CLH_OK; rep++) {
         if ((rc = run_batch(c, bt, false))) break;
-        if (adj && (rc = clh_batch_vjp(c, ord, dvbar.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr)))
+        if (adj && (rc = clh_batch_vjp(c, ord, dvbar.p, glnp.p, nullptr, nullptr, nullptr, nullptr, nullptr)))
           break;
         if ((rc = clh_synchronize(c))) break;
         const float ms[4] = {c->last_ms[3], c->last_ms[2], adj ? c->last_vjp_ms[1] : 1e30f,
@@ -1949,6 +1970,9 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
     }
   }
   c->timing = was_timing;
+  (void)hipStreamSynchronize(c->stream);
+  dvbar.release();  // local buffers (hipFree)
+  glnp.release();
   if (rc) {
     c->gg_lds[0] = lds_was[0];
     c->gg_lds[1] = lds_was[1];
