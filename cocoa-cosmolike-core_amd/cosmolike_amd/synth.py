@@ -89,6 +89,38 @@ def sigma8_unnorm(om: float, h: float, ns: float) -> float:
     return math.sqrt(integrate.simpson(integrand, x=lk))
 
 
+def fpt_ia_tables(cosmo: Cosmology, n: int = 1100, kmin: float = 1e-4, kmax: float = 1e2):
+    """Synthetic stand-ins for the FAST-PT IA power spectra, in set_IA_PS form:
+    PS[12][n] on k_i = exp(ln kmin + i (ln kmax - ln kmin) / n) (h/Mpc), rows
+    0..9 the tt E/B, ta dE1/dE2/0E0E/0B0B and mix A/B/D_EE/D_BB terms in
+    (Mpc/h)^3 (smooth functions of the linear P(k) at z = 0 with distinct
+    amplitudes and signs), row 10 = k, row 11 = P_lin.  They exercise every
+    term of the TATT integrands; they are not one-loop integrals."""
+    lk = np.log(kmin) + np.arange(n) * (np.log(kmax) - np.log(kmin)) / n
+    k = np.exp(lk)
+    amp = (cosmo.sigma8 / sigma8_unnorm(cosmo.omega_m, cosmo.h, cosmo.ns)) ** 2
+    plin = amp * k**cosmo.ns * t_bbks(k, cosmo.omega_m, cosmo.h) ** 2
+    shape = plin * (k / (k + 0.3)) ** 2
+    coef = [0.8, 0.3, -0.5, 0.25, 0.6, 0.2, 0.4, -0.35, 0.5, 0.15]
+    out = np.empty((12, n))
+    for t, c in enumerate(coef):
+        out[t] = c * shape * (1.0 + 0.1 * t * np.tanh(np.log(k / 0.1)))
+    out[10] = k
+    out[11] = plin
+    return np.ascontiguousarray(out.reshape(-1)), kmin, kmax
+
+
+def tatt_problem(cosmo: Cosmology | None = None, n_ell: int = 500, A2: float = -0.8, eta2: float = 0.5,
+                 bta: float = 0.7) -> Problem:
+    """The full 3x2pt layout with TATT intrinsic alignments (redshift
+    evolution: A1 = 0.5, eta = 0; A2, eta2, b_TA as given)."""
+    cosmo = cosmo or Cosmology()
+    p = full_3x2pt_problem(cosmo, n_ell)
+    p.ia_model, p.ia2_A, p.ia2_eta, p.ia_bta = 1, A2, eta2, bta
+    p.ia_ps, p.ia_ps_kmin, p.ia_ps_kmax = fpt_ia_tables(cosmo)
+    return p
+
+
 def power_table(cosmo: Cosmology, nk: int = 400, nz: int = 100, kmin_log10=-4.0,
                 kmax_log10=2.0, zmax=3.0):
     """(log10k[nk], z[nz], lnP[nk*nz] k-major) in (Mpc/h)^3, k in h/Mpc."""
@@ -201,6 +233,15 @@ class Problem:
     ia_mode: int = 3             # IA_REDSHIFT_EVOLUTION (cosmolike/IA.h:13)
     h: float = 0.7
     pm: np.ndarray | None = None  # point-mass amplitudes per lens bin
+    # TATT intrinsic alignments (IA_MODEL = 1): A2 (value, exponent), b_TA and
+    # the FAST-PT IA tables in set_IA_PS form ([12][N], k range in h/Mpc)
+    ia_model: int = 0
+    ia2_A: float = 0.0
+    ia2_eta: float = 0.0
+    ia_bta: float = 0.0
+    ia_ps: np.ndarray | None = None
+    ia_ps_kmin: float = 0.0
+    ia_ps_kmax: float = 0.0
 
     @property
     def ns(self):

@@ -57,6 +57,13 @@ struct oc_ctx {
   int mask_n;
   int ia_mode;
   double ia_bin[OC_MAXB], ia_A, ia_eta, oneplusz0, c1rhocrit;
+  /* TATT (IA_MODEL = 1): nuisance.ia[1][*] (A2), ia[2][*] (b_TA) and the
+     FAST-PT IA tables of set_IA_PS (FPTIA.tab[0..9], k in c/H0 units) */
+  int ia_model;
+  double ia2_bin[OC_MAXB], ia2_A, ia2_eta, bta_bin[OC_MAXB], bta_A;
+  double* fpt; /* [10][fpt_n] */
+  int fpt_n;
+  double fpt_kmin, fpt_kmax;
   /* caches */
   int g_valid;
   double *gt_tab, *gl_tab; /* [ns][N_a], [nl][N_a] */
@@ -102,6 +109,7 @@ void oc_free(oc_ctx* c) {
   free(c->l_tab); free(c->l_zc); free(c->l_y); free(c->l_c);
   free(c->gt_tab); free(c->gl_tab); free(c->K);
   free(c->mask);
+  free(c->fpt);
   free(c);
 }
 
@@ -626,6 +634,47 @@ static void IA_A1_Z1Z2(oc_ctx* c, double a, double growfac_a, int n1, int n2, do
   res[1] = A2 * x;
 }
 
+/* IA_A2_Z1Z2 (IA.c:305-361): A2 (Omega_m c1rhocrit / D^2), same evolution
+   form as A1 with its own amplitude and exponent */
+static void IA_A2_Z1Z2(oc_ctx* c, double a, double growfac_a, int n1, int n2, double res[2]) {
+  double A1 = 0.0, A2 = 0.0;
+  switch (c->ia_mode) {
+    case 0: break;
+    case 2: A1 = c->ia2_bin[n1]; A2 = c->ia2_bin[n2]; break;
+    case 3: A1 = c->ia2_A * pow(1.0 / (a * c->oneplusz0), c->ia2_eta); A2 = A1; break;
+    default: oc_set_err(c, "IA mode %d not supported", c->ia_mode);
+  }
+  const double x = c->omega_m * c->c1rhocrit / (growfac_a * growfac_a);
+  res[0] = A1 * x;
+  res[1] = A2 * x;
+}
+
+/* IA_BTA_Z1Z2 (IA.c:370-415): b_TA per bin or one value; 1 without IA */
+static void IA_BTA_Z1Z2(oc_ctx* c, int n1, int n2, double res[2]) {
+  switch (c->ia_mode) {
+    case 0: res[0] = res[1] = 1.0; break;
+    case 2: res[0] = c->bta_bin[n1]; res[1] = c->bta_bin[n2]; break;
+    case 3: res[0] = res[1] = c->bta_A; break;
+    default: oc_set_err(c, "IA mode %d not supported", c->ia_mode); res[0] = res[1] = 0.0;
+  }
+}
+
+/* FAST-PT IA table t at k (cosmo2D.c:1414-1447): uniform in ln k with step
+   (ln kmax - ln kmin) / N, zero outside [kmin, kmax], last entry at the top */
+static double fpt_at(oc_ctx* c, int t, double k, double g4) {
+  const double lnk = log(k);
+  const double l0 = log(c->fpt_kmin), l1 = log(c->fpt_kmax), dl = (l1 - l0) / c->fpt_n;
+  if (lnk < l0 || lnk > l1) return 0.0;
+  const double r = (lnk - l0) / dl;
+  const int i = (int)floor(r);
+  const double* tab = c->fpt + (size_t)t * c->fpt_n;
+  if (i + 1 >= c->fpt_n) return g4 * tab[c->fpt_n - 1];
+  const double w = r - i;
+  return g4 * (w * (tab[i + 1] - tab[i]) + tab[i]);
+}
+
+double oc_fpt_at(oc_ctx* c, int t, double k, double g4) { return fpt_at(c, t, k, g4); }
+
 /* amin_lens / amax_lens: redshift_spline.c:50-79 */
 static double amin_lens(oc_ctx* c, int ni) {
   const double zmax = (c->l_zmax[ni] - c->l_zmean[ni]) * c->lens_st[ni] + c->l_zmean[ni];
@@ -650,11 +699,40 @@ int oc_npairs_gs(oc_ctx* c) { return c->nl * c->ns; }
 /* ---------------------------------------------------------------------- */
 /* Limber integrands                                                       */
 /* ---------------------------------------------------------------------- */
-/* int_for_C_ss_tomo_limber_core, NLA branch: cosmo2D.c:1370-1515 */
+/* int_for_C_ss_tomo_limber_core: cosmo2D.c:1370-1515 (TATT branch
+   :1395-1487, NLA :1488-1504); k = (l + 1/2) / f_K */
 static double ss_core(oc_ctx* c, double a, double fK, double PK, double growfac_a,
                       double dchida, double ell_prefactor, int n1, int n2, double WK1,
-                      double WK2, double WS1, double WS2, int EE) {
+                      double WK2, double WS1, double WS2, int EE, double k) {
   double ans;
+  if (c->ia_model == 1) {
+    const double g4 = growfac_a * growfac_a * growfac_a * growfac_a;
+    double AX[2];
+    IA_A1_Z1Z2(c, a, growfac_a, n1, n2, AX);
+    const double C11 = AX[0], C12 = AX[1];
+    IA_A2_Z1Z2(c, a, growfac_a, n1, n2, AX);
+    const double C21 = AX[0], C22 = AX[1];
+    IA_BTA_Z1Z2(c, n1, n2, AX);
+    const double bta1 = AX[0], bta2 = AX[1];
+    if (1 == EE) {
+      const double tt = fpt_at(c, 0, k, g4), ta_dE1 = fpt_at(c, 2, k, g4), ta_dE2 = fpt_at(c, 3, k, g4);
+      const double ta = fpt_at(c, 4, k, g4), mixA = fpt_at(c, 6, k, g4), mixB = fpt_at(c, 7, k, g4);
+      const double mixEE = fpt_at(c, 8, k, g4);
+      ans = WK1 * WK2 * PK
+          - WS1 * WK2 * (C11 * PK + C11 * bta1 * (ta_dE1 + ta_dE2) - 5 * C21 * (mixA + mixB))
+          - WS2 * WK1 * (C12 * PK + C12 * bta2 * (ta_dE1 + ta_dE2) - 5 * C22 * (mixA + mixB))
+          + WS1 * WS2 * (C11 * C12 * PK
+                         + C11 * C12 * (bta1 * bta2 * ta + (bta1 + bta2) * (ta_dE1 + ta_dE2))
+                         - 5. * (C11 * C22 + C12 * C21) * (mixA + mixB)
+                         - 5. * (C11 * bta1 * C22 + C12 * bta2 * C21) * mixEE
+                         + 25. * C21 * C22 * tt);
+    } else {
+      const double tt = fpt_at(c, 1, k, g4), ta = fpt_at(c, 5, k, g4), mix = fpt_at(c, 9, k, g4);
+      ans = WS1 * WS2 * (C11 * C12 * bta1 * bta2 * ta - 5. * (C11 * bta1 * C22 + C12 * bta2 * C21) * mix +
+                         25. * C21 * C22 * tt);
+    }
+    return ans * (dchida / (fK * fK)) * ell_prefactor;
+  }
   if (1 == EE) {
     double IA[2];
     IA_A1_Z1Z2(c, a, growfac_a, n1, n2, IA);
@@ -679,15 +757,33 @@ static double int_ss(oc_ctx* c, double a, int n1, int n2, double l, int EE) {
   const double ell_prefactor = l * (l - 1.) * (l + 1.) * (l + 2.) / ell4;
   const double WK1 = W_kappa(c, a, fK, n1), WK2 = W_kappa(c, a, fK, n2);
   const double WS1 = W_source(c, a, n1, hh), WS2 = W_source(c, a, n2, hh);
-  return ss_core(c, a, fK, PK, growfac_a, dchida, ell_prefactor, n1, n2, WK1, WK2, WS1, WS2, EE);
+  return ss_core(c, a, fK, PK, growfac_a, dchida, ell_prefactor, n1, n2, WK1, WK2, WS1, WS2, EE, k);
 }
 
-/* int_for_C_gs_tomo_limber_core, NLA branch: cosmo2D.c:1890-2073 */
+/* int_for_C_gs_tomo_limber_core: cosmo2D.c:1890-2073 (TATT branch
+   :1919-2006 and NLA :2007-2065, without RSD and one-loop bias, which are
+   off by default); k = (l + 1/2) / f_K */
 static double gs_core(oc_ctx* c, double a, double fK, double PK, double growfac_a,
                       double dchida, double ell_prefactor, double ell_prefactor2, int nl,
-                      int ns, double WK, double WS, double WGAL, double WMAG) {
+                      int ns, double WK, double WS, double WGAL, double WMAG, double k) {
   const double b1 = c->b1[nl];
   const double bmag = c->bmag[nl];
+  if (c->ia_model == 1) {
+    const double g4 = growfac_a * growfac_a * growfac_a * growfac_a;
+    const double mixA = fpt_at(c, 6, k, g4), mixB = fpt_at(c, 7, k, g4);
+    const double ta_dE1 = fpt_at(c, 2, k, g4), ta_dE2 = fpt_at(c, 3, k, g4);
+    double AX[2];
+    IA_A1_Z1Z2(c, a, growfac_a, ns, ns, AX);
+    const double C1ZS = AX[0];
+    IA_BTA_Z1Z2(c, ns, ns, AX);
+    const double btazs = AX[0];
+    IA_A2_Z1Z2(c, a, growfac_a, ns, ns, AX);
+    const double C2ZS = AX[0];
+    const double ans = WK * ((WGAL * b1 + WMAG * ell_prefactor * bmag + 0.0) * PK + WGAL * 0.0)
+                     - WS * (WGAL * b1 + WMAG * ell_prefactor * bmag) *
+                           (C1ZS * PK + C1ZS * btazs * (ta_dE1 + ta_dE2) - 5 * C2ZS * (mixA + mixB));
+    return ans * (dchida / (fK * fK)) * ell_prefactor2;
+  }
   double IA[2];
   IA_A1_Z1Z2(c, a, growfac_a, ns, ns, IA);
   const double C1ZS = IA[0];
@@ -711,7 +807,7 @@ static double int_gs(oc_ctx* c, double a, int nl, int ns, double l) {
   const double WS = W_source(c, a, ns, hh);
   const double WGAL = W_gal(c, a, nl, hh);
   const double WMAG = W_mag(c, a, fK, nl);
-  return gs_core(c, a, fK, PK, growfac_a, dchida, ell_prefactor, ell_prefactor2, nl, ns, WK, WS, WGAL, WMAG);
+  return gs_core(c, a, fK, PK, growfac_a, dchida, ell_prefactor, ell_prefactor2, nl, ns, WK, WS, WGAL, WMAG, k);
 }
 
 /* int_for_C_gg_tomo_limber (linear bias, RSD on, HOD off): cosmo2D.c:2490-2608 */
@@ -804,7 +900,7 @@ static void gx_lim(oc_ctx* c, double lim[3]) {
 }
 
 /* C_ss_tomo_limber table: cosmo2D.c:1655-1747 */
-static void table_ss(oc_ctx* c, double* tab /* [npairs][nell] */) {
+static void table_ss(oc_ctx* c, double* tab /* [npairs][nell] */, int EE) {
   const int nell = c->n_ell, npts = gl_order_ssgs(c->hdi), ns = c->ns;
   const int np = oc_npairs_ss(c);
   double lim[3];
@@ -848,7 +944,7 @@ static void table_ss(oc_ctx* c, double* tab /* [npairs][nell] */) {
       for (int p = 0; p < npts; p++)
         sum += ss_core(c, A[p], FK[p], PK[(size_t)i * npts + p], GF[p], DC[p], pref[i], z1, z2,
                        W[z1 * npts + p], W[z2 * npts + p], W[(ns + z1) * npts + p],
-                       W[(ns + z2) * npts + p], 1) * WT[p];
+                       W[(ns + z2) * npts + p], EE, (lx[i] + 0.5) / FK[p]) * WT[p];
       tab[(size_t)k * nell + i] = sum;
     }
   free(A); free(WT); free(FK); free(GF); free(HH); free(DC);
@@ -905,7 +1001,8 @@ static void table_gs(oc_ctx* c, double* tab /* [nl*ns][nell] */) {
         double sum = 0.0;
         for (int p = 0; p < npts; p++)
           sum += gs_core(c, A[p], FK[p], PK[(size_t)i * npts + p], GF[p], DC[p], p1[i], p2[i], zl,
-                         zs, WK[zs * npts + p], WS[zs * npts + p], WG[p], WM[p]) * WT[p];
+                         zs, WK[zs * npts + p], WS[zs * npts + p], WG[p], WM[p], (lx[i] + 0.5) / FK[p]) *
+                 WT[p];
         tab[(size_t)(zl * ns + zs) * nell + i] = sum;
       }
     free(PK); free(WK); free(WS);
@@ -944,7 +1041,7 @@ static void fill(const double* tab, const double lim[3], int n, int lmin, int lm
 int oc_cl_tables(oc_ctx* c, double* ss, double* gs, double* gg) {
   c->status = 0;
   if (ensure_g(c)) return -1;
-  if (ss) table_ss(c, ss);
+  if (ss) table_ss(c, ss, 1);
   if (gs && c->nl > 0) table_gs(c, gs);
   if (gg && c->nl > 0) table_gg(c, gg);
   return c->status;
@@ -962,6 +1059,30 @@ int oc_cl_ss_nointerp(oc_ctx* c, const double* l, int nl, double* out) {
       int z1, z2;
       ss_pair(c->ns, k, &z1, &z2);
       out[(size_t)k * nl + i] = C_ss_nointerp(c, &r, l[i], z1, z2, 1);
+    }
+  return c->status;
+}
+
+/* B-mode C_ss (C_ss_tomo_limber(l, ni, nj, EE = 0)): table at the table ells
+   and direct GL at arbitrary ell */
+int oc_cl_tables_bb(oc_ctx* c, double* ss_bb) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  table_ss(c, ss_bb, 0);
+  return c->status;
+}
+int oc_cl_ss_bb_nointerp(oc_ctx* c, const double* l, int nl, double* out) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  gl_rule r;
+  gl_make(&r, gl_order_ssgs(c->hdi));
+  const int np = oc_npairs_ss(c);
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < np; k++)
+    for (int i = 0; i < nl; i++) {
+      int z1, z2;
+      ss_pair(c->ns, k, &z1, &z2);
+      out[(size_t)k * nl + i] = C_ss_nointerp(c, &r, l[i], z1, z2, 0);
     }
   return c->status;
 }
@@ -992,6 +1113,26 @@ int oc_cl_gg_nointerp(oc_ctx* c, const double* l, int nl, double* out) {
 /* Cl arrays exactly as xi_pm_tomo / w_gammat_tomo / w_gg_tomo(limber=1) build
  * them: l=0 -> 0, 1 <= l < LMIN_tab direct GL, LMIN_tab <= l < LMAX fill
  * (cosmo2D.c:468-496, 655-674, 831-849) */
+/* C_ss at every integer ell < LMAX as xi_pm_tomo builds it (EE = 1: E mode,
+   0: B mode, nonzero only under TATT) */
+static void full_ss(oc_ctx* c, const gl_rule* r64, double* ss, int EE) {
+  const int L = c->lmax, nell = c->n_ell, np = oc_npairs_ss(c);
+  double* tab = malloc(sizeof(double) * (size_t)np * nell);
+  table_ss(c, tab, EE);
+  double lim[3];
+  ss_lim(c, lim);
+  #pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < np; k++) {
+    int z1, z2;
+    ss_pair(c->ns, k, &z1, &z2);
+    double* out = ss + (size_t)k * L;
+    out[0] = 0.0;
+    for (int l = 1; l < OC_LMIN_TAB; l++) out[l] = C_ss_nointerp(c, r64, (double)l, z1, z2, EE);
+    fill(tab + (size_t)k * nell, lim, nell, OC_LMIN_TAB, L, out);
+  }
+  free(tab);
+}
+
 int oc_cl_full(oc_ctx* c, double* ss, double* gs, double* gg) {
   c->status = 0;
   if (ensure_g(c)) return -1;
@@ -999,23 +1140,7 @@ int oc_cl_full(oc_ctx* c, double* ss, double* gs, double* gg) {
   gl_rule r64, r96;
   gl_make(&r64, gl_order_ssgs(c->hdi));
   gl_make(&r96, gl_order_gg(c->hdi));
-  if (ss) {
-    const int np = oc_npairs_ss(c);
-    double* tab = malloc(sizeof(double) * (size_t)np * nell);
-    table_ss(c, tab);
-    double lim[3];
-    ss_lim(c, lim);
-    #pragma omp parallel for schedule(dynamic)
-    for (int k = 0; k < np; k++) {
-      int z1, z2;
-      ss_pair(c->ns, k, &z1, &z2);
-      double* out = ss + (size_t)k * L;
-      out[0] = 0.0;
-      for (int l = 1; l < OC_LMIN_TAB; l++) out[l] = C_ss_nointerp(c, &r64, (double)l, z1, z2, 1);
-      fill(tab + (size_t)k * nell, lim, nell, OC_LMIN_TAB, L, out);
-    }
-    free(tab);
-  }
+  if (ss) full_ss(c, &r64, ss, 1);
   if (gs && c->nl > 0) {
     const int np = oc_npairs_gs(c);
     double* tab = malloc(sizeof(double) * (size_t)np * nell);
@@ -1215,6 +1340,22 @@ int oc_xi_pm(oc_ctx* c, double* xip, double* xim) {
   const int np = oc_npairs_ss(c), L = c->lmax;
   double* cl = malloc(sizeof(double) * (size_t)np * L);
   if (oc_cl_full(c, cl, NULL, NULL)) { free(cl); return -1; }
+  if (c->ia_model == 1) {
+    /* xi+- from EE +- BB (cosmo2D.c:517-533); BB vanishes under NLA */
+    double* bb = malloc(sizeof(double) * (size_t)np * L);
+    gl_rule r64;
+    gl_make(&r64, gl_order_ssgs(c->hdi));
+    full_ss(c, &r64, bb, 0);
+    double* cm = malloc(sizeof(double) * (size_t)np * L);
+    for (size_t i = 0; i < (size_t)np * L; i++) {
+      cm[i] = cl[i] - bb[i];
+      cl[i] += bb[i];
+    }
+    contract(c->K, cl, np, c->ntheta, L, xip);
+    contract(c->K + (size_t)c->ntheta * L, cm, np, c->ntheta, L, xim);
+    free(bb); free(cm); free(cl);
+    return c->status;
+  }
   /* BB = 0 under NLA, so EE+BB = EE-BB = EE */
   contract(c->K, cl, np, c->ntheta, L, xip);
   contract(c->K + (size_t)c->ntheta * L, cl, np, c->ntheta, L, xim);
@@ -1399,6 +1540,38 @@ double oc_chi2(const double* dv, const double* data, const double* inv_cov, cons
     chi2 += x * row;
   }
   return chi2;
+}
+
+/* init_IA(IA_MODEL = 1) plus set_nuisance_IA's A2 and b_TA
+   (generic_interface.cpp:588-632, 1701-1774): mode 2 per-bin arrays, mode 3
+   (A2, eta2) and one b_TA */
+int oc_set_ia_tatt(oc_ctx* c, int model, const double* A2_bin, double A2, double eta2, const double* bta_bin,
+                   double bta) {
+  c->ia_model = model;
+  c->ia2_A = A2;
+  c->ia2_eta = eta2;
+  c->bta_A = bta;
+  for (int i = 0; i < OC_MAXB; i++) {
+    c->ia2_bin[i] = A2_bin ? A2_bin[i < c->ns ? i : 0] : 0.0;
+    c->bta_bin[i] = bta_bin ? bta_bin[i < c->ns ? i : 0] : 0.0;
+  }
+  return 0;
+}
+
+/* set_IA_PS (generic_interface.cpp:941-1001): PS[12][N], tables 0..9 in
+   (Mpc/h)^3 -> divided by (c/H0)^3, k range in h/Mpc -> times c/H0 */
+int oc_set_ia_ps(oc_ctx* c, const double* PS, double kmin, double kmax, double cutoff, int N) {
+  (void)cutoff;
+  if (N < 2) { oc_set_err(c, "set_IA_PS: N < 2"); return -1; }
+  free(c->fpt);
+  c->fpt = malloc(sizeof(double) * 10 * (size_t)N);
+  const double h3 = OC_COVERH0 * OC_COVERH0 * OC_COVERH0;
+  for (int t = 0; t < 10; t++)
+    for (int j = 0; j < N; j++) c->fpt[(size_t)t * N + j] = PS[(size_t)t * N + j] / h3;
+  c->fpt_n = N;
+  c->fpt_kmin = kmin * OC_COVERH0;
+  c->fpt_kmax = kmax * OC_COVERH0;
+  return 0;
 }
 
 int oc_set_pointmass(oc_ctx* c, const double* pm, int n) {

@@ -64,6 +64,16 @@ int oc_cl_tables(oc_ctx* c, double* ss, double* gs, double* gg);
 int oc_cl_ss_nointerp(oc_ctx* c, const double* l, int nl, double* out);
 int oc_cl_gs_nointerp(oc_ctx* c, const double* l, int nl, double* out);
 int oc_cl_gg_nointerp(oc_ctx* c, const double* l, int nl, double* out);
+/* B-mode C_ss (nonzero under TATT): table at the table ells, direct GL */
+int oc_cl_tables_bb(oc_ctx* c, double* ss_bb);
+int oc_cl_ss_bb_nointerp(oc_ctx* c, const double* l, int nl, double* out);
+/* TATT: IA model (0 NLA, 1 TATT), A2 and b_TA (per bin for mode 2; value and
+ * exponent / one value for mode 3), and the FAST-PT IA tables of set_IA_PS */
+int oc_set_ia_tatt(oc_ctx* c, int model, const double* A2_bin, double A2, double eta2, const double* bta_bin,
+                   double bta);
+int oc_set_ia_ps(oc_ctx* c, const double* PS, double kmin, double kmax, double cutoff, int N);
+/* one FAST-PT IA table value at k (c/H0 units) times g4, as the integrands read it */
+double oc_fpt_at(oc_ctx* c, int t, double k, double g4);
 /* Fourier-space 3x2pt data vector at the given ells (compute_Mx2pt_N_masked<1,3>) */
 int oc_datavector_fourier(oc_ctx* c, const double* ell, int ncl, double lmax_shear, const int ord[3],
                           const int* mask, double* dv);

@@ -54,6 +54,12 @@ def _load():
         lib.oc_set_pointmass.argtypes = [vp, _D, ctypes.c_int]
         lib.oc_set_mask.restype = ctypes.c_int
         lib.oc_set_mask.argtypes = [vp, _I, ctypes.c_int]
+        for name in ("oc_cl_tables_bb", "oc_cl_ss_bb_nointerp", "oc_set_ia_tatt", "oc_set_ia_ps"):
+            getattr(lib, name).restype = ctypes.c_int
+        lib.oc_set_ia_tatt.argtypes = [vp, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double, _D, ctypes.c_double]
+        lib.oc_set_ia_ps.argtypes = [vp, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        lib.oc_fpt_at.restype = ctypes.c_double
+        lib.oc_fpt_at.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         lib.oc_datavector_fourier.restype = ctypes.c_int
         lib.oc_datavector_fourier.argtypes = [vp, _D, ctypes.c_int, ctypes.c_double,
                                               ctypes.POINTER(ctypes.c_int), _I, _D]
@@ -178,6 +184,9 @@ class Oracle:
                           prob.b1 if prob.lens is not None else None,
                           prob.bmag if prob.lens is not None else None, prob.shear_m)
         self.set_ia(prob.ia_mode, prob.ia_A, prob.ia_eta)
+        if getattr(prob, "ia_model", 0) == 1:
+            self.set_ia_tatt(1, prob.ia2_A, prob.ia2_eta, prob.ia_bta)
+            self.set_ia_ps(prob.ia_ps, prob.ia_ps_kmin, prob.ia_ps_kmax)
         if prob.lens is not None and getattr(prob, "pm", None) is not None:
             self.set_pointmass(prob.pm)
         return self
@@ -193,6 +202,31 @@ class Oracle:
             return
         m = np.ascontiguousarray(np.asarray(mask, dtype=np.int32))
         self._chk(self.lib.oc_set_mask(self.c, m.ctypes.data_as(_I), len(m)))
+
+    def set_ia_tatt(self, model=1, A2=0.0, eta2=0.0, bta=0.0, A2_bin=None, bta_bin=None):
+        a2b = None if A2_bin is None else _arr(A2_bin)
+        btb = None if bta_bin is None else _arr(bta_bin)
+        self._chk(self.lib.oc_set_ia_tatt(self.c, int(model), _p(a2b), float(A2), float(eta2), _p(btb),
+                                          float(bta)))
+
+    def set_ia_ps(self, PS, kmin, kmax, cutoff=1.0e4):
+        PS = _arr(PS)
+        n = PS.size // 12
+        self._chk(self.lib.oc_set_ia_ps(self.c, _p(PS), float(kmin), float(kmax), float(cutoff), n))
+
+    def fpt_at(self, t, k, g4=1.0):
+        return self.lib.oc_fpt_at(self.c, int(t), float(k), float(g4))
+
+    def cl_tables_bb(self):
+        out = np.empty((self.n_ss, self.n_ell))
+        self._chk(self.lib.oc_cl_tables_bb(self.c, _p(out)))
+        return out
+
+    def cl_ss_bb_nointerp(self, ells):
+        ells = _arr(ells)
+        out = np.empty((self.n_ss, len(ells)))
+        self._chk(self.lib.oc_cl_ss_bb_nointerp(self.c, _p(ells), len(ells), _p(out)))
+        return out
 
     def set_kernel_precision(self, extended: bool):
         self.lib.oc_set_kernel_precision(self.c, 1 if extended else 0)

@@ -549,3 +549,64 @@ def test_fourier_datavector_pins(p3):
             o.datavector_fourier(ells, 1e9, bad)
         with pytest.raises(O.OracleError):
             o.datavector(bad)
+
+
+# ----------------------------------------------------------------- TATT IA
+def _tatt(A2=-0.8, eta2=0.5, bta=0.7, A1=0.5):
+    p = synth.tatt_problem(A2=A2, eta2=eta2, bta=bta)
+    p.ia_A = A1
+    return p
+
+
+def test_tatt_reduces_to_nla():
+    """A2 = 0 and b_TA = 0 turn every TATT term off (cosmo2D.c:1452-1486,
+    :2000-2004): E mode and gs equal NLA, B mode vanishes."""
+    nla = O.Oracle().load(synth.full_3x2pt_problem())
+    t = O.Oracle().load(_tatt(A2=0.0, bta=0.0))
+    e0, g0, w0 = nla.cl_tables()
+    e1, g1, w1 = t.cl_tables()
+    # same terms, summed in another order where lensing and IA nearly cancel:
+    # equal to rounding, measured against each row's largest value
+    for x1, x0 in ((e1, e0), (g1, g0)):
+        assert np.all(np.abs(x1 - x0) <= 1e-13 * np.abs(x0).max(axis=1, keepdims=True))
+    assert np.array_equal(w1, w0)
+    assert np.all(t.cl_tables_bb() == 0.0)
+    d1, d0 = t.datavector(), nla.datavector()
+    assert np.abs(d1 - d0).max() <= 1e-12 * np.abs(d0).max()
+
+
+def test_tatt_b_mode_scaling():
+    """With A1 = 0 the B mode is 25 C21 C22 tt_BB: quadratic in A2."""
+    b1 = O.Oracle().load(_tatt(A2=-0.8, A1=0.0)).cl_tables_bb()
+    b2 = O.Oracle().load(_tatt(A2=-1.6, A1=0.0)).cl_tables_bb()
+    assert np.abs(b1).max() > 0
+    assert np.allclose(b2, 4.0 * b1, rtol=1e-12, atol=0)
+
+
+def test_tatt_e_mode_quadratic_in_bta():
+    """E mode: b_TA enters linearly and as b_TA^2 (one value for all bins in
+    redshift-evolution mode), so third differences in b_TA vanish."""
+    e = [O.Oracle().load(_tatt(bta=b)).cl_tables()[0] for b in (0.0, 1.0, 2.0, 3.0)]
+    d3 = e[3] - 3 * e[2] + 3 * e[1] - e[0]
+    assert np.abs(d3).max() <= 1e-10 * np.abs(e[0]).max()
+    assert np.abs(e[1] - e[0]).max() > 1e-6 * np.abs(e[0]).max()
+
+
+def test_fpt_table_lookup():
+    """FAST-PT IA tables (set_IA_PS units: /(c/H0)^3, k x c/H0), read on the
+    uniform ln k grid of step (ln kmax - ln kmin)/N: exact at grid points,
+    linear in ln k between them, zero outside [kmin, kmax]."""
+    p = _tatt()
+    o = O.Oracle().load(p)
+    n = p.ia_ps.size // 12
+    tab = p.ia_ps.reshape(12, n)
+    h3 = COVERH0 ** 3
+    for t in (0, 3, 9):
+        for i in (0, 17, 500, n - 2):
+            k = tab[10][i] * COVERH0
+            assert np.isclose(o.fpt_at(t, k * (1 + 1e-13), 2.0), 2.0 * tab[t][i] / h3, rtol=1e-9)
+        i = 300
+        km = np.sqrt(tab[10][i] * tab[10][i + 1]) * COVERH0
+        assert np.isclose(o.fpt_at(t, km), 0.5 * (tab[t][i] + tab[t][i + 1]) / h3, rtol=1e-9)
+    assert o.fpt_at(0, p.ia_ps_kmin * COVERH0 * 0.99) == 0.0
+    assert o.fpt_at(0, p.ia_ps_kmax * COVERH0 * 1.01) == 0.0
