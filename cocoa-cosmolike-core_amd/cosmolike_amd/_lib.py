@@ -30,6 +30,9 @@ class clh_draw(ctypes.Structure):
         ("b1", D), ("bmag", D), ("shear_m", D),
         ("ia_A", ctypes.c_double), ("ia_eta", ctypes.c_double), ("ia_bin", D),
         ("pm", D),
+        ("ia2_A", ctypes.c_double), ("ia2_eta", ctypes.c_double), ("ia2_bin", D),
+        ("bta_A", ctypes.c_double), ("bta_bin", D),
+        ("fpt", D), ("fpt_n", ctypes.c_int), ("fpt_kmin", ctypes.c_double), ("fpt_kmax", ctypes.c_double),
     ]
 
 
@@ -74,6 +77,8 @@ def lib():
             "clh_set_nuisance_linear_bias": (i32, [VP, D, i32]),
             "clh_set_nuisance_magnification_bias": (i32, [VP, D, i32]),
             "clh_set_nuisance_ia": (i32, [VP, D, i32]),
+            "clh_set_nuisance_ia_tatt": (i32, [VP, D, D, D, i32]),
+            "clh_set_IA_PS": (i32, [VP, D, f64, f64, f64, i32]),
             "clh_set_nuisance_pointmass": (i32, [VP, D, i32]),
             "clh_set_vjp_checkpoint": (i32, [VP, i32]),
             "clh_init_binning_fourier": (i32, [VP, i32, i32, i32, i32]),
@@ -90,6 +95,7 @@ def lib():
             "clh_gammat": (i32, [VP, D]),
             "clh_wtheta": (i32, [VP, D]),
             "clh_cl_tables": (i32, [VP, D, D, D]),
+            "clh_cl_tables_bb": (i32, [VP, D]),
             "clh_cl_nointerp": (i32, [VP, i32, D, i32, D]),
             "clh_batch_upload": (i32, [VP, i32, ctypes.POINTER(clh_draw)]),
             "clh_layout_capture": (i32, [VP, ctypes.POINTER(VP)]),

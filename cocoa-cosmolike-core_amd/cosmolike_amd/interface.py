@@ -159,14 +159,31 @@ class Cosmolike:
     def set_nuisance_magnification_bias(self, bmag):
         self._setv(self.L.clh_set_nuisance_magnification_bias, bmag)
 
-    def set_nuisance_IA(self, A1):
-        self._setv(self.L.clh_set_nuisance_ia, A1)
+    def set_nuisance_IA(self, A1, A2=None, BTA=None):
+        """set_nuisance_IA: A1 only (NLA), or A1, A2 and b_TA (TATT); mode 2
+        one value per source bin in each, mode 3 (A, eta), (A2, eta2), (b_TA,)."""
+        if A2 is None and BTA is None:
+            self._setv(self.L.clh_set_nuisance_ia, A1)
+            return
+        A1, A2, BTA = _arr(A1), _arr(A2), _arr(BTA)
+        n = min(len(A1), len(A2))
+        BTA = np.resize(BTA, n)
+        self._chk(self.L.clh_set_nuisance_ia_tatt(self.h, _p(A1), _p(A2), _p(BTA), n))
+
+    def set_IA_PS(self, PS, kmin, kmax, cutoff=1.0e4):
+        """set_IA_PS: the FAST-PT IA spectra PS[12][N] on N log-spaced k in
+        [kmin, kmax) h/Mpc."""
+        PS = _arr(PS)
+        n = PS.size // 12
+        if PS.size != 12 * n:
+            raise ValueError("PS must hold 12 rows of N entries")
+        self._chk(self.L.clh_set_IA_PS(self.h, _p(PS), float(kmin), float(kmax), float(cutoff), n))
 
     def load(self, prob, legendre_mode=LEGENDRE_FOLDED):
         """Configure the layout and current draw from a synth.Problem."""
         self.init_ntable(prob.n_ell, 225, prob.lmax, 0)
         self.init_binning_real_space(prob.ntheta, prob.theta_min_arcmin, prob.theta_max_arcmin)
-        self.init_IA(0, prob.ia_mode)
+        self.init_IA(getattr(prob, "ia_model", 0), prob.ia_mode)
         self.set_legendre_mode(legendre_mode)
         self.set_source_sample(prob.source)
         if prob.lens is not None:
@@ -188,7 +205,12 @@ class Cosmolike:
             self.set_nuisance_magnification_bias(prob.bmag)
             if getattr(prob, "pm", None) is not None:
                 self.set_nuisance_pointmass(prob.pm)
-        if prob.ia_mode == 3:
+        tatt = getattr(prob, "ia_model", 0) == 1
+        if tatt:
+            self.set_IA_PS(prob.ia_ps, prob.ia_ps_kmin, prob.ia_ps_kmax)
+        if prob.ia_mode == 3 and tatt:
+            self.set_nuisance_IA([prob.ia_A, prob.ia_eta], [prob.ia2_A, prob.ia2_eta], [prob.ia_bta])
+        elif prob.ia_mode == 3:
             self.set_nuisance_IA([prob.ia_A, prob.ia_eta])
 
     def init_binning_fourier(self, nells, lmin, lmax, lmax_shear):
@@ -293,10 +315,17 @@ class Cosmolike:
         self._chk(self.L.clh_cl_tables(self.h, _p(ss), _p(gs), _p(gg)))
         return ss, gs[: self.n_gs], gg[: self.nl]
 
+    def cl_tables_bb(self, n_ell):
+        """B-mode C_ss table [n_ss][N_ell] (zero under NLA)."""
+        bb = np.empty((self.n_ss, n_ell))
+        self._chk(self.L.clh_cl_tables_bb(self.h, _p(bb)))
+        return bb
+
     def cl_nointerp(self, probe, ells):
+        """probe "ss" (E mode), "gs", "gg" or "ss_bb" (B mode)."""
         ells = _arr(ells)
-        idx = {"ss": 0, "gs": 1, "gg": 2}[probe]
-        n = {"ss": self.n_ss, "gs": self.n_gs, "gg": self.nl}[probe]
+        idx = {"ss": 0, "gs": 1, "gg": 2, "ss_bb": 3}[probe]
+        n = {"ss": self.n_ss, "gs": self.n_gs, "gg": self.nl, "ss_bb": self.n_ss}[probe]
         out = np.empty((n, len(ells)))
         self._chk(self.L.clh_cl_nointerp(self.h, idx, _p(ells), len(ells), _p(out)))
         return out
@@ -324,6 +353,11 @@ class Cosmolike:
             d.b1, d.bmag = _p(vals["b1"]), _p(vals["bmag"])
             d.pm = _p(vals["pm"])
             d.ia_A, d.ia_eta = p.ia_A, p.ia_eta
+            if getattr(p, "ia_model", 0) == 1:
+                vals["fpt"] = _arr(p.ia_ps)
+                d.ia2_A, d.ia2_eta, d.bta_A = p.ia2_A, p.ia2_eta, p.ia_bta
+                d.fpt, d.fpt_n = _p(vals["fpt"]), vals["fpt"].size // 12
+                d.fpt_kmin, d.fpt_kmax = p.ia_ps_kmin, p.ia_ps_kmax
         return arr, keep
 
     def batch_upload(self, problems):

@@ -145,6 +145,35 @@ __device__ __forceinline__ double clh_ia_c1(const DrawDesc& d, double a, double 
   return A * (d.omega_m * 0.01389 / growfac_a);
 }
 
+// IA_A2_Z1 (IA.c:305-368): A2 Omega_m c1rhocrit / D^2; redshift evolution
+// A2 (1 / (a (1+z0)))^eta2
+__device__ __forceinline__ double clh_ia_c2(const DrawDesc& d, double a, double growfac_a, int b) {
+  double A = 0.0;
+  if (d.ia_mode == 2) A = d.par->ia2_bin[b];
+  else if (d.ia_mode == 3) A = d.par->ia2_A * pow(1.0 / (a * 1.62), d.par->ia2_eta);
+  return A * (d.omega_m * 0.01389 / (growfac_a * growfac_a));
+}
+
+// IA_BTA_Z1 (IA.c:370-420): b_TA per bin or one value; 1 without IA
+__device__ __forceinline__ double clh_ia_bta(const DrawDesc& d, int b) {
+  if (d.ia_mode == 2) return d.par->bta_bin[b];
+  if (d.ia_mode == 3) return d.par->bta_A;
+  return 1.0;
+}
+
+// FAST-PT IA table t at ln k (cosmo2D.c:1414-1447): uniform in ln k with the
+// step (ln kmax - ln kmin) / N, zero outside the range, the last entry at its
+// top end; linear between entries
+__device__ __forceinline__ double clh_fpt(const DrawDesc& d, int t, double lnk) {
+  if (lnk < d.fpt_l0 || lnk > d.fpt_l1) return 0.0;
+  const double r = (lnk - d.fpt_l0) / d.fpt_dl;
+  const int i = (int)floor(r);
+  const double* __restrict__ tab = d.fpt + (size_t)t * d.fpt_n;
+  if (i + 1 >= d.fpt_n) return tab[d.fpt_n - 1];
+  const double w = r - i;
+  return w * (tab[i + 1] - tab[i]) + tab[i];
+}
+
 // W_RSD: radial_weights.c:87-105 (hoverh0 via dchi_dz, f via f_growth)
 __device__ __forceinline__ double clh_w_rsd(const DrawDesc& d, double l, double a0, double a1,
                                             int ni) {

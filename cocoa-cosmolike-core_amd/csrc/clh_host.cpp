@@ -23,6 +23,7 @@
 hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, int keep, hipStream_t s);
+hipError_t clh_launch_limber_sx_tatt(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles, int keep,
                                 hipStream_t s);
 hipError_t clh_launch_legendre(const DrawDesc* D, const int4* work, int n, hipStream_t s);
@@ -171,6 +172,12 @@ struct Draw {  // one draw's cosmology + nuisance (host copy)
   double ia_bin[CLH_MAXB] = {0};
   double pm[CLH_MAXB] = {0};
   double ia_A = 0.0, ia_eta = 0.0;
+  // TATT: A2 and b_TA (per bin, or value / exponent), FAST-PT IA tables
+  double ia2_bin[CLH_MAXB] = {0}, bta_bin[CLH_MAXB] = {0};
+  double ia2_A = 0.0, ia2_eta = 0.0, bta_A = 0.0;
+  dvec fpt;  // [10][fpt_n], already divided by (c/H0)^3
+  int fpt_n = 0;
+  double fpt_kmin = 0.0, fpt_kmax = 0.0;  // in H0/c
   Draw() { for (int i = 0; i < CLH_MAXB; i++) lens_st[i] = 1.0; }
 };
 
@@ -246,6 +253,7 @@ struct Batch {
   DBuf<double> d_ggk, d_sxk, d_dgk;  // dgk: lens-efficiency shift tangents
   int dv_size = 0;
   int nknz = 0;        // common ln P table size (0 if the draws differ)
+  bool tatt = false;   // some draw uses TATT intrinsic alignments
   int ord[3] = {0, 1, 2};
   void release() {
     d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
@@ -272,6 +280,7 @@ struct clh_ctx {
   uint64_t version = 1, cached_version = 0;
   dvec cache_raw;   // raw (uncalibrated, ord 0,1,2) data vector of `cur`
   dvec cache_cl;    // cl rows of `cur`
+  dvec cache_bb;    // B-mode C_ss rows of `cur` (zero under NLA)
   // device copies of the layout
   DevLayout dl;                                        // device copy of L
   std::map<uint64_t, std::unique_ptr<LayoutCopy>> rl;  // layouts of a multi-layout batch
@@ -583,6 +592,8 @@ static int check_draw(clh_ctx* c, const Layout& L, const Draw& d, int idx) {
   if (d.lk.size() < 2 || d.pz.size() < 2 || d.lnP.size() != d.lk.size() * d.pz.size())
     return fail(c, CLH_ERR_STATE, "draw %d: P(k,z) not set", idx);
   if (!(d.omega_m > 0)) return fail(c, CLH_ERR_STATE, "draw %d: Omega_m not set", idx);
+  if (L.ia_model == CLH_IA_TATT && (d.fpt_n < 2 || d.fpt.size() != (size_t)10 * d.fpt_n))
+    return fail(c, CLH_ERR_STATE, "draw %d: TATT needs the FAST-PT IA tables (set_IA_PS)", idx);
   for (int i = 1; i < nc; i++)
     if (!(d.chi_z[i] > d.chi_z[i - 1]) || !(d.chi[i] > d.chi[i - 1]))
       return fail(c, CLH_ERR_DOMAIN, "draw %d: chi(z) table not strictly increasing", idx);
@@ -922,9 +933,12 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     const int ns = L.src.nbin, nl = L.lens.nbin;
     const int nrow = ells ? (int)ells->size() : L.nrow();
     const size_t node_sz = node_size(ns, nl);
+    const bool tatt = L.ia_model == CLH_IA_TATT;
     woff[i + 1] = woff[i] + al((size_t)(ns + nl) * L.n_a) + al(node_sz) +
-                  al((size_t)(L.n_ss() + L.n_gs() + nl) * nrow);
-    toff[i + 1] = toff[i] + 2 * d.chi_z.size() + 2 * d.G_z.size() + d.lk.size() + d.pz.size() + d.lnP.size() + 32;
+                  al((size_t)(L.n_ss() + L.n_gs() + nl) * nrow) +
+                  (tatt ? al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS) + al((size_t)L.n_ss() * nrow) : 0);
+    toff[i + 1] = toff[i] + 2 * d.chi_z.size() + 2 * d.G_z.size() + d.lk.size() + d.pz.size() + d.lnP.size() +
+                  (tatt ? d.fpt.size() : 0) + 32;
     dvoff[i + 1] = dvoff[i] + L.dv_size();
     max_nrow = std::max(max_nrow, nrow);
     if (lays[i] != lays[0]) uniform = false;
@@ -955,6 +969,13 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.chi_z = push(d.chi_z); q.chi_v = push(d.chi);
     q.G_z = push(d.G_z); q.G_v = push(d.G);
     q.lk = push(d.lk); q.pz = push(d.pz); q.lnP = push(d.lnP);
+    if (L.ia_model == CLH_IA_TATT) {
+      q.fpt = push(d.fpt);
+      q.fpt_n = d.fpt_n;
+      q.fpt_l0 = std::log(d.fpt_kmin);
+      q.fpt_l1 = std::log(d.fpt_kmax);
+      q.fpt_dl = (q.fpt_l1 - q.fpt_l0) / d.fpt_n;
+    }
     q.chi_n = (int)d.chi_z.size(); q.G_n = (int)d.G_z.size();
     q.nk = (int)d.lk.size(); q.nzp = (int)d.pz.size();
     q.omega_m = d.omega_m;
@@ -972,6 +993,9 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
       pp.pm[b] = d.pm[b];
     }
     pp.ia_A = d.ia_A; pp.ia_eta = d.ia_eta;
+    for (int b = 0; b < CLH_MAXB; b++) { pp.ia2_bin[b] = d.ia2_bin[b]; pp.bta_bin[b] = d.bta_bin[b]; }
+    pp.ia2_A = d.ia2_A; pp.ia2_eta = d.ia2_eta; pp.bta_A = d.bta_A;
+    q.ia_model = L.ia_model;
     q.par = bt.d_par.p + i;
     q.ia_mode = L.ia_mode;
     q.ell_ss = ells ? bt.d_ells.p : lay + DL.off_ell_ss;
@@ -991,6 +1015,10 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.gtab = wb;
     q.nodes = wb + al(gtab_sz);
     q.cl = q.nodes + al(node_sz);
+    if (L.ia_model == CLH_IA_TATT) {
+      q.tatt = q.cl + al((size_t)(L.n_ss() + L.n_gs() + nl) * nrow);
+      q.cl_bb = q.tatt + al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS);
+    }
     q.dv = dv_out ? dv_out + dvoff[i] : bt.d_dv.p + dvoff[i];
     dv_starts(L, ord, q.dv_start);
     q.node_ss = 0;
@@ -1024,6 +1052,8 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   bt.dv_size = lays[0]->dv_size();
   bt.nrow_custom = ells ? (int)ells->size() : 0;
   bt.state_valid = false;
+  bt.tatt = false;
+  for (int i = 0; i < B; i++) bt.tatt = bt.tatt || lays[i]->ia_model == CLH_IA_TATT;
   {  // the gg kernels stage chi(z), G(z) and one lens n(z) spline in LDS when
      // the largest draw's tables fit in 44 KB (64 KB per block with the static LDS)
     size_t st = 0;
@@ -1068,7 +1098,10 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, mixed, c->gg_lds[0] ? bt.gg_stage : 0,
                                  bt.keep_ckpt && bt.ggk_per ? 1 : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], s));
-  HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, mixed, bt.keep_ckpt ? 1 : 0, s));
+  if (bt.tatt)  // TATT ss/gs tiles: FP64 in every precision mode
+    HIPCHK(c, clh_launch_limber_sx_tatt(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
+  else
+    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, mixed, bt.keep_ckpt ? 1 : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
   if (legendre) {
     if (c->legendre_mode == CLH_LEGENDRE_FULL)
@@ -1099,6 +1132,10 @@ static int ensure_current(clh_ctx* c) {
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->cache_cl.data(), c->single.desc[0].cl, ncl * sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
+  c->cache_bb.assign((size_t)c->L.n_ss() * c->L.nrow(), 0.0);
+  if (c->single.desc[0].cl_bb)
+    HIPCHK(c, hipMemcpyAsync(c->cache_bb.data(), c->single.desc[0].cl_bb, c->cache_bb.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->cached_version = c->version;
   return CLH_OK;
@@ -1201,7 +1238,8 @@ int clh_init_binning_real_space(clh_ctx* c, int ntheta, double tmin, double tmax
 
 int clh_init_ia(clh_ctx* c, int model, int evol) {
   if (!c) return CLH_ERR_ARG;
-  if (model != 0) return fail(c, CLH_ERR_ARG, "IA_MODEL=%d not implemented (NLA=0 only)", model);
+  if (model != 0 && model != CLH_IA_TATT)
+    return fail(c, CLH_ERR_ARG, "IA_MODEL=%d not implemented (NLA=0, TATT=1)", model);
   if (evol != 0 && evol != 2 && evol != 3) return fail(c, CLH_ERR_ARG, "IA redshift evolution %d not supported", evol);
   c->L.ia_model = model;
   c->L.ia_mode = evol;
@@ -1305,6 +1343,51 @@ int clh_set_nuisance_ia(clh_ctx* c, const double* A1, int n) {
     if (n < c->L.src.nbin) return fail(c, CLH_ERR_ARG, "set_nuisance_IA: need one A per source bin");
     for (int i = 0; i < c->L.src.nbin; i++) c->cur.ia_bin[i] = A1[i];
   }
+  c->version++;
+  return CLH_OK;
+}
+
+// set_nuisance_IA(A1, A2, BTA) (generic_interface.cpp:1701-1774): mode 2 one
+// value per source bin in each array; mode 3 A1 = (A, eta), A2 = (A2, eta2),
+// BTA[0] = b_TA
+int clh_set_nuisance_ia_tatt(clh_ctx* c, const double* A1, const double* A2, const double* BTA, int n) {
+  if (!c) return CLH_ERR_ARG;
+  if (!A1 || !A2 || !BTA || n <= 0) return fail(c, CLH_ERR_ARG, "set_nuisance_IA: empty");
+  if (has_nan(A1, n) || has_nan(A2, n) || has_nan(BTA, n)) return fail(c, CLH_ERR_NAN, "set_nuisance_IA: NaN");
+  if (c->L.ia_mode == 3) {
+    if (n < 2) return fail(c, CLH_ERR_ARG, "set_nuisance_IA: need (A, eta), (A2, eta2), b_TA");
+    c->cur.ia_A = A1[0];
+    c->cur.ia_eta = A1[1];
+    c->cur.ia2_A = A2[0];
+    c->cur.ia2_eta = A2[1];
+    c->cur.bta_A = BTA[0];
+  } else if (c->L.ia_mode == 2) {
+    if (n < c->L.src.nbin) return fail(c, CLH_ERR_ARG, "set_nuisance_IA: need one value per source bin");
+    for (int i = 0; i < c->L.src.nbin; i++) {
+      c->cur.ia_bin[i] = A1[i];
+      c->cur.ia2_bin[i] = A2[i];
+      c->cur.bta_bin[i] = BTA[i];
+    }
+  }
+  c->version++;
+  return CLH_OK;
+}
+
+// set_IA_PS (generic_interface.cpp:941-1001): PS[12][N], rows 0..9 the
+// FAST-PT IA spectra in (Mpc/h)^3, k range in h/Mpc; stored as the reference
+// stores them (spectra / (c/H0)^3, k * c/H0)
+int clh_set_IA_PS(clh_ctx* c, const double* PS, double kmin, double kmax, double cutoff, int N) {
+  if (!c) return CLH_ERR_ARG;
+  (void)cutoff;
+  if (!PS || N < 2) return fail(c, CLH_ERR_ARG, "set_IA_PS: N=%d", N);
+  if (!(kmin > 0) || !(kmax > kmin)) return fail(c, CLH_ERR_ARG, "set_IA_PS: bad k range");
+  if (has_nan(PS, 12 * N)) return fail(c, CLH_ERR_NAN, "set_IA_PS: NaN");
+  const double h3 = CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0;
+  c->cur.fpt.resize((size_t)10 * N);
+  for (size_t i = 0; i < (size_t)10 * N; i++) c->cur.fpt[i] = PS[i] / h3;
+  c->cur.fpt_n = N;
+  c->cur.fpt_kmin = kmin * CLH_COVERH0;
+  c->cur.fpt_kmax = kmax * CLH_COVERH0;
   c->version++;
   return CLH_OK;
 }
@@ -1583,10 +1666,23 @@ int clh_cl_tables(clh_ctx* c, double* ss, double* gs, double* gg) {
   return CLH_OK;
 }
 
+// C_ss_tomo_limber(l, ni, nj, EE = 0) tables (cosmo2D.c:1617): the B mode,
+// zero under NLA
+int clh_cl_tables_bb(clh_ctx* c, double* bb) {
+  if (!c || !bb) return CLH_ERR_ARG;
+  int rc = ensure_current(c);
+  if (rc) return rc;
+  const int nrow = c->L.nrow(), ne = c->L.n_ell;
+  for (int q = 0; q < c->L.n_ss(); q++)
+    std::copy(c->cache_bb.begin() + (size_t)q * nrow + CLH_NLOW, c->cache_bb.begin() + (size_t)(q + 1) * nrow,
+              bb + (size_t)q * ne);
+  return CLH_OK;
+}
+
 int clh_cl_nointerp(clh_ctx* c, int probe, const double* ell, int nl_, double* out) {
   if (!c || !ell || !out || nl_ <= 0) return CLH_ERR_ARG;
-  if (probe < 0 || probe > 2) return fail(c, CLH_ERR_ARG, "probe %d", probe);
-  if (probe > 0 && c->L.lens.nbin == 0) return fail(c, CLH_ERR_STATE, "lens sample not set");
+  if (probe < 0 || probe > 3) return fail(c, CLH_ERR_ARG, "probe %d", probe);
+  if ((probe == 1 || probe == 2) && c->L.lens.nbin == 0) return fail(c, CLH_ERR_STATE, "lens sample not set");
   if (has_nan(ell, nl_)) return fail(c, CLH_ERR_NAN, "ell: NaN");
   dvec ells(ell, ell + nl_);
   const int ord[3] = {0, 1, 2};
@@ -1595,11 +1691,35 @@ int clh_cl_nointerp(clh_ctx* c, int probe, const double* ell, int nl_, double* o
   if (rc) return rc;
   if ((rc = run_batch(c, c->scratch, false))) return rc;
   const int nss = c->L.n_ss(), ngs = c->L.n_gs(), nl = c->L.lens.nbin;
+  if (probe == 3) {  // B mode (EE = 0)
+    const double* bb = c->scratch.desc[0].cl_bb;
+    if (bb)
+      HIPCHK(c, hipMemcpyAsync(out, bb, (size_t)nss * nl_ * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    else
+      std::fill(out, out + (size_t)nss * nl_, 0.0);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLH_OK;
+  }
   const int col0 = probe == 0 ? 0 : probe == 1 ? nss : nss + ngs;
   const int ncol = probe == 0 ? nss : probe == 1 ? ngs : nl;
   HIPCHK(c, hipMemcpyAsync(out, c->scratch.desc[0].cl + (size_t)col0 * nl_, (size_t)ncol * nl_ * sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLH_OK;
+}
+
+// the table sizes, NaNs and required arrays of one uploaded draw
+static int check_upload(clh_ctx* c, const clh_draw& s, const Layout& L, int i) {
+  if (s.chi_n < 5 || s.G_n < 2 || s.nk < 2 || s.npz < 2) return fail(c, CLH_ERR_ARG, "draw %d: table sizes", i);
+  if (has_nan(s.chi_z, s.chi_n) || has_nan(s.chi, s.chi_n) || has_nan(s.G_z, s.G_n) || has_nan(s.G, s.G_n) ||
+      has_nan(s.log10k, s.nk) || has_nan(s.pk_z, s.npz) || has_nan(s.lnP, s.nk * s.npz))
+    return fail(c, CLH_ERR_NAN, "draw %d: NaN in a table", i);
+  if (L.lens.nbin > 0 && !s.b1) return fail(c, CLH_ERR_ARG, "draw %d: b1 missing", i);
+  if (L.ia_model == CLH_IA_TATT) {
+    if (!s.fpt || s.fpt_n < 2 || !(s.fpt_kmin > 0) || !(s.fpt_kmax > s.fpt_kmin))
+      return fail(c, CLH_ERR_ARG, "draw %d: TATT needs the FAST-PT IA tables", i);
+    if (has_nan(s.fpt, 10 * s.fpt_n)) return fail(c, CLH_ERR_NAN, "draw %d: NaN in the IA tables", i);
+  }
   return CLH_OK;
 }
 
@@ -1626,6 +1746,23 @@ static void fill_draw(Draw& d, const clh_draw& s, const Layout& L) {
   }
   d.ia_A = s.ia_A;
   d.ia_eta = s.ia_eta;
+  if (L.ia_model == CLH_IA_TATT) {
+    for (int i = 0; i < L.src.nbin; i++) {
+      d.ia2_bin[i] = s.ia2_bin ? s.ia2_bin[i] : 0.0;
+      d.bta_bin[i] = s.bta_bin ? s.bta_bin[i] : 0.0;
+    }
+    d.ia2_A = s.ia2_A;
+    d.ia2_eta = s.ia2_eta;
+    d.bta_A = s.bta_A;
+    if (s.fpt && s.fpt_n >= 2) {
+      const double h3 = CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0;
+      d.fpt.resize((size_t)10 * s.fpt_n);
+      for (size_t i = 0; i < d.fpt.size(); i++) d.fpt[i] = s.fpt[i] / h3;
+      d.fpt_n = s.fpt_n;
+      d.fpt_kmin = s.fpt_kmin * CLH_COVERH0;
+      d.fpt_kmax = s.fpt_kmax * CLH_COVERH0;
+    }
+  }
 }
 
 int clh_batch_upload(clh_ctx* c, int B, const clh_draw* draws) {
@@ -1634,11 +1771,8 @@ int clh_batch_upload(clh_ctx* c, int B, const clh_draw* draws) {
   std::vector<const Draw*> ptr(B);
   for (int i = 0; i < B; i++) {
     const clh_draw& s = draws[i];
-    if (s.chi_n < 5 || s.G_n < 2 || s.nk < 2 || s.npz < 2) return fail(c, CLH_ERR_ARG, "draw %d: table sizes", i);
-    if (has_nan(s.chi_z, s.chi_n) || has_nan(s.chi, s.chi_n) || has_nan(s.G_z, s.G_n) || has_nan(s.G, s.G_n) ||
-        has_nan(s.log10k, s.nk) || has_nan(s.pk_z, s.npz) || has_nan(s.lnP, s.nk * s.npz))
-      return fail(c, CLH_ERR_NAN, "draw %d: NaN in a table", i);
-    if (c->L.lens.nbin > 0 && !s.b1) return fail(c, CLH_ERR_ARG, "draw %d: b1 missing", i);
+    const int rc = check_upload(c, s, c->L, i);
+    if (rc) return rc;
     fill_draw(ds[i], s, c->L);
     ptr[i] = &ds[i];
   }
@@ -1690,11 +1824,8 @@ int clh_batch_upload_layouts(clh_ctx* c, int B, const clh_draw* draws, const clh
     const clh_draw& s = draws[i];
     if (!layouts[i]) return fail(c, CLH_ERR_ARG, "draw %d: no layout", i);
     const Layout& L = layouts[i]->L;
-    if (s.chi_n < 5 || s.G_n < 2 || s.nk < 2 || s.npz < 2) return fail(c, CLH_ERR_ARG, "draw %d: table sizes", i);
-    if (has_nan(s.chi_z, s.chi_n) || has_nan(s.chi, s.chi_n) || has_nan(s.G_z, s.G_n) || has_nan(s.G, s.G_n) ||
-        has_nan(s.log10k, s.nk) || has_nan(s.pk_z, s.npz) || has_nan(s.lnP, s.nk * s.npz))
-      return fail(c, CLH_ERR_NAN, "draw %d: NaN in a table", i);
-    if (L.lens.nbin > 0 && !s.b1) return fail(c, CLH_ERR_ARG, "draw %d: b1 missing", i);
+    const int rc = check_upload(c, s, L, i);
+    if (rc) return rc;
     fill_draw(ds[i], s, L);
     ptr[i] = &ds[i];
   }
@@ -1750,7 +1881,7 @@ long long clh_batch_datavector_offset(clh_ctx* c, int i) {
 // the context asks for it and the batch has one layout.  `changed`: some
 // descriptor changed on the host and must be uploaded.
 static int prepare_checkpoint(clh_ctx* c, Batch& cb, bool& changed) {
-  cb.keep_ckpt = c->vjp_ckpt && cb.uniform && !cb.nrow_custom;
+  cb.keep_ckpt = c->vjp_ckpt && cb.uniform && !cb.nrow_custom && !cb.tatt;
   if (cb.keep_ckpt) {
     const size_t nl = cb.lays[0]->lens.nbin, nr = cb.lays[0]->nrow();
     cb.ggk_per = (nl * nr * 3 * CLH_NP_GG + 31) & ~size_t(31);
@@ -1924,7 +2055,7 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   if (rc0) return rc0;
   for (int i = 0; i < bt.B; i++) bt.desc[i].dv = bt.d_dv.p + bt.dv_off[i];
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
-  const bool adj = bt.uniform && bt.nknz > 0;
+  const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt;  // the VJP refuses TATT
   DBuf<double> dvbar, glnp;  // unit cotangent and ln P gradient of the adjoint stages
   if (adj) {
     hipError_t e = dvbar.ensure((size_t)bt.dv_off[bt.B]);
@@ -2020,6 +2151,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if ((bt.ctx_layout && !c->dl.valid) || c->batch_epoch != c->mode_epoch)
     return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
   if (!bt.uniform) return fail(c, CLH_ERR_ARG, "VJP needs a batch with one layout for every draw");
+  if (bt.tatt) return fail(c, CLH_ERR_ARG, "VJP of TATT intrinsic alignments is not implemented");
   const Layout& L = *bt.lays[0];
   if (!ord_ok(L, ord))
     return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");

@@ -14,6 +14,7 @@
 #define CLH_W_SS 0
 #define CLH_W_GS 1
 #define CLH_W_GG 2
+#define CLH_IA_TATT 1        // nuisance.IA_MODEL (cosmolike/IA.h:8)
 
 // Precision modes (CLH_PRECISION_* of the C ABI)
 #define CLH_PREC_FP64 0
@@ -110,6 +111,13 @@ struct DrawDesc {
   double* run;          // [64 (1 + nl) + 96 nl][nk] per-GL-node ln P cotangent runs, or null
   double* sxk;          // [1 + nl][nrow][64] ss / gs checkpoint of the forward (S = P fac per
                         // row and node) for the VJP, or null
+  // TATT intrinsic alignments (ia_model == CLH_IA_TATT)
+  int ia_model;
+  double* cl_bb;        // [n_ss][nrow] B-mode C_ss rows, or null (NLA: B mode is zero)
+  double* tatt;         // [1 + nl][2 ns + 1][64] per ss / gs node: B_b, D_b, D(a)^4
+  const double* fpt;    // [10][fpt_n] FAST-PT IA tables (set_IA_PS units: / (c/H0)^3)
+  int fpt_n;
+  double fpt_l0, fpt_l1, fpt_dl;  // ln k range (k in H0/c) and step (l1 - l0) / fpt_n
   double* ggk;          // [nl][nrow][3][96] gg integrand checkpoint of the forward (bracket t,
                         // P fac, dW_RSD/d dz per row and node) for the VJP, or null
   double* g_src;        // [ns]
@@ -125,6 +133,9 @@ struct DrawParams {
   double ia_bin[CLH_MAXB];
   double pm[CLH_MAXB];  // point-mass amplitudes per lens bin (PointMass::pm_)
   double ia_A, ia_eta;
+  // TATT: nuisance.ia[1][*] (A2: per bin, or value and exponent) and ia[2][*] (b_TA)
+  double ia2_bin[CLH_MAXB], bta_bin[CLH_MAXB];
+  double ia2_A, ia2_eta, bta_A;
 };
 
 // Node buffer layout (doubles):

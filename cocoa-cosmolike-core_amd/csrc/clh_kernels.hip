@@ -249,6 +249,16 @@ __global__ __launch_bounds__(128) void k_nodes(const DrawDesc* __restrict__ D,
     lut[7 * np + p] = clh_nz_lens(d, zz0, zl) * hf0;
     lut[8 * np + p] = clh_dnz_lens(d, zz0, zl) * hf0;
   }
+  if (d.tatt && seg <= d.nl) {
+    // TATT node vectors (B_b = W_S C1 b_TA, D_b = W_S C2) and D(a)^4
+    double* t = d.tatt + (size_t)seg * (2 * d.ns + 1) * np;
+    for (int bb = 0; bb < d.ns; bb++) {
+      const double WS = clh_nz_src(d, z, bb) * hh;
+      t[bb * np + p] = WS * clh_ia_c1(d, a, gf, bb) * clh_ia_bta(d, bb);
+      t[(d.ns + bb) * np + p] = WS * clh_ia_c2(d, a, gf, bb);
+    }
+    t[2 * d.ns * np + p] = gf * gf * gf * gf;
+  }
   if (seg == 0) {
     double* o = d.nodes + d.node_ss;
     o[p] = a;
@@ -415,6 +425,146 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
 }
 
 // ----------------------------------------------------------------------------
+// k_limber_sx_tatt: ss and gs tiles under TATT intrinsic alignments
+// (int_for_C_ss_tomo_limber_core, cosmo2D.c:1395-1487; gs, :1919-2006),
+// FP64.  Work item as k_limber_sx.  Per node (k_nodes): A_b = W_K - C1 W_S
+// (the NLA weight), B_b = W_S C1 b_TA, D_b = W_S C2, g4 = D(a)^4.  Per
+// (row, node) the P(k,z) product S = P fac and the FAST-PT IA functions at
+// k = (l + 1/2)/chi, each times g4 fac: E = ta_dE1 + ta_dE2, M = mixA + mixB,
+// T = ta_EE, X = mix_EE, TT = tt_EE and, for the B mode, T' = ta_BB,
+// X' = mix_BB, TT' = tt_BB.  The reference integrands, expanded per node:
+//   EE_ij = A_iA_j S - (B_iA_j + A_iB_j) E + 5 (D_iA_j + A_iD_j) M + B_iB_j T
+//           - 5 (B_iD_j + D_iB_j) X + 25 D_iD_j TT
+//   BB_ij = B_iB_j T' - 5 (B_iD_j + D_iB_j) X' + 25 D_iD_j TT'
+//   gs_(l,s) = gal_l (A_s S - B_s E + 5 D_s M),  gal = W_gal b1 + W_mag pref bmag
+// Block of 256 threads: 4 rows x 64 nodes per step (each thread walks rows
+// r, r + 4, ... in increasing ell), then the pair sums from LDS.  A draw of a
+// ragged batch without TATT (d.tatt null) gets B = D = 0, which leaves the
+// NLA sums.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx_tatt(const DrawDesc* __restrict__ D,
+                                                              const int4* __restrict__ work) {
+  constexpr int np = CLH_NP_SS, sp = CLH_NP_SS + 1, NK = 9;
+  __shared__ double K[4 * NK * sp];        // [row][function][node]
+  __shared__ double V[3 * CLH_MAXB * np];  // A, B, D per source bin and node
+  __shared__ double G[2 * np];             // gs: W_gal, W_mag
+  __shared__ double L10[CLH_TILE_ROWS], LN[CLH_TILE_ROWS];
+  __shared__ double s_lk[CLH_LDS_LK];
+  const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
+  const DrawDesc& d = D[w.x];
+  const int kind = w.y & 3, zl = w.y >> 2, row0 = w.z, nr = w.w;
+  const bool ss = kind == CLH_W_SS;
+  const int tid = threadIdx.x, ns = d.ns;
+  const double* o = clh_sx_nodes(d, kind, zl);
+  const double* lut = o + clh_sx_lut_col(d, kind) * np;
+  const bool tatt = d.tatt != nullptr;
+  const double* tv = tatt ? d.tatt + (size_t)(ss ? 0 : 1 + zl) * (2 * ns + 1) * np : nullptr;
+  const double* ells = ss ? d.ell_ss : d.ell_gx;
+  const int woff = ss ? 3 : 5;
+  for (int i = tid; i < ns * np; i += CLH_BLOCK) {
+    V[i] = o[woff * np + i];
+    V[CLH_MAXB * np + i] = tatt ? tv[i] : 0.0;
+    V[2 * CLH_MAXB * np + i] = tatt ? tv[ns * np + i] : 0.0;
+  }
+  if (!ss)
+    for (int i = tid; i < 2 * np; i += CLH_BLOCK) G[i] = o[3 * np + i];
+  for (int t = tid; t < nr; t += CLH_BLOCK) {
+    const double ell = ells[row0 + t] + 0.5;
+    L10[t] = log10(ell);
+    LN[t] = log(ell);
+  }
+  const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
+  __syncthreads();
+  const int p = tid & 63, wv = tid >> 6;
+  const double l10fk = lut[2 * np + p], fac = o[2 * np + p], lnfk = log(o[np + p]);
+  const double g4f = tatt ? tv[2 * ns * np + p] * fac : 0.0;
+  ClhPkNode pkn;
+  pkn.init(d, (int)lut[p], lut[np + p]);
+  constexpr double c3 = 1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
+  // FAST-PT rows of E, M, T, X, TT, T', X', TT' (E and M are sums of two rows)
+  const double b1 = ss ? 0.0 : d.par->b1[zl], bmag = ss ? 0.0 : d.par->bmag[zl];
+  const double* Av = V;
+  const double* Bv = V + CLH_MAXB * np;
+  const double* Dv = V + 2 * CLH_MAXB * np;
+  for (int base = 0; base < nr; base += 4) {
+    const int rr = base + wv;
+    double* k = K + wv * NK * sp;
+    if (rr < nr) {
+      double dx;
+      k[p] = exp(pkn.lnp(lk, L10[rr] - l10fk, dx)) * c3 * fac;
+      const double lnk = LN[rr] - lnfk;
+      if (!tatt) {
+        for (int f = 1; f < NK; f++) k[f * sp + p] = 0.0;
+      } else {
+        k[sp + p] = g4f * (clh_fpt(d, 2, lnk) + clh_fpt(d, 3, lnk));
+        k[2 * sp + p] = g4f * (clh_fpt(d, 6, lnk) + clh_fpt(d, 7, lnk));
+      }
+      if (ss && tatt) {
+        k[3 * sp + p] = g4f * clh_fpt(d, 4, lnk);
+        k[4 * sp + p] = g4f * clh_fpt(d, 8, lnk);
+        k[5 * sp + p] = g4f * clh_fpt(d, 0, lnk);
+        k[6 * sp + p] = g4f * clh_fpt(d, 5, lnk);
+        k[7 * sp + p] = g4f * clh_fpt(d, 9, lnk);
+        k[8 * sp + p] = g4f * clh_fpt(d, 1, lnk);
+      }
+    }
+    __syncthreads();
+    if (ss) {
+      const int nout = (d.cl_bb ? 2 : 1) * d.n_ss * 4;
+      for (int q = tid; q < nout; q += CLH_BLOCK) {
+        const int bb_mode = q >= d.n_ss * 4;
+        const int qq = bb_mode ? q - d.n_ss * 4 : q;
+        const int pair = qq >> 2, r = qq & 3, row = base + r;
+        if (row >= nr) continue;
+        int bi, bj;
+        clh_ss_pair(ns, pair, bi, bj);
+        const double* kr = K + r * NK * sp;
+        const double *Ai = Av + bi * np, *Aj = Av + bj * np, *Bi = Bv + bi * np, *Bj = Bv + bj * np;
+        const double *Di = Dv + bi * np, *Dj = Dv + bj * np;
+        double acc = 0.0;
+        if (!bb_mode) {
+          for (int n = 0; n < np; n++) {
+            const double ai = Ai[n], aj = Aj[n], bi_ = Bi[n], bj_ = Bj[n], di = Di[n], dj = Dj[n];
+            acc += ai * aj * kr[n] - (bi_ * aj + ai * bj_) * kr[sp + n] + 5.0 * (di * aj + ai * dj) * kr[2 * sp + n] +
+                   bi_ * bj_ * kr[3 * sp + n] - 5.0 * (bi_ * dj + di * bj_) * kr[4 * sp + n] +
+                   25.0 * di * dj * kr[5 * sp + n];
+          }
+        } else {
+          for (int n = 0; n < np; n++) {
+            const double bi_ = Bi[n], bj_ = Bj[n], di = Di[n], dj = Dj[n];
+            acc += bi_ * bj_ * kr[6 * sp + n] - 5.0 * (bi_ * dj + di * bj_) * kr[7 * sp + n] +
+                   25.0 * di * dj * kr[8 * sp + n];
+          }
+        }
+        const double l = ells[row0 + row], ell = l + 0.5;
+        const double pref = l * (l - 1.) * (l + 1.) * (l + 2.) / (ell * ell * ell * ell);
+        double* out = bb_mode ? d.cl_bb : d.cl;
+        out[(size_t)pair * d.nrow + row0 + row] = acc * pref;
+      }
+    } else {
+      const int nout = ns * 4;
+      for (int q = tid; q < nout; q += CLH_BLOCK) {
+        const int zs = q >> 2, r = q & 3, row = base + r;
+        if (row >= nr) continue;
+        const double l = ells[row0 + row], ell = l + 0.5;
+        const double pref1 = l * (l + 1.) / (ell * ell);
+        const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+        const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+        const double* kr = K + r * NK * sp;
+        const double *As = Av + zs * np, *Bs = Bv + zs * np, *Ds = Dv + zs * np;
+        double acc = 0.0;
+        for (int n = 0; n < np; n++) {
+          const double gal = G[n] * b1 + G[np + n] * pref1 * bmag;
+          acc += gal * (As[n] * kr[n] - Bs[n] * kr[sp + n] + 5.0 * Ds[n] * kr[2 * sp + n]);
+        }
+        d.cl[(size_t)(d.n_ss + zl * ns + zs) * d.nrow + row0 + row] = acc * pref2;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------------------------------------
 // k_limber_gg: gg tiles.  Work item (draw, zl, row0, nrows); 192 threads = two
 // groups of 96 (one thread per GL node, ClhGgNode started from the k_nodes
 // search results; chi(z), G(z) and the lens n(z) spline staged in LDS when
@@ -536,6 +686,9 @@ __global__ __launch_bounds__(256) void k_legendre(const DrawDesc* __restrict__ D
   const double* __restrict__ K = d.kfold + (size_t)kind * nrow * nt;
   const int q = c0 + lm;
   const double* __restrict__ C = d.cl + (size_t)(colbase + (q < ncolk ? q : 0)) * nrow;
+  // xi+- read EE +- BB when the B mode is present (TATT), cosmo2D.c:517-533
+  const double* __restrict__ CB = (kind <= 1 && d.cl_bb) ? d.cl_bb + (size_t)(q < ncolk ? q : 0) * nrow : nullptr;
+  const double sb = kind == 0 ? 1.0 : -1.0;
   const bool qin = q < ncolk, t0in = lm < nt, t1in = 16 + lm < nt, two = nt > 16;
   clh_v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
   const int nstep = (nrow + 3) >> 2;
@@ -545,7 +698,7 @@ __global__ __launch_bounds__(256) void k_legendre(const DrawDesc* __restrict__ D
     for (int u = 0; u < CLH_LEG_UNROLL; u++) {
       const int r = 4 * (s0 + 4 * u) + lk;
       const bool rin = r < nrow;
-      a[u] = (rin && qin) ? C[r] : 0.0;
+      a[u] = (rin && qin) ? (CB ? C[r] + sb * CB[r] : C[r]) : 0.0;
       b0[u] = (rin && t0in) ? K[(size_t)r * nt + lm] : 0.0;
       b1[u] = (rin && t1in) ? K[(size_t)r * nt + 16 + lm] : 0.0;
     }
@@ -599,6 +752,8 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre_full(const DrawDesc* __r
     if (tt < ntask) {
       const int col = (tt < d.n_ss) ? tt : tt - d.n_ss;
       v = d.cl[(size_t)col * nrow + (i % nrow)];
+      if (d.cl_bb && tt < 2 * d.n_ss)  // xi+- from EE +- BB (TATT)
+        v += (tt < d.n_ss ? 1.0 : -1.0) * d.cl_bb[(size_t)col * nrow + (i % nrow)];
     }
     colbuf[i] = v;
   }
@@ -653,6 +808,11 @@ hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, 
 hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_gtab<true>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  return hipGetLastError();
+}
+hipError_t clh_launch_limber_sx_tatt(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_limber_sx_tatt, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, int keep, hipStream_t s) {

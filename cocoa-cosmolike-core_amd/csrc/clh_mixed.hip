@@ -45,6 +45,10 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre_mx(const DrawDesc* __res
   const int q = col0 + j;
   const bool colok = q < ncol;
   const double* __restrict__ C = d.cl + (size_t)(cbase + (colok ? q : 0)) * nrow + 4 * h;
+  // xi+- read EE +- BB when the B mode is present (TATT)
+  const double* __restrict__ CB =
+      (kind <= 1 && d.cl_bb) ? d.cl_bb + (size_t)(colok ? q : 0) * nrow + 4 * h : nullptr;
+  const double sb = kind == 0 ? 1.0 : -1.0;
   clh_v16f acc;
 #pragma unroll
   for (int v = 0; v < 16; v++) acc[v] = 0.0f;
@@ -54,9 +58,13 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre_mx(const DrawDesc* __res
     if (g < gfull) {
 #pragma unroll
       for (int t = 0; t < 4; t++) c[t] = C[8 * g + t];
+      if (CB)
+        for (int t = 0; t < 4; t++) c[t] += sb * CB[8 * g + t];
     } else {
 #pragma unroll
       for (int t = 0; t < 4; t++) c[t] = (colok && 8 * g + 4 * h + t < nrow) ? C[8 * g + t] : 0.0;
+      if (CB)
+        for (int t = 0; t < 4; t++) c[t] += (colok && 8 * g + 4 * h + t < nrow) ? sb * CB[8 * g + t] : 0.0;
     }
     if (MODE == 0) {
       const float4 a = *reinterpret_cast<const float4*>(d.kmx32 + (((size_t)kind * ngrp + g) * 64 + lane) * 4);

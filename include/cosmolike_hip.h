@@ -71,8 +71,10 @@ int clh_init_ntable(clh_ctx* ctx, int n_ell, int n_a, int lmax, int high_def);
 /* init_binning_real_space (generic_interface.cpp:479-505), arcmin */
 int clh_init_binning_real_space(clh_ctx* ctx, int ntheta, double theta_min_arcmin,
                                 double theta_max_arcmin);
-/* init_IA (generic_interface.cpp:628-634): ia_model 0 (NLA) only;
- * ia_redshift_evol 0 (none), 2 (per-bin A) or 3 (A ((1+z)/1.62)^eta) */
+/* init_IA (generic_interface.cpp:628-634): ia_model 0 (NLA) or 1 (TATT);
+ * ia_redshift_evol 0 (none), 2 (per-bin A) or 3 (A ((1+z)/1.62)^eta).
+ * TATT computes the E- and B-mode C_ss (xi+- = EE +- BB transforms) and the
+ * TATT gamma_t in FP64 in every precision mode; clh_batch_vjp refuses it. */
 int clh_init_ia(clh_ctx* ctx, int ia_model, int ia_redshift_evol);
 /* set_cosmological_parameters (generic_interface.cpp:912-939) */
 int clh_set_cosmological_parameters(clh_ctx* ctx, double omega_matter, double hubble);
@@ -97,6 +99,15 @@ int clh_set_nuisance_linear_bias(clh_ctx* ctx, const double* b1, int n);
 int clh_set_nuisance_magnification_bias(clh_ctx* ctx, const double* bmag, int n);
 /* set_nuisance_IA: mode 2 uses A1[0..ns-1]; mode 3 uses A1[0]=A, A1[1]=eta */
 int clh_set_nuisance_ia(clh_ctx* ctx, const double* A1, int n);
+/* set_nuisance_IA with A2 and b_TA (generic_interface.cpp:1701-1774), for
+ * TATT: mode 2 one value per source bin in each of A1, A2, BTA; mode 3
+ * A1 = (A, eta), A2 = (A2, eta2), BTA[0] = b_TA */
+int clh_set_nuisance_ia_tatt(clh_ctx* ctx, const double* A1, const double* A2, const double* BTA, int n);
+/* set_IA_PS (generic_interface.cpp:941-1001): PS row-major [12][N], rows
+ * 0..9 the FAST-PT IA spectra (Mpc/h)^3 on N log-spaced k in [kmin, kmax)
+ * h/Mpc (rows 10, 11 unused by the Limber integrands); cutoff is accepted and
+ * ignored as the reference's lookup ignores it */
+int clh_set_IA_PS(clh_ctx* ctx, const double* PS, double kmin, double kmax, double cutoff, int N);
 
 /* ----------------------------------------------------------- bulk compute */
 /* PointMass::set_pm_vector (generic_interface.hpp:362-365): point-mass
@@ -154,8 +165,12 @@ int clh_wtheta(clh_ctx* ctx, double* out);
 /* Limber tables at the N_ell log-spaced ell (C_ss/gs/gg_tomo_limber tables,
  * cosmo2D.c:1617, 2195, 2693): ss[n_ss][N_ell], gs[n_gs][N_ell], gg[nl][N_ell] */
 int clh_cl_tables(clh_ctx* ctx, double* ss, double* gs, double* gg);
+/* the B-mode C_ss table, C_ss_tomo_limber(l, ni, nj, EE = 0) (cosmo2D.c:1617):
+ * bb[n_ss][N_ell]; zero under NLA */
+int clh_cl_tables_bb(clh_ctx* ctx, double* bb);
 /* Direct GL C_ell at arbitrary ell (C_*_tomo_limber_nointerp,
- * cosmo2D.c:1562, 2125, 2672): probe 0=ss(EE), 1=gs, 2=gg; out[pair][nl] */
+ * cosmo2D.c:1562, 2125, 2672): probe 0=ss(EE), 1=gs, 2=gg, 3=ss(BB);
+ * out[pair][nl] */
 int clh_cl_nointerp(clh_ctx* ctx, int probe, const double* ell, int nl, double* out);
 
 /* ------------------------------------------------ device-resident batches */
@@ -172,6 +187,11 @@ typedef struct {
   const double* b1; const double* bmag; const double* shear_m;
   double ia_A; double ia_eta; const double* ia_bin;
   const double* pm;  /* point-mass amplitudes per lens bin (NULL = 0) */
+  /* TATT only (ignored for NLA): A2 / b_TA as ia_A / ia_bin, and the
+   * set_IA_PS tables PS [12][fpt_n] over [fpt_kmin, fpt_kmax) h/Mpc */
+  double ia2_A; double ia2_eta; const double* ia2_bin;
+  double bta_A; const double* bta_bin;
+  const double* fpt; int fpt_n; double fpt_kmin; double fpt_kmax;
 } clh_draw;
 
 /* uploads B draws into device memory owned by the context */

@@ -89,6 +89,30 @@ def test_setters_validate(ctx):
     assert ctx.tile_rows() == [32, 32, 16, 64]
 
 
+def test_tatt_setters_validate(ctx):
+    p = synth.tatt_problem()
+    with pytest.raises(CosmolikeError) as e:
+        ctx.init_IA(2, 3)                             # no such IA model
+    assert e.value.code == -1
+    ctx.load(p)                                       # TATT layout and tables
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_IA_PS(p.ia_ps, 1.0, 0.5)              # kmax < kmin
+    assert e.value.code == -1
+    bad = p.ia_ps.copy().reshape(12, -1)
+    bad[3, 7] = np.nan
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_IA_PS(bad, p.ia_ps_kmin, p.ia_ps_kmax)
+    assert e.value.code == -2
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_nuisance_IA([0.5, 0.0], [np.nan, 0.0], [1.0])
+    assert e.value.code == -2
+    ctx.init_IA(1, 2)
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_nuisance_IA(np.ones(3), np.ones(3), np.ones(3))  # fewer than p.ns
+    assert e.value.code == -1
+    ctx.set_nuisance_IA(np.ones(p.ns), np.ones(p.ns), np.ones(p.ns))
+
+
 def test_datavector_sizes(ctx):
     p = synth.full_3x2pt_problem()
     ctx.load(p)
