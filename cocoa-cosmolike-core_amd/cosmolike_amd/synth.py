@@ -134,6 +134,35 @@ def power_table(cosmo: Cosmology, nk: int = 400, nz: int = 100, kmin_log10=-4.0,
     return log10k, z, np.ascontiguousarray(lnP.reshape(-1))
 
 
+def linear_power_table(cosmo: Cosmology, nk: int = 300, nz: int = 60, kmin_log10=-4.0, kmax_log10=2.0,
+                       zmax=4.0, k_nl=0.3):
+    """Linear P(k,z) companion of ``power_table`` on its own grid: the
+    ``power_table`` spectrum divided by a smooth nonlinear boost
+    1 + (k D(z) / k_nl)^2, so P_lin < P_nl at high k and the two differ where
+    the non-Limber correction (Limber with P_nl minus Limber with P_lin) reads
+    them.  k-major ln P, (Mpc/h)^3, k in h/Mpc."""
+    log10k = np.linspace(kmin_log10, kmax_log10, nk)
+    z = np.linspace(0.0, zmax, nz)
+    k = 10.0**log10k
+    amp = (cosmo.sigma8 / sigma8_unnorm(cosmo.omega_m, cosmo.h, cosmo.ns)) ** 2
+    p0 = amp * k**cosmo.ns * t_bbks(k, cosmo.omega_m, cosmo.h) ** 2
+    d = growth_D(z, cosmo.omega_m)
+    lnP = np.log(p0)[:, None] + 2.0 * np.log(d)[None, :] - np.log1p((k[:, None] * d[None, :] / k_nl) ** 2)
+    return log10k, z, np.ascontiguousarray(lnP.reshape(-1))
+
+
+def nonlimber_problem(cosmo: Cosmology | None = None, n_ell: int = 500, bmag=None) -> Problem:
+    """The full 3x2pt layout with the non-Limber w(theta) below ell = 150
+    (like.adopt_limber_gg = 0) and its linear P(k,z)."""
+    cosmo = cosmo or Cosmology()
+    p = full_3x2pt_problem(cosmo, n_ell)
+    p.plin_log10k, p.plin_z, p.lnPL = linear_power_table(cosmo)
+    p.limber_gg = 0
+    if bmag is not None:
+        p.bmag = np.asarray(bmag, dtype=np.float64)
+    return p
+
+
 def background(cosmo: Cosmology):
     z = background_grid()
     chi = comoving_distance(z, cosmo.omega_m)
@@ -242,6 +271,11 @@ class Problem:
     ia_ps: np.ndarray | None = None
     ia_ps_kmin: float = 0.0
     ia_ps_kmax: float = 0.0
+    # non-Limber w (like.adopt_limber_gg = 0) and its linear P(k,z)
+    limber_gg: int = 1
+    plin_log10k: np.ndarray | None = None
+    plin_z: np.ndarray | None = None
+    lnPL: np.ndarray | None = None
 
     @property
     def ns(self):

@@ -17,6 +17,7 @@
  */
 #include "cl_oracle.h"
 
+#include <complex.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdint.h>
@@ -64,6 +65,13 @@ struct oc_ctx {
   double* fpt; /* [10][fpt_n] */
   int fpt_n;
   double fpt_kmin, fpt_kmax;
+  /* non-Limber w (C_cl_tomo): h0, linear P(k,z) (set_linear_power_spectrum)
+     and like.adopt_limber_gg (1 here by default; the reference default 0) */
+  double h0;
+  int nkL, nzL;
+  double *lkL, *pzL, *lnPL;
+  int limber_gg;
+  int nl_lmax[OC_MAXB]; /* LMAX[i] of the last C_cl_tomo run */
   /* caches */
   int g_valid;
   double *gt_tab, *gl_tab; /* [ns][N_a], [nl][N_a] */
@@ -97,6 +105,8 @@ oc_ctx* oc_create(void) {
   c->c1rhocrit = 0.01389;
   c->K_ld = 1;
   c->oneplusz0 = 1.62; /* set_nuisance_IA, generic_interface.cpp:1756 */
+  c->h0 = 0.7;
+  c->limber_gg = 1;
   for (int i = 0; i < OC_MAXB; i++) c->lens_st[i] = 1.0;
   return c;
 }
@@ -110,6 +120,7 @@ void oc_free(oc_ctx* c) {
   free(c->gt_tab); free(c->gl_tab); free(c->K);
   free(c->mask);
   free(c->fpt);
+  free(c->lkL); free(c->pzL); free(c->lnPL);
   free(c);
 }
 
@@ -811,7 +822,24 @@ static double int_gs(oc_ctx* c, double a, int nl, int ns, double l) {
 }
 
 /* int_for_C_gg_tomo_limber (linear bias, RSD on, HOD off): cosmo2D.c:2490-2608 */
-static double int_gg(oc_ctx* c, double a, int ni, double l) {
+/* p_lin: cosmo3D.c:306-358 (bilinear in log10 k, z of ln P_lin) */
+static double p_lin(oc_ctx* c, double k, double a) {
+  const double log10k = log10(k / OC_COVERH0);
+  const double z = 1.0 / a - 1.0;
+  const int nk = c->nkL, nz = c->nzL;
+  const int i = bsearch_lo(c->lkL, nk, log10k);
+  const int j = bsearch_lo(c->pzL, nz, z);
+  const double dx = (log10k - c->lkL[i]) / (c->lkL[i + 1] - c->lkL[i]);
+  const double dy = (z - c->pzL[j]) / (c->pzL[j + 1] - c->pzL[j]);
+  const double* P = c->lnPL;
+  const double out = (1 - dx) * (1 - dy) * P[i * nz + j] + (1 - dx) * dy * P[i * nz + j + 1] +
+                     dx * (1 - dy) * P[(i + 1) * nz + j] + dx * dy * P[(i + 1) * nz + j + 1];
+  return exp(out) / (OC_COVERH0 * OC_COVERH0 * OC_COVERH0);
+}
+
+/* int_for_C_gg_tomo_limber: cosmo2D.c:2489-2606 (RSD on, no HOD, no one-loop
+   bias); use_lin selects p_lin over Pdelta */
+static double int_gg_ps(oc_ctx* c, double a, int ni, double l, int use_lin) {
   double fK, dchida;
   chi_all(c, a, &fK, &dchida);
   const double hh = hoverh0v2(a, dchida);
@@ -832,7 +860,7 @@ static double int_gg(oc_ctx* c, double a, int ni, double l) {
   const double WRSDi = W_RSD(c, ell, a_0, a_1, ni);
   res *= (WGALi * b1i + WMAGi * ell_prefactor * bmagi + WRSDi);
   res *= (WGALi * b1i + WMAGi * ell_prefactor * bmagi + WRSDi);
-  res *= p_nonlin(c, k, a);
+  res *= use_lin ? p_lin(c, k, a) : p_nonlin(c, k, a);
   return (res + 0.0) * dchida / (fK * fK);
 }
 
@@ -872,7 +900,11 @@ static double C_gs_nointerp(oc_ctx* c, const gl_rule* r, double l, int nl, int n
 }
 
 /* C_gg_tomo_limber_linpsopt_nointerp: cosmo2D.c:2614-2666 */
+static double C_gg_nointerp_ps(oc_ctx* c, const gl_rule* r, double l, int ni, int use_lin);
 static double C_gg_nointerp(oc_ctx* c, const gl_rule* r, double l, int ni) {
+  return C_gg_nointerp_ps(c, r, l, ni, 0);
+}
+static double C_gg_nointerp_ps(oc_ctx* c, const gl_rule* r, double l, int ni, int use_lin) {
   const double amin = amin_lens(c, ni), amax = amax_lens(c, ni);
   if (!(amin > 0) || !(amin < 1) || !(amax > 0) || !(amax < 1) || !(amin < amax)) {
     oc_set_err(c, "0 < amin < amax < 1 not true");
@@ -882,7 +914,7 @@ static double C_gg_nointerp(oc_ctx* c, const gl_rule* r, double l, int ni) {
   for (int p = 0; p < r->n; p++) {
     double x, w;
     gl_point(r, amin, amax, p, &x, &w);
-    s += w * int_gg(c, x, ni, l);
+    s += w * int_gg_ps(c, x, ni, l, use_lin);
   }
   return s;
 }
@@ -1374,12 +1406,25 @@ int oc_gammat(oc_ctx* c, double* out) {
   return c->status;
 }
 
+static int C_cl_tomo(oc_ctx* c, double* Cl, double tol);
+#define OC_LMAX_NOLIMBER 150 /* structs.c:45 */
+
+/* w_gg_tomo (cosmo2D.c:739-907): limber_gg = 1 fills every ell from the
+   Limber tables (ell < LMIN_tab direct); 0 takes ell < LMAX_NOLIMBER from
+   C_cl_tomo with tolerance 0.01 and the Limber fill above */
 int oc_wtheta(oc_ctx* c, double* out) {
   c->status = 0;
   if (!c->K_valid && build_kernels(c)) return -1;
   const int L = c->lmax;
   double* cl = malloc(sizeof(double) * (size_t)c->nl * L);
   if (oc_cl_full(c, NULL, NULL, cl)) { free(cl); return -1; }
+  if (!c->limber_gg) {
+    double* nl = malloc(sizeof(double) * (size_t)c->nl * OC_LMAX_NOLIMBER);
+    if (C_cl_tomo(c, nl, 0.01)) { free(nl); free(cl); return -1; }
+    for (int i = 0; i < c->nl; i++)
+      for (int l = 0; l < OC_LMAX_NOLIMBER && l < L; l++) cl[(size_t)i * L + l] = nl[(size_t)i * OC_LMAX_NOLIMBER + l];
+    free(nl);
+  }
   contract(c->K + (size_t)3 * c->ntheta * L, cl, c->nl, c->ntheta, L, out);
   free(cl);
   return c->status;
@@ -1707,4 +1752,338 @@ int oc_chi_all(oc_ctx* c, const double* a, int n, double* chi, double* dchida) {
 int oc_pdelta(oc_ctx* c, const double* k, const double* a, int n, double* out) {
   for (int i = 0; i < n; i++) out[i] = p_nonlin(c, k[i], a[i]);
   return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Non-Limber clustering C_ell (C_cl_tomo, cosmo2D.c:4004-4769)            */
+/* ---------------------------------------------------------------------- */
+#define OC_NL_NCHI 500       /* Ntable.NL_Nchi, structs.c:272 */
+#define OC_NL_BLOCK 16       /* cosmo2D.c:4672 */
+
+/* next_fft_size (cosmo2D.c:4023-4034): the smallest m >= n whose prime
+   factors are all <= 7 */
+static long size_7smooth(long n) {
+  static const int pr[4] = {2, 3, 5, 7};
+  for (;; n++) {
+    long m = n;
+    for (int i = 0; i < 4; i++)
+      while (m % pr[i] == 0) m /= pr[i];
+    if (m == 1) return n;
+  }
+}
+
+/* Unnormalised complex DFT out[k] = sum_j in[j stride] e^{sign 2 pi i jk/n}
+   by recursive Cooley-Tukey over the smallest prime factor p of n (sub-DFTs
+   of the p decimated sequences, then one p-point DFT per output group).  It
+   stands in for FFTW's fftw_plan_dft_r2c_1d / c2r_1d (the reference's
+   cfftlog_ells_p1/p2); pinned against numpy.fft in the tests. */
+static void fft_rec(const double complex* in, double complex* out, int n, int stride, int sign) {
+  if (n == 1) { out[0] = in[0]; return; }
+  int p = 2;
+  while (n % p) p++;
+  const int m = n / p;
+  for (int r = 0; r < p; r++) fft_rec(in + (size_t)r * stride, out + (size_t)r * m, m, stride * p, sign);
+  double complex ybuf[16];
+  double complex* y = p <= 16 ? ybuf : (double complex*)malloc(sizeof(double complex) * p);
+  const double w0 = 2.0 * M_PI / n;
+  for (int k = 0; k < m; k++) {
+    for (int r = 0; r < p; r++) y[r] = out[(size_t)r * m + k];
+    for (int q = 0; q < p; q++) {
+      const long kk = k + (long)q * m;
+      double complex s = 0.0;
+      for (int r = 0; r < p; r++) {
+        const long t = ((long)r * kk) % n;
+        const double ang = sign * w0 * (double)t;
+        s += y[r] * (cos(ang) + I * sin(ang));
+      }
+      out[kk] = s;
+    }
+  }
+  if (y != ybuf) free(y);
+}
+
+int oc_fft(const double* re, const double* im, int n, int sign, double* out_re, double* out_im) {
+  double complex* a = malloc(sizeof(double complex) * n);
+  double complex* b = malloc(sizeof(double complex) * n);
+  for (int i = 0; i < n; i++) a[i] = re[i] + I * (im ? im[i] : 0.0);
+  fft_rec(a, b, n, 1, sign < 0 ? -1 : 1);
+  for (int i = 0; i < n; i++) { out_re[i] = creal(b[i]); out_im[i] = cimag(b[i]); }
+  free(a); free(b);
+  return 0;
+}
+
+/* ln Gamma(a) for complex a: Lanczos approximation (g = 7, 9 terms, the
+   coefficient table of cfftlog_ells_p2, cosmo2D.c:4160-4168, i.e. the
+   published Godfrey set), with the reflection formula below Re a = 1/2
+   (cosmo2D.c:4270-4297).  The imaginary part is defined modulo 2 pi. */
+static const double lanczos_p[9] = {0.99999999999980993227684700473478, 676.520368121885098567009190444019,
+                                    -1259.13921672240287047156078755283, 771.3234287776530788486528258894,
+                                    -176.61502916214059906584551354, 12.507343278686904814458936853,
+                                    -0.13857109526572011689554707, 9.984369578019570859563e-6,
+                                    1.50563273514931155834e-7};
+static double complex lanczos_lngamma1(double complex zz) { /* ln Gamma(zz + 1), Re zz > -1/2 */
+  double complex s = lanczos_p[0];
+  for (int w = 1; w < 9; w++) s += lanczos_p[w] / (zz + w);
+  const double complex t = zz + 7.5;
+  return 0.5 * log(2 * M_PI) + (zz + 0.5) * clog(t) - t + clog(s);
+}
+static double complex clngamma(double complex a) {
+  if (creal(a) < 0.5) return clog(M_PI) - clog(csin(M_PI * a)) - lanczos_lngamma1(-a);
+  return lanczos_lngamma1(a - 1.0);
+}
+int oc_clngamma(double re, double im, double* out) {
+  const double complex v = clngamma(re + I * im);
+  out[0] = creal(v);
+  out[1] = cimag(v);
+  return 0;
+}
+
+/* FFTLog configuration of C_cl_tomo (cosmo2D.c:4566-4580): density, RSD, magnification */
+typedef struct { double nu, cww; int deriv, npad; } nl_cfg;
+static const nl_cfg NL_CFG[3] = {{1.0, 0.25, 0, 200}, {1.01, 0.25, 2, 500}, {1.0, 0.25, 0, 500}};
+
+/* g_l(z) of cfftlog_ells_p2 (cosmo2D.c:4253-4469): derivative 0:
+   2^z Gamma((l+z)/2) / Gamma((3+l-z)/2); derivative 2:
+   (z-1)(z-2) 2^(z-2) Gamma((l+z-2)/2) / Gamma((5+l-z)/2) */
+static double complex nl_gl_direct(int deriv, int l, double complex z) {
+  if (deriv == 0) return cexp(z * log(2.) + clngamma(0.5 * (l + z)) - clngamma(0.5 * (3 + l - z)));
+  return (z - 1) * (z - 2) * cexp((z - 2) * log(2.) + clngamma(0.5 * (l + z - 2)) - clngamma(0.5 * (5 + l - z)));
+}
+/* the two-step recurrence the reference uses inside an ell block */
+static double complex nl_gl_step(int deriv, int l, double complex z) {
+  return deriv == 0 ? (l - 2 + z) / (l + 1 - z) : (l - 4 + z) / (l + 3 - z);
+}
+
+/* One FFTLog transform of fx on the log grid x (cfftlog_ells_p1 + p2 for one
+   (bin, config)): Fy[l][q] at y = (l + 1) / x[nx-1-q] for l in [ls, le).
+   Fy ~ int dx/x fx(x) j_l^(deriv)(x y). */
+typedef struct {
+  int N0, N2, nh;
+  double dlnx, nu;
+  double complex* T; /* windowed forward transform [nh] */
+  double* eta;       /* [nh] */
+} nl_fwd;
+
+static void nl_forward(const double* x, const double* fx, int nx, const nl_cfg* cf, nl_fwd* F) {
+  F->N0 = cf->npad;
+  long raw = 2 * (long)cf->npad + nx;
+  if (raw % 2) raw++;
+  long N2 = size_7smooth(raw);
+  if (N2 % 2) N2 = size_7smooth(N2 + 1);
+  F->N2 = (int)N2;
+  F->nh = F->N2 / 2 + 1;
+  F->nu = cf->nu;
+  F->dlnx = log(x[1] / x[0]);
+  double complex* in = calloc(F->N2, sizeof(double complex));
+  double complex* out = malloc(sizeof(double complex) * F->N2);
+  for (int q = 0; q < nx; q++) in[F->N0 + q] = fx[q] / pow(x[q], cf->nu);
+  fft_rec(in, out, F->N2, 1, -1);
+  F->T = malloc(sizeof(double complex) * F->nh);
+  for (int m = 0; m < F->nh; m++) F->T[m] = out[m];
+  const int halfN = F->N2 / 2;
+  const int kmax = (int)(halfN * cf->cww);
+  for (int k = 0; k <= kmax; k++) { /* c_window_cfft (cosmo2D.c:4103-4116) */
+    const double W = (double)k / kmax - sin(2. * M_PI * k / kmax) / (2. * M_PI);
+    F->T[halfN - k] *= W;
+  }
+  F->eta = malloc(sizeof(double) * F->nh);
+  for (int m = 0; m < F->nh; m++) F->eta[m] = (2.0 * M_PI / (F->dlnx * F->N2)) * m;
+  free(in); free(out);
+}
+
+static void nl_backward(const double* x, int nx, const nl_cfg* cf, const nl_fwd* F, int ls, int le,
+                        double* Fy /* [le-ls][nx] */) {
+  const int N2 = F->N2, nh = F->nh;
+  double complex* gl = malloc(sizeof(double complex) * (size_t)(le - ls) * nh);
+  for (int m = 0; m < nh; m++) {
+    const double complex z = F->nu + I * F->eta[m];
+    for (int l = ls; l < le; l++)
+      gl[(size_t)(l - ls) * nh + m] =
+          (l < ls + 2) ? nl_gl_direct(cf->deriv, l, z) : gl[(size_t)(l - ls - 2) * nh + m] * nl_gl_step(cf->deriv, l, z);
+  }
+  const double base = x[0] / exp(2 * F->N0 * F->dlnx);
+  double complex* spec = malloc(sizeof(double complex) * N2);
+  double complex* out = malloc(sizeof(double complex) * N2);
+  for (int l = ls; l < le; l++) {
+    const double lnbase = log(base * (l + 1.) / x[nx - 1]);
+    for (int m = 0; m < nh; m++) {
+      const double ph = -F->eta[m] * lnbase;
+      const double complex v = F->T[m] * (cos(ph) + I * sin(ph)) * gl[(size_t)(l - ls) * nh + m];
+      spec[m] = conj(v);
+    }
+    /* c2r: Hermitian extension, imaginary parts of the DC and Nyquist terms dropped */
+    spec[0] = creal(spec[0]);
+    spec[N2 / 2] = creal(spec[N2 / 2]);
+    for (int m = 1; m < N2 / 2; m++) spec[N2 - m] = conj(spec[m]);
+    fft_rec(spec, out, N2, 1, +1);
+    const double pre = sqrt(M_PI) / (4. * N2 * pow(l + 1., cf->nu));
+    for (int q = 0; q < nx; q++) Fy[(size_t)(l - ls) * nx + q] = creal(out[F->N0 + q]) * pre * pow(x[nx - 1 - q], cf->nu);
+  }
+  free(gl); free(spec); free(out);
+}
+
+static void nl_free(nl_fwd* F) { free(F->T); free(F->eta); }
+
+int oc_fftlog(const double* x, const double* fx, int nx, int cfg, int ls, int le, double* Fy) {
+  if (cfg < 0 || cfg > 2 || nx < 2 || le <= ls || ls < 0) return -1;
+  nl_fwd F;
+  nl_forward(x, fx, nx, &NL_CFG[cfg], &F);
+  nl_backward(x, nx, &NL_CFG[cfg], &F, ls, le, Fy);
+  nl_free(&F);
+  return 0;
+}
+
+/* C_cl_tomo (cosmo2D.c:4528-4769) with tol: Cl[nl][OC_LMAX_NOLIMBER] */
+static int C_cl_tomo(oc_ctx* c, double* Cl, double tol) {
+  if (!c->lnPL) { oc_set_err(c, "non-Limber w needs the linear P(k,z) (set_linear_power_spectrum)"); return -1; }
+  const int nb = c->nl, nchi = OC_NL_NCHI, LN = OC_LMAX_NOLIMBER;
+  const double rc = OC_COVERH0 / c->h0;
+  const double chi_min = chi_of_a(c, 1. / (1.0 + 0.002)) * rc;
+  const double chi_max = chi_of_a(c, 1. / (1.0 + 4.0)) * rc;
+  const double dlnchi = log(chi_max / chi_min) / ((double)nchi - 1.0);
+  const double dlnk = dlnchi;
+  double* x = malloc(sizeof(double) * nchi);
+  double* fx = calloc((size_t)nb * 3 * nchi, sizeof(double));
+  for (int j = 0; j < nchi; j++) {
+    x[j] = chi_min * exp(dlnchi * j);
+    const double chi = x[j] / rc;
+    const double a = a_chi(c, chi);
+    const double z = 1. / a - 1.;
+    const double hh = hoverh0(c, a);
+    const double D = growfac(c, a), f = f_growth(c, z);
+    for (int i = 0; i < nb; i++) {
+      if (z < c->l_zmin[i] || z > c->l_zmax[i]) continue;
+      const double pf = nz_lens(c, z, i);
+      const double WM = W_mag(c, a, chi, i);
+      fx[((size_t)i * 3 + 0) * nchi + j] = chi * pf * D * hh * c->b1[i];
+      fx[((size_t)i * 3 + 1) * nchi + j] = -chi * pf * D * hh * f;
+      fx[((size_t)i * 3 + 2) * nchi + j] = (WM / chi / (rc * rc)) * D;
+    }
+  }
+  int bmag_zero = 1;
+  for (int i = 0; i < nb; i++)
+    if (fabs(c->bmag[i]) > 1.e-12) bmag_zero = 0;
+  const int S2 = bmag_zero ? 2 : 3;
+  nl_fwd* F = malloc(sizeof(nl_fwd) * (size_t)nb * 3);
+  for (int i = 0; i < nb; i++)
+    for (int jj = 0; jj < S2; jj++) nl_forward(x, fx + ((size_t)i * 3 + jj) * nchi, nchi, &NL_CFG[jj], &F[i * 3 + jj]);
+  /* the gg Limber table (C_gg_tomo_limber) for the ells past convergence */
+  double* tab = malloc(sizeof(double) * (size_t)nb * c->n_ell);
+  table_gg(c, tab);
+  double lim[3];
+  gx_lim(c, lim);
+  gl_rule r96;
+  gl_make(&r96, gl_order_gg(c->hdi));
+  int conv[OC_MAXB];
+  for (int i = 0; i < nb; i++) { conv[i] = 0; c->nl_lmax[i] = LN; }
+  double* Fy = malloc(sizeof(double) * 3 * OC_NL_BLOCK * nchi);
+  for (int ks = 0; ks < LN;) {
+    int all = 1;
+    for (int i = 0; i < nb; i++) all &= conv[i];
+    if (all) break;
+    const int ke = ks + OC_NL_BLOCK < LN ? ks + OC_NL_BLOCK : LN;
+    for (int i = 0; i < nb; i++) {
+      if (conv[i]) continue;
+      for (int jj = 0; jj < 3; jj++) {
+        if (jj < S2) nl_backward(x, nchi, &NL_CFG[jj], &F[i * 3 + jj], ks, ke, Fy + (size_t)jj * OC_NL_BLOCK * nchi);
+        else memset(Fy + (size_t)jj * OC_NL_BLOCK * nchi, 0, sizeof(double) * OC_NL_BLOCK * nchi);
+      }
+      const int kk = ke;
+      for (int k = ks; k < kk; k++) {
+        double tcl = 0.0;
+        const double ell_prefactor = k * (k + 1.);
+        for (int q = 0; q < nchi; q++) {
+          const double ty = (k + 1.) / x[nchi - 1 - q];
+          const double k1 = ty * rc;
+          const size_t o = (size_t)(k - ks) * nchi + q;
+          const double Fv = Fy[o] + Fy[OC_NL_BLOCK * nchi + o] +
+                            c->bmag[i] * ell_prefactor * Fy[2 * OC_NL_BLOCK * nchi + o] / (ty * ty);
+          tcl += Fv * Fv * (k1 * k1 * k1) * p_lin(c, k1, 1.0);
+        }
+        Cl[(size_t)i * LN + k] = tcl * dlnk * 2. / M_PI + C_gg_nointerp_ps(c, &r96, (double)k, i, 0) -
+                                 C_gg_nointerp_ps(c, &r96, (double)k, i, 1);
+      }
+      const int L = kk - 1;
+      const double denom = C_gg_nointerp(c, &r96, (double)L, i);
+      if (fabs(denom) > 1e-300) {
+        const double dev = Cl[(size_t)i * LN + L] / denom - 1.0;
+        if (isfinite(dev) && fabs(dev) < tol) { conv[i] = 1; c->nl_lmax[i] = kk; }
+      }
+    }
+    ks = ke;
+  }
+  for (int i = 0; i < nb; i++)
+    for (int k = c->nl_lmax[i]; k < LN; k++)
+      Cl[(size_t)i * LN + k] = (k > OC_LMIN_TAB) ? interpol1d(tab + (size_t)i * c->n_ell, c->n_ell, lim[0], lim[1], lim[2], log((double)k))
+                                                 : C_gg_nointerp(c, &r96, (double)k, i);
+  for (int i = 0; i < nb; i++)
+    for (int jj = 0; jj < S2; jj++) nl_free(&F[i * 3 + jj]);
+  free(F); free(Fy); free(tab); free(x); free(fx);
+  return c->status;
+}
+
+int oc_cl_nonlimber(oc_ctx* c, double* cl /* [nl][150] */, int* lmax /* [nl] or NULL */) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  if (C_cl_tomo(c, cl, 0.01)) return -1;
+  if (lmax)
+    for (int i = 0; i < c->nl; i++) lmax[i] = c->nl_lmax[i];
+  return c->status;
+}
+
+int oc_set_h0(oc_ctx* c, double h0) { c->h0 = h0; return 0; }
+int oc_set_limber_gg(oc_ctx* c, int limber) { c->limber_gg = limber != 0; return 0; }
+
+/* set_linear_power_spectrum (generic_interface.cpp:1194-1270): lnPL k-major */
+int oc_set_plin(oc_ctx* c, const double* lk, int nk, const double* z, int nz, const double* lnP) {
+  c->status = 0;
+  if (nk < 2 || nz < 2 || any_nan(lk, nk) || any_nan(z, nz) || any_nan(lnP, nk * nz)) {
+    oc_set_err(c, "bad linear P(k,z)"); return -1;
+  }
+  free(c->lkL); free(c->pzL); free(c->lnPL);
+  c->lkL = dup(lk, nk); c->pzL = dup(z, nz); c->lnPL = dup(lnP, nk * nz);
+  c->nkL = nk; c->nzL = nz;
+  return 0;
+}
+
+/* the FFTLog inputs of C_cl_tomo (cosmo2D.c:4586-4628): x[nchi] (Mpc) and
+   fx[nl][3][nchi] (density, RSD, magnification), for the tests */
+int oc_nl_inputs(oc_ctx* c, double* x_out, double* fx_out) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  const int nb = c->nl, nchi = OC_NL_NCHI;
+  const double rc = OC_COVERH0 / c->h0;
+  const double chi_min = chi_of_a(c, 1. / (1.0 + 0.002)) * rc;
+  const double chi_max = chi_of_a(c, 1. / (1.0 + 4.0)) * rc;
+  const double dlnchi = log(chi_max / chi_min) / ((double)nchi - 1.0);
+  for (size_t i = 0; i < (size_t)nb * 3 * nchi; i++) fx_out[i] = 0.0;
+  for (int j = 0; j < nchi; j++) {
+    const double x = chi_min * exp(dlnchi * j);
+    x_out[j] = x;
+    const double chi = x / rc;
+    const double a = a_chi(c, chi);
+    const double z = 1. / a - 1.;
+    const double hh = hoverh0(c, a);
+    const double D = growfac(c, a), f = f_growth(c, z);
+    for (int i = 0; i < nb; i++) {
+      if (z < c->l_zmin[i] || z > c->l_zmax[i]) continue;
+      const double pf = nz_lens(c, z, i);
+      fx_out[((size_t)i * 3 + 0) * nchi + j] = chi * pf * D * hh * c->b1[i];
+      fx_out[((size_t)i * 3 + 1) * nchi + j] = -chi * pf * D * hh * f;
+      fx_out[((size_t)i * 3 + 2) * nchi + j] = (W_mag(c, a, chi, i) / chi / (rc * rc)) * D;
+    }
+  }
+  return c->status;
+}
+
+/* C_gg_tomo_limber_linpsopt_nointerp(l, i, i, use_linear_ps = 1) */
+int oc_cl_gg_lin_nointerp(oc_ctx* c, const double* l, int nl, double* out) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  if (!c->lnPL) { oc_set_err(c, "linear P(k,z) not set"); return -1; }
+  gl_rule r;
+  gl_make(&r, gl_order_gg(c->hdi));
+  for (int k = 0; k < c->nl; k++)
+    for (int i = 0; i < nl; i++) out[(size_t)k * nl + i] = C_gg_nointerp_ps(c, &r, l[i], k, 1);
+  return c->status;
 }

@@ -103,6 +103,28 @@ int oc_npairs_gs(oc_ctx* c);
 void oc_touch(oc_ctx* c);
 const char* oc_last_error(oc_ctx* c);
 
+/* ---- non-Limber clustering (C_cl_tomo, cosmo2D.c:4004-4769) ---- */
+/* h0 = H0 / 100 (set_cosmological_parameters); default 0.7 */
+int oc_set_h0(oc_ctx* c, double h0);
+/* set_linear_power_spectrum (generic_interface.cpp:1194-1270), lnP k-major */
+int oc_set_plin(oc_ctx* c, const double* log10k, int nk, const double* z, int nz, const double* lnP);
+/* like.adopt_limber_gg: 1 (default here) Limber w, 0 non-Limber below ell 150 */
+int oc_set_limber_gg(oc_ctx* c, int limber);
+/* C_cl_tomo with tolerance 0.01: cl[nl][150] (ell 0..149) and the ell where
+   each bin switched to Limber (LMAX[i]) */
+int oc_cl_nonlimber(oc_ctx* c, double* cl, int* lmax);
+/* one FFTLog transform (configuration cfg 0 density, 1 RSD, 2 magnification
+   of C_cl_tomo) of fx on the log grid x: Fy[l - ls][q] at y = (l+1)/x[nx-1-q] */
+int oc_fftlog(const double* x, const double* fx, int nx, int cfg, int ls, int le, double* Fy);
+/* the DFT behind it (sign -1 forward, +1 backward, unnormalised) */
+int oc_fft(const double* re, const double* im, int n, int sign, double* out_re, double* out_im);
+/* complex ln Gamma (Lanczos g = 7, 9 terms; imaginary part modulo 2 pi) */
+int oc_clngamma(double re, double im, double* out);
+/* FFTLog inputs of C_cl_tomo: x[500] (Mpc), fx[nl][3][500] */
+int oc_nl_inputs(oc_ctx* c, double* x, double* fx);
+/* Limber C_gg with the linear P(k,z) at arbitrary ell: out[nl][n] */
+int oc_cl_gg_lin_nointerp(oc_ctx* c, const double* l, int n, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,20 @@ def _load():
                                               ctypes.POINTER(ctypes.c_int), _I, _D]
         lib.oc_chi2.restype = ctypes.c_double
         lib.oc_chi2.argtypes = [_D, _D, _D, _I, ctypes.c_int]
+        for name in ("oc_set_h0", "oc_set_plin", "oc_set_limber_gg", "oc_cl_nonlimber", "oc_fftlog", "oc_fft",
+                     "oc_clngamma"):
+            getattr(lib, name).restype = ctypes.c_int
+        lib.oc_set_h0.argtypes = [vp, ctypes.c_double]
+        lib.oc_set_plin.argtypes = [vp, _D, ctypes.c_int, _D, ctypes.c_int, _D]
+        lib.oc_set_limber_gg.argtypes = [vp, ctypes.c_int]
+        lib.oc_cl_nonlimber.argtypes = [vp, _D, _I]
+        lib.oc_fftlog.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
+        lib.oc_fft.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, _D, _D]
+        lib.oc_clngamma.argtypes = [ctypes.c_double, ctypes.c_double, _D]
+        lib.oc_nl_inputs.restype = ctypes.c_int
+        lib.oc_nl_inputs.argtypes = [vp, _D, _D]
+        lib.oc_cl_gg_lin_nointerp.restype = ctypes.c_int
+        lib.oc_cl_gg_lin_nointerp.argtypes = [vp, _D, ctypes.c_int, _D]
         _lib = lib
     return _lib
 
@@ -189,7 +203,42 @@ class Oracle:
             self.set_ia_ps(prob.ia_ps, prob.ia_ps_kmin, prob.ia_ps_kmax)
         if prob.lens is not None and getattr(prob, "pm", None) is not None:
             self.set_pointmass(prob.pm)
+        self.set_h0(getattr(prob, "h", 0.7))
+        if getattr(prob, "lnPL", None) is not None:
+            self.set_plin(prob.plin_log10k, prob.plin_z, prob.lnPL)
+        self.set_limber_gg(getattr(prob, "limber_gg", 1))
         return self
+
+    # ----------------------------------------------------- non-Limber w
+    def set_h0(self, h0):
+        self._chk(self.lib.oc_set_h0(self.c, float(h0)))
+
+    def set_plin(self, log10k, z, lnPL):
+        lk, z, lnPL = map(_arr, (log10k, z, lnPL))
+        self._chk(self.lib.oc_set_plin(self.c, _p(lk), len(lk), _p(z), len(z), _p(lnPL)))
+
+    def set_limber_gg(self, limber):
+        self._chk(self.lib.oc_set_limber_gg(self.c, int(limber)))
+
+    def cl_nonlimber(self):
+        """C_cl_tomo: (cl[nl][150] at ell 0..149, LMAX[nl] where Limber takes over)."""
+        out = np.empty((self.nl, 150))
+        lmax = np.empty(self.nl, dtype=np.int32)
+        self._chk(self.lib.oc_cl_nonlimber(self.c, _p(out), lmax.ctypes.data_as(_I)))
+        return out, lmax
+
+    def nl_inputs(self):
+        """FFTLog inputs of C_cl_tomo: x[500] (Mpc) and fx[nl][3][500]."""
+        x = np.empty(500)
+        fx = np.empty((self.nl, 3, 500))
+        self._chk(self.lib.oc_nl_inputs(self.c, _p(x), _p(fx)))
+        return x, fx
+
+    def cl_gg_lin_nointerp(self, ells):
+        ells = _arr(ells)
+        out = np.empty((self.nl, len(ells)))
+        self._chk(self.lib.oc_cl_gg_lin_nointerp(self.c, _p(ells), len(ells), _p(out)))
+        return out
 
     def set_pointmass(self, pm):
         pm = _arr(pm)
@@ -347,3 +396,27 @@ class Oracle:
         o = (ctypes.c_int * 3)(*ord)
         self._chk(self.lib.oc_datavector_3x2pt(self.c, o, _p(out)))
         return out
+
+
+def fftlog(x, fx, cfg, ls, le):
+    """One FFTLog transform of C_cl_tomo's configuration cfg (0 density, 1 RSD,
+    2 magnification): Fy[l - ls][q] at y = (l + 1) / x[-1 - q]."""
+    x, fx = _arr(x), _arr(fx)
+    out = np.empty((le - ls, len(x)))
+    if _load().oc_fftlog(_p(x), _p(fx), len(x), int(cfg), int(ls), int(le), _p(out)):
+        raise ValueError("oc_fftlog: bad arguments")
+    return out
+
+
+def fft(v, sign=-1):
+    v = np.asarray(v, dtype=np.complex128)
+    re, im = _arr(v.real), _arr(v.imag)
+    ore, oim = np.empty(len(v)), np.empty(len(v))
+    _load().oc_fft(_p(re), _p(im), len(v), int(sign), _p(ore), _p(oim))
+    return ore + 1j * oim
+
+
+def clngamma(z):
+    out = np.empty(2)
+    _load().oc_clngamma(float(np.real(z)), float(np.imag(z)), _p(out))
+    return out[0] + 1j * out[1]
