@@ -33,6 +33,8 @@ class clh_draw(ctypes.Structure):
         ("ia2_A", ctypes.c_double), ("ia2_eta", ctypes.c_double), ("ia2_bin", D),
         ("bta_A", ctypes.c_double), ("bta_bin", D),
         ("fpt", D), ("fpt_n", ctypes.c_int), ("fpt_kmin", ctypes.c_double), ("fpt_kmax", ctypes.c_double),
+        ("h", ctypes.c_double),
+        ("plin_log10k", D), ("plin_z", D), ("lnPL", D), ("plin_nk", ctypes.c_int), ("plin_nz", ctypes.c_int),
     ]
 
 
@@ -79,6 +81,9 @@ def lib():
             "clh_set_nuisance_ia": (i32, [VP, D, i32]),
             "clh_set_nuisance_ia_tatt": (i32, [VP, D, D, D, i32]),
             "clh_set_IA_PS": (i32, [VP, D, f64, f64, f64, i32]),
+            "clh_set_linear_power_spectrum": (i32, [VP, D, i32, D, i32, D]),
+            "clh_set_adopt_limber_gg": (i32, [VP, i32]),
+            "clh_cl_nonlimber": (i32, [VP, D, ctypes.POINTER(ctypes.c_int)]),
             "clh_set_nuisance_pointmass": (i32, [VP, D, i32]),
             "clh_set_vjp_checkpoint": (i32, [VP, i32]),
             "clh_init_binning_fourier": (i32, [VP, i32, i32, i32, i32]),

@@ -170,6 +170,21 @@ class Cosmolike:
         BTA = np.resize(BTA, n)
         self._chk(self.L.clh_set_nuisance_ia_tatt(self.h, _p(A1), _p(A2), _p(BTA), n))
 
+    def set_linear_power_spectrum(self, log10k, z, lnPL):
+        lk, z, lnPL = _arr(log10k), _arr(z), _arr(lnPL)
+        self._chk(self.L.clh_set_linear_power_spectrum(self.h, _p(lk), len(lk), _p(z), len(z), _p(lnPL)))
+
+    def set_adopt_limber_gg(self, limber):
+        """like.adopt_limber_gg: 1 Limber w(theta); 0 the non-Limber C_gg below ell 150."""
+        self._chk(self.L.clh_set_adopt_limber_gg(self.h, int(limber)))
+
+    def cl_nonlimber(self):
+        """C_cl_tomo as w uses it: (cl[nl][150] at ell 0..149, LMAX[nl])."""
+        cl = np.empty((self.nl, 150))
+        lmax = np.empty(self.nl, dtype=np.int32)
+        self._chk(self.L.clh_cl_nonlimber(self.h, _p(cl), lmax.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return cl, lmax
+
     def set_IA_PS(self, PS, kmin, kmax, cutoff=1.0e4):
         """set_IA_PS: the FAST-PT IA spectra PS[12][N] on N log-spaced k in
         [kmin, kmax) h/Mpc."""
@@ -185,6 +200,7 @@ class Cosmolike:
         self.init_binning_real_space(prob.ntheta, prob.theta_min_arcmin, prob.theta_max_arcmin)
         self.init_IA(getattr(prob, "ia_model", 0), prob.ia_mode)
         self.set_legendre_mode(legendre_mode)
+        self.set_adopt_limber_gg(getattr(prob, "limber_gg", 1))
         self.set_source_sample(prob.source)
         if prob.lens is not None:
             self.set_lens_sample(prob.lens)
@@ -196,6 +212,8 @@ class Cosmolike:
         self.set_distances(prob.chi_z, prob.chi)
         self.set_growth(prob.G_z, prob.G)
         self.set_non_linear_power_spectrum(prob.log10k, prob.pk_z, prob.lnP)
+        if getattr(prob, "lnPL", None) is not None:
+            self.set_linear_power_spectrum(prob.plin_log10k, prob.plin_z, prob.lnPL)
         self.set_nuisance_shear_photoz(prob.src_dz)
         self.set_nuisance_shear_calib(prob.shear_m)
         if prob.lens is not None:
@@ -353,6 +371,12 @@ class Cosmolike:
             d.b1, d.bmag = _p(vals["b1"]), _p(vals["bmag"])
             d.pm = _p(vals["pm"])
             d.ia_A, d.ia_eta = p.ia_A, p.ia_eta
+            d.h = getattr(p, "h", 0.7)
+            if getattr(p, "lnPL", None) is not None:
+                for k in ("plin_log10k", "plin_z", "lnPL"):
+                    vals[k] = _arr(getattr(p, k))
+                d.plin_log10k, d.plin_z, d.lnPL = _p(vals["plin_log10k"]), _p(vals["plin_z"]), _p(vals["lnPL"])
+                d.plin_nk, d.plin_nz = len(vals["plin_log10k"]), len(vals["plin_z"])
             if getattr(p, "ia_model", 0) == 1:
                 vals["fpt"] = _arr(p.ia_ps)
                 d.ia2_A, d.ia2_eta, d.bta_A = p.ia2_A, p.ia2_eta, p.ia_bta

@@ -24,6 +24,9 @@ hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, int keep, hipStream_t s);
 hipError_t clh_launch_limber_sx_tatt(const DrawDesc* D, const int4* work, int n, hipStream_t s);
+hipError_t clh_launch_nl_forward(const DrawDesc* D, const int2* wprep, int nprep, const int4* wfwd, int nfwd,
+                                 const int4* wgl, int ngl, const int4* widft, int nidft, hipStream_t s);
+hipError_t clh_launch_nl_cl(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles, int keep,
                                 hipStream_t s);
 hipError_t clh_launch_legendre(const DrawDesc* D, const int4* work, int n, hipStream_t s);
@@ -158,6 +161,8 @@ struct Layout {
   dvec ell_f;               // Fourier-space ell bins (init_binning_fourier), like.ell
   double lmax_shear = 0;    // like.lmax_shear
   std::vector<int> mask;    // likelihood mask in the (0,1,2) block order; empty = all ones
+  bool nonlimber = false;   // !like.adopt_limber_gg: w from the non-Limber C_ell below ell 150
+  dvec kwlow;               // [ntheta][CLH_NL_L] w kernel at integer ell (ell 0 = 0)
   int nrow() const { return CLH_NLOW + n_ell; }
   int n_ss() const { return src.nbin * (src.nbin + 1) / 2; }
   int n_gs() const { return src.nbin * lens.nbin; }
@@ -178,6 +183,8 @@ struct Draw {  // one draw's cosmology + nuisance (host copy)
   dvec fpt;  // [10][fpt_n], already divided by (c/H0)^3
   int fpt_n = 0;
   double fpt_kmin = 0.0, fpt_kmax = 0.0;  // in H0/c
+  double h0 = 0.0;           // H0 / 100 km/s/Mpc (set_cosmological_parameters)
+  dvec lkL, pzL, lnPL;       // linear P(k,z) (set_linear_power_spectrum)
   Draw() { for (int i = 0; i < CLH_MAXB; i++) lens_st[i] = 1.0; }
 };
 
@@ -205,7 +212,8 @@ struct DevLayout {  // device copy of one survey layout
   int ngrp = 0;
   DBuf<int> mask;       // likelihood mask (null when the layout has none)
   size_t off_s_zc = 0, off_s_coef = 0, off_l_zc = 0, off_l_coef = 0, off_ell_ss = 0,
-         off_ell_gx = 0, off_kfold = 0, off_kfoldT = 0, off_gl = 0, off_kfull = 0, off_fill = 0, off_theta = 0;
+         off_ell_gx = 0, off_kfold = 0, off_kfoldT = 0, off_gl = 0, off_kfull = 0, off_fill = 0, off_theta = 0,
+         off_kwlow = 0, off_nltw = 0;
   void release() { buf.release(); k32.release(); ksc.release(); kbf.release(); mask.release(); valid = false; }
 };
 
@@ -254,12 +262,23 @@ struct Batch {
   int dv_size = 0;
   int nknz = 0;        // common ln P table size (0 if the draws differ)
   bool tatt = false;   // some draw uses TATT intrinsic alignments
+  // non-Limber w: per such draw two views of it (gg Limber rows at ell =
+  // 1..149 with P_nl and with P_lin) and the FFTLog work lists
+  bool nl_any = false;
+  std::vector<DrawDesc> desc_nl;
+  DBuf<DrawDesc> d_desc_nl;
+  DBuf<double> d_nl_ells;
+  DBuf<int2> d_wn_nl, d_wnl_prep, d_wnl_cl;
+  DBuf<int4> d_wgg_nl, d_wnl_fwd, d_wnl_gl, d_wnl_idft;
+  int n_wn_nl = 0, n_wgg_nl = 0, n_wnl_prep = 0, n_wnl_fwd = 0, n_wnl_gl = 0, n_wnl_idft = 0, n_wnl_cl = 0;
   int ord[3] = {0, 1, 2};
   void release() {
     d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
     d_vjp.release(); d_gscr.release(); d_run.release(); d_wlg.release(); d_ggk.release(); d_sxk.release(); d_dgk.release();
     d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa4.release();
     d_wsx.release(); d_wgg.release(); d_wga.release(); d_wmx.release(); d_wlf.release();
+    d_desc_nl.release(); d_nl_ells.release(); d_wn_nl.release(); d_wnl_prep.release(); d_wnl_cl.release();
+    d_wgg_nl.release(); d_wnl_fwd.release(); d_wnl_gl.release(); d_wnl_idft.release();
   }
 };
 
@@ -281,6 +300,7 @@ struct clh_ctx {
   dvec cache_raw;   // raw (uncalibrated, ord 0,1,2) data vector of `cur`
   dvec cache_cl;    // cl rows of `cur`
   dvec cache_bb;    // B-mode C_ss rows of `cur` (zero under NLA)
+  dvec cache_nl;    // non-Limber C_ell [nl][150] then LMAX [nl] of `cur` (empty under Limber w)
   // device copies of the layout
   DevLayout dl;                                        // device copy of L
   std::map<uint64_t, std::unique_ptr<LayoutCopy>> rl;  // layouts of a multi-layout batch
@@ -442,7 +462,7 @@ struct KernelKey {
   }
 };
 struct KernelSet {
-  dvec kfold, ell_ss, ell_gx;
+  dvec kfold, ell_ss, ell_gx, kwlow;
 };
 // folded kernels depend only on the binning and Ntable: shared process-wide
 static std::map<KernelKey, std::shared_ptr<const KernelSet>> g_kernel_cache;
@@ -459,6 +479,7 @@ static void build_kernels(Layout& L, bool full) {
       L.kfold = it->second->kfold;
       L.ell_ss = it->second->ell_ss;
       L.ell_gx = it->second->ell_gx;
+      L.kwlow = it->second->kwlow;
       L.kfull.clear();
       L.kern_valid = true;
       return;
@@ -471,12 +492,14 @@ static void build_kernels(Layout& L, bool full) {
   ks->kfold = L.kfold;
   ks->ell_ss = L.ell_ss;
   ks->ell_gx = L.ell_gx;
+  ks->kwlow = L.kwlow;
   g_kernel_cache[key] = ks;
 }
 
 static void build_kernels_uncached(Layout& L, bool full) {
   const int nt = L.ntheta, Lm = L.lmax, nrow = L.nrow();
   L.kfold.assign((size_t)4 * nrow * nt, 0.0);
+  L.kwlow.assign((size_t)nt * CLH_NL_L, 0.0);
   if (full) L.kfull.assign((size_t)4 * Lm * nt, 0.0); else L.kfull.clear();
   double lim_ss[3], lim_gx[3];
   lim_ss[0] = std::log(std::fmax(CLH_LMIN_TAB - 1., 1.0));
@@ -532,6 +555,7 @@ static void build_kernels_uncached(Layout& L, bool full) {
                        (2 - lf) * (xmin * Pmin[l] - xmax * Pmax[l]) -
                        2.0L / (2 * lf + 1) * (Pmin[l + 1] - Pmax[l + 1])));
       g[3] = (double)((1.0L / dx) * (1.0L / (4 * pi)) * (Pmin[l + 1] - Pmax[l + 1] - Pmin[l - 1] + Pmax[l - 1]));
+      if (l < CLH_NL_L) L.kwlow[(size_t)i * CLH_NL_L + l] = g[3];
       for (int k = 0; k < 4; k++) {
         if (full) L.kfull[((size_t)k * Lm + l) * nt + i] = g[k];
         double* kf = L.kfold.data() + (size_t)k * nrow * nt;
@@ -594,6 +618,16 @@ static int check_draw(clh_ctx* c, const Layout& L, const Draw& d, int idx) {
   if (!(d.omega_m > 0)) return fail(c, CLH_ERR_STATE, "draw %d: Omega_m not set", idx);
   if (L.ia_model == CLH_IA_TATT && (d.fpt_n < 2 || d.fpt.size() != (size_t)10 * d.fpt_n))
     return fail(c, CLH_ERR_STATE, "draw %d: TATT needs the FAST-PT IA tables (set_IA_PS)", idx);
+  if (L.nonlimber && L.lens.nbin > 0) {
+    if (d.lkL.size() < 2 || d.pzL.size() < 2 || d.lnPL.size() != d.lkL.size() * d.pzL.size())
+      return fail(c, CLH_ERR_STATE, "draw %d: non-Limber w needs the linear P(k,z) (set_linear_power_spectrum)", idx);
+    if (!(d.h0 > 0)) return fail(c, CLH_ERR_STATE, "draw %d: non-Limber w needs H0 (set_cosmological_parameters)", idx);
+    for (size_t i = 1; i < d.lkL.size(); i++)
+      if (!(d.lkL[i] > d.lkL[i - 1])) return fail(c, CLH_ERR_DOMAIN, "draw %d: linear log10k not increasing", idx);
+    for (size_t i = 1; i < d.pzL.size(); i++)
+      if (!(d.pzL[i] > d.pzL[i - 1])) return fail(c, CLH_ERR_DOMAIN, "draw %d: linear P z not increasing", idx);
+    if (!(d.chi_z[nc - 2] > 4.0)) return fail(c, CLH_ERR_DOMAIN, "draw %d: chi(z) must extend beyond z = 4", idx);
+  }
   for (int i = 1; i < nc; i++)
     if (!(d.chi_z[i] > d.chi_z[i - 1]) || !(d.chi[i] > d.chi[i - 1]))
       return fail(c, CLH_ERR_DOMAIN, "draw %d: chi(z) table not strictly increasing", idx);
@@ -636,6 +670,29 @@ static double bf16_value(uint16_t h) {
   return f;
 }
 
+// FFTLog transform lengths of C_cl_tomo (cosmo2D.c:4632-4654): zero padding
+// N_pad = 200, 500, 500 (density, RSD, magnification) on each side of the
+// NL_Nchi nodes, rounded up to an even length without prime factors above 7
+static long nl_fft_size(long n) {
+  for (;; n++) {
+    long m = n;
+    for (int p : {2, 3, 5, 7})
+      while (m % p == 0) m /= p;
+    if (m == 1) return n;
+  }
+}
+static void nl_sizes(int N[3], int N0[3]) {
+  static const int pad[3] = {200, 500, 500};
+  for (int j = 0; j < 3; j++) {
+    N0[j] = pad[j];
+    long raw = 2L * pad[j] + CLH_NL_NCHI;
+    if (raw % 2) raw++;
+    long n = nl_fft_size(raw);
+    if (n % 2) n = nl_fft_size(n + 1);
+    N[j] = (int)n;
+  }
+}
+
 static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   const bool full = c->legendre_mode == CLH_LEGENDRE_FULL;
   if (!L.kern_valid || (full && L.kfull.empty()))
@@ -676,6 +733,21 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
     }
   }
   D.off_fill = take(fill.size());
+  // non-Limber w: the w kernel at integer ell < 150, and cos / sin of
+  // 2 pi t / N_j for the three transform lengths
+  int nlN[3], nlN0[3];
+  nl_sizes(nlN, nlN0);
+  dvec tw;
+  for (int j = 0; j < 3; j++) {
+    if (nlN[j] > 1500) return fail(c, CLH_ERR_STATE, "non-Limber transform length %d above 1500", nlN[j]);
+    for (int t = 0; t < nlN[j]; t++) {
+      const long double ang = 2.0L * 3.14159265358979323846264338327950288L * t / nlN[j];
+      tw.push_back((double)std::cos(ang));
+      tw.push_back((double)std::sin(ang));
+    }
+  }
+  D.off_kwlow = take(L.kwlow.size());
+  D.off_nltw = take(tw.size());
   HIPCHK(c, D.buf.ensure(off));
   double* base = D.buf.p;
   auto put = [&](size_t o, const double* src, size_t n) -> hipError_t {
@@ -699,6 +771,8 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   HIPCHK(c, put(D.off_kfoldT, kfoldT.data(), kfoldT.size()));
   HIPCHK(c, put(D.off_gl, glbuf, sizeof glbuf / sizeof(double)));
   HIPCHK(c, put(D.off_theta, L.theta.data(), L.theta.size()));
+  HIPCHK(c, put(D.off_kwlow, L.kwlow.data(), L.kwlow.size()));
+  HIPCHK(c, put(D.off_nltw, tw.data(), tw.size()));
   if (!L.mask.empty()) {
     HIPCHK(c, D.mask.ensure(L.mask.size()));
     HIPCHK(c, hipMemcpyAsync(D.mask.p, L.mask.data(), L.mask.size() * sizeof(int), hipMemcpyHostToDevice,
@@ -902,6 +976,41 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   HIPCHK(c, up4(bt.d_wa4, wa, bt.n_wa));
   bt.adj_lds_ss = lds_ss;
   bt.adj_lds_gs = lds_gs;
+  // non-Limber w: the views' gg nodes and rows (ell = 1..149), then the FFTLog
+  // stages per draw (node blocks; bin x configuration x m blocks;
+  // configuration x m blocks; bin x 16-ell x 64-node tiles; bins)
+  std::vector<int2> wn_nl, wnl_prep, wnl_cl;
+  std::vector<int4> wgg_nl, wnl_fwd, wnl_gl, wnl_idft;
+  int v = 0;
+  for (int i = 0; i < bt.B; i++) {
+    if (!bt.desc[i].nlw) continue;
+    const int nl = bt.desc[i].nl;
+    for (int u = v; u < v + 2; u++) {
+      for (int sg = nl + 1; sg <= 2 * nl; sg++) wn_nl.push_back(make_int2(u, sg));
+      row_tiles(CLH_NL_L - 1, rows[1], tiles);
+      for (int zl = 0; zl < nl; zl++)
+        for (const int2& t : tiles) wgg_nl.push_back(make_int4(u, zl, t.x, t.y));
+    }
+    v += 2;
+    for (int b = 0; b < 2; b++) wnl_prep.push_back(make_int2(i, b));
+    const int nmb = (CLH_NL_NH + 255) / 256;
+    for (int zl = 0; zl < nl; zl++)
+      for (int j = 0; j < 3; j++)
+        for (int mb = 0; mb < nmb; mb++) wnl_fwd.push_back(make_int4(i, zl, j, mb));
+    for (int j = 0; j < 3; j++)
+      for (int mb = 0; mb < nmb; mb++) wnl_gl.push_back(make_int4(i, j, mb, 0));
+    for (int zl = 0; zl < nl; zl++)
+      for (int lt = 0; lt < CLH_NL_LT / 16; lt++)
+        for (int qt = 0; qt < CLH_NL_QT / 4; qt++) wnl_idft.push_back(make_int4(i, zl, lt, qt));
+    for (int zl = 0; zl < nl; zl++) wnl_cl.push_back(make_int2(i, zl));
+  }
+  HIPCHK(c, up2(bt.d_wn_nl, wn_nl, bt.n_wn_nl));
+  HIPCHK(c, up4(bt.d_wgg_nl, wgg_nl, bt.n_wgg_nl));
+  HIPCHK(c, up2(bt.d_wnl_prep, wnl_prep, bt.n_wnl_prep));
+  HIPCHK(c, up4(bt.d_wnl_fwd, wnl_fwd, bt.n_wnl_fwd));
+  HIPCHK(c, up4(bt.d_wnl_gl, wnl_gl, bt.n_wnl_gl));
+  HIPCHK(c, up4(bt.d_wnl_idft, wnl_idft, bt.n_wnl_idft));
+  HIPCHK(c, up2(bt.d_wnl_cl, wnl_cl, bt.n_wnl_cl));
   return CLH_OK;
 
 }
@@ -934,11 +1043,15 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     const int nrow = ells ? (int)ells->size() : L.nrow();
     const size_t node_sz = node_size(ns, nl);
     const bool tatt = L.ia_model == CLH_IA_TATT;
+    const bool nlw = L.nonlimber && nl > 0 && !ells;
     woff[i + 1] = woff[i] + al((size_t)(ns + nl) * L.n_a) + al(node_sz) +
                   al((size_t)(L.n_ss() + L.n_gs() + nl) * nrow) +
-                  (tatt ? al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS) + al((size_t)L.n_ss() * nrow) : 0);
+                  (tatt ? al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS) + al((size_t)L.n_ss() * nrow) : 0) +
+                  (nlw ? clh_nl_off(nl).total + 2 * (al(node_sz) + al((size_t)(L.n_ss() + L.n_gs() + nl) *
+                                                                       (CLH_NL_L - 1)))
+                       : 0);
     toff[i + 1] = toff[i] + 2 * d.chi_z.size() + 2 * d.G_z.size() + d.lk.size() + d.pz.size() + d.lnP.size() +
-                  (tatt ? d.fpt.size() : 0) + 32;
+                  (tatt ? d.fpt.size() : 0) + (nlw ? d.lkL.size() + d.pzL.size() + d.lnPL.size() : 0) + 32;
     dvoff[i + 1] = dvoff[i] + L.dv_size();
     max_nrow = std::max(max_nrow, nrow);
     if (lays[i] != lays[0]) uniform = false;
@@ -954,6 +1067,7 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   std::vector<double> htab(toff[B]);
   bt.desc.assign(B, DrawDesc());
   bt.par.assign(B, DrawParams());
+  std::vector<std::pair<int, double*>> nl_views;  // (draw, start of its two views' buffers)
   HIPCHK(c, bt.d_par.ensure(B));
   for (int i = 0; i < B; i++) {
     const Draw& d = *draws[i];
@@ -969,6 +1083,13 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.chi_z = push(d.chi_z); q.chi_v = push(d.chi);
     q.G_z = push(d.G_z); q.G_v = push(d.G);
     q.lk = push(d.lk); q.pz = push(d.pz); q.lnP = push(d.lnP);
+    const bool nlw = L.nonlimber && nl > 0 && !ells;
+    if (nlw) {
+      q.lkL = push(d.lkL); q.pzL = push(d.pzL); q.lnPL = push(d.lnPL);
+      q.nkL = (int)d.lkL.size(); q.nzL = (int)d.pzL.size();
+      q.h0 = d.h0;
+      nl_sizes(q.nl_N, q.nl_N0);
+    }
     if (L.ia_model == CLH_IA_TATT) {
       q.fpt = push(d.fpt);
       q.fpt_n = d.fpt_n;
@@ -1015,9 +1136,26 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.gtab = wb;
     q.nodes = wb + al(gtab_sz);
     q.cl = q.nodes + al(node_sz);
+    double* wend = q.cl + al((size_t)(L.n_ss() + L.n_gs() + nl) * nrow);
     if (L.ia_model == CLH_IA_TATT) {
-      q.tatt = q.cl + al((size_t)(L.n_ss() + L.n_gs() + nl) * nrow);
+      q.tatt = wend;
       q.cl_bb = q.tatt + al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS);
+      wend = q.cl_bb + al((size_t)L.n_ss() * nrow);
+    }
+    {  // ln-ell grid of the gs / gg tables (cosmo2D.c:2204-2207, 2703-2706)
+      q.lim_gx[0] = std::log(std::fmax((double)CLH_LMIN_TAB, 1.0));
+      q.lim_gx[1] = std::log((double)(L.lmax + 1));
+      q.lim_gx[2] = (q.lim_gx[1] - q.lim_gx[0]) / ((double)L.n_ell - 1.0);
+      q.lim_ss[0] = std::log(std::fmax(CLH_LMIN_TAB - 1., 1.0));
+      q.lim_ss[1] = q.lim_gx[1];
+      q.lim_ss[2] = (q.lim_ss[1] - q.lim_ss[0]) / ((double)L.n_ell - 1.0);
+    }
+    if (nlw) {
+      q.nlw = wend;
+      wend += clh_nl_off(nl).total;
+      q.nl_tw = lay + DL.off_nltw;
+      q.nl_pw = lay + DL.off_kwlow;
+      nl_views.push_back({i, wend});
     }
     q.dv = dv_out ? dv_out + dvoff[i] : bt.d_dv.p + dvoff[i];
     dv_starts(L, ord, q.dv_start);
@@ -1036,6 +1174,48 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     }
     q.apply_calib = apply_calib;
   }
+  // non-Limber views: draw copies that integrate the gg Limber rows at
+  // ell = 1..149, with P_nl and with P_lin, into their own nodes and rows
+  bt.desc_nl.clear();
+  if (!nl_views.empty()) {
+    if (!bt.d_nl_ells.p) {
+      dvec e(CLH_NL_L - 1);
+      for (int l = 1; l < CLH_NL_L; l++) e[l - 1] = (double)l;
+      HIPCHK(c, bt.d_nl_ells.ensure(e.size()));
+      HIPCHK(c, hipMemcpy(bt.d_nl_ells.p, e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    for (auto& v : nl_views) {
+      DrawDesc& q = bt.desc[v.first];
+      const Layout& L = *lays[v.first];
+      const size_t node_sz = node_size(L.src.nbin, L.lens.nbin);
+      const size_t cl_sz = (size_t)(L.n_ss() + L.n_gs() + L.lens.nbin) * (CLH_NL_L - 1);
+      double* p = v.second;
+      for (int lin = 0; lin < 2; lin++) {
+        DrawDesc u = q;
+        u.ell_ss = u.ell_gx = bt.d_nl_ells.p;
+        u.nrow = CLH_NL_L - 1;
+        u.nodes = p; p += al(node_sz);
+        u.cl = p; p += al(cl_sz);
+        u.dv = nullptr;
+        u.nlw = nullptr;
+        u.tatt = nullptr;
+        u.cl_bb = nullptr;
+        u.kfull = nullptr;
+        u.fill = nullptr;
+        u.apply_calib = 0;
+        if (lin) {
+          u.lk = q.lkL; u.pz = q.pzL; u.lnP = const_cast<double*>(q.lnPL);
+          u.nk = q.nkL; u.nzp = q.nzL;
+        }
+        (lin ? q.nl_llin : q.nl_lnl) = u.cl + (size_t)(L.n_ss() + L.n_gs()) * (CLH_NL_L - 1);
+        bt.desc_nl.push_back(u);
+      }
+    }
+    HIPCHK(c, bt.d_desc_nl.ensure(bt.desc_nl.size()));
+    HIPCHK(c, hipMemcpyAsync(bt.d_desc_nl.p, bt.desc_nl.data(), bt.desc_nl.size() * sizeof(DrawDesc),
+                             hipMemcpyHostToDevice, c->stream));
+  }
+  bt.nl_any = !nl_views.empty();
   HIPCHK(c, hipMemcpyAsync(bt.d_tab.p, htab.data(), toff[B] * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, bt.d_desc.ensure(B));
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
@@ -1103,6 +1283,12 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   else
     HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, mixed, bt.keep_ckpt ? 1 : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
+  if (legendre && bt.nl_any) {  // non-Limber w: FP64 in every precision mode
+    HIPCHK(c, clh_launch_nodes(bt.d_desc_nl.p, bt.d_wn_nl.p, bt.n_wn_nl, s));
+    HIPCHK(c, clh_launch_limber_gg(bt.d_desc_nl.p, bt.d_wgg_nl.p, bt.n_wgg_nl, 0, c->gg_lds[0] ? bt.gg_stage : 0, 0, s));
+    HIPCHK(c, clh_launch_nl_forward(bt.d_desc.p, bt.d_wnl_prep.p, bt.n_wnl_prep, bt.d_wnl_fwd.p, bt.n_wnl_fwd,
+                                    bt.d_wnl_gl.p, bt.n_wnl_gl, bt.d_wnl_idft.p, bt.n_wnl_idft, s));
+  }
   if (legendre) {
     if (c->legendre_mode == CLH_LEGENDRE_FULL)
       HIPCHK(c, clh_launch_legendre_full(bt.d_desc.p, bt.d_wl2.p, bt.n_wl2, bt.max_nrow, s));
@@ -1110,6 +1296,7 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
       HIPCHK(c, clh_launch_legendre_mx(bt.d_desc.p, bt.d_wmx.p, bt.n_wmx, c->precision, s));
     else
       HIPCHK(c, clh_launch_legendre(bt.d_desc.p, bt.d_wlf.p, bt.n_wlf, s));
+    if (bt.nl_any) HIPCHK(c, clh_launch_nl_cl(bt.d_desc.p, bt.d_wnl_cl.p, bt.n_wnl_cl, s));
   }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[5], s));
   bt.state_valid = true;
@@ -1132,6 +1319,13 @@ static int ensure_current(clh_ctx* c) {
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->cache_cl.data(), c->single.desc[0].cl, ncl * sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
+  c->cache_nl.clear();
+  if (c->single.desc[0].nlw) {
+    const ClhNlOff off = clh_nl_off(c->L.lens.nbin);
+    c->cache_nl.resize((size_t)c->L.lens.nbin * (CLH_NL_L + 1));
+    HIPCHK(c, hipMemcpyAsync(c->cache_nl.data(), c->single.desc[0].nlw + off.used,
+                             c->cache_nl.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
   c->cache_bb.assign((size_t)c->L.n_ss() * c->L.nrow(), 0.0);
   if (c->single.desc[0].cl_bb)
     HIPCHK(c, hipMemcpyAsync(c->cache_bb.data(), c->single.desc[0].cl_bb, c->cache_bb.size() * sizeof(double),
@@ -1251,6 +1445,7 @@ int clh_set_cosmological_parameters(clh_ctx* c, double om, double hubble) {
   if (!c) return CLH_ERR_ARG;
   if (std::isnan(om) || std::isnan(hubble)) return fail(c, CLH_ERR_NAN, "NaN cosmological parameter");
   c->cur.omega_m = om;
+  if (hubble > 0) c->cur.h0 = hubble / 100.0;
   c->version++;
   return CLH_OK;
 }
@@ -1281,6 +1476,32 @@ int clh_set_non_linear_power_spectrum(clh_ctx* c, const double* lk, int nk, cons
   c->cur.pz.assign(z, z + nz);
   c->cur.lnP.assign(lnP, lnP + (size_t)nk * nz);
   c->version++;
+  return CLH_OK;
+}
+
+// set_linear_power_spectrum (generic_interface.cpp:1194-1270): ln P_lin
+// k-major, (Mpc/h)^3, k in h/Mpc; read by the non-Limber w only
+int clh_set_linear_power_spectrum(clh_ctx* c, const double* lk, int nk, const double* z, int nz,
+                                  const double* lnP) {
+  if (!c) return CLH_ERR_ARG;
+  if (!lk || !z || !lnP || nk < 2 || nz < 2) return fail(c, CLH_ERR_ARG, "set_linear_power_spectrum: sizes");
+  if (has_nan(lk, nk) || has_nan(z, nz) || has_nan(lnP, nk * nz)) return fail(c, CLH_ERR_NAN, "P_lin(k,z): NaN");
+  c->cur.lkL.assign(lk, lk + nk);
+  c->cur.pzL.assign(z, z + nz);
+  c->cur.lnPL.assign(lnP, lnP + (size_t)nk * nz);
+  c->version++;
+  return CLH_OK;
+}
+
+// like.adopt_limber_gg (generic_interface.cpp:269): 1 Limber w(theta) at every
+// ell, 0 the non-Limber C_ell below ell 150 (C_cl_tomo)
+int clh_set_adopt_limber_gg(clh_ctx* c, int limber) {
+  if (!c) return CLH_ERR_ARG;
+  if (limber != 0 && limber != 1) return fail(c, CLH_ERR_ARG, "adopt_limber_gg must be 0 or 1");
+  if (c->L.nonlimber != (limber == 0)) {
+    c->L.nonlimber = limber == 0;
+    c->version++;
+  }
   return CLH_OK;
 }
 
@@ -1666,6 +1887,20 @@ int clh_cl_tables(clh_ctx* c, double* ss, double* gs, double* gg) {
   return CLH_OK;
 }
 
+// C_cl_tomo (cosmo2D.c:4528-4769) as w_gg_tomo(limber = 0) reads it
+int clh_cl_nonlimber(clh_ctx* c, double* cl, int* lmax) {
+  if (!c || !cl) return CLH_ERR_ARG;
+  if (!c->L.nonlimber) return fail(c, CLH_ERR_STATE, "non-Limber w not selected (clh_set_adopt_limber_gg)");
+  if (c->L.lens.nbin == 0) return fail(c, CLH_ERR_STATE, "lens sample not set");
+  int rc = ensure_current(c);
+  if (rc) return rc;
+  const int nl = c->L.lens.nbin;
+  std::copy(c->cache_nl.begin(), c->cache_nl.begin() + (size_t)nl * CLH_NL_L, cl);
+  if (lmax)
+    for (int i = 0; i < nl; i++) lmax[i] = (int)c->cache_nl[(size_t)nl * CLH_NL_L + i];
+  return CLH_OK;
+}
+
 // C_ss_tomo_limber(l, ni, nj, EE = 0) tables (cosmo2D.c:1617): the B mode,
 // zero under NLA
 int clh_cl_tables_bb(clh_ctx* c, double* bb) {
@@ -1720,6 +1955,12 @@ static int check_upload(clh_ctx* c, const clh_draw& s, const Layout& L, int i) {
       return fail(c, CLH_ERR_ARG, "draw %d: TATT needs the FAST-PT IA tables", i);
     if (has_nan(s.fpt, 10 * s.fpt_n)) return fail(c, CLH_ERR_NAN, "draw %d: NaN in the IA tables", i);
   }
+  if (L.nonlimber && L.lens.nbin > 0) {
+    if (!s.plin_log10k || !s.plin_z || !s.lnPL || s.plin_nk < 2 || s.plin_nz < 2 || !(s.h > 0))
+      return fail(c, CLH_ERR_ARG, "draw %d: non-Limber w needs the linear P(k,z) and h", i);
+    if (has_nan(s.plin_log10k, s.plin_nk) || has_nan(s.plin_z, s.plin_nz) || has_nan(s.lnPL, s.plin_nk * s.plin_nz))
+      return fail(c, CLH_ERR_NAN, "draw %d: NaN in the linear P(k,z)", i);
+  }
   return CLH_OK;
 }
 
@@ -1746,6 +1987,12 @@ static void fill_draw(Draw& d, const clh_draw& s, const Layout& L) {
   }
   d.ia_A = s.ia_A;
   d.ia_eta = s.ia_eta;
+  d.h0 = s.h;
+  if (s.plin_log10k && s.plin_z && s.lnPL && s.plin_nk > 0 && s.plin_nz > 0) {
+    d.lkL.assign(s.plin_log10k, s.plin_log10k + s.plin_nk);
+    d.pzL.assign(s.plin_z, s.plin_z + s.plin_nz);
+    d.lnPL.assign(s.lnPL, s.lnPL + (size_t)s.plin_nk * s.plin_nz);
+  }
   if (L.ia_model == CLH_IA_TATT) {
     for (int i = 0; i < L.src.nbin; i++) {
       d.ia2_bin[i] = s.ia2_bin ? s.ia2_bin[i] : 0.0;
@@ -1881,7 +2128,7 @@ long long clh_batch_datavector_offset(clh_ctx* c, int i) {
 // the context asks for it and the batch has one layout.  `changed`: some
 // descriptor changed on the host and must be uploaded.
 static int prepare_checkpoint(clh_ctx* c, Batch& cb, bool& changed) {
-  cb.keep_ckpt = c->vjp_ckpt && cb.uniform && !cb.nrow_custom && !cb.tatt;
+  cb.keep_ckpt = c->vjp_ckpt && cb.uniform && !cb.nrow_custom && !cb.tatt && !cb.nl_any;
   if (cb.keep_ckpt) {
     const size_t nl = cb.lays[0]->lens.nbin, nr = cb.lays[0]->nrow();
     cb.ggk_per = (nl * nr * 3 * CLH_NP_GG + 31) & ~size_t(31);
@@ -2055,7 +2302,7 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   if (rc0) return rc0;
   for (int i = 0; i < bt.B; i++) bt.desc[i].dv = bt.d_dv.p + bt.dv_off[i];
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
-  const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt;  // the VJP refuses TATT
+  const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt && !bt.nl_any;  // the VJP refuses these
   DBuf<double> dvbar, glnp;  // unit cotangent and ln P gradient of the adjoint stages
   if (adj) {
     hipError_t e = dvbar.ensure((size_t)bt.dv_off[bt.B]);
@@ -2152,6 +2399,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
   if (!bt.uniform) return fail(c, CLH_ERR_ARG, "VJP needs a batch with one layout for every draw");
   if (bt.tatt) return fail(c, CLH_ERR_ARG, "VJP of TATT intrinsic alignments is not implemented");
+  if (bt.nl_any) return fail(c, CLH_ERR_ARG, "VJP of the non-Limber w(theta) is not implemented");
   const Layout& L = *bt.lays[0];
   if (!ord_ok(L, ord))
     return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");
@@ -2288,12 +2536,12 @@ static int n_shear_idx(int ns, int ni, int nj) {  // N_shear: redshift_spline.c:
 }
 
 // interpol1d over one table column (basics.c:457-480)
-static double table_lookup(const clh_ctx* c, int col, bool ss, double l) {
+static double table_lookup(const clh_ctx* c, int col, bool ss, double l, bool bb = false) {
   const Layout& L = c->L;
   const double lim0 = std::log(std::fmax(ss ? CLH_LMIN_TAB - 1. : (double)CLH_LMIN_TAB, 1.0));
   const double lim1 = std::log((double)(L.lmax + 1));
   const double dx = (lim1 - lim0) / ((double)L.n_ell - 1.);
-  const double* f = c->cache_cl.data() + (size_t)col * L.nrow() + CLH_NLOW;
+  const double* f = (bb ? c->cache_bb.data() : c->cache_cl.data()) + (size_t)col * L.nrow() + CLH_NLOW;
   const double x = std::log(l);
   if (x < lim0) return f[0];
   const double r = (x - lim0) / dx;
@@ -2340,7 +2588,8 @@ double w_gammat_tomo(const int nt, const int ni, const int nj, const int limber)
 double w_gg_tomo(const int nt, const int ni, const int nj, const int limber) {
   clh_ctx* c = clh_default();
   if (!c) return nanv();
-  if (limber != 1) { fail(c, CLH_ERR_ARG, "non-Limber w(theta) is not implemented (limber=1 only)"); return nanv(); }
+  if (limber != 0 && limber != 1) { fail(c, CLH_ERR_ARG, "limber must be 0 or 1"); return nanv(); }
+  if (clh_set_adopt_limber_gg(c, limber)) return nanv();
   if (nt < 0 || nt > c->L.ntheta - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number nt = %d", nt); return nanv(); }
   if (ni < 0 || ni > c->L.lens.nbin - 1 || nj < 0 || nj > c->L.lens.nbin - 1) {
     fail(c, CLH_ERR_ARG, "error in selecting bin number (ni,nj) = [%d,%d]", ni, nj);
@@ -2357,8 +2606,7 @@ double C_ss_tomo_limber(const double l, const int ni, const int nj, const int EE
   const int ns = c->L.src.nbin;
   if (ni < 0 || ni > ns - 1 || nj < 0 || nj > ns - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number (ni,nj) = [%d,%d]", ni, nj); return nanv(); }
   if (ensure_current(c)) return nanv();
-  if (EE != 1) return 0.0;  // BB table is identically zero under NLA
-  return table_lookup(c, n_shear_idx(ns, ni, nj), true, l);
+  return table_lookup(c, n_shear_idx(ns, ni, nj), true, l, EE != 1);  // EE = 0: the B mode
 }
 
 double C_gs_tomo_limber(const double l, const int ni, const int nj) {

@@ -40,6 +40,45 @@
 #define CLH_LUT_SX 3
 #define CLH_LUT_GG 9
 
+// Non-Limber clustering (C_cl_tomo, cosmolike/cosmo2D.c:4528-4769): NL_Nchi
+// log-spaced chi nodes (structs.c:272), ell < LMAX_NOLIMBER (structs.c:45) in
+// blocks of 16 (cosmo2D.c:4672), three FFTLog configurations (density, RSD,
+// magnification) whose transform lengths N_j and zero padding N0_j follow
+// from NL_Nchi; a draw's work area holds, at the offsets of ClhNlOff:
+//   x[512]                 chi nodes (Mpc)
+//   fb[nl][3][512]         FFTLog inputs f(x) / x^nu
+//   T[nl][3][NH][2]        windowed forward transforms (complex)
+//   G[3][LT][NH][2]        g_l(nu + i eta_m) e^{-i eta_m ln base_l} per ell row
+//   part[nl][LT][QT]       partial q sums of the C_ell integrand
+//   used[nl][L]            the C_ell the w sum reads below LMAX_NOLIMBER
+//   lmx[nl]                the ell where each bin switched to Limber (LMAX)
+//   scal[8]                chi_min, dln chi, c/H0 in Mpc
+#define CLH_NL_NCHI 500
+#define CLH_NL_L 150
+#define CLH_NL_LT 160          // ell rows padded to 16
+#define CLH_NL_NH 752          // N/2 + 1 padded to even, N <= 1500
+#define CLH_NL_QT 32           // q tiles of 16 columns (512 columns)
+struct ClhNlOff {
+  size_t x, fb, T, G, part, used, lmx, scal, total;
+};
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline ClhNlOff clh_nl_off(int nl) {
+  ClhNlOff o;
+  size_t p = 0;
+  o.x = p; p += 512;
+  o.fb = p; p += (size_t)nl * 3 * 512;
+  o.T = p; p += (size_t)nl * 3 * CLH_NL_NH * 2;
+  o.G = p; p += (size_t)3 * CLH_NL_LT * CLH_NL_NH * 2;
+  o.part = p; p += (size_t)nl * CLH_NL_LT * CLH_NL_QT;
+  o.used = p; p += (size_t)nl * CLH_NL_L;
+  o.lmx = p; p += (size_t)nl;
+  o.scal = p; p += 8;
+  o.total = (p + 31) & ~(size_t)31;
+  return o;
+}
+
 // Everything one draw needs on the device.  Plain data; lives in an array in
 // device memory and is read with uniform (scalar) loads.
 struct DrawParams;
@@ -120,6 +159,18 @@ struct DrawDesc {
   double fpt_l0, fpt_l1, fpt_dl;  // ln k range (k in H0/c) and step (l1 - l0) / fpt_n
   double* ggk;          // [nl][nrow][3][96] gg integrand checkpoint of the forward (bracket t,
                         // P fac, dW_RSD/d dz per row and node) for the VJP, or null
+  // non-Limber w (like.adopt_limber_gg = 0), nlw null otherwise
+  double* nlw;                // work area, ClhNlOff offsets
+  const double* nl_tw;        // [3][2 N_j] cos, sin of 2 pi t / N_j (layout)
+  const double* nl_pw;        // [ntheta][CLH_NL_L] w kernel at integer ell (layout)
+  const double* nl_lnl;       // [nl][CLH_NL_L - 1] gg Limber rows at ell = 1..149 with P_nl
+  const double* nl_llin;      // the same with the linear P(k,z)
+  const double* lkL;          // linear P(k,z): log10 k, z, ln P k-major
+  const double* pzL;
+  const double* lnPL;
+  int nkL, nzL;
+  int nl_N[3], nl_N0[3];      // transform lengths and zero padding per configuration
+  double h0;                  // H0 / 100 km/s/Mpc
   double* g_src;        // [ns]
   double* g_lens;       // [nl]
   double* g_b1;         // [nl]

@@ -76,7 +76,8 @@ int clh_init_binning_real_space(clh_ctx* ctx, int ntheta, double theta_min_arcmi
  * TATT computes the E- and B-mode C_ss (xi+- = EE +- BB transforms) and the
  * TATT gamma_t in FP64 in every precision mode; clh_batch_vjp refuses it. */
 int clh_init_ia(clh_ctx* ctx, int ia_model, int ia_redshift_evol);
-/* set_cosmological_parameters (generic_interface.cpp:912-939) */
+/* set_cosmological_parameters (generic_interface.cpp:912-939); H0 in
+ * km/s/Mpc sets c/H0 in Mpc for the non-Limber w */
 int clh_set_cosmological_parameters(clh_ctx* ctx, double omega_matter, double hubble);
 /* set_distances (generic_interface.cpp:1085-1143): z, chi [Mpc/h], n >= 5 */
 int clh_set_distances(clh_ctx* ctx, const double* z, const double* chi, int n);
@@ -86,6 +87,16 @@ int clh_set_growth(clh_ctx* ctx, const double* z, const double* G, int n);
  * lnP k-major, lnP[i*nz + j], P in (Mpc/h)^3, k in h/Mpc */
 int clh_set_non_linear_power_spectrum(clh_ctx* ctx, const double* log10k, int nk,
                                       const double* z, int nz, const double* lnP);
+/* set_linear_power_spectrum (generic_interface.cpp:1194-1270), the same
+ * layout; read only by the non-Limber w (p_lin, cosmo3D.c:306-358) */
+int clh_set_linear_power_spectrum(clh_ctx* ctx, const double* log10k, int nk,
+                                  const double* z, int nz, const double* lnP);
+/* like.adopt_limber_gg (generic_interface.cpp:269): 1 (this library's
+ * default) Limber w(theta) at every ell; 0 (the reference's default) the
+ * non-Limber C_gg of C_cl_tomo (cosmo2D.c:4528-4769) below ell 150, FFTLog
+ * with tolerance 0.01, Limber above.  Needs the linear P(k,z) and H0;
+ * clh_batch_vjp refuses it. */
+int clh_set_adopt_limber_gg(clh_ctx* ctx, int limber);
 /* set_source_sample / set_lens_sample (generic_interface.cpp:1907-1999,
  * 1802-1883): row-major (nrows, ntomo+1) table, column 0 = z */
 int clh_set_source_sample(clh_ctx* ctx, const double* table, int nrows, int ntomo);
@@ -158,13 +169,17 @@ int clh_datavector_size(clh_ctx* ctx);
 int clh_compute_3x2pt_real(clh_ctx* ctx, const int ord[3], double* dv, int n);
 /* per-probe arrays as the reference's static caches hold them (q = pair*Ntheta + i),
  * without calibration: xi_pm_tomo (cosmo2D.c:353), w_gammat_tomo (:562),
- * w_gg_tomo with limber=1 (:739) */
+ * w_gg_tomo (:739; Limber or not as clh_set_adopt_limber_gg selects) */
 int clh_xi_pm(clh_ctx* ctx, double* xip, double* xim);
 int clh_gammat(clh_ctx* ctx, double* out);
 int clh_wtheta(clh_ctx* ctx, double* out);
 /* Limber tables at the N_ell log-spaced ell (C_ss/gs/gg_tomo_limber tables,
  * cosmo2D.c:1617, 2195, 2693): ss[n_ss][N_ell], gs[n_gs][N_ell], gg[nl][N_ell] */
 int clh_cl_tables(clh_ctx* ctx, double* ss, double* gs, double* gg);
+/* C_cl_tomo's C_gg as w_gg_tomo(limber = 0) uses it: cl[nl][150] at ell
+ * 0..149 (non-Limber below each bin's LMAX, Limber from there) and LMAX[nl]
+ * (NULL = skip) */
+int clh_cl_nonlimber(clh_ctx* ctx, double* cl, int* lmax);
 /* the B-mode C_ss table, C_ss_tomo_limber(l, ni, nj, EE = 0) (cosmo2D.c:1617):
  * bb[n_ss][N_ell]; zero under NLA */
 int clh_cl_tables_bb(clh_ctx* ctx, double* bb);
@@ -192,6 +207,10 @@ typedef struct {
   double ia2_A; double ia2_eta; const double* ia2_bin;
   double bta_A; const double* bta_bin;
   const double* fpt; int fpt_n; double fpt_kmin; double fpt_kmax;
+  /* non-Limber w only (clh_set_adopt_limber_gg(ctx, 0)): h = H0/100 and the
+   * linear P(k,z) as clh_set_linear_power_spectrum takes it */
+  double h;
+  const double* plin_log10k; const double* plin_z; const double* lnPL; int plin_nk; int plin_nz;
 } clh_draw;
 
 /* uploads B draws into device memory owned by the context */
@@ -273,7 +292,9 @@ int clh_set_vjp_checkpoint(clh_ctx* ctx, int on);
 /* ---------------------------------------- reference-named scalar entries */
 /* Identical signatures to cosmolike/cosmo2D.h:30-96, served from clh_default().
  * On an error they return NaN and record the message (clh_last_error on
- * clh_default()) instead of exiting. */
+ * clh_default()) instead of exiting.  xi_pm_tomo and w_gammat_tomo take
+ * limber = 1 only; w_gg_tomo's limber selects the Limber (1) or non-Limber
+ * (0) w for the default context, as clh_set_adopt_limber_gg does. */
 double xi_pm_tomo(const int pm, const int nt, const int ni, const int nj, const int limber);
 double w_gammat_tomo(const int nt, const int ni, const int nj, const int limber);
 double w_gg_tomo(const int nt, const int ni, const int nj, const int limber);
