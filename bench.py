@@ -42,6 +42,8 @@ def _build_one(item):
     kind, spec = item
     if kind == "full":
         return synth.full_3x2pt_problem(spec)
+    if kind == "nl":
+        return synth.nonlimber_problem(spec)
     cs, ns, nl, nt = spec
     return synth.ragged_3x2pt_problem(cs, ns, nl, nt)
 
@@ -60,6 +62,11 @@ def _build(items):
 def make_draws(B, seed):
     from cosmolike_amd import synth
     return _build([("full", cos) for cos, *_ in synth.random_draws(B, seed=seed, ragged=False)])
+
+
+def make_nonlimber_draws(B, seed):
+    from cosmolike_amd import synth
+    return _build([("nl", cos) for cos, *_ in synth.random_draws(B, seed=seed, ragged=False)])
 
 
 def make_ragged_draws(n, seed):
@@ -157,6 +164,8 @@ def main():
                     help="also copy 512 MiB device to device once (HBM-counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--ragged", type=int, default=512,
                     help="draws of the multi-layout leg (3-10 src x 2-6 lens bins, 20-30 theta bins; 0 = skip)")
+    ap.add_argument("--nonlimber", type=int, default=128,
+                    help="draws of the non-Limber w leg (adopt_limber_gg = 0; 0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the draws of a batch "
@@ -169,7 +178,8 @@ def main():
     t0 = time.perf_counter()
     draws = make_draws(args.batch, args.seed)
     probs = make_ragged_draws(args.ragged, args.seed + 1) if args.ragged > 0 else []
-    log(f"bench: built {len(draws)} + {len(probs)} draws in {time.perf_counter() - t0:.1f}s")
+    nlprobs = make_nonlimber_draws(args.nonlimber, args.seed + 2) if args.nonlimber > 0 else []
+    log(f"bench: built {len(draws)} + {len(probs)} + {len(nlprobs)} draws in {time.perf_counter() - t0:.1f}s")
 
     import torch
     from cosmolike_amd import Cosmolike, LEGENDRE_FOLDED, LEGENDRE_FULL
@@ -327,6 +337,34 @@ def main():
         log(f"bench: ragged {ragged['value']:.1f} dv/s")
         del cr, rdv
 
+    # non-Limber w (the reference's default adopt_limber_gg = 0): FFTLog C_gg
+    # below ell 150 for every lens bin of every draw, FP64, graph replay
+    nonlimber = None
+    if args.nonlimber > 0:
+        t1 = time.perf_counter()
+        cn = Cosmolike(device=0).load(nlprobs[0], mode)
+        cn.batch_upload(nlprobs)
+        ndv = torch.empty(len(nlprobs) * n, dtype=torch.float64, device="cuda:0")
+        cn.batch_use_graph(True)
+        for _ in range(2):
+            cn.batch_forward(ndv.data_ptr())
+        cn.synchronize()
+        torch.cuda.synchronize()
+        log(f"bench: non-Limber batch of {len(nlprobs)} draws set up in {time.perf_counter() - t1:.1f}s")
+        nn = max(3, args.steps // 4)
+        t1 = time.perf_counter()
+        for _ in range(nn):
+            cn.batch_forward(ndv.data_ptr())
+        cn.synchronize()
+        torch.cuda.synchronize()
+        eln = time.perf_counter() - t1
+        assert torch.isfinite(ndv).all().item()
+        nonlimber = {"value": len(nlprobs) * nn / eln, "unit": "data vectors/s", "draws": len(nlprobs),
+                     "ms_per_step": 1e3 * eln / nn, "precision": "fp64",
+                     "w": "non-Limber C_gg below ell 150 (C_cl_tomo FFTLog, tolerance 0.01), Limber above"}
+        log(f"bench: non-Limber {nonlimber['value']:.1f} dv/s")
+        del cn, ndv
+
     # the plain FP64 drop-in path: one draw at a time through the setters and
     # clh_compute_3x2pt_real (host buffers in and out), as the reference's
     # interface calls it
@@ -369,6 +407,7 @@ def main():
         "fwd_plus_vjp_value": pair_value,
         "plain_fp64_single_draw": plain,
         "ragged_batch": ragged,
+        "nonlimber_batch": nonlimber,
         "speedup_vs_plain_path": {"forward": value / plain["value"],
                                   "forward_plus_vjp": (pair_value / plain["value"]) if pair_value else None},
         "roofline": roofline,

@@ -25,7 +25,8 @@ hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStrea
 hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, int keep, hipStream_t s);
 hipError_t clh_launch_limber_sx_tatt(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_nl_forward(const DrawDesc* D, const int2* wprep, int nprep, const int4* wfwd, int nfwd,
-                                 const int4* wgl, int ngl, const int4* widft, int nidft, hipStream_t s);
+                                 const int4* wgl, int ngl, const int4* wchk, int nchk, const int2* wbin, int nbin,
+                                 const int4* widft, int nidft, hipStream_t s);
 hipError_t clh_launch_nl_cl(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles, int keep,
                                 hipStream_t s);
@@ -269,8 +270,9 @@ struct Batch {
   DBuf<DrawDesc> d_desc_nl;
   DBuf<double> d_nl_ells;
   DBuf<int2> d_wn_nl, d_wnl_prep, d_wnl_cl;
-  DBuf<int4> d_wgg_nl, d_wnl_fwd, d_wnl_gl, d_wnl_idft;
-  int n_wn_nl = 0, n_wgg_nl = 0, n_wnl_prep = 0, n_wnl_fwd = 0, n_wnl_gl = 0, n_wnl_idft = 0, n_wnl_cl = 0;
+  DBuf<int4> d_wgg_nl, d_wnl_fwd, d_wnl_gl, d_wnl_chk, d_wnl_idft;
+  int n_wn_nl = 0, n_wgg_nl = 0, n_wnl_prep = 0, n_wnl_fwd = 0, n_wnl_gl = 0, n_wnl_chk = 0, n_wnl_idft = 0,
+      n_wnl_cl = 0;
   int ord[3] = {0, 1, 2};
   void release() {
     d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
@@ -278,7 +280,7 @@ struct Batch {
     d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa4.release();
     d_wsx.release(); d_wgg.release(); d_wga.release(); d_wmx.release(); d_wlf.release();
     d_desc_nl.release(); d_nl_ells.release(); d_wn_nl.release(); d_wnl_prep.release(); d_wnl_cl.release();
-    d_wgg_nl.release(); d_wnl_fwd.release(); d_wnl_gl.release(); d_wnl_idft.release();
+    d_wgg_nl.release(); d_wnl_fwd.release(); d_wnl_gl.release(); d_wnl_chk.release(); d_wnl_idft.release();
   }
 };
 
@@ -978,9 +980,9 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   bt.adj_lds_gs = lds_gs;
   // non-Limber w: the views' gg nodes and rows (ell = 1..149), then the FFTLog
   // stages per draw (node blocks; bin x configuration x m blocks;
-  // configuration x m blocks; bin x 16-ell x 64-node tiles; bins)
+  // configuration x m blocks; bin x 16-ell x 256-node tiles; bins)
   std::vector<int2> wn_nl, wnl_prep, wnl_cl;
-  std::vector<int4> wgg_nl, wnl_fwd, wnl_gl, wnl_idft;
+  std::vector<int4> wgg_nl, wnl_fwd, wnl_gl, wnl_chk, wnl_idft;
   int v = 0;
   for (int i = 0; i < bt.B; i++) {
     if (!bt.desc[i].nlw) continue;
@@ -1000,8 +1002,10 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
     for (int j = 0; j < 3; j++)
       for (int mb = 0; mb < nmb; mb++) wnl_gl.push_back(make_int4(i, j, mb, 0));
     for (int zl = 0; zl < nl; zl++)
+      for (int qt = 0; qt < CLH_NL_QT / 16; qt++) wnl_chk.push_back(make_int4(i, zl, 0, qt));
+    for (int zl = 0; zl < nl; zl++)
       for (int lt = 0; lt < CLH_NL_LT / 16; lt++)
-        for (int qt = 0; qt < CLH_NL_QT / 4; qt++) wnl_idft.push_back(make_int4(i, zl, lt, qt));
+        for (int qt = 0; qt < CLH_NL_QT / 16; qt++) wnl_idft.push_back(make_int4(i, zl, lt, qt));
     for (int zl = 0; zl < nl; zl++) wnl_cl.push_back(make_int2(i, zl));
   }
   HIPCHK(c, up2(bt.d_wn_nl, wn_nl, bt.n_wn_nl));
@@ -1009,6 +1013,7 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   HIPCHK(c, up2(bt.d_wnl_prep, wnl_prep, bt.n_wnl_prep));
   HIPCHK(c, up4(bt.d_wnl_fwd, wnl_fwd, bt.n_wnl_fwd));
   HIPCHK(c, up4(bt.d_wnl_gl, wnl_gl, bt.n_wnl_gl));
+  HIPCHK(c, up4(bt.d_wnl_chk, wnl_chk, bt.n_wnl_chk));
   HIPCHK(c, up4(bt.d_wnl_idft, wnl_idft, bt.n_wnl_idft));
   HIPCHK(c, up2(bt.d_wnl_cl, wnl_cl, bt.n_wnl_cl));
   return CLH_OK;
@@ -1287,7 +1292,8 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
     HIPCHK(c, clh_launch_nodes(bt.d_desc_nl.p, bt.d_wn_nl.p, bt.n_wn_nl, s));
     HIPCHK(c, clh_launch_limber_gg(bt.d_desc_nl.p, bt.d_wgg_nl.p, bt.n_wgg_nl, 0, c->gg_lds[0] ? bt.gg_stage : 0, 0, s));
     HIPCHK(c, clh_launch_nl_forward(bt.d_desc.p, bt.d_wnl_prep.p, bt.n_wnl_prep, bt.d_wnl_fwd.p, bt.n_wnl_fwd,
-                                    bt.d_wnl_gl.p, bt.n_wnl_gl, bt.d_wnl_idft.p, bt.n_wnl_idft, s));
+                                    bt.d_wnl_gl.p, bt.n_wnl_gl, bt.d_wnl_chk.p, bt.n_wnl_chk, bt.d_wnl_cl.p,
+                                    bt.n_wnl_cl, bt.d_wnl_idft.p, bt.n_wnl_idft, s));
   }
   if (legendre) {
     if (c->legendre_mode == CLH_LEGENDRE_FULL)

@@ -48,8 +48,9 @@
 //   x[512]                 chi nodes (Mpc)
 //   fb[nl][3][512]         FFTLog inputs f(x) / x^nu
 //   T[nl][3][NH][2]        windowed forward transforms (complex)
-//   G[3][LT][NH][2]        g_l(nu + i eta_m) e^{-i eta_m ln base_l} per ell row
+//   G[3][NH][LT][2]        g_l(nu + i eta_m) e^{-i eta_m ln base_l}, ell fastest
 //   part[nl][LT][QT]       partial q sums of the C_ell integrand
+//   partc[nl][16][QT]      the same at the block ends ell = 15, 31, ..., 149
 //   used[nl][L]            the C_ell the w sum reads below LMAX_NOLIMBER
 //   lmx[nl]                the ell where each bin switched to Limber (LMAX)
 //   scal[8]                chi_min, dln chi, c/H0 in Mpc
@@ -59,7 +60,7 @@
 #define CLH_NL_NH 752          // N/2 + 1 padded to even, N <= 1500
 #define CLH_NL_QT 32           // q tiles of 16 columns (512 columns)
 struct ClhNlOff {
-  size_t x, fb, T, G, part, used, lmx, scal, total;
+  size_t x, fb, T, G, part, partc, used, lmx, scal, total;
 };
 #ifdef __HIPCC__
 __host__ __device__
@@ -72,6 +73,7 @@ inline ClhNlOff clh_nl_off(int nl) {
   o.T = p; p += (size_t)nl * 3 * CLH_NL_NH * 2;
   o.G = p; p += (size_t)3 * CLH_NL_LT * CLH_NL_NH * 2;
   o.part = p; p += (size_t)nl * CLH_NL_LT * CLH_NL_QT;
+  o.partc = p; p += (size_t)nl * 16 * CLH_NL_QT;
   o.used = p; p += (size_t)nl * CLH_NL_L;
   o.lmx = p; p += (size_t)nl;
   o.scal = p; p += 8;

@@ -14,14 +14,17 @@
 //   k_nl_idft  the inverse DFT restricted to the kept outputs as an FP64 MFMA
 //              product: Out[l][q] = sum_m w_m Re(conj(T G) e^{2 pi i m n / N}),
 //              fused with the C_ell integrand F^2 k^3 P_lin(k) and its sum
-//              over a 16-node slice of q
-//   k_nl_cl    C_ell = 2/pi dln k sum + C_Limber(P_nl) - C_Limber(P_lin),
-//              the reference's block-convergence switch to Limber (|C/C_Limber
-//              - 1| < 0.01 at the end of a 16-ell block), and the change of
-//              w(theta) against the Limber C_ell the folded Legendre kernel
-//              already summed: dw = sum_l Pw[theta][l] (C_used - C_Limber)[l]
-// Every ell < 150 is transformed (the reference stops a bin at convergence;
-// the values it keeps are the same).
+//              over a 16-node slice of q; first for the ten block-end ells
+//              (15, 31, ..., 143, 149) only, then for the 16-ell tiles below
+//              each bin's LMAX
+//   k_nl_lmax  the reference's block-convergence switch to Limber: LMAX is the
+//              end of the first 16-ell block whose last C_ell is within 1% of
+//              the Limber C_ell (cosmo2D.c:4740-4750)
+//   k_nl_cl    C_ell = 2/pi dln k sum + C_Limber(P_nl) - C_Limber(P_lin) and
+//              the change of w(theta) against the Limber C_ell the folded
+//              Legendre kernel already summed:
+//              dw = sum_l Pw[theta][l] (C_used - C_Limber)[l]
+// As in the reference, no ell at or above a bin's LMAX is transformed.
 #include "clh_device.hpp"
 
 typedef double clh_v4d __attribute__((ext_vector_type(4)));
@@ -230,9 +233,9 @@ __global__ __launch_bounds__(256) void k_nl_gl(const DrawDesc* __restrict__ D, c
   const int N = d.nl_N[j], N0 = d.nl_N0[j], nh = N / 2 + 1;
   const int m = w.z * 256 + threadIdx.x;
   if (m >= CLH_NL_NH) return;
-  double2* G = reinterpret_cast<double2*>(d.nlw + off.G) + (size_t)j * CLH_NL_LT * CLH_NL_NH + m;
+  double2* G = reinterpret_cast<double2*>(d.nlw + off.G) + ((size_t)j * CLH_NL_NH + m) * CLH_NL_LT;
   if (m >= nh || (j == 2 && !clh_nl_mag(d))) {
-    for (int l = 0; l < CLH_NL_LT; l++) G[(size_t)l * CLH_NL_NH] = make_double2(0.0, 0.0);
+    for (int l = 0; l < CLH_NL_LT; l++) G[l] = make_double2(0.0, 0.0);
     return;
   }
   const double* x = d.nlw + off.x;
@@ -249,40 +252,53 @@ __global__ __launch_bounds__(256) void k_nl_gl(const DrawDesc* __restrict__ D, c
     g1 = g;
     double s, c;
     sincos(-eta * log(base * (l + 1.) / x[CLH_NL_NCHI - 1]), &s, &c);
-    G[(size_t)l * CLH_NL_NH] = cmul(g, make_double2(c, s));
+    G[l] = cmul(g, make_double2(c, s));
   }
-  for (int l = CLH_NL_L; l < CLH_NL_LT; l++) G[(size_t)l * CLH_NL_NH] = make_double2(0.0, 0.0);
+  for (int l = CLH_NL_L; l < CLH_NL_LT; l++) G[l] = make_double2(0.0, 0.0);
 }
 
 // ----------------------------------------------------------------------------
 // k_nl_idft: the inverse real transforms of every (bin, ell) kept at the 500
 // original nodes, as an FP64 MFMA product over m, fused with the C_ell
 // integrand (cosmo2D.c:4492-4524 and 4702-4734).  Work item (draw, bin,
-// 16-ell tile, 64-node tile); four waves, one 16 x 16 output tile each.
+// 16-ell tile, 256-node tile).
 // Lane layout of v_mfma_f64_16x16x4: A[r][k] on lane r + 16 k, B[k][n] on
-// lane n + 16 k, D[4 v + l / 16][l % 16] in value v of lane l.  K index
+// lane n + 16 k, D[4 v + l / 16][l % 16] in value v of lane l.  A block of
+// four waves covers 256 columns, each wave four 16-column tiles that share
+// its A operand (four independent accumulation chains).  K index
 // 4 s + k -> m = 2 s + k / 2, real (k even) or imaginary (k odd) part:
 //   A[l][.] = (Re Z, Im Z) with Z = T[m] G[l][m],
 //   B[.][q] = w_m (cos, sin)(2 pi m n / N), n = N0 + q, w_m = 1 at m = 0 and
 // N/2, 2 elsewhere (the c2r sum over the Hermitian half), so that
 // sum A B = sum_m w_m Re(conj(Z) e^{2 pi i m n / N}).
+// CHECK: the tile's rows are the ten block-end ells (row r -> min(16 r + 15,
+// 149), rows 10..15 empty) and the sums go to partc; otherwise the tile is
+// ells 16 w.z .. 16 w.z + 15 and is skipped at or above the bin's LMAX.
 // ----------------------------------------------------------------------------
+__device__ __forceinline__ int clh_nl_row(bool check, int l0, int r) {
+  if (!check) return l0 + r;
+  return r < 10 ? (16 * r + 15 < CLH_NL_L ? 16 * r + 15 : CLH_NL_L - 1) : CLH_NL_LT - 1;
+}
+
+template <bool CHECK>
 __global__ __launch_bounds__(256) void k_nl_idft(const DrawDesc* __restrict__ D, const int4* __restrict__ work) {
+  constexpr int SUB = 4;  // 16-column output tiles per wave, sharing the A operand
   __shared__ double tw[2 * 1500];
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
   const int i = w.y, l0 = 16 * w.z;
   const ClhNlOff off = clh_nl_off(d.nl);
+  if (!CHECK && l0 >= (int)d.nlw[off.lmx + i]) return;  // whole block: ells the reference never transforms
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = lane & 15, kq = lane >> 4, mo = kq >> 1, cpart = kq & 1;
-  const int q0 = 64 * w.w + 16 * wv;
-  const int qcol = q0 + r;
-  const double bm = qcol < CLH_NL_NCHI ? 1.0 : 0.0;
+  const int lrow = clh_nl_row(CHECK, l0, r);
+  const int q0 = 256 * w.w + 64 * wv;  // the wave's columns q0 .. q0 + 63
   const bool mag = clh_nl_mag(d);
   const double2* Tall = reinterpret_cast<const double2*>(d.nlw + off.T);
   const double2* Gall = reinterpret_cast<const double2*>(d.nlw + off.G);
-  clh_v4d acc[3];
-  for (int j = 0; j < 3; j++) acc[j] = (clh_v4d){0.0, 0.0, 0.0, 0.0};
+  clh_v4d acc[3][SUB];
+  for (int j = 0; j < 3; j++)
+    for (int u = 0; u < SUB; u++) acc[j][u] = (clh_v4d){0.0, 0.0, 0.0, 0.0};
   for (int j = 0; j < 3; j++) {
     if (j == 2 && !mag) break;
     const int N = d.nl_N[j], N0 = d.nl_N0[j], nh = N / 2 + 1;
@@ -290,55 +306,108 @@ __global__ __launch_bounds__(256) void k_nl_idft(const DrawDesc* __restrict__ D,
     __syncthreads();
     for (int t = threadIdx.x; t < 2 * N; t += 256) tw[t] = d.nl_tw[tw0 + t];
     __syncthreads();
-    const double2* T = Tall + (size_t)(i * 3 + j) * CLH_NL_NH;
-    const double2* G = Gall + ((size_t)j * CLH_NL_LT + l0 + r) * CLH_NL_NH;
-    const int n = N0 + (qcol < CLH_NL_NCHI ? qcol : 0);
-    int t = (mo * n) % N;
-    const int dt = (2 * n) % N;
-    const int nsteps = (nh + 1) / 2;
-    clh_v4d a4 = acc[j];
-#pragma unroll 4
-    for (int s = 0; s < nsteps; s++) {
-      const int m = 2 * s + mo;
-      const double2 Z = cmul(T[m], G[m]);
-      const double a = cpart ? Z.y : Z.x;
-      const double wm = (m == 0 || 2 * m == N) ? 1.0 : 2.0;
-      const double b = bm * wm * tw[2 * t + cpart];
-      a4 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, a4, 0, 0, 0);
-      t += dt;
-      if (t >= N) t -= N;
+    const double2* T = Tall + (size_t)(i * 3 + j) * CLH_NL_NH + mo;
+    const double2* G = Gall + ((size_t)j * CLH_NL_NH + mo) * CLH_NL_LT + lrow;
+    int t[SUB], dt[SUB];
+    double bm[SUB];
+    for (int u = 0; u < SUB; u++) {
+      const int q = q0 + 16 * u + r;
+      const int n = N0 + (q < CLH_NL_NCHI ? q : 0);
+      bm[u] = q < CLH_NL_NCHI ? 2.0 : 0.0;  // w_m = 2 except at m = 0 and N/2
+      t[u] = (mo * n) % N;
+      dt[u] = (2 * n) % N;
     }
-    acc[j] = a4;
+    const int nsteps = (nh + 1) / 2;
+    clh_v4d a4[SUB];
+    for (int u = 0; u < SUB; u++) a4[u] = acc[j][u];
+    double2 Tn = T[0], Gn = G[0];
+    for (int s = 0; s < nsteps; s++) {
+      const double2 Z = cmul(Tn, Gn);
+      if (s + 1 < nsteps) {  // next step's operands, loaded ahead of this step's MFMAs
+        Tn = T[2 * (s + 1)];
+        Gn = G[(size_t)2 * (s + 1) * CLH_NL_LT];
+      }
+      const int m = 2 * s + mo;
+      const double a = cpart ? Z.y : Z.x;
+      const double half = (m == 0 || 2 * m == N) ? 0.5 : 1.0;
+#pragma unroll
+      for (int u = 0; u < SUB; u++) {
+        const double b = bm[u] * half * tw[2 * t[u] + cpart];
+        a4[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, a4[u], 0, 0, 0);
+        t[u] += dt[u];
+        if (t[u] >= N) t[u] -= N;
+      }
+    }
+    for (int u = 0; u < SUB; u++) acc[j][u] = a4[u];
   }
   // epilogue: F = sum_j Fy_j (magnification with b_mag l(l+1) / y^2),
   // integrand F^2 k^3 P_lin(k, a = 1) at k = y c/H0, y = (l + 1) / x[499 - q]
   const double* x = d.nlw + off.x;
   const double rc = d.nlw[off.scal + 2];
   const double bmag = d.par->bmag[i];
-  double* part = d.nlw + off.part;
-  const int q = qcol;
+  double* part = d.nlw + (CHECK ? off.partc : off.part);
   for (int v = 0; v < 4; v++) {
-    const int l = l0 + 4 * v + kq;
-    double val = 0.0;
-    if (l < CLH_NL_L && q < CLH_NL_NCHI) {
-      const double xq = x[CLH_NL_NCHI - 1 - q];
-      const double ty = (l + 1.) / xq;
-      double F = 0.0;
-      for (int j = 0; j < 3; j++) {
-        if (j == 2 && !mag) break;
-        const double nu = c_nl_nu[j];
-        const double pre = sqrt(M_PI) / (4.0 * d.nl_N[j] * pow(l + 1., nu)) * pow(xq, nu);
-        const double fy = acc[j][v] * pre;
-        F += j == 2 ? bmag * (l * (l + 1.)) * fy / (ty * ty) : fy;
+    const int row = 4 * v + kq;
+    const int l = clh_nl_row(CHECK, l0, row);
+    for (int u = 0; u < SUB; u++) {
+      const int q = q0 + 16 * u + r;
+      double val = 0.0;
+      if (l < CLH_NL_L && q < CLH_NL_NCHI) {
+        const double xq = x[CLH_NL_NCHI - 1 - q];
+        const double ty = (l + 1.) / xq;
+        double F = 0.0;
+        for (int j = 0; j < 3; j++) {
+          if (j == 2 && !mag) break;
+          const double nu = c_nl_nu[j];
+          const double pre = sqrt(M_PI) / (4.0 * d.nl_N[j] * pow(l + 1., nu)) * pow(xq, nu);
+          const double fy = acc[j][u][v] * pre;
+          F += j == 2 ? bmag * (l * (l + 1.)) * fy / (ty * ty) : fy;
+        }
+        const double k1 = ty * rc;
+        val = F * F * (k1 * k1 * k1) * clh_plin0(d, k1);
       }
-      const double k1 = ty * rc;
-      val = F * F * (k1 * k1 * k1) * clh_plin0(d, k1);
+      val += __shfl_xor(val, 1);
+      val += __shfl_xor(val, 2);
+      val += __shfl_xor(val, 4);
+      val += __shfl_xor(val, 8);
+      if (r == 0)
+        part[((size_t)i * (CHECK ? 16 : CLH_NL_LT) + (CHECK ? row : l)) * CLH_NL_QT + 16 * w.w + 4 * wv + u] = val;
     }
-    val += __shfl_xor(val, 1);
-    val += __shfl_xor(val, 2);
-    val += __shfl_xor(val, 4);
-    val += __shfl_xor(val, 8);
-    if (r == 0) part[((size_t)i * CLH_NL_LT + l) * CLH_NL_QT + 4 * w.w + wv] = val;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// k_nl_lmax: per (draw, lens bin), one wave: the non-Limber C_ell at the ten
+// block ends and the first block whose end agrees with Limber to 1%
+// (cosmo2D.c:4740-4750; LMAX = 150 if none does)
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_nl_lmax(const DrawDesc* __restrict__ D, const int2* __restrict__ work) {
+  __shared__ double dev[16];
+  const int2 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const int i = w.y, r = threadIdx.x;
+  const ClhNlOff off = clh_nl_off(d.nl);
+  if (r < 10) {
+    const int L = clh_nl_row(true, 0, r);
+    const double* part = d.nlw + off.partc + ((size_t)i * 16 + r) * CLH_NL_QT;
+    double s = 0.0;
+    for (int t = 0; t < CLH_NL_QT; t++) s += part[t];
+    const double lnl = d.nl_lnl[(size_t)i * (CLH_NL_L - 1) + L - 1];
+    const double llin = d.nl_llin[(size_t)i * (CLH_NL_L - 1) + L - 1];
+    const double cl = s * d.nlw[off.scal + 1] * 2. / M_PI + lnl - llin;
+    double dv = INFINITY;
+    if (fabs(lnl) > 1e-300) dv = cl / lnl - 1.0;
+    dev[r] = (isfinite(dv) && fabs(dv) < 0.01) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  if (r == 0) {
+    int lmax = CLH_NL_L;
+    for (int b = 0; b < 10; b++)
+      if (dev[b] != 0.0) {
+        lmax = 16 * (b + 1) < CLH_NL_L ? 16 * (b + 1) : CLH_NL_L;
+        break;
+      }
+    d.nlw[off.lmx + i] = (double)lmax;
   }
 }
 
@@ -361,29 +430,13 @@ __global__ __launch_bounds__(256) void k_nl_cl(const DrawDesc* __restrict__ D, c
   const double dlnk = d.nlw[off.scal + 1];
   const double* lnl = d.nl_lnl + (size_t)i * (CLH_NL_L - 1);
   const double* llin = d.nl_llin + (size_t)i * (CLH_NL_L - 1);
-  if (l < CLH_NL_L) {
+  if (l < CLH_NL_L && l < (int)d.nlw[off.lmx + i]) {  // the transformed ells
     const double* part = d.nlw + off.part + ((size_t)i * CLH_NL_LT + l) * CLH_NL_QT;
     double s = 0.0;
     for (int t = 0; t < CLH_NL_QT; t++) s += part[t];
     NL[l] = s * dlnk * 2. / M_PI + (l >= 1 ? lnl[l - 1] - llin[l - 1] : 0.0);
   }
-  __syncthreads();
-  if (l == 0) {
-    int lmax = CLH_NL_L;
-    for (int ks = 0; ks < CLH_NL_L; ks += 16) {
-      const int ke = ks + 16 < CLH_NL_L ? ks + 16 : CLH_NL_L;
-      const double denom = lnl[ke - 2];  // Limber C at L = ke - 1
-      if (fabs(denom) > 1e-300) {
-        const double dev = NL[ke - 1] / denom - 1.0;
-        if (isfinite(dev) && fabs(dev) < 0.01) {
-          lmax = ke;
-          break;
-        }
-      }
-    }
-    s_lmax = lmax;
-    d.nlw[off.lmx + i] = (double)lmax;
-  }
+  if (l == 0) s_lmax = (int)d.nlw[off.lmx + i];
   __syncthreads();
   if (l < CLH_NL_L) {
     const double* tab = d.cl + (size_t)(d.n_ss + d.n_gs + i) * d.nrow;
@@ -413,12 +466,15 @@ __global__ __launch_bounds__(256) void k_nl_cl(const DrawDesc* __restrict__ D, c
 }
 
 hipError_t clh_launch_nl_forward(const DrawDesc* D, const int2* wprep, int nprep, const int4* wfwd, int nfwd,
-                                 const int4* wgl, int ngl, const int4* widft, int nidft, hipStream_t s) {
+                                 const int4* wgl, int ngl, const int4* wchk, int nchk, const int2* wbin, int nbin,
+                                 const int4* widft, int nidft, hipStream_t s) {
   if (nprep <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_nl_prep, dim3(nprep), dim3(256), 0, s, D, wprep);
   hipLaunchKernelGGL(k_nl_fwd, dim3(nfwd), dim3(256), 0, s, D, wfwd);
   hipLaunchKernelGGL(k_nl_gl, dim3(ngl), dim3(256), 0, s, D, wgl);
-  hipLaunchKernelGGL(k_nl_idft, dim3(nidft), dim3(256), 0, s, D, widft);
+  hipLaunchKernelGGL(k_nl_idft<true>, dim3(nchk), dim3(256), 0, s, D, wchk);
+  hipLaunchKernelGGL(k_nl_lmax, dim3(nbin), dim3(64), 0, s, D, wbin);
+  hipLaunchKernelGGL(k_nl_idft<false>, dim3(nidft), dim3(256), 0, s, D, widft);
   return hipGetLastError();
 }
 
