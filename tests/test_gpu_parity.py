@@ -180,8 +180,7 @@ def test_reference_scalar_entries(full):
         assert abs(L.C_ss_tomo_limber(l, 0, 3, 1) / ref - 1) < RTOL
     assert np.isnan(L.xi_pm_tomo(1, 0, 0, 99, 1))
     assert np.isnan(L.w_gg_tomo(0, 0, 1, 1))
-    assert np.isnan(L.w_gg_tomo(0, 0, 0, 0))  # non-Limber w: no linear P(k,z) set
-    c.set_adopt_limber_gg(1)
+    assert np.isnan(L.w_gg_tomo(0, 0, 0, 2))  # limber is 0 or 1
 
 
 def test_error_paths_leave_output(full):
