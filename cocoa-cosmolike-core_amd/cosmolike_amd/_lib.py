@@ -35,6 +35,9 @@ class clh_draw(ctypes.Structure):
         ("fpt", D), ("fpt_n", ctypes.c_int), ("fpt_kmin", ctypes.c_double), ("fpt_kmax", ctypes.c_double),
         ("h", ctypes.c_double),
         ("plin_log10k", D), ("plin_z", D), ("lnPL", D), ("plin_nk", ctypes.c_int), ("plin_nz", ctypes.c_int),
+        ("b2", D), ("bs2", D), ("b3", D), ("bk", D),
+        ("fptb", D), ("fptb_n", ctypes.c_int), ("fptb_kmin", ctypes.c_double), ("fptb_kmax", ctypes.c_double),
+        ("fptb_sigma4", ctypes.c_double),
     ]
 
 
@@ -82,6 +85,9 @@ def lib():
             "clh_set_nuisance_ia_tatt": (i32, [VP, D, D, D, i32]),
             "clh_set_IA_PS": (i32, [VP, D, f64, f64, f64, i32]),
             "clh_set_linear_power_spectrum": (i32, [VP, D, i32, D, i32, D]),
+            "clh_set_nuisance_nonlinear_bias": (i32, [VP, D, D, i32]),
+            "clh_set_nuisance_nonlocal_bias": (i32, [VP, D, D, i32]),
+            "clh_set_bias_PS": (i32, [VP, D, f64, f64, f64, f64, i32]),
             "clh_set_adopt_limber_gg": (i32, [VP, i32]),
             "clh_cl_nonlimber": (i32, [VP, D, ctypes.POINTER(ctypes.c_int)]),
             "clh_set_nuisance_pointmass": (i32, [VP, D, i32]),

@@ -170,6 +170,30 @@ class Cosmolike:
         BTA = np.resize(BTA, n)
         self._chk(self.L.clh_set_nuisance_ia_tatt(self.h, _p(A1), _p(A2), _p(BTA), n))
 
+    def set_nuisance_nonlinear_bias(self, b1, b2):
+        """b2 per lens bin (bs2 = -4/7 (b1 - 1) where b2 != 0)."""
+        b1, b2 = _arr(b1), _arr(b2)
+        self._chk(self.L.clh_set_nuisance_nonlinear_bias(self.h, _p(b1), _p(b2), len(b2)))
+
+    def set_nuisance_nonlocal_bias(self, b3, bk):
+        b3, bk = _arr(b3), _arr(bk)
+        self._chk(self.L.clh_set_nuisance_nonlocal_bias(self.h, _p(b3), _p(bk), len(b3)))
+
+    def set_nuisance_bias_fastpt(self, b1, b2, bmag, b3, bk):
+        """set_nuisance_bias_fastpt: linear, nonlinear, magnification and nonlocal bias."""
+        self.set_nuisance_linear_bias(b1)
+        self.set_nuisance_nonlinear_bias(b1, b2)
+        self.set_nuisance_magnification_bias(bmag)
+        self.set_nuisance_nonlocal_bias(b3, bk)
+
+    def set_bias_PS(self, PS, kmin, kmax, sigma4, cutoff=1.0e4):
+        """set_bias_PS: the FAST-PT bias spectra PS[12][N] on N log-spaced k in [kmin, kmax) h/Mpc."""
+        PS = _arr(PS)
+        n = PS.size // 12
+        if PS.size != 12 * n:
+            raise ValueError("PS must hold 12 rows of N entries")
+        self._chk(self.L.clh_set_bias_PS(self.h, _p(PS), float(kmin), float(kmax), float(cutoff), float(sigma4), n))
+
     def set_linear_power_spectrum(self, log10k, z, lnPL):
         lk, z, lnPL = _arr(log10k), _arr(z), _arr(lnPL)
         self._chk(self.L.clh_set_linear_power_spectrum(self.h, _p(lk), len(lk), _p(z), len(z), _p(lnPL)))
@@ -223,6 +247,14 @@ class Cosmolike:
             self.set_nuisance_magnification_bias(prob.bmag)
             if getattr(prob, "pm", None) is not None:
                 self.set_nuisance_pointmass(prob.pm)
+            if getattr(prob, "b2", None) is not None:
+                self.set_bias_PS(prob.bias_ps, prob.bias_ps_kmin, prob.bias_ps_kmax, prob.bias_ps_sigma4)
+                self.set_nuisance_nonlinear_bias(prob.b1, prob.b2)
+                self.set_nuisance_nonlocal_bias(prob.b3, prob.bk)
+            else:
+                z = np.zeros(prob.nl)
+                self.set_nuisance_nonlinear_bias(prob.b1, z)
+                self.set_nuisance_nonlocal_bias(z, z)
         tatt = getattr(prob, "ia_model", 0) == 1
         if tatt:
             self.set_IA_PS(prob.ia_ps, prob.ia_ps_kmin, prob.ia_ps_kmax)
@@ -372,6 +404,12 @@ class Cosmolike:
             d.pm = _p(vals["pm"])
             d.ia_A, d.ia_eta = p.ia_A, p.ia_eta
             d.h = getattr(p, "h", 0.7)
+            if p.lens is not None and getattr(p, "b2", None) is not None:
+                for k in ("b2", "b3", "bk", "bias_ps"):
+                    vals[k] = _arr(getattr(p, k))
+                d.b2, d.b3, d.bk = _p(vals["b2"]), _p(vals["b3"]), _p(vals["bk"])
+                d.fptb, d.fptb_n = _p(vals["bias_ps"]), vals["bias_ps"].size // 12
+                d.fptb_kmin, d.fptb_kmax, d.fptb_sigma4 = p.bias_ps_kmin, p.bias_ps_kmax, p.bias_ps_sigma4
             if getattr(p, "lnPL", None) is not None:
                 for k in ("plin_log10k", "plin_z", "lnPL"):
                     vals[k] = _arr(getattr(p, k))

@@ -110,6 +110,46 @@ def fpt_ia_tables(cosmo: Cosmology, n: int = 1100, kmin: float = 1e-4, kmax: flo
     return np.ascontiguousarray(out.reshape(-1)), kmin, kmax
 
 
+def fpt_bias_tables(cosmo: Cosmology, n: int = 1100, kmin: float = 1e-4, kmax: float = 1e2):
+    """Synthetic stand-ins for the FAST-PT one-loop bias spectra, in
+    set_bias_PS form: PS[12][n] on k_i = exp(ln kmin + i (ln kmax - ln kmin) / n)
+    (h/Mpc), rows 0..5 Pd1d2, Pd2d2, Pd1s2, Pd2s2, Ps2s2, Pd1p3 in (Mpc/h)^3
+    (smooth functions of P_lin(k, z = 0) of tens of per cent of it, distinct
+    amplitudes and signs),
+    row 6 = k, row 7 = P_lin, rows 8..11 zero; and sigma4 = Pd2d2(kmin) / 2 as
+    get_FPT_bias sets it.  They exercise every term of the one-loop
+    integrands; they are not one-loop integrals."""
+    lk = np.log(kmin) + np.arange(n) * (np.log(kmax) - np.log(kmin)) / n
+    k = np.exp(lk)
+    amp = (cosmo.sigma8 / sigma8_unnorm(cosmo.omega_m, cosmo.h, cosmo.ns)) ** 2
+    plin = amp * k**cosmo.ns * t_bbks(k, cosmo.omega_m, cosmo.h) ** 2
+    shape = plin * (k / (k + 0.2)) ** 2
+    coef = [0.5, 0.3, -0.35, 0.2, 0.15, -0.12]
+    out = np.zeros((12, n))
+    for t, c in enumerate(coef):
+        out[t] = c * shape * (1.0 + 0.15 * t * np.tanh(np.log(k / 0.2)))
+    out[1] += 2.0 * 40.0  # Pd2d2 tends to 2 sigma4 at low k
+    out[6] = k
+    out[7] = plin
+    return np.ascontiguousarray(out.reshape(-1)), kmin, kmax, 0.5 * out[1][0]
+
+
+def oneloop_problem(cosmo: Cosmology | None = None, n_ell: int = 500, b3=None, bk=None) -> Problem:
+    """The full 3x2pt layout with one-loop galaxy bias: b2 from b1
+    (Lazeyras et al. form, as B2_FROM_B1 in cosmolike/bias.c:85-91),
+    bs2 = -4/7 (b1 - 1), b3 and bK per lens bin, and synthetic FAST-PT bias
+    tables."""
+    cosmo = cosmo or Cosmology()
+    p = full_3x2pt_problem(cosmo, n_ell)
+    b1 = p.b1
+    p.b2 = 0.412 - 2.143 * b1 + 0.929 * b1**2 + 0.008 * b1**3
+    p.b3 = np.asarray(b3, dtype=np.float64) if b3 is not None else 0.3 * (b1 - 1.0)
+    # bK multiplies k^2 P with k in H0/c units (k = 3000 at 1 h/Mpc)
+    p.bk = np.asarray(bk, dtype=np.float64) if bk is not None else np.linspace(-4e-8, 4e-8, len(b1))
+    p.bias_ps, p.bias_ps_kmin, p.bias_ps_kmax, p.bias_ps_sigma4 = fpt_bias_tables(cosmo)
+    return p
+
+
 def tatt_problem(cosmo: Cosmology | None = None, n_ell: int = 500, A2: float = -0.8, eta2: float = 0.5,
                  bta: float = 0.7) -> Problem:
     """The full 3x2pt layout with TATT intrinsic alignments (redshift
@@ -271,6 +311,15 @@ class Problem:
     ia_ps: np.ndarray | None = None
     ia_ps_kmin: float = 0.0
     ia_ps_kmax: float = 0.0
+    # one-loop galaxy bias: b2 (bs2 follows from b1), b3, bK per lens bin and
+    # the FAST-PT bias tables in set_bias_PS form ([12][N], k range in h/Mpc)
+    b2: np.ndarray | None = None
+    b3: np.ndarray | None = None
+    bk: np.ndarray | None = None
+    bias_ps: np.ndarray | None = None
+    bias_ps_kmin: float = 0.0
+    bias_ps_kmax: float = 0.0
+    bias_ps_sigma4: float = 0.0
     # non-Limber w (like.adopt_limber_gg = 0) and its linear P(k,z)
     limber_gg: int = 1
     plin_log10k: np.ndarray | None = None

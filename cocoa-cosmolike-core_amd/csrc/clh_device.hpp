@@ -161,17 +161,56 @@ __device__ __forceinline__ double clh_ia_bta(const DrawDesc& d, int b) {
   return 1.0;
 }
 
-// FAST-PT IA table t at ln k (cosmo2D.c:1414-1447): uniform in ln k with the
-// step (ln kmax - ln kmin) / N, zero outside the range, the last entry at its
-// top end; linear between entries
-__device__ __forceinline__ double clh_fpt(const DrawDesc& d, int t, double lnk) {
-  if (lnk < d.fpt_l0 || lnk > d.fpt_l1) return 0.0;
-  const double r = (lnk - d.fpt_l0) / d.fpt_dl;
+// A FAST-PT table at ln k (cosmo2D.c:1414-1447, 2566-2593): uniform in ln k
+// with the step (ln kmax - ln kmin) / N, zero outside the range, the last
+// entry at its top end; linear between entries (interpol1d, basics.c:457-480)
+__device__ __forceinline__ double clh_ptab(const double* __restrict__ tab, int n, double l0, double l1,
+                                           double dl, double lnk) {
+  if (lnk < l0 || lnk > l1) return 0.0;
+  const double r = (lnk - l0) / dl;
   const int i = (int)floor(r);
-  const double* __restrict__ tab = d.fpt + (size_t)t * d.fpt_n;
-  if (i + 1 >= d.fpt_n) return tab[d.fpt_n - 1];
+  if (i + 1 >= n) return tab[n - 1];
   const double w = r - i;
   return w * (tab[i + 1] - tab[i]) + tab[i];
+}
+// FAST-PT IA table t (set_IA_PS)
+__device__ __forceinline__ double clh_fpt(const DrawDesc& d, int t, double lnk) {
+  return clh_ptab(d.fpt + (size_t)t * d.fpt_n, d.fpt_n, d.fpt_l0, d.fpt_l1, d.fpt_dl, lnk);
+}
+// FAST-PT bias table t (set_bias_PS)
+__device__ __forceinline__ double clh_fptb(const DrawDesc& d, int t, double lnk) {
+  return clh_ptab(d.fptb + (size_t)t * d.fptb_n, d.fptb_n, d.fptb_l0, d.fptb_l1, d.fptb_dl, lnk);
+}
+
+// one-loop bias term of the gg integrand without W_gal^2 (cosmo2D.c:2560-2603):
+// g4 fac (b1 b2 d1d2 + b2^2/4 d2d2 + b1 bs2 d1s2 + b2 bs2/2 d2s2 + bs2^2/4 s2s2
+// + b1 b3 d1p3) + 2 b1 bK k^2 P fac, with d2d2, d2s2, s2s2 less 2, 4/3, 8/9
+// sigma4 inside the table range
+__device__ __forceinline__ double clh_oneloop_gg(const DrawDesc& d, int zl, double lnk, double k, double g4fac,
+                                                 double pfac) {
+  const bool in = !(lnk < d.fptb_l0 || lnk > d.fptb_l1);
+  const double s4 = d.fptb_s4;
+  const double d1d2 = clh_fptb(d, 0, lnk);
+  const double d2d2 = in ? clh_fptb(d, 1, lnk) - 2. * s4 : 0.0;
+  const double d1s2 = clh_fptb(d, 2, lnk);
+  const double d2s2 = in ? clh_fptb(d, 3, lnk) - 4. / 3. * s4 : 0.0;
+  const double s2s2 = in ? clh_fptb(d, 4, lnk) - 8. / 9. * s4 : 0.0;
+  const double d1p3 = clh_fptb(d, 5, lnk);
+  const double b1 = d.par->b1[zl], b2 = d.par->b2[zl], bs2 = d.par->bs2[zl], b3 = d.par->b3[zl],
+               bk = d.par->bk[zl];
+  return g4fac * (b1 * b2 * d1d2 + 0.25 * b2 * b2 * d2d2 + b1 * bs2 * d1s2 + 0.5 * b2 * bs2 * d2s2 +
+                  0.25 * bs2 * bs2 * s2s2 + b1 * b3 * d1p3) +
+         2 * b1 * bk * k * k * pfac;
+}
+
+// one-loop bias term of the gs integrand times fac (cosmo2D.c:2027-2057):
+// 0.5 g4 fac (b2 d1d2 + bs2 d1s2 + b3 d1p3) + bK k^2 P fac
+__device__ __forceinline__ double clh_oneloop_gs(const DrawDesc& d, int zl, double lnk, double k, double g4fac,
+                                                 double pfac) {
+  return 0.5 * g4fac *
+             (d.par->b2[zl] * clh_fptb(d, 0, lnk) + d.par->bs2[zl] * clh_fptb(d, 2, lnk) +
+              d.par->b3[zl] * clh_fptb(d, 5, lnk)) +
+         d.par->bk[zl] * k * k * pfac;
 }
 
 // W_RSD: radial_weights.c:87-105 (hoverh0 via dchi_dz, f via f_growth)
@@ -390,7 +429,7 @@ __device__ __forceinline__ const double* clh_stage(const double* __restrict__ g,
 // chi_1/chi = (ell+1)/ell, the two W_RSD coefficients (radial_weights.c:98-103)
 // and the magnification prefactor l(l+1)/ell^2.
 struct ClhGgRow {
-  double l10, rr1, c0, c1, pref;
+  double l10, rr1, c0, c1, pref, ell;
 };
 
 __device__ __forceinline__ ClhGgRow clh_gg_row(double l) {
@@ -401,6 +440,7 @@ __device__ __forceinline__ ClhGgRow clh_gg_row(double l) {
   R.c0 = (1 + 8. * ell) / ((2. * ell + 1.) * (2. * ell + 1.));
   R.c1 = 4. / (2 * ell + 3.) * sqrt((2 * ell + 1.) / (2 * ell + 3.));
   R.pref = l * (l + 1.) / (ell * ell);
+  R.ell = ell;
   return R;
 }
 

@@ -22,14 +22,15 @@
 
 hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
-hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, int keep, hipStream_t s);
+hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, int keep, hipStream_t s,
+                                int ol = 0);
 hipError_t clh_launch_limber_sx_tatt(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_nl_forward(const DrawDesc* D, const int2* wprep, int nprep, const int4* wfwd, int nfwd,
                                  const int4* wgl, int ngl, const int4* wchk, int nchk, const int2* wbin, int nbin,
                                  const int4* widft, int nidft, hipStream_t s);
 hipError_t clh_launch_nl_cl(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles, int keep,
-                                hipStream_t s);
+                                hipStream_t s, int ol = 0);
 hipError_t clh_launch_legendre(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_fill(double* p, size_t n, double v, hipStream_t s);
@@ -185,6 +186,16 @@ struct Draw {  // one draw's cosmology + nuisance (host copy)
   int fpt_n = 0;
   double fpt_kmin = 0.0, fpt_kmax = 0.0;  // in H0/c
   double h0 = 0.0;           // H0 / 100 km/s/Mpc (set_cosmological_parameters)
+  // one-loop galaxy bias: b2, bs2, b3, bK per lens bin and the FAST-PT bias tables
+  double b2[CLH_MAXB] = {0}, bs2[CLH_MAXB] = {0}, b3[CLH_MAXB] = {0}, bk[CLH_MAXB] = {0};
+  dvec fptb;                 // [6][fptb_n], already divided by (c/H0)^3
+  int fptb_n = 0;
+  double fptb_kmin = 0.0, fptb_kmax = 0.0, fptb_s4 = 0.0;  // k in H0/c; sigma4 / (c/H0)^3
+  bool has_b2(int nl) const {
+    for (int i = 0; i < nl; i++)
+      if (b2[i] != 0.0) return true;
+    return false;
+  }
   dvec lkL, pzL, lnPL;       // linear P(k,z) (set_linear_power_spectrum)
   Draw() { for (int i = 0; i < CLH_MAXB; i++) lens_st[i] = 1.0; }
 };
@@ -263,6 +274,7 @@ struct Batch {
   int dv_size = 0;
   int nknz = 0;        // common ln P table size (0 if the draws differ)
   bool tatt = false;   // some draw uses TATT intrinsic alignments
+  bool oneloop = false;  // some draw has one-loop galaxy bias
   // non-Limber w: per such draw two views of it (gg Limber rows at ell =
   // 1..149 with P_nl and with P_lin) and the FFTLog work lists
   bool nl_any = false;
@@ -620,6 +632,12 @@ static int check_draw(clh_ctx* c, const Layout& L, const Draw& d, int idx) {
   if (!(d.omega_m > 0)) return fail(c, CLH_ERR_STATE, "draw %d: Omega_m not set", idx);
   if (L.ia_model == CLH_IA_TATT && (d.fpt_n < 2 || d.fpt.size() != (size_t)10 * d.fpt_n))
     return fail(c, CLH_ERR_STATE, "draw %d: TATT needs the FAST-PT IA tables (set_IA_PS)", idx);
+  if (d.has_b2(L.lens.nbin)) {
+    if (d.fptb_n < 2 || d.fptb.size() != (size_t)6 * d.fptb_n)
+      return fail(c, CLH_ERR_STATE, "draw %d: one-loop bias needs the FAST-PT bias tables (set_bias_PS)", idx);
+    if (L.ia_model == CLH_IA_TATT)
+      return fail(c, CLH_ERR_ARG, "draw %d: one-loop bias together with TATT is not implemented", idx);
+  }
   if (L.nonlimber && L.lens.nbin > 0) {
     if (d.lkL.size() < 2 || d.pzL.size() < 2 || d.lnPL.size() != d.lkL.size() * d.pzL.size())
       return fail(c, CLH_ERR_STATE, "draw %d: non-Limber w needs the linear P(k,z) (set_linear_power_spectrum)", idx);
@@ -1056,7 +1074,8 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
                                                                        (CLH_NL_L - 1)))
                        : 0);
     toff[i + 1] = toff[i] + 2 * d.chi_z.size() + 2 * d.G_z.size() + d.lk.size() + d.pz.size() + d.lnP.size() +
-                  (tatt ? d.fpt.size() : 0) + (nlw ? d.lkL.size() + d.pzL.size() + d.lnPL.size() : 0) + 32;
+                  (tatt ? d.fpt.size() : 0) + (nlw ? d.lkL.size() + d.pzL.size() + d.lnPL.size() : 0) +
+                  (d.has_b2(nl) ? d.fptb.size() : 0) + 32;
     dvoff[i + 1] = dvoff[i] + L.dv_size();
     max_nrow = std::max(max_nrow, nrow);
     if (lays[i] != lays[0]) uniform = false;
@@ -1095,6 +1114,15 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
       q.h0 = d.h0;
       nl_sizes(q.nl_N, q.nl_N0);
     }
+    if (d.has_b2(nl)) {
+      q.oneloop = 1;
+      q.fptb = push(d.fptb);
+      q.fptb_n = d.fptb_n;
+      q.fptb_l0 = std::log(d.fptb_kmin);
+      q.fptb_l1 = std::log(d.fptb_kmax);
+      q.fptb_dl = (q.fptb_l1 - q.fptb_l0) / d.fptb_n;
+      q.fptb_s4 = d.fptb_s4;
+    }
     if (L.ia_model == CLH_IA_TATT) {
       q.fpt = push(d.fpt);
       q.fpt_n = d.fpt_n;
@@ -1121,6 +1149,7 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     pp.ia_A = d.ia_A; pp.ia_eta = d.ia_eta;
     for (int b = 0; b < CLH_MAXB; b++) { pp.ia2_bin[b] = d.ia2_bin[b]; pp.bta_bin[b] = d.bta_bin[b]; }
     pp.ia2_A = d.ia2_A; pp.ia2_eta = d.ia2_eta; pp.bta_A = d.bta_A;
+    for (int b = 0; b < CLH_MAXB; b++) { pp.b2[b] = d.b2[b]; pp.bs2[b] = d.bs2[b]; pp.b3[b] = d.b3[b]; pp.bk[b] = d.bk[b]; }
     q.ia_model = L.ia_model;
     q.par = bt.d_par.p + i;
     q.ia_mode = L.ia_mode;
@@ -1208,9 +1237,10 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
         u.kfull = nullptr;
         u.fill = nullptr;
         u.apply_calib = 0;
-        if (lin) {
+        if (lin) {  // C_gg_tomo_limber_linpsopt_nointerp(use_linear_ps = 1): no one-loop term
           u.lk = q.lkL; u.pz = q.pzL; u.lnP = const_cast<double*>(q.lnPL);
           u.nk = q.nkL; u.nzp = q.nzL;
+          u.oneloop = 0;
         }
         (lin ? q.nl_llin : q.nl_lnl) = u.cl + (size_t)(L.n_ss() + L.n_gs()) * (CLH_NL_L - 1);
         bt.desc_nl.push_back(u);
@@ -1238,7 +1268,11 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   bt.nrow_custom = ells ? (int)ells->size() : 0;
   bt.state_valid = false;
   bt.tatt = false;
-  for (int i = 0; i < B; i++) bt.tatt = bt.tatt || lays[i]->ia_model == CLH_IA_TATT;
+  bt.oneloop = false;
+  for (int i = 0; i < B; i++) {
+    bt.tatt = bt.tatt || lays[i]->ia_model == CLH_IA_TATT;
+    bt.oneloop = bt.oneloop || bt.desc[i].oneloop;
+  }
   {  // the gg kernels stage chi(z), G(z) and one lens n(z) spline in LDS when
      // the largest draw's tables fit in 44 KB (64 KB per block with the static LDS)
     size_t st = 0;
@@ -1281,16 +1315,18 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], s));
   const bool mixed = c->precision != CLH_PREC_FP64;
   HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, mixed, c->gg_lds[0] ? bt.gg_stage : 0,
-                                 bt.keep_ckpt && bt.ggk_per ? 1 : 0, s));
+                                 bt.keep_ckpt && bt.ggk_per ? 1 : 0, s, bt.oneloop ? 1 : 0));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], s));
   if (bt.tatt)  // TATT ss/gs tiles: FP64 in every precision mode
     HIPCHK(c, clh_launch_limber_sx_tatt(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
   else
-    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, mixed, bt.keep_ckpt ? 1 : 0, s));
+    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, mixed, bt.keep_ckpt ? 1 : 0, s,
+                                   bt.oneloop ? 1 : 0));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
   if (legendre && bt.nl_any) {  // non-Limber w: FP64 in every precision mode
     HIPCHK(c, clh_launch_nodes(bt.d_desc_nl.p, bt.d_wn_nl.p, bt.n_wn_nl, s));
-    HIPCHK(c, clh_launch_limber_gg(bt.d_desc_nl.p, bt.d_wgg_nl.p, bt.n_wgg_nl, 0, c->gg_lds[0] ? bt.gg_stage : 0, 0, s));
+    HIPCHK(c, clh_launch_limber_gg(bt.d_desc_nl.p, bt.d_wgg_nl.p, bt.n_wgg_nl, 0, c->gg_lds[0] ? bt.gg_stage : 0, 0, s,
+                                   bt.oneloop ? 1 : 0));
     HIPCHK(c, clh_launch_nl_forward(bt.d_desc.p, bt.d_wnl_prep.p, bt.n_wnl_prep, bt.d_wnl_fwd.p, bt.n_wnl_fwd,
                                     bt.d_wnl_gl.p, bt.n_wnl_gl, bt.d_wnl_chk.p, bt.n_wnl_chk, bt.d_wnl_cl.p,
                                     bt.n_wnl_cl, bt.d_wnl_idft.p, bt.n_wnl_idft, s));
@@ -1495,6 +1531,58 @@ int clh_set_linear_power_spectrum(clh_ctx* c, const double* lk, int nk, const do
   c->cur.lkL.assign(lk, lk + nk);
   c->cur.pzL.assign(z, z + nz);
   c->cur.lnPL.assign(lnP, lnP + (size_t)nk * nz);
+  c->version++;
+  return CLH_OK;
+}
+
+// set_nuisance_nonlinear_bias (generic_interface.cpp:1533-1568): b2 per lens
+// bin, and bs2 = -4/7 (b1 - 1) unless b2 = 0 (then 0)
+int clh_set_nuisance_nonlinear_bias(clh_ctx* c, const double* B1, const double* B2, int n) {
+  if (!c) return CLH_ERR_ARG;
+  const int nl = c->L.lens.nbin;
+  if (nl == 0) return fail(c, CLH_ERR_STATE, "set_nuisance_nonlinear_bias: lens sample not set");
+  if (!B1 || !B2 || n != nl) return fail(c, CLH_ERR_ARG, "set_nuisance_nonlinear_bias: need %d values", nl);
+  if (has_nan(B1, n) || has_nan(B2, n)) return fail(c, CLH_ERR_NAN, "set_nuisance_nonlinear_bias: NaN");
+  for (int i = 0; i < nl; i++) {
+    c->cur.b2[i] = B2[i];
+    c->cur.bs2[i] = B2[i] == 0.0 ? 0.0 : (-4. / 7.) * (B1[i] - 1.0);
+  }
+  c->version++;
+  return CLH_OK;
+}
+
+// set_nuisance_nonlocal_bias (generic_interface.cpp:1610-1669): b3 and bK per lens bin
+int clh_set_nuisance_nonlocal_bias(clh_ctx* c, const double* B3, const double* BK, int n) {
+  if (!c) return CLH_ERR_ARG;
+  const int nl = c->L.lens.nbin;
+  if (nl == 0) return fail(c, CLH_ERR_STATE, "set_nuisance_nonlocal_bias: lens sample not set");
+  if (!B3 || !BK || n != nl) return fail(c, CLH_ERR_ARG, "set_nuisance_nonlocal_bias: need %d values", nl);
+  if (has_nan(B3, n) || has_nan(BK, n)) return fail(c, CLH_ERR_NAN, "set_nuisance_nonlocal_bias: NaN");
+  for (int i = 0; i < nl; i++) {
+    c->cur.b3[i] = B3[i];
+    c->cur.bk[i] = BK[i];
+  }
+  c->version++;
+  return CLH_OK;
+}
+
+// set_bias_PS (generic_interface.cpp:1008-1076): PS[12][N], rows 0..5 the
+// FAST-PT bias spectra (d1d2, d2d2, d1s2, d2s2, s2s2, d1p3) in (Mpc/h)^3,
+// k range and cutoff in h/Mpc, sigma4 in (Mpc/h)^3; stored as the reference
+// stores them (/ (c/H0)^3, k * c/H0)
+int clh_set_bias_PS(clh_ctx* c, const double* PS, double kmin, double kmax, double cutoff, double sigma4, int N) {
+  if (!c) return CLH_ERR_ARG;
+  (void)cutoff;
+  if (!PS || N < 2) return fail(c, CLH_ERR_ARG, "set_bias_PS: N=%d", N);
+  if (!(kmin > 0) || !(kmax > kmin)) return fail(c, CLH_ERR_ARG, "set_bias_PS: bad k range");
+  if (has_nan(PS, 12 * N) || std::isnan(sigma4)) return fail(c, CLH_ERR_NAN, "set_bias_PS: NaN");
+  const double h3 = CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0;
+  c->cur.fptb.resize((size_t)6 * N);
+  for (size_t i = 0; i < (size_t)6 * N; i++) c->cur.fptb[i] = PS[i] / h3;
+  c->cur.fptb_n = N;
+  c->cur.fptb_kmin = kmin * CLH_COVERH0;
+  c->cur.fptb_kmax = kmax * CLH_COVERH0;
+  c->cur.fptb_s4 = sigma4 / h3;
   c->version++;
   return CLH_OK;
 }
@@ -1961,6 +2049,12 @@ static int check_upload(clh_ctx* c, const clh_draw& s, const Layout& L, int i) {
       return fail(c, CLH_ERR_ARG, "draw %d: TATT needs the FAST-PT IA tables", i);
     if (has_nan(s.fpt, 10 * s.fpt_n)) return fail(c, CLH_ERR_NAN, "draw %d: NaN in the IA tables", i);
   }
+  if (s.fptb) {
+    if (s.fptb_n < 2 || !(s.fptb_kmin > 0) || !(s.fptb_kmax > s.fptb_kmin))
+      return fail(c, CLH_ERR_ARG, "draw %d: bad FAST-PT bias tables", i);
+    if (has_nan(s.fptb, 12 * s.fptb_n) || std::isnan(s.fptb_sigma4))
+      return fail(c, CLH_ERR_NAN, "draw %d: NaN in the bias tables", i);
+  }
   if (L.nonlimber && L.lens.nbin > 0) {
     if (!s.plin_log10k || !s.plin_z || !s.lnPL || s.plin_nk < 2 || s.plin_nz < 2 || !(s.h > 0))
       return fail(c, CLH_ERR_ARG, "draw %d: non-Limber w needs the linear P(k,z) and h", i);
@@ -1993,6 +2087,21 @@ static void fill_draw(Draw& d, const clh_draw& s, const Layout& L) {
   }
   d.ia_A = s.ia_A;
   d.ia_eta = s.ia_eta;
+  for (int i = 0; i < L.lens.nbin; i++) {
+    d.b2[i] = s.b2 ? s.b2[i] : 0.0;
+    d.bs2[i] = s.bs2 ? s.bs2[i] : (d.b2[i] == 0.0 ? 0.0 : (-4. / 7.) * (d.b1[i] - 1.0));
+    d.b3[i] = s.b3 ? s.b3[i] : 0.0;
+    d.bk[i] = s.bk ? s.bk[i] : 0.0;
+  }
+  if (s.fptb && s.fptb_n >= 2) {
+    const double h3 = CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0;
+    d.fptb.resize((size_t)6 * s.fptb_n);
+    for (size_t i = 0; i < d.fptb.size(); i++) d.fptb[i] = s.fptb[i] / h3;
+    d.fptb_n = s.fptb_n;
+    d.fptb_kmin = s.fptb_kmin * CLH_COVERH0;
+    d.fptb_kmax = s.fptb_kmax * CLH_COVERH0;
+    d.fptb_s4 = s.fptb_sigma4 / h3;
+  }
   d.h0 = s.h;
   if (s.plin_log10k && s.plin_z && s.lnPL && s.plin_nk > 0 && s.plin_nz > 0) {
     d.lkL.assign(s.plin_log10k, s.plin_log10k + s.plin_nk);
@@ -2134,7 +2243,7 @@ long long clh_batch_datavector_offset(clh_ctx* c, int i) {
 // the context asks for it and the batch has one layout.  `changed`: some
 // descriptor changed on the host and must be uploaded.
 static int prepare_checkpoint(clh_ctx* c, Batch& cb, bool& changed) {
-  cb.keep_ckpt = c->vjp_ckpt && cb.uniform && !cb.nrow_custom && !cb.tatt && !cb.nl_any;
+  cb.keep_ckpt = c->vjp_ckpt && cb.uniform && !cb.nrow_custom && !cb.tatt && !cb.nl_any && !cb.oneloop;
   if (cb.keep_ckpt) {
     const size_t nl = cb.lays[0]->lens.nbin, nr = cb.lays[0]->nrow();
     cb.ggk_per = (nl * nr * 3 * CLH_NP_GG + 31) & ~size_t(31);
@@ -2308,7 +2417,7 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   if (rc0) return rc0;
   for (int i = 0; i < bt.B; i++) bt.desc[i].dv = bt.d_dv.p + bt.dv_off[i];
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
-  const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt && !bt.nl_any;  // the VJP refuses these
+  const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt && !bt.nl_any && !bt.oneloop;  // the VJP refuses these
   DBuf<double> dvbar, glnp;  // unit cotangent and ln P gradient of the adjoint stages
   if (adj) {
     hipError_t e = dvbar.ensure((size_t)bt.dv_off[bt.B]);
@@ -2406,6 +2515,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (!bt.uniform) return fail(c, CLH_ERR_ARG, "VJP needs a batch with one layout for every draw");
   if (bt.tatt) return fail(c, CLH_ERR_ARG, "VJP of TATT intrinsic alignments is not implemented");
   if (bt.nl_any) return fail(c, CLH_ERR_ARG, "VJP of the non-Limber w(theta) is not implemented");
+  if (bt.oneloop) return fail(c, CLH_ERR_ARG, "VJP of the one-loop galaxy bias terms is not implemented");
   const Layout& L = *bt.lays[0];
   if (!ord_ok(L, ord))
     return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");

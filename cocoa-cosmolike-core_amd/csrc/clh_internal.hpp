@@ -161,6 +161,11 @@ struct DrawDesc {
   double fpt_l0, fpt_l1, fpt_dl;  // ln k range (k in H0/c) and step (l1 - l0) / fpt_n
   double* ggk;          // [nl][nrow][3][96] gg integrand checkpoint of the forward (bracket t,
                         // P fac, dW_RSD/d dz per row and node) for the VJP, or null
+  // one-loop galaxy bias (some b2 != 0, has_b2_galaxies): FAST-PT bias tables
+  int oneloop;
+  const double* fptb;         // [6][fptb_n] d1d2, d2d2, d1s2, d2s2, s2s2, d1p3 (/ (c/H0)^3)
+  int fptb_n;
+  double fptb_l0, fptb_l1, fptb_dl, fptb_s4;  // ln k range (k in H0/c), step, sigma4 (/ (c/H0)^3)
   // non-Limber w (like.adopt_limber_gg = 0), nlw null otherwise
   double* nlw;                // work area, ClhNlOff offsets
   const double* nl_tw;        // [3][2 N_j] cos, sin of 2 pi t / N_j (layout)
@@ -189,6 +194,8 @@ struct DrawParams {
   // TATT: nuisance.ia[1][*] (A2: per bin, or value and exponent) and ia[2][*] (b_TA)
   double ia2_bin[CLH_MAXB], bta_bin[CLH_MAXB];
   double ia2_A, ia2_eta, bta_A;
+  // one-loop galaxy bias: nuisance.gb[1], gb[2], gb[3], gb[5] per lens bin
+  double b2[CLH_MAXB], bs2[CLH_MAXB], b3[CLH_MAXB], bk[CLH_MAXB];
 };
 
 // Node buffer layout (doubles):

@@ -316,7 +316,7 @@ struct ClhKahan {
   }
 };
 
-template <bool F32, bool KEEP>
+template <bool F32, bool KEEP, bool OL = false>
 __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restrict__ D,
                                                          const int4* __restrict__ work) {
   // rows of S / Sf padded by one element: the pair sums read 16 rows at one
@@ -329,10 +329,12 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
   __shared__ float Wf[CLH_MAXB * CLH_NP_SS];
   __shared__ double L10[CLH_TILE_ROWS];
   __shared__ double s_lk[CLH_LDS_LK];
+  __shared__ double O[OL ? CLH_SUB_ROWS * sp : 1];  // gs one-loop bias term per (row, node)
   const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
   const int kind = w.y & 3, zl = w.y >> 2, row0 = w.z, nr = w.w;
   const bool f32 = F32 && kind == CLH_W_SS;
+  const bool ol = OL && kind != CLH_W_SS && d.oneloop;
   const int tid = threadIdx.x;
   constexpr int np = CLH_NP_SS;
   const double* o = clh_sx_nodes(d, kind, zl);
@@ -351,6 +353,12 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
   ClhPkNode pkn;
   pkn.init(d, (int)lut[p], lut[np + p]);
   constexpr double c3 = 1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
+  double ol_g4f = 0.0, ol_fk = 1.0;
+  if (ol) {
+    const double g = clh_growfac(d, o[p]);
+    ol_g4f = g * g * g * g * fac;
+    ol_fk = o[np + p];
+  }
   constexpr int per = CLH_SUB_ROWS / 4;
   for (int sub = 0; sub * CLH_SUB_ROWS < nr; sub++) {
     const int base = sub * CLH_SUB_ROWS;
@@ -367,6 +375,14 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
         Sf[rl * sp + p] = 0.0f;
       }
       if (!f32) S[rl * sp + p] = s;
+      if (OL) {
+        double ov = 0.0;
+        if (ol && rr < nr) {
+          const double k = (ells[row0 + rr] + 0.5) / ol_fk;
+          ov = clh_oneloop_gs(d, zl, log(k), k, ol_g4f, s);
+        }
+        O[rl * sp + p] = ov;
+      }
       if (KEEP && rr < nr)
         d.sxk[((size_t)(kind == CLH_W_SS ? 0 : 1 + zl) * d.nrow + row0 + rr) * np + p] =
             f32 ? (double)Sf[rl * sp + p] : s;
@@ -415,8 +431,15 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
         const double* Bz = W + (2 + zs) * np;
         const double* Sr = S + rl * sp;
         double acc = 0.0;
+        if (ol) {  // + W_gal x the one-loop term (cosmo2D.c:2059-2060)
+          const double* Or = O + rl * sp;
 #pragma unroll 8
-        for (int pp = 0; pp < np; pp++) acc += Bz[pp] * (W[pp] * b1 + W[np + pp] * pref1 * bmag) * Sr[pp];
+          for (int pp = 0; pp < np; pp++)
+            acc += Bz[pp] * ((W[pp] * b1 + W[np + pp] * pref1 * bmag) * Sr[pp] + W[pp] * Or[pp]);
+        } else {
+#pragma unroll 8
+          for (int pp = 0; pp < np; pp++) acc += Bz[pp] * (W[pp] * b1 + W[np + pp] * pref1 * bmag) * Sr[pp];
+        }
         d.cl[(size_t)(d.n_ss + zl * d.ns + zs) * d.nrow + row0 + rr] = acc * pref2;
       }
     }
@@ -576,7 +599,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx_tatt(const DrawDesc* __
 // in the draw's checkpoint (d.ggk), which the VJP then reads instead of
 // evaluating the integrand again.
 // ----------------------------------------------------------------------------
-template <bool F32, bool KEEP>
+template <bool F32, bool KEEP, bool OL = false>
 __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ D,
                                                    const int4* __restrict__ work, int stage) {
   __shared__ double V[CLH_SUB_ROWS * CLH_NP_GG];
@@ -593,6 +616,16 @@ __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ 
   const int p = threadIdx.x % np, half = threadIdx.x / np;
   ClhGgNode<KEEP> nd;
   nd.init(d, T, clh_gg_nodes(d, zl), p, zl);
+  // OL: the one-loop bias term W_gal^2 (...) of draws with some b2 != 0
+  double ol_wg2 = 0.0, ol_g4f = 0.0, ol_lnfk = 0.0, ol_fk = 1.0;
+  if (OL && d.oneloop) {
+    const double* o = clh_gg_nodes(d, zl);
+    const double g = clh_growfac(d, o[p]);
+    ol_wg2 = o[3 * np + p] * o[3 * np + p];
+    ol_g4f = g * g * g * g * o[2 * np + p];
+    ol_fk = o[np + p];
+    ol_lnfk = log(ol_fk);
+  }
   constexpr int per = CLH_SUB_ROWS / 2;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int sub = 0; sub * CLH_SUB_ROWS < nr; sub++) {
@@ -602,8 +635,15 @@ __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ 
       double v = 0.0;
       if (base + rl < nr) {
         double t = 0.0, pf = 0.0, dwr = 0.0, dx;
-        if (nd.template eval<F32>(d, T, R[base + rl], t, pf, dwr, dx)) v = t * t * pf;
-        else t = pf = dwr = 0.0;
+        if (nd.template eval<F32>(d, T, R[base + rl], t, pf, dwr, dx)) {
+          v = t * t * pf;
+          if (OL && d.oneloop) {
+            const double ell = R[base + rl].ell;
+            v += ol_wg2 * clh_oneloop_gg(d, zl, log(ell) - ol_lnfk, ell / ol_fk, ol_g4f, pf);
+          }
+        } else {
+          t = pf = dwr = 0.0;
+        }
         if (KEEP) {
           double* g = d.ggk + ((size_t)(zl * d.nrow + row0 + base + rl) * 3) * np + p;
           g[0] = t;
@@ -815,20 +855,25 @@ hipError_t clh_launch_limber_sx_tatt(const DrawDesc* D, const int4* work, int n,
   hipLaunchKernelGGL(k_limber_sx_tatt, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }
-hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, int keep, hipStream_t s) {
+hipError_t clh_launch_limber_sx(const DrawDesc* D, const int4* work, int n, int f32, int keep, hipStream_t s,
+                                int ol) {
   if (n <= 0) return hipSuccess;
-  if (f32 && keep) hipLaunchKernelGGL((k_limber_sx<true, true>), dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  if (ol && f32) hipLaunchKernelGGL((k_limber_sx<true, false, true>), dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  else if (ol) hipLaunchKernelGGL((k_limber_sx<false, false, true>), dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  else if (f32 && keep) hipLaunchKernelGGL((k_limber_sx<true, true>), dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   else if (f32) hipLaunchKernelGGL((k_limber_sx<true, false>), dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   else if (keep) hipLaunchKernelGGL((k_limber_sx<false, true>), dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   else hipLaunchKernelGGL((k_limber_sx<false, false>), dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles,
-                                int keep, hipStream_t s) {
+                                int keep, hipStream_t s, int ol) {
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)stage_doubles;
   const int stage = stage_doubles > 0;
-  if (f32 && keep) hipLaunchKernelGGL((k_limber_gg<true, true>), dim3(n), dim3(192), lds, s, D, work, stage);
+  if (ol && f32) hipLaunchKernelGGL((k_limber_gg<true, false, true>), dim3(n), dim3(192), lds, s, D, work, stage);
+  else if (ol) hipLaunchKernelGGL((k_limber_gg<false, false, true>), dim3(n), dim3(192), lds, s, D, work, stage);
+  else if (f32 && keep) hipLaunchKernelGGL((k_limber_gg<true, true>), dim3(n), dim3(192), lds, s, D, work, stage);
   else if (f32) hipLaunchKernelGGL((k_limber_gg<true, false>), dim3(n), dim3(192), lds, s, D, work, stage);
   else if (keep) hipLaunchKernelGGL((k_limber_gg<false, true>), dim3(n), dim3(192), lds, s, D, work, stage);
   else hipLaunchKernelGGL((k_limber_gg<false, false>), dim3(n), dim3(192), lds, s, D, work, stage);

@@ -114,6 +114,19 @@ int clh_set_nuisance_ia(clh_ctx* ctx, const double* A1, int n);
  * TATT: mode 2 one value per source bin in each of A1, A2, BTA; mode 3
  * A1 = (A, eta), A2 = (A2, eta2), BTA[0] = b_TA */
 int clh_set_nuisance_ia_tatt(clh_ctx* ctx, const double* A1, const double* A2, const double* BTA, int n);
+/* set_nuisance_nonlinear_bias (generic_interface.cpp:1533-1568): b2 per lens
+ * bin; bs2 = -4/7 (b1 - 1) where b2 != 0, else 0.  Any b2 != 0 switches on
+ * the one-loop bias terms of the gg and gs integrands (cosmo2D.c:1970-2057,
+ * 2560-2603; FP64 gs, gg in the precision mode); with TATT and in
+ * clh_batch_vjp they are refused. */
+int clh_set_nuisance_nonlinear_bias(clh_ctx* ctx, const double* B1, const double* B2, int n);
+/* set_nuisance_nonlocal_bias (generic_interface.cpp:1610-1669): b3, bK per lens bin */
+int clh_set_nuisance_nonlocal_bias(clh_ctx* ctx, const double* B3, const double* BK, int n);
+/* set_bias_PS (generic_interface.cpp:1008-1076): PS row-major [12][N], rows
+ * 0..5 the FAST-PT bias spectra Pd1d2, Pd2d2, Pd1s2, Pd2s2, Ps2s2, Pd1p3
+ * (Mpc/h)^3 on N log-spaced k in [kmin, kmax) h/Mpc; sigma4 (Mpc/h)^3;
+ * cutoff accepted and ignored as the integrands ignore it */
+int clh_set_bias_PS(clh_ctx* ctx, const double* PS, double kmin, double kmax, double cutoff, double sigma4, int N);
 /* set_IA_PS (generic_interface.cpp:941-1001): PS row-major [12][N], rows
  * 0..9 the FAST-PT IA spectra (Mpc/h)^3 on N log-spaced k in [kmin, kmax)
  * h/Mpc (rows 10, 11 unused by the Limber integrands); cutoff is accepted and
@@ -211,6 +224,11 @@ typedef struct {
    * linear P(k,z) as clh_set_linear_power_spectrum takes it */
   double h;
   const double* plin_log10k; const double* plin_z; const double* lnPL; int plin_nk; int plin_nz;
+  /* one-loop galaxy bias per lens bin (NULL = 0; bs2 NULL = -4/7 (b1 - 1)
+   * where b2 != 0, as set_nuisance_nonlinear_bias sets it) and the
+   * set_bias_PS tables PS [12][fptb_n] over [fptb_kmin, fptb_kmax) h/Mpc */
+  const double* b2; const double* bs2; const double* b3; const double* bk;
+  const double* fptb; int fptb_n; double fptb_kmin; double fptb_kmax; double fptb_sigma4;
 } clh_draw;
 
 /* uploads B draws into device memory owned by the context */

@@ -72,6 +72,12 @@ struct oc_ctx {
   double *lkL, *pzL, *lnPL;
   int limber_gg;
   int nl_lmax[OC_MAXB]; /* LMAX[i] of the last C_cl_tomo run */
+  /* one-loop galaxy bias: nuisance.gb[1..3,5] (b2, bs2, b3, bK) and the
+     FAST-PT bias tables of set_bias_PS (FPTbias.tab[0..5], k in c/H0 units) */
+  double b2[OC_MAXB], bs2[OC_MAXB], b3[OC_MAXB], bk[OC_MAXB];
+  double* fptb; /* [6][fptb_n] */
+  int fptb_n;
+  double fptb_kmin, fptb_kmax, fptb_s4;
   /* caches */
   int g_valid;
   double *gt_tab, *gl_tab; /* [ns][N_a], [nl][N_a] */
@@ -121,6 +127,7 @@ void oc_free(oc_ctx* c) {
   free(c->mask);
   free(c->fpt);
   free(c->lkL); free(c->pzL); free(c->lnPL);
+  free(c->fptb);
   free(c);
 }
 
@@ -771,6 +778,20 @@ static double int_ss(oc_ctx* c, double a, int n1, int n2, double l, int EE) {
   return ss_core(c, a, fK, PK, growfac_a, dchida, ell_prefactor, n1, n2, WK1, WK2, WS1, WS2, EE, k);
 }
 
+static int has_b2(oc_ctx* c);
+static double fptb_at(oc_ctx* c, int t, double lnk);
+
+/* one-loop bias term of the gs integrand (cosmo2D.c:1970-1992, 2027-2057):
+   0.5 g4 (b2 d1d2 + bs2 d1s2 + b3 d1p3) + bK k^2 P when has_b2_galaxies */
+static double gs_oneloop(oc_ctx* c, int nl, double k, double PK, double growfac_a) {
+  if (!has_b2(c)) return 0.0;
+  if (!c->fptb) { oc_set_err(c, "one-loop bias needs the FAST-PT bias tables (set_bias_PS)"); return NAN; }
+  const double lnk = log(k);
+  const double g4 = growfac_a * growfac_a * growfac_a * growfac_a;
+  return 0.5 * g4 * (c->b2[nl] * fptb_at(c, 0, lnk) + c->bs2[nl] * fptb_at(c, 2, lnk) + c->b3[nl] * fptb_at(c, 5, lnk)) +
+         c->bk[nl] * k * k * PK;
+}
+
 /* int_for_C_gs_tomo_limber_core: cosmo2D.c:1890-2073 (TATT branch
    :1919-2006 and NLA :2007-2065, without RSD and one-loop bias, which are
    off by default); k = (l + 1/2) / f_K */
@@ -790,7 +811,7 @@ static double gs_core(oc_ctx* c, double a, double fK, double PK, double growfac_
     const double btazs = AX[0];
     IA_A2_Z1Z2(c, a, growfac_a, ns, ns, AX);
     const double C2ZS = AX[0];
-    const double ans = WK * ((WGAL * b1 + WMAG * ell_prefactor * bmag + 0.0) * PK + WGAL * 0.0)
+    const double ans = WK * ((WGAL * b1 + WMAG * ell_prefactor * bmag + 0.0) * PK + WGAL * gs_oneloop(c, nl, k, PK, growfac_a))
                      - WS * (WGAL * b1 + WMAG * ell_prefactor * bmag) *
                            (C1ZS * PK + C1ZS * btazs * (ta_dE1 + ta_dE2) - 5 * C2ZS * (mixA + mixB));
     return ans * (dchida / (fK * fK)) * ell_prefactor2;
@@ -798,7 +819,8 @@ static double gs_core(oc_ctx* c, double a, double fK, double PK, double growfac_
   double IA[2];
   IA_A1_Z1Z2(c, a, growfac_a, ns, ns, IA);
   const double C1ZS = IA[0];
-  const double ans = (WK - WS * C1ZS) * ((WGAL * b1 + WMAG * ell_prefactor * bmag + 0.0) * PK + WGAL * 0.0);
+  const double ans = (WK - WS * C1ZS) *
+                     ((WGAL * b1 + WMAG * ell_prefactor * bmag + 0.0) * PK + WGAL * gs_oneloop(c, nl, k, PK, growfac_a));
   return ans * (dchida / (fK * fK)) * ell_prefactor2;
 }
 
@@ -837,8 +859,24 @@ static double p_lin(oc_ctx* c, double k, double a) {
   return exp(out) / (OC_COVERH0 * OC_COVERH0 * OC_COVERH0);
 }
 
-/* int_for_C_gg_tomo_limber: cosmo2D.c:2489-2606 (RSD on, no HOD, no one-loop
-   bias); use_lin selects p_lin over Pdelta */
+/* has_b2_galaxies (cosmo2D.c:282-288) */
+static int has_b2(oc_ctx* c) {
+  for (int i = 0; i < c->nl; i++)
+    if (c->b2[i] != 0.0) return 1;
+  return 0;
+}
+
+/* FPTbias.tab[t] at ln k (cosmo2D.c:2566-2593): uniform in ln k with step
+   (ln kmax - ln kmin) / N, zero outside the range */
+static double fptb_at(oc_ctx* c, int t, double lnk) {
+  const double l0 = log(c->fptb_kmin), l1 = log(c->fptb_kmax), dl = (l1 - l0) / c->fptb_n;
+  if (lnk < l0 || lnk > l1) return 0.0;
+  return interpol1d(c->fptb + (size_t)t * c->fptb_n, c->fptb_n, l0, l1, dl, lnk);
+}
+
+/* int_for_C_gg_tomo_limber: cosmo2D.c:2489-2606 (RSD on, no HOD); use_lin
+   selects p_lin over Pdelta; the one-loop bias term when has_b2_galaxies and
+   not use_lin (:2560-2603) */
 static double int_gg_ps(oc_ctx* c, double a, int ni, double l, int use_lin) {
   double fK, dchida;
   chi_all(c, a, &fK, &dchida);
@@ -860,8 +898,29 @@ static double int_gg_ps(oc_ctx* c, double a, int ni, double l, int use_lin) {
   const double WRSDi = W_RSD(c, ell, a_0, a_1, ni);
   res *= (WGALi * b1i + WMAGi * ell_prefactor * bmagi + WRSDi);
   res *= (WGALi * b1i + WMAGi * ell_prefactor * bmagi + WRSDi);
-  res *= use_lin ? p_lin(c, k, a) : p_nonlin(c, k, a);
-  return (res + 0.0) * dchida / (fK * fK);
+  const double PK = use_lin ? p_lin(c, k, a) : p_nonlin(c, k, a);
+  res *= PK;
+  double oneloop = 0.0;
+  if (has_b2(c) && !use_lin) {
+    if (!c->fptb) { oc_set_err(c, "one-loop bias needs the FAST-PT bias tables (set_bias_PS)"); return NAN; }
+    const double lnk = log(k);
+    const double l0 = log(c->fptb_kmin), l1 = log(c->fptb_kmax);
+    const int in = !(lnk < l0 || lnk > l1);
+    const double s4 = c->fptb_s4;
+    const double d1d2 = fptb_at(c, 0, lnk);
+    const double d2d2 = in ? fptb_at(c, 1, lnk) - 2. * s4 : 0.0;
+    const double d1s2 = fptb_at(c, 2, lnk);
+    const double d2s2 = in ? fptb_at(c, 3, lnk) - 4. / 3. * s4 : 0.0;
+    const double s2s2 = in ? fptb_at(c, 4, lnk) - 8. / 9. * s4 : 0.0;
+    const double d1p3 = fptb_at(c, 5, lnk);
+    const double g = growfac(c, a), g4 = g * g * g * g;
+    const double b2 = c->b2[ni], bs2 = c->bs2[ni], b3 = c->b3[ni], bk = c->bk[ni];
+    oneloop = WGALi * WGALi *
+              (g4 * (b1i * b2 * d1d2 + 0.25 * b2 * b2 * d2d2 + b1i * bs2 * d1s2 + 0.5 * b2 * bs2 * d2s2 +
+                     0.25 * bs2 * bs2 * s2s2 + b1i * b3 * d1p3) +
+               (2 * b1i * bk * k * k * PK));
+  }
+  return (res + oneloop) * dchida / (fK * fK);
 }
 
 static int ensure_g(oc_ctx* c) {
@@ -2086,4 +2145,38 @@ int oc_cl_gg_lin_nointerp(oc_ctx* c, const double* l, int nl, double* out) {
   for (int k = 0; k < c->nl; k++)
     for (int i = 0; i < nl; i++) out[(size_t)k * nl + i] = C_gg_nointerp_ps(c, &r, l[i], k, 1);
   return c->status;
+}
+
+/* set_nuisance_nonlinear_bias / set_nuisance_nonlocal_bias
+   (generic_interface.cpp:1533-1669): b2 per lens bin with
+   bs2 = -4/7 (b1 - 1) unless b2 = 0 (then 0); b3 and bK per lens bin.
+   NULL b3 / bk leave them unchanged. */
+int oc_set_bias_nl(oc_ctx* c, const double* b1, const double* b2, const double* b3, const double* bk) {
+  for (int i = 0; i < c->nl; i++) {
+    if (b2) {
+      c->b2[i] = b2[i];
+      c->bs2[i] = (b2[i] == 0.0) ? 0.0 : (-4. / 7.) * (b1[i] - 1.0);
+    }
+    if (b3) c->b3[i] = b3[i];
+    if (bk) c->bk[i] = bk[i];
+  }
+  return 0;
+}
+
+/* set_bias_PS (generic_interface.cpp:1008-1076): PS[12][N], rows 0..5 the
+   FAST-PT bias spectra (d1d2, d2d2, d1s2, d2s2, s2s2, d1p3) in (Mpc/h)^3 ->
+   / (c/H0)^3; k range in h/Mpc -> * c/H0; sigma4 in (Mpc/h)^3 -> / (c/H0)^3 */
+int oc_set_bias_ps(oc_ctx* c, const double* PS, double kmin, double kmax, double cutoff, double sigma4, int N) {
+  (void)cutoff;
+  if (N < 2) { oc_set_err(c, "set_bias_PS: N < 2"); return -1; }
+  free(c->fptb);
+  c->fptb = malloc(sizeof(double) * 6 * (size_t)N);
+  const double h3 = OC_COVERH0 * OC_COVERH0 * OC_COVERH0;
+  for (int t = 0; t < 6; t++)
+    for (int j = 0; j < N; j++) c->fptb[(size_t)t * N + j] = PS[(size_t)t * N + j] / h3;
+  c->fptb_n = N;
+  c->fptb_kmin = kmin * OC_COVERH0;
+  c->fptb_kmax = kmax * OC_COVERH0;
+  c->fptb_s4 = sigma4 / h3;
+  return 0;
 }

@@ -125,6 +125,12 @@ int oc_nl_inputs(oc_ctx* c, double* x, double* fx);
 /* Limber C_gg with the linear P(k,z) at arbitrary ell: out[nl][n] */
 int oc_cl_gg_lin_nointerp(oc_ctx* c, const double* l, int n, double* out);
 
+/* ---- one-loop galaxy bias (cosmo2D.c:1970-2057, 2560-2603) ---- */
+/* b2 (with bs2 = -4/7 (b1 - 1) unless b2 = 0), b3, bK per lens bin; NULL = unchanged */
+int oc_set_bias_nl(oc_ctx* c, const double* b1, const double* b2, const double* b3, const double* bk);
+/* set_bias_PS: PS[12][N] FAST-PT bias spectra (rows 0..5 used), k range h/Mpc */
+int oc_set_bias_ps(oc_ctx* c, const double* PS, double kmin, double kmax, double cutoff, double sigma4, int N);
+
 #ifdef __cplusplus
 }
 #endif

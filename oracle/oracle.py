@@ -79,6 +79,11 @@ def _load():
         lib.oc_nl_inputs.argtypes = [vp, _D, _D]
         lib.oc_cl_gg_lin_nointerp.restype = ctypes.c_int
         lib.oc_cl_gg_lin_nointerp.argtypes = [vp, _D, ctypes.c_int, _D]
+        lib.oc_set_bias_nl.restype = ctypes.c_int
+        lib.oc_set_bias_nl.argtypes = [vp, _D, _D, _D, _D]
+        lib.oc_set_bias_ps.restype = ctypes.c_int
+        lib.oc_set_bias_ps.argtypes = [vp, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int]
         _lib = lib
     return _lib
 
@@ -207,7 +212,20 @@ class Oracle:
         if getattr(prob, "lnPL", None) is not None:
             self.set_plin(prob.plin_log10k, prob.plin_z, prob.lnPL)
         self.set_limber_gg(getattr(prob, "limber_gg", 1))
+        if getattr(prob, "b2", None) is not None:
+            self.set_bias_nl(prob.b1, prob.b2, prob.b3, prob.bk)
+            self.set_bias_ps(prob.bias_ps, prob.bias_ps_kmin, prob.bias_ps_kmax, prob.bias_ps_sigma4)
         return self
+
+    # ------------------------------------------------- one-loop galaxy bias
+    def set_bias_nl(self, b1, b2=None, b3=None, bk=None):
+        v = [None if x is None else _arr(x) for x in (b1, b2, b3, bk)]
+        self._chk(self.lib.oc_set_bias_nl(self.c, *[_p(x) for x in v]))
+
+    def set_bias_ps(self, PS, kmin, kmax, sigma4, cutoff=1.0e4):
+        PS = _arr(PS)
+        self._chk(self.lib.oc_set_bias_ps(self.c, _p(PS), float(kmin), float(kmax), float(cutoff), float(sigma4),
+                                          PS.size // 12))
 
     # ----------------------------------------------------- non-Limber w
     def set_h0(self, h0):
