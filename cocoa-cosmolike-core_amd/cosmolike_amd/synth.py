@@ -35,7 +35,7 @@ def _E(z, om):
     return np.sqrt(om * (1.0 + z) ** 3 + (1.0 - om))
 
 
-def background_grid() -> np.ndarray:
+def background_grid(cmb: bool = False) -> np.ndarray:
     """z grid for chi(z) and G(z): dense below z=4, sparser up to z=50.
 
     chi_all reads chi[j+2] (cosmolike/cosmo3D.c:66-67) and the gg RSD term
@@ -44,7 +44,11 @@ def background_grid() -> np.ndarray:
     """
     z1 = np.linspace(0.0, 4.0, 401)
     z2 = np.geomspace(4.05, 50.0, 120)
-    return np.concatenate([z1, z2])
+    if not cmb:
+        return np.concatenate([z1, z2])
+    # CMB lensing (g_cmb, redshift_spline.c:1111-1125) reads chi(z = 1090)
+    z3 = np.geomspace(52.0, 1300.0, 80)
+    return np.concatenate([z1, z2, z3])
 
 
 def comoving_distance(z: np.ndarray, om: float) -> np.ndarray:
@@ -150,6 +154,37 @@ def oneloop_problem(cosmo: Cosmology | None = None, n_ell: int = 500, b3=None, b
     return p
 
 
+def ells_fourier(nells: int, lmin: float, lmax: float) -> np.ndarray:
+    """like.ell of init_binning_fourier (generic_interface.cpp:425-470): the
+    geometric centres of nells log-spaced bins in [lmin, lmax]."""
+    e = lmin * (lmax / lmin) ** (np.arange(nells + 1) / nells)
+    return np.sqrt(e[:-1] * e[1:])
+
+
+def healpix_window_like(nside: int = 2048, lmax: int = 6000) -> np.ndarray:
+    """A smooth stand-in for a HEALPix pixel window (ell = 0..lmax): a
+    Gaussian of the pixel scale, not the HEALPix tabulation."""
+    l = np.arange(lmax + 1.0)
+    return np.exp(-0.5 * (l / (2.0 * nside)) ** 2)
+
+
+def sixpt_problem(cosmo: Cosmology | None = None, n_ell: int = 500, fwhm_arcmin: float = 1.4,
+                  cmb_lmin: int = 8, cmb_lmax: int = 4000, window: bool = True, nells: int = 20,
+                  ell_min: float = 30.0, ell_max: float = 3000.0) -> Problem:
+    """The full 3x2pt layout with the CMB lensing probes of 6x2pt (w_gk,
+    w_ks in the same 26 theta bins, C_kk at nells log-centred ells): a
+    background that reaches z = 1300, a beam of fwhm_arcmin over
+    [cmb_lmin, cmb_lmax] and optionally a pixel-window-like filter."""
+    cosmo = cosmo or Cosmology()
+    p = full_3x2pt_problem(cosmo, n_ell)
+    p.chi_z, p.chi, p.G_z, p.G = background(cosmo, cmb=True)
+    p.sixpt = 1
+    p.cmb_fwhm, p.cmb_lmin, p.cmb_lmax = fwhm_arcmin, cmb_lmin, cmb_lmax
+    p.cmb_win = healpix_window_like() if window else None
+    p.kk_nells, p.kk_lmin, p.kk_lmax = nells, ell_min, ell_max
+    return p
+
+
 def tatt_problem(cosmo: Cosmology | None = None, n_ell: int = 500, A2: float = -0.8, eta2: float = 0.5,
                  bta: float = 0.7) -> Problem:
     """The full 3x2pt layout with TATT intrinsic alignments (redshift
@@ -203,8 +238,8 @@ def nonlimber_problem(cosmo: Cosmology | None = None, n_ell: int = 500, bmag=Non
     return p
 
 
-def background(cosmo: Cosmology):
-    z = background_grid()
+def background(cosmo: Cosmology, cmb: bool = False):
+    z = background_grid(cmb)
     chi = comoving_distance(z, cosmo.omega_m)
     d = growth_D(z, cosmo.omega_m)
     G = d * (1.0 + z)  # G = D/a
@@ -320,6 +355,15 @@ class Problem:
     bias_ps_kmin: float = 0.0
     bias_ps_kmax: float = 0.0
     bias_ps_sigma4: float = 0.0
+    # 6x2pt: CMB lensing probes, the CMB filter and the kk ell bins
+    sixpt: int = 0
+    cmb_fwhm: float | None = None
+    cmb_lmin: int = 0
+    cmb_lmax: int = 0
+    cmb_win: np.ndarray | None = None
+    kk_nells: int = 0
+    kk_lmin: float = 0.0
+    kk_lmax: float = 0.0
     # non-Limber w (like.adopt_limber_gg = 0) and its linear P(k,z)
     limber_gg: int = 1
     plin_log10k: np.ndarray | None = None

@@ -78,6 +78,11 @@ struct oc_ctx {
   double* fptb; /* [6][fptb_n] */
   int fptb_n;
   double fptb_kmin, fptb_kmax, fptb_s4;
+  /* CMB lensing cross-correlations (init_cmb_cross_correlation): beam FWHM
+     (rad), ell range of the beam filter, HEALPix pixel window (optional) */
+  double cmb_fwhm;
+  int cmb_lmin, cmb_lmax, cmb_nwin;
+  double* cmb_win;
   /* caches */
   int g_valid;
   double *gt_tab, *gl_tab; /* [ns][N_a], [nl][N_a] */
@@ -128,6 +133,7 @@ void oc_free(oc_ctx* c) {
   free(c->fpt);
   free(c->lkL); free(c->pzL); free(c->lnPL);
   free(c->fptb);
+  free(c->cmb_win);
   free(c);
 }
 
@@ -2179,4 +2185,236 @@ int oc_set_bias_ps(oc_ctx* c, const double* PS, double kmin, double kmax, double
   c->fptb_kmax = kmax * OC_COVERH0;
   c->fptb_s4 = sigma4 / h3;
   return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* 6x2pt: CMB lensing convergence cross galaxies (gk), shear (ks), kk     */
+/* ---------------------------------------------------------------------- */
+/* g_cmb (redshift_spline.c:1111-1125) for a flat background (f_K(chi) = chi,
+   Omega_v = 1 - Omega_m as set_cosmological_parameters sets it) */
+static double g_cmb(oc_ctx* c, double a) {
+  const double chi_cmb = chi_of_a(c, 1. / 1091.);
+  return (chi_cmb - chi_of_a(c, a)) / chi_cmb;
+}
+/* W_k (radial_weights.c:107-113) */
+static double W_k(oc_ctx* c, double a, double fK) { return (1.5 * c->omega_m * fK / a) * g_cmb(c, a); }
+
+/* int_for_C_gk_tomo_limber (cosmo2D.c:2857-2954), RSD off (include_RSD_GK = 0) */
+static double int_gk(oc_ctx* c, double a, int nl, double l) {
+  const double ell = l + 0.5;
+  double fK, dchida;
+  chi_all(c, a, &fK, &dchida);
+  const double hh = hoverh0v2(a, dchida);
+  const double k = ell / fK;
+  const double WK = W_k(c, a, fK);
+  const double WGAL = W_gal(c, a, nl, hh), WMAG = W_mag(c, a, fK, nl);
+  const double ell_prefactor = l * (l + 1.) / (ell * ell);
+  const double PK = p_nonlin(c, k, a);
+  const double res = WK * (WGAL * c->b1[nl] + WMAG * ell_prefactor * c->bmag[nl]) * PK;
+  double oneloop = 0.0;
+  if (has_b2(c)) {
+    if (!c->fptb) { oc_set_err(c, "one-loop bias needs the FAST-PT bias tables (set_bias_PS)"); return NAN; }
+    const double g = growfac(c, a), g4 = g * g * g * g, lnk = log(k);
+    oneloop = WK * WGAL *
+              (g4 * (0.5 * c->b2[nl] * fptb_at(c, 0, lnk) + 0.5 * c->bs2[nl] * fptb_at(c, 2, lnk) +
+                     0.5 * c->b3[nl] * fptb_at(c, 5, lnk)) +
+               c->bk[nl] * k * k * PK);
+  }
+  return ((res + oneloop) * dchida / (fK * fK)) * ell_prefactor;
+}
+
+/* int_for_C_ks_tomo_limber (cosmo2D.c:3178-3211) */
+static double int_ks(oc_ctx* c, double a, int ns, double l) {
+  const double ell = l + 0.5;
+  const double growfac_a = growfac(c, a);
+  double fK, dchida;
+  chi_all(c, a, &fK, &dchida);
+  const double hh = hoverh0v2(a, dchida);
+  const double k = ell / fK;
+  const double PK = p_nonlin(c, k, a);
+  const double WK1 = W_kappa(c, a, fK, ns), WK2 = W_k(c, a, fK);
+  const double ell_prefactor1 = l * (l + 1.) / (ell * ell);
+  const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+  const double ell_prefactor2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+  double IA[2];
+  IA_A1_Z1Z2(c, a, growfac_a, ns, ns, IA);
+  const double WS1 = W_source(c, a, ns, hh) * IA[0];
+  return ((WK1 - WS1) * WK2 * PK * dchida / (fK * fK)) * ell_prefactor1 * ell_prefactor2;
+}
+
+/* int_for_C_kk_limber (cosmo2D.c:3428-3446) */
+static double int_kk(oc_ctx* c, double a, double l) {
+  double fK, dchida;
+  chi_all(c, a, &fK, &dchida);
+  const double ell = l + 0.5;
+  const double k = ell / fK;
+  const double WK = W_k(c, a, fK);
+  const double ell_prefactor = l * (l + 1.0) / (ell * ell);
+  return WK * WK * p_nonlin(c, k, a) * (dchida / (fK * fK)) * ell_prefactor * ell_prefactor;
+}
+
+/* C_gk_tomo_limber_nointerp (:2960-3015), C_ks_tomo_limber_nointerp
+   (:3217-3264; amin_source / amax_source, redshift_spline.c:27-39),
+   C_kk_limber_nointerp (:3449-3481): 96-point GL */
+static double C_cmb_nointerp(oc_ctx* c, const gl_rule* r, int probe, double l, int bin) {
+  double amin, amax;
+  if (probe == 3) { amin = amin_lens(c, bin); amax = amax_lens(c, bin); }
+  else if (probe == 4) { amin = 1. / (c->s_zmax_all + 1.); amax = 1. / (1. + fmax(c->s_zmin_all, 0.001)); }
+  else { amin = OC_A_MIN * (1. + 1.e-5); amax = 0.99999; }
+  double s = 0.0;
+  for (int p = 0; p < r->n; p++) {
+    double x, w;
+    gl_point(r, amin, amax, p, &x, &w);
+    s += w * (probe == 3 ? int_gk(c, x, bin, l) : probe == 4 ? int_ks(c, x, bin, l) : int_kk(c, x, l));
+  }
+  return s;
+}
+
+int oc_set_cmb(oc_ctx* c, double fwhm_arcmin, int lmin, int lmax, const double* win, int nwin) {
+  c->cmb_fwhm = fwhm_arcmin * 2.90888208665721580e-4; /* init_cmb_cross_correlation, generic_interface.cpp:519 */
+  c->cmb_lmin = lmin;
+  c->cmb_lmax = lmax;
+  free(c->cmb_win);
+  c->cmb_win = NULL;
+  c->cmb_nwin = 0;
+  if (win && nwin > 0) {
+    c->cmb_win = dup(win, nwin);
+    c->cmb_nwin = nwin;
+  }
+  return 0;
+}
+
+/* beam_cmb x w_pixel (cosmo2D.c:268-279, 1003-1010) */
+static double cmb_filter(oc_ctx* c, int l) {
+  const double s = c->cmb_fwhm / sqrt(16.0 * log(2.0));
+  double f = (l < c->cmb_lmin || l > c->cmb_lmax) ? 0.0 : exp(-l * (l + 1.0) * s * s);
+  if (c->cmb_nwin > 0) f *= (l < c->cmb_nwin) ? c->cmb_win[l] : 0.0;
+  return f;
+}
+
+int oc_cl_cmb_nointerp(oc_ctx* c, int probe, const double* l, int n, double* out) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  if (probe < 3 || probe > 5) { oc_set_err(c, "probe %d", probe); return -1; }
+  gl_rule r;
+  gl_make(&r, gl_order_gg(c->hdi));
+  const int nb = probe == 3 ? c->nl : probe == 4 ? c->ns : 1;
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int b = 0; b < nb; b++)
+    for (int i = 0; i < n; i++) out[(size_t)b * n + i] = C_cmb_nointerp(c, &r, probe, l[i], b);
+  return c->status;
+}
+
+/* C_gk / C_ks / C_kk tables at the gs/gg ell grid (cosmo2D.c:3020-3060,
+   3270-3310, 3484-3516) */
+static void table_cmb(oc_ctx* c, int probe, double* tab /* [nb][nell] */) {
+  const int nell = c->n_ell;
+  double lim[3];
+  gx_lim(c, lim);
+  gl_rule r;
+  gl_make(&r, gl_order_gg(c->hdi));
+  const int nb = probe == 3 ? c->nl : probe == 4 ? c->ns : 1;
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int b = 0; b < nb; b++)
+    for (int i = 0; i < nell; i++) tab[(size_t)b * nell + i] = C_cmb_nointerp(c, &r, probe, exp(lim[0] + i * lim[2]), b);
+}
+
+/* w_gk_tomo / w_ks_tomo (cosmo2D.c:913-1296): C_ell direct below LMIN_tab,
+   the table fill above, times the CMB filter, then the w (gk) or gamma_t
+   (ks) bin-averaged kernel; out[nb][ntheta] */
+static int w_cmb(oc_ctx* c, int probe, double* out) {
+  if (!c->K_valid && build_kernels(c)) return -1;
+  if (ensure_g(c)) return -1;
+  const int L = c->lmax, nell = c->n_ell, nb = probe == 3 ? c->nl : c->ns;
+  double* tab = malloc(sizeof(double) * (size_t)nb * nell);
+  double* cl = malloc(sizeof(double) * (size_t)nb * L);
+  table_cmb(c, probe, tab);
+  double lim[3];
+  gx_lim(c, lim);
+  gl_rule r;
+  gl_make(&r, gl_order_gg(c->hdi));
+  #pragma omp parallel for schedule(dynamic)
+  for (int b = 0; b < nb; b++) {
+    double* o = cl + (size_t)b * L;
+    o[0] = 0.0;
+    for (int l = 1; l < OC_LMIN_TAB; l++) o[l] = C_cmb_nointerp(c, &r, probe, (double)l, b) * cmb_filter(c, l);
+    fill(tab + (size_t)b * nell, lim, nell, OC_LMIN_TAB, L, o);
+    for (int l = OC_LMIN_TAB; l < L; l++) o[l] *= cmb_filter(c, l);
+  }
+  contract(c->K + (size_t)(probe == 3 ? 3 : 2) * c->ntheta * L, cl, nb, c->ntheta, L, out);
+  free(tab); free(cl);
+  return c->status;
+}
+int oc_w_gk(oc_ctx* c, double* out) { c->status = 0; return w_cmb(c, 3, out); }
+int oc_w_ks(oc_ctx* c, double* out) { c->status = 0; return w_cmb(c, 4, out); }
+
+/* the kk block at like.ell (generic_interface.hpp:1085-1094): direct at
+   ell <= LMIN_tab, C_kk_limber's table interpolation above */
+int oc_cl_kk(oc_ctx* c, const double* ell, int n, double* out) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  double* tab = malloc(sizeof(double) * (size_t)c->n_ell);
+  table_cmb(c, 5, tab);
+  double lim[3];
+  gx_lim(c, lim);
+  gl_rule r;
+  gl_make(&r, gl_order_gg(c->hdi));
+  for (int i = 0; i < n; i++)
+    out[i] = (ell[i] <= OC_LMIN_TAB) ? C_cmb_nointerp(c, &r, 5, ell[i], 0)
+                                    : interpol1d(tab, c->n_ell, lim[0], lim[1], lim[2], log(ell[i]));
+  free(tab);
+  return c->status;
+}
+
+int oc_datavector_size_6x2pt(oc_ctx* c, int ncl) {
+  return c->ntheta * (2 * oc_npairs_ss(c) + oc_npairs_gs(c) + 2 * c->nl + c->ns) + ncl;
+}
+
+/* compute_Mx2pt_N_masked<0,6> (generic_interface.hpp:1137-1159) with every
+   mask entry 1: blocks xi+- | gamma_t | w | w_gk | w_ks | kk placed by ord
+   (compute_data_vector_Mx2pt_N_starts<0,6>, :770-791); point mass and
+   (1+m) on gamma_t, (1+m_s) on w_ks (add_calib_and_set_mask_X_N, :800-920);
+   kk at the ncl ells (like.ell) */
+int oc_datavector_6x2pt(oc_ctx* c, const int ord[6], const double* ell, int ncl, double* dv) {
+  c->status = 0;
+  const int nt = c->ntheta, nss = oc_npairs_ss(c), ngs = oc_npairs_gs(c), nl = c->nl, ns = c->ns;
+  const int sizes[6] = {2 * nt * nss, nt * ngs, nt * nl, nt * nl, nt * ns, ncl};
+  for (int i = 0; i < 6; i++)
+    if (ord[i] < 0 || ord[i] > 5) { oc_set_err(c, "ord"); return -1; }
+  int idx[6] = {0, 1, 2, 3, 4, 5};
+  for (int i = 1; i < 6; i++)
+    for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) { int t = idx[j]; idx[j] = idx[j - 1]; idx[j - 1] = t; }
+  int start[6];
+  int n = 0;
+  for (int i = 0; i < 6; i++) {
+    n += sizes[i];
+    start[i] = 0;
+    for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+  }
+  for (int i = 0; i < 6; i++) {
+    if (start[i] + sizes[i] > n) { oc_set_err(c, "ord: blocks do not tile the data vector"); return -1; }
+    for (int j = 0; j < i; j++)
+      if (sizes[i] && sizes[j] && start[i] < start[j] + sizes[j] && start[j] < start[i] + sizes[i]) {
+        oc_set_err(c, "ord: blocks do not tile the data vector");
+        return -1;
+      }
+  }
+  const int n3 = oc_datavector_size(c);
+  double* d3 = malloc(sizeof(double) * n3);
+  const int ord3[3] = {0, 1, 2};
+  if (oc_datavector_3x2pt(c, ord3, d3)) { free(d3); return -1; }
+  double* gk = malloc(sizeof(double) * (size_t)nl * nt);
+  double* ks = malloc(sizeof(double) * (size_t)ns * nt);
+  double* kk = malloc(sizeof(double) * (size_t)(ncl > 0 ? ncl : 1));
+  if (oc_w_gk(c, gk) || oc_w_ks(c, ks) || oc_cl_kk(c, ell, ncl, kk)) { free(d3); free(gk); free(ks); free(kk); return -1; }
+  for (int i = 0; i < n; i++) dv[i] = 0.0;
+  for (int k = 0; k < sizes[0]; k++) dv[start[0] + k] = d3[k];
+  for (int k = 0; k < sizes[1]; k++) dv[start[1] + k] = d3[sizes[0] + k];
+  for (int k = 0; k < sizes[2]; k++) dv[start[2] + k] = d3[sizes[0] + sizes[1] + k];
+  for (int k = 0; k < sizes[3]; k++) dv[start[3] + k] = gk[k];
+  for (int z = 0; z < ns; z++)
+    for (int i = 0; i < nt; i++) dv[start[4] + nt * z + i] = ks[z * nt + i] * (1.0 + c->shear_m[z]);
+  for (int k = 0; k < ncl; k++) dv[start[5] + k] = kk[k];
+  free(d3); free(gk); free(ks); free(kk);
+  return c->status;
 }

@@ -131,6 +131,18 @@ int oc_set_bias_nl(oc_ctx* c, const double* b1, const double* b2, const double* 
 /* set_bias_PS: PS[12][N] FAST-PT bias spectra (rows 0..5 used), k range h/Mpc */
 int oc_set_bias_ps(oc_ctx* c, const double* PS, double kmin, double kmax, double cutoff, double sigma4, int N);
 
+/* ---- 6x2pt CMB lensing probes (cosmo2D.c:913-1296, 2857-3516) ---- */
+/* init_cmb_cross_correlation: beam FWHM in arcmin, filter ell range, HEALPix
+   pixel window per ell (NULL / 0: none) */
+int oc_set_cmb(oc_ctx* c, double fwhm_arcmin, int lmin, int lmax, const double* win, int nwin);
+/* direct GL C_ell: probe 3 gk [nl][n], 4 ks [ns][n], 5 kk [1][n] */
+int oc_cl_cmb_nointerp(oc_ctx* c, int probe, const double* l, int n, double* out);
+int oc_w_gk(oc_ctx* c, double* out);  /* [nl][ntheta] */
+int oc_w_ks(oc_ctx* c, double* out);  /* [ns][ntheta] */
+int oc_cl_kk(oc_ctx* c, const double* ell, int n, double* out);
+int oc_datavector_size_6x2pt(oc_ctx* c, int ncl);
+int oc_datavector_6x2pt(oc_ctx* c, const int ord[6], const double* ell, int ncl, double* dv);
+
 #ifdef __cplusplus
 }
 #endif

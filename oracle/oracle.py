@@ -81,6 +81,16 @@ def _load():
         lib.oc_cl_gg_lin_nointerp.argtypes = [vp, _D, ctypes.c_int, _D]
         lib.oc_set_bias_nl.restype = ctypes.c_int
         lib.oc_set_bias_nl.argtypes = [vp, _D, _D, _D, _D]
+        for name in ("oc_set_cmb", "oc_cl_cmb_nointerp", "oc_w_gk", "oc_w_ks", "oc_cl_kk", "oc_datavector_size_6x2pt",
+                     "oc_datavector_6x2pt"):
+            getattr(lib, name).restype = ctypes.c_int
+        lib.oc_set_cmb.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int]
+        lib.oc_cl_cmb_nointerp.argtypes = [vp, ctypes.c_int, _D, ctypes.c_int, _D]
+        lib.oc_w_gk.argtypes = [vp, _D]
+        lib.oc_w_ks.argtypes = [vp, _D]
+        lib.oc_cl_kk.argtypes = [vp, _D, ctypes.c_int, _D]
+        lib.oc_datavector_size_6x2pt.argtypes = [vp, ctypes.c_int]
+        lib.oc_datavector_6x2pt.argtypes = [vp, ctypes.POINTER(ctypes.c_int), _D, ctypes.c_int, _D]
         lib.oc_set_bias_ps.restype = ctypes.c_int
         lib.oc_set_bias_ps.argtypes = [vp, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int]
@@ -212,10 +222,50 @@ class Oracle:
         if getattr(prob, "lnPL", None) is not None:
             self.set_plin(prob.plin_log10k, prob.plin_z, prob.lnPL)
         self.set_limber_gg(getattr(prob, "limber_gg", 1))
+        if getattr(prob, "cmb_fwhm", None) is not None:
+            self.set_cmb(prob.cmb_fwhm, prob.cmb_lmin, prob.cmb_lmax, prob.cmb_win)
         if getattr(prob, "b2", None) is not None:
             self.set_bias_nl(prob.b1, prob.b2, prob.b3, prob.bk)
             self.set_bias_ps(prob.bias_ps, prob.bias_ps_kmin, prob.bias_ps_kmax, prob.bias_ps_sigma4)
         return self
+
+    # ------------------------------------------------------------- 6x2pt
+    def set_cmb(self, fwhm_arcmin, lmin, lmax, win=None):
+        w = None if win is None else _arr(win)
+        self._chk(self.lib.oc_set_cmb(self.c, float(fwhm_arcmin), int(lmin), int(lmax), _p(w),
+                                      0 if w is None else len(w)))
+
+    def cl_cmb_nointerp(self, probe, ells):
+        ells = _arr(ells)
+        idx = {"gk": 3, "ks": 4, "kk": 5}[probe]
+        nb = {"gk": self.nl, "ks": self.ns, "kk": 1}[probe]
+        out = np.empty((nb, len(ells)))
+        self._chk(self.lib.oc_cl_cmb_nointerp(self.c, idx, _p(ells), len(ells), _p(out)))
+        return out
+
+    def w_gk(self):
+        out = np.empty((self.nl, self.ntheta))
+        self._chk(self.lib.oc_w_gk(self.c, _p(out)))
+        return out
+
+    def w_ks(self):
+        out = np.empty((self.ns, self.ntheta))
+        self._chk(self.lib.oc_w_ks(self.c, _p(out)))
+        return out
+
+    def cl_kk(self, ells):
+        ells = _arr(ells)
+        out = np.empty(len(ells))
+        self._chk(self.lib.oc_cl_kk(self.c, _p(ells), len(ells), _p(out)))
+        return out
+
+    def datavector_6x2pt(self, ells, ord=(0, 1, 2, 3, 4, 5)):
+        ells = _arr(ells)
+        n = self.lib.oc_datavector_size_6x2pt(self.c, len(ells))
+        out = np.empty(n)
+        o = (ctypes.c_int * 6)(*ord)
+        self._chk(self.lib.oc_datavector_6x2pt(self.c, o, _p(ells), len(ells), _p(out)))
+        return out
 
     # ------------------------------------------------- one-loop galaxy bias
     def set_bias_nl(self, b1, b2=None, b3=None, bk=None):
