@@ -194,6 +194,23 @@ class Cosmolike:
             raise ValueError("PS must hold 12 rows of N entries")
         self._chk(self.L.clh_set_bias_PS(self.h, _p(PS), float(kmin), float(kmax), float(cutoff), float(sigma4), n))
 
+    def init_probes(self, probes="3x2pt"):
+        """init_probes: "3x2pt" or "6x2pt" (adds w_gk, w_ks and C_kk blocks)."""
+        self._chk(self.L.clh_init_probes(self.h, probes.encode()))
+
+    def init_cmb_cross_correlation(self, lmin, lmax, fwhm_arcmin, healpix_window=None):
+        w = None if healpix_window is None else _arr(healpix_window)
+        self._chk(self.L.clh_init_cmb_cross_correlation(self.h, int(lmin), int(lmax), float(fwhm_arcmin), _p(w),
+                                                        0 if w is None else len(w)))
+
+    def compute_6x2pt_real(self, ord=(0, 1, 2, 3, 4, 5)):
+        """compute_Mx2pt_N_masked<0,6>(ord)."""
+        n = self.datavector_size()
+        out = np.empty(n)
+        o = (ctypes.c_int * 6)(*ord)
+        self._chk(self.L.clh_compute_6x2pt_real(self.h, o, _p(out), n))
+        return out
+
     def set_linear_power_spectrum(self, log10k, z, lnPL):
         lk, z, lnPL = _arr(log10k), _arr(z), _arr(lnPL)
         self._chk(self.L.clh_set_linear_power_spectrum(self.h, _p(lk), len(lk), _p(z), len(z), _p(lnPL)))
@@ -225,6 +242,12 @@ class Cosmolike:
         self.init_IA(getattr(prob, "ia_model", 0), prob.ia_mode)
         self.set_legendre_mode(legendre_mode)
         self.set_adopt_limber_gg(getattr(prob, "limber_gg", 1))
+        if getattr(prob, "sixpt", 0):
+            self.init_probes("6x2pt")
+            self.init_cmb_cross_correlation(prob.cmb_lmin, prob.cmb_lmax, prob.cmb_fwhm, prob.cmb_win)
+            self.init_binning_fourier(prob.kk_nells, int(prob.kk_lmin), int(prob.kk_lmax), int(prob.kk_lmax))
+        else:
+            self.init_probes("3x2pt")
         self.set_source_sample(prob.source)
         if prob.lens is not None:
             self.set_lens_sample(prob.lens)

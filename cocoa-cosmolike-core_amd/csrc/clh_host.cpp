@@ -29,6 +29,7 @@ hipError_t clh_launch_nl_forward(const DrawDesc* D, const int2* wprep, int nprep
                                  const int4* wgl, int ngl, const int4* wchk, int nchk, const int2* wbin, int nbin,
                                  const int4* widft, int nidft, hipStream_t s);
 hipError_t clh_launch_nl_cl(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_cmb(const DrawDesc* D, const int4* work, int n, const int* wplace, int nplace, hipStream_t s);
 hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles, int keep,
                                 hipStream_t s, int ol = 0);
 hipError_t clh_launch_legendre(const DrawDesc* D, const int4* work, int n, hipStream_t s);
@@ -165,10 +166,20 @@ struct Layout {
   std::vector<int> mask;    // likelihood mask in the (0,1,2) block order; empty = all ones
   bool nonlimber = false;   // !like.adopt_limber_gg: w from the non-Limber C_ell below ell 150
   dvec kwlow;               // [ntheta][CLH_NL_L] w kernel at integer ell (ell 0 = 0)
+  // 6x2pt (init_probes("6x2pt")): CMB filter (init_cmb_cross_correlation) and
+  // the folded w_gk / w_ks kernels with it
+  bool sixpt = false;
+  double cmb_fwhm = 0.0;    // rad
+  int cmb_lmin = 0, cmb_lmax = 0;
+  dvec cmb_win;             // HEALPix pixel window per ell (empty: none)
+  dvec kf6;                 // [2][nrow][ntheta]
   int nrow() const { return CLH_NLOW + n_ell; }
   int n_ss() const { return src.nbin * (src.nbin + 1) / 2; }
   int n_gs() const { return src.nbin * lens.nbin; }
-  int dv_size() const { return ntheta * (2 * n_ss() + n_gs() + lens.nbin); }
+  int dv_size3() const { return ntheta * (2 * n_ss() + n_gs() + lens.nbin); }
+  int dv_size() const {
+    return dv_size3() + (sixpt ? ntheta * (lens.nbin + src.nbin) + (int)ell_f.size() : 0);
+  }
 };
 
 struct Draw {  // one draw's cosmology + nuisance (host copy)
@@ -225,7 +236,7 @@ struct DevLayout {  // device copy of one survey layout
   DBuf<int> mask;       // likelihood mask (null when the layout has none)
   size_t off_s_zc = 0, off_s_coef = 0, off_l_zc = 0, off_l_coef = 0, off_ell_ss = 0,
          off_ell_gx = 0, off_kfold = 0, off_kfoldT = 0, off_gl = 0, off_kfull = 0, off_fill = 0, off_theta = 0,
-         off_kwlow = 0, off_nltw = 0;
+         off_kwlow = 0, off_nltw = 0, off_kf6 = 0, off_ellf = 0;
   void release() { buf.release(); k32.release(); ksc.release(); kbf.release(); mask.release(); valid = false; }
 };
 
@@ -275,6 +286,10 @@ struct Batch {
   int nknz = 0;        // common ln P table size (0 if the draws differ)
   bool tatt = false;   // some draw uses TATT intrinsic alignments
   bool oneloop = false;  // some draw has one-loop galaxy bias
+  bool cmb_any = false;  // some draw computes the 6x2pt CMB lensing blocks
+  DBuf<int4> d_wcmb;
+  DBuf<int> d_wcmbp;
+  int n_wcmb = 0, n_wcmbp = 0;
   // non-Limber w: per such draw two views of it (gg Limber rows at ell =
   // 1..149 with P_nl and with P_lin) and the FFTLog work lists
   bool nl_any = false;
@@ -293,6 +308,7 @@ struct Batch {
     d_wsx.release(); d_wgg.release(); d_wga.release(); d_wmx.release(); d_wlf.release();
     d_desc_nl.release(); d_nl_ells.release(); d_wn_nl.release(); d_wnl_prep.release(); d_wnl_cl.release();
     d_wgg_nl.release(); d_wnl_fwd.release(); d_wnl_gl.release(); d_wnl_chk.release(); d_wnl_idft.release();
+    d_wcmb.release(); d_wcmbp.release();
   }
 };
 
@@ -308,6 +324,7 @@ struct clh_ctx {
   int legendre_mode = CLH_LEGENDRE_FOLDED;
   int precision = CLH_PREC_FP64;
   int tile_rows[4] = {16, 64, 64, 64};  // rows per tile: fwd ss/gs, fwd gg, adj ss/gs, adj gg
+  const int* ord6 = nullptr;            // 6-block order of a clh_compute_6x2pt_real call in progress
   int gg_lds[2] = {0, 1};               // stage tables in LDS: fwd gg, adj gg
   int vjp_ckpt = 0;                     // batch forwards keep the gg checkpoint for a VJP
   uint64_t version = 1, cached_version = 0;
@@ -486,6 +503,10 @@ static void build_kernels_uncached(Layout& L, bool full);
 
 static void build_kernels(Layout& L, bool full) {
   const KernelKey key{L.ntheta, L.n_ell, L.lmax, L.vtmin, L.vtmax};
+  if (L.sixpt) {  // the CMB filter enters the folded 6x2pt kernels: not shared
+    build_kernels_uncached(L, full);
+    return;
+  }
   if (!full) {
     std::lock_guard<std::mutex> lk(g_kernel_mu);
     auto it = g_kernel_cache.find(key);
@@ -514,6 +535,14 @@ static void build_kernels_uncached(Layout& L, bool full) {
   const int nt = L.ntheta, Lm = L.lmax, nrow = L.nrow();
   L.kfold.assign((size_t)4 * nrow * nt, 0.0);
   L.kwlow.assign((size_t)nt * CLH_NL_L, 0.0);
+  if (L.sixpt) L.kf6.assign((size_t)2 * nrow * nt, 0.0); else L.kf6.clear();
+  // beam_cmb x w_pixel (cosmo2D.c:268-279, 1003-1010)
+  const double cmb_s = L.cmb_fwhm / std::sqrt(16.0 * std::log(2.0));
+  auto cmbf = [&](int l) {
+    double f = (l < L.cmb_lmin || l > L.cmb_lmax) ? 0.0 : std::exp(-l * (l + 1.0) * cmb_s * cmb_s);
+    if (!L.cmb_win.empty()) f *= (l < (int)L.cmb_win.size()) ? L.cmb_win[l] : 0.0;
+    return f;
+  };
   if (full) L.kfull.assign((size_t)4 * Lm * nt, 0.0); else L.kfull.clear();
   double lim_ss[3], lim_gx[3];
   lim_ss[0] = std::log(std::fmax(CLH_LMIN_TAB - 1., 1.0));
@@ -570,6 +599,23 @@ static void build_kernels_uncached(Layout& L, bool full) {
                        2.0L / (2 * lf + 1) * (Pmin[l + 1] - Pmax[l + 1])));
       g[3] = (double)((1.0L / dx) * (1.0L / (4 * pi)) * (Pmin[l + 1] - Pmax[l + 1] - Pmin[l - 1] + Pmax[l - 1]));
       if (l < CLH_NL_L) L.kwlow[(size_t)i * CLH_NL_L + l] = g[3];
+      if (L.sixpt) {  // w_gk: w kernel, w_ks: gamma_t kernel, both times the CMB filter
+        const double f = cmbf(l);
+        for (int k6 = 0; k6 < 2; k6++) {
+          const double gv = (k6 == 0 ? g[3] : g[2]) * f;
+          double* kf = L.kf6.data() + (size_t)k6 * nrow * nt;
+          if (l < CLH_LMIN_TAB) {
+            kf[(size_t)(l - 1) * nt + i] += gv;
+          } else {
+            const double r = (std::log((double)l) - lim_gx[0]) * (1.0 / lim_gx[2]);
+            const int ii = (int)std::floor(r);
+            const int ic = ii < 0 ? 0 : (ii >= L.n_ell - 1 ? L.n_ell - 2 : ii);
+            const double t = r - ic;
+            kf[(size_t)(CLH_NLOW + ic) * nt + i] += gv * (1.0 - t);
+            kf[(size_t)(CLH_NLOW + ic + 1) * nt + i] += gv * t;
+          }
+        }
+      }
       for (int k = 0; k < 4; k++) {
         if (full) L.kfull[((size_t)k * Lm + l) * nt + i] = g[k];
         double* kf = L.kfold.data() + (size_t)k * nrow * nt;
@@ -637,6 +683,11 @@ static int check_draw(clh_ctx* c, const Layout& L, const Draw& d, int idx) {
       return fail(c, CLH_ERR_STATE, "draw %d: one-loop bias needs the FAST-PT bias tables (set_bias_PS)", idx);
     if (L.ia_model == CLH_IA_TATT)
       return fail(c, CLH_ERR_ARG, "draw %d: one-loop bias together with TATT is not implemented", idx);
+  }
+  if (L.sixpt) {
+    if (L.lens.nbin == 0) return fail(c, CLH_ERR_STATE, "6x2pt needs the lens sample");
+    if (!(d.chi_z[nc - 2] > 1090.0))
+      return fail(c, CLH_ERR_DOMAIN, "draw %d: 6x2pt needs chi(z) two nodes beyond z = 1090 (g_cmb)", idx);
   }
   if (L.nonlimber && L.lens.nbin > 0) {
     if (d.lkL.size() < 2 || d.pzL.size() < 2 || d.lnPL.size() != d.lkL.size() * d.pzL.size())
@@ -768,6 +819,8 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   }
   D.off_kwlow = take(L.kwlow.size());
   D.off_nltw = take(tw.size());
+  D.off_kf6 = take(L.kf6.size());
+  D.off_ellf = take(L.ell_f.size());
   HIPCHK(c, D.buf.ensure(off));
   double* base = D.buf.p;
   auto put = [&](size_t o, const double* src, size_t n) -> hipError_t {
@@ -793,6 +846,8 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   HIPCHK(c, put(D.off_theta, L.theta.data(), L.theta.size()));
   HIPCHK(c, put(D.off_kwlow, L.kwlow.data(), L.kwlow.size()));
   HIPCHK(c, put(D.off_nltw, tw.data(), tw.size()));
+  HIPCHK(c, put(D.off_kf6, L.kf6.data(), L.kf6.size()));
+  HIPCHK(c, put(D.off_ellf, L.ell_f.data(), L.ell_f.size()));
   if (!L.mask.empty()) {
     HIPCHK(c, D.mask.ensure(L.mask.size()));
     HIPCHK(c, hipMemcpyAsync(D.mask.p, L.mask.data(), L.mask.size() * sizeof(int), hipMemcpyHostToDevice,
@@ -903,13 +958,70 @@ static void dv_sizes(const Layout& L, int sizes[3]) {
   sizes[0] = 2 * nt * L.n_ss(); sizes[1] = nt * L.n_gs(); sizes[2] = nt * L.lens.nbin;
 }
 
-static void dv_starts(const Layout& L, const int ord[3], int start[3]) {
-  int sizes[3];
+// compute_data_vector_Mx2pt_N_starts<0,6> (generic_interface.hpp:770-791)
+// for the six blocks of 6x2pt; the same formula as block_starts
+static bool blocks_tile6(const int sizes[6], const int ord[6], int start[6]) {
+  for (int i = 0; i < 6; i++)
+    if (ord[i] < 0 || ord[i] > 5) return false;
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < i; j++)
+      if (ord[i] == ord[j]) return false;
+  int idx[6] = {0, 1, 2, 3, 4, 5};
+  for (int i = 1; i < 6; i++)
+    for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) std::swap(idx[j], idx[j - 1]);
+  int n = 0;
+  for (int i = 0; i < 6; i++) {
+    n += sizes[i];
+    start[i] = 0;
+    for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+  }
+  for (int i = 0; i < 6; i++) {
+    if (start[i] + sizes[i] > n) return false;
+    for (int j = 0; j < i; j++)
+      if (sizes[i] > 0 && sizes[j] > 0 && start[i] < start[j] + sizes[j] && start[j] < start[i] + sizes[i])
+        return false;
+  }
+  return true;
+}
+static void dv_sizes6(const Layout& L, int sizes[6]) {
   dv_sizes(L, sizes);
-  block_starts(sizes, ord, start);
+  sizes[3] = L.ntheta * L.lens.nbin;
+  sizes[4] = L.ntheta * L.src.nbin;
+  sizes[5] = (int)L.ell_f.size();
+}
+// a 3-entry ord on a 6x2pt layout keeps gk, ks, kk after the 3x2pt blocks
+static void ord6_of(const int ord[3], int o6[6]) {
+  for (int i = 0; i < 3; i++) o6[i] = ord[i];
+  o6[3] = 3; o6[4] = 4; o6[5] = 5;
+}
+
+// block starts of the 3x2pt blocks (start) and, for 6x2pt, of gk, ks, kk
+// (start6); ord has 3 entries, or 6 when ord6 is set
+static void dv_starts(const Layout& L, const int ord[3], int start[3], int start6[3] = nullptr,
+                      const int* ord6 = nullptr) {
+  if (!L.sixpt) {
+    int sizes[3];
+    dv_sizes(L, sizes);
+    block_starts(sizes, ord, start);
+    return;
+  }
+  int sizes[6], o6[6], st[6];
+  dv_sizes6(L, sizes);
+  if (ord6) for (int i = 0; i < 6; i++) o6[i] = ord6[i];
+  else ord6_of(ord, o6);
+  (void)blocks_tile6(sizes, o6, st);
+  for (int i = 0; i < 3; i++) start[i] = st[i];
+  if (start6)
+    for (int i = 0; i < 3; i++) start6[i] = st[3 + i];
 }
 
 static bool ord_ok(const Layout& L, const int ord[3]) {
+  if (L.sixpt) {
+    int sizes[6], o6[6], st[6];
+    dv_sizes6(L, sizes);
+    ord6_of(ord, o6);
+    return blocks_tile6(sizes, o6, st);
+  }
   int sizes[3];
   dv_sizes(L, sizes);
   return blocks_tile(sizes, ord);
@@ -1034,6 +1146,25 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   HIPCHK(c, up4(bt.d_wnl_chk, wnl_chk, bt.n_wnl_chk));
   HIPCHK(c, up4(bt.d_wnl_idft, wnl_idft, bt.n_wnl_idft));
   HIPCHK(c, up2(bt.d_wnl_cl, wnl_cl, bt.n_wnl_cl));
+  // 6x2pt: gk per lens bin, ks per source bin, kk; tiles of 32 rows
+  std::vector<int4> wcmb;
+  std::vector<int> wcmbp;
+  for (int i = 0; i < bt.B; i++) {
+    const DrawDesc& q = bt.desc[i];
+    if (!q.cl6) continue;
+    wcmbp.push_back(i);
+    for (int r0 = 0; r0 < q.nrow6; r0 += 32) {
+      const int nr = std::min(32, q.nrow6 - r0);
+      for (int b = 0; b < q.nl; b++) wcmb.push_back(make_int4(i, 0 + 4 * b, r0, nr));
+      for (int b = 0; b < q.ns; b++) wcmb.push_back(make_int4(i, 1 + 4 * b, r0, nr));
+      wcmb.push_back(make_int4(i, 2, r0, nr));
+    }
+  }
+  HIPCHK(c, up4(bt.d_wcmb, wcmb, bt.n_wcmb));
+  bt.n_wcmbp = (int)wcmbp.size();
+  HIPCHK(c, bt.d_wcmbp.ensure(std::max<size_t>(wcmbp.size(), 1)));
+  if (!wcmbp.empty())
+    HIPCHK(c, hipMemcpyAsync(bt.d_wcmbp.p, wcmbp.data(), wcmbp.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
   return CLH_OK;
 
 }
@@ -1072,7 +1203,8 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
                   (tatt ? al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS) + al((size_t)L.n_ss() * nrow) : 0) +
                   (nlw ? clh_nl_off(nl).total + 2 * (al(node_sz) + al((size_t)(L.n_ss() + L.n_gs() + nl) *
                                                                        (CLH_NL_L - 1)))
-                       : 0);
+                       : 0) +
+                  (L.sixpt && !ells ? al((size_t)(nl + ns + 1) * (nrow + L.ell_f.size())) : 0);
     toff[i + 1] = toff[i] + 2 * d.chi_z.size() + 2 * d.G_z.size() + d.lk.size() + d.pz.size() + d.lnP.size() +
                   (tatt ? d.fpt.size() : 0) + (nlw ? d.lkL.size() + d.pzL.size() + d.lnPL.size() : 0) +
                   (d.has_b2(nl) ? d.fptb.size() : 0) + 32;
@@ -1184,6 +1316,17 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
       q.lim_ss[1] = q.lim_gx[1];
       q.lim_ss[2] = (q.lim_ss[1] - q.lim_ss[0]) / ((double)L.n_ell - 1.0);
     }
+    if (L.sixpt && !ells) {
+      q.nrow6 = nrow + (int)L.ell_f.size();
+      q.kf6 = lay + DL.off_kf6;
+      q.ell_f = lay + DL.off_ellf;
+      q.n_f = (int)L.ell_f.size();
+      double cc, dd;
+      host_chi_all(d, 1.0 / 1091.0, cc, dd);
+      q.chi_cmb = cc;
+      q.cl6 = wend;
+      wend += al((size_t)(nl + ns + 1) * q.nrow6);
+    }
     if (nlw) {
       q.nlw = wend;
       wend += clh_nl_off(nl).total;
@@ -1192,7 +1335,7 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
       nl_views.push_back({i, wend});
     }
     q.dv = dv_out ? dv_out + dvoff[i] : bt.d_dv.p + dvoff[i];
-    dv_starts(L, ord, q.dv_start);
+    dv_starts(L, ord, q.dv_start, q.dv6_start, c->ord6);
     q.node_ss = 0;
     q.node_gs = CLH_NP_SS * (3 + ns + CLH_LUT_SX);
     q.node_gs_stride = CLH_NP_SS * (5 + ns + CLH_LUT_SX);
@@ -1269,7 +1412,9 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   bt.state_valid = false;
   bt.tatt = false;
   bt.oneloop = false;
+  bt.cmb_any = false;
   for (int i = 0; i < B; i++) {
+    bt.cmb_any = bt.cmb_any || bt.desc[i].cl6 != nullptr;
     bt.tatt = bt.tatt || lays[i]->ia_model == CLH_IA_TATT;
     bt.oneloop = bt.oneloop || bt.desc[i].oneloop;
   }
@@ -1323,6 +1468,8 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
     HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, mixed, bt.keep_ckpt ? 1 : 0, s,
                                    bt.oneloop ? 1 : 0));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
+  if (legendre && bt.cmb_any)  // 6x2pt CMB lensing blocks: FP64 in every precision mode
+    HIPCHK(c, clh_launch_cmb(bt.d_desc.p, bt.d_wcmb.p, bt.n_wcmb, bt.d_wcmbp.p, bt.n_wcmbp, s));
   if (legendre && bt.nl_any) {  // non-Limber w: FP64 in every precision mode
     HIPCHK(c, clh_launch_nodes(bt.d_desc_nl.p, bt.d_wn_nl.p, bt.n_wn_nl, s));
     HIPCHK(c, clh_launch_limber_gg(bt.d_desc_nl.p, bt.d_wgg_nl.p, bt.n_wgg_nl, 0, c->gg_lds[0] ? bt.gg_stage : 0, 0, s,
@@ -1587,6 +1734,39 @@ int clh_set_bias_PS(clh_ctx* c, const double* PS, double kmin, double kmax, doub
   return CLH_OK;
 }
 
+// init_probes (generic_interface.cpp:640-701) for "3x2pt" and "6x2pt"
+int clh_init_probes(clh_ctx* c, const char* probes) {
+  if (!c || !probes) return CLH_ERR_ARG;
+  const std::string p(probes);
+  if (p != "3x2pt" && p != "6x2pt") return fail(c, CLH_ERR_ARG, "init_probes: '%s' not supported (3x2pt, 6x2pt)", probes);
+  const bool six = p == "6x2pt";
+  if (six != c->L.sixpt) {
+    c->L.sixpt = six;
+    c->L.kern_valid = false;
+    c->dl.valid = false;
+    c->L.mask.clear();
+    c->version++;
+  }
+  return CLH_OK;
+}
+
+// init_cmb_cross_correlation (generic_interface.cpp:508-526): filter ell
+// range, beam FWHM in arcmin and the HEALPix pixel window (column 1 of the
+// reference's file, one value per ell; NULL / 0 = none)
+int clh_init_cmb_cross_correlation(clh_ctx* c, int lmin, int lmax, double fwhm_arcmin, const double* win, int nwin) {
+  if (!c) return CLH_ERR_ARG;
+  if (std::isnan(fwhm_arcmin) || fwhm_arcmin < 0) return fail(c, CLH_ERR_ARG, "init_cmb_cross_correlation: fwhm");
+  if (win && nwin > 0 && has_nan(win, nwin)) return fail(c, CLH_ERR_NAN, "init_cmb_cross_correlation: NaN window");
+  c->L.cmb_fwhm = fwhm_arcmin * 2.90888208665721580e-4;
+  c->L.cmb_lmin = lmin;
+  c->L.cmb_lmax = lmax;
+  if (win && nwin > 0) c->L.cmb_win.assign(win, win + nwin); else c->L.cmb_win.clear();
+  c->L.kern_valid = false;
+  c->dl.valid = false;
+  c->version++;
+  return CLH_OK;
+}
+
 // like.adopt_limber_gg (generic_interface.cpp:269): 1 Limber w(theta) at every
 // ell, 0 the non-Limber C_ell below ell 150 (C_cl_tomo)
 int clh_set_adopt_limber_gg(clh_ctx* c, int limber) {
@@ -1715,9 +1895,15 @@ int clh_set_nuisance_pointmass(clh_ctx* c, const double* pm, int n) {
 // data-vector index in the block order `ord` of index k in the (0,1,2) order
 static int dv_index_from_canonical(const Layout& L, const int ord[3], int k) {
   const int nt = L.ntheta;
+  const int c3 = L.dv_size3();
+  int start[3], start6[3];
+  dv_starts(L, ord, start, start6);
+  if (L.sixpt && k >= c3) {  // gk, ks, kk keep their canonical order after the 3x2pt blocks
+    const int cs[3] = {c3, c3 + nt * L.lens.nbin, c3 + nt * (L.lens.nbin + L.src.nbin)};
+    const int b = k >= cs[2] ? 2 : (k >= cs[1] ? 1 : 0);
+    return start6[b] + (k - cs[b]);
+  }
   const int canon_start[3] = {0, 2 * nt * L.n_ss(), nt * (2 * L.n_ss() + L.n_gs())};
-  int start[3];
-  dv_starts(L, ord, start);
   const int b = k >= canon_start[2] ? 2 : (k >= canon_start[1] ? 1 : 0);
   return start[b] + (k - canon_start[b]);
 }
@@ -1866,6 +2052,8 @@ int clh_init_binning_fourier(clh_ctx* c, int nells, int lmin, int lmax, int lmax
   const double logdl = (std::log((double)lmax) - std::log((double)lmin)) / (double)nells;
   for (int i = 0; i < nells; i++) c->L.ell_f[i] = std::exp(std::log((double)lmin) + (i + 0.5) * logdl);
   c->L.lmax_shear = lmax_shear;
+  c->dl.valid = false;  // like.ell also feeds the 6x2pt kk block
+  if (c->L.sixpt) c->L.mask.clear();
   c->version++;
   return CLH_OK;
 }
@@ -1935,6 +2123,29 @@ int clh_compute_3x2pt_real(clh_ctx* c, const int ord[3], double* dv, int n) {
   HIPCHK(c, hipMemcpyAsync(dv, c->single.desc[0].dv, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->cached_version = 0;  // the single batch now holds a calibrated vector
+  return CLH_OK;
+}
+
+// compute_Mx2pt_N_masked<0,6>(ord) (generic_interface.hpp:1137-1159): the
+// 3x2pt blocks, w_gk, w_ks (x (1 + m)) and C_kk at like.ell, placed by ord
+int clh_compute_6x2pt_real(clh_ctx* c, const int ord[6], double* dv, int n) {
+  if (!c || !ord || !dv) return CLH_ERR_ARG;
+  if (!c->L.sixpt) return fail(c, CLH_ERR_STATE, "6x2pt not selected (clh_init_probes(ctx, \"6x2pt\"))");
+  if (n != c->L.dv_size()) return fail(c, CLH_ERR_ARG, "dv size %d != %d", n, c->L.dv_size());
+  int sizes[6], st[6];
+  dv_sizes6(c->L, sizes);
+  if (!blocks_tile6(sizes, ord, st))
+    return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0..5 whose blocks tile the data vector");
+  const int ord3[3] = {0, 1, 2};
+  std::vector<const Draw*> v{&c->cur};
+  c->ord6 = ord;
+  int rc = build_batch_ctx(c, c->single, v, 1, ord3, nullptr, nullptr);
+  c->ord6 = nullptr;
+  if (rc) return rc;
+  if ((rc = run_batch(c, c->single, true))) return rc;
+  HIPCHK(c, hipMemcpyAsync(dv, c->single.desc[0].dv, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->cached_version = 0;
   return CLH_OK;
 }
 
@@ -2243,7 +2454,8 @@ long long clh_batch_datavector_offset(clh_ctx* c, int i) {
 // the context asks for it and the batch has one layout.  `changed`: some
 // descriptor changed on the host and must be uploaded.
 static int prepare_checkpoint(clh_ctx* c, Batch& cb, bool& changed) {
-  cb.keep_ckpt = c->vjp_ckpt && cb.uniform && !cb.nrow_custom && !cb.tatt && !cb.nl_any && !cb.oneloop;
+  cb.keep_ckpt = c->vjp_ckpt && cb.uniform && !cb.nrow_custom && !cb.tatt && !cb.nl_any && !cb.oneloop &&
+                 !cb.cmb_any;
   if (cb.keep_ckpt) {
     const size_t nl = cb.lays[0]->lens.nbin, nr = cb.lays[0]->nrow();
     cb.ggk_per = (nl * nr * 3 * CLH_NP_GG + 31) & ~size_t(31);
@@ -2417,7 +2629,8 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   if (rc0) return rc0;
   for (int i = 0; i < bt.B; i++) bt.desc[i].dv = bt.d_dv.p + bt.dv_off[i];
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
-  const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt && !bt.nl_any && !bt.oneloop;  // the VJP refuses these
+  const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt && !bt.nl_any && !bt.oneloop &&
+                   !bt.lays[0]->sixpt;  // the VJP refuses these
   DBuf<double> dvbar, glnp;  // unit cotangent and ln P gradient of the adjoint stages
   if (adj) {
     hipError_t e = dvbar.ensure((size_t)bt.dv_off[bt.B]);
@@ -2516,6 +2729,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (bt.tatt) return fail(c, CLH_ERR_ARG, "VJP of TATT intrinsic alignments is not implemented");
   if (bt.nl_any) return fail(c, CLH_ERR_ARG, "VJP of the non-Limber w(theta) is not implemented");
   if (bt.oneloop) return fail(c, CLH_ERR_ARG, "VJP of the one-loop galaxy bias terms is not implemented");
+  if (bt.lays[0]->sixpt) return fail(c, CLH_ERR_ARG, "VJP of the 6x2pt data vector is not implemented");
   const Layout& L = *bt.lays[0];
   if (!ord_ok(L, ord))
     return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");

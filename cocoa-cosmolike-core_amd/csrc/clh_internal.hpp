@@ -166,6 +166,15 @@ struct DrawDesc {
   const double* fptb;         // [6][fptb_n] d1d2, d2d2, d1s2, d2s2, s2s2, d1p3 (/ (c/H0)^3)
   int fptb_n;
   double fptb_l0, fptb_l1, fptb_dl, fptb_s4;  // ln k range (k in H0/c), step, sigma4 (/ (c/H0)^3)
+  // 6x2pt CMB lensing probes (cl6 null otherwise): C_gk [nl], C_ks [ns], C_kk
+  // columns of nrow6 = nrow + n_f rows (the gs/gg ell rows, then like.ell)
+  double* cl6;
+  int nrow6;
+  const double* kf6;          // [2][nrow][ntheta] folded w (gk) and gamma_t (ks) kernels x CMB filter
+  const double* ell_f;        // [n_f] like.ell (C_kk block)
+  int n_f;
+  double chi_cmb;             // chi(z = 1090) in c/H0 (g_cmb, redshift_spline.c:1111-1125)
+  int dv6_start[3];           // data-vector starts of the gk, ks, kk blocks
   // non-Limber w (like.adopt_limber_gg = 0), nlw null otherwise
   double* nlw;                // work area, ClhNlOff offsets
   const double* nl_tw;        // [3][2 N_j] cos, sin of 2 pi t / N_j (layout)

@@ -155,6 +155,29 @@ int clh_datavector_size_fourier(clh_ctx* ctx);
  * entries in the same order as dv, masked entries are 0 */
 int clh_compute_3x2pt_fourier(clh_ctx* ctx, const int ord[3], const int* mask, double* dv, int n);
 
+/* ------------------------------------------------------------------ 6x2pt */
+/* init_probes (generic_interface.cpp:640-701): "3x2pt" (default) or
+ * "6x2pt".  6x2pt adds w_gk(theta) per lens bin and w_ks(theta) per source
+ * bin in the real-space theta bins (w_gk_tomo, w_ks_tomo, cosmo2D.c:913-1296)
+ * and C_kk at like.ell (the clh_init_binning_fourier bins;
+ * generic_interface.hpp:1085-1094) to the real-space data vector, whose size
+ * clh_datavector_size then reports; the 3-entry ord of the other entry
+ * points keeps gk, ks, kk after the 3x2pt blocks.  Needs chi(z) beyond
+ * z = 1090 (g_cmb) and clh_init_cmb_cross_correlation; clh_batch_vjp
+ * refuses it.  Real space only, as in the reference (whose Fourier gk / ks
+ * are not implemented); the kk band-power binning is not built. */
+int clh_init_probes(clh_ctx* ctx, const char* probes);
+/* init_cmb_cross_correlation (generic_interface.cpp:508-526): the CMB filter
+ * of w_gk / w_ks, exp(-l(l+1) s^2) with s = FWHM / sqrt(16 ln 2) on
+ * [lmin, lmax] (beam_cmb, cosmo2D.c:268-271), times the HEALPix pixel window
+ * win[l] (l < nwin, 0 above; NULL = none) */
+int clh_init_cmb_cross_correlation(clh_ctx* ctx, int lmin, int lmax, double fwhm_arcmin, const double* win,
+                                   int nwin);
+/* compute_Mx2pt_N_masked<0,6>(ord) (generic_interface.hpp:1137-1159): the
+ * six blocks xi+- | gamma_t | w | w_gk | w_ks | kk placed by ord (6 entries),
+ * calibrations and mask as compute_Mx2pt_N_masked applies them */
+int clh_compute_6x2pt_real(clh_ctx* ctx, const int ord[6], double* dv, int n);
+
 /* ------------------------------------------------------------ likelihood */
 /* IP::set_mask (generic_interface.hpp:1234-1331): 0/1 per data-vector entry,
  * in the block order ord (NULL clears). Masked entries of every computed

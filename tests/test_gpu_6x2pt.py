@@ -1,0 +1,138 @@
+"""6x2pt (init_probes("6x2pt")) on the GPU vs the FP64 oracle: the CMB lensing
+probes w_gk(theta), w_ks(theta) (with the CMB beam and pixel filter) and
+C_kk at the Fourier ell bins, the whole six-block real-space data vector in
+two block orders and both Legendre modes, batches with graph replay, FP32
+mode, and the VJP refusal.
+
+Bar: 1e-6 relative per entry with a floor of 1e-3 of the block row's largest
+magnitude (the FP64 parity metric); FP32 mode 1e-4.
+"""
+import numpy as np
+import pytest
+
+from cosmolike_amd import Cosmolike, CosmolikeError, LEGENDRE_FOLDED, LEGENDRE_FULL, synth
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-6
+
+
+def rel_err(gpu, ref):
+    gpu = np.atleast_2d(gpu)
+    ref = np.atleast_2d(ref)
+    floor = 1e-3 * np.abs(ref).max(axis=1, keepdims=True)
+    den = np.maximum(np.abs(ref), floor)
+    den[den == 0] = 1.0
+    return float((np.abs(gpu - ref) / den).max())
+
+
+def blocks(dv, p, nf):
+    nt, nss, ngs, nl, ns = p.ntheta, p.ns * (p.ns + 1) // 2, p.ns * p.nl, p.nl, p.ns
+    out, o = {}, 0
+    for name, rows, width in (("xi+", nss, nt), ("xi-", nss, nt), ("gammat", ngs, nt), ("w", nl, nt),
+                              ("gk", nl, nt), ("ks", ns, nt), ("kk", 1, nf)):
+        out[name] = dv[o:o + rows * width].reshape(rows, width)
+        o += rows * width
+    assert o == len(dv)
+    return out
+
+
+def dv_err(dv, ref, p, nf):
+    g, r = blocks(dv, p, nf), blocks(ref, p, nf)
+    return {k: rel_err(g[k], r[k]) for k in g}
+
+
+def _problem(seed=7):
+    cos = synth.random_draws(1, seed=seed, ragged=False)[0][0]
+    p = synth.sixpt_problem(cos)
+    rng = np.random.default_rng(seed)
+    p.src_dz = rng.normal(0, 0.01, p.ns)
+    p.lens_dz = rng.normal(0, 0.005, p.nl)
+    p.shear_m = rng.normal(0, 0.01, p.ns)
+    p.bmag = np.linspace(-0.5, 1.0, p.nl)
+    return p
+
+
+@pytest.fixture(scope="module")
+def six():
+    p = _problem()
+    o = O.Oracle().load(p)
+    ells = synth.ells_fourier(p.kk_nells, p.kk_lmin, p.kk_lmax)
+    return p, o, ells, o.datavector_6x2pt(ells)
+
+
+@pytest.mark.parametrize("mode", [LEGENDRE_FOLDED, LEGENDRE_FULL])
+def test_6x2pt_datavector(six, mode):
+    p, o, ells, ref = six
+    c = Cosmolike().load(p, mode)
+    assert c.datavector_size() == len(ref)
+    got = c.compute_6x2pt_real()
+    e = dv_err(got, ref, p, len(ells))
+    print(mode, e)
+    assert max(e.values()) < RTOL
+    # the 3-entry entry point gives the same vector (gk, ks, kk after the 3x2pt blocks)
+    assert np.array_equal(c.compute_3x2pt_real(), got)
+
+
+def test_6x2pt_block_order(six):
+    p, o, ells, ref = six
+    ord6 = (3, 4, 5, 0, 1, 2)  # involutive: the reference formula tiles only those
+    c = Cosmolike().load(p)
+    got = c.compute_6x2pt_real(ord6)
+    want = o.datavector_6x2pt(ells, ord6)
+    assert rel_err(got.reshape(1, -1), want.reshape(1, -1)) < RTOL
+    with pytest.raises(CosmolikeError):
+        c.compute_6x2pt_real((1, 2, 0, 3, 4, 5))  # a 3-cycle: blocks would not tile
+
+
+def _forward(c, n, B):
+    import torch
+    out = torch.full((B * n,), float("nan"), dtype=torch.float64, device="cuda:0")
+    c.batch_forward(out.data_ptr())
+    c.synchronize()
+    torch.cuda.synchronize()
+    return out.cpu().numpy().reshape(B, n)
+
+
+def test_6x2pt_batch_and_graph(six):
+    p, o, ells, ref = six
+    draws = [_problem(seed=s) for s in (31, 32, 33)]
+    c = Cosmolike().load(draws[0])
+    c.batch_upload(draws)
+    n = c.datavector_size()
+    got = _forward(c, n, len(draws))
+    for i, q in enumerate(draws):
+        e = dv_err(got[i], O.Oracle().load(q).datavector_6x2pt(ells), q, len(ells))
+        print(i, e)
+        assert max(e.values()) < RTOL, i
+    c.batch_use_graph(True)
+    for _ in range(2):
+        assert np.array_equal(_forward(c, n, len(draws)), got)
+    c.batch_use_graph(False)
+
+
+def test_6x2pt_fp32(six):
+    p, o, ells, ref = six
+    c = Cosmolike().load(p)
+    c.set_precision("fp32")
+    c.batch_upload([p])
+    e = dv_err(_forward(c, c.datavector_size(), 1)[0], ref, p, len(ells))
+    print(e)
+    assert max(e.values()) < 1e-4
+
+
+def test_6x2pt_vjp_refused(six):
+    import torch
+    p, o, ells, ref = six
+    c = Cosmolike().load(p)
+    c.batch_upload([p])
+    n = c.datavector_size()
+    _forward(c, n, 1)
+    bar = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    g = torch.zeros(p.ns, dtype=torch.float64, device="cuda:0")
+    with pytest.raises(CosmolikeError, match="6x2pt"):
+        c.batch_vjp(bar.data_ptr(), g_src_dz=g.data_ptr())
+    c.init_probes("3x2pt")
+    assert c.datavector_size() == len(ref) - (p.nl + p.ns) * p.ntheta - len(ells)
