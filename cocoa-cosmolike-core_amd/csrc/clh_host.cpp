@@ -861,7 +861,7 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   // |K|); the contraction scales that band's C_ell rows by 2^e.
   const int nrow = L.nrow(), nt = L.ntheta, ngrp = (nrow + 7) / 8;
   std::vector<float> k32((size_t)4 * ngrp * 64 * 4, 0.f), ksc((size_t)4 * ngrp, 1.f);
-  std::vector<uint16_t> kbf((size_t)4 * ngrp * 64 * 8, 0);
+  std::vector<uint16_t> kbf((size_t)4 * ngrp * 64 * 12, 0);
   for (int k = 0; k < 4; k++) {
     const double* K = L.kfold.data() + (size_t)k * nrow * nt;
     for (int g0 = 0; g0 < ngrp; g0 += 4) {
@@ -879,9 +879,14 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
             const size_t o = ((size_t)k * ngrp + g) * 64 + l;
             k32[o * 4 + t] = (float)v;
             const double vs = std::ldexp(v, -e);
+            // three BF16 parts: hi, mid, lo (each of what the previous left)
             const uint16_t hi = bf16_rne((float)vs);
-            kbf[o * 8 + t] = hi;
-            kbf[o * 8 + 4 + t] = bf16_rne((float)(vs - bf16_value(hi)));
+            const double r1 = vs - bf16_value(hi);
+            const uint16_t mid = bf16_rne((float)r1);
+            const uint16_t lo = bf16_rne((float)(r1 - bf16_value(mid)));
+            kbf[o * 12 + t] = hi;
+            kbf[o * 12 + 4 + t] = mid;
+            kbf[o * 12 + 8 + t] = lo;
           }
       }
     }

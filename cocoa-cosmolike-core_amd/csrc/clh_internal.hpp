@@ -117,7 +117,7 @@ struct DrawDesc {
   // folded kernels packed as MFMA A operands, rows in groups of 8
   // (group g, lane l = theta + 32 h, item t <-> row 8 g + 4 h + t):
   const float* kmx32;        // [4][ngrp][64][4]   FP32
-  const uint16_t* kmxbf;     // [4][ngrp][64][8]   BF16 hi[4], lo[4] of the band-scaled kernel
+  const uint16_t* kmxbf;     // [4][ngrp][64][12]  BF16 hi[4], mid[4], lo[4] of the band-scaled kernel
   const float* kmxsc;        // [4][ngrp] power-of-two factor applied to C_ell of the band
   int kmx_ngrp;
 

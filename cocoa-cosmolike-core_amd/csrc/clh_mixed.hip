@@ -17,8 +17,11 @@
 // j, which is both the B-operand slice of v_mfma_f32_32x32x8_bf16 and, one
 // row per instruction, of four v_mfma_f32_32x32x2_f32.
 //   MODE 0: FP32 operands.
-//   MODE 1: BF16 operands, 3-term split a*b ~ ah*bh + ah*bl + al*bh with
-//           a = ah + al, b = bh + bl (each part rounded to nearest even).
+//   MODE 1: BF16 operands, each split into three parts a = a0 + a1 + a2,
+//           b = b0 + b1 + b2 (each part the nearest-even BF16 of what the
+//           previous parts left), and a*b ~ a2 b0 + a1 b1 + a0 b2 + a1 b0 +
+//           a0 b1 + a0 b0 (six MFMAs, smallest first; the dropped products
+//           are below FP32 rounding).
 // The accumulator tile of lane l, register v holds theta = 8(v/4) + 4h + v%4.
 // ----------------------------------------------------------------------------
 typedef float clh_v16f __attribute__((ext_vector_type(16)));
@@ -73,25 +76,34 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre_mx(const DrawDesc* __res
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, (float)c[2], acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, (float)c[3], acc, 0, 0, 0);
     } else {
-      const uint4 a = *reinterpret_cast<const uint4*>(d.kmxbf + (((size_t)kind * ngrp + g) * 64 + lane) * 8);
+      const uint2* ap = reinterpret_cast<const uint2*>(d.kmxbf + (((size_t)kind * ngrp + g) * 64 + lane) * 12);
+      const uint2 u0 = ap[0], u1 = ap[1], u2 = ap[2];
       const float sc = d.kmxsc[kind * ngrp + g];
-      clh_v4s bh, bl;
+      clh_v4s b0, b1, b2;
 #pragma unroll
       for (int t = 0; t < 4; t++) {
         const float f = (float)c[t] * sc;
-        const unsigned hi = clh_bf16_rne(f);
-        const float rem = f - __uint_as_float(hi << 16);
-        bh[t] = (short)hi;
-        bl[t] = (short)clh_bf16_rne(rem);
+        const unsigned h0 = clh_bf16_rne(f);
+        const float r1 = f - __uint_as_float(h0 << 16);
+        const unsigned h1 = clh_bf16_rne(r1);
+        const float r2 = r1 - __uint_as_float(h1 << 16);
+        b0[t] = (short)h0;
+        b1[t] = (short)h1;
+        b2[t] = (short)clh_bf16_rne(r2);
       }
-      clh_v4s ah, al;
-      ah[0] = (short)(a.x & 0xffffu); ah[1] = (short)(a.x >> 16);
-      ah[2] = (short)(a.y & 0xffffu); ah[3] = (short)(a.y >> 16);
-      al[0] = (short)(a.z & 0xffffu); al[1] = (short)(a.z >> 16);
-      al[2] = (short)(a.w & 0xffffu); al[3] = (short)(a.w >> 16);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bh, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bl, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bh, acc, 0, 0, 0);
+      clh_v4s a0, a1, a2;
+      a0[0] = (short)(u0.x & 0xffffu); a0[1] = (short)(u0.x >> 16);
+      a0[2] = (short)(u0.y & 0xffffu); a0[3] = (short)(u0.y >> 16);
+      a1[0] = (short)(u1.x & 0xffffu); a1[1] = (short)(u1.x >> 16);
+      a1[2] = (short)(u1.y & 0xffffu); a1[3] = (short)(u1.y >> 16);
+      a2[0] = (short)(u2.x & 0xffffu); a2[1] = (short)(u2.x >> 16);
+      a2[2] = (short)(u2.y & 0xffffu); a2[3] = (short)(u2.y >> 16);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, b0, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, b1, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a0, b2, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, b0, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a0, b1, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a0, b0, acc, 0, 0, 0);
     }
   }
   if (!colok) return;
