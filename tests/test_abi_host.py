@@ -113,6 +113,42 @@ def test_tatt_setters_validate(ctx):
     ctx.set_nuisance_IA(np.ones(p.ns), np.ones(p.ns), np.ones(p.ns))
 
 
+def test_new_setters_validate(ctx):
+    p = synth.full_3x2pt_problem()
+    ctx.load(p)
+    with pytest.raises(CosmolikeError) as e:
+        ctx.init_probes("5x2pt")                      # only 3x2pt and 6x2pt
+    assert e.value.code == -1
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_adopt_limber_gg(2)
+    assert e.value.code == -1
+    bad = p.lnP.copy()
+    bad[5] = np.nan
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_linear_power_spectrum(p.log10k, p.pk_z, bad)
+    assert e.value.code == -2
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_nuisance_nonlinear_bias(p.b1, np.zeros(3))  # 5 lens bins
+    assert e.value.code == -1
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_nuisance_nonlocal_bias(np.zeros(5), np.full(5, np.nan))
+    assert e.value.code == -2
+    tab, kmin, kmax, s4 = synth.fpt_bias_tables(synth.Cosmology(), n=64)
+    with pytest.raises(CosmolikeError) as e:
+        ctx.set_bias_PS(tab, kmax, kmin, s4)            # kmax < kmin
+    assert e.value.code == -1
+    with pytest.raises(CosmolikeError) as e:
+        ctx.init_cmb_cross_correlation(8, 4000, float("nan"))
+    assert e.value.code == -1
+    # 6x2pt adds (nl + ns) ntheta + N_ell(Fourier) entries to the real-space vector
+    n3 = ctx.datavector_size()
+    ctx.init_binning_fourier(20, 30, 3000, 3000)
+    ctx.init_probes("6x2pt")
+    assert ctx.datavector_size() == n3 + (p.nl + p.ns) * p.ntheta + 20
+    ctx.init_probes("3x2pt")
+    assert ctx.datavector_size() == n3
+
+
 def test_datavector_sizes(ctx):
     p = synth.full_3x2pt_problem()
     ctx.load(p)
