@@ -88,6 +88,9 @@ def lib():
             "clh_init_probes": (i32, [VP, ctypes.c_char_p]),
             "clh_init_cmb_cross_correlation": (i32, [VP, i32, i32, f64, D, i32]),
             "clh_compute_6x2pt_real": (i32, [VP, ctypes.POINTER(ctypes.c_int), D, i32]),
+            "clh_init_cmb_auto_bandpower": (i32, [VP, i32, i32, i32, D, D, f64]),
+            "clh_cmb_kk_bandpower_info": (i32, [VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                                ctypes.POINTER(ctypes.c_int), D]),
             "clh_set_nuisance_nonlinear_bias": (i32, [VP, D, D, i32]),
             "clh_set_nuisance_nonlocal_bias": (i32, [VP, D, D, i32]),
             "clh_set_bias_PS": (i32, [VP, D, f64, f64, f64, f64, i32]),

@@ -203,6 +203,26 @@ class Cosmolike:
         self._chk(self.L.clh_init_cmb_cross_correlation(self.h, int(lmin), int(lmax), float(fwhm_arcmin), _p(w),
                                                         0 if w is None else len(w)))
 
+    def init_cmb_auto_bandpower(self, nbins, lmin, lmax, binning_matrix, theory_offset=None, alpha=1.0):
+        """init_cmb_auto_bandpower: the kk block as nbins band powers; the
+        binning matrix [nbins][lmax - lmin + 1] and offset [nbins] as arrays."""
+        M = _arr(binning_matrix)
+        if M.size != int(nbins) * (int(lmax) - int(lmin) + 1):
+            raise ValueError("binning matrix must hold nbins x (lmax - lmin + 1) entries")
+        off = None if theory_offset is None else _arr(theory_offset)
+        if off is not None and off.size != int(nbins):
+            raise ValueError("theory offset must hold nbins entries")
+        self._chk(self.L.clh_init_cmb_auto_bandpower(self.h, int(nbins), int(lmin), int(lmax), _p(M), _p(off),
+                                                     float(alpha)))
+
+    def kk_bandpower_info(self):
+        """(nbins, lmin, lmax, alpha) of the kk band powers (nbins 0: none)."""
+        n, lo, hi = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        a = np.zeros(1)
+        self._chk(self.L.clh_cmb_kk_bandpower_info(self.h, ctypes.byref(n), ctypes.byref(lo), ctypes.byref(hi),
+                                                   _p(a)))
+        return n.value, lo.value, hi.value, float(a[0])
+
     def compute_6x2pt_real(self, ord=(0, 1, 2, 3, 4, 5)):
         """compute_Mx2pt_N_masked<0,6>(ord)."""
         n = self.datavector_size()
@@ -246,6 +266,9 @@ class Cosmolike:
             self.init_probes("6x2pt")
             self.init_cmb_cross_correlation(prob.cmb_lmin, prob.cmb_lmax, prob.cmb_fwhm, prob.cmb_win)
             self.init_binning_fourier(prob.kk_nells, int(prob.kk_lmin), int(prob.kk_lmax), int(prob.kk_lmax))
+            if getattr(prob, "kk_bp", None) is not None:
+                nbp, lo, hi, M, off = prob.kk_bp
+                self.init_cmb_auto_bandpower(nbp, lo, hi, M, off)
         else:
             self.init_probes("3x2pt")
         self.set_source_sample(prob.source)

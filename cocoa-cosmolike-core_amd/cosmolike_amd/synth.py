@@ -185,6 +185,21 @@ def sixpt_problem(cosmo: Cosmology | None = None, n_ell: int = 500, fwhm_arcmin:
     return p
 
 
+def kk_bandpower_binning(nbp: int = 8, lmin: int = 8, lmax: int = 400):
+    """A synthetic kk band-power binning in init_cmb_auto_bandpower form:
+    nbp top-hat bins in ln L over [lmin, lmax], each normalised to unit sum
+    with an L(L+1) weight ramp, and a small theory offset per bin."""
+    L = np.arange(lmin, lmax + 1)
+    edges = np.unique(np.round(np.geomspace(lmin, lmax + 1, nbp + 1)).astype(int))
+    M = np.zeros((nbp, len(L)))
+    for j in range(nbp):
+        sel = (L >= edges[j]) & (L < edges[j + 1])
+        w = (L * (L + 1.0)) * sel
+        M[j] = w / w.sum()
+    off = 1e-9 * np.linspace(1.0, 2.0, nbp)
+    return nbp, lmin, lmax, np.ascontiguousarray(M.reshape(-1)), off
+
+
 def tatt_problem(cosmo: Cosmology | None = None, n_ell: int = 500, A2: float = -0.8, eta2: float = 0.5,
                  bta: float = 0.7) -> Problem:
     """The full 3x2pt layout with TATT intrinsic alignments (redshift
@@ -364,6 +379,7 @@ class Problem:
     kk_nells: int = 0
     kk_lmin: float = 0.0
     kk_lmax: float = 0.0
+    kk_bp: tuple | None = None   # (nbp, lmin, lmax, M [nbp*(lmax-lmin+1)], offset [nbp]) band powers
     # non-Limber w (like.adopt_limber_gg = 0) and its linear P(k,z)
     limber_gg: int = 1
     plin_log10k: np.ndarray | None = None

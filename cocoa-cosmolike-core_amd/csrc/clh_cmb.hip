@@ -109,19 +109,36 @@ __global__ __launch_bounds__(128) void k_limber_cmb(const DrawDesc* __restrict__
   }
 }
 
+// C_kk at ell from the kk column C: the direct row `direct` (ell <= LMIN_tab)
+// or the table's linear interpolation in ln ell (C_kk_limber,
+// cosmo2D.c:3484-3516)
+__device__ __forceinline__ double clh_ckk(const DrawDesc& d, const double* C, double l, int direct) {
+  if (l <= CLH_LMIN_TAB) return C[d.nrow + direct];
+  const double* f = C + CLH_NLOW;
+  const double x = log(l);
+  const int n = d.n_ell;
+  if (x < d.lim_gx[0]) return f[0];
+  const double r = (x - d.lim_gx[0]) / d.lim_gx[2];
+  const int i0 = (int)floor(r);
+  return (i0 + 1 >= n) ? f[n - 1] : (r - i0) * (f[i0 + 1] - f[i0]) + f[i0];
+}
+
 // ----------------------------------------------------------------------------
 // k_cmb_place: the gk, ks and kk blocks of one draw.  w_gk / w_ks: the folded
 // kernels (w and gamma_t kernels times the CMB filter, ln-ell fill folded in)
-// against the C_gk / C_ks rows; (1 + m_s) on ks; C_kk at like.ell: direct at
-// ell <= LMIN_tab, the table's interpolation above (C_kk_limber,
-// cosmo2D.c:3484-3516).  Masked entries are 0.
+// against the C_gk / C_ks rows; (1 + m_s) on ks; the kk block either C_kk at
+// like.ell or, with band powers (init_cmb_auto_bandpower), the block sums
+// sum_L M[j][L - lmin] C_kk(L) - offset[j] over integer L
+// (generic_interface.hpp:1095-1118).  Masked entries are 0.
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_cmb_place(const DrawDesc* __restrict__ D, const int* __restrict__ work) {
   const DrawDesc& d = D[work[blockIdx.x]];
   const int nt = d.ntheta, nrow = d.nrow;
   const int base = nt * (2 * d.n_ss + d.n_gs + d.nl);  // canonical start of the gk block
   const int nw = (d.nl + d.ns) * nt;
-  for (int t = threadIdx.x; t < nw + d.n_f; t += 256) {
+  const double* Ckk = d.cl6 + (size_t)(d.nl + d.ns) * d.nrow6;
+  const int nf = d.kk_nbp > 0 ? 0 : d.n_f;
+  for (int t = threadIdx.x; t < nw + nf; t += 256) {
     double v;
     int idx, canon;
     if (t < nw) {
@@ -141,26 +158,30 @@ __global__ __launch_bounds__(256) void k_cmb_place(const DrawDesc* __restrict__ 
       canon = base + t;
     } else {
       const int i = t - nw;
-      const double l = d.ell_f[i];
-      const double* C = d.cl6 + (size_t)(d.nl + d.ns) * d.nrow6;
-      if (l <= CLH_LMIN_TAB) {
-        v = C[nrow + i];
-      } else {
-        const double* f = C + CLH_NLOW;
-        const double x = log(l);
-        const int n = d.n_ell;
-        if (x < d.lim_gx[0]) {
-          v = f[0];
-        } else {
-          const double r = (x - d.lim_gx[0]) / d.lim_gx[2];
-          const int i0 = (int)floor(r);
-          v = (i0 + 1 >= n) ? f[n - 1] : (r - i0) * (f[i0 + 1] - f[i0]) + f[i0];
-        }
-      }
+      v = clh_ckk(d, Ckk, d.ell_f[i], i);
       idx = d.dv6_start[2] + i;
       canon = base + nw + i;
     }
     d.dv[idx] = clh_masked(d, canon) ? 0.0 : v;
+  }
+  if (d.kk_nbp > 0) {  // band powers: one block-wide sum over L per band
+    __shared__ double part[4];
+    for (int j = 0; j < d.kk_nbp; j++) {
+      const double* M = d.kk_M + (size_t)j * d.kk_nL;
+      double v = 0.0;
+      for (int i = threadIdx.x; i < d.kk_nL; i += 256) {
+        const int L = d.kk_lmin + i;
+        v += clh_ckk(d, Ckk, (double)L, i) * M[i];  // direct row i holds L = lmin + i
+      }
+      v = clh_wave_sum(v);
+      if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const double bp = (part[0] + part[1]) + (part[2] + part[3]) - d.kk_off[j];
+        d.dv[d.dv6_start[2] + j] = clh_masked(d, base + nw + j) ? 0.0 : bp;
+      }
+      __syncthreads();
+    }
   }
 }
 

@@ -173,12 +173,26 @@ struct Layout {
   int cmb_lmin = 0, cmb_lmax = 0;
   dvec cmb_win;             // HEALPix pixel window per ell (empty: none)
   dvec kf6;                 // [2][nrow][ntheta]
+  // kk band powers (init_cmb_auto_bandpower); kk_nbp = 0: C_kk at like.ell
+  int kk_nbp = 0, kk_lmin = 0, kk_lmax = 0;
+  dvec kk_M;                // [kk_nbp][kk_lmax - kk_lmin + 1] binning matrix
+  dvec kk_off;              // [kk_nbp] theory offset
+  double kk_alpha = 1.0;    // Hartlap factor of the kkkk covariance block
+  int kk_size() const { return kk_nbp > 0 ? kk_nbp : (int)ell_f.size(); }
+  // the ells of the direct (untabulated) C_kk rows: like.ell, or the band
+  // powers' integer L <= LMIN_tab
+  dvec kk_direct_ells() const {
+    if (kk_nbp <= 0) return ell_f;
+    dvec e;
+    for (int l = kk_lmin; l <= std::min(kk_lmax, CLH_LMIN_TAB); l++) e.push_back((double)l);
+    return e;
+  }
   int nrow() const { return CLH_NLOW + n_ell; }
   int n_ss() const { return src.nbin * (src.nbin + 1) / 2; }
   int n_gs() const { return src.nbin * lens.nbin; }
   int dv_size3() const { return ntheta * (2 * n_ss() + n_gs() + lens.nbin); }
   int dv_size() const {
-    return dv_size3() + (sixpt ? ntheta * (lens.nbin + src.nbin) + (int)ell_f.size() : 0);
+    return dv_size3() + (sixpt ? ntheta * (lens.nbin + src.nbin) + kk_size() : 0);
   }
 };
 
@@ -236,7 +250,7 @@ struct DevLayout {  // device copy of one survey layout
   DBuf<int> mask;       // likelihood mask (null when the layout has none)
   size_t off_s_zc = 0, off_s_coef = 0, off_l_zc = 0, off_l_coef = 0, off_ell_ss = 0,
          off_ell_gx = 0, off_kfold = 0, off_kfoldT = 0, off_gl = 0, off_kfull = 0, off_fill = 0, off_theta = 0,
-         off_kwlow = 0, off_nltw = 0, off_kf6 = 0, off_ellf = 0;
+         off_kwlow = 0, off_nltw = 0, off_kf6 = 0, off_ellf = 0, off_kkM = 0, off_kkoff = 0;
   void release() { buf.release(); k32.release(); ksc.release(); kbf.release(); mask.release(); valid = false; }
 };
 
@@ -820,7 +834,10 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   D.off_kwlow = take(L.kwlow.size());
   D.off_nltw = take(tw.size());
   D.off_kf6 = take(L.kf6.size());
-  D.off_ellf = take(L.ell_f.size());
+  const dvec kk_ells = L.kk_direct_ells();
+  D.off_ellf = take(kk_ells.size());
+  D.off_kkM = take(L.kk_M.size());
+  D.off_kkoff = take(L.kk_off.size());
   HIPCHK(c, D.buf.ensure(off));
   double* base = D.buf.p;
   auto put = [&](size_t o, const double* src, size_t n) -> hipError_t {
@@ -847,7 +864,9 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   HIPCHK(c, put(D.off_kwlow, L.kwlow.data(), L.kwlow.size()));
   HIPCHK(c, put(D.off_nltw, tw.data(), tw.size()));
   HIPCHK(c, put(D.off_kf6, L.kf6.data(), L.kf6.size()));
-  HIPCHK(c, put(D.off_ellf, L.ell_f.data(), L.ell_f.size()));
+  HIPCHK(c, put(D.off_ellf, kk_ells.data(), kk_ells.size()));
+  HIPCHK(c, put(D.off_kkM, L.kk_M.data(), L.kk_M.size()));
+  HIPCHK(c, put(D.off_kkoff, L.kk_off.data(), L.kk_off.size()));
   if (!L.mask.empty()) {
     HIPCHK(c, D.mask.ensure(L.mask.size()));
     HIPCHK(c, hipMemcpyAsync(D.mask.p, L.mask.data(), L.mask.size() * sizeof(int), hipMemcpyHostToDevice,
@@ -992,7 +1011,7 @@ static void dv_sizes6(const Layout& L, int sizes[6]) {
   dv_sizes(L, sizes);
   sizes[3] = L.ntheta * L.lens.nbin;
   sizes[4] = L.ntheta * L.src.nbin;
-  sizes[5] = (int)L.ell_f.size();
+  sizes[5] = L.kk_size();
 }
 // a 3-entry ord on a 6x2pt layout keeps gk, ks, kk after the 3x2pt blocks
 static void ord6_of(const int ord[3], int o6[6]) {
@@ -1209,7 +1228,7 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
                   (nlw ? clh_nl_off(nl).total + 2 * (al(node_sz) + al((size_t)(L.n_ss() + L.n_gs() + nl) *
                                                                        (CLH_NL_L - 1)))
                        : 0) +
-                  (L.sixpt && !ells ? al((size_t)(nl + ns + 1) * (nrow + L.ell_f.size())) : 0);
+                  (L.sixpt && !ells ? al((size_t)(nl + ns + 1) * (nrow + L.kk_direct_ells().size())) : 0);
     toff[i + 1] = toff[i] + 2 * d.chi_z.size() + 2 * d.G_z.size() + d.lk.size() + d.pz.size() + d.lnP.size() +
                   (tatt ? d.fpt.size() : 0) + (nlw ? d.lkL.size() + d.pzL.size() + d.lnPL.size() : 0) +
                   (d.has_b2(nl) ? d.fptb.size() : 0) + 32;
@@ -1322,10 +1341,15 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
       q.lim_ss[2] = (q.lim_ss[1] - q.lim_ss[0]) / ((double)L.n_ell - 1.0);
     }
     if (L.sixpt && !ells) {
-      q.nrow6 = nrow + (int)L.ell_f.size();
+      q.n_f = (int)L.kk_direct_ells().size();
+      q.nrow6 = nrow + q.n_f;
       q.kf6 = lay + DL.off_kf6;
       q.ell_f = lay + DL.off_ellf;
-      q.n_f = (int)L.ell_f.size();
+      q.kk_nbp = L.kk_nbp;
+      q.kk_lmin = L.kk_lmin;
+      q.kk_nL = L.kk_lmax - L.kk_lmin + 1;
+      q.kk_M = lay + DL.off_kkM;
+      q.kk_off = lay + DL.off_kkoff;
       double cc, dd;
       host_chi_all(d, 1.0 / 1091.0, cc, dd);
       q.chi_cmb = cc;
@@ -1769,6 +1793,47 @@ int clh_init_cmb_cross_correlation(clh_ctx* c, int lmin, int lmax, double fwhm_a
   c->L.kern_valid = false;
   c->dl.valid = false;
   c->version++;
+  return CLH_OK;
+}
+
+// init_cmb_auto_bandpower (generic_interface.cpp:541-560, IPCMB::
+// set_kk_binning_bandpower / set_kk_binning_mat / set_kk_theory_offset,
+// :2512-2606): the kk block becomes nbins band powers
+// sum_L M[j][L - lmin] C_kk(L) - offset[j] over L = lmin..lmax.  The matrix
+// and the offset are the arrays the reference reads from its two files
+// (offset NULL = zeros); alpha is the Hartlap factor the reference divides the
+// kkkk covariance block by before inverting it: clh_like_set_inv_cov takes the
+// inverse the caller formed, so alpha is kept for clh_cmb_kk_bandpower_info.
+int clh_init_cmb_auto_bandpower(clh_ctx* c, int nbins, int lmin, int lmax, const double* binning_matrix,
+                                const double* theory_offset, double alpha) {
+  if (!c) return CLH_ERR_ARG;
+  if (!(nbins > 0) || !(lmin > 0) || !(lmax >= lmin))
+    return fail(c, CLH_ERR_ARG, "init_cmb_auto_bandpower: bad binning (nbins=%d, lmin=%d, lmax=%d)", nbins, lmin,
+                lmax);
+  if (!binning_matrix) return fail(c, CLH_ERR_ARG, "init_cmb_auto_bandpower: no binning matrix");
+  if (!(alpha > 0)) return fail(c, CLH_ERR_ARG, "init_cmb_auto_bandpower: alpha must be > 0");
+  const size_t nm = (size_t)nbins * (size_t)(lmax - lmin + 1);
+  if (has_nan(binning_matrix, (int)nm) || (theory_offset && has_nan(theory_offset, nbins)))
+    return fail(c, CLH_ERR_NAN, "init_cmb_auto_bandpower: NaN in the binning matrix or offset");
+  Layout& L = c->L;
+  L.kk_nbp = nbins;
+  L.kk_lmin = lmin;
+  L.kk_lmax = lmax;
+  L.kk_M.assign(binning_matrix, binning_matrix + nm);
+  if (theory_offset) L.kk_off.assign(theory_offset, theory_offset + nbins); else L.kk_off.assign(nbins, 0.0);
+  L.kk_alpha = alpha;
+  c->dl.valid = false;
+  if (L.sixpt) L.mask.clear();  // the data-vector size changes
+  c->version++;
+  return CLH_OK;
+}
+
+int clh_cmb_kk_bandpower_info(clh_ctx* c, int* nbins, int* lmin, int* lmax, double* alpha) {
+  if (!c) return CLH_ERR_ARG;
+  if (nbins) *nbins = c->L.kk_nbp;
+  if (lmin) *lmin = c->L.kk_lmin;
+  if (lmax) *lmax = c->L.kk_lmax;
+  if (alpha) *alpha = c->L.kk_alpha;
   return CLH_OK;
 }
 

@@ -167,12 +167,16 @@ struct DrawDesc {
   int fptb_n;
   double fptb_l0, fptb_l1, fptb_dl, fptb_s4;  // ln k range (k in H0/c), step, sigma4 (/ (c/H0)^3)
   // 6x2pt CMB lensing probes (cl6 null otherwise): C_gk [nl], C_ks [ns], C_kk
-  // columns of nrow6 = nrow + n_f rows (the gs/gg ell rows, then like.ell)
+  // columns of nrow6 = nrow + n_f rows (the gs/gg ell rows, then the direct
+  // C_kk ells)
   double* cl6;
   int nrow6;
   const double* kf6;          // [2][nrow][ntheta] folded w (gk) and gamma_t (ks) kernels x CMB filter
-  const double* ell_f;        // [n_f] like.ell (C_kk block)
+  const double* ell_f;        // [n_f] direct C_kk ells: like.ell, or band-power L <= LMIN_tab
   int n_f;
+  int kk_nbp, kk_lmin, kk_nL; // kk band powers (kk_nbp 0: C_kk at like.ell), L = kk_lmin + [0, kk_nL)
+  const double* kk_M;         // [kk_nbp][kk_nL] binning matrix
+  const double* kk_off;       // [kk_nbp] theory offset
   double chi_cmb;             // chi(z = 1090) in c/H0 (g_cmb, redshift_spline.c:1111-1125)
   int dv6_start[3];           // data-vector starts of the gk, ks, kk blocks
   // non-Limber w (like.adopt_limber_gg = 0), nlw null otherwise

@@ -165,7 +165,8 @@ int clh_compute_3x2pt_fourier(clh_ctx* ctx, const int ord[3], const int* mask, d
  * points keeps gk, ks, kk after the 3x2pt blocks.  Needs chi(z) beyond
  * z = 1090 (g_cmb) and clh_init_cmb_cross_correlation; clh_batch_vjp
  * refuses it.  Real space only, as in the reference (whose Fourier gk / ks
- * are not implemented); the kk band-power binning is not built. */
+ * are not implemented).  With clh_init_cmb_auto_bandpower the kk block holds
+ * band powers instead of C_kk at like.ell. */
 int clh_init_probes(clh_ctx* ctx, const char* probes);
 /* init_cmb_cross_correlation (generic_interface.cpp:508-526): the CMB filter
  * of w_gk / w_ks, exp(-l(l+1) s^2) with s = FWHM / sqrt(16 ln 2) on
@@ -173,6 +174,20 @@ int clh_init_probes(clh_ctx* ctx, const char* probes);
  * win[l] (l < nwin, 0 above; NULL = none) */
 int clh_init_cmb_cross_correlation(clh_ctx* ctx, int lmin, int lmax, double fwhm_arcmin, const double* win,
                                    int nwin);
+/* init_cmb_auto_bandpower (generic_interface.cpp:541-560; IPCMB::
+ * set_kk_binning_bandpower, set_kk_binning_mat, set_kk_theory_offset,
+ * generic_interface.cpp:2512-2606): the kk block becomes nbins band powers
+ * sum_L M[j][L - lmin] C_kk(L) - offset[j], L = lmin..lmax, C_kk direct at
+ * L <= LMIN_tab and from the table above (generic_interface.hpp:1095-1118).
+ * binning_matrix is [nbins][lmax - lmin + 1] row-major and theory_offset
+ * [nbins] (NULL = zeros): the tables the reference reads from its two files.
+ * alpha, the Hartlap factor the reference divides the kkkk covariance block
+ * by, is stored only (clh_like_set_inv_cov takes an inverse the caller
+ * formed).  nbins, lmin > 0, lmax >= lmin, alpha > 0. */
+int clh_init_cmb_auto_bandpower(clh_ctx* ctx, int nbins, int lmin, int lmax, const double* binning_matrix,
+                                const double* theory_offset, double alpha);
+/* the band-power binning in use (nbins 0: C_kk at like.ell); any pointer may be NULL */
+int clh_cmb_kk_bandpower_info(clh_ctx* ctx, int* nbins, int* lmin, int* lmax, double* alpha);
 /* compute_Mx2pt_N_masked<0,6>(ord) (generic_interface.hpp:1137-1159): the
  * six blocks xi+- | gamma_t | w | w_gk | w_ks | kk placed by ord (6 entries),
  * calibrations and mask as compute_Mx2pt_N_masked applies them */

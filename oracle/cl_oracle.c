@@ -83,6 +83,10 @@ struct oc_ctx {
   double cmb_fwhm;
   int cmb_lmin, cmb_lmax, cmb_nwin;
   double* cmb_win;
+  /* kk band powers (init_cmb_auto_bandpower): binning matrix
+     [nbp][lmax - lmin + 1] and theory offset [nbp]; nbp = 0: C_kk at like.ell */
+  int kk_nbp, kk_lmin, kk_lmax;
+  double *kk_M, *kk_off;
   /* caches */
   int g_valid;
   double *gt_tab, *gl_tab; /* [ns][N_a], [nl][N_a] */
@@ -134,6 +138,7 @@ void oc_free(oc_ctx* c) {
   free(c->lkL); free(c->pzL); free(c->lnPL);
   free(c->fptb);
   free(c->cmb_win);
+  free(c->kk_M); free(c->kk_off);
   free(c);
 }
 
@@ -2366,8 +2371,43 @@ int oc_cl_kk(oc_ctx* c, const double* ell, int n, double* out) {
   return c->status;
 }
 
+int oc_set_kk_bandpower(oc_ctx* c, int nbp, int lmin, int lmax, const double* M, const double* off) {
+  free(c->kk_M); free(c->kk_off);
+  c->kk_M = c->kk_off = NULL;
+  c->kk_nbp = 0;
+  if (nbp <= 0) return 0;
+  c->kk_nbp = nbp;
+  c->kk_lmin = lmin;
+  c->kk_lmax = lmax;
+  c->kk_M = dup(M, nbp * (lmax - lmin + 1));
+  c->kk_off = malloc(sizeof(double) * nbp);
+  for (int j = 0; j < nbp; j++) c->kk_off[j] = off ? off[j] : 0.0;
+  return 0;
+}
+
+/* the kk block as band powers (generic_interface.hpp:1095-1118):
+   sum_L M[j][L - lmin] C_kk(L) - offset[j], C_kk direct at L <= LMIN_tab */
+static int kk_bandpower(oc_ctx* c, double* out) {
+  double* tab = malloc(sizeof(double) * (size_t)c->n_ell);
+  table_cmb(c, 5, tab);
+  double lim[3];
+  gx_lim(c, lim);
+  gl_rule r;
+  gl_make(&r, gl_order_gg(c->hdi));
+  const int nL = c->kk_lmax - c->kk_lmin + 1;
+  for (int j = 0; j < c->kk_nbp; j++) out[j] = 0.0;
+  for (int L = c->kk_lmin; L <= c->kk_lmax; L++) {
+    const double ckk = (L <= OC_LMIN_TAB) ? C_cmb_nointerp(c, &r, 5, (double)L, 0)
+                                          : interpol1d(tab, c->n_ell, lim[0], lim[1], lim[2], log((double)L));
+    for (int j = 0; j < c->kk_nbp; j++) out[j] += ckk * c->kk_M[(size_t)j * nL + (L - c->kk_lmin)];
+  }
+  for (int j = 0; j < c->kk_nbp; j++) out[j] -= c->kk_off[j];
+  free(tab);
+  return c->status;
+}
+
 int oc_datavector_size_6x2pt(oc_ctx* c, int ncl) {
-  return c->ntheta * (2 * oc_npairs_ss(c) + oc_npairs_gs(c) + 2 * c->nl + c->ns) + ncl;
+  return c->ntheta * (2 * oc_npairs_ss(c) + oc_npairs_gs(c) + 2 * c->nl + c->ns) + (c->kk_nbp > 0 ? c->kk_nbp : ncl);
 }
 
 /* compute_Mx2pt_N_masked<0,6> (generic_interface.hpp:1137-1159) with every
@@ -2378,7 +2418,8 @@ int oc_datavector_size_6x2pt(oc_ctx* c, int ncl) {
 int oc_datavector_6x2pt(oc_ctx* c, const int ord[6], const double* ell, int ncl, double* dv) {
   c->status = 0;
   const int nt = c->ntheta, nss = oc_npairs_ss(c), ngs = oc_npairs_gs(c), nl = c->nl, ns = c->ns;
-  const int sizes[6] = {2 * nt * nss, nt * ngs, nt * nl, nt * nl, nt * ns, ncl};
+  const int nkk = c->kk_nbp > 0 ? c->kk_nbp : ncl;
+  const int sizes[6] = {2 * nt * nss, nt * ngs, nt * nl, nt * nl, nt * ns, nkk};
   for (int i = 0; i < 6; i++)
     if (ord[i] < 0 || ord[i] > 5) { oc_set_err(c, "ord"); return -1; }
   int idx[6] = {0, 1, 2, 3, 4, 5};
@@ -2405,8 +2446,11 @@ int oc_datavector_6x2pt(oc_ctx* c, const int ord[6], const double* ell, int ncl,
   if (oc_datavector_3x2pt(c, ord3, d3)) { free(d3); return -1; }
   double* gk = malloc(sizeof(double) * (size_t)nl * nt);
   double* ks = malloc(sizeof(double) * (size_t)ns * nt);
-  double* kk = malloc(sizeof(double) * (size_t)(ncl > 0 ? ncl : 1));
-  if (oc_w_gk(c, gk) || oc_w_ks(c, ks) || oc_cl_kk(c, ell, ncl, kk)) { free(d3); free(gk); free(ks); free(kk); return -1; }
+  double* kk = malloc(sizeof(double) * (size_t)(nkk > 0 ? nkk : 1));
+  if (oc_w_gk(c, gk) || oc_w_ks(c, ks) || (c->kk_nbp > 0 ? kk_bandpower(c, kk) : oc_cl_kk(c, ell, ncl, kk))) {
+    free(d3); free(gk); free(ks); free(kk);
+    return -1;
+  }
   for (int i = 0; i < n; i++) dv[i] = 0.0;
   for (int k = 0; k < sizes[0]; k++) dv[start[0] + k] = d3[k];
   for (int k = 0; k < sizes[1]; k++) dv[start[1] + k] = d3[sizes[0] + k];
@@ -2414,7 +2458,7 @@ int oc_datavector_6x2pt(oc_ctx* c, const int ord[6], const double* ell, int ncl,
   for (int k = 0; k < sizes[3]; k++) dv[start[3] + k] = gk[k];
   for (int z = 0; z < ns; z++)
     for (int i = 0; i < nt; i++) dv[start[4] + nt * z + i] = ks[z * nt + i] * (1.0 + c->shear_m[z]);
-  for (int k = 0; k < ncl; k++) dv[start[5] + k] = kk[k];
+  for (int k = 0; k < nkk; k++) dv[start[5] + k] = kk[k];
   free(d3); free(gk); free(ks); free(kk);
   return c->status;
 }

@@ -140,6 +140,9 @@ int oc_cl_cmb_nointerp(oc_ctx* c, int probe, const double* l, int n, double* out
 int oc_w_gk(oc_ctx* c, double* out);  /* [nl][ntheta] */
 int oc_w_ks(oc_ctx* c, double* out);  /* [ns][ntheta] */
 int oc_cl_kk(oc_ctx* c, const double* ell, int n, double* out);
+/* init_cmb_auto_bandpower: the kk block as nbp band powers of C_kk(L),
+   L = lmin..lmax (M [nbp][lmax - lmin + 1], offset [nbp] or NULL = 0); nbp = 0 clears */
+int oc_set_kk_bandpower(oc_ctx* c, int nbp, int lmin, int lmax, const double* M, const double* off);
 int oc_datavector_size_6x2pt(oc_ctx* c, int ncl);
 int oc_datavector_6x2pt(oc_ctx* c, const int ord[6], const double* ell, int ncl, double* dv);
 

@@ -91,6 +91,8 @@ def _load():
         lib.oc_cl_kk.argtypes = [vp, _D, ctypes.c_int, _D]
         lib.oc_datavector_size_6x2pt.argtypes = [vp, ctypes.c_int]
         lib.oc_datavector_6x2pt.argtypes = [vp, ctypes.POINTER(ctypes.c_int), _D, ctypes.c_int, _D]
+        lib.oc_set_kk_bandpower.restype = ctypes.c_int
+        lib.oc_set_kk_bandpower.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]
         lib.oc_set_bias_ps.restype = ctypes.c_int
         lib.oc_set_bias_ps.argtypes = [vp, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int]
@@ -224,6 +226,9 @@ class Oracle:
         self.set_limber_gg(getattr(prob, "limber_gg", 1))
         if getattr(prob, "cmb_fwhm", None) is not None:
             self.set_cmb(prob.cmb_fwhm, prob.cmb_lmin, prob.cmb_lmax, prob.cmb_win)
+        if getattr(prob, "kk_bp", None) is not None:
+            nbp, lmin, lmax, M, off = prob.kk_bp
+            self.set_kk_bandpower(nbp, lmin, lmax, M, off)
         if getattr(prob, "b2", None) is not None:
             self.set_bias_nl(prob.b1, prob.b2, prob.b3, prob.bk)
             self.set_bias_ps(prob.bias_ps, prob.bias_ps_kmin, prob.bias_ps_kmax, prob.bias_ps_sigma4)
@@ -234,6 +239,11 @@ class Oracle:
         w = None if win is None else _arr(win)
         self._chk(self.lib.oc_set_cmb(self.c, float(fwhm_arcmin), int(lmin), int(lmax), _p(w),
                                       0 if w is None else len(w)))
+
+    def set_kk_bandpower(self, nbp, lmin, lmax, M, offset=None):
+        M = _arr(M)
+        off = None if offset is None else _arr(offset)
+        self._chk(self.lib.oc_set_kk_bandpower(self.c, int(nbp), int(lmin), int(lmax), _p(M), _p(off)))
 
     def cl_cmb_nointerp(self, probe, ells):
         ells = _arr(ells)

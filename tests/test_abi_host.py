@@ -149,6 +149,33 @@ def test_new_setters_validate(ctx):
     assert ctx.datavector_size() == n3
 
 
+def test_kk_bandpower_setter(ctx):
+    """init_cmb_auto_bandpower: validation as IPCMB::set_kk_binning_bandpower
+    (nbins, lmin, lmax > 0), NaN refusal, and the kk block size becomes nbins."""
+    p = synth.sixpt_problem()
+    ctx.load(p)
+    n6 = ctx.datavector_size()
+    assert ctx.kk_bandpower_info() == (0, 0, 0, 1.0)
+    M = np.ones((4, 100 - 8 + 1))
+    for bad in ((0, 8, 100), (4, 0, 100), (4, 50, 10)):
+        with pytest.raises((CosmolikeError, ValueError)):
+            ctx.init_cmb_auto_bandpower(*bad, np.ones(max(bad[0], 1) * max(bad[2] - bad[1] + 1, 1)))
+    with pytest.raises(CosmolikeError) as e:
+        ctx.init_cmb_auto_bandpower(4, 8, 100, M, alpha=0.0)
+    assert e.value.code == -1
+    Mn = M.copy()
+    Mn[2, 7] = np.nan
+    with pytest.raises(CosmolikeError) as e:
+        ctx.init_cmb_auto_bandpower(4, 8, 100, Mn)
+    assert e.value.code == -2
+    with pytest.raises(ValueError):
+        ctx.init_cmb_auto_bandpower(4, 8, 100, M, np.zeros(3))
+    assert ctx.kk_bandpower_info()[0] == 0 and ctx.datavector_size() == n6
+    ctx.init_cmb_auto_bandpower(4, 8, 100, M, np.zeros(4), alpha=0.9)
+    assert ctx.kk_bandpower_info() == (4, 8, 100, 0.9)
+    assert ctx.datavector_size() == n6 - p.kk_nells + 4
+
+
 def test_datavector_sizes(ctx):
     p = synth.full_3x2pt_problem()
     ctx.load(p)

@@ -2,7 +2,7 @@
 probes w_gk(theta), w_ks(theta) (with the CMB beam and pixel filter) and
 C_kk at the Fourier ell bins, the whole six-block real-space data vector in
 two block orders and both Legendre modes, batches with graph replay, FP32
-mode, and the VJP refusal.
+mode, kk band powers, and the VJP refusal.
 
 Bar: 1e-6 relative per entry with a floor of 1e-3 of the block row's largest
 magnitude (the FP64 parity metric); FP32 mode 1e-4.
@@ -121,6 +121,47 @@ def test_6x2pt_fp32(six):
     e = dv_err(_forward(c, c.datavector_size(), 1)[0], ref, p, len(ells))
     print(e)
     assert max(e.values()) < 1e-4
+
+
+@pytest.mark.parametrize("binning", ["smooth", "onehot"])
+def test_6x2pt_kk_bandpower(six, binning):
+    """init_cmb_auto_bandpower: the kk block as band powers
+    sum_L M[j][L - lmin] C_kk(L) - offset[j]; the five other blocks unchanged.
+    The one-hot binning puts single L on both sides of LMIN_tab = 20 (direct
+    rows below, table above); the smooth one spans L = 8..400."""
+    p, o, ells, ref = six
+    if binning == "smooth":
+        bp = synth.kk_bandpower_binning()
+    else:
+        lmin, lmax = 5, 40
+        picks = [5, 12, 19, 20, 21, 33, 40]
+        M = np.zeros((len(picks), lmax - lmin + 1))
+        for j, L in enumerate(picks):
+            M[j, L - lmin] = 1.0
+        bp = (len(picks), lmin, lmax, M.reshape(-1), np.linspace(-2e-9, 2e-9, len(picks)))
+    q = _problem()
+    q.kk_bp = bp
+    oq = O.Oracle().load(q)
+    want = oq.datavector_6x2pt(ells)
+    c = Cosmolike().load(q)
+    assert c.kk_bandpower_info()[:3] == bp[:3]
+    assert c.datavector_size() == len(want) == len(ref) - len(ells) + bp[0]
+    got = c.compute_6x2pt_real()
+    n5 = len(ref) - len(ells)
+    assert rel_err(got[:n5].reshape(1, -1), ref[:n5].reshape(1, -1)) < RTOL
+    e = np.abs(got[n5:] - want[n5:]) / np.abs(want[n5:])
+    print(binning, e.max())
+    assert e.max() < RTOL
+    # batched with graph replay, and the block order moves kk whole
+    c.batch_upload([q, q])
+    n = c.datavector_size()
+    b = _forward(c, n, 2)
+    assert rel_err(b, np.stack([want, want])) < RTOL
+    c.batch_use_graph(True)
+    assert np.array_equal(_forward(c, n, 2), b)
+    c.batch_use_graph(False)
+    rev = c.compute_6x2pt_real((3, 4, 5, 0, 1, 2))
+    assert rel_err(rev.reshape(1, -1), oq.datavector_6x2pt(ells, (3, 4, 5, 0, 1, 2)).reshape(1, -1)) < RTOL
 
 
 def test_6x2pt_vjp_refused(six):

@@ -126,3 +126,42 @@ def test_6x2pt_assembly(six):
     for b in range(5, -1, -1):
         assert np.array_equal(rev[pos:pos + sizes[b]], dv6[starts[b]:starts[b] + sizes[b]]), b
         pos += sizes[b]
+
+
+def test_kk_bandpower(six):
+    """init_cmb_auto_bandpower: the kk block becomes sum_L M[j][L - lmin]
+    C_kk(L) - offset[j] (generic_interface.hpp:1095-1118). One-hot rows give
+    C_kk at single multipoles on both sides of LMIN_tab (direct at L <= 20,
+    the table above); the map is linear in M; the other blocks stay as they
+    are and the block size is the number of band powers."""
+    p, o = six
+    ells = synth.ells_fourier(p.kk_nells, p.kk_lmin, p.kk_lmax)
+    dv6 = o.datavector_6x2pt(ells)
+    n5 = len(dv6) - len(ells)
+    lmin, lmax = 8, 400
+    picks = [8, 15, 20, 21, 22, 57, 400]
+    M = np.zeros((len(picks), lmax - lmin + 1))
+    for j, L in enumerate(picks):
+        M[j, L - lmin] = 1.0
+    off = np.linspace(-1e-9, 3e-9, len(picks))
+    o.set_kk_bandpower(len(picks), lmin, lmax, M.reshape(-1), off)
+    try:
+        got = o.datavector_6x2pt(ells)
+        assert len(got) == n5 + len(picks)
+        assert np.array_equal(got[:n5], dv6[:n5])
+        ref = np.array([o.cl_kk([float(L)])[0] for L in picks]) - off
+        assert np.allclose(got[n5:], ref, rtol=1e-14, atol=0)
+        # direct below LMIN_tab, the table above: these agree only closely
+        direct = o.cl_cmb_nointerp("kk", [21.0, 57.0])[0]
+        assert np.allclose(got[n5 + 3:n5 + 6:2] + off[3:6:2], direct, rtol=2e-3)
+        # linear in M: a normalised band equals the weighted one-hot sum
+        nbp, lo, hi, Mb, offb = synth.kk_bandpower_binning()
+        o.set_kk_bandpower(nbp, lo, hi, Mb, offb)
+        bp = o.datavector_6x2pt(ells)[n5:]
+        Ls = np.arange(lo, hi + 1.0)
+        ckk = np.array([o.cl_kk([L])[0] for L in Ls])
+        assert np.allclose(bp, Mb.reshape(nbp, -1) @ ckk - offb, rtol=1e-12, atol=0)
+        assert np.all(bp > 0)
+    finally:
+        o.set_kk_bandpower(0, 0, 0, np.zeros(1))
+    assert np.array_equal(o.datavector_6x2pt(ells), dv6)
