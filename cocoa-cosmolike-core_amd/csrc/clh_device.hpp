@@ -494,11 +494,24 @@ __device__ __forceinline__ ClhGgTabs clh_gg_tabs(const DrawDesc& d, int zl, doub
 // increasing log10 k (p_nonlin, cosmo3D.c:362-417): the bracketing log10 k
 // values, the reciprocal spacing and the four ln P values of the current cell
 // stay in registers; memory is read only when log10 k leaves the cell (the
-// index is the one clh_bs would return).
+// index is the one clh_bs would return).  Successive rows mostly move one
+// cell up, so the log10 k knot and the ln P pair of row i + 2 are loaded one
+// move ahead: a one-cell step then needs no load to complete (the values are
+// the same ones set_cell would read).
 struct ClhPkNode {
   int i, j, nk, nz;
   double dy, lo, hi, inv_k, p00, p01, p10, p11;
+  double nhi, n0, n1;  // lk[i + 2], ln P[i + 2][j], ln P[i + 2][j + 1] (i + 2 <= nk - 1)
   const double* __restrict__ lnP;
+
+  __device__ __forceinline__ void ahead(const double* __restrict__ lk) {
+    if (i + 2 <= nk - 1) {
+      nhi = lk[i + 2];
+      const double* __restrict__ P = lnP + (size_t)(i + 2) * nz + j;
+      n0 = P[0];
+      n1 = P[1];
+    }
+  }
 
   __device__ __forceinline__ void init(const DrawDesc& d, int j_, double dy_) {
     i = -1;
@@ -517,6 +530,7 @@ struct ClhPkNode {
     p01 = P[1];
     p10 = P[nz];
     p11 = P[nz + 1];
+    ahead(lk);
   }
   // ln P at log10 k; dx: the log10 k weight of the cell (i, j)
   __device__ __forceinline__ double lnp(const double* __restrict__ lk, double log10k, double& dx) {
@@ -524,8 +538,21 @@ struct ClhPkNode {
       i = clh_bs(lk, nk, log10k);
       set_cell(lk);
     } else if (!((log10k >= lo || i == 0) && (log10k < hi || i == nk - 2))) {
-      i = clh_walk(lk, nk, log10k, i);
-      set_cell(lk);
+      if (i + 1 <= nk - 2 && log10k >= hi && (log10k < nhi || i + 1 == nk - 2)) {
+        // one cell up: the clh_walk index is i + 1
+        i++;
+        lo = hi;
+        hi = nhi;
+        inv_k = 1.0 / (hi - lo);
+        p00 = p10;
+        p01 = p11;
+        p10 = n0;
+        p11 = n1;
+        ahead(lk);
+      } else {
+        i = clh_walk(lk, nk, log10k, i);
+        set_cell(lk);
+      }
     }
     dx = (log10k - lo) * inv_k;
     return (1 - dx) * (1 - dy) * p00 + (1 - dx) * dy * p01 + dx * (1 - dy) * p10 + dx * dy * p11;
