@@ -18,6 +18,8 @@
 
 #define CLH_BLOCK 256
 
+typedef double clh_v4d __attribute__((ext_vector_type(4)));  // an FP64 MFMA 16x16 accumulator
+
 // ----------------------------------------------------------------------------
 // block-wide inclusive scan of one double per thread (256 threads, wave64)
 // ----------------------------------------------------------------------------
@@ -297,58 +299,54 @@ __global__ __launch_bounds__(128) void k_nodes(const DrawDesc* __restrict__ D,
 // CLH_SUB_ROWS.  One thread per GL node (4 waves x 64 nodes) walks 4 rows of
 // every sub-tile in increasing ell, so the log10 k index of the P(k,z) lookup
 // moves by a step or two between rows; the z index, its weight and
-// log10(chi c/H0) come from k_nodes.  P fac of the sub-tile goes to LDS, then
-// the pair sums read it with the node weights (also in LDS).
-// F32: ss tiles with FP32 P(k,z) (expf of the FP64 bilinear value), products
-// and Kahan-compensated node sums; gs tiles always run in FP64 (their lensing
-// and intrinsic-alignment terms can cancel to 1e-3 of their size, which FP32
-// rounding does not survive at 1e-4).
+// log10(chi c/H0) come from k_nodes.  S = P fac of the sub-tile goes to LDS.
+// The pair sums are then products on FP64 MFMA (v_mfma_f64_16x16x4f64) over
+// the 64 nodes: ss C[pair][row] = sum_p (A_i A_j)[p] S[row][p] in tiles of 16
+// pairs x 16 rows (one tile per wave in turn); gs C[zs][row] = pref2 (b1 G1 +
+// pref1 bmag G2 (+ G3)), G1 = (B_zs W_gal) S^T, G2 = (B_zs W_mag) S^T and,
+// with the one-loop bias, G3 = (B_zs W_gal) O^T.
+// F32: the ss rows' P(k,z) as expf of the FP64 bilinear value (products and
+// sums stay FP64).
 // KEEP: S = P fac of every (row, node) also goes to the draw's checkpoint
 // (d.sxk) for the VJP.
 // ----------------------------------------------------------------------------
-struct ClhKahan {
-  float s = 0.0f, c = 0.0f;
-  __device__ __forceinline__ void add(float x) {
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-};
-
 template <bool F32, bool KEEP, bool OL = false>
 __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restrict__ D,
                                                          const int4* __restrict__ work) {
-  // rows of S / Sf padded by one element: the pair sums read 16 rows at one
-  // node index per wave, which an unpadded 64-element stride maps onto a
-  // single LDS bank
+  // rows of S / O padded by one element: the MFMA B operand reads 16 rows at
+  // 4 node indices per wave, which a 64-element stride maps onto few banks
   constexpr int sp = CLH_NP_SS + 1;
   __shared__ double S[CLH_SUB_ROWS * sp];
   __shared__ double W[(CLH_MAXB + 2) * CLH_NP_SS];
-  __shared__ float Sf[CLH_SUB_ROWS * sp];
-  __shared__ float Wf[CLH_MAXB * CLH_NP_SS];
   __shared__ double L10[CLH_TILE_ROWS];
   __shared__ double s_lk[CLH_LDS_LK];
   __shared__ double O[OL ? CLH_SUB_ROWS * sp : 1];  // gs one-loop bias term per (row, node)
+  __shared__ unsigned char PB[2][CLH_MAXB * (CLH_MAXB + 1) / 2];  // ss pair -> (bin i, bin j)
   const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
   const int kind = w.y & 3, zl = w.y >> 2, row0 = w.z, nr = w.w;
-  const bool f32 = F32 && kind == CLH_W_SS;
-  const bool ol = OL && kind != CLH_W_SS && d.oneloop;
+  const bool ss = kind == CLH_W_SS;
+  const bool f32 = F32 && ss;
+  const bool ol = OL && !ss && d.oneloop;
   const int tid = threadIdx.x;
   constexpr int np = CLH_NP_SS;
   const double* o = clh_sx_nodes(d, kind, zl);
   const double* lut = o + clh_sx_lut_col(d, kind) * np;
-  const double* ells = (kind == CLH_W_SS) ? d.ell_ss : d.ell_gx;
+  const double* ells = ss ? d.ell_ss : d.ell_gx;
   for (int t = tid; t < nr; t += CLH_BLOCK) L10[t] = log10(ells[row0 + t] + 0.5);
-  const int nw = (kind == CLH_W_SS) ? d.ns : d.ns + 2;
-  if (f32)
-    for (int i = tid; i < nw * np; i += CLH_BLOCK) Wf[i] = (float)o[3 * np + i];
-  else
-    for (int i = tid; i < nw * np; i += CLH_BLOCK) W[i] = o[3 * np + i];
+  const int nw = ss ? d.ns : d.ns + 2;
+  for (int i = tid; i < nw * np; i += CLH_BLOCK) W[i] = o[3 * np + i];
+  if (ss)
+    for (int q = tid; q < d.n_ss; q += CLH_BLOCK) {
+      int bi, bj;
+      clh_ss_pair(d.ns, q, bi, bj);
+      PB[0][q] = (unsigned char)bi;
+      PB[1][q] = (unsigned char)bj;
+    }
   const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
   __syncthreads();
   const int p = tid & 63, wv = tid >> 6;
+  const int lm = p & 15, lq = p >> 4;  // MFMA lane coordinates
   const double l10fk = lut[2 * np + p], fac = o[2 * np + p];
   ClhPkNode pkn;
   pkn.init(d, (int)lut[p], lut[np + p]);
@@ -369,12 +367,9 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
       if (rr < nr) {
         double dx;
         const double out = pkn.lnp(lk, L10[rr] - l10fk, dx);
-        if (f32) Sf[rl * sp + p] = expf((float)out) * (float)(c3 * fac);
-        else s = exp(out) * c3 * fac;
-      } else if (f32) {
-        Sf[rl * sp + p] = 0.0f;
+        s = f32 ? (double)(expf((float)out) * (float)(c3 * fac)) : exp(out) * c3 * fac;
       }
-      if (!f32) S[rl * sp + p] = s;
+      S[rl * sp + p] = s;
       if (OL) {
         double ov = 0.0;
         if (ol && rr < nr) {
@@ -383,64 +378,67 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx(const DrawDesc* __restr
         }
         O[rl * sp + p] = ov;
       }
-      if (KEEP && rr < nr)
-        d.sxk[((size_t)(kind == CLH_W_SS ? 0 : 1 + zl) * d.nrow + row0 + rr) * np + p] =
-            f32 ? (double)Sf[rl * sp + p] : s;
+      if (KEEP && rr < nr) d.sxk[((size_t)(ss ? 0 : 1 + zl) * d.nrow + row0 + rr) * np + p] = s;
     }
     __syncthreads();
-    if (kind == CLH_W_SS) {
-      const int nout = d.n_ss * CLH_SUB_ROWS;
-      for (int qq = tid; qq < nout; qq += CLH_BLOCK) {
-        const int pair = qq / CLH_SUB_ROWS, rl = qq % CLH_SUB_ROWS;
-        const int rr = base + rl;
-        if (rr >= nr) continue;
-        int bi, bj;
-        clh_ss_pair(d.ns, pair, bi, bj);
-        double acc;
-        if (f32) {
-          const float* Ai = Wf + bi * np;
-          const float* Aj = Wf + bj * np;
-          const float* Sr = Sf + rl * sp;
-          ClhKahan k;
-#pragma unroll 8
-          for (int pp = 0; pp < np; pp++) k.add(Ai[pp] * Aj[pp] * Sr[pp]);
-          acc = (double)k.s;
-        } else {
-          const double* Ai = W + bi * np;
-          const double* Aj = W + bj * np;
-          const double* Sr = S + rl * sp;
-          acc = 0.0;
-#pragma unroll 8
-          for (int pp = 0; pp < np; pp++) acc += Ai[pp] * Aj[pp] * Sr[pp];
+    // MFMA operands: A[m][k] on lane m + 16 k, B[k][n] on lane n + 16 k; the
+    // result D[4 v + lq][lm] is value v of lane p.  n = lm is the row.
+    const double* Sb = S + lm * sp + lq;
+    const int rr = base + lm;
+    if (ss) {
+      const int ntile = (d.n_ss + 15) >> 4;
+      for (int t = wv; t < ntile; t += 4) {
+        const int pr = 16 * t + lm;
+        const bool pin = pr < d.n_ss;
+        const double* Ai = W + (pin ? PB[0][pr] : 0) * np + lq;
+        const double* Aj = W + (pin ? PB[1][pr] : 0) * np + lq;
+        clh_v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int st = 0; st < np / 4; st++) {
+          const double a = pin ? Ai[4 * st] * Aj[4 * st] : 0.0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Sb[4 * st], acc, 0, 0, 0);
         }
-        const double l = ells[row0 + rr], ell = l + 0.5, ell4 = ell * ell * ell * ell;
-        const double pref = l * (l - 1.) * (l + 1.) * (l + 2.) / ell4;
-        d.cl[(size_t)pair * d.nrow + row0 + rr] = acc * pref;
+        if (rr < nr) {
+          const double l = ells[row0 + rr], ell = l + 0.5, ell4 = ell * ell * ell * ell;
+          const double pref = l * (l - 1.) * (l + 1.) * (l + 2.) / ell4;
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int pair = 16 * t + 4 * v + lq;
+            if (pair < d.n_ss) d.cl[(size_t)pair * d.nrow + row0 + rr] = acc[v] * pref;
+          }
+        }
       }
     } else {
-      const int nout = d.ns * CLH_SUB_ROWS;
+      const int ntile = (d.ns + 15) >> 4;
       const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
-      for (int qq = tid; qq < nout; qq += CLH_BLOCK) {
-        const int zs = qq / CLH_SUB_ROWS, rl = qq % CLH_SUB_ROWS;
-        const int rr = base + rl;
-        if (rr >= nr) continue;
-        const double l = ells[row0 + rr], ell = l + 0.5;
-        const double pref1 = l * (l + 1.) / (ell * ell);
-        const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
-        const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
-        const double* Bz = W + (2 + zs) * np;
-        const double* Sr = S + rl * sp;
-        double acc = 0.0;
-        if (ol) {  // + W_gal x the one-loop term (cosmo2D.c:2059-2060)
-          const double* Or = O + rl * sp;
-#pragma unroll 8
-          for (int pp = 0; pp < np; pp++)
-            acc += Bz[pp] * ((W[pp] * b1 + W[np + pp] * pref1 * bmag) * Sr[pp] + W[pp] * Or[pp]);
-        } else {
-#pragma unroll 8
-          for (int pp = 0; pp < np; pp++) acc += Bz[pp] * (W[pp] * b1 + W[np + pp] * pref1 * bmag) * Sr[pp];
+      const double* Ob = O + lm * sp + lq;
+      for (int t = wv; t < ntile; t += 4) {
+        const int zs = 16 * t + lm;
+        const bool zin = zs < d.ns;
+        const double* Bz = W + (2 + (zin ? zs : 0)) * np + lq;
+        clh_v4d g1 = {0.0, 0.0, 0.0, 0.0}, g2 = g1, g3 = g1;
+#pragma unroll 4
+        for (int st = 0; st < np / 4; st++) {
+          const double wb = zin ? Bz[4 * st] : 0.0;
+          const double a1 = wb * W[4 * st + lq], a2 = wb * W[np + 4 * st + lq];
+          const double bs = Sb[4 * st];
+          g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bs, g1, 0, 0, 0);
+          g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bs, g2, 0, 0, 0);
+          if (ol) g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, Ob[4 * st], g3, 0, 0, 0);
         }
-        d.cl[(size_t)(d.n_ss + zl * d.ns + zs) * d.nrow + row0 + rr] = acc * pref2;
+        if (rr < nr) {
+          const double l = ells[row0 + rr], ell = l + 0.5;
+          const double pref1 = l * (l + 1.) / (ell * ell);
+          const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+          const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int z = 16 * t + 4 * v + lq;
+            if (z < d.ns)
+              d.cl[(size_t)(d.n_ss + zl * d.ns + z) * d.nrow + row0 + rr] =
+                  (g1[v] * b1 + g2[v] * pref1 * bmag + g3[v]) * pref2;
+          }
+        }
       }
     }
     __syncthreads();
@@ -711,7 +709,6 @@ __device__ __forceinline__ void clh_place(const DrawDesc& d, int kind, int q, in
 // A[m][k] on lane m + 16 k, B[k][n] on lane n + 16 k, D[4 v + l / 16][l % 16]
 // in value v of lane l.
 // ----------------------------------------------------------------------------
-typedef double clh_v4d __attribute__((ext_vector_type(4)));
 #define CLH_LEG_UNROLL 8
 __global__ __launch_bounds__(256) void k_legendre(const DrawDesc* __restrict__ D,
                                                   const int4* __restrict__ work) {
