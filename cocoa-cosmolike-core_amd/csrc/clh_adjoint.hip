@@ -334,7 +334,7 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
       }
     }
   } else {
-    ClhPkNode pkn;
+    ClhPkNodeT<false> pkn;  // no P-row look-ahead: it would cost a wave per SIMD
     pkn.init(d, j, dy);
     for (int rr = 0; rr < T; rr++) {
       double dx;
@@ -428,7 +428,7 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
       }
     }
   } else {
-    ClhGgNode<true> nd;
+    ClhGgNode<true, false> nd;  // no P-row look-ahead: it would cost a wave per SIMD
     nd.init(d, T, o, p, zl);
     for (int rr = r_lo; rr < r_hi; rr++) {
       const double cb = Cr[rr];

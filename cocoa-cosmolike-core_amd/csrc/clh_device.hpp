@@ -495,17 +495,19 @@ __device__ __forceinline__ ClhGgTabs clh_gg_tabs(const DrawDesc& d, int zl, doub
 // values, the reciprocal spacing and the four ln P values of the current cell
 // stay in registers; memory is read only when log10 k leaves the cell (the
 // index is the one clh_bs would return).  Successive rows mostly move one
-// cell up, so the log10 k knot and the ln P pair of row i + 2 are loaded one
-// move ahead: a one-cell step then needs no load to complete (the values are
-// the same ones set_cell would read).
-struct ClhPkNode {
+// cell up, so with AHEAD the log10 k knot and the ln P pair of row i + 2 are
+// loaded one move ahead: a one-cell step then needs no load to complete (the
+// values are the same ones set_cell would read).  The look-ahead costs 6
+// VGPRs, so kernels near an occupancy step leave it off.
+template <bool AHEAD = true>
+struct ClhPkNodeT {
   int i, j, nk, nz;
   double dy, lo, hi, inv_k, p00, p01, p10, p11;
   double nhi, n0, n1;  // lk[i + 2], ln P[i + 2][j], ln P[i + 2][j + 1] (i + 2 <= nk - 1)
   const double* __restrict__ lnP;
 
   __device__ __forceinline__ void ahead(const double* __restrict__ lk) {
-    if (i + 2 <= nk - 1) {
+    if (AHEAD && i + 2 <= nk - 1) {
       nhi = lk[i + 2];
       const double* __restrict__ P = lnP + (size_t)(i + 2) * nz + j;
       n0 = P[0];
@@ -538,7 +540,7 @@ struct ClhPkNode {
       i = clh_bs(lk, nk, log10k);
       set_cell(lk);
     } else if (!((log10k >= lo || i == 0) && (log10k < hi || i == nk - 2))) {
-      if (i + 1 <= nk - 2 && log10k >= hi && (log10k < nhi || i + 1 == nk - 2)) {
+      if (AHEAD && i + 1 <= nk - 2 && log10k >= hi && (log10k < nhi || i + 1 == nk - 2)) {
         // one cell up: the clh_walk index is i + 1
         i++;
         lo = hi;
@@ -558,6 +560,8 @@ struct ClhPkNode {
     return (1 - dx) * (1 - dy) * p00 + (1 - dx) * dy * p01 + dx * (1 - dy) * p10 + dx * dy * p11;
   }
 };
+
+typedef ClhPkNodeT<true> ClhPkNode;
 
 // The log10 k cell of a P(k,z) lookup along increasing k without the ln P
 // values: the same index and weight dx as ClhPkNode::lnp.
@@ -598,7 +602,7 @@ struct ClhBracket {
   }
 };
 
-template <bool ADJ>
+template <bool ADJ, bool AHEAD = true>
 struct ClhGgNode {
   double fK, fac, WG, WM, b1, bmag, l10fk, t0, t0d, inv_st, dzl, zm, chi_amin;
   ClhBracket bc, bz, bg, bn;           // chi_v, chi_z, G_z, lens n(z) knots
@@ -606,7 +610,7 @@ struct ClhGgNode {
   double zjj, inv_z, up, down;         // dchi/dz stencil payload
   double gv_lo, slG;                   // G(z) interval payload
   double cf0, cf1, cf2, cf3;           // lens spline piece
-  ClhPkNode pk;
+  ClhPkNodeT<AHEAD> pk;
   int zl, jj;
 
   __device__ __forceinline__ void set_c(const ClhGgTabs& T) {

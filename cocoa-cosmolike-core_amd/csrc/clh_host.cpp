@@ -2745,6 +2745,50 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
       }
     }
   }
+  // forward + VJP (checkpointing forward, tuned for pairs): the per-stage
+  // minima need not make the fastest whole pass (a stage's tile and LDS
+  // choice changes what the next stage finds in cache), so the forward gg
+  // tile and both LDS choices are also checked jointly, timing whole passes
+  if (adj && bt.keep_ckpt && rc == CLH_OK) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    float best_all = 1e30f;
+    int jt[4] = {pick[0], pick[1], pick[2], pick[3]}, jl[2] = {pick_lds[0], pick_lds[1]};
+    for (int g = 0; g < 2 && he == hipSuccess && rc == CLH_OK; g++) {
+      const int t[4] = {pick[0], g == 0 ? pick[1] : (pick[1] == 64 ? 32 : 64), pick[2], pick[3]};
+      if ((rc = build_work(c, bt, t))) break;
+      for (int l0 = 0; l0 < nst && rc == CLH_OK; l0++)
+        for (int l1 = 0; l1 < nst && rc == CLH_OK; l1++) {
+          c->gg_lds[0] = l0;
+          c->gg_lds[1] = l1;
+          for (int rep = 0; rep < 3 && rc == CLH_OK; rep++) {
+            if ((he = hipEventRecord(e0, c->stream)) != hipSuccess) break;
+            if ((rc = run_batch(c, bt, false))) break;
+            if ((rc = clh_batch_vjp(c, ord, dvbar.p, glnp.p, nullptr, nullptr, nullptr, nullptr, nullptr))) break;
+            if ((he = hipEventRecord(e1, c->stream)) != hipSuccess) break;
+            if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
+            float ms = 0.0f;
+            if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+            if (ms < best_all) {
+              best_all = ms;
+              for (int k = 0; k < 4; k++) jt[k] = t[k];
+              jl[0] = l0;
+              jl[1] = l1;
+            }
+          }
+          if (he != hipSuccess) break;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc == CLH_OK && he != hipSuccess) rc = fail(c, CLH_ERR_HIP, "autotune: %s", hipGetErrorString(he));
+    if (rc == CLH_OK) {
+      for (int k = 0; k < 4; k++) pick[k] = jt[k];
+      pick_lds[0] = jl[0];
+      pick_lds[1] = jl[1];
+    }
+  }
   c->timing = was_timing;
   (void)hipStreamSynchronize(c->stream);
   dvbar.release();  // local buffers (hipFree)

@@ -612,7 +612,7 @@ __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ 
   const ClhGgTabs T = clh_gg_tabs(d, zl, dyn, stage, s_lk);
   __syncthreads();
   const int p = threadIdx.x % np, half = threadIdx.x / np;
-  ClhGgNode<KEEP> nd;
+  ClhGgNode<KEEP, !KEEP && !OL> nd;  // the look-ahead only where it keeps 3 waves per SIMD
   nd.init(d, T, clh_gg_nodes(d, zl), p, zl);
   // OL: the one-loop bias term W_gal^2 (...) of draws with some b2 != 0
   double ol_wg2 = 0.0, ol_g4f = 0.0, ol_lnfk = 0.0, ol_fk = 1.0;
