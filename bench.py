@@ -117,7 +117,7 @@ def probe_errors(dv, ref, ns, nl, nt):
 def measured_traffic(kernel, batch, precision):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE scaled by a plain device copy), or None."""
-    path = os.path.join(ROOT, "profiles", f"projection_3x2pt_b{batch}_{precision}_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"projection_3x2pt_b{batch}_{precision}_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
