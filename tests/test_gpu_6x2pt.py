@@ -177,3 +177,25 @@ def test_6x2pt_vjp_refused(six):
         c.batch_vjp(bar.data_ptr(), g_src_dz=g.data_ptr())
     c.init_probes("3x2pt")
     assert c.datavector_size() == len(ref) - (p.nl + p.ns) * p.ntheta - len(ells)
+
+
+def test_6x2pt_mask_with_bandpowers():
+    """IP::set_mask on the 6x2pt vector (generic_interface.hpp:895-921): masked
+    entries of every block, band powers included, are 0; the others are the
+    unmasked values."""
+    q = _problem(seed=41)
+    q.kk_bp = synth.kk_bandpower_binning()
+    ells = synth.ells_fourier(q.kk_nells, q.kk_lmin, q.kk_lmax)
+    want = O.Oracle().load(q).datavector_6x2pt(ells)
+    c = Cosmolike().load(q)
+    n = c.datavector_size()
+    assert n == len(want)
+    mask = (np.random.default_rng(41).random(n) > 0.3).astype(np.int32)
+    mask[-3:] = (0, 1, 0)  # band powers on both sides
+    c.like_set_mask(mask)
+    got = c.compute_6x2pt_real()
+    assert np.all(got[mask == 0] == 0.0)
+    keep = mask == 1
+    assert rel_err(got[keep].reshape(1, -1), want[keep].reshape(1, -1)) < RTOL
+    c.like_set_mask(None)
+    assert rel_err(c.compute_6x2pt_real().reshape(1, -1), want.reshape(1, -1)) < RTOL
