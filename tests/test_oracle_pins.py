@@ -610,3 +610,53 @@ def test_fpt_table_lookup():
         assert np.isclose(o.fpt_at(t, km), 0.5 * (tab[t][i] + tab[t][i + 1]) / h3, rtol=1e-9)
     assert o.fpt_at(0, p.ia_ps_kmin * COVERH0 * 0.99) == 0.0
     assert o.fpt_at(0, p.ia_ps_kmax * COVERH0 * 1.01) == 0.0
+
+
+def test_rsd_term_vs_continuum(p3):
+    """W_RSD (radial_weights.c:87-105, used at cosmo2D.c:2534-2547) at low ell,
+    where it matters: C_gg(b) = b^2 A + b B + C0 is quadratic in b, so three
+    bias values separate the RSD cross term B = 2 b int W_gal W_RSD P and the
+    RSD square C0 = int W_RSD^2 P from the density term A.  Each is compared
+    with a continuum integral over z of the same Limber kernels with W_RSD =
+    c0 n(z0) E(z0) f(z0) - c1 n(z1) E(z1) f(z1), c0 = (1 + 8L)/(2L + 1)^2,
+    c1 = 4/(2L + 3) sqrt((2L + 1)/(2L + 3)), L = l + 1/2, chi_1 = chi (L + 1)/L
+    and f = 1 - (1 + z) dln G/dz from the G(z) table.  At l = 5 and 10 they
+    agree within 2e-2 while B / A is 1.8-22%; a flipped a_1 sign, l in place of
+    l + 1/2 in c0 and c1, or chi_1 = chi (L + 3/2)/L each moves B by more than
+    that in at least one of these cases."""
+    P, D, chi, E = _continuum_setup(p3)
+    lens = _nz_splines(p3.lens)
+    zt = np.linspace(0.0, 4.0, 40001)
+    ct = chi(zt)
+    G = np.interp(zt, p3.G_z, p3.G)
+    ft = 1 - (1 + zt) * np.gradient(np.log(G), zt)
+    o = O.Oracle().load(p3)
+    ells = [5.0, 10.0]
+    C = []
+    for s in (0.0, 1.0, 2.0):
+        o.set_nuisance(b1=p3.b1 * s)
+        C.append(o.cl_nointerp("gg", ells))
+    o.set_nuisance(b1=p3.b1)
+    C0 = C[0]
+    A = (C[2] - 2 * C[1] + C[0]) / 2
+    B = C[1] - C[0] - A
+    z = np.linspace(1e-3, 2.0, 200001)
+    cz = chi(z)
+    for zl, cases in ((0, ells), (2, ells), (4, [10.0])):
+        b = p3.b1[zl]
+        for l in cases:
+            n = ells.index(l)
+            L = l + 0.5
+            c0 = (1 + 8 * L) / (2 * L + 1) ** 2
+            c1 = 4 / (2 * L + 3) * np.sqrt((2 * L + 1) / (2 * L + 3))
+            z1 = np.interp(cz * (L + 1) / L, ct, zt)
+            wr = c0 * lens[zl](z) * E(z) * np.interp(z, zt, ft) - c1 * lens[zl](z1) * E(z1) * np.interp(z1, zt, ft)
+            wg = lens[zl](z) * E(z)
+            m = P(L / cz, z) / E(z) / cz**2
+            a_ref = np.trapezoid(m * wg * wg, z) * b * b
+            b_ref = np.trapezoid(m * 2 * wg * wr, z) * b
+            c_ref = np.trapezoid(m * wr * wr, z)
+            assert abs(A[zl, n] / a_ref - 1) < 1e-3, (zl, l)
+            assert abs(b_ref / a_ref) > 1e-2, (zl, l)  # the RSD term is really exercised
+            assert abs(B[zl, n] / b_ref - 1) < 2e-2, (zl, l, B[zl, n], b_ref)
+            assert abs(C0[zl, n] / c_ref - 1) < 2e-2, (zl, l, C0[zl, n], c_ref)
