@@ -2,7 +2,8 @@
 probes w_gk(theta), w_ks(theta) (with the CMB beam and pixel filter) and
 C_kk at the Fourier ell bins, the whole six-block real-space data vector in
 two block orders and both Legendre modes, batches with graph replay, FP32
-mode, kk band powers, and the VJP refusal.
+mode, kk band powers, masks, batches of different 6x2pt layouts, and the
+VJP refusal.
 
 Bar: 1e-6 relative per entry with a floor of 1e-3 of the block row's largest
 magnitude (the FP64 parity metric); FP32 mode 1e-4.
@@ -199,3 +200,30 @@ def test_6x2pt_mask_with_bandpowers():
     assert rel_err(got[keep].reshape(1, -1), want[keep].reshape(1, -1)) < RTOL
     c.like_set_mask(None)
     assert rel_err(c.compute_6x2pt_real().reshape(1, -1), want.reshape(1, -1)) < RTOL
+
+
+def test_6x2pt_ragged_layouts():
+    """Draws of different 6x2pt layouts in one batch (per-draw captured
+    layouts): C_kk at like.ell with 26 theta bins, and band powers with 20
+    theta bins; each draw against the oracle with its own layout."""
+    qa = _problem(seed=51)
+    qb = _problem(seed=52)
+    qb.ntheta = 20
+    qb.kk_bp = synth.kk_bandpower_binning(nbp=6, lmin=10, lmax=300)
+    refs, nfs, lays = [], [], []
+    c = Cosmolike()
+    for q in (qa, qb):
+        ells = synth.ells_fourier(q.kk_nells, q.kk_lmin, q.kk_lmax)
+        refs.append(O.Oracle().load(q).datavector_6x2pt(ells))
+        nfs.append(q.kk_bp[0] if q.kk_bp is not None else len(ells))
+        c.load(q)
+        lays.append(c.capture_layout())
+    order = [0, 1, 0, 1]
+    c.batch_upload_layouts([(qa, qb)[k] for k in order], [lays[k] for k in order])
+    offs = c.batch_offsets()
+    assert offs[-1] == sum(len(refs[k]) for k in order)
+    got = _forward(c, offs[-1], 1)[0]
+    for i, k in enumerate(order):
+        e = dv_err(got[offs[i]:offs[i + 1]], refs[k], (qa, qb)[k], nfs[k])
+        print(i, k, e)
+        assert max(e.values()) < RTOL, (i, k)
