@@ -281,6 +281,7 @@ struct Batch {
   int n_wlg = 0, lnp_ic = 0;  // ln P gather: work items (draw, first k row), k rows per item
   DBuf<int4> d_wl, d_wsx, d_wgg, d_wga, d_wmx, d_wlf;
   int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0, n_wsx = 0, n_wgg = 0, n_wga = 0, n_wmx = 0, n_wlf = 0;
+  int n_wsx_std = 0;  // ss/gs tiles of non-TATT draws, first in d_wsx; the TATT draws' tiles follow
   int nrow_custom = 0;  // rows of a custom-ell batch (0: the layouts' rows)
   int gg_stage = 0;     // LDS doubles for the gg kernels' staged tables (0: not staged)
   // the device state (lens-efficiency tables, node weights, ss/gs C_ell rows)
@@ -1056,7 +1057,7 @@ static bool ord_ok(const Layout& L, const int ord[3]) {
 // adjoint ss/gs and adjoint gg Limber stages (1..CLH_TILE_ROWS).
 static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   std::vector<int2> wg, wn, wl2;
-  std::vector<int4> wl, wl_gs, wsx, wgg, wga, wmx, wlf, wa;
+  std::vector<int4> wl, wl_gs, wsx, wsx_t, wgg, wga, wmx, wlf, wa;
   int lds_ss = 0, lds_gs = 0;
   std::vector<int2> tiles, atiles_ss, atiles_gs;
   for (int i = 0; i < bt.B; i++) {
@@ -1067,10 +1068,12 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
     for (int b = 0; b < ns + nl; b++) wg.push_back(make_int2(i, b));
     for (int sg = 0; sg <= 2 * nl; sg++) wn.push_back(make_int2(i, sg));
     // forward Limber: ss, gs and gg tiles (draw, kind + 4 zl, row0, rows)
+    // (TATT draws' tiles in their own list: they run in k_limber_sx_tatt)
     row_tiles(nrow, rows[0], tiles);
-    for (const int2& t : tiles) wsx.push_back(make_int4(i, CLH_W_SS, t.x, t.y));
+    std::vector<int4>& ws = L.ia_model == CLH_IA_TATT ? wsx_t : wsx;
+    for (const int2& t : tiles) ws.push_back(make_int4(i, CLH_W_SS, t.x, t.y));
     for (int zl = 0; zl < nl; zl++)
-      for (const int2& t : tiles) wsx.push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
+      for (const int2& t : tiles) ws.push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
     row_tiles(nrow, rows[1], tiles);
     for (int zl = 0; zl < nl; zl++)
       for (const int2& t : tiles) wgg.push_back(make_int4(i, zl, t.x, t.y));
@@ -1124,6 +1127,8 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   bt.n_wl_ss = (int)wl.size();
   wl.insert(wl.end(), wl_gs.begin(), wl_gs.end());
   HIPCHK(c, up4(bt.d_wl, wl, bt.n_wl));
+  bt.n_wsx_std = (int)wsx.size();
+  wsx.insert(wsx.end(), wsx_t.begin(), wsx_t.end());
   HIPCHK(c, up4(bt.d_wsx, wsx, bt.n_wsx));
   HIPCHK(c, up4(bt.d_wgg, wgg, bt.n_wgg));
   HIPCHK(c, up4(bt.d_wga, wga, bt.n_wga));
@@ -1491,11 +1496,12 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, mixed, c->gg_lds[0] ? bt.gg_stage : 0,
                                  bt.keep_ckpt && bt.ggk_per ? 1 : 0, s, bt.oneloop ? 1 : 0));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], s));
-  if (bt.tatt)  // TATT ss/gs tiles: FP64 in every precision mode
-    HIPCHK(c, clh_launch_limber_sx_tatt(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, s));
-  else
-    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, mixed, bt.keep_ckpt ? 1 : 0, s,
-                                   bt.oneloop ? 1 : 0));
+  // ss/gs tiles: the other draws' in k_limber_sx, then the TATT draws' in
+  // k_limber_sx_tatt (FP64 in every precision mode)
+  HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx_std, mixed, bt.keep_ckpt ? 1 : 0, s,
+                                 bt.oneloop ? 1 : 0));
+  if (bt.n_wsx > bt.n_wsx_std)
+    HIPCHK(c, clh_launch_limber_sx_tatt(bt.d_desc.p, bt.d_wsx.p + bt.n_wsx_std, bt.n_wsx - bt.n_wsx_std, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
   if (legendre && bt.cmb_any)  // 6x2pt CMB lensing blocks: FP64 in every precision mode
     HIPCHK(c, clh_launch_cmb(bt.d_desc.p, bt.d_wcmb.p, bt.n_wcmb, bt.d_wcmbp.p, bt.n_wcmbp, s));
