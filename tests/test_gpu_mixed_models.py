@@ -1,6 +1,7 @@
 """One batch whose draws use different models through per-draw captured
 layouts: NLA 3x2pt, TATT intrinsic alignments, the one-loop galaxy bias,
-non-Limber w and 6x2pt, launched together, each draw against the FP64 oracle
+non-Limber w, 6x2pt, and 6x2pt with the non-Limber w and kk band powers,
+launched together, each draw against the FP64 oracle
 with its own layout.
 
 Bar: 1e-6 relative per entry with a floor of 1e-3 of the block row's largest
@@ -44,9 +45,13 @@ def block_errs(dv, ref, p, nf=0):
 @pytest.mark.parametrize("mode,tol", [("fp64", RTOL), ("fp32", 1e-4), ("bf16x3", 5e-4)])
 def test_models_in_one_batch(mode, tol):
     import torch
-    cos = [c for c, *_ in synth.random_draws(5, seed=61, ragged=False)]
+    cos = [c for c, *_ in synth.random_draws(6, seed=61, ragged=False)]
     probs = [synth.full_3x2pt_problem(cos[0]), synth.tatt_problem(cos[1]), synth.oneloop_problem(cos[2]),
-             synth.nonlimber_problem(cos[3]), synth.sixpt_problem(cos[4])]
+             synth.nonlimber_problem(cos[3]), synth.sixpt_problem(cos[4]), synth.sixpt_problem(cos[5])]
+    six_nl = probs[5]
+    six_nl.plin_log10k, six_nl.plin_z, six_nl.lnPL = synth.linear_power_table(cos[5])
+    six_nl.limber_gg = 0
+    six_nl.kk_bp = synth.kk_bandpower_binning()
     rng = np.random.default_rng(61)
     for p in probs:
         p.src_dz = rng.normal(0, 0.01, p.ns)
@@ -57,7 +62,7 @@ def test_models_in_one_batch(mode, tol):
         if getattr(p, "sixpt", 0):
             ells = synth.ells_fourier(p.kk_nells, p.kk_lmin, p.kk_lmax)
             refs.append(o.datavector_6x2pt(ells))
-            nfs.append(len(ells))
+            nfs.append(p.kk_bp[0] if getattr(p, "kk_bp", None) is not None else len(ells))
         else:
             refs.append(o.datavector())
             nfs.append(0)
@@ -66,7 +71,7 @@ def test_models_in_one_batch(mode, tol):
     for p in probs:
         c.load(p)
         lays.append(c.capture_layout())
-    order = [0, 1, 2, 3, 4, 0]
+    order = [0, 1, 2, 3, 4, 5, 0]
     c.set_precision(mode)
     c.batch_upload_layouts([probs[k] for k in order], [lays[k] for k in order])
     offs = c.batch_offsets()
