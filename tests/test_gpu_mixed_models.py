@@ -85,3 +85,15 @@ def test_models_in_one_batch(mode, tol):
         e = block_errs(got[offs[i]:offs[i + 1]], refs[k], probs[k], nfs[k])
         print(i, k, e)
         assert max(e.values()) < tol, (i, k)
+    # the same batch as one graph, replayed: bit for bit the eager result
+    c.batch_use_graph(True)
+    try:
+        for _ in range(2):
+            out.fill_(float("nan"))
+            c.batch_forward(out.data_ptr())
+            c.synchronize()
+            torch.cuda.synchronize()
+            assert np.array_equal(out.cpu().numpy(), got)
+    finally:
+        c.batch_use_graph(False)
+        c.set_precision("fp64")
