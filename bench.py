@@ -75,7 +75,9 @@ def make_ragged_draws(n, seed):
 
 
 def algorithmic_work(p):
-    """Per-draw algorithmic work of each kernel (FLOP), as defined in DESIGN.md."""
+    """Per-draw algorithmic work of each kernel (FLOP): 20 per pair-node Limber
+    integrand evaluation plus 40 per P(k,z) lookup, 2 per multiply-add of the
+    Legendre contraction."""
     nss, ngs, ngg = p.ns * (p.ns + 1) // 2, p.ns * p.nl, p.nl
     rows = p.n_ell + 19
     return {
