@@ -61,7 +61,7 @@ def test_c2_datavector_mixed(c2, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
-def test_c1_shear_mixed(mode):
+def test_shear_only_mixed(mode):
     p = synth.shear_only_problem()
     o = O.Oracle().load(p)
     xp, xm = o.xi_pm()
@@ -70,7 +70,7 @@ def test_c1_shear_mixed(mode):
     gp, gm = c.xi_pm()
     errs = probe_errors(np.concatenate([gp.ravel(), gm.ravel()]),
                         np.concatenate([xp.ravel(), xm.ravel()]), p.ns, 0, p.ntheta)
-    _report("C1 " + mode, errs)
+    _report("shear-only " + mode, errs)
     assert max(v[0] for v in errs.values()) <= TOL, errs
 
 
