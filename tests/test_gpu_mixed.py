@@ -40,7 +40,7 @@ def _report(tag, errs):
 
 
 @pytest.fixture(scope="module")
-def c2():
+def full3x2():
     p = synth.full_3x2pt_problem()
     rng = np.random.default_rng(11)
     p.src_dz = rng.normal(0, 0.01, p.ns)
@@ -50,8 +50,8 @@ def c2():
 
 
 @pytest.mark.parametrize("mode", MODES)
-def test_c2_datavector_mixed(c2, mode):
-    p, ref = c2
+def test_c2_datavector_mixed(full3x2, mode):
+    p, ref = full3x2
     c = Cosmolike().load(p)
     c.set_precision(mode)
     dv = c.compute_3x2pt_real()
@@ -96,8 +96,8 @@ def test_batch_mixed_random_cosmologies(mode):
     assert worst <= TOL
 
 
-def test_precision_switch_back_to_fp64(c2):
-    p, ref = c2
+def test_precision_switch_back_to_fp64(full3x2):
+    p, ref = full3x2
     c = Cosmolike().load(p)
     c.set_precision("bf16x3")
     c.compute_3x2pt_real()
