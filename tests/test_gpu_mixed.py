@@ -50,7 +50,7 @@ def full3x2():
 
 
 @pytest.mark.parametrize("mode", MODES)
-def test_c2_datavector_mixed(full3x2, mode):
+def test_full_3x2pt_datavector_mixed(full3x2, mode):
     p, ref = full3x2
     c = Cosmolike().load(p)
     c.set_precision(mode)
