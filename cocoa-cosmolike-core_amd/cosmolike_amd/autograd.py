@@ -7,7 +7,8 @@ data vectors, FP64, on the GPU.  The backward pass runs the library's
 hand-written adjoint kernels (``clh_batch_vjp``).
 
 Shapes (float64, cuda, contiguous): lnP [B, nk*nz] (k-major), src_dz [B, ns],
-lens_dz [B, nl], b1 [B, nl], ia_A [B], shear_m [B, ns].
+lens_dz [B, nl], b1 [B, nl], ia_A [B], shear_m [B, ns].  An input given as
+None takes the value uploaded with the batch (not that of an earlier call).
 """
 from __future__ import annotations
 
@@ -23,13 +24,17 @@ def _ptr(t):
 class Projection3x2pt(torch.autograd.Function):
     @staticmethod
     def forward(ctx, lib_ctx: Cosmolike, lnP, src_dz, lens_dz, b1, ia_A, shear_m, ord=(0, 1, 2)):
-        ins = [t.contiguous() if t is not None else None for t in (lnP, src_dz, lens_dz, b1, ia_A, shear_m)]
-        for t in ins:
+        given = [t.contiguous() if t is not None else None for t in (lnP, src_dz, lens_dz, b1, ia_A, shear_m)]
+        for t in given:
             if t is not None and (t.dtype != torch.float64 or not t.is_cuda):
                 raise TypeError("inputs must be float64 CUDA tensors")
+        dev = next((t.device for t in given if t is not None), torch.device("cuda", torch.cuda.current_device()))
+        # every input is passed explicitly: a None takes the uploaded value,
+        # never what an earlier call left on the device
+        base = lib_ctx.uploaded_params(dev)
+        ins = [t if t is not None else b for t, b in zip(given, base)]
         B = lib_ctx.batch_size
         n = lib_ctx.datavector_size()
-        dev = ins[0].device
         out = torch.empty(B, n, dtype=torch.float64, device=dev)
         lib_ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
         lib_ctx.batch_set_params(*[_ptr(t) for t in ins])

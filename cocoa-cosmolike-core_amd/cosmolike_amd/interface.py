@@ -474,6 +474,28 @@ class Cosmolike:
         self._chk(self.L.clh_batch_upload(self.h, len(problems), arr))
         self.batch_size = len(problems)
         self.param_generation = getattr(self, "param_generation", 0) + 1
+        # the uploaded values of the differentiable inputs, [B][...] in the
+        # order of batch_set_params (None where the layout has no such input)
+        lens = problems[0].lens is not None
+        self._uploaded = [
+            np.stack([_arr(p.lnP).ravel() for p in problems]),
+            np.stack([_arr(p.src_dz) for p in problems]),
+            np.stack([_arr(p.lens_dz) for p in problems]) if lens else None,
+            np.stack([_arr(p.b1) for p in problems]) if lens else None,
+            np.array([float(p.ia_A) for p in problems]),
+            np.stack([_arr(p.shear_m) for p in problems]),
+        ]
+        self._uploaded_dev = None
+
+    def uploaded_params(self, device):
+        """The uploaded batch's (lnP, src_dz, lens_dz, b1, ia_A, shear_m) as
+        float64 tensors on `device` (None where the layout has no such input)."""
+        import torch
+        key = (self.param_generation, str(device))
+        if getattr(self, "_uploaded_dev", None) is None or self._uploaded_dev[0] != key:
+            t = [None if a is None else torch.from_numpy(a).to(device) for a in self._uploaded]
+            self._uploaded_dev = (key, t)
+        return self._uploaded_dev[1]
 
     def capture_layout(self):
         """Snapshot of the current survey layout (clh_layout_capture)."""

@@ -268,6 +268,7 @@ struct Batch {
   // are not held to the occupancy of the ss ones
   int adj_lds_ss = 0, adj_lds_gs = 0, n_wl_ss = 0;
   bool ctx_layout = true;     // every draw uses the context's own layout
+  uint64_t lay_epoch = 0;     // the context's layout_epoch when a ctx_layout batch was built
   bool uniform = true;        // every draw has the same layout
   std::vector<const Layout*> lays;
   std::vector<long long> dv_off;  // [B+1] data-vector offsets of the draws
@@ -347,11 +348,20 @@ struct clh_ctx {
   dvec cache_cl;    // cl rows of `cur`
   dvec cache_bb;    // B-mode C_ss rows of `cur` (zero under NLA)
   dvec cache_nl;    // non-Limber C_ell [nl][150] then LMAX [nl] of `cur` (empty under Limber w)
+  // w(theta) of `cur` through the path adopt_limber_gg does not select
+  // (w_gg_tomo with the other `limber`), on a copy of L
+  Layout alt_L;
+  dvec alt_w;
+  uint64_t alt_w_version = 0;
+  int alt_w_limber = -1;
   // device copies of the layout
   DevLayout dl;                                        // device copy of L
   std::map<uint64_t, std::unique_ptr<LayoutCopy>> rl;  // layouts of a multi-layout batch
   uint64_t mode_epoch = 0;                             // bumped when device layouts go stale
   uint64_t batch_epoch = 0;
+  // bumped by every change of the context's layout L; a batch uploaded on L
+  // (ctx_layout) is refused once it differs from the batch's lay_epoch
+  uint64_t layout_epoch = 0;
   Batch single, batch, scratch;
   bool timing = false;
   bool use_graph = false;             // replay the batch forward from a hipGraph
@@ -369,6 +379,13 @@ struct clh_ctx {
   uint64_t like_mask_version = ~0ull;  // layout version the device mask copy was made for
   rocblas_handle blas = nullptr;
 };
+
+// the context's layout L changed: its device copy is stale and batches
+// uploaded on it are refused
+static void layout_changed(clh_ctx* c) {
+  c->dl.valid = false;
+  c->layout_epoch++;
+}
 
 static int fail(clh_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -1480,6 +1497,7 @@ static int build_batch_ctx(clh_ctx* c, Batch& bt, const std::vector<const Draw*>
   const std::vector<const DevLayout*> devs(draws.size(), &c->dl);
   rc = build_batch(c, bt, draws, lays, devs, apply_calib, ord, ells, dv_out);
   bt.ctx_layout = true;
+  bt.lay_epoch = c->layout_epoch;
   return rc;
 }
 
@@ -1559,6 +1577,19 @@ static int ensure_current(clh_ctx* c) {
   return CLH_OK;
 }
 
+// the uploaded batch no longer matches the layouts it was built on: the
+// Legendre mode changed, or (for a batch on the context's own layout) any
+// layout setter ran since the upload.  The per-draw data-vector slices must
+// still follow the layouts' sizes.
+static bool batch_stale(const clh_ctx* c, const Batch& bt) {
+  if (c->batch_epoch != c->mode_epoch) return true;
+  if (bt.ctx_layout && (bt.lay_epoch != c->layout_epoch || !c->dl.valid)) return true;
+  if ((int)bt.lays.size() != bt.B || (int)bt.dv_off.size() != bt.B + 1) return true;
+  for (int i = 0; i < bt.B; i++)
+    if (bt.dv_off[i + 1] - bt.dv_off[i] != (long long)bt.lays[i]->dv_size()) return true;
+  return false;
+}
+
 // ============================================================== C ABI
 extern "C" {
 
@@ -1598,7 +1629,7 @@ int clh_set_legendre_mode(clh_ctx* c, int mode) {
   if (mode != CLH_LEGENDRE_FOLDED && mode != CLH_LEGENDRE_FULL) return fail(c, CLH_ERR_ARG, "bad mode %d", mode);
   if (mode != c->legendre_mode) {
     c->legendre_mode = mode;
-    c->dl.valid = false;
+    layout_changed(c);
     c->mode_epoch++;
     c->version++;
   }
@@ -1625,7 +1656,7 @@ int clh_init_ntable(clh_ctx* c, int n_ell, int n_a, int lmax, int hdi) {
   if (hdi != 0) return fail(c, CLH_ERR_ARG, "high_def_integration=%d not implemented (0 only)", hdi);
   c->L.n_ell = n_ell; c->L.n_a = n_a; c->L.lmax = lmax; c->L.hdi = hdi;
   c->L.kern_valid = false;
-  c->dl.valid = false;
+  layout_changed(c);
   c->version++;
   return CLH_OK;
 }
@@ -1649,7 +1680,7 @@ int clh_init_binning_real_space(clh_ctx* c, int ntheta, double tmin, double tmax
   }
   if (!c->L.mask.empty() && (int)c->L.mask.size() != c->L.dv_size()) c->L.mask.clear();
   c->L.kern_valid = false;
-  c->dl.valid = false;
+  layout_changed(c);
   c->version++;
   return CLH_OK;
 }
@@ -1659,6 +1690,7 @@ int clh_init_ia(clh_ctx* c, int model, int evol) {
   if (model != 0 && model != CLH_IA_TATT)
     return fail(c, CLH_ERR_ARG, "IA_MODEL=%d not implemented (NLA=0, TATT=1)", model);
   if (evol != 0 && evol != 2 && evol != 3) return fail(c, CLH_ERR_ARG, "IA redshift evolution %d not supported", evol);
+  if (model != c->L.ia_model || evol != c->L.ia_mode) layout_changed(c);
   c->L.ia_model = model;
   c->L.ia_mode = evol;
   c->version++;
@@ -1778,7 +1810,7 @@ int clh_init_probes(clh_ctx* c, const char* probes) {
   if (six != c->L.sixpt) {
     c->L.sixpt = six;
     c->L.kern_valid = false;
-    c->dl.valid = false;
+    layout_changed(c);
     c->L.mask.clear();
     c->version++;
   }
@@ -1797,7 +1829,7 @@ int clh_init_cmb_cross_correlation(clh_ctx* c, int lmin, int lmax, double fwhm_a
   c->L.cmb_lmax = lmax;
   if (win && nwin > 0) c->L.cmb_win.assign(win, win + nwin); else c->L.cmb_win.clear();
   c->L.kern_valid = false;
-  c->dl.valid = false;
+  layout_changed(c);
   c->version++;
   return CLH_OK;
 }
@@ -1828,7 +1860,7 @@ int clh_init_cmb_auto_bandpower(clh_ctx* c, int nbins, int lmin, int lmax, const
   L.kk_M.assign(binning_matrix, binning_matrix + nm);
   if (theory_offset) L.kk_off.assign(theory_offset, theory_offset + nbins); else L.kk_off.assign(nbins, 0.0);
   L.kk_alpha = alpha;
-  c->dl.valid = false;
+  layout_changed(c);
   if (L.sixpt) L.mask.clear();  // the data-vector size changes
   c->version++;
   return CLH_OK;
@@ -1850,6 +1882,7 @@ int clh_set_adopt_limber_gg(clh_ctx* c, int limber) {
   if (limber != 0 && limber != 1) return fail(c, CLH_ERR_ARG, "adopt_limber_gg must be 0 or 1");
   if (c->L.nonlimber != (limber == 0)) {
     c->L.nonlimber = limber == 0;
+    layout_changed(c);
     c->version++;
   }
   return CLH_OK;
@@ -1859,7 +1892,7 @@ int clh_set_source_sample(clh_ctx* c, const double* t, int nrows, int ntomo) {
   if (!c) return CLH_ERR_ARG;
   int rc = build_sample(c, c->L.src, t, nrows, ntomo, true);
   if (rc) return rc;
-  c->dl.valid = false;
+  layout_changed(c);
   c->version++;
   return CLH_OK;
 }
@@ -1869,7 +1902,7 @@ int clh_set_lens_sample(clh_ctx* c, const double* t, int nrows, int ntomo) {
   int rc = build_sample(c, c->L.lens, t, nrows, ntomo, false);
   if (rc) return rc;
   lens_zmean(c->L);
-  c->dl.valid = false;
+  layout_changed(c);
   c->version++;
   return CLH_OK;
 }
@@ -2004,7 +2037,7 @@ int clh_like_set_mask(clh_ctx* c, const int* mask, int n, const int ord[3]) {
     if (kept == 0) return fail(c, CLH_ERR_DOMAIN, "set_mask: the mask leaves no data points");
     c->L.mask = std::move(m);
   }
-  c->dl.valid = false;
+  layout_changed(c);
   c->version++;
   return CLH_OK;
 }
@@ -2090,6 +2123,7 @@ int clh_batch_chi2(clh_ctx* c, const int ord[3], const double* dv_device, double
   if (!c || !ord || !dv_device || !chi2_device) return CLH_ERR_ARG;
   Batch& bt = c->batch;
   if (bt.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
+  if (batch_stale(c, bt)) return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
   if (!bt.uniform) return fail(c, CLH_ERR_ARG, "chi2 needs a batch with one layout for every draw");
   return chi2_run(c, *bt.lays[0], bt.B, ord, dv_device, chi2_device, grad_device);
 }
@@ -2128,7 +2162,7 @@ int clh_init_binning_fourier(clh_ctx* c, int nells, int lmin, int lmax, int lmax
   const double logdl = (std::log((double)lmax) - std::log((double)lmin)) / (double)nells;
   for (int i = 0; i < nells; i++) c->L.ell_f[i] = std::exp(std::log((double)lmin) + (i + 0.5) * logdl);
   c->L.lmax_shear = lmax_shear;
-  c->dl.valid = false;  // like.ell also feeds the 6x2pt kk block
+  layout_changed(c);  // like.ell also feeds the 6x2pt kk block
   if (c->L.sixpt) c->L.mask.clear();
   c->version++;
   return CLH_OK;
@@ -2561,7 +2595,7 @@ static int prepare_checkpoint(clh_ctx* c, Batch& cb, bool& changed) {
 int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
   if (!c || !ord || !dv_device) return CLH_ERR_ARG;
   if (c->batch.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
-  if ((c->batch.ctx_layout && !c->dl.valid) || c->batch_epoch != c->mode_epoch)
+  if (batch_stale(c, c->batch))
     return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
   for (int i = 0; i < c->batch.B; i++)
     if ((i == 0 || c->batch.lays[i] != c->batch.lays[i - 1]) && !ord_ok(*c->batch.lays[i], ord))
@@ -2676,7 +2710,7 @@ int clh_set_tile_rows(clh_ctx* c, const int rows[4]) {
   for (int k = 0; k < 4; k++)
     if (rows[k] < 1 || rows[k] > CLH_TILE_ROWS) return fail(c, CLH_ERR_ARG, "tile rows %d not in 1..%d", rows[k], CLH_TILE_ROWS);
   for (int k = 0; k < 4; k++) c->tile_rows[k] = rows[k];
-  if (c->batch.B > 0) {
+  if (c->batch.B > 0 && !batch_stale(c, c->batch)) {  // a stale batch is refused by every batch call
     int rc = build_work(c, c->batch, c->tile_rows);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2695,7 +2729,7 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   if (!c) return CLH_ERR_ARG;
   Batch& bt = c->batch;
   if (bt.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
-  if ((bt.ctx_layout && !c->dl.valid) || c->batch_epoch != c->mode_epoch)
+  if (batch_stale(c, bt))
     return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
   HIPCHK(c, hipSetDevice(c->device));
   // the forward writes into the batch's own output buffer while tuning
@@ -2828,6 +2862,7 @@ int clh_batch_set_params(clh_ctx* c, const double* lnP, const double* src_dz, co
   if (!c) return CLH_ERR_ARG;
   Batch& bt = c->batch;
   if (bt.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
+  if (batch_stale(c, bt)) return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
   if (lnP && bt.nknz <= 0) return fail(c, CLH_ERR_ARG, "draws of the batch have different P(k,z) shapes");
   if (!bt.uniform) return fail(c, CLH_ERR_ARG, "per-draw parameter arrays need a batch with one layout");
   HIPCHK(c, hipSetDevice(c->device));
@@ -2843,7 +2878,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   Batch& bt = c->batch;
   if (bt.B <= 0) return fail(c, CLH_ERR_STATE, "no batch uploaded");
   if (bt.nknz <= 0) return fail(c, CLH_ERR_ARG, "draws of the batch have different P(k,z) shapes");
-  if ((bt.ctx_layout && !c->dl.valid) || c->batch_epoch != c->mode_epoch)
+  if (batch_stale(c, bt))
     return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
   if (!bt.uniform) return fail(c, CLH_ERR_ARG, "VJP needs a batch with one layout for every draw");
   if (bt.tatt) return fail(c, CLH_ERR_ARG, "VJP of TATT intrinsic alignments is not implemented");
@@ -3000,6 +3035,41 @@ static double table_lookup(const clh_ctx* c, int col, bool ss, double l, bool bb
   return (r - i) * (f[i + 1] - f[i]) + f[i];
 }
 
+// w(theta) of the current draw with the Limber (limber = 1) or non-Limber
+// (0) C_gg below ell 150, [nl][ntheta]: the cached data vector when the
+// context's adopt_limber_gg selects that path, else one evaluation of the
+// other variant on a copy of the layout (cached by the context version)
+static const double* w_block(clh_ctx* c, int limber) {
+  if (limber == (c->L.nonlimber ? 0 : 1)) {
+    if (ensure_current(c)) return nullptr;
+    return c->cache_raw.data() + (size_t)(2 * c->L.n_ss() + c->L.n_gs()) * c->L.ntheta;
+  }
+  if (c->alt_w_version == c->version && c->alt_w_limber == limber) return c->alt_w.data();
+  if (check_layout(c, c->L) || ensure_device(c)) return nullptr;
+  const bool full = c->legendre_mode == CLH_LEGENDRE_FULL;
+  if (!c->L.kern_valid || (full && c->L.kfull.empty())) build_kernels(c->L, full);
+  if (!c->dl.valid && upload_layout(c, c->L, c->dl)) return nullptr;
+  c->alt_L = c->L;  // the device copy of L serves both variants
+  c->alt_L.nonlimber = limber == 0;
+  const int ord[3] = {0, 1, 2};
+  std::vector<const Draw*> v{&c->cur};
+  const std::vector<const Layout*> lays{&c->alt_L};
+  const std::vector<const DevLayout*> devs{&c->dl};
+  if (build_batch(c, c->scratch, v, lays, devs, 0, ord, nullptr, nullptr)) return nullptr;
+  if (run_batch(c, c->scratch, true)) return nullptr;
+  const int nw = c->L.lens.nbin * c->L.ntheta;
+  c->alt_w.resize(nw);
+  const double* src = c->scratch.desc[0].dv + (size_t)(2 * c->L.n_ss() + c->L.n_gs()) * c->L.ntheta;
+  if (hipMemcpyAsync(c->alt_w.data(), src, nw * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    fail(c, CLH_ERR_HIP, "w_gg_tomo: copy of w(theta) failed");
+    return nullptr;
+  }
+  c->alt_w_version = c->version;
+  c->alt_w_limber = limber;
+  return c->alt_w.data();
+}
+
 extern "C" {
 
 clh_ctx* clh_default(void) {
@@ -3039,15 +3109,16 @@ double w_gg_tomo(const int nt, const int ni, const int nj, const int limber) {
   clh_ctx* c = clh_default();
   if (!c) return nanv();
   if (limber != 0 && limber != 1) { fail(c, CLH_ERR_ARG, "limber must be 0 or 1"); return nanv(); }
-  if (clh_set_adopt_limber_gg(c, limber)) return nanv();
   if (nt < 0 || nt > c->L.ntheta - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number nt = %d", nt); return nanv(); }
   if (ni < 0 || ni > c->L.lens.nbin - 1 || nj < 0 || nj > c->L.lens.nbin - 1) {
     fail(c, CLH_ERR_ARG, "error in selecting bin number (ni,nj) = [%d,%d]", ni, nj);
     return nanv();
   }
   if (ni != nj) { fail(c, CLH_ERR_ARG, "ni != nj tomography not supported"); return nanv(); }
-  if (ensure_current(c)) return nanv();
-  return c->cache_raw[(2 * c->L.n_ss() + c->L.n_gs()) * c->L.ntheta + ni * c->L.ntheta + nt];
+  // `limber` picks the path of this evaluation only (cosmo2D.c:845-857); the
+  // context's adopt_limber_gg, which the data vectors follow, is left as set
+  const double* w = w_block(c, limber);
+  return w ? w[ni * c->L.ntheta + nt] : nanv();
 }
 
 double C_ss_tomo_limber(const double l, const int ni, const int nj, const int EE) {
@@ -3089,9 +3160,12 @@ double C_ss_tomo_limber_nointerp(const double l, const int ni, const int nj, con
   if (!c) return nanv();
   const int ns = c->L.src.nbin;
   if (ni < 0 || ni > ns - 1 || nj < 0 || nj > ns - 1) { fail(c, CLH_ERR_ARG, "invalid bin input (ni, nj) = (%d, %d)", ni, nj); return nanv(); }
-  if (init == 1 || EE != 1) return 0.0;
+  if (init == 1) return 0.0;
+  // EE = 0: the B mode, zero under NLA (cosmo2D.c:1488-1504) and integrated
+  // under TATT (:1562-1601)
+  if (EE != 1 && c->L.ia_model != CLH_IA_TATT) return 0.0;
   std::vector<double> out(c->L.n_ss());
-  if (clh_cl_nointerp(c, 0, &l, 1, out.data())) return nanv();
+  if (clh_cl_nointerp(c, EE != 1 ? 3 : 0, &l, 1, out.data())) return nanv();
   return out[n_shear_idx(ns, ni, nj)];
 }
 

@@ -40,13 +40,16 @@ extern "C" {
 #define CLH_LEGENDRE_FULL 1
 
 /* Arithmetic of the Limber and folded-Legendre stages.  FP64 is the plain
- * path (the reference's arithmetic, default).  FP32: Limber integrands and
- * Kahan-compensated node sums in FP32 (table searches, chi(z)/growth tables,
- * lens efficiencies and Legendre kernels stay FP64-built), Legendre
- * contraction on FP32 MFMA.  BF16X3: as FP32, with the Legendre contraction
- * on BF16 MFMA using a 3-term hi/lo split of both operands (hi*hi + hi*lo +
- * lo*hi), FP32 accumulation and power-of-two scaling of 32-row ell bands.
- * FULL Legendre mode always contracts in FP64. */
+ * path (the reference's arithmetic, default).  FP32: the ss and gg Limber
+ * integrands take P(k,z) as the FP32 exp of the FP64 bilinear ln P; the node
+ * sums stay FP64 (ss pair sums on FP64 MFMA), and the gs rows, table
+ * searches, chi(z)/growth tables, lens efficiencies and Legendre kernels stay
+ * FP64; Legendre contraction on FP32 MFMA.  BF16X3: as FP32, with the
+ * Legendre contraction on BF16 MFMA: each operand split into three BF16
+ * parts a0 + a1 + a2 (each the nearest BF16 of what the previous left), the
+ * six products a2b0 + a1b1 + a0b2 + a1b0 + a0b1 + a0b0 summed smallest first
+ * in FP32, with power-of-two scaling of 32-row ell bands.  FULL Legendre mode
+ * always contracts in FP64. */
 #define CLH_PRECISION_FP64 0
 #define CLH_PRECISION_FP32 1
 #define CLH_PRECISION_BF16X3 2

@@ -1,0 +1,151 @@
+"""Context state across calls: what one entry point leaves behind must not
+change what another computes.
+
+- A batch uploaded on the context's own layout is refused (CLH_ERR_STATE,
+  before any kernel runs) once a layout setter changed that layout, even
+  after a single-draw call re-uploaded the layout's device copy.
+- w_gg_tomo(..., limber) selects the path of that evaluation only: the
+  data vector keeps following adopt_limber_gg.
+- C_ss_tomo_limber_nointerp(l, ni, nj, EE = 0, 0) is the B mode: zero under
+  NLA, the integrated B mode under TATT (cosmo2D.c:1562-1601).
+- The autograd front end: an input passed as None takes the uploaded value,
+  so a backward after another forward uses this forward's parameters.
+
+Bar: 1e-6 relative against the FP64 oracle (floor 1e-3 of the row's largest
+magnitude), gradients identical to those of a fresh forward.
+"""
+import numpy as np
+import pytest
+
+from cosmolike_amd import Cosmolike, CosmolikeError, synth
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-6
+ERR_STATE = -3
+
+
+def rel_err(gpu, ref):
+    gpu = np.atleast_2d(gpu)
+    ref = np.atleast_2d(ref)
+    floor = 1e-3 * np.abs(ref).max(axis=1, keepdims=True)
+    den = np.maximum(np.abs(ref), floor)
+    den[den == 0] = 1.0
+    return float((np.abs(gpu - ref) / den).max())
+
+
+def test_batch_refused_after_layout_change():
+    import torch
+    draws = [synth.full_3x2pt_problem(cos) for cos, *_r in synth.random_draws(2, seed=21, ragged=False)]
+    c = Cosmolike().load(draws[0])
+    c.batch_upload(draws)
+    n = c.datavector_size()
+    out = torch.zeros(2 * n, dtype=torch.float64, device="cuda:0")
+    c.batch_forward(out.data_ptr())
+    c.synchronize()
+    # a different source sample (fewer bins), then a single-draw call that
+    # uploads the new layout's device copy
+    small = synth.ragged_3x2pt_problem(synth.random_draws(1, seed=22, ragged=False)[0][0], 4, 3, 20)
+    c.load(small)
+    c.xi_pm()
+    sentinel = torch.full((2 * n,), 7.0, dtype=torch.float64, device="cuda:0")
+    with pytest.raises(CosmolikeError) as e:
+        c.batch_forward(sentinel.data_ptr())
+    assert e.value.code == ERR_STATE
+    g = torch.zeros(2 * n, dtype=torch.float64, device="cuda:0")
+    with pytest.raises(CosmolikeError) as e:
+        c.batch_vjp(g.data_ptr())
+    assert e.value.code == ERR_STATE
+    with pytest.raises(CosmolikeError) as e:
+        c.autotune()
+    assert e.value.code == ERR_STATE
+    torch.cuda.synchronize()
+    assert torch.all(sentinel == 7.0)
+    # a new upload on the new layout works
+    c.batch_upload([small])
+    out2 = torch.zeros(c.datavector_size(), dtype=torch.float64, device="cuda:0")
+    c.batch_forward(out2.data_ptr())
+    c.synchronize()
+    assert rel_err(out2.cpu().numpy(), O.Oracle().load(small).datavector()) < RTOL
+
+
+def test_w_gg_tomo_limber_argument_leaves_datavector():
+    cos = synth.random_draws(1, seed=3, ragged=False)[0][0]
+    p = synth.nonlimber_problem(cos)
+    c = Cosmolike.default()
+    c.load(p)
+    c.set_adopt_limber_gg(0)  # the data vector follows the non-Limber w
+    o = O.Oracle().load(p)
+    ref_nl = o.datavector()
+    before = c.compute_3x2pt_real()
+    assert rel_err(before, ref_nl) < RTOL
+    nt = p.ntheta
+    L = c.L
+    # the Limber w through the scalar entry ...
+    pl = synth.nonlimber_problem(cos)
+    pl.limber_gg = 1
+    w_limber = O.Oracle().load(pl).datavector()[-p.nl * nt:].reshape(p.nl, nt)
+    got = np.array([[L.w_gg_tomo(t, i, i, 1) for t in range(nt)] for i in range(p.nl)])
+    assert rel_err(got, w_limber) < RTOL
+    # ... leaves the context's choice: the data vector is still non-Limber
+    after = c.compute_3x2pt_real()
+    assert np.array_equal(after, before)
+    w_nl = ref_nl[-p.nl * nt:].reshape(p.nl, nt)
+    got0 = np.array([[L.w_gg_tomo(t, i, i, 0) for t in range(nt)] for i in range(p.nl)])
+    assert rel_err(got0, w_nl) < RTOL
+    assert rel_err(w_nl, w_limber) > 1e-3  # the two paths really differ
+    c.set_adopt_limber_gg(1)
+
+
+def test_b_mode_nointerp_scalar():
+    cos = synth.random_draws(1, seed=71, ragged=False)[0][0]
+    p = synth.tatt_problem(cos)
+    c = Cosmolike.default()
+    c.load(p)
+    L = c.L
+    ells = np.array([3.0, 12.0, 150.0, 2500.0])
+    ref = O.Oracle().load(p).cl_ss_bb_nointerp(ells)
+    assert np.abs(ref).max() > 0
+    q = 0
+    for i in range(p.ns):
+        for j in range(i, p.ns):
+            if (i + j) % 3 == 0:
+                got = np.array([L.C_ss_tomo_limber_nointerp(l, i, j, 0, 0) for l in ells])
+                assert rel_err(got, ref[q]) < RTOL, (i, j)
+            q += 1
+    # under NLA the B mode is zero
+    c.load(synth.full_3x2pt_problem(cos))
+    assert L.C_ss_tomo_limber_nointerp(150.0, 0, 1, 0, 0) == 0.0
+
+
+def test_autograd_none_input_uses_uploaded_value():
+    import torch
+    from cosmolike_amd.autograd import projection_3x2pt
+    draws = [synth.full_3x2pt_problem(cos) for cos, *_r in synth.random_draws(2, seed=31, ragged=False)]
+    c = Cosmolike().load(draws[0])
+    c.batch_upload(draws)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.array(a), dtype=torch.float64, device=dev)
+    lnP0 = t([d.lnP.ravel() for d in draws])
+    src = t([d.src_dz for d in draws])
+    v = t(np.random.default_rng(5).normal(size=(2, c.datavector_size())))
+    # forward A: lnP not given (the uploaded table), src_dz differentiable
+    a_src = src.clone().requires_grad_(True)
+    out_a = projection_3x2pt(c, None, a_src, None, None, None, None)
+    # forward B with another lnP table in between
+    out_b = projection_3x2pt(c, lnP0 + 0.05, src.clone(), None, None, None, None)
+    (out_a * v).sum().backward()
+    g_after = a_src.grad.clone()
+    # the same forward with lnP passed explicitly, alone
+    r_src = src.clone().requires_grad_(True)
+    out_r = projection_3x2pt(c, lnP0, r_src, None, None, None, None)
+    (out_r * v).sum().backward()
+    torch.cuda.synchronize()
+    assert torch.equal(out_a, out_r)
+    assert not torch.equal(out_a, out_b)
+    # equal up to the order of the adjoint's atomic sums (checkpointed vs
+    # recomputed forward state)
+    ga, gr = g_after.cpu().numpy(), r_src.grad.cpu().numpy()
+    np.testing.assert_allclose(ga, gr, rtol=1e-9, atol=1e-11 * np.abs(gr).max())
