@@ -225,11 +225,31 @@ struct Draw {  // one draw's cosmology + nuisance (host copy)
   Draw() { for (int i = 0; i < CLH_MAXB; i++) lens_st[i] = 1.0; }
 };
 
+// While set (clh_compute_3x2pt_real_scratch), DBuf::ensure takes its memory
+// from this bump allocator over caller-provided device memory instead of
+// hipMalloc; such slices are never freed.
+thread_local ClhArena* t_arena = nullptr;
+
 template <class T>
 struct DBuf {  // device buffer that grows
   T* p = nullptr;
   size_t cap = 0;
+  bool ext = false;  // a slice of an arena: not owned
   hipError_t ensure(size_t n) {
+    if (t_arena) {
+      if (p && !ext) return hipErrorInvalidValue;  // an owned buffer is never traded for scratch
+      const size_t bytes = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~size_t(255);
+      if (t_arena->used + bytes > t_arena->cap) {
+        t_arena->exhausted = true;
+        return hipErrorOutOfMemory;
+      }
+      p = (T*)(t_arena->base + t_arena->used);
+      t_arena->used += bytes;
+      cap = n;
+      ext = true;
+      return hipSuccess;
+    }
+    if (ext) { p = nullptr; cap = 0; ext = false; }
     if (n <= cap) return hipSuccess;
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -238,7 +258,7 @@ struct DBuf {  // device buffer that grows
     if (e == hipSuccess) cap = n;
     return e;
   }
-  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  void release() { if (p && !ext) (void)hipFree(p); p = nullptr; cap = 0; ext = false; }
 };
 
 struct DevLayout {  // device copy of one survey layout
@@ -363,6 +383,7 @@ struct clh_ctx {
   // (ctx_layout) is refused once it differs from the batch's lay_epoch
   uint64_t layout_epoch = 0;
   Batch single, batch, scratch;
+  Batch arena;  // clh_compute_3x2pt_real_scratch: every buffer a slice of the caller's scratch
   bool timing = false;
   bool use_graph = false;             // replay the batch forward from a hipGraph
   hipGraphExec_t gexec = nullptr;     // captured forward of `batch`
@@ -1611,7 +1632,7 @@ void clh_destroy(clh_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
   }
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  c->single.release(); c->batch.release(); c->scratch.release();
+  c->single.release(); c->batch.release(); c->scratch.release(); c->arena.release();
   c->dl.release();
   c->rl.clear();
   c->like_data.release(); c->like_icov.release(); c->like_mask.release();
@@ -2235,6 +2256,53 @@ int clh_compute_3x2pt_real(clh_ctx* c, const int ord[3], double* dv, int n) {
   c->cached_version = 0;  // the single batch now holds a calibrated vector
   return CLH_OK;
 }
+
+}  // extern "C"
+
+// clh_compute_3x2pt_real with every device buffer of the draw (tables,
+// lens-efficiency tables, nodes, C_ell rows, descriptors, work lists, data
+// vector) carved from `scratch` (bytes long, 256-byte aligned): no device
+// allocation and no free.  The layout's device copy must already be current
+// (an earlier clh_compute_3x2pt_real after the last layout setter made it).
+int clh_compute_3x2pt_real_scratch(clh_ctx* c, const int ord[3], double* dv, int n, void* scratch, size_t bytes) {
+  if (!c || !ord || !dv || !scratch) return CLH_ERR_ARG;
+  if (n != c->L.dv_size()) return fail(c, CLH_ERR_ARG, "dv size %d != %d", n, c->L.dv_size());
+  if (!ord_ok(c->L, ord))
+    return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");
+  int rc = check_layout(c, c->L);
+  if (rc) return rc;
+  if (c->L.sixpt) return fail(c, CLH_ERR_ARG, "allocation-free compute: 6x2pt is not supported");
+  if (c->L.nonlimber && c->L.lens.nbin > 0)
+    return fail(c, CLH_ERR_ARG, "allocation-free compute: the non-Limber w is not supported");
+  const bool full = c->legendre_mode == CLH_LEGENDRE_FULL;
+  if (!c->own_stream || !c->L.kern_valid || (full && c->L.kfull.empty()) || !c->dl.valid)
+    return fail(c, CLH_ERR_STATE,
+                "allocation-free compute: the layout's device copy is not current (call "
+                "clh_compute_3x2pt_real once after the last layout setter)");
+  ClhArena a{(char*)scratch, bytes, 0, false};
+  const std::vector<const Draw*> v{&c->cur};
+  const std::vector<const Layout*> lays{&c->L};
+  const std::vector<const DevLayout*> devs{&c->dl};
+  t_arena = &a;
+  rc = build_batch(c, c->arena, v, lays, devs, 1, ord, nullptr, nullptr);
+  t_arena = nullptr;
+  if (rc) {
+    if (a.exhausted)
+      return fail(c, CLH_ERR_ARG,
+                  "allocation-free compute: this layout and draw need more than the %zu bytes of scratch", bytes);
+    return rc;
+  }
+  if ((rc = run_batch(c, c->arena, true))) return rc;
+  HIPCHK(c, hipMemcpyAsync(dv, c->arena.desc[0].dv, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLH_OK;
+}
+
+int clh_context_device(const clh_ctx* c) { return c ? c->device : -1; }
+
+void clh_record_error(clh_ctx* c, int code, const char* msg) { (void)fail(c, code, "%s", msg); }
+
+extern "C" {
 
 // compute_Mx2pt_N_masked<0,6>(ord) (generic_interface.hpp:1137-1159): the
 // 3x2pt blocks, w_gk, w_ks (x (1 + m)) and C_kk at like.ell, placed by ord

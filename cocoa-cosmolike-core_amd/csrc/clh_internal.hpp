@@ -81,6 +81,19 @@ inline ClhNlOff clh_nl_off(int nl) {
   return o;
 }
 
+// Bump allocator over device memory the caller provides (the allocation-free
+// entry point's scratch array)
+struct ClhArena {
+  char* base;
+  size_t cap, used;
+  bool exhausted;
+};
+struct clh_ctx;
+// clh_compute_3x2pt_real with every device buffer carved from `scratch`
+int clh_compute_3x2pt_real_scratch(clh_ctx* c, const int ord[3], double* dv, int n, void* scratch, size_t bytes);
+int clh_context_device(const clh_ctx* c);
+void clh_record_error(clh_ctx* c, int code, const char* msg);
+
 // Everything one draw needs on the device.  Plain data; lives in an array in
 // device memory and is read with uniform (scalar) loads.
 struct DrawParams;

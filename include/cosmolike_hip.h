@@ -221,6 +221,18 @@ int clh_datavector_size(clh_ctx* ctx);
  * all-ones mask: xi+ | xi- | gamma_t | w blocks placed by ord, shear
  * calibration applied; dv has clh_datavector_size() entries (host memory) */
 int clh_compute_3x2pt_real(clh_ctx* ctx, const int ord[3], double* dv, int n);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): clh_compute_3x2pt_real with every device buffer of the
+ * draw (its tables, lens-efficiency tables, nodes, C_ell rows, descriptors,
+ * work lists and the data vector) in a fixed 16 MiB array per GPU inside the
+ * library; it calls no hipMalloc or hipFree.  The layout's device copy must
+ * be current: call clh_compute_3x2pt_real once after the last layout setter
+ * (CLH_ERR_STATE otherwise).  A layout and draw that need more than 16 MiB
+ * are refused with CLH_ERR_ARG; 6x2pt and the non-Limber w are refused.
+ * Calls that overlap on one GPU (several threads or contexts) race on the
+ * shared array. */
+int clh_compute_3x2pt_real_noalloc(clh_ctx* ctx, const int ord[3], double* dv, int n);
 /* per-probe arrays as the reference's static caches hold them (q = pair*Ntheta + i),
  * without calibration: xi_pm_tomo (cosmo2D.c:353), w_gammat_tomo (:562),
  * w_gg_tomo (:739; Limber or not as clh_set_adopt_limber_gg selects) */
