@@ -502,17 +502,32 @@ __device__ __forceinline__ ClhGgTabs clh_gg_tabs(const DrawDesc& d, int zl, doub
 template <bool AHEAD = true>
 struct ClhPkNodeT {
   int i, j, nk, nz;
-  double dy, lo, hi, inv_k, p00, p01, p10, p11;
+  // cell state: lo / hi bound the walk test (-inf / +inf at the table ends,
+  // where clh_bs extrapolates), x0 = lk[i], inv_k = 1 / (lk[i+1] - lk[i]),
+  // L0 / L1 the ln P interpolated in z at lk[i] / lk[i+1], so that
+  // (1-dx)(1-dy)p00 + (1-dx)dy p01 + dx(1-dy)p10 + dx dy p11 = L0 + dx (L1 - L0)
+  double dy, lo, hi, x0, inv_k, L0, L1;
   double nhi, n0, n1;  // lk[i + 2], ln P[i + 2][j], ln P[i + 2][j + 1] (i + 2 <= nk - 1)
   const double* __restrict__ lnP;
 
+  __device__ __forceinline__ double zint(const double* __restrict__ P) const {
+    return (1 - dy) * P[0] + dy * P[1];
+  }
   __device__ __forceinline__ void ahead(const double* __restrict__ lk) {
     if (AHEAD && i + 2 <= nk - 1) {
+      // used only at the next step, so their latency overlaps the rows between
       nhi = lk[i + 2];
       const double* __restrict__ P = lnP + (size_t)(i + 2) * nz + j;
       n0 = P[0];
       n1 = P[1];
     }
+  }
+  __device__ __forceinline__ void bounds(const double* __restrict__ lk) {
+    x0 = lk[i];
+    const double x1 = lk[i + 1];
+    inv_k = 1.0 / (x1 - x0);
+    lo = i == 0 ? -INFINITY : x0;
+    hi = i == nk - 2 ? INFINITY : x1;
   }
 
   __device__ __forceinline__ void init(const DrawDesc& d, int j_, double dy_) {
@@ -524,14 +539,10 @@ struct ClhPkNodeT {
     lnP = d.lnP;
   }
   __device__ __forceinline__ void set_cell(const double* __restrict__ lk) {
-    lo = lk[i];
-    hi = lk[i + 1];
-    inv_k = 1.0 / (hi - lo);
+    bounds(lk);
     const double* __restrict__ P = lnP + (size_t)i * nz + j;
-    p00 = P[0];
-    p01 = P[1];
-    p10 = P[nz];
-    p11 = P[nz + 1];
+    L0 = zint(P);
+    L1 = zint(P + nz);
     ahead(lk);
   }
   // ln P at log10 k; dx: the log10 k weight of the cell (i, j)
@@ -539,25 +550,24 @@ struct ClhPkNodeT {
     if (i < 0) {
       i = clh_bs(lk, nk, log10k);
       set_cell(lk);
-    } else if (!((log10k >= lo || i == 0) && (log10k < hi || i == nk - 2))) {
-      if (AHEAD && i + 1 <= nk - 2 && log10k >= hi && (log10k < nhi || i + 1 == nk - 2)) {
+    } else if (!(log10k >= lo && log10k < hi)) {
+      if (AHEAD && log10k >= hi && (i + 1 == nk - 2 || log10k < nhi)) {
         // one cell up: the clh_walk index is i + 1
         i++;
-        lo = hi;
-        hi = nhi;
-        inv_k = 1.0 / (hi - lo);
-        p00 = p10;
-        p01 = p11;
-        p10 = n0;
-        p11 = n1;
+        x0 = hi;  // finite: the old cell was not the last
+        lo = x0;
+        hi = i == nk - 2 ? INFINITY : nhi;
+        inv_k = 1.0 / (nhi - x0);
+        L0 = L1;
+        L1 = (1 - dy) * n0 + dy * n1;
         ahead(lk);
       } else {
         i = clh_walk(lk, nk, log10k, i);
         set_cell(lk);
       }
     }
-    dx = (log10k - lo) * inv_k;
-    return (1 - dx) * (1 - dy) * p00 + (1 - dx) * dy * p01 + dx * (1 - dy) * p10 + dx * dy * p11;
+    dx = (log10k - x0) * inv_k;
+    return L0 + dx * (L1 - L0);
   }
 };
 
@@ -604,77 +614,134 @@ struct ClhBracket {
 
 template <bool ADJ, bool AHEAD = true>
 struct ClhGgNode {
-  double fK, fac, WG, WM, b1, bmag, l10fk, t0, t0d, inv_st, dzl, zm, chi_amin;
-  ClhBracket bc, bz, bg, bn;           // chi_v, chi_z, G_z, lens n(z) knots
-  double cz_lo, cz_hi, inv_c;          // chi(z) interval payload
-  double zjj, inv_z, up, down;         // dchi/dz stencil payload
-  double gv_lo, slG;                   // G(z) interval payload
-  double cf0, cf1, cf2, cf3;           // lens spline piece
+  // row-independent values of the node
+  double fK, wgb, wmb, t0, t0d, c3fac, l10fk, chi_amin;
+  double inv_st, zs0, zlo, zhi;  // lens n(z) argument zs = z1 inv_st + zs0; first and last knot
+  // The a_1 side walks over the chi(z) table (a_chi), its z column (the
+  // dchi/dz stencil), the G(z) table and the lens n(z) knots.  Per table the
+  // current interval j, the bounds of the walk test (-inf / +inf at the table
+  // ends, where the reference's bisection extrapolates) and the interval's
+  // coefficients, read from the draw's interval records (DrawDesc::crec,
+  // grec) when the interval changes.  Successive rows move chi_1 down, so a
+  // change is almost always one interval down: one record load, no search.
+  int jc, jz, jg, jn, chi_n, G_n, l_rows, same;
+  const double* __restrict__ crec;
+  const double* __restrict__ grec;
+  double clo, chi_, czc, zA, zS;         // chi interval: z1 = zA + c1v zS; czc = z at its lower end
+  double zlo_, zhi_, D0, D1;             // z interval: dchi/dz = D0 + z1 D1
+  double glo, ghi, G0, slG;              // G interval (own bounds only off the chi grid): G = G0 + z1 slG
+  double nlo, nhi_, cf0, cf1, cf2, cf3;  // lens spline piece, coefficients / stretch
   ClhPkNodeT<AHEAD> pk;
-  int zl, jj;
 
-  __device__ __forceinline__ void set_c(const ClhGgTabs& T) {
-    cz_lo = T.cz[bc.j];
-    cz_hi = T.cz[bc.j + 1];
-    inv_c = 1.0 / (bc.hi - bc.lo);
+  __device__ __forceinline__ void rec_z(const double* __restrict__ r) {  // z interval jz from its record
+    D0 = r[4];
+    D1 = r[5];
+    if (same) {
+      G0 = r[6];
+      slG = r[7];
+    }
   }
-  __device__ __forceinline__ void set_z(const DrawDesc& d, const ClhGgTabs& T) {
-    jj = bz.j > d.chi_n - 3 ? d.chi_n - 3 : bz.j;
-    const double* zz = T.cz;
-    const double* cc = T.cv;
-    zjj = zz[jj];
-    inv_z = 1.0 / (zz[jj + 1] - zz[jj]);
-    up = (cc[jj + 2] - cc[jj]) / (zz[jj + 2] - zz[jj]);
-    down = (jj > 0) ? (cc[jj + 1] - cc[jj - 1]) / (zz[jj + 1] - zz[jj - 1]) : (cc[jj + 1] - cc[jj]) * inv_z;
+  // chi interval jc and its z interval (the same index unless rounding put
+  // z1 on the next knot; the per-row z test catches that) from the records
+  __device__ __forceinline__ void set_c() {
+    const double* __restrict__ r = crec + 8 * (size_t)jc;
+    clo = jc == 0 ? -INFINITY : r[0];
+    chi_ = jc == chi_n - 2 ? INFINITY : r[8];
+    czc = r[1];
+    zA = r[2];
+    zS = r[3];
+    jz = jc;
+    zlo_ = jc == 0 ? -INFINITY : czc;
+    zhi_ = jc == chi_n - 2 ? INFINITY : r[9];
+    rec_z(r);
   }
-  __device__ __forceinline__ void set_g(const ClhGgTabs& T) {
-    gv_lo = T.gv[bg.j];
-    slG = (T.gv[bg.j + 1] - gv_lo) / (bg.hi - bg.lo);
+  __device__ __forceinline__ void move_c(const ClhGgTabs& T, double c1v) {
+    if (c1v < clo && jc > 0) {  // one interval down: the upper bounds are the old lower ones
+      const double* __restrict__ r = crec + 8 * (size_t)(jc - 1);
+      const double lo1 = r[0];
+      if (jc == 1 || c1v >= lo1) {
+        chi_ = clo;
+        zhi_ = czc;
+        jc--;
+        clo = jc == 0 ? -INFINITY : lo1;
+        czc = r[1];
+        zA = r[2];
+        zS = r[3];
+        jz = jc;
+        zlo_ = jc == 0 ? -INFINITY : czc;
+        rec_z(r);
+        return;
+      }
+    }
+    jc = clh_walk(T.cv, chi_n, c1v, jc);
+    set_c();
+  }
+  __device__ __forceinline__ void move_z(const ClhGgTabs& T, double z1) {
+    jz = clh_walk(T.cz, chi_n, z1, jz);
+    const double* __restrict__ r = crec + 8 * (size_t)jz;
+    zlo_ = jz == 0 ? -INFINITY : r[1];
+    zhi_ = jz == chi_n - 2 ? INFINITY : r[9];
+    rec_z(r);
+  }
+  __device__ __forceinline__ void set_g() {
+    const double* __restrict__ r = grec + 3 * (size_t)jg;
+    glo = jg == 0 ? -INFINITY : r[0];
+    ghi = jg == G_n - 2 ? INFINITY : r[3];
+    G0 = r[1];
+    slG = r[2];
   }
   __device__ __forceinline__ void set_n(const ClhGgTabs& T) {
-    const double* cf = T.lcf + 4 * bn.j;
-    cf0 = cf[0];
-    cf1 = cf[1];
-    cf2 = cf[2];
-    cf3 = cf[3];
+    nlo = T.lzc[jn];
+    nhi_ = T.lzc[jn + 1];
+    const double* cf = T.lcf + 4 * jn;
+    cf0 = cf[0] * inv_st;
+    cf1 = cf[1] * inv_st;
+    cf2 = cf[2] * inv_st;
+    cf3 = cf[3] * inv_st;
   }
 
   // o: the node block (a, fK, fac, W_gal, W_mag, then the search results of
   // k_nodes: P(k,z) z index and weight, log10(chi c/H0), the a_0 side of W_RSD
   // and the table indices at a_0 from which the walks start)
   __device__ __forceinline__ void init(const DrawDesc& d, const ClhGgTabs& T, const double* __restrict__ o,
-                                      int p, int zl_) {
+                                      int p, int zl) {
     constexpr int np = CLH_NP_GG;
-    zl = zl_;
+    constexpr double c3 = 1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
+    chi_n = d.chi_n;
+    G_n = d.G_n;
+    l_rows = d.l_rows;
+    same = d.zg_same;
+    crec = d.crec;
+    grec = d.grec;
     fK = o[np + p];
-    fac = o[2 * np + p];
-    WG = o[3 * np + p];
-    WM = o[4 * np + p];
-    b1 = d.par->b1[zl];
-    bmag = d.par->bmag[zl];
+    c3fac = c3 * o[2 * np + p];
+    wgb = o[3 * np + p] * d.par->b1[zl];
+    wmb = o[4 * np + p] * d.par->bmag[zl];
     const double* lut = o + 5 * np;
     pk.init(d, (int)lut[p], lut[np + p]);
     l10fk = lut[2 * np + p];
-    bc.set(T.cv, (int)lut[3 * np + p]);
-    bz.set(T.cz, (int)lut[4 * np + p]);
-    bg.set(T.gz, (int)lut[5 * np + p]);
-    bn.set(T.lzc, (int)lut[6 * np + p]);
+    jc = (int)lut[3 * np + p];
+    jg = (int)lut[5 * np + p];
+    jn = (int)lut[6 * np + p];
     t0 = lut[7 * np + p];
     t0d = ADJ ? lut[8 * np + p] : 0.0;
     inv_st = 1.0 / d.par->lens_st[zl];
-    dzl = d.par->lens_dz[zl];
-    zm = d.l_zmean[zl];
+    const double zm = d.l_zmean[zl];
+    zs0 = (-d.par->lens_dz[zl] - zm) * inv_st + zm;
+    zlo = d.l_zc[0];
+    zhi = d.l_zc[l_rows - 1];
     chi_amin = d.chi_amin;
-    set_c(T);
-    set_z(d, T);
-    set_g(T);
+    set_c();
+    if (!same) set_g();
     set_n(T);
   }
 
   // false: chi_1 beyond chi(a_min), integrand zero (cosmo2D.c:2512-2514).
   // t: the bracket; pf = P fac; dx: log10 k weight of the P cell (pk.i, pk.j).
   // T: the draw's tables (staged in LDS when they fit).  F32: P(k,z) as expf
-  // of the FP64 bilinear value.
+  // of the FP64 bilinear value.  Each interval is the one the reference's
+  // bisection finds; inside it the interpolations are the interval's linear
+  // (n(z): cubic) pieces.
   template <bool F32 = false>
   __device__ __forceinline__ bool eval(const DrawDesc& d, const ClhGgTabs& T, const ClhGgRow& R,
                                        double& t, double& pf, double& dwr, double& dx) {
@@ -682,35 +749,37 @@ struct ClhGgNode {
     if (chi_1 > chi_amin) return false;
     // a_chi(chi_1) (cosmo3D.c:109-133)
     const double c1v = chi_1 * CLH_COVERH0;
-    if (bc.move(T.cv, d.chi_n, c1v)) set_c(T);
-    const double z1 = cz_lo + (c1v - bc.lo) * inv_c * (cz_hi - cz_lo);
+    if (!(c1v >= clo && c1v < chi_)) move_c(T, c1v);
+    const double z1 = zA + c1v * zS;
     // dchi/dz at z1 (cosmo3D.c:64-81 stencils)
-    if (bz.move(T.cz, d.chi_n, z1)) set_z(d, T);
-    const double Dz = down + (z1 - zjj) * inv_z * (up - down);
-    // G(z1) and f_growth (cosmo3D.c:200-228)
-    if (bg.move(T.gz, d.G_n, z1)) set_g(T);
-    const double G = gv_lo + (z1 - bg.lo) * slG;
-    // hoverh0 * f_growth = (c/H0)/(dchi/dz) * (1 - G'(1+z)/G)
-    const double hf1 = CLH_COVERH0 * (G - slG * (1.0 + z1)) / (Dz * G);
+    if (!(z1 >= zlo_ && z1 < zhi_)) move_z(T, z1);
+    // G(z1) and f_growth (cosmo3D.c:200-228): hoverh0 f_growth =
+    // (c/H0) / (dchi/dz) (1 - G'(1+z)/G), where G - G'(1+z) = G0 - slG on the interval
+    if (!same && !(z1 >= glo && z1 < ghi)) {
+      jg = clh_walk(T.gz, G_n, z1, jg);
+      set_g();
+    }
+    const double hf1 = CLH_COVERH0 * (G0 - slG) / ((D0 + z1 * D1) * (G0 + z1 * slG));
     // nz_lens(z1) (redshift_spline.c:736-762) and its shift derivative
     double n1 = 0.0, dn1 = 0.0;
     {
-      const double zs = (z1 - dzl - zm) * inv_st + zm;
-      const int nr = d.l_rows;
-      if (!(zs <= T.lzc[0] || zs >= T.lzc[nr - 1])) {
-        if (bn.move(T.lzc, nr, zs)) set_n(T);
-        const double x = zs - bn.lo;
-        n1 = (cf0 + x * (cf1 + x * (cf2 + x * cf3))) * inv_st;
-        if (ADJ) dn1 = -(cf1 + x * (2.0 * cf2 + 3.0 * x * cf3)) * (inv_st * inv_st);
+      const double zs = z1 * inv_st + zs0;
+      if (!(zs <= zlo || zs >= zhi)) {
+        if (!(zs >= nlo && zs < nhi_)) {
+          jn = clh_walk(T.lzc, l_rows, zs, jn);
+          set_n(T);
+        }
+        const double x = zs - nlo;
+        n1 = cf0 + x * (cf1 + x * (cf2 + x * cf3));
+        if (ADJ) dn1 = -(cf1 + x * (2.0 * cf2 + 3.0 * x * cf3)) * inv_st;
       }
     }
     const double wr = R.c0 * t0 - R.c1 * hf1 * n1;
     if (ADJ) dwr = R.c0 * t0d - R.c1 * hf1 * dn1;
-    t = WG * b1 + WM * R.pref * bmag + wr;
+    t = wgb + R.pref * wmb + wr;
     const double out = pk.lnp(T.lk, R.l10 - l10fk, dx);
-    constexpr double c3 = 1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
-    if (F32) pf = (double)(expf((float)out) * (float)(c3 * fac));
-    else pf = exp(out) * c3 * fac;
+    if (F32) pf = (double)(expf((float)out) * (float)c3fac);
+    else pf = exp(out) * c3fac;
     return true;
   }
 };

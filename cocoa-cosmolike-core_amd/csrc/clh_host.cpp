@@ -1266,6 +1266,7 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     const bool tatt = L.ia_model == CLH_IA_TATT;
     const bool nlw = L.nonlimber && nl > 0 && !ells;
     woff[i + 1] = woff[i] + al((size_t)(ns + nl) * L.n_a) + al(node_sz) +
+                  al(8 * d.chi_z.size()) + al(3 * d.G_z.size()) +
                   al((size_t)(L.n_ss() + L.n_gs() + nl) * nrow) +
                   (tatt ? al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS) + al((size_t)L.n_ss() * nrow) : 0) +
                   (nlw ? clh_nl_off(nl).total + 2 * (al(node_sz) + al((size_t)(L.n_ss() + L.n_gs() + nl) *
@@ -1370,6 +1371,11 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.nodes = wb + al(gtab_sz);
     q.cl = q.nodes + al(node_sz);
     double* wend = q.cl + al((size_t)(L.n_ss() + L.n_gs() + nl) * nrow);
+    q.crec = wend;
+    wend += al(8 * d.chi_z.size());
+    q.grec = wend;
+    wend += al(3 * d.G_z.size());
+    q.zg_same = d.G_z == d.chi_z ? 1 : 0;
     if (L.ia_model == CLH_IA_TATT) {
       q.tatt = wend;
       q.cl_bb = q.tatt + al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS);

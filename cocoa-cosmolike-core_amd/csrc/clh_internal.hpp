@@ -140,6 +140,13 @@ struct DrawDesc {
 
   // per-draw work buffers
   double* gtab;    // [ns + nl][n_a] lens-efficiency tables (g_tomo, then g_lens)
+  // per-interval coefficients of the chi(z), dchi/dz and G(z) tables read by
+  // the gg integrand's a_1 walks (written by k_gtab, clh_cell_records):
+  // crec [chi_n][8] = chi, z, zA, zS, D0, D1, G0, slG of interval j; grec
+  // [G_n][3] = z, G0, slG of the G(z) intervals when G_z differs from chi_z
+  double* crec;
+  double* grec;
+  int zg_same;     // G(z) is tabulated on the z grid of chi(z)
   double* nodes;   // node data, offsets below
   double* cl;      // [n_ss + n_gs + nl][nrow]
   double* dv;      // data vector

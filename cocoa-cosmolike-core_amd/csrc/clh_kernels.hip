@@ -58,6 +58,46 @@ __device__ __forceinline__ double clh_block_sum(double v, double* lds) {
 // TANGENT: also the same prefix sums over d(n)/d(photo-z shift of the bin),
 // the derivative of the table with respect to that shift (into dgtab).
 // ----------------------------------------------------------------------------
+// The interval records of a draw's chi(z) and G(z) tables (DrawDesc::crec,
+// grec) that the gg integrand's a_1 walks read instead of recomputing the
+// interval slopes and the dchi/dz stencil (cosmo3D.c:40-133, 200-228) each
+// time a walk changes interval.  One block.
+__device__ void clh_cell_records(const DrawDesc& d) {
+  const int n = d.chi_n;
+  const double* zz = d.chi_z;
+  const double* cc = d.chi_v;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double* r = d.crec + 8 * (size_t)j;
+    r[0] = cc[j];
+    r[1] = zz[j];
+    if (j > n - 2) continue;
+    const double zS = (zz[j + 1] - zz[j]) / (cc[j + 1] - cc[j]);
+    r[2] = zz[j] - cc[j] * zS;
+    r[3] = zS;
+    const int jj = j > n - 3 ? n - 3 : j;  // chi_all's stencil clamp (cosmo3D.c:58-67)
+    const double inv_z = 1.0 / (zz[jj + 1] - zz[jj]);
+    const double up = (cc[jj + 2] - cc[jj]) / (zz[jj + 2] - zz[jj]);
+    const double down = (jj > 0) ? (cc[jj + 1] - cc[jj - 1]) / (zz[jj + 1] - zz[jj - 1]) : (cc[jj + 1] - cc[jj]) * inv_z;
+    const double D1 = inv_z * (up - down);
+    r[4] = down - zz[jj] * D1;
+    r[5] = D1;
+    if (d.zg_same) {
+      const double slG = (d.G_v[j + 1] - d.G_v[j]) / (d.G_z[j + 1] - d.G_z[j]);
+      r[6] = d.G_v[j] - d.G_z[j] * slG;
+      r[7] = slG;
+    }
+  }
+  if (!d.zg_same)
+    for (int j = threadIdx.x; j < d.G_n; j += blockDim.x) {
+      double* r = d.grec + 3 * (size_t)j;
+      r[0] = d.G_z[j];
+      if (j > d.G_n - 2) continue;
+      const double slG = (d.G_v[j + 1] - d.G_v[j]) / (d.G_z[j + 1] - d.G_z[j]);
+      r[1] = d.G_v[j] - d.G_z[j] * slG;
+      r[2] = slG;
+    }
+}
+
 template <bool TANGENT>
 __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__ D,
                                                     const int2* __restrict__ work) {
@@ -66,6 +106,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   __shared__ double red[8];
   const int2 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
+  if (w.y == 0) clh_cell_records(d);  // once per draw: the block of its first bin
   const bool src = w.y < d.ns;
   const int b = src ? w.y : w.y - d.ns;
   const int x = 25;  // 25 (1 + |high_def_integration|), high_def_integration = 0
