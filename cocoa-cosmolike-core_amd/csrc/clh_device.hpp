@@ -534,8 +534,8 @@ struct ClhPkNodeT {
     i = -1;
     j = j_;
     dy = dy_;
-    nk = d.nk;
-    nz = d.nzp;
+    nk = __builtin_amdgcn_readfirstlane(d.nk);  // the draw's: equal in every lane
+    nz = __builtin_amdgcn_readfirstlane(d.nzp);
     lnP = d.lnP;
   }
   __device__ __forceinline__ void set_cell(const double* __restrict__ lk) {
@@ -612,6 +612,23 @@ struct ClhBracket {
   }
 };
 
+// A value known to be equal across the wave, moved to scalar registers
+// (every lane holds it; readfirstlane states it for the compiler).
+__device__ __forceinline__ int clh_uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double clh_uni(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <class T>
+__device__ __forceinline__ const T* clh_uni(const T* p) {
+  const long long b = (long long)p;
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return (const T*)(((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <bool ADJ, bool AHEAD = true>
 struct ClhGgNode {
   // row-independent values of the node
@@ -627,10 +644,11 @@ struct ClhGgNode {
   int jc, jz, jg, jn, chi_n, G_n, l_rows, same;
   const double* __restrict__ crec;
   const double* __restrict__ grec;
+  const double* __restrict__ nrec;  // this lens bin's n(z) interval records
   double clo, chi_, czc, zA, zS;         // chi interval: z1 = zA + c1v zS; czc = z at its lower end
   double zlo_, zhi_, D0, D1;             // z interval: dchi/dz = D0 + z1 D1
   double glo, ghi, G0, slG;              // G interval (own bounds only off the chi grid): G = G0 + z1 slG
-  double nlo, nhi_, cf0, cf1, cf2, cf3;  // lens spline piece, coefficients / stretch
+  double nlo, nhi_, cf0, cf1, cf2, cf3;  // lens spline interval: knots, coefficients / stretch
   ClhPkNodeT<AHEAD> pk;
 
   __device__ __forceinline__ void rec_z(const double* __restrict__ r) {  // z interval jz from its record
@@ -690,14 +708,32 @@ struct ClhGgNode {
     G0 = r[1];
     slG = r[2];
   }
-  __device__ __forceinline__ void set_n(const ClhGgTabs& T) {
-    nlo = T.lzc[jn];
-    nhi_ = T.lzc[jn + 1];
-    const double* cf = T.lcf + 4 * jn;
-    cf0 = cf[0] * inv_st;
-    cf1 = cf[1] * inv_st;
-    cf2 = cf[2] * inv_st;
-    cf3 = cf[3] * inv_st;
+  __device__ __forceinline__ void rec_n(const double* __restrict__ r) {
+    cf0 = r[1];
+    cf1 = r[2];
+    cf2 = r[3];
+    cf3 = r[4];
+  }
+  __device__ __forceinline__ void set_n() {
+    const double* __restrict__ r = nrec + 5 * (size_t)jn;
+    nlo = r[0];
+    nhi_ = r[5];
+    rec_n(r);
+  }
+  __device__ __forceinline__ void move_n(const ClhGgTabs& T, double zs) {
+    if (zs < nlo && jn > 0) {  // one knot down (zs falls as chi_1 does)
+      const double* __restrict__ r = nrec + 5 * (size_t)(jn - 1);
+      const double lo1 = r[0];
+      if (zs >= lo1) {
+        nhi_ = nlo;
+        nlo = lo1;
+        jn--;
+        rec_n(r);
+        return;
+      }
+    }
+    jn = clh_walk(T.lzc, l_rows, zs, jn);
+    set_n();
   }
 
   // o: the node block (a, fK, fac, W_gal, W_mag, then the search results of
@@ -707,12 +743,14 @@ struct ClhGgNode {
                                       int p, int zl) {
     constexpr int np = CLH_NP_GG;
     constexpr double c3 = 1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
-    chi_n = d.chi_n;
-    G_n = d.G_n;
-    l_rows = d.l_rows;
-    same = d.zg_same;
-    crec = d.crec;
-    grec = d.grec;
+    // the draw's and lens bin's values are the same in every lane: scalar registers
+    chi_n = clh_uni(d.chi_n);
+    G_n = clh_uni(d.G_n);
+    l_rows = clh_uni(d.l_rows);
+    same = clh_uni(d.zg_same);
+    crec = clh_uni(d.crec);
+    grec = clh_uni(d.grec);
+    nrec = clh_uni(d.nrec + (size_t)zl * 5 * d.l_rows);
     fK = o[np + p];
     c3fac = c3 * o[2 * np + p];
     wgb = o[3 * np + p] * d.par->b1[zl];
@@ -725,15 +763,15 @@ struct ClhGgNode {
     jn = (int)lut[6 * np + p];
     t0 = lut[7 * np + p];
     t0d = ADJ ? lut[8 * np + p] : 0.0;
-    inv_st = 1.0 / d.par->lens_st[zl];
+    inv_st = clh_uni(1.0 / d.par->lens_st[zl]);
     const double zm = d.l_zmean[zl];
-    zs0 = (-d.par->lens_dz[zl] - zm) * inv_st + zm;
-    zlo = d.l_zc[0];
-    zhi = d.l_zc[l_rows - 1];
-    chi_amin = d.chi_amin;
+    zs0 = clh_uni((-d.par->lens_dz[zl] - zm) * inv_st + zm);
+    zlo = clh_uni(d.l_zc[0]);
+    zhi = clh_uni(d.l_zc[l_rows - 1]);
+    chi_amin = clh_uni(d.chi_amin);
     set_c();
     if (!same) set_g();
-    set_n(T);
+    set_n();
   }
 
   // false: chi_1 beyond chi(a_min), integrand zero (cosmo2D.c:2512-2514).
@@ -765,10 +803,7 @@ struct ClhGgNode {
     {
       const double zs = z1 * inv_st + zs0;
       if (!(zs <= zlo || zs >= zhi)) {
-        if (!(zs >= nlo && zs < nhi_)) {
-          jn = clh_walk(T.lzc, l_rows, zs, jn);
-          set_n(T);
-        }
+        if (!(zs >= nlo && zs < nhi_)) move_n(T, zs);
         const double x = zs - nlo;
         n1 = cf0 + x * (cf1 + x * (cf2 + x * cf3));
         if (ADJ) dn1 = -(cf1 + x * (2.0 * cf2 + 3.0 * x * cf3)) * inv_st;

@@ -1266,7 +1266,7 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     const bool tatt = L.ia_model == CLH_IA_TATT;
     const bool nlw = L.nonlimber && nl > 0 && !ells;
     woff[i + 1] = woff[i] + al((size_t)(ns + nl) * L.n_a) + al(node_sz) +
-                  al(8 * d.chi_z.size()) + al(3 * d.G_z.size()) +
+                  al(8 * d.chi_z.size()) + al(3 * d.G_z.size()) + al((size_t)5 * nl * L.lens.rows) +
                   al((size_t)(L.n_ss() + L.n_gs() + nl) * nrow) +
                   (tatt ? al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS) + al((size_t)L.n_ss() * nrow) : 0) +
                   (nlw ? clh_nl_off(nl).total + 2 * (al(node_sz) + al((size_t)(L.n_ss() + L.n_gs() + nl) *
@@ -1376,6 +1376,8 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.grec = wend;
     wend += al(3 * d.G_z.size());
     q.zg_same = d.G_z == d.chi_z ? 1 : 0;
+    q.nrec = wend;
+    wend += al((size_t)5 * nl * L.lens.rows);
     if (L.ia_model == CLH_IA_TATT) {
       q.tatt = wend;
       q.cl_bb = q.tatt + al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS);

@@ -147,6 +147,9 @@ struct DrawDesc {
   double* crec;
   double* grec;
   int zg_same;     // G(z) is tabulated on the z grid of chi(z)
+  // nrec [nl][l_rows][5] = knot z and the spline coefficients / lens stretch
+  // of each lens n(z) interval (k_gtab, lens blocks)
+  double* nrec;
   double* nodes;   // node data, offsets below
   double* cl;      // [n_ss + n_gs + nl][nrow]
   double* dv;      // data vector

@@ -107,6 +107,17 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   const int2 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
   if (w.y == 0) clh_cell_records(d);  // once per draw: the block of its first bin
+  if (w.y >= d.ns) {  // the lens bin's n(z) interval records (ClhGgNode)
+    const int zl = w.y - d.ns, nr = d.l_rows;
+    const double inv_st = 1.0 / d.par->lens_st[zl];
+    const double* cf = d.l_coef + (size_t)zl * 4 * (nr - 1);
+    double* r = d.nrec + (size_t)zl * 5 * nr;
+    for (int j = threadIdx.x; j < nr; j += blockDim.x) {
+      r[5 * j] = d.l_zc[j];
+      if (j < nr - 1)
+        for (int k = 0; k < 4; k++) r[5 * j + 1 + k] = cf[4 * j + k] * inv_st;
+    }
+  }
   const bool src = w.y < d.ns;
   const int b = src ? w.y : w.y - d.ns;
   const int x = 25;  // 25 (1 + |high_def_integration|), high_def_integration = 0
