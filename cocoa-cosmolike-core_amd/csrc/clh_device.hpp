@@ -532,6 +532,8 @@ struct ClhPkNodeT {
 
   __device__ __forceinline__ void init(const DrawDesc& d, int j_, double dy_) {
     i = -1;
+    lo = INFINITY;  // empty cell: the first lookup searches
+    hi = -INFINITY;
     j = j_;
     dy = dy_;
     nk = __builtin_amdgcn_readfirstlane(d.nk);  // the draw's: equal in every lane
@@ -547,11 +549,11 @@ struct ClhPkNodeT {
   }
   // ln P at log10 k; dx: the log10 k weight of the cell (i, j)
   __device__ __forceinline__ double lnp(const double* __restrict__ lk, double log10k, double& dx) {
-    if (i < 0) {
-      i = clh_bs(lk, nk, log10k);
-      set_cell(lk);
-    } else if (!(log10k >= lo && log10k < hi)) {
-      if (AHEAD && log10k >= hi && (i + 1 == nk - 2 || log10k < nhi)) {
+    if (!(log10k >= lo && log10k < hi)) {
+      if (i < 0) {
+        i = clh_bs(lk, nk, log10k);
+        set_cell(lk);
+      } else if (AHEAD && log10k >= hi && (i + 1 == nk - 2 || log10k < nhi)) {
         // one cell up: the clh_walk index is i + 1
         i++;
         x0 = hi;  // finite: the old cell was not the last
@@ -612,23 +614,6 @@ struct ClhBracket {
   }
 };
 
-// A value known to be equal across the wave, moved to scalar registers
-// (every lane holds it; readfirstlane states it for the compiler).
-__device__ __forceinline__ int clh_uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ double clh_uni(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
-  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-template <class T>
-__device__ __forceinline__ const T* clh_uni(const T* p) {
-  const long long b = (long long)p;
-  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
-  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return (const T*)(((long long)hi << 32) | (unsigned int)lo);
-}
-
 template <bool ADJ, bool AHEAD = true>
 struct ClhGgNode {
   // row-independent values of the node
@@ -641,17 +626,18 @@ struct ClhGgNode {
   // coefficients, read from the draw's interval records (DrawDesc::crec,
   // grec) when the interval changes.  Successive rows move chi_1 down, so a
   // change is almost always one interval down: one record load, no search.
-  int jc, jz, jg, jn, chi_n, G_n, l_rows, same;
+  int jc, jg, jn, chi_n, G_n, l_rows, same;
+  bool alive;  // chi_1 <= chi(a_min) on the current chi interval
   const double* __restrict__ crec;
   const double* __restrict__ grec;
   const double* __restrict__ nrec;  // this lens bin's n(z) interval records
   double clo, chi_, czc, zA, zS;         // chi interval: z1 = zA + c1v zS; czc = z at its lower end
-  double zlo_, zhi_, D0, D1;             // z interval: dchi/dz = D0 + z1 D1
+  double D0, D1;                         // z interval (the chi interval's): dchi/dz = D0 + z1 D1
   double glo, ghi, G0, slG;              // G interval (own bounds only off the chi grid): G = G0 + z1 slG
   double nlo, nhi_, cf0, cf1, cf2, cf3;  // lens spline interval: knots, coefficients / stretch
   ClhPkNodeT<AHEAD> pk;
 
-  __device__ __forceinline__ void rec_z(const double* __restrict__ r) {  // z interval jz from its record
+  __device__ __forceinline__ void rec_z(const double* __restrict__ r) {  // the z interval's coefficients
     D0 = r[4];
     D1 = r[5];
     if (same) {
@@ -659,47 +645,45 @@ struct ClhGgNode {
       slG = r[7];
     }
   }
-  // chi interval jc and its z interval (the same index unless rounding put
-  // z1 on the next knot; the per-row z test catches that) from the records
-  __device__ __forceinline__ void set_c() {
+  // chi interval jc from the records.  Its z interval, for the dchi/dz
+  // stencil and (same grid) G(z), has the same index: z1 interpolated on the
+  // chi interval lies between the interval's z knots, the reference's second
+  // bisection (on z1) can differ only when z1 rounds onto a knot, where the
+  // stencil is continuous.  The upper bound is capped at the chi(a_min) cut
+  // (cosmo2D.c:2512-2514), beyond which the integrand is zero.
+  __device__ __forceinline__ void set_c(double c1v) {
+    if (c1v > chi_amin * CLH_COVERH0) {  // beyond the cut until chi_1 falls below it
+      alive = false;
+      clo = chi_amin * CLH_COVERH0;
+      chi_ = INFINITY;
+      return;
+    }
+    alive = true;
     const double* __restrict__ r = crec + 8 * (size_t)jc;
     clo = jc == 0 ? -INFINITY : r[0];
-    chi_ = jc == chi_n - 2 ? INFINITY : r[8];
+    chi_ = fmin(jc == chi_n - 2 ? INFINITY : r[8], chi_amin * CLH_COVERH0);
     czc = r[1];
     zA = r[2];
     zS = r[3];
-    jz = jc;
-    zlo_ = jc == 0 ? -INFINITY : czc;
-    zhi_ = jc == chi_n - 2 ? INFINITY : r[9];
     rec_z(r);
   }
   __device__ __forceinline__ void move_c(const ClhGgTabs& T, double c1v) {
-    if (c1v < clo && jc > 0) {  // one interval down: the upper bounds are the old lower ones
+    if (alive && c1v < clo && jc > 0) {  // one interval down: the upper bound is the old lower one
       const double* __restrict__ r = crec + 8 * (size_t)(jc - 1);
       const double lo1 = r[0];
       if (jc == 1 || c1v >= lo1) {
         chi_ = clo;
-        zhi_ = czc;
         jc--;
         clo = jc == 0 ? -INFINITY : lo1;
         czc = r[1];
         zA = r[2];
         zS = r[3];
-        jz = jc;
-        zlo_ = jc == 0 ? -INFINITY : czc;
         rec_z(r);
         return;
       }
     }
     jc = clh_walk(T.cv, chi_n, c1v, jc);
-    set_c();
-  }
-  __device__ __forceinline__ void move_z(const ClhGgTabs& T, double z1) {
-    jz = clh_walk(T.cz, chi_n, z1, jz);
-    const double* __restrict__ r = crec + 8 * (size_t)jz;
-    zlo_ = jz == 0 ? -INFINITY : r[1];
-    zhi_ = jz == chi_n - 2 ? INFINITY : r[9];
-    rec_z(r);
+    set_c(c1v);
   }
   __device__ __forceinline__ void set_g() {
     const double* __restrict__ r = grec + 3 * (size_t)jg;
@@ -714,17 +698,27 @@ struct ClhGgNode {
     cf2 = r[3];
     cf3 = r[4];
   }
-  __device__ __forceinline__ void set_n() {
+  // the lens n(z) interval of zs: a spline piece inside the knots, zero
+  // (and an interval reaching to -inf / +inf) outside them
+  __device__ __forceinline__ void set_n(const ClhGgTabs& T, double zs) {
+    if (zs <= zlo || zs >= zhi) {
+      nlo = zs <= zlo ? -INFINITY : zhi;
+      nhi_ = zs <= zlo ? zlo : INFINITY;
+      cf0 = cf1 = cf2 = cf3 = 0.0;
+      jn = l_rows - 1;  // above the knots, one step down is the last interval
+      return;
+    }
+    jn = clh_walk(T.lzc, l_rows, zs, jn);
     const double* __restrict__ r = nrec + 5 * (size_t)jn;
     nlo = r[0];
     nhi_ = r[5];
     rec_n(r);
   }
   __device__ __forceinline__ void move_n(const ClhGgTabs& T, double zs) {
-    if (zs < nlo && jn > 0) {  // one knot down (zs falls as chi_1 does)
+    if (zs < nlo && nlo > zlo) {  // one knot down inside the knots (zs falls as chi_1 does)
       const double* __restrict__ r = nrec + 5 * (size_t)(jn - 1);
       const double lo1 = r[0];
-      if (zs >= lo1) {
+      if (zs >= lo1 && zs > zlo) {
         nhi_ = nlo;
         nlo = lo1;
         jn--;
@@ -732,8 +726,7 @@ struct ClhGgNode {
         return;
       }
     }
-    jn = clh_walk(T.lzc, l_rows, zs, jn);
-    set_n();
+    set_n(T, zs);
   }
 
   // o: the node block (a, fK, fac, W_gal, W_mag, then the search results of
@@ -743,14 +736,13 @@ struct ClhGgNode {
                                       int p, int zl) {
     constexpr int np = CLH_NP_GG;
     constexpr double c3 = 1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
-    // the draw's and lens bin's values are the same in every lane: scalar registers
-    chi_n = clh_uni(d.chi_n);
-    G_n = clh_uni(d.G_n);
-    l_rows = clh_uni(d.l_rows);
-    same = clh_uni(d.zg_same);
-    crec = clh_uni(d.crec);
-    grec = clh_uni(d.grec);
-    nrec = clh_uni(d.nrec + (size_t)zl * 5 * d.l_rows);
+    chi_n = d.chi_n;
+    G_n = d.G_n;
+    l_rows = d.l_rows;
+    same = d.zg_same;
+    crec = d.crec;
+    grec = d.grec;
+    nrec = d.nrec + (size_t)zl * 5 * d.l_rows;
     fK = o[np + p];
     c3fac = c3 * o[2 * np + p];
     wgb = o[3 * np + p] * d.par->b1[zl];
@@ -763,15 +755,21 @@ struct ClhGgNode {
     jn = (int)lut[6 * np + p];
     t0 = lut[7 * np + p];
     t0d = ADJ ? lut[8 * np + p] : 0.0;
-    inv_st = clh_uni(1.0 / d.par->lens_st[zl]);
+    inv_st = 1.0 / d.par->lens_st[zl];
     const double zm = d.l_zmean[zl];
-    zs0 = clh_uni((-d.par->lens_dz[zl] - zm) * inv_st + zm);
-    zlo = clh_uni(d.l_zc[0]);
-    zhi = clh_uni(d.l_zc[l_rows - 1]);
-    chi_amin = clh_uni(d.chi_amin);
-    set_c();
+    zs0 = (-d.par->lens_dz[zl] - zm) * inv_st + zm;
+    zlo = d.l_zc[0];
+    zhi = d.l_zc[l_rows - 1];
+    chi_amin = d.chi_amin;
+    alive = false;   // empty intervals: the first row walks from the a_0 indices
+    clo = INFINITY;
+    chi_ = -INFINITY;
+    nlo = -INFINITY;  // (not a one-knot step either)
+    nhi_ = -INFINITY;
+    czc = zA = zS = D1 = slG = 0.0;  // finite until a row lies inside the cut
+    D0 = G0 = 1.0;
+    cf0 = cf1 = cf2 = cf3 = 0.0;
     if (!same) set_g();
-    set_n();
   }
 
   // false: chi_1 beyond chi(a_min), integrand zero (cosmo2D.c:2512-2514).
@@ -783,14 +781,11 @@ struct ClhGgNode {
   template <bool F32 = false>
   __device__ __forceinline__ bool eval(const DrawDesc& d, const ClhGgTabs& T, const ClhGgRow& R,
                                        double& t, double& pf, double& dwr, double& dx) {
-    const double chi_1 = fK * R.rr1;
-    if (chi_1 > chi_amin) return false;
-    // a_chi(chi_1) (cosmo3D.c:109-133)
-    const double c1v = chi_1 * CLH_COVERH0;
+    // a_chi(chi_1) (cosmo3D.c:109-133), chi_1 = chi (ell+1)/ell
+    const double c1v = fK * R.rr1 * CLH_COVERH0;
     if (!(c1v >= clo && c1v < chi_)) move_c(T, c1v);
+    // dchi/dz at z1 (cosmo3D.c:64-81 stencils) on the same interval
     const double z1 = zA + c1v * zS;
-    // dchi/dz at z1 (cosmo3D.c:64-81 stencils)
-    if (!(z1 >= zlo_ && z1 < zhi_)) move_z(T, z1);
     // G(z1) and f_growth (cosmo3D.c:200-228): hoverh0 f_growth =
     // (c/H0) / (dchi/dz) (1 - G'(1+z)/G), where G - G'(1+z) = G0 - slG on the interval
     if (!same && !(z1 >= glo && z1 < ghi)) {
@@ -799,22 +794,18 @@ struct ClhGgNode {
     }
     const double hf1 = CLH_COVERH0 * (G0 - slG) / ((D0 + z1 * D1) * (G0 + z1 * slG));
     // nz_lens(z1) (redshift_spline.c:736-762) and its shift derivative
-    double n1 = 0.0, dn1 = 0.0;
-    {
-      const double zs = z1 * inv_st + zs0;
-      if (!(zs <= zlo || zs >= zhi)) {
-        if (!(zs >= nlo && zs < nhi_)) move_n(T, zs);
-        const double x = zs - nlo;
-        n1 = cf0 + x * (cf1 + x * (cf2 + x * cf3));
-        if (ADJ) dn1 = -(cf1 + x * (2.0 * cf2 + 3.0 * x * cf3)) * inv_st;
-      }
-    }
+    const double zs = z1 * inv_st + zs0;
+    if (!(zs >= nlo && zs < nhi_)) move_n(T, zs);
+    const double x = nlo > -INFINITY ? zs - nlo : 0.0;  // below the knots: zero coefficients
+    const double n1 = cf0 + x * (cf1 + x * (cf2 + x * cf3));
+    double dn1 = 0.0;
+    if (ADJ) dn1 = -(cf1 + x * (2.0 * cf2 + 3.0 * x * cf3)) * inv_st;
     const double wr = R.c0 * t0 - R.c1 * hf1 * n1;
     if (ADJ) dwr = R.c0 * t0d - R.c1 * hf1 * dn1;
     t = wgb + R.pref * wmb + wr;
     const double out = pk.lnp(T.lk, R.l10 - l10fk, dx);
     if (F32) pf = (double)(expf((float)out) * (float)c3fac);
     else pf = exp(out) * c3fac;
-    return true;
+    return alive;
   }
 };
