@@ -129,6 +129,67 @@ def measured_traffic(kernel, batch, precision):
     return None
 
 
+def time_pairs(c, dv, dvbar, p0, B, n, ckpt, steps, fwd_s):
+    """Forward + VJP pairs over the uploaded batch (gradients of ln P, the
+    photo-z shifts, b1, A_IA, m); ckpt: the forward keeps the Limber
+    integrands for the VJP instead of the VJP recomputing them.  fwd_s: the
+    forward's own time per step, subtracted for the VJP rate."""
+    import torch
+    ns, nl = p0.ns, p0.nl
+    nknz = p0.lnP.size
+    g = {k: torch.empty(B * m, dtype=torch.float64, device="cuda:0")
+         for k, m in (("lnP", nknz), ("src", ns), ("lens", nl), ("b1", nl), ("A", 1), ("m", ns))}
+    ptrs = [g[k].data_ptr() for k in ("lnP", "src", "lens", "b1", "A", "m")]
+    c.set_vjp_checkpoint(ckpt)
+    tiles = c.autotune()  # forward and adjoint stages tuned for pairs
+    for _ in range(2):
+        c.batch_forward(dv.data_ptr())
+        c.batch_vjp(dvbar.data_ptr(), *ptrs)
+    c.synchronize()
+    nv = max(3, steps // 2)
+    t1 = time.perf_counter()
+    for _ in range(nv):
+        c.batch_forward(dv.data_ptr())
+        c.batch_vjp(dvbar.data_ptr(), *ptrs)
+    c.synchronize()
+    t_pair = (time.perf_counter() - t1) / nv
+    c.enable_timing(True)
+    ms = {}
+    for _ in range(3):
+        c.batch_forward(dv.data_ptr())
+        c.batch_vjp(dvbar.data_ptr(), *ptrs)
+        c.synchronize()
+        for k, v in c.kernel_times().items():
+            if k.endswith("_adj") or k == "k_lnp_gather":
+                ms[k] = ms.get(k, 0.0) + v / 3
+    c.enable_timing(False)
+    c.set_vjp_checkpoint(False)
+    return {"fwd_plus_vjp_value": B / t_pair, "vjp_value": B / max(t_pair - fwd_s, 1e-9),
+            "vjp_kernel_ms": ms, "tile_rows": tiles}
+
+
+def speedups(value, vjp, mixed, plain):
+    """Throughput of each batched leg over the plain FP64 single-draw path."""
+    out = {"forward": value / plain}
+    for k, v in (vjp or {}).items():
+        out[f"forward_plus_vjp_{k}"] = v["fwd_plus_vjp_value"] / plain
+    for k, v in (mixed or {}).items():
+        out[f"{k}_forward"] = v["value"] / plain
+        if "fwd_plus_vjp_value" in v:
+            out[f"{k}_forward_plus_vjp"] = v["fwd_plus_vjp_value"] / plain
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(draws, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the FP64 CPU restatement, timed as the CPU reference path
@@ -147,6 +208,7 @@ def cpu_baseline(draws, seconds):
             break
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": n / el, "unit": "data vectors/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"{n} draws of the same 3x2pt layout (FP64 oracle, OpenMP), {el:.1f} s"}
 
 
@@ -162,6 +224,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--vjp", type=int, default=1, help="also time the batched VJP")
+    ap.add_argument("--mixed", type=int, default=1,
+                    help="also time the FP32 and BF16X3 forward and forward + VJP (with --precision fp64)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="also copy 512 MiB device to device once (HBM-counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--ragged", type=int, default=512,
@@ -242,62 +306,64 @@ def main():
 
     # forward + VJP: one autograd step per draw (forward, then the VJP of the
     # whole data vector against a cotangent), timed as pairs; the VJP rate is
-    # the pair time minus the forward time.  As under torch.autograd, the
-    # forward keeps the gg integrand checkpoint and the VJP reuses the
-    # forward's device state (node weights, lens efficiencies, C_ell rows).
-    vjp_value = None
-    pair_value = None
-    vjp_ms = {}
+    # the pair time minus the forward time.  The VJP reuses the forward's
+    # device state (node weights, lens efficiencies, C_ell rows).  Two
+    # adjoints: recomputing the Limber integrands (the default), or reading a
+    # checkpoint the forward stored (clh_set_vjp_checkpoint).
+    vjp = {}
     if args.vjp:
         B = args.batch
-        ns, nl = draws[0].ns, draws[0].nl
-        nknz = draws[0].lnP.size
         dvbar = torch.randn(B * n, dtype=torch.float64, device="cuda:0")
-        g = {k: torch.empty(B * m, dtype=torch.float64, device="cuda:0")
-             for k, m in (("lnP", nknz), ("src", ns), ("lens", nl), ("b1", nl), ("A", 1), ("m", ns))}
-        ptrs = [g[k].data_ptr() for k in ("lnP", "src", "lens", "b1", "A", "m")]
-        c.set_vjp_checkpoint(True)
-        pair_tiles = c.autotune()  # the checkpointed forward and adjoint have their own best tiles
-        log(f"bench: autotuned rows per tile for forward+VJP pairs {pair_tiles}, gg tables in LDS {c.gg_lds}")
-        for _ in range(2):
-            c.batch_forward(dv.data_ptr())
-            c.batch_vjp(dvbar.data_ptr(), *ptrs)
-        c.synchronize()
-        t1 = time.perf_counter()
-        nv = max(3, args.steps // 2)
-        for _ in range(nv):
-            c.batch_forward(dv.data_ptr())
-            c.batch_vjp(dvbar.data_ptr(), *ptrs)
-        c.synchronize()
-        t_pair = (time.perf_counter() - t1) / nv
-        pair_value = B / t_pair
-        vjp_value = B / max(t_pair - el / args.steps, 1e-9)
-        log(f"bench: forward+VJP {pair_value:.1f} per s; VJP share {vjp_value:.1f} per s")
-        c.enable_timing(True)
-        for _ in range(3):
-            c.batch_forward(dv.data_ptr())
-            c.batch_vjp(dvbar.data_ptr(), *ptrs)
-            c.synchronize()
-            for k, v in c.kernel_times().items():
-                if k.endswith("_adj") or k == "k_lnp_gather":
-                    vjp_ms[k] = vjp_ms.get(k, 0.0) + v / 3
-        c.enable_timing(False)
-        c.set_vjp_checkpoint(False)
+        for name, ck in (("recompute", False), ("checkpoint", True)):
+            vjp[name] = time_pairs(c, dv, dvbar, draws[0], B, n, ck, args.steps, el / args.steps)
+            log(f"bench: forward+VJP ({name}) {vjp[name]}")
         c.set_tile_rows(tile_rows)
-        log(f"bench: VJP Limber kernel ms {vjp_ms}")
+
+    # the split-precision forward and forward + VJP on the same batch, with
+    # each probe's error against the plain FP64 GPU path
+    mixed = {}
+    if args.mixed and args.precision == "fp64":
+        ref = dv.view(args.batch, n).cpu().numpy()
+        p0 = draws[0]
+        for prec in ("fp32", "bf16x3"):
+            c.set_precision(prec)
+            mt = c.autotune()
+            mdv = torch.empty_like(dv)
+            for _ in range(args.warmup):
+                c.batch_forward(mdv.data_ptr())
+            c.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                c.batch_forward(mdv.data_ptr())
+            c.synchronize()
+            elm = time.perf_counter() - t1
+            acc_m = probe_errors(mdv.view(args.batch, n).cpu().numpy(), ref, p0.ns, p0.nl, p0.ntheta)
+            rec_m = {"value": args.batch * args.steps / elm, "unit": "data vectors/s", "tile_rows": mt,
+                     "accuracy_vs_fp64_path": acc_m,
+                     "max_rel_err": max(v["max"] for v in acc_m.values())}
+            if args.vjp:
+                pr = time_pairs(c, mdv, dvbar, p0, args.batch, n, False, args.steps, elm / args.steps)
+                rec_m["fwd_plus_vjp_value"] = pr["fwd_plus_vjp_value"]
+                rec_m["vjp_value"] = pr["vjp_value"]
+            mixed[prec] = rec_m
+            log(f"bench: {prec} {rec_m}")
+            del mdv
+        c.set_precision("fp64")
+        c.set_tile_rows(tile_rows)
+
     work = algorithmic_work(draws[0])
     dom = max(acc, key=acc.get)
     flop1, peak = kernel_work_and_peak(dom, work, args.precision)
     flop = flop1 * args.batch
     achieved = flop / (acc[dom] * 1e-3) / 1e12 if acc[dom] > 0 else 0.0
-    # the Limber kernels compute on the FP64 (FP32) vector ALUs, whose MI355X
-    # peak equals the dense MFMA peak of the same type: the compute roofline
-    # ("mfma" in the record's vocabulary) is the one that bounds them
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": peak,
+    # the Limber kernels compute on the FP64 (FP32) vector ALUs: their roofline
+    # is the vector peak of that type (on MI355X equal to the dense MFMA one)
+    roofline = {"bound": "valu" if "legendre" not in dom else "mfma", "achieved": achieved, "peak": peak,
                 "unit": "TFLOP/s", "frac": achieved / peak,
                 "units": "vector ALU" if "legendre" not in dom else "MFMA",
                 "traffic": measured_traffic(dom, args.batch, args.precision), "kernel": dom,
-                "kernel_ms": acc[dom], "kernel_ms_all": acc, "vjp_kernel_ms": vjp_ms}
+                "kernel_ms": acc[dom], "kernel_ms_all": acc,
+                "vjp_kernel_ms": vjp["recompute"]["vjp_kernel_ms"] if vjp else None}
     log(f"bench: kernel ms {acc}")
 
     # sanity: the last batch agrees with itself (finite values)
@@ -407,15 +473,16 @@ def main():
                    "tile_rows": tile_rows,
                    "datavector_len": n},
         "accuracy_vs_fp64_path": accuracy,
-        "vjp_value": vjp_value,
+        "vjp_value": vjp["recompute"]["vjp_value"] if vjp else None,
         "vjp_unit": "VJPs/s (full data vector -> lnP, dz_src, dz_lens, b1, A_IA, m; FP64 adjoint kernels "
-                    "reading the forward's gg checkpoint, timed as forward+VJP pairs minus the forward)",
-        "fwd_plus_vjp_value": pair_value,
+                    "recomputing the Limber integrands, timed as forward+VJP pairs minus the forward)",
+        "fwd_plus_vjp_value": vjp["recompute"]["fwd_plus_vjp_value"] if vjp else None,
+        "vjp_legs": vjp,
+        "mixed_precision": mixed,
         "plain_fp64_single_draw": plain,
         "ragged_batch": ragged,
         "nonlimber_batch": nonlimber,
-        "speedup_vs_plain_path": {"forward": value / plain["value"],
-                                  "forward_plus_vjp": (pair_value / plain["value"]) if pair_value else None},
+        "speedup_vs_plain_path": speedups(value, vjp, mixed, plain["value"]),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

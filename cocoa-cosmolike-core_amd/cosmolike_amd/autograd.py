@@ -4,7 +4,9 @@
 batch of B draws (uploaded beforehand with ``Cosmolike.batch_upload``; the
 tensors below override their differentiable inputs) to the B real-space 3x2pt
 data vectors, FP64, on the GPU.  The backward pass runs the library's
-hand-written adjoint kernels (``clh_batch_vjp``).
+hand-written adjoint kernels (``clh_batch_vjp``), which recompute the Limber
+integrands; ``checkpoint=True`` has the forward store them for the backward
+instead (the same gradients, no faster at batch 512).
 
 Shapes (float64, cuda, contiguous): lnP [B, nk*nz] (k-major), src_dz [B, ns],
 lens_dz [B, nl], b1 [B, nl], ia_A [B], shear_m [B, ns].  An input given as
@@ -23,7 +25,8 @@ def _ptr(t):
 
 class Projection3x2pt(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, lib_ctx: Cosmolike, lnP, src_dz, lens_dz, b1, ia_A, shear_m, ord=(0, 1, 2)):
+    def forward(ctx, lib_ctx: Cosmolike, lnP, src_dz, lens_dz, b1, ia_A, shear_m, ord=(0, 1, 2),
+                checkpoint=False):
         given = [t.contiguous() if t is not None else None for t in (lnP, src_dz, lens_dz, b1, ia_A, shear_m)]
         for t in given:
             if t is not None and (t.dtype != torch.float64 or not t.is_cuda):
@@ -38,8 +41,9 @@ class Projection3x2pt(torch.autograd.Function):
         out = torch.empty(B, n, dtype=torch.float64, device=dev)
         lib_ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
         lib_ctx.batch_set_params(*[_ptr(t) for t in ins])
-        # keep the gg integrand for the backward pass when one will follow
-        lib_ctx.set_vjp_checkpoint(any(ctx.needs_input_grad[1:7]))
+        # the backward recomputes the Limber integrands (chi-slices are not
+        # stored); checkpoint=True keeps them from this forward instead
+        lib_ctx.set_vjp_checkpoint(bool(checkpoint) and any(ctx.needs_input_grad[1:7]))
         lib_ctx.batch_forward(out.data_ptr(), ord)
         ctx.lib_ctx = lib_ctx
         ctx.ord = ord
@@ -63,8 +67,8 @@ class Projection3x2pt(torch.autograd.Function):
         grads = [torch.empty_like(t) if (t is not None and need) else None
                  for t, need in zip(ins, ctx.needs_input_grad[1:7])]
         c.batch_vjp(g.data_ptr(), *[_ptr(t) for t in grads], ord=ctx.ord)
-        return (None, *grads, None)
+        return (None, *grads, None, None)
 
 
-def projection_3x2pt(lib_ctx, lnP, src_dz, lens_dz, b1, ia_A, shear_m, ord=(0, 1, 2)):
-    return Projection3x2pt.apply(lib_ctx, lnP, src_dz, lens_dz, b1, ia_A, shear_m, ord)
+def projection_3x2pt(lib_ctx, lnP, src_dz, lens_dz, b1, ia_A, shear_m, ord=(0, 1, 2), checkpoint=False):
+    return Projection3x2pt.apply(lib_ctx, lnP, src_dz, lens_dz, b1, ia_A, shear_m, ord, checkpoint)

@@ -145,7 +145,6 @@ def test_autograd_none_input_uses_uploaded_value():
     torch.cuda.synchronize()
     assert torch.equal(out_a, out_r)
     assert not torch.equal(out_a, out_b)
-    # equal up to the order of the adjoint's atomic sums (checkpointed vs
-    # recomputed forward state)
+    # equal up to the order of the adjoint's atomic sums
     ga, gr = g_after.cpu().numpy(), r_src.grad.cpu().numpy()
     np.testing.assert_allclose(ga, gr, rtol=1e-9, atol=1e-11 * np.abs(gr).max())
