@@ -160,7 +160,7 @@ def time_pairs(c, dv, dvbar, p0, B, n, ckpt, steps, fwd_s):
         c.batch_vjp(dvbar.data_ptr(), *ptrs)
         c.synchronize()
         for k, v in c.kernel_times().items():
-            if k.endswith("_adj") or k == "k_lnp_gather":
+            if k.endswith("_adj") or k in ("k_lnp_gather", "lens_limits"):
                 ms[k] = ms.get(k, 0.0) + v / 3
     c.enable_timing(False)
     c.set_vjp_checkpoint(False)
@@ -300,7 +300,7 @@ def main():
         c.batch_forward(dv.data_ptr())
         c.synchronize()
         for k, v in c.kernel_times().items():
-            if not k.endswith("_adj") and k != "k_lnp_gather":
+            if not k.endswith("_adj") and k not in ("k_lnp_gather", "lens_limits"):
                 acc[k] = acc.get(k, 0.0) + v / nt
     c.enable_timing(False)
 

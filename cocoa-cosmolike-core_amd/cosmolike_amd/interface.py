@@ -565,7 +565,7 @@ class Cosmolike:
         self._chk(self.L.clh_enable_timing(self.h, 1 if on else 0))
 
     def kernel_times(self):
-        names = (ctypes.c_char_p * 8)()
-        ms = (ctypes.c_float * 8)()
-        n = self.L.clh_last_kernel_times(self.h, names, ms, 8)
+        names = (ctypes.c_char_p * 9)()
+        ms = (ctypes.c_float * 9)()
+        n = self.L.clh_last_kernel_times(self.h, names, ms, 9)
         return {names[i].decode(): float(ms[i]) for i in range(n)}

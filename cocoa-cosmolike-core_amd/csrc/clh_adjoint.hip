@@ -519,6 +519,47 @@ __global__ __launch_bounds__(128) void k_nodes_adj(const DrawDesc* __restrict__ 
 }
 
 // ----------------------------------------------------------------------------
+// k_lim_grad: the part of the lens-shift gradient carried by the integration
+// limits.  amin_lens / amax_lens move with |dz| (redshift_spline.c:50-79), so
+// the GL nodes of the bin's gs and gg integrals move with it; the adjoint
+// kernels differentiate at fixed nodes.  P / M: views of the draws whose
+// limits take |dz| + eps / |dz| - eps (DrawDesc::lim_eps) with their gs and gg
+// C_ell rows.  The integrands jump where a node crosses a knot of a table
+// whose slope enters them (the growth rate from G(z), the dchi/dz stencil):
+// a row's forward and backward differences then disagree, and the one of
+// smaller magnitude is the row's slope (a jump adds |jump| / eps to the
+// other); else their mean.  One block per (draw, lens bin):
+// sign(dz) sum_(c, r) cbar[c][r] slope[c][r] over the bin's gs columns and
+// its gg column, into g_lens; nothing for |dz| <= 2 eps (the forward's kink
+// at dz = 0, where the dependence on |dz| is symmetric).
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_lim_grad(const DrawDesc* __restrict__ D, const DrawDesc* __restrict__ P,
+                                                  const DrawDesc* __restrict__ M, const int2* __restrict__ work) {
+  __shared__ double red[4];
+  const int2 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const double* __restrict__ cp = P[w.x].cl;
+  const double* __restrict__ cm = M[w.x].cl;
+  const double eps = P[w.x].lim_eps;
+  const int zl = w.y, ns = d.ns, nrow = d.nrow;
+  const double dz = d.par->lens_dz[zl];
+  if (!(fabs(dz) > 2.0 * eps)) return;
+  double acc = 0.0;
+  const int n = (ns + 1) * nrow;
+  for (int t = threadIdx.x; t < n; t += 256) {
+    const int k = t / nrow, r = t % nrow;
+    const int col = k < ns ? d.n_ss + zl * ns + k : d.n_ss + d.n_gs + zl;
+    const size_t o = (size_t)col * nrow + r;
+    const double c0 = d.cl[o], sp = cp[o] - c0, sm = c0 - cm[o];
+    const double s = fabs(sp - sm) <= 0.05 * (fabs(sp) + fabs(sm)) ? 0.5 * (sp + sm)
+                   : (fabs(sp) < fabs(sm) ? sp : sm);
+    acc += d.cbar[o] * s;
+  }
+  const double sum = clh_block_sum(acc, red);
+  if (threadIdx.x == 0 && sum != 0.0) atomicAdd(d.g_lens + zl, (dz > 0 ? sum : -sum) / eps);
+}
+
+// ----------------------------------------------------------------------------
 // k_lnp_gather: the ln P cotangent of a draw from its per-node runs.  Work
 // item (draw, i0): lnP rows i0 .. i0 + ic - 1, every z column.  Node q (ss,
 // gs, gg blocks in run-slab order) with z column j and weight dy adds
@@ -591,6 +632,12 @@ __global__ __launch_bounds__(256) void k_lnp_gather(const DrawDesc* __restrict__
 }
 
 // ----------------------------------------------------------------------------
+hipError_t clh_launch_lim_grad(const DrawDesc* D, const DrawDesc* P, const DrawDesc* M, const int2* work, int n,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lim_grad, dim3(n), dim3(256), 0, s, D, P, M, work);
+  return hipGetLastError();
+}
 hipError_t clh_launch_fill(double* p, size_t n, double v, hipStream_t s) {
   if (!p || n == 0) return hipSuccess;
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);

@@ -234,15 +234,32 @@ __device__ __forceinline__ void clh_ss_pair(int ns, int q, int& i, int& j) {
   j = i + q;
 }
 
+// sum of v over the block (blockDim.x <= 256: four waves); lds holds 4 doubles
+__device__ __forceinline__ double clh_block_sum(double v, double* lds) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  const double s = lds[0] + lds[1] + lds[2] + lds[3];
+  __syncthreads();
+  return s;
+}
+
 // amin_lens / amax_lens: redshift_spline.c:50-79
+// |dz| of the lens limits (lim_eps: see DrawDesc)
+__device__ __forceinline__ double clh_lim_adz(const DrawDesc& d, int ni) {
+  const double dz = d.par->lens_dz[ni];
+  return fabs(dz) + (dz != 0.0 ? d.lim_eps : 0.0);
+}
 __device__ __forceinline__ double clh_amin_lens(const DrawDesc& d, int ni) {
   const double zmax = (d.l_zmax[ni] - d.l_zmean[ni]) * d.par->lens_st[ni] + d.l_zmean[ni];
-  return 1. / (1 + zmax + 2. * fabs(d.par->lens_dz[ni]));
+  return 1. / (1 + zmax + 2. * clh_lim_adz(d, ni));
 }
 __device__ __forceinline__ double clh_amax_lens(const DrawDesc& d, int ni) {
   const double zmin = (d.l_zmin[ni] - d.l_zmean[ni]) * d.par->lens_st[ni] + d.l_zmean[ni];
   if (d.par->bmag[ni] != 0) return 1. / (1. + fmax(d.s_zmin_all, 0.001));
-  return 1. / (1 + fmax(zmin - 2. * fabs(d.par->lens_dz[ni]), 0.001));
+  return 1. / (1 + fmax(zmin - 2. * clh_lim_adz(d, ni), 0.001));
 }
 
 // ------------------------------------------------ walked-index evaluations

@@ -49,6 +49,8 @@ hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, 
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
                                    hipStream_t s);
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_lim_grad(const DrawDesc* D, const DrawDesc* P, const DrawDesc* M, const int2* work, int n,
+                               hipStream_t s);
 hipError_t clh_launch_lnp_gather(const DrawDesc* D, const int2* work, int n, int ic, int nz, int nl,
                                  hipStream_t s);
 hipError_t clh_launch_legendre_mx(const DrawDesc* D, const int4* work, int n, int mode, hipStream_t s);
@@ -303,6 +305,16 @@ struct Batch {
   DBuf<int4> d_wl, d_wsx, d_wgg, d_wga, d_wmx, d_wlf;
   int n_wg = 0, n_wn = 0, n_wl = 0, n_wl2 = 0, n_wa = 0, n_wsx = 0, n_wgg = 0, n_wga = 0, n_wmx = 0, n_wlf = 0;
   int n_wsx_std = 0;  // ss/gs tiles of non-TATT draws, first in d_wsx; the TATT draws' tiles follow
+  // the VJP's lens-limit term: two views of each draw whose lens integration
+  // limits take |dz| + lim_eps and |dz| - lim_eps (own node data and C_ell
+  // rows in d_lim), the gs tiles of the non-TATT draws and the (draw, lens
+  // bin) items
+  std::vector<DrawDesc> desc_lim;
+  DBuf<DrawDesc> d_desc_lim;
+  DBuf<double> d_lim;
+  DBuf<int4> d_wsx_gs;
+  DBuf<int2> d_wlim;
+  int n_wsx_gs = 0, n_wlim = 0;
   int nrow_custom = 0;  // rows of a custom-ell batch (0: the layouts' rows)
   int gg_stage = 0;     // LDS doubles for the gg kernels' staged tables (0: not staged)
   // the device state (lens-efficiency tables, node weights, ss/gs C_ell rows)
@@ -316,6 +328,9 @@ struct Batch {
   // ss / gs: P fac): written by forwards with keep_ckpt, current while
   // ckpt_fresh (cleared with state_valid and by forwards without it)
   bool keep_ckpt = false, ckpt_fresh = false;
+  // with state_valid: the gs and gg C_ell rows are FP64 (the lens-limit term
+  // of the VJP differences them against its views)
+  bool rows64 = false;
   size_t ggk_per = 0, sxk_per = 0, dgk_per = 0;  // doubles per draw
   DBuf<double> d_ggk, d_sxk, d_dgk;  // dgk: lens-efficiency shift tangents
   int dv_size = 0;
@@ -345,6 +360,7 @@ struct Batch {
     d_desc_nl.release(); d_nl_ells.release(); d_wn_nl.release(); d_wnl_prep.release(); d_wnl_cl.release();
     d_wgg_nl.release(); d_wnl_fwd.release(); d_wnl_gl.release(); d_wnl_chk.release(); d_wnl_idft.release();
     d_wcmb.release(); d_wcmbp.release();
+    d_desc_lim.release(); d_lim.release(); d_wsx_gs.release(); d_wlim.release();
   }
 };
 
@@ -388,9 +404,9 @@ struct clh_ctx {
   bool use_graph = false;             // replay the batch forward from a hipGraph
   hipGraphExec_t gexec = nullptr;     // captured forward of `batch`
   hipStream_t gstream = nullptr;      // stream the graph was captured on
-  hipEvent_t ev[10] = {nullptr};
+  hipEvent_t ev[11] = {nullptr};
   float last_ms[5] = {0, 0, 0, 0, 0};
-  float last_vjp_ms[3] = {0, 0, 0};
+  float last_vjp_ms[4] = {0, 0, 0, 0};
   bool vjp_timed = false;
   // likelihood (IP): data and inverse covariance in the block order like_ord
   int like_n = 0, like_ord[3] = {0, 1, 2};
@@ -1094,8 +1110,8 @@ static bool ord_ok(const Layout& L, const int ord[3]) {
 // the whole batch).  rows: rows per tile of the forward ss/gs, forward gg,
 // adjoint ss/gs and adjoint gg Limber stages (1..CLH_TILE_ROWS).
 static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
-  std::vector<int2> wg, wn, wl2;
-  std::vector<int4> wl, wl_gs, wsx, wsx_t, wgg, wga, wmx, wlf, wa;
+  std::vector<int2> wg, wn, wl2, wlim;
+  std::vector<int4> wl, wl_gs, wsx, wsx_t, wsx_gs, wgg, wga, wmx, wlf, wa;
   int lds_ss = 0, lds_gs = 0;
   std::vector<int2> tiles, atiles_ss, atiles_gs;
   for (int i = 0; i < bt.B; i++) {
@@ -1111,7 +1127,11 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
     std::vector<int4>& ws = L.ia_model == CLH_IA_TATT ? wsx_t : wsx;
     for (const int2& t : tiles) ws.push_back(make_int4(i, CLH_W_SS, t.x, t.y));
     for (int zl = 0; zl < nl; zl++)
-      for (const int2& t : tiles) ws.push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
+      for (const int2& t : tiles) {
+        ws.push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
+        if (L.ia_model != CLH_IA_TATT) wsx_gs.push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
+      }
+    for (int zl = 0; zl < nl; zl++) wlim.push_back(make_int2(i, zl));
     row_tiles(nrow, rows[1], tiles);
     for (int zl = 0; zl < nl; zl++)
       for (const int2& t : tiles) wgg.push_back(make_int4(i, zl, t.x, t.y));
@@ -1169,6 +1189,8 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   wsx.insert(wsx.end(), wsx_t.begin(), wsx_t.end());
   HIPCHK(c, up4(bt.d_wsx, wsx, bt.n_wsx));
   HIPCHK(c, up4(bt.d_wgg, wgg, bt.n_wgg));
+  HIPCHK(c, up4(bt.d_wsx_gs, wsx_gs, bt.n_wsx_gs));
+  HIPCHK(c, up2(bt.d_wlim, wlim, bt.n_wlim));
   HIPCHK(c, up4(bt.d_wga, wga, bt.n_wga));
   HIPCHK(c, up4(bt.d_wmx, wmx, bt.n_wmx));
   HIPCHK(c, up4(bt.d_wlf, wlf, bt.n_wlf));
@@ -1572,6 +1594,7 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[5], s));
   bt.state_valid = true;
   bt.ckpt_fresh = bt.keep_ckpt;
+  bt.rows64 = !mixed;
   return CLH_OK;
 }
 
@@ -2715,6 +2738,7 @@ int clh_batch_forward(clh_ctx* c, const int ord[3], double* dv_device) {
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     c->batch.state_valid = true;
     c->batch.ckpt_fresh = c->batch.keep_ckpt;
+    c->batch.rows64 = c->precision == CLH_PREC_FP64;
     return CLH_OK;
   }
   return run_batch(c, c->batch, true);
@@ -2736,7 +2760,7 @@ int clh_synchronize(clh_ctx* c) {
   if (c->timing) {
     for (int k = 0; k < 5; k++) HIPCHK(c, hipEventElapsedTime(&c->last_ms[k], c->ev[k], c->ev[k + 1]));
     if (c->vjp_timed)
-      for (int k = 0; k < 3; k++) HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[k], c->ev[6 + k], c->ev[7 + k]));
+      for (int k = 0; k < 4; k++) HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[k], c->ev[6 + k], c->ev[7 + k]));
   }
   return CLH_OK;
 }
@@ -2764,13 +2788,13 @@ int clh_enable_timing(clh_ctx* c, int on) {
 }
 
 int clh_last_kernel_times(clh_ctx* c, const char** names, float* ms, int cap) {
-  static const char* nm[8] = {"k_gtab", "k_nodes", "k_limber_gg", "k_limber_sx", "k_legendre",
-                              "k_limber_gg_adj", "k_limber_sx_adj", "k_lnp_gather"};
+  static const char* nm[9] = {"k_gtab", "k_nodes", "k_limber_gg", "k_limber_sx", "k_legendre",
+                              "k_limber_gg_adj", "k_limber_sx_adj", "k_lnp_gather", "lens_limits"};
   static const char* nm_full = "k_legendre_full";
   static const char* nm32[3] = {"k_limber_gg_f32", "k_limber_sx_f32", "k_legendre_mx"};
   if (!c) return CLH_ERR_ARG;
   const bool mixed = c->precision != CLH_PREC_FP64;
-  const int n = c->vjp_timed ? 8 : 5;
+  const int n = c->vjp_timed ? 9 : 5;
   for (int k = 0; k < n && k < cap; k++) {
     if (names) {
       names[k] = (mixed && k >= 2 && k < 5) ? nm32[k - 2] : nm[k];
@@ -3052,6 +3076,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
   if (!bt.state_valid) {  // else the last forward / VJP left them for these parameters
     bt.ckpt_fresh = false;  // a checkpoint predates these parameters
+    bt.rows64 = false;      // the gg rows predate them too
     HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
     HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, 0, 0, s));
   }
@@ -3070,6 +3095,46 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (g_lnP) HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[9], s));
   c->vjp_timed = c->timing;
+  if (g_lens_dz && nl > 0) {
+    // The lens integration limits move with |dz| (redshift_spline.c:50-79),
+    // and the GL nodes of the bin's gs and gg integrals with them; the
+    // adjoint kernels above hold the nodes fixed.  That part comes from the
+    // gs / gg rows of two views whose limits take |dz| +- eps, against the
+    // FP64 rows at these parameters, contracted with their cotangent
+    // (k_lim_grad).  eps: the rounding of the rows' differences costs about
+    // 1e-3 of the term (itself below 1% of the lens-shift gradient), and about
+    // one row in 1e4 has a node crossing a table knot within eps.
+    constexpr double eps = 1e-8;
+    const int gg_st = c->gg_lds[0] ? bt.gg_stage : 0;
+    if (!bt.rows64) {  // base rows: gs (unless recomputed above in FP64) and gg
+      if (bt.state_valid) HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx_gs.p, bt.n_wsx_gs, 0, 0, s));
+      HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, 0, gg_st, 0, s));
+      bt.rows64 = true;
+    }
+    const size_t nsz = al(node_size(ns, nl)), csz = al((size_t)ncol * std::max(nrow, bt.desc[0].nrow));
+    HIPCHK(c, bt.d_lim.ensure((nsz + csz) * 2 * B));
+    HIPCHK(c, bt.d_desc_lim.ensure(2 * B));
+    bt.desc_lim.resize(2 * B);
+    for (int i = 0; i < 2 * B; i++) {
+      DrawDesc& q = bt.desc_lim[i];
+      q = bt.desc[i % B];
+      q.lim_eps = i < B ? eps : -eps;
+      q.nodes = bt.d_lim.p + (nsz + csz) * i;
+      q.cl = q.nodes + nsz;
+      q.dv = nullptr; q.cl_bb = nullptr; q.tatt = nullptr; q.cl6 = nullptr; q.nlw = nullptr;
+      q.ggk = nullptr; q.sxk = nullptr;
+    }
+    HIPCHK(c, hipMemcpyAsync(bt.d_desc_lim.p, bt.desc_lim.data(), 2 * B * sizeof(DrawDesc), hipMemcpyHostToDevice,
+                             s));
+    for (int h = 0; h < 2; h++) {
+      const DrawDesc* V = bt.d_desc_lim.p + h * B;
+      HIPCHK(c, clh_launch_nodes(V, bt.d_wn.p, bt.n_wn, s));
+      HIPCHK(c, clh_launch_limber_gg(V, bt.d_wgg.p, bt.n_wgg, 0, gg_st, 0, s));
+      HIPCHK(c, clh_launch_limber_sx(V, bt.d_wsx_gs.p, bt.n_wsx_gs, 0, 0, s));
+    }
+    HIPCHK(c, clh_launch_lim_grad(bt.d_desc.p, bt.d_desc_lim.p, bt.d_desc_lim.p + B, bt.d_wlim.p, bt.n_wlim, s));
+  }
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[10], s));
   bt.state_valid = true;
   HIPCHK(c, clh_launch_nodes_adj(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   return CLH_OK;

@@ -150,6 +150,9 @@ struct DrawDesc {
   // nrec [nl][l_rows][5] = knot z and the spline coefficients / lens stretch
   // of each lens n(z) interval (k_gtab, lens blocks)
   double* nrec;
+  // the lens integration limits take |dz| + lim_eps for dz != 0 (0 in a
+  // draw; a small step in the views that give the VJP the limits' motion)
+  double lim_eps;
   double* nodes;   // node data, offsets below
   double* cl;      // [n_ss + n_gs + nl][nrow]
   double* dv;      // data vector

@@ -38,16 +38,6 @@ __device__ __forceinline__ double clh_block_scan(double v, double* lds /* >= 4 *
   return v + base;
 }
 
-__device__ __forceinline__ double clh_block_sum(double v, double* lds) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if (lane == 0) lds[wid] = v;
-  __syncthreads();
-  const double s = lds[0] + lds[1] + lds[2] + lds[3];
-  __syncthreads();
-  return s;
-}
 
 // ----------------------------------------------------------------------------
 // k_gtab: one block per (draw, bin).  bin < ns -> g_tomo of source bin,
