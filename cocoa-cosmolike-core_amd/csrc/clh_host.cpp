@@ -19,6 +19,7 @@
 
 #include "../../include/cosmolike_hip.h"
 #include "clh_internal.hpp"
+#include "clh_place.hpp"
 
 hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
@@ -1000,36 +1001,13 @@ static void row_tiles(int nrow, int cap, std::vector<int2>& out) {
   for (int r = 0; r < nrow; r += tr) out.push_back(make_int2(r, std::min(tr, nrow - r)));
 }
 
-// block starts of compute_data_vector_Mx2pt_N_starts
-// (generic_interface.hpp:770-791): indices = stable ascending sort of ord,
-// start(i) = sum of sizes(indices(j)) over j < indices(i)
+// block starts of compute_data_vector_Mx2pt_N_starts (generic_interface.hpp:770-791)
 static void block_starts(const int sizes[3], const int ord[3], int start[3]) {
-  int idx[3] = {0, 1, 2};
-  for (int i = 1; i < 3; i++)
-    for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) std::swap(idx[j], idx[j - 1]);
-  for (int i = 0; i < 3; i++) start[i] = 0;
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
+  clh_block_starts(sizes, ord, 3, start);
 }
-
-// the formula above tiles the data vector only for an involutive order
-// ((0,1,2), (1,0,2), (2,1,0), (0,2,1)); a 3-cycle such as (1,2,0) makes
-// blocks overlap or run past the end, so such orders are refused
 static bool blocks_tile(const int sizes[3], const int ord[3]) {
-  if (!ord) return false;
-  for (int i = 0; i < 3; i++)
-    if (ord[i] < 0 || ord[i] > 2) return false;
-  if (ord[0] == ord[1] || ord[0] == ord[2] || ord[1] == ord[2]) return false;
   int start[3];
-  block_starts(sizes, ord, start);
-  const int n = sizes[0] + sizes[1] + sizes[2];
-  for (int i = 0; i < 3; i++) {
-    if (start[i] < 0 || start[i] + sizes[i] > n) return false;
-    for (int j = 0; j < i; j++)
-      if (sizes[i] > 0 && sizes[j] > 0 && start[i] < start[j] + sizes[j] && start[j] < start[i] + sizes[i])
-        return false;
-  }
-  return true;
+  return clh_blocks_tile(sizes, ord, 3, start);
 }
 
 static void dv_sizes(const Layout& L, int sizes[3]) {
@@ -1038,29 +1016,9 @@ static void dv_sizes(const Layout& L, int sizes[3]) {
 }
 
 // compute_data_vector_Mx2pt_N_starts<0,6> (generic_interface.hpp:770-791)
-// for the six blocks of 6x2pt; the same formula as block_starts
+// for the six blocks of 6x2pt
 static bool blocks_tile6(const int sizes[6], const int ord[6], int start[6]) {
-  for (int i = 0; i < 6; i++)
-    if (ord[i] < 0 || ord[i] > 5) return false;
-  for (int i = 0; i < 6; i++)
-    for (int j = 0; j < i; j++)
-      if (ord[i] == ord[j]) return false;
-  int idx[6] = {0, 1, 2, 3, 4, 5};
-  for (int i = 1; i < 6; i++)
-    for (int j = i; j > 0 && ord[idx[j - 1]] > ord[idx[j]]; j--) std::swap(idx[j], idx[j - 1]);
-  int n = 0;
-  for (int i = 0; i < 6; i++) {
-    n += sizes[i];
-    start[i] = 0;
-    for (int j = 0; j < idx[i]; j++) start[i] += sizes[idx[j]];
-  }
-  for (int i = 0; i < 6; i++) {
-    if (start[i] + sizes[i] > n) return false;
-    for (int j = 0; j < i; j++)
-      if (sizes[i] > 0 && sizes[j] > 0 && start[i] < start[j] + sizes[j] && start[j] < start[i] + sizes[i])
-        return false;
-  }
-  return true;
+  return clh_blocks_tile(sizes, ord, 6, start);
 }
 static void dv_sizes6(const Layout& L, int sizes[6]) {
   dv_sizes(L, sizes);
@@ -2252,20 +2210,9 @@ int clh_compute_3x2pt_fourier(clh_ctx* c, const int ord[3], const int* mask, dou
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   std::vector<double> out(n, 0.0);
-  const double* m = c->cur.shear_m;
-  for (int q = 0; q < nss; q++) {
-    int z1 = 0, z2 = q;
-    while (z2 >= ns - z1) { z2 -= ns - z1; z1++; }
-    z2 += z1;
-    for (int i = 0; i < ncl; i++)
-      if (L.ell_f[i] < L.lmax_shear)
-        out[start[0] + ncl * q + i] = cl[(size_t)q * ncl + i] * (1.0 + m[z1]) * (1.0 + m[z2]);
-  }
-  for (int q = 0; q < ngs; q++)
-    for (int i = 0; i < ncl; i++)
-      out[start[1] + ncl * q + i] = cl[(size_t)(nss + q) * ncl + i] * (1.0 + m[q % ns]);
-  for (int q = 0; q < nl; q++)
-    for (int i = 0; i < ncl; i++) out[start[2] + ncl * q + i] = cl[(size_t)(nss + ngs + q) * ncl + i];
+  if (clh_place_fourier(ns, nl, ncl, L.ell_f.data(), L.lmax_shear, c->cur.shear_m, start, cl.data(), out.data(), n) !=
+      n)
+    return fail(c, CLH_ERR_STATE, "Fourier placement does not fill the %d entries", n);
   if (mask)
     for (int k = 0; k < n; k++)
       if (!mask[k]) out[k] = 0.0;
