@@ -97,6 +97,8 @@ def lib():
             "clh_set_adopt_limber_gg": (i32, [VP, i32]),
             "clh_cl_nonlimber": (i32, [VP, D, ctypes.POINTER(ctypes.c_int)]),
             "clh_set_nuisance_pointmass": (i32, [VP, D, i32]),
+            "clh_init_ggl_exclude": (i32, [VP, I, i32]),
+            "clh_test_zoverlap": (i32, [VP, i32, i32]),
             "clh_set_vjp_checkpoint": (i32, [VP, i32]),
             "clh_init_binning_fourier": (i32, [VP, i32, i32, i32, i32]),
             "clh_datavector_size_fourier": (i32, [VP]),
@@ -142,6 +144,10 @@ def lib():
             "C_ss_tomo_limber_nointerp": (f64, [f64, i32, i32, i32, i32]),
             "C_gs_tomo_limber_nointerp": (f64, [f64, i32, i32, i32]),
             "C_gg_tomo_limber_nointerp": (f64, [f64, i32, i32, i32]),
+            "test_zoverlap": (i32, [i32, i32]),
+            "N_ggl": (i32, [i32, i32]),
+            "ZL": (i32, [i32]),
+            "ZS": (i32, [i32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -274,6 +274,7 @@ class Cosmolike:
         self.set_source_sample(prob.source)
         if prob.lens is not None:
             self.set_lens_sample(prob.lens)
+        self.init_ggl_exclude(getattr(prob, "ggl_exclude", None))
         self.set_draw(prob)
         return self
 
@@ -329,6 +330,24 @@ class Cosmolike:
         self._chk(self.L.clh_compute_3x2pt_fourier(
             self.h, I3, None if m is None else m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(out), n))
         return out
+
+    def init_ggl_exclude(self, pairs):
+        """init_ggl_exclude: (lens, source) pairs left out of gamma_t, flat
+        [zl0, zs0, zl1, zs1, ...] or a list of pairs; None or empty clears."""
+        a = np.ascontiguousarray(np.asarray([] if pairs is None else pairs, dtype=np.int32).ravel())
+        self._chk(self.L.clh_init_ggl_exclude(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+                                              if a.size else None, int(a.size)))
+
+    def test_zoverlap(self, zl, zs):
+        r = self.L.clh_test_zoverlap(self.h, int(zl), int(zs))
+        if r < 0:
+            self._chk(r)
+        return r
+
+    @property
+    def n_ggl(self):
+        """Included gs pairs (tomo.ggl_Npowerspectra)."""
+        return sum(self.test_zoverlap(zl, zs) for zl in range(self.nl) for zs in range(self.ns))
 
     def set_nuisance_pointmass(self, pm):
         pm = _arr(pm)
@@ -395,9 +414,11 @@ class Cosmolike:
         return xp.reshape(self.n_ss, self.ntheta), xm.reshape(self.n_ss, self.ntheta)
 
     def gammat(self):
-        out = np.empty(self.n_gs * self.ntheta)
+        """[included pairs][ntheta] (every pair without init_ggl_exclude)."""
+        n = self.n_ggl
+        out = np.empty(n * self.ntheta)
         self._chk(self.L.clh_gammat(self.h, _p(out)))
-        return out.reshape(self.n_gs, self.ntheta)
+        return out.reshape(n, self.ntheta)
 
     def wtheta(self):
         out = np.empty(self.nl * self.ntheta)

@@ -94,13 +94,14 @@ __global__ __launch_bounds__(256) void k_leg_adjoint(const DrawDesc* __restrict_
         canon = nt * q + th + (sk ? nt * nss : 0);
         idx = d.dv_start[0] + canon;
       } else if (kind == 2) {
-        canon = 2 * nt * nss + nt * q + th;
-        idx = d.dv_start[1] + nt * q + th;
+        const int r = d.gs_row ? d.gs_row[q] : q;  // -1: an excluded pair, no data-vector row
+        canon = 2 * nt * nss + nt * r + th;
+        idx = r < 0 ? -1 : d.dv_start[1] + nt * r + th;
       } else {
-        canon = nt * (2 * nss + ngs) + nt * q + th;
+        canon = nt * (2 * nss + d.n_gs_dv) + nt * q + th;
         idx = d.dv_start[2] + nt * q + th;
       }
-      v = clh_masked(d, canon) ? 0.0 : d.dvbar[idx];  // masked entries are constant 0
+      v = (idx < 0 || clh_masked(d, canon)) ? 0.0 : d.dvbar[idx];  // masked entries are constant 0
     }
     vb[sk][cc][th] = v;
   }

@@ -134,7 +134,7 @@ __device__ __forceinline__ double clh_ckk(const DrawDesc& d, const double* C, do
 __global__ __launch_bounds__(256) void k_cmb_place(const DrawDesc* __restrict__ D, const int* __restrict__ work) {
   const DrawDesc& d = D[work[blockIdx.x]];
   const int nt = d.ntheta, nrow = d.nrow;
-  const int base = nt * (2 * d.n_ss + d.n_gs + d.nl);  // canonical start of the gk block
+  const int base = nt * (2 * d.n_ss + d.n_gs_dv + d.nl);  // canonical start of the gk block
   const int nw = (d.nl + d.ns) * nt;
   const double* Ckk = d.cl6 + (size_t)(d.nl + d.ns) * d.nrow6;
   const int nf = d.kk_nbp > 0 ? 0 : d.n_f;

@@ -190,10 +190,34 @@ struct Layout {
     for (int l = kk_lmin; l <= std::min(kk_lmax, CLH_LMIN_TAB); l++) e.push_back((double)l);
     return e;
   }
+  // excluded (lens, source) pairs of gamma_t / C_gs (init_ggl_exclude,
+  // generic_interface.cpp:874-896), flat (zl, zs) pairs.  Every pair keeps
+  // its C_ell column (n_gs() = ns nl); the data vector holds the included
+  // ones in N_ggl order (redshift_spline.c:199-223).
+  std::vector<int> ggl_ex;
+  bool zoverlap(int zl, int zs) const {  // test_zoverlap (redshift_spline.c:115-145)
+    for (size_t k = 0; k + 1 < ggl_ex.size(); k += 2)
+      if (ggl_ex[k] == zl && ggl_ex[k + 1] == zs) return false;
+    return true;
+  }
+  // data-vector row of C_gs column zl ns + zs, -1 when excluded (N_ggl)
+  std::vector<int> gs_rows() const {
+    std::vector<int> r(n_gs());
+    int n = 0;
+    for (int zl = 0; zl < lens.nbin; zl++)
+      for (int zs = 0; zs < src.nbin; zs++) r[zl * src.nbin + zs] = zoverlap(zl, zs) ? n++ : -1;
+    return r;
+  }
   int nrow() const { return CLH_NLOW + n_ell; }
   int n_ss() const { return src.nbin * (src.nbin + 1) / 2; }
   int n_gs() const { return src.nbin * lens.nbin; }
-  int dv_size3() const { return ntheta * (2 * n_ss() + n_gs() + lens.nbin); }
+  int n_gs_dv() const {  // tomo.ggl_Npowerspectra
+    int n = 0;
+    for (int zl = 0; zl < lens.nbin; zl++)
+      for (int zs = 0; zs < src.nbin; zs++) n += zoverlap(zl, zs) ? 1 : 0;
+    return n;
+  }
+  int dv_size3() const { return ntheta * (2 * n_ss() + n_gs_dv() + lens.nbin); }
   int dv_size() const {
     return dv_size3() + (sixpt ? ntheta * (lens.nbin + src.nbin) + kk_size() : 0);
   }
@@ -271,10 +295,13 @@ struct DevLayout {  // device copy of one survey layout
   DBuf<uint16_t> kbf;
   int ngrp = 0;
   DBuf<int> mask;       // likelihood mask (null when the layout has none)
+  DBuf<int> gsrow;      // data-vector row per C_gs column (null without excluded pairs)
   size_t off_s_zc = 0, off_s_coef = 0, off_l_zc = 0, off_l_coef = 0, off_ell_ss = 0,
          off_ell_gx = 0, off_kfold = 0, off_kfoldT = 0, off_gl = 0, off_kfull = 0, off_fill = 0, off_theta = 0,
          off_kwlow = 0, off_nltw = 0, off_kf6 = 0, off_ellf = 0, off_kkM = 0, off_kkoff = 0;
-  void release() { buf.release(); k32.release(); ksc.release(); kbf.release(); mask.release(); valid = false; }
+  void release() {
+    buf.release(); k32.release(); ksc.release(); kbf.release(); mask.release(); gsrow.release(); valid = false;
+  }
 };
 
 struct LayoutCopy {  // a context's own copy of a captured layout, with its device copy
@@ -928,6 +955,11 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
     HIPCHK(c, hipMemcpyAsync(D.mask.p, L.mask.data(), L.mask.size() * sizeof(int), hipMemcpyHostToDevice,
                              c->stream));
   }
+  if (!L.ggl_ex.empty() && L.n_gs() > 0) {
+    const std::vector<int> rows = L.gs_rows();
+    HIPCHK(c, D.gsrow.ensure(rows.size()));
+    HIPCHK(c, hipMemcpyAsync(D.gsrow.p, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  }
   if (full) HIPCHK(c, put(D.off_kfull, L.kfull.data(), L.kfull.size()));
   if (full) HIPCHK(c, put(D.off_fill, fill.data(), fill.size()));
   // Folded kernels as MFMA A operands (k_legendre_mx): group g of 8 rows,
@@ -1012,7 +1044,7 @@ static bool blocks_tile(const int sizes[3], const int ord[3]) {
 
 static void dv_sizes(const Layout& L, int sizes[3]) {
   const int nt = L.ntheta;
-  sizes[0] = 2 * nt * L.n_ss(); sizes[1] = nt * L.n_gs(); sizes[2] = nt * L.lens.nbin;
+  sizes[0] = 2 * nt * L.n_ss(); sizes[1] = nt * L.n_gs_dv(); sizes[2] = nt * L.lens.nbin;
 }
 
 // compute_data_vector_Mx2pt_N_starts<0,6> (generic_interface.hpp:770-791)
@@ -1344,6 +1376,8 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.kmx_ngrp = DL.ngrp;
     q.theta = lay + DL.off_theta;
     q.mask = L.mask.empty() ? nullptr : DL.mask.p;
+    q.gs_row = (L.ggl_ex.empty() || L.n_gs() == 0) ? nullptr : DL.gsrow.p;
+    q.n_gs_dv = L.n_gs_dv();
     const size_t gtab_sz = (size_t)(ns + nl) * L.n_a;
     const size_t node_sz = node_size(ns, nl);
     double* wb = bt.d_work.p + woff[i];
@@ -2011,6 +2045,29 @@ int clh_set_nuisance_pointmass(clh_ctx* c, const double* pm, int n) {
   return set_vec(c, c->cur.pm, pm, n, c ? c->L.lens.nbin : 0, "set_pointmass");
 }
 
+// init_ggl_exclude (generic_interface.cpp:874-896): n/2 (lens, source) pairs,
+// flat; n = 0 clears.  The pairs hold until the next call, whatever samples
+// are set later (test_zoverlap reads them against the current bin counts).
+int clh_init_ggl_exclude(clh_ctx* c, const int* pairs, int n) {
+  if (!c) return CLH_ERR_ARG;
+  if (n < 0 || (n > 1 && !pairs)) return fail(c, CLH_ERR_ARG, "init_ggl_exclude: %d entries", n);
+  if (n / 2 > CLH_MAXB * CLH_MAXB) return fail(c, CLH_ERR_ARG, "init_ggl_exclude: %d pairs", n / 2);
+  c->L.ggl_ex.assign(pairs ? pairs : nullptr, pairs ? pairs + 2 * (n / 2) : nullptr);
+  if (!c->L.mask.empty() && (int)c->L.mask.size() != c->L.dv_size()) c->L.mask.clear();
+  layout_changed(c);
+  c->version++;
+  return CLH_OK;
+}
+
+// test_zoverlap (redshift_spline.c:115-145): 1 if the pair is in the data
+// vector, 0 if excluded, CLH_ERR_ARG for bins out of range
+int clh_test_zoverlap(clh_ctx* c, int zl, int zs) {
+  if (!c) return CLH_ERR_ARG;
+  if (zl < 0 || zl >= c->L.lens.nbin || zs < 0 || zs >= c->L.src.nbin)
+    return fail(c, CLH_ERR_ARG, "invalid bin input (ni, nj) = (%d, %d)", zl, zs);
+  return c->L.zoverlap(zl, zs) ? 1 : 0;
+}
+
 // data-vector index in the block order `ord` of index k in the (0,1,2) order
 static int dv_index_from_canonical(const Layout& L, const int ord[3], int k) {
   const int nt = L.ntheta;
@@ -2022,7 +2079,7 @@ static int dv_index_from_canonical(const Layout& L, const int ord[3], int k) {
     const int b = k >= cs[2] ? 2 : (k >= cs[1] ? 1 : 0);
     return start6[b] + (k - cs[b]);
   }
-  const int canon_start[3] = {0, 2 * nt * L.n_ss(), nt * (2 * L.n_ss() + L.n_gs())};
+  const int canon_start[3] = {0, 2 * nt * L.n_ss(), nt * (2 * L.n_ss() + L.n_gs_dv())};
   const int b = k >= canon_start[2] ? 2 : (k >= canon_start[1] ? 1 : 0);
   return start[b] + (k - canon_start[b]);
 }
@@ -2180,7 +2237,7 @@ int clh_init_binning_fourier(clh_ctx* c, int nells, int lmin, int lmax, int lmax
 
 int clh_datavector_size_fourier(clh_ctx* c) {
   if (!c) return 0;
-  return (int)c->L.ell_f.size() * (c->L.n_ss() + c->L.n_gs() + c->L.lens.nbin);
+  return (int)c->L.ell_f.size() * (c->L.n_ss() + c->L.n_gs_dv() + c->L.lens.nbin);
 }
 
 // compute_Mx2pt_N_masked<1,3> (generic_interface.hpp:1137-1159, 968-1041)
@@ -2195,7 +2252,7 @@ int clh_compute_3x2pt_fourier(clh_ctx* c, const int ord[3], const int* mask, dou
   if (ncl == 0) return fail(c, CLH_ERR_STATE, "Fourier binning not initialized");
   if (n != clh_datavector_size_fourier(c)) return fail(c, CLH_ERR_ARG, "dv size %d != %d", n, clh_datavector_size_fourier(c));
   const int nss = L.n_ss(), ngs = L.n_gs(), nl = L.lens.nbin, ns = L.src.nbin;
-  const int sizes[3] = {ncl * nss, ncl * ngs, ncl * nl};
+  const int sizes[3] = {ncl * nss, ncl * L.n_gs_dv(), ncl * nl};
   if (!blocks_tile(sizes, ord))
     return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");
   int start[3];
@@ -2210,8 +2267,9 @@ int clh_compute_3x2pt_fourier(clh_ctx* c, const int ord[3], const int* mask, dou
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   std::vector<double> out(n, 0.0);
-  if (clh_place_fourier(ns, nl, ncl, L.ell_f.data(), L.lmax_shear, c->cur.shear_m, start, cl.data(), out.data(), n) !=
-      n)
+  const std::vector<int> rows = L.gs_rows();
+  if (clh_place_fourier(ns, nl, ncl, L.ell_f.data(), L.lmax_shear, c->cur.shear_m, rows.data(), start, cl.data(),
+                        out.data(), n) != n)
     return fail(c, CLH_ERR_STATE, "Fourier placement does not fill the %d entries", n);
   if (mask)
     for (int k = 0; k < n; k++)
@@ -2320,7 +2378,7 @@ int clh_gammat(clh_ctx* c, double* out) {
   int rc = ensure_current(c);
   if (rc) return rc;
   const int m = c->L.n_ss() * c->L.ntheta;
-  std::copy(c->cache_raw.begin() + 2 * m, c->cache_raw.begin() + 2 * m + c->L.n_gs() * c->L.ntheta, out);
+  std::copy(c->cache_raw.begin() + 2 * m, c->cache_raw.begin() + 2 * m + c->L.n_gs_dv() * c->L.ntheta, out);
   return CLH_OK;
 }
 
@@ -2328,7 +2386,7 @@ int clh_wtheta(clh_ctx* c, double* out) {
   if (!c || !out) return CLH_ERR_ARG;
   int rc = ensure_current(c);
   if (rc) return rc;
-  const int m = c->L.ntheta * (2 * c->L.n_ss() + c->L.n_gs());
+  const int m = c->L.ntheta * (2 * c->L.n_ss() + c->L.n_gs_dv());
   std::copy(c->cache_raw.begin() + m, c->cache_raw.begin() + m + c->L.lens.nbin * c->L.ntheta, out);
   return CLH_OK;
 }
@@ -3130,7 +3188,7 @@ static double table_lookup(const clh_ctx* c, int col, bool ss, double l, bool bb
 static const double* w_block(clh_ctx* c, int limber) {
   if (limber == (c->L.nonlimber ? 0 : 1)) {
     if (ensure_current(c)) return nullptr;
-    return c->cache_raw.data() + (size_t)(2 * c->L.n_ss() + c->L.n_gs()) * c->L.ntheta;
+    return c->cache_raw.data() + (size_t)(2 * c->L.n_ss() + c->L.n_gs_dv()) * c->L.ntheta;
   }
   if (c->alt_w_version == c->version && c->alt_w_limber == limber) return c->alt_w.data();
   if (check_layout(c, c->L) || ensure_device(c)) return nullptr;
@@ -3147,7 +3205,7 @@ static const double* w_block(clh_ctx* c, int limber) {
   if (run_batch(c, c->scratch, true)) return nullptr;
   const int nw = c->L.lens.nbin * c->L.ntheta;
   c->alt_w.resize(nw);
-  const double* src = c->scratch.desc[0].dv + (size_t)(2 * c->L.n_ss() + c->L.n_gs()) * c->L.ntheta;
+  const double* src = c->scratch.desc[0].dv + (size_t)(2 * c->L.n_ss() + c->L.n_gs_dv()) * c->L.ntheta;
   if (hipMemcpyAsync(c->alt_w.data(), src, nw * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
     fail(c, CLH_ERR_HIP, "w_gg_tomo: copy of w(theta) failed");
@@ -3188,8 +3246,9 @@ double w_gammat_tomo(const int nt, const int ni, const int nj, const int limber)
     fail(c, CLH_ERR_ARG, "error in selecting bin number (ni, nj) = [%d,%d]", ni, nj);
     return nanv();
   }
+  if (!c->L.zoverlap(ni, nj)) return 0.0;  // cosmo2D.c:722-732
   if (ensure_current(c)) return nanv();
-  const int q = (ni * c->L.src.nbin + nj) * c->L.ntheta + nt;
+  const int q = c->L.gs_rows()[ni * c->L.src.nbin + nj] * c->L.ntheta + nt;
   return c->cache_raw[2 * c->L.n_ss() * c->L.ntheta + q];
 }
 
@@ -3225,6 +3284,7 @@ double C_gs_tomo_limber(const double l, const int ni, const int nj) {
     fail(c, CLH_ERR_ARG, "error in selecting bin number (ni, nj) = [%d,%d]", ni, nj);
     return nanv();
   }
+  if (!c->L.zoverlap(ni, nj)) return 0.0;  // cosmo2D.c:2376-2391
   if (ensure_current(c)) return nanv();
   return table_lookup(c, c->L.n_ss() + ni * c->L.src.nbin + nj, false, l);
 }
@@ -3283,5 +3343,35 @@ double C_gg_tomo_limber_nointerp(const double l, const int ni, const int nj, con
   if (clh_cl_nointerp(c, 2, &l, 1, out.data())) return nanv();
   return out[ni];
 }
+
+// the gs pair maps of redshift_spline.c:115-223 on the default context
+int test_zoverlap(int ni, int nj) {
+  clh_ctx* c = clh_default();
+  if (!c) return -1;
+  const int r = clh_test_zoverlap(c, ni, nj);
+  return r < 0 ? -1 : r;
+}
+
+int N_ggl(int ni, int nj) {
+  clh_ctx* c = clh_default();
+  if (!c) return -1;
+  if (ni < 0 || ni >= c->L.lens.nbin || nj < 0 || nj >= c->L.src.nbin) {
+    fail(c, CLH_ERR_ARG, "invalid bin input (ni, nj) = (%d, %d)", ni, nj);
+    return -1;
+  }
+  return c->L.gs_rows()[ni * c->L.src.nbin + nj];
+}
+
+static int ggl_pair_bin(int nz, bool lens) {
+  clh_ctx* c = clh_default();
+  if (!c) return -1;
+  const std::vector<int> rows = c->L.gs_rows();
+  for (size_t q = 0; q < rows.size(); q++)
+    if (rows[q] == nz && nz >= 0) return lens ? (int)q / c->L.src.nbin : (int)q % c->L.src.nbin;
+  fail(c, CLH_ERR_ARG, "invalid bin input %d (max %d)", nz, c->L.n_gs_dv());
+  return -1;
+}
+int ZL(int ni) { return ggl_pair_bin(ni, true); }
+int ZS(int nj) { return ggl_pair_bin(nj, false); }
 
 }  // extern "C"

@@ -165,6 +165,10 @@ struct DrawDesc {
   int apply_calib;  // 1: shear calibration, point mass and mask applied to the data vector
   const double* theta;  // [ntheta] bin-averaged theta [rad] (compute_binning_real_space)
   const int* mask;      // [dv size] 0/1 in the (0,1,2) block order, or null (all ones)
+  // data-vector row of C_gs column zl ns + zs, -1 for an excluded pair
+  // (init_ggl_exclude), or null (every pair, row = column); n_gs_dv rows
+  const int* gs_row;
+  int n_gs_dv;
 
   // adjoint (VJP) buffers of the draw
   const double* dvbar;  // cotangent of the data vector

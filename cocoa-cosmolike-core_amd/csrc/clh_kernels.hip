@@ -724,17 +724,18 @@ __device__ __forceinline__ void clh_place(const DrawDesc& d, int kind, int q, in
     idx = d.dv_start[0] + o;
     canon = o;
   } else if (kind == 2) {
-    const int zs = q % d.ns;
+    const int zs = q % d.ns, r = d.gs_row ? d.gs_row[q] : q;
+    if (r < 0) return;  // an excluded pair has no data-vector row
     if (d.apply_calib) {
       f = 1.0 + d.par->shear_m[zs];
       const double pmf = clh_pm_fac(d, q / d.ns, zs);
       if (pmf != 0.0) acc += pmf / (d.theta[th] * d.theta[th]);
     }
-    idx = d.dv_start[1] + nt * q + th;
-    canon = 2 * nt * d.n_ss + nt * q + th;
+    idx = d.dv_start[1] + nt * r + th;
+    canon = 2 * nt * d.n_ss + nt * r + th;
   } else {
     idx = d.dv_start[2] + nt * q + th;
-    canon = nt * (2 * d.n_ss + d.n_gs) + nt * q + th;
+    canon = nt * (2 * d.n_ss + d.n_gs_dv) + nt * q + th;
   }
   d.dv[idx] = clh_masked(d, canon) ? 0.0 : acc * f;
 }

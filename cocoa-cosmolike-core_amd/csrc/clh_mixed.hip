@@ -116,16 +116,17 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre_mx(const DrawDesc* __res
     canon0 = nt * q + (kind == 1 ? nt * d.n_ss : 0);
     base = d.dv_start[0] + canon0;
   } else if (kind == 2) {
-    const int zs = q % d.ns;
+    const int zs = q % d.ns, r = d.gs_row ? d.gs_row[q] : q;
+    if (r < 0) return;  // an excluded pair has no data-vector row
     if (d.apply_calib) {
       f = 1.0 + d.par->shear_m[zs];
       pmf = clh_pm_fac(d, q / d.ns, zs);  // point mass (generic_interface.cpp:2626-2641)
     }
-    base = d.dv_start[1] + nt * q;
-    canon0 = 2 * nt * d.n_ss + nt * q;
+    base = d.dv_start[1] + nt * r;
+    canon0 = 2 * nt * d.n_ss + nt * r;
   } else {
     base = d.dv_start[2] + nt * q;
-    canon0 = nt * (2 * d.n_ss + d.n_gs) + nt * q;
+    canon0 = nt * (2 * d.n_ss + d.n_gs_dv) + nt * q;
   }
 #pragma unroll
   for (int v = 0; v < 16; v++) {

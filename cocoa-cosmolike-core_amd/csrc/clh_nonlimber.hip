@@ -460,7 +460,7 @@ __global__ __launch_bounds__(256) void k_nl_cl(const DrawDesc* __restrict__ D, c
     double dw = 0.0;
     for (int k = 1; k < CLH_NL_L; k++) dw += pw[k] * DL[k];
     const int nt = d.ntheta;
-    const int canon = nt * (2 * d.n_ss + d.n_gs) + nt * i + l;
+    const int canon = nt * (2 * d.n_ss + d.n_gs_dv) + nt * i + l;
     if (!clh_masked(d, canon)) d.dv[d.dv_start[2] + nt * i + l] += dw;
   }
 }

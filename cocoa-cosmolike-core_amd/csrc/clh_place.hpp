@@ -5,6 +5,7 @@
 #pragma once
 
 #include <utility>
+#include <vector>
 
 // compute_data_vector_Mx2pt_N_starts (generic_interface.hpp:770-791) for M
 // blocks: indices = stable ascending sort of ord, start(i) = sum of
@@ -46,15 +47,30 @@ inline bool clh_blocks_tile(const int* sizes, const int* ord, int M, int* start)
 
 // The Fourier-space 3x2pt vector (compute_Mx2pt_N_masked<1,3>,
 // generic_interface.cpp): blocks ss (EE, zero at ell >= lmax_shear), gs, gg
-// of ncl ells each, ss pairs in N_shear order (z1 <= z2), gs pairs as
-// zl * ns + zs with calibration (1 + m_zs), ss with (1 + m_z1)(1 + m_z2).
-// cl: [n_ss + n_gs + nl][ncl] rows.  Every write is checked against n
-// (returns -1 and writes nothing if a block would leave out[0, n)); returns
-// the number of entries written.
+// of ncl ells each, ss pairs in N_shear order (z1 <= z2), gs columns
+// zl * ns + zs at data-vector row gs_row[zl * ns + zs] (-1: an excluded pair,
+// not placed; null: every pair, row = column) with calibration (1 + m_zs),
+// ss with (1 + m_z1)(1 + m_z2).  cl: [n_ss + ns nl + nl][ncl] rows.  Every
+// write is checked against n (returns -1 and writes nothing if a block would
+// leave out[0, n)); returns the number of entries written.
 inline int clh_place_fourier(int ns, int nl, int ncl, const double* ell, double lmax_shear, const double* m,
-                             const int start[3], const double* cl, double* out, int n) {
+                             const int* gs_row, const int start[3], const double* cl, double* out, int n) {
   const int nss = ns * (ns + 1) / 2, ngs = ns * nl;
-  const int rows[3] = {nss, ngs, nl};
+  // the included pairs' rows are 0 .. ngs_dv - 1, each once
+  int ngs_dv = 0;
+  std::vector<char> seen(ngs > 0 ? ngs : 1, 0);
+  for (int q = 0; q < ngs; q++) {
+    const int r = gs_row ? gs_row[q] : q;
+    if (r < -1 || r >= ngs) return -1;
+    if (r >= 0) {
+      if (seen[r]) return -1;
+      seen[r] = 1;
+      ngs_dv++;
+    }
+  }
+  for (int r = 0; r < ngs_dv; r++)
+    if (!seen[r]) return -1;
+  const int rows[3] = {nss, ngs_dv, nl};
   for (int k = 0; k < 3; k++)
     if (start[k] < 0 || (long long)start[k] + (long long)rows[k] * ncl > n) return -1;
   int w = 0;
@@ -69,9 +85,12 @@ inline int clh_place_fourier(int ns, int nl, int ncl, const double* ell, double 
       out[start[0] + ncl * q + i] =
           ell[i] < lmax_shear ? cl[(long long)q * ncl + i] * (1.0 + m[z1]) * (1.0 + m[z2]) : 0.0;
   }
-  for (int q = 0; q < ngs; q++)
+  for (int q = 0; q < ngs; q++) {
+    const int r = gs_row ? gs_row[q] : q;
+    if (r < 0) continue;
     for (int i = 0; i < ncl; i++, w++)
-      out[start[1] + ncl * q + i] = cl[(long long)(nss + q) * ncl + i] * (1.0 + m[q % ns]);
+      out[start[1] + ncl * r + i] = cl[(long long)(nss + q) * ncl + i] * (1.0 + m[q % ns]);
+  }
   for (int q = 0; q < nl; q++)
     for (int i = 0; i < ncl; i++, w++) out[start[2] + ncl * q + i] = cl[(long long)(nss + ngs + q) * ncl + i];
   return w;

@@ -144,6 +144,17 @@ int clh_set_IA_PS(clh_ctx* ctx, const double* PS, double kmin, double kmax, doub
  * generic_interface.cpp:2626-2641; add_calib_and_set_mask_X_N,
  * generic_interface.hpp:843-858). Default 0. */
 int clh_set_nuisance_pointmass(clh_ctx* ctx, const double* pm, int n);
+/* init_ggl_exclude (generic_interface.cpp:874-896): n/2 (lens, source) bin
+ * pairs, flat [zl0, zs0, zl1, zs1, ...], left out of gamma_t / C_gs; n = 0
+ * clears.  The gamma_t block of the data vectors then holds the included
+ * pairs in N_ggl order (lens major; tomo.ggl_Npowerspectra of them,
+ * generic_interface.cpp:775-802), clh_gammat returns those rows, and
+ * w_gammat_tomo / C_gs_tomo_limber return 0 for an excluded pair
+ * (cosmo2D.c:722-732, 2376-2391).  A mask of the old size is dropped. */
+int clh_init_ggl_exclude(clh_ctx* ctx, const int* pairs, int n);
+/* test_zoverlap (redshift_spline.c:115-145): 1 included, 0 excluded,
+ * CLH_ERR_ARG for bins out of range */
+int clh_test_zoverlap(clh_ctx* ctx, int zl, int zs);
 
 /* ---------------------------------------------------- Fourier-space 3x2pt */
 /* init_binning_fourier (generic_interface.cpp:425-470): like.ell at the
@@ -234,8 +245,9 @@ int clh_compute_3x2pt_real(clh_ctx* ctx, const int ord[3], double* dv, int n);
  * shared array. */
 int clh_compute_3x2pt_real_noalloc(clh_ctx* ctx, const int ord[3], double* dv, int n);
 /* per-probe arrays as the reference's static caches hold them (q = pair*Ntheta + i),
- * without calibration: xi_pm_tomo (cosmo2D.c:353), w_gammat_tomo (:562),
- * w_gg_tomo (:739; Limber or not as clh_set_adopt_limber_gg selects) */
+ * without calibration: xi_pm_tomo (cosmo2D.c:353), w_gammat_tomo (:562; the
+ * included pairs, clh_init_ggl_exclude), w_gg_tomo (:739; Limber or not as
+ * clh_set_adopt_limber_gg selects) */
 int clh_xi_pm(clh_ctx* ctx, double* xip, double* xim);
 int clh_gammat(clh_ctx* ctx, double* out);
 int clh_wtheta(clh_ctx* ctx, double* out);
@@ -377,6 +389,13 @@ double C_ss_tomo_limber_nointerp(const double l, const int ni, const int nj, con
                                  const int init);
 double C_gs_tomo_limber_nointerp(const double l, const int nl, const int ns, const int init);
 double C_gg_tomo_limber_nointerp(const double l, const int ni, const int nj, const int init);
+/* gs pair maps, redshift_spline.h (redshift_spline.c:115-223): test_zoverlap,
+ * N_ggl (data-vector pair index of (lens, source), -1 if excluded), ZL / ZS
+ * (lens / source bin of pair index n); -1 on a bin out of range */
+int test_zoverlap(int ni, int nj);
+int N_ggl(int ni, int nj);
+int ZL(int ni);
+int ZS(int nj);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,8 @@ struct oc_ctx {
   double src_dz[OC_MAXB], lens_dz[OC_MAXB], lens_st[OC_MAXB], b1[OC_MAXB],
       bmag[OC_MAXB], shear_m[OC_MAXB];
   double pm[OC_MAXB]; /* point-mass amplitudes per lens bin (PointMass::pm_) */
+  int ggl_ex[2 * OC_MAXB * OC_MAXB]; /* excluded (lens, source) pairs (tomo.ggl_exclude) */
+  int n_ggl_ex;
   int* mask;          /* likelihood mask in the (0,1,2) block order, or NULL */
   int mask_n;
   int ia_mode;
@@ -724,6 +726,33 @@ static void ss_pair(int ns, int q, int* z1, int* z2) {
 }
 int oc_npairs_ss(oc_ctx* c) { return c->ns * (c->ns + 1) / 2; }
 int oc_npairs_gs(oc_ctx* c) { return c->nl * c->ns; }
+
+/* test_zoverlap (redshift_spline.c:115-145): 0 for a listed (lens, source)
+   pair of tomo.ggl_exclude, else 1 */
+int oc_test_zoverlap(oc_ctx* c, int zl, int zs) {
+  for (int k = 0; k < c->n_ggl_ex; k++)
+    if (c->ggl_ex[2 * k] == zl && c->ggl_ex[2 * k + 1] == zs) return 0;
+  return 1;
+}
+
+/* tomo.ggl_Npowerspectra (init_ntomo_powerspectra, generic_interface.cpp:775-802):
+   the included pairs, which the gamma_t block holds in N_ggl order (lens
+   major, source minor; redshift_spline.c:199-223) */
+int oc_npairs_ggl(oc_ctx* c) {
+  int n = 0;
+  for (int zl = 0; zl < c->nl; zl++)
+    for (int zs = 0; zs < c->ns; zs++) n += oc_test_zoverlap(c, zl, zs);
+  return n;
+}
+
+/* init_ggl_exclude (generic_interface.cpp:874-896): n entries, pairs
+   (lens, source) */
+int oc_set_ggl_exclude(oc_ctx* c, const int* pairs, int n) {
+  if (n < 0 || n / 2 > OC_MAXB * OC_MAXB || (n > 0 && !pairs)) return -1;
+  c->n_ggl_ex = n / 2;
+  for (int k = 0; k < 2 * c->n_ggl_ex; k++) c->ggl_ex[k] = pairs[k];
+  return 0;
+}
 
 /* ---------------------------------------------------------------------- */
 /* Limber integrands                                                       */
@@ -1501,7 +1530,7 @@ int oc_wtheta(oc_ctx* c, double* out) {
 }
 
 int oc_datavector_size(oc_ctx* c) {
-  return c->ntheta * (2 * oc_npairs_ss(c) + oc_npairs_gs(c) + c->nl);
+  return c->ntheta * (2 * oc_npairs_ss(c) + oc_npairs_ggl(c) + c->nl);
 }
 
 /* compute_Mx2pt_N_masked<0,3> with every mask entry 1 and no point mass
@@ -1553,7 +1582,7 @@ static int block_starts(const int sizes[3], const int ord[3], int start[3]) {
 int oc_datavector_3x2pt(oc_ctx* c, const int ord[3], double* dv) {
   c->status = 0;
   const int nt = c->ntheta, nss = oc_npairs_ss(c), ngs = oc_npairs_gs(c), ngg = c->nl;
-  const int sizes[3] = {2 * nt * nss, nt * ngs, nt * ngg};
+  const int sizes[3] = {2 * nt * nss, nt * oc_npairs_ggl(c), nt * ngg};
   int start[3];
   if (block_starts(sizes, ord, start)) { oc_set_err(c, "ord: blocks do not tile the data vector"); return -1; }
   const int n = sizes[0] + sizes[1] + sizes[2];
@@ -1581,16 +1610,18 @@ int oc_datavector_3x2pt(oc_ctx* c, const int ord[3], double* dv) {
      generic_interface.hpp:843-858; PointMass::get_pm, generic_interface.cpp:2626-2641) */
   double* theta = malloc(sizeof(double) * nt);
   binning_theta(c, theta);
-  for (int nz = 0; nz < ngs; nz++) {
+  for (int nz = 0, r = 0; nz < ngs; nz++) {
     const int zs = nz % c->ns, zl = nz / c->ns;
+    if (!oc_test_zoverlap(c, zl, zs)) continue;
     for (int i = 0; i < nt; i++)
-      dv[start[1] + nt * nz + i] = (gt[nz * nt + i] + point_mass(c, zl, zs, theta[i])) * (1.0 + c->shear_m[zs]);
+      dv[start[1] + nt * r + i] = (gt[nz * nt + i] + point_mass(c, zl, zs, theta[i])) * (1.0 + c->shear_m[zs]);
+    r++;
   }
   for (int nz = 0; nz < ngg; nz++)
     for (int i = 0; i < nt; i++) dv[start[2] + nt * nz + i] = w[nz * nt + i];
   /* masked entries are 0 */
   if (c->mask) {
-    const int cs[3] = {0, 2 * nt * nss, nt * (2 * nss + ngs)};
+    const int cs[3] = {0, 2 * nt * nss, 2 * nt * nss + sizes[1]};
     for (int k = 0; k < n; k++) {
       if (c->mask[k]) continue;
       const int b = k >= cs[2] ? 2 : (k >= cs[1] ? 1 : 0);
@@ -1609,7 +1640,7 @@ int oc_datavector_3x2pt(oc_ctx* c, const int ord[3], double* dv) {
 int oc_datavector_fourier(oc_ctx* c, const double* ell, int ncl, double lmax_shear, const int ord[3],
                           const int* mask, double* dv) {
   const int nss = oc_npairs_ss(c), ngs = oc_npairs_gs(c), ngg = c->nl;
-  const int sizes[3] = {ncl * nss, ncl * ngs, ncl * ngg};
+  const int sizes[3] = {ncl * nss, ncl * oc_npairs_ggl(c), ncl * ngg};
   int start[3];
   c->status = 0;
   if (block_starts(sizes, ord, start)) { oc_set_err(c, "ord: blocks do not tile the data vector"); return -1; }
@@ -1628,8 +1659,11 @@ int oc_datavector_fourier(oc_ctx* c, const double* ell, int ncl, double lmax_she
       dv[k] = ell[i] < lmax_shear ? ss[nz * ncl + i] * (1.0 + c->shear_m[z1]) * (1.0 + c->shear_m[z2]) : 0.0;
     }
   }
-  for (int nz = 0; nz < ngs; nz++)
-    for (int i = 0; i < ncl; i++) dv[start[1] + ncl * nz + i] = gs[nz * ncl + i] * (1.0 + c->shear_m[nz % c->ns]);
+  for (int nz = 0, r = 0; nz < ngs; nz++) {
+    if (!oc_test_zoverlap(c, nz / c->ns, nz % c->ns)) continue;
+    for (int i = 0; i < ncl; i++) dv[start[1] + ncl * r + i] = gs[nz * ncl + i] * (1.0 + c->shear_m[nz % c->ns]);
+    r++;
+  }
   for (int nz = 0; nz < ngg; nz++)
     for (int i = 0; i < ncl; i++) dv[start[2] + ncl * nz + i] = gg[nz * ncl + i];
   if (mask)
@@ -2407,7 +2441,7 @@ static int kk_bandpower(oc_ctx* c, double* out) {
 }
 
 int oc_datavector_size_6x2pt(oc_ctx* c, int ncl) {
-  return c->ntheta * (2 * oc_npairs_ss(c) + oc_npairs_gs(c) + 2 * c->nl + c->ns) + (c->kk_nbp > 0 ? c->kk_nbp : ncl);
+  return c->ntheta * (2 * oc_npairs_ss(c) + oc_npairs_ggl(c) + 2 * c->nl + c->ns) + (c->kk_nbp > 0 ? c->kk_nbp : ncl);
 }
 
 /* compute_Mx2pt_N_masked<0,6> (generic_interface.hpp:1137-1159) with every
@@ -2417,7 +2451,7 @@ int oc_datavector_size_6x2pt(oc_ctx* c, int ncl) {
    kk at the ncl ells (like.ell) */
 int oc_datavector_6x2pt(oc_ctx* c, const int ord[6], const double* ell, int ncl, double* dv) {
   c->status = 0;
-  const int nt = c->ntheta, nss = oc_npairs_ss(c), ngs = oc_npairs_gs(c), nl = c->nl, ns = c->ns;
+  const int nt = c->ntheta, nss = oc_npairs_ss(c), ngs = oc_npairs_ggl(c), nl = c->nl, ns = c->ns;
   const int nkk = c->kk_nbp > 0 ? c->kk_nbp : ncl;
   const int sizes[6] = {2 * nt * nss, nt * ngs, nt * nl, nt * nl, nt * ns, nkk};
   for (int i = 0; i < 6; i++)

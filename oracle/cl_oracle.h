@@ -99,6 +99,9 @@ int oc_datavector_size(oc_ctx* c);
 
 int oc_npairs_ss(oc_ctx* c);
 int oc_npairs_gs(oc_ctx* c);
+int oc_test_zoverlap(oc_ctx* c, int zl, int zs);
+int oc_npairs_ggl(oc_ctx* c);
+int oc_set_ggl_exclude(oc_ctx* c, const int* pairs, int n);
 /* invalidates the cosmology-dependent caches (a new draw) */
 void oc_touch(oc_ctx* c);
 const char* oc_last_error(oc_ctx* c);

@@ -54,6 +54,12 @@ def _load():
         lib.oc_set_pointmass.argtypes = [vp, _D, ctypes.c_int]
         lib.oc_set_mask.restype = ctypes.c_int
         lib.oc_set_mask.argtypes = [vp, _I, ctypes.c_int]
+        lib.oc_set_ggl_exclude.restype = ctypes.c_int
+        lib.oc_set_ggl_exclude.argtypes = [vp, _I, ctypes.c_int]
+        lib.oc_test_zoverlap.restype = ctypes.c_int
+        lib.oc_test_zoverlap.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        lib.oc_npairs_ggl.restype = ctypes.c_int
+        lib.oc_npairs_ggl.argtypes = [vp]
         for name in ("oc_cl_tables_bb", "oc_cl_ss_bb_nointerp", "oc_set_ia_tatt", "oc_set_ia_ps"):
             getattr(lib, name).restype = ctypes.c_int
         lib.oc_set_ia_tatt.argtypes = [vp, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double, _D, ctypes.c_double]
@@ -220,6 +226,8 @@ class Oracle:
             self.set_ia_ps(prob.ia_ps, prob.ia_ps_kmin, prob.ia_ps_kmax)
         if prob.lens is not None and getattr(prob, "pm", None) is not None:
             self.set_pointmass(prob.pm)
+        if getattr(prob, "ggl_exclude", None) is not None:
+            self.set_ggl_exclude(prob.ggl_exclude)
         self.set_h0(getattr(prob, "h", 0.7))
         if getattr(prob, "lnPL", None) is not None:
             self.set_plin(prob.plin_log10k, prob.plin_z, prob.lnPL)
@@ -321,6 +329,17 @@ class Oracle:
     def set_pointmass(self, pm):
         pm = _arr(pm)
         self._chk(self.lib.oc_set_pointmass(self.c, _p(pm), len(pm)))
+
+    def set_ggl_exclude(self, pairs):
+        """Excluded (lens, source) pairs, flat [zl0, zs0, zl1, zs1, ...] (init_ggl_exclude)."""
+        a = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).ravel())
+        self._chk(self.lib.oc_set_ggl_exclude(self.c, a.ctypes.data_as(_I) if a.size else None, int(a.size)))
+
+    def test_zoverlap(self, zl, zs):
+        return self.lib.oc_test_zoverlap(self.c, int(zl), int(zs))
+
+    def n_ggl(self):
+        return self.lib.oc_npairs_ggl(self.c)
 
     def set_mask(self, mask):
         """Mask in the (0,1,2) block order (None clears)."""
@@ -460,7 +479,7 @@ class Oracle:
 
     def datavector_fourier(self, ells, lmax_shear, ord=(0, 1, 2), mask=None):
         ells = _arr(ells)
-        n = len(ells) * (self.n_ss + self.n_gs + self.nl)
+        n = len(ells) * (self.n_ss + self.n_ggl() + self.nl)
         out = np.empty(n)
         o = (ctypes.c_int * 3)(*ord)
         m = None if mask is None else np.ascontiguousarray(np.asarray(mask, dtype=np.int32))

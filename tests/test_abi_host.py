@@ -238,3 +238,47 @@ def test_layout_capture_is_a_snapshot(ctx):
     assert b.datavector_size() == 20 * (2 * 55 + 50 + 5)
     assert a.datavector_size() == 4290  # unchanged by later setters
     del a, b
+
+
+EXCL = [(0, 0), (0, 3), (2, 9), (4, 1), (4, 9)]
+
+
+def test_ggl_exclude_sizes_and_maps(ctx):
+    """init_ggl_exclude (generic_interface.cpp:874-896): the gamma_t block
+    loses the excluded pairs (tomo.ggl_Npowerspectra, :775-802); the pair maps
+    test_zoverlap / N_ggl / ZL / ZS (redshift_spline.c:115-223) follow."""
+    p = synth.full_3x2pt_problem()
+    ctx.load(p)
+    nt = p.ntheta
+    assert ctx.datavector_size() == nt * (2 * 55 + 50 + 5)
+    ctx.init_binning_fourier(4, 30, 3000, 1000)
+    ctx.init_ggl_exclude(EXCL)
+    assert ctx.n_ggl == 45
+    assert ctx.datavector_size() == nt * (2 * 55 + 45 + 5)
+    assert ctx.datavector_size_fourier() == 4 * (55 + 45 + 5)
+    for zl in range(5):
+        for zs in range(10):
+            assert ctx.test_zoverlap(zl, zs) == (0 if (zl, zs) in EXCL else 1)
+    with pytest.raises(CosmolikeError):
+        ctx.test_zoverlap(5, 0)
+    # a mask of the old size is dropped; the flat form is accepted too
+    ctx.init_ggl_exclude(np.array(EXCL[:2]).ravel())
+    assert ctx.datavector_size() == nt * (2 * 55 + 48 + 5)
+    ctx.init_ggl_exclude(None)
+    assert ctx.datavector_size() == nt * (2 * 55 + 50 + 5)
+    # the reference-named maps on the default context
+    d = Cosmolike.default()
+    d.load(p)
+    d.init_ggl_exclude(EXCL)
+    L = d.L
+    n = 0
+    for zl in range(5):
+        for zs in range(10):
+            inc = (zl, zs) not in EXCL
+            assert L.test_zoverlap(zl, zs) == int(inc)
+            assert L.N_ggl(zl, zs) == (n if inc else -1)
+            if inc:
+                assert (L.ZL(n), L.ZS(n)) == (zl, zs)
+                n += 1
+    assert n == 45 and L.ZL(45) == -1 and L.N_ggl(0, 10) == -1 and L.test_zoverlap(-1, 0) == -1
+    d.init_ggl_exclude(None)
