@@ -99,6 +99,7 @@ def lib():
             "clh_set_nuisance_pointmass": (i32, [VP, D, i32]),
             "clh_init_ggl_exclude": (i32, [VP, I, i32]),
             "clh_test_zoverlap": (i32, [VP, i32, i32]),
+            "clh_int_for_limber": (i32, [VP, i32, D, D, i32, D]),
             "clh_set_vjp_checkpoint": (i32, [VP, i32]),
             "clh_init_binning_fourier": (i32, [VP, i32, i32, i32, i32]),
             "clh_datavector_size_fourier": (i32, [VP]),
@@ -148,6 +149,20 @@ def lib():
             "N_ggl": (i32, [i32, i32]),
             "ZL": (i32, [i32]),
             "ZS": (i32, [i32]),
+            "w_gk_tomo": (f64, [i32, i32, i32]),
+            "w_ks_tomo": (f64, [i32, i32, i32]),
+            "C_gk_tomo_limber": (f64, [f64, i32]),
+            "C_ks_tomo_limber": (f64, [f64, i32]),
+            "C_kk_limber": (f64, [f64]),
+            "C_gk_tomo_limber_nointerp": (f64, [f64, i32, i32]),
+            "C_ks_tomo_limber_nointerp": (f64, [f64, i32, i32]),
+            "C_kk_limber_nointerp": (f64, [f64, i32]),
+            "int_for_C_ss_tomo_limber": (f64, [f64, D]),
+            "int_for_C_gs_tomo_limber": (f64, [f64, D]),
+            "int_for_C_gg_tomo_limber": (f64, [f64, D]),
+            "int_for_C_gk_tomo_limber": (f64, [f64, D]),
+            "int_for_C_ks_tomo_limber": (f64, [f64, D]),
+            "int_for_C_kk_limber": (f64, [f64, D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -9,7 +9,7 @@ CSRC = os.path.join(os.path.dirname(PKG), "csrc")
 LIB = os.path.join(PKG, "libcosmolike_hip.so")
 
 SOURCES = ["clh_kernels.hip", "clh_adjoint.hip", "clh_mixed.hip", "clh_like.hip", "clh_nonlimber.hip", "clh_cmb.hip",
-           "clh_noalloc.hip", "clh_host.cpp"]
+           "clh_noalloc.hip", "clh_integrand.hip", "clh_host.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
     # packed FP32 instructions stay off in every HIP build of this library

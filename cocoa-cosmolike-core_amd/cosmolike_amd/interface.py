@@ -344,6 +344,15 @@ class Cosmolike:
             self._chk(r)
         return r
 
+    def int_for(self, probe, params, a):
+        """Limber integrand of probe 0 ss .. 5 kk at the scale factors a, with the
+        reference's parameter array (clh_int_for_limber)."""
+        a = _arr(a)
+        ar = _arr(params)
+        out = np.empty_like(a)
+        self._chk(self.L.clh_int_for_limber(self.h, int(probe), _p(ar), _p(a), len(a), _p(out)))
+        return out
+
     @property
     def n_ggl(self):
         """Included gs pairs (tomo.ggl_Npowerspectra)."""
@@ -439,10 +448,12 @@ class Cosmolike:
         return bb
 
     def cl_nointerp(self, probe, ells):
-        """probe "ss" (E mode), "gs", "gg" or "ss_bb" (B mode)."""
+        """probe "ss" (E mode), "gs", "gg", "ss_bb" (B mode), and with the 6x2pt
+        probes "gk", "ks", "kk"."""
         ells = _arr(ells)
-        idx = {"ss": 0, "gs": 1, "gg": 2, "ss_bb": 3}[probe]
-        n = {"ss": self.n_ss, "gs": self.n_gs, "gg": self.nl, "ss_bb": self.n_ss}[probe]
+        idx = {"ss": 0, "gs": 1, "gg": 2, "ss_bb": 3, "gk": 4, "ks": 5, "kk": 6}[probe]
+        n = {"ss": self.n_ss, "gs": self.n_gs, "gg": self.nl, "ss_bb": self.n_ss, "gk": self.nl, "ks": self.ns,
+             "kk": 1}[probe]
         out = np.empty((n, len(ells)))
         self._chk(self.L.clh_cl_nointerp(self.h, idx, _p(ells), len(ells), _p(out)))
         return out

@@ -52,6 +52,8 @@ hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, 
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_lim_grad(const DrawDesc* D, const DrawDesc* P, const DrawDesc* M, const int2* work, int n,
                                hipStream_t s);
+hipError_t clh_launch_int_for(const DrawDesc* D, int probe, int i, int j, int f1, int f2, double l, const double* a,
+                              int n, double* out, hipStream_t s);
 hipError_t clh_launch_lnp_gather(const DrawDesc* D, const int2* work, int n, int ic, int nz, int nl,
                                  hipStream_t s);
 hipError_t clh_launch_legendre_mx(const DrawDesc* D, const int4* work, int n, int mode, hipStream_t s);
@@ -411,6 +413,8 @@ struct clh_ctx {
   dvec cache_raw;   // raw (uncalibrated, ord 0,1,2) data vector of `cur`
   dvec cache_cl;    // cl rows of `cur`
   dvec cache_bb;    // B-mode C_ss rows of `cur` (zero under NLA)
+  dvec cache_cl6;   // 6x2pt: C_gk [nl], C_ks [ns], C_kk columns of cache_nrow6 rows (empty otherwise)
+  int cache_nrow6 = 0;
   dvec cache_nl;    // non-Limber C_ell [nl][150] then LMAX [nl] of `cur` (empty under Limber w)
   // w(theta) of `cur` through the path adopt_limber_gg does not select
   // (w_gg_tomo with the other `limber`), on a copy of L
@@ -435,6 +439,7 @@ struct clh_ctx {
   hipEvent_t ev[11] = {nullptr};
   float last_ms[5] = {0, 0, 0, 0, 0};
   float last_vjp_ms[4] = {0, 0, 0, 0};
+  DBuf<double> d_intf;  // scale factors and values of clh_int_for_limber
   bool vjp_timed = false;
   // likelihood (IP): data and inverse covariance in the block order like_ord
   int like_n = 0, like_ord[3] = {0, 1, 2};
@@ -1231,7 +1236,7 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   for (int i = 0; i < bt.B; i++) {
     const DrawDesc& q = bt.desc[i];
     if (!q.cl6) continue;
-    wcmbp.push_back(i);
+    if (!bt.nrow_custom) wcmbp.push_back(i);  // custom ells: C_ell columns only, no placement
     for (int r0 = 0; r0 < q.nrow6; r0 += 32) {
       const int nr = std::min(32, q.nrow6 - r0);
       for (int b = 0; b < q.nl; b++) wcmb.push_back(make_int4(i, 0 + 4 * b, r0, nr));
@@ -1284,9 +1289,9 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
                   (nlw ? clh_nl_off(nl).total + 2 * (al(node_sz) + al((size_t)(L.n_ss() + L.n_gs() + nl) *
                                                                        (CLH_NL_L - 1)))
                        : 0) +
-                  (L.sixpt && !ells ? al((size_t)(nl + ns + 1) * (nrow + L.kk_direct_ells().size())) : 0);
+                  (L.sixpt ? al((size_t)(nl + ns + 1) * (nrow + (ells ? 0 : L.kk_direct_ells().size()))) : 0);
     toff[i + 1] = toff[i] + 2 * d.chi_z.size() + 2 * d.G_z.size() + d.lk.size() + d.pz.size() + d.lnP.size() +
-                  (tatt ? d.fpt.size() : 0) + (nlw ? d.lkL.size() + d.pzL.size() + d.lnPL.size() : 0) +
+                  (tatt ? d.fpt.size() : 0) + d.lkL.size() + d.pzL.size() + d.lnPL.size() +
                   (d.has_b2(nl) ? d.fptb.size() : 0) + 32;
     dvoff[i + 1] = dvoff[i] + L.dv_size();
     max_nrow = std::max(max_nrow, nrow);
@@ -1320,9 +1325,11 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.G_z = push(d.G_z); q.G_v = push(d.G);
     q.lk = push(d.lk); q.pz = push(d.pz); q.lnP = push(d.lnP);
     const bool nlw = L.nonlimber && nl > 0 && !ells;
-    if (nlw) {
+    if (!d.lkL.empty()) {  // linear P(k,z): the non-Limber w and int_for_C_gg_tomo_limber
       q.lkL = push(d.lkL); q.pzL = push(d.pzL); q.lnPL = push(d.lnPL);
       q.nkL = (int)d.lkL.size(); q.nzL = (int)d.pzL.size();
+    }
+    if (nlw) {
       q.h0 = d.h0;
       nl_sizes(q.nl_N, q.nl_N0);
     }
@@ -1405,16 +1412,18 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
       q.lim_ss[1] = q.lim_gx[1];
       q.lim_ss[2] = (q.lim_ss[1] - q.lim_ss[0]) / ((double)L.n_ell - 1.0);
     }
-    if (L.sixpt && !ells) {
-      q.n_f = (int)L.kk_direct_ells().size();
+    if (L.sixpt) {  // custom ells: the C_gk / C_ks / C_kk columns at those ells only
+      q.n_f = ells ? 0 : (int)L.kk_direct_ells().size();
       q.nrow6 = nrow + q.n_f;
-      q.kf6 = lay + DL.off_kf6;
-      q.ell_f = lay + DL.off_ellf;
-      q.kk_nbp = L.kk_nbp;
-      q.kk_lmin = L.kk_lmin;
-      q.kk_nL = L.kk_lmax - L.kk_lmin + 1;
-      q.kk_M = lay + DL.off_kkM;
-      q.kk_off = lay + DL.off_kkoff;
+      if (!ells) {
+        q.kf6 = lay + DL.off_kf6;
+        q.ell_f = lay + DL.off_ellf;
+        q.kk_nbp = L.kk_nbp;
+        q.kk_lmin = L.kk_lmin;
+        q.kk_nL = L.kk_lmax - L.kk_lmin + 1;
+        q.kk_M = lay + DL.off_kkM;
+        q.kk_off = lay + DL.off_kkoff;
+      }
       double cc, dd;
       host_chi_all(d, 1.0 / 1091.0, cc, dd);
       q.chi_cmb = cc;
@@ -1564,8 +1573,8 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   if (bt.n_wsx > bt.n_wsx_std)
     HIPCHK(c, clh_launch_limber_sx_tatt(bt.d_desc.p, bt.d_wsx.p + bt.n_wsx_std, bt.n_wsx - bt.n_wsx_std, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
-  if (legendre && bt.cmb_any)  // 6x2pt CMB lensing blocks: FP64 in every precision mode
-    HIPCHK(c, clh_launch_cmb(bt.d_desc.p, bt.d_wcmb.p, bt.n_wcmb, bt.d_wcmbp.p, bt.n_wcmbp, s));
+  if (bt.cmb_any)  // 6x2pt CMB lensing columns and blocks: FP64 in every precision mode
+    HIPCHK(c, clh_launch_cmb(bt.d_desc.p, bt.d_wcmb.p, bt.n_wcmb, bt.d_wcmbp.p, legendre ? bt.n_wcmbp : 0, s));
   if (legendre && bt.nl_any) {  // non-Limber w: FP64 in every precision mode
     HIPCHK(c, clh_launch_nodes(bt.d_desc_nl.p, bt.d_wn_nl.p, bt.n_wn_nl, s));
     HIPCHK(c, clh_launch_limber_gg(bt.d_desc_nl.p, bt.d_wgg_nl.p, bt.n_wgg_nl, 0, c->gg_lds[0] ? bt.gg_stage : 0, 0, s,
@@ -1611,6 +1620,14 @@ static int ensure_current(clh_ctx* c) {
     c->cache_nl.resize((size_t)c->L.lens.nbin * (CLH_NL_L + 1));
     HIPCHK(c, hipMemcpyAsync(c->cache_nl.data(), c->single.desc[0].nlw + off.used,
                              c->cache_nl.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  c->cache_cl6.clear();
+  if (c->single.desc[0].cl6) {
+    const DrawDesc& q = c->single.desc[0];
+    c->cache_cl6.resize((size_t)(q.nl + q.ns + 1) * q.nrow6);
+    c->cache_nrow6 = q.nrow6;
+    HIPCHK(c, hipMemcpyAsync(c->cache_cl6.data(), q.cl6, c->cache_cl6.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
   }
   c->cache_bb.assign((size_t)c->L.n_ss() * c->L.nrow(), 0.0);
   if (c->single.desc[0].cl_bb)
@@ -1660,6 +1677,7 @@ void clh_destroy(clh_ctx* c) {
   c->rl.clear();
   c->like_data.release(); c->like_icov.release(); c->like_mask.release();
   c->like_R.release(); c->like_Y.release();
+  c->d_intf.release();
   if (c->blas) (void)rocblas_destroy_handle(c->blas);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -2068,6 +2086,72 @@ int clh_test_zoverlap(clh_ctx* c, int zl, int zs) {
   return c->L.zoverlap(zl, zs) ? 1 : 0;
 }
 
+// The Limber integrands at n scale factors (int_for_C_*_limber, cosmo2D.c),
+// params as the reference's arrays; evaluated on the device for the current
+// draw (its tables and lens efficiencies from the cached forward)
+int clh_int_for_limber(clh_ctx* c, int probe, const double* ar, const double* a, int n, double* out) {
+  if (!c) return CLH_ERR_ARG;
+  if (!ar || !a || !out || n < 0 || probe < 0 || probe > 5) return fail(c, CLH_ERR_ARG, "int_for: bad arguments");
+  if (n == 0) return CLH_OK;
+  const Layout& L = c->L;
+  const int np[6] = {5, 4, 5, 3, 2, 1};
+  for (int k = 0; k < np[probe]; k++)
+    if (std::isnan(ar[k])) return fail(c, CLH_ERR_NAN, "int_for: NaN parameter %d", k);
+  for (int t = 0; t < n; t++)
+    if (!(a[t] > 0) || !(a[t] < 1)) return fail(c, CLH_ERR_ARG, "a>0 and a<1 not true");
+  const int ns = L.src.nbin, nl = L.lens.nbin;
+  int i = 0, j = 0, f1 = 0, f2 = 0;
+  double l = 0.0;
+  auto bad = [&](int v, int hi) { return v < 0 || v >= hi; };
+  switch (probe) {
+    case 0:
+      i = (int)ar[0]; j = (int)ar[1]; l = ar[2]; f1 = (int)ar[3]; f2 = (int)ar[4];
+      if (bad(i, ns) || bad(j, ns)) return fail(c, CLH_ERR_ARG, "error in selecting bin number (ni,nj) = [%d,%d]", i, j);
+      break;
+    case 1:
+      i = (int)ar[0]; j = (int)ar[1]; l = ar[2]; f1 = (int)ar[3];
+      if (bad(i, nl) || bad(j, ns)) return fail(c, CLH_ERR_ARG, "error in selecting bin number (nl, ns) = [%d,%d]", i, j);
+      break;
+    case 2:
+      i = (int)ar[0]; j = (int)ar[1]; l = ar[2]; f1 = (int)ar[3]; f2 = (int)ar[4];
+      if (bad(i, nl) || bad(j, nl)) return fail(c, CLH_ERR_ARG, "error in selecting bin number (ni,nj) = [%d,%d]", i, j);
+      if (i != j) return fail(c, CLH_ERR_ARG, "cross-tomography (ni,nj) = (%d,%d) bins not supported", i, j);
+      break;
+    case 3:
+      i = (int)ar[0]; l = ar[1]; f1 = (int)ar[2];
+      if (bad(i, nl)) return fail(c, CLH_ERR_ARG, "error in selecting bin number ni = %d", i);
+      break;
+    case 4:
+      i = (int)ar[0]; l = ar[1];
+      if (bad(i, ns)) return fail(c, CLH_ERR_ARG, "error in selecting bin number ni = %d", i);
+      break;
+    default:
+      l = ar[0];
+  }
+  if (probe >= 3 && !L.sixpt) return fail(c, CLH_ERR_STATE, "CMB lensing probes need init_probes(\"6x2pt\")");
+  int rc = ensure_current(c);
+  if (rc) return rc;
+  const DrawDesc& q = c->single.desc[0];
+  // the one-loop term: FAST-PT bias tables of the draw (set_bias_PS), or no
+  // bias terms for the bin, which make it zero
+  const bool ol = (probe == 1 || probe == 3) ? f1 == 1 : (probe == 2 && f2 == 1 && f1 == 0);
+  if (ol && !q.fptb) {
+    const Draw& d = c->cur;
+    if (d.b2[i] != 0 || d.bs2[i] != 0 || d.b3[i] != 0 || d.bk[i] != 0)
+      return fail(c, CLH_ERR_STATE, "one-loop bias needs the FAST-PT bias tables (set_bias_PS)");
+    if (probe == 2) f2 = 0; else f1 = 0;
+  }
+  if (probe == 2 && f1 == 1 && !q.lnPL) return fail(c, CLH_ERR_STATE, "use_linear_ps: linear P(k,z) not set");
+  if (probe == 0 && L.ia_model == CLH_IA_TATT && !q.fpt) return fail(c, CLH_ERR_STATE, "TATT: FAST-PT IA tables not set");
+  HIPCHK(c, c->d_intf.ensure(2 * (size_t)n));
+  double* da = c->d_intf.p;
+  HIPCHK(c, hipMemcpyAsync(da, a, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, clh_launch_int_for(c->single.d_desc.p, probe, i, j, f1, f2, l, da, n, da + n, c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, da + n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLH_OK;
+}
+
 // data-vector index in the block order `ord` of index k in the (0,1,2) order
 static int dv_index_from_canonical(const Layout& L, const int ord[3], int k) {
   const int nt = L.ntheta;
@@ -2435,8 +2519,10 @@ int clh_cl_tables_bb(clh_ctx* c, double* bb) {
 
 int clh_cl_nointerp(clh_ctx* c, int probe, const double* ell, int nl_, double* out) {
   if (!c || !ell || !out || nl_ <= 0) return CLH_ERR_ARG;
-  if (probe < 0 || probe > 3) return fail(c, CLH_ERR_ARG, "probe %d", probe);
-  if ((probe == 1 || probe == 2) && c->L.lens.nbin == 0) return fail(c, CLH_ERR_STATE, "lens sample not set");
+  if (probe < 0 || probe > 6) return fail(c, CLH_ERR_ARG, "probe %d", probe);
+  if ((probe == 1 || probe == 2 || probe == 4) && c->L.lens.nbin == 0)
+    return fail(c, CLH_ERR_STATE, "lens sample not set");
+  if (probe >= 4 && !c->L.sixpt) return fail(c, CLH_ERR_STATE, "CMB lensing probes need init_probes(\"6x2pt\")");
   if (has_nan(ell, nl_)) return fail(c, CLH_ERR_NAN, "ell: NaN");
   dvec ells(ell, ell + nl_);
   const int ord[3] = {0, 1, 2};
@@ -2445,6 +2531,16 @@ int clh_cl_nointerp(clh_ctx* c, int probe, const double* ell, int nl_, double* o
   if (rc) return rc;
   if ((rc = run_batch(c, c->scratch, false))) return rc;
   const int nss = c->L.n_ss(), ngs = c->L.n_gs(), nl = c->L.lens.nbin;
+  if (probe >= 4) {  // gk [nl], ks [ns], kk [1] columns of nrow6 = nl_ rows
+    const DrawDesc& q = c->scratch.desc[0];
+    const int col0 = probe == 4 ? 0 : probe == 5 ? nl : nl + c->L.src.nbin;
+    const int ncol = probe == 4 ? nl : probe == 5 ? c->L.src.nbin : 1;
+    if (!q.cl6 || q.nrow6 != nl_) return fail(c, CLH_ERR_STATE, "6x2pt columns missing");
+    HIPCHK(c, hipMemcpyAsync(out, q.cl6 + (size_t)col0 * nl_, (size_t)ncol * nl_ * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLH_OK;
+  }
   if (probe == 3) {  // B mode (EE = 0)
     const double* bb = c->scratch.desc[0].cl_bb;
     if (bb)
@@ -3167,12 +3263,13 @@ static int n_shear_idx(int ns, int ni, int nj) {  // N_shear: redshift_spline.c:
 }
 
 // interpol1d over one table column (basics.c:457-480)
-static double table_lookup(const clh_ctx* c, int col, bool ss, double l, bool bb = false) {
+static double table_lookup(const clh_ctx* c, int col, bool ss, double l, bool bb = false, bool cmb = false) {
   const Layout& L = c->L;
   const double lim0 = std::log(std::fmax(ss ? CLH_LMIN_TAB - 1. : (double)CLH_LMIN_TAB, 1.0));
   const double lim1 = std::log((double)(L.lmax + 1));
   const double dx = (lim1 - lim0) / ((double)L.n_ell - 1.);
-  const double* f = (bb ? c->cache_bb.data() : c->cache_cl.data()) + (size_t)col * L.nrow() + CLH_NLOW;
+  const double* f = cmb ? c->cache_cl6.data() + (size_t)col * c->cache_nrow6 + CLH_NLOW
+                        : (bb ? c->cache_bb.data() : c->cache_cl.data()) + (size_t)col * L.nrow() + CLH_NLOW;
   const double x = std::log(l);
   if (x < lim0) return f[0];
   const double r = (x - lim0) / dx;
@@ -3343,6 +3440,88 @@ double C_gg_tomo_limber_nointerp(const double l, const int ni, const int nj, con
   if (clh_cl_nointerp(c, 2, &l, 1, out.data())) return nanv();
   return out[ni];
 }
+
+// 6x2pt CMB lensing scalars on the default context: w_gk_tomo (cosmo2D.c:913),
+// w_ks_tomo (:1107) from the cached raw data vector (the gk and ks blocks
+// after the 3x2pt ones, without calibration), C_gk / C_ks / C_kk tables at the
+// gs grid of ln ell (:3020, 3270, 3484) and the direct integrals at any ell
+// (:2960, 3217, 3449; init = 1 returns 0, the reference's cache set-up call)
+static clh_ctx* sixpt_default() {
+  clh_ctx* c = clh_default();
+  if (!c) return nullptr;
+  if (!c->L.sixpt) {
+    fail(c, CLH_ERR_STATE, "CMB lensing probes need init_probes(\"6x2pt\")");
+    return nullptr;
+  }
+  return c;
+}
+static double cmb_w(int nt, int ni, int limber, bool ks) {
+  clh_ctx* c = sixpt_default();
+  if (!c) return nanv();
+  const Layout& L = c->L;
+  if (limber != 1) { fail(c, CLH_ERR_ARG, "NonLimber not implemented"); return nanv(); }
+  const int nb = ks ? L.src.nbin : L.lens.nbin;
+  if (nt < 0 || nt > L.ntheta - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number nt = %d", nt); return nanv(); }
+  if (ni < 0 || ni > nb - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number ni = %d", ni); return nanv(); }
+  if (ensure_current(c)) return nanv();
+  const size_t o = (size_t)L.dv_size3() + (size_t)(ks ? L.lens.nbin : 0) * L.ntheta + (size_t)ni * L.ntheta + nt;
+  return c->cache_raw[o];
+}
+double w_gk_tomo(const int nt, const int ni, const int limber) { return cmb_w(nt, ni, limber, false); }
+double w_ks_tomo(const int nt, const int ni, const int limber) { return cmb_w(nt, ni, limber, true); }
+
+static double cmb_table(int col, double l) {
+  clh_ctx* c = sixpt_default();
+  if (!c || ensure_current(c)) return nanv();
+  if (c->cache_cl6.empty()) { fail(c, CLH_ERR_STATE, "6x2pt columns missing"); return nanv(); }
+  return table_lookup(c, col, false, l, false, true);
+}
+double C_gk_tomo_limber(const double l, const int ni) {
+  clh_ctx* c = sixpt_default();
+  if (!c) return nanv();
+  if (ni < 0 || ni > c->L.lens.nbin - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number ni = %d", ni); return nanv(); }
+  return cmb_table(ni, l);
+}
+double C_ks_tomo_limber(const double l, const int ni) {
+  clh_ctx* c = sixpt_default();
+  if (!c) return nanv();
+  if (ni < 0 || ni > c->L.src.nbin - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number ni = %d", ni); return nanv(); }
+  return cmb_table(c->L.lens.nbin + ni, l);
+}
+double C_kk_limber(const double l) {
+  clh_ctx* c = sixpt_default();
+  if (!c) return nanv();
+  return cmb_table(c->L.lens.nbin + c->L.src.nbin, l);
+}
+static double cmb_direct(int probe, double l, int bin, int init) {
+  clh_ctx* c = sixpt_default();
+  if (!c) return nanv();
+  const int nb = probe == 4 ? c->L.lens.nbin : probe == 5 ? c->L.src.nbin : 1;
+  if (bin < 0 || bin > nb - 1) { fail(c, CLH_ERR_ARG, "error in selecting bin number ni = %d", bin); return nanv(); }
+  if (init == 1) return 0.0;
+  std::vector<double> out(nb);
+  if (clh_cl_nointerp(c, probe, &l, 1, out.data())) return nanv();
+  return out[bin];
+}
+double C_gk_tomo_limber_nointerp(const double l, const int nl, const int init) { return cmb_direct(4, l, nl, init); }
+double C_ks_tomo_limber_nointerp(const double l, const int ns, const int init) { return cmb_direct(5, l, ns, init); }
+double C_kk_limber_nointerp(const double l, const int init) { return cmb_direct(6, l, 0, init); }
+
+// int_for_C_*_limber (cosmo2D.h:116-130) at one scale factor on the default
+// context; NaN (message recorded) where the reference exits
+static double int_for_default(int probe, double a, void* params) {
+  clh_ctx* c = clh_default();
+  if (!c) return nanv();
+  double out = nanv();
+  if (clh_int_for_limber(c, probe, (const double*)params, &a, 1, &out)) return nanv();
+  return out;
+}
+double int_for_C_ss_tomo_limber(double a, void* params) { return int_for_default(0, a, params); }
+double int_for_C_gs_tomo_limber(double a, void* params) { return int_for_default(1, a, params); }
+double int_for_C_gg_tomo_limber(double a, void* params) { return int_for_default(2, a, params); }
+double int_for_C_gk_tomo_limber(double a, void* params) { return int_for_default(3, a, params); }
+double int_for_C_ks_tomo_limber(double a, void* params) { return int_for_default(4, a, params); }
+double int_for_C_kk_limber(double a, void* params) { return int_for_default(5, a, params); }
 
 // the gs pair maps of redshift_spline.c:115-223 on the default context
 int test_zoverlap(int ni, int nj) {

@@ -1,0 +1,153 @@
+// The Limber integrands as functions of the scale factor (int_for_C_*_limber,
+// cosmolike/cosmo2D.c): the same per-point physics the projection kernels
+// sum over their GL nodes, here at caller-given a for one (probe, bins, ell),
+// one thread per point.  Parameters follow the reference's arrays:
+//   probe 0 ss {n1, n2, l, EE, deriv}          cosmo2D.c:1521-1555
+//         1 gs {nl, ns, l, nonlinear_bias}     :2079-2119
+//         2 gg {ni, nj, l, use_linear_ps, nonlinear_bias}  :2489-2606
+//         3 gk {nl, l, nonlinear_bias}         :2857-2954
+//         4 ks {ns, l}                         :3178-3211
+//         5 kk {l}                             :3428-3446
+#include "clh_device.hpp"
+
+struct ClhIntArgs {
+  int probe, i, j, f1, f2;
+  double l;
+};
+
+// p_lin: cosmo3D.c:306-358 (bilinear in log10 k, z of ln P_lin)
+__device__ __forceinline__ double clh_p_lin(const DrawDesc& d, double k, double a) {
+  const double log10k = log10(k / CLH_COVERH0);
+  const double z = 1.0 / a - 1.0;
+  const int nz = d.nzL;
+  const int i = clh_bs(d.lkL, d.nkL, log10k);
+  const int j = clh_bs(d.pzL, nz, z);
+  const double dx = (log10k - d.lkL[i]) / (d.lkL[i + 1] - d.lkL[i]);
+  const double dy = (z - d.pzL[j]) / (d.pzL[j + 1] - d.pzL[j]);
+  const double* P = d.lnPL + (size_t)i * nz + j;
+  const double out = (1 - dx) * (1 - dy) * P[0] + (1 - dx) * dy * P[1] + dx * (1 - dy) * P[nz] + dx * dy * P[nz + 1];
+  return exp(out) / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
+}
+
+__device__ double clh_int_for(const DrawDesc& d, const ClhIntArgs& q, double a) {
+  double fK, dchida;
+  clh_chi_all(d, a, fK, dchida);
+  const double hh = clh_hoverh0v2(a, dchida);
+  const double l = q.l, ell = l + 0.5, k = ell / fK, z = 1.0 / a - 1.0;
+  const double wpre = 1.5 * d.omega_m * fK / a;  // W_kappa / g
+  const double fac = dchida / (fK * fK);
+  const double pref = l * (l + 1.) / (ell * ell);  // 1812.05995 eqs 74-79
+  const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+  const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+  const bool tatt = d.ia_model == CLH_IA_TATT;
+  switch (q.probe) {
+    case 0: {  // int_for_C_ss_tomo_limber_core (cosmo2D.c:1370-1515)
+      const int n1 = q.i, n2 = q.j, EE = q.f1;
+      const double g = clh_growfac(d, a);
+      const double PK = clh_p_nonlin(d, k, a);
+      const double ell4 = ell * ell * ell * ell;
+      const double p4 = l * (l - 1.) * (l + 1.) * (l + 2.) / ell4;
+      const double WK1 = wpre * clh_g_tomo(d, a, n1), WK2 = wpre * clh_g_tomo(d, a, n2);
+      const double WS1 = clh_nz_src(d, z, n1) * hh, WS2 = clh_nz_src(d, z, n2) * hh;
+      const double C11 = clh_ia_c1(d, a, g, n1), C12 = clh_ia_c1(d, a, g, n2);
+      double ans;
+      if (tatt) {
+        const double g4 = g * g * g * g, lnk = log(k);
+        const double C21 = clh_ia_c2(d, a, g, n1), C22 = clh_ia_c2(d, a, g, n2);
+        const double bta1 = clh_ia_bta(d, n1), bta2 = clh_ia_bta(d, n2);
+        if (EE == 1) {
+          const double tt = g4 * clh_fpt(d, 0, lnk), ta_dE1 = g4 * clh_fpt(d, 2, lnk),
+                       ta_dE2 = g4 * clh_fpt(d, 3, lnk), ta = g4 * clh_fpt(d, 4, lnk),
+                       mixA = g4 * clh_fpt(d, 6, lnk), mixB = g4 * clh_fpt(d, 7, lnk),
+                       mixEE = g4 * clh_fpt(d, 8, lnk);
+          ans = WK1 * WK2 * PK - WS1 * WK2 * (C11 * PK + C11 * bta1 * (ta_dE1 + ta_dE2) - 5 * C21 * (mixA + mixB)) -
+                WS2 * WK1 * (C12 * PK + C12 * bta2 * (ta_dE1 + ta_dE2) - 5 * C22 * (mixA + mixB)) +
+                WS1 * WS2 *
+                    (C11 * C12 * PK + C11 * C12 * (bta1 * bta2 * ta + (bta1 + bta2) * (ta_dE1 + ta_dE2)) -
+                     5. * (C11 * C22 + C12 * C21) * (mixA + mixB) - 5. * (C11 * bta1 * C22 + C12 * bta2 * C21) * mixEE +
+                     25. * C21 * C22 * tt);
+        } else {
+          const double tt = g4 * clh_fpt(d, 1, lnk), ta = g4 * clh_fpt(d, 5, lnk), mix = g4 * clh_fpt(d, 9, lnk);
+          ans = WS1 * WS2 *
+                (C11 * C12 * bta1 * bta2 * ta - 5. * (C11 * bta1 * C22 + C12 * bta2 * C21) * mix + 25. * C21 * C22 * tt);
+        }
+      } else {
+        ans = EE == 1 ? WK1 * WK2 * PK - WS1 * WK2 * C11 * PK - WS2 * WK1 * C12 * PK + WS1 * WS2 * C11 * C12 * PK : 0.0;
+      }
+      return q.f2 == 1 ? ans * (dchida / fK) * p4 : ans * fac * p4;
+    }
+    case 1: {  // int_for_C_gs_tomo_limber_core (cosmo2D.c:1890-2073), no RSD (include_RSD_GS = 0)
+      const int zl = q.i, zs = q.j;
+      const double g = clh_growfac(d, a);
+      const double PK = clh_p_nonlin(d, k, a);
+      const double WK = wpre * clh_g_tomo(d, a, zs), WS = clh_nz_src(d, z, zs) * hh;
+      const double WGAL = clh_nz_lens(d, z, zl) * hh, WMAG = wpre * clh_g_lens(d, a, zl);
+      const double gal = WGAL * d.par->b1[zl] + WMAG * pref * d.par->bmag[zl];
+      const double g4 = g * g * g * g;
+      const double ol = q.f1 == 1 ? clh_oneloop_gs(d, zl, log(k), k, g4, PK) : 0.0;
+      const double C1 = clh_ia_c1(d, a, g, zs);
+      double ans;
+      if (tatt) {
+        const double lnk = log(k);
+        const double mixA = g4 * clh_fpt(d, 6, lnk), mixB = g4 * clh_fpt(d, 7, lnk),
+                     ta_dE1 = g4 * clh_fpt(d, 2, lnk), ta_dE2 = g4 * clh_fpt(d, 3, lnk);
+        const double bta = clh_ia_bta(d, zs), C2 = clh_ia_c2(d, a, g, zs);
+        ans = WK * (gal * PK + WGAL * ol) - WS * gal * (C1 * PK + C1 * bta * (ta_dE1 + ta_dE2) - 5 * C2 * (mixA + mixB));
+      } else {
+        ans = (WK - WS * C1) * (gal * PK + WGAL * ol);
+      }
+      return ans * fac * pref2;
+    }
+    case 2: {  // int_for_C_gg_tomo_limber (cosmo2D.c:2489-2606), RSD on, no HOD
+      const int zl = q.i;
+      if ((ell + 1.0) / k > d.chi_amin) return 0.0;  // chi_1 beyond chi(a_min)
+      const double a0 = clh_a_chi(d, ell / k), a1 = clh_a_chi(d, (ell + 1.0) / k);
+      const double WGAL = clh_nz_lens(d, z, zl) * hh, WMAG = wpre * clh_g_lens(d, a, zl);
+      const double t = WGAL * d.par->b1[zl] + WMAG * pref * d.par->bmag[zl] + clh_w_rsd(d, ell, a0, a1, zl);
+      const double PK = q.f1 == 1 ? clh_p_lin(d, k, a) : clh_p_nonlin(d, k, a);
+      double ol = 0.0;
+      if (q.f2 == 1 && q.f1 == 0) {
+        const double g = clh_growfac(d, a);
+        ol = WGAL * WGAL * clh_oneloop_gg(d, zl, log(k), k, g * g * g * g, PK);
+      }
+      return (t * t * PK + ol) * fac;
+    }
+    case 3: {  // int_for_C_gk_tomo_limber (cosmo2D.c:2857-2954), RSD off
+      const int zl = q.i;
+      const double WK = wpre * ((d.chi_cmb - fK) / d.chi_cmb);
+      const double WGAL = clh_nz_lens(d, z, zl) * hh, WMAG = wpre * clh_g_lens(d, a, zl);
+      const double PK = clh_p_nonlin(d, k, a);
+      double r = WK * (WGAL * d.par->b1[zl] + WMAG * pref * d.par->bmag[zl]) * PK;
+      if (q.f1 == 1) {
+        const double g = clh_growfac(d, a);
+        r += WK * WGAL * clh_oneloop_gs(d, zl, log(k), k, g * g * g * g, PK);
+      }
+      return r * fac * pref;
+    }
+    case 4: {  // int_for_C_ks_tomo_limber (cosmo2D.c:3178-3211)
+      const int zs = q.i;
+      const double g = clh_growfac(d, a);
+      const double WK1 = wpre * clh_g_tomo(d, a, zs), WK2 = wpre * ((d.chi_cmb - fK) / d.chi_cmb);
+      const double WS1 = clh_nz_src(d, z, zs) * hh * clh_ia_c1(d, a, g, zs);
+      return ((WK1 - WS1) * WK2 * clh_p_nonlin(d, k, a) * fac) * pref * pref2;
+    }
+    default: {  // int_for_C_kk_limber (cosmo2D.c:3428-3446)
+      const double WK = wpre * ((d.chi_cmb - fK) / d.chi_cmb);
+      return WK * WK * clh_p_nonlin(d, k, a) * fac * pref * pref;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_int_for(const DrawDesc* __restrict__ D, ClhIntArgs q,
+                                                 const double* __restrict__ a, int n, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = clh_int_for(*D, q, a[t]);
+}
+
+hipError_t clh_launch_int_for(const DrawDesc* D, int probe, int i, int j, int f1, int f2, double l, const double* a,
+                              int n, double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  ClhIntArgs q{probe, i, j, f1, f2, l};
+  hipLaunchKernelGGL(k_int_for, dim3((n + 255) / 256), dim3(256), 0, s, D, q, a, n, out);
+  return hipGetLastError();
+}

@@ -155,6 +155,18 @@ int clh_init_ggl_exclude(clh_ctx* ctx, const int* pairs, int n);
 /* test_zoverlap (redshift_spline.c:115-145): 1 included, 0 excluded,
  * CLH_ERR_ARG for bins out of range */
 int clh_test_zoverlap(clh_ctx* ctx, int zl, int zs);
+/* The Limber integrands of cosmo2D.c as functions of the scale factor, at
+ * the n points a[] (host arrays, 0 < a < 1) for the current draw, evaluated
+ * on the device; ar is the reference's parameter array: probe 0 ss {n1, n2,
+ * l, EE, deriv} (int_for_C_ss_tomo_limber, cosmo2D.c:1521-1555), 1 gs {nl,
+ * ns, l, nonlinear_bias} (:2079-2119), 2 gg {ni, nj, l, use_linear_ps,
+ * nonlinear_bias} (:2489-2606), 3 gk {nl, l, nonlinear_bias} (:2857-2954),
+ * 4 ks {ns, l} (:3178-3211), 5 kk {l} (:3428-3446).  RSD enters gg only
+ * (include_RSD_GS / _GK = 0, as the data vectors use them); deriv = 1 gives
+ * the d C / d ln k integrand; nonlinear_bias = 1 needs the FAST-PT bias
+ * tables (set_bias_PS) unless the bin has no bias terms; gk, ks, kk need the
+ * 6x2pt probes. */
+int clh_int_for_limber(clh_ctx* ctx, int probe, const double* ar, const double* a, int n, double* out);
 
 /* ---------------------------------------------------- Fourier-space 3x2pt */
 /* init_binning_fourier (generic_interface.cpp:425-470): like.ell at the
@@ -262,7 +274,8 @@ int clh_cl_nonlimber(clh_ctx* ctx, double* cl, int* lmax);
  * bb[n_ss][N_ell]; zero under NLA */
 int clh_cl_tables_bb(clh_ctx* ctx, double* bb);
 /* Direct GL C_ell at arbitrary ell (C_*_tomo_limber_nointerp,
- * cosmo2D.c:1562, 2125, 2672): probe 0=ss(EE), 1=gs, 2=gg, 3=ss(BB);
+ * cosmo2D.c:1562, 2125, 2672, 2960, 3217, 3449): probe 0=ss(EE), 1=gs,
+ * 2=gg, 3=ss(BB), and with the 6x2pt probes 4=gk, 5=ks, 6=kk;
  * out[pair][nl] */
 int clh_cl_nointerp(clh_ctx* ctx, int probe, const double* ell, int nl, double* out);
 
@@ -396,6 +409,25 @@ int test_zoverlap(int ni, int nj);
 int N_ggl(int ni, int nj);
 int ZL(int ni);
 int ZS(int nj);
+/* 6x2pt CMB lensing (cosmo2D.h:43-45, 66-70, 98-102), after
+ * init_probes("6x2pt"): w_gk_tomo / w_ks_tomo without calibration (limber =
+ * 1 only), the C_gk / C_ks / C_kk tables and the direct integrals (init = 1
+ * returns 0) */
+double w_gk_tomo(const int nt, const int ni, const int limber);
+double w_ks_tomo(const int nt, const int ni, const int limber);
+double C_gk_tomo_limber(const double l, const int ni);
+double C_ks_tomo_limber(const double l, const int ni);
+double C_kk_limber(const double l);
+double C_gk_tomo_limber_nointerp(const double l, const int nl, const int init);
+double C_ks_tomo_limber_nointerp(const double l, const int ns, const int init);
+double C_kk_limber_nointerp(const double l, const int init);
+/* the integrands, cosmo2D.h:116-130 (clh_int_for_limber at one point) */
+double int_for_C_ss_tomo_limber(double a, void* params);
+double int_for_C_gs_tomo_limber(double a, void* params);
+double int_for_C_gg_tomo_limber(double a, void* params);
+double int_for_C_gk_tomo_limber(double a, void* params);
+double int_for_C_ks_tomo_limber(double a, void* params);
+double int_for_C_kk_limber(double a, void* params);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,9 @@ struct oc_ctx {
       bmag[OC_MAXB], shear_m[OC_MAXB];
   double pm[OC_MAXB]; /* point-mass amplitudes per lens bin (PointMass::pm_) */
   int ggl_ex[2 * OC_MAXB * OC_MAXB]; /* excluded (lens, source) pairs (tomo.ggl_exclude) */
+  int nlb_set;  /* 1: the one-loop bias term follows nlb_on (the integrands' nonlinear_bias
+                   argument), 0: has_b2_galaxies as the nointerp integrals pass it */
+  int nlb_on;
   int n_ggl_ex;
   int* mask;          /* likelihood mask in the (0,1,2) block order, or NULL */
   int mask_n;
@@ -820,11 +823,15 @@ static double int_ss(oc_ctx* c, double a, int n1, int n2, double l, int EE) {
 
 static int has_b2(oc_ctx* c);
 static double fptb_at(oc_ctx* c, int t, double lnk);
+/* the integrands' nonlinear_bias argument: has_b2_galaxies() in the C_ell
+   integrals (cosmo2D.c:2154, 2643, 2988), the caller's value in an integrand
+   evaluated on its own */
+static int nlb(oc_ctx* c) { return c->nlb_set ? c->nlb_on : has_b2(c); }
 
 /* one-loop bias term of the gs integrand (cosmo2D.c:1970-1992, 2027-2057):
    0.5 g4 (b2 d1d2 + bs2 d1s2 + b3 d1p3) + bK k^2 P when has_b2_galaxies */
 static double gs_oneloop(oc_ctx* c, int nl, double k, double PK, double growfac_a) {
-  if (!has_b2(c)) return 0.0;
+  if (!nlb(c)) return 0.0;
   if (!c->fptb) { oc_set_err(c, "one-loop bias needs the FAST-PT bias tables (set_bias_PS)"); return NAN; }
   const double lnk = log(k);
   const double g4 = growfac_a * growfac_a * growfac_a * growfac_a;
@@ -941,7 +948,7 @@ static double int_gg_ps(oc_ctx* c, double a, int ni, double l, int use_lin) {
   const double PK = use_lin ? p_lin(c, k, a) : p_nonlin(c, k, a);
   res *= PK;
   double oneloop = 0.0;
-  if (has_b2(c) && !use_lin) {
+  if (nlb(c) && !use_lin) {
     if (!c->fptb) { oc_set_err(c, "one-loop bias needs the FAST-PT bias tables (set_bias_PS)"); return NAN; }
     const double lnk = log(k);
     const double l0 = log(c->fptb_kmin), l1 = log(c->fptb_kmax);
@@ -2251,7 +2258,7 @@ static double int_gk(oc_ctx* c, double a, int nl, double l) {
   const double PK = p_nonlin(c, k, a);
   const double res = WK * (WGAL * c->b1[nl] + WMAG * ell_prefactor * c->bmag[nl]) * PK;
   double oneloop = 0.0;
-  if (has_b2(c)) {
+  if (nlb(c)) {
     if (!c->fptb) { oc_set_err(c, "one-loop bias needs the FAST-PT bias tables (set_bias_PS)"); return NAN; }
     const double g = growfac(c, a), g4 = g * g * g * g, lnk = log(k);
     oneloop = WK * WGAL *
@@ -2290,6 +2297,38 @@ static double int_kk(oc_ctx* c, double a, double l) {
   const double WK = W_k(c, a, fK);
   const double ell_prefactor = l * (l + 1.0) / (ell * ell);
   return WK * WK * p_nonlin(c, k, a) * (dchida / (fK * fK)) * ell_prefactor * ell_prefactor;
+}
+
+/* The Limber integrands at scale factors a[0..n) with the reference's
+   parameter arrays (int_for_C_*_tomo_limber, cosmo2D.c:1521-1555,
+   2079-2119, 2489-2606, 2857-2954, 3178-3211, 3428-3446): probe 0 ss
+   {n1, n2, l, EE, deriv}, 1 gs {nl, ns, l, nonlinear_bias}, 2 gg {ni, nj, l,
+   use_linear_ps, nonlinear_bias}, 3 gk {nl, l, nonlinear_bias}, 4 ks {ns, l},
+   5 kk {l}.  ss with deriv = 1 returns the d C / d ln k integrand (dchi/da /
+   f_K instead of / f_K^2). */
+int oc_int_for(oc_ctx* c, int probe, const double* ar, const double* a, int n, double* out) {
+  c->status = 0;
+  if (ensure_g(c)) return -1;
+  c->nlb_set = 1;
+  c->nlb_on = probe == 1 ? (int)ar[3] : probe == 2 ? (int)ar[4] : probe == 3 ? (int)ar[2] : 0;
+  for (int i = 0; i < n; i++) {
+    double v;
+    switch (probe) {
+      case 0: {
+        v = int_ss(c, a[i], (int)ar[0], (int)ar[1], ar[2], (int)ar[3]);
+        if ((int)ar[4] == 1) v *= chi_of_a(c, a[i]);
+        break;
+      }
+      case 1: v = int_gs(c, a[i], (int)ar[0], (int)ar[1], ar[2]); break;
+      case 2: v = int_gg_ps(c, a[i], (int)ar[0], ar[2], (int)ar[3]); break;
+      case 3: v = int_gk(c, a[i], (int)ar[0], ar[1]); break;
+      case 4: v = int_ks(c, a[i], (int)ar[0], ar[1]); break;
+      default: v = int_kk(c, a[i], ar[0]);
+    }
+    out[i] = v;
+  }
+  c->nlb_set = 0;
+  return c->status;
 }
 
 /* C_gk_tomo_limber_nointerp (:2960-3015), C_ks_tomo_limber_nointerp

@@ -102,6 +102,7 @@ int oc_npairs_gs(oc_ctx* c);
 int oc_test_zoverlap(oc_ctx* c, int zl, int zs);
 int oc_npairs_ggl(oc_ctx* c);
 int oc_set_ggl_exclude(oc_ctx* c, const int* pairs, int n);
+int oc_int_for(oc_ctx* c, int probe, const double* ar, const double* a, int n, double* out);
 /* invalidates the cosmology-dependent caches (a new draw) */
 void oc_touch(oc_ctx* c);
 const char* oc_last_error(oc_ctx* c);

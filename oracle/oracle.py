@@ -60,6 +60,8 @@ def _load():
         lib.oc_test_zoverlap.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.oc_npairs_ggl.restype = ctypes.c_int
         lib.oc_npairs_ggl.argtypes = [vp]
+        lib.oc_int_for.restype = ctypes.c_int
+        lib.oc_int_for.argtypes = [vp, ctypes.c_int, _D, _D, ctypes.c_int, _D]
         for name in ("oc_cl_tables_bb", "oc_cl_ss_bb_nointerp", "oc_set_ia_tatt", "oc_set_ia_ps"):
             getattr(lib, name).restype = ctypes.c_int
         lib.oc_set_ia_tatt.argtypes = [vp, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double, _D, ctypes.c_double]
@@ -334,6 +336,15 @@ class Oracle:
         """Excluded (lens, source) pairs, flat [zl0, zs0, zl1, zs1, ...] (init_ggl_exclude)."""
         a = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).ravel())
         self._chk(self.lib.oc_set_ggl_exclude(self.c, a.ctypes.data_as(_I) if a.size else None, int(a.size)))
+
+    def int_for(self, probe, params, a):
+        """Limber integrand of probe 0 ss .. 5 kk at the scale factors a, with the
+        reference's parameter array (int_for_C_*_tomo_limber)."""
+        a = _arr(a)
+        ar = _arr(params)
+        out = np.empty_like(a)
+        self._chk(self.lib.oc_int_for(self.c, int(probe), _p(ar), _p(a), len(a), _p(out)))
+        return out
 
     def test_zoverlap(self, zl, zs):
         return self.lib.oc_test_zoverlap(self.c, int(zl), int(zs))
