@@ -849,7 +849,7 @@ static double bf16_value(uint16_t h) {
 static long nl_fft_size(long n) {
   for (;; n++) {
     long m = n;
-    for (int p : {2, 3, 5, 7})
+    for (int p : {2, 3, 5})  // the radices of the device FFT (clh_fft.hpp)
       while (m % p == 0) m /= p;
     if (m == 1) return n;
   }
@@ -910,6 +910,8 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   // 2 pi t / N_j for the three transform lengths
   int nlN[3], nlN0[3];
   nl_sizes(nlN, nlN0);
+  if (nlN[1] != nlN[2] || nlN[0] + nlN[1] > 2400)  // the transforms' LDS layout (k_nl_fft)
+    return fail(c, CLH_ERR_STATE, "non-Limber transform lengths %d, %d, %d not supported", nlN[0], nlN[1], nlN[2]);
   dvec tw;
   for (int j = 0; j < 3; j++) {
     if (nlN[j] > 1500) return fail(c, CLH_ERR_STATE, "non-Limber transform length %d above 1500", nlN[j]);
@@ -1194,7 +1196,7 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   bt.adj_lds_gs = lds_gs;
   // non-Limber w: the views' gg nodes and rows (ell = 1..149), then the FFTLog
   // stages per draw (node blocks; bin x configuration x m blocks;
-  // configuration x m blocks; bin x 16-ell x 256-node tiles; bins)
+  // configuration x m blocks; the ten block-end ells; every ell; bins)
   std::vector<int2> wn_nl, wnl_prep, wnl_cl;
   std::vector<int4> wgg_nl, wnl_fwd, wnl_gl, wnl_chk, wnl_idft;
   int v = 0;
@@ -1211,15 +1213,11 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
     for (int b = 0; b < 2; b++) wnl_prep.push_back(make_int2(i, b));
     const int nmb = (CLH_NL_NH + 255) / 256;
     for (int zl = 0; zl < nl; zl++)
-      for (int j = 0; j < 3; j++)
-        for (int mb = 0; mb < nmb; mb++) wnl_fwd.push_back(make_int4(i, zl, j, mb));
+      for (int j = 0; j < 3; j++) wnl_fwd.push_back(make_int4(i, zl, j, 0));
     for (int j = 0; j < 3; j++)
       for (int mb = 0; mb < nmb; mb++) wnl_gl.push_back(make_int4(i, j, mb, 0));
-    for (int zl = 0; zl < nl; zl++)
-      for (int qt = 0; qt < CLH_NL_QT / 16; qt++) wnl_chk.push_back(make_int4(i, zl, 0, qt));
-    for (int zl = 0; zl < nl; zl++)
-      for (int lt = 0; lt < CLH_NL_LT / 16; lt++)
-        for (int qt = 0; qt < CLH_NL_QT / 16; qt++) wnl_idft.push_back(make_int4(i, zl, lt, qt));
+    for (int r = 0; r < 10; r++) wnl_chk.push_back(make_int4(i, r, 0, 0));  // the block-end ells
+    for (int l = 0; l < CLH_NL_L; l++) wnl_idft.push_back(make_int4(i, l, 0, 0));
     for (int zl = 0; zl < nl; zl++) wnl_cl.push_back(make_int2(i, zl));
   }
   HIPCHK(c, up2(bt.d_wn_nl, wn_nl, bt.n_wn_nl));

@@ -58,9 +58,8 @@
 #define CLH_NL_L 150
 #define CLH_NL_LT 160          // ell rows padded to 16
 #define CLH_NL_NH 752          // N/2 + 1 padded to even, N <= 1500
-#define CLH_NL_QT 32           // q tiles of 16 columns (512 columns)
 struct ClhNlOff {
-  size_t x, fb, T, G, part, partc, used, lmx, scal, total;
+  size_t x, xnu, fb, T, G, part, partc, used, lmx, scal, total;
 };
 #ifdef __HIPCC__
 __host__ __device__
@@ -69,11 +68,12 @@ inline ClhNlOff clh_nl_off(int nl) {
   ClhNlOff o;
   size_t p = 0;
   o.x = p; p += 512;
+  o.xnu = p; p += 512;  // x^1.01 (the RSD configuration's nu)
   o.fb = p; p += (size_t)nl * 3 * 512;
   o.T = p; p += (size_t)nl * 3 * CLH_NL_NH * 2;
   o.G = p; p += (size_t)3 * CLH_NL_LT * CLH_NL_NH * 2;
-  o.part = p; p += (size_t)nl * CLH_NL_LT * CLH_NL_QT;
-  o.partc = p; p += (size_t)nl * 16 * CLH_NL_QT;
+  o.part = p; p += (size_t)nl * CLH_NL_LT;   // sum_q F^2 k^3 P_lin per (bin, ell)
+  o.partc = p; p += (size_t)nl * 16;         // the same at the block-end ells
   o.used = p; p += (size_t)nl * CLH_NL_L;
   o.lmx = p; p += (size_t)nl;
   o.scal = p; p += 8;

@@ -9,14 +9,15 @@
 // g_l(nu + i eta_m) e^{-i eta_m ln base_l} and one inverse real FFT, of which
 // the 500 outputs at the original nodes are kept.  Here:
 //   k_nl_prep  FFTLog inputs f(x) / x^nu                     (thread per node)
-//   k_nl_fwd   the forward DFT T_ij[m], windowed             (thread per m)
+//   k_nl_fwd   the forward r2c FFT T_ij[m], windowed         (one per bin, j)
 //   k_nl_gl    G_j[l][m] = g_l e^{-i eta_m ln base_l}         (thread per m)
-//   k_nl_idft  the inverse DFT restricted to the kept outputs as an FP64 MFMA
-//              product: Out[l][q] = sum_m w_m Re(conj(T G) e^{2 pi i m n / N}),
-//              fused with the C_ell integrand F^2 k^3 P_lin(k) and its sum
-//              over a 16-node slice of q; first for the ten block-end ells
-//              (15, 31, ..., 143, 149) only, then for the 16-ell tiles below
-//              each bin's LMAX
+//   k_nl_fft   per (draw, ell) and lens bin: the inverse real transforms
+//              Out[n] = sum_m w_m Re(conj(T G) e^{2 pi i m n / N}) as one
+//              mixed-radix Stockham FFT of length N/2 in LDS each
+//              (csrc/clh_fft.hpp), fused with the C_ell integrand
+//              F^2 k^3 P_lin(k) at the 500 kept outputs and its sum; first
+//              for the ten block-end ells (15, 31, ..., 143, 149) only, then
+//              for every ell below each bin's LMAX
 //   k_nl_lmax  the reference's block-convergence switch to Limber: LMAX is the
 //              end of the first 16-ell block whose last C_ell is within 1% of
 //              the Limber C_ell (cosmo2D.c:4740-4750)
@@ -26,6 +27,17 @@
 //              dw = sum_l Pw[theta][l] (C_used - C_Limber)[l]
 // As in the reference, no ell at or above a bin's LMAX is transformed.
 #include "clh_device.hpp"
+#include "clh_fft.hpp"
+
+// e^{2 pi i t / M} as the product of two LDS tables: lo[t % S], hi[t / S]
+struct ClhTwid {
+  const clh_c2* lo;
+  const clh_c2* hi;
+  int S;
+  __device__ __forceinline__ clh_c2 operator()(int t) const { return clh_c2_mul(hi[t / S], lo[t % S]); }
+};
+#define CLH_NL_TWS 28  // S: S * S >= the largest M (750)
+
 
 typedef double clh_v4d __attribute__((ext_vector_type(4)));
 
@@ -146,6 +158,7 @@ __global__ __launch_bounds__(256) void k_nl_prep(const DrawDesc* __restrict__ D,
   double* fb = W + off.fb;
   if (q >= CLH_NL_NCHI) {
     W[off.x + q] = 0.0;
+    W[off.xnu + q] = 0.0;
     for (int i = 0; i < d.nl; i++)
       for (int j = 0; j < 3; j++) fb[(i * 3 + j) * 512 + q] = 0.0;
     return;
@@ -160,6 +173,7 @@ __global__ __launch_bounds__(256) void k_nl_prep(const DrawDesc* __restrict__ D,
   const double hh = 1.0 / ((a * a) * dc);
   const double Dg = clh_growfac(d, a), f = clh_f_growth(d, z);
   const double xnu1 = pow(x, 1.01);
+  W[off.xnu + q] = xnu1;
   for (int i = 0; i < d.nl; i++) {
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
     if (!(z < d.l_zmin[i] || z > d.l_zmax[i])) {
@@ -178,44 +192,64 @@ __global__ __launch_bounds__(256) void k_nl_prep(const DrawDesc* __restrict__ D,
 // ----------------------------------------------------------------------------
 // k_nl_fwd: T_ij[m] = sum_q fb[q] e^{-2 pi i m (N0 + q) / N} (the r2c FFT of
 // the zero-padded input, cosmo2D.c:4085-4098) and the c-window
-// (cosmo2D.c:4100-4116).  Work item (draw, bin, configuration, 256-m block).
+// (cosmo2D.c:4100-4116).  Work item (draw, bin, configuration): one r2c
+// transform of length N as a complex Stockham FFT of length N / 2 in LDS
+// (conj of the + sign transform of the real input).
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_nl_fwd(const DrawDesc* __restrict__ D, const int4* __restrict__ work) {
-  __shared__ double tw[2 * 1500];
+  __shared__ clh_c2 buf[2][CLH_NL_NH - 2];
+  __shared__ clh_c2 twl[CLH_NL_TWS], twh[CLH_NL_TWS];
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
-  const int i = w.y, j = w.z;
+  const int i = w.y, j = w.z, tid = threadIdx.x;
   const ClhNlOff off = clh_nl_off(d.nl);
-  const int N = d.nl_N[j], N0 = d.nl_N0[j], nh = N / 2 + 1;
-  const int tw0 = j == 0 ? 0 : (j == 1 ? 2 * d.nl_N[0] : 2 * (d.nl_N[0] + d.nl_N[1]));
-  for (int t = threadIdx.x; t < 2 * N; t += 256) tw[t] = d.nl_tw[tw0 + t];
-  __syncthreads();
-  const int m = w.w * 256 + threadIdx.x;
-  if (m >= CLH_NL_NH) return;
-  double2* T = reinterpret_cast<double2*>(d.nlw + off.T) + (size_t)(i * 3 + j) * CLH_NL_NH;
-  if (m >= nh || (j == 2 && !clh_nl_mag(d))) {
-    T[m] = make_double2(0.0, 0.0);
+  const int N = d.nl_N[j], N0 = d.nl_N0[j], M = N / 2, nh = N / 2 + 1;
+  const double* tw = d.nl_tw + (j == 0 ? 0 : (j == 1 ? 2 * d.nl_N[0] : 2 * (d.nl_N[0] + d.nl_N[1])));
+  clh_c2* T = reinterpret_cast<clh_c2*>(d.nlw + off.T) + (size_t)(i * 3 + j) * CLH_NL_NH;
+  if (j == 2 && !clh_nl_mag(d)) {
+    for (int m = tid; m < CLH_NL_NH; m += 256) T[m] = clh_c2{0.0, 0.0};
     return;
   }
+  for (int u = tid; u < 2 * CLH_NL_TWS; u += 256) {  // e^{2 pi i t / M} = entry 2 t of the N table
+    const int hi = u / CLH_NL_TWS, t = u % CLH_NL_TWS;
+    const int tt = (hi ? t * CLH_NL_TWS : t) % M;
+    (hi ? twh : twl)[t] = clh_c2{tw[4 * tt], tw[4 * tt + 1]};
+  }
   const double* fb = d.nlw + off.fb + (i * 3 + j) * 512;
-  int t = (m * N0) % N;
-  double re = 0.0, im = 0.0;
-  for (int q = 0; q < CLH_NL_NCHI; q++) {
-    const double v = fb[q];
-    re += v * tw[2 * t];
-    im -= v * tw[2 * t + 1];
-    t += m;
-    if (t >= N) t -= N;
+  for (int n = tid; n < M; n += 256) {
+    const int a = 2 * n - N0, b = 2 * n + 1 - N0;
+    buf[0][n] = clh_c2{(a >= 0 && a < CLH_NL_NCHI) ? fb[a] : 0.0, (b >= 0 && b < CLH_NL_NCHI) ? fb[b] : 0.0};
+  }
+  __syncthreads();
+  const ClhTwid twM{twl, twh, CLH_NL_TWS};
+  int src = 0, Ns = 1;
+  for (int m = M; m > 1;) {  // N / 2 factors into 2, 3, 5 (checked on the host)
+    const int R = clh_fft_radix(m);
+    const ClhFftStage st = clh_fft_stage(M, R, Ns);
+    for (int jb = tid; jb < st.stride; jb += 256) clh_fft_bfly(buf[src], buf[src ^ 1], st, jb, twM);
+    __syncthreads();
+    src ^= 1;
+    Ns *= R;
+    m /= R;
   }
   const int halfN = N / 2;
   const int kmax = (int)(halfN * 0.25);
-  if (m >= halfN - kmax) {
-    const int k = halfN - m;
-    const double Wc = (double)k / kmax - sin(2. * M_PI * k / kmax) / (2. * M_PI);
-    re *= Wc;
-    im *= Wc;
+  for (int m = tid; m < CLH_NL_NH; m += 256) {
+    if (m >= nh) {
+      T[m] = clh_c2{0.0, 0.0};
+      continue;
+    }
+    const clh_c2 e = m == M ? clh_c2{-1.0, 0.0} : clh_c2{tw[2 * m], tw[2 * m + 1]};  // e^{2 pi i m / N}
+    const clh_c2 y = clh_r2c_unpack(buf[src][m % M], buf[src][(M - m) % M], e);
+    double re = y.re, im = -y.im;  // e^{-2 pi i ...}: the conjugate for real input
+    if (m >= halfN - kmax) {
+      const int k = halfN - m;
+      const double Wc = (double)k / kmax - sin(2. * M_PI * k / kmax) / (2. * M_PI);
+      re *= Wc;
+      im *= Wc;
+    }
+    T[m] = clh_c2{re, im};
   }
-  T[m] = make_double2(re, im);
 }
 
 // ----------------------------------------------------------------------------
@@ -233,9 +267,10 @@ __global__ __launch_bounds__(256) void k_nl_gl(const DrawDesc* __restrict__ D, c
   const int N = d.nl_N[j], N0 = d.nl_N0[j], nh = N / 2 + 1;
   const int m = w.z * 256 + threadIdx.x;
   if (m >= CLH_NL_NH) return;
-  double2* G = reinterpret_cast<double2*>(d.nlw + off.G) + ((size_t)j * CLH_NL_NH + m) * CLH_NL_LT;
+  // G_j stored [l][m] (m fastest): these stores and the transforms' reads coalesce
+  double2* G = reinterpret_cast<double2*>(d.nlw + off.G) + (size_t)j * CLH_NL_LT * CLH_NL_NH + m;
   if (m >= nh || (j == 2 && !clh_nl_mag(d))) {
-    for (int l = 0; l < CLH_NL_LT; l++) G[l] = make_double2(0.0, 0.0);
+    for (int l = 0; l < CLH_NL_LT; l++) G[(size_t)l * CLH_NL_NH] = make_double2(0.0, 0.0);
     return;
   }
   const double* x = d.nlw + off.x;
@@ -252,126 +287,173 @@ __global__ __launch_bounds__(256) void k_nl_gl(const DrawDesc* __restrict__ D, c
     g1 = g;
     double s, c;
     sincos(-eta * log(base * (l + 1.) / x[CLH_NL_NCHI - 1]), &s, &c);
-    G[l] = cmul(g, make_double2(c, s));
+    G[(size_t)l * CLH_NL_NH] = cmul(g, make_double2(c, s));
   }
-  for (int l = CLH_NL_L; l < CLH_NL_LT; l++) G[l] = make_double2(0.0, 0.0);
+  for (int l = CLH_NL_L; l < CLH_NL_LT; l++) G[(size_t)l * CLH_NL_NH] = make_double2(0.0, 0.0);
 }
 
 // ----------------------------------------------------------------------------
-// k_nl_idft: the inverse real transforms of every (bin, ell) kept at the 500
-// original nodes, as an FP64 MFMA product over m, fused with the C_ell
-// integrand (cosmo2D.c:4492-4524 and 4702-4734).  Work item (draw, bin,
-// 16-ell tile, 256-node tile).
-// Lane layout of v_mfma_f64_16x16x4: A[r][k] on lane r + 16 k, B[k][n] on
-// lane n + 16 k, D[4 v + l / 16][l % 16] in value v of lane l.  A block of
-// four waves covers 256 columns, each wave four 16-column tiles that share
-// its A operand (four independent accumulation chains).  K index
-// 4 s + k -> m = 2 s + k / 2, real (k even) or imaginary (k odd) part:
-//   A[l][.] = (Re Z, Im Z) with Z = T[m] G[l][m],
-//   B[.][q] = w_m (cos, sin)(2 pi m n / N), n = N0 + q, w_m = 1 at m = 0 and
-// N/2, 2 elsewhere (the c2r sum over the Hermitian half), so that
-// sum A B = sum_m w_m Re(conj(Z) e^{2 pi i m n / N}).
-// CHECK: the tile's rows are the ten block-end ells (row r -> min(16 r + 15,
-// 149), rows 10..15 empty) and the sums go to partc; otherwise the tile is
-// ells 16 w.z .. 16 w.z + 15 and is skipped at or above the bin's LMAX.
+// k_nl_fft: the inverse real transforms of one ell for every lens bin below
+// its LMAX, at the 500 original nodes, fused with the C_ell integrand
+// (cosmo2D.c:4492-4524 and 4702-4734).  Work item (draw, ell) or, with
+// CHECK, (draw, r) for the block-end ell of row r (ten rows).  Per bin and
+// configuration j: X_m = conj(T_j[m] G_j[l][m]) for m = 0 .. N/2 (the
+// imaginary parts at m = 0 and N/2 dropped, as the c2r transform does),
+// packed into one complex transform of length M = N/2 whose outputs z_n
+// hold Out[2n] + i Out[2n+1] (Stockham stages ping-ponging between two LDS
+// buffers, twiddles from the layout's cos / sin table of 2 pi t / N);
+// Out[N0 + q] then enters F(q) = sum_j Fy_j (magnification with
+// b_mag l(l+1) / y^2), and the bin's sum_q F^2 k^3 P_lin(k) at k = y c/H0,
+// y = (l + 1) / x[499 - q], goes to part (partc with CHECK).
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ int clh_nl_row(bool check, int l0, int r) {
   if (!check) return l0 + r;
   return r < 10 ? (16 * r + 15 < CLH_NL_L ? 16 * r + 15 : CLH_NL_L - 1) : CLH_NL_LT - 1;
 }
 
+// number of Stockham stages of a length-M transform
+__device__ __forceinline__ int clh_fft_nst(int M) {
+  int n = 0;
+  for (int m = M; m > 1; m /= clh_fft_radix(m)) n++;
+  return n;
+}
+
+// Two halves of 128 threads run the density (j = 0) and RSD (j = 1)
+// transforms of a bin side by side, with barriers shared; the magnification
+// transform (j = 2) follows on the second half.
 template <bool CHECK>
-__global__ __launch_bounds__(256) void k_nl_idft(const DrawDesc* __restrict__ D, const int4* __restrict__ work) {
-  constexpr int SUB = 4;  // 16-column output tiles per wave, sharing the A operand
-  __shared__ double tw[2 * 1500];
-  const int4 w = work[blockIdx.x];
+__global__ __launch_bounds__(256) void k_nl_fft(const DrawDesc* __restrict__ D, const int4* __restrict__ work) {
+  __shared__ clh_c2 fbuf[2 * (CLH_NL_NH - 2) + 2 * 450];  // per half two buffers of its M (M0 + M1 <= 1200)
+  __shared__ clh_c2 twl[2][CLH_NL_TWS], twh[2][CLH_NL_TWS];  // per M (configuration 0; 1 and 2)
+  __shared__ double kP[CLH_NL_NCHI];
+  __shared__ double Fh[2][512];  // the linear P column at z = 0 first, then F(q) per half
+  __shared__ double red[4];
+  const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
-  const int i = w.y, l0 = 16 * w.z;
+  const int r = w.y, l = CHECK ? clh_nl_row(true, 0, r) : r;
   const ClhNlOff off = clh_nl_off(d.nl);
-  if (!CHECK && l0 >= (int)d.nlw[off.lmx + i]) return;  // whole block: ells the reference never transforms
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int r = lane & 15, kq = lane >> 4, mo = kq >> 1, cpart = kq & 1;
-  const int lrow = clh_nl_row(CHECK, l0, r);
-  const int q0 = 256 * w.w + 64 * wv;  // the wave's columns q0 .. q0 + 63
-  const bool mag = clh_nl_mag(d);
-  const double2* Tall = reinterpret_cast<const double2*>(d.nlw + off.T);
-  const double2* Gall = reinterpret_cast<const double2*>(d.nlw + off.G);
-  clh_v4d acc[3][SUB];
-  for (int j = 0; j < 3; j++)
-    for (int u = 0; u < SUB; u++) acc[j][u] = (clh_v4d){0.0, 0.0, 0.0, 0.0};
-  for (int j = 0; j < 3; j++) {
-    if (j == 2 && !mag) break;
-    const int N = d.nl_N[j], N0 = d.nl_N0[j], nh = N / 2 + 1;
-    const int tw0 = j == 0 ? 0 : (j == 1 ? 2 * d.nl_N[0] : 2 * (d.nl_N[0] + d.nl_N[1]));
-    __syncthreads();
-    for (int t = threadIdx.x; t < 2 * N; t += 256) tw[t] = d.nl_tw[tw0 + t];
-    __syncthreads();
-    const double2* T = Tall + (size_t)(i * 3 + j) * CLH_NL_NH + mo;
-    const double2* G = Gall + ((size_t)j * CLH_NL_NH + mo) * CLH_NL_LT + lrow;
-    int t[SUB], dt[SUB];
-    double bm[SUB];
-    for (int u = 0; u < SUB; u++) {
-      const int q = q0 + 16 * u + r;
-      const int n = N0 + (q < CLH_NL_NCHI ? q : 0);
-      bm[u] = q < CLH_NL_NCHI ? 2.0 : 0.0;  // w_m = 2 except at m = 0 and N/2
-      t[u] = (mo * n) % N;
-      dt[u] = (2 * n) % N;
-    }
-    const int nsteps = (nh + 1) / 2;
-    clh_v4d a4[SUB];
-    for (int u = 0; u < SUB; u++) a4[u] = acc[j][u];
-    double2 Tn = T[0], Gn = G[0];
-    for (int s = 0; s < nsteps; s++) {
-      const double2 Z = cmul(Tn, Gn);
-      if (s + 1 < nsteps) {  // next step's operands, loaded ahead of this step's MFMAs
-        Tn = T[2 * (s + 1)];
-        Gn = G[(size_t)2 * (s + 1) * CLH_NL_LT];
-      }
-      const int m = 2 * s + mo;
-      const double a = cpart ? Z.y : Z.x;
-      const double half = (m == 0 || 2 * m == N) ? 0.5 : 1.0;
-#pragma unroll
-      for (int u = 0; u < SUB; u++) {
-        const double b = bm[u] * half * tw[2 * t[u] + cpart];
-        a4[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, a4[u], 0, 0, 0);
-        t[u] += dt[u];
-        if (t[u] >= N) t[u] -= N;
-      }
-    }
-    for (int u = 0; u < SUB; u++) acc[j][u] = a4[u];
+  const int tid = threadIdx.x, h = tid >> 7, ht = tid & 127;
+  if (!CHECK) {  // nothing to do when every bin switched to Limber at or below l
+    bool any = false;
+    for (int i = 0; i < d.nl; i++) any = any || l < (int)d.nlw[off.lmx + i];
+    if (!any) return;
   }
-  // epilogue: F = sum_j Fy_j (magnification with b_mag l(l+1) / y^2),
-  // integrand F^2 k^3 P_lin(k, a = 1) at k = y c/H0, y = (l + 1) / x[499 - q]
+  // twiddle tables of M = N_j / 2 (N_1 = N_2) from the layout's cos / sin of 2 pi t / N_j
+  for (int u = tid; u < 4 * CLH_NL_TWS; u += 256) {
+    const int c = u / (2 * CLH_NL_TWS), hi = (u / CLH_NL_TWS) & 1, t = u % CLH_NL_TWS;
+    const int N = d.nl_N[c], M = N / 2;
+    const double* tw = d.nl_tw + (c == 0 ? 0 : 2 * d.nl_N[0]);
+    const int tt = (hi ? t * CLH_NL_TWS : t) % M;  // e^{2 pi i tt / M} = entry 2 tt of the N table
+    (hi ? twh : twl)[c][t] = clh_c2{tw[4 * tt], tw[4 * tt + 1]};
+  }
+  // k^3 P_lin(k, z = 0) at k = (l + 1) / x[499 - q] c/H0: the z = 0 column
+  // of ln P_lin staged in LDS (interpolation in k of the z-interpolated
+  // column is the bilinear value of p_lin, cosmo3D.c:306-358)
   const double* x = d.nlw + off.x;
+  const double* xnu = d.nlw + off.xnu;
   const double rc = d.nlw[off.scal + 2];
-  const double bmag = d.par->bmag[i];
-  double* part = d.nlw + (CHECK ? off.partc : off.part);
-  for (int v = 0; v < 4; v++) {
-    const int row = 4 * v + kq;
-    const int l = clh_nl_row(CHECK, l0, row);
-    for (int u = 0; u < SUB; u++) {
-      const int q = q0 + 16 * u + r;
-      double val = 0.0;
-      if (l < CLH_NL_L && q < CLH_NL_NCHI) {
-        const double xq = x[CLH_NL_NCHI - 1 - q];
-        const double ty = (l + 1.) / xq;
-        double F = 0.0;
-        for (int j = 0; j < 3; j++) {
-          if (j == 2 && !mag) break;
-          const double nu = c_nl_nu[j];
-          const double pre = sqrt(M_PI) / (4.0 * d.nl_N[j] * pow(l + 1., nu)) * pow(xq, nu);
-          const double fy = acc[j][u][v] * pre;
-          F += j == 2 ? bmag * (l * (l + 1.)) * fy / (ty * ty) : fy;
+  const bool stage = d.nkL <= 512;
+  double* s_lk = Fh[0];
+  double* s_col = Fh[1];
+  if (stage) {
+    const int nz = d.nzL, j0 = clh_bs(d.pzL, nz, 0.0);
+    const double dy = (0.0 - d.pzL[j0]) / (d.pzL[j0 + 1] - d.pzL[j0]);
+    for (int k = tid; k < d.nkL; k += 256) {
+      s_lk[k] = d.lkL[k];
+      const double* P = d.lnPL + (size_t)k * nz + j0;
+      s_col[k] = (1 - dy) * P[0] + dy * P[1];
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < CLH_NL_NCHI; q += 256) {
+    const double xq = x[CLH_NL_NCHI - 1 - q];
+    const double k1 = (l + 1.) / xq * rc;
+    double pl;
+    if (stage) {
+      const double log10k = log10(k1 / CLH_COVERH0);
+      const int i = clh_bs(s_lk, d.nkL, log10k);
+      const double dx = (log10k - s_lk[i]) / (s_lk[i + 1] - s_lk[i]);
+      pl = exp((1 - dx) * s_col[i] + dx * s_col[i + 1]) / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
+    } else {
+      pl = clh_plin0(d, k1);
+    }
+    kP[q] = (k1 * k1 * k1) * pl;
+  }
+  const bool mag = clh_nl_mag(d);
+  const clh_c2* Tall = reinterpret_cast<const clh_c2*>(d.nlw + off.T);
+  const clh_c2* Gall = reinterpret_cast<const clh_c2*>(d.nlw + off.G);
+  const int M0 = d.nl_N[0] / 2;
+  clh_c2* mybuf = h == 0 ? fbuf : fbuf + 2 * M0;  // this half's two buffers
+  for (int i = 0; i < d.nl; i++) {
+    if (!CHECK && l >= (int)d.nlw[off.lmx + i]) continue;  // uniform over the block
+    __syncthreads();  // kP / the previous bin's sum done with Fh
+    for (int q = ht; q < 512; q += 128) Fh[h][q] = 0.0;
+    const double bmag = d.par->bmag[i];
+    for (int round = 0; round < (mag ? 2 : 1); round++) {
+      const int j = round == 0 ? h : 2;
+      const bool act = round == 0 || h == 1;
+      const int N = d.nl_N[j], N0 = d.nl_N0[j], M = N / 2, c = j == 0 ? 0 : 1;
+      const double* tw = d.nl_tw + (j == 0 ? 0 : (j == 1 ? 2 * d.nl_N[0] : 2 * (d.nl_N[0] + d.nl_N[1])));
+      clh_c2* b0 = mybuf;
+      clh_c2* b1 = mybuf + M;
+      if (act) {
+        const clh_c2* T = Tall + (size_t)(i * 3 + j) * CLH_NL_NH;
+        const clh_c2* G = Gall + ((size_t)j * CLH_NL_LT + l) * CLH_NL_NH;
+        for (int k = ht; k < M; k += 128) {
+          const clh_c2 a = clh_c2_mul(T[k], G[k]), b = clh_c2_mul(T[M - k], G[M - k]);
+          clh_c2 Xk{a.re, -a.im}, XMk{b.re, -b.im};
+          if (k == 0) Xk.im = XMk.im = 0.0;
+          b0[k] = clh_c2r_pack(Xk, XMk, clh_c2{tw[2 * k], tw[2 * k + 1]});
         }
-        const double k1 = ty * rc;
-        val = F * F * (k1 * k1 * k1) * clh_plin0(d, k1);
       }
-      val += __shfl_xor(val, 1);
-      val += __shfl_xor(val, 2);
-      val += __shfl_xor(val, 4);
-      val += __shfl_xor(val, 8);
-      if (r == 0)
-        part[((size_t)i * (CHECK ? 16 : CLH_NL_LT) + (CHECK ? row : l)) * CLH_NL_QT + 16 * w.w + 4 * wv + u] = val;
+      __syncthreads();
+      // the stage loop runs the larger stage count of the active halves;
+      // N / 2 factors into 2, 3, 5 (nl_fft_size, checked on the host)
+      const int nst = round == 0 ? max(clh_fft_nst(M0), clh_fft_nst(d.nl_N[1] / 2)) : clh_fft_nst(M);
+      const ClhTwid twM{twl[c], twh[c], CLH_NL_TWS};
+      clh_c2* src = b0;
+      clh_c2* dst = b1;
+      int Ns = 1, m = M;
+      for (int s = 0; s < nst; s++) {
+        if (act && m > 1) {
+          const int R = clh_fft_radix(m);
+          const ClhFftStage st = clh_fft_stage(M, R, Ns);
+          for (int jb = ht; jb < st.stride; jb += 128) clh_fft_bfly(src, dst, st, jb, twM);
+          Ns *= R;
+          m /= R;
+          clh_c2* t = src;
+          src = dst;
+          dst = t;
+        }
+        __syncthreads();
+      }
+      if (act) {
+        const double nu = c_nl_nu[j];
+        const double pre = sqrt(M_PI) / (4.0 * N * (nu == 1.0 ? l + 1. : pow(l + 1., nu)));
+        for (int q = ht; q < CLH_NL_NCHI; q += 128) {
+          const int n = N0 + q;
+          const double o = (n & 1) ? src[n >> 1].im : src[n >> 1].re;
+          const double xq = x[CLH_NL_NCHI - 1 - q];
+          const double fy = o * pre * (nu == 1.0 ? xq : xnu[CLH_NL_NCHI - 1 - q]);
+          if (j == 2) {
+            const double ty = (l + 1.) / xq;
+            Fh[h][q] += bmag * (l * (l + 1.)) * fy / (ty * ty);
+          } else {
+            Fh[h][q] += fy;
+          }
+        }
+      }
+      __syncthreads();  // this round's readers done with the buffers, F complete
+    }
+    double s = 0.0;
+    for (int q = tid; q < CLH_NL_NCHI; q += 256) {
+      const double F = Fh[0][q] + Fh[1][q];
+      s += F * F * kP[q];
+    }
+    s = clh_block_sum(s, red);
+    if (tid == 0) {
+      double* part = d.nlw + (CHECK ? off.partc : off.part);
+      part[(size_t)i * (CHECK ? 16 : CLH_NL_LT) + (CHECK ? r : l)] = s;
     }
   }
 }
@@ -389,9 +471,7 @@ __global__ __launch_bounds__(64) void k_nl_lmax(const DrawDesc* __restrict__ D, 
   const ClhNlOff off = clh_nl_off(d.nl);
   if (r < 10) {
     const int L = clh_nl_row(true, 0, r);
-    const double* part = d.nlw + off.partc + ((size_t)i * 16 + r) * CLH_NL_QT;
-    double s = 0.0;
-    for (int t = 0; t < CLH_NL_QT; t++) s += part[t];
+    const double s = d.nlw[off.partc + (size_t)i * 16 + r];
     const double lnl = d.nl_lnl[(size_t)i * (CLH_NL_L - 1) + L - 1];
     const double llin = d.nl_llin[(size_t)i * (CLH_NL_L - 1) + L - 1];
     const double cl = s * d.nlw[off.scal + 1] * 2. / M_PI + lnl - llin;
@@ -431,9 +511,7 @@ __global__ __launch_bounds__(256) void k_nl_cl(const DrawDesc* __restrict__ D, c
   const double* lnl = d.nl_lnl + (size_t)i * (CLH_NL_L - 1);
   const double* llin = d.nl_llin + (size_t)i * (CLH_NL_L - 1);
   if (l < CLH_NL_L && l < (int)d.nlw[off.lmx + i]) {  // the transformed ells
-    const double* part = d.nlw + off.part + ((size_t)i * CLH_NL_LT + l) * CLH_NL_QT;
-    double s = 0.0;
-    for (int t = 0; t < CLH_NL_QT; t++) s += part[t];
+    const double s = d.nlw[off.part + (size_t)i * CLH_NL_LT + l];
     NL[l] = s * dlnk * 2. / M_PI + (l >= 1 ? lnl[l - 1] - llin[l - 1] : 0.0);
   }
   if (l == 0) s_lmax = (int)d.nlw[off.lmx + i];
@@ -472,9 +550,9 @@ hipError_t clh_launch_nl_forward(const DrawDesc* D, const int2* wprep, int nprep
   hipLaunchKernelGGL(k_nl_prep, dim3(nprep), dim3(256), 0, s, D, wprep);
   hipLaunchKernelGGL(k_nl_fwd, dim3(nfwd), dim3(256), 0, s, D, wfwd);
   hipLaunchKernelGGL(k_nl_gl, dim3(ngl), dim3(256), 0, s, D, wgl);
-  hipLaunchKernelGGL(k_nl_idft<true>, dim3(nchk), dim3(256), 0, s, D, wchk);
+  hipLaunchKernelGGL(k_nl_fft<true>, dim3(nchk), dim3(256), 0, s, D, wchk);
   hipLaunchKernelGGL(k_nl_lmax, dim3(nbin), dim3(64), 0, s, D, wbin);
-  hipLaunchKernelGGL(k_nl_idft<false>, dim3(nidft), dim3(256), 0, s, D, widft);
+  hipLaunchKernelGGL(k_nl_fft<false>, dim3(nidft), dim3(256), 0, s, D, widft);
   return hipGetLastError();
 }
 

@@ -5,7 +5,9 @@ the ln P run accumulator of the adjoint (csrc/clh_run.hpp against direct
 sums, tiles in either order; tests/run_check.cpp) and the data-vector
 placement (csrc/clh_place.hpp: block starts of every order, the Fourier
 placement into a buffer of exactly the vector's size; tests/place_check.cpp,
-built with AddressSanitizer).  The oracle, built with AddressSanitizer, runs
+built with AddressSanitizer) and the non-Limber FFTs (csrc/clh_fft.hpp: the
+mixed-radix Stockham stages, c2r and r2c packing against direct sums;
+tests/fft_check.cpp, AddressSanitizer).  The oracle, built with AddressSanitizer, runs
 the Fourier-then-real sequence of the GPU Fourier test (one ell, block
 order (0, 2, 1) among them)."""
 import os
@@ -21,13 +23,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ASAN = ["-fsanitize=address", "-fno-omit-frame-pointer", "-g"]
 
 
-@pytest.mark.parametrize("name", ["search_check", "run_check", "place_check"])
+@pytest.mark.parametrize("name", ["search_check", "run_check", "place_check", "fft_check"])
 def test_host_helpers(tmp_path, name):
     cxx = shutil.which("g++")
     if cxx is None:
         pytest.skip("no host C++ compiler")
     exe = tmp_path / name
-    extra = ASAN if name == "place_check" else []
+    extra = ASAN if name in ("place_check", "fft_check") else []
     subprocess.run([cxx, "-O2", "-std=c++17", *extra, "-I", os.path.join(ROOT, "cocoa-cosmolike-core_amd", "csrc"),
                     os.path.join(ROOT, "tests", name + ".cpp"), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True)
