@@ -20,6 +20,7 @@
 #include "../../include/cosmolike_hip.h"
 #include "clh_internal.hpp"
 #include "clh_place.hpp"
+#include "clh_fft.hpp"
 
 hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s);
@@ -849,7 +850,7 @@ static double bf16_value(uint16_t h) {
 static long nl_fft_size(long n) {
   for (;; n++) {
     long m = n;
-    for (int p : {2, 3, 5})  // the radices of the device FFT (clh_fft.hpp)
+    for (int p : {2, 3, 5, 7})  // next_fft_size (cosmo2D.c:4023-4034)
       while (m % p == 0) m /= p;
     if (m == 1) return n;
   }
@@ -910,7 +911,11 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   // 2 pi t / N_j for the three transform lengths
   int nlN[3], nlN0[3];
   nl_sizes(nlN, nlN0);
-  if (nlN[1] != nlN[2] || nlN[0] + nlN[1] > 2400)  // the transforms' LDS layout (k_nl_fft)
+  // the device transforms (clh_fft.hpp) take N / 2 with factors 2, 3, 5 and the
+  // LDS layout of k_nl_fft; the reference's paddings give N = 900, 1500, 1500
+  for (int j = 0; j < 3; j++)
+    if (!clh_fft_ok(nlN[j] / 2)) return fail(c, CLH_ERR_STATE, "non-Limber transform length %d has a factor 7", nlN[j]);
+  if (nlN[1] != nlN[2] || nlN[0] + nlN[1] > 2400)
     return fail(c, CLH_ERR_STATE, "non-Limber transform lengths %d, %d, %d not supported", nlN[0], nlN[1], nlN[2]);
   dvec tw;
   for (int j = 0; j < 3; j++) {
