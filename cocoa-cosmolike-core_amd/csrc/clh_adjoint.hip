@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 
 #include "clh_device.hpp"
 #include "clh_run.hpp"
@@ -277,7 +278,7 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
   const double b1 = ss ? 0.0 : d.par->b1[zl], bmag = ss ? 0.0 : d.par->bmag[zl];
   double twg = 0.0, twm = 0.0;
   ClhLnPRun sc;
-  sc.R = clh_run_row(d, kind, zl, p);
+  sc.out.R = clh_run_row(d, kind, zl, p);
   // per row: S = P fac of the node (from the forward's checkpoint, or the
   // P(k,z) lookup again), its k cell (i, dx), then the cotangent terms
   auto row = [&](int rr, double S, int i, double dx) {
@@ -316,7 +317,7 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
       twm += t * pref1;
       g = t * gp;
     }
-    if (sc.R) sc.add(i, dx, g);
+    if (sc.out.R) sc.add(i, dx, g);
   };
   if constexpr (CK) {
     // the forward's checkpoint of S (d.sxk); eight rows loaded ahead of their use
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
       row(rr, exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac, pkn.i, dx);
     }
   }
-  if (sc.R) sc.done();
+  if (sc.out.R) sc.done();
   if (ss) {
 #pragma unroll
     for (int b = 0; b < NS; b++)
@@ -366,7 +367,101 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
 // Per (row, node) with integrand t^2 P fac, t = W_gal b1 + W_mag pref bmag +
 // W_RSD: ln P cotangent cbar t^2 P fac; tb = 2 cbar t P fac feeds b1 (W_gal),
 // the W_gal / W_mag node cotangents and the lens shift through dW_RSD/d dz.
+//
+// The ln P cotangent: the 96 nodes of a lens bin sit in a narrow z range,
+// about ten nodes per z column of P(k,z), so they share cells.  The block
+// folds its nodes' runs over (k cell, z column) in an LDS band: per column
+// the cells its nodes reach over the tile's rows (from the bisection of the
+// first and last rows' log10 k), packed column after column.  After the rows
+// the band is added into g_lnP with atomics (the gather has stored the ss /
+// gs part there before this launch).  A column past CLH_BAND_COLS or beyond
+// CLH_BAND_CAP doubles, and any cell outside its column's range, is added
+// into g_lnP directly.
 // ----------------------------------------------------------------------------
+#define CLH_BAND_CAP 1024
+#define CLH_BAND_COLS 64
+struct ClhBandLds {
+  double A[CLH_BAND_CAP];
+  int lo[CLH_BAND_COLS], hi[CLH_BAND_COLS], start[CLH_BAND_COLS];
+  int jlo, nval, used;
+};
+
+// Band set-up of a gg block (every thread calls it; ends synced).  Node p of
+// the block's first group reports its columns jp, jp + 1 and cells
+// i(row 0) .. i(row nr - 1) + 1; out: the node's ClhBand.
+__device__ __forceinline__ void clh_band_setup(ClhBandLds& bd, const DrawDesc& d, const double* __restrict__ lk,
+                                               const ClhGgRow* R, int nr, int jp, double dyp, double l10fk,
+                                               bool reporter, ClhBand& out) {
+  const int t = threadIdx.x;
+  for (int u = t; u < CLH_BAND_CAP; u += blockDim.x) bd.A[u] = 0.0;
+  if (t < CLH_BAND_COLS) {
+    bd.lo[t] = INT_MAX;
+    bd.hi[t] = -1;
+  }
+  if (t == 0) bd.jlo = INT_MAX;
+  __syncthreads();
+  if (reporter) atomicMin(&bd.jlo, jp);
+  __syncthreads();
+  const int jlo = bd.jlo, c0 = jp - jlo;
+  if (reporter) {
+    const int i0 = clh_bs(lk, d.nk, R[0].l10 - l10fk);
+    const int i1 = clh_bs(lk, d.nk, R[nr - 1].l10 - l10fk) + 1;
+    for (int c = c0; c <= c0 + 1 && c < CLH_BAND_COLS; c++) {
+      atomicMin(&bd.lo[c], i0);
+      atomicMax(&bd.hi[c], i1);
+    }
+  }
+  __syncthreads();
+  if (t < 64) {  // first wave: column starts, an inclusive scan of the widths
+    const int w = bd.hi[t] >= bd.lo[t] ? bd.hi[t] - bd.lo[t] + 1 : 0;
+    int incl = w;
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+      const int u = __shfl_up(incl, k, 64);
+      if (t >= k) incl += u;
+    }
+    const bool room = incl <= CLH_BAND_CAP;
+    bd.start[t] = room ? incl - w : -1;
+    // the columns with room form a prefix (incl never decreases); the last
+    // of them ends the band
+    const int nval = (int)__popcll(__ballot(room));
+    if (t == 0) bd.nval = nval;
+    if (nval > 0 && t == nval - 1) bd.used = incl;
+    if (nval == 0 && t == 0) bd.used = 0;
+  }
+  __syncthreads();
+  out.G = d.g_lnP;
+  out.nz = d.nzp;
+  out.j = jp;
+  out.dy = dyp;
+  for (int c = 0; c < 2; c++) {
+    const int cc = c0 + c;
+    if (cc < CLH_BAND_COLS && bd.start[cc] >= 0) {
+      out.lo[c] = bd.lo[cc];
+      out.hi[c] = bd.hi[cc];
+      out.b[c] = bd.start[cc] - bd.lo[cc];
+    }
+  }
+}
+
+// Adds the band into g_lnP (after a sync): cell u of the band lies in the
+// last column whose start is <= u.
+__device__ __forceinline__ void clh_band_flush(const ClhBandLds& bd, const DrawDesc& d) {
+  const int nv = bd.nval, used = bd.used, jlo = bd.jlo, nz = d.nzp;
+  for (int u = threadIdx.x; u < used; u += blockDim.x) {
+    const double v = bd.A[u];
+    if (v == 0.0) continue;
+    int lo = 0, hi = nv - 1;  // start[lo] <= u < start[hi + 1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (bd.start[mid] <= u) lo = mid;
+      else hi = mid - 1;
+    }
+    const int i = bd.lo[lo] + (u - bd.start[lo]);
+    atomicAdd(d.g_lnP + (size_t)i * nz + jlo + lo, v);
+  }
+}
+
 template <bool CK>
 __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restrict__ D,
                                                        const int4* __restrict__ work, int stage) {
@@ -374,6 +469,7 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
   __shared__ double Cr[CLH_TILE_ROWS];
   __shared__ double s_lk[CLH_LDS_LK];
   __shared__ double red[2][3];
+  __shared__ ClhBandLds bd;
   extern __shared__ double dyn[];  // staged chi(z), G(z), lens n(z) tables
   const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
@@ -387,18 +483,22 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
   __syncthreads();
   const int p = threadIdx.x % np, half = threadIdx.x / np;
   double tg = 0.0, tm = 0.0, b1p = 0.0, lzp = 0.0;
-  ClhLnPRun sc;
-  sc.R = clh_run_row(d, CLH_W_GG, zl, p);
+  const double* o = clh_gg_nodes(d, zl);
+  const double* lut = o + 5 * np;
+  // the ln P cotangent is wanted when the draw has a run slab (uniform per block)
+  const bool lnp = d.run != nullptr;
+  ClhLnPRunT<ClhBand> sc;
+  sc.out.A = bd.A;  // (unconditionally: an LDS pointer throughout, ds_add_f64)
+  if (lnp) clh_band_setup(bd, d, T.lk, R, nr, (int)lut[p], lut[np + p], lut[2 * np + p], half == 0, sc.out);
   const int hn = (nr + 1) / 2;
   const int r_lo = half * hn, r_hi = half ? nr : hn;
-  const double* o = clh_gg_nodes(d, zl);
   const double WG = o[3 * np + p];
   if constexpr (CK) {
     // the forward's checkpoint (d.ggk): (t, P fac, dW_RSD) per row; only the
     // k cell of the ln P cotangent is walked again
     ClhKCell kc;
     kc.nk = d.nk;
-    const double l10fk = o[5 * np + 2 * np + p];
+    const double l10fk = lut[2 * np + p];
     // eight rows of the checkpoint are loaded before any is used, so their
     // latencies overlap
     const double* gk = d.ggk + ((size_t)(zl * d.nrow + row0) * 3) * np + p;
@@ -420,7 +520,7 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
         const double t = tv[u], pf = pv[u], dwr = wv[u];
         if (cb == 0.0 || pf == 0.0) continue;  // pf = 0: chi_1 beyond chi(a_min), zero integrand
         const double dx = kc.dx(T.lk, R[rr].l10 - l10fk);
-        if (sc.R) sc.add(kc.i, dx, cb * t * t * pf);
+        if (lnp) sc.add(kc.i, dx, cb * t * t * pf);
         const double tb = cb * 2.0 * t * pf;
         tg += tb;
         tm += tb * R[rr].pref;
@@ -436,7 +536,7 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
       if (cb == 0.0) continue;
       double t, pf, dwr, dx;
       if (!nd.eval(d, T, R[rr], t, pf, dwr, dx)) continue;
-      if (sc.R) sc.add(nd.pk.i, dx, cb * t * t * pf);
+      if (lnp) sc.add(nd.pk.i, dx, cb * t * t * pf);
       const double tb = cb * 2.0 * t * pf;
       tg += tb;
       tm += tb * R[rr].pref;
@@ -444,7 +544,7 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
       lzp += tb * dwr;
     }
   }
-  if (sc.R) sc.done();
+  if (lnp) sc.done();
   const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
   const size_t base = (size_t)2 * d.nl * CLH_NP_SS + (size_t)zl * 2 * np;
   if (tg != 0.0) atomicAdd(d.wbar + base + p, tg * b1);
@@ -460,6 +560,7 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
     if (sb != 0.0) atomicAdd(d.g_b1 + zl, sb);
     if (sl != 0.0) atomicAdd(d.g_lens + zl, sl);
   }
+  if (lnp) clh_band_flush(bd, d);
 }
 
 // ----------------------------------------------------------------------------
@@ -561,14 +662,15 @@ __global__ __launch_bounds__(256) void k_lim_grad(const DrawDesc* __restrict__ D
 }
 
 // ----------------------------------------------------------------------------
-// k_lnp_gather: the ln P cotangent of a draw from its per-node runs.  Work
-// item (draw, i0): lnP rows i0 .. i0 + ic - 1, every z column.  Node q (ss,
-// gs, gg blocks in run-slab order) with z column j and weight dy adds
-// R_q[i] (1 - dy) to cell (i, j) and R_q[i] dy to (i, j + 1) in an LDS tile
-// (LDS atomics: nodes of two groups may share a column); the tile then goes
-// out with coalesced stores and covers every cell, so g_lnP needs no zeroing.
-// The nodes' (j, dy) are staged in LDS first; group g of ic lanes takes nodes
-// g, g + 256/ic, ..., lane l reading R_q[i0 + l], four nodes per step.
+// k_lnp_gather: the ss / gs part of a draw's ln P cotangent from the per-node
+// runs.  Work item (draw, i0): lnP rows i0 .. i0 + ic - 1, every z column.
+// Node q (ss, then gs blocks in run-slab order) with z column j and weight dy
+// adds R_q[i] (1 - dy) to cell (i, j) and R_q[i] dy to (i, j + 1) in an LDS
+// tile (LDS atomics: nodes of two groups may share a column); the tile then
+// goes out with coalesced stores and covers every cell, so g_lnP needs no
+// zeroing (k_limber_gg_adj adds the gg part on top afterwards).  The nodes'
+// (j, dy) are staged in LDS first; group g of ic lanes takes nodes g, g +
+// 256/ic, ..., lane l reading R_q[i0 + l], four nodes per step.
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_lnp_gather(const DrawDesc* __restrict__ D,
                                                     const int2* __restrict__ work, int ic) {
@@ -576,25 +678,16 @@ __global__ __launch_bounds__(256) void k_lnp_gather(const DrawDesc* __restrict__
   const int2 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
   const int nz = d.nzp, nk = d.nk, i0 = w.y;
-  const int n_sx = CLH_NP_SS * (1 + d.nl), total = n_sx + CLH_NP_GG * d.nl;
+  const int total = CLH_NP_SS * (1 + d.nl);
   double* s_dy = A + ic * nz;
   int* s_j = (int*)(s_dy + total);
   for (int t = threadIdx.x; t < ic * nz; t += 256) A[t] = 0.0;
   for (int q = threadIdx.x; q < total; q += 256) {
-    const double* lut;
-    int p, np;
-    if (q < n_sx) {
-      const int kind = q < CLH_NP_SS ? CLH_W_SS : CLH_W_GS;
-      lut = clh_sx_nodes(d, kind, q / CLH_NP_SS - 1) + clh_sx_lut_col(d, kind) * CLH_NP_SS;
-      p = q % CLH_NP_SS;
-      np = CLH_NP_SS;
-    } else {
-      lut = clh_gg_nodes(d, (q - n_sx) / CLH_NP_GG) + 5 * CLH_NP_GG;
-      p = (q - n_sx) % CLH_NP_GG;
-      np = CLH_NP_GG;
-    }
+    const int kind = q < CLH_NP_SS ? CLH_W_SS : CLH_W_GS;
+    const double* lut = clh_sx_nodes(d, kind, q / CLH_NP_SS - 1) + clh_sx_lut_col(d, kind) * CLH_NP_SS;
+    const int p = q % CLH_NP_SS;
     s_j[q] = (int)lut[p];
-    s_dy[q] = lut[np + p];
+    s_dy[q] = lut[CLH_NP_SS + p];
   }
   __syncthreads();
   const int li = threadIdx.x % ic, grp = threadIdx.x / ic, ng = 256 / ic;
@@ -697,7 +790,7 @@ hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, 
 hipError_t clh_launch_lnp_gather(const DrawDesc* D, const int2* work, int n, int ic, int nz, int nl,
                                  hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const size_t total = (size_t)CLH_NP_SS * (1 + nl) + (size_t)CLH_NP_GG * nl;
+  const size_t total = (size_t)CLH_NP_SS * (1 + nl);
   const size_t lds = sizeof(double) * ((size_t)ic * nz + total) + sizeof(int) * total;
   hipLaunchKernelGGL(k_lnp_gather, dim3(n), dim3(256), lds, s, D, work, ic);
   return hipGetLastError();

@@ -420,15 +420,12 @@ __device__ __forceinline__ const double* clh_gg_nodes(const DrawDesc& d, int zl)
   return d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride;
 }
 
-// Row of the per-node ln P cotangent runs of GL node p: the ss nodes, then
-// 64 per gs lens bin, then 96 per gg lens bin (null when the caller did not
-// ask for the ln P gradient).
+// Row of the per-node ln P cotangent runs of ss / gs GL node p: the ss
+// nodes, then 64 per gs lens bin (null when the caller did not ask for the
+// ln P gradient).  The gg nodes fold theirs in LDS (k_limber_gg_adj).
 __device__ __forceinline__ double* clh_run_row(const DrawDesc& d, int kind, int zl, int p) {
   if (!d.run) return nullptr;
-  size_t r;
-  if (kind == CLH_W_SS) r = p;
-  else if (kind == CLH_W_GS) r = (size_t)CLH_NP_SS * (1 + zl) + p;
-  else r = (size_t)CLH_NP_SS * (1 + d.nl) + (size_t)CLH_NP_GG * zl + p;
+  const size_t r = kind == CLH_W_SS ? (size_t)p : (size_t)CLH_NP_SS * (1 + zl) + p;
   return d.run + r * d.nk;
 }
 

@@ -7,25 +7,25 @@
 #pragma once
 
 #ifdef __HIPCC__
-#define CLH_HD __host__ __device__
+#define CLH_FFT_HD __host__ __device__
 #else
-#define CLH_HD
+#define CLH_FFT_HD
 #endif
 
 struct clh_c2 {
   double re, im;
 };
 
-CLH_HD inline clh_c2 clh_c2_mul(clh_c2 a, clh_c2 b) {
+CLH_FFT_HD inline clh_c2 clh_c2_mul(clh_c2 a, clh_c2 b) {
   return clh_c2{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
 // The stage radices of a length-M transform: 5s, then 3s, 2s (the next
 // radix of the still unfactored part m; 0 when m has another factor)
-CLH_HD inline int clh_fft_radix(int m) {
+CLH_FFT_HD inline int clh_fft_radix(int m) {
   return m % 5 == 0 ? 5 : m % 3 == 0 ? 3 : m % 2 == 0 ? 2 : 0;
 }
-CLH_HD inline bool clh_fft_ok(int M) {
+CLH_FFT_HD inline bool clh_fft_ok(int M) {
   int m = M;
   while (m > 1) {
     const int R = clh_fft_radix(m);
@@ -38,7 +38,7 @@ CLH_HD inline bool clh_fft_ok(int M) {
 // v[k] <- sum_r v[r] e^{+2 pi i r k / R} for R = 2, 3, 5 (a template
 // argument, so that v stays in registers)
 template <int R>
-CLH_HD inline void clh_dft_small(clh_c2* v) {
+CLH_FFT_HD inline void clh_dft_small(clh_c2* v) {
   if (R == 2) {
     const clh_c2 a = v[0], b = v[1];
     v[0] = clh_c2{a.re + b.re, a.im + b.im};
@@ -74,7 +74,7 @@ struct ClhFftStage {
   int M, R, Ns, stride, tsc;  // stride = M / R, tsc = M / (Ns R)
   float rNs;                  // 1 / Ns
 };
-CLH_HD inline ClhFftStage clh_fft_stage(int M, int R, int Ns) {
+CLH_FFT_HD inline ClhFftStage clh_fft_stage(int M, int R, int Ns) {
   return ClhFftStage{M, R, Ns, M / R, M / (Ns * R), 1.0f / (float)Ns};
 }
 
@@ -83,7 +83,7 @@ CLH_HD inline ClhFftStage clh_fft_stage(int M, int R, int Ns) {
 // out[(j / Ns) Ns R + (j mod Ns) + r Ns].  After every stage, out holds the
 // inverse DFT sum_k in[k] e^{+2 pi i k n / M} in natural order.
 template <int R, class TW>
-CLH_HD inline void clh_fft_bfly_r(const clh_c2* in, clh_c2* out, const ClhFftStage& st, int j, const TW& tw) {
+CLH_FFT_HD inline void clh_fft_bfly_r(const clh_c2* in, clh_c2* out, const ClhFftStage& st, int j, const TW& tw) {
   clh_c2 v[R];
   int jq = (int)((float)j * st.rNs);  // j / Ns, exact after one correction either way (j < 2^20)
   if (jq * st.Ns > j) jq--;
@@ -101,7 +101,7 @@ CLH_HD inline void clh_fft_bfly_r(const clh_c2* in, clh_c2* out, const ClhFftSta
   for (int r = 0; r < R; r++) out[o + r * st.Ns] = v[r];
 }
 template <class TW>
-CLH_HD inline void clh_fft_bfly(const clh_c2* in, clh_c2* out, const ClhFftStage& st, int j, const TW& tw) {
+CLH_FFT_HD inline void clh_fft_bfly(const clh_c2* in, clh_c2* out, const ClhFftStage& st, int j, const TW& tw) {
   switch (st.R) {
     case 2: clh_fft_bfly_r<2>(in, out, st, j, tw); break;
     case 3: clh_fft_bfly_r<3>(in, out, st, j, tw); break;
@@ -113,7 +113,7 @@ CLH_HD inline void clh_fft_bfly(const clh_c2* in, clh_c2* out, const ClhFftStage
 // holds the c2r inverse transform of length N = 2M: z_n = x[2n] + i x[2n+1],
 // x[n'] = sum_{m=0}^{N-1} X_m e^{2 pi i m n' / N} for a Hermitian X given by
 // X_0 .. X_M.  e = e^{+2 pi i k / N}.
-CLH_HD inline clh_c2 clh_c2r_pack(clh_c2 Xk, clh_c2 XMk, clh_c2 e) {
+CLH_FFT_HD inline clh_c2 clh_c2r_pack(clh_c2 Xk, clh_c2 XMk, clh_c2 e) {
   const clh_c2 s{Xk.re + XMk.re, Xk.im - XMk.im};  // X_k + conj(X_{M-k})
   const clh_c2 d{Xk.re - XMk.re, Xk.im + XMk.im};  // X_k - conj(X_{M-k})
   const clh_c2 ed = clh_c2_mul(e, d);
@@ -124,7 +124,7 @@ CLH_HD inline clh_c2 clh_c2r_pack(clh_c2 Xk, clh_c2 XMk, clh_c2 e) {
 // i x[2n+1] (real x of length N = 2M) gives Y_k = sum_{n'} x[n'] e^{+2 pi i
 // k n' / N} for 0 <= k <= M as E_k + e O_k, E_k = (Z_k + conj(Z_{M-k})) / 2,
 // O_k = (Z_k - conj(Z_{M-k})) / (2 i), e = e^{+2 pi i k / N} (Z_M = Z_0).
-CLH_HD inline clh_c2 clh_r2c_unpack(clh_c2 Zk, clh_c2 ZMk, clh_c2 e) {
+CLH_FFT_HD inline clh_c2 clh_r2c_unpack(clh_c2 Zk, clh_c2 ZMk, clh_c2 e) {
   const clh_c2 E{0.5 * (Zk.re + ZMk.re), 0.5 * (Zk.im - ZMk.im)};
   const clh_c2 O{0.5 * (Zk.im + ZMk.im), -0.5 * (Zk.re - ZMk.re)};  // (Z_k - conj(Z_{M-k})) / (2 i)
   const clh_c2 eo = clh_c2_mul(e, O);

@@ -2863,8 +2863,13 @@ int clh_synchronize(clh_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->timing) {
     for (int k = 0; k < 5; k++) HIPCHK(c, hipEventElapsedTime(&c->last_ms[k], c->ev[k], c->ev[k + 1]));
-    if (c->vjp_timed)
-      for (int k = 0; k < 4; k++) HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[k], c->ev[6 + k], c->ev[7 + k]));
+    if (c->vjp_timed) {
+      // launch order sx_adj, gather, gg_adj, lens limits; kept as gg_adj,
+      // sx_adj, gather, lens limits
+      static const int slot[4] = {1, 2, 0, 3};
+      for (int k = 0; k < 4; k++)
+        HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[slot[k]], c->ev[6 + k], c->ev[7 + k]));
+    }
   }
   return CLH_OK;
 }
@@ -3103,12 +3108,13 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
                n_dg = al((size_t)(ns + nl) * L.n_a);
   const size_t per = n_cbar + n_raw + n_abar + n_bbar + n_wbar + n_dg;
   HIPCHK(c, bt.d_vjp.ensure(per * B));
-  // per-node ln P cotangent runs and the gather work list (k rows per item:
-  // the LDS tile of k rows x nz columns plus the nodes' (j, dy) fits 64 KB)
+  // per-node ln P cotangent runs of the ss / gs nodes and the gather work
+  // list (k rows per item: the LDS tile of k rows x nz columns plus the
+  // nodes' (j, dy) fits 64 KB); the gg nodes fold theirs in LDS
   const int nk = bt.desc[0].nk, nzp = bt.desc[0].nzp;
-  const size_t n_run = al((size_t)(CLH_NP_SS * (1 + nl) + CLH_NP_GG * nl) * nk);
+  const size_t n_run = al((size_t)CLH_NP_SS * (1 + nl) * nk);
   if (g_lnP) {
-    const size_t nq = (size_t)CLH_NP_SS * (1 + nl) + (size_t)CLH_NP_GG * nl;
+    const size_t nq = (size_t)CLH_NP_SS * (1 + nl);
     auto lds = [&](int ic) { return sizeof(double) * ((size_t)ic * nzp + nq) + sizeof(int) * nq; };
     int ic = 16;
     while (ic > 2 && lds(ic) > 65536) ic /= 2;
@@ -3167,7 +3173,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // zero the accumulators (fill kernels, not memsets: safe under graph capture)
   HIPCHK(c, clh_launch_fill(bt.d_vjp.p, per * B, 0.0, s));
   HIPCHK(c, clh_launch_fill(bt.d_gscr.p, gper * B, 0.0, s));
-  if (g_lnP) HIPCHK(c, clh_launch_fill(bt.d_run.p, n_run * B, 0.0, s));  // g_lnP: every cell written by the gather
+  if (g_lnP) HIPCHK(c, clh_launch_fill(bt.d_run.p, n_run * B, 0.0, s));  // g_lnP: every cell stored by the gather
   if (g_src_dz) HIPCHK(c, clh_launch_fill(g_src_dz, (size_t)ns * B, 0.0, s));
   if (g_lens_dz && nl) HIPCHK(c, clh_launch_fill(g_lens_dz, (size_t)nl * B, 0.0, s));
   if (g_b1 && nl) HIPCHK(c, clh_launch_fill(g_b1, (size_t)nl * B, 0.0, s));
@@ -3187,16 +3193,18 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa4.p, bt.n_wa, s));
   HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
+  // ss / gs adjoint, the gather that stores their part of g_lnP, then the gg
+  // adjoint adding its part on top (events 6 .. 9 in this order)
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));
-  const bool use_ggk = bt.state_valid && bt.ckpt_fresh && bt.ggk_per;
-  HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga,
-                                     (c->gg_lds[1] && !use_ggk) ? bt.gg_stage : 0, use_ggk ? 1 : 0, s));
-  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
   HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, ck_all ? 1 : 0, s));
   HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, ns,
                                      ck_all ? 1 : 0, s));
-  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
   if (g_lnP) HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
+  const bool use_ggk = bt.state_valid && bt.ckpt_fresh && bt.ggk_per;
+  HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga,
+                                     (c->gg_lds[1] && !use_ggk) ? bt.gg_stage : 0, use_ggk ? 1 : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[9], s));
   c->vjp_timed = c->timing;
   if (g_lens_dz && nl > 0) {

@@ -179,7 +179,7 @@ struct DrawDesc {
   double* wbar;         // [nl][2][64] gs nodes (W_gal, W_mag), then [nl][2][96] gg nodes
   double* dgtab;        // [ns+nl][n_a] d g / d(photo-z shift of its own bin)
   double* g_lnP;        // [nk][nzp]
-  double* run;          // [64 (1 + nl) + 96 nl][nk] per-GL-node ln P cotangent runs, or null
+  double* run;          // [64 (1 + nl)][nk] per-GL-node ln P cotangent runs of the ss / gs nodes, or null
   double* sxk;          // [1 + nl][nrow][64] ss / gs checkpoint of the forward (S = P fac per
                         // row and node) for the VJP, or null
   // TATT intrinsic alignments (ia_model == CLH_IA_TATT)
