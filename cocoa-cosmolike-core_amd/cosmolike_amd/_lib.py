@@ -101,6 +101,7 @@ def lib():
             "clh_test_zoverlap": (i32, [VP, i32, i32]),
             "clh_int_for_limber": (i32, [VP, i32, D, D, i32, D]),
             "clh_set_vjp_checkpoint": (i32, [VP, i32]),
+            "clh_set_vjp_lens_limits": (i32, [VP, i32]),
             "clh_init_binning_fourier": (i32, [VP, i32, i32, i32, i32]),
             "clh_datavector_size_fourier": (i32, [VP]),
             "clh_compute_3x2pt_fourier": (i32, [VP, I, I, D, i32]),

@@ -593,6 +593,12 @@ class Cosmolike:
         """Batch forwards keep the gg integrand for a following VJP."""
         self._chk(self.L.clh_set_vjp_checkpoint(self.h, 1 if on else 0))
 
+    def set_vjp_lens_limits(self, mode=0):
+        """Lens-limit part of the lens-shift gradient: 0 from the nodes'
+        motion inside the adjoint kernels (default), 1 by finite-difference
+        views (a cross-check), 2 left out (the gradient at fixed limits)."""
+        self._chk(self.L.clh_set_vjp_lens_limits(self.h, int(mode)))
+
     def enable_timing(self, on=True):
         self._chk(self.L.clh_enable_timing(self.h, 1 if on else 0))
 

@@ -219,8 +219,15 @@ __global__ __launch_bounds__(64) void k_m_grad(const DrawDesc* __restrict__ D) {
 // LDS: ss [ns][ns][nrows], gs [ns][nrows] (dynamic).  With the forward's
 // checkpoint (d.sxk) the P(k,z) lookup is replaced by its stored S = P fac
 // and only the k cell is walked again.
+// LIM (gs tiles, no checkpoint): also the lens-limit part of the lens-shift
+// gradient, sum over rows and bins of cbar dC/d dz through the motion of the
+// nodes (k_nodes_tan): with g the row's ln P cotangent, the node's share is
+// a_t (sum_b B_b' acc_b + W_gal' b1 twg + W_mag' bmag twm + dlog10k' sum g
+// dlnP/dlog10k + dy' sum g dlnP/dy) + (ln(weight fac))_t sum g, where acc,
+// twg, twm are the node cotangents above (the derivative of C_gs along the
+// node's a, contracted with cbar, is built from them).
 // ----------------------------------------------------------------------------
-template <int NS, bool CK>
+template <int NS, bool CK, bool LIM = false>
 __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict__ D,
                                                       const int4* __restrict__ work) {
   // NS: the source-bin count when it is at most 10 (loops fully unrolled to
@@ -238,7 +245,8 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
   const int p = threadIdx.x;
   constexpr int np = CLH_NP_SS;
   const int nrow = d.nrow, ns = EXACT ? NS : d.ns;
-  const bool ss = kind == CLH_W_SS;
+  static_assert(!(LIM && CK), "the lens-limit term walks P(k,z) again");
+  const bool ss = !LIM && kind == CLH_W_SS;  // LIM: launched on gs tiles only
   const double* o = clh_sx_nodes(d, kind, zl);
   const double* lut = o + clh_sx_lut_col(d, kind) * np;
   const double* ells = ss ? d.ell_ss : d.ell_gx;
@@ -318,7 +326,9 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
       g = t * gp;
     }
     if (sc.out.R) sc.add(i, dx, g);
+    return g;
   };
+  double slk = 0.0, slz = 0.0, sgl = 0.0;  // LIM: sum g dlnP/dlog10k, sum g dlnP/dy, sum g
   if constexpr (CK) {
     // the forward's checkpoint of S (d.sxk); eight rows loaded ahead of their use
     const double* sk = d.sxk + ((size_t)(ss ? 0 : 1 + zl) * nrow + row0) * np + p;
@@ -336,12 +346,17 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
       }
     }
   } else {
-    ClhPkNodeT<false> pkn;  // no P-row look-ahead: it would cost a wave per SIMD
+    ClhPkNodeT<false, LIM> pkn;  // no P-row look-ahead: it would cost a wave per SIMD
     pkn.init(d, j, dy);
     for (int rr = 0; rr < T; rr++) {
       double dx;
       const double out = pkn.lnp(lk, L10[rr] - l10fk, dx);
-      row(rr, exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac, pkn.i, dx);
+      const double g = row(rr, exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac, pkn.i, dx);
+      if (LIM) {
+        slk += g * pkn.dlnp_dk();
+        slz += g * pkn.dlnp_dy(dx);
+        sgl += g;
+      }
     }
   }
   if (sc.out.R) sc.done();
@@ -357,6 +372,18 @@ __global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict
     atomicAdd(d.wbar + ((size_t)zl * 2 + 1) * np + p, twm * bmag);
     const double b1p = clh_wave_sum(twg * WG);
     if (p == 0 && b1p != 0.0) atomicAdd(d.g_b1 + zl, b1p);
+    if (LIM) {
+      const double* tn = d.ntan + (size_t)zl * (6 + ns) * np;
+      double sb = 0.0;
+#pragma unroll
+      for (int b = 0; b < NS; b++)
+        if (EXACT || b < ns) sb += tn[(6 + b) * np + p] * acc[b];
+      double lim = tn[p] * (sb + tn[4 * np + p] * b1 * twg + tn[5 * np + p] * bmag * twm + tn[2 * np + p] * slk +
+                            tn[3 * np + p] * slz) +
+                   tn[np + p] * sgl;
+      lim = clh_wave_sum(lim);
+      if (p == 0 && lim != 0.0) atomicAdd(d.g_lens + zl, lim);
+    }
   }
 }
 
@@ -462,7 +489,16 @@ __device__ __forceinline__ void clh_band_flush(const ClhBandLds& bd, const DrawD
   }
 }
 
-template <bool CK>
+// LIM (no checkpoint): also the lens-limit part of the lens-shift gradient
+// (k_nodes_tan gives the nodes' motion).  Along the node's a the integrand
+// t^2 P fac changes through t (W_gal b1, W_mag pref bmag, the a_0 half c0 t0
+// and the a_1 half c1 hf1 n1 of W_RSD, whose chi_1 = chi (ell+1)/ell moves
+// with the node's chi) and through ln P and fac; contracted with cbar, the
+// node's share is a_t (W_gal' b1 sum tb + W_mag' bmag sum tb pref + t0' sum tb
+// c0 - chi' sum tb c1 d(hf1 n1)/dchi + dlog10k' sum g dlnP/dlog10k + dy' sum g
+// dlnP/dy) + (ln(weight fac))_t sum g, with tb = 2 cbar t P fac and g =
+// cbar t^2 P fac the row's cotangents of t and ln P.
+template <bool CK, bool LIM = false>
 __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restrict__ D,
                                                        const int4* __restrict__ work, int stage) {
   __shared__ ClhGgRow R[CLH_TILE_ROWS];
@@ -470,6 +506,7 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
   __shared__ double s_lk[CLH_LDS_LK];
   __shared__ double red[2][3];
   __shared__ ClhBandLds bd;
+  static_assert(!(LIM && CK), "the lens-limit term evaluates the integrand again");
   extern __shared__ double dyn[];  // staged chi(z), G(z), lens n(z) tables
   const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
@@ -493,6 +530,7 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
   const int hn = (nr + 1) / 2;
   const int r_lo = half * hn, r_hi = half ? nr : hn;
   const double WG = o[3 * np + p];
+  double sc0 = 0.0, sa1 = 0.0, slk = 0.0, slz = 0.0, sgl = 0.0;  // LIM sums
   if constexpr (CK) {
     // the forward's checkpoint (d.ggk): (t, P fac, dW_RSD) per row; only the
     // k cell of the ln P cotangent is walked again
@@ -529,19 +567,27 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
       }
     }
   } else {
-    ClhGgNode<true, false> nd;  // no P-row look-ahead: it would cost a wave per SIMD
+    ClhGgNode<true, false, LIM> nd;  // no P-row look-ahead: it would cost a wave per SIMD
     nd.init(d, T, o, p, zl);
     for (int rr = r_lo; rr < r_hi; rr++) {
       const double cb = Cr[rr];
       if (cb == 0.0) continue;
       double t, pf, dwr, dx;
       if (!nd.eval(d, T, R[rr], t, pf, dwr, dx)) continue;
-      if (lnp) sc.add(nd.pk.i, dx, cb * t * t * pf);
+      const double g = cb * t * t * pf;
+      if (lnp) sc.add(nd.pk.i, dx, g);
       const double tb = cb * 2.0 * t * pf;
       tg += tb;
       tm += tb * R[rr].pref;
       b1p += tb * WG;
       lzp += tb * dwr;
+      if (LIM) {
+        sc0 += tb * R[rr].c0;
+        sa1 += tb * R[rr].c1 * nd.dhn;
+        slk += g * nd.pk.dlnp_dk();
+        slz += g * nd.pk.dlnp_dy(dx);
+        sgl += g;
+      }
     }
   }
   if (lnp) sc.done();
@@ -549,6 +595,13 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
   const size_t base = (size_t)2 * d.nl * CLH_NP_SS + (size_t)zl * 2 * np;
   if (tg != 0.0) atomicAdd(d.wbar + base + p, tg * b1);
   if (tm != 0.0 && bmag != 0.0) atomicAdd(d.wbar + base + np + p, tm * bmag);
+  if (LIM) {
+    const double* tn = d.ntan + (size_t)d.nl * (6 + d.ns) * CLH_NP_SS + (size_t)zl * 8 * np;
+    const double lim = tn[p] * (tn[4 * np + p] * b1 * tg + tn[5 * np + p] * bmag * tm + tn[6 * np + p] * sc0 -
+                                tn[7 * np + p] * sa1 + tn[2 * np + p] * slk + tn[3 * np + p] * slz) +
+                       tn[np + p] * sgl;
+    lzp += lim;
+  }
   b1p = clh_wave_sum(b1p);
   lzp = clh_wave_sum(lzp);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -617,6 +670,138 @@ __global__ __launch_bounds__(128) void k_nodes_adj(const DrawDesc* __restrict__ 
   if ((p & 63) == 0) {
     if (gA != 0.0) atomicAdd(d.g_A, gA);
     if (zl >= 0 && gl != 0.0) atomicAdd(d.g_lens + zl, gl);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// k_nodes_tan: the motion of the gs and gg GL nodes of lens bin zl with the
+// bin's photo-z shift, for the lens-limit part of the lens-shift gradient.
+// amin_lens = 1/(1 + zmax + 2|dz|) and, without magnification, amax_lens =
+// 1/(1 + max(zmin - 2|dz|, 0.001)) (redshift_spline.c:50-79), so node n sits
+// at a = B + A x_n with weight A w_n and both move with dz.  Per node: a_t =
+// da/d dz, the relative change of its weight times fac (fac = A w dchi/da /
+// chi^2), and the derivatives along a (at fixed ell) of everything the gs /
+// gg integrands read from the node: log10 k = log10(ell) - log10(chi c/H0),
+// the P(k,z) z weight dy, W_gal, W_mag, the source weights B_zs (gs), the
+// a_0 half of W_RSD and chi (gg).  Each is the derivative of the forward's
+// own piecewise arithmetic on the node's table intervals (chi_all's linear
+// chi and stencil dchi/dz, cosmo3D.c:40-92; the growth table; the n(z)
+// spline pieces; the lens-efficiency lerp).  Segment layout as k_nodes; the
+// ss segment has none.  dz = 0: zero (the limits' kink).
+// ----------------------------------------------------------------------------
+// slope of interpol1d (basics.c:457-480) at x: the interval's, zero where it
+// extrapolates by a constant
+__device__ __forceinline__ double clh_interpol1d_slope(const double* __restrict__ f, int n, double a, double dx,
+                                                       double x) {
+  if (x < a) return 0.0;
+  const int i = (int)floor((x - a) / dx);
+  if (i + 1 >= n) return 0.0;
+  return (f[i + 1] - f[i]) / dx;
+}
+
+__global__ __launch_bounds__(128) void k_nodes_tan(const DrawDesc* __restrict__ D, const int2* __restrict__ work) {
+  const int2 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const int seg = w.y;
+  if (seg == 0) return;
+  const bool gs = seg <= d.nl;
+  const int zl = gs ? seg - 1 : seg - 1 - d.nl, ns = d.ns;
+  const int np = gs ? CLH_NP_SS : CLH_NP_GG;
+  const int p = threadIdx.x;
+  if (p >= np) return;
+  double* tn = gs ? d.ntan + (size_t)zl * (6 + ns) * CLH_NP_SS
+                  : d.ntan + (size_t)d.nl * (6 + ns) * CLH_NP_SS + (size_t)zl * 8 * CLH_NP_GG;
+  const double* o = gs ? d.nodes + d.node_gs + (size_t)zl * d.node_gs_stride
+                       : d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride;
+  const double* lut = o + (gs ? 5 + ns : 5) * np;
+  const double dz = d.par->lens_dz[zl];
+  const double sg = dz > 0.0 ? 1.0 : (dz < 0.0 ? -1.0 : 0.0);
+  // the limits per unit dz
+  const double amin = clh_amin_lens(d, zl), amax = clh_amax_lens(d, zl);
+  const double amin_t = -2.0 * amin * amin * sg;
+  double amax_t = 0.0;
+  if (d.par->bmag[zl] == 0) {
+    const double zmin = (d.l_zmin[zl] - d.l_zmean[zl]) * d.par->lens_st[zl] + d.l_zmean[zl];
+    if (zmin - 2. * clh_lim_adz(d, zl) > 0.001) amax_t = 2.0 * amax * amax * sg;
+  }
+  const double* glx = d.gl + (np == CLH_NP_SS ? 0 : 2 * CLH_NP_SS);
+  const double A = 0.5 * (amax - amin), A_t = 0.5 * (amax_t - amin_t), B_t = 0.5 * (amax_t + amin_t);
+  const double u = o[p];
+  const double u_t = B_t + A_t * glx[p];
+  constexpr double C = CLH_COVERH0;
+  // chi_all at a = u (cosmo3D.c:40-92): chi and the stencil dchi/dz linear in z on interval j
+  const double z = 1.0 / u - 1.0, z_u = -1.0 / (u * u);
+  const double* __restrict__ zz = d.chi_z;
+  const double* __restrict__ cc = d.chi_v;
+  int j = clh_bs(zz, d.chi_n, z);
+  if (j > d.chi_n - 3) j = d.chi_n - 3;
+  const double hz = zz[j + 1] - zz[j];
+  const double dyc = (z - zz[j]) / hz;
+  const double fK = (cc[j] + dyc * (cc[j + 1] - cc[j])) / C;
+  const double fK_u = (cc[j + 1] - cc[j]) / hz * z_u / C;
+  const double up = (cc[j + 2] - cc[j]) / (zz[j + 2] - zz[j]);
+  const double down = j > 0 ? (cc[j + 1] - cc[j - 1]) / (zz[j + 1] - zz[j - 1]) : (cc[j + 1] - cc[j]) / hz;
+  const double out1 = down + dyc * (up - down);
+  const double dchida = out1 / C / (u * u);
+  const double dchida_u = (up - down) / hz * z_u / C / (u * u) - 2.0 * dchida / u;
+  const double hh = 1.0 / ((u * u) * dchida);
+  const double hh_u = -hh * (2.0 / u + dchida_u / dchida);
+  const int jp = (int)lut[0 * np + p];
+  tn[p] = u_t;
+  tn[np + p] = A_t / A + (dchida_u / dchida - 2.0 * fK_u / fK) * u_t;
+  tn[2 * np + p] = -fK_u / (fK * M_LN10);
+  tn[3 * np + p] = z_u / (d.pz[jp + 1] - d.pz[jp]);
+  // W_gal = n_lens(z) hh, W_mag = wk_pre g_lens (n_lens' = -d n_lens / d dz)
+  const double nlv = clh_nz_lens(d, z, zl), nl_z = -clh_dnz_lens(d, z, zl);
+  tn[4 * np + p] = nl_z * z_u * hh + nlv * hh_u;
+  const double wk_pre = 1.5 * d.omega_m * fK / u;
+  const double wk_u = 1.5 * d.omega_m * (fK_u / u - fK / (u * u));
+  const double gla = 1.0 / (d.l_zmax_all + 1.0), gda = (0.999999 - gla) / ((double)d.n_a - 1.0);
+  const bool gl_in = !(u < gla || u > 1.0 - gda);
+  const double* gtl = d.gtab + (size_t)(ns + zl) * d.n_a;
+  const double gL = clh_g_lens(d, u, zl), gL_u = gl_in ? clh_interpol1d_slope(gtl, d.n_a, gla, gda, u) : 0.0;
+  tn[5 * np + p] = wk_u * gL + wk_pre * gL_u;
+  if (gs) {
+    // B_zs = wk_pre g_tomo - n_src hh C1 (NLA, IA.c:238-296)
+    const int jg = clh_bs(d.G_z, d.G_n, z);
+    const double slG = (d.G_v[jg + 1] - d.G_v[jg]) / (d.G_z[jg + 1] - d.G_z[jg]);
+    const double Gz = d.G_v[jg] + (z - d.G_z[jg]) * slG;
+    const double gf = Gz * u / d.G0, gf_u = (slG * z_u * u + Gz) / d.G0;
+    const double gsa = 1.0 / (d.s_zmax_all + 1.0), gsd = (0.999999 - gsa) / ((double)d.n_a - 1.0);
+    const bool gs_in = !(u <= gsa || u > 1.0 - gsd);
+    for (int zs = 0; zs < ns; zs++) {
+      const double gT = clh_g_tomo(d, u, zs);
+      const double gT_u = gs_in ? clh_interpol1d_slope(d.gtab + (size_t)zs * d.n_a, d.n_a, gsa, gsd, u) : 0.0;
+      const double nsv = clh_nz_src(d, z, zs), ns_z = -clh_dnz_src(d, z, zs);
+      const double WS = nsv * hh, WS_u = ns_z * z_u * hh + nsv * hh_u;
+      const double C1 = clh_ia_c1(d, u, gf, zs);
+      const double C1_u = -C1 * gf_u / gf - (d.ia_mode == 3 ? C1 * d.par->ia_eta / u : 0.0);
+      tn[(6 + zs) * np + p] = wk_u * gT + wk_pre * gT_u - WS_u * C1 - WS * C1_u;
+    }
+  } else {
+    // t0 = n_lens(z0) hf0 at a_0 = a_chi(chi) (radial_weights.c:98-99):
+    // z0 linear in chi on the chi(z) interval jc, hf0 = (c/H0) f(z0) / out1(z0)
+    // with the stencil out1 = dchi/dz and f = 1 - slG (1 + z) / G (cosmo3D.c:200-228)
+    const double chi = fK * C;
+    const int jc = (int)lut[3 * np + p];
+    const double sz = (zz[jc + 1] - zz[jc]) / (cc[jc + 1] - cc[jc]);
+    const double z0 = zz[jc] + (chi - cc[jc]) * sz;
+    const double z0_u = C * fK_u * sz;
+    int j0 = clh_bs(zz, d.chi_n, z0);
+    if (j0 > d.chi_n - 3) j0 = d.chi_n - 3;
+    const double hz0 = zz[j0 + 1] - zz[j0];
+    const double up0 = (cc[j0 + 2] - cc[j0]) / (zz[j0 + 2] - zz[j0]);
+    const double dn0 = j0 > 0 ? (cc[j0 + 1] - cc[j0 - 1]) / (zz[j0 + 1] - zz[j0 - 1]) : (cc[j0 + 1] - cc[j0]) / hz0;
+    const double o1 = dn0 + (z0 - zz[j0]) / hz0 * (up0 - dn0), o1_z = (up0 - dn0) / hz0;
+    const int jg = clh_bs(d.G_z, d.G_n, z0);
+    const double slG = (d.G_v[jg + 1] - d.G_v[jg]) / (d.G_z[jg + 1] - d.G_z[jg]);
+    const double Gz = d.G_v[jg] + (z0 - d.G_z[jg]) * slG;
+    const double f = 1.0 - slG * (1.0 + z0) / Gz;
+    const double f_z = -slG / Gz * (1.0 - (1.0 + z0) * slG / Gz);
+    const double hf0 = C * f / o1, hf0_z = C * (f_z / o1 - f * o1_z / (o1 * o1));
+    const double n0 = clh_nz_lens(d, z0, zl), n0_z = -clh_dnz_lens(d, z0, zl);
+    tn[6 * np + p] = (n0_z * hf0 + n0 * hf0_z) * z0_u;
+    tn[7 * np + p] = fK_u;
   }
 }
 
@@ -754,14 +939,16 @@ hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int max_npair, int ns,
-                                   int ck, hipStream_t s) {
-  // max_npair: largest npair x rows of any work item (doubles of LDS)
+                                   int ck, int lim, hipStream_t s) {
+  // max_npair: largest npair x rows of any work item (doubles of LDS); lim:
+  // gs tiles with the lens-limit term (no checkpoint)
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)max_npair;  // max over items of npair x rows
-#define CLH_SXADJ(NSV)                                                                            \
-  do {                                                                                            \
-    if (ck) hipLaunchKernelGGL((k_limber_sx_adj<NSV, true>), dim3(n), dim3(64), lds, s, D, work);  \
-    else hipLaunchKernelGGL((k_limber_sx_adj<NSV, false>), dim3(n), dim3(64), lds, s, D, work);    \
+#define CLH_SXADJ(NSV)                                                                                \
+  do {                                                                                                \
+    if (lim) hipLaunchKernelGGL((k_limber_sx_adj<NSV, false, true>), dim3(n), dim3(64), lds, s, D, work); \
+    else if (ck) hipLaunchKernelGGL((k_limber_sx_adj<NSV, true>), dim3(n), dim3(64), lds, s, D, work);  \
+    else hipLaunchKernelGGL((k_limber_sx_adj<NSV, false>), dim3(n), dim3(64), lds, s, D, work);        \
   } while (0)
   switch (ns) {
     case 1: CLH_SXADJ(1); break;
@@ -780,10 +967,13 @@ hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, 
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
-                                   hipStream_t s) {
+                                   int lim, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)stage_doubles;
-  if (ck) hipLaunchKernelGGL((k_limber_gg_adj<true>), dim3(n), dim3(192), lds, s, D, work, stage_doubles > 0 ? 1 : 0);
+  if (lim)
+    hipLaunchKernelGGL((k_limber_gg_adj<false, true>), dim3(n), dim3(192), lds, s, D, work,
+                       stage_doubles > 0 ? 1 : 0);
+  else if (ck) hipLaunchKernelGGL((k_limber_gg_adj<true>), dim3(n), dim3(192), lds, s, D, work, stage_doubles > 0 ? 1 : 0);
   else hipLaunchKernelGGL((k_limber_gg_adj<false>), dim3(n), dim3(192), lds, s, D, work, stage_doubles > 0 ? 1 : 0);
   return hipGetLastError();
 }
@@ -793,6 +983,11 @@ hipError_t clh_launch_lnp_gather(const DrawDesc* D, const int2* work, int n, int
   const size_t total = (size_t)CLH_NP_SS * (1 + nl);
   const size_t lds = sizeof(double) * ((size_t)ic * nz + total) + sizeof(int) * total;
   hipLaunchKernelGGL(k_lnp_gather, dim3(n), dim3(256), lds, s, D, work, ic);
+  return hipGetLastError();
+}
+hipError_t clh_launch_nodes_tan(const DrawDesc* D, const int2* work, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nodes_tan, dim3(n), dim3(128), 0, s, D, work);
   return hipGetLastError();
 }
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s) {

@@ -513,9 +513,14 @@ __device__ __forceinline__ ClhGgTabs clh_gg_tabs(const DrawDesc& d, int zl, doub
 // loaded one move ahead: a one-cell step then needs no load to complete (the
 // values are the same ones set_cell would read).  The look-ahead costs 6
 // VGPRs, so kernels near an occupancy step leave it off.
-template <bool AHEAD = true>
+// TAN: the cell's ln P differences along z at its two k knots, M0 = ln P[i][j+1]
+// - ln P[i][j] and M1 = ln P[i+1][j+1] - ln P[i+1][j], are kept as well, so
+// that d ln P / d dy = (1 - dx) M0 + dx M1 (and d ln P / d log10 k =
+// (L1 - L0) inv_k) is at hand for the tangent of the lookup.
+template <bool AHEAD = true, bool TAN = false>
 struct ClhPkNodeT {
   int i, j, nk, nz;
+  double M0, M1;
   // cell state: lo / hi bound the walk test (-inf / +inf at the table ends,
   // where clh_bs extrapolates), x0 = lk[i], inv_k = 1 / (lk[i+1] - lk[i]),
   // L0 / L1 the ln P interpolated in z at lk[i] / lk[i+1], so that
@@ -559,6 +564,10 @@ struct ClhPkNodeT {
     const double* __restrict__ P = lnP + (size_t)i * nz + j;
     L0 = zint(P);
     L1 = zint(P + nz);
+    if (TAN) {
+      M0 = P[1] - P[0];
+      M1 = P[nz + 1] - P[nz];
+    }
     ahead(lk);
   }
   // ln P at log10 k; dx: the log10 k weight of the cell (i, j)
@@ -576,6 +585,10 @@ struct ClhPkNodeT {
         inv_k = 1.0 / (nhi - x0);
         L0 = L1;
         L1 = (1 - dy) * n0 + dy * n1;
+        if (TAN) {
+          M0 = M1;
+          M1 = n1 - n0;
+        }
         ahead(lk);
       } else {
         i = clh_walk(lk, nk, log10k, i);
@@ -585,6 +598,9 @@ struct ClhPkNodeT {
     dx = (log10k - x0) * inv_k;
     return L0 + dx * (L1 - L0);
   }
+  // TAN: d ln P / d log10 k and d ln P / d dy at the last lookup (weight dx)
+  __device__ __forceinline__ double dlnp_dk() const { return (L1 - L0) * inv_k; }
+  __device__ __forceinline__ double dlnp_dy(double dx) const { return (1 - dx) * M0 + dx * M1; }
 };
 
 typedef ClhPkNodeT<true> ClhPkNode;
@@ -628,8 +644,13 @@ struct ClhBracket {
   }
 };
 
-template <bool ADJ, bool AHEAD = true>
+// TAN (with ADJ): eval also leaves in dhn the derivative of the a_1 half's
+// hf1 n1 (W_RSD = c0 t0 - c1 hf1 n1) with respect to the node's chi at the
+// row, on the row's table intervals, and the P(k,z) walk keeps its z
+// differences: the pieces of the integrand's derivative along the node's a.
+template <bool ADJ, bool AHEAD = true, bool TAN = false>
 struct ClhGgNode {
+  static_assert(ADJ || !TAN, "the tangent uses the adjoint's n(z) derivative");
   // row-independent values of the node
   double fK, wgb, wmb, t0, t0d, c3fac, l10fk, chi_amin;
   double inv_st, zs0, zlo, zhi;  // lens n(z) argument zs = z1 inv_st + zs0; first and last knot
@@ -649,7 +670,8 @@ struct ClhGgNode {
   double D0, D1;                         // z interval (the chi interval's): dchi/dz = D0 + z1 D1
   double glo, ghi, G0, slG;              // G interval (own bounds only off the chi grid): G = G0 + z1 slG
   double nlo, nhi_, cf0, cf1, cf2, cf3;  // lens spline interval: knots, coefficients / stretch
-  ClhPkNodeT<AHEAD> pk;
+  double dhn;                            // TAN: d(hf1 n1) / d chi of the last row
+  ClhPkNodeT<AHEAD, TAN> pk;
 
   __device__ __forceinline__ void rec_z(const double* __restrict__ r) {  // the z interval's coefficients
     D0 = r[4];
@@ -806,7 +828,14 @@ struct ClhGgNode {
       jg = clh_walk(T.gz, G_n, z1, jg);
       set_g();
     }
-    const double hf1 = CLH_COVERH0 * (G0 - slG) / ((D0 + z1 * D1) * (G0 + z1 * slG));
+    const double DG = D0 + z1 * D1, GG = G0 + z1 * slG;
+    double hf1, inv = 0.0;
+    if (TAN) {
+      inv = 1.0 / (DG * GG);
+      hf1 = CLH_COVERH0 * (G0 - slG) * inv;
+    } else {
+      hf1 = CLH_COVERH0 * (G0 - slG) / (DG * GG);
+    }
     // nz_lens(z1) (redshift_spline.c:736-762) and its shift derivative
     const double zs = z1 * inv_st + zs0;
     if (!(zs >= nlo && zs < nhi_)) move_n(T, zs);
@@ -814,6 +843,11 @@ struct ClhGgNode {
     const double n1 = cf0 + x * (cf1 + x * (cf2 + x * cf3));
     double dn1 = 0.0;
     if (ADJ) dn1 = -(cf1 + x * (2.0 * cf2 + 3.0 * x * cf3)) * inv_st;
+    if (TAN) {
+      // d(hf1 n1)/d z1 = hf1 (n1' - n1 (D1 / DG + slG / GG)), n1' = -dn1;
+      // z1 = zA + chi rr1 (c/H0) zS on the chi interval
+      dhn = hf1 * (-dn1 - n1 * (D1 * GG + slG * DG) * inv) * (zS * R.rr1 * CLH_COVERH0);
+    }
     const double wr = R.c0 * t0 - R.c1 * hf1 * n1;
     if (ADJ) dwr = R.c0 * t0d - R.c1 * hf1 * dn1;
     t = wgb + R.pref * wmb + wr;

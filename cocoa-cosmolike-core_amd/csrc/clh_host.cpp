@@ -47,9 +47,10 @@ hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nk
 hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
 hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles, int ns, int ck,
-                                   hipStream_t s);
+                                   int lim, hipStream_t s);
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
-                                   hipStream_t s);
+                                   int lim, hipStream_t s);
+hipError_t clh_launch_nodes_tan(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_lim_grad(const DrawDesc* D, const DrawDesc* P, const DrawDesc* M, const int2* work, int n,
                                hipStream_t s);
@@ -410,6 +411,7 @@ struct clh_ctx {
   const int* ord6 = nullptr;            // 6-block order of a clh_compute_6x2pt_real call in progress
   int gg_lds[2] = {0, 1};               // stage tables in LDS: fwd gg, adj gg
   int vjp_ckpt = 0;                     // batch forwards keep the gg checkpoint for a VJP
+  int vjp_lim = 0;                      // lens-limit term: 0 nodes' motion, 1 finite-difference views, 2 none
   uint64_t version = 1, cached_version = 0;
   dvec cache_raw;   // raw (uncalibrated, ord 0,1,2) data vector of `cur`
   dvec cache_cl;    // cl rows of `cur`
@@ -2890,6 +2892,15 @@ int clh_set_vjp_checkpoint(clh_ctx* c, int on) {
   return CLH_OK;
 }
 
+int clh_set_vjp_lens_limits(clh_ctx* c, int mode) {
+  if (!c || mode < 0 || mode > 2) return CLH_ERR_ARG;
+  if (mode != c->vjp_lim) {
+    c->vjp_lim = mode;
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  }
+  return CLH_OK;
+}
+
 int clh_enable_timing(clh_ctx* c, int on) {
   if (!c) return CLH_ERR_ARG;
   c->timing = on != 0;
@@ -3105,8 +3116,13 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   const size_t n_cbar = al((size_t)ncol * nrow), n_raw = al(bt.dv_size), n_abar = al((size_t)ns * CLH_NP_SS),
                n_bbar = al((size_t)nl * ns * CLH_NP_SS),
                n_wbar = al((size_t)nl * 2 * CLH_NP_SS + (size_t)nl * 2 * CLH_NP_GG),
-               n_dg = al((size_t)(ns + nl) * L.n_a);
-  const size_t per = n_cbar + n_raw + n_abar + n_bbar + n_wbar + n_dg;
+               n_dg = al((size_t)(ns + nl) * L.n_a),
+               n_tan = al((size_t)nl * (6 + ns) * CLH_NP_SS + (size_t)nl * 8 * CLH_NP_GG);
+  const size_t per = n_cbar + n_raw + n_abar + n_bbar + n_wbar + n_dg + n_tan;
+  // the lens-limit part of the lens-shift gradient from the nodes' motion
+  // (k_nodes_tan and the LIM adjoint instantiations), unless the
+  // finite-difference views are asked for
+  const bool lim_tan = g_lens_dz && nl > 0 && c->vjp_lim == 0;
   HIPCHK(c, bt.d_vjp.ensure(per * B));
   // per-node ln P cotangent runs of the ss / gs nodes and the gather work
   // list (k rows per item: the LDS tile of k rows x nz columns plus the
@@ -3162,6 +3178,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     q.ggk = (ck && bt.ggk_per) ? bt.d_ggk.p + bt.ggk_per * i : nullptr;
     q.sxk = ck ? bt.d_sxk.p + bt.sxk_per * i : nullptr;
     q.dgtab = ck ? bt.d_dgk.p + bt.dgk_per * i : v;
+    q.ntan = lim_tan ? v + n_dg : nullptr;
     q.g_src = g_src_dz ? g_src_dz + (size_t)ns * i : g + al(bt.nknz);
     q.g_lens = g_lens_dz ? g_lens_dz + (size_t)nl * i : g + al(bt.nknz) + al(CLH_MAXB);
     q.g_b1 = g_b1 ? g_b1 + (size_t)nl * i : g + al(bt.nknz) + 2 * al(CLH_MAXB);
@@ -3193,21 +3210,25 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa4.p, bt.n_wa, s));
   HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
+  if (lim_tan) HIPCHK(c, clh_launch_nodes_tan(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
   // ss / gs adjoint, the gather that stores their part of g_lnP, then the gg
   // adjoint adding its part on top (events 6 .. 9 in this order)
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, ck_all ? 1 : 0, s));
+  // (with the lens-limit term the gs and gg adjoints evaluate their
+  // integrands again: the term needs the P(k,z) cells and the a_1 walks)
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, ck_all ? 1 : 0, 0, s));
   HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, ns,
-                                     ck_all ? 1 : 0, s));
+                                     (ck_all && !lim_tan) ? 1 : 0, lim_tan ? 1 : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
   if (g_lnP) HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
-  const bool use_ggk = bt.state_valid && bt.ckpt_fresh && bt.ggk_per;
+  const bool use_ggk = bt.state_valid && bt.ckpt_fresh && bt.ggk_per && !lim_tan;
   HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga,
-                                     (c->gg_lds[1] && !use_ggk) ? bt.gg_stage : 0, use_ggk ? 1 : 0, s));
+                                     (c->gg_lds[1] && !use_ggk) ? bt.gg_stage : 0, use_ggk ? 1 : 0,
+                                     lim_tan ? 1 : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[9], s));
   c->vjp_timed = c->timing;
-  if (g_lens_dz && nl > 0) {
+  if (g_lens_dz && nl > 0 && c->vjp_lim == 1) {
     // The lens integration limits move with |dz| (redshift_spline.c:50-79),
     // and the GL nodes of the bin's gs and gg integrals with them; the
     // adjoint kernels above hold the nodes fixed.  That part comes from the

@@ -182,6 +182,13 @@ struct DrawDesc {
   double* run;          // [64 (1 + nl)][nk] per-GL-node ln P cotangent runs of the ss / gs nodes, or null
   double* sxk;          // [1 + nl][nrow][64] ss / gs checkpoint of the forward (S = P fac per
                         // row and node) for the VJP, or null
+  // the motion of the gs / gg nodes with the lens photo-z shift (the lens
+  // integration limits move with |dz|), per node of lens bin zl, for the VJP
+  // (k_nodes_tan), or null: gs [nl][6 + ns][64] = da/d dz, d ln(GL weight x
+  // fac)/d dz, then per unit a: d log10 k, d dy (P(k,z) z weight), W_gal,
+  // W_mag, B_zs[ns]; then gg [nl][8][96] = the first six, t0 (the a_0 half of
+  // W_RSD) and chi
+  double* ntan;
   // TATT intrinsic alignments (ia_model == CLH_IA_TATT)
   int ia_model;
   double* cl_bb;        // [n_ss][nrow] B-mode C_ss rows, or null (NLA: B mode is zero)

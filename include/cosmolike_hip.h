@@ -385,6 +385,18 @@ int clh_enable_timing(clh_ctx* ctx, int on);
  * memory).  After a mixed-precision forward the checkpoint holds its FP32
  * P(k,z) products.  Default off.  No reference counterpart. */
 int clh_set_vjp_checkpoint(clh_ctx* ctx, int on);
+/* How clh_batch_vjp obtains the part of the lens-shift gradient that comes
+ * from the lens integration limits, which move with |dz| (amin_lens /
+ * amax_lens, redshift_spline.c:50-79), and the gs / gg GL nodes with them.
+ * mode 0 (default): the derivative of the integrands along each node's
+ * motion, inside the gs and gg adjoint kernels (exact for the discrete
+ * forward away from the tables' knots).  mode 1: finite differences of two
+ * views whose limits take |dz| +- 1e-8, a cross-check of mode 0 at about
+ * twice the VJP's cost.  mode 2: left out (the gradient at fixed limits).
+ * With mode 0 and the lens-shift gradient requested,
+ * the gs and gg adjoints evaluate their integrands again even when a
+ * checkpoint is present.  No reference counterpart. */
+int clh_set_vjp_lens_limits(clh_ctx* ctx, int mode);
 
 /* ---------------------------------------- reference-named scalar entries */
 /* Identical signatures to cosmolike/cosmo2D.h:30-96, served from clh_default().

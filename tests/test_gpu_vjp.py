@@ -206,3 +206,53 @@ def test_vjp_checkpoint_matches_recompute(setup):
         same(run(P, True, update_after=P2), run(P2, False))
     finally:
         c.set_vjp_checkpoint(False)
+
+
+def test_lens_limit_term_tangent_vs_views():
+    """The lens-limit part of the lens-shift gradient: the GL nodes of a lens
+    bin's gs and gg integrals move with |dz| (amin_lens / amax_lens,
+    redshift_spline.c:50-79).  The default (clh_set_vjp_lens_limits 0) takes
+    it from the nodes' motion inside the adjoint kernels; mode 1 from
+    finite-difference views of the forward (|dz| +- 1e-8, per-row knot-jump
+    rejection); mode 2 leaves it out.  The term (mode - mode 2) of modes 0 and
+    1 agree to 2e-3 of its size, on draws with lens shifts of both signs,
+    stretched lens bins and magnification bias (amax then fixed); the other
+    gradients do not depend on the mode."""
+    import torch
+    draws = []
+    for i, (cos, *_r) in enumerate(synth.random_draws(3, seed=11, ragged=False)):
+        p = synth.full_3x2pt_problem(cos)
+        rng = np.random.default_rng(300 + i)
+        p.src_dz = rng.normal(0, 0.01, p.ns)
+        p.lens_dz = rng.normal(0, 0.02, p.nl)
+        p.lens_stretch = 1 + rng.normal(0, 0.05, p.nl)
+        p.b1 = np.linspace(1.2, 2.0, p.nl) * (1 + 0.05 * i)
+        if i == 1:
+            p.bmag = rng.normal(0, 1.0, p.nl)
+        draws.append(p)
+    c = Cosmolike(device=0).load(draws[0])
+    c.batch_upload(draws)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.array(a), dtype=torch.float64, device=dev)
+    P = dict(lnP=t([d.lnP for d in draws]), src_dz=t([d.src_dz for d in draws]),
+             lens_dz=t([d.lens_dz for d in draws]), b1=t([d.b1 for d in draws]),
+             ia_A=t([d.ia_A for d in draws]), shear_m=t([d.shear_m for d in draws]))
+    v = t(np.random.default_rng(43).normal(size=(len(draws), c.datavector_size())))
+    g = {}
+    try:
+        for mode in (0, 1, 2):
+            c.set_vjp_lens_limits(mode)
+            g[mode] = _vjp(c, P, v)
+    finally:
+        c.set_vjp_lens_limits(0)
+    term0 = g[0]["lens_dz"] - g[2]["lens_dz"]
+    term1 = g[1]["lens_dz"] - g[2]["lens_dz"]
+    size = np.abs(term1).max(axis=1)
+    scale = np.abs(g[1]["lens_dz"]).max(axis=1)
+    err = np.abs(term0 - term1).max(axis=1) / size
+    print("lens-limit term / gradient scale", size / scale, "tangent vs views", err)
+    assert (size > 1e-4 * scale).all()
+    assert err.max() < 2e-3, (term0, term1)
+    for k in ("lnP", "src_dz", "b1", "ia_A", "shear_m"):
+        a, b = g[0][k], g[2][k]
+        assert np.abs(a - b).max() <= 1e-11 * np.abs(b).max(), k
