@@ -498,14 +498,19 @@ __device__ __forceinline__ void clh_band_flush(const ClhBandLds& bd, const DrawD
 // c0 - chi' sum tb c1 d(hf1 n1)/dchi + dlog10k' sum g dlnP/dlog10k + dy' sum g
 // dlnP/dy) + (ln(weight fac))_t sum g, with tb = 2 cbar t P fac and g =
 // cbar t^2 P fac the row's cotangents of t and ln P.
+// Three waves per SIMD are asked for: the LIM instantiation would otherwise
+// take 192 VGPRs (two waves) and run 8% slower than with a few spills.
 template <bool CK, bool LIM = false>
-__global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restrict__ D,
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k_limber_gg_adj(const DrawDesc* __restrict__ D,
                                                        const int4* __restrict__ work, int stage) {
   __shared__ ClhGgRow R[CLH_TILE_ROWS];
   __shared__ double Cr[CLH_TILE_ROWS];
   __shared__ double s_lk[CLH_LDS_LK];
   __shared__ double red[2][3];
   __shared__ ClhBandLds bd;
+  // LIM: per node t0' and chi' (per unit a), dlog10k/da, d dy/da and the a_0
+  // half of dW_RSD/d dz, read per row from LDS to spare registers
+  __shared__ double TC[LIM ? 5 * CLH_NP_GG : 1];
   static_assert(!(LIM && CK), "the lens-limit term evaluates the integrand again");
   extern __shared__ double dyn[];  // staged chi(z), G(z), lens n(z) tables
   const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
@@ -517,6 +522,17 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
     Cr[t] = d.cbar[(size_t)(d.n_ss + d.n_gs + zl) * d.nrow + row0 + t];
   }
   const ClhGgTabs T = clh_gg_tabs(d, zl, dyn, stage, s_lk);
+  if (LIM) {
+    const double* tn = d.ntan + (size_t)d.nl * (6 + d.ns) * CLH_NP_SS + (size_t)zl * 8 * np;
+    const double* lt = clh_gg_nodes(d, zl) + 5 * np;
+    for (int q = threadIdx.x; q < np; q += 192) {
+      TC[q] = tn[6 * np + q];
+      TC[np + q] = tn[7 * np + q];
+      TC[2 * np + q] = tn[2 * np + q];
+      TC[3 * np + q] = tn[3 * np + q];
+      TC[4 * np + q] = lt[8 * np + q];
+    }
+  }
   __syncthreads();
   const int p = threadIdx.x % np, half = threadIdx.x / np;
   double tg = 0.0, tm = 0.0, b1p = 0.0, lzp = 0.0;
@@ -530,7 +546,7 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
   const int hn = (nr + 1) / 2;
   const int r_lo = half * hn, r_hi = half ? nr : hn;
   const double WG = o[3 * np + p];
-  double sc0 = 0.0, sa1 = 0.0, slk = 0.0, slz = 0.0, sgl = 0.0;  // LIM sums
+  double sl = 0.0, sgl = 0.0;  // LIM sums
   if constexpr (CK) {
     // the forward's checkpoint (d.ggk): (t, P fac, dW_RSD) per row; only the
     // k cell of the ln P cotangent is walked again
@@ -562,7 +578,6 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
         const double tb = cb * 2.0 * t * pf;
         tg += tb;
         tm += tb * R[rr].pref;
-        b1p += tb * WG;
         lzp += tb * dwr;
       }
     }
@@ -579,14 +594,15 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
       const double tb = cb * 2.0 * t * pf;
       tg += tb;
       tm += tb * R[rr].pref;
-      b1p += tb * WG;
-      lzp += tb * dwr;
       if (LIM) {
-        sc0 += tb * R[rr].c0;
-        sa1 += tb * R[rr].c1 * nd.dhn;
-        slk += g * nd.pk.dlnp_dk();
-        slz += g * nd.pk.dlnp_dy(dx);
+        const double* tc = TC + p;
+        const double c0 = R[rr].c0;
+        lzp += tb * (dwr + c0 * tc[4 * np]);
+        sl += tb * (c0 * tc[0] - R[rr].c1 * nd.dhn * tc[np]) +
+              g * (nd.pk.dlnp_dk() * tc[2 * np] + nd.pk.dlnp_dy(dx) * tc[3 * np]);
         sgl += g;
+      } else {
+        lzp += tb * dwr;
       }
     }
   }
@@ -597,12 +613,9 @@ __global__ __launch_bounds__(192) void k_limber_gg_adj(const DrawDesc* __restric
   if (tm != 0.0 && bmag != 0.0) atomicAdd(d.wbar + base + np + p, tm * bmag);
   if (LIM) {
     const double* tn = d.ntan + (size_t)d.nl * (6 + d.ns) * CLH_NP_SS + (size_t)zl * 8 * np;
-    const double lim = tn[p] * (tn[4 * np + p] * b1 * tg + tn[5 * np + p] * bmag * tm + tn[6 * np + p] * sc0 -
-                                tn[7 * np + p] * sa1 + tn[2 * np + p] * slk + tn[3 * np + p] * slz) +
-                       tn[np + p] * sgl;
-    lzp += lim;
+    lzp += tn[p] * (tn[4 * np + p] * b1 * tg + tn[5 * np + p] * bmag * tm + sl) + tn[np + p] * sgl;
   }
-  b1p = clh_wave_sum(b1p);
+  b1p = clh_wave_sum(WG * tg);  // d/d b1 through W_gal b1: W_gal sum tb
   lzp = clh_wave_sum(lzp);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) { red[0][wid] = b1p; red[1][wid] = lzp; }

@@ -790,7 +790,7 @@ struct ClhGgNode {
     jg = (int)lut[5 * np + p];
     jn = (int)lut[6 * np + p];
     t0 = lut[7 * np + p];
-    t0d = ADJ ? lut[8 * np + p] : 0.0;
+    t0d = (ADJ && !TAN) ? lut[8 * np + p] : 0.0;
     inv_st = 1.0 / d.par->lens_st[zl];
     const double zm = d.l_zmean[zl];
     zs0 = (-d.par->lens_dz[zl] - zm) * inv_st + zm;
@@ -849,7 +849,8 @@ struct ClhGgNode {
       dhn = hf1 * (-dn1 - n1 * (D1 * GG + slG * DG) * inv) * (zS * R.rr1 * CLH_COVERH0);
     }
     const double wr = R.c0 * t0 - R.c1 * hf1 * n1;
-    if (ADJ) dwr = R.c0 * t0d - R.c1 * hf1 * dn1;
+    // (TAN: without the a_0 half c0 t0d, which the caller adds per node)
+    if (ADJ) dwr = (TAN ? 0.0 : R.c0 * t0d) - R.c1 * hf1 * dn1;
     t = wgb + R.pref * wmb + wr;
     const double out = pk.lnp(T.lk, R.l10 - l10fk, dx);
     if (F32) pf = (double)(expf((float)out) * (float)c3fac);
