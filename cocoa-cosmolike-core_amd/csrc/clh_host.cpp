@@ -439,7 +439,8 @@ struct clh_ctx {
   bool use_graph = false;             // replay the batch forward from a hipGraph
   hipGraphExec_t gexec = nullptr;     // captured forward of `batch`
   hipStream_t gstream = nullptr;      // stream the graph was captured on
-  hipEvent_t ev[11] = {nullptr};
+  hipEvent_t ev[13] = {nullptr};  // 0-5 forward stages, 6-10 VJP stages, 11-12 k_nodes_tan
+  bool vjp_lim_tan = false;         // the last timed VJP took the lens-limit term from k_nodes_tan
   float last_ms[5] = {0, 0, 0, 0, 0};
   float last_vjp_ms[4] = {0, 0, 0, 0};
   DBuf<double> d_intf;  // scale factors and values of clh_int_for_limber
@@ -2871,6 +2872,9 @@ int clh_synchronize(clh_ctx* c) {
       static const int slot[4] = {1, 2, 0, 3};
       for (int k = 0; k < 4; k++)
         HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[slot[k]], c->ev[6 + k], c->ev[7 + k]));
+      // lens limits from the nodes' motion: k_nodes_tan (the rest of the
+      // term is inside the gs / gg adjoint kernels' times)
+      if (c->vjp_lim_tan) HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[3], c->ev[11], c->ev[12]));
     }
   }
   return CLH_OK;
@@ -2961,14 +2965,18 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
   const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt && !bt.nl_any && !bt.oneloop &&
                    !bt.lays[0]->sixpt;  // the VJP refuses these
-  DBuf<double> dvbar, glnp;  // unit cotangent and ln P gradient of the adjoint stages
+  // unit cotangent, ln P and lens-shift gradients of the adjoint stages (the
+  // ln P runs and the lens-limit term are part of what is timed)
+  DBuf<double> dvbar, glnp, glens;
   if (adj) {
     hipError_t e = dvbar.ensure((size_t)bt.dv_off[bt.B]);
     if (e == hipSuccess) e = clh_launch_fill(dvbar.p, (size_t)bt.dv_off[bt.B], 1.0, c->stream);
-    if (e == hipSuccess) e = glnp.ensure((size_t)bt.nknz * bt.B);  // the ln P runs are part of what is timed
+    if (e == hipSuccess) e = glnp.ensure((size_t)bt.nknz * bt.B);
+    if (e == hipSuccess) e = glens.ensure((size_t)CLH_MAXB * bt.B);
     if (e != hipSuccess) {
       dvbar.release();
       glnp.release();
+      glens.release();
       return fail(c, CLH_ERR_HIP, "autotune: %s", hipGetErrorString(e));
     }
   }
@@ -2989,7 +2997,7 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
       c->gg_lds[0] = c->gg_lds[1] = st;
       for (int rep = 0; rep < 3 && rc == CLH_OK; rep++) {
         if ((rc = run_batch(c, bt, false))) break;
-        if (adj && (rc = clh_batch_vjp(c, ord, dvbar.p, glnp.p, nullptr, nullptr, nullptr, nullptr, nullptr)))
+        if (adj && (rc = clh_batch_vjp(c, ord, dvbar.p, glnp.p, nullptr, glens.p, nullptr, nullptr, nullptr)))
           break;
         if ((rc = clh_synchronize(c))) break;
         const float ms[4] = {c->last_ms[3], c->last_ms[2], adj ? c->last_vjp_ms[1] : 1e30f,
@@ -3025,7 +3033,7 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
           for (int rep = 0; rep < 3 && rc == CLH_OK; rep++) {
             if ((he = hipEventRecord(e0, c->stream)) != hipSuccess) break;
             if ((rc = run_batch(c, bt, false))) break;
-            if ((rc = clh_batch_vjp(c, ord, dvbar.p, glnp.p, nullptr, nullptr, nullptr, nullptr, nullptr))) break;
+            if ((rc = clh_batch_vjp(c, ord, dvbar.p, glnp.p, nullptr, glens.p, nullptr, nullptr, nullptr))) break;
             if ((he = hipEventRecord(e1, c->stream)) != hipSuccess) break;
             if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
             float ms = 0.0f;
@@ -3053,6 +3061,7 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   (void)hipStreamSynchronize(c->stream);
   dvbar.release();  // local buffers (hipFree)
   glnp.release();
+  glens.release();
   if (rc) {
     c->gg_lds[0] = lds_was[0];
     c->gg_lds[1] = lds_was[1];
@@ -3210,7 +3219,12 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa4.p, bt.n_wa, s));
   HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
-  if (lim_tan) HIPCHK(c, clh_launch_nodes_tan(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
+  if (lim_tan) {
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[11], s));
+    HIPCHK(c, clh_launch_nodes_tan(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[12], s));
+  }
+  c->vjp_lim_tan = lim_tan;
   // ss / gs adjoint, the gather that stores their part of g_lnP, then the gg
   // adjoint adding its part on top (events 6 .. 9 in this order)
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));

@@ -373,7 +373,9 @@ int clh_get_tile_rows(clh_ctx* ctx, int rows[4]);
 int clh_autotune(clh_ctx* ctx, int rows_out[4], int lds_out[2]);
 /* per-kernel timing of the last clh_batch_forward (and, after a
  * clh_batch_vjp, its two Limber adjoint stages, the ln P gather and the
- * lens-limit term of the lens-shift gradient) with
+ * lens-limit term of the lens-shift gradient: the finite-difference views, or
+ * k_nodes_tan when the term comes from the nodes' motion, whose remainder is
+ * then part of the adjoint stages) with
  * HIP events on the context stream: names[i], milliseconds[i]; returns the
  * kernel count */
 int clh_last_kernel_times(clh_ctx* ctx, const char** names, float* ms, int cap);
