@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "clh_device.hpp"
 #include "clh_run.hpp"
@@ -500,7 +501,9 @@ __device__ __forceinline__ void clh_band_flush(const ClhBandLds& bd, const DrawD
 // cbar t^2 P fac the row's cotangents of t and ln P.
 // Three waves per SIMD are asked for: the LIM instantiation would otherwise
 // take 192 VGPRs (two waves) and run 8% slower than with a few spills.
-template <bool CK, bool LIM = false>
+// F32: the integrand evaluated in FP32 (as k_limber_gg<F32>), the cotangent
+// sums in FP64 (the mixed-precision modes' VJP).
+template <bool CK, bool LIM = false, bool F32 = false>
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k_limber_gg_adj(const DrawDesc* __restrict__ D,
                                                        const int4* __restrict__ work, int stage) {
   __shared__ ClhGgRow R[CLH_TILE_ROWS];
@@ -582,12 +585,13 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k
       }
     }
   } else {
-    ClhGgNode<true, false, LIM> nd;  // no P-row look-ahead: it would cost a wave per SIMD
+    typedef typename std::conditional<F32, float, double>::type real;
+    ClhGgNode<true, false, LIM, real> nd;  // no P-row look-ahead: it would cost a wave per SIMD
     nd.init(d, T, o, p, zl);
     for (int rr = r_lo; rr < r_hi; rr++) {
       const double cb = Cr[rr];
       if (cb == 0.0) continue;
-      double t, pf, dwr, dx;
+      real t, pf, dwr, dx;
       if (!nd.eval(d, T, R[rr], t, pf, dwr, dx)) continue;
       const double g = cb * t * t * pf;
       if (lnp) sc.add(nd.pk.i, dx, g);
@@ -980,12 +984,14 @@ hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, 
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
-                                   int lim, hipStream_t s) {
+                                   int lim, int f32, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)stage_doubles;
-  if (lim)
-    hipLaunchKernelGGL((k_limber_gg_adj<false, true>), dim3(n), dim3(192), lds, s, D, work,
-                       stage_doubles > 0 ? 1 : 0);
+  const int st = stage_doubles > 0 ? 1 : 0;
+  if (lim && f32) hipLaunchKernelGGL((k_limber_gg_adj<false, true, true>), dim3(n), dim3(192), lds, s, D, work, st);
+  else if (lim) hipLaunchKernelGGL((k_limber_gg_adj<false, true>), dim3(n), dim3(192), lds, s, D, work, st);
+  else if (f32 && !ck)
+    hipLaunchKernelGGL((k_limber_gg_adj<false, false, true>), dim3(n), dim3(192), lds, s, D, work, st);
   else if (ck) hipLaunchKernelGGL((k_limber_gg_adj<true>), dim3(n), dim3(192), lds, s, D, work, stage_doubles > 0 ? 1 : 0);
   else hipLaunchKernelGGL((k_limber_gg_adj<false>), dim3(n), dim3(192), lds, s, D, work, stage_doubles > 0 ? 1 : 0);
   return hipGetLastError();

@@ -517,20 +517,22 @@ __device__ __forceinline__ ClhGgTabs clh_gg_tabs(const DrawDesc& d, int zl, doub
 // - ln P[i][j] and M1 = ln P[i+1][j+1] - ln P[i+1][j], are kept as well, so
 // that d ln P / d dy = (1 - dx) M0 + dx M1 (and d ln P / d log10 k =
 // (L1 - L0) inv_k) is at hand for the tangent of the lookup.
-template <bool AHEAD = true, bool TAN = false>
+// R: the arithmetic type of the walk and the interpolation (float in the
+// FP32 integrands; the table stays FP64 in memory).
+template <bool AHEAD = true, bool TAN = false, typename R = double>
 struct ClhPkNodeT {
   int i, j, nk, nz;
-  double M0, M1;
+  R M0, M1;
   // cell state: lo / hi bound the walk test (-inf / +inf at the table ends,
   // where clh_bs extrapolates), x0 = lk[i], inv_k = 1 / (lk[i+1] - lk[i]),
   // L0 / L1 the ln P interpolated in z at lk[i] / lk[i+1], so that
   // (1-dx)(1-dy)p00 + (1-dx)dy p01 + dx(1-dy)p10 + dx dy p11 = L0 + dx (L1 - L0)
-  double dy, lo, hi, x0, inv_k, L0, L1;
-  double nhi, n0, n1;  // lk[i + 2], ln P[i + 2][j], ln P[i + 2][j + 1] (i + 2 <= nk - 1)
+  R dy, lo, hi, x0, inv_k, L0, L1;
+  R nhi, n0, n1;  // lk[i + 2], ln P[i + 2][j], ln P[i + 2][j + 1] (i + 2 <= nk - 1)
   const double* __restrict__ lnP;
 
-  __device__ __forceinline__ double zint(const double* __restrict__ P) const {
-    return (1 - dy) * P[0] + dy * P[1];
+  __device__ __forceinline__ R zint(const double* __restrict__ P) const {
+    return (1 - dy) * (R)P[0] + dy * (R)P[1];
   }
   __device__ __forceinline__ void ahead(const double* __restrict__ lk) {
     if (AHEAD && i + 2 <= nk - 1) {
@@ -543,13 +545,13 @@ struct ClhPkNodeT {
   }
   __device__ __forceinline__ void bounds(const double* __restrict__ lk) {
     x0 = lk[i];
-    const double x1 = lk[i + 1];
-    inv_k = 1.0 / (x1 - x0);
+    const R x1 = lk[i + 1];
+    inv_k = R(1) / (x1 - x0);
     lo = i == 0 ? -INFINITY : x0;
     hi = i == nk - 2 ? INFINITY : x1;
   }
 
-  __device__ __forceinline__ void init(const DrawDesc& d, int j_, double dy_) {
+  __device__ __forceinline__ void init(const DrawDesc& d, int j_, R dy_) {
     i = -1;
     lo = INFINITY;  // empty cell: the first lookup searches
     hi = -INFINITY;
@@ -571,7 +573,7 @@ struct ClhPkNodeT {
     ahead(lk);
   }
   // ln P at log10 k; dx: the log10 k weight of the cell (i, j)
-  __device__ __forceinline__ double lnp(const double* __restrict__ lk, double log10k, double& dx) {
+  __device__ __forceinline__ R lnp(const double* __restrict__ lk, R log10k, R& dx) {
     if (!(log10k >= lo && log10k < hi)) {
       if (i < 0) {
         i = clh_bs(lk, nk, log10k);
@@ -582,7 +584,7 @@ struct ClhPkNodeT {
         x0 = hi;  // finite: the old cell was not the last
         lo = x0;
         hi = i == nk - 2 ? INFINITY : nhi;
-        inv_k = 1.0 / (nhi - x0);
+        inv_k = R(1) / (nhi - x0);
         L0 = L1;
         L1 = (1 - dy) * n0 + dy * n1;
         if (TAN) {
@@ -599,8 +601,8 @@ struct ClhPkNodeT {
     return L0 + dx * (L1 - L0);
   }
   // TAN: d ln P / d log10 k and d ln P / d dy at the last lookup (weight dx)
-  __device__ __forceinline__ double dlnp_dk() const { return (L1 - L0) * inv_k; }
-  __device__ __forceinline__ double dlnp_dy(double dx) const { return (1 - dx) * M0 + dx * M1; }
+  __device__ __forceinline__ R dlnp_dk() const { return (L1 - L0) * inv_k; }
+  __device__ __forceinline__ R dlnp_dy(R dx) const { return (1 - dx) * M0 + dx * M1; }
 };
 
 typedef ClhPkNodeT<true> ClhPkNode;
@@ -648,12 +650,14 @@ struct ClhBracket {
 // hf1 n1 (W_RSD = c0 t0 - c1 hf1 n1) with respect to the node's chi at the
 // row, on the row's table intervals, and the P(k,z) walk keeps its z
 // differences: the pieces of the integrand's derivative along the node's a.
-template <bool ADJ, bool AHEAD = true, bool TAN = false>
+// R: the arithmetic type of the integrand (float in the FP32 mode's gg
+// kernels: tables and records stay FP64 in memory, the walks compare in R).
+template <bool ADJ, bool AHEAD = true, bool TAN = false, typename R = double>
 struct ClhGgNode {
   static_assert(ADJ || !TAN, "the tangent uses the adjoint's n(z) derivative");
   // row-independent values of the node
-  double fK, wgb, wmb, t0, t0d, c3fac, l10fk, chi_amin;
-  double inv_st, zs0, zlo, zhi;  // lens n(z) argument zs = z1 inv_st + zs0; first and last knot
+  R fK, wgb, wmb, t0, t0d, c3fac, l10fk, chi_cut;  // chi_cut = chi(a_min) in Mpc
+  R inv_st, zs0, zlo, zhi;  // lens n(z) argument zs = z1 inv_st + zs0; first and last knot
   // The a_1 side walks over the chi(z) table (a_chi), its z column (the
   // dchi/dz stencil), the G(z) table and the lens n(z) knots.  Per table the
   // current interval j, the bounds of the walk test (-inf / +inf at the table
@@ -666,12 +670,12 @@ struct ClhGgNode {
   const double* __restrict__ crec;
   const double* __restrict__ grec;
   const double* __restrict__ nrec;  // this lens bin's n(z) interval records
-  double clo, chi_, czc, zA, zS;         // chi interval: z1 = zA + c1v zS; czc = z at its lower end
-  double D0, D1;                         // z interval (the chi interval's): dchi/dz = D0 + z1 D1
-  double glo, ghi, G0, slG;              // G interval (own bounds only off the chi grid): G = G0 + z1 slG
-  double nlo, nhi_, cf0, cf1, cf2, cf3;  // lens spline interval: knots, coefficients / stretch
-  double dhn;                            // TAN: d(hf1 n1) / d chi of the last row
-  ClhPkNodeT<AHEAD, TAN> pk;
+  R clo, chi_, czc, zA, zS;         // chi interval: z1 = zA + c1v zS; czc = z at its lower end
+  R D0, D1;                         // z interval (the chi interval's): dchi/dz = D0 + z1 D1
+  R glo, ghi, G0, slG;              // G interval (own bounds only off the chi grid): G = G0 + z1 slG
+  R nlo, nhi_, cf0, cf1, cf2, cf3;  // lens spline interval: knots, coefficients / stretch
+  R dhn;                            // TAN: d(hf1 n1) / d chi of the last row
+  ClhPkNodeT<AHEAD, TAN, R> pk;
 
   __device__ __forceinline__ void rec_z(const double* __restrict__ r) {  // the z interval's coefficients
     D0 = r[4];
@@ -687,26 +691,26 @@ struct ClhGgNode {
   // bisection (on z1) can differ only when z1 rounds onto a knot, where the
   // stencil is continuous.  The upper bound is capped at the chi(a_min) cut
   // (cosmo2D.c:2512-2514), beyond which the integrand is zero.
-  __device__ __forceinline__ void set_c(double c1v) {
-    if (c1v > chi_amin * CLH_COVERH0) {  // beyond the cut until chi_1 falls below it
+  __device__ __forceinline__ void set_c(R c1v) {
+    if (c1v > chi_cut) {  // beyond the cut until chi_1 falls below it
       alive = false;
-      clo = chi_amin * CLH_COVERH0;
+      clo = chi_cut;
       chi_ = INFINITY;
       return;
     }
     alive = true;
     const double* __restrict__ r = crec + 8 * (size_t)jc;
     clo = jc == 0 ? -INFINITY : r[0];
-    chi_ = fmin(jc == chi_n - 2 ? INFINITY : r[8], chi_amin * CLH_COVERH0);
+    chi_ = fmin(jc == chi_n - 2 ? (R)INFINITY : (R)r[8], chi_cut);
     czc = r[1];
     zA = r[2];
     zS = r[3];
     rec_z(r);
   }
-  __device__ __forceinline__ void move_c(const ClhGgTabs& T, double c1v) {
+  __device__ __forceinline__ void move_c(const ClhGgTabs& T, R c1v) {
     if (alive && c1v < clo && jc > 0) {  // one interval down: the upper bound is the old lower one
       const double* __restrict__ r = crec + 8 * (size_t)(jc - 1);
-      const double lo1 = r[0];
+      const R lo1 = r[0];
       if (jc == 1 || c1v >= lo1) {
         chi_ = clo;
         jc--;
@@ -736,7 +740,7 @@ struct ClhGgNode {
   }
   // the lens n(z) interval of zs: a spline piece inside the knots, zero
   // (and an interval reaching to -inf / +inf) outside them
-  __device__ __forceinline__ void set_n(const ClhGgTabs& T, double zs) {
+  __device__ __forceinline__ void set_n(const ClhGgTabs& T, R zs) {
     if (zs <= zlo || zs >= zhi) {
       nlo = zs <= zlo ? -INFINITY : zhi;
       nhi_ = zs <= zlo ? zlo : INFINITY;
@@ -750,10 +754,10 @@ struct ClhGgNode {
     nhi_ = r[5];
     rec_n(r);
   }
-  __device__ __forceinline__ void move_n(const ClhGgTabs& T, double zs) {
+  __device__ __forceinline__ void move_n(const ClhGgTabs& T, R zs) {
     if (zs < nlo && nlo > zlo) {  // one knot down inside the knots (zs falls as chi_1 does)
       const double* __restrict__ r = nrec + 5 * (size_t)(jn - 1);
-      const double lo1 = r[0];
+      const R lo1 = r[0];
       if (zs >= lo1 && zs > zlo) {
         nhi_ = nlo;
         nlo = lo1;
@@ -792,11 +796,11 @@ struct ClhGgNode {
     t0 = lut[7 * np + p];
     t0d = (ADJ && !TAN) ? lut[8 * np + p] : 0.0;
     inv_st = 1.0 / d.par->lens_st[zl];
-    const double zm = d.l_zmean[zl];
+    const R zm = d.l_zmean[zl];
     zs0 = (-d.par->lens_dz[zl] - zm) * inv_st + zm;
     zlo = d.l_zc[0];
     zhi = d.l_zc[l_rows - 1];
-    chi_amin = d.chi_amin;
+    chi_cut = d.chi_amin * CLH_COVERH0;
     alive = false;   // empty intervals: the first row walks from the a_0 indices
     clo = INFINITY;
     chi_ = -INFINITY;
@@ -810,50 +814,53 @@ struct ClhGgNode {
 
   // false: chi_1 beyond chi(a_min), integrand zero (cosmo2D.c:2512-2514).
   // t: the bracket; pf = P fac; dx: log10 k weight of the P cell (pk.i, pk.j).
-  // T: the draw's tables (staged in LDS when they fit).  F32: P(k,z) as expf
-  // of the FP64 bilinear value.  Each interval is the one the reference's
+  // T: the draw's tables (staged in LDS when they fit).  F32 (R = double):
+  // P(k,z) as expf of the FP64 bilinear value; R = float: all in FP32.  Each interval is the one the reference's
   // bisection finds; inside it the interpolations are the interval's linear
   // (n(z): cubic) pieces.
   template <bool F32 = false>
-  __device__ __forceinline__ bool eval(const DrawDesc& d, const ClhGgTabs& T, const ClhGgRow& R,
-                                       double& t, double& pf, double& dwr, double& dx) {
+  __device__ __forceinline__ bool eval(const DrawDesc& d, const ClhGgTabs& T, const ClhGgRow& row,
+                                       R& t, R& pf, R& dwr, R& dx) {
+    constexpr R C = CLH_COVERH0;
     // a_chi(chi_1) (cosmo3D.c:109-133), chi_1 = chi (ell+1)/ell
-    const double c1v = fK * R.rr1 * CLH_COVERH0;
+    const R c1v = fK * (R)row.rr1 * C;
     if (!(c1v >= clo && c1v < chi_)) move_c(T, c1v);
     // dchi/dz at z1 (cosmo3D.c:64-81 stencils) on the same interval
-    const double z1 = zA + c1v * zS;
+    const R z1 = zA + c1v * zS;
     // G(z1) and f_growth (cosmo3D.c:200-228): hoverh0 f_growth =
     // (c/H0) / (dchi/dz) (1 - G'(1+z)/G), where G - G'(1+z) = G0 - slG on the interval
     if (!same && !(z1 >= glo && z1 < ghi)) {
       jg = clh_walk(T.gz, G_n, z1, jg);
       set_g();
     }
-    const double DG = D0 + z1 * D1, GG = G0 + z1 * slG;
-    double hf1, inv = 0.0;
+    const R DG = D0 + z1 * D1, GG = G0 + z1 * slG;
+    R hf1, inv = 0;
     if (TAN) {
-      inv = 1.0 / (DG * GG);
-      hf1 = CLH_COVERH0 * (G0 - slG) * inv;
+      inv = R(1) / (DG * GG);
+      hf1 = C * (G0 - slG) * inv;
     } else {
-      hf1 = CLH_COVERH0 * (G0 - slG) / (DG * GG);
+      hf1 = C * (G0 - slG) / (DG * GG);
     }
     // nz_lens(z1) (redshift_spline.c:736-762) and its shift derivative
-    const double zs = z1 * inv_st + zs0;
+    const R zs = z1 * inv_st + zs0;
     if (!(zs >= nlo && zs < nhi_)) move_n(T, zs);
-    const double x = nlo > -INFINITY ? zs - nlo : 0.0;  // below the knots: zero coefficients
-    const double n1 = cf0 + x * (cf1 + x * (cf2 + x * cf3));
-    double dn1 = 0.0;
-    if (ADJ) dn1 = -(cf1 + x * (2.0 * cf2 + 3.0 * x * cf3)) * inv_st;
+    const R x = nlo > -INFINITY ? zs - nlo : R(0);  // below the knots: zero coefficients
+    const R n1 = cf0 + x * (cf1 + x * (cf2 + x * cf3));
+    R dn1 = 0;
+    if (ADJ) dn1 = -(cf1 + x * (R(2) * cf2 + R(3) * x * cf3)) * inv_st;
     if (TAN) {
       // d(hf1 n1)/d z1 = hf1 (n1' - n1 (D1 / DG + slG / GG)), n1' = -dn1;
       // z1 = zA + chi rr1 (c/H0) zS on the chi interval
-      dhn = hf1 * (-dn1 - n1 * (D1 * GG + slG * DG) * inv) * (zS * R.rr1 * CLH_COVERH0);
+      dhn = hf1 * (-dn1 - n1 * (D1 * GG + slG * DG) * inv) * (zS * (R)row.rr1 * C);
     }
-    const double wr = R.c0 * t0 - R.c1 * hf1 * n1;
+    const R c0 = row.c0, c1 = row.c1;
+    const R wr = c0 * t0 - c1 * hf1 * n1;
     // (TAN: without the a_0 half c0 t0d, which the caller adds per node)
-    if (ADJ) dwr = (TAN ? 0.0 : R.c0 * t0d) - R.c1 * hf1 * dn1;
-    t = wgb + R.pref * wmb + wr;
-    const double out = pk.lnp(T.lk, R.l10 - l10fk, dx);
-    if (F32) pf = (double)(expf((float)out) * (float)c3fac);
+    if (ADJ) dwr = (TAN ? R(0) : c0 * t0d) - c1 * hf1 * dn1;
+    t = wgb + (R)row.pref * wmb + wr;
+    const R out = pk.lnp(T.lk, (R)row.l10 - l10fk, dx);
+    if constexpr (sizeof(R) == 4) pf = expf(out) * c3fac;
+    else if (F32) pf = (R)(expf((float)out) * (float)c3fac);
     else pf = exp(out) * c3fac;
     return alive;
   }

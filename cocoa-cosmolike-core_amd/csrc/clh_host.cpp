@@ -49,7 +49,7 @@ hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
 hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles, int ns, int ck,
                                    int lim, hipStream_t s);
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
-                                   int lim, hipStream_t s);
+                                   int lim, int f32, hipStream_t s);
 hipError_t clh_launch_nodes_tan(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_lim_grad(const DrawDesc* D, const DrawDesc* P, const DrawDesc* M, const int2* work, int n,
@@ -3239,7 +3239,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   const bool use_ggk = bt.state_valid && bt.ckpt_fresh && bt.ggk_per && !lim_tan;
   HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga,
                                      (c->gg_lds[1] && !use_ggk) ? bt.gg_stage : 0, use_ggk ? 1 : 0,
-                                     lim_tan ? 1 : 0, s));
+                                     lim_tan ? 1 : 0, c->precision != CLH_PREC_FP64 ? 1 : 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[9], s));
   c->vjp_timed = c->timing;
   if (g_lens_dz && nl > 0 && c->vjp_lim == 1) {

@@ -13,6 +13,8 @@
 //             shear calibration (cosmo2D.c:487-534; generic_interface.hpp:800-862)
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "clh_device.hpp"
 #include "clh_internal.hpp"
 
@@ -634,7 +636,9 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_limber_sx_tatt(const DrawDesc* __
 // stage != 0).  Sub-tile s covers rows 16 s .. 16 s + 15 of the tile:
 // group h takes rows 16 s + 8 h .. + 7, so each thread still walks its rows in
 // increasing ell.  Row sums over the 96 nodes: one wave per row.
-// F32: P(k,z) as expf of the FP64 bilinear value.
+// F32: the integrand in FP32 arithmetic (walks, interpolations, W_RSD, the
+// P(k,z) bilinear value and its exp; tables stay FP64 in memory); the t^2 P
+// fac of each node goes into the FP64 row sums.
 // KEEP: also store the bracket t, P fac and dW_RSD / d dz of every (row, node)
 // in the draw's checkpoint (d.ggk), which the VJP then reads instead of
 // evaluating the integrand again.
@@ -654,7 +658,8 @@ __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ 
   const ClhGgTabs T = clh_gg_tabs(d, zl, dyn, stage, s_lk);
   __syncthreads();
   const int p = threadIdx.x % np, half = threadIdx.x / np;
-  ClhGgNode<KEEP, !KEEP && !OL> nd;  // the look-ahead only where it keeps 3 waves per SIMD
+  typedef typename std::conditional<F32, float, double>::type real;
+  ClhGgNode<KEEP, !KEEP && !OL, false, real> nd;  // the look-ahead only where it keeps 3 waves per SIMD
   nd.init(d, T, clh_gg_nodes(d, zl), p, zl);
   // OL: the one-loop bias term W_gal^2 (...) of draws with some b2 != 0
   double ol_wg2 = 0.0, ol_g4f = 0.0, ol_lnfk = 0.0, ol_fk = 1.0;
@@ -674,9 +679,9 @@ __global__ __launch_bounds__(192) void k_limber_gg(const DrawDesc* __restrict__ 
       const int rl = half * per + q;
       double v = 0.0;
       if (base + rl < nr) {
-        double t = 0.0, pf = 0.0, dwr = 0.0, dx;
-        if (nd.template eval<F32>(d, T, R[base + rl], t, pf, dwr, dx)) {
-          v = t * t * pf;
+        real t = 0, pf = 0, dwr = 0, dx;
+        if (nd.template eval<false>(d, T, R[base + rl], t, pf, dwr, dx)) {
+          v = (double)t * t * pf;
           if (OL && d.oneloop) {
             const double ell = R[base + rl].ell;
             v += ol_wg2 * clh_oneloop_gg(d, zl, log(ell) - ol_lnfk, ell / ol_fk, ol_g4f, pf);
