@@ -257,7 +257,7 @@ class Cosmolike:
 
     def load(self, prob, legendre_mode=LEGENDRE_FOLDED):
         """Configure the layout and current draw from a synth.Problem."""
-        self.init_ntable(prob.n_ell, 225, prob.lmax, 0)
+        self.init_ntable(prob.n_ell, 225, prob.lmax, getattr(prob, "hdi", 0))
         self.init_binning_real_space(prob.ntheta, prob.theta_min_arcmin, prob.theta_max_arcmin)
         self.init_IA(getattr(prob, "ia_model", 0), prob.ia_mode)
         self.set_legendre_mode(legendre_mode)

@@ -341,6 +341,7 @@ class Problem:
     theta_max_arcmin: float
     n_ell: int = 300
     lmax: int = 50000
+    hdi: int = 0                 # Ntable.high_def_integration
     src_dz: np.ndarray | None = None
     lens_dz: np.ndarray | None = None
     lens_stretch: np.ndarray | None = None

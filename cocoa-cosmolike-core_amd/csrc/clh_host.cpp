@@ -54,6 +54,7 @@ hipError_t clh_launch_nodes_tan(const DrawDesc* D, const int2* work, int n, hipS
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_lim_grad(const DrawDesc* D, const DrawDesc* P, const DrawDesc* M, const int2* work, int n,
                                hipStream_t s);
+hipError_t clh_launch_limber_hdi(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_int_for(const DrawDesc* D, int probe, int i, int j, int f1, int f2, double l, const double* a,
                               int n, double* out, hipStream_t s);
 hipError_t clh_launch_lnp_gather(const DrawDesc* D, const int2* work, int n, int ic, int nz, int nl,
@@ -370,6 +371,11 @@ struct Batch {
   bool tatt = false;   // some draw uses TATT intrinsic alignments
   bool oneloop = false;  // some draw has one-loop galaxy bias
   bool cmb_any = false;  // some draw computes the 6x2pt CMB lensing blocks
+  // draws at high_def_integration != 0: their Limber rows from k_limber_hdi
+  // (draw, column, row0, rows) instead of the node kernels
+  bool hdi_any = false;
+  DBuf<int4> d_whd;
+  int n_whd = 0;
   DBuf<int4> d_wcmb;
   DBuf<int> d_wcmbp;
   int n_wcmb = 0, n_wcmbp = 0;
@@ -392,7 +398,7 @@ struct Batch {
     d_desc_nl.release(); d_nl_ells.release(); d_wn_nl.release(); d_wnl_prep.release(); d_wnl_cl.release();
     d_wgg_nl.release(); d_wnl_fwd.release(); d_wnl_gl.release(); d_wnl_chk.release(); d_wnl_idft.release();
     d_wcmb.release(); d_wcmbp.release();
-    d_desc_lim.release(); d_lim.release(); d_wsx_gs.release(); d_wlim.release();
+    d_desc_lim.release(); d_lim.release(); d_wsx_gs.release(); d_wlim.release(); d_whd.release();
   }
 };
 
@@ -558,10 +564,23 @@ static int build_sample(clh_ctx* c, Sample& S, const double* table, int nrows, i
   return CLH_OK;
 }
 
+// GL orders of the Limber integrals by high_def_integration: ss and gs
+// (cosmo2D.c:1577-1581, 2145-2149), gg (:2635-2639)
+static int gl_order_ssgs(int hdi) {
+  hdi = std::abs(hdi);
+  return hdi == 0 ? 64 : hdi == 1 ? 96 : hdi == 2 ? 128 : hdi == 3 ? 256 : 512;
+}
+static int gl_order_gg(int hdi) {
+  hdi = std::abs(hdi);
+  return hdi == 0 ? 96 : hdi == 1 ? 128 : hdi == 2 ? 256 : hdi == 3 ? 512 : 1024;
+}
+
 // zmean (redshift_spline.c:794-844) as set_lens_sample evaluates it
-// (generic_interface.cpp:1874-1876): zero shift, unit stretch, 256-point GL
+// (generic_interface.cpp:1874-1876): zero shift, unit stretch, 256 / 512 /
+// 1024-point GL by high_def_integration (:809-811)
 static void lens_zmean(Layout& L) {
-  const int n = 256;
+  const int hd = std::abs(L.hdi);
+  const int n = hd == 0 ? 256 : hd == 1 ? 512 : 1024;
   std::vector<double> x(n), w(n);
   gauss_legendre(n, x.data(), w.data());
   const int rows = L.lens.rows;
@@ -874,9 +893,18 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   const bool full = c->legendre_mode == CLH_LEGENDRE_FULL;
   if (!L.kern_valid || (full && L.kfull.empty()))
     return fail(c, CLH_ERR_STATE, "layout kernels not built for the Legendre mode");
-  double glbuf[2 * CLH_NP_SS + 2 * CLH_NP_GG];
+  // GL rules: 64 and 96 points, then at high_def_integration != 0 the ss / gs
+  // and gg rules of that level (k_limber_hdi)
+  const int hss = gl_order_ssgs(L.hdi), hgg = gl_order_gg(L.hdi);
+  std::vector<double> glv(2 * CLH_NP_SS + 2 * CLH_NP_GG + (L.hdi ? 2 * hss + 2 * hgg : 0));
+  double* glbuf = glv.data();
   gauss_legendre(CLH_NP_SS, glbuf, glbuf + CLH_NP_SS);
   gauss_legendre(CLH_NP_GG, glbuf + 2 * CLH_NP_SS, glbuf + 2 * CLH_NP_SS + CLH_NP_GG);
+  if (L.hdi) {
+    double* h = glbuf + 2 * CLH_NP_SS + 2 * CLH_NP_GG;
+    gauss_legendre(hss, h, h + hss);
+    gauss_legendre(hgg, h + 2 * hss, h + 2 * hss + hgg);
+  }
   size_t off = 0;
   auto take = [&](size_t n) { size_t o = off; off += (n + 31) & ~size_t(31); return o; };
   D.off_s_zc = take(L.src.zc.size());
@@ -887,7 +915,7 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
   D.off_ell_gx = take(L.ell_gx.size());
   D.off_kfold = take(L.kfold.size());
   D.off_kfoldT = take(L.kfold.size());
-  D.off_gl = take(sizeof glbuf / sizeof(double));
+  D.off_gl = take(glv.size());
   D.off_theta = take(L.theta.size());
   D.off_kfull = take(full ? L.kfull.size() : 0);
   // FULL mode: index and weight of the ln-ell fill at every integer ell
@@ -957,7 +985,7 @@ static int upload_layout(clh_ctx* c, const Layout& L, DevLayout& D) {
         for (size_t t = 0; t < nt; t++) kfoldT[(k * nt + t) * nr + r] = L.kfold[(k * nr + r) * nt + t];
   }
   HIPCHK(c, put(D.off_kfoldT, kfoldT.data(), kfoldT.size()));
-  HIPCHK(c, put(D.off_gl, glbuf, sizeof glbuf / sizeof(double)));
+  HIPCHK(c, put(D.off_gl, glbuf, glv.size()));
   HIPCHK(c, put(D.off_theta, L.theta.data(), L.theta.size()));
   HIPCHK(c, put(D.off_kwlow, L.kwlow.data(), L.kwlow.size()));
   HIPCHK(c, put(D.off_nltw, tw.data(), tw.size()));
@@ -1116,7 +1144,7 @@ static bool ord_ok(const Layout& L, const int ord[3]) {
 // adjoint ss/gs and adjoint gg Limber stages (1..CLH_TILE_ROWS).
 static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   std::vector<int2> wg, wn, wl2, wlim;
-  std::vector<int4> wl, wl_gs, wsx, wsx_t, wsx_gs, wgg, wga, wmx, wlf, wa;
+  std::vector<int4> wl, wl_gs, wsx, wsx_t, wsx_gs, wgg, wga, wmx, wlf, wa, whd;
   int lds_ss = 0, lds_gs = 0;
   std::vector<int2> tiles, atiles_ss, atiles_gs;
   for (int i = 0; i < bt.B; i++) {
@@ -1125,6 +1153,16 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
     const int nrow = bt.nrow_custom ? bt.nrow_custom : L.nrow();
     const int n_ss = L.n_ss(), n_gs = L.n_gs();
     for (int b = 0; b < ns + nl; b++) wg.push_back(make_int2(i, b));
+    const int ntask = 2 * n_ss + n_gs + nl;
+    if (L.hdi != 0) {
+      // every column (and the TATT B-mode ss columns) in tiles of CLH_HDI_ROWS
+      // rows; no node, adjoint or lens-limit items (the VJP refuses the batch)
+      const int ncol = n_ss + n_gs + nl + (L.ia_model == CLH_IA_TATT ? n_ss : 0);
+      for (int col = 0; col < ncol; col++)
+        for (int r0 = 0; r0 < nrow; r0 += CLH_HDI_ROWS)
+          whd.push_back(make_int4(i, col, r0, std::min(CLH_HDI_ROWS, nrow - r0)));
+    }
+    if (L.hdi == 0) {
     for (int sg = 0; sg <= 2 * nl; sg++) wn.push_back(make_int2(i, sg));
     // forward Limber: ss, gs and gg tiles (draw, kind + 4 zl, row0, rows)
     // (TATT draws' tiles in their own list: they run in k_limber_sx_tatt)
@@ -1158,7 +1196,7 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
     row_tiles(nrow, rows[3], tiles);
     for (int zl = 0; zl < nl; zl++)
       for (const int2& t : tiles) wga.push_back(make_int4(i, zl, t.x, t.y));
-    const int ntask = 2 * n_ss + n_gs + nl;
+    }  // L.hdi == 0
     for (int t = 0; t < ntask; t += 8) wl2.push_back(make_int2(i, t));
     // FP64 Legendre on MFMA: 16 columns of one probe kind per item, forward
     // (xi+, xi-, gamma_t, w kernels) and adjoint (ss with both xi kernels, gs, gg)
@@ -1200,6 +1238,7 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   HIPCHK(c, up4(bt.d_wmx, wmx, bt.n_wmx));
   HIPCHK(c, up4(bt.d_wlf, wlf, bt.n_wlf));
   HIPCHK(c, up4(bt.d_wa4, wa, bt.n_wa));
+  HIPCHK(c, up4(bt.d_whd, whd, bt.n_whd));
   bt.adj_lds_ss = lds_ss;
   bt.adj_lds_gs = lds_gs;
   // non-Limber w: the views' gg nodes and rows (ell = 1..149), then the FFTLog
@@ -1273,6 +1312,10 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   for (int i = 0; i < B; i++) {
     if ((rc = check_layout(c, *lays[i]))) return rc;
     if ((rc = check_draw(c, *lays[i], *draws[i], i))) return rc;
+    const Layout& L = *lays[i];
+    if (L.hdi != 0 && (L.sixpt || (L.nonlimber && L.lens.nbin > 0 && !ells)))
+      return fail(c, CLH_ERR_ARG, "high_def_integration=%d with %s is not implemented", L.hdi,
+                  L.sixpt ? "the 6x2pt probes" : "the non-Limber w(theta)");
   }
   auto al = [](size_t n) { return (n + 31) & ~size_t(31); };
   // per-draw sizes and offsets (layouts may differ between draws)
@@ -1385,6 +1428,10 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
     q.kfull = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + DL.off_kfull : nullptr;
     q.fill = (c->legendre_mode == CLH_LEGENDRE_FULL) ? lay + DL.off_fill : nullptr;
     q.gl = lay + DL.off_gl;
+    q.gx = 25 * (1 + std::abs(L.hdi));
+    q.np_ss = L.hdi ? gl_order_ssgs(L.hdi) : CLH_NP_SS;
+    q.np_gg = L.hdi ? gl_order_gg(L.hdi) : CLH_NP_GG;
+    q.gl_hd = L.hdi ? q.gl + 2 * CLH_NP_SS + 2 * CLH_NP_GG : nullptr;
     q.kmx32 = DL.k32.p; q.kmxbf = DL.kbf.p; q.kmxsc = DL.ksc.p;
     q.kmx_ngrp = DL.ngrp;
     q.theta = lay + DL.off_theta;
@@ -1522,7 +1569,9 @@ static int build_batch(clh_ctx* c, Batch& bt, const std::vector<const Draw*>& dr
   bt.tatt = false;
   bt.oneloop = false;
   bt.cmb_any = false;
+  bt.hdi_any = false;
   for (int i = 0; i < B; i++) {
+    bt.hdi_any = bt.hdi_any || lays[i]->hdi != 0;
     bt.cmb_any = bt.cmb_any || bt.desc[i].cl6 != nullptr;
     bt.tatt = bt.tatt || lays[i]->ia_model == CLH_IA_TATT;
     bt.oneloop = bt.oneloop || bt.desc[i].oneloop;
@@ -1578,6 +1627,8 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
                                  bt.oneloop ? 1 : 0));
   if (bt.n_wsx > bt.n_wsx_std)
     HIPCHK(c, clh_launch_limber_sx_tatt(bt.d_desc.p, bt.d_wsx.p + bt.n_wsx_std, bt.n_wsx - bt.n_wsx_std, s));
+  // draws at high_def_integration != 0: n-point GL rows of every column (FP64)
+  if (bt.n_whd) HIPCHK(c, clh_launch_limber_hdi(bt.d_desc.p, bt.d_whd.p, bt.n_whd, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], s));
   if (bt.cmb_any)  // 6x2pt CMB lensing columns and blocks: FP64 in every precision mode
     HIPCHK(c, clh_launch_cmb(bt.d_desc.p, bt.d_wcmb.p, bt.n_wcmb, bt.d_wcmbp.p, legendre ? bt.n_wcmbp : 0, s));
@@ -1721,8 +1772,8 @@ int clh_init_ntable(clh_ctx* c, int n_ell, int n_a, int lmax, int hdi) {
   if (!c) return CLH_ERR_ARG;
   if (n_ell < 2 || n_a < 2 || n_a > 1025 || lmax < CLH_LMIN_TAB + 2)
     return fail(c, CLH_ERR_ARG, "bad Ntable (N_ell=%d, N_a=%d, LMAX=%d)", n_ell, n_a, lmax);
-  if (hdi != 0) return fail(c, CLH_ERR_ARG, "high_def_integration=%d not implemented (0 only)", hdi);
   c->L.n_ell = n_ell; c->L.n_a = n_a; c->L.lmax = lmax; c->L.hdi = hdi;
+  if (c->L.lens.set) lens_zmean(c->L);  // its GL order follows high_def_integration
   c->L.kern_valid = false;
   layout_changed(c);
   c->version++;
@@ -2771,7 +2822,7 @@ long long clh_batch_datavector_offset(clh_ctx* c, int i) {
 // descriptor changed on the host and must be uploaded.
 static int prepare_checkpoint(clh_ctx* c, Batch& cb, bool& changed) {
   cb.keep_ckpt = c->vjp_ckpt && cb.uniform && !cb.nrow_custom && !cb.tatt && !cb.nl_any && !cb.oneloop &&
-                 !cb.cmb_any;
+                 !cb.cmb_any && !cb.hdi_any;
   if (cb.keep_ckpt) {
     const size_t nl = cb.lays[0]->lens.nbin, nr = cb.lays[0]->nrow();
     cb.ggk_per = (nl * nr * 3 * CLH_NP_GG + 31) & ~size_t(31);
@@ -2964,7 +3015,7 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   for (int i = 0; i < bt.B; i++) bt.desc[i].dv = bt.d_dv.p + bt.dv_off[i];
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
   const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt && !bt.nl_any && !bt.oneloop &&
-                   !bt.lays[0]->sixpt;  // the VJP refuses these
+                   !bt.lays[0]->sixpt && !bt.hdi_any;  // the VJP refuses these
   // unit cotangent, ln P and lens-shift gradients of the adjoint stages (the
   // ln P runs and the lens-limit term are part of what is timed)
   DBuf<double> dvbar, glnp, glens;
@@ -3114,6 +3165,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (bt.nl_any) return fail(c, CLH_ERR_ARG, "VJP of the non-Limber w(theta) is not implemented");
   if (bt.oneloop) return fail(c, CLH_ERR_ARG, "VJP of the one-loop galaxy bias terms is not implemented");
   if (bt.lays[0]->sixpt) return fail(c, CLH_ERR_ARG, "VJP of the 6x2pt data vector is not implemented");
+  if (bt.hdi_any) return fail(c, CLH_ERR_ARG, "VJP at high_def_integration != 0 is not implemented");
   const Layout& L = *bt.lays[0];
   if (!ord_ok(L, ord))
     return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");

@@ -144,6 +144,85 @@ __global__ __launch_bounds__(256) void k_int_for(const DrawDesc* __restrict__ D,
   if (t < n) out[t] = clh_int_for(*D, q, a[t]);
 }
 
+// ----------------------------------------------------------------------------
+// k_limber_hdi: the Limber C_ell rows of a draw at high_def_integration != 0
+// (Ntable.high_def_integration): n-point GL of the integrands above over the
+// reference's ranges, n = 96 / 128 / 256 / 512 for ss and gs and 128 / 256 /
+// 512 / 1024 for gg (cosmo2D.c:1577-1581, 2145-2149, 2635-2639); ss on
+// [1/(1 + z_max,src), 1/(1 + max(z_min,src, 1e-6))] (:1675-1676), gs and gg on
+// [amin_lens, amax_lens] (:2159-2160, 2647-2648).  Work item (draw, column,
+// row0, rows): column < n_ss + n_gs + nl is an ss (EE), gs or gg column of
+// d.cl, column - (n_ss + n_gs + nl) a B-mode ss column of d.cl_bb (TATT).
+// Rows go two at a time to 128 threads each (n <= 128) or one at a time to
+// all 256; a thread sums its nodes in a fixed order and the row's sum is a
+// fixed-order wave and LDS reduction, so repeated launches agree bit for bit.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_limber_hdi(const DrawDesc* __restrict__ D, const int4* __restrict__ work) {
+  __shared__ double part[4];
+  const int4 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const int ncol = d.n_ss + d.n_gs + d.nl;
+  const bool bb = w.y >= ncol;
+  const int col = bb ? w.y - ncol : w.y, row0 = w.z, nr = w.w;
+  ClhIntArgs q{0, 0, 0, 0, 0, 0.0};
+  double amin, amax;
+  int np;
+  const double* gx;
+  if (col < d.n_ss) {
+    int i, j;
+    clh_ss_pair(d.ns, col, i, j);
+    q = ClhIntArgs{0, i, j, bb ? 0 : 1, 0, 0.0};
+    amin = 1. / (d.s_zmax_all + 1.);
+    amax = 1. / (1. + fmax(d.s_zmin_all, 1e-6));
+    np = d.np_ss;
+    gx = d.gl_hd;
+  } else if (col < d.n_ss + d.n_gs) {
+    const int zl = (col - d.n_ss) / d.ns, zs = (col - d.n_ss) % d.ns;
+    q = ClhIntArgs{1, zl, zs, d.oneloop, 0, 0.0};
+    amin = clh_amin_lens(d, zl);
+    amax = clh_amax_lens(d, zl);
+    np = d.np_ss;
+    gx = d.gl_hd;
+  } else {
+    const int zl = col - d.n_ss - d.n_gs;
+    q = ClhIntArgs{2, zl, zl, 0, d.oneloop, 0.0};
+    amin = clh_amin_lens(d, zl);
+    amax = clh_amax_lens(d, zl);
+    np = d.np_gg;
+    gx = d.gl_hd + 2 * d.np_ss;
+  }
+  const double A = 0.5 * (amax - amin), B = 0.5 * (amax + amin);
+  const double* ells = col < d.n_ss ? d.ell_ss : d.ell_gx;
+  double* out = bb ? d.cl_bb + (size_t)col * d.nrow : d.cl + (size_t)col * d.nrow;
+  const int per = np <= 128 ? 128 : 256, rpb = 256 / per;  // threads per row, rows in flight
+  const int lr = threadIdx.x % per, sub = threadIdx.x / per;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, wpr = per / 64;
+  for (int r0 = 0; r0 < nr; r0 += rpb) {
+    const int r = r0 + sub;
+    double v = 0.0;
+    if (r < nr) {
+      ClhIntArgs qr = q;
+      qr.l = ells[row0 + r];
+      for (int n = lr; n < np; n += per) v += A * gx[np + n] * clh_int_for(d, qr, B + A * gx[n]);
+    }
+    v = clh_wave_sum(v);
+    if (lane == 0) part[wv] = v;
+    __syncthreads();
+    if (threadIdx.x < rpb && r0 + (int)threadIdx.x < nr) {
+      double t = 0.0;
+      for (int k = 0; k < wpr; k++) t += part[threadIdx.x * wpr + k];
+      out[row0 + r0 + threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t clh_launch_limber_hdi(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_limber_hdi, dim3(n), dim3(256), 0, s, D, work);
+  return hipGetLastError();
+}
+
 hipError_t clh_launch_int_for(const DrawDesc* D, int probe, int i, int j, int f1, int f2, double l, const double* a,
                               int n, double* out, hipStream_t s) {
   if (n <= 0) return hipSuccess;

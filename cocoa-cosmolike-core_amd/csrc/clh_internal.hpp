@@ -9,6 +9,7 @@
 #define CLH_A_MIN (1.0 / (1.0 + 40.0))  // limits.a_min (structs.c:41)
 #define CLH_NP_SS 64         // GL order for ss/gs at high_def_integration = 0
 #define CLH_NP_GG 96         // GL order for gg at high_def_integration = 0
+#define CLH_HDI_ROWS 16      // rows per work item of k_limber_hdi (high_def_integration != 0)
 
 // Work-item kinds of the Limber kernel
 #define CLH_W_SS 0
@@ -160,6 +161,12 @@ struct DrawDesc {
   int node_ss, node_gs, node_gg;  // offsets into nodes
   int node_gs_stride, node_gg_stride;
   double chi_amin;  // chi(a_min) for the gg RSD cut
+  // Ntable.high_def_integration: gx = 25 (1 + |hdi|) fine-grid points per
+  // lens-efficiency interval (redshift_spline.c:865, 939, 1027); for hdi != 0
+  // the Limber rows come from k_limber_hdi with np_ss (ss, gs) and np_gg
+  // (gg) GL nodes, gl_hd = x, w of the ss / gs rule then x, w of the gg rule
+  int gx, np_ss, np_gg;
+  const double* gl_hd;
   double G0;        // G(z=0) by the growth table's own interpolation
   const double* gl; // GL abscissae/weights on [-1,1]: x64 w64 x96 w96
   int apply_calib;  // 1: shear calibration, point mass and mask applied to the data vector

@@ -112,7 +112,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   }
   const bool src = w.y < d.ns;
   const int b = src ? w.y : w.y - d.ns;
-  const int x = 25;  // 25 (1 + |high_def_integration|), high_def_integration = 0
+  const int x = d.gx;  // 25 (1 + |high_def_integration|)
   const int nint = d.n_a - 1;
   const int Na = x * nint + 1;
   const double amax = 0.999999;

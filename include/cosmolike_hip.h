@@ -69,7 +69,13 @@ int clh_set_precision(clh_ctx* ctx, int mode);
 /* ---------------------------------------------------------------- setters */
 /* init_accuracy_boost / init_ntable_lmax (generic_interface.cpp:284-342):
  * N_ell log-spaced Limber ell, N_a lens-efficiency table size, LMAX,
- * high_def_integration (only 0 is implemented: 64/96-point GL). */
+ * high_def_integration (Ntab.high_def_integration, structs.h:52-126): the
+ * Limber GL orders 64 / 96 / 128 / 256 / 512 (ss, gs) and 96 / 128 / 256 /
+ * 512 / 1024 (gg) for |hdi| = 0 .. 4+ (cosmo2D.c:1577-1581, 2635-2639), the
+ * lens-efficiency fine grid 25 (1 + |hdi|) (redshift_spline.c:865) and the
+ * lens zmean GL order 256 / 512 / 1024 (:809-811).  At hdi != 0 the Limber
+ * rows are direct n-point sums of the integrands (the node kernels serve
+ * hdi = 0); the 6x2pt probes, the non-Limber w and the VJP refuse it. */
 int clh_init_ntable(clh_ctx* ctx, int n_ell, int n_a, int lmax, int high_def);
 /* init_binning_real_space (generic_interface.cpp:479-505), arcmin */
 int clh_init_binning_real_space(clh_ctx* ctx, int ntheta, double theta_min_arcmin,

@@ -211,7 +211,7 @@ class Oracle:
 
     def load(self, prob):
         """Configure from a ``cosmolike_amd.synth.Problem``."""
-        self.set_ntable(prob.n_ell, 225, prob.lmax, 0)
+        self.set_ntable(prob.n_ell, 225, prob.lmax, getattr(prob, "hdi", 0))
         self.set_binning(prob.ntheta, prob.theta_min_arcmin, prob.theta_max_arcmin)
         self.set_cosmology(prob.omega_m, prob.chi_z, prob.chi, prob.G_z, prob.G, prob.log10k,
                            prob.pk_z, prob.lnP)

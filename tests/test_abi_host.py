@@ -53,8 +53,10 @@ def test_setters_validate(ctx):
         ctx.init_binning_real_space(40, 2.5, 250.0)  # more than 32 theta bins
     assert e.value.code == -1
     with pytest.raises(CosmolikeError) as e:
-        ctx.init_ntable(300, 225, 50000, 1)           # high_def_integration != 0
+        ctx.init_ntable(300, 1, 50000, 1)             # N_a < 2
     assert e.value.code == -1
+    ctx.init_ntable(300, 225, 50000, 3)               # high_def_integration != 0 is accepted
+    ctx.init_ntable(p.n_ell, 225, p.lmax, 0)
     bad = p.chi.copy()
     bad[3] = np.nan
     with pytest.raises(CosmolikeError) as e:
