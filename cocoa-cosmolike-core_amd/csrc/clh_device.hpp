@@ -266,15 +266,26 @@ __device__ __forceinline__ double clh_amax_lens(const DrawDesc& d, int ni) {
 // The same values as clh_chi_all (chi only), clh_nz_src / clh_nz_lens and
 // their shift derivatives, for a thread that evaluates them along monotone z:
 // the table index is walked from the previous call (start with j = -1).
-__device__ __forceinline__ double clh_chi_w(const DrawDesc& d, double a, int& j) {
-  const double z = 1.0 / a - 1.0;
-  const double* __restrict__ zz = d.chi_z;
-  const double* __restrict__ cc = d.chi_v;
-  j = (j < 0) ? clh_bs(zz, d.chi_n, z) : clh_walk(zz, d.chi_n, z, j);
-  const int jj = j > d.chi_n - 3 ? d.chi_n - 3 : j;
-  const double dy = (z - zz[jj]) / (zz[jj + 1] - zz[jj]);
-  return (cc[jj] + dy * (cc[jj + 1] - cc[jj])) / CLH_COVERH0;
-}
+// chi at z along monotone z (chi_all's linear chi, cosmo3D.c:40-92): the
+// interval's knots and reciprocal width are kept until the walk moves
+struct ClhChiWalk {
+  int j = -1, jj = 0;
+  double z0 = 0.0, c0 = 0.0, dc = 0.0, inv = 0.0;
+  __device__ __forceinline__ double chi(const DrawDesc& d, double z) {
+    const double* __restrict__ zz = d.chi_z;
+    const int jn = (j < 0) ? clh_bs(zz, d.chi_n, z) : clh_walk(zz, d.chi_n, z, j);
+    if (jn != j) {
+      j = jn;
+      jj = j > d.chi_n - 3 ? d.chi_n - 3 : j;
+      const double* __restrict__ cc = d.chi_v;
+      z0 = zz[jj];
+      inv = 1.0 / (zz[jj + 1] - z0);
+      c0 = cc[jj];
+      dc = cc[jj + 1] - c0;
+    }
+    return (c0 + (z - z0) * inv * dc) * (1.0 / CLH_COVERH0);
+  }
+};
 
 // n and d n / d dz of source bin b at z (D: also the derivative)
 template <bool DER>

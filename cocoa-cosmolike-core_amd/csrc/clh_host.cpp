@@ -35,7 +35,7 @@ hipError_t clh_launch_cmb(const DrawDesc* D, const int4* work, int n, const int*
 hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles, int keep,
                                 hipStream_t s, int ol = 0);
 hipError_t clh_launch_legendre(const DrawDesc* D, const int4* work, int n, hipStream_t s);
-hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, int tables, hipStream_t s);
 hipError_t clh_launch_fill(double* p, size_t n, double v, hipStream_t s);
 hipError_t clh_launch_chi2_resid(int B, int n, const double* dv, const double* data, const double* mask,
                                  double* R, hipStream_t s);
@@ -1612,7 +1612,7 @@ static int run_batch(clh_ctx* c, Batch& bt, bool legendre) {
   hipStream_t s = c->stream;
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], s));
   // with the checkpoint, the tables come with their photo-z shift tangents
-  if (bt.keep_ckpt) HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
+  if (bt.keep_ckpt) HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, 1, s));
   else HIPCHK(c, clh_launch_gtab(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], s));
   HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
@@ -3261,7 +3261,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // them; the gg rows carry no calibration and are not recomputed)
   const bool ck_all = bt.state_valid && bt.ckpt_fresh;
   if (!ck_all)  // else the checkpointing forward left tables and shift tangents
-    HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, s));
+    HIPCHK(c, clh_launch_gtab_tangent(bt.d_desc.p, bt.d_wg.p, bt.n_wg, bt.state_valid ? 0 : 1, s));
   if (!bt.state_valid) {  // else the last forward / VJP left them for these parameters
     bt.ckpt_fresh = false;  // a checkpoint predates these parameters
     bt.rows64 = false;      // the gg rows predate them too

@@ -90,16 +90,20 @@ __device__ void clh_cell_records(const DrawDesc& d) {
     }
 }
 
-template <bool TANGENT>
+// MODE 0: the tables; 1: the tables and their shift tangents; 2: the
+// tangents only (the tables and interval records of these parameters are
+// already in place).
+template <int MODE>
 __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__ D,
                                                     const int2* __restrict__ work) {
-  __shared__ double sP[1024], sQ[1024];
+  constexpr bool TANGENT = MODE >= 1, TABLES = MODE <= 1;
+  __shared__ double sP[TABLES ? 1024 : 1], sQ[TABLES ? 1024 : 1];
   __shared__ double sPd[TANGENT ? 1024 : 1], sQd[TANGENT ? 1024 : 1];
   __shared__ double red[8];
   const int2 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
-  if (w.y == 0) clh_cell_records(d);  // once per draw: the block of its first bin
-  if (w.y >= d.ns) {  // the lens bin's n(z) interval records (ClhGgNode)
+  if (TABLES && w.y == 0) clh_cell_records(d);  // once per draw: the block of its first bin
+  if (TABLES && w.y >= d.ns) {  // the lens bin's n(z) interval records (ClhGgNode)
     const int zl = w.y - d.ns, nr = d.l_rows;
     const double inv_st = 1.0 / d.par->lens_st[zl];
     const double* cf = d.l_coef + (size_t)zl * 4 * (nr - 1);
@@ -129,10 +133,12 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
     for (int i = 1 + threadIdx.x; i < Na0; i += CLH_BLOCK) {
       const double apl = amin_shear + (i - 1) * da0;
       const double aph = amin_shear + i * da0;
-      const double fl = clh_nz_lens(d, 1.0 / apl - 1.0, b) / (apl * apl);
-      const double fh = clh_nz_lens(d, 1.0 / aph - 1.0, b) / (aph * aph);
-      p += 0.5 * da0 * (fl + fh);
-      q += 0.5 * da0 * (fl / clh_chi(d, apl) + fh / clh_chi(d, aph));
+      if (TABLES) {
+        const double fl = clh_nz_lens(d, 1.0 / apl - 1.0, b) / (apl * apl);
+        const double fh = clh_nz_lens(d, 1.0 / aph - 1.0, b) / (aph * aph);
+        p += 0.5 * da0 * (fl + fh);
+        q += 0.5 * da0 * (fl / clh_chi(d, apl) + fh / clh_chi(d, aph));
+      }
       if (TANGENT) {
         const double gl = clh_dnz_lens(d, 1.0 / apl - 1.0, b) / (apl * apl);
         const double gh = clh_dnz_lens(d, 1.0 / aph - 1.0, b) / (aph * aph);
@@ -140,8 +146,10 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
         qd += 0.5 * da0 * (gl / clh_chi(d, apl) + gh / clh_chi(d, aph));
       }
     }
-    P0 = clh_block_sum(p, red);
-    Q0 = clh_block_sum(q, red);
+    if (TABLES) {
+      P0 = clh_block_sum(p, red);
+      Q0 = clh_block_sum(q, red);
+    }
     if (TANGENT) {
       P0d = clh_block_sum(pd, red);
       Q0d = clh_block_sum(qd, red);
@@ -152,24 +160,28 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   for (int t = threadIdx.x; t < nint; t += CLH_BLOCK) {
     double p = 0.0, q = 0.0, pd = 0.0, qd = 0.0;
     double fprev = 0.0, gprev = 0.0, fdprev = 0.0, gdprev = 0.0;
-    int jn = -1, jc = -1;
+    int jn = -1;
+    ClhChiWalk cw;
     for (int s2 = 0; s2 <= x; s2++) {
       const int i = t * x + s2;
       const double a = amin + i * da;
+      const double ia = 1.0 / a, z = ia - 1.0;
       double n, dn;
-      if (src) clh_nz_src_w<TANGENT>(d, 1. / a - 1., b, jn, n, dn);
-      else clh_nz_lens_w<TANGENT>(d, 1.0 / a - 1.0, b, jn, n, dn);
-      const double chi = clh_chi_w(d, amin + i * da, jc);
-      const double f = n / (a * a);
-      const double g = f / chi;
-      if (s2 > 0) {
-        p += 0.5 * da * (fprev + f);
-        q += 0.5 * da * (gprev + g);
+      if (src) clh_nz_src_w<TANGENT>(d, z, b, jn, n, dn);
+      else clh_nz_lens_w<TANGENT>(d, z, b, jn, n, dn);
+      const double chi = cw.chi(d, z);
+      if (TABLES) {
+        const double f = n * ia * ia;
+        const double g = f / chi;
+        if (s2 > 0) {
+          p += 0.5 * da * (fprev + f);
+          q += 0.5 * da * (gprev + g);
+        }
+        fprev = f;
+        gprev = g;
       }
-      fprev = f;
-      gprev = g;
       if (TANGENT) {
-        const double fd = dn / (a * a);
+        const double fd = dn * ia * ia;
         const double gd = fd / chi;
         if (s2 > 0) {
           pd += 0.5 * da * (fdprev + fd);
@@ -179,8 +191,10 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
         gdprev = gd;
       }
     }
-    sP[t] = p;
-    sQ[t] = q;
+    if (TABLES) {
+      sP[t] = p;
+      sQ[t] = q;
+    }
     if (TANGENT) {
       sPd[t] = pd;
       sQd[t] = qd;
@@ -194,16 +208,22 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   const int t1 = min(t0 + cpt, nint);
   double cp = 0.0, cq = 0.0, cpd = 0.0, cqd = 0.0;
   for (int t = t0; t < t1; t++) {
-    cp += sP[t];
-    cq += sQ[t];
+    if (TABLES) {
+      cp += sP[t];
+      cq += sQ[t];
+    }
     if (TANGENT) {
       cpd += sPd[t];
       cqd += sQd[t];
     }
   }
-  const double ip = clh_block_scan(cp, red);
-  const double iq = clh_block_scan(cq, red);
-  double P = P0 + ip - cp, Q = Q0 + iq - cq;
+  double P = 0.0, Q = 0.0;
+  if (TABLES) {
+    const double ip = clh_block_scan(cp, red);
+    const double iq = clh_block_scan(cq, red);
+    P = P0 + ip - cp;
+    Q = Q0 + iq - cq;
+  }
   double Pd = 0.0, Qd = 0.0;
   if (TANGENT) {
     const double ipd = clh_block_scan(cpd, red);
@@ -214,11 +234,13 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   double* tab = d.gtab + (size_t)w.y * d.n_a;
   double* dtab = TANGENT ? d.dgtab + (size_t)w.y * d.n_a : nullptr;
   for (int t = t0; t < t1; t++) {
-    P += sP[t];
-    Q += sQ[t];
     const int i = (t + 1) * x;
     const double chi = clh_chi(d, amin + i * da);
-    tab[t + 1] = P - chi * Q;
+    if (TABLES) {
+      P += sP[t];
+      Q += sQ[t];
+      tab[t + 1] = P - chi * Q;
+    }
     if (TANGENT) {
       Pd += sPd[t];
       Qd += sQd[t];
@@ -227,7 +249,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   }
   if (threadIdx.x == 0) {
     const double chi0 = clh_chi(d, amin);
-    tab[0] = src ? 0.0 : P0 - chi0 * Q0;
+    if (TABLES) tab[0] = src ? 0.0 : P0 - chi0 * Q0;
     if (TANGENT) dtab[0] = src ? 0.0 : P0d - chi0 * Q0d;
   }
 }
@@ -870,7 +892,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_legendre_full(const DrawDesc* __r
 // ----------------------------------------------------------------------------
 hipError_t clh_launch_gtab(const DrawDesc* D, const int2* work, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gtab<false>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  hipLaunchKernelGGL(k_gtab<0>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }
 hipError_t clh_launch_nodes(const DrawDesc* D, const int2* work, int n, hipStream_t s) {
@@ -890,9 +912,10 @@ hipError_t clh_launch_legendre_full(const DrawDesc* D, const int2* work, int n, 
   hipLaunchKernelGGL(k_legendre_full, dim3(n), dim3(CLH_BLOCK), lds, s, D, work);
   return hipGetLastError();
 }
-hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, hipStream_t s) {
+hipError_t clh_launch_gtab_tangent(const DrawDesc* D, const int2* work, int n, int tables, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gtab<true>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  if (tables) hipLaunchKernelGGL(k_gtab<1>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
+  else hipLaunchKernelGGL(k_gtab<2>, dim3(n), dim3(CLH_BLOCK), 0, s, D, work);
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_sx_tatt(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
