@@ -208,203 +208,16 @@ __global__ __launch_bounds__(64) void k_m_grad(const DrawDesc* __restrict__ D) {
 }
 
 // ----------------------------------------------------------------------------
-// k_limber_sx_adj: adjoint of k_limber_sx.  One wave per work item (draw,
-// kind + 4 zl, row0, nrows); thread = GL node, walking the tile's rows in
-// increasing ell with the forward's P(k,z) recomputation.  Per (row, node):
-//  ss: u_b = sum_j f_bj cbar_(b,j) A_j (f = 2 on the diagonal) gives both
-//      abar_b += pref0 S u_b and the ln P cotangent pref0 S sum_b A_b u_b / 2
-//      (the cotangents sit in LDS as the full symmetric [ns][ns][rows] array
-//      with f folded in, so the b, j loops address it linearly);
-//  gs: bbar_zs += pref2 cbar_zs g S, W_gal / W_mag / b1 cotangents and
-//      pref2 S g sum_zs cbar_zs B_zs for ln P, with g = W_gal b1 + W_mag pref1 bmag.
-// LDS: ss [ns][ns][nrows], gs [ns][nrows] (dynamic).  With the forward's
-// checkpoint (d.sxk) the P(k,z) lookup is replaced by its stored S = P fac
-// and only the k cell is walked again.
-// LIM (gs tiles, no checkpoint): also the lens-limit part of the lens-shift
-// gradient, sum over rows and bins of cbar dC/d dz through the motion of the
-// nodes (k_nodes_tan): with g the row's ln P cotangent, the node's share is
-// a_t (sum_b B_b' acc_b + W_gal' b1 twg + W_mag' bmag twm + dlog10k' sum g
-// dlnP/dlog10k + dy' sum g dlnP/dy) + (ln(weight fac))_t sum g, where acc,
-// twg, twm are the node cotangents above (the derivative of C_gs along the
-// node's a, contracted with cbar, is built from them).
-// ----------------------------------------------------------------------------
-template <int NS, bool CK, bool LIM = false>
-__global__ __launch_bounds__(64) void k_limber_sx_adj(const DrawDesc* __restrict__ D,
-                                                      const int4* __restrict__ work) {
-  // NS: the source-bin count when it is at most 10 (loops fully unrolled to
-  // it), else CLH_MAXB with the count checked at run time.  CK: read the
-  // forward's checkpoint instead of looking P(k,z) up again (its own
-  // instantiation, so the lighter path gets its own register budget)
-  constexpr bool EXACT = NS < CLH_MAXB;
-  extern __shared__ double Cb[];  // ss: [ns][ns][T], gs: [ns][T]
-  __shared__ double L10[CLH_TILE_ROWS];
-  __shared__ double PR[3][CLH_TILE_ROWS];  // per-row prefactors (ss: pref0; gs: pref1, pref2)
-  __shared__ double s_lk[CLH_LDS_LK];
-  const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
-  const DrawDesc& d = D[w.x];
-  const int kind = w.y & 3, zl = w.y >> 2, row0 = w.z, T = w.w;
-  const int p = threadIdx.x;
-  constexpr int np = CLH_NP_SS;
-  const int nrow = d.nrow, ns = EXACT ? NS : d.ns;
-  static_assert(!(LIM && CK), "the lens-limit term walks P(k,z) again");
-  const bool ss = !LIM && kind == CLH_W_SS;  // LIM: launched on gs tiles only
-  const double* o = clh_sx_nodes(d, kind, zl);
-  const double* lut = o + clh_sx_lut_col(d, kind) * np;
-  const double* ells = ss ? d.ell_ss : d.ell_gx;
-  if (ss) {
-    for (int i = p; i < ns * ns * T; i += 64) {
-      const int b = i / (ns * T), jj = (i / T) % ns, r = row0 + i % T;
-      Cb[i] = (b == jj ? 2.0 : 1.0) * d.cbar[(size_t)clh_ss_index(ns, b, jj) * nrow + r];
-    }
-  } else {
-    const int col0 = d.n_ss + zl * ns;
-    for (int i = p; i < ns * T; i += 64) Cb[i] = d.cbar[(size_t)(col0 + i / T) * nrow + row0 + i % T];
-  }
-  for (int t = p; t < T; t += 64) {
-    const double l = ells[row0 + t], ell = l + 0.5;
-    L10[t] = log10(ell);
-    if (ss) {
-      PR[0][t] = l * (l - 1.) * (l + 1.) * (l + 2.) / (ell * ell * ell * ell);
-    } else {
-      const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
-      PR[1][t] = l * (l + 1.) / (ell * ell);
-      PR[2][t] = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
-    }
-  }
-  const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
-  __syncthreads();
-  const double fac = o[2 * np + p];
-  const int j = (int)lut[p];
-  const double dy = lut[np + p], l10fk = lut[2 * np + p];
-  const int woff = ss ? 3 : 5;
-  double Wn[NS], acc[NS];
-#pragma unroll
-  for (int b = 0; b < NS; b++) {
-    Wn[b] = (EXACT || b < ns) ? o[(woff + b) * np + p] : 0.0;
-    acc[b] = 0.0;
-  }
-  const double WG = ss ? 0.0 : o[3 * np + p], WM = ss ? 0.0 : o[4 * np + p];
-  const double b1 = ss ? 0.0 : d.par->b1[zl], bmag = ss ? 0.0 : d.par->bmag[zl];
-  double twg = 0.0, twm = 0.0;
-  ClhLnPRun sc;
-  sc.out.R = clh_run_row(d, kind, zl, p);
-  // per row: S = P fac of the node (from the forward's checkpoint, or the
-  // P(k,z) lookup again), its k cell (i, dx), then the cotangent terms
-  auto row = [&](int rr, double S, int i, double dx) {
-    double g;
-    if (ss) {
-      const double c = PR[0][rr] * S;
-      double sb = 0.0;
-      const double* cr = Cb + rr;
-#pragma unroll
-      for (int b = 0; b < NS; b++) {
-        if (EXACT || b < ns) {
-          double u = 0.0;
-#pragma unroll
-          for (int jj = 0; jj < NS; jj++)
-            if (EXACT || jj < ns) u += cr[(b * ns + jj) * T] * Wn[jj];
-          acc[b] += c * u;
-          sb += Wn[b] * u;
-        }
-      }
-      g = 0.5 * sb * c;
-    } else {
-      const double pref1 = PR[1][rr], pref2 = PR[2][rr];
-      const double gp = WG * b1 + WM * pref1 * bmag;
-      const double cg = pref2 * gp * S;
-      double sv = 0.0;
-#pragma unroll
-      for (int b = 0; b < NS; b++) {
-        if (EXACT || b < ns) {
-          const double cb = Cb[b * T + rr];
-          sv += cb * Wn[b];
-          acc[b] += cg * cb;
-        }
-      }
-      const double t = pref2 * S * sv;
-      twg += t;
-      twm += t * pref1;
-      g = t * gp;
-    }
-    if (sc.out.R) sc.add(i, dx, g);
-    return g;
-  };
-  double slk = 0.0, slz = 0.0, sgl = 0.0;  // LIM: sum g dlnP/dlog10k, sum g dlnP/dy, sum g
-  if constexpr (CK) {
-    // the forward's checkpoint of S (d.sxk); eight rows loaded ahead of their use
-    const double* sk = d.sxk + ((size_t)(ss ? 0 : 1 + zl) * nrow + row0) * np + p;
-    ClhKCell kc;
-    kc.nk = d.nk;
-    for (int r8 = 0; r8 < T; r8 += 8) {
-      double sv8[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) sv8[u] = sk[(size_t)(r8 + u < T ? r8 + u : T - 1) * np];
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        if (r8 + u >= T) break;
-        const double dx = kc.dx(lk, L10[r8 + u] - l10fk);
-        row(r8 + u, sv8[u], kc.i, dx);
-      }
-    }
-  } else {
-    ClhPkNodeT<false, LIM> pkn;  // no P-row look-ahead: it would cost a wave per SIMD
-    pkn.init(d, j, dy);
-    for (int rr = 0; rr < T; rr++) {
-      double dx;
-      const double out = pkn.lnp(lk, L10[rr] - l10fk, dx);
-      const double g = row(rr, exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac, pkn.i, dx);
-      if (LIM) {
-        slk += g * pkn.dlnp_dk();
-        slz += g * pkn.dlnp_dy(dx);
-        sgl += g;
-      }
-    }
-  }
-  if (sc.out.R) sc.done();
-  if (ss) {
-#pragma unroll
-    for (int b = 0; b < NS; b++)
-      if ((EXACT || b < ns) && acc[b] != 0.0) atomicAdd(d.abar + (size_t)b * np + p, acc[b]);
-  } else {
-#pragma unroll
-    for (int b = 0; b < NS; b++)
-      if ((EXACT || b < ns) && acc[b] != 0.0) atomicAdd(d.bbar + ((size_t)zl * ns + b) * np + p, acc[b]);
-    atomicAdd(d.wbar + ((size_t)zl * 2 + 0) * np + p, twg * b1);
-    atomicAdd(d.wbar + ((size_t)zl * 2 + 1) * np + p, twm * bmag);
-    const double b1p = clh_wave_sum(twg * WG);
-    if (p == 0 && b1p != 0.0) atomicAdd(d.g_b1 + zl, b1p);
-    if (LIM) {
-      const double* tn = d.ntan + (size_t)zl * (6 + ns) * np;
-      double sb = 0.0;
-#pragma unroll
-      for (int b = 0; b < NS; b++)
-        if (EXACT || b < ns) sb += tn[(6 + b) * np + p] * acc[b];
-      double lim = tn[p] * (sb + tn[4 * np + p] * b1 * twg + tn[5 * np + p] * bmag * twm + tn[2 * np + p] * slk +
-                            tn[3 * np + p] * slz) +
-                   tn[np + p] * sgl;
-      lim = clh_wave_sum(lim);
-      if (p == 0 && lim != 0.0) atomicAdd(d.g_lens + zl, lim);
-    }
-  }
-}
-
-// ----------------------------------------------------------------------------
-// k_limber_gg_adj: adjoint of k_limber_gg; work item (draw, zl, row0,
-// nrows), two groups of 96 node threads, group h walking rows
-// h ceil(nrows/2) .. of the tile (ClhGgNode from the k_nodes search results).
-// Per (row, node) with integrand t^2 P fac, t = W_gal b1 + W_mag pref bmag +
-// W_RSD: ln P cotangent cbar t^2 P fac; tb = 2 cbar t P fac feeds b1 (W_gal),
-// the W_gal / W_mag node cotangents and the lens shift through dW_RSD/d dz.
-//
-// The ln P cotangent: the 96 nodes of a lens bin sit in a narrow z range,
-// about ten nodes per z column of P(k,z), so they share cells.  The block
-// folds its nodes' runs over (k cell, z column) in an LDS band: per column
-// the cells its nodes reach over the tile's rows (from the bisection of the
-// first and last rows' log10 k), packed column after column.  After the rows
-// the band is added into g_lnP with atomics (the gather has stored the ss /
-// gs part there before this launch).  A column past CLH_BAND_COLS or beyond
-// CLH_BAND_CAP doubles, and any cell outside its column's range, is added
-// into g_lnP directly.
+// The ln P cotangent band of a gs or gg block.  The 64 nodes of a gs tile
+// and the 96 of a gg tile sit in one lens bin's z range, about ten nodes per
+// z column of P(k,z), so they share cells: the block folds its nodes' runs
+// over (k cell, z column) in an LDS band, per column the cells its nodes
+// reach over the tile's rows (from the bisection of the first and last rows'
+// log10 k), packed column after column.  After the rows the band is added
+// into g_lnP with atomics (the gather has stored the ss part there before
+// these launches).  A column past CLH_BAND_COLS or beyond CLH_BAND_CAP
+// doubles, and any cell outside its column's range, is added into g_lnP
+// directly.
 // ----------------------------------------------------------------------------
 #define CLH_BAND_CAP 1024
 #define CLH_BAND_COLS 64
@@ -414,11 +227,12 @@ struct ClhBandLds {
   int jlo, nval, used;
 };
 
-// Band set-up of a gg block (every thread calls it; ends synced).  Node p of
-// the block's first group reports its columns jp, jp + 1 and cells
-// i(row 0) .. i(row nr - 1) + 1; out: the node's ClhBand.
+// Band set-up of a gs or gg block (every thread calls it; ends synced).
+// Each reporting node (a gg node reports from the block's first group)
+// gives its columns jp, jp + 1 and cells i(first row) .. i(last row) + 1;
+// out: the node's ClhBand.
 __device__ __forceinline__ void clh_band_setup(ClhBandLds& bd, const DrawDesc& d, const double* __restrict__ lk,
-                                               const ClhGgRow* R, int nr, int jp, double dyp, double l10fk,
+                                               double l10_first, double l10_last, int jp, double dyp, double l10fk,
                                                bool reporter, ClhBand& out) {
   const int t = threadIdx.x;
   for (int u = t; u < CLH_BAND_CAP; u += blockDim.x) bd.A[u] = 0.0;
@@ -432,8 +246,8 @@ __device__ __forceinline__ void clh_band_setup(ClhBandLds& bd, const DrawDesc& d
   __syncthreads();
   const int jlo = bd.jlo, c0 = jp - jlo;
   if (reporter) {
-    const int i0 = clh_bs(lk, d.nk, R[0].l10 - l10fk);
-    const int i1 = clh_bs(lk, d.nk, R[nr - 1].l10 - l10fk) + 1;
+    const int i0 = clh_bs(lk, d.nk, l10_first - l10fk);
+    const int i1 = clh_bs(lk, d.nk, l10_last - l10fk) + 1;
     for (int c = c0; c <= c0 + 1 && c < CLH_BAND_COLS; c++) {
       atomicMin(&bd.lo[c], i0);
       atomicMax(&bd.hi[c], i1);
@@ -499,6 +313,214 @@ __device__ __forceinline__ void clh_band_flush(const ClhBandLds& bd, const DrawD
 // c0 - chi' sum tb c1 d(hf1 n1)/dchi + dlog10k' sum g dlnP/dlog10k + dy' sum g
 // dlnP/dy) + (ln(weight fac))_t sum g, with tb = 2 cbar t P fac and g =
 // cbar t^2 P fac the row's cotangents of t and ln P.
+// ----------------------------------------------------------------------------
+// k_limber_sx_adj: adjoint of k_limber_sx.  One wave per work item (draw,
+// kind + 4 zl, row0, nrows); thread = GL node, walking the tile's rows in
+// increasing ell with the forward's P(k,z) recomputation.  Per (row, node):
+//  ss: u_b = sum_j f_bj cbar_(b,j) A_j (f = 2 on the diagonal) gives both
+//      abar_b += pref0 S u_b and the ln P cotangent pref0 S sum_b A_b u_b / 2
+//      (the cotangents sit in LDS as the full symmetric [ns][ns][rows] array
+//      with f folded in, so the b, j loops address it linearly);
+//  gs: bbar_zs += pref2 cbar_zs g S, W_gal / W_mag / b1 cotangents and
+//      pref2 S g sum_zs cbar_zs B_zs for ln P, with g = W_gal b1 + W_mag pref1 bmag.
+// LDS: ss [ns][ns][nrows], gs [ns][nrows] (dynamic).  With the forward's
+// checkpoint (d.sxk) the P(k,z) lookup is replaced by its stored S = P fac
+// and only the k cell is walked again.
+// LIM (gs tiles, no checkpoint): also the lens-limit part of the lens-shift
+// gradient, sum over rows and bins of cbar dC/d dz through the motion of the
+// nodes (k_nodes_tan): with g the row's ln P cotangent, the node's share is
+// a_t (sum_b B_b' acc_b + W_gal' b1 twg + W_mag' bmag twm + dlog10k' sum g
+// dlnP/dlog10k + dy' sum g dlnP/dy) + (ln(weight fac))_t sum g, where acc,
+// twg, twm are the node cotangents above (the derivative of C_gs along the
+// node's a, contracted with cbar, is built from them).
+// ----------------------------------------------------------------------------
+// GS: the launch holds gs tiles only; their ln P runs fold into the block's
+// band (added into g_lnP after the gather), the ss tiles' runs go to the run
+// slab.
+template <int NS, bool CK, bool LIM = false, bool GS = false>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NS < CLH_MAXB ? 3 : 1))) void k_limber_sx_adj(const DrawDesc* __restrict__ D,
+                                                      const int4* __restrict__ work) {
+  // NS: the source-bin count when it is at most 10 (loops fully unrolled to
+  // it), else CLH_MAXB with the count checked at run time.  CK: read the
+  // forward's checkpoint instead of looking P(k,z) up again (its own
+  // instantiation, so the lighter path gets its own register budget)
+  constexpr bool EXACT = NS < CLH_MAXB;
+  extern __shared__ double Cb[];  // ss: [ns][ns][T], gs: [ns][T]
+  __shared__ double L10[CLH_TILE_ROWS];
+  __shared__ double PR[3][CLH_TILE_ROWS];  // per-row prefactors (ss: pref0; gs: pref1, pref2)
+  __shared__ double s_lk[CLH_LDS_LK];
+  __shared__ typename std::conditional<GS, ClhBandLds, int>::type bd;
+  const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
+  const DrawDesc& d = D[w.x];
+  const int kind = w.y & 3, zl = w.y >> 2, row0 = w.z, T = w.w;
+  const int p = threadIdx.x;
+  constexpr int np = CLH_NP_SS;
+  const int nrow = d.nrow, ns = EXACT ? NS : d.ns;
+  static_assert(!(LIM && CK), "the lens-limit term walks P(k,z) again");
+  static_assert(GS || !LIM, "the lens-limit term is on gs tiles");
+  const bool ss = !GS && kind == CLH_W_SS;
+  const double* o = clh_sx_nodes(d, kind, zl);
+  const double* lut = o + clh_sx_lut_col(d, kind) * np;
+  const double* ells = ss ? d.ell_ss : d.ell_gx;
+  if (ss) {
+    for (int i = p; i < ns * ns * T; i += 64) {
+      const int b = i / (ns * T), jj = (i / T) % ns, r = row0 + i % T;
+      Cb[i] = (b == jj ? 2.0 : 1.0) * d.cbar[(size_t)clh_ss_index(ns, b, jj) * nrow + r];
+    }
+  } else {
+    const int col0 = d.n_ss + zl * ns;
+    for (int i = p; i < ns * T; i += 64) Cb[i] = d.cbar[(size_t)(col0 + i / T) * nrow + row0 + i % T];
+  }
+  for (int t = p; t < T; t += 64) {
+    const double l = ells[row0 + t], ell = l + 0.5;
+    L10[t] = log10(ell);
+    if (ss) {
+      PR[0][t] = l * (l - 1.) * (l + 1.) * (l + 2.) / (ell * ell * ell * ell);
+    } else {
+      const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+      PR[1][t] = l * (l + 1.) / (ell * ell);
+      PR[2][t] = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+    }
+  }
+  const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
+  __syncthreads();
+  const double fac = o[2 * np + p];
+  const int j = (int)lut[p];
+  const double dy = lut[np + p], l10fk = lut[2 * np + p];
+  const int woff = ss ? 3 : 5;
+  double Wn[NS], acc[NS];
+#pragma unroll
+  for (int b = 0; b < NS; b++) {
+    Wn[b] = (EXACT || b < ns) ? o[(woff + b) * np + p] : 0.0;
+    acc[b] = 0.0;
+  }
+  const double WG = ss ? 0.0 : o[3 * np + p], WM = ss ? 0.0 : o[4 * np + p];
+  const double b1 = ss ? 0.0 : d.par->b1[zl], bmag = ss ? 0.0 : d.par->bmag[zl];
+  double twg = 0.0, twm = 0.0;
+  // the ln P cotangent is wanted when the draw has a run slab (uniform per block)
+  const bool lnp = d.run != nullptr;
+  ClhLnPRunT<typename std::conditional<GS, ClhBand, ClhSlab>::type> sc;
+  if constexpr (GS) {
+    sc.out.A = bd.A;
+    if (lnp) clh_band_setup(bd, d, lk, L10[0], L10[T - 1], j, dy, l10fk, true, sc.out);
+  } else {
+    sc.out.R = clh_run_row(d, kind, zl, p);
+  }
+  // per row: S = P fac of the node (from the forward's checkpoint, or the
+  // P(k,z) lookup again), its k cell (i, dx), then the cotangent terms
+  auto row = [&](int rr, double S, int i, double dx) {
+    double g;
+    if (ss) {
+      const double c = PR[0][rr] * S;
+      double sb = 0.0;
+      const double* cr = Cb + rr;
+#pragma unroll
+      for (int b = 0; b < NS; b++) {
+        if (EXACT || b < ns) {
+          double u = 0.0;
+#pragma unroll
+          for (int jj = 0; jj < NS; jj++)
+            if (EXACT || jj < ns) u += cr[(b * ns + jj) * T] * Wn[jj];
+          acc[b] += c * u;
+          sb += Wn[b] * u;
+        }
+      }
+      g = 0.5 * sb * c;
+    } else {
+      const double pref1 = PR[1][rr], pref2 = PR[2][rr];
+      const double gp = WG * b1 + WM * pref1 * bmag;
+      const double cg = pref2 * gp * S;
+      double sv = 0.0;
+#pragma unroll
+      for (int b = 0; b < NS; b++) {
+        if (EXACT || b < ns) {
+          const double cb = Cb[b * T + rr];
+          sv += cb * Wn[b];
+          acc[b] += cg * cb;
+        }
+      }
+      const double t = pref2 * S * sv;
+      twg += t;
+      twm += t * pref1;
+      g = t * gp;
+    }
+    if (lnp) sc.add(i, dx, g);
+    return g;
+  };
+  double slk = 0.0, slz = 0.0, sgl = 0.0;  // LIM: sum g dlnP/dlog10k, sum g dlnP/dy, sum g
+  if constexpr (CK) {
+    // the forward's checkpoint of S (d.sxk); eight rows loaded ahead of their use
+    const double* sk = d.sxk + ((size_t)(ss ? 0 : 1 + zl) * nrow + row0) * np + p;
+    ClhKCell kc;
+    kc.nk = d.nk;
+    for (int r8 = 0; r8 < T; r8 += 8) {
+      double sv8[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) sv8[u] = sk[(size_t)(r8 + u < T ? r8 + u : T - 1) * np];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        if (r8 + u >= T) break;
+        const double dx = kc.dx(lk, L10[r8 + u] - l10fk);
+        row(r8 + u, sv8[u], kc.i, dx);
+      }
+    }
+  } else {
+    ClhPkNodeT<false, LIM> pkn;  // no P-row look-ahead: it would cost a wave per SIMD
+    pkn.init(d, j, dy);
+    for (int rr = 0; rr < T; rr++) {
+      double dx;
+      const double out = pkn.lnp(lk, L10[rr] - l10fk, dx);
+      const double g = row(rr, exp(out) * (1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0)) * fac, pkn.i, dx);
+      if (LIM) {
+        slk += g * pkn.dlnp_dk();
+        slz += g * pkn.dlnp_dy(dx);
+        sgl += g;
+      }
+    }
+  }
+  if (lnp) sc.done();
+  if (ss) {
+#pragma unroll
+    for (int b = 0; b < NS; b++)
+      if ((EXACT || b < ns) && acc[b] != 0.0) atomicAdd(d.abar + (size_t)b * np + p, acc[b]);
+  } else {
+#pragma unroll
+    for (int b = 0; b < NS; b++)
+      if ((EXACT || b < ns) && acc[b] != 0.0) atomicAdd(d.bbar + ((size_t)zl * ns + b) * np + p, acc[b]);
+    atomicAdd(d.wbar + ((size_t)zl * 2 + 0) * np + p, twg * b1);
+    atomicAdd(d.wbar + ((size_t)zl * 2 + 1) * np + p, twm * bmag);
+    const double b1p = clh_wave_sum(twg * WG);
+    if (p == 0 && b1p != 0.0) atomicAdd(d.g_b1 + zl, b1p);
+    if (LIM) {
+      const double* tn = d.ntan + (size_t)zl * (6 + ns) * np;
+      double sb = 0.0;
+#pragma unroll
+      for (int b = 0; b < NS; b++)
+        if (EXACT || b < ns) sb += tn[(6 + b) * np + p] * acc[b];
+      double lim = tn[p] * (sb + tn[4 * np + p] * b1 * twg + tn[5 * np + p] * bmag * twm + tn[2 * np + p] * slk +
+                            tn[3 * np + p] * slz) +
+                   tn[np + p] * sgl;
+      lim = clh_wave_sum(lim);
+      if (p == 0 && lim != 0.0) atomicAdd(d.g_lens + zl, lim);
+    }
+  }
+  if constexpr (GS) {
+    if (lnp) {
+      __syncthreads();
+      clh_band_flush(bd, d);
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// k_limber_gg_adj: adjoint of k_limber_gg; work item (draw, zl, row0,
+// nrows), two groups of 96 node threads, group h walking rows
+// h ceil(nrows/2) .. of the tile (ClhGgNode from the k_nodes search results).
+// Per (row, node) with integrand t^2 P fac, t = W_gal b1 + W_mag pref bmag +
+// W_RSD: ln P cotangent cbar t^2 P fac; tb = 2 cbar t P fac feeds b1 (W_gal),
+// the W_gal / W_mag node cotangents and the lens shift through dW_RSD/d dz.
+// The ln P cotangent goes through the block's band (above).
+// ----------------------------------------------------------------------------
 // Three waves per SIMD are asked for: the LIM instantiation would otherwise
 // take 192 VGPRs (two waves) and run 8% slower than with a few spills.
 // F32: the integrand evaluated in FP32 (as k_limber_gg<F32>), the cotangent
@@ -545,7 +567,9 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k
   const bool lnp = d.run != nullptr;
   ClhLnPRunT<ClhBand> sc;
   sc.out.A = bd.A;  // (unconditionally: an LDS pointer throughout, ds_add_f64)
-  if (lnp) clh_band_setup(bd, d, T.lk, R, nr, (int)lut[p], lut[np + p], lut[2 * np + p], half == 0, sc.out);
+  if (lnp)
+    clh_band_setup(bd, d, T.lk, R[0].l10, R[nr - 1].l10, (int)lut[p], lut[np + p], lut[2 * np + p], half == 0,
+                   sc.out);
   const int hn = (nr + 1) / 2;
   const int r_lo = half * hn, r_hi = half ? nr : hn;
   const double WG = o[3 * np + p];
@@ -864,15 +888,15 @@ __global__ __launch_bounds__(256) void k_lim_grad(const DrawDesc* __restrict__ D
 }
 
 // ----------------------------------------------------------------------------
-// k_lnp_gather: the ss / gs part of a draw's ln P cotangent from the per-node
+// k_lnp_gather: the ss part of a draw's ln P cotangent from the per-node
 // runs.  Work item (draw, i0): lnP rows i0 .. i0 + ic - 1, every z column.
-// Node q (ss, then gs blocks in run-slab order) with z column j and weight dy
-// adds R_q[i] (1 - dy) to cell (i, j) and R_q[i] dy to (i, j + 1) in an LDS
-// tile (LDS atomics: nodes of two groups may share a column); the tile then
-// goes out with coalesced stores and covers every cell, so g_lnP needs no
-// zeroing (k_limber_gg_adj adds the gg part on top afterwards).  The nodes'
-// (j, dy) are staged in LDS first; group g of ic lanes takes nodes g, g +
-// 256/ic, ..., lane l reading R_q[i0 + l], four nodes per step.
+// ss node q with z column j and weight dy adds R_q[i] (1 - dy) to cell
+// (i, j) and R_q[i] dy to (i, j + 1) in an LDS tile (LDS atomics: nodes of
+// two groups may share a column); the tile then goes out with coalesced
+// stores and covers every cell, so g_lnP needs no zeroing (the gs and gg
+// adjoints add their bands on top afterwards).  The nodes' (j, dy) are
+// staged in LDS first; group g of ic lanes takes nodes g, g + 256/ic, ...,
+// lane l reading R_q[i0 + l], four nodes per step.
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_lnp_gather(const DrawDesc* __restrict__ D,
                                                     const int2* __restrict__ work, int ic) {
@@ -880,14 +904,13 @@ __global__ __launch_bounds__(256) void k_lnp_gather(const DrawDesc* __restrict__
   const int2 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
   const int nz = d.nzp, nk = d.nk, i0 = w.y;
-  const int total = CLH_NP_SS * (1 + d.nl);
+  const int total = CLH_NP_SS;  // the ss nodes (gs and gg fold theirs in LDS bands)
   double* s_dy = A + ic * nz;
   int* s_j = (int*)(s_dy + total);
   for (int t = threadIdx.x; t < ic * nz; t += 256) A[t] = 0.0;
   for (int q = threadIdx.x; q < total; q += 256) {
-    const int kind = q < CLH_NP_SS ? CLH_W_SS : CLH_W_GS;
-    const double* lut = clh_sx_nodes(d, kind, q / CLH_NP_SS - 1) + clh_sx_lut_col(d, kind) * CLH_NP_SS;
-    const int p = q % CLH_NP_SS;
+    const double* lut = clh_sx_nodes(d, CLH_W_SS, 0) + clh_sx_lut_col(d, CLH_W_SS) * CLH_NP_SS;
+    const int p = q;
     s_j[q] = (int)lut[p];
     s_dy[q] = lut[CLH_NP_SS + p];
   }
@@ -956,16 +979,19 @@ hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int max_npair, int ns,
-                                   int ck, int lim, hipStream_t s) {
-  // max_npair: largest npair x rows of any work item (doubles of LDS); lim:
-  // gs tiles with the lens-limit term (no checkpoint)
+                                   int ck, int lim, int gs, hipStream_t s) {
+  // max_npair: largest npair x rows of any work item (doubles of LDS); gs:
+  // the items are gs tiles (LDS band); lim: with the lens-limit term (no
+  // checkpoint)
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)max_npair;  // max over items of npair x rows
-#define CLH_SXADJ(NSV)                                                                                \
-  do {                                                                                                \
-    if (lim) hipLaunchKernelGGL((k_limber_sx_adj<NSV, false, true>), dim3(n), dim3(64), lds, s, D, work); \
-    else if (ck) hipLaunchKernelGGL((k_limber_sx_adj<NSV, true>), dim3(n), dim3(64), lds, s, D, work);  \
-    else hipLaunchKernelGGL((k_limber_sx_adj<NSV, false>), dim3(n), dim3(64), lds, s, D, work);        \
+#define CLH_SXADJ(NSV)                                                                                        \
+  do {                                                                                                        \
+    if (gs && lim) hipLaunchKernelGGL((k_limber_sx_adj<NSV, false, true, true>), dim3(n), dim3(64), lds, s, D, work); \
+    else if (gs && ck) hipLaunchKernelGGL((k_limber_sx_adj<NSV, true, false, true>), dim3(n), dim3(64), lds, s, D, work); \
+    else if (gs) hipLaunchKernelGGL((k_limber_sx_adj<NSV, false, false, true>), dim3(n), dim3(64), lds, s, D, work); \
+    else if (ck) hipLaunchKernelGGL((k_limber_sx_adj<NSV, true>), dim3(n), dim3(64), lds, s, D, work);          \
+    else hipLaunchKernelGGL((k_limber_sx_adj<NSV, false>), dim3(n), dim3(64), lds, s, D, work);                \
   } while (0)
   switch (ns) {
     case 1: CLH_SXADJ(1); break;
@@ -999,7 +1025,8 @@ hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, 
 hipError_t clh_launch_lnp_gather(const DrawDesc* D, const int2* work, int n, int ic, int nz, int nl,
                                  hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const size_t total = (size_t)CLH_NP_SS * (1 + nl);
+  (void)nl;
+  const size_t total = CLH_NP_SS;  // the ss nodes
   const size_t lds = sizeof(double) * ((size_t)ic * nz + total) + sizeof(int) * total;
   hipLaunchKernelGGL(k_lnp_gather, dim3(n), dim3(256), lds, s, D, work, ic);
   return hipGetLastError();

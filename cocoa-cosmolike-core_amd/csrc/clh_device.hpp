@@ -431,13 +431,14 @@ __device__ __forceinline__ const double* clh_gg_nodes(const DrawDesc& d, int zl)
   return d.nodes + d.node_gg + (size_t)zl * d.node_gg_stride;
 }
 
-// Row of the per-node ln P cotangent runs of ss / gs GL node p: the ss
-// nodes, then 64 per gs lens bin (null when the caller did not ask for the
-// ln P gradient).  The gg nodes fold theirs in LDS (k_limber_gg_adj).
+// Row of the per-node ln P cotangent runs of ss GL node p (null when the
+// caller did not ask for the ln P gradient).  The gs and gg nodes fold
+// theirs in LDS bands (k_limber_sx_adj<..., GS>, k_limber_gg_adj).
 __device__ __forceinline__ double* clh_run_row(const DrawDesc& d, int kind, int zl, int p) {
+  (void)kind;
+  (void)zl;
   if (!d.run) return nullptr;
-  const size_t r = kind == CLH_W_SS ? (size_t)p : (size_t)CLH_NP_SS * (1 + zl) + p;
-  return d.run + r * d.nk;
+  return d.run + (size_t)p * d.nk;
 }
 
 // stage a table of n doubles in LDS when it fits (cap entries), else use it

@@ -47,7 +47,7 @@ hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nk
 hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
 hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles, int ns, int ck,
-                                   int lim, hipStream_t s);
+                                   int lim, int gs, hipStream_t s);
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
                                    int lim, int f32, hipStream_t s);
 hipError_t clh_launch_nodes_tan(const DrawDesc* D, const int2* work, int n, hipStream_t s);
@@ -445,7 +445,7 @@ struct clh_ctx {
   bool use_graph = false;             // replay the batch forward from a hipGraph
   hipGraphExec_t gexec = nullptr;     // captured forward of `batch`
   hipStream_t gstream = nullptr;      // stream the graph was captured on
-  hipEvent_t ev[13] = {nullptr};  // 0-5 forward stages, 6-10 VJP stages, 11-12 k_nodes_tan
+  hipEvent_t ev[14] = {nullptr};  // 0-5 forward stages, 6-10 and 13 VJP stages, 11-12 k_nodes_tan
   bool vjp_lim_tan = false;         // the last timed VJP took the lens-limit term from k_nodes_tan
   float last_ms[5] = {0, 0, 0, 0, 0};
   float last_vjp_ms[4] = {0, 0, 0, 0};
@@ -2919,10 +2919,16 @@ int clh_synchronize(clh_ctx* c) {
     for (int k = 0; k < 5; k++) HIPCHK(c, hipEventElapsedTime(&c->last_ms[k], c->ev[k], c->ev[k + 1]));
     if (c->vjp_timed) {
       // launch order sx_adj, gather, gg_adj, lens limits; kept as gg_adj,
-      // sx_adj, gather, lens limits
-      static const int slot[4] = {1, 2, 0, 3};
-      for (int k = 0; k < 4; k++)
-        HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[slot[k]], c->ev[6 + k], c->ev[7 + k]));
+      // launch order: ss adjoint (6-7), gather (7-8), gs adjoint (8-13), gg
+      // adjoint (13-9), lens limits (9-10); kept as gg_adj, sx_adj (ss + gs),
+      // gather, lens limits
+      float t_ss = 0, t_gs = 0;
+      HIPCHK(c, hipEventElapsedTime(&t_ss, c->ev[6], c->ev[7]));
+      HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[2], c->ev[7], c->ev[8]));
+      HIPCHK(c, hipEventElapsedTime(&t_gs, c->ev[8], c->ev[13]));
+      HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[0], c->ev[13], c->ev[9]));
+      HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[3], c->ev[9], c->ev[10]));
+      c->last_vjp_ms[1] = t_ss + t_gs;
       // lens limits from the nodes' motion: k_nodes_tan (the rest of the
       // term is inside the gs / gg adjoint kernels' times)
       if (c->vjp_lim_tan) HIPCHK(c, hipEventElapsedTime(&c->last_vjp_ms[3], c->ev[11], c->ev[12]));
@@ -3189,9 +3195,9 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // list (k rows per item: the LDS tile of k rows x nz columns plus the
   // nodes' (j, dy) fits 64 KB); the gg nodes fold theirs in LDS
   const int nk = bt.desc[0].nk, nzp = bt.desc[0].nzp;
-  const size_t n_run = al((size_t)CLH_NP_SS * (1 + nl) * nk);
+  const size_t n_run = al((size_t)CLH_NP_SS * nk);  // the ss nodes' rows
   if (g_lnP) {
-    const size_t nq = (size_t)CLH_NP_SS * (1 + nl);
+    const size_t nq = (size_t)CLH_NP_SS;
     auto lds = [&](int ic) { return sizeof(double) * ((size_t)ic * nzp + nq) + sizeof(int) * nq; };
     int ic = 16;
     while (ic > 2 && lds(ic) > 65536) ic /= 2;
@@ -3277,17 +3283,19 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[12], s));
   }
   c->vjp_lim_tan = lim_tan;
-  // ss / gs adjoint, the gather that stores their part of g_lnP, then the gg
-  // adjoint adding its part on top (events 6 .. 9 in this order)
+  // ss adjoint, the gather that stores its part of g_lnP (every cell), then
+  // the gs and gg adjoints adding theirs from their LDS bands on top (events
+  // 6, 7, 8, 13, 9 in this order)
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));
-  // (with the lens-limit term the gs and gg adjoints evaluate their
-  // integrands again: the term needs the P(k,z) cells and the a_1 walks)
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, ck_all ? 1 : 0, 0, s));
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, ns,
-                                     (ck_all && !lim_tan) ? 1 : 0, lim_tan ? 1 : 0, s));
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, ck_all ? 1 : 0, 0, 0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
   if (g_lnP) HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
+  // (with the lens-limit term the gs and gg adjoints evaluate their
+  // integrands again: the term needs the P(k,z) cells and the a_1 walks)
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, ns,
+                                     (ck_all && !lim_tan) ? 1 : 0, lim_tan ? 1 : 0, 1, s));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[13], s));
   const bool use_ggk = bt.state_valid && bt.ckpt_fresh && bt.ggk_per && !lim_tan;
   HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga,
                                      (c->gg_lds[1] && !use_ggk) ? bt.gg_stage : 0, use_ggk ? 1 : 0,
