@@ -336,8 +336,12 @@ __device__ __forceinline__ void clh_band_flush(const ClhBandLds& bd, const DrawD
 // ----------------------------------------------------------------------------
 // GS: the launch holds gs tiles only; their ln P runs fold into the block's
 // band (added into g_lnP after the gather), the ss tiles' runs go to the run
-// slab.
-template <int NS, bool CK, bool LIM = false, bool GS = false>
+// slab.  OL (gs, no checkpoint, no lens-limit term): draws with one-loop
+// galaxy bias add W_gal O per (row, node) inside the source weights' factor,
+// O = g^4 fac (b2 d1d2 + bs2 d1s2 + b3 d1p3) / 2 + bK k^2 P fac
+// (cosmo2D.c:2027-2057): its cotangents go to B_zs, W_gal and, through the
+// bK term, ln P (bs2 held fixed: d/d b1 is taken at the uploaded bs2).
+template <int NS, bool CK, bool LIM = false, bool GS = false, bool OL = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NS < CLH_MAXB ? 3 : 1))) void k_limber_sx_adj(const DrawDesc* __restrict__ D,
                                                       const int4* __restrict__ work) {
   // NS: the source-bin count when it is at most 10 (loops fully unrolled to
@@ -358,6 +362,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NS < CLH_MAX
   const int nrow = d.nrow, ns = EXACT ? NS : d.ns;
   static_assert(!(LIM && CK), "the lens-limit term walks P(k,z) again");
   static_assert(GS || !LIM, "the lens-limit term is on gs tiles");
+  static_assert(!OL || (GS && !CK && !LIM), "one-loop terms: gs tiles without checkpoint or lens-limit term");
   const bool ss = !GS && kind == CLH_W_SS;
   const double* o = clh_sx_nodes(d, kind, zl);
   const double* lut = o + clh_sx_lut_col(d, kind) * np;
@@ -396,7 +401,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NS < CLH_MAX
   }
   const double WG = ss ? 0.0 : o[3 * np + p], WM = ss ? 0.0 : o[4 * np + p];
   const double b1 = ss ? 0.0 : d.par->b1[zl], bmag = ss ? 0.0 : d.par->bmag[zl];
-  double twg = 0.0, twm = 0.0;
+  double twg = 0.0, twm = 0.0, tow = 0.0;  // tow: OL part of the W_gal cotangent
+  const bool ol = OL && d.oneloop;
+  double ol_g4f = 0.0, ol_fk = 1.0, ol_bk = 0.0;
+  if (ol) {
+    const double gr = clh_growfac(d, o[p]);
+    ol_g4f = gr * gr * gr * gr * fac;
+    ol_fk = o[np + p];
+    ol_bk = d.par->bk[zl];
+  }
   // the ln P cotangent is wanted when the draw has a run slab (uniform per block)
   const bool lnp = d.run != nullptr;
   ClhLnPRunT<typename std::conditional<GS, ClhBand, ClhSlab>::type> sc;
@@ -429,7 +442,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NS < CLH_MAX
     } else {
       const double pref1 = PR[1][rr], pref2 = PR[2][rr];
       const double gp = WG * b1 + WM * pref1 * bmag;
-      const double cg = pref2 * gp * S;
+      double O = 0.0, Op = 0.0;  // one-loop term and its P-proportional part
+      if (ol) {
+        const double k = (ells[row0 + rr] + 0.5) / ol_fk;
+        O = clh_oneloop_gs(d, zl, log(k), k, ol_g4f, S);
+        Op = ol_bk * k * k * S;
+      }
+      const double cg = pref2 * (gp * S + WG * O);
       double sv = 0.0;
 #pragma unroll
       for (int b = 0; b < NS; b++) {
@@ -443,6 +462,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NS < CLH_MAX
       twg += t;
       twm += t * pref1;
       g = t * gp;
+      if (ol) {
+        g += pref2 * sv * WG * Op;
+        tow += pref2 * sv * O;
+      }
     }
     if (lnp) sc.add(i, dx, g);
     return g;
@@ -487,7 +510,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NS < CLH_MAX
 #pragma unroll
     for (int b = 0; b < NS; b++)
       if ((EXACT || b < ns) && acc[b] != 0.0) atomicAdd(d.bbar + ((size_t)zl * ns + b) * np + p, acc[b]);
-    atomicAdd(d.wbar + ((size_t)zl * 2 + 0) * np + p, twg * b1);
+    atomicAdd(d.wbar + ((size_t)zl * 2 + 0) * np + p, twg * b1 + tow);
     atomicAdd(d.wbar + ((size_t)zl * 2 + 1) * np + p, twm * bmag);
     const double b1p = clh_wave_sum(twg * WG);
     if (p == 0 && b1p != 0.0) atomicAdd(d.g_b1 + zl, b1p);
@@ -525,7 +548,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NS < CLH_MAX
 // take 192 VGPRs (two waves) and run 8% slower than with a few spills.
 // F32: the integrand evaluated in FP32 (as k_limber_gg<F32>), the cotangent
 // sums in FP64 (the mixed-precision modes' VJP).
-template <bool CK, bool LIM = false, bool F32 = false>
+// OL (no checkpoint, no lens-limit term): draws with one-loop galaxy bias
+// add W_gal^2 (g^4 fac Q(k) + 2 b1 bK k^2 P fac) per (row, node)
+// (cosmo2D.c:2560-2603): its cotangents go to W_gal, b1 (at the uploaded
+// bs2) and, through the bK term, ln P.
+template <bool CK, bool LIM = false, bool F32 = false, bool OL = false>
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k_limber_gg_adj(const DrawDesc* __restrict__ D,
                                                        const int4* __restrict__ work, int stage) {
   __shared__ ClhGgRow R[CLH_TILE_ROWS];
@@ -537,6 +564,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k
   // half of dW_RSD/d dz, read per row from LDS to spare registers
   __shared__ double TC[LIM ? 5 * CLH_NP_GG : 1];
   static_assert(!(LIM && CK), "the lens-limit term evaluates the integrand again");
+  static_assert(!OL || (!CK && !LIM), "one-loop terms: no checkpoint, no lens-limit term");
   extern __shared__ double dyn[];  // staged chi(z), G(z), lens n(z) tables
   const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
@@ -574,6 +602,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k
   const int r_lo = half * hn, r_hi = half ? nr : hn;
   const double WG = o[3 * np + p];
   double sl = 0.0, sgl = 0.0;  // LIM sums
+  double tol = 0.0, b1o = 0.0;  // OL sums: W_gal and b1 cotangents of the one-loop term
   if constexpr (CK) {
     // the forward's checkpoint (d.ggk): (t, P fac, dW_RSD) per row; only the
     // k cell of the ln P cotangent is walked again
@@ -612,12 +641,29 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k
     typedef typename std::conditional<F32, float, double>::type real;
     ClhGgNode<true, false, LIM, real> nd;  // no P-row look-ahead: it would cost a wave per SIMD
     nd.init(d, T, o, p, zl);
+    const bool ol = OL && d.oneloop;
+    double ol_wg2 = 0.0, ol_g4f = 0.0, ol_lnfk = 0.0, ol_fk = 1.0;
+    if (ol) {
+      const double gr = clh_growfac(d, o[p]);
+      ol_wg2 = WG * WG;
+      ol_g4f = gr * gr * gr * gr * o[2 * np + p];
+      ol_fk = o[np + p];
+      ol_lnfk = log(ol_fk);
+    }
     for (int rr = r_lo; rr < r_hi; rr++) {
       const double cb = Cr[rr];
       if (cb == 0.0) continue;
       real t, pf, dwr, dx;
       if (!nd.eval(d, T, R[rr], t, pf, dwr, dx)) continue;
-      const double g = cb * t * t * pf;
+      double g = cb * t * t * pf;
+      if (ol) {
+        const double ell = R[rr].ell;
+        double tot, pp, db1;
+        clh_oneloop_gg_parts(d, zl, log(ell) - ol_lnfk, ell / ol_fk, ol_g4f, pf, tot, pp, db1);
+        g += cb * ol_wg2 * pp;
+        tol += cb * 2.0 * WG * tot;
+        b1o += cb * ol_wg2 * db1;
+      }
       if (lnp) sc.add(nd.pk.i, dx, g);
       const double tb = cb * 2.0 * t * pf;
       tg += tb;
@@ -637,13 +683,13 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k
   if (lnp) sc.done();
   const double b1 = d.par->b1[zl], bmag = d.par->bmag[zl];
   const size_t base = (size_t)2 * d.nl * CLH_NP_SS + (size_t)zl * 2 * np;
-  if (tg != 0.0) atomicAdd(d.wbar + base + p, tg * b1);
+  if (tg != 0.0 || tol != 0.0) atomicAdd(d.wbar + base + p, tg * b1 + tol);
   if (tm != 0.0 && bmag != 0.0) atomicAdd(d.wbar + base + np + p, tm * bmag);
   if (LIM) {
     const double* tn = d.ntan + (size_t)d.nl * (6 + d.ns) * CLH_NP_SS + (size_t)zl * 8 * np;
     lzp += tn[p] * (tn[4 * np + p] * b1 * tg + tn[5 * np + p] * bmag * tm + sl) + tn[np + p] * sgl;
   }
-  b1p = clh_wave_sum(WG * tg);  // d/d b1 through W_gal b1: W_gal sum tb
+  b1p = clh_wave_sum(WG * tg + b1o);  // d/d b1 through W_gal b1 (W_gal sum tb) and the one-loop term
   lzp = clh_wave_sum(lzp);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) { red[0][wid] = b1p; red[1][wid] = lzp; }
@@ -979,7 +1025,8 @@ hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int max_npair, int ns,
-                                   int ck, int lim, int gs, hipStream_t s) {
+                                   int ck, int lim, int gs, int ol, hipStream_t s) {
+  // ol: gs tiles of a batch with one-loop bias (no checkpoint, no lens-limit term)
   // max_npair: largest npair x rows of any work item (doubles of LDS); gs:
   // the items are gs tiles (LDS band); lim: with the lens-limit term (no
   // checkpoint)
@@ -987,7 +1034,8 @@ hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, 
   const size_t lds = sizeof(double) * (size_t)max_npair;  // max over items of npair x rows
 #define CLH_SXADJ(NSV)                                                                                        \
   do {                                                                                                        \
-    if (gs && lim) hipLaunchKernelGGL((k_limber_sx_adj<NSV, false, true, true>), dim3(n), dim3(64), lds, s, D, work); \
+    if (gs && ol) hipLaunchKernelGGL((k_limber_sx_adj<NSV, false, false, true, true>), dim3(n), dim3(64), lds, s, D, work); \
+    else if (gs && lim) hipLaunchKernelGGL((k_limber_sx_adj<NSV, false, true, true>), dim3(n), dim3(64), lds, s, D, work); \
     else if (gs && ck) hipLaunchKernelGGL((k_limber_sx_adj<NSV, true, false, true>), dim3(n), dim3(64), lds, s, D, work); \
     else if (gs) hipLaunchKernelGGL((k_limber_sx_adj<NSV, false, false, true>), dim3(n), dim3(64), lds, s, D, work); \
     else if (ck) hipLaunchKernelGGL((k_limber_sx_adj<NSV, true>), dim3(n), dim3(64), lds, s, D, work);          \
@@ -1010,10 +1058,15 @@ hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, 
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
-                                   int lim, int f32, hipStream_t s) {
+                                   int lim, int f32, int ol, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)stage_doubles;
   const int st = stage_doubles > 0 ? 1 : 0;
+  if (ol) {  // a batch with one-loop bias: no checkpoint, no lens-limit term
+    if (f32) hipLaunchKernelGGL((k_limber_gg_adj<false, false, true, true>), dim3(n), dim3(192), lds, s, D, work, st);
+    else hipLaunchKernelGGL((k_limber_gg_adj<false, false, false, true>), dim3(n), dim3(192), lds, s, D, work, st);
+    return hipGetLastError();
+  }
   if (lim && f32) hipLaunchKernelGGL((k_limber_gg_adj<false, true, true>), dim3(n), dim3(192), lds, s, D, work, st);
   else if (lim) hipLaunchKernelGGL((k_limber_gg_adj<false, true>), dim3(n), dim3(192), lds, s, D, work, st);
   else if (f32 && !ck)

@@ -203,6 +203,28 @@ __device__ __forceinline__ double clh_oneloop_gg(const DrawDesc& d, int zl, doub
          2 * b1 * bk * k * k * pfac;
 }
 
+// The gg one-loop term (clh_oneloop_gg) split for the adjoint: tot, its
+// P-proportional part 2 b1 bK k^2 P fac (the part with a ln P cotangent) and
+// its derivative with respect to b1 at fixed bs2.
+__device__ __forceinline__ void clh_oneloop_gg_parts(const DrawDesc& d, int zl, double lnk, double k, double g4fac,
+                                                     double pfac, double& tot, double& pp, double& db1) {
+  const bool in = !(lnk < d.fptb_l0 || lnk > d.fptb_l1);
+  const double s4 = d.fptb_s4;
+  const double d1d2 = clh_fptb(d, 0, lnk);
+  const double d2d2 = in ? clh_fptb(d, 1, lnk) - 2. * s4 : 0.0;
+  const double d1s2 = clh_fptb(d, 2, lnk);
+  const double d2s2 = in ? clh_fptb(d, 3, lnk) - 4. / 3. * s4 : 0.0;
+  const double s2s2 = in ? clh_fptb(d, 4, lnk) - 8. / 9. * s4 : 0.0;
+  const double d1p3 = clh_fptb(d, 5, lnk);
+  const double b1 = d.par->b1[zl], b2 = d.par->b2[zl], bs2 = d.par->bs2[zl], b3 = d.par->b3[zl],
+               bk = d.par->bk[zl];
+  pp = 2 * b1 * bk * k * k * pfac;
+  tot = g4fac * (b1 * b2 * d1d2 + 0.25 * b2 * b2 * d2d2 + b1 * bs2 * d1s2 + 0.5 * b2 * bs2 * d2s2 +
+                 0.25 * bs2 * bs2 * s2s2 + b1 * b3 * d1p3) +
+        pp;
+  db1 = g4fac * (b2 * d1d2 + bs2 * d1s2 + b3 * d1p3) + 2 * bk * k * k * pfac;
+}
+
 // one-loop bias term of the gs integrand times fac (cosmo2D.c:2027-2057):
 // 0.5 g4 fac (b2 d1d2 + bs2 d1s2 + b3 d1p3) + bK k^2 P fac
 __device__ __forceinline__ double clh_oneloop_gs(const DrawDesc& d, int zl, double lnk, double k, double g4fac,

@@ -47,9 +47,9 @@ hipError_t clh_launch_set_params(const DrawDesc* D, DrawParams* P, int B, int nk
 hipError_t clh_launch_leg_adjoint(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_m_grad(const DrawDesc* D, int B, hipStream_t s);
 hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles, int ns, int ck,
-                                   int lim, int gs, hipStream_t s);
+                                   int lim, int gs, int ol, hipStream_t s);
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
-                                   int lim, int f32, hipStream_t s);
+                                   int lim, int f32, int ol, hipStream_t s);
 hipError_t clh_launch_nodes_tan(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_lim_grad(const DrawDesc* D, const DrawDesc* P, const DrawDesc* M, const int2* work, int n,
@@ -3020,8 +3020,8 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   if (rc0) return rc0;
   for (int i = 0; i < bt.B; i++) bt.desc[i].dv = bt.d_dv.p + bt.dv_off[i];
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
-  const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt && !bt.nl_any && !bt.oneloop &&
-                   !bt.lays[0]->sixpt && !bt.hdi_any;  // the VJP refuses these
+  const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt && !bt.nl_any && !bt.lays[0]->sixpt &&
+                   !bt.hdi_any;  // the VJP refuses these
   // unit cotangent, ln P and lens-shift gradients of the adjoint stages (the
   // ln P runs and the lens-limit term are part of what is timed)
   DBuf<double> dvbar, glnp, glens;
@@ -3169,7 +3169,6 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (!bt.uniform) return fail(c, CLH_ERR_ARG, "VJP needs a batch with one layout for every draw");
   if (bt.tatt) return fail(c, CLH_ERR_ARG, "VJP of TATT intrinsic alignments is not implemented");
   if (bt.nl_any) return fail(c, CLH_ERR_ARG, "VJP of the non-Limber w(theta) is not implemented");
-  if (bt.oneloop) return fail(c, CLH_ERR_ARG, "VJP of the one-loop galaxy bias terms is not implemented");
   if (bt.lays[0]->sixpt) return fail(c, CLH_ERR_ARG, "VJP of the 6x2pt data vector is not implemented");
   if (bt.hdi_any) return fail(c, CLH_ERR_ARG, "VJP at high_def_integration != 0 is not implemented");
   const Layout& L = *bt.lays[0];
@@ -3189,7 +3188,11 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // the lens-limit part of the lens-shift gradient from the nodes' motion
   // (k_nodes_tan and the LIM adjoint instantiations), unless the
   // finite-difference views are asked for
-  const bool lim_tan = g_lens_dz && nl > 0 && c->vjp_lim == 0;
+  // (one-loop batches take the term from the finite-difference views: the
+  // node-motion tangent does not cover the one-loop terms)
+  const bool lim_tan = g_lens_dz && nl > 0 && c->vjp_lim == 0 && !bt.oneloop;
+  const bool lim_fd = g_lens_dz && nl > 0 && (c->vjp_lim == 1 || (c->vjp_lim == 0 && bt.oneloop));
+  const int ol = bt.oneloop ? 1 : 0;
   HIPCHK(c, bt.d_vjp.ensure(per * B));
   // per-node ln P cotangent runs of the ss / gs nodes and the gather work
   // list (k rows per item: the LDS tile of k rows x nz columns plus the
@@ -3272,7 +3275,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     bt.ckpt_fresh = false;  // a checkpoint predates these parameters
     bt.rows64 = false;      // the gg rows predate them too
     HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
-    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, 0, 0, s));
+    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, 0, 0, s, ol));
   }
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa4.p, bt.n_wa, s));
@@ -3287,22 +3290,23 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // the gs and gg adjoints adding theirs from their LDS bands on top (events
   // 6, 7, 8, 13, 9 in this order)
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, ck_all ? 1 : 0, 0, 0, s));
+  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, ck_all ? 1 : 0, 0, 0, 0,
+                                     s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
   if (g_lnP) HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
   // (with the lens-limit term the gs and gg adjoints evaluate their
   // integrands again: the term needs the P(k,z) cells and the a_1 walks)
   HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, ns,
-                                     (ck_all && !lim_tan) ? 1 : 0, lim_tan ? 1 : 0, 1, s));
+                                     (ck_all && !lim_tan) ? 1 : 0, lim_tan ? 1 : 0, 1, ol, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[13], s));
   const bool use_ggk = bt.state_valid && bt.ckpt_fresh && bt.ggk_per && !lim_tan;
   HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga,
                                      (c->gg_lds[1] && !use_ggk) ? bt.gg_stage : 0, use_ggk ? 1 : 0,
-                                     lim_tan ? 1 : 0, c->precision != CLH_PREC_FP64 ? 1 : 0, s));
+                                     lim_tan ? 1 : 0, c->precision != CLH_PREC_FP64 ? 1 : 0, ol, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[9], s));
   c->vjp_timed = c->timing;
-  if (g_lens_dz && nl > 0 && c->vjp_lim == 1) {
+  if (lim_fd) {
     // The lens integration limits move with |dz| (redshift_spline.c:50-79),
     // and the GL nodes of the bin's gs and gg integrals with them; the
     // adjoint kernels above hold the nodes fixed.  That part comes from the
@@ -3314,8 +3318,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     constexpr double eps = 1e-8;
     const int gg_st = c->gg_lds[0] ? bt.gg_stage : 0;
     if (!bt.rows64) {  // base rows: gs (unless recomputed above in FP64) and gg
-      if (bt.state_valid) HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx_gs.p, bt.n_wsx_gs, 0, 0, s));
-      HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, 0, gg_st, 0, s));
+      if (bt.state_valid) HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx_gs.p, bt.n_wsx_gs, 0, 0, s, ol));
+      HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, 0, gg_st, 0, s, ol));
       bt.rows64 = true;
     }
     const size_t nsz = al(node_size(ns, nl)), csz = al((size_t)ncol * std::max(nrow, bt.desc[0].nrow));
@@ -3336,8 +3340,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     for (int h = 0; h < 2; h++) {
       const DrawDesc* V = bt.d_desc_lim.p + h * B;
       HIPCHK(c, clh_launch_nodes(V, bt.d_wn.p, bt.n_wn, s));
-      HIPCHK(c, clh_launch_limber_gg(V, bt.d_wgg.p, bt.n_wgg, 0, gg_st, 0, s));
-      HIPCHK(c, clh_launch_limber_sx(V, bt.d_wsx_gs.p, bt.n_wsx_gs, 0, 0, s));
+      HIPCHK(c, clh_launch_limber_gg(V, bt.d_wgg.p, bt.n_wgg, 0, gg_st, 0, s, ol));
+      HIPCHK(c, clh_launch_limber_sx(V, bt.d_wsx_gs.p, bt.n_wsx_gs, 0, 0, s, ol));
     }
     HIPCHK(c, clh_launch_lim_grad(bt.d_desc.p, bt.d_desc_lim.p, bt.d_desc_lim.p + B, bt.d_wlim.p, bt.n_wlim, s));
   }

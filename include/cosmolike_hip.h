@@ -126,8 +126,11 @@ int clh_set_nuisance_ia_tatt(clh_ctx* ctx, const double* A1, const double* A2, c
 /* set_nuisance_nonlinear_bias (generic_interface.cpp:1533-1568): b2 per lens
  * bin; bs2 = -4/7 (b1 - 1) where b2 != 0, else 0.  Any b2 != 0 switches on
  * the one-loop bias terms of the gg and gs integrands (cosmo2D.c:1970-2057,
- * 2560-2603; FP64 gs, gg in the precision mode); with TATT and in
- * clh_batch_vjp they are refused. */
+ * 2560-2603; FP64 gs, gg in the precision mode); with TATT they are
+ * refused.  The VJP differentiates them with bs2 held at its value (d/d b1
+ * at fixed bs2, as after a b1-only update; the reference recomputes bs2 only
+ * when b2 changes), and takes their lens-limit part from finite-difference
+ * views. */
 int clh_set_nuisance_nonlinear_bias(clh_ctx* ctx, const double* B1, const double* B2, int n);
 /* set_nuisance_nonlocal_bias (generic_interface.cpp:1610-1669): b3, bK per lens bin */
 int clh_set_nuisance_nonlocal_bias(clh_ctx* ctx, const double* B3, const double* BK, int n);
@@ -360,7 +363,9 @@ int clh_batch_set_params(clh_ctx* ctx, const double* lnP, const double* src_dz,
  * as clh_batch_forward) with the cotangent dv_bar [B][n] (device). Writes, for
  * every non-NULL device pointer, d<dv_bar, dv>/d(input): g_lnP [B][nk*nz],
  * g_src_dz [B][ns], g_lens_dz [B][nl], g_b1 [B][nl], g_ia_A [B], g_m [B][ns].
- * Differentiates the discrete algorithm (no reference counterpart). */
+ * Differentiates the discrete algorithm (no reference counterpart).  Refused
+ * (CLH_ERR_ARG) for batches with TATT, the non-Limber w, 6x2pt, more than
+ * one layout or high_def_integration != 0. */
 int clh_batch_vjp(clh_ctx* ctx, const int ord[3], const double* dv_bar, double* g_lnP,
                   double* g_src_dz, double* g_lens_dz, double* g_b1, double* g_ia_A, double* g_m);
 /* Rows per work tile of the four Limber stages: rows[0] forward ss/gs,

@@ -2,7 +2,7 @@
 set_bias_PS; cosmolike/cosmo2D.c:1970-2057, 2560-2603) on the GPU vs the FP64
 oracle: Limber tables, the real-space data vector in both Legendre modes,
 batches with graph replay, FP32 mode, with the non-Limber w, and the
-refusals (VJP; together with TATT).
+refusal together with TATT (the VJP is in test_gpu_vjp.py).
 
 Bar: 1e-6 relative per entry with a floor of 1e-3 of the row's largest
 magnitude (the FP64 parity metric); FP32 mode 1e-4.
@@ -128,16 +128,10 @@ def test_oneloop_with_nonlimber_w():
 
 
 def test_oneloop_refusals(ol):
-    import torch
+    """TATT together with the one-loop bias is refused (the TATT gs one-loop
+    term multiplies W_K alone); the VJP of one-loop batches is tested
+    against finite differences in test_gpu_vjp.py."""
     p, *_ = ol
-    c = Cosmolike().load(p)
-    c.batch_upload([p])
-    n = c.datavector_size()
-    _forward(c, n, 1)
-    bar = torch.ones(n, dtype=torch.float64, device="cuda:0")
-    g = torch.zeros(p.ns, dtype=torch.float64, device="cuda:0")
-    with pytest.raises(CosmolikeError, match="one-loop"):
-        c.batch_vjp(bar.data_ptr(), g_src_dz=g.data_ptr())
     t = synth.tatt_problem()
     t.b2, t.b3, t.bk = p.b2, p.b3, p.bk
     t.bias_ps, t.bias_ps_kmin, t.bias_ps_kmax, t.bias_ps_sigma4 = p.bias_ps, p.bias_ps_kmin, p.bias_ps_kmax, \
