@@ -71,7 +71,9 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_set_params(const DrawDesc* __rest
 // vector of the column), summed over the waves through LDS, goes to the
 // draw's `raw` buffer for the shear-calibration gradient, with the point-mass
 // term of gamma_t.  (The FULL-mode forward has the same derivative with
-// respect to the rows.)
+// respect to the rows.)  TATT draws (cbar_bb set): the B-mode rows enter
+// xi+ = K+ (EE + BB), xi- = K- (EE - BB), so their cotangent takes the xi-
+// part with the opposite sign, and T adds their share.
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_leg_adjoint(const DrawDesc* __restrict__ D,
                                                      const int4* __restrict__ work) {
@@ -84,6 +86,7 @@ __global__ __launch_bounds__(256) void k_leg_adjoint(const DrawDesc* __restrict_
   const int kind = w.y, c0 = w.z;
   const int nt = d.ntheta, nrow = d.nrow, nss = d.n_ss, ngs = d.n_gs;
   const int ncolk = kind == 0 ? nss : (kind == 2 ? ngs : d.nl);
+  const bool bbm = kind == 0 && d.cbar_bb != nullptr;  // TATT: B-mode cotangents as well
   const int colbase = kind == 0 ? 0 : (kind == 2 ? nss : nss + ngs);
   const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6, lm = l & 15, lk = l >> 4;
   for (int i = tid; i < 2 * 16 * 33; i += 256) {
@@ -147,6 +150,15 @@ __global__ __launch_bounds__(256) void k_leg_adjoint(const DrawDesc* __restrict_
       const int sk = u >> 3, st = u & 7;
       if (st < nst && (sk == 0 || kind == 0)) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
     }
+    // TATT: the B-mode rows enter xi+ with + and xi- with - (cosmo2D.c:517-533)
+    v4d accb = {0.0, 0.0, 0.0, 0.0};
+    if (bbm) {
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int sk = u >> 3, st = u & 7;
+        if (st < nst) accb = __builtin_amdgcn_mfma_f64_16x16x4f64(sk ? -av[u] : av[u], bv[u], accb, 0, 0, 0);
+      }
+    }
     if (!rin) continue;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -155,6 +167,11 @@ __global__ __launch_bounds__(256) void k_leg_adjoint(const DrawDesc* __restrict_
       const size_t o = (size_t)(colbase + q) * nrow + r;
       d.cbar[o] = acc[v] * cf[4 * v + lk];
       if (kind != 3) t[v] += acc[v] * d.cl[o];
+      if (bbm) {
+        const size_t ob = (size_t)q * nrow + r;
+        d.cbar_bb[ob] = accb[v] * cf[4 * v + lk];
+        t[v] += accb[v] * d.cl_bb[ob];
+      }
     }
   }
   // T per column: over the 16 lanes of each column group, then the 4 waves
@@ -536,6 +553,169 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NS < CLH_MAX
 }
 
 // ----------------------------------------------------------------------------
+// k_tatt_adj: adjoint of k_limber_sx_tatt (TATT ss and gs tiles; separate
+// launches, GS the gs ones).  One wave per work item (draw, kind + 4 zl,
+// row0, rows), thread = GL node walking the tile's rows with the forward's
+// P(k,z) lookup; the node vectors A_b = W_K - C1 W_S, B_b = W_S C1 b_TA,
+// D_b = W_S C2 (k_nodes) and their cotangents stay in registers (NS source
+// bins, at most 10).  Per (row, node), with S = P fac and E, M, T, X, TT, T',
+// X', TT' the FAST-PT IA functions times D^4 fac at k = (l+1/2)/chi:
+//  ss, F / G the EE / BB row cotangents as symmetric [ns][ns] matrices with
+//  the diagonal doubled, u^V_b = sum_j F_bj V_j, v^V_b = sum_j G_bj V_j:
+//    A_b: pref (u^A S - u^B E + 5 u^D M)
+//    B_b: pref (-u^A E + u^B T - 5 u^D X + v^B T' - 5 v^D X')
+//    D_b: pref (5 u^A M - 5 u^B X + 25 u^D TT - 5 v^B X' + 25 v^D TT')
+//    ln P: pref S sum_b A_b u^A_b / 2
+//  gs (lens bin l, gal = W_gal b1 + W_mag pref1 b_mag), c_s the row's gs
+//  cotangents: A_s += pref2 gal S c_s, B_s -= pref2 gal E c_s, D_s += 5 pref2
+//  gal M c_s; W_gal, W_mag and b1 through pref2 sum_s c_s (A_s S - B_s E +
+//  5 D_s M); ln P pref2 gal S sum_s c_s A_s.
+// The FAST-PT tables are inputs (no ln P dependence).  The ss runs go to the
+// run slab, the gs runs to the block's band (after the gather).
+// ----------------------------------------------------------------------------
+template <int NS, bool GS>
+__global__ __launch_bounds__(64) void k_tatt_adj(const DrawDesc* __restrict__ D, const int4* __restrict__ work) {
+  extern __shared__ double Cb[];  // ss: EE [ns][ns][T] then BB [ns][ns][T]; gs: [ns][T]
+  __shared__ double L10[CLH_TILE_ROWS], LN[CLH_TILE_ROWS];
+  __shared__ double PR[3][CLH_TILE_ROWS];
+  __shared__ double s_lk[CLH_LDS_LK];
+  __shared__ typename std::conditional<GS, ClhBandLds, int>::type bd;
+  const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
+  const DrawDesc& d = D[w.x];
+  const int zl = w.y >> 2, row0 = w.z, T = w.w;
+  const int kind = GS ? CLH_W_GS : CLH_W_SS;
+  const int p = threadIdx.x;
+  constexpr int np = CLH_NP_SS, ns = NS;
+  const int nrow = d.nrow;
+  const double* o = clh_sx_nodes(d, kind, zl);
+  const double* lut = o + clh_sx_lut_col(d, kind) * np;
+  const double* ells = GS ? d.ell_gx : d.ell_ss;
+  const double* tv = d.tatt + (size_t)(GS ? 1 + zl : 0) * (2 * ns + 1) * np;
+  if (!GS) {
+    for (int i = p; i < ns * ns * T; i += 64) {
+      const int b = i / (ns * T), jj = (i / T) % ns, r = row0 + i % T;
+      const size_t q = (size_t)clh_ss_index(ns, b, jj) * nrow + r;
+      const double f = b == jj ? 2.0 : 1.0;
+      Cb[i] = f * d.cbar[q];
+      Cb[ns * ns * T + i] = f * d.cbar_bb[q];
+    }
+  } else {
+    const int col0 = d.n_ss + zl * ns;
+    for (int i = p; i < ns * T; i += 64) Cb[i] = d.cbar[(size_t)(col0 + i / T) * nrow + row0 + i % T];
+  }
+  for (int t = p; t < T; t += 64) {
+    const double l = ells[row0 + t], ell = l + 0.5;
+    L10[t] = log10(ell);
+    LN[t] = log(ell);
+    PR[0][t] = l * (l - 1.) * (l + 1.) * (l + 2.) / (ell * ell * ell * ell);
+    const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+    PR[1][t] = l * (l + 1.) / (ell * ell);
+    PR[2][t] = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+  }
+  const double* lk = clh_stage(d.lk, d.nk, s_lk, CLH_LDS_LK);
+  __syncthreads();
+  const double fac = o[2 * np + p];
+  const int j = (int)lut[p];
+  const double dy = lut[np + p], l10fk = lut[2 * np + p], lnfk = log(o[np + p]);
+  const double g4f = tv[2 * ns * np + p] * fac;
+  const int woff = GS ? 5 : 3;
+  double Av[NS], Bv[NS], Dv[NS], aA[NS], aB[NS], aD[NS];
+#pragma unroll
+  for (int b = 0; b < NS; b++) {
+    Av[b] = o[(woff + b) * np + p];
+    Bv[b] = tv[b * np + p];
+    Dv[b] = tv[(ns + b) * np + p];
+    aA[b] = aB[b] = aD[b] = 0.0;
+  }
+  const double WG = GS ? o[3 * np + p] : 0.0, WM = GS ? o[4 * np + p] : 0.0;
+  const double b1 = GS ? d.par->b1[zl] : 0.0, bmag = GS ? d.par->bmag[zl] : 0.0;
+  double twg = 0.0, twm = 0.0;
+  const bool lnp = d.run != nullptr;  // uniform per block
+  ClhLnPRunT<typename std::conditional<GS, ClhBand, ClhSlab>::type> sc;
+  if constexpr (GS) {
+    sc.out.A = bd.A;
+    if (lnp) clh_band_setup(bd, d, lk, L10[0], L10[T - 1], j, dy, l10fk, true, sc.out);
+  } else {
+    sc.out.R = clh_run_row(d, kind, zl, p);
+  }
+  ClhPkNodeT<false> pkn;
+  pkn.init(d, j, dy);
+  constexpr double c3 = 1.0 / (CLH_COVERH0 * CLH_COVERH0 * CLH_COVERH0);
+  for (int rr = 0; rr < T; rr++) {
+    double dx;
+    const double S = exp(pkn.lnp(lk, L10[rr] - l10fk, dx)) * c3 * fac;
+    const double lnk = LN[rr] - lnfk;
+    const double E = g4f * (clh_fpt(d, 2, lnk) + clh_fpt(d, 3, lnk));
+    const double M = g4f * (clh_fpt(d, 6, lnk) + clh_fpt(d, 7, lnk));
+    double g;
+    if constexpr (!GS) {
+      const double Tt = g4f * clh_fpt(d, 4, lnk), X = g4f * clh_fpt(d, 8, lnk), TT = g4f * clh_fpt(d, 0, lnk);
+      const double Tp = g4f * clh_fpt(d, 5, lnk), Xp = g4f * clh_fpt(d, 9, lnk), TTp = g4f * clh_fpt(d, 1, lnk);
+      const double pref = PR[0][rr];
+      double sAA = 0.0;
+#pragma unroll
+      for (int b = 0; b < NS; b++) {
+        double uA = 0.0, uB = 0.0, uD = 0.0, vB = 0.0, vD = 0.0;
+        const double* F = Cb + (size_t)(b * ns) * T + rr;
+        const double* G = F + (size_t)ns * ns * T;
+#pragma unroll
+        for (int jj = 0; jj < NS; jj++) {
+          const double f = F[jj * T], gg = G[jj * T];
+          uA += f * Av[jj];
+          uB += f * Bv[jj];
+          uD += f * Dv[jj];
+          vB += gg * Bv[jj];
+          vD += gg * Dv[jj];
+        }
+        aA[b] += pref * (uA * S - uB * E + 5.0 * uD * M);
+        aB[b] += pref * (-uA * E + uB * Tt - 5.0 * uD * X + vB * Tp - 5.0 * vD * Xp);
+        aD[b] += pref * (5.0 * uA * M - 5.0 * uB * X + 25.0 * uD * TT - 5.0 * vB * Xp + 25.0 * vD * TTp);
+        sAA += Av[b] * uA;
+      }
+      g = 0.5 * pref * S * sAA;
+    } else {
+      const double pref1 = PR[1][rr], pref2 = PR[2][rr];
+      const double gal = WG * b1 + WM * pref1 * bmag;
+      const double cS = pref2 * gal * S, cE = pref2 * gal * E, cM = 5.0 * pref2 * gal * M;
+      double stot = 0.0, sA = 0.0;
+#pragma unroll
+      for (int b = 0; b < NS; b++) {
+        const double cb = Cb[b * T + rr];
+        aA[b] += cS * cb;
+        aB[b] -= cE * cb;
+        aD[b] += cM * cb;
+        stot += cb * (Av[b] * S - Bv[b] * E + 5.0 * Dv[b] * M);
+        sA += cb * Av[b];
+      }
+      const double t = pref2 * stot;
+      twg += t;
+      twm += t * pref1;
+      g = pref2 * gal * S * sA;
+    }
+    if (lnp) sc.add(pkn.i, dx, g);
+  }
+  if (lnp) sc.done();
+  double* tb = d.tbar + (size_t)(GS ? 1 + zl : 0) * 2 * ns * np;
+#pragma unroll
+  for (int b = 0; b < NS; b++) {
+    double* ab = GS ? d.bbar + ((size_t)zl * ns + b) * np : d.abar + (size_t)b * np;
+    if (aA[b] != 0.0) atomicAdd(ab + p, aA[b]);
+    if (aB[b] != 0.0) atomicAdd(tb + (size_t)b * np + p, aB[b]);
+    if (aD[b] != 0.0) atomicAdd(tb + (size_t)(ns + b) * np + p, aD[b]);
+  }
+  if constexpr (GS) {
+    atomicAdd(d.wbar + ((size_t)zl * 2 + 0) * np + p, twg * b1);
+    atomicAdd(d.wbar + ((size_t)zl * 2 + 1) * np + p, twm * bmag);
+    const double b1p = clh_wave_sum(twg * WG);
+    if (p == 0 && b1p != 0.0) atomicAdd(d.g_b1 + zl, b1p);
+    if (lnp) {
+      __syncthreads();
+      clh_band_flush(bd, d);
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
 // k_limber_gg_adj: adjoint of k_limber_gg; work item (draw, zl, row0,
 // nrows), two groups of 96 node threads, group h walking rows
 // h ceil(nrows/2) .. of the tile (ClhGgNode from the k_nodes search results).
@@ -742,6 +922,17 @@ __global__ __launch_bounds__(128) void k_nodes_adj(const DrawDesc* __restrict__ 
         const double dWK = wk_pre * clh_dg_tomo(d, a, b);
         gs = bar * (dWK - C1 * dWS);
         gA += bar * (-c1u * WS);
+      }
+      if (d.tbar) {  // TATT: B_b = W_S C1 b_TA and D_b = W_S C2 (IA.c:238-420)
+        const double* tb = d.tbar + (size_t)seg * 2 * d.ns * CLH_NP_SS;
+        const double tB = tb[(size_t)b * CLH_NP_SS + p], tD = tb[(size_t)(d.ns + b) * CLH_NP_SS + p];
+        if (tB != 0.0 || tD != 0.0) {
+          const double C1 = clh_ia_c1(d, a, gf, b), C2 = clh_ia_c2(d, a, gf, b), bta = clh_ia_bta(d, b);
+          const double WS = clh_nz_src(d, z, b) * hh;
+          const double dWS = clh_dnz_src(d, z, b) * hh;
+          gs += tB * dWS * C1 * bta + tD * dWS * C2;
+          gA += tB * WS * c1u * bta;
+        }
       }
     }
     gs = clh_wave_sum(gs);
@@ -1055,6 +1246,32 @@ hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, 
     default: CLH_SXADJ(CLH_MAXB); break;
   }
 #undef CLH_SXADJ
+  return hipGetLastError();
+}
+hipError_t clh_launch_tatt_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles, int ns, int gs,
+                               hipStream_t s) {
+  // lds_doubles: largest cotangent tile of the items (ss: 2 ns^2 rows, gs: ns rows)
+  if (n <= 0) return hipSuccess;
+  const size_t lds = sizeof(double) * (size_t)lds_doubles;
+#define CLH_TADJ(NSV)                                                                           \
+  do {                                                                                          \
+    if (gs) hipLaunchKernelGGL((k_tatt_adj<NSV, true>), dim3(n), dim3(64), lds, s, D, work);      \
+    else hipLaunchKernelGGL((k_tatt_adj<NSV, false>), dim3(n), dim3(64), lds, s, D, work);        \
+  } while (0)
+  switch (ns) {
+    case 1: CLH_TADJ(1); break;
+    case 2: CLH_TADJ(2); break;
+    case 3: CLH_TADJ(3); break;
+    case 4: CLH_TADJ(4); break;
+    case 5: CLH_TADJ(5); break;
+    case 6: CLH_TADJ(6); break;
+    case 7: CLH_TADJ(7); break;
+    case 8: CLH_TADJ(8); break;
+    case 9: CLH_TADJ(9); break;
+    case 10: CLH_TADJ(10); break;
+    default: return hipErrorInvalidValue;  // the host refuses more than 10 source bins
+  }
+#undef CLH_TADJ
   return hipGetLastError();
 }
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,

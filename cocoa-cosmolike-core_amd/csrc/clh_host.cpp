@@ -51,6 +51,9 @@ hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, 
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
                                    int lim, int f32, int ol, hipStream_t s);
 hipError_t clh_launch_nodes_tan(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_tatt_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles, int ns, int gs,
+                               hipStream_t s);
+hipError_t clh_launch_limber_sx_tatt(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_nodes_adj(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_lim_grad(const DrawDesc* D, const DrawDesc* P, const DrawDesc* M, const int2* work, int n,
                                hipStream_t s);
@@ -345,9 +348,9 @@ struct Batch {
   std::vector<DrawDesc> desc_lim;
   DBuf<DrawDesc> d_desc_lim;
   DBuf<double> d_lim;
-  DBuf<int4> d_wsx_gs;
+  DBuf<int4> d_wsx_gs, d_wsx_gs_t;  // the gs tiles of the non-TATT / TATT draws
   DBuf<int2> d_wlim;
-  int n_wsx_gs = 0, n_wlim = 0;
+  int n_wsx_gs = 0, n_wsx_gs_t = 0, n_wlim = 0;
   int nrow_custom = 0;  // rows of a custom-ell batch (0: the layouts' rows)
   int gg_stage = 0;     // LDS doubles for the gg kernels' staged tables (0: not staged)
   // the device state (lens-efficiency tables, node weights, ss/gs C_ell rows)
@@ -398,7 +401,8 @@ struct Batch {
     d_desc_nl.release(); d_nl_ells.release(); d_wn_nl.release(); d_wnl_prep.release(); d_wnl_cl.release();
     d_wgg_nl.release(); d_wnl_fwd.release(); d_wnl_gl.release(); d_wnl_chk.release(); d_wnl_idft.release();
     d_wcmb.release(); d_wcmbp.release();
-    d_desc_lim.release(); d_lim.release(); d_wsx_gs.release(); d_wlim.release(); d_whd.release();
+    d_desc_lim.release(); d_lim.release(); d_wsx_gs.release(); d_wsx_gs_t.release(); d_wlim.release();
+    d_whd.release();
   }
 };
 
@@ -1144,7 +1148,7 @@ static bool ord_ok(const Layout& L, const int ord[3]) {
 // adjoint ss/gs and adjoint gg Limber stages (1..CLH_TILE_ROWS).
 static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   std::vector<int2> wg, wn, wl2, wlim;
-  std::vector<int4> wl, wl_gs, wsx, wsx_t, wsx_gs, wgg, wga, wmx, wlf, wa, whd;
+  std::vector<int4> wl, wl_gs, wsx, wsx_t, wsx_gs, wsx_gs_t, wgg, wga, wmx, wlf, wa, whd;
   int lds_ss = 0, lds_gs = 0;
   std::vector<int2> tiles, atiles_ss, atiles_gs;
   for (int i = 0; i < bt.B; i++) {
@@ -1172,7 +1176,7 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
     for (int zl = 0; zl < nl; zl++)
       for (const int2& t : tiles) {
         ws.push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
-        if (L.ia_model != CLH_IA_TATT) wsx_gs.push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
+        (L.ia_model != CLH_IA_TATT ? wsx_gs : wsx_gs_t).push_back(make_int4(i, CLH_W_GS + 4 * zl, t.x, t.y));
       }
     for (int zl = 0; zl < nl; zl++) wlim.push_back(make_int2(i, zl));
     row_tiles(nrow, rows[1], tiles);
@@ -1180,7 +1184,8 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
       for (const int2& t : tiles) wgg.push_back(make_int4(i, zl, t.x, t.y));
     // adjoint ss/gs tiles: the tile's cotangent rows ([ns][ns][rows] for ss,
     // [ns][rows] for gs) fit in 64 KB of LDS
-    const int cap_ss = std::max(1, std::min(rows[2], 8192 / std::max(ns * ns, 1)));
+    // (TATT: the EE and BB cotangents, 2 ns^2 doubles per row, within 48 KB)
+    const int cap_ss = std::max(1, std::min(rows[2], (L.ia_model == CLH_IA_TATT ? 3072 : 8192) / std::max(ns * ns, 1)));
     const int cap_gs = std::max(1, std::min(rows[2], 8192 / std::max(ns, 1)));
     row_tiles(nrow, cap_ss, atiles_ss);
     row_tiles(nrow, cap_gs, atiles_gs);
@@ -1233,6 +1238,7 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   HIPCHK(c, up4(bt.d_wsx, wsx, bt.n_wsx));
   HIPCHK(c, up4(bt.d_wgg, wgg, bt.n_wgg));
   HIPCHK(c, up4(bt.d_wsx_gs, wsx_gs, bt.n_wsx_gs));
+  HIPCHK(c, up4(bt.d_wsx_gs_t, wsx_gs_t, bt.n_wsx_gs_t));
   HIPCHK(c, up2(bt.d_wlim, wlim, bt.n_wlim));
   HIPCHK(c, up4(bt.d_wga, wga, bt.n_wga));
   HIPCHK(c, up4(bt.d_wmx, wmx, bt.n_wmx));
@@ -3020,8 +3026,8 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   if (rc0) return rc0;
   for (int i = 0; i < bt.B; i++) bt.desc[i].dv = bt.d_dv.p + bt.dv_off[i];
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
-  const bool adj = bt.uniform && bt.nknz > 0 && !bt.tatt && !bt.nl_any && !bt.lays[0]->sixpt &&
-                   !bt.hdi_any;  // the VJP refuses these
+  const bool adj = bt.uniform && bt.nknz > 0 && !(bt.tatt && bt.lays[0]->src.nbin > 10) && !bt.nl_any &&
+                   !bt.lays[0]->sixpt && !bt.hdi_any;  // the VJP refuses these
   // unit cotangent, ln P and lens-shift gradients of the adjoint stages (the
   // ln P runs and the lens-limit term are part of what is timed)
   DBuf<double> dvbar, glnp, glens;
@@ -3167,7 +3173,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (batch_stale(c, bt))
     return fail(c, CLH_ERR_STATE, "layout changed since the batch upload");
   if (!bt.uniform) return fail(c, CLH_ERR_ARG, "VJP needs a batch with one layout for every draw");
-  if (bt.tatt) return fail(c, CLH_ERR_ARG, "VJP of TATT intrinsic alignments is not implemented");
+  if (bt.tatt && bt.lays[0]->src.nbin > 10)
+    return fail(c, CLH_ERR_ARG, "VJP under TATT takes at most 10 source bins (%d)", bt.lays[0]->src.nbin);
   if (bt.nl_any) return fail(c, CLH_ERR_ARG, "VJP of the non-Limber w(theta) is not implemented");
   if (bt.lays[0]->sixpt) return fail(c, CLH_ERR_ARG, "VJP of the 6x2pt data vector is not implemented");
   if (bt.hdi_any) return fail(c, CLH_ERR_ARG, "VJP at high_def_integration != 0 is not implemented");
@@ -3183,15 +3190,17 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
                n_bbar = al((size_t)nl * ns * CLH_NP_SS),
                n_wbar = al((size_t)nl * 2 * CLH_NP_SS + (size_t)nl * 2 * CLH_NP_GG),
                n_dg = al((size_t)(ns + nl) * L.n_a),
-               n_tan = al((size_t)nl * (6 + ns) * CLH_NP_SS + (size_t)nl * 8 * CLH_NP_GG);
-  const size_t per = n_cbar + n_raw + n_abar + n_bbar + n_wbar + n_dg + n_tan;
+               n_tan = al((size_t)nl * (6 + ns) * CLH_NP_SS + (size_t)nl * 8 * CLH_NP_GG),
+               n_cbb = bt.tatt ? al((size_t)L.n_ss() * nrow) : 0,
+               n_tb = bt.tatt ? al((size_t)(1 + nl) * 2 * ns * CLH_NP_SS) : 0;
+  const size_t per = n_cbar + n_raw + n_abar + n_bbar + n_wbar + n_dg + n_tan + n_cbb + n_tb;
   // the lens-limit part of the lens-shift gradient from the nodes' motion
   // (k_nodes_tan and the LIM adjoint instantiations), unless the
   // finite-difference views are asked for
   // (one-loop batches take the term from the finite-difference views: the
   // node-motion tangent does not cover the one-loop terms)
-  const bool lim_tan = g_lens_dz && nl > 0 && c->vjp_lim == 0 && !bt.oneloop;
-  const bool lim_fd = g_lens_dz && nl > 0 && (c->vjp_lim == 1 || (c->vjp_lim == 0 && bt.oneloop));
+  const bool lim_tan = g_lens_dz && nl > 0 && c->vjp_lim == 0 && !bt.oneloop && !bt.tatt;
+  const bool lim_fd = g_lens_dz && nl > 0 && (c->vjp_lim == 1 || (c->vjp_lim == 0 && (bt.oneloop || bt.tatt)));
   const int ol = bt.oneloop ? 1 : 0;
   HIPCHK(c, bt.d_vjp.ensure(per * B));
   // per-node ln P cotangent runs of the ss / gs nodes and the gather work
@@ -3249,6 +3258,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     q.sxk = ck ? bt.d_sxk.p + bt.sxk_per * i : nullptr;
     q.dgtab = ck ? bt.d_dgk.p + bt.dgk_per * i : v;
     q.ntan = lim_tan ? v + n_dg : nullptr;
+    q.cbar_bb = bt.tatt ? v + n_dg + n_tan : nullptr;
+    q.tbar = bt.tatt ? v + n_dg + n_tan + n_cbb : nullptr;
     q.g_src = g_src_dz ? g_src_dz + (size_t)ns * i : g + al(bt.nknz);
     q.g_lens = g_lens_dz ? g_lens_dz + (size_t)nl * i : g + al(bt.nknz) + al(CLH_MAXB);
     q.g_b1 = g_b1 ? g_b1 + (size_t)nl * i : g + al(bt.nknz) + 2 * al(CLH_MAXB);
@@ -3275,7 +3286,9 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     bt.ckpt_fresh = false;  // a checkpoint predates these parameters
     bt.rows64 = false;      // the gg rows predate them too
     HIPCHK(c, clh_launch_nodes(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
-    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx, 0, 0, s, ol));
+    HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx_std, 0, 0, s, ol));
+    if (bt.n_wsx > bt.n_wsx_std)  // the TATT draws' tiles (EE and BB rows)
+      HIPCHK(c, clh_launch_limber_sx_tatt(bt.d_desc.p, bt.d_wsx.p + bt.n_wsx_std, bt.n_wsx - bt.n_wsx_std, s));
   }
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa4.p, bt.n_wa, s));
@@ -3290,15 +3303,21 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // the gs and gg adjoints adding theirs from their LDS bands on top (events
   // 6, 7, 8, 13, 9 in this order)
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[6], s));
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, ck_all ? 1 : 0, 0, 0, 0,
-                                     s));
+  if (bt.tatt)  // TATT: the EE and BB cotangent tiles
+    HIPCHK(c, clh_launch_tatt_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, 2 * bt.adj_lds_ss, ns, 0, s));
+  else
+    HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, ck_all ? 1 : 0, 0, 0,
+                                       0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
   if (g_lnP) HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
   // (with the lens-limit term the gs and gg adjoints evaluate their
   // integrands again: the term needs the P(k,z) cells and the a_1 walks)
-  HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, ns,
-                                     (ck_all && !lim_tan) ? 1 : 0, lim_tan ? 1 : 0, 1, ol, s));
+  if (bt.tatt)
+    HIPCHK(c, clh_launch_tatt_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, ns, 1, s));
+  else
+    HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p + bt.n_wl_ss, bt.n_wl - bt.n_wl_ss, bt.adj_lds_gs, ns,
+                                       (ck_all && !lim_tan) ? 1 : 0, lim_tan ? 1 : 0, 1, ol, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[13], s));
   const bool use_ggk = bt.state_valid && bt.ckpt_fresh && bt.ggk_per && !lim_tan;
   HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga,
@@ -3322,17 +3341,20 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
       HIPCHK(c, clh_launch_limber_gg(bt.d_desc.p, bt.d_wgg.p, bt.n_wgg, 0, gg_st, 0, s, ol));
       bt.rows64 = true;
     }
-    const size_t nsz = al(node_size(ns, nl)), csz = al((size_t)ncol * std::max(nrow, bt.desc[0].nrow));
-    HIPCHK(c, bt.d_lim.ensure((nsz + csz) * 2 * B));
+    // (TATT: the views' own B_b, D_b node vectors for their gs tiles)
+    const size_t nsz = al(node_size(ns, nl)), csz = al((size_t)ncol * std::max(nrow, bt.desc[0].nrow)),
+                 tsz = bt.tatt ? al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS) : 0;
+    HIPCHK(c, bt.d_lim.ensure((nsz + csz + tsz) * 2 * B));
     HIPCHK(c, bt.d_desc_lim.ensure(2 * B));
     bt.desc_lim.resize(2 * B);
     for (int i = 0; i < 2 * B; i++) {
       DrawDesc& q = bt.desc_lim[i];
       q = bt.desc[i % B];
       q.lim_eps = i < B ? eps : -eps;
-      q.nodes = bt.d_lim.p + (nsz + csz) * i;
+      q.nodes = bt.d_lim.p + (nsz + csz + tsz) * i;
       q.cl = q.nodes + nsz;
-      q.dv = nullptr; q.cl_bb = nullptr; q.tatt = nullptr; q.cl6 = nullptr; q.nlw = nullptr;
+      q.dv = nullptr; q.cl_bb = nullptr; q.cl6 = nullptr; q.nlw = nullptr;
+      q.tatt = bt.tatt ? q.cl + csz : nullptr;
       q.ggk = nullptr; q.sxk = nullptr;
     }
     HIPCHK(c, hipMemcpyAsync(bt.d_desc_lim.p, bt.desc_lim.data(), 2 * B * sizeof(DrawDesc), hipMemcpyHostToDevice,
@@ -3342,6 +3364,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
       HIPCHK(c, clh_launch_nodes(V, bt.d_wn.p, bt.n_wn, s));
       HIPCHK(c, clh_launch_limber_gg(V, bt.d_wgg.p, bt.n_wgg, 0, gg_st, 0, s, ol));
       HIPCHK(c, clh_launch_limber_sx(V, bt.d_wsx_gs.p, bt.n_wsx_gs, 0, 0, s, ol));
+      if (bt.n_wsx_gs_t) HIPCHK(c, clh_launch_limber_sx_tatt(V, bt.d_wsx_gs_t.p, bt.n_wsx_gs_t, s));
     }
     HIPCHK(c, clh_launch_lim_grad(bt.d_desc.p, bt.d_desc_lim.p, bt.d_desc_lim.p + B, bt.d_wlim.p, bt.n_wlim, s));
   }

@@ -189,6 +189,11 @@ struct DrawDesc {
   double* run;          // [64 (1 + nl)][nk] per-GL-node ln P cotangent runs of the ss / gs nodes, or null
   double* sxk;          // [1 + nl][nrow][64] ss / gs checkpoint of the forward (S = P fac per
                         // row and node) for the VJP, or null
+  // TATT VJP: cotangents of the B-mode ss rows [n_ss][nrow] and of the node
+  // vectors B_b, D_b at the ss and gs nodes [(1 + nl)][2][ns][64] (null
+  // otherwise)
+  double* cbar_bb;
+  double* tbar;
   // the motion of the gs / gg nodes with the lens photo-z shift (the lens
   // integration limits move with |dz|), per node of lens bin zl, for the VJP
   // (k_nodes_tan), or null: gs [nl][6 + ns][64] = da/d dz, d ln(GL weight x

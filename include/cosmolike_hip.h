@@ -83,7 +83,8 @@ int clh_init_binning_real_space(clh_ctx* ctx, int ntheta, double theta_min_arcmi
 /* init_IA (generic_interface.cpp:628-634): ia_model 0 (NLA) or 1 (TATT);
  * ia_redshift_evol 0 (none), 2 (per-bin A) or 3 (A ((1+z)/1.62)^eta).
  * TATT computes the E- and B-mode C_ss (xi+- = EE +- BB transforms) and the
- * TATT gamma_t in FP64 in every precision mode; clh_batch_vjp refuses it. */
+ * TATT gamma_t in FP64 in every precision mode; clh_batch_vjp takes TATT
+ * batches of at most 10 source bins. */
 int clh_init_ia(clh_ctx* ctx, int ia_model, int ia_redshift_evol);
 /* set_cosmological_parameters (generic_interface.cpp:912-939); H0 in
  * km/s/Mpc sets c/H0 in Mpc for the non-Limber w */
@@ -364,8 +365,10 @@ int clh_batch_set_params(clh_ctx* ctx, const double* lnP, const double* src_dz,
  * every non-NULL device pointer, d<dv_bar, dv>/d(input): g_lnP [B][nk*nz],
  * g_src_dz [B][ns], g_lens_dz [B][nl], g_b1 [B][nl], g_ia_A [B], g_m [B][ns].
  * Differentiates the discrete algorithm (no reference counterpart).  Refused
- * (CLH_ERR_ARG) for batches with TATT, the non-Limber w, 6x2pt, more than
- * one layout or high_def_integration != 0. */
+ * (CLH_ERR_ARG) for batches with the non-Limber w, 6x2pt, TATT with more
+ * than 10 source bins, more than one layout or high_def_integration != 0.
+ * Under TATT the gradient runs through A1 (ia_A, redshift evolution mode 3)
+ * with A2, eta2 and b_TA held at their uploaded values. */
 int clh_batch_vjp(clh_ctx* ctx, const int ord[3], const double* dv_bar, double* g_lnP,
                   double* g_src_dz, double* g_lens_dz, double* g_b1, double* g_ia_A, double* g_m);
 /* Rows per work tile of the four Limber stages: rows[0] forward ss/gs,

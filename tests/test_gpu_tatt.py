@@ -1,7 +1,8 @@
 """TATT intrinsic alignments (IA_MODEL = TATT) on the GPU vs the FP64 oracle:
 E- and B-mode C_ss tables, the TATT C_gs, xi+- from EE +- BB, gamma_t and w
 of the real-space 3x2pt data vector, direct-GL C_ell, batches, the precision
-modes (TATT ss / gs tiles stay FP64) and the refused VJP.
+modes (TATT ss / gs tiles stay FP64) and the VJP's refusal above 10 source
+bins (the VJP itself is in test_gpu_vjp.py).
 
 The FAST-PT IA spectra are synthetic tables of the set_IA_PS shape (no
 FAST-PT here); the oracle reads the same tables.
@@ -155,9 +156,13 @@ def test_tatt_precision_modes(tatt, mode, tol):
     assert max(e.values()) < tol
 
 
-def test_tatt_vjp_refused(tatt):
+def test_tatt_vjp_refused_above_10_source_bins():
     import torch
-    p, *_ = tatt
+    cos = synth.random_draws(1, seed=5, ragged=False)[0][0]
+    p = synth.ragged_3x2pt_problem(cos, 11, 2, 20)
+    q = synth.tatt_problem(cos)
+    p.ia_model, p.ia2_A, p.ia2_eta, p.ia_bta = q.ia_model, q.ia2_A, q.ia2_eta, q.ia_bta
+    p.ia_ps, p.ia_ps_kmin, p.ia_ps_kmax = q.ia_ps, q.ia_ps_kmin, q.ia_ps_kmax
     c = Cosmolike().load(p)
     c.batch_upload([p])
     n = c.datavector_size()
