@@ -202,8 +202,9 @@ __global__ __launch_bounds__(256) void k_leg_adjoint(const DrawDesc* __restrict_
 
 // ----------------------------------------------------------------------------
 // k_m_grad: one block per draw, one thread per source bin.  With T[col] the
-// cotangent-weighted uncalibrated column sums of k_leg_adjoint:
-//   d/dm_i = sum_{ss pairs (a,b) with i} d[(1+m_a)(1+m_b)]/dm_i T + sum_zl T[(zl, i)].
+// cotangent-weighted uncalibrated column sums of k_leg_adjoint (and, for
+// 6x2pt, of k_cmb_place_adj's ks columns after the 3x2pt ones):
+//   d/dm_i = sum_{ss pairs (a,b) with i} d[(1+m_a)(1+m_b)]/dm_i T + sum_zl T[(zl, i)] + T[ks i].
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_m_grad(const DrawDesc* __restrict__ D) {
   const DrawDesc& d = D[blockIdx.x];
@@ -220,6 +221,7 @@ __global__ __launch_bounds__(64) void k_m_grad(const DrawDesc* __restrict__ D) {
       g += der * d.raw[q];
     }
     for (int zl = 0; zl < d.nl; zl++) g += d.raw[d.n_ss + zl * d.ns + i];
+    if (d.cbar6) g += d.raw[d.n_ss + d.n_gs + d.nl + i];  // 6x2pt: the ks column of source bin i
   }
   d.g_m[i] = g;
 }
@@ -1095,7 +1097,7 @@ __global__ __launch_bounds__(128) void k_nodes_tan(const DrawDesc* __restrict__ 
 // smaller magnitude is the row's slope (a jump adds |jump| / eps to the
 // other); else their mean.  One block per (draw, lens bin):
 // sign(dz) sum_(c, r) cbar[c][r] slope[c][r] over the bin's gs columns and
-// its gg column, into g_lens; nothing for |dz| <= 2 eps (the forward's kink
+// its gg column (and its C_gk column under 6x2pt), into g_lens; nothing for |dz| <= 2 eps (the forward's kink
 // at dz = 0, where the dependence on |dz| is symmetric).
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_lim_grad(const DrawDesc* __restrict__ D, const DrawDesc* __restrict__ P,
@@ -1119,6 +1121,16 @@ __global__ __launch_bounds__(256) void k_lim_grad(const DrawDesc* __restrict__ D
     const double s = fabs(sp - sm) <= 0.05 * (fabs(sp) + fabs(sm)) ? 0.5 * (sp + sm)
                    : (fabs(sp) < fabs(sm) ? sp : sm);
     acc += d.cbar[o] * s;
+  }
+  if (d.cbar6) {  // 6x2pt: the bin's C_gk column (its limits are the gs / gg ones)
+    const double *c6 = d.cl6 + (size_t)zl * d.nrow6, *cb6 = d.cbar6 + (size_t)zl * d.nrow6;
+    const double *p6 = P[w.x].cl6 + (size_t)zl * d.nrow6, *m6 = M[w.x].cl6 + (size_t)zl * d.nrow6;
+    for (int r = threadIdx.x; r < d.nrow6; r += 256) {
+      const double c0 = c6[r], sp = p6[r] - c0, sm = c0 - m6[r];
+      const double s = fabs(sp - sm) <= 0.05 * (fabs(sp) + fabs(sm)) ? 0.5 * (sp + sm)
+                     : (fabs(sp) < fabs(sm) ? sp : sm);
+      acc += cb6[r] * s;
+    }
   }
   const double sum = clh_block_sum(acc, red);
   if (threadIdx.x == 0 && sum != 0.0) atomicAdd(d.g_lens + zl, (dz > 0 ? sum : -sum) / eps);

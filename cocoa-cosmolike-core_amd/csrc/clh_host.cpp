@@ -32,6 +32,8 @@ hipError_t clh_launch_nl_forward(const DrawDesc* D, const int2* wprep, int nprep
                                  const int4* widft, int nidft, hipStream_t s);
 hipError_t clh_launch_nl_cl(const DrawDesc* D, const int2* work, int n, hipStream_t s);
 hipError_t clh_launch_cmb(const DrawDesc* D, const int4* work, int n, const int* wplace, int nplace, hipStream_t s);
+hipError_t clh_launch_cmb_place_adj(const DrawDesc* D, const int* wplace, int nplace, hipStream_t s);
+hipError_t clh_launch_limber_cmb_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_limber_gg(const DrawDesc* D, const int4* work, int n, int f32, int stage_doubles, int keep,
                                 hipStream_t s, int ol = 0);
 hipError_t clh_launch_legendre(const DrawDesc* D, const int4* work, int n, hipStream_t s);
@@ -3027,7 +3029,7 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   for (int i = 0; i < bt.B; i++) bt.desc[i].dv = bt.d_dv.p + bt.dv_off[i];
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
   const bool adj = bt.uniform && bt.nknz > 0 && !(bt.tatt && bt.lays[0]->src.nbin > 10) && !bt.nl_any &&
-                   !bt.lays[0]->sixpt && !bt.hdi_any;  // the VJP refuses these
+                   !bt.hdi_any && !bt.nrow_custom;  // the VJP refuses these
   // unit cotangent, ln P and lens-shift gradients of the adjoint stages (the
   // ln P runs and the lens-limit term are part of what is timed)
   DBuf<double> dvbar, glnp, glens;
@@ -3176,7 +3178,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (bt.tatt && bt.lays[0]->src.nbin > 10)
     return fail(c, CLH_ERR_ARG, "VJP under TATT takes at most 10 source bins (%d)", bt.lays[0]->src.nbin);
   if (bt.nl_any) return fail(c, CLH_ERR_ARG, "VJP of the non-Limber w(theta) is not implemented");
-  if (bt.lays[0]->sixpt) return fail(c, CLH_ERR_ARG, "VJP of the 6x2pt data vector is not implemented");
+  if (bt.nrow_custom) return fail(c, CLH_ERR_ARG, "VJP of a custom-ell batch (no data vector)");
   if (bt.hdi_any) return fail(c, CLH_ERR_ARG, "VJP at high_def_integration != 0 is not implemented");
   const Layout& L = *bt.lays[0];
   if (!ord_ok(L, ord))
@@ -3193,14 +3195,19 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
                n_tan = al((size_t)nl * (6 + ns) * CLH_NP_SS + (size_t)nl * 8 * CLH_NP_GG),
                n_cbb = bt.tatt ? al((size_t)L.n_ss() * nrow) : 0,
                n_tb = bt.tatt ? al((size_t)(1 + nl) * 2 * ns * CLH_NP_SS) : 0;
-  const size_t per = n_cbar + n_raw + n_abar + n_bbar + n_wbar + n_dg + n_tan + n_cbb + n_tb;
+  const bool six = L.sixpt;
+  const int nrow6 = six ? bt.desc[0].nrow6 : 0;
+  const size_t n_c6 = six ? al((size_t)(nl + ns + 1) * nrow6) : 0;
+  const size_t per = n_cbar + n_raw + n_abar + n_bbar + n_wbar + n_dg + n_tan + n_cbb + n_tb + n_c6;
   // the lens-limit part of the lens-shift gradient from the nodes' motion
   // (k_nodes_tan and the LIM adjoint instantiations), unless the
   // finite-difference views are asked for
-  // (one-loop batches take the term from the finite-difference views: the
-  // node-motion tangent does not cover the one-loop terms)
-  const bool lim_tan = g_lens_dz && nl > 0 && c->vjp_lim == 0 && !bt.oneloop && !bt.tatt;
-  const bool lim_fd = g_lens_dz && nl > 0 && (c->vjp_lim == 1 || (c->vjp_lim == 0 && (bt.oneloop || bt.tatt)));
+  // (one-loop, TATT and 6x2pt batches take the term from the
+  // finite-difference views: the node-motion tangent covers neither the
+  // one-loop and TATT terms nor the gk integral's nodes)
+  const bool lim_tan = g_lens_dz && nl > 0 && c->vjp_lim == 0 && !bt.oneloop && !bt.tatt && !six;
+  const bool lim_fd =
+      g_lens_dz && nl > 0 && (c->vjp_lim == 1 || (c->vjp_lim == 0 && (bt.oneloop || bt.tatt || six)));
   const int ol = bt.oneloop ? 1 : 0;
   HIPCHK(c, bt.d_vjp.ensure(per * B));
   // per-node ln P cotangent runs of the ss / gs nodes and the gather work
@@ -3260,6 +3267,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     q.ntan = lim_tan ? v + n_dg : nullptr;
     q.cbar_bb = bt.tatt ? v + n_dg + n_tan : nullptr;
     q.tbar = bt.tatt ? v + n_dg + n_tan + n_cbb : nullptr;
+    q.cbar6 = six ? v + n_dg + n_tan + n_cbb + n_tb : nullptr;
     q.g_src = g_src_dz ? g_src_dz + (size_t)ns * i : g + al(bt.nknz);
     q.g_lens = g_lens_dz ? g_lens_dz + (size_t)nl * i : g + al(bt.nknz) + al(CLH_MAXB);
     q.g_b1 = g_b1 ? g_b1 + (size_t)nl * i : g + al(bt.nknz) + 2 * al(CLH_MAXB);
@@ -3289,9 +3297,12 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     HIPCHK(c, clh_launch_limber_sx(bt.d_desc.p, bt.d_wsx.p, bt.n_wsx_std, 0, 0, s, ol));
     if (bt.n_wsx > bt.n_wsx_std)  // the TATT draws' tiles (EE and BB rows)
       HIPCHK(c, clh_launch_limber_sx_tatt(bt.d_desc.p, bt.d_wsx.p + bt.n_wsx_std, bt.n_wsx - bt.n_wsx_std, s));
+    if (six)  // the C_gk / C_ks / C_kk columns (d/dm reads C_ks; the views' base rows)
+      HIPCHK(c, clh_launch_cmb(bt.d_desc.p, bt.d_wcmb.p, bt.n_wcmb, nullptr, 0, s));
   }
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa4.p, bt.n_wa, s));
+  if (six) HIPCHK(c, clh_launch_cmb_place_adj(bt.d_desc.p, bt.d_wcmbp.p, bt.n_wcmbp, s));
   HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
   if (lim_tan) {
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[11], s));
@@ -3323,6 +3334,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc.p, bt.d_wga.p, bt.n_wga,
                                      (c->gg_lds[1] && !use_ggk) ? bt.gg_stage : 0, use_ggk ? 1 : 0,
                                      lim_tan ? 1 : 0, c->precision != CLH_PREC_FP64 ? 1 : 0, ol, s));
+  // 6x2pt: the gk / ks / kk Limber adjoint (its ln P runs added after the gather)
+  if (six) HIPCHK(c, clh_launch_limber_cmb_adj(bt.d_desc.p, bt.d_wcmb.p, bt.n_wcmb, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[9], s));
   c->vjp_timed = c->timing;
   if (lim_fd) {
@@ -3343,18 +3356,20 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     }
     // (TATT: the views' own B_b, D_b node vectors for their gs tiles)
     const size_t nsz = al(node_size(ns, nl)), csz = al((size_t)ncol * std::max(nrow, bt.desc[0].nrow)),
-                 tsz = bt.tatt ? al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS) : 0;
-    HIPCHK(c, bt.d_lim.ensure((nsz + csz + tsz) * 2 * B));
+                 tsz = bt.tatt ? al((size_t)(1 + nl) * (2 * ns + 1) * CLH_NP_SS) : 0,
+                 vsz = nsz + csz + tsz + n_c6;  // (6x2pt: the views' own cl6 columns)
+    HIPCHK(c, bt.d_lim.ensure(vsz * 2 * B));
     HIPCHK(c, bt.d_desc_lim.ensure(2 * B));
     bt.desc_lim.resize(2 * B);
     for (int i = 0; i < 2 * B; i++) {
       DrawDesc& q = bt.desc_lim[i];
       q = bt.desc[i % B];
       q.lim_eps = i < B ? eps : -eps;
-      q.nodes = bt.d_lim.p + (nsz + csz + tsz) * i;
+      q.nodes = bt.d_lim.p + vsz * i;
       q.cl = q.nodes + nsz;
-      q.dv = nullptr; q.cl_bb = nullptr; q.cl6 = nullptr; q.nlw = nullptr;
+      q.dv = nullptr; q.cl_bb = nullptr; q.nlw = nullptr;
       q.tatt = bt.tatt ? q.cl + csz : nullptr;
+      q.cl6 = six ? q.cl + csz + tsz : nullptr;
       q.ggk = nullptr; q.sxk = nullptr;
     }
     HIPCHK(c, hipMemcpyAsync(bt.d_desc_lim.p, bt.desc_lim.data(), 2 * B * sizeof(DrawDesc), hipMemcpyHostToDevice,
@@ -3365,6 +3380,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
       HIPCHK(c, clh_launch_limber_gg(V, bt.d_wgg.p, bt.n_wgg, 0, gg_st, 0, s, ol));
       HIPCHK(c, clh_launch_limber_sx(V, bt.d_wsx_gs.p, bt.n_wsx_gs, 0, 0, s, ol));
       if (bt.n_wsx_gs_t) HIPCHK(c, clh_launch_limber_sx_tatt(V, bt.d_wsx_gs_t.p, bt.n_wsx_gs_t, s));
+      if (six) HIPCHK(c, clh_launch_cmb(V, bt.d_wcmb.p, bt.n_wcmb, nullptr, 0, s));
     }
     HIPCHK(c, clh_launch_lim_grad(bt.d_desc.p, bt.d_desc_lim.p, bt.d_desc_lim.p + B, bt.d_wlim.p, bt.n_wlim, s));
   }

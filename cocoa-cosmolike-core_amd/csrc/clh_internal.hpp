@@ -194,6 +194,8 @@ struct DrawDesc {
   // otherwise)
   double* cbar_bb;
   double* tbar;
+  // 6x2pt VJP: cotangent of the cl6 columns [nl + ns + 1][nrow6] (null otherwise)
+  double* cbar6;
   // the motion of the gs / gg nodes with the lens photo-z shift (the lens
   // integration limits move with |dz|), per node of lens bin zl, for the VJP
   // (k_nodes_tan), or null: gs [nl][6 + ns][64] = da/d dz, d ln(GL weight x

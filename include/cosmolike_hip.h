@@ -365,8 +365,9 @@ int clh_batch_set_params(clh_ctx* ctx, const double* lnP, const double* src_dz,
  * every non-NULL device pointer, d<dv_bar, dv>/d(input): g_lnP [B][nk*nz],
  * g_src_dz [B][ns], g_lens_dz [B][nl], g_b1 [B][nl], g_ia_A [B], g_m [B][ns].
  * Differentiates the discrete algorithm (no reference counterpart).  Refused
- * (CLH_ERR_ARG) for batches with the non-Limber w, 6x2pt, TATT with more
- * than 10 source bins, more than one layout or high_def_integration != 0.
+ * (CLH_ERR_ARG) for batches with the non-Limber w, TATT with more than 10
+ * source bins, more than one layout, custom ells or high_def_integration
+ * != 0.  With 6x2pt the w_gk, w_ks and C_kk blocks are included.
  * Under TATT the gradient runs through A1 (ia_A, redshift evolution mode 3)
  * with A2, eta2 and b_TA held at their uploaded values. */
 int clh_batch_vjp(clh_ctx* ctx, const int ord[3], const double* dv_bar, double* g_lnP,

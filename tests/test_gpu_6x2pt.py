@@ -3,7 +3,7 @@ probes w_gk(theta), w_ks(theta) (with the CMB beam and pixel filter) and
 C_kk at the Fourier ell bins, the whole six-block real-space data vector in
 two block orders and both Legendre modes, batches with graph replay, FP32
 mode, kk band powers, masks, batches of different 6x2pt layouts, and the
-VJP refusal.
+switch back to 3x2pt (the VJP is in test_gpu_vjp.py).
 
 Bar: 1e-6 relative per entry with a floor of 1e-3 of the block row's largest
 magnitude (the FP64 parity metric); FP32 mode 1e-4.
@@ -165,17 +165,10 @@ def test_6x2pt_kk_bandpower(six, binning):
     assert rel_err(rev.reshape(1, -1), oq.datavector_6x2pt(ells, (3, 4, 5, 0, 1, 2)).reshape(1, -1)) < RTOL
 
 
-def test_6x2pt_vjp_refused(six):
-    import torch
+def test_6x2pt_back_to_3x2pt(six):
     p, o, ells, ref = six
     c = Cosmolike().load(p)
-    c.batch_upload([p])
-    n = c.datavector_size()
-    _forward(c, n, 1)
-    bar = torch.ones(n, dtype=torch.float64, device="cuda:0")
-    g = torch.zeros(p.ns, dtype=torch.float64, device="cuda:0")
-    with pytest.raises(CosmolikeError, match="6x2pt"):
-        c.batch_vjp(bar.data_ptr(), g_src_dz=g.data_ptr())
+    assert c.datavector_size() == len(ref)
     c.init_probes("3x2pt")
     assert c.datavector_size() == len(ref) - (p.nl + p.ns) * p.ntheta - len(ells)
 

@@ -332,9 +332,15 @@ def _groups_vs_fd(c, P, v, B, tag):
                 dm = _fwd(c, dict(P, **{group: xm.reshape(x0.shape)}))
             fd[:, j] = ((dp - dm) * v).sum(dim=1).cpu().numpy() / (2 * h)
         _check(g[group].reshape(fd.shape), fd, tag + " " + group)
+    _lnP_dir_vs_fd(c, P, v, g, tag)
+
+
+def _lnP_dir_vs_fd(c, P, v, g, tag, h=1e-4):
+    """The ln P gradient g along a random direction against the central
+    difference of v . forward (step h)."""
+    import torch
     rng = np.random.default_rng(10)
     u = torch.tensor(rng.normal(size=P["lnP"].shape), dtype=torch.float64, device=P["lnP"].device)
-    h = 1e-4
     with torch.no_grad():
         fd = ((_fwd(c, dict(P, lnP=P["lnP"] + h * u)) - _fwd(c, dict(P, lnP=P["lnP"] - h * u))) * v).sum(dim=1)
     _check((g["lnP"] * u.cpu().numpy()).sum(axis=1), fd.cpu().numpy() / (2 * h), tag + " lnP")
@@ -375,3 +381,85 @@ def test_vjp_tatt_vs_gpu_fd():
         floor = 1e-3 * np.abs(ref).max()
         assert (np.abs(out[i] - ref) / np.maximum(np.abs(ref), floor)).max() < 1e-6
     _groups_vs_fd(c, P, v, len(draws), "tatt")
+
+
+def _sixpt_draws(seed, bp=None):
+    draws = []
+    for i, (cos, *_r) in enumerate(synth.random_draws(2, seed=seed, ragged=False)):
+        p = synth.sixpt_problem(cos)
+        rng = np.random.default_rng(seed + 10 * i)
+        p.src_dz = rng.normal(0, 0.01, p.ns)
+        p.lens_dz = rng.normal(0, 0.005, p.nl)
+        p.shear_m = rng.normal(0, 0.01, p.ns)
+        p.ia_A = 0.5 + 0.2 * i
+        if i == 1:
+            p.bmag = rng.normal(0, 1.0, p.nl)
+        if bp is not None:
+            p.kk_bp = bp
+        draws.append(p)
+    return draws
+
+
+def _sixpt_setup(draws, mask_seed=None):
+    import torch
+    c = Cosmolike(device=0).load(draws[0])
+    if mask_seed is not None:
+        n = c.datavector_size()
+        mask = (np.random.default_rng(mask_seed).random(n) > 0.2).astype(np.int32)
+        c.like_set_mask(mask)
+    c.batch_upload(draws)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.array(a), dtype=torch.float64, device=dev)
+    P = dict(lnP=t([d.lnP for d in draws]), src_dz=t([d.src_dz for d in draws]),
+             lens_dz=t([d.lens_dz for d in draws]), b1=t([d.b1 for d in draws]),
+             ia_A=t([d.ia_A for d in draws]), shear_m=t([d.shear_m for d in draws]))
+    v = t(np.random.default_rng(46).normal(size=(len(draws), c.datavector_size())))
+    return c, P, v
+
+
+def test_vjp_6x2pt_vs_gpu_fd():
+    """The VJP of the 6x2pt data vector: the w_gk, w_ks and C_kk blocks back
+    through the CMB-filtered kernels and C_kk(ell) to the C_gk / C_ks / C_kk
+    rows, and through their Limber integrands to ln P, the lens shifts (W_gal,
+    W_mag; the gk limits' motion from the finite-difference views), b1, the
+    source shifts and A_IA (C_ks) and m (w_ks).  The forward against the
+    oracle first; then every scalar group and a ln P direction against
+    differences of the GPU forward at 1e-4, magnification bias on one draw."""
+    draws = _sixpt_draws(31)
+    c, P, v = _sixpt_setup(draws)
+    ells = synth.ells_fourier(draws[0].kk_nells, draws[0].kk_lmin, draws[0].kk_lmax)
+    out = _fwd(c, P).detach().cpu().numpy()
+    for i, d in enumerate(draws):
+        ref = O.Oracle().load(d).datavector_6x2pt(ells)
+        floor = 1e-3 * np.abs(ref).max()
+        assert (np.abs(out[i] - ref) / np.maximum(np.abs(ref), floor)).max() < 1e-6
+    _groups_vs_fd(c, P, v, len(draws), "6x2pt")
+    # cotangents on the CMB blocks alone (w_gk, w_ks, C_kk after the 3x2pt
+    # blocks), and on C_kk alone (ln P only), so that the small C_kk entries
+    # are not hidden behind the others.  The C_kk integral runs to z ~ 1000,
+    # far past the P(k,z) table (z <= 3), where the bilinear z weights
+    # extrapolate with |dz| in the hundreds: a random ln P step of 1e-4 moves
+    # ln P there by 0.01-0.1 and the central difference carries that
+    # curvature, so its step is 1e-6.
+    p = draws[0]
+    n3 = (p.ns * (p.ns + 1) + p.ns * p.nl + p.nl) * p.ntheta
+    v6 = v.clone()
+    v6[:, :n3] = 0.0
+    _groups_vs_fd(c, P, v6, len(draws), "6x2pt CMB blocks")
+    vkk = v.clone()
+    vkk[:, :n3 + (p.nl + p.ns) * p.ntheta] = 0.0
+    _lnP_dir_vs_fd(c, P, vkk, _vjp(c, P, vkk), "6x2pt kk", h=1e-6)
+
+
+def test_vjp_6x2pt_bandpowers_mask_vs_gpu_fd():
+    """As above with kk band powers (init_cmb_auto_bandpower: the cotangent of
+    every band spread over C_kk(L), L = lmin..lmax, through the direct rows
+    below LMIN_tab and the table rows above) and a mask over every block
+    (masked entries carry no cotangent)."""
+    draws = _sixpt_draws(37, bp=synth.kk_bandpower_binning())
+    c, P, v = _sixpt_setup(draws, mask_seed=8)
+    _groups_vs_fd(c, P, v, len(draws), "6x2pt bp")
+    nbp = draws[0].kk_bp[0]
+    vkk = v.clone()
+    vkk[:, :-nbp] = 0.0
+    _lnP_dir_vs_fd(c, P, vkk, _vjp(c, P, vkk), "6x2pt bp kk", h=1e-6)
