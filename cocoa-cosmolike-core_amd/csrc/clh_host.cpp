@@ -31,6 +31,9 @@ hipError_t clh_launch_nl_forward(const DrawDesc* D, const int2* wprep, int nprep
                                  const int4* wgl, int ngl, const int4* wchk, int nchk, const int2* wbin, int nbin,
                                  const int4* widft, int nidft, hipStream_t s);
 hipError_t clh_launch_nl_cl(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_nl_cl_adj(const DrawDesc* D, const int2* wbin, int nbin, hipStream_t s);
+hipError_t clh_launch_nl_tangent(const DrawDesc* D, const int2* wprep, int nprep, const int4* wfwd4, int nfwd4,
+                                 const int4* widft, int nidft, hipStream_t s);
 hipError_t clh_launch_cmb(const DrawDesc* D, const int4* work, int n, const int* wplace, int nplace, hipStream_t s);
 hipError_t clh_launch_cmb_place_adj(const DrawDesc* D, const int* wplace, int nplace, hipStream_t s);
 hipError_t clh_launch_limber_cmb_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s);
@@ -388,20 +391,22 @@ struct Batch {
   // 1..149 with P_nl and with P_lin) and the FFTLog work lists
   bool nl_any = false;
   std::vector<DrawDesc> desc_nl;
-  DBuf<DrawDesc> d_desc_nl;
+  DBuf<DrawDesc> d_desc_nl, d_desc_nlv;
   DBuf<double> d_nl_ells;
   DBuf<int2> d_wn_nl, d_wnl_prep, d_wnl_cl;
-  DBuf<int4> d_wgg_nl, d_wnl_fwd, d_wnl_gl, d_wnl_chk, d_wnl_idft;
-  int n_wn_nl = 0, n_wgg_nl = 0, n_wnl_prep = 0, n_wnl_fwd = 0, n_wnl_gl = 0, n_wnl_chk = 0, n_wnl_idft = 0,
-      n_wnl_cl = 0;
+  DBuf<int4> d_wgg_nl, d_wnl_fwd, d_wnl_fwd4, d_wnl_gl, d_wnl_chk, d_wnl_idft;
+  int n_wn_nl = 0, n_wgg_nl = 0, n_wnl_prep = 0, n_wnl_fwd = 0, n_wnl_fwd4 = 0, n_wnl_gl = 0, n_wnl_chk = 0,
+      n_wnl_idft = 0, n_wnl_cl = 0;
+  std::vector<DrawDesc> desc_nlv;  // the views' descriptors with the VJP's buffers
   int ord[3] = {0, 1, 2};
   void release() {
     d_desc.release(); d_par.release(); d_tab.release(); d_work.release(); d_dv.release(); d_ells.release();
     d_vjp.release(); d_gscr.release(); d_run.release(); d_wlg.release(); d_ggk.release(); d_sxk.release(); d_dgk.release();
     d_wg.release(); d_wn.release(); d_wl2.release(); d_wl.release(); d_wa4.release();
     d_wsx.release(); d_wgg.release(); d_wga.release(); d_wmx.release(); d_wlf.release();
-    d_desc_nl.release(); d_nl_ells.release(); d_wn_nl.release(); d_wnl_prep.release(); d_wnl_cl.release();
-    d_wgg_nl.release(); d_wnl_fwd.release(); d_wnl_gl.release(); d_wnl_chk.release(); d_wnl_idft.release();
+    d_desc_nl.release(); d_desc_nlv.release(); d_nl_ells.release(); d_wn_nl.release(); d_wnl_prep.release(); d_wnl_cl.release();
+    d_wgg_nl.release(); d_wnl_fwd.release(); d_wnl_fwd4.release(); d_wnl_gl.release(); d_wnl_chk.release();
+    d_wnl_idft.release();
     d_wcmb.release(); d_wcmbp.release();
     d_desc_lim.release(); d_lim.release(); d_wsx_gs.release(); d_wsx_gs_t.release(); d_wlim.release();
     d_whd.release();
@@ -1253,7 +1258,7 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   // stages per draw (node blocks; bin x configuration x m blocks;
   // configuration x m blocks; the ten block-end ells; every ell; bins)
   std::vector<int2> wn_nl, wnl_prep, wnl_cl;
-  std::vector<int4> wgg_nl, wnl_fwd, wnl_gl, wnl_chk, wnl_idft;
+  std::vector<int4> wgg_nl, wnl_fwd, wnl_fwd4, wnl_gl, wnl_chk, wnl_idft;
   int v = 0;
   for (int i = 0; i < bt.B; i++) {
     if (!bt.desc[i].nlw) continue;
@@ -1269,6 +1274,8 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
     const int nmb = (CLH_NL_NH + 255) / 256;
     for (int zl = 0; zl < nl; zl++)
       for (int j = 0; j < 3; j++) wnl_fwd.push_back(make_int4(i, zl, j, 0));
+    for (int zl = 0; zl < nl; zl++)  // the VJP's four tangent sets per bin
+      for (int j = 0; j < 4; j++) wnl_fwd4.push_back(make_int4(i, zl, j, 0));
     for (int j = 0; j < 3; j++)
       for (int mb = 0; mb < nmb; mb++) wnl_gl.push_back(make_int4(i, j, mb, 0));
     for (int r = 0; r < 10; r++) wnl_chk.push_back(make_int4(i, r, 0, 0));  // the block-end ells
@@ -1279,6 +1286,7 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
   HIPCHK(c, up4(bt.d_wgg_nl, wgg_nl, bt.n_wgg_nl));
   HIPCHK(c, up2(bt.d_wnl_prep, wnl_prep, bt.n_wnl_prep));
   HIPCHK(c, up4(bt.d_wnl_fwd, wnl_fwd, bt.n_wnl_fwd));
+  HIPCHK(c, up4(bt.d_wnl_fwd4, wnl_fwd4, bt.n_wnl_fwd4));
   HIPCHK(c, up4(bt.d_wnl_gl, wnl_gl, bt.n_wnl_gl));
   HIPCHK(c, up4(bt.d_wnl_chk, wnl_chk, bt.n_wnl_chk));
   HIPCHK(c, up4(bt.d_wnl_idft, wnl_idft, bt.n_wnl_idft));
@@ -3177,7 +3185,10 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   if (!bt.uniform) return fail(c, CLH_ERR_ARG, "VJP needs a batch with one layout for every draw");
   if (bt.tatt && bt.lays[0]->src.nbin > 10)
     return fail(c, CLH_ERR_ARG, "VJP under TATT takes at most 10 source bins (%d)", bt.lays[0]->src.nbin);
-  if (bt.nl_any) return fail(c, CLH_ERR_ARG, "VJP of the non-Limber w(theta) is not implemented");
+  if (bt.nl_any && (bt.oneloop || bt.tatt || bt.lays[0]->sixpt || c->vjp_lim == 1))
+    return fail(c, CLH_ERR_ARG,
+                "VJP of the non-Limber w(theta) together with one-loop bias, TATT or 6x2pt, or with the "
+                "finite-difference lens limits, is not implemented");
   if (bt.nrow_custom) return fail(c, CLH_ERR_ARG, "VJP of a custom-ell batch (no data vector)");
   if (bt.hdi_any) return fail(c, CLH_ERR_ARG, "VJP at high_def_integration != 0 is not implemented");
   const Layout& L = *bt.lays[0];
@@ -3198,7 +3209,12 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   const bool six = L.sixpt;
   const int nrow6 = six ? bt.desc[0].nrow6 : 0;
   const size_t n_c6 = six ? al((size_t)(nl + ns + 1) * nrow6) : 0;
-  const size_t per = n_cbar + n_raw + n_abar + n_bbar + n_wbar + n_dg + n_tan + n_cbb + n_tb + n_c6;
+  // non-Limber w: the tangent area, the two views' row cotangents and (with
+  // the node tangents) the views' own node tangents
+  const bool nlv = bt.nl_any;
+  const size_t n_nlt = nlv ? clh_nl_tan(nl).total : 0, n_nlc = nlv ? al((size_t)2 * ncol * (CLH_NL_L - 1)) : 0;
+  const size_t per0 = n_cbar + n_raw + n_abar + n_bbar + n_wbar + n_dg + n_tan + n_cbb + n_tb + n_c6;
+  const size_t per = per0 + n_nlt + n_nlc + (nlv ? 2 * n_tan : 0);
   // the lens-limit part of the lens-shift gradient from the nodes' motion
   // (k_nodes_tan and the LIM adjoint instantiations), unless the
   // finite-difference views are asked for
@@ -3268,6 +3284,9 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     q.cbar_bb = bt.tatt ? v + n_dg + n_tan : nullptr;
     q.tbar = bt.tatt ? v + n_dg + n_tan + n_cbb : nullptr;
     q.cbar6 = six ? v + n_dg + n_tan + n_cbb + n_tb : nullptr;
+    double* vn = bt.d_vjp.p + per * i + per0;  // (non-Limber buffers after the others)
+    q.nlt = nlv ? vn : nullptr;
+    q.nlc_bar = nlv ? vn + n_nlt : nullptr;
     q.g_src = g_src_dz ? g_src_dz + (size_t)ns * i : g + al(bt.nknz);
     q.g_lens = g_lens_dz ? g_lens_dz + (size_t)nl * i : g + al(bt.nknz) + al(CLH_MAXB);
     q.g_b1 = g_b1 ? g_b1 + (size_t)nl * i : g + al(bt.nknz) + 2 * al(CLH_MAXB);
@@ -3276,6 +3295,33 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   }
   hipStream_t s = c->stream;
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), B * sizeof(DrawDesc), hipMemcpyHostToDevice, s));
+  if (nlv) {
+    // the non-Limber views (P_nl, then P_lin, per draw) with the VJP's
+    // buffers: their gg rows at ell = 1..149 have their own cotangents; their
+    // gg nodes sit where the draw's do, so the node cotangents, b1, lens
+    // shift and (P_nl view) ln P go to the draw's accumulators
+    if (bt.desc_nl.size() != (size_t)2 * B) return fail(c, CLH_ERR_STATE, "non-Limber views missing");
+    bt.desc_nlv.assign(bt.desc_nl.begin(), bt.desc_nl.end());
+    for (int i = 0; i < B; i++)
+      for (int lin = 0; lin < 2; lin++) {
+        const DrawDesc& q = bt.desc[i];
+        DrawDesc& u = bt.desc_nlv[2 * i + lin];
+        u.cbar = q.nlc_bar + (size_t)lin * ncol * (CLH_NL_L - 1);
+        u.wbar = q.wbar;
+        u.abar = q.abar;
+        u.bbar = q.bbar;
+        u.run = lin ? nullptr : q.run;  // (the P_lin rows carry no ln P cotangent)
+        u.g_lnP = q.g_lnP;
+        u.g_src = q.g_src; u.g_lens = q.g_lens; u.g_b1 = q.g_b1; u.g_A = q.g_A; u.g_m = q.g_m;
+        u.dgtab = q.dgtab;
+        u.ntan = lim_tan ? q.nlc_bar + n_nlc + (size_t)lin * n_tan : nullptr;  // (their own z weights)
+        u.dvbar = nullptr; u.raw = nullptr; u.ggk = nullptr; u.sxk = nullptr;
+        u.cbar_bb = nullptr; u.tbar = nullptr; u.cbar6 = nullptr; u.nlt = nullptr; u.nlc_bar = nullptr;
+      }
+    HIPCHK(c, bt.d_desc_nlv.ensure(2 * B));
+    HIPCHK(c, hipMemcpyAsync(bt.d_desc_nlv.p, bt.desc_nlv.data(), 2 * B * sizeof(DrawDesc), hipMemcpyHostToDevice,
+                             s));
+  }
   // zero the accumulators (fill kernels, not memsets: safe under graph capture)
   HIPCHK(c, clh_launch_fill(bt.d_vjp.p, per * B, 0.0, s));
   HIPCHK(c, clh_launch_fill(bt.d_gscr.p, gper * B, 0.0, s));
@@ -3299,14 +3345,24 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
       HIPCHK(c, clh_launch_limber_sx_tatt(bt.d_desc.p, bt.d_wsx.p + bt.n_wsx_std, bt.n_wsx - bt.n_wsx_std, s));
     if (six)  // the C_gk / C_ks / C_kk columns (d/dm reads C_ks; the views' base rows)
       HIPCHK(c, clh_launch_cmb(bt.d_desc.p, bt.d_wcmb.p, bt.n_wcmb, nullptr, 0, s));
+    if (nlv) {  // the non-Limber views' nodes and rows, the FFTLog stages and the LMAX switch
+      HIPCHK(c, clh_launch_nodes(bt.d_desc_nl.p, bt.d_wn_nl.p, bt.n_wn_nl, s));
+      HIPCHK(c, clh_launch_limber_gg(bt.d_desc_nl.p, bt.d_wgg_nl.p, bt.n_wgg_nl, 0, 0, 0, s, 0));
+      HIPCHK(c, clh_launch_nl_forward(bt.d_desc.p, bt.d_wnl_prep.p, bt.n_wnl_prep, bt.d_wnl_fwd.p, bt.n_wnl_fwd,
+                                      bt.d_wnl_gl.p, bt.n_wnl_gl, bt.d_wnl_chk.p, bt.n_wnl_chk, bt.d_wnl_cl.p,
+                                      bt.n_wnl_cl, bt.d_wnl_idft.p, bt.n_wnl_idft, s));
+    }
   }
   // reverse sweep
   HIPCHK(c, clh_launch_leg_adjoint(bt.d_desc.p, bt.d_wa4.p, bt.n_wa, s));
   if (six) HIPCHK(c, clh_launch_cmb_place_adj(bt.d_desc.p, bt.d_wcmbp.p, bt.n_wcmbp, s));
+  // non-Limber w: its cotangent onto the gg rows, the views' rows and the FFTLog sums
+  if (nlv) HIPCHK(c, clh_launch_nl_cl_adj(bt.d_desc.p, bt.d_wnl_cl.p, bt.n_wnl_cl, s));
   HIPCHK(c, clh_launch_m_grad(bt.d_desc.p, B, s));
   if (lim_tan) {
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[11], s));
     HIPCHK(c, clh_launch_nodes_tan(bt.d_desc.p, bt.d_wn.p, bt.n_wn, s));
+    if (nlv) HIPCHK(c, clh_launch_nodes_tan(bt.d_desc_nlv.p, bt.d_wn_nl.p, bt.n_wn_nl, s));
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[12], s));
   }
   c->vjp_lim_tan = lim_tan;
@@ -3336,6 +3392,11 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
                                      lim_tan ? 1 : 0, c->precision != CLH_PREC_FP64 ? 1 : 0, ol, s));
   // 6x2pt: the gk / ks / kk Limber adjoint (its ln P runs added after the gather)
   if (six) HIPCHK(c, clh_launch_limber_cmb_adj(bt.d_desc.p, bt.d_wcmb.p, bt.n_wcmb, s));
+  if (nlv) {  // non-Limber w: the views' gg rows (FP64), then the FFTLog sums' tangents
+    HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc_nlv.p, bt.d_wgg_nl.p, bt.n_wgg_nl, 0, 0, lim_tan ? 1 : 0, 0, 0, s));
+    HIPCHK(c, clh_launch_nl_tangent(bt.d_desc.p, bt.d_wnl_prep.p, bt.n_wnl_prep, bt.d_wnl_fwd4.p, bt.n_wnl_fwd4,
+                                    bt.d_wnl_idft.p, bt.n_wnl_idft, s));
+  }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[9], s));
   c->vjp_timed = c->timing;
   if (lim_fd) {

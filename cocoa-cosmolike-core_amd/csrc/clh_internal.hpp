@@ -82,6 +82,26 @@ inline ClhNlOff clh_nl_off(int nl) {
   return o;
 }
 
+// VJP tangent area of a non-Limber draw (DrawDesc::nlt): FFTLog inputs of
+// the two tangent directions per lens bin (sets 0-2: d/d dz of the density,
+// RSD and magnification inputs; set 3: d/d b1 of the density input), their
+// transforms, and the cotangent of sum_q F^2 k^3 P_lin per (bin, ell)
+struct ClhNlTan {
+  size_t fb, T, sbar, total;
+};
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline ClhNlTan clh_nl_tan(int nl) {
+  ClhNlTan o;
+  size_t p = 0;
+  o.fb = p; p += (size_t)nl * 4 * 512;
+  o.T = p; p += (size_t)nl * 4 * CLH_NL_NH * 2;
+  o.sbar = p; p += (size_t)nl * CLH_NL_LT;
+  o.total = (p + 31) & ~(size_t)31;
+  return o;
+}
+
 // Bump allocator over device memory the caller provides (the allocation-free
 // entry point's scratch array)
 struct ClhArena {
@@ -196,6 +216,11 @@ struct DrawDesc {
   double* tbar;
   // 6x2pt VJP: cotangent of the cl6 columns [nl + ns + 1][nrow6] (null otherwise)
   double* cbar6;
+  // non-Limber VJP (null otherwise): the tangent area (ClhNlTan) and the
+  // cotangents of the two views' C_ell rows at ell = 1..149, [2][ncol][149]
+  // (P_nl view, then P_lin view; only the gg columns are used)
+  double* nlt;
+  double* nlc_bar;
   // the motion of the gs / gg nodes with the lens photo-z shift (the lens
   // integration limits move with |dz|), per node of lens bin zl, for the VJP
   // (k_nodes_tan), or null: gs [nl][6 + ns][64] = da/d dz, d ln(GL weight x

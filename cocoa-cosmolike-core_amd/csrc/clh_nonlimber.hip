@@ -196,16 +196,21 @@ __global__ __launch_bounds__(256) void k_nl_prep(const DrawDesc* __restrict__ D,
 // transform of length N as a complex Stockham FFT of length N / 2 in LDS
 // (conj of the + sign transform of the real input).
 // ----------------------------------------------------------------------------
+// TAN: the VJP's tangent inputs (DrawDesc::nlt, ClhNlTan), work item (draw,
+// bin, set): sets 0-2 in configurations 0-2, set 3 in configuration 0.
+template <bool TAN>
 __global__ __launch_bounds__(256) void k_nl_fwd(const DrawDesc* __restrict__ D, const int4* __restrict__ work) {
   __shared__ clh_c2 buf[2][CLH_NL_NH - 2];
   __shared__ clh_c2 twl[CLH_NL_TWS], twh[CLH_NL_TWS];
   const int4 w = work[blockIdx.x];
   const DrawDesc& d = D[w.x];
-  const int i = w.y, j = w.z, tid = threadIdx.x;
+  const int i = w.y, set = w.z, j = TAN ? (set < 3 ? set : 0) : set, tid = threadIdx.x;
   const ClhNlOff off = clh_nl_off(d.nl);
+  const ClhNlTan ot = clh_nl_tan(d.nl);
   const int N = d.nl_N[j], N0 = d.nl_N0[j], M = N / 2, nh = N / 2 + 1;
   const double* tw = d.nl_tw + (j == 0 ? 0 : (j == 1 ? 2 * d.nl_N[0] : 2 * (d.nl_N[0] + d.nl_N[1])));
-  clh_c2* T = reinterpret_cast<clh_c2*>(d.nlw + off.T) + (size_t)(i * 3 + j) * CLH_NL_NH;
+  clh_c2* T = TAN ? reinterpret_cast<clh_c2*>(d.nlt + ot.T) + (size_t)(i * 4 + set) * CLH_NL_NH
+                  : reinterpret_cast<clh_c2*>(d.nlw + off.T) + (size_t)(i * 3 + j) * CLH_NL_NH;
   if (j == 2 && !clh_nl_mag(d)) {
     for (int m = tid; m < CLH_NL_NH; m += 256) T[m] = clh_c2{0.0, 0.0};
     return;
@@ -215,7 +220,7 @@ __global__ __launch_bounds__(256) void k_nl_fwd(const DrawDesc* __restrict__ D, 
     const int tt = (hi ? t * CLH_NL_TWS : t) % M;
     (hi ? twh : twl)[t] = clh_c2{tw[4 * tt], tw[4 * tt + 1]};
   }
-  const double* fb = d.nlw + off.fb + (i * 3 + j) * 512;
+  const double* fb = TAN ? d.nlt + ot.fb + (i * 4 + set) * 512 : d.nlw + off.fb + (i * 3 + j) * 512;
   for (int n = tid; n < M; n += 256) {
     const int a = 2 * n - N0, b = 2 * n + 1 - N0;
     buf[0][n] = clh_c2{(a >= 0 && a < CLH_NL_NCHI) ? fb[a] : 0.0, (b >= 0 && b < CLH_NL_NCHI) ? fb[b] : 0.0};
@@ -321,17 +326,25 @@ __device__ __forceinline__ int clh_fft_nst(int M) {
 // Two halves of 128 threads run the density (j = 0) and RSD (j = 1)
 // transforms of a bin side by side, with barriers shared; the magnification
 // transform (j = 2) follows on the second half.
-template <bool CHECK>
+// TAN (the VJP): per bin whose ell carries a cotangent sbar of the sum,
+// three passes: F from the forward's transforms, then F_z from the lens
+// shift's tangent transforms (sets 0-2) and F_b from the b1 tangent (set 3,
+// density only); sbar sum_q 2 F F_z k^3 P_lin goes to g_lens, sbar sum_q
+// 2 F F_b k^3 P_lin to g_b1 (the sum is linear in each transformed input).
+template <bool CHECK, bool TAN = false>
 __global__ __launch_bounds__(256) void k_nl_fft(const DrawDesc* __restrict__ D, const int4* __restrict__ work) {
+  static_assert(!(CHECK && TAN), "the tangent passes run at the transformed ells");
   __shared__ clh_c2 fbuf[2 * (CLH_NL_NH - 2) + 2 * 450];  // per half two buffers of its M (M0 + M1 <= 1200)
   __shared__ clh_c2 twl[2][CLH_NL_TWS], twh[2][CLH_NL_TWS];  // per M (configuration 0; 1 and 2)
   __shared__ double kP[CLH_NL_NCHI];
   __shared__ double Fh[2][512];  // the linear P column at z = 0 first, then F(q) per half
   __shared__ double red[4];
+  __shared__ double Fk[TAN ? 512 : 1];  // TAN: F of the first pass
   const int4 w = work[clh_xcd_item(blockIdx.x, gridDim.x)];
   const DrawDesc& d = D[w.x];
   const int r = w.y, l = CHECK ? clh_nl_row(true, 0, r) : r;
   const ClhNlOff off = clh_nl_off(d.nl);
+  const ClhNlTan ot = clh_nl_tan(d.nl);
   const int tid = threadIdx.x, h = tid >> 7, ht = tid & 127;
   if (!CHECK) {  // nothing to do when every bin switched to Limber at or below l
     bool any = false;
@@ -386,18 +399,23 @@ __global__ __launch_bounds__(256) void k_nl_fft(const DrawDesc* __restrict__ D, 
   clh_c2* mybuf = h == 0 ? fbuf : fbuf + 2 * M0;  // this half's two buffers
   for (int i = 0; i < d.nl; i++) {
     if (!CHECK && l >= (int)d.nlw[off.lmx + i]) continue;  // uniform over the block
-    __syncthreads();  // kP / the previous bin's sum done with Fh
+    const double sbar = TAN ? d.nlt[ot.sbar + (size_t)i * CLH_NL_LT + l] : 0.0;
+    if (TAN && sbar == 0.0) continue;  // (uniform)
+    for (int pass = 0; pass < (TAN ? 3 : 1); pass++) {
+    __syncthreads();  // kP / the previous bin's (pass's) sum done with Fh
     for (int q = ht; q < 512; q += 128) Fh[h][q] = 0.0;
     const double bmag = d.par->bmag[i];
-    for (int round = 0; round < (mag ? 2 : 1); round++) {
+    for (int round = 0; round < (pass == 2 ? 1 : (mag ? 2 : 1)); round++) {
       const int j = round == 0 ? h : 2;
-      const bool act = round == 0 || h == 1;
+      const bool act = pass == 2 ? h == 0 : (round == 0 || h == 1);
       const int N = d.nl_N[j], N0 = d.nl_N0[j], M = N / 2, c = j == 0 ? 0 : 1;
       const double* tw = d.nl_tw + (j == 0 ? 0 : (j == 1 ? 2 * d.nl_N[0] : 2 * (d.nl_N[0] + d.nl_N[1])));
       clh_c2* b0 = mybuf;
       clh_c2* b1 = mybuf + M;
       if (act) {
-        const clh_c2* T = Tall + (size_t)(i * 3 + j) * CLH_NL_NH;
+        const clh_c2* T = pass == 0 ? Tall + (size_t)(i * 3 + j) * CLH_NL_NH
+                                    : reinterpret_cast<const clh_c2*>(d.nlt + ot.T) +
+                                          (size_t)(i * 4 + (pass == 1 ? j : 3)) * CLH_NL_NH;
         const clh_c2* G = Gall + ((size_t)j * CLH_NL_LT + l) * CLH_NL_NH;
         for (int k = ht; k < M; k += 128) {
           const clh_c2 a = clh_c2_mul(T[k], G[k]), b = clh_c2_mul(T[M - k], G[M - k]);
@@ -445,16 +463,28 @@ __global__ __launch_bounds__(256) void k_nl_fft(const DrawDesc* __restrict__ D, 
       }
       __syncthreads();  // this round's readers done with the buffers, F complete
     }
-    double s = 0.0;
-    for (int q = tid; q < CLH_NL_NCHI; q += 256) {
-      const double F = Fh[0][q] + Fh[1][q];
-      s += F * F * kP[q];
+    if constexpr (TAN) {
+      if (pass == 0) {
+        for (int q = tid; q < CLH_NL_NCHI; q += 256) Fk[q] = Fh[0][q] + Fh[1][q];
+      } else {  // (each thread reads the Fk entries it wrote)
+        double s = 0.0;
+        for (int q = tid; q < CLH_NL_NCHI; q += 256) s += 2.0 * Fk[q] * (Fh[0][q] + Fh[1][q]) * kP[q];
+        s = clh_block_sum(s, red);
+        if (tid == 0 && s != 0.0) atomicAdd(pass == 1 ? d.g_lens + i : d.g_b1 + i, sbar * s);
+      }
+    } else {
+      double s = 0.0;
+      for (int q = tid; q < CLH_NL_NCHI; q += 256) {
+        const double F = Fh[0][q] + Fh[1][q];
+        s += F * F * kP[q];
+      }
+      s = clh_block_sum(s, red);
+      if (tid == 0) {
+        double* part = d.nlw + (CHECK ? off.partc : off.part);
+        part[(size_t)i * (CHECK ? 16 : CLH_NL_LT) + (CHECK ? r : l)] = s;
+      }
     }
-    s = clh_block_sum(s, red);
-    if (tid == 0) {
-      double* part = d.nlw + (CHECK ? off.partc : off.part);
-      part[(size_t)i * (CHECK ? 16 : CLH_NL_LT) + (CHECK ? r : l)] = s;
-    }
+    }  // pass
   }
 }
 
@@ -543,12 +573,132 @@ __global__ __launch_bounds__(256) void k_nl_cl(const DrawDesc* __restrict__ D, c
   }
 }
 
+// ----------------------------------------------------------------------------
+// Adjoint (VJP) of the non-Limber w.
+// k_nl_prep_tan: the tangent FFTLog inputs (ClhNlTan sets) at the chi nodes
+// of k_nl_prep: d/d dz of the density b1 n D H/H0 chi, RSD -f n D H/H0 chi
+// and magnification W_mag D / chi / (c/H0)^2 inputs (n(z - dz) and g_lens
+// through their shift tangents), and d/d b1 of the density input.  The bin's
+// z range (l_zmin .. l_zmax) does not move with dz.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_nl_prep_tan(const DrawDesc* __restrict__ D, const int2* __restrict__ work) {
+  const int2 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const ClhNlOff off = clh_nl_off(d.nl);
+  const ClhNlTan ot = clh_nl_tan(d.nl);
+  double* fb = d.nlt + ot.fb;
+  const int q = w.y * 256 + threadIdx.x;
+  if (q >= 512) return;
+  if (q >= CLH_NL_NCHI) {
+    for (int i = 0; i < d.nl; i++)
+      for (int j = 0; j < 4; j++) fb[(i * 4 + j) * 512 + q] = 0.0;
+    return;
+  }
+  const double rc = d.nlw[off.scal + 2];
+  const double x = d.nlw[off.x + q], xnu1 = d.nlw[off.xnu + q];
+  const double chi = x / rc;
+  const double a = clh_a_chi(d, chi);
+  const double z = 1. / a - 1.;
+  double cc, dc;
+  clh_chi_all(d, a, cc, dc);
+  const double hh = 1.0 / ((a * a) * dc);
+  const double Dg = clh_growfac(d, a), f = clh_f_growth(d, z);
+  for (int i = 0; i < d.nl; i++) {
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, tb = 0.0;
+    if (!(z < d.l_zmin[i] || z > d.l_zmax[i])) {
+      const double pf = clh_nz_lens(d, z, i), dpf = clh_dnz_lens(d, z, i);
+      const double dWM = 1.5 * d.omega_m * chi / a * clh_dg_lens(d, a, i);
+      t0 = chi * dpf * Dg * hh * d.par->b1[i];
+      t1 = -chi * dpf * Dg * hh * f;
+      t2 = (dWM / chi / (rc * rc)) * Dg;
+      tb = chi * pf * Dg * hh;
+    }
+    fb[(i * 4 + 0) * 512 + q] = t0 / x;
+    fb[(i * 4 + 1) * 512 + q] = t1 / xnu1;
+    fb[(i * 4 + 2) * 512 + q] = t2 / x;
+    fb[(i * 4 + 3) * 512 + q] = tb / x;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// k_nl_cl_adj: adjoint of k_nl_cl per (draw, lens bin), thread = ell.  With u
+// the masked w cotangent of the bin, dL[l] = sum_theta Pw[theta][l] u[theta]
+// is the cotangent of DL = C_used - C_L:
+//  - below LMAX: sbar[l] = dL 2/pi dln k (the FFTLog sum, k_nl_fft<TAN>),
+//    +dL on the P_nl view's row l and -dL on the P_lin view's;
+//  - from LMAX on: +dL on the table rows of the interpolated value (l >
+//    LMIN_tab) or on the P_nl view's row (l <= LMIN_tab);
+//  - -dL on what the folded kernel summed: the direct row l - 1 (l <
+//    LMIN_tab) or the table rows (l >= LMIN_tab) of the gg column.
+// The LMAX switch is piecewise constant in the parameters.  Runs after
+// k_leg_adjoint (which stored the gg column's cotangent) and before the gg
+// adjoints read it.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ void clh_nl_interp_adj(const DrawDesc& d, double* tab, int l, double v) {
+  const double r = (log((double)l) - d.lim_gx[0]) * (1.0 / d.lim_gx[2]);
+  const int ii = (int)floor(r);
+  const int ic = ii < 0 ? 0 : (ii >= d.n_ell - 1 ? d.n_ell - 2 : ii);
+  const double t = r - ic;
+  atomicAdd(tab + CLH_NLOW + ic, v * (1.0 - t));
+  atomicAdd(tab + CLH_NLOW + ic + 1, v * t);
+}
+
+__global__ __launch_bounds__(256) void k_nl_cl_adj(const DrawDesc* __restrict__ D, const int2* __restrict__ work) {
+  __shared__ double u[32];
+  const int2 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const int i = w.y, l = threadIdx.x, nt = d.ntheta;
+  const ClhNlOff off = clh_nl_off(d.nl);
+  const ClhNlTan ot = clh_nl_tan(d.nl);
+  if (l < nt) {
+    const int canon = nt * (2 * d.n_ss + d.n_gs_dv) + nt * i + l;
+    u[l] = clh_masked(d, canon) ? 0.0 : d.dvbar[d.dv_start[2] + nt * i + l];
+  }
+  __syncthreads();
+  if (l >= CLH_NL_LT) return;
+  double* sbar = d.nlt + ot.sbar + (size_t)i * CLH_NL_LT;
+  if (l == 0 || l >= CLH_NL_L) {
+    sbar[l] = 0.0;
+    return;
+  }
+  double x = 0.0;
+  for (int th = 0; th < nt; th++) x += d.nl_pw[(size_t)th * CLH_NL_L + l] * u[th];
+  const int lmax = (int)d.nlw[off.lmx + i];
+  const int ncol = d.n_ss + d.n_gs + d.nl, col = d.n_ss + d.n_gs + i;
+  double* vnl = d.nlc_bar + (size_t)col * (CLH_NL_L - 1);
+  double* vlin = d.nlc_bar + (size_t)(ncol + col) * (CLH_NL_L - 1);
+  double* tab = d.cbar + (size_t)col * d.nrow;
+  sbar[l] = l < lmax ? x * d.nlw[off.scal + 1] * 2. / M_PI : 0.0;
+  vnl[l - 1] = (l < lmax || l <= CLH_LMIN_TAB) ? x : 0.0;
+  vlin[l - 1] = l < lmax ? -x : 0.0;
+  if (x == 0.0) return;
+  if (l >= lmax && l > CLH_LMIN_TAB) clh_nl_interp_adj(d, tab, l, x);  // C_used: the table value
+  if (l < CLH_LMIN_TAB) atomicAdd(tab + (l - 1), -x);                  // C_L: the direct row
+  else clh_nl_interp_adj(d, tab, l, -x);                               // C_L: the table value
+}
+
+hipError_t clh_launch_nl_cl_adj(const DrawDesc* D, const int2* wbin, int nbin, hipStream_t s) {
+  if (nbin <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nl_cl_adj, dim3(nbin), dim3(256), 0, s, D, wbin);
+  return hipGetLastError();
+}
+
+// the tangent inputs, their transforms and the tangent passes (g_b1, g_lens)
+hipError_t clh_launch_nl_tangent(const DrawDesc* D, const int2* wprep, int nprep, const int4* wfwd4, int nfwd4,
+                                 const int4* widft, int nidft, hipStream_t s) {
+  if (nprep <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nl_prep_tan, dim3(nprep), dim3(256), 0, s, D, wprep);
+  hipLaunchKernelGGL(k_nl_fwd<true>, dim3(nfwd4), dim3(256), 0, s, D, wfwd4);
+  hipLaunchKernelGGL((k_nl_fft<false, true>), dim3(nidft), dim3(256), 0, s, D, widft);
+  return hipGetLastError();
+}
+
 hipError_t clh_launch_nl_forward(const DrawDesc* D, const int2* wprep, int nprep, const int4* wfwd, int nfwd,
                                  const int4* wgl, int ngl, const int4* wchk, int nchk, const int2* wbin, int nbin,
                                  const int4* widft, int nidft, hipStream_t s) {
   if (nprep <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_nl_prep, dim3(nprep), dim3(256), 0, s, D, wprep);
-  hipLaunchKernelGGL(k_nl_fwd, dim3(nfwd), dim3(256), 0, s, D, wfwd);
+  hipLaunchKernelGGL(k_nl_fwd<false>, dim3(nfwd), dim3(256), 0, s, D, wfwd);
   hipLaunchKernelGGL(k_nl_gl, dim3(ngl), dim3(256), 0, s, D, wgl);
   hipLaunchKernelGGL(k_nl_fft<true>, dim3(nchk), dim3(256), 0, s, D, wchk);
   hipLaunchKernelGGL(k_nl_lmax, dim3(nbin), dim3(64), 0, s, D, wbin);

@@ -105,7 +105,7 @@ int clh_set_linear_power_spectrum(clh_ctx* ctx, const double* log10k, int nk,
  * default) Limber w(theta) at every ell; 0 (the reference's default) the
  * non-Limber C_gg of C_cl_tomo (cosmo2D.c:4528-4769) below ell 150, FFTLog
  * with tolerance 0.01, Limber above.  Needs the linear P(k,z) and H0;
- * clh_batch_vjp refuses it. */
+ * clh_batch_vjp includes it (not with one-loop bias, TATT or 6x2pt). */
 int clh_set_adopt_limber_gg(clh_ctx* ctx, int limber);
 /* set_source_sample / set_lens_sample (generic_interface.cpp:1907-1999,
  * 1802-1883): row-major (nrows, ntomo+1) table, column 0 = z */
@@ -200,7 +200,7 @@ int clh_compute_3x2pt_fourier(clh_ctx* ctx, const int ord[3], const int* mask, d
  * clh_datavector_size then reports; the 3-entry ord of the other entry
  * points keeps gk, ks, kk after the 3x2pt blocks.  Needs chi(z) beyond
  * z = 1090 (g_cmb) and clh_init_cmb_cross_correlation; clh_batch_vjp
- * refuses it.  Real space only, as in the reference (whose Fourier gk / ks
+ * includes these blocks.  Real space only, as in the reference (whose Fourier gk / ks
  * are not implemented).  With clh_init_cmb_auto_bandpower the kk block holds
  * band powers instead of C_kk at like.ell. */
 int clh_init_probes(clh_ctx* ctx, const char* probes);
@@ -365,9 +365,11 @@ int clh_batch_set_params(clh_ctx* ctx, const double* lnP, const double* src_dz,
  * every non-NULL device pointer, d<dv_bar, dv>/d(input): g_lnP [B][nk*nz],
  * g_src_dz [B][ns], g_lens_dz [B][nl], g_b1 [B][nl], g_ia_A [B], g_m [B][ns].
  * Differentiates the discrete algorithm (no reference counterpart).  Refused
- * (CLH_ERR_ARG) for batches with the non-Limber w, TATT with more than 10
- * source bins, more than one layout, custom ells or high_def_integration
- * != 0.  With 6x2pt the w_gk, w_ks and C_kk blocks are included.
+ * (CLH_ERR_ARG) for batches with TATT with more than 10 source bins, more
+ * than one layout, custom ells or high_def_integration != 0, and for the
+ * non-Limber w together with one-loop bias, TATT, 6x2pt or
+ * clh_set_vjp_lens_limits(1).  With 6x2pt the w_gk, w_ks and C_kk blocks
+ * are included; with the non-Limber w its correction below ell 150.
  * Under TATT the gradient runs through A1 (ia_A, redshift evolution mode 3)
  * with A2, eta2 and b_TA held at their uploaded values. */
 int clh_batch_vjp(clh_ctx* ctx, const int ord[3], const double* dv_bar, double* g_lnP,

@@ -2,7 +2,8 @@
 the GPU vs the FP64 oracle: the C_gg the w sum reads (non-Limber below each
 bin's LMAX, Limber above), LMAX itself, the real-space data vector in both
 Legendre modes and with magnification, batches with graph replay, the
-reference-named w_gg_tomo(limber = 0), and the refusals.
+reference-named w_gg_tomo(limber = 0), and the refusals (the VJP is in
+test_gpu_vjp.py).
 
 Bar: 1e-6 relative per entry with a floor of 1e-3 of the row's largest
 magnitude (the FP64 parity metric); LMAX identical.
@@ -152,9 +153,15 @@ def test_nonlimber_refusals(nl):
     n = c.datavector_size()
     _forward(c, n, 1)
     bar = torch.ones(n, dtype=torch.float64, device="cuda:0")
-    g = torch.zeros(p.ns, dtype=torch.float64, device="cuda:0")
-    with pytest.raises(CosmolikeError, match="non-Limber"):
-        c.batch_vjp(bar.data_ptr(), g_src_dz=g.data_ptr())
+    g = torch.zeros(p.nl, dtype=torch.float64, device="cuda:0")
+    # the VJP (test_gpu_vjp.py) takes the lens-limit term of the views'
+    # rows from the node tangents only
+    c.set_vjp_lens_limits(1)
+    try:
+        with pytest.raises(CosmolikeError, match="non-Limber"):
+            c.batch_vjp(bar.data_ptr(), g_lens_dz=g.data_ptr())
+    finally:
+        c.set_vjp_lens_limits(0)
     q = _problem()
     q.lnPL = None  # no linear P(k,z)
     c2 = Cosmolike().load(q)
