@@ -56,6 +56,7 @@ hipError_t clh_launch_limber_sx_adj(const DrawDesc* D, const int4* work, int n, 
 hipError_t clh_launch_limber_gg_adj(const DrawDesc* D, const int4* work, int n, int stage_doubles, int ck,
                                    int lim, int f32, int ol, hipStream_t s);
 hipError_t clh_launch_nodes_tan(const DrawDesc* D, const int2* work, int n, hipStream_t s);
+hipError_t clh_launch_limber_hdi_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s);
 hipError_t clh_launch_tatt_adj(const DrawDesc* D, const int4* work, int n, int lds_doubles, int ns, int gs,
                                hipStream_t s);
 hipError_t clh_launch_limber_sx_tatt(const DrawDesc* D, const int4* work, int n, hipStream_t s);
@@ -1167,11 +1168,13 @@ static int build_work(clh_ctx* c, Batch& bt, const int rows[4]) {
     const int ntask = 2 * n_ss + n_gs + nl;
     if (L.hdi != 0) {
       // every column (and the TATT B-mode ss columns) in tiles of CLH_HDI_ROWS
-      // rows; no node, adjoint or lens-limit items (the VJP refuses the batch)
+      // rows, for the forward and its adjoint (k_limber_hdi_adj); the
+      // lens-limit items of the VJP's finite-difference views; no node items
       const int ncol = n_ss + n_gs + nl + (L.ia_model == CLH_IA_TATT ? n_ss : 0);
       for (int col = 0; col < ncol; col++)
         for (int r0 = 0; r0 < nrow; r0 += CLH_HDI_ROWS)
           whd.push_back(make_int4(i, col, r0, std::min(CLH_HDI_ROWS, nrow - r0)));
+      for (int zl = 0; zl < nl; zl++) wlim.push_back(make_int2(i, zl));
     }
     if (L.hdi == 0) {
     for (int sg = 0; sg <= 2 * nl; sg++) wn.push_back(make_int2(i, sg));
@@ -3037,7 +3040,7 @@ int clh_autotune(clh_ctx* c, int rows_out[4], int lds_out[2]) {
   for (int i = 0; i < bt.B; i++) bt.desc[i].dv = bt.d_dv.p + bt.dv_off[i];
   HIPCHK(c, hipMemcpyAsync(bt.d_desc.p, bt.desc.data(), bt.B * sizeof(DrawDesc), hipMemcpyHostToDevice, c->stream));
   const bool adj = bt.uniform && bt.nknz > 0 && !(bt.tatt && bt.lays[0]->src.nbin > 10) && !bt.nl_any &&
-                   !bt.hdi_any && !bt.nrow_custom;  // the VJP refuses these
+                   !bt.hdi_any && !bt.nrow_custom;  // (no adjoint tiles to tune for these)
   // unit cotangent, ln P and lens-shift gradients of the adjoint stages (the
   // ln P runs and the lens-limit term are part of what is timed)
   DBuf<double> dvbar, glnp, glens;
@@ -3190,7 +3193,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
                 "VJP of the non-Limber w(theta) together with one-loop bias, TATT or 6x2pt, or with the "
                 "finite-difference lens limits, is not implemented");
   if (bt.nrow_custom) return fail(c, CLH_ERR_ARG, "VJP of a custom-ell batch (no data vector)");
-  if (bt.hdi_any) return fail(c, CLH_ERR_ARG, "VJP at high_def_integration != 0 is not implemented");
+  if (bt.hdi_any && (bt.tatt || bt.oneloop))
+    return fail(c, CLH_ERR_ARG, "VJP at high_def_integration != 0 with TATT or one-loop bias is not implemented");
   const Layout& L = *bt.lays[0];
   if (!ord_ok(L, ord))
     return fail(c, CLH_ERR_ARG, "ord must be a permutation of 0,1,2 whose blocks tile the data vector");
@@ -3221,9 +3225,12 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
   // (one-loop, TATT and 6x2pt batches take the term from the
   // finite-difference views: the node-motion tangent covers neither the
   // one-loop and TATT terms nor the gk integral's nodes)
-  const bool lim_tan = g_lens_dz && nl > 0 && c->vjp_lim == 0 && !bt.oneloop && !bt.tatt && !six;
+  // (and so do batches at high_def_integration != 0, whose nodes k_limber_hdi
+  // places itself)
+  const bool hdi = bt.hdi_any;
+  const bool lim_tan = g_lens_dz && nl > 0 && c->vjp_lim == 0 && !bt.oneloop && !bt.tatt && !six && !hdi;
   const bool lim_fd =
-      g_lens_dz && nl > 0 && (c->vjp_lim == 1 || (c->vjp_lim == 0 && (bt.oneloop || bt.tatt || six)));
+      g_lens_dz && nl > 0 && (c->vjp_lim == 1 || (c->vjp_lim == 0 && (bt.oneloop || bt.tatt || six || hdi)));
   const int ol = bt.oneloop ? 1 : 0;
   HIPCHK(c, bt.d_vjp.ensure(per * B));
   // per-node ln P cotangent runs of the ss / gs nodes and the gather work
@@ -3345,6 +3352,8 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
       HIPCHK(c, clh_launch_limber_sx_tatt(bt.d_desc.p, bt.d_wsx.p + bt.n_wsx_std, bt.n_wsx - bt.n_wsx_std, s));
     if (six)  // the C_gk / C_ks / C_kk columns (d/dm reads C_ks; the views' base rows)
       HIPCHK(c, clh_launch_cmb(bt.d_desc.p, bt.d_wcmb.p, bt.n_wcmb, nullptr, 0, s));
+    if (hdi)  // the n-point GL rows of every column
+      HIPCHK(c, clh_launch_limber_hdi(bt.d_desc.p, bt.d_whd.p, bt.n_whd, s));
     if (nlv) {  // the non-Limber views' nodes and rows, the FFTLog stages and the LMAX switch
       HIPCHK(c, clh_launch_nodes(bt.d_desc_nl.p, bt.d_wn_nl.p, bt.n_wn_nl, s));
       HIPCHK(c, clh_launch_limber_gg(bt.d_desc_nl.p, bt.d_wgg_nl.p, bt.n_wgg_nl, 0, 0, 0, s, 0));
@@ -3376,7 +3385,12 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
     HIPCHK(c, clh_launch_limber_sx_adj(bt.d_desc.p, bt.d_wl.p, bt.n_wl_ss, bt.adj_lds_ss, ns, ck_all ? 1 : 0, 0, 0,
                                        0, s));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[7], s));
-  if (g_lnP) HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
+  if (g_lnP) {
+    if (hdi)  // no ss nodes (k_limber_hdi places its own): g_lnP starts at zero
+      HIPCHK(c, clh_launch_fill(g_lnP, (size_t)bt.nknz * B, 0.0, s));
+    else
+      HIPCHK(c, clh_launch_lnp_gather(bt.d_desc.p, bt.d_wlg.p, bt.n_wlg, bt.lnp_ic, nzp, nl, s));
+  }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[8], s));
   // (with the lens-limit term the gs and gg adjoints evaluate their
   // integrands again: the term needs the P(k,z) cells and the a_1 walks)
@@ -3392,6 +3406,9 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
                                      lim_tan ? 1 : 0, c->precision != CLH_PREC_FP64 ? 1 : 0, ol, s));
   // 6x2pt: the gk / ks / kk Limber adjoint (its ln P runs added after the gather)
   if (six) HIPCHK(c, clh_launch_limber_cmb_adj(bt.d_desc.p, bt.d_wcmb.p, bt.n_wcmb, s));
+  // high_def_integration != 0: the adjoint of every column's n-point rows (the
+  // node-based adjoints above have no items; the gather stored zeros)
+  if (hdi) HIPCHK(c, clh_launch_limber_hdi_adj(bt.d_desc.p, bt.d_whd.p, bt.n_whd, s));
   if (nlv) {  // non-Limber w: the views' gg rows (FP64), then the FFTLog sums' tangents
     HIPCHK(c, clh_launch_limber_gg_adj(bt.d_desc_nlv.p, bt.d_wgg_nl.p, bt.n_wgg_nl, 0, 0, lim_tan ? 1 : 0, 0, 0, s));
     HIPCHK(c, clh_launch_nl_tangent(bt.d_desc.p, bt.d_wnl_prep.p, bt.n_wnl_prep, bt.d_wnl_fwd4.p, bt.n_wnl_fwd4,
@@ -3442,6 +3459,7 @@ int clh_batch_vjp(clh_ctx* c, const int ord[3], const double* dv_bar, double* g_
       HIPCHK(c, clh_launch_limber_sx(V, bt.d_wsx_gs.p, bt.n_wsx_gs, 0, 0, s, ol));
       if (bt.n_wsx_gs_t) HIPCHK(c, clh_launch_limber_sx_tatt(V, bt.d_wsx_gs_t.p, bt.n_wsx_gs_t, s));
       if (six) HIPCHK(c, clh_launch_cmb(V, bt.d_wcmb.p, bt.n_wcmb, nullptr, 0, s));
+      if (hdi) HIPCHK(c, clh_launch_limber_hdi(V, bt.d_whd.p, bt.n_whd, s));
     }
     HIPCHK(c, clh_launch_lim_grad(bt.d_desc.p, bt.d_desc_lim.p, bt.d_desc_lim.p + B, bt.d_wlim.p, bt.n_wlim, s));
   }

@@ -217,6 +217,202 @@ __global__ __launch_bounds__(256) void k_limber_hdi(const DrawDesc* __restrict__
   }
 }
 
+// ----------------------------------------------------------------------------
+// k_limber_hdi_adj: adjoint of k_limber_hdi for NLA draws without one-loop
+// terms (the VJP refuses the others at high_def_integration != 0).  Work
+// item (draw, column, row0, rows) as the forward; thread = GL node n (n =
+// tid, tid + 256, ...) walking the tile's rows.  Per (row, node) with c the
+// row's cotangent: c dv/d ln P into the node's bilinear P(k,z) cell (a run
+// per cell, added to g_lnP when the k cell changes; after the gather), and
+// row sums that carry the node's radial weights to the parameters:
+//   ss: v = A_1 A_2 P fac p4 w, A_b = W_K,b - C1_b W_S,b: source shifts
+//       through dA_b = wpre dg_b - C1_b dW_S,b, A_IA through -c1 W_S,b;
+//   gs: v = (W_K - C1 W_S)(W_gal b1 + W_mag pref b_mag) P fac pref2 w: the
+//       source shift, A_IA, b1, and the lens shift through W_gal and W_mag;
+//   gg: v = t^2 P fac w, t = W_gal b1 + W_mag pref b_mag + W_RSD: b1 and the
+//       lens shift through W_gal, W_mag and the two n(z) of W_RSD.
+// The motion of the gs / gg limits with |dz| comes from the
+// finite-difference views.
+// ----------------------------------------------------------------------------
+// d W_RSD / d dz of the lens bin (clh_w_rsd with the n(z) shift tangents)
+__device__ __forceinline__ double clh_w_rsd_ddz(const DrawDesc& d, double l, double a0, double a1, int ni) {
+  double c0, dc0, c1, dc1;
+  clh_chi_all(d, a0, c0, dc0);
+  clh_chi_all(d, a1, c1, dc1);
+  const double h0 = 1.0 / ((a0 * a0) * dc0);
+  const double h1 = 1.0 / ((a1 * a1) * dc1);
+  double w = (1 + 8. * l) / ((2. * l + 1.) * (2. * l + 1.)) * clh_dnz_lens(d, 1. / a0 - 1., ni) * h0 *
+             clh_f_growth(d, 1.0 / a0 - 1.0);
+  w -= 4. / (2 * l + 3.) * sqrt((2 * l + 1.) / (2 * l + 3.)) * clh_dnz_lens(d, 1. / a1 - 1., ni) * h1 *
+       clh_f_growth(d, 1.0 / a1 - 1.0);
+  return w;
+}
+
+__global__ __launch_bounds__(256) void k_limber_hdi_adj(const DrawDesc* __restrict__ D,
+                                                        const int4* __restrict__ work) {
+  __shared__ double red[4];
+  __shared__ double cr[CLH_HDI_ROWS];
+  const int4 w = work[blockIdx.x];
+  const DrawDesc& d = D[w.x];
+  const int ncol = d.n_ss + d.n_gs + d.nl;
+  const int col = w.y, row0 = w.z, nr = w.w;
+  if (col >= ncol) return;  // (B-mode columns: TATT)
+  int kind, bi, bj;  // ss: source bins bi <= bj; gs: lens bi, source bj; gg: lens bi
+  double amin, amax;
+  int np;
+  const double* gx;
+  if (col < d.n_ss) {
+    kind = 0;
+    clh_ss_pair(d.ns, col, bi, bj);
+    amin = 1. / (d.s_zmax_all + 1.);
+    amax = 1. / (1. + fmax(d.s_zmin_all, 1e-6));
+    np = d.np_ss;
+    gx = d.gl_hd;
+  } else if (col < d.n_ss + d.n_gs) {
+    kind = 1;
+    bi = (col - d.n_ss) / d.ns;
+    bj = (col - d.n_ss) % d.ns;
+    amin = clh_amin_lens(d, bi);
+    amax = clh_amax_lens(d, bi);
+    np = d.np_ss;
+    gx = d.gl_hd;
+  } else {
+    kind = 2;
+    bi = bj = col - d.n_ss - d.n_gs;
+    amin = clh_amin_lens(d, bi);
+    amax = clh_amax_lens(d, bi);
+    np = d.np_gg;
+    gx = d.gl_hd + 2 * d.np_ss;
+  }
+  for (int t = threadIdx.x; t < nr; t += 256) cr[t] = d.cbar[(size_t)col * d.nrow + row0 + t];
+  __syncthreads();
+  const double* ells = kind == 0 ? d.ell_ss : d.ell_gx;
+  const bool lnp = d.run != nullptr;  // the ln P cotangent is wanted
+  const double A = 0.5 * (amax - amin), B = 0.5 * (amax + amin);
+  const double b1 = kind ? d.par->b1[bi] : 0.0, bmag = kind ? d.par->bmag[bi] : 0.0;
+  const int nz = d.nzp;
+  double gi = 0.0, gj = 0.0, gl = 0.0, gb = 0.0, gA = 0.0;
+  for (int n = threadIdx.x; n < np; n += 256) {
+    const double a = B + A * gx[n], wn = A * gx[np + n];
+    double fK, dchida;
+    clh_chi_all(d, a, fK, dchida);
+    const double hh = clh_hoverh0v2(a, dchida), z = 1.0 / a - 1.0;
+    const double wpre = 1.5 * d.omega_m * fK / a, fac = dchida / (fK * fK);
+    const double g = clh_growfac(d, a), c1u = clh_ia_c1_unit(d, a, g);
+    // node constants: ss A_i, A_j (and their shift / A_IA tangents); gs, gg
+    // the source and lens weights
+    double X0 = 0, X1 = 0, dX0 = 0, dX1 = 0, aX0 = 0, aX1 = 0, WG = 0, WM = 0, dWG = 0, dWM = 0;
+    if (kind == 0) {
+      const double C1i = clh_ia_c1(d, a, g, bi), C1j = clh_ia_c1(d, a, g, bj);
+      const double WSi = clh_nz_src(d, z, bi) * hh, WSj = clh_nz_src(d, z, bj) * hh;
+      X0 = wpre * clh_g_tomo(d, a, bi) - C1i * WSi;
+      X1 = wpre * clh_g_tomo(d, a, bj) - C1j * WSj;
+      dX0 = wpre * clh_dg_tomo(d, a, bi) - C1i * clh_dnz_src(d, z, bi) * hh;
+      dX1 = wpre * clh_dg_tomo(d, a, bj) - C1j * clh_dnz_src(d, z, bj) * hh;
+      aX0 = -c1u * WSi;
+      aX1 = -c1u * WSj;
+    } else {
+      WG = clh_nz_lens(d, z, bi) * hh;
+      WM = wpre * clh_g_lens(d, a, bi);
+      dWG = clh_dnz_lens(d, z, bi) * hh;
+      dWM = wpre * clh_dg_lens(d, a, bi);
+      if (kind == 1) {
+        const double C1 = clh_ia_c1(d, a, g, bj), WS = clh_nz_src(d, z, bj) * hh;
+        X0 = wpre * clh_g_tomo(d, a, bj) - WS * C1;
+        dX0 = wpre * clh_dg_tomo(d, a, bj) - C1 * clh_dnz_src(d, z, bj) * hh;
+        aX0 = -c1u * WS;
+      }
+    }
+    double S0 = 0.0, S1 = 0.0;
+    int ci = -1;
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+    auto flush = [&]() {
+      if (ci < 0) return;
+      double* gp = d.g_lnP + ci;
+      if (q0 != 0.0) atomicAdd(gp, q0);
+      if (q1 != 0.0) atomicAdd(gp + 1, q1);
+      if (q2 != 0.0) atomicAdd(gp + nz, q2);
+      if (q3 != 0.0) atomicAdd(gp + nz + 1, q3);
+    };
+    for (int r = 0; r < nr; r++) {
+      const double c = cr[r];
+      if (c == 0.0) continue;
+      const double l = ells[row0 + r], ell = l + 0.5, k = ell / fK;
+      double gv;  // c dv / d ln P
+      if (kind == 2 && (ell + 1.0) / k > d.chi_amin) continue;  // chi_1 beyond chi(a_min): zero
+      int cell;
+      double wc[4];
+      const double PK = clh_p_nonlin_cells(d, k, a, cell, wc);
+      const double pref = l * (l + 1.) / (ell * ell);
+      if (kind == 0) {
+        const double p4 = l * (l - 1.) * (l + 1.) * (l + 2.) / (ell * ell * ell * ell);
+        const double s = c * PK * fac * p4 * wn;
+        S0 += s;
+        gv = s * X0 * X1;
+      } else if (kind == 1) {
+        const double tmp = (l - 1.) * l * (l + 1.) * (l + 2.);
+        const double pref2 = (tmp > 0) ? sqrt(tmp) / (ell * ell) : 0.0;
+        const double s = c * PK * fac * pref2 * wn;
+        S0 += s;
+        S1 += s * pref;
+        gv = s * X0 * (WG * b1 + WM * pref * bmag);
+      } else {
+        const double a0 = clh_a_chi(d, ell / k), a1 = clh_a_chi(d, (ell + 1.0) / k);
+        const double t = WG * b1 + WM * pref * bmag + clh_w_rsd(d, ell, a0, a1, bi);
+        const double s = 2.0 * c * t * PK * fac * wn;
+        gv = 0.5 * s * t;
+        gb += s * WG;
+        gl += s * (dWG * b1 + dWM * pref * bmag + clh_w_rsd_ddz(d, ell, a0, a1, bi));
+      }
+      if (lnp) {
+        if (cell != ci) {
+          flush();
+          ci = cell;
+          q0 = q1 = q2 = q3 = 0.0;
+        }
+        q0 += gv * wc[0];
+        q1 += gv * wc[1];
+        q2 += gv * wc[2];
+        q3 += gv * wc[3];
+      }
+    }
+    if (lnp) flush();
+    if (kind == 0) {
+      gi += S0 * X1 * dX0;
+      gj += S0 * X0 * dX1;
+      gA += S0 * (X1 * aX0 + X0 * aX1);
+    } else if (kind == 1) {
+      const double Gs = b1 * WG * S0 + bmag * WM * S1;
+      gj += dX0 * Gs;
+      gA += aX0 * Gs;
+      gb += X0 * WG * S0;
+      gl += X0 * (b1 * dWG * S0 + bmag * dWM * S1);
+    }
+  }
+  gi = clh_block_sum(gi, red);
+  gj = clh_block_sum(gj, red);
+  gl = clh_block_sum(gl, red);
+  gb = clh_block_sum(gb, red);
+  gA = clh_block_sum(gA, red);
+  if (threadIdx.x == 0) {
+    if (kind == 0) {
+      if (gi != 0.0) atomicAdd(d.g_src + bi, gi);
+      if (gj != 0.0) atomicAdd(d.g_src + bj, gj);
+    } else {
+      if (kind == 1 && gj != 0.0) atomicAdd(d.g_src + bj, gj);
+      if (gl != 0.0) atomicAdd(d.g_lens + bi, gl);
+      if (gb != 0.0) atomicAdd(d.g_b1 + bi, gb);
+    }
+    if (gA != 0.0) atomicAdd(d.g_A, gA);
+  }
+}
+
+hipError_t clh_launch_limber_hdi_adj(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_limber_hdi_adj, dim3(n), dim3(256), 0, s, D, work);
+  return hipGetLastError();
+}
+
 hipError_t clh_launch_limber_hdi(const DrawDesc* D, const int4* work, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_limber_hdi, dim3(n), dim3(256), 0, s, D, work);

@@ -75,7 +75,8 @@ int clh_set_precision(clh_ctx* ctx, int mode);
  * lens-efficiency fine grid 25 (1 + |hdi|) (redshift_spline.c:865) and the
  * lens zmean GL order 256 / 512 / 1024 (:809-811).  At hdi != 0 the Limber
  * rows are direct n-point sums of the integrands (the node kernels serve
- * hdi = 0); the 6x2pt probes, the non-Limber w and the VJP refuse it. */
+ * hdi = 0); the 6x2pt probes and the non-Limber w refuse it, and so does
+ * the VJP under TATT or one-loop bias. */
 int clh_init_ntable(clh_ctx* ctx, int n_ell, int n_a, int lmax, int high_def);
 /* init_binning_real_space (generic_interface.cpp:479-505), arcmin */
 int clh_init_binning_real_space(clh_ctx* ctx, int ntheta, double theta_min_arcmin,
@@ -366,8 +367,9 @@ int clh_batch_set_params(clh_ctx* ctx, const double* lnP, const double* src_dz,
  * g_src_dz [B][ns], g_lens_dz [B][nl], g_b1 [B][nl], g_ia_A [B], g_m [B][ns].
  * Differentiates the discrete algorithm (no reference counterpart).  Refused
  * (CLH_ERR_ARG) for batches with TATT with more than 10 source bins, more
- * than one layout, custom ells or high_def_integration != 0, and for the
- * non-Limber w together with one-loop bias, TATT, 6x2pt or
+ * than one layout, custom ells, high_def_integration != 0 under TATT or
+ * one-loop bias, and for the non-Limber w together with one-loop bias, TATT,
+ * 6x2pt or
  * clh_set_vjp_lens_limits(1).  With 6x2pt the w_gk, w_ks and C_kk blocks
  * are included; with the non-Limber w its correction below ell 150.
  * Under TATT the gradient runs through A1 (ia_A, redshift evolution mode 3)

@@ -4,8 +4,9 @@ At |hdi| >= 1 the Limber rows are n-point GL sums of the integrands
 (k_limber_hdi: 96 .. 512 nodes for ss / gs, 128 .. 1024 for gg), the lens
 efficiencies use 25 (1 + |hdi|) fine-grid points per interval and the lens
 zmean a 512 / 1024-point rule.  Bar: 1e-6 relative (metric of the FP64 parity
-tests), 1e-4 in the FP32 mode; graph replays bit-identical; the VJP, the
-6x2pt probes and the non-Limber w refuse hdi != 0.
+tests), 1e-4 in the FP32 mode; graph replays bit-identical; the 6x2pt
+probes and the non-Limber w refuse hdi != 0, and so does the VJP under TATT
+(the VJP of NLA draws is in test_gpu_vjp.py).
 """
 import numpy as np
 import pytest
@@ -141,7 +142,8 @@ def test_hdi_mixed_levels_batch_graph():
 
 def test_hdi_refusals():
     import torch
-    p = _c2(1)
+    p = synth.tatt_problem()
+    p.hdi = 1
     c = Cosmolike().load(p)
     c.batch_upload([p])
     n = c.datavector_size()
