@@ -137,13 +137,15 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
         const double fl = clh_nz_lens(d, 1.0 / apl - 1.0, b) / (apl * apl);
         const double fh = clh_nz_lens(d, 1.0 / aph - 1.0, b) / (aph * aph);
         p += 0.5 * da0 * (fl + fh);
-        q += 0.5 * da0 * (fl / clh_chi(d, apl) + fh / clh_chi(d, aph));
+        // beyond the bin's n(z) support both ends are zero and so is their
+        // n / (chi a^2) term (+0 exactly): chi(a), a bisection, only where it counts
+        if (fl != 0.0 || fh != 0.0) q += 0.5 * da0 * (fl / clh_chi(d, apl) + fh / clh_chi(d, aph));
       }
       if (TANGENT) {
         const double gl = clh_dnz_lens(d, 1.0 / apl - 1.0, b) / (apl * apl);
         const double gh = clh_dnz_lens(d, 1.0 / aph - 1.0, b) / (aph * aph);
         pd += 0.5 * da0 * (gl + gh);
-        qd += 0.5 * da0 * (gl / clh_chi(d, apl) + gh / clh_chi(d, aph));
+        if (gl != 0.0 || gh != 0.0) qd += 0.5 * da0 * (gl / clh_chi(d, apl) + gh / clh_chi(d, aph));
       }
     }
     if (TABLES) {
