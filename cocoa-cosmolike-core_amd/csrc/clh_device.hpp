@@ -304,12 +304,17 @@ __device__ __forceinline__ double clh_amax_lens(const DrawDesc& d, int ni) {
 struct ClhChiWalk {
   int j = -1, jj = 0;
   double z0 = 0.0, c0 = 0.0, dc = 0.0, inv = 0.0;
+  double lo = 0.0, hi = 0.0;  // chi_z[j], chi_z[j + 1]: the walk test without loads
   __device__ __forceinline__ double chi(const DrawDesc& d, double z) {
     const double* __restrict__ zz = d.chi_z;
-    const int jn = (j < 0) ? clh_bs(zz, d.chi_n, z) : clh_walk(zz, d.chi_n, z, j);
-    if (jn != j) {
-      j = jn;
-      jj = j > d.chi_n - 3 ? d.chi_n - 3 : j;
+    const int n = d.chi_n;
+    // clh_walk keeps j exactly when this test holds
+    const bool stay = j >= 0 && (z >= lo || j == 0) && (z < hi || j == n - 2);
+    if (!stay) {
+      j = (j < 0) ? clh_bs(zz, n, z) : clh_walk(zz, n, z, j);
+      lo = zz[j];
+      hi = zz[j + 1];
+      jj = j > n - 3 ? n - 3 : j;
       const double* __restrict__ cc = d.chi_v;
       z0 = zz[jj];
       inv = 1.0 / (zz[jj + 1] - z0);
@@ -319,36 +324,6 @@ struct ClhChiWalk {
     return (c0 + (z - z0) * inv * dc) * (1.0 / CLH_COVERH0);
   }
 };
-
-// n and d n / d dz of source bin b at z (D: also the derivative)
-template <bool DER>
-__device__ __forceinline__ void clh_nz_src_w(const DrawDesc& d, double z, int b, int& j, double& n,
-                                             double& dn) {
-  const double zz = z - d.par->src_dz[b];
-  const int rows = d.s_rows;
-  n = dn = 0.0;
-  if (zz < d.s_zc[0] || zz > d.s_zc[rows - 1]) return;
-  j = (j < 0) ? clh_bs(d.s_zc, rows, zz) : clh_walk(d.s_zc, rows, zz, j);
-  const double* c = d.s_coef + (size_t)b * 4 * (rows - 1) + 4 * j;
-  const double x = zz - d.s_zc[j];
-  n = c[0] + x * (c[1] + x * (c[2] + x * c[3]));
-  if (DER) dn = -(c[1] + x * (2.0 * c[2] + 3.0 * x * c[3]));
-}
-
-template <bool DER>
-__device__ __forceinline__ void clh_nz_lens_w(const DrawDesc& d, double z, int b, int& j, double& n,
-                                              double& dn) {
-  const double s = d.par->lens_st[b];
-  const double zz = (z - d.par->lens_dz[b] - d.l_zmean[b]) / s + d.l_zmean[b];
-  const int rows = d.l_rows;
-  n = dn = 0.0;
-  if (zz <= d.l_zc[0] || zz >= d.l_zc[rows - 1]) return;
-  j = (j < 0) ? clh_bs(d.l_zc, rows, zz) : clh_walk(d.l_zc, rows, zz, j);
-  const double* c = d.l_coef + (size_t)b * 4 * (rows - 1) + 4 * j;
-  const double x = zz - d.l_zc[j];
-  n = (c[0] + x * (c[1] + x * (c[2] + x * c[3]))) / s;
-  if (DER) dn = -(c[1] + x * (2.0 * c[2] + 3.0 * x * c[3])) / (s * s);
-}
 
 // ---------------------------------------------------------------- derivatives
 // d/dx of the spline piece: b + dx (2c + 3 dx d)
@@ -688,6 +663,74 @@ struct ClhBracket {
     if ((v >= lo || j == 0) && (v < hi || j == n - 2)) return false;
     set(x, clh_walk(x, n, v, j));
     return true;
+  }
+};
+
+// n and d n / d dz of one source or lens bin along a run of z (the fine grid
+// of k_gtab): the knot interval's bounds and spline coefficients stay in
+// registers and are read again only when z leaves the interval (the index
+// clh_bs would return).  Source bins: the spline at z - dz; lens bins: at
+// (z - dz - zmean) / stretch + zmean, divided by the stretch (clh_nz_lens);
+// zero outside the knots.
+template <bool DER>
+struct ClhNzRun {
+  const double* __restrict__ zc;
+  const double* __restrict__ cf;
+  int rows;
+  bool lens, have = false;
+  double dz, s, zm;
+  ClhBracket br;
+  double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+  __device__ __forceinline__ void init(const DrawDesc& d, bool src, int b) {
+    lens = !src;
+    if (src) {
+      zc = d.s_zc;
+      rows = d.s_rows;
+      cf = d.s_coef + (size_t)b * 4 * (rows - 1);
+      dz = d.par->src_dz[b];
+      s = 1.0;
+      zm = 0.0;
+    } else {
+      zc = d.l_zc;
+      rows = d.l_rows;
+      cf = d.l_coef + (size_t)b * 4 * (rows - 1);
+      dz = d.par->lens_dz[b];
+      s = d.par->lens_st[b];
+      zm = d.l_zmean[b];
+    }
+  }
+  __device__ __forceinline__ void coef() {
+    const double* __restrict__ c = cf + 4 * br.j;
+    c0 = c[0];
+    c1 = c[1];
+    c2 = c[2];
+    c3 = c[3];
+  }
+  __device__ __forceinline__ void eval(double z, double& n, double& dn) {
+    n = dn = 0.0;
+    double zz;
+    if (lens) {
+      zz = (z - dz - zm) / s + zm;
+      if (zz <= zc[0] || zz >= zc[rows - 1]) return;
+    } else {
+      zz = z - dz;
+      if (zz < zc[0] || zz > zc[rows - 1]) return;
+    }
+    if (!have) {
+      br.set(zc, clh_bs(zc, rows, zz));
+      have = true;
+      coef();
+    } else if (br.move(zc, rows, zz)) {
+      coef();
+    }
+    const double x = zz - br.lo;
+    if (lens) {
+      n = (c0 + x * (c1 + x * (c2 + x * c3))) / s;
+      if (DER) dn = -(c1 + x * (2.0 * c2 + 3.0 * x * c3)) / (s * s);
+    } else {
+      n = c0 + x * (c1 + x * (c2 + x * c3));
+      if (DER) dn = -(c1 + x * (2.0 * c2 + 3.0 * x * c3));
+    }
   }
 };
 

@@ -162,15 +162,15 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   for (int t = threadIdx.x; t < nint; t += CLH_BLOCK) {
     double p = 0.0, q = 0.0, pd = 0.0, qd = 0.0;
     double fprev = 0.0, gprev = 0.0, fdprev = 0.0, gdprev = 0.0;
-    int jn = -1;
+    ClhNzRun<TANGENT> nw;
+    nw.init(d, src, b);
     ClhChiWalk cw;
     for (int s2 = 0; s2 <= x; s2++) {
       const int i = t * x + s2;
       const double a = amin + i * da;
       const double ia = 1.0 / a, z = ia - 1.0;
       double n, dn;
-      if (src) clh_nz_src_w<TANGENT>(d, z, b, jn, n, dn);
-      else clh_nz_lens_w<TANGENT>(d, z, b, jn, n, dn);
+      nw.eval(z, n, dn);
       const double chi = cw.chi(d, z);
       if (TABLES) {
         const double f = n * ia * ia;
