@@ -670,15 +670,15 @@ struct ClhBracket {
 // of k_gtab): the knot interval's bounds and spline coefficients stay in
 // registers and are read again only when z leaves the interval (the index
 // clh_bs would return).  Source bins: the spline at z - dz; lens bins: at
-// (z - dz - zmean) / stretch + zmean, divided by the stretch (clh_nz_lens);
-// zero outside the knots.
+// (z - dz - zmean) / stretch + zmean, divided by the stretch (clh_nz_lens),
+// the stretch's reciprocal taken once; zero outside the knots.
 template <bool DER>
 struct ClhNzRun {
   const double* __restrict__ zc;
   const double* __restrict__ cf;
   int rows;
   bool lens, have = false;
-  double dz, s, zm;
+  double dz, is, zm;  // shift, 1 / stretch, zmean
   ClhBracket br;
   double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
   __device__ __forceinline__ void init(const DrawDesc& d, bool src, int b) {
@@ -688,14 +688,14 @@ struct ClhNzRun {
       rows = d.s_rows;
       cf = d.s_coef + (size_t)b * 4 * (rows - 1);
       dz = d.par->src_dz[b];
-      s = 1.0;
+      is = 1.0;
       zm = 0.0;
     } else {
       zc = d.l_zc;
       rows = d.l_rows;
       cf = d.l_coef + (size_t)b * 4 * (rows - 1);
       dz = d.par->lens_dz[b];
-      s = d.par->lens_st[b];
+      is = 1.0 / d.par->lens_st[b];
       zm = d.l_zmean[b];
     }
   }
@@ -710,7 +710,7 @@ struct ClhNzRun {
     n = dn = 0.0;
     double zz;
     if (lens) {
-      zz = (z - dz - zm) / s + zm;
+      zz = (z - dz - zm) * is + zm;
       if (zz <= zc[0] || zz >= zc[rows - 1]) return;
     } else {
       zz = z - dz;
@@ -725,8 +725,8 @@ struct ClhNzRun {
     }
     const double x = zz - br.lo;
     if (lens) {
-      n = (c0 + x * (c1 + x * (c2 + x * c3))) / s;
-      if (DER) dn = -(c1 + x * (2.0 * c2 + 3.0 * x * c3)) / (s * s);
+      n = (c0 + x * (c1 + x * (c2 + x * c3))) * is;
+      if (DER) dn = -(c1 + x * (2.0 * c2 + 3.0 * x * c3)) * (is * is);
     } else {
       n = c0 + x * (c1 + x * (c2 + x * c3));
       if (DER) dn = -(c1 + x * (2.0 * c2 + 3.0 * x * c3));

@@ -168,13 +168,16 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
     for (int s2 = 0; s2 <= x; s2++) {
       const int i = t * x + s2;
       const double a = amin + i * da;
-      const double ia = 1.0 / a, z = ia - 1.0;
+      // per-point divisions as the reciprocal with two Newton steps (clh_rcp):
+      // a and chi are normal and positive on the grid
+      const double ia = clh_rcp(a), z = ia - 1.0;
       double n, dn;
       nw.eval(z, n, dn);
       const double chi = cw.chi(d, z);
+      const double ichi = clh_rcp(chi);
       if (TABLES) {
         const double f = n * ia * ia;
-        const double g = f / chi;
+        const double g = f * ichi;
         if (s2 > 0) {
           p += 0.5 * da * (fprev + f);
           q += 0.5 * da * (gprev + g);
@@ -184,7 +187,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
       }
       if (TANGENT) {
         const double fd = dn * ia * ia;
-        const double gd = fd / chi;
+        const double gd = fd * ichi;
         if (s2 > 0) {
           pd += 0.5 * da * (fdprev + fd);
           qd += 0.5 * da * (gdprev + gd);
