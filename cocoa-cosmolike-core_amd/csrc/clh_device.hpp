@@ -29,6 +29,18 @@ __device__ __forceinline__ void clh_chi_all(const DrawDesc& d, double a, double&
   dchida = out1;
 }
 
+// chi(a) of clh_chi_all with its interval found by walking from j0 (the
+// index of its bisection, whatever the start; j0 < 0: the bisection)
+__device__ __forceinline__ double clh_chi_near(const DrawDesc& d, double a, int j0) {
+  const double z = 1.0 / a - 1.0;
+  const double* __restrict__ zz = d.chi_z;
+  const double* __restrict__ cc = d.chi_v;
+  int j = j0 < 0 ? clh_bs(zz, d.chi_n, z) : clh_walk(zz, d.chi_n, z, j0);
+  if (j > d.chi_n - 3) j = d.chi_n - 3;
+  const double dy = (z - zz[j]) / (zz[j + 1] - zz[j]);
+  return (cc[j] + dy * (cc[j + 1] - cc[j])) / CLH_COVERH0;
+}
+
 __device__ __forceinline__ double clh_chi(const DrawDesc& d, double a) {
   double c, dc;
   clh_chi_all(d, a, c, dc);

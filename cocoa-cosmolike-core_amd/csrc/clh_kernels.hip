@@ -159,6 +159,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   }
 
   // per-interval trapezoid sums of n/a^2 and n/(chi a^2)
+  int jlast = -1;  // the thread's last chi(z) interval: where the table loop's walks start
   for (int t = threadIdx.x; t < nint; t += CLH_BLOCK) {
     double p = 0.0, q = 0.0, pd = 0.0, qd = 0.0;
     double fprev = 0.0, gprev = 0.0, fdprev = 0.0, gdprev = 0.0;
@@ -196,6 +197,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
         gdprev = gd;
       }
     }
+    jlast = cw.j;
     if (TABLES) {
       sP[t] = p;
       sQ[t] = q;
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(CLH_BLOCK) void k_gtab(const DrawDesc* __restrict__
   double* dtab = TANGENT ? d.dgtab + (size_t)w.y * d.n_a : nullptr;
   for (int t = t0; t < t1; t++) {
     const int i = (t + 1) * x;
-    const double chi = clh_chi(d, amin + i * da);
+    const double chi = clh_chi_near(d, amin + i * da, jlast);
     if (TABLES) {
       P += sP[t];
       Q += sQ[t];
